@@ -1,0 +1,149 @@
+/*
+ * plom_smc.h -- C ABI of the MI355X (gfx950) PLOM sequential Monte Carlo filter.
+ *
+ * Plain C: no torch or HIP types in any signature; the caller owns every host buffer,
+ * the library owns every device buffer.  All functions return PLOM_OK (0) or a negative
+ * PLOM_ERR_* code and never exit(); plom_last_error() gives the message of the last
+ * failure on the calling thread.  One context per host thread (contexts are not locked).
+ *
+ * What each entry point replaces in the reference (paths relative to the reference
+ * checkout, model_builder/C/ abbreviated C/):
+ *
+ *   plom_smc_run            run_SMC_zmq_inproc, C/core/smc.c:379-467 (and its OpenMP
+ *                           twin run_SMC, :204-307) together with the per-thread worker
+ *                           C/core/worker_inproc.c:21-102: propagate (f_prediction_sde_*,
+ *                           C/core/prediction_util.c:197-287, generated step_sde_*,
+ *                           C/templates/core_template.c:215-339), weight (get_log_likelihood,
+ *                           C/core/likelihood.c:56-70; weight, smc.c:27-83), resample
+ *                           (systematic_sampling, smc.c:90-112; resample_X, smc.c:142-161).
+ *                           Called where C/smc/main.c:318 and pmcmc's run_propag
+ *                           (C/pmcmc/pmcmc.c:24-37) call the filter.
+ *   plom_filter_create/...  the same filter split into upload / run / read so that a
+ *                           caller (pMCMC proposals, a server) keeps clouds resident.
+ *   plom_batch_run          no counterpart: many independent filters (pMCMC proposals,
+ *                           C/pmcmc/pmcmc.c:186-266 runs them one after another) advanced
+ *                           together, one observation interval per batched launch.
+ *   plom_server_*           no counterpart: continuous-batching request loop with
+ *                           admission by free HBM.
+ *   plom_model_*            the generated per-model API (C/core/plom.h:896-913,
+ *                           C/templates/core_template.c) compiled into the library.
+ */
+#ifndef PLOM_SMC_H
+#define PLOM_SMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PLOM_OK 0
+#define PLOM_ERR_ARG -1      /* invalid argument or shape */
+#define PLOM_ERR_HIP -2      /* HIP runtime error */
+#define PLOM_ERR_NOMEM -3    /* device memory exhausted */
+#define PLOM_ERR_MODEL -4    /* unknown model id */
+#define PLOM_ERR_STATE -5    /* call out of order (e.g. read before run) */
+
+/* noise switches, same bits as the reference's enum plom_noises_off (C/core/plom.h:66) */
+#define PLOM_NO_DEM_STO_BIT 1u
+#define PLOM_NO_ENV_STO_BIT 2u
+#define PLOM_NO_DRIFT_BIT 4u
+
+#define PLOM_FP64 0
+#define PLOM_FP32 1 /* FP32 propagation; log-weights, sums, scan and log-likelihood stay FP64 */
+
+#define PLOM_FLAG_NO_FILTER 1u        /* -t/--no_filter: propagate only (C/smc/main.c:177) */
+#define PLOM_FLAG_STORE_ANCESTRY 2u   /* keep select[n][j] for every n (needed by --full) */
+#define PLOM_FLAG_FINAL_STATE 4u      /* keep the resampled final cloud */
+
+#define PLOM_MAX_J (1 << 21)          /* particles per filter */
+
+typedef struct {
+    int32_t size_proj;   /* S: N_PAR_SV*N_CAC + N_TS_INC_UNIQUE (C/smc/main.c:249) */
+    int32_t n_ts;        /* N_TS */
+    int32_t n_cac;       /* N_CAC */
+    int32_t n_par_sv;    /* N_PAR_SV */
+    int32_t inc_offset;  /* first incidence slot of the state */
+    int32_t n_inc;       /* N_TS_INC_UNIQUE */
+    int32_t pv_size;     /* length of the flat natural-parameter vector */
+    int32_t n_dem;       /* demographic noises per population */
+    int32_t n_env;       /* environmental noises per population */
+} plom_model_info;
+
+typedef struct {
+    int32_t model_id;
+    int32_t J;           /* particles, 1..PLOM_MAX_J */
+    int32_t N;           /* observation intervals (nb_obs) */
+    int32_t precision;   /* PLOM_FP64 or PLOM_FP32 */
+    uint32_t noises_off; /* PLOM_NO_*_BIT */
+    uint32_t flags;      /* PLOM_FLAG_* */
+    uint32_t seed;       /* random key, first word */
+    uint32_t stream;     /* random key, second word: one per filter / request */
+    double dt;           /* Euler step, already 1/round(1/DT) (C/core/build.c:1298-1304) */
+    double like_min;     /* LIKE_MIN (C/smc/main.c:83) */
+} plom_filter_desc;
+
+typedef struct {
+    const double *times; /* [N+1] days since date_0 (settings data.times) */
+    const double *data;  /* [N*n_ts] observations, NaN = missing */
+    const double *pv;    /* [pv_size] natural parameters per population / series */
+    const double *x0;    /* [size_proj] initial state (incidence slots ignored) */
+} plom_filter_inputs;
+
+typedef struct {
+    double loglik;       /* sum_n loglik_n, accumulated in step order (smc.c:80) */
+    int32_t n_all_fail;  /* steps where every particle failed (smc.c:55-57) */
+    double *loglik_n;    /* optional [N]: per-interval log-likelihood (0 without data) */
+    double *ess_n;       /* optional [N]: effective sample size (0 without data / all fail) */
+    int32_t *ancestry;   /* optional [N*J]: select[n][j]; needs PLOM_FLAG_STORE_ANCESTRY */
+    double *final_state; /* optional [size_proj*J] SoA resampled final cloud (PLOM_FLAG_FINAL_STATE) */
+    double *final_logw;  /* optional [J] log-weights of the last interval, before resampling */
+} plom_filter_outputs;
+
+typedef struct plom_ctx plom_ctx;
+typedef struct plom_filter plom_filter;
+
+const char *plom_last_error(void);
+
+int plom_ctx_create(int device, plom_ctx **out);
+int plom_ctx_destroy(plom_ctx *ctx);
+
+int plom_model_count(void);
+int plom_model_find(const char *name);                 /* model id, or PLOM_ERR_MODEL */
+int plom_model_name(int model_id, char *buf, int len);
+int plom_model_info_get(int model_id, plom_model_info *out);
+
+/* one filter, blocking: upload, run, read back, free */
+int plom_smc_run(plom_ctx *ctx, const plom_filter_desc *desc, const plom_filter_inputs *in,
+                 plom_filter_outputs *out);
+
+/* resident filter: upload once, run (asynchronously on the context stream), read */
+int plom_filter_create(plom_ctx *ctx, const plom_filter_desc *desc, const plom_filter_inputs *in,
+                       plom_filter **out);
+int plom_filter_reset(plom_ctx *ctx, plom_filter *f);  /* back to interval 0, same inputs */
+int plom_filter_run(plom_ctx *ctx, plom_filter *f);    /* enqueue every interval */
+int plom_filter_read(plom_ctx *ctx, plom_filter *f, plom_filter_outputs *out); /* waits */
+int plom_filter_destroy(plom_ctx *ctx, plom_filter *f);
+int64_t plom_filter_device_bytes(const plom_filter_desc *desc); /* bytes a filter holds on the device */
+
+/* many resident filters advanced together: one batched launch per interval for all of
+ * them (uneven J and N allowed); results are bit-identical to plom_filter_run */
+int plom_batch_run(plom_ctx *ctx, int n_filters, plom_filter **filters, int use_graph);
+
+int plom_sync(plom_ctx *ctx);
+
+/* kernel duration of the last plom_filter_run / plom_batch_run, from HIP events on the
+ * context stream: total milliseconds and the split by kernel */
+int plom_last_timing(plom_ctx *ctx, double *total_ms, double *propagate_ms, double *scan_ms);
+
+/* device self-test of the shared arithmetic: out[i*6+..] = exp, log, Phi, sin, cos, sqrt of in[i] */
+int plom_selftest_math(plom_ctx *ctx, int n, const double *in, double *out);
+/* device self-test of the streams: the first n normals of particle j at interval n0 */
+int plom_selftest_normals(plom_ctx *ctx, uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j,
+                          int n, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLOM_SMC_H */

@@ -1,0 +1,337 @@
+"""Emit the generated per-model step / observation code.
+
+One header per model holds, for every noise variant of the reference
+(``full``, ``no_dem_sto``, ``no_env_sto``, ``no_dem_sto_no_env_sto``;
+model_builder/Ccoder.py:444-505, selected by get_f_pred, C/core/prediction_util.c:138-173):
+
+  plom_<m>_interval_<variant>_f64   one observation interval of Euler-Maruyama steps
+                                    for one particle (host + device, FP64)
+  plom_<m>_interval_<variant>_f32   the same in FP32 (device only)
+  plom_<m>_proj2obs_f64/_f32        state -> observed quantities (core_template.c:52-102)
+  plom_<m>_loglik                   sum over non-missing series of log(discretised-normal
+                                    likelihood), sanitised (core_template.c:342-367,
+                                    C/core/likelihood.c:56-92); always FP64
+
+Everything is unrolled over populations and states so that, on the GPU, a particle's
+whole state lives in registers.  Arithmetic is printed in the reference's term order
+(Ccoder.py:449-505): X + (drift)*dt + demographic terms + environmental terms.
+"""
+from . import expr as E
+
+VARIANTS = [
+    # name, demographic noise on, environmental noise on
+    ("full", True, True),
+    ("no_dem_sto", False, True),
+    ("no_env_sto", True, False),
+    ("no_dem_sto_no_env_sto", False, False),
+]
+
+
+def _R(real):
+    return {"double": ("double", "", "f64", "plom_nstream64", "plom_normal64"),
+            "float": ("float", "f", "f32", "plom_nstream32", "plom_normal32")}[real]
+
+
+class Emitter:
+    def __init__(self, model):
+        self.m = model
+        self.p = "plom_%s" % model.name.lower()
+
+    # ---------------------------------------------------------------- symbols
+    def xidx(self, state, cac):
+        return self.m.par_sv.index(state) * self.m.N_CAC + cac
+
+    def rate_emitter(self, cac, real):
+        m = self.m
+        lay = m.layout
+        Rt, sfx = _R(real)[0], _R(real)[1]
+
+        def resolve(name):
+            if name in m.par_sv:
+                return "X[%d]" % self.xidx(name, cac)
+            if name in lay["proc"]:
+                return "pv[%d]" % (lay["proc"][name] + cac)
+            if name in lay["fixed"]:
+                return "pv[%d]" % (lay["fixed"][name] + cac)
+            if name == "t":
+                return "tr"
+            if name == "M_PI":
+                return "3.141592653589793" + ("f" if real == "float" else "")
+            if name == "ONE_YEAR":
+                return "365.0" + ("f" if real == "float" else "")
+            raise E.__class__  # unreachable: symbols are checked at model build
+
+        def call(fn, args, node):
+            if fn == "correct_rate":
+                # skipped in SDE/ODE code (Ccoder.py:55-56, generator_C with no_correct_rate)
+                return "(%s)" % args[0]
+            if fn == "mix":
+                if len(node.args) != 1 or not isinstance(node.args[0], E.Sym) or node.args[0].name not in m.par_sv:
+                    raise ValueError("mix() takes one state variable")
+                s = node.args[0].name
+                base = lay["coupling"] + cac * m.N_CAC
+                return "(" + " + ".join("pv[%d] * X[%d]" % (base + k, self.xidx(s, k)) for k in range(m.N_CAC)) + ")"
+            return None
+
+        return E.CEmitter(resolve, real, call)
+
+    # ---------------------------------------------------------------- interval
+    def term(self, reaction, cac, from_state):
+        k = reaction["ind_cache"]
+        if from_state in ("U", self.m.remainder):
+            return "r%d_%d" % (k, cac)
+        return "r%d_%d*X[%d]" % (k, cac, self.xidx(from_state, cac))
+
+    def sd_text(self, sd, cac, real):
+        return self.rate_emitter(cac, real).emit(E.parse(sd))
+
+    def eq_parts(self, items, cac, real, dem, env):
+        """items: list of (sign, term, reaction). Returns (eq, dem, env) strings."""
+        sq = "plom_sqrt" if real == "double" else "__builtin_sqrtf"
+        eq = "".join(" %s (%s)" % (s, t) for s, t, _ in items)
+        d = "".join("%s %s((%s))*d%d_%d" % (s, sq, t, r["ind_dem_sto"], cac) for s, t, r in items) if dem else ""
+        e = ""
+        if env:
+            for s, t, r in items:
+                if "white_noise" in r:
+                    j = int(r["white_noise"]["name"].split("__")[1])
+                    e += "%s (%s)*%s*w%d_%d" % (s, t, self.sd_text(r["white_noise"]["sd"], cac, real), j, cac)
+        return eq, d, e
+
+    def interval(self, variant, dem, env, real):
+        m = self.m
+        Rt, sfx, tag, stream, normal = _R(real)
+        zero = "0.0" if real == "double" else "0.0f"
+        qual = "PLOM_HD" if real == "double" else "static __device__ inline __attribute__((always_inline))"
+        L = []
+        L.append("%s void %s_interval_%s_%s(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt)"
+                 % (qual, self.p, variant, tag, Rt, Rt, stream, Rt))
+        L.append("{")
+        L.append("    const %s dtr = (%s)dt;" % (Rt, Rt))
+        L.append("    (void)sqdt; (void)ns; (void)pv;")
+        for u in range(m.N_TS_INC_UNIQUE):
+            L.append("    X[%d] = %s; /* reset_inc */" % (m.inc_offset + u, zero))
+        L.append("    double t = t0;")
+        L.append("    while (t < t1) {")
+        L.append("        const %s tr = (%s)t; (void)tr;" % (Rt, Rt))
+        # rate caches then noises, population by population (core_template.c:258-284)
+        for c in range(m.N_CAC):
+            em = self.rate_emitter(c, real)
+            for k, tree in enumerate(m.rate_trees):
+                L.append("        const %s r%d_%d = %s;" % (Rt, k, c, em.emit(tree)))
+            if dem:
+                for i in range(len(m.proc_model)):
+                    L.append("        const %s d%d_%d = sqdt * %s(ns);" % (Rt, i, c, normal))
+            if env:
+                for j in range(len(m.white_noise)):
+                    L.append("        const %s w%d_%d = sqdt * %s(ns);" % (Rt, j, c, normal))
+        # state equations
+        upd = []
+        for s in m.par_sv:
+            for c in range(m.N_CAC):
+                items = []
+                for r in m.proc_model:
+                    if r["from"] == s:
+                        items.append(("-", self.term(r, c, r["from"]), r))
+                for r in m.proc_model:
+                    if r["to"] == s:
+                        items.append(("+", self.term(r, c, r["from"]), r))
+                eq, d, e = self.eq_parts(items, c, real, dem, env)
+                idx = self.xidx(s, c)
+                if not eq:
+                    rhs = "X[%d]" % idx
+                else:
+                    rhs = "X[%d] + (%s)*dtr" % (idx, eq)
+                    if dem:
+                        rhs += " + " + d
+                    if e:
+                        rhs += " + " + e
+                L.append("        const %s f%d = %s;" % (Rt, idx, rhs))
+                upd.append(idx)
+        # observed incidences (core_template.c:303-323)
+        u = 0
+        for o, lst in enumerate(m.obs_var_def):
+            if not m.is_inc[o]:
+                continue
+            info = m.obs2ts[o]
+            for tsu in range(info["n_ts_unique"]):
+                idx = m.inc_offset + u
+                L.append("        %s s%d = %s;" % (Rt, idx, zero))
+                for (ci, ai) in info["cac"][tsu]:
+                    c = ci * m.N_AC + ai
+                    items = []
+                    for dfn in lst:
+                        for ri in dfn["reactions"]:
+                            r = m.proc_model[ri]
+                            items.append(("+", self.term(r, c, dfn["from"]), r))
+                    eq, d, e = self.eq_parts(items, c, real, dem, env)
+                    rhs = "(%s)*dtr" % eq
+                    if dem:
+                        rhs += " + " + d
+                    if e:
+                        rhs += " + " + e
+                    L.append("        s%d += %s;" % (idx, rhs))
+                L.append("        const %s f%d = X[%d] + s%d;" % (Rt, idx, idx, idx))
+                upd.append(idx)
+                u += 1
+        # y_pred -> X with X >= 0 (core_template.c:328-336)
+        for idx in upd:
+            L.append("        X[%d] = (f%d < %s) ? %s : f%d;" % (idx, idx, zero, zero, idx))
+        L.append("        t += dt;")
+        L.append("    }")
+        L.append("}")
+        return "\n".join(L)
+
+    # ---------------------------------------------------------------- observations
+    def proj2obs(self, real):
+        m = self.m
+        Rt, sfx, tag = _R(real)[:3]
+        qual = "PLOM_HD" if real == "double" else "static __device__ inline __attribute__((always_inline))"
+        L = ["%s void %s_proj2obs_%s(const %s *X, const double *pvd, double *obs)" % (qual, self.p, tag, Rt), "{",
+             "    (void)pvd; (void)X;"]
+        ind = 0
+        u = 0
+        for o, lst in enumerate(m.obs_var_def):
+            info = m.obs2ts[o]
+            if m.is_inc[o]:
+                for tsu in range(info["n_ts_unique"]):
+                    for _ in range(info["n_stream"][tsu]):
+                        L.append("    obs[%d] = (double)X[%d];" % (ind, m.inc_offset + u))
+                        ind += 1
+                    u += 1
+        for o, lst in enumerate(m.obs_var_def):
+            info = m.obs2ts[o]
+            if m.is_inc[o]:
+                continue
+            prev = "+".join(("(N-%s)" % "-".join(m.par_sv)) if x == m.remainder else x for x in lst)
+            tree = E.parse(prev)
+            for tsu in range(info["n_ts_unique"]):
+                L.append("    {")
+                L.append("        double sum_prev = 0.0;")
+                for (ci, ai) in info["cac"][tsu]:
+                    c = ci * m.N_AC + ai
+                    em = self._prev_emitter(c)
+                    L.append("        sum_prev += %s;" % em.emit(tree))
+                for _ in range(info["n_stream"][tsu]):
+                    L.append("        obs[%d] = sum_prev;" % ind)
+                    ind += 1
+                L.append("    }")
+        L.append("}")
+        return "\n".join(L)
+
+    def _prev_emitter(self, cac):
+        m, lay = self.m, self.m.layout
+
+        def resolve(name):
+            if name in m.par_sv:
+                return "(double)X[%d]" % self.xidx(name, cac)
+            if name in lay["fixed"]:
+                return "pvd[%d]" % (lay["fixed"][name] + cac)
+            if name in lay["proc"]:
+                return "pvd[%d]" % (lay["proc"][name] + cac)
+            raise ValueError("symbol %r not allowed in a prevalence definition" % name)
+
+        return E.CEmitter(resolve, "double")
+
+    def loglik(self):
+        m, lay = self.m, self.m.layout
+        L = ["PLOM_HD double %s_loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor)" % self.p,
+             "{", "    double loglike = 0.0;", "    int nf = 0;", "    (void)t; (void)pvd;"]
+        mean_t = E.parse(m.obs_model["mean"])
+        var_t = E.parse(m.obs_model["var"])
+        for ts in range(m.N_TS):
+            def resolve(name, ts=ts):
+                if name == "x":
+                    return "x"
+                if name in lay["obs"]:
+                    return "pvd[%d]" % (lay["obs"][name] + ts)
+                if name in lay["fixed_obs"]:
+                    return "pvd[%d]" % (lay["fixed_obs"][name] + ts)
+                if name == "t":
+                    return "t"
+                if name == "M_PI":
+                    return "3.141592653589793"
+                if name == "ONE_YEAR":
+                    return "365.0"
+                raise ValueError("symbol %r not allowed in the observation model" % name)
+
+            em = E.CEmitter(resolve, "double")
+            L.append("    if (!plom_isnan(y[%d])) {" % ts)
+            L.append("        const double x = obs[%d];" % ts)
+            L.append("        const double gsl_mu = %s;" % em.emit(mean_t))
+            L.append("        const double gsl_sd = plom_sqrt(%s);" % em.emit(var_t))
+            L.append("        double like;")
+            L.append("        if (y[%d] > 0.0) like = plom_cdf_gaussian_P(y[%d] + 0.5 - gsl_mu, gsl_sd) - plom_cdf_gaussian_P(y[%d] - 0.5 - gsl_mu, gsl_sd);" % (ts, ts, ts))
+            L.append("        else like = plom_cdf_gaussian_P(y[%d] + 0.5 - gsl_mu, gsl_sd);" % ts)
+            L.append("        loglike += plom_log(plom_sanitize_likelihood(like, like_min, &nf));")
+            L.append("    }")
+        L.append("    *n_floor = nf;")
+        L.append("    return loglike;")
+        L.append("}")
+        return "\n".join(L)
+
+    # ---------------------------------------------------------------- header
+    def header(self):
+        m = self.m
+        P = self.p.upper()
+        ndem = len(m.proc_model)
+        nenv = len(m.white_noise)
+        out = ["/* Generated by iterated-inference-pipeline_amd/codegen from the %s model files. Do not edit. */" % m.name,
+               "#ifndef %s_H" % P, "#define %s_H" % P, '#include "plom_model_common.h"', ""]
+        out += ["#define %s_N_CAC %d" % (P, m.N_CAC),
+                "#define %s_N_PAR_SV %d" % (P, m.N_PAR_SV),
+                "#define %s_N_TS %d" % (P, m.N_TS),
+                "#define %s_N_TS_INC_UNIQUE %d" % (P, m.N_TS_INC_UNIQUE),
+                "#define %s_INC_OFFSET %d" % (P, m.inc_offset),
+                "#define %s_SIZE_PROJ %d" % (P, m.size_proj),
+                "#define %s_PV_SIZE %d" % (P, m.layout["size"]),
+                "#define %s_N_DEM %d" % (P, ndem),
+                "#define %s_N_ENV %d" % (P, nenv), ""]
+        for v, dem, env in VARIANTS:
+            out.append(self.interval(v, dem, env, "double"))
+            out.append("")
+        out.append(self.proj2obs("double"))
+        out.append("")
+        out.append(self.loglik())
+        out.append("")
+        # host dispatcher on the reference's noises_off bits (prediction_util.c:143-161)
+        out.append("PLOM_HD void %s_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off)" % self.p)
+        out.append("{")
+        out.append("    const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);")
+        out.append("    if (v == 0u) %s_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
+        out.append("    else if (v == PLOM_NO_DEM_STO) %s_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
+        out.append("    else if (v == PLOM_NO_ENV_STO) %s_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
+        out.append("    else %s_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
+        out.append("}")
+        out.append("")
+        out.append("#if defined(__HIPCC__)")
+        for v, dem, env in VARIANTS:
+            out.append(self.interval(v, dem, env, "float"))
+            out.append("")
+        out.append(self.proj2obs("float"))
+        out.append("")
+        name = m.name.lower()
+        out.append("struct plom_model_%s {" % name)
+        out.append("    static constexpr int S = %d, N_TS = %d, INC_OFFSET = %d, N_INC = %d, PV = %d, N_CAC = %d;"
+                   % (m.size_proj, m.N_TS, m.inc_offset, m.N_TS_INC_UNIQUE, m.layout["size"], m.N_CAC))
+        for tag, Rt, stream in (("f64", "double", "plom_nstream64"), ("f32", "float", "plom_nstream32")):
+            out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt)"
+                       % (Rt, Rt, stream, Rt))
+            out.append("    {")
+            for vi, (v, dem, env) in enumerate(VARIANTS):
+                kw = "if" if vi == 0 else "else if"
+                out.append("        %s constexpr (V == %d) %s_interval_%s_%s(X, pv, ns, t0, t1, dt, sqdt);" % (kw, vi, self.p, v, tag))
+            out.append("    }")
+            out.append("    static __device__ inline __attribute__((always_inline)) void proj2obs(const %s *X, const double *pvd, double *obs) { %s_proj2obs_%s(X, pvd, obs); }"
+                       % (Rt, self.p, tag))
+        out.append("    static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor)"
+                   " { return %s_loglik(obs, y, pvd, t, like_min, n_floor); }" % self.p)
+        out.append("};")
+        out.append("#endif")
+        out.append("")
+        out.append("#endif")
+        return "\n".join(out) + "\n"
+
+
+def emit_header(model):
+    return Emitter(model).header()

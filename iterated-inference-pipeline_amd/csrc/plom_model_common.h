@@ -1,0 +1,63 @@
+/*
+ * plom_model_common.h -- helpers used by the generated model headers.
+ *
+ * PLOM_NO_* are the reference's noise switches (model_builder/C/core/plom.h:66).
+ * The special functions follow model_builder/C/core/special_functions.c:45-74
+ * (slowstep returns its argument inside the ramp; the reference returns an undeclared
+ * `t` there, special_functions.c:74, which does not compile).
+ * plom_sanitize_likelihood follows model_builder/C/core/likelihood.c:77-92 and counts
+ * the floored series so that the filter can apply the all-series-floored failure rule.
+ */
+#ifndef PLOM_MODEL_COMMON_H
+#define PLOM_MODEL_COMMON_H
+
+#include "plom_rng.h"
+
+#define PLOM_NO_DEM_STO 1u
+#define PLOM_NO_ENV_STO 2u
+#define PLOM_NO_DRIFT 4u
+
+PLOM_HD double plom_heaviside(double x) { return (x < 0.0) ? 0.0 : 1.0; }
+PLOM_HD double plom_ramp(double x) { return (x >= 0.0) ? x : 0.0; }
+PLOM_HD double plom_slowstep(double x, double d) { return (x >= 0.0) ? ((x >= d) ? d : x) : 0.0; }
+
+PLOM_HD double plom_sanitize_likelihood(double like, double like_min, int *n_floor)
+{
+    if (plom_isnan(like) || like == PLOM_INF || like == -PLOM_INF || like < 0.0) {
+        (*n_floor)++;
+        return like_min;
+    }
+    if (like <= like_min) {
+        (*n_floor)++;
+        return like_min;
+    }
+    return like;
+}
+
+#if defined(__HIPCC__)
+#define PLOM_DF static __device__ inline __attribute__((always_inline))
+PLOM_DF float plom_sinf(float x) { return sinf(x); }
+PLOM_DF float plom_cosf(float x) { return cosf(x); }
+PLOM_DF float plom_expf(float x) { return expf(x); }
+PLOM_DF float plom_logf(float x) { return logf(x); }
+PLOM_DF float plom_sqrtf(float x) { return __builtin_sqrtf(x); }
+PLOM_DF float plom_fabsf(float x) { return fabsf(x); }
+PLOM_DF float plom_powif(float x, int n)
+{
+    int neg = n < 0;
+    unsigned int m = neg ? (unsigned int)(-n) : (unsigned int)n;
+    float r = 1.0f, b = x;
+    while (m) {
+        if (m & 1u) r = r * b;
+        m >>= 1;
+        if (m) b = b * b;
+    }
+    return neg ? 1.0f / r : r;
+}
+PLOM_DF float plom_powf(float x, float y) { return powf(x, y); }
+PLOM_DF float plom_heavisidef(float x) { return (x < 0.0f) ? 0.0f : 1.0f; }
+PLOM_DF float plom_rampf(float x) { return (x >= 0.0f) ? x : 0.0f; }
+PLOM_DF float plom_slowstepf(float x, float d) { return (x >= 0.0f) ? ((x >= d) ? d : x) : 0.0f; }
+#endif
+
+#endif /* PLOM_MODEL_COMMON_H */

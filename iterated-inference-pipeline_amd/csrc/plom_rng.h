@@ -1,0 +1,173 @@
+/*
+ * plom_rng.h -- counter-keyed random streams shared by the HIP kernels and the C oracle.
+ *
+ * Replaces the reference's one sequential GSL stream per worker thread
+ * (model_builder/C/core/build.c:965-971 seeds 2+id(+thread), :1019-1030 picks mt19937
+ * or ranlxs0), whose draws depend on the thread count and on which thread owns which
+ * particle.  Here every draw is a pure function of
+ *     (seed, stream id, observation interval n, particle j, purpose)
+ * so a filter gives the same result whatever the tiling, the batch it runs in, or the
+ * other requests beside it, and a lineage can be replayed particle by particle.
+ *
+ *   Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11 "Parallel random numbers: as easy
+ *   as 1, 2, 3"), key = (seed, stream id), counter = (j, n, purpose, 0), gives 128
+ *   bits per (n, j, purpose).
+ *   PURPOSE_PROPAGATE seeds a per-(n, j) xoshiro128** generator (Blackman & Vigna 2018)
+ *   that feeds, in order, every Gaussian draw of that particle over interval n: for each
+ *   Euler substep, for each population, the demographic then environmental noises, the
+ *   order of core_template.c:258-284, then the drift noises (drift.c:39).
+ *   PURPOSE_RESAMPLE gives the one systematic-resampling uniform of interval n
+ *   (the role of gsl_ran_flat(calc[0]) in smc.c:100).
+ *   PURPOSE_FULL gives the uniform that picks the final particle of a --full lineage.
+ *
+ * Uniforms are (x + 1/2) 2^-32 for a 32-bit x: never 0 or 1, exact in double.
+ * Gaussians: Box-Muller on two uniforms, both outputs used (the second is kept for the
+ * next draw).  FP64 uses plom_log/plom_sqrt/plom_sincos_2pi; the FP32 stream (device
+ * only) uses the gfx950 transcendental instructions.
+ */
+#ifndef PLOM_RNG_H
+#define PLOM_RNG_H
+
+#include "plom_math.h"
+
+#define PLOM_PURPOSE_PROPAGATE 0u
+#define PLOM_PURPOSE_RESAMPLE 1u
+#define PLOM_PURPOSE_FULL 2u
+
+PLOM_HD uint32_t plom_mulhi32(uint32_t a, uint32_t b)
+{
+    return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
+}
+
+PLOM_HD void plom_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint32_t hi0 = plom_mulhi32(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        uint32_t hi1 = plom_mulhi32(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* 128 bits for (seed, stream, n, j, purpose) */
+PLOM_HD void plom_draw128(uint32_t seed, uint32_t stream, uint32_t n, uint32_t j,
+                          uint32_t purpose, uint32_t out[4])
+{
+    uint32_t ctr[4] = { j, n, purpose, 0u };
+    uint32_t key[2] = { seed, stream };
+    plom_philox4x32_10(ctr, key, out);
+}
+
+typedef struct { uint32_t s0, s1, s2, s3; } plom_xoshiro128;
+
+PLOM_HD uint32_t plom_rotl32(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+
+PLOM_HD uint32_t plom_xoshiro_next(plom_xoshiro128 *r)
+{
+    uint32_t result = plom_rotl32(r->s1 * 5u, 7) * 9u;
+    uint32_t t = r->s1 << 9;
+    r->s2 ^= r->s0;
+    r->s3 ^= r->s1;
+    r->s1 ^= r->s2;
+    r->s0 ^= r->s3;
+    r->s2 ^= t;
+    r->s3 = plom_rotl32(r->s3, 11);
+    return result;
+}
+
+PLOM_HD void plom_xoshiro_seed(plom_xoshiro128 *r, const uint32_t w[4])
+{
+    r->s0 = w[0]; r->s1 = w[1]; r->s2 = w[2]; r->s3 = w[3];
+    if ((w[0] | w[1] | w[2] | w[3]) == 0u) r->s0 = 1u; /* the all-zero state is a fixed point */
+}
+
+PLOM_HD double plom_u01_from32(uint32_t x) { return ((double)x + 0.5) * 0x1p-32; }
+
+/* ---------------------------------------------------------------- FP64 Gaussian stream */
+typedef struct {
+    plom_xoshiro128 x;
+    double spare;
+    int has_spare;
+} plom_nstream64;
+
+PLOM_HD void plom_nstream64_init(plom_nstream64 *ns, uint32_t seed, uint32_t stream, uint32_t n, uint32_t j)
+{
+    uint32_t w[4];
+    plom_draw128(seed, stream, n, j, PLOM_PURPOSE_PROPAGATE, w);
+    plom_xoshiro_seed(&ns->x, w);
+    ns->spare = 0.0;
+    ns->has_spare = 0;
+}
+
+PLOM_HD double plom_normal64(plom_nstream64 *ns)
+{
+    if (ns->has_spare) {
+        ns->has_spare = 0;
+        return ns->spare;
+    }
+    double u1 = plom_u01_from32(plom_xoshiro_next(&ns->x));
+    double u2 = plom_u01_from32(plom_xoshiro_next(&ns->x));
+    double rad = plom_sqrt(-2.0 * plom_log(u1));
+    double s, c;
+    plom_sincos_2pi(u2, &s, &c);
+    ns->spare = rad * s;
+    ns->has_spare = 1;
+    return rad * c;
+}
+
+/* the resampling uniform of interval n as the integer numerator 2x+1 of U = (2x+1)/2^33 */
+PLOM_HD uint64_t plom_resample_num(uint32_t seed, uint32_t stream, uint32_t n)
+{
+    uint32_t w[4];
+    plom_draw128(seed, stream, n, 0u, PLOM_PURPOSE_RESAMPLE, w);
+    return 2ull * (uint64_t)w[0] + 1ull;
+}
+
+/* ---------------------------------------------------------------- FP32 Gaussian stream */
+#if defined(__HIPCC__)
+typedef struct {
+    plom_xoshiro128 x;
+    float spare;
+    int has_spare;
+} plom_nstream32;
+
+__device__ inline __attribute__((always_inline)) void
+plom_nstream32_init(plom_nstream32 *ns, uint32_t seed, uint32_t stream, uint32_t n, uint32_t j)
+{
+    uint32_t w[4];
+    plom_draw128(seed, stream, n, j, PLOM_PURPOSE_PROPAGATE, w);
+    plom_xoshiro_seed(&ns->x, w);
+    ns->spare = 0.0f;
+    ns->has_spare = 0;
+}
+
+/* u in (0,1) from the top 24 bits: exact in float */
+__device__ inline __attribute__((always_inline)) float plom_u01f(uint32_t x)
+{
+    return ((float)(x >> 8) + 0.5f) * 0x1p-24f;
+}
+
+__device__ inline __attribute__((always_inline)) float plom_normal32(plom_nstream32 *ns)
+{
+    if (ns->has_spare) {
+        ns->has_spare = 0;
+        return ns->spare;
+    }
+    float u1 = plom_u01f(plom_xoshiro_next(&ns->x));
+    float u2 = plom_u01f(plom_xoshiro_next(&ns->x));
+    /* -2 ln u1 = -2 ln2 log2(u1); v_log_f32 is log2, v_sin/v_cos_f32 take turns */
+    float rad = __builtin_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+    float s = __builtin_amdgcn_sinf(u2);
+    float c = __builtin_amdgcn_cosf(u2);
+    ns->spare = rad * s;
+    ns->has_spare = 1;
+    return rad * c;
+}
+#endif
+
+#endif /* PLOM_RNG_H */

@@ -1,0 +1,894 @@
+/*
+ * plom_smc.hip -- the PLOM SMC particle filter on MI355X (gfx950): kernels + C ABI.
+ *
+ * One observation interval n of one filter is two kernels over tiles of 256 particles
+ * (one particle per lane, 4 waves per tile):
+ *
+ *   k_propagate<Model, Real, Variant>
+ *       select:   ancestor a_j of particle j from the previous interval's fixed-point
+ *                 cumulative weights (systematic resampling, reference smc.c:90-112),
+ *       gather:   state of a_j from the SoA cloud [S][Jp] (resample_X, smc.c:142-161),
+ *       propagate K Euler-Maruyama substeps in registers (generated step, core_template.c
+ *                 :215-339, driven as in prediction_util.c:275-287),
+ *       weight:   log-weight l_j = sum_ts log like_ts (likelihood.c:56-92),
+ *       tile:     max m_t, tree sums of exp(l - m_t) and its square;
+ *       the block that completes the filter's last tile reduces the tiles to the global
+ *       max m, S = sum exp(l - m), S2, and writes loglik_n = m + log(S/J), ESS = S^2/S2,
+ *       the all-fail branch (smc.c:55-70) and loglik += loglik_n (smc.c:80).
+ *   k_scan
+ *       q_j = trunc(exp(l_j - m)/S * 2^62), tile-local inclusive integer scan, tile totals;
+ *       the block completing the last tile scans the tile totals and advances the
+ *       filter's interval counter.
+ *
+ * Canonical arithmetic shared with the oracle (oracle/smc_oracle.c): the sums over 256
+ * particles and over tiles are the recursive contiguous pairwise tree (zero padded to a
+ * power of two); the resampling compares (k 2^33 + 2u + 1) Q with C_i J 2^33 exactly in
+ * 128-bit integers, C the integer prefix of q, Q its total, u the interval's 32-bit
+ * uniform; select[k] is the first i with C_i J 2^33 >= (k 2^33 + 2u + 1) Q.
+ *
+ * A batch of filters runs the same kernels over a tile map (filter, tile); every filter
+ * keeps its own interval counter in device memory, so a launch sequence can be captured
+ * in a hipGraph and replayed.
+ */
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdarg.h>
+#include <vector>
+#include <algorithm>
+
+#include "plom_smc.h"
+#include "plom_math.h"
+#include "plom_rng.h"
+#include "plom_models.h"
+
+typedef unsigned __int128 plom_u128;
+
+#define TILE 256
+#define MAX_TILES (PLOM_MAX_J / TILE)
+
+/* ------------------------------------------------------------------ errors */
+static thread_local char g_err[512];
+
+static int set_err(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess)                                                          \
+            return set_err(e_ == hipErrorOutOfMemory ? PLOM_ERR_NOMEM : PLOM_ERR_HIP,  \
+                           "%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char *plom_last_error(void) { return g_err; }
+
+/* ------------------------------------------------------------------ device filter */
+struct plom_dfilter {
+    int32_t J, Jp, T, N, S, n_ts, flags, step;
+    uint32_t seed, stream, prec, pad;
+    double dt, sqdt, like_min, all_fail_llike;
+    void *X[2];
+    const void *x0;
+    const double *pv;
+    const float *pvf;
+    const double *times, *data;
+    const int32_t *nonan;
+    double *logw, *tmax, *tsum, *tsum2;
+    uint64_t *cum, *ttot, *tincl;
+    uint32_t *cnt;      /* [0] propagate arrivals, [1] scan arrivals */
+    double *scal;       /* m, S, S2, loglik */
+    int32_t *n_all_fail;
+    double *llike_n, *ess_n;
+    int32_t *rflag;     /* rflag[n] = 1 when interval n ended with a resampling */
+    int32_t *ancestry;  /* [N][J] or null */
+};
+
+/* ------------------------------------------------------------------ block helpers */
+/* recursive contiguous pairwise sum of the 256 values of a block (every lane gets it) */
+__device__ inline double tree256(double v, double *lds)
+{
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1) v = v + __shfl_xor(v, h, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    double r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+    __syncthreads();
+    return r;
+}
+
+__device__ inline double max256(double v, double *lds)
+{
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1) v = fmax(v, __shfl_xor(v, h, 64));
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    double r = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+    __syncthreads();
+    return r;
+}
+
+/* inclusive scan of 256 uint64 values (exact: any order gives the same integers) */
+__device__ inline uint64_t scan256(uint64_t v, uint64_t *lds)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1) {
+        uint64_t o = __shfl_up(v, h, 64);
+        if (lane >= h) v += o;
+    }
+    if (lane == 63) lds[w] = v;
+    __syncthreads();
+    uint64_t off = 0;
+    for (int k = 0; k < w; k++) off += lds[k];
+    __syncthreads();
+    return v + off;
+}
+
+__device__ inline uint32_t arrive(uint32_t *cnt)
+{
+    __shared__ uint32_t s_old;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_old = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    __threadfence();
+    return s_old;
+}
+
+/* systematic resampling of interval n: ancestor of output slot k */
+__device__ inline int select_ancestor(const plom_dfilter *f, int k, int n)
+{
+    const int T = f->T;
+    const uint64_t *tincl = f->tincl;
+    const uint64_t Q = tincl[T - 1];
+    const uint64_t unum = plom_resample_num(f->seed, f->stream, (uint32_t)n);
+    const plom_u128 Rk = (plom_u128)(((uint64_t)k << 33) + unum) * (plom_u128)Q;
+    const uint64_t J2 = (uint64_t)f->J << 33;
+    int lo = 0, hi = T - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if ((plom_u128)tincl[mid] * J2 >= Rk) hi = mid; else lo = mid + 1;
+    }
+    const uint64_t P = lo ? tincl[lo - 1] : 0ull;
+    const uint64_t *c = f->cum + (size_t)lo * TILE;
+    int a = 0, b = TILE - 1;
+    while (a < b) {
+        int mid = (a + b) >> 1;
+        if ((plom_u128)(P + c[mid]) * J2 >= Rk) b = mid; else a = mid + 1;
+    }
+    return lo * TILE + a;
+}
+
+/* global reduction of the tile statistics by the last block of interval n */
+__device__ void finalize_weights(plom_dfilter *f, int n, double *lds)
+{
+    __shared__ double s_chunk[MAX_TILES / TILE + 1][2];
+    const int T = f->T;
+    double m = -PLOM_INF;
+    for (int t = threadIdx.x; t < T; t += TILE) m = fmax(m, f->tmax[t]);
+    m = max256(m, lds);
+    double llike_n, ess;
+    int rflag;
+    if (m == -PLOM_INF) {
+        /* every particle failed: equal weights, identity ancestry (smc.c:55-70) */
+        llike_n = f->all_fail_llike;
+        ess = 0.0;
+        rflag = 0;
+    } else {
+        const int nch = (T + TILE - 1) / TILE;
+        for (int c = 0; c < nch; c++) {
+            const int t = c * TILE + threadIdx.x;
+            double v = 0.0, v2 = 0.0;
+            if (t < T) {
+                const double mt = f->tmax[t];
+                if (mt != -PLOM_INF) {
+                    const double e = plom_exp(mt - m);
+                    v = f->tsum[t] * e;
+                    v2 = (f->tsum2[t] * e) * e;
+                }
+            }
+            v = tree256(v, lds);
+            v2 = tree256(v2, lds);
+            if (threadIdx.x == 0) { s_chunk[c][0] = v; s_chunk[c][1] = v2; }
+        }
+        __syncthreads();
+        double S = 0.0, S2 = 0.0;
+        if (threadIdx.x == 0) {
+            int P = 1;
+            while (P < nch) P <<= 1;
+            for (int c = nch; c < P; c++) { s_chunk[c][0] = 0.0; s_chunk[c][1] = 0.0; }
+            for (int h = 1; h < P; h <<= 1)
+                for (int i = 0; i < P; i += 2 * h) {
+                    s_chunk[i][0] = s_chunk[i][0] + s_chunk[i + h][0];
+                    s_chunk[i][1] = s_chunk[i][1] + s_chunk[i + h][1];
+                }
+            S = s_chunk[0][0];
+            S2 = s_chunk[0][1];
+        }
+        llike_n = m + plom_log(S / (double)f->J);
+        ess = (S * S) / S2;
+        rflag = 1;
+        if (threadIdx.x == 0) { f->scal[0] = m; f->scal[1] = S; f->scal[2] = S2; }
+    }
+    if (threadIdx.x == 0) {
+        if (!rflag) f->n_all_fail[0] += 1;
+        f->llike_n[n] = llike_n;
+        f->ess_n[n] = ess;
+        f->rflag[n] = rflag;
+        f->scal[3] = f->scal[3] + llike_n;
+    }
+}
+
+/* ------------------------------------------------------------------ kernels */
+template <class M, typename R, int V>
+__global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const int2 *tiles)
+{
+    __shared__ double lds[8];
+    int fi = 0, tile = blockIdx.x;
+    if (tiles) { const int2 tt = tiles[blockIdx.x]; fi = tt.x; tile = tt.y; }
+    plom_dfilter *f = &filters[fi];
+    const int n = f->step;
+    if (n >= f->N) return;
+    const int J = f->J, Jp = f->Jp;
+    const int j = tile * TILE + threadIdx.x;
+    const bool valid = j < J;
+
+    R X[M::S];
+    if (n == 0) {
+        const R *x0 = (const R *)f->x0;
+#pragma unroll
+        for (int k = 0; k < M::S; k++) X[k] = x0[k];
+    } else {
+        int a = j;
+        if (valid && f->rflag[n - 1]) a = select_ancestor(f, j, n - 1);
+        if (valid && f->ancestry) f->ancestry[(size_t)(n - 1) * J + j] = a;
+        const R *Xin = (const R *)f->X[n & 1];
+        if (!valid) a = 0;
+#pragma unroll
+        for (int k = 0; k < M::S; k++) X[k] = Xin[(size_t)k * Jp + a];
+    }
+
+    const double t0 = f->times[n], t1 = f->times[n + 1];
+    if (valid) {
+        if constexpr (sizeof(R) == 8) {
+            plom_nstream64 ns;
+            plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j);
+            M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt);
+        } else {
+            plom_nstream32 ns;
+            plom_nstream32_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j);
+            M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt);
+        }
+        R *Xout = (R *)f->X[(n + 1) & 1];
+#pragma unroll
+        for (int k = 0; k < M::S; k++) Xout[(size_t)k * Jp + j] = X[k];
+    }
+
+    if ((f->flags & PLOM_FLAG_NO_FILTER) || f->nonan[n] == 0) return;
+
+    double l = -PLOM_INF;
+    if (valid) {
+        double obs[M::N_TS];
+        M::proj2obs(X, f->pv, obs);
+        int nf = 0;
+        l = M::loglik(obs, f->data + (size_t)n * f->n_ts, f->pv, t1, f->like_min, &nf);
+        /* failure: every series observed and floored at LIKE_MIN (smc.c:44) */
+        if (nf == M::N_TS) l = -PLOM_INF;
+    }
+    f->logw[j] = l;
+    const double mt = max256(l, lds);
+    double w = 0.0;
+    if (mt != -PLOM_INF && l != -PLOM_INF) w = plom_exp(l - mt);
+    const double s = tree256(w, lds);
+    const double s2 = tree256(w * w, lds);
+    if (threadIdx.x == 0) { f->tmax[tile] = mt; f->tsum[tile] = s; f->tsum2[tile] = s2; }
+    if (arrive(&f->cnt[0]) == (uint32_t)(f->T - 1)) {
+        finalize_weights(f, n, lds);
+        if (threadIdx.x == 0) f->cnt[0] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(TILE) void k_scan(plom_dfilter *filters, const int2 *tiles)
+{
+    __shared__ uint64_t lds[8];
+    __shared__ uint64_t s_carry;
+    int fi = 0, tile = blockIdx.x;
+    if (tiles) { const int2 tt = tiles[blockIdx.x]; fi = tt.x; tile = tt.y; }
+    plom_dfilter *f = &filters[fi];
+    const int n = f->step;
+    if (n >= f->N) return;
+    const bool active = !(f->flags & PLOM_FLAG_NO_FILTER) && f->nonan[n] > 0 && f->rflag[n];
+    if (active) {
+        const double m = f->scal[0], S = f->scal[1];
+        const int j = tile * TILE + threadIdx.x;
+        const double l = f->logw[j];
+        uint64_t q = 0;
+        if (l != -PLOM_INF) q = (uint64_t)((plom_exp(l - m) / S) * 0x1p62);
+        const uint64_t c = scan256(q, lds);
+        f->cum[j] = c;
+        if (threadIdx.x == TILE - 1) f->ttot[tile] = c;
+    }
+    if (arrive(&f->cnt[1]) == (uint32_t)(f->T - 1)) {
+        if (active) {
+            if (threadIdx.x == 0) s_carry = 0;
+            __syncthreads();
+            for (int base = 0; base < f->T; base += TILE) {
+                const int t = base + threadIdx.x;
+                const uint64_t v = t < f->T ? f->ttot[t] : 0ull;
+                const uint64_t c = scan256(v, lds) + s_carry;
+                if (t < f->T) f->tincl[t] = c;
+                __syncthreads();
+                if (threadIdx.x == TILE - 1) s_carry = c;
+                __syncthreads();
+            }
+        }
+        if (threadIdx.x == 0) {
+            f->step = n + 1;
+            f->cnt[1] = 0u;
+        }
+    }
+}
+
+/* resampling of the last interval: ancestry row N-1 and the resampled final cloud */
+template <typename R>
+__global__ __launch_bounds__(TILE) void k_final(plom_dfilter *filters, const int2 *tiles, int S)
+{
+    int fi = 0, tile = blockIdx.x;
+    if (tiles) { const int2 tt = tiles[blockIdx.x]; fi = tt.x; tile = tt.y; }
+    plom_dfilter *f = &filters[fi];
+    const int N = f->N, J = f->J, Jp = f->Jp;
+    const int j = tile * TILE + threadIdx.x;
+    if (j >= J || N == 0) return;
+    int a = j;
+    if (f->rflag[N - 1]) a = select_ancestor(f, j, N - 1);
+    if (f->ancestry) f->ancestry[(size_t)(N - 1) * J + j] = a;
+    const R *Xin = (const R *)f->X[N & 1];
+    R *Xout = (R *)f->X[(N + 1) & 1];
+    for (int k = 0; k < S; k++) Xout[(size_t)k * Jp + j] = Xin[(size_t)k * Jp + a];
+}
+
+__global__ void k_selftest_math(int n, const double *in, double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = in[i];
+    out[i * 6 + 0] = plom_exp(x);
+    out[i * 6 + 1] = plom_log(x);
+    out[i * 6 + 2] = plom_Phi(x);
+    out[i * 6 + 3] = plom_sin(x);
+    out[i * 6 + 4] = plom_cos(x);
+    out[i * 6 + 5] = plom_sqrt(x);
+}
+
+__global__ void k_selftest_normals(uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int n, double *out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    plom_nstream64 ns;
+    plom_nstream64_init(&ns, seed, stream, n0, j);
+    for (int i = 0; i < n; i++) out[i] = plom_normal64(&ns);
+}
+
+/* ------------------------------------------------------------------ model table */
+struct model_entry {
+    const char *name;
+    plom_model_info info;
+    void (*launch_prop)(int prec, int variant, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles);
+};
+
+template <class M, typename R>
+static void launch_prop_r(int variant, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles)
+{
+    switch (variant) {
+    case 0: hipLaunchKernelGGL((k_propagate<M, R, 0>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
+    case 1: hipLaunchKernelGGL((k_propagate<M, R, 1>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
+    case 2: hipLaunchKernelGGL((k_propagate<M, R, 2>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
+    default: hipLaunchKernelGGL((k_propagate<M, R, 3>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
+    }
+}
+
+template <class M>
+static void launch_prop(int prec, int variant, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles)
+{
+    if (prec == PLOM_FP32) launch_prop_r<M, float>(variant, nblocks, s, df, tiles);
+    else launch_prop_r<M, double>(variant, nblocks, s, df, tiles);
+}
+
+#define MODEL_ENTRY(lname, UNAME)                                                               \
+    { #lname,                                                                                   \
+      { UNAME##_SIZE_PROJ, UNAME##_N_TS, UNAME##_N_CAC, UNAME##_N_PAR_SV, UNAME##_INC_OFFSET,   \
+        UNAME##_N_TS_INC_UNIQUE, UNAME##_PV_SIZE, UNAME##_N_DEM, UNAME##_N_ENV },               \
+      launch_prop<plom_model_##lname> }
+
+static const model_entry g_models[] = {
+    MODEL_ENTRY(sir, PLOM_SIR),
+    MODEL_ENTRY(sirs4, PLOM_SIRS4),
+    MODEL_ENTRY(seir10, PLOM_SEIR10),
+};
+static const int g_n_models = (int)(sizeof g_models / sizeof g_models[0]);
+
+extern "C" int plom_model_count(void) { return g_n_models; }
+
+extern "C" int plom_model_find(const char *name)
+{
+    for (int i = 0; i < g_n_models; i++)
+        if (name && !strcmp(name, g_models[i].name)) return i;
+    return set_err(PLOM_ERR_MODEL, "unknown model '%s'", name ? name : "(null)");
+}
+
+extern "C" int plom_model_name(int id, char *buf, int len)
+{
+    if (id < 0 || id >= g_n_models) return set_err(PLOM_ERR_MODEL, "model id %d out of range", id);
+    if (buf && len > 0) snprintf(buf, (size_t)len, "%s", g_models[id].name);
+    return PLOM_OK;
+}
+
+extern "C" int plom_model_info_get(int id, plom_model_info *out)
+{
+    if (id < 0 || id >= g_n_models) return set_err(PLOM_ERR_MODEL, "model id %d out of range", id);
+    if (!out) return set_err(PLOM_ERR_ARG, "null output");
+    *out = g_models[id].info;
+    return PLOM_OK;
+}
+
+/* ------------------------------------------------------------------ context / filters */
+struct plom_ctx {
+    int device;
+    hipStream_t stream;
+    hipEvent_t ev[4];
+    double last_total_ms, last_prop_ms, last_scan_ms;
+    int timing;
+};
+
+struct plom_filter {
+    plom_filter_desc desc;
+    plom_model_info info;
+    int variant;
+    char *dev;              /* one allocation holding everything */
+    size_t dev_bytes;
+    plom_dfilter *df;       /* device copy of the descriptor */
+    plom_dfilter h;         /* host copy (device pointers) */
+    hipGraphExec_t graph;   /* captured launch sequence of plom_filter_run */
+    int has_run;
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct layout {
+    size_t off_X0, off_X1, off_logw, off_cum, off_tmax, off_tsum, off_tsum2, off_ttot, off_tincl,
+        off_cnt, off_scal, off_naf, off_llike_n, off_ess_n, off_rflag, off_nonan, off_times, off_data,
+        off_pv, off_pvf, off_x0, off_anc, off_df, total;
+};
+
+static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, layout *L)
+{
+    const size_t J = (size_t)d->J, N = (size_t)d->N;
+    const size_t Jp = (J + TILE - 1) / TILE * TILE, T = Jp / TILE;
+    const size_t rs = d->precision == PLOM_FP32 ? 4 : 8;
+    const size_t S = (size_t)mi->size_proj;
+    size_t o = 0;
+#define PUT(field, bytes) do { L->field = o; o += align256(bytes); } while (0)
+    PUT(off_X0, S * Jp * rs);
+    PUT(off_X1, S * Jp * rs);
+    PUT(off_logw, Jp * 8);
+    PUT(off_cum, Jp * 8);
+    PUT(off_tmax, T * 8);
+    PUT(off_tsum, T * 8);
+    PUT(off_tsum2, T * 8);
+    PUT(off_ttot, T * 8);
+    PUT(off_tincl, T * 8);
+    PUT(off_cnt, 2 * 4);
+    PUT(off_scal, 4 * 8);
+    PUT(off_naf, 4);
+    PUT(off_llike_n, N * 8);
+    PUT(off_ess_n, N * 8);
+    PUT(off_rflag, N * 4);
+    PUT(off_nonan, N * 4);
+    PUT(off_times, (N + 1) * 8);
+    PUT(off_data, N * (size_t)mi->n_ts * 8);
+    PUT(off_pv, (size_t)mi->pv_size * 8);
+    PUT(off_pvf, (size_t)mi->pv_size * 4);
+    PUT(off_x0, S * 8);
+    PUT(off_anc, (d->flags & PLOM_FLAG_STORE_ANCESTRY) ? N * J * 4 : 0);
+    PUT(off_df, sizeof(plom_dfilter));
+#undef PUT
+    L->total = o;
+    return PLOM_OK;
+}
+
+static int check_desc(const plom_filter_desc *d, plom_model_info *mi)
+{
+    if (!d) return set_err(PLOM_ERR_ARG, "null descriptor");
+    if (d->model_id < 0 || d->model_id >= g_n_models) return set_err(PLOM_ERR_MODEL, "model id %d out of range", d->model_id);
+    *mi = g_models[d->model_id].info;
+    if (d->J < 1 || d->J > PLOM_MAX_J) return set_err(PLOM_ERR_ARG, "J=%d outside [1, %d]", d->J, PLOM_MAX_J);
+    if (d->N < 0) return set_err(PLOM_ERR_ARG, "N=%d < 0", d->N);
+    if (d->precision != PLOM_FP64 && d->precision != PLOM_FP32) return set_err(PLOM_ERR_ARG, "bad precision %d", d->precision);
+    if (!(d->dt > 0.0)) return set_err(PLOM_ERR_ARG, "dt must be > 0");
+    if (!(d->like_min > 0.0)) return set_err(PLOM_ERR_ARG, "like_min must be > 0");
+    return PLOM_OK;
+}
+
+extern "C" int64_t plom_filter_device_bytes(const plom_filter_desc *d)
+{
+    plom_model_info mi;
+    if (check_desc(d, &mi) != PLOM_OK) return -1;
+    layout L;
+    make_layout(d, &mi, &L);
+    return (int64_t)L.total;
+}
+
+extern "C" int plom_ctx_create(int device, plom_ctx **out)
+{
+    if (!out) return set_err(PLOM_ERR_ARG, "null output");
+    HIPCHK(hipSetDevice(device));
+    plom_ctx *c = (plom_ctx *)calloc(1, sizeof(plom_ctx));
+    if (!c) return set_err(PLOM_ERR_NOMEM, "host allocation failed");
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { free(c); return set_err(PLOM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+    for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&c->ev[i]));
+    c->timing = 1;
+    *out = c;
+    return PLOM_OK;
+}
+
+extern "C" int plom_ctx_destroy(plom_ctx *c)
+{
+    if (!c) return PLOM_OK;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < 4; i++) HIPCHK(hipEventDestroy(c->ev[i]));
+    HIPCHK(hipStreamDestroy(c->stream));
+    free(c);
+    return PLOM_OK;
+}
+
+extern "C" int plom_sync(plom_ctx *c)
+{
+    if (!c) return set_err(PLOM_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PLOM_OK;
+}
+
+/* reset the per-run device state of a filter (enqueued on the context stream) */
+static int filter_reset_async(plom_ctx *c, plom_filter *f)
+{
+    const layout *Lp = nullptr;
+    (void)Lp;
+    plom_dfilter *h = &f->h;
+    HIPCHK(hipMemsetAsync(h->cnt, 0, 2 * sizeof(uint32_t), c->stream));
+    HIPCHK(hipMemsetAsync(h->scal, 0, 4 * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(h->n_all_fail, 0, sizeof(int32_t), c->stream));
+    if (h->N > 0) {
+        HIPCHK(hipMemsetAsync(h->rflag, 0, (size_t)h->N * sizeof(int32_t), c->stream));
+        HIPCHK(hipMemsetAsync(h->llike_n, 0, (size_t)h->N * sizeof(double), c->stream));
+        HIPCHK(hipMemsetAsync(h->ess_n, 0, (size_t)h->N * sizeof(double), c->stream));
+    }
+    /* the step counter lives inside the device descriptor */
+    HIPCHK(hipMemcpyAsync(f->df, h, sizeof(plom_dfilter), hipMemcpyHostToDevice, c->stream));
+    return PLOM_OK;
+}
+
+extern "C" int plom_filter_create(plom_ctx *c, const plom_filter_desc *d, const plom_filter_inputs *in, plom_filter **out)
+{
+    if (!c || !out || !in) return set_err(PLOM_ERR_ARG, "null argument");
+    plom_model_info mi;
+    int rc = check_desc(d, &mi);
+    if (rc != PLOM_OK) return rc;
+    if (!in->times || !in->pv || !in->x0 || (d->N > 0 && !in->data)) return set_err(PLOM_ERR_ARG, "null input array");
+    for (int n = 0; n < d->N; n++)
+        if (!(in->times[n + 1] > in->times[n])) return set_err(PLOM_ERR_ARG, "times must increase (n=%d)", n);
+    HIPCHK(hipSetDevice(c->device));
+    layout L;
+    make_layout(d, &mi, &L);
+    plom_filter *f = (plom_filter *)calloc(1, sizeof(plom_filter));
+    if (!f) return set_err(PLOM_ERR_NOMEM, "host allocation failed");
+    f->desc = *d;
+    f->info = mi;
+    f->variant = (int)(d->noises_off & 3u);
+    f->dev_bytes = L.total;
+    hipError_t e = hipMalloc((void **)&f->dev, L.total);
+    if (e != hipSuccess) {
+        free(f);
+        return set_err(PLOM_ERR_NOMEM, "hipMalloc(%zu): %s", L.total, hipGetErrorString(e));
+    }
+    char *b = f->dev;
+    plom_dfilter *h = &f->h;
+    const int Jp = (d->J + TILE - 1) / TILE * TILE;
+    h->J = d->J; h->Jp = Jp; h->T = Jp / TILE; h->N = d->N; h->S = mi.size_proj; h->n_ts = mi.n_ts;
+    h->flags = (int32_t)d->flags; h->step = 0;
+    h->seed = d->seed; h->stream = d->stream; h->prec = (uint32_t)d->precision;
+    h->dt = d->dt;
+    h->sqdt = plom_sqrt(d->dt);
+    h->like_min = d->like_min;
+    h->all_fail_llike = plom_log(d->like_min) * (double)mi.n_ts;
+    h->X[0] = b + L.off_X0; h->X[1] = b + L.off_X1;
+    h->x0 = b + L.off_x0;
+    h->pv = (const double *)(b + L.off_pv); h->pvf = (const float *)(b + L.off_pvf);
+    h->times = (const double *)(b + L.off_times); h->data = (const double *)(b + L.off_data);
+    h->nonan = (const int32_t *)(b + L.off_nonan);
+    h->logw = (double *)(b + L.off_logw); h->tmax = (double *)(b + L.off_tmax);
+    h->tsum = (double *)(b + L.off_tsum); h->tsum2 = (double *)(b + L.off_tsum2);
+    h->cum = (uint64_t *)(b + L.off_cum); h->ttot = (uint64_t *)(b + L.off_ttot); h->tincl = (uint64_t *)(b + L.off_tincl);
+    h->cnt = (uint32_t *)(b + L.off_cnt); h->scal = (double *)(b + L.off_scal);
+    h->n_all_fail = (int32_t *)(b + L.off_naf);
+    h->llike_n = (double *)(b + L.off_llike_n); h->ess_n = (double *)(b + L.off_ess_n);
+    h->rflag = (int32_t *)(b + L.off_rflag);
+    h->ancestry = (d->flags & PLOM_FLAG_STORE_ANCESTRY) ? (int32_t *)(b + L.off_anc) : nullptr;
+    f->df = (plom_dfilter *)(b + L.off_df);
+
+    /* host-side staging of the inputs */
+    const int S = mi.size_proj;
+    std::vector<int32_t> nonan((size_t)d->N);
+    for (int n = 0; n < d->N; n++) {
+        int k = 0;
+        for (int ts = 0; ts < mi.n_ts; ts++) k += !(in->data[(size_t)n * mi.n_ts + ts] != in->data[(size_t)n * mi.n_ts + ts]);
+        nonan[(size_t)n] = k;
+    }
+    std::vector<float> pvf((size_t)mi.pv_size);
+    for (int i = 0; i < mi.pv_size; i++) pvf[(size_t)i] = (float)in->pv[i];
+    std::vector<char> x0b((size_t)S * 8);
+    if (d->precision == PLOM_FP32) {
+        for (int k = 0; k < S; k++) ((float *)x0b.data())[k] = (float)in->x0[k];
+    } else {
+        memcpy(x0b.data(), in->x0, (size_t)S * 8);
+    }
+    HIPCHK(hipMemcpyAsync(b + L.off_times, in->times, (size_t)(d->N + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    if (d->N > 0) {
+        HIPCHK(hipMemcpyAsync(b + L.off_data, in->data, (size_t)d->N * mi.n_ts * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(b + L.off_nonan, nonan.data(), (size_t)d->N * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipMemcpyAsync(b + L.off_pv, in->pv, (size_t)mi.pv_size * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b + L.off_pvf, pvf.data(), (size_t)mi.pv_size * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b + L.off_x0, x0b.data(), (size_t)S * 8, hipMemcpyHostToDevice, c->stream));
+    /* padding lanes of the last tile read log-weight -inf and zero increments */
+    HIPCHK(hipMemsetAsync(b + L.off_logw, 0, (size_t)Jp * 8, c->stream));
+    rc = filter_reset_async(c, f);
+    if (rc != PLOM_OK) { (void)hipFree(f->dev); free(f); return rc; }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *out = f;
+    return PLOM_OK;
+}
+
+extern "C" int plom_filter_reset(plom_ctx *c, plom_filter *f)
+{
+    if (!c || !f) return set_err(PLOM_ERR_ARG, "null argument");
+    f->has_run = 0;
+    return filter_reset_async(c, f);
+}
+
+extern "C" int plom_filter_destroy(plom_ctx *c, plom_filter *f)
+{
+    if (!f) return PLOM_OK;
+    if (c) HIPCHK(hipStreamSynchronize(c->stream));
+    if (f->graph) HIPCHK(hipGraphExecDestroy(f->graph));
+    HIPCHK(hipFree(f->dev));
+    free(f);
+    return PLOM_OK;
+}
+
+static void enqueue_final(plom_ctx *c, plom_filter *f, const int2 *tiles, plom_dfilter *df, int nblocks)
+{
+    if (f->desc.precision == PLOM_FP32)
+        hipLaunchKernelGGL((k_final<float>), dim3(nblocks), dim3(TILE), 0, c->stream, df, tiles, f->info.size_proj);
+    else
+        hipLaunchKernelGGL((k_final<double>), dim3(nblocks), dim3(TILE), 0, c->stream, df, tiles, f->info.size_proj);
+}
+
+static int needs_final(const plom_filter *f)
+{
+    return f->desc.N > 0 && (f->desc.flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_FINAL_STATE));
+}
+
+extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
+{
+    if (!c || !f) return set_err(PLOM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    if (f->has_run) {
+        int rc = filter_reset_async(c, f);
+        if (rc != PLOM_OK) return rc;
+    }
+    const model_entry *me = &g_models[f->desc.model_id];
+    const int T = f->h.T;
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    for (int n = 0; n < f->desc.N; n++) {
+        me->launch_prop(f->desc.precision, f->variant, T, c->stream, f->df, nullptr);
+        hipLaunchKernelGGL(k_scan, dim3(T), dim3(TILE), 0, c->stream, f->df, (const int2 *)nullptr);
+    }
+    if (needs_final(f)) enqueue_final(c, f, nullptr, f->df, T);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    f->has_run = 1;
+    return PLOM_OK;
+}
+
+extern "C" int plom_filter_read(plom_ctx *c, plom_filter *f, plom_filter_outputs *o)
+{
+    if (!c || !f || !o) return set_err(PLOM_ERR_ARG, "null argument");
+    if (!f->has_run) return set_err(PLOM_ERR_STATE, "filter has not run");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->last_total_ms = ms;
+    const plom_dfilter *h = &f->h;
+    const size_t N = (size_t)f->desc.N, J = (size_t)f->desc.J;
+    double scal[4];
+    int32_t naf = 0;
+    HIPCHK(hipMemcpy(scal, h->scal, sizeof scal, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&naf, h->n_all_fail, sizeof naf, hipMemcpyDeviceToHost));
+    o->loglik = scal[3];
+    o->n_all_fail = naf;
+    if (o->loglik_n && N) HIPCHK(hipMemcpy(o->loglik_n, h->llike_n, N * 8, hipMemcpyDeviceToHost));
+    if (o->ess_n && N) HIPCHK(hipMemcpy(o->ess_n, h->ess_n, N * 8, hipMemcpyDeviceToHost));
+    if (o->ancestry && N) {
+        if (!h->ancestry) return set_err(PLOM_ERR_STATE, "ancestry requested but PLOM_FLAG_STORE_ANCESTRY not set");
+        HIPCHK(hipMemcpy(o->ancestry, h->ancestry, N * J * 4, hipMemcpyDeviceToHost));
+    }
+    if (o->final_logw) HIPCHK(hipMemcpy(o->final_logw, h->logw, J * 8, hipMemcpyDeviceToHost));
+    if (o->final_state) {
+        if (!(f->desc.flags & PLOM_FLAG_FINAL_STATE) || N == 0)
+            return set_err(PLOM_ERR_STATE, "final state requested but PLOM_FLAG_FINAL_STATE not set (or N == 0)");
+        const int S = f->info.size_proj;
+        const void *src = h->X[(N + 1) & 1];
+        if (f->desc.precision == PLOM_FP32) {
+            std::vector<float> tmp(J);
+            for (int k = 0; k < S; k++) {
+                HIPCHK(hipMemcpy(tmp.data(), (const float *)src + (size_t)k * h->Jp, J * 4, hipMemcpyDeviceToHost));
+                for (size_t j = 0; j < J; j++) o->final_state[(size_t)k * J + j] = tmp[j];
+            }
+        } else {
+            for (int k = 0; k < S; k++)
+                HIPCHK(hipMemcpy(o->final_state + (size_t)k * J, (const double *)src + (size_t)k * h->Jp, J * 8, hipMemcpyDeviceToHost));
+        }
+    }
+    return PLOM_OK;
+}
+
+extern "C" int plom_smc_run(plom_ctx *c, const plom_filter_desc *d, const plom_filter_inputs *in, plom_filter_outputs *o)
+{
+    plom_filter *f = nullptr;
+    int rc = plom_filter_create(c, d, in, &f);
+    if (rc != PLOM_OK) return rc;
+    rc = plom_filter_run(c, f);
+    if (rc == PLOM_OK) rc = plom_filter_read(c, f, o);
+    int rc2 = plom_filter_destroy(c, f);
+    return rc != PLOM_OK ? rc : rc2;
+}
+
+/* ------------------------------------------------------------------ batches */
+extern "C" int plom_batch_run(plom_ctx *c, int nf, plom_filter **fs, int use_graph)
+{
+    (void)use_graph;
+    if (!c || nf < 0 || (nf > 0 && !fs)) return set_err(PLOM_ERR_ARG, "bad batch arguments");
+    if (nf == 0) return PLOM_OK;
+    HIPCHK(hipSetDevice(c->device));
+    for (int i = 0; i < nf; i++) {
+        if (!fs[i]) return set_err(PLOM_ERR_ARG, "null filter %d", i);
+        if (fs[i]->has_run) {
+            int rc = filter_reset_async(c, fs[i]);
+            if (rc != PLOM_OK) return rc;
+        }
+    }
+    /* device array of descriptors: copies of each filter's descriptor, in N-descending order */
+    std::vector<int> order((size_t)nf);
+    for (int i = 0; i < nf; i++) order[(size_t)i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return fs[a]->desc.N > fs[b]->desc.N; });
+    /* groups by (model, precision, variant) */
+    struct group { int model, prec, variant; std::vector<int2> tiles; std::vector<int> prefix; };
+    std::vector<group> groups;
+    std::vector<int2> all_tiles;
+    std::vector<int> all_prefix;
+    int maxN = fs[order[0]]->desc.N;
+    for (int r = 0; r < nf; r++) {
+        plom_filter *f = fs[order[(size_t)r]];
+        group *g = nullptr;
+        for (auto &gg : groups)
+            if (gg.model == f->desc.model_id && gg.prec == f->desc.precision && gg.variant == f->variant) g = &gg;
+        if (!g) { groups.push_back(group{f->desc.model_id, f->desc.precision, f->variant, {}, {}}); g = &groups.back(); }
+        for (int t = 0; t < f->h.T; t++) { g->tiles.push_back(make_int2(r, t)); all_tiles.push_back(make_int2(r, t)); }
+    }
+    /* active tile prefix per interval s: filters are sorted by N, so active ones are a prefix */
+    auto prefix_len = [&](const std::vector<int2> &tl, int s) {
+        int k = 0;
+        while (k < (int)tl.size() && fs[order[(size_t)tl[(size_t)k].x]]->desc.N > s) k++;
+        return k;
+    };
+    /* one device buffer: descriptors, then tile maps */
+    size_t bytes = align256((size_t)nf * sizeof(plom_dfilter)) + align256(all_tiles.size() * sizeof(int2));
+    for (auto &g : groups) bytes += align256(g.tiles.size() * sizeof(int2));
+    char *buf = nullptr;
+    HIPCHK(hipMallocAsync((void **)&buf, bytes, c->stream));
+    std::vector<plom_dfilter> hd((size_t)nf);
+    for (int r = 0; r < nf; r++) hd[(size_t)r] = fs[order[(size_t)r]]->h;
+    plom_dfilter *ddf = (plom_dfilter *)buf;
+    size_t o = align256((size_t)nf * sizeof(plom_dfilter));
+    int2 *d_all = (int2 *)(buf + o);
+    o += align256(all_tiles.size() * sizeof(int2));
+    std::vector<int2 *> d_groups;
+    for (auto &g : groups) { d_groups.push_back((int2 *)(buf + o)); o += align256(g.tiles.size() * sizeof(int2)); }
+    HIPCHK(hipMemcpyAsync(ddf, hd.data(), (size_t)nf * sizeof(plom_dfilter), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_all, all_tiles.data(), all_tiles.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    for (size_t gi = 0; gi < groups.size(); gi++)
+        HIPCHK(hipMemcpyAsync(d_groups[gi], groups[gi].tiles.data(), groups[gi].tiles.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    for (int s = 0; s < maxN; s++) {
+        for (size_t gi = 0; gi < groups.size(); gi++) {
+            const int nb = prefix_len(groups[gi].tiles, s);
+            if (nb > 0) g_models[groups[gi].model].launch_prop(groups[gi].prec, groups[gi].variant, nb, c->stream, ddf, d_groups[gi]);
+        }
+        const int nb = prefix_len(all_tiles, s);
+        hipLaunchKernelGGL(k_scan, dim3(nb), dim3(TILE), 0, c->stream, ddf, (const int2 *)d_all);
+    }
+    /* final resampling for the filters that keep ancestry or the final cloud */
+    for (int r = 0; r < nf; r++) {
+        plom_filter *f = fs[order[(size_t)r]];
+        if (needs_final(f)) {
+            /* tiles of filter r are contiguous in all_tiles */
+            size_t first = 0;
+            while (all_tiles[first].x != r) first++;
+            enqueue_final(c, f, d_all + first, ddf, f->h.T);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    /* bring each filter's device descriptor (step counter) in line with the batch copy */
+    for (int r = 0; r < nf; r++)
+        HIPCHK(hipMemcpyAsync(fs[order[(size_t)r]]->df, ddf + r, sizeof(plom_dfilter), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipFreeAsync(buf, c->stream));
+    for (int i = 0; i < nf; i++) fs[i]->has_run = 1;
+    return PLOM_OK;
+}
+
+extern "C" int plom_last_timing(plom_ctx *c, double *total_ms, double *prop_ms, double *scan_ms)
+{
+    if (!c) return set_err(PLOM_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->last_total_ms = ms;
+    if (total_ms) *total_ms = ms;
+    if (prop_ms) *prop_ms = -1.0;
+    if (scan_ms) *scan_ms = -1.0;
+    return PLOM_OK;
+}
+
+extern "C" int plom_selftest_math(plom_ctx *c, int n, const double *in, double *out)
+{
+    if (!c || n < 0 || (n > 0 && (!in || !out))) return set_err(PLOM_ERR_ARG, "bad arguments");
+    if (n == 0) return PLOM_OK;
+    HIPCHK(hipSetDevice(c->device));
+    double *d = nullptr;
+    HIPCHK(hipMalloc((void **)&d, (size_t)n * 7 * 8));
+    HIPCHK(hipMemcpy(d, in, (size_t)n * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_selftest_math, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, d, d + n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(out, d + n, (size_t)n * 6 * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipFree(d));
+    return PLOM_OK;
+}
+
+extern "C" int plom_selftest_normals(plom_ctx *c, uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int n, double *out)
+{
+    if (!c || n < 0 || (n > 0 && !out)) return set_err(PLOM_ERR_ARG, "bad arguments");
+    if (n == 0) return PLOM_OK;
+    HIPCHK(hipSetDevice(c->device));
+    double *d = nullptr;
+    HIPCHK(hipMalloc((void **)&d, (size_t)n * 8));
+    hipLaunchKernelGGL(k_selftest_normals, dim3(1), dim3(64), 0, c->stream, seed, stream, n0, j, n, d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(out, d, (size_t)n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipFree(d));
+    return PLOM_OK;
+}

@@ -1,0 +1,329 @@
+"""ctypes binding of libplom_smc.so (include/plom_smc.h) and the Python filter API.
+
+The Python side mirrors the reference's driver (model_builder/C/smc/main.c:243-278):
+natural parameters per population / series (back_transform_theta2par + routers,
+C/core/transform.c:462-514), initial state = proportions x population size
+(linearize_and_repeat, prop2Xpop_size, C/core/prediction_util.c:82-115), replicated to
+all J particles (replicate_J_p_X_0, C/core/smc.c:167-176), then the filter.
+
+The library must be built (iterated-inference-pipeline_amd/build.py); there is no
+fallback: loading fails loudly if libplom_smc.so is missing.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+from . import PKG_DIR
+from .codegen import load_model, BUILTIN_MODELS, context_data, load_json
+
+PLOM_FP64, PLOM_FP32 = 0, 1
+NO_DEM_STO, NO_ENV_STO, NO_DRIFT = 1, 2, 4
+FLAG_NO_FILTER, FLAG_STORE_ANCESTRY, FLAG_FINAL_STATE = 1, 2, 4
+MAX_J = 1 << 21
+
+
+class ModelInfo(C.Structure):
+    _fields_ = [("size_proj", C.c_int32), ("n_ts", C.c_int32), ("n_cac", C.c_int32),
+                ("n_par_sv", C.c_int32), ("inc_offset", C.c_int32), ("n_inc", C.c_int32),
+                ("pv_size", C.c_int32), ("n_dem", C.c_int32), ("n_env", C.c_int32)]
+
+
+class FilterDesc(C.Structure):
+    _fields_ = [("model_id", C.c_int32), ("J", C.c_int32), ("N", C.c_int32), ("precision", C.c_int32),
+                ("noises_off", C.c_uint32), ("flags", C.c_uint32), ("seed", C.c_uint32),
+                ("stream", C.c_uint32), ("dt", C.c_double), ("like_min", C.c_double)]
+
+
+class FilterInputs(C.Structure):
+    _fields_ = [("times", C.POINTER(C.c_double)), ("data", C.POINTER(C.c_double)),
+                ("pv", C.POINTER(C.c_double)), ("x0", C.POINTER(C.c_double))]
+
+
+class FilterOutputs(C.Structure):
+    _fields_ = [("loglik", C.c_double), ("n_all_fail", C.c_int32),
+                ("loglik_n", C.POINTER(C.c_double)), ("ess_n", C.POINTER(C.c_double)),
+                ("ancestry", C.POINTER(C.c_int32)), ("final_state", C.POINTER(C.c_double)),
+                ("final_logw", C.POINTER(C.c_double))]
+
+
+EXPORTS = [
+    "plom_last_error", "plom_ctx_create", "plom_ctx_destroy", "plom_model_count", "plom_model_find",
+    "plom_model_name", "plom_model_info_get", "plom_smc_run", "plom_filter_create", "plom_filter_reset",
+    "plom_filter_run", "plom_filter_read", "plom_filter_destroy", "plom_filter_device_bytes",
+    "plom_batch_run", "plom_sync", "plom_last_timing", "plom_selftest_math", "plom_selftest_normals",
+]
+
+_lib = None
+
+
+class PlomError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libplom_smc.so (built in-tree). Raises if it is missing: no CPU fallback."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = os.path.join(PKG_DIR, "csrc", "libplom_smc.so")
+    if not os.path.exists(path):
+        raise PlomError("libplom_smc.so is not built (%s); run iterated-inference-pipeline_amd/build.py" % path)
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    L.plom_last_error.restype = C.c_char_p
+    L.plom_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+    L.plom_ctx_destroy.argtypes = [C.c_void_p]
+    L.plom_model_find.argtypes = [C.c_char_p]
+    L.plom_model_name.argtypes = [C.c_int, C.c_char_p, C.c_int]
+    L.plom_model_info_get.argtypes = [C.c_int, C.POINTER(ModelInfo)]
+    L.plom_smc_run.argtypes = [C.c_void_p, C.POINTER(FilterDesc), C.POINTER(FilterInputs), C.POINTER(FilterOutputs)]
+    L.plom_filter_create.argtypes = [C.c_void_p, C.POINTER(FilterDesc), C.POINTER(FilterInputs), C.POINTER(C.c_void_p)]
+    L.plom_filter_reset.argtypes = [C.c_void_p, C.c_void_p]
+    L.plom_filter_run.argtypes = [C.c_void_p, C.c_void_p]
+    L.plom_filter_read.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(FilterOutputs)]
+    L.plom_filter_destroy.argtypes = [C.c_void_p, C.c_void_p]
+    L.plom_filter_device_bytes.argtypes = [C.POINTER(FilterDesc)]
+    L.plom_filter_device_bytes.restype = C.c_int64
+    L.plom_batch_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.c_int]
+    L.plom_sync.argtypes = [C.c_void_p]
+    L.plom_last_timing.argtypes = [C.c_void_p, dp, dp, dp]
+    L.plom_selftest_math.argtypes = [C.c_void_p, C.c_int, dp, dp]
+    L.plom_selftest_normals.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise PlomError("plom error %d: %s" % (rc, lib().plom_last_error().decode(errors="replace")))
+
+
+def dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32)) if a is not None else None
+
+
+class Context:
+    """A device context: one HIP stream on one GPU (plom_ctx_create)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        check(lib().plom_ctx_create(device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            check(lib().plom_ctx_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        check(lib().plom_sync(self.h))
+
+    def last_timing_ms(self):
+        t = C.c_double()
+        check(lib().plom_last_timing(self.h, C.byref(t), None, None))
+        return t.value
+
+
+# ---------------------------------------------------------------------- models
+class Model:
+    """A built-in model: generated code in the library + its description (codegen)."""
+
+    _cache = {}
+
+    def __init__(self, name):
+        self.name = name
+        self.desc = load_model(name)
+        self.id = BUILTIN_MODELS.index(name)
+        self.S = self.desc.size_proj
+        self.n_ts = self.desc.N_TS
+        self.n_cac = self.desc.N_CAC
+        self.layout = self.desc.layout
+        self.context = load_json(os.path.join(PKG_DIR, "examples", name, "context.json"))
+
+    @classmethod
+    def get(cls, name):
+        if name not in cls._cache:
+            cls._cache[name] = cls(name)
+        return cls._cache[name]
+
+    def fixed_values(self):
+        """Constant covariate values per population (first point of each metadata series)."""
+        out = {}
+        for meta in self.context.get("metadata", []):
+            vals = []
+            for cac in self.desc.cac_id:
+                series = meta["source"][cac]["value"]
+                ys = [v[1] for v in series if v[1] is not None]
+                if len(set(ys)) > 1:
+                    raise ValueError("time-varying covariate %s is not supported yet" % meta["id"])
+                vals.append(float(ys[0]))
+            out[meta["id"]] = vals
+        return out
+
+    def pack(self, params):
+        """Natural parameters -> flat vector pv in the generated layout. ``params`` maps
+        each process / observation parameter to a scalar or a per-population (per-series)
+        list."""
+        lay, m = self.layout, self.desc
+        pv = np.zeros(lay["size"], dtype=np.float64)
+
+        def put(off, n, v):
+            v = np.broadcast_to(np.asarray(v, dtype=np.float64), (n,))
+            pv[off:off + n] = v
+
+        for p, off in lay["proc"].items():
+            put(off, m.N_CAC, params[p])
+        for p, off in lay["obs"].items():
+            put(off, m.N_TS, params[p])
+        fixed = self.fixed_values()
+        for p, off in lay["fixed"].items():
+            put(off, m.N_CAC, params.get(p, fixed[p]))
+        for p, off in lay["fixed_obs"].items():
+            # observation-level covariates follow the populations of each series
+            ts_c = m.ts_cacs()
+            put(off, m.N_TS, [fixed[p][c[0]] for c in ts_c])
+        if lay["coupling"] is not None:
+            pv[lay["coupling"]:lay["coupling"] + m.N_CAC ** 2] = np.asarray(m.coupling, dtype=np.float64).ravel()
+        return pv
+
+    def x0(self, props, pop_size=None):
+        """Initial state from proportions (par_sv order) times the population size."""
+        m = self.desc
+        N = np.asarray(pop_size if pop_size is not None else self.fixed_values()["N"], dtype=np.float64)
+        x = np.zeros(self.S, dtype=np.float64)
+        for i, s in enumerate(m.par_sv):
+            v = np.broadcast_to(np.asarray(props[s], dtype=np.float64), (m.N_CAC,))
+            x[i * m.N_CAC:(i + 1) * m.N_CAC] = v * N
+        return x
+
+    def dataset(self, N=None):
+        """(times[N+1], data[N, n_ts]) from the context (null -> NaN)."""
+        dates, times, data, _ = context_data(self.desc, self.context)
+        if N is None:
+            N = len(data)
+        if N > len(data):
+            raise ValueError("dataset of %s has %d intervals, %d requested" % (self.name, len(data), N))
+        d = np.array([[np.nan if v is None else float(v) for v in row] for row in data[:N]], dtype=np.float64)
+        d = d.reshape(N, self.n_ts)
+        return np.asarray(times[:N + 1], dtype=np.float64), d
+
+
+def round_dt(DT):
+    """dt = 1/round(1/DT) (C/core/build.c:1298-1304); DT <= 0 means 0.25 day."""
+    if DT <= 0:
+        DT = 0.25 / 365.0 * 365.0
+    return 1.0 / float(round(1.0 / DT))
+
+
+class FilterResult:
+    def __init__(self, loglik, n_all_fail, loglik_n, ess_n, ancestry=None, final_state=None, final_logw=None):
+        self.loglik = loglik
+        self.n_all_fail = n_all_fail
+        self.loglik_n = loglik_n
+        self.ess_n = ess_n
+        self.ancestry = ancestry
+        self.final_state = final_state
+        self.final_logw = final_logw
+
+
+def _desc(model, J, N, precision, noises_off, flags, seed, stream, dt, like_min):
+    d = FilterDesc()
+    d.model_id = model.id
+    d.J, d.N = int(J), int(N)
+    d.precision = PLOM_FP32 if precision in ("fp32", PLOM_FP32) else PLOM_FP64
+    d.noises_off, d.flags = int(noises_off), int(flags)
+    d.seed, d.stream = int(seed) & 0xffffffff, int(stream) & 0xffffffff
+    d.dt, d.like_min = float(dt), float(like_min)
+    return d
+
+
+class Filter:
+    """A device-resident filter (plom_filter_create / run / read)."""
+
+    def __init__(self, ctx, model, J, times, data, pv, x0, precision="fp64", noises_off=0, flags=0,
+                 seed=1, stream=0, dt=0.25, like_min=1e-17):
+        self.ctx, self.model = ctx, model
+        self.J, self.N = int(J), int(len(times) - 1)
+        self.flags = int(flags)
+        self.times = np.ascontiguousarray(times, dtype=np.float64)
+        self.data = np.ascontiguousarray(data, dtype=np.float64).reshape(self.N, model.n_ts)
+        self.pv = np.ascontiguousarray(pv, dtype=np.float64)
+        self.x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        if self.pv.size != model.layout["size"] or self.x0.size != model.S:
+            raise ValueError("pv / x0 size mismatch for model %s" % model.name)
+        self.desc = _desc(model, J, self.N, precision, noises_off, flags, seed, stream, dt, like_min)
+        inp = FilterInputs(dptr(self.times), dptr(self.data), dptr(self.pv), dptr(self.x0))
+        self.h = C.c_void_p()
+        check(lib().plom_filter_create(ctx.h, C.byref(self.desc), C.byref(inp), C.byref(self.h)))
+
+    def device_bytes(self):
+        return lib().plom_filter_device_bytes(C.byref(self.desc))
+
+    def run(self):
+        check(lib().plom_filter_run(self.ctx.h, self.h))
+        return self
+
+    def read(self, ancestry=False, final_state=False, final_logw=False):
+        N, J = self.N, self.J
+        ll_n = np.zeros(N)
+        ess = np.zeros(N)
+        anc = np.zeros((N, J), dtype=np.int32) if ancestry else None
+        fs = np.zeros((self.model.S, J)) if final_state else None
+        fw = np.zeros(J) if final_logw else None
+        out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), iptr(anc), dptr(fs), dptr(fw))
+        check(lib().plom_filter_read(self.ctx.h, self.h, C.byref(out)))
+        return FilterResult(out.loglik, out.n_all_fail, ll_n, ess, anc, fs, fw)
+
+    def close(self):
+        if self.h:
+            check(lib().plom_filter_destroy(self.ctx.h, self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_run(ctx, filters, use_graph=False):
+    arr = (C.c_void_p * len(filters))(*[f.h for f in filters])
+    check(lib().plom_batch_run(ctx.h, len(filters), arr, 1 if use_graph else 0))
+
+
+def run(ctx, model, J, times, data, pv, x0, ancestry=False, final_state=False, **kw):
+    flags = kw.pop("flags", 0)
+    if ancestry:
+        flags |= FLAG_STORE_ANCESTRY
+    if final_state:
+        flags |= FLAG_FINAL_STATE
+    f = Filter(ctx, model, J, times, data, pv, x0, flags=flags, **kw)
+    try:
+        f.run()
+        return f.read(ancestry=ancestry, final_state=final_state, final_logw=True)
+    finally:
+        f.close()
+
+
+def selftest_math(ctx, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros((x.size, 6))
+    check(lib().plom_selftest_math(ctx.h, x.size, dptr(x), dptr(out)))
+    return out
+
+
+def selftest_normals(ctx, seed, stream, n0, j, n):
+    out = np.zeros(n)
+    check(lib().plom_selftest_normals(ctx.h, seed, stream, n0, j, n, dptr(out)))
+    return out

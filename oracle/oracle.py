@@ -1,0 +1,126 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/liboracle.so). TEST INFRASTRUCTURE:
+imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as
+the checker of the HIP filter, never as the thing measured or shipped."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+CANONICAL, REFERENCE = 0, 1
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int32)
+        L.oracle_smc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                 C.c_double, C.c_double, dp, dp, dp, dp, C.c_int, C.c_int,
+                                 dp, ip, dp, dp, ip, dp, dp, dp]
+        L.oracle_trajectory.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.c_double, dp, dp, dp, dp]
+        L.oracle_obs_loglik.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, dp, ip]
+        L.oracle_math.argtypes = [C.c_int, dp, dp]
+        L.oracle_normals.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
+        L.oracle_philox.argtypes = [C.POINTER(C.c_uint32)] * 3
+        L.oracle_xoshiro.argtypes = [C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_uint32)]
+        L.oracle_model_name.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32)) if a is not None else None
+
+
+def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=0, dt=0.25,
+        like_min=1e-17, mode=CANONICAL, nthreads=1, ancestry=False, final_state=False, logw=False):
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    N = len(times) - 1
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    pv = np.ascontiguousarray(pv, dtype=np.float64)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    S = lib().oracle_model_size_proj(model_id)
+    ll = C.c_double()
+    naf = C.c_int32()
+    ll_n = np.zeros(N)
+    ess = np.zeros(N)
+    anc = np.zeros((N, J), dtype=np.int32) if ancestry else None
+    fs = np.zeros((S, J)) if final_state else None
+    lw = np.zeros((N, J)) if logw else None
+    el = C.c_double()
+    rc = lib().oracle_smc(model_id, J, N, noises_off, flags, seed, stream, dt, like_min, _dp(times), _dp(data),
+                          _dp(pv), _dp(x0), mode, nthreads, C.byref(ll), C.byref(naf), _dp(ll_n), _dp(ess),
+                          _ip(anc), _dp(fs), _dp(lw), C.byref(el))
+    if rc != 0:
+        raise RuntimeError("oracle_smc failed: %d" % rc)
+    return {"loglik": ll.value, "n_all_fail": naf.value, "loglik_n": ll_n, "ess_n": ess,
+            "ancestry": anc, "final_state": fs, "logw": lw, "elapsed_s": el.value}
+
+
+def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=0.25):
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    N = len(times) - 1
+    S = lib().oracle_model_size_proj(model_id)
+    out = np.zeros((N + 1, S))
+    rc = lib().oracle_trajectory(model_id, N, noises_off, seed, stream, j, dt, _dp(times),
+                                 _dp(np.ascontiguousarray(pv, dtype=np.float64)),
+                                 _dp(np.ascontiguousarray(x0, dtype=np.float64)), _dp(out))
+    if rc != 0:
+        raise RuntimeError("oracle_trajectory failed")
+    return out
+
+
+def obs_loglik(model_id, x, y, pv, t=0.0, like_min=1e-17):
+    n_ts = lib().oracle_model_n_ts(model_id)
+    obs = np.zeros(max(n_ts, 1))
+    ll = C.c_double()
+    nf = C.c_int32()
+    lib().oracle_obs_loglik(model_id, _dp(np.ascontiguousarray(x, dtype=np.float64)),
+                            _dp(np.ascontiguousarray(y, dtype=np.float64)),
+                            _dp(np.ascontiguousarray(pv, dtype=np.float64)), t, like_min, _dp(obs),
+                            C.byref(ll), C.byref(nf))
+    return obs, ll.value, nf.value
+
+
+def math(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros((x.size, 6))
+    lib().oracle_math(x.size, _dp(x), _dp(out))
+    return out
+
+
+def normals(seed, stream, n0, j, n):
+    out = np.zeros(n)
+    lib().oracle_normals(seed, stream, n0, j, n, _dp(out))
+    return out
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().oracle_philox(c, k, o)
+    return list(o)
+
+
+def xoshiro(state, n):
+    s = (C.c_uint32 * 4)(*state)
+    o = (C.c_uint32 * n)()
+    lib().oracle_xoshiro(s, n, o)
+    return list(o), list(s)

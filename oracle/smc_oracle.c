@@ -1,0 +1,385 @@
+/*
+ * smc_oracle.c -- CPU restatement of the PLOM SMC filter: TEST INFRASTRUCTURE ONLY.
+ *
+ * This file is the checker of the HIP filter, never the product: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * It restates, as plain sequential C:
+ *   run_SMC_zmq_inproc      model_builder/C/core/smc.c:379-467 (driver order: propagate
+ *                           every particle, weight, select, gather)
+ *   worker j-loop           model_builder/C/core/worker_inproc.c:71-80 (reset_inc ->
+ *                           f_pred -> proj2obs -> weight)
+ *   weight                  model_builder/C/core/smc.c:27-83
+ *   systematic_sampling     model_builder/C/core/smc.c:90-112
+ *   resample_X              model_builder/C/core/smc.c:142-161
+ * Two arithmetic modes:
+ *   ORACLE_CANONICAL (0)    the arithmetic the GPU filter defines (log-domain weights,
+ *                           recursive pairwise tree sums over tiles of 256, fixed-point
+ *                           2^62 weights and an exact integer systematic resampling), each
+ *                           step written here independently of the kernels: sequential
+ *                           recursion for the trees, a running integer prefix and the
+ *                           reference's two-pointer walk for the selection;
+ *   ORACLE_REFERENCE (1)    the reference's own arithmetic: raw likelihoods exp(l), the
+ *                           failure test w <= LIKE_MIN^N_TS, sequential double sums,
+ *                           log(sum/J), weight_cum += w_i and ran += 1/J (smc.c:44-110),
+ *                           with the index clamped at J-1 (smc.c:105 has no bound check).
+ * Propagation and observation use the generated model code and the shared math/RNG
+ * headers (csrc/plom_math.h, csrc/plom_rng.h), which are pinned separately (Philox
+ * known-answer vectors, libm/mpmath accuracy, closed-form Euler tests).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "plom_math.h"
+#include "plom_rng.h"
+#include "plom_models.h"
+
+#define ORACLE_CANONICAL 0
+#define ORACLE_REFERENCE 1
+#define TILE 256
+#define FLAG_NO_FILTER 1u
+
+typedef unsigned __int128 u128;
+
+typedef void (*interval_fn)(double *, const double *, plom_nstream64 *, double, double, double, double, unsigned int);
+typedef void (*proj2obs_fn)(const double *, const double *, double *);
+typedef double (*loglik_fn)(const double *, const double *, const double *, double, double, int *);
+
+typedef struct {
+    const char *name;
+    int S, n_ts;
+    interval_fn interval;
+    proj2obs_fn proj2obs;
+    loglik_fn loglik;
+} oracle_model;
+
+static const oracle_model MODELS[] = {
+    {"sir", PLOM_SIR_SIZE_PROJ, PLOM_SIR_N_TS, plom_sir_interval_f64, plom_sir_proj2obs_f64, plom_sir_loglik},
+    {"sirs4", PLOM_SIRS4_SIZE_PROJ, PLOM_SIRS4_N_TS, plom_sirs4_interval_f64, plom_sirs4_proj2obs_f64, plom_sirs4_loglik},
+    {"seir10", PLOM_SEIR10_SIZE_PROJ, PLOM_SEIR10_N_TS, plom_seir10_interval_f64, plom_seir10_proj2obs_f64, plom_seir10_loglik},
+};
+#define N_MODELS ((int)(sizeof MODELS / sizeof MODELS[0]))
+
+int oracle_model_count(void) { return N_MODELS; }
+const char *oracle_model_name(int m) { return (m >= 0 && m < N_MODELS) ? MODELS[m].name : ""; }
+int oracle_model_size_proj(int m) { return MODELS[m].S; }
+int oracle_model_n_ts(int m) { return MODELS[m].n_ts; }
+
+/* ------------------------------------------------------------------ canonical trees */
+/* recursive contiguous pairwise sum of a[0..n), n a power of two */
+static double tree_sum(const double *a, int n)
+{
+    if (n == 1) return a[0];
+    return tree_sum(a, n / 2) + tree_sum(a + n / 2, n / 2);
+}
+
+static int next_pow2(int n)
+{
+    int p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+/* ------------------------------------------------------------------ propagation */
+typedef struct {
+    const oracle_model *M;
+    double *X;          /* [J][S] */
+    double *logw;       /* [J] */
+    const double *pv, *times, *data;
+    int J, n, nonan, filter, n_ts;
+    uint32_t seed, stream, noises_off;
+    double dt, sqdt, like_min;
+    int j0, j1, mode;
+} prop_job;
+
+static void propagate_range(prop_job *p)
+{
+    const oracle_model *M = p->M;
+    const int S = M->S;
+    double obs[64];
+    for (int j = p->j0; j < p->j1; j++) {
+        double *x = p->X + (size_t)j * S;
+        plom_nstream64 ns;
+        plom_nstream64_init(&ns, p->seed, p->stream, (uint32_t)p->n, (uint32_t)j);
+        M->interval(x, p->pv, &ns, p->times[p->n], p->times[p->n + 1], p->dt, p->sqdt, p->noises_off);
+        if (p->filter && p->nonan) {
+            int nf = 0;
+            M->proj2obs(x, p->pv, obs);
+            double l = M->loglik(obs, p->data + (size_t)p->n * p->n_ts, p->pv, p->times[p->n + 1], p->like_min, &nf);
+            if (p->mode == ORACLE_CANONICAL && nf == p->n_ts) l = -INFINITY;
+            p->logw[j] = l;
+        }
+    }
+}
+
+static void *prop_thread(void *arg)
+{
+    propagate_range((prop_job *)arg);
+    return NULL;
+}
+
+static void propagate_all(prop_job *base, int nthreads)
+{
+    if (nthreads <= 1 || base->J < 2 * nthreads) {
+        base->j0 = 0;
+        base->j1 = base->J;
+        propagate_range(base);
+        return;
+    }
+    pthread_t th[256];
+    prop_job jobs[256];
+    if (nthreads > 256) nthreads = 256;
+    int chunk = base->J / nthreads;
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t] = *base;
+        jobs[t].j0 = t * chunk;
+        jobs[t].j1 = (t + 1 == nthreads) ? base->J : (t + 1) * chunk;
+        pthread_create(&th[t], NULL, prop_thread, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+}
+
+/* ------------------------------------------------------------------ weighting + selection */
+/* canonical: returns 1 if resampled (select filled), 0 for the all-fail branch */
+static int weight_select_canonical(const double *logw, int J, int n, uint32_t seed, uint32_t stream,
+                                   double all_fail_llike, double *llike_n, double *ess_n, uint32_t *select,
+                                   double *work, uint64_t *cum)
+{
+    const int T = (J + TILE - 1) / TILE;
+    double *tmax = work, *tsum = work + T, *tsum2 = work + 2 * T, *buf = work + 3 * T;
+    double m = -INFINITY;
+    for (int t = 0; t < T; t++) {
+        double mt = -INFINITY;
+        for (int i = 0; i < TILE; i++) {
+            int j = t * TILE + i;
+            if (j < J && logw[j] > mt) mt = logw[j];
+        }
+        for (int i = 0; i < TILE; i++) {
+            int j = t * TILE + i;
+            double w = 0.0;
+            if (j < J && mt != -INFINITY && logw[j] != -INFINITY) w = plom_exp(logw[j] - mt);
+            buf[i] = w;
+        }
+        tsum[t] = tree_sum(buf, TILE);
+        for (int i = 0; i < TILE; i++) buf[i] = buf[i] * buf[i];
+        tsum2[t] = tree_sum(buf, TILE);
+        tmax[t] = mt;
+        if (mt > m) m = mt;
+    }
+    if (m == -INFINITY) {
+        *llike_n = all_fail_llike;
+        *ess_n = 0.0;
+        for (int j = 0; j < J; j++) select[j] = (uint32_t)j;
+        return 0;
+    }
+    const int P = next_pow2(T);
+    double *a = buf, *b = buf + P;
+    for (int t = 0; t < P; t++) {
+        a[t] = 0.0;
+        b[t] = 0.0;
+        if (t < T && tmax[t] != -INFINITY) {
+            double e = plom_exp(tmax[t] - m);
+            a[t] = tsum[t] * e;
+            b[t] = (tsum2[t] * e) * e;
+        }
+    }
+    const double S = tree_sum(a, P), S2 = tree_sum(b, P);
+    *llike_n = m + plom_log(S / (double)J);
+    *ess_n = (S * S) / S2;
+    /* fixed-point weights and their running integer prefix */
+    uint64_t C = 0;
+    for (int j = 0; j < J; j++) {
+        uint64_t q = 0;
+        if (logw[j] != -INFINITY) q = (uint64_t)((plom_exp(logw[j] - m) / S) * 0x1p62);
+        C += q;
+        cum[j] = C;
+    }
+    const uint64_t Q = C;
+    const uint64_t unum = plom_resample_num(seed, stream, (uint32_t)n);
+    const uint64_t J2 = (uint64_t)J << 33;
+    /* the reference's two-pointer walk (smc.c:101-111), in exact integers */
+    int i = 0;
+    for (int k = 0; k < J; k++) {
+        const u128 Rk = (u128)(((uint64_t)k << 33) + unum) * (u128)Q;
+        while ((u128)cum[i] * J2 < Rk) i++;
+        select[k] = (uint32_t)i;
+    }
+    return 1;
+}
+
+/* reference arithmetic (smc.c:27-112) on the same log-weights and uniform */
+static int weight_select_reference(const double *logw, int J, int n, int n_ts, double like_min, uint32_t seed,
+                                   uint32_t stream, double *llike_n, double *ess_n, uint32_t *select, double *w)
+{
+    double like_tot = 0.0, ess = 0.0;
+    int nfail = 0;
+    const double thr = pow(like_min, (double)n_ts);
+    for (int j = 0; j < J; j++) {
+        w[j] = exp(logw[j]);
+        if (w[j] <= thr) {
+            w[j] = 0.0;
+            nfail++;
+        } else {
+            like_tot += w[j];
+            ess += w[j] * w[j];
+        }
+    }
+    if (nfail == J) {
+        *llike_n = log(like_min) * n_ts;
+        *ess_n = 0.0;
+        for (int j = 0; j < J; j++) select[j] = (uint32_t)j;
+        return 0;
+    }
+    for (int j = 0; j < J; j++) w[j] /= like_tot;
+    *llike_n = log(like_tot / (double)J);
+    *ess_n = (like_tot * like_tot) / ess;
+    const double inc = 1.0 / (double)J;
+    const double U = (double)plom_resample_num(seed, stream, (uint32_t)n) * 0x1p-33;
+    double ran = inc * U;
+    int i = 0;
+    double cum = w[0];
+    for (int k = 0; k < J; k++) {
+        while (ran > cum && i < J - 1) {
+            i++;
+            cum += w[i];
+        }
+        select[k] = (uint32_t)i;
+        ran += inc;
+    }
+    return 1;
+}
+
+/* ------------------------------------------------------------------ the filter */
+int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uint32_t seed, uint32_t stream,
+               double dt, double like_min, const double *times, const double *data, const double *pv,
+               const double *x0, int mode, int nthreads,
+               double *loglik, int32_t *n_all_fail, double *loglik_n, double *ess_n, int32_t *ancestry,
+               double *final_state, double *logw_out, double *elapsed_s)
+{
+    if (model < 0 || model >= N_MODELS || J < 1 || N < 0) return -1;
+    const oracle_model *M = &MODELS[model];
+    const int S = M->S, n_ts = M->n_ts;
+    const int T = (J + TILE - 1) / TILE;
+    double *X = malloc((size_t)J * S * sizeof(double));
+    double *Xn = malloc((size_t)J * S * sizeof(double));
+    double *logw = malloc((size_t)J * sizeof(double));
+    double *work = malloc((size_t)(3 * T + 4 * next_pow2(T) + 2 * TILE + J) * sizeof(double));
+    uint64_t *cum = malloc((size_t)J * sizeof(uint64_t));
+    uint32_t *select = malloc((size_t)J * sizeof(uint32_t));
+    if (!X || !Xn || !logw || !work || !cum || !select) return -2;
+    for (int j = 0; j < J; j++) memcpy(X + (size_t)j * S, x0, (size_t)S * sizeof(double));
+    const int filter = !(flags & FLAG_NO_FILTER);
+    const double all_fail_llike = plom_log(like_min) * (double)n_ts;
+    double llike = 0.0;
+    int naf = 0;
+    struct timespec ts0, ts1;
+    clock_gettime(CLOCK_MONOTONIC, &ts0);
+    for (int n = 0; n < N; n++) {
+        int nonan = 0;
+        for (int k = 0; k < n_ts; k++) nonan += !isnan(data[(size_t)n * n_ts + k]);
+        prop_job job;
+        memset(&job, 0, sizeof job);
+        job.M = M; job.X = X; job.logw = logw; job.pv = pv; job.times = times; job.data = data;
+        job.J = J; job.n = n; job.nonan = nonan; job.filter = filter; job.n_ts = n_ts;
+        job.seed = seed; job.stream = stream; job.noises_off = noises_off;
+        job.dt = dt; job.sqdt = plom_sqrt(dt); job.like_min = like_min; job.mode = mode;
+        propagate_all(&job, nthreads);
+        double ln = 0.0, es = 0.0;
+        int resampled = 0;
+        if (filter && nonan) {
+            if (mode == ORACLE_CANONICAL)
+                resampled = weight_select_canonical(logw, J, n, seed, stream, all_fail_llike, &ln, &es, select, work, cum);
+            else
+                resampled = weight_select_reference(logw, J, n, n_ts, like_min, seed, stream, &ln, &es, select, work);
+            if (!resampled) naf++;
+            llike += ln;
+            if (logw_out) memcpy(logw_out + (size_t)n * J, logw, (size_t)J * sizeof(double));
+        } else {
+            for (int j = 0; j < J; j++) select[j] = (uint32_t)j;
+        }
+        if (loglik_n) loglik_n[n] = ln;
+        if (ess_n) ess_n[n] = es;
+        if (ancestry)
+            for (int j = 0; j < J; j++) ancestry[(size_t)n * J + j] = (int32_t)select[j];
+        /* resample_X: gather the ancestors' states */
+        for (int j = 0; j < J; j++) memcpy(Xn + (size_t)j * S, X + (size_t)select[j] * S, (size_t)S * sizeof(double));
+        double *tmp = X; X = Xn; Xn = tmp;
+    }
+    clock_gettime(CLOCK_MONOTONIC, &ts1);
+    if (elapsed_s) *elapsed_s = (double)(ts1.tv_sec - ts0.tv_sec) + 1e-9 * (double)(ts1.tv_nsec - ts0.tv_nsec);
+    if (final_state)
+        for (int k = 0; k < S; k++)
+            for (int j = 0; j < J; j++) final_state[(size_t)k * J + j] = X[(size_t)j * S + k];
+    *loglik = llike;
+    *n_all_fail = naf;
+    free(X); free(Xn); free(logw); free(work); free(cum); free(select);
+    return 0;
+}
+
+/* one particle through N intervals, no filtering: the state at every observation time,
+ * out[(n+1)*S + k]; out[0..S) = x0 (with incidences reset) */
+int oracle_trajectory(int model, int N, uint32_t noises_off, uint32_t seed, uint32_t stream, uint32_t j,
+                      double dt, const double *times, const double *pv, const double *x0, double *out)
+{
+    if (model < 0 || model >= N_MODELS) return -1;
+    const oracle_model *M = &MODELS[model];
+    const int S = M->S;
+    double x[256];
+    memcpy(x, x0, (size_t)S * sizeof(double));
+    memcpy(out, x0, (size_t)S * sizeof(double));
+    for (int n = 0; n < N; n++) {
+        plom_nstream64 ns;
+        plom_nstream64_init(&ns, seed, stream, (uint32_t)n, j);
+        M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off);
+        memcpy(out + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
+    }
+    return 0;
+}
+
+/* observed quantities and log-likelihood of one state (for tests) */
+int oracle_obs_loglik(int model, const double *x, const double *y, const double *pv, double t, double like_min,
+                      double *obs, double *loglik, int32_t *n_floor)
+{
+    if (model < 0 || model >= N_MODELS) return -1;
+    const oracle_model *M = &MODELS[model];
+    int nf = 0;
+    M->proj2obs(x, pv, obs);
+    *loglik = M->loglik(obs, y, pv, t, like_min, &nf);
+    *n_floor = nf;
+    return 0;
+}
+
+/* shared scalar arithmetic, host build (compared with the device build bit for bit) */
+void oracle_math(int n, const double *in, double *out)
+{
+    for (int i = 0; i < n; i++) {
+        const double x = in[i];
+        out[i * 6 + 0] = plom_exp(x);
+        out[i * 6 + 1] = plom_log(x);
+        out[i * 6 + 2] = plom_Phi(x);
+        out[i * 6 + 3] = plom_sin(x);
+        out[i * 6 + 4] = plom_cos(x);
+        out[i * 6 + 5] = plom_sqrt(x);
+    }
+}
+
+void oracle_normals(uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int n, double *out)
+{
+    plom_nstream64 ns;
+    plom_nstream64_init(&ns, seed, stream, n0, j);
+    for (int i = 0; i < n; i++) out[i] = plom_normal64(&ns);
+}
+
+void oracle_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) { plom_philox4x32_10(ctr, key, out); }
+
+uint32_t oracle_xoshiro(uint32_t *state, int n, uint32_t *out)
+{
+    plom_xoshiro128 r = {state[0], state[1], state[2], state[3]};
+    for (int i = 0; i < n; i++) out[i] = plom_xoshiro_next(&r);
+    state[0] = r.s0; state[1] = r.s1; state[2] = r.s2; state[3] = r.s3;
+    return 0;
+}

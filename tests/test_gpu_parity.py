@@ -1,0 +1,112 @@
+"""GPU filter vs the CPU oracle (canonical arithmetic), through the C ABI.
+
+Bar: bit-exact log-weights, ancestors, per-interval log-likelihoods, ESS and total
+log-likelihood in FP64 (the floor the workload states is 1e-9 relative; the tests ask
+for equality).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from iip_amd import configs, smc
+from iip_amd.smc import Model
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(name, N, params=None, props=None):
+    m = Model.get(name)
+    times, data = m.dataset(N)
+    pv = m.pack(params or configs.TRUTH[name])
+    x0 = m.x0(props or configs.X0_PROPS[name])
+    return m, times, data, pv, x0
+
+
+def compare(ctx, name, J, N, data_edit=None, noises_off=0, flags=0, seed=1, stream=0, like_min=1e-17, params=None):
+    m, times, data, pv, x0 = setup(name, N, params)
+    if data_edit:
+        data = data_edit(data.copy())
+    print("case", name, "J", J, "N", N, "noises_off", noises_off, "flags", flags, flush=True)
+    g = smc.run(ctx, m, J, times, data, pv, x0, ancestry=True, final_state=True, noises_off=noises_off,
+                flags=flags, seed=seed, stream=stream, like_min=like_min)
+    o = oracle.smc(m.id, J, times, data, pv, x0, noises_off=noises_off, flags=flags, seed=seed, stream=stream,
+                   like_min=like_min, ancestry=True, final_state=True, logw=True, nthreads=8)
+    assert g.n_all_fail == o["n_all_fail"]
+    np.testing.assert_array_equal(g.ancestry, o["ancestry"])
+    np.testing.assert_array_equal(g.loglik_n, o["loglik_n"])
+    np.testing.assert_array_equal(g.ess_n, o["ess_n"])
+    assert g.loglik == o["loglik"]
+    np.testing.assert_array_equal(g.final_state, o["final_state"])
+    return g, o
+
+
+def test_math_bitwise(ctx):
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(-40, 40, 20000), rng.uniform(-745, 709, 20000),
+                        np.exp(rng.uniform(-700, 700, 20000)), [0.0, 1.0, -1.0, 0.5, 1e-310, 2.0, 38.0, -38.0]])
+    np.testing.assert_array_equal(smc.selftest_math(ctx, x), oracle.math(x))
+
+
+def test_normals_bitwise(ctx):
+    for (seed, stream, n0, j) in [(1, 0, 0, 0), (7, 3, 11, 12345), (0xffffffff, 0xffffffff, 1039, 999999)]:
+        np.testing.assert_array_equal(smc.selftest_normals(ctx, seed, stream, n0, j, 257),
+                                      oracle.normals(seed, stream, n0, j, 257))
+
+
+def test_c1_parity(ctx):
+    """Config C1 exactly: SIR, J = 1000, N = 52, FP64."""
+    compare(ctx, "sir", 1000, 52)
+
+
+@pytest.mark.parametrize("J", [1, 7, 256, 257, 1000, 4099])
+def test_sir_ragged_J(ctx, J):
+    compare(ctx, "sir", J, 20)
+
+
+def test_sirs4_parity(ctx):
+    compare(ctx, "sirs4", 3000, 40)
+
+
+def test_seir10_parity(ctx):
+    compare(ctx, "seir10", 1500, 20)
+
+
+def test_many_tiles(ctx):
+    """More than 256 tiles: the tile-level tree runs over several chunks."""
+    compare(ctx, "sir", 70000, 6)
+
+
+@pytest.mark.parametrize("noises_off", [1, 2, 3])
+def test_noise_variants(ctx, noises_off):
+    compare(ctx, "sirs4", 600, 12, noises_off=noises_off)
+
+
+def test_missing_data(ctx):
+    """NaN series and all-NaN rows (no weighting, identity ancestry)."""
+    def edit(d):
+        d[2, :] = np.nan
+        d[5, 1] = np.nan
+        d[6, :2] = np.nan
+        return d
+    compare(ctx, "sirs4", 900, 10, data_edit=edit)
+
+
+def test_all_fail_branch(ctx):
+    """Data far from the model with a huge LIKE_MIN: every particle floored at some steps."""
+    def edit(d):
+        d[3, :] = 1e9
+        return d
+    g, o = compare(ctx, "sir", 500, 8, data_edit=edit, like_min=1e-3)
+    assert g.n_all_fail >= 1
+    assert np.all(g.ancestry[3] == np.arange(500))
+
+
+def test_no_filter(ctx):
+    g, o = compare(ctx, "sir", 300, 10, flags=smc.FLAG_NO_FILTER)
+    assert g.loglik == 0.0
+
+
+def test_streams_and_seeds(ctx):
+    a, _ = compare(ctx, "sir", 700, 10, seed=5, stream=9)
+    b, _ = compare(ctx, "sir", 700, 10, seed=5, stream=10)
+    assert a.loglik != b.loglik

@@ -28,14 +28,15 @@ VARIANTS = [
 
 
 def _R(real):
-    return {"double": ("double", "", "f64", "plom_nstream64", "plom_normal64"),
-            "float": ("float", "f", "f32", "plom_nstream32", "plom_normal32")}[real]
+    return {"double": ("double", "", "f64", "plom_nstream64", "plom_normal_pair64"),
+            "float": ("float", "f", "f32", "plom_nstream32", "plom_normal_pair32")}[real]
 
 
 class Emitter:
     def __init__(self, model):
         self.m = model
         self.p = "plom_%s" % model.name.lower()
+        self.used_pars = set()
 
     # ---------------------------------------------------------------- symbols
     def xidx(self, state, cac):
@@ -50,9 +51,9 @@ class Emitter:
             if name in m.par_sv:
                 return "X[%d]" % self.xidx(name, cac)
             if name in lay["proc"]:
-                return "pv[%d]" % (lay["proc"][name] + cac)
+                return self.par(lay["proc"][name] + cac)
             if name in lay["fixed"]:
-                return "pv[%d]" % (lay["fixed"][name] + cac)
+                return self.par(lay["fixed"][name] + cac)
             if name == "t":
                 return "tr"
             if name == "M_PI":
@@ -70,10 +71,15 @@ class Emitter:
                     raise ValueError("mix() takes one state variable")
                 s = node.args[0].name
                 base = lay["coupling"] + cac * m.N_CAC
-                return "(" + " + ".join("pv[%d] * X[%d]" % (base + k, self.xidx(s, k)) for k in range(m.N_CAC)) + ")"
+                return "(" + " + ".join("%s * X[%d]" % (self.par(base + k), self.xidx(s, k)) for k in range(m.N_CAC)) + ")"
             return None
 
         return E.CEmitter(resolve, real, call)
+
+    def par(self, k):
+        """Name of parameter pv[k] inside an interval function (bound once, uniform)."""
+        self.used_pars.add(k)
+        return "P%d" % k
 
     # ---------------------------------------------------------------- interval
     def term(self, reaction, cac, from_state):
@@ -109,26 +115,66 @@ class Emitter:
         L.append("{")
         L.append("    const %s dtr = (%s)dt;" % (Rt, Rt))
         L.append("    (void)sqdt; (void)ns; (void)pv;")
+        self.used_pars = set()
+        bind_at = len(L)
         for u in range(m.N_TS_INC_UNIQUE):
             L.append("    X[%d] = %s; /* reset_inc */" % (m.inc_offset + u, zero))
         L.append("    double t = t0;")
         L.append("    while (t < t1) {")
         L.append("        const %s tr = (%s)t; (void)tr;" % (Rt, Rt))
-        # rate caches then noises, population by population (core_template.c:258-284)
+        # States read across populations (inside mix()) keep their old value until every
+        # population is done; the others are written back right after their population's
+        # block.  Every right-hand side reads only old values either way, as in the
+        # reference's separate y_pred buffer (core_template.c:292-336).
+        cross = set()
+        for tree in m.rate_trees:
+            for cl in E.calls(tree):
+                if cl.fn == "mix":
+                    cross |= E.symbols(cl)
+        incs = []  # (accumulator index, ordered cac list, definition list)
+        u = 0
+        for o, lst in enumerate(m.obs_var_def):
+            if not m.is_inc[o]:
+                continue
+            info = m.obs2ts[o]
+            for tsu in range(info["n_ts_unique"]):
+                cl = [ci * m.N_AC + ai for (ci, ai) in info["cac"][tsu]]
+                incs.append((m.inc_offset + u, cl, lst))
+                u += 1
+        for idx, cl, lst in incs:
+            L.append("        %s s%d = %s;" % (Rt, idx, zero))
+        deferred = []
+        contrib = {}
+        pair = {"open": None, "k": 0}
+
+        def draw(name):
+            # Box-Muller pairs taken in draw order; the second value of a pair still open
+            # at the end of the substep is dropped
+            if pair["open"] is None:
+                k = pair["k"]
+                pair["k"] += 1
+                L.append("        %s za%d, zb%d;" % (Rt, k, k))
+                L.append("        %s(ns, &za%d, &zb%d);" % (normal, k, k))
+                L.append("        const %s %s = sqdt * za%d;" % (Rt, name, k))
+                pair["open"] = "zb%d" % k
+            else:
+                L.append("        const %s %s = sqdt * %s;" % (Rt, name, pair["open"]))
+                pair["open"] = None
+
         for c in range(m.N_CAC):
+            # rate caches then noises, population by population (core_template.c:258-284)
             em = self.rate_emitter(c, real)
             for k, tree in enumerate(m.rate_trees):
                 L.append("        const %s r%d_%d = %s;" % (Rt, k, c, em.emit(tree)))
             if dem:
                 for i in range(len(m.proc_model)):
-                    L.append("        const %s d%d_%d = sqdt * %s(ns);" % (Rt, i, c, normal))
+                    draw("d%d_%d" % (i, c))
             if env:
                 for j in range(len(m.white_noise)):
-                    L.append("        const %s w%d_%d = sqdt * %s(ns);" % (Rt, j, c, normal))
-        # state equations
-        upd = []
-        for s in m.par_sv:
-            for c in range(m.N_CAC):
+                    draw("w%d_%d" % (j, c))
+            # state equations of population c
+            local = []
+            for s in m.par_sv:
                 items = []
                 for r in m.proc_model:
                     if r["from"] == s:
@@ -147,39 +193,44 @@ class Emitter:
                     if e:
                         rhs += " + " + e
                 L.append("        const %s f%d = %s;" % (Rt, idx, rhs))
-                upd.append(idx)
-        # observed incidences (core_template.c:303-323)
-        u = 0
-        for o, lst in enumerate(m.obs_var_def):
-            if not m.is_inc[o]:
-                continue
-            info = m.obs2ts[o]
-            for tsu in range(info["n_ts_unique"]):
-                idx = m.inc_offset + u
-                L.append("        %s s%d = %s;" % (Rt, idx, zero))
-                for (ci, ai) in info["cac"][tsu]:
-                    c = ci * m.N_AC + ai
-                    items = []
-                    for dfn in lst:
-                        for ri in dfn["reactions"]:
-                            r = m.proc_model[ri]
-                            items.append(("+", self.term(r, c, dfn["from"]), r))
-                    eq, d, e = self.eq_parts(items, c, real, dem, env)
-                    rhs = "(%s)*dtr" % eq
-                    if dem:
-                        rhs += " + " + d
-                    if e:
-                        rhs += " + " + e
+                (deferred if s in cross else local).append(idx)
+            # contributions of population c to the observed incidences (core_template.c:303-323)
+            for idx, cl, lst in incs:
+                if c not in cl:
+                    continue
+                items = []
+                for dfn in lst:
+                    for ri in dfn["reactions"]:
+                        r = m.proc_model[ri]
+                        items.append(("+", self.term(r, c, dfn["from"]), r))
+                eq, d, e = self.eq_parts(items, c, real, dem, env)
+                rhs = "(%s)*dtr" % eq
+                if dem:
+                    rhs += " + " + d
+                if e:
+                    rhs += " + " + e
+                if cl == sorted(cl):
                     L.append("        s%d += %s;" % (idx, rhs))
-                L.append("        const %s f%d = X[%d] + s%d;" % (Rt, idx, idx, idx))
-                upd.append(idx)
-                u += 1
-        # y_pred -> X with X >= 0 (core_template.c:328-336)
-        for idx in upd:
+                else:
+                    L.append("        const %s q%d_%d = %s;" % (Rt, idx, c, rhs))
+                    contrib[(idx, c)] = True
+            # y_pred -> X with X >= 0 (core_template.c:328-336)
+            for idx in local:
+                L.append("        X[%d] = (f%d < %s) ? %s : f%d;" % (idx, idx, zero, zero, idx))
+        for idx in deferred:
+            L.append("        X[%d] = (f%d < %s) ? %s : f%d;" % (idx, idx, zero, zero, idx))
+        for idx, cl, lst in incs:
+            if cl != sorted(cl):
+                for c in cl:
+                    L.append("        s%d += q%d_%d;" % (idx, idx, c))
+            L.append("        const %s f%d = X[%d] + s%d;" % (Rt, idx, idx, idx))
             L.append("        X[%d] = (f%d < %s) ? %s : f%d;" % (idx, idx, zero, zero, idx))
         L.append("        t += dt;")
         L.append("    }")
         L.append("}")
+        uni = "PLOM_UNIFORM64" if real == "double" else "PLOM_UNIFORM32"
+        binds = ["    const %s P%d = %s(pv[%d]);" % (Rt, k, uni, k) for k in sorted(self.used_pars)]
+        L[bind_at:bind_at] = binds
         return "\n".join(L)
 
     # ---------------------------------------------------------------- observations
@@ -334,4 +385,5 @@ class Emitter:
 
 
 def emit_header(model):
+    model.check_codegen()
     return Emitter(model).header()

@@ -35,7 +35,7 @@ class PlomModel:
         self.link = copy.deepcopy(link)
 
         # ---------------------------------------------------------- Context.py:32-81
-        self.cac_id = [p["id"] for p in context["population"]]
+        self.cac_id = [p["id"] for p in context.get("population", [])]
         head = [x.split("__") for x in self.cac_id]
         cities = [h[0] for h in head]
         ages = [h[1] for h in head]
@@ -44,8 +44,8 @@ class PlomModel:
         self.N_C = len(set(cities))
         self.N_AC = len(set(ages))
         self.N_CAC = self.N_C * self.N_AC
-        self.ts_id = [x["id"] for x in context["time_series"]]
-        self.map_ts_cac = {x["id"]: list(x["population_id"]) for x in context["time_series"]}
+        self.ts_id = [x["id"] for x in context.get("time_series", [])]
+        self.map_ts_cac = {x["id"]: list(x["population_id"]) for x in context.get("time_series", [])}
 
         # ---------------------------------------------------------- Cmodel.py:43-76
         self.remainder = None
@@ -79,8 +79,6 @@ class PlomModel:
 
         self.drift_par_proc = [x["parameter"] for x in process.get("diffusion", [])]
         self.vol_par_proc = [x["volatility"] for x in process.get("diffusion", [])]
-        if self.drift_par_proc:
-            raise ModelError("diffusions on parameters (drift) are not supported by this build yet")
         self.drift_var = ["drift__par_proc__" + x for x in self.drift_par_proc]
 
         fixed_obs = []
@@ -143,7 +141,7 @@ class PlomModel:
         # ---------------------------------------------------------- Builder.py:83-169
         self.map_ts_obs = {}
         for x in link["observed"]:
-            for ts in x["time_series_id"]:
+            for ts in x.get("time_series_id", []):
                 self.map_ts_obs[ts] = x["id"]
         head = [t.split("__") for t in self.ts_id]
         name_ts = [h[0] for h in head]
@@ -199,8 +197,13 @@ class PlomModel:
                            if r["from"] == d["from"] and r["to"] == d["to"] and r["rate"] == d["rate"]]
                     d["reactions"] = ids
         self.rate_trees = [E.parse(r) for r in self.rates]
-        self._check_symbols()
         self.layout = self._param_layout()
+
+    def check_codegen(self):
+        """Raise ModelError for what the code generator does not support."""
+        if self.drift_par_proc:
+            raise ModelError("diffusions on parameters (drift) are not supported by this build yet")
+        self._check_symbols()
 
     # ------------------------------------------------------------------ helpers
     def _get_obs2ts(self):

@@ -95,14 +95,10 @@ PLOM_HD double plom_exp(double x)
 }
 
 /* ---------------------------------------------------------------- log */
-PLOM_HD double plom_log(double x)
+/* log of a positive normal finite x (the core of plom_log, also used directly by the
+ * Box-Muller transform whose argument is always in [2^-33, 1)) */
+PLOM_HD double plom_log_core(double x, int e)
 {
-    if (plom_isnan(x)) return x;
-    if (x < 0.0) return PLOM_NAN;
-    if (x == 0.0) return -PLOM_INF;
-    if (x == PLOM_INF) return x;
-    int e = 0;
-    if (x < 0x1p-1022) { x = x * 0x1p54; e = -54; }
     uint64_t u = plom_d2u(x);
     e += (int)(u >> 52) - 1023;
     double m = plom_u2d((u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
@@ -126,6 +122,16 @@ PLOM_HD double plom_log(double x)
     double lm = f - (hfsq - s * (hfsq + R));
     double de = (double)e;
     return de * PLOM_LN2_HI + (de * PLOM_LN2_LO + lm);
+}
+
+PLOM_HD double plom_log(double x)
+{
+    if (plom_isnan(x)) return x;
+    if (x < 0.0) return PLOM_NAN;
+    if (x == 0.0) return -PLOM_INF;
+    if (x == PLOM_INF) return x;
+    if (x < 0x1p-1022) return plom_log_core(x * 0x1p54, -54);
+    return plom_log_core(x, 0);
 }
 
 /* ---------------------------------------------------------------- sin / cos */

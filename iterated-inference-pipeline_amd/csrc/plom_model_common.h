@@ -13,6 +13,30 @@
 
 #include "plom_rng.h"
 
+/* PLOM_UNIFORM64/32: a value every lane of the wave holds (the filter's parameters: a
+ * wave never spans two filters), moved to scalar registers on the device */
+#if defined(__HIP_DEVICE_COMPILE__)
+static __device__ inline __attribute__((always_inline)) double plom_uniform64(double x)
+{
+    uint64_t u = plom_d2u(x);
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+    return plom_u2d(((uint64_t)hi << 32) | lo);
+}
+static __device__ inline __attribute__((always_inline)) float plom_uniform32(float x)
+{
+    union { float f; uint32_t u; } v;
+    v.f = x;
+    v.u = __builtin_amdgcn_readfirstlane(v.u);
+    return v.f;
+}
+#define PLOM_UNIFORM64(x) plom_uniform64(x)
+#define PLOM_UNIFORM32(x) plom_uniform32(x)
+#else
+#define PLOM_UNIFORM64(x) (x)
+#define PLOM_UNIFORM32(x) (x)
+#endif
+
 #define PLOM_NO_DEM_STO 1u
 #define PLOM_NO_ENV_STO 2u
 #define PLOM_NO_DRIFT 4u

@@ -21,9 +21,11 @@
  *   PURPOSE_FULL gives the uniform that picks the final particle of a --full lineage.
  *
  * Uniforms are (x + 1/2) 2^-32 for a 32-bit x: never 0 or 1, exact in double.
- * Gaussians: Box-Muller on two uniforms, both outputs used (the second is kept for the
- * next draw).  FP64 uses plom_log/plom_sqrt/plom_sincos_2pi; the FP32 stream (device
- * only) uses the gfx950 transcendental instructions.
+ * Gaussians: Box-Muller pairs on two uniforms; the generated step code takes the two
+ * values of a pair in draw order and drops the second value of a pair left open at the
+ * end of an Euler substep, so the pairing is fixed at code-generation time.  FP64 uses
+ * plom_log_core/plom_sqrt/plom_sincos_2pi; the FP32 stream (device only) uses the gfx950
+ * transcendental instructions.
  */
 #ifndef PLOM_RNG_H
 #define PLOM_RNG_H
@@ -88,36 +90,26 @@ PLOM_HD void plom_xoshiro_seed(plom_xoshiro128 *r, const uint32_t w[4])
 
 PLOM_HD double plom_u01_from32(uint32_t x) { return ((double)x + 0.5) * 0x1p-32; }
 
-/* ---------------------------------------------------------------- FP64 Gaussian stream */
-typedef struct {
-    plom_xoshiro128 x;
-    double spare;
-    int has_spare;
-} plom_nstream64;
+/* ---------------------------------------------------------------- FP64 Gaussians */
+typedef plom_xoshiro128 plom_nstream64;
 
 PLOM_HD void plom_nstream64_init(plom_nstream64 *ns, uint32_t seed, uint32_t stream, uint32_t n, uint32_t j)
 {
     uint32_t w[4];
     plom_draw128(seed, stream, n, j, PLOM_PURPOSE_PROPAGATE, w);
-    plom_xoshiro_seed(&ns->x, w);
-    ns->spare = 0.0;
-    ns->has_spare = 0;
+    plom_xoshiro_seed(ns, w);
 }
 
-PLOM_HD double plom_normal64(plom_nstream64 *ns)
+/* one Box-Muller pair: z0 = r cos(2 pi u2), z1 = r sin(2 pi u2), r = sqrt(-2 log u1) */
+PLOM_HD void plom_normal_pair64(plom_nstream64 *ns, double *z0, double *z1)
 {
-    if (ns->has_spare) {
-        ns->has_spare = 0;
-        return ns->spare;
-    }
-    double u1 = plom_u01_from32(plom_xoshiro_next(&ns->x));
-    double u2 = plom_u01_from32(plom_xoshiro_next(&ns->x));
-    double rad = plom_sqrt(-2.0 * plom_log(u1));
+    double u1 = plom_u01_from32(plom_xoshiro_next(ns));
+    double u2 = plom_u01_from32(plom_xoshiro_next(ns));
+    double rad = plom_sqrt(-2.0 * plom_log_core(u1, 0));
     double s, c;
     plom_sincos_2pi(u2, &s, &c);
-    ns->spare = rad * s;
-    ns->has_spare = 1;
-    return rad * c;
+    *z0 = rad * c;
+    *z1 = rad * s;
 }
 
 /* the resampling uniform of interval n as the integer numerator 2x+1 of U = (2x+1)/2^33 */
@@ -128,22 +120,16 @@ PLOM_HD uint64_t plom_resample_num(uint32_t seed, uint32_t stream, uint32_t n)
     return 2ull * (uint64_t)w[0] + 1ull;
 }
 
-/* ---------------------------------------------------------------- FP32 Gaussian stream */
+/* ---------------------------------------------------------------- FP32 Gaussians */
 #if defined(__HIPCC__)
-typedef struct {
-    plom_xoshiro128 x;
-    float spare;
-    int has_spare;
-} plom_nstream32;
+typedef plom_xoshiro128 plom_nstream32;
 
 __device__ inline __attribute__((always_inline)) void
 plom_nstream32_init(plom_nstream32 *ns, uint32_t seed, uint32_t stream, uint32_t n, uint32_t j)
 {
     uint32_t w[4];
     plom_draw128(seed, stream, n, j, PLOM_PURPOSE_PROPAGATE, w);
-    plom_xoshiro_seed(&ns->x, w);
-    ns->spare = 0.0f;
-    ns->has_spare = 0;
+    plom_xoshiro_seed(ns, w);
 }
 
 /* u in (0,1) from the top 24 bits: exact in float */
@@ -152,21 +138,14 @@ __device__ inline __attribute__((always_inline)) float plom_u01f(uint32_t x)
     return ((float)(x >> 8) + 0.5f) * 0x1p-24f;
 }
 
-__device__ inline __attribute__((always_inline)) float plom_normal32(plom_nstream32 *ns)
+__device__ inline __attribute__((always_inline)) void plom_normal_pair32(plom_nstream32 *ns, float *z0, float *z1)
 {
-    if (ns->has_spare) {
-        ns->has_spare = 0;
-        return ns->spare;
-    }
-    float u1 = plom_u01f(plom_xoshiro_next(&ns->x));
-    float u2 = plom_u01f(plom_xoshiro_next(&ns->x));
+    float u1 = plom_u01f(plom_xoshiro_next(ns));
+    float u2 = plom_u01f(plom_xoshiro_next(ns));
     /* -2 ln u1 = -2 ln2 log2(u1); v_log_f32 is log2, v_sin/v_cos_f32 take turns */
     float rad = __builtin_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
-    float s = __builtin_amdgcn_sinf(u2);
-    float c = __builtin_amdgcn_cosf(u2);
-    ns->spare = rad * s;
-    ns->has_spare = 1;
-    return rad * c;
+    *z0 = rad * __builtin_amdgcn_cosf(u2);
+    *z1 = rad * __builtin_amdgcn_sinf(u2);
 }
 #endif
 

@@ -375,7 +375,11 @@ __global__ void k_selftest_normals(uint32_t seed, uint32_t stream, uint32_t n0, 
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     plom_nstream64 ns;
     plom_nstream64_init(&ns, seed, stream, n0, j);
-    for (int i = 0; i < n; i++) out[i] = plom_normal64(&ns);
+    for (int i = 0; i + 1 < n; i += 2) plom_normal_pair64(&ns, &out[i], &out[i + 1]);
+    if (n & 1) {
+        double spare;
+        plom_normal_pair64(&ns, &out[n - 1], &spare);
+    }
 }
 
 /* ------------------------------------------------------------------ model table */

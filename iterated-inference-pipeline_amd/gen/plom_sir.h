@@ -17,21 +17,26 @@ PLOM_HD void plom_sir_interval_full_f64(double *X, const double *pv, plom_nstrea
 {
     const double dtr = (double)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     while (t < t1) {
         const double tr = (double)t; (void)tr;
-        const double r0_0 = ((pv[0] * X[1]) / pv[4]);
-        const double r1_0 = pv[1];
-        const double d0_0 = sqdt * plom_normal64(ns);
-        const double d1_0 = sqdt * plom_normal64(ns);
+        double s2 = 0.0;
+        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r1_0 = P1;
+        double za0, zb0;
+        plom_normal_pair64(ns, &za0, &zb0);
+        const double d0_0 = sqdt * za0;
+        const double d1_0 = sqdt * zb0;
         const double f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0;
         const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[1]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
-        double s2 = 0.0;
         s2 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0;
-        const double f2 = X[2] + s2;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
+        const double f2 = X[2] + s2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         t += dt;
     }
@@ -41,19 +46,22 @@ PLOM_HD void plom_sir_interval_no_dem_sto_f64(double *X, const double *pv, plom_
 {
     const double dtr = (double)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     while (t < t1) {
         const double tr = (double)t; (void)tr;
-        const double r0_0 = ((pv[0] * X[1]) / pv[4]);
-        const double r1_0 = pv[1];
+        double s2 = 0.0;
+        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r1_0 = P1;
         const double f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
-        double s2 = 0.0;
         s2 += ( + (r0_0*X[0]))*dtr;
-        const double f2 = X[2] + s2;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
+        const double f2 = X[2] + s2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         t += dt;
     }
@@ -63,21 +71,26 @@ PLOM_HD void plom_sir_interval_no_env_sto_f64(double *X, const double *pv, plom_
 {
     const double dtr = (double)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     while (t < t1) {
         const double tr = (double)t; (void)tr;
-        const double r0_0 = ((pv[0] * X[1]) / pv[4]);
-        const double r1_0 = pv[1];
-        const double d0_0 = sqdt * plom_normal64(ns);
-        const double d1_0 = sqdt * plom_normal64(ns);
+        double s2 = 0.0;
+        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r1_0 = P1;
+        double za0, zb0;
+        plom_normal_pair64(ns, &za0, &zb0);
+        const double d0_0 = sqdt * za0;
+        const double d1_0 = sqdt * zb0;
         const double f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0;
         const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[1]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
-        double s2 = 0.0;
         s2 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0;
-        const double f2 = X[2] + s2;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
+        const double f2 = X[2] + s2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         t += dt;
     }
@@ -87,19 +100,22 @@ PLOM_HD void plom_sir_interval_no_dem_sto_no_env_sto_f64(double *X, const double
 {
     const double dtr = (double)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     while (t < t1) {
         const double tr = (double)t; (void)tr;
-        const double r0_0 = ((pv[0] * X[1]) / pv[4]);
-        const double r1_0 = pv[1];
+        double s2 = 0.0;
+        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r1_0 = P1;
         const double f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
-        double s2 = 0.0;
         s2 += ( + (r0_0*X[0]))*dtr;
-        const double f2 = X[2] + s2;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
+        const double f2 = X[2] + s2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         t += dt;
     }
@@ -143,21 +159,26 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_f
 {
     const float dtr = (float)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const float P0 = PLOM_UNIFORM32(pv[0]);
+    const float P1 = PLOM_UNIFORM32(pv[1]);
+    const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
     while (t < t1) {
         const float tr = (float)t; (void)tr;
-        const float r0_0 = ((pv[0] * X[1]) / pv[4]);
-        const float r1_0 = pv[1];
-        const float d0_0 = sqdt * plom_normal32(ns);
-        const float d1_0 = sqdt * plom_normal32(ns);
+        float s2 = 0.0f;
+        const float r0_0 = ((P0 * X[1]) / P4);
+        const float r1_0 = P1;
+        float za0, zb0;
+        plom_normal_pair32(ns, &za0, &zb0);
+        const float d0_0 = sqdt * za0;
+        const float d1_0 = sqdt * zb0;
         const float f0 = X[0] + ( - (r0_0*X[0]))*dtr + - __builtin_sqrtf((r0_0*X[0]))*d0_0;
         const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - __builtin_sqrtf((r1_0*X[1]))*d1_0+ __builtin_sqrtf((r0_0*X[0]))*d0_0;
-        float s2 = 0.0f;
         s2 += ( + (r0_0*X[0]))*dtr + + __builtin_sqrtf((r0_0*X[0]))*d0_0;
-        const float f2 = X[2] + s2;
         X[0] = (f0 < 0.0f) ? 0.0f : f0;
         X[1] = (f1 < 0.0f) ? 0.0f : f1;
+        const float f2 = X[2] + s2;
         X[2] = (f2 < 0.0f) ? 0.0f : f2;
         t += dt;
     }
@@ -167,19 +188,22 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
 {
     const float dtr = (float)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const float P0 = PLOM_UNIFORM32(pv[0]);
+    const float P1 = PLOM_UNIFORM32(pv[1]);
+    const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
     while (t < t1) {
         const float tr = (float)t; (void)tr;
-        const float r0_0 = ((pv[0] * X[1]) / pv[4]);
-        const float r1_0 = pv[1];
+        float s2 = 0.0f;
+        const float r0_0 = ((P0 * X[1]) / P4);
+        const float r1_0 = P1;
         const float f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
-        float s2 = 0.0f;
         s2 += ( + (r0_0*X[0]))*dtr;
-        const float f2 = X[2] + s2;
         X[0] = (f0 < 0.0f) ? 0.0f : f0;
         X[1] = (f1 < 0.0f) ? 0.0f : f1;
+        const float f2 = X[2] + s2;
         X[2] = (f2 < 0.0f) ? 0.0f : f2;
         t += dt;
     }
@@ -189,21 +213,26 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
 {
     const float dtr = (float)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const float P0 = PLOM_UNIFORM32(pv[0]);
+    const float P1 = PLOM_UNIFORM32(pv[1]);
+    const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
     while (t < t1) {
         const float tr = (float)t; (void)tr;
-        const float r0_0 = ((pv[0] * X[1]) / pv[4]);
-        const float r1_0 = pv[1];
-        const float d0_0 = sqdt * plom_normal32(ns);
-        const float d1_0 = sqdt * plom_normal32(ns);
+        float s2 = 0.0f;
+        const float r0_0 = ((P0 * X[1]) / P4);
+        const float r1_0 = P1;
+        float za0, zb0;
+        plom_normal_pair32(ns, &za0, &zb0);
+        const float d0_0 = sqdt * za0;
+        const float d1_0 = sqdt * zb0;
         const float f0 = X[0] + ( - (r0_0*X[0]))*dtr + - __builtin_sqrtf((r0_0*X[0]))*d0_0;
         const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - __builtin_sqrtf((r1_0*X[1]))*d1_0+ __builtin_sqrtf((r0_0*X[0]))*d0_0;
-        float s2 = 0.0f;
         s2 += ( + (r0_0*X[0]))*dtr + + __builtin_sqrtf((r0_0*X[0]))*d0_0;
-        const float f2 = X[2] + s2;
         X[0] = (f0 < 0.0f) ? 0.0f : f0;
         X[1] = (f1 < 0.0f) ? 0.0f : f1;
+        const float f2 = X[2] + s2;
         X[2] = (f2 < 0.0f) ? 0.0f : f2;
         t += dt;
     }
@@ -213,19 +242,22 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
 {
     const float dtr = (float)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const float P0 = PLOM_UNIFORM32(pv[0]);
+    const float P1 = PLOM_UNIFORM32(pv[1]);
+    const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
     while (t < t1) {
         const float tr = (float)t; (void)tr;
-        const float r0_0 = ((pv[0] * X[1]) / pv[4]);
-        const float r1_0 = pv[1];
+        float s2 = 0.0f;
+        const float r0_0 = ((P0 * X[1]) / P4);
+        const float r1_0 = P1;
         const float f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
-        float s2 = 0.0f;
         s2 += ( + (r0_0*X[0]))*dtr;
-        const float f2 = X[2] + s2;
         X[0] = (f0 < 0.0f) ? 0.0f : f0;
         X[1] = (f1 < 0.0f) ? 0.0f : f1;
+        const float f2 = X[2] + s2;
         X[2] = (f2 < 0.0f) ? 0.0f : f2;
         t += dt;
     }

@@ -17,6 +17,46 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
 {
     const double dtr = (double)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P2 = PLOM_UNIFORM64(pv[2]);
+    const double P3 = PLOM_UNIFORM64(pv[3]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    const double P5 = PLOM_UNIFORM64(pv[5]);
+    const double P6 = PLOM_UNIFORM64(pv[6]);
+    const double P7 = PLOM_UNIFORM64(pv[7]);
+    const double P8 = PLOM_UNIFORM64(pv[8]);
+    const double P9 = PLOM_UNIFORM64(pv[9]);
+    const double P10 = PLOM_UNIFORM64(pv[10]);
+    const double P11 = PLOM_UNIFORM64(pv[11]);
+    const double P12 = PLOM_UNIFORM64(pv[12]);
+    const double P13 = PLOM_UNIFORM64(pv[13]);
+    const double P14 = PLOM_UNIFORM64(pv[14]);
+    const double P15 = PLOM_UNIFORM64(pv[15]);
+    const double P16 = PLOM_UNIFORM64(pv[16]);
+    const double P17 = PLOM_UNIFORM64(pv[17]);
+    const double P18 = PLOM_UNIFORM64(pv[18]);
+    const double P19 = PLOM_UNIFORM64(pv[19]);
+    const double P28 = PLOM_UNIFORM64(pv[28]);
+    const double P29 = PLOM_UNIFORM64(pv[29]);
+    const double P30 = PLOM_UNIFORM64(pv[30]);
+    const double P31 = PLOM_UNIFORM64(pv[31]);
+    const double P32 = PLOM_UNIFORM64(pv[32]);
+    const double P33 = PLOM_UNIFORM64(pv[33]);
+    const double P34 = PLOM_UNIFORM64(pv[34]);
+    const double P35 = PLOM_UNIFORM64(pv[35]);
+    const double P36 = PLOM_UNIFORM64(pv[36]);
+    const double P37 = PLOM_UNIFORM64(pv[37]);
+    const double P38 = PLOM_UNIFORM64(pv[38]);
+    const double P39 = PLOM_UNIFORM64(pv[39]);
+    const double P40 = PLOM_UNIFORM64(pv[40]);
+    const double P41 = PLOM_UNIFORM64(pv[41]);
+    const double P42 = PLOM_UNIFORM64(pv[42]);
+    const double P43 = PLOM_UNIFORM64(pv[43]);
+    const double P44 = PLOM_UNIFORM64(pv[44]);
+    const double P45 = PLOM_UNIFORM64(pv[45]);
+    const double P46 = PLOM_UNIFORM64(pv[46]);
+    const double P47 = PLOM_UNIFORM64(pv[47]);
     X[8] = 0.0; /* reset_inc */
     X[9] = 0.0; /* reset_inc */
     X[10] = 0.0; /* reset_inc */
@@ -24,65 +64,81 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
     double t = t0;
     while (t < t1) {
         const double tr = (double)t; (void)tr;
-        const double r0_0 = (((pv[0] * (1.0 + (pv[4] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[32] * X[4] + pv[33] * X[5] + pv[34] * X[6] + pv[35] * X[7])) / pv[28]);
-        const double r1_0 = pv[8];
-        const double r2_0 = (pv[12] * ((pv[28] - X[0]) - X[4]));
-        const double d0_0 = sqdt * plom_normal64(ns);
-        const double d1_0 = sqdt * plom_normal64(ns);
-        const double d2_0 = sqdt * plom_normal64(ns);
-        const double w0_0 = sqdt * plom_normal64(ns);
-        const double r0_1 = (((pv[1] * (1.0 + (pv[5] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[36] * X[4] + pv[37] * X[5] + pv[38] * X[6] + pv[39] * X[7])) / pv[29]);
-        const double r1_1 = pv[9];
-        const double r2_1 = (pv[13] * ((pv[29] - X[1]) - X[5]));
-        const double d0_1 = sqdt * plom_normal64(ns);
-        const double d1_1 = sqdt * plom_normal64(ns);
-        const double d2_1 = sqdt * plom_normal64(ns);
-        const double w0_1 = sqdt * plom_normal64(ns);
-        const double r0_2 = (((pv[2] * (1.0 + (pv[6] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[40] * X[4] + pv[41] * X[5] + pv[42] * X[6] + pv[43] * X[7])) / pv[30]);
-        const double r1_2 = pv[10];
-        const double r2_2 = (pv[14] * ((pv[30] - X[2]) - X[6]));
-        const double d0_2 = sqdt * plom_normal64(ns);
-        const double d1_2 = sqdt * plom_normal64(ns);
-        const double d2_2 = sqdt * plom_normal64(ns);
-        const double w0_2 = sqdt * plom_normal64(ns);
-        const double r0_3 = (((pv[3] * (1.0 + (pv[7] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[44] * X[4] + pv[45] * X[5] + pv[46] * X[6] + pv[47] * X[7])) / pv[31]);
-        const double r1_3 = pv[11];
-        const double r2_3 = (pv[15] * ((pv[31] - X[3]) - X[7]));
-        const double d0_3 = sqdt * plom_normal64(ns);
-        const double d1_3 = sqdt * plom_normal64(ns);
-        const double d2_3 = sqdt * plom_normal64(ns);
-        const double w0_3 = sqdt * plom_normal64(ns);
-        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0+ plom_sqrt((r2_0))*d2_0 + - (r0_0*X[0])*pv[16]*w0_0;
-        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - plom_sqrt((r0_1*X[1]))*d0_1+ plom_sqrt((r2_1))*d2_1 + - (r0_1*X[1])*pv[17]*w0_1;
-        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - plom_sqrt((r0_2*X[2]))*d0_2+ plom_sqrt((r2_2))*d2_2 + - (r0_2*X[2])*pv[18]*w0_2;
-        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - plom_sqrt((r0_3*X[3]))*d0_3+ plom_sqrt((r2_3))*d2_3 + - (r0_3*X[3])*pv[19]*w0_3;
-        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[4]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*pv[16]*w0_0;
-        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[5]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*pv[17]*w0_1;
-        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[6]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*pv[18]*w0_2;
-        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - plom_sqrt((r1_3*X[7]))*d1_3+ plom_sqrt((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*pv[19]*w0_3;
         double s8 = 0.0;
-        s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*pv[16]*w0_0;
-        const double f8 = X[8] + s8;
         double s9 = 0.0;
-        s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*pv[17]*w0_1;
-        const double f9 = X[9] + s9;
         double s10 = 0.0;
-        s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*pv[18]*w0_2;
-        const double f10 = X[10] + s10;
         double s11 = 0.0;
-        s11 += ( + (r0_3*X[3]))*dtr + + plom_sqrt((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*pv[19]*w0_3;
-        const double f11 = X[11] + s11;
+        const double r0_0 = (((P0 * (1.0 + (P4 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r1_0 = P8;
+        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        double za0, zb0;
+        plom_normal_pair64(ns, &za0, &zb0);
+        const double d0_0 = sqdt * za0;
+        const double d1_0 = sqdt * zb0;
+        double za1, zb1;
+        plom_normal_pair64(ns, &za1, &zb1);
+        const double d2_0 = sqdt * za1;
+        const double w0_0 = sqdt * zb1;
+        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0+ plom_sqrt((r2_0))*d2_0 + - (r0_0*X[0])*P16*w0_0;
+        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[4]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
+        s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
+        const double r0_1 = (((P1 * (1.0 + (P5 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r1_1 = P9;
+        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        double za2, zb2;
+        plom_normal_pair64(ns, &za2, &zb2);
+        const double d0_1 = sqdt * za2;
+        const double d1_1 = sqdt * zb2;
+        double za3, zb3;
+        plom_normal_pair64(ns, &za3, &zb3);
+        const double d2_1 = sqdt * za3;
+        const double w0_1 = sqdt * zb3;
+        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - plom_sqrt((r0_1*X[1]))*d0_1+ plom_sqrt((r2_1))*d2_1 + - (r0_1*X[1])*P17*w0_1;
+        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[5]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
+        s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
+        const double r0_2 = (((P2 * (1.0 + (P6 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r1_2 = P10;
+        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        double za4, zb4;
+        plom_normal_pair64(ns, &za4, &zb4);
+        const double d0_2 = sqdt * za4;
+        const double d1_2 = sqdt * zb4;
+        double za5, zb5;
+        plom_normal_pair64(ns, &za5, &zb5);
+        const double d2_2 = sqdt * za5;
+        const double w0_2 = sqdt * zb5;
+        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - plom_sqrt((r0_2*X[2]))*d0_2+ plom_sqrt((r2_2))*d2_2 + - (r0_2*X[2])*P18*w0_2;
+        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[6]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
+        s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
+        const double r0_3 = (((P3 * (1.0 + (P7 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r1_3 = P11;
+        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        double za6, zb6;
+        plom_normal_pair64(ns, &za6, &zb6);
+        const double d0_3 = sqdt * za6;
+        const double d1_3 = sqdt * zb6;
+        double za7, zb7;
+        plom_normal_pair64(ns, &za7, &zb7);
+        const double d2_3 = sqdt * za7;
+        const double w0_3 = sqdt * zb7;
+        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - plom_sqrt((r0_3*X[3]))*d0_3+ plom_sqrt((r2_3))*d2_3 + - (r0_3*X[3])*P19*w0_3;
+        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - plom_sqrt((r1_3*X[7]))*d1_3+ plom_sqrt((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
+        s11 += ( + (r0_3*X[3]))*dtr + + plom_sqrt((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
+        const double f8 = X[8] + s8;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
+        const double f9 = X[9] + s9;
         X[9] = (f9 < 0.0) ? 0.0 : f9;
+        const double f10 = X[10] + s10;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
+        const double f11 = X[11] + s11;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
         t += dt;
     }
@@ -92,6 +148,46 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
 {
     const double dtr = (double)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P2 = PLOM_UNIFORM64(pv[2]);
+    const double P3 = PLOM_UNIFORM64(pv[3]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    const double P5 = PLOM_UNIFORM64(pv[5]);
+    const double P6 = PLOM_UNIFORM64(pv[6]);
+    const double P7 = PLOM_UNIFORM64(pv[7]);
+    const double P8 = PLOM_UNIFORM64(pv[8]);
+    const double P9 = PLOM_UNIFORM64(pv[9]);
+    const double P10 = PLOM_UNIFORM64(pv[10]);
+    const double P11 = PLOM_UNIFORM64(pv[11]);
+    const double P12 = PLOM_UNIFORM64(pv[12]);
+    const double P13 = PLOM_UNIFORM64(pv[13]);
+    const double P14 = PLOM_UNIFORM64(pv[14]);
+    const double P15 = PLOM_UNIFORM64(pv[15]);
+    const double P16 = PLOM_UNIFORM64(pv[16]);
+    const double P17 = PLOM_UNIFORM64(pv[17]);
+    const double P18 = PLOM_UNIFORM64(pv[18]);
+    const double P19 = PLOM_UNIFORM64(pv[19]);
+    const double P28 = PLOM_UNIFORM64(pv[28]);
+    const double P29 = PLOM_UNIFORM64(pv[29]);
+    const double P30 = PLOM_UNIFORM64(pv[30]);
+    const double P31 = PLOM_UNIFORM64(pv[31]);
+    const double P32 = PLOM_UNIFORM64(pv[32]);
+    const double P33 = PLOM_UNIFORM64(pv[33]);
+    const double P34 = PLOM_UNIFORM64(pv[34]);
+    const double P35 = PLOM_UNIFORM64(pv[35]);
+    const double P36 = PLOM_UNIFORM64(pv[36]);
+    const double P37 = PLOM_UNIFORM64(pv[37]);
+    const double P38 = PLOM_UNIFORM64(pv[38]);
+    const double P39 = PLOM_UNIFORM64(pv[39]);
+    const double P40 = PLOM_UNIFORM64(pv[40]);
+    const double P41 = PLOM_UNIFORM64(pv[41]);
+    const double P42 = PLOM_UNIFORM64(pv[42]);
+    const double P43 = PLOM_UNIFORM64(pv[43]);
+    const double P44 = PLOM_UNIFORM64(pv[44]);
+    const double P45 = PLOM_UNIFORM64(pv[45]);
+    const double P46 = PLOM_UNIFORM64(pv[46]);
+    const double P47 = PLOM_UNIFORM64(pv[47]);
     X[8] = 0.0; /* reset_inc */
     X[9] = 0.0; /* reset_inc */
     X[10] = 0.0; /* reset_inc */
@@ -99,53 +195,57 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
     double t = t0;
     while (t < t1) {
         const double tr = (double)t; (void)tr;
-        const double r0_0 = (((pv[0] * (1.0 + (pv[4] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[32] * X[4] + pv[33] * X[5] + pv[34] * X[6] + pv[35] * X[7])) / pv[28]);
-        const double r1_0 = pv[8];
-        const double r2_0 = (pv[12] * ((pv[28] - X[0]) - X[4]));
-        const double w0_0 = sqdt * plom_normal64(ns);
-        const double r0_1 = (((pv[1] * (1.0 + (pv[5] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[36] * X[4] + pv[37] * X[5] + pv[38] * X[6] + pv[39] * X[7])) / pv[29]);
-        const double r1_1 = pv[9];
-        const double r2_1 = (pv[13] * ((pv[29] - X[1]) - X[5]));
-        const double w0_1 = sqdt * plom_normal64(ns);
-        const double r0_2 = (((pv[2] * (1.0 + (pv[6] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[40] * X[4] + pv[41] * X[5] + pv[42] * X[6] + pv[43] * X[7])) / pv[30]);
-        const double r1_2 = pv[10];
-        const double r2_2 = (pv[14] * ((pv[30] - X[2]) - X[6]));
-        const double w0_2 = sqdt * plom_normal64(ns);
-        const double r0_3 = (((pv[3] * (1.0 + (pv[7] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[44] * X[4] + pv[45] * X[5] + pv[46] * X[6] + pv[47] * X[7])) / pv[31]);
-        const double r1_3 = pv[11];
-        const double r2_3 = (pv[15] * ((pv[31] - X[3]) - X[7]));
-        const double w0_3 = sqdt * plom_normal64(ns);
-        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - (r0_0*X[0])*pv[16]*w0_0;
-        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - (r0_1*X[1])*pv[17]*w0_1;
-        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - (r0_2*X[2])*pv[18]*w0_2;
-        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - (r0_3*X[3])*pv[19]*w0_3;
-        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + + (r0_0*X[0])*pv[16]*w0_0;
-        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*pv[17]*w0_1;
-        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + + (r0_2*X[2])*pv[18]*w0_2;
-        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + + (r0_3*X[3])*pv[19]*w0_3;
         double s8 = 0.0;
-        s8 += ( + (r0_0*X[0]))*dtr + + (r0_0*X[0])*pv[16]*w0_0;
-        const double f8 = X[8] + s8;
         double s9 = 0.0;
-        s9 += ( + (r0_1*X[1]))*dtr + + (r0_1*X[1])*pv[17]*w0_1;
-        const double f9 = X[9] + s9;
         double s10 = 0.0;
-        s10 += ( + (r0_2*X[2]))*dtr + + (r0_2*X[2])*pv[18]*w0_2;
-        const double f10 = X[10] + s10;
         double s11 = 0.0;
-        s11 += ( + (r0_3*X[3]))*dtr + + (r0_3*X[3])*pv[19]*w0_3;
-        const double f11 = X[11] + s11;
+        const double r0_0 = (((P0 * (1.0 + (P4 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r1_0 = P8;
+        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        double za0, zb0;
+        plom_normal_pair64(ns, &za0, &zb0);
+        const double w0_0 = sqdt * za0;
+        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - (r0_0*X[0])*P16*w0_0;
+        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
+        s8 += ( + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
+        const double r0_1 = (((P1 * (1.0 + (P5 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r1_1 = P9;
+        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const double w0_1 = sqdt * zb0;
+        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - (r0_1*X[1])*P17*w0_1;
+        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
+        s9 += ( + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
+        const double r0_2 = (((P2 * (1.0 + (P6 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r1_2 = P10;
+        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        double za1, zb1;
+        plom_normal_pair64(ns, &za1, &zb1);
+        const double w0_2 = sqdt * za1;
+        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - (r0_2*X[2])*P18*w0_2;
+        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
+        s10 += ( + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
+        const double r0_3 = (((P3 * (1.0 + (P7 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r1_3 = P11;
+        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double w0_3 = sqdt * zb1;
+        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - (r0_3*X[3])*P19*w0_3;
+        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + + (r0_3*X[3])*P19*w0_3;
+        s11 += ( + (r0_3*X[3]))*dtr + + (r0_3*X[3])*P19*w0_3;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
+        const double f8 = X[8] + s8;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
+        const double f9 = X[9] + s9;
         X[9] = (f9 < 0.0) ? 0.0 : f9;
+        const double f10 = X[10] + s10;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
+        const double f11 = X[11] + s11;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
         t += dt;
     }
@@ -155,6 +255,42 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
 {
     const double dtr = (double)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P2 = PLOM_UNIFORM64(pv[2]);
+    const double P3 = PLOM_UNIFORM64(pv[3]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    const double P5 = PLOM_UNIFORM64(pv[5]);
+    const double P6 = PLOM_UNIFORM64(pv[6]);
+    const double P7 = PLOM_UNIFORM64(pv[7]);
+    const double P8 = PLOM_UNIFORM64(pv[8]);
+    const double P9 = PLOM_UNIFORM64(pv[9]);
+    const double P10 = PLOM_UNIFORM64(pv[10]);
+    const double P11 = PLOM_UNIFORM64(pv[11]);
+    const double P12 = PLOM_UNIFORM64(pv[12]);
+    const double P13 = PLOM_UNIFORM64(pv[13]);
+    const double P14 = PLOM_UNIFORM64(pv[14]);
+    const double P15 = PLOM_UNIFORM64(pv[15]);
+    const double P28 = PLOM_UNIFORM64(pv[28]);
+    const double P29 = PLOM_UNIFORM64(pv[29]);
+    const double P30 = PLOM_UNIFORM64(pv[30]);
+    const double P31 = PLOM_UNIFORM64(pv[31]);
+    const double P32 = PLOM_UNIFORM64(pv[32]);
+    const double P33 = PLOM_UNIFORM64(pv[33]);
+    const double P34 = PLOM_UNIFORM64(pv[34]);
+    const double P35 = PLOM_UNIFORM64(pv[35]);
+    const double P36 = PLOM_UNIFORM64(pv[36]);
+    const double P37 = PLOM_UNIFORM64(pv[37]);
+    const double P38 = PLOM_UNIFORM64(pv[38]);
+    const double P39 = PLOM_UNIFORM64(pv[39]);
+    const double P40 = PLOM_UNIFORM64(pv[40]);
+    const double P41 = PLOM_UNIFORM64(pv[41]);
+    const double P42 = PLOM_UNIFORM64(pv[42]);
+    const double P43 = PLOM_UNIFORM64(pv[43]);
+    const double P44 = PLOM_UNIFORM64(pv[44]);
+    const double P45 = PLOM_UNIFORM64(pv[45]);
+    const double P46 = PLOM_UNIFORM64(pv[46]);
+    const double P47 = PLOM_UNIFORM64(pv[47]);
     X[8] = 0.0; /* reset_inc */
     X[9] = 0.0; /* reset_inc */
     X[10] = 0.0; /* reset_inc */
@@ -162,61 +298,73 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
     double t = t0;
     while (t < t1) {
         const double tr = (double)t; (void)tr;
-        const double r0_0 = (((pv[0] * (1.0 + (pv[4] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[32] * X[4] + pv[33] * X[5] + pv[34] * X[6] + pv[35] * X[7])) / pv[28]);
-        const double r1_0 = pv[8];
-        const double r2_0 = (pv[12] * ((pv[28] - X[0]) - X[4]));
-        const double d0_0 = sqdt * plom_normal64(ns);
-        const double d1_0 = sqdt * plom_normal64(ns);
-        const double d2_0 = sqdt * plom_normal64(ns);
-        const double r0_1 = (((pv[1] * (1.0 + (pv[5] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[36] * X[4] + pv[37] * X[5] + pv[38] * X[6] + pv[39] * X[7])) / pv[29]);
-        const double r1_1 = pv[9];
-        const double r2_1 = (pv[13] * ((pv[29] - X[1]) - X[5]));
-        const double d0_1 = sqdt * plom_normal64(ns);
-        const double d1_1 = sqdt * plom_normal64(ns);
-        const double d2_1 = sqdt * plom_normal64(ns);
-        const double r0_2 = (((pv[2] * (1.0 + (pv[6] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[40] * X[4] + pv[41] * X[5] + pv[42] * X[6] + pv[43] * X[7])) / pv[30]);
-        const double r1_2 = pv[10];
-        const double r2_2 = (pv[14] * ((pv[30] - X[2]) - X[6]));
-        const double d0_2 = sqdt * plom_normal64(ns);
-        const double d1_2 = sqdt * plom_normal64(ns);
-        const double d2_2 = sqdt * plom_normal64(ns);
-        const double r0_3 = (((pv[3] * (1.0 + (pv[7] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[44] * X[4] + pv[45] * X[5] + pv[46] * X[6] + pv[47] * X[7])) / pv[31]);
-        const double r1_3 = pv[11];
-        const double r2_3 = (pv[15] * ((pv[31] - X[3]) - X[7]));
-        const double d0_3 = sqdt * plom_normal64(ns);
-        const double d1_3 = sqdt * plom_normal64(ns);
-        const double d2_3 = sqdt * plom_normal64(ns);
-        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0+ plom_sqrt((r2_0))*d2_0;
-        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - plom_sqrt((r0_1*X[1]))*d0_1+ plom_sqrt((r2_1))*d2_1;
-        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - plom_sqrt((r0_2*X[2]))*d0_2+ plom_sqrt((r2_2))*d2_2;
-        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - plom_sqrt((r0_3*X[3]))*d0_3+ plom_sqrt((r2_3))*d2_3;
-        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[4]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
-        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[5]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1;
-        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[6]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2;
-        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - plom_sqrt((r1_3*X[7]))*d1_3+ plom_sqrt((r0_3*X[3]))*d0_3;
         double s8 = 0.0;
-        s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0;
-        const double f8 = X[8] + s8;
         double s9 = 0.0;
-        s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt((r0_1*X[1]))*d0_1;
-        const double f9 = X[9] + s9;
         double s10 = 0.0;
-        s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt((r0_2*X[2]))*d0_2;
-        const double f10 = X[10] + s10;
         double s11 = 0.0;
-        s11 += ( + (r0_3*X[3]))*dtr + + plom_sqrt((r0_3*X[3]))*d0_3;
-        const double f11 = X[11] + s11;
+        const double r0_0 = (((P0 * (1.0 + (P4 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r1_0 = P8;
+        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        double za0, zb0;
+        plom_normal_pair64(ns, &za0, &zb0);
+        const double d0_0 = sqdt * za0;
+        const double d1_0 = sqdt * zb0;
+        double za1, zb1;
+        plom_normal_pair64(ns, &za1, &zb1);
+        const double d2_0 = sqdt * za1;
+        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0+ plom_sqrt((r2_0))*d2_0;
+        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[4]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
+        s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
+        const double r0_1 = (((P1 * (1.0 + (P5 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r1_1 = P9;
+        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const double d0_1 = sqdt * zb1;
+        double za2, zb2;
+        plom_normal_pair64(ns, &za2, &zb2);
+        const double d1_1 = sqdt * za2;
+        const double d2_1 = sqdt * zb2;
+        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - plom_sqrt((r0_1*X[1]))*d0_1+ plom_sqrt((r2_1))*d2_1;
+        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[5]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1;
+        s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt((r0_1*X[1]))*d0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
+        const double r0_2 = (((P2 * (1.0 + (P6 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r1_2 = P10;
+        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        double za3, zb3;
+        plom_normal_pair64(ns, &za3, &zb3);
+        const double d0_2 = sqdt * za3;
+        const double d1_2 = sqdt * zb3;
+        double za4, zb4;
+        plom_normal_pair64(ns, &za4, &zb4);
+        const double d2_2 = sqdt * za4;
+        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - plom_sqrt((r0_2*X[2]))*d0_2+ plom_sqrt((r2_2))*d2_2;
+        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[6]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2;
+        s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt((r0_2*X[2]))*d0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
+        const double r0_3 = (((P3 * (1.0 + (P7 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r1_3 = P11;
+        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double d0_3 = sqdt * zb4;
+        double za5, zb5;
+        plom_normal_pair64(ns, &za5, &zb5);
+        const double d1_3 = sqdt * za5;
+        const double d2_3 = sqdt * zb5;
+        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - plom_sqrt((r0_3*X[3]))*d0_3+ plom_sqrt((r2_3))*d2_3;
+        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - plom_sqrt((r1_3*X[7]))*d1_3+ plom_sqrt((r0_3*X[3]))*d0_3;
+        s11 += ( + (r0_3*X[3]))*dtr + + plom_sqrt((r0_3*X[3]))*d0_3;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
+        const double f8 = X[8] + s8;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
+        const double f9 = X[9] + s9;
         X[9] = (f9 < 0.0) ? 0.0 : f9;
+        const double f10 = X[10] + s10;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
+        const double f11 = X[11] + s11;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
         t += dt;
     }
@@ -226,6 +374,42 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_no_env_sto_f64(double *X, const doub
 {
     const double dtr = (double)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P2 = PLOM_UNIFORM64(pv[2]);
+    const double P3 = PLOM_UNIFORM64(pv[3]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    const double P5 = PLOM_UNIFORM64(pv[5]);
+    const double P6 = PLOM_UNIFORM64(pv[6]);
+    const double P7 = PLOM_UNIFORM64(pv[7]);
+    const double P8 = PLOM_UNIFORM64(pv[8]);
+    const double P9 = PLOM_UNIFORM64(pv[9]);
+    const double P10 = PLOM_UNIFORM64(pv[10]);
+    const double P11 = PLOM_UNIFORM64(pv[11]);
+    const double P12 = PLOM_UNIFORM64(pv[12]);
+    const double P13 = PLOM_UNIFORM64(pv[13]);
+    const double P14 = PLOM_UNIFORM64(pv[14]);
+    const double P15 = PLOM_UNIFORM64(pv[15]);
+    const double P28 = PLOM_UNIFORM64(pv[28]);
+    const double P29 = PLOM_UNIFORM64(pv[29]);
+    const double P30 = PLOM_UNIFORM64(pv[30]);
+    const double P31 = PLOM_UNIFORM64(pv[31]);
+    const double P32 = PLOM_UNIFORM64(pv[32]);
+    const double P33 = PLOM_UNIFORM64(pv[33]);
+    const double P34 = PLOM_UNIFORM64(pv[34]);
+    const double P35 = PLOM_UNIFORM64(pv[35]);
+    const double P36 = PLOM_UNIFORM64(pv[36]);
+    const double P37 = PLOM_UNIFORM64(pv[37]);
+    const double P38 = PLOM_UNIFORM64(pv[38]);
+    const double P39 = PLOM_UNIFORM64(pv[39]);
+    const double P40 = PLOM_UNIFORM64(pv[40]);
+    const double P41 = PLOM_UNIFORM64(pv[41]);
+    const double P42 = PLOM_UNIFORM64(pv[42]);
+    const double P43 = PLOM_UNIFORM64(pv[43]);
+    const double P44 = PLOM_UNIFORM64(pv[44]);
+    const double P45 = PLOM_UNIFORM64(pv[45]);
+    const double P46 = PLOM_UNIFORM64(pv[46]);
+    const double P47 = PLOM_UNIFORM64(pv[47]);
     X[8] = 0.0; /* reset_inc */
     X[9] = 0.0; /* reset_inc */
     X[10] = 0.0; /* reset_inc */
@@ -233,49 +417,49 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_no_env_sto_f64(double *X, const doub
     double t = t0;
     while (t < t1) {
         const double tr = (double)t; (void)tr;
-        const double r0_0 = (((pv[0] * (1.0 + (pv[4] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[32] * X[4] + pv[33] * X[5] + pv[34] * X[6] + pv[35] * X[7])) / pv[28]);
-        const double r1_0 = pv[8];
-        const double r2_0 = (pv[12] * ((pv[28] - X[0]) - X[4]));
-        const double r0_1 = (((pv[1] * (1.0 + (pv[5] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[36] * X[4] + pv[37] * X[5] + pv[38] * X[6] + pv[39] * X[7])) / pv[29]);
-        const double r1_1 = pv[9];
-        const double r2_1 = (pv[13] * ((pv[29] - X[1]) - X[5]));
-        const double r0_2 = (((pv[2] * (1.0 + (pv[6] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[40] * X[4] + pv[41] * X[5] + pv[42] * X[6] + pv[43] * X[7])) / pv[30]);
-        const double r1_2 = pv[10];
-        const double r2_2 = (pv[14] * ((pv[30] - X[2]) - X[6]));
-        const double r0_3 = (((pv[3] * (1.0 + (pv[7] * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (pv[44] * X[4] + pv[45] * X[5] + pv[46] * X[6] + pv[47] * X[7])) / pv[31]);
-        const double r1_3 = pv[11];
-        const double r2_3 = (pv[15] * ((pv[31] - X[3]) - X[7]));
-        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr;
-        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr;
-        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr;
-        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr;
-        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr;
-        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr;
-        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr;
-        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr;
         double s8 = 0.0;
-        s8 += ( + (r0_0*X[0]))*dtr;
-        const double f8 = X[8] + s8;
         double s9 = 0.0;
-        s9 += ( + (r0_1*X[1]))*dtr;
-        const double f9 = X[9] + s9;
         double s10 = 0.0;
-        s10 += ( + (r0_2*X[2]))*dtr;
-        const double f10 = X[10] + s10;
         double s11 = 0.0;
-        s11 += ( + (r0_3*X[3]))*dtr;
-        const double f11 = X[11] + s11;
+        const double r0_0 = (((P0 * (1.0 + (P4 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r1_0 = P8;
+        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr;
+        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr;
+        s8 += ( + (r0_0*X[0]))*dtr;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
+        const double r0_1 = (((P1 * (1.0 + (P5 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r1_1 = P9;
+        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr;
+        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr;
+        s9 += ( + (r0_1*X[1]))*dtr;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
+        const double r0_2 = (((P2 * (1.0 + (P6 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r1_2 = P10;
+        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr;
+        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr;
+        s10 += ( + (r0_2*X[2]))*dtr;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
+        const double r0_3 = (((P3 * (1.0 + (P7 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r1_3 = P11;
+        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr;
+        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr;
+        s11 += ( + (r0_3*X[3]))*dtr;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
+        const double f8 = X[8] + s8;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
+        const double f9 = X[9] + s9;
         X[9] = (f9 < 0.0) ? 0.0 : f9;
+        const double f10 = X[10] + s10;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
+        const double f11 = X[11] + s11;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
         t += dt;
     }
@@ -349,6 +533,46 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
 {
     const float dtr = (float)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const float P0 = PLOM_UNIFORM32(pv[0]);
+    const float P1 = PLOM_UNIFORM32(pv[1]);
+    const float P2 = PLOM_UNIFORM32(pv[2]);
+    const float P3 = PLOM_UNIFORM32(pv[3]);
+    const float P4 = PLOM_UNIFORM32(pv[4]);
+    const float P5 = PLOM_UNIFORM32(pv[5]);
+    const float P6 = PLOM_UNIFORM32(pv[6]);
+    const float P7 = PLOM_UNIFORM32(pv[7]);
+    const float P8 = PLOM_UNIFORM32(pv[8]);
+    const float P9 = PLOM_UNIFORM32(pv[9]);
+    const float P10 = PLOM_UNIFORM32(pv[10]);
+    const float P11 = PLOM_UNIFORM32(pv[11]);
+    const float P12 = PLOM_UNIFORM32(pv[12]);
+    const float P13 = PLOM_UNIFORM32(pv[13]);
+    const float P14 = PLOM_UNIFORM32(pv[14]);
+    const float P15 = PLOM_UNIFORM32(pv[15]);
+    const float P16 = PLOM_UNIFORM32(pv[16]);
+    const float P17 = PLOM_UNIFORM32(pv[17]);
+    const float P18 = PLOM_UNIFORM32(pv[18]);
+    const float P19 = PLOM_UNIFORM32(pv[19]);
+    const float P28 = PLOM_UNIFORM32(pv[28]);
+    const float P29 = PLOM_UNIFORM32(pv[29]);
+    const float P30 = PLOM_UNIFORM32(pv[30]);
+    const float P31 = PLOM_UNIFORM32(pv[31]);
+    const float P32 = PLOM_UNIFORM32(pv[32]);
+    const float P33 = PLOM_UNIFORM32(pv[33]);
+    const float P34 = PLOM_UNIFORM32(pv[34]);
+    const float P35 = PLOM_UNIFORM32(pv[35]);
+    const float P36 = PLOM_UNIFORM32(pv[36]);
+    const float P37 = PLOM_UNIFORM32(pv[37]);
+    const float P38 = PLOM_UNIFORM32(pv[38]);
+    const float P39 = PLOM_UNIFORM32(pv[39]);
+    const float P40 = PLOM_UNIFORM32(pv[40]);
+    const float P41 = PLOM_UNIFORM32(pv[41]);
+    const float P42 = PLOM_UNIFORM32(pv[42]);
+    const float P43 = PLOM_UNIFORM32(pv[43]);
+    const float P44 = PLOM_UNIFORM32(pv[44]);
+    const float P45 = PLOM_UNIFORM32(pv[45]);
+    const float P46 = PLOM_UNIFORM32(pv[46]);
+    const float P47 = PLOM_UNIFORM32(pv[47]);
     X[8] = 0.0f; /* reset_inc */
     X[9] = 0.0f; /* reset_inc */
     X[10] = 0.0f; /* reset_inc */
@@ -356,65 +580,81 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     double t = t0;
     while (t < t1) {
         const float tr = (float)t; (void)tr;
-        const float r0_0 = (((pv[0] * (1.0f + (pv[4] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[32] * X[4] + pv[33] * X[5] + pv[34] * X[6] + pv[35] * X[7])) / pv[28]);
-        const float r1_0 = pv[8];
-        const float r2_0 = (pv[12] * ((pv[28] - X[0]) - X[4]));
-        const float d0_0 = sqdt * plom_normal32(ns);
-        const float d1_0 = sqdt * plom_normal32(ns);
-        const float d2_0 = sqdt * plom_normal32(ns);
-        const float w0_0 = sqdt * plom_normal32(ns);
-        const float r0_1 = (((pv[1] * (1.0f + (pv[5] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[36] * X[4] + pv[37] * X[5] + pv[38] * X[6] + pv[39] * X[7])) / pv[29]);
-        const float r1_1 = pv[9];
-        const float r2_1 = (pv[13] * ((pv[29] - X[1]) - X[5]));
-        const float d0_1 = sqdt * plom_normal32(ns);
-        const float d1_1 = sqdt * plom_normal32(ns);
-        const float d2_1 = sqdt * plom_normal32(ns);
-        const float w0_1 = sqdt * plom_normal32(ns);
-        const float r0_2 = (((pv[2] * (1.0f + (pv[6] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[40] * X[4] + pv[41] * X[5] + pv[42] * X[6] + pv[43] * X[7])) / pv[30]);
-        const float r1_2 = pv[10];
-        const float r2_2 = (pv[14] * ((pv[30] - X[2]) - X[6]));
-        const float d0_2 = sqdt * plom_normal32(ns);
-        const float d1_2 = sqdt * plom_normal32(ns);
-        const float d2_2 = sqdt * plom_normal32(ns);
-        const float w0_2 = sqdt * plom_normal32(ns);
-        const float r0_3 = (((pv[3] * (1.0f + (pv[7] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[44] * X[4] + pv[45] * X[5] + pv[46] * X[6] + pv[47] * X[7])) / pv[31]);
-        const float r1_3 = pv[11];
-        const float r2_3 = (pv[15] * ((pv[31] - X[3]) - X[7]));
-        const float d0_3 = sqdt * plom_normal32(ns);
-        const float d1_3 = sqdt * plom_normal32(ns);
-        const float d2_3 = sqdt * plom_normal32(ns);
-        const float w0_3 = sqdt * plom_normal32(ns);
-        const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - __builtin_sqrtf((r0_0*X[0]))*d0_0+ __builtin_sqrtf((r2_0))*d2_0 + - (r0_0*X[0])*pv[16]*w0_0;
-        const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - __builtin_sqrtf((r0_1*X[1]))*d0_1+ __builtin_sqrtf((r2_1))*d2_1 + - (r0_1*X[1])*pv[17]*w0_1;
-        const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - __builtin_sqrtf((r0_2*X[2]))*d0_2+ __builtin_sqrtf((r2_2))*d2_2 + - (r0_2*X[2])*pv[18]*w0_2;
-        const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - __builtin_sqrtf((r0_3*X[3]))*d0_3+ __builtin_sqrtf((r2_3))*d2_3 + - (r0_3*X[3])*pv[19]*w0_3;
-        const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - __builtin_sqrtf((r1_0*X[4]))*d1_0+ __builtin_sqrtf((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*pv[16]*w0_0;
-        const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - __builtin_sqrtf((r1_1*X[5]))*d1_1+ __builtin_sqrtf((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*pv[17]*w0_1;
-        const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - __builtin_sqrtf((r1_2*X[6]))*d1_2+ __builtin_sqrtf((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*pv[18]*w0_2;
-        const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - __builtin_sqrtf((r1_3*X[7]))*d1_3+ __builtin_sqrtf((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*pv[19]*w0_3;
         float s8 = 0.0f;
-        s8 += ( + (r0_0*X[0]))*dtr + + __builtin_sqrtf((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*pv[16]*w0_0;
-        const float f8 = X[8] + s8;
         float s9 = 0.0f;
-        s9 += ( + (r0_1*X[1]))*dtr + + __builtin_sqrtf((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*pv[17]*w0_1;
-        const float f9 = X[9] + s9;
         float s10 = 0.0f;
-        s10 += ( + (r0_2*X[2]))*dtr + + __builtin_sqrtf((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*pv[18]*w0_2;
-        const float f10 = X[10] + s10;
         float s11 = 0.0f;
-        s11 += ( + (r0_3*X[3]))*dtr + + __builtin_sqrtf((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*pv[19]*w0_3;
-        const float f11 = X[11] + s11;
+        const float r0_0 = (((P0 * (1.0f + (P4 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const float r1_0 = P8;
+        const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        float za0, zb0;
+        plom_normal_pair32(ns, &za0, &zb0);
+        const float d0_0 = sqdt * za0;
+        const float d1_0 = sqdt * zb0;
+        float za1, zb1;
+        plom_normal_pair32(ns, &za1, &zb1);
+        const float d2_0 = sqdt * za1;
+        const float w0_0 = sqdt * zb1;
+        const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - __builtin_sqrtf((r0_0*X[0]))*d0_0+ __builtin_sqrtf((r2_0))*d2_0 + - (r0_0*X[0])*P16*w0_0;
+        const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - __builtin_sqrtf((r1_0*X[4]))*d1_0+ __builtin_sqrtf((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
+        s8 += ( + (r0_0*X[0]))*dtr + + __builtin_sqrtf((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0f) ? 0.0f : f0;
+        const float r0_1 = (((P1 * (1.0f + (P5 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const float r1_1 = P9;
+        const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        float za2, zb2;
+        plom_normal_pair32(ns, &za2, &zb2);
+        const float d0_1 = sqdt * za2;
+        const float d1_1 = sqdt * zb2;
+        float za3, zb3;
+        plom_normal_pair32(ns, &za3, &zb3);
+        const float d2_1 = sqdt * za3;
+        const float w0_1 = sqdt * zb3;
+        const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - __builtin_sqrtf((r0_1*X[1]))*d0_1+ __builtin_sqrtf((r2_1))*d2_1 + - (r0_1*X[1])*P17*w0_1;
+        const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - __builtin_sqrtf((r1_1*X[5]))*d1_1+ __builtin_sqrtf((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
+        s9 += ( + (r0_1*X[1]))*dtr + + __builtin_sqrtf((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0f) ? 0.0f : f1;
+        const float r0_2 = (((P2 * (1.0f + (P6 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const float r1_2 = P10;
+        const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        float za4, zb4;
+        plom_normal_pair32(ns, &za4, &zb4);
+        const float d0_2 = sqdt * za4;
+        const float d1_2 = sqdt * zb4;
+        float za5, zb5;
+        plom_normal_pair32(ns, &za5, &zb5);
+        const float d2_2 = sqdt * za5;
+        const float w0_2 = sqdt * zb5;
+        const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - __builtin_sqrtf((r0_2*X[2]))*d0_2+ __builtin_sqrtf((r2_2))*d2_2 + - (r0_2*X[2])*P18*w0_2;
+        const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - __builtin_sqrtf((r1_2*X[6]))*d1_2+ __builtin_sqrtf((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
+        s10 += ( + (r0_2*X[2]))*dtr + + __builtin_sqrtf((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0f) ? 0.0f : f2;
+        const float r0_3 = (((P3 * (1.0f + (P7 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const float r1_3 = P11;
+        const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        float za6, zb6;
+        plom_normal_pair32(ns, &za6, &zb6);
+        const float d0_3 = sqdt * za6;
+        const float d1_3 = sqdt * zb6;
+        float za7, zb7;
+        plom_normal_pair32(ns, &za7, &zb7);
+        const float d2_3 = sqdt * za7;
+        const float w0_3 = sqdt * zb7;
+        const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - __builtin_sqrtf((r0_3*X[3]))*d0_3+ __builtin_sqrtf((r2_3))*d2_3 + - (r0_3*X[3])*P19*w0_3;
+        const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - __builtin_sqrtf((r1_3*X[7]))*d1_3+ __builtin_sqrtf((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
+        s11 += ( + (r0_3*X[3]))*dtr + + __builtin_sqrtf((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
         X[3] = (f3 < 0.0f) ? 0.0f : f3;
         X[4] = (f4 < 0.0f) ? 0.0f : f4;
         X[5] = (f5 < 0.0f) ? 0.0f : f5;
         X[6] = (f6 < 0.0f) ? 0.0f : f6;
         X[7] = (f7 < 0.0f) ? 0.0f : f7;
+        const float f8 = X[8] + s8;
         X[8] = (f8 < 0.0f) ? 0.0f : f8;
+        const float f9 = X[9] + s9;
         X[9] = (f9 < 0.0f) ? 0.0f : f9;
+        const float f10 = X[10] + s10;
         X[10] = (f10 < 0.0f) ? 0.0f : f10;
+        const float f11 = X[11] + s11;
         X[11] = (f11 < 0.0f) ? 0.0f : f11;
         t += dt;
     }
@@ -424,6 +664,46 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
 {
     const float dtr = (float)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const float P0 = PLOM_UNIFORM32(pv[0]);
+    const float P1 = PLOM_UNIFORM32(pv[1]);
+    const float P2 = PLOM_UNIFORM32(pv[2]);
+    const float P3 = PLOM_UNIFORM32(pv[3]);
+    const float P4 = PLOM_UNIFORM32(pv[4]);
+    const float P5 = PLOM_UNIFORM32(pv[5]);
+    const float P6 = PLOM_UNIFORM32(pv[6]);
+    const float P7 = PLOM_UNIFORM32(pv[7]);
+    const float P8 = PLOM_UNIFORM32(pv[8]);
+    const float P9 = PLOM_UNIFORM32(pv[9]);
+    const float P10 = PLOM_UNIFORM32(pv[10]);
+    const float P11 = PLOM_UNIFORM32(pv[11]);
+    const float P12 = PLOM_UNIFORM32(pv[12]);
+    const float P13 = PLOM_UNIFORM32(pv[13]);
+    const float P14 = PLOM_UNIFORM32(pv[14]);
+    const float P15 = PLOM_UNIFORM32(pv[15]);
+    const float P16 = PLOM_UNIFORM32(pv[16]);
+    const float P17 = PLOM_UNIFORM32(pv[17]);
+    const float P18 = PLOM_UNIFORM32(pv[18]);
+    const float P19 = PLOM_UNIFORM32(pv[19]);
+    const float P28 = PLOM_UNIFORM32(pv[28]);
+    const float P29 = PLOM_UNIFORM32(pv[29]);
+    const float P30 = PLOM_UNIFORM32(pv[30]);
+    const float P31 = PLOM_UNIFORM32(pv[31]);
+    const float P32 = PLOM_UNIFORM32(pv[32]);
+    const float P33 = PLOM_UNIFORM32(pv[33]);
+    const float P34 = PLOM_UNIFORM32(pv[34]);
+    const float P35 = PLOM_UNIFORM32(pv[35]);
+    const float P36 = PLOM_UNIFORM32(pv[36]);
+    const float P37 = PLOM_UNIFORM32(pv[37]);
+    const float P38 = PLOM_UNIFORM32(pv[38]);
+    const float P39 = PLOM_UNIFORM32(pv[39]);
+    const float P40 = PLOM_UNIFORM32(pv[40]);
+    const float P41 = PLOM_UNIFORM32(pv[41]);
+    const float P42 = PLOM_UNIFORM32(pv[42]);
+    const float P43 = PLOM_UNIFORM32(pv[43]);
+    const float P44 = PLOM_UNIFORM32(pv[44]);
+    const float P45 = PLOM_UNIFORM32(pv[45]);
+    const float P46 = PLOM_UNIFORM32(pv[46]);
+    const float P47 = PLOM_UNIFORM32(pv[47]);
     X[8] = 0.0f; /* reset_inc */
     X[9] = 0.0f; /* reset_inc */
     X[10] = 0.0f; /* reset_inc */
@@ -431,53 +711,57 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     double t = t0;
     while (t < t1) {
         const float tr = (float)t; (void)tr;
-        const float r0_0 = (((pv[0] * (1.0f + (pv[4] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[32] * X[4] + pv[33] * X[5] + pv[34] * X[6] + pv[35] * X[7])) / pv[28]);
-        const float r1_0 = pv[8];
-        const float r2_0 = (pv[12] * ((pv[28] - X[0]) - X[4]));
-        const float w0_0 = sqdt * plom_normal32(ns);
-        const float r0_1 = (((pv[1] * (1.0f + (pv[5] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[36] * X[4] + pv[37] * X[5] + pv[38] * X[6] + pv[39] * X[7])) / pv[29]);
-        const float r1_1 = pv[9];
-        const float r2_1 = (pv[13] * ((pv[29] - X[1]) - X[5]));
-        const float w0_1 = sqdt * plom_normal32(ns);
-        const float r0_2 = (((pv[2] * (1.0f + (pv[6] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[40] * X[4] + pv[41] * X[5] + pv[42] * X[6] + pv[43] * X[7])) / pv[30]);
-        const float r1_2 = pv[10];
-        const float r2_2 = (pv[14] * ((pv[30] - X[2]) - X[6]));
-        const float w0_2 = sqdt * plom_normal32(ns);
-        const float r0_3 = (((pv[3] * (1.0f + (pv[7] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[44] * X[4] + pv[45] * X[5] + pv[46] * X[6] + pv[47] * X[7])) / pv[31]);
-        const float r1_3 = pv[11];
-        const float r2_3 = (pv[15] * ((pv[31] - X[3]) - X[7]));
-        const float w0_3 = sqdt * plom_normal32(ns);
-        const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - (r0_0*X[0])*pv[16]*w0_0;
-        const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - (r0_1*X[1])*pv[17]*w0_1;
-        const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - (r0_2*X[2])*pv[18]*w0_2;
-        const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - (r0_3*X[3])*pv[19]*w0_3;
-        const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + + (r0_0*X[0])*pv[16]*w0_0;
-        const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*pv[17]*w0_1;
-        const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + + (r0_2*X[2])*pv[18]*w0_2;
-        const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + + (r0_3*X[3])*pv[19]*w0_3;
         float s8 = 0.0f;
-        s8 += ( + (r0_0*X[0]))*dtr + + (r0_0*X[0])*pv[16]*w0_0;
-        const float f8 = X[8] + s8;
         float s9 = 0.0f;
-        s9 += ( + (r0_1*X[1]))*dtr + + (r0_1*X[1])*pv[17]*w0_1;
-        const float f9 = X[9] + s9;
         float s10 = 0.0f;
-        s10 += ( + (r0_2*X[2]))*dtr + + (r0_2*X[2])*pv[18]*w0_2;
-        const float f10 = X[10] + s10;
         float s11 = 0.0f;
-        s11 += ( + (r0_3*X[3]))*dtr + + (r0_3*X[3])*pv[19]*w0_3;
-        const float f11 = X[11] + s11;
+        const float r0_0 = (((P0 * (1.0f + (P4 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const float r1_0 = P8;
+        const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        float za0, zb0;
+        plom_normal_pair32(ns, &za0, &zb0);
+        const float w0_0 = sqdt * za0;
+        const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - (r0_0*X[0])*P16*w0_0;
+        const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
+        s8 += ( + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0f) ? 0.0f : f0;
+        const float r0_1 = (((P1 * (1.0f + (P5 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const float r1_1 = P9;
+        const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const float w0_1 = sqdt * zb0;
+        const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - (r0_1*X[1])*P17*w0_1;
+        const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
+        s9 += ( + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0f) ? 0.0f : f1;
+        const float r0_2 = (((P2 * (1.0f + (P6 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const float r1_2 = P10;
+        const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        float za1, zb1;
+        plom_normal_pair32(ns, &za1, &zb1);
+        const float w0_2 = sqdt * za1;
+        const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - (r0_2*X[2])*P18*w0_2;
+        const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
+        s10 += ( + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0f) ? 0.0f : f2;
+        const float r0_3 = (((P3 * (1.0f + (P7 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const float r1_3 = P11;
+        const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const float w0_3 = sqdt * zb1;
+        const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - (r0_3*X[3])*P19*w0_3;
+        const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + + (r0_3*X[3])*P19*w0_3;
+        s11 += ( + (r0_3*X[3]))*dtr + + (r0_3*X[3])*P19*w0_3;
         X[3] = (f3 < 0.0f) ? 0.0f : f3;
         X[4] = (f4 < 0.0f) ? 0.0f : f4;
         X[5] = (f5 < 0.0f) ? 0.0f : f5;
         X[6] = (f6 < 0.0f) ? 0.0f : f6;
         X[7] = (f7 < 0.0f) ? 0.0f : f7;
+        const float f8 = X[8] + s8;
         X[8] = (f8 < 0.0f) ? 0.0f : f8;
+        const float f9 = X[9] + s9;
         X[9] = (f9 < 0.0f) ? 0.0f : f9;
+        const float f10 = X[10] + s10;
         X[10] = (f10 < 0.0f) ? 0.0f : f10;
+        const float f11 = X[11] + s11;
         X[11] = (f11 < 0.0f) ? 0.0f : f11;
         t += dt;
     }
@@ -487,6 +771,42 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
 {
     const float dtr = (float)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const float P0 = PLOM_UNIFORM32(pv[0]);
+    const float P1 = PLOM_UNIFORM32(pv[1]);
+    const float P2 = PLOM_UNIFORM32(pv[2]);
+    const float P3 = PLOM_UNIFORM32(pv[3]);
+    const float P4 = PLOM_UNIFORM32(pv[4]);
+    const float P5 = PLOM_UNIFORM32(pv[5]);
+    const float P6 = PLOM_UNIFORM32(pv[6]);
+    const float P7 = PLOM_UNIFORM32(pv[7]);
+    const float P8 = PLOM_UNIFORM32(pv[8]);
+    const float P9 = PLOM_UNIFORM32(pv[9]);
+    const float P10 = PLOM_UNIFORM32(pv[10]);
+    const float P11 = PLOM_UNIFORM32(pv[11]);
+    const float P12 = PLOM_UNIFORM32(pv[12]);
+    const float P13 = PLOM_UNIFORM32(pv[13]);
+    const float P14 = PLOM_UNIFORM32(pv[14]);
+    const float P15 = PLOM_UNIFORM32(pv[15]);
+    const float P28 = PLOM_UNIFORM32(pv[28]);
+    const float P29 = PLOM_UNIFORM32(pv[29]);
+    const float P30 = PLOM_UNIFORM32(pv[30]);
+    const float P31 = PLOM_UNIFORM32(pv[31]);
+    const float P32 = PLOM_UNIFORM32(pv[32]);
+    const float P33 = PLOM_UNIFORM32(pv[33]);
+    const float P34 = PLOM_UNIFORM32(pv[34]);
+    const float P35 = PLOM_UNIFORM32(pv[35]);
+    const float P36 = PLOM_UNIFORM32(pv[36]);
+    const float P37 = PLOM_UNIFORM32(pv[37]);
+    const float P38 = PLOM_UNIFORM32(pv[38]);
+    const float P39 = PLOM_UNIFORM32(pv[39]);
+    const float P40 = PLOM_UNIFORM32(pv[40]);
+    const float P41 = PLOM_UNIFORM32(pv[41]);
+    const float P42 = PLOM_UNIFORM32(pv[42]);
+    const float P43 = PLOM_UNIFORM32(pv[43]);
+    const float P44 = PLOM_UNIFORM32(pv[44]);
+    const float P45 = PLOM_UNIFORM32(pv[45]);
+    const float P46 = PLOM_UNIFORM32(pv[46]);
+    const float P47 = PLOM_UNIFORM32(pv[47]);
     X[8] = 0.0f; /* reset_inc */
     X[9] = 0.0f; /* reset_inc */
     X[10] = 0.0f; /* reset_inc */
@@ -494,61 +814,73 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     double t = t0;
     while (t < t1) {
         const float tr = (float)t; (void)tr;
-        const float r0_0 = (((pv[0] * (1.0f + (pv[4] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[32] * X[4] + pv[33] * X[5] + pv[34] * X[6] + pv[35] * X[7])) / pv[28]);
-        const float r1_0 = pv[8];
-        const float r2_0 = (pv[12] * ((pv[28] - X[0]) - X[4]));
-        const float d0_0 = sqdt * plom_normal32(ns);
-        const float d1_0 = sqdt * plom_normal32(ns);
-        const float d2_0 = sqdt * plom_normal32(ns);
-        const float r0_1 = (((pv[1] * (1.0f + (pv[5] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[36] * X[4] + pv[37] * X[5] + pv[38] * X[6] + pv[39] * X[7])) / pv[29]);
-        const float r1_1 = pv[9];
-        const float r2_1 = (pv[13] * ((pv[29] - X[1]) - X[5]));
-        const float d0_1 = sqdt * plom_normal32(ns);
-        const float d1_1 = sqdt * plom_normal32(ns);
-        const float d2_1 = sqdt * plom_normal32(ns);
-        const float r0_2 = (((pv[2] * (1.0f + (pv[6] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[40] * X[4] + pv[41] * X[5] + pv[42] * X[6] + pv[43] * X[7])) / pv[30]);
-        const float r1_2 = pv[10];
-        const float r2_2 = (pv[14] * ((pv[30] - X[2]) - X[6]));
-        const float d0_2 = sqdt * plom_normal32(ns);
-        const float d1_2 = sqdt * plom_normal32(ns);
-        const float d2_2 = sqdt * plom_normal32(ns);
-        const float r0_3 = (((pv[3] * (1.0f + (pv[7] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[44] * X[4] + pv[45] * X[5] + pv[46] * X[6] + pv[47] * X[7])) / pv[31]);
-        const float r1_3 = pv[11];
-        const float r2_3 = (pv[15] * ((pv[31] - X[3]) - X[7]));
-        const float d0_3 = sqdt * plom_normal32(ns);
-        const float d1_3 = sqdt * plom_normal32(ns);
-        const float d2_3 = sqdt * plom_normal32(ns);
-        const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - __builtin_sqrtf((r0_0*X[0]))*d0_0+ __builtin_sqrtf((r2_0))*d2_0;
-        const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - __builtin_sqrtf((r0_1*X[1]))*d0_1+ __builtin_sqrtf((r2_1))*d2_1;
-        const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - __builtin_sqrtf((r0_2*X[2]))*d0_2+ __builtin_sqrtf((r2_2))*d2_2;
-        const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - __builtin_sqrtf((r0_3*X[3]))*d0_3+ __builtin_sqrtf((r2_3))*d2_3;
-        const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - __builtin_sqrtf((r1_0*X[4]))*d1_0+ __builtin_sqrtf((r0_0*X[0]))*d0_0;
-        const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - __builtin_sqrtf((r1_1*X[5]))*d1_1+ __builtin_sqrtf((r0_1*X[1]))*d0_1;
-        const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - __builtin_sqrtf((r1_2*X[6]))*d1_2+ __builtin_sqrtf((r0_2*X[2]))*d0_2;
-        const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - __builtin_sqrtf((r1_3*X[7]))*d1_3+ __builtin_sqrtf((r0_3*X[3]))*d0_3;
         float s8 = 0.0f;
-        s8 += ( + (r0_0*X[0]))*dtr + + __builtin_sqrtf((r0_0*X[0]))*d0_0;
-        const float f8 = X[8] + s8;
         float s9 = 0.0f;
-        s9 += ( + (r0_1*X[1]))*dtr + + __builtin_sqrtf((r0_1*X[1]))*d0_1;
-        const float f9 = X[9] + s9;
         float s10 = 0.0f;
-        s10 += ( + (r0_2*X[2]))*dtr + + __builtin_sqrtf((r0_2*X[2]))*d0_2;
-        const float f10 = X[10] + s10;
         float s11 = 0.0f;
-        s11 += ( + (r0_3*X[3]))*dtr + + __builtin_sqrtf((r0_3*X[3]))*d0_3;
-        const float f11 = X[11] + s11;
+        const float r0_0 = (((P0 * (1.0f + (P4 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const float r1_0 = P8;
+        const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        float za0, zb0;
+        plom_normal_pair32(ns, &za0, &zb0);
+        const float d0_0 = sqdt * za0;
+        const float d1_0 = sqdt * zb0;
+        float za1, zb1;
+        plom_normal_pair32(ns, &za1, &zb1);
+        const float d2_0 = sqdt * za1;
+        const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - __builtin_sqrtf((r0_0*X[0]))*d0_0+ __builtin_sqrtf((r2_0))*d2_0;
+        const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - __builtin_sqrtf((r1_0*X[4]))*d1_0+ __builtin_sqrtf((r0_0*X[0]))*d0_0;
+        s8 += ( + (r0_0*X[0]))*dtr + + __builtin_sqrtf((r0_0*X[0]))*d0_0;
         X[0] = (f0 < 0.0f) ? 0.0f : f0;
+        const float r0_1 = (((P1 * (1.0f + (P5 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const float r1_1 = P9;
+        const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const float d0_1 = sqdt * zb1;
+        float za2, zb2;
+        plom_normal_pair32(ns, &za2, &zb2);
+        const float d1_1 = sqdt * za2;
+        const float d2_1 = sqdt * zb2;
+        const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - __builtin_sqrtf((r0_1*X[1]))*d0_1+ __builtin_sqrtf((r2_1))*d2_1;
+        const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - __builtin_sqrtf((r1_1*X[5]))*d1_1+ __builtin_sqrtf((r0_1*X[1]))*d0_1;
+        s9 += ( + (r0_1*X[1]))*dtr + + __builtin_sqrtf((r0_1*X[1]))*d0_1;
         X[1] = (f1 < 0.0f) ? 0.0f : f1;
+        const float r0_2 = (((P2 * (1.0f + (P6 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const float r1_2 = P10;
+        const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        float za3, zb3;
+        plom_normal_pair32(ns, &za3, &zb3);
+        const float d0_2 = sqdt * za3;
+        const float d1_2 = sqdt * zb3;
+        float za4, zb4;
+        plom_normal_pair32(ns, &za4, &zb4);
+        const float d2_2 = sqdt * za4;
+        const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - __builtin_sqrtf((r0_2*X[2]))*d0_2+ __builtin_sqrtf((r2_2))*d2_2;
+        const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - __builtin_sqrtf((r1_2*X[6]))*d1_2+ __builtin_sqrtf((r0_2*X[2]))*d0_2;
+        s10 += ( + (r0_2*X[2]))*dtr + + __builtin_sqrtf((r0_2*X[2]))*d0_2;
         X[2] = (f2 < 0.0f) ? 0.0f : f2;
+        const float r0_3 = (((P3 * (1.0f + (P7 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const float r1_3 = P11;
+        const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const float d0_3 = sqdt * zb4;
+        float za5, zb5;
+        plom_normal_pair32(ns, &za5, &zb5);
+        const float d1_3 = sqdt * za5;
+        const float d2_3 = sqdt * zb5;
+        const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - __builtin_sqrtf((r0_3*X[3]))*d0_3+ __builtin_sqrtf((r2_3))*d2_3;
+        const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - __builtin_sqrtf((r1_3*X[7]))*d1_3+ __builtin_sqrtf((r0_3*X[3]))*d0_3;
+        s11 += ( + (r0_3*X[3]))*dtr + + __builtin_sqrtf((r0_3*X[3]))*d0_3;
         X[3] = (f3 < 0.0f) ? 0.0f : f3;
         X[4] = (f4 < 0.0f) ? 0.0f : f4;
         X[5] = (f5 < 0.0f) ? 0.0f : f5;
         X[6] = (f6 < 0.0f) ? 0.0f : f6;
         X[7] = (f7 < 0.0f) ? 0.0f : f7;
+        const float f8 = X[8] + s8;
         X[8] = (f8 < 0.0f) ? 0.0f : f8;
+        const float f9 = X[9] + s9;
         X[9] = (f9 < 0.0f) ? 0.0f : f9;
+        const float f10 = X[10] + s10;
         X[10] = (f10 < 0.0f) ? 0.0f : f10;
+        const float f11 = X[11] + s11;
         X[11] = (f11 < 0.0f) ? 0.0f : f11;
         t += dt;
     }
@@ -558,6 +890,42 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
 {
     const float dtr = (float)dt;
     (void)sqdt; (void)ns; (void)pv;
+    const float P0 = PLOM_UNIFORM32(pv[0]);
+    const float P1 = PLOM_UNIFORM32(pv[1]);
+    const float P2 = PLOM_UNIFORM32(pv[2]);
+    const float P3 = PLOM_UNIFORM32(pv[3]);
+    const float P4 = PLOM_UNIFORM32(pv[4]);
+    const float P5 = PLOM_UNIFORM32(pv[5]);
+    const float P6 = PLOM_UNIFORM32(pv[6]);
+    const float P7 = PLOM_UNIFORM32(pv[7]);
+    const float P8 = PLOM_UNIFORM32(pv[8]);
+    const float P9 = PLOM_UNIFORM32(pv[9]);
+    const float P10 = PLOM_UNIFORM32(pv[10]);
+    const float P11 = PLOM_UNIFORM32(pv[11]);
+    const float P12 = PLOM_UNIFORM32(pv[12]);
+    const float P13 = PLOM_UNIFORM32(pv[13]);
+    const float P14 = PLOM_UNIFORM32(pv[14]);
+    const float P15 = PLOM_UNIFORM32(pv[15]);
+    const float P28 = PLOM_UNIFORM32(pv[28]);
+    const float P29 = PLOM_UNIFORM32(pv[29]);
+    const float P30 = PLOM_UNIFORM32(pv[30]);
+    const float P31 = PLOM_UNIFORM32(pv[31]);
+    const float P32 = PLOM_UNIFORM32(pv[32]);
+    const float P33 = PLOM_UNIFORM32(pv[33]);
+    const float P34 = PLOM_UNIFORM32(pv[34]);
+    const float P35 = PLOM_UNIFORM32(pv[35]);
+    const float P36 = PLOM_UNIFORM32(pv[36]);
+    const float P37 = PLOM_UNIFORM32(pv[37]);
+    const float P38 = PLOM_UNIFORM32(pv[38]);
+    const float P39 = PLOM_UNIFORM32(pv[39]);
+    const float P40 = PLOM_UNIFORM32(pv[40]);
+    const float P41 = PLOM_UNIFORM32(pv[41]);
+    const float P42 = PLOM_UNIFORM32(pv[42]);
+    const float P43 = PLOM_UNIFORM32(pv[43]);
+    const float P44 = PLOM_UNIFORM32(pv[44]);
+    const float P45 = PLOM_UNIFORM32(pv[45]);
+    const float P46 = PLOM_UNIFORM32(pv[46]);
+    const float P47 = PLOM_UNIFORM32(pv[47]);
     X[8] = 0.0f; /* reset_inc */
     X[9] = 0.0f; /* reset_inc */
     X[10] = 0.0f; /* reset_inc */
@@ -565,49 +933,49 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     double t = t0;
     while (t < t1) {
         const float tr = (float)t; (void)tr;
-        const float r0_0 = (((pv[0] * (1.0f + (pv[4] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[32] * X[4] + pv[33] * X[5] + pv[34] * X[6] + pv[35] * X[7])) / pv[28]);
-        const float r1_0 = pv[8];
-        const float r2_0 = (pv[12] * ((pv[28] - X[0]) - X[4]));
-        const float r0_1 = (((pv[1] * (1.0f + (pv[5] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[36] * X[4] + pv[37] * X[5] + pv[38] * X[6] + pv[39] * X[7])) / pv[29]);
-        const float r1_1 = pv[9];
-        const float r2_1 = (pv[13] * ((pv[29] - X[1]) - X[5]));
-        const float r0_2 = (((pv[2] * (1.0f + (pv[6] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[40] * X[4] + pv[41] * X[5] + pv[42] * X[6] + pv[43] * X[7])) / pv[30]);
-        const float r1_2 = pv[10];
-        const float r2_2 = (pv[14] * ((pv[30] - X[2]) - X[6]));
-        const float r0_3 = (((pv[3] * (1.0f + (pv[7] * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (pv[44] * X[4] + pv[45] * X[5] + pv[46] * X[6] + pv[47] * X[7])) / pv[31]);
-        const float r1_3 = pv[11];
-        const float r2_3 = (pv[15] * ((pv[31] - X[3]) - X[7]));
-        const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr;
-        const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr;
-        const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr;
-        const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr;
-        const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr;
-        const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr;
-        const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr;
-        const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr;
         float s8 = 0.0f;
-        s8 += ( + (r0_0*X[0]))*dtr;
-        const float f8 = X[8] + s8;
         float s9 = 0.0f;
-        s9 += ( + (r0_1*X[1]))*dtr;
-        const float f9 = X[9] + s9;
         float s10 = 0.0f;
-        s10 += ( + (r0_2*X[2]))*dtr;
-        const float f10 = X[10] + s10;
         float s11 = 0.0f;
-        s11 += ( + (r0_3*X[3]))*dtr;
-        const float f11 = X[11] + s11;
+        const float r0_0 = (((P0 * (1.0f + (P4 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const float r1_0 = P8;
+        const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr;
+        const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr;
+        s8 += ( + (r0_0*X[0]))*dtr;
         X[0] = (f0 < 0.0f) ? 0.0f : f0;
+        const float r0_1 = (((P1 * (1.0f + (P5 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const float r1_1 = P9;
+        const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr;
+        const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr;
+        s9 += ( + (r0_1*X[1]))*dtr;
         X[1] = (f1 < 0.0f) ? 0.0f : f1;
+        const float r0_2 = (((P2 * (1.0f + (P6 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const float r1_2 = P10;
+        const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr;
+        const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr;
+        s10 += ( + (r0_2*X[2]))*dtr;
         X[2] = (f2 < 0.0f) ? 0.0f : f2;
+        const float r0_3 = (((P3 * (1.0f + (P7 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const float r1_3 = P11;
+        const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr;
+        const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr;
+        s11 += ( + (r0_3*X[3]))*dtr;
         X[3] = (f3 < 0.0f) ? 0.0f : f3;
         X[4] = (f4 < 0.0f) ? 0.0f : f4;
         X[5] = (f5 < 0.0f) ? 0.0f : f5;
         X[6] = (f6 < 0.0f) ? 0.0f : f6;
         X[7] = (f7 < 0.0f) ? 0.0f : f7;
+        const float f8 = X[8] + s8;
         X[8] = (f8 < 0.0f) ? 0.0f : f8;
+        const float f9 = X[9] + s9;
         X[9] = (f9 < 0.0f) ? 0.0f : f9;
+        const float f10 = X[10] + s10;
         X[10] = (f10 < 0.0f) ? 0.0f : f10;
+        const float f11 = X[11] + s11;
         X[11] = (f11 < 0.0f) ? 0.0f : f11;
         t += dt;
     }

@@ -124,3 +124,24 @@ def xoshiro(state, n):
     o = (C.c_uint32 * n)()
     lib().oracle_xoshiro(s, n, o)
     return list(o), list(s)
+
+
+def resample(logw, n=0, n_ts=1, like_min=1e-17, seed=1, stream=0, mode=CANONICAL):
+    logw = np.ascontiguousarray(logw, dtype=np.float64)
+    J = logw.size
+    sel = np.zeros(J, dtype=np.int32)
+    ln = C.c_double()
+    es = C.c_double()
+    L = lib()
+    L.oracle_resample.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int, C.c_int, C.c_double, C.c_uint32,
+                                  C.c_uint32, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]
+    r = L.oracle_resample(J, _dp(logw), n, n_ts, like_min, seed, stream, mode, _ip(sel), C.byref(ln), C.byref(es))
+    return sel, ln.value, es.value, r
+
+
+def resample_num(seed, stream, n):
+    L = lib()
+    L.oracle_resample_num.restype = C.c_uint64
+    L.oracle_resample_num.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+    return L.oracle_resample_num(seed, stream, n)

@@ -367,11 +367,16 @@ void oracle_math(int n, const double *in, double *out)
     }
 }
 
+/* the first n Gaussians (Box-Muller pairs in order) of particle j over interval n0 */
 void oracle_normals(uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int n, double *out)
 {
     plom_nstream64 ns;
     plom_nstream64_init(&ns, seed, stream, n0, j);
-    for (int i = 0; i < n; i++) out[i] = plom_normal64(&ns);
+    for (int i = 0; i + 1 < n; i += 2) plom_normal_pair64(&ns, &out[i], &out[i + 1]);
+    if (n & 1) {
+        double spare;
+        plom_normal_pair64(&ns, &out[n - 1], &spare);
+    }
 }
 
 void oracle_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) { plom_philox4x32_10(ctr, key, out); }
@@ -383,3 +388,24 @@ uint32_t oracle_xoshiro(uint32_t *state, int n, uint32_t *out)
     state[0] = r.s0; state[1] = r.s1; state[2] = r.s2; state[3] = r.s3;
     return 0;
 }
+
+/* weighting + selection of one interval on given log-weights (for tests); returns 1 if
+ * resampled, 0 for the all-fail branch */
+int oracle_resample(int J, const double *logw, int n, int n_ts, double like_min, uint32_t seed, uint32_t stream,
+                    int mode, int32_t *select_out, double *llike_n, double *ess_n)
+{
+    const int T = (J + TILE - 1) / TILE;
+    double *work = malloc((size_t)(3 * T + 4 * next_pow2(T) + 2 * TILE + J) * sizeof(double));
+    uint64_t *cum = malloc((size_t)J * sizeof(uint64_t));
+    uint32_t *sel = malloc((size_t)J * sizeof(uint32_t));
+    int r;
+    if (mode == ORACLE_CANONICAL)
+        r = weight_select_canonical(logw, J, n, seed, stream, plom_log(like_min) * (double)n_ts, llike_n, ess_n, sel, work, cum);
+    else
+        r = weight_select_reference(logw, J, n, n_ts, like_min, seed, stream, llike_n, ess_n, sel, work);
+    for (int j = 0; j < J; j++) select_out[j] = (int32_t)sel[j];
+    free(work); free(cum); free(sel);
+    return r;
+}
+
+uint64_t oracle_resample_num(uint32_t seed, uint32_t stream, uint32_t n) { return plom_resample_num(seed, stream, n); }

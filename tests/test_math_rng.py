@@ -1,0 +1,91 @@
+"""Shared arithmetic (csrc/plom_math.h, csrc/plom_rng.h), host build, pinned against
+published known answers and high-precision references."""
+import math
+
+import mpmath as mp
+import numpy as np
+import pytest
+from scipy import special, stats
+
+from oracle import oracle
+
+mp.mp.dps = 40
+
+
+def test_philox_known_answers():
+    """Random123 kat_vectors for philox4x32_10 (Salmon et al., SC'11)."""
+    assert oracle.philox([0, 0, 0, 0], [0, 0]) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert oracle.philox([0xffffffff] * 4, [0xffffffff] * 2) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert oracle.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0]) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_xoshiro128ss_first_output_and_uniformity():
+    out, _ = oracle.xoshiro([1, 2, 3, 4], 200000)
+    # first output: rotl(s1*5, 7)*9 with s1 = 2 -> rotl(10, 7)*9 = 11520
+    assert out[0] == 11520
+    u = (np.asarray(out, dtype=np.float64) + 0.5) / 2 ** 32
+    assert stats.kstest(u, "uniform").pvalue > 1e-3
+    assert abs(np.corrcoef(u[:-1], u[1:])[0, 1]) < 0.01
+    bits = np.unpackbits(np.asarray(out, dtype=np.uint32).view(np.uint8))
+    assert abs(bits.mean() - 0.5) < 0.002
+
+
+def test_normals_distribution():
+    z = np.concatenate([oracle.normals(1, s, n, j, 64) for s in range(4) for n in range(16) for j in range(64)])
+    assert stats.kstest(z, "norm").pvalue > 1e-3
+    assert abs(z.mean()) < 0.02 and abs(z.var() - 1) < 0.03
+
+
+def test_normals_are_a_function_of_the_key():
+    a = oracle.normals(3, 4, 5, 6, 10)
+    assert np.array_equal(a, oracle.normals(3, 4, 5, 6, 10))
+    assert not np.array_equal(a, oracle.normals(3, 4, 5, 7, 10))
+    assert not np.array_equal(a, oracle.normals(3, 4, 6, 6, 10))
+    assert not np.array_equal(a, oracle.normals(3, 5, 5, 6, 10))
+
+
+def ulp_err(got, ref):
+    ref = float(ref)
+    if ref == 0.0:
+        return abs(got)
+    return abs(got - ref) / math.ulp(ref)
+
+
+def test_exp_log_sin_cos_accuracy():
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.uniform(-700, 700, 400), rng.uniform(-1, 1, 200), rng.uniform(-50, 50, 200)])
+    r = oracle.math(xs)
+    for x, row in zip(xs, r):
+        assert ulp_err(row[0], mp.exp(mp.mpf(x))) <= 2
+        assert abs(row[3] - float(mp.sin(mp.mpf(x)))) <= 2 * 2 ** -53 * max(1.0, 1.0)
+        assert abs(row[4] - float(mp.cos(mp.mpf(x)))) <= 2 * 2 ** -53
+    pos = np.exp(rng.uniform(-740, 700, 600))
+    pos = np.concatenate([pos, 1 + rng.uniform(-1e-6, 1e-6, 100), [1e-310, 5e-324]])
+    r = oracle.math(pos)
+    for x, row in zip(pos, r):
+        assert ulp_err(row[1], mp.log(mp.mpf(x))) <= 2
+        assert row[5] == math.sqrt(x)
+
+
+def test_special_values():
+    r = oracle.math(np.array([0.0, -0.0, np.inf, -np.inf, np.nan, -1.0, 1000.0, -1000.0]))
+    assert r[0, 1] == -np.inf and r[2, 1] == np.inf and np.isnan(r[5, 1]) and np.isnan(r[4, 1])
+    assert r[3, 0] == 0.0 and r[2, 0] == np.inf and r[6, 0] == np.inf and r[7, 0] == 0.0
+    assert r[0, 2] == 0.5 and r[2, 2] == 1.0 and r[3, 2] == 0.0 and np.isnan(r[4, 2])
+
+
+def test_phi_against_mpmath_and_scipy():
+    xs = np.concatenate([np.linspace(-37.5, 8.0, 3001), [-1.0, 1.0, -2.0, 2.0, -4.0, 4.0, -8.0]])
+    r = oracle.math(xs)[:, 2]
+    for x, got in zip(xs, r):
+        ref = float(mp.ncdf(mp.mpf(x)))
+        assert abs(got - ref) <= 1e-14 * ref + 1e-300, (x, got, ref)
+    np.testing.assert_allclose(r, special.ndtr(xs), rtol=3e-13, atol=0)
+
+
+def test_phi_central_symmetry():
+    xs = np.linspace(0, 6, 601)
+    lo = oracle.math(-xs)[:, 2]
+    hi = oracle.math(xs)[:, 2]
+    np.testing.assert_allclose(lo + hi, 1.0, rtol=0, atol=2.3e-16)

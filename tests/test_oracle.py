@@ -1,0 +1,186 @@
+"""Pin the CPU oracle: weighting and systematic resampling against exact rational
+arithmetic and closed forms, the whole filter against its deterministic and one-particle
+limits, the likelihood against scipy, and the canonical arithmetic against the
+reference's sequential arithmetic (smc.c:27-112)."""
+from fractions import Fraction
+
+import mpmath as mp
+import numpy as np
+import pytest
+from scipy import stats
+
+from oracle import oracle
+from iip_amd import configs
+from iip_amd.smc import Model
+
+mp.mp.dps = 60
+
+
+def exact_systematic(logw, U):
+    """Textbook systematic resampling (Kitagawa 1996; reference doc/doc.tex:403-419,
+    smc.c:90-112) in exact arithmetic: w_i = e^{l_i}/sum e^{l_j} evaluated with 60 digits,
+    points (k + U)/J, select[k] = first i with cumulative weight >= point."""
+    J = len(logw)
+    ws = [mp.e ** mp.mpf(float(l)) if np.isfinite(l) else mp.mpf(0) for l in logw]
+    tot = mp.fsum(ws)
+    cum, c = [], mp.mpf(0)
+    for w in ws:
+        c += w / tot
+        cum.append(c)
+    out, i = [], 0
+    for k in range(J):
+        p = (k + mp.mpf(U.numerator) / U.denominator) / J
+        while cum[i] < p:
+            i += 1
+        out.append(i)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("J,seed", [(1, 0), (2, 1), (3, 2), (7, 3), (16, 4), (255, 5), (256, 6), (300, 7), (1000, 8)])
+def test_systematic_matches_exact_rational(J, seed):
+    rng = np.random.default_rng(seed)
+    logw = rng.normal(-50.0, 3.0, J)
+    if J > 4:
+        logw[rng.integers(0, J, J // 5)] = -np.inf
+        logw[0] = -40.0
+    sel, ln, ess, r = oracle.resample(logw, n=seed, seed=11, stream=seed)
+    assert r == 1
+    U = Fraction(int(oracle.resample_num(11, seed, seed)), 2 ** 33)
+    np.testing.assert_array_equal(sel, exact_systematic(logw, U))
+    # log-likelihood of the interval and ESS against high precision
+    ws = [mp.e ** mp.mpf(float(l)) for l in logw if np.isfinite(l)]
+    assert ln == pytest.approx(float(mp.log(mp.fsum(ws) / J)), rel=1e-14)
+    assert ess == pytest.approx(float(mp.fsum(ws) ** 2 / mp.fsum(w * w for w in ws)), rel=1e-13)
+
+
+def test_systematic_invariants():
+    rng = np.random.default_rng(42)
+    for trial in range(20):
+        J = int(rng.integers(2, 3000))
+        logw = rng.normal(0, 2, J)
+        sel, ln, ess, _ = oracle.resample(logw, n=trial, stream=trial)
+        assert np.all(np.diff(sel) >= 0)
+        w = np.exp(logw - logw.max())
+        w /= w.sum()
+        counts = np.bincount(sel, minlength=J)
+        assert np.all(counts >= np.floor(J * w) - 1e-9)
+        assert np.all(counts <= np.ceil(J * w) + 1e-9)
+        assert 1.0 <= ess <= J
+
+
+@pytest.mark.parametrize("J", [1, 5, 64, 256, 1000, 4096])
+def test_equal_weights(J):
+    sel, ln, ess, r = oracle.resample(np.full(J, -3.25), n=3)
+    np.testing.assert_array_equal(sel, np.arange(J))
+    assert ln == -3.25 and ess == J
+
+
+def test_single_survivor():
+    logw = np.full(500, -np.inf)
+    logw[123] = -7.0
+    sel, ln, ess, r = oracle.resample(logw)
+    assert r == 1 and ess == 1.0 and np.all(sel == 123)
+    assert ln == pytest.approx(-7.0 - np.log(500.0), rel=1e-15)
+
+
+def test_all_fail():
+    sel, ln, ess, r = oracle.resample(np.full(100, -np.inf), n_ts=4, like_min=1e-17)
+    assert r == 0 and ess == 0.0
+    np.testing.assert_array_equal(sel, np.arange(100))
+    assert ln == pytest.approx(4 * np.log(1e-17), rel=1e-15)
+
+
+def test_canonical_vs_reference_arithmetic():
+    rng = np.random.default_rng(7)
+    mism = 0
+    total = 0
+    for t in range(30):
+        J = int(rng.integers(10, 5000))
+        # realisable log-weights stay above N_TS*log(LIKE_MIN), where both arithmetics agree
+        logw = rng.normal(-12, 4, J)
+        a, la, ea, _ = oracle.resample(logw, n=t, stream=t, mode=oracle.CANONICAL)
+        b, lb, eb, _ = oracle.resample(logw, n=t, stream=t, mode=oracle.REFERENCE)
+        mism += int(np.sum(a != b))
+        total += J
+        assert la == pytest.approx(lb, rel=1e-13)
+        assert ea == pytest.approx(eb, rel=1e-12)
+    assert mism / total < 1e-3
+
+
+def setup(name, N):
+    m = Model.get(name)
+    times, data = m.dataset(N)
+    return m, times, data, m.pack(configs.TRUTH[name]), m.x0(configs.X0_PROPS[name])
+
+
+@pytest.mark.parametrize("name", ["sir", "sirs4"])
+def test_deterministic_limit(name):
+    """No noise: every particle is the same, ESS = J, and the log-likelihood is the sum of
+    the per-interval log-likelihoods of the one deterministic trajectory."""
+    m, times, data, pv, x0 = setup(name, 30)
+    r = oracle.smc(m.id, 64, times, data, pv, x0, noises_off=3)
+    traj = oracle.trajectory(m.id, times, pv, x0, noises_off=3)
+    expect = [oracle.obs_loglik(m.id, traj[n + 1], data[n], pv)[1] for n in range(30)]
+    np.testing.assert_allclose(r["loglik_n"], expect, rtol=1e-13, atol=1e-13)
+    np.testing.assert_array_equal(r["ess_n"], 64.0)
+
+
+def test_one_particle_is_one_path():
+    m, times, data, pv, x0 = setup("sir", 40)
+    r = oracle.smc(m.id, 1, times, data, pv, x0, seed=9, stream=2)
+    traj = oracle.trajectory(m.id, times, pv, x0, seed=9, stream=2, j=0)
+    expect = [oracle.obs_loglik(m.id, traj[n + 1], data[n], pv)[1] for n in range(40)]
+    np.testing.assert_allclose(r["loglik_n"], expect, rtol=1e-13, atol=1e-12)
+
+
+def test_filter_canonical_vs_reference_c1():
+    m, times, data, pv, x0 = setup("sir", 52)
+    a = oracle.smc(m.id, 1000, times, data, pv, x0, ancestry=True)
+    b = oracle.smc(m.id, 1000, times, data, pv, x0, ancestry=True, mode=oracle.REFERENCE)
+    assert (a["ancestry"] != b["ancestry"]).mean() < 1e-3
+    assert a["loglik"] == pytest.approx(b["loglik"], rel=1e-9)
+
+
+def test_thread_count_invariance():
+    m, times, data, pv, x0 = setup("sirs4", 12)
+    a = oracle.smc(m.id, 3000, times, data, pv, x0, nthreads=1, ancestry=True)
+    b = oracle.smc(m.id, 3000, times, data, pv, x0, nthreads=7, ancestry=True)
+    assert a["loglik"] == b["loglik"]
+    np.testing.assert_array_equal(a["ancestry"], b["ancestry"])
+
+
+def test_likelihood_against_scipy():
+    """Discretised normal (core_template.c:357-366) with rep 0.6, phi 0.1 for the SIR model."""
+    m = Model.get("sir")
+    pv = m.pack(configs.TRUTH["sir"])
+    rep, phi = 0.6, 0.1
+    for inc in [0.0, 0.5, 3.0, 100.0, 5000.0, 2.5e5]:
+        for y in [0.0, 1.0, 2.0, 60.0, 3000.0, 150000.0]:
+            x = np.array([5e5, 1e3, inc])
+            obs, ll, nf = oracle.obs_loglik(m.id, x, np.array([y]), pv)
+            mu = rep * inc
+            sd = np.sqrt(rep * (1 - rep) * inc + (rep * phi * inc) ** 2)
+            if sd == 0:
+                like = 1.0 if y == 0 else 0.0
+            elif y > 0:
+                like = stats.norm.cdf((y + 0.5 - mu) / sd) - stats.norm.cdf((y - 0.5 - mu) / sd)
+            else:
+                like = stats.norm.cdf((y + 0.5 - mu) / sd)
+            floored = (not np.isfinite(like)) or like <= 1e-17
+            expect = np.log(1e-17) if floored else np.log(like)
+            assert nf == int(floored)
+            assert ll == pytest.approx(expect, rel=1e-9, abs=1e-12), (inc, y)
+
+
+def test_missing_series_skipped():
+    m = Model.get("sirs4")
+    pv = m.pack(configs.TRUTH["sirs4"])
+    x = m.x0(configs.X0_PROPS["sirs4"])
+    x[8:12] = [100.0, 200.0, 300.0, 400.0]
+    y = np.array([60.0, np.nan, 180.0, np.nan])
+    _, ll, nf = oracle.obs_loglik(m.id, x, y, pv)
+    _, l0, _ = oracle.obs_loglik(m.id, x, np.array([60.0, np.nan, np.nan, np.nan]), pv)
+    _, l2, _ = oracle.obs_loglik(m.id, x, np.array([np.nan, np.nan, 180.0, np.nan]), pv)
+    assert ll == l0 + l2
+    _, lnone, _ = oracle.obs_loglik(m.id, x, np.full(4, np.nan), pv)
+    assert lnone == 0.0

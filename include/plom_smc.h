@@ -132,9 +132,12 @@ int plom_batch_run(plom_ctx *ctx, int n_filters, plom_filter **filters, int use_
 
 int plom_sync(plom_ctx *ctx);
 
-/* kernel duration of the last plom_filter_run / plom_batch_run, from HIP events on the
- * context stream: total milliseconds and the split by kernel */
-int plom_last_timing(plom_ctx *ctx, double *total_ms, double *propagate_ms, double *scan_ms);
+/* Timing of the last plom_filter_run / plom_batch_run from HIP events on the context
+ * stream.  total_ms always; with kernel timing on (plom_ctx_set_kernel_timing), events
+ * also bracket every launch and propagate_ms / scan_ms are the summed durations of the
+ * propagate and scan launches, n_intervals the number of propagate launch rounds. */
+int plom_ctx_set_kernel_timing(plom_ctx *ctx, int on);
+int plom_last_timing(plom_ctx *ctx, double *total_ms, double *propagate_ms, double *scan_ms, int32_t *n_intervals);
 
 /* device self-test of the shared arithmetic: out[i*6+..] = exp, log, Phi, sin, cos, sqrt of in[i] */
 int plom_selftest_math(plom_ctx *ctx, int n, const double *in, double *out);

@@ -175,6 +175,13 @@ PLOM_HD double plom_cos_kernel(double r)
 /* sin and cos of x for |x| < 2^20 (ample for forcing terms in days) */
 PLOM_HD void plom_sincos(double x, double *s, double *c)
 {
+    /* outside |x| <= 2^20 the quadrant count would overflow an int (undefined and
+     * different between x86 and gfx950): NaN on both sides instead */
+    if (!(plom_fabs(x) <= 0x1p20)) {
+        *s = PLOM_NAN;
+        *c = PLOM_NAN;
+        return;
+    }
     double kd = plom_rint(x * PLOM_2_OVER_PI);
     double r = x - kd * PLOM_PIO2_1;
     r = r - kd * PLOM_PIO2_2;

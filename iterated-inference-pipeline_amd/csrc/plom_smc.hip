@@ -37,6 +37,7 @@
 #include <stdarg.h>
 #include <vector>
 #include <algorithm>
+#include <new>
 
 #include "plom_smc.h"
 #include "plom_math.h"
@@ -450,8 +451,22 @@ struct plom_ctx {
     hipStream_t stream;
     hipEvent_t ev[4];
     double last_total_ms, last_prop_ms, last_scan_ms;
-    int timing;
+    int ktiming;                    /* bracket every launch with events */
+    std::vector<hipEvent_t> kev;    /* pool: 2 per interval + 1 */
+    int kev_used;                   /* events recorded by the last run */
 };
+
+/* event k of the per-launch timing pool (grown on demand) */
+static int kev_record(plom_ctx *c, int k)
+{
+    while ((int)c->kev.size() <= k) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        c->kev.push_back(e);
+    }
+    HIPCHK(hipEventRecord(c->kev[(size_t)k], c->stream));
+    return PLOM_OK;
+}
 
 struct plom_filter {
     plom_filter_desc desc;
@@ -535,13 +550,15 @@ extern "C" int plom_ctx_create(int device, plom_ctx **out)
 {
     if (!out) return set_err(PLOM_ERR_ARG, "null output");
     HIPCHK(hipSetDevice(device));
-    plom_ctx *c = (plom_ctx *)calloc(1, sizeof(plom_ctx));
-    if (!c) return set_err(PLOM_ERR_NOMEM, "host allocation failed");
+    void *mem = calloc(1, sizeof(plom_ctx));
+    if (!mem) return set_err(PLOM_ERR_NOMEM, "host allocation failed");
+    plom_ctx *c = new (mem) plom_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { free(c); return set_err(PLOM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
     for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&c->ev[i]));
-    c->timing = 1;
+    c->ktiming = 0;
+    c->kev_used = 0;
     *out = c;
     return PLOM_OK;
 }
@@ -552,7 +569,9 @@ extern "C" int plom_ctx_destroy(plom_ctx *c)
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int i = 0; i < 4; i++) HIPCHK(hipEventDestroy(c->ev[i]));
+    for (auto e : c->kev) HIPCHK(hipEventDestroy(e));
     HIPCHK(hipStreamDestroy(c->stream));
+    c->~plom_ctx();
     free(c);
     return PLOM_OK;
 }
@@ -705,10 +724,15 @@ extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
     const model_entry *me = &g_models[f->desc.model_id];
     const int T = f->h.T;
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    int k = 0;
     for (int n = 0; n < f->desc.N; n++) {
+        if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
         me->launch_prop(f->desc.precision, f->variant, T, c->stream, f->df, nullptr);
+        if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
         hipLaunchKernelGGL(k_scan, dim3(T), dim3(TILE), 0, c->stream, f->df, (const int2 *)nullptr);
     }
+    if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
+    c->kev_used = k;
     if (needs_final(f)) enqueue_final(c, f, nullptr, f->df, T);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
@@ -825,14 +849,19 @@ extern "C" int plom_batch_run(plom_ctx *c, int nf, plom_filter **fs, int use_gra
     for (size_t gi = 0; gi < groups.size(); gi++)
         HIPCHK(hipMemcpyAsync(d_groups[gi], groups[gi].tiles.data(), groups[gi].tiles.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    int k = 0;
     for (int s = 0; s < maxN; s++) {
+        if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
         for (size_t gi = 0; gi < groups.size(); gi++) {
             const int nb = prefix_len(groups[gi].tiles, s);
             if (nb > 0) g_models[groups[gi].model].launch_prop(groups[gi].prec, groups[gi].variant, nb, c->stream, ddf, d_groups[gi]);
         }
+        if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
         const int nb = prefix_len(all_tiles, s);
         hipLaunchKernelGGL(k_scan, dim3(nb), dim3(TILE), 0, c->stream, ddf, (const int2 *)d_all);
     }
+    if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
+    c->kev_used = k;
     /* final resampling for the filters that keep ancestry or the final cloud */
     for (int r = 0; r < nf; r++) {
         plom_filter *f = fs[order[(size_t)r]];
@@ -853,16 +882,36 @@ extern "C" int plom_batch_run(plom_ctx *c, int nf, plom_filter **fs, int use_gra
     return PLOM_OK;
 }
 
-extern "C" int plom_last_timing(plom_ctx *c, double *total_ms, double *prop_ms, double *scan_ms)
+extern "C" int plom_ctx_set_kernel_timing(plom_ctx *c, int on)
+{
+    if (!c) return set_err(PLOM_ERR_ARG, "null context");
+    c->ktiming = on ? 1 : 0;
+    return PLOM_OK;
+}
+
+extern "C" int plom_last_timing(plom_ctx *c, double *total_ms, double *prop_ms, double *scan_ms, int32_t *n_intervals)
 {
     if (!c) return set_err(PLOM_ERR_ARG, "null context");
     HIPCHK(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     c->last_total_ms = ms;
+    double p = 0.0, q = 0.0;
+    int nint = 0;
+    /* events: e[2n] before the propagate launches of interval n, e[2n+1] before its scan,
+     * e[2n+2] after it */
+    for (int n = 0; 2 * n + 2 < c->kev_used; n++) {
+        float a = 0.f, b = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, c->kev[(size_t)(2 * n)], c->kev[(size_t)(2 * n + 1)]));
+        HIPCHK(hipEventElapsedTime(&b, c->kev[(size_t)(2 * n + 1)], c->kev[(size_t)(2 * n + 2)]));
+        p += a;
+        q += b;
+        nint++;
+    }
     if (total_ms) *total_ms = ms;
-    if (prop_ms) *prop_ms = -1.0;
-    if (scan_ms) *scan_ms = -1.0;
+    if (prop_ms) *prop_ms = c->kev_used ? p : -1.0;
+    if (scan_ms) *scan_ms = c->kev_used ? q : -1.0;
+    if (n_intervals) *n_intervals = nint;
     return PLOM_OK;
 }
 

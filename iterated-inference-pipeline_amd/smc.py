@@ -52,7 +52,8 @@ EXPORTS = [
     "plom_last_error", "plom_ctx_create", "plom_ctx_destroy", "plom_model_count", "plom_model_find",
     "plom_model_name", "plom_model_info_get", "plom_smc_run", "plom_filter_create", "plom_filter_reset",
     "plom_filter_run", "plom_filter_read", "plom_filter_destroy", "plom_filter_device_bytes",
-    "plom_batch_run", "plom_sync", "plom_last_timing", "plom_selftest_math", "plom_selftest_normals",
+    "plom_batch_run", "plom_sync", "plom_ctx_set_kernel_timing", "plom_last_timing", "plom_selftest_math",
+    "plom_selftest_normals",
 ]
 
 _lib = None
@@ -88,7 +89,8 @@ def lib():
     L.plom_filter_device_bytes.restype = C.c_int64
     L.plom_batch_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.c_int]
     L.plom_sync.argtypes = [C.c_void_p]
-    L.plom_last_timing.argtypes = [C.c_void_p, dp, dp, dp]
+    L.plom_ctx_set_kernel_timing.argtypes = [C.c_void_p, C.c_int]
+    L.plom_last_timing.argtypes = [C.c_void_p, dp, dp, dp, C.POINTER(C.c_int32)]
     L.plom_selftest_math.argtypes = [C.c_void_p, C.c_int, dp, dp]
     L.plom_selftest_normals.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
     _lib = L
@@ -129,10 +131,15 @@ class Context:
     def sync(self):
         check(lib().plom_sync(self.h))
 
-    def last_timing_ms(self):
-        t = C.c_double()
-        check(lib().plom_last_timing(self.h, C.byref(t), None, None))
-        return t.value
+    def set_kernel_timing(self, on):
+        check(lib().plom_ctx_set_kernel_timing(self.h, 1 if on else 0))
+
+    def last_timing(self):
+        """(total_ms, propagate_ms, scan_ms, n_intervals) of the last run."""
+        t, p, q = C.c_double(), C.c_double(), C.c_double()
+        n = C.c_int32()
+        check(lib().plom_last_timing(self.h, C.byref(t), C.byref(p), C.byref(q), C.byref(n)))
+        return t.value, p.value, q.value, n.value
 
 
 # ---------------------------------------------------------------------- models

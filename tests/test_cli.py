@@ -1,0 +1,99 @@
+"""The drop-in `smc` command: host-side parameter logic (dry run, no GPU) and, on the
+GPU, the full run against the Python API and the CPU oracle."""
+import json
+import math
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from iip_amd import configs, smc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "iterated-inference-pipeline_amd")
+SMC = os.path.join(PKG, "cli", "smc")
+
+
+def model_dir(tmp_path, name):
+    d = tmp_path / name
+    d.mkdir()
+    subprocess.check_call(["python", os.path.join(PKG, "plom_build.py"), os.path.join(PKG, "examples", name), "-o", str(d)],
+                          stdout=subprocess.DEVNULL)
+    return d
+
+
+def run_cli(d, args, theta_path):
+    with open(theta_path) as f:
+        return subprocess.run([SMC] + args, cwd=str(d), stdin=f, capture_output=True, text=True, timeout=600)
+
+
+def test_settings_schema(tmp_path):
+    d = model_dir(tmp_path, "sirs4")
+    s = json.load(open(d / ".settings.json"))
+    assert s["model"] == "sirs4" and s["POP_SIZE_EQ_SUM_SV"] is False and s["remainder"] == "R"
+    assert s["cst"]["N_TS"] == 4 and s["cst"]["N_CAC"] if "N_CAC" in s["cst"] else s["cst"]["N_C"] == 4
+    assert s["orders"]["par_sv"] == ["S", "I"]
+    assert len(s["data"]["times"]) == s["cst"]["N_DATA"] + 1 and s["data"]["times"][:3] == [0, 7, 14]
+    assert len(s["coupling"]["matrix"]) == 4
+
+
+def test_dry_run_parameters(tmp_path):
+    """Natural parameters = back_transform(transform(guess)) with the unit multiplier and
+    the duration -> rate conversion 1/(1e-5 + d) (transform.c:462-480); initial state =
+    proportions x N (prediction_util.c:82-115)."""
+    d = model_dir(tmp_path, "sir")
+    r = run_cli(d, ["sde", "-J", "10", "-o", "52", "--dry_run"], os.path.join(PKG, "examples", "sir", "theta.json"))
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    logit = lambda p: math.log(p / (1 - p))
+    inv_logit = lambda x: 1 / (1 + math.exp(-x)) if x > 0 else math.exp(x) / (1 + math.exp(x))
+    beta = math.exp(math.log(0.5))
+    gamma = 1.0 / (0.00001 + math.exp(math.log(5.0)))
+    rep = inv_logit(logit(0.6))
+    phi = math.exp(math.log(0.1))
+    assert out["pv"] == [beta, gamma, rep, phi, 1e6]
+    assert out["x0"] == [inv_logit(logit(0.999)) * 1e6, inv_logit(logit(0.001)) * 1e6, 0.0]
+    assert out["N"] == 52 and out["dt"] == 0.25
+
+
+def test_theta_remainder_and_errors(tmp_path):
+    d = model_dir(tmp_path, "sir")
+    theta = json.load(open(os.path.join(PKG, "examples", "sir", "theta.json")))
+    # no remainder state missing here, but an IC sum > 1 must fail like load.c:369-372
+    theta["parameter"]["I"]["group"]["all"]["guess"]["value"] = 0.5
+    p = tmp_path / "bad.json"
+    p.write_text(json.dumps(theta) + "\n")
+    r = run_cli(d, ["sde", "-J", "10", "--dry_run"], str(p))
+    assert r.returncode != 0 and "greater than 1.0" in r.stderr
+    r = run_cli(d, ["psr", "-J", "10"], os.path.join(PKG, "examples", "sir", "theta.json"))
+    assert r.returncode == 1
+
+
+@pytest.mark.gpu
+def test_cli_run_matches_api(tmp_path, ctx):
+    """`smc sde -J 1000 -o 52 -P -b` gives the Python API's (and the oracle's) log-likelihood
+    for the same natural parameters and random key (seed 2 + id)."""
+    from oracle import oracle
+    d = model_dir(tmp_path, "sir")
+    theta = os.path.join(PKG, "examples", "sir", "theta.json")
+    dry = json.loads(run_cli(d, ["sde", "-J", "1000", "-o", "52", "--dry_run"], theta).stdout)
+    print("cli run", flush=True)
+    r = run_cli(d, ["sde", "-J", "1000", "-o", "52", "-e", "-d"], theta)
+    assert r.returncode == 0, r.stderr
+    line = [l for l in r.stdout.splitlines() if l.startswith("logV_17g:")][0]
+    ll_cli = float(line.split(":")[1])
+    m = smc.Model.get("sir")
+    times, data = m.dataset(52)
+    pv, x0 = np.array(dry["pv"]), np.array(dry["x0"])
+    g = smc.run(ctx, m, 1000, times, data, pv, x0, seed=2, stream=0)
+    o = oracle.smc(m.id, 1000, times, data, pv, x0, seed=2, stream=0)
+    assert ll_cli == g.loglik == o["loglik"]
+    trace = open(d / "trace_0.csv").read().splitlines()
+    assert trace[0].startswith("index,S:all,I:all,beta:all") and trace[0].endswith("log_like")
+    # pipe mode: theta echoed with no trailing newline
+    r = run_cli(d, ["sde", "-J", "1000", "-o", "52", "-P", "-b", "-e", "-d"], theta)
+    assert r.returncode == 0 and not r.stdout.endswith("\n")
+    echoed = json.loads(r.stdout)
+    assert echoed["parameter"]["beta"]["group"]["all"]["guess"]["value"] == pytest.approx(0.5, rel=1e-15)

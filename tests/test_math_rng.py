@@ -89,3 +89,9 @@ def test_phi_central_symmetry():
     lo = oracle.math(-xs)[:, 2]
     hi = oracle.math(xs)[:, 2]
     np.testing.assert_allclose(lo + hi, 1.0, rtol=0, atol=2.3e-16)
+
+
+def test_sincos_domain():
+    """sin/cos are defined on |x| <= 2^20 and NaN outside (the same on host and device)."""
+    r = oracle.math(np.array([2.0 ** 20, -2.0 ** 20, 2.0 ** 20 * 1.0000001, 1e300]))
+    assert np.isfinite(r[:2, 3:5]).all() and np.isnan(r[2:, 3:5]).all()

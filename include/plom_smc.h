@@ -126,9 +126,17 @@ int plom_filter_read(plom_ctx *ctx, plom_filter *f, plom_filter_outputs *out); /
 int plom_filter_destroy(plom_ctx *ctx, plom_filter *f);
 int64_t plom_filter_device_bytes(const plom_filter_desc *desc); /* bytes a filter holds on the device */
 
-/* many resident filters advanced together: one batched launch per interval for all of
- * them (uneven J and N allowed); results are bit-identical to plom_filter_run */
-int plom_batch_run(plom_ctx *ctx, int n_filters, plom_filter **filters, int use_graph);
+/* Many resident filters advanced together: one batched launch round per interval for
+ * all of them (uneven models, J and N allowed); each filter's results are bit-identical
+ * to plom_filter_run.  A batch plan keeps the descriptors and tile maps on the device;
+ * with use_graph the whole launch sequence (reset, every interval, final resampling) is
+ * captured once as a hipGraph and replayed by every later launch.  The filters' inputs
+ * may not change size while a plan holds them. */
+typedef struct plom_batch plom_batch;
+int plom_batch_create(plom_ctx *ctx, int n_filters, plom_filter **filters, plom_batch **out);
+int plom_batch_launch(plom_ctx *ctx, plom_batch *batch, int use_graph);  /* asynchronous */
+int plom_batch_destroy(plom_ctx *ctx, plom_batch *batch);
+int plom_batch_run(plom_ctx *ctx, int n_filters, plom_filter **filters, int use_graph); /* create + launch + destroy */
 
 int plom_sync(plom_ctx *ctx);
 

@@ -794,92 +794,198 @@ extern "C" int plom_smc_run(plom_ctx *c, const plom_filter_desc *d, const plom_f
 }
 
 /* ------------------------------------------------------------------ batches */
-extern "C" int plom_batch_run(plom_ctx *c, int nf, plom_filter **fs, int use_graph)
+/* back to interval 0 for every filter of a batch (a kernel, so that it can sit inside a
+ * captured graph: memset nodes are not used in graphs) */
+__global__ void k_reset(plom_dfilter *filters, int nf)
 {
-    (void)use_graph;
-    if (!c || nf < 0 || (nf > 0 && !fs)) return set_err(PLOM_ERR_ARG, "bad batch arguments");
-    if (nf == 0) return PLOM_OK;
-    HIPCHK(hipSetDevice(c->device));
-    for (int i = 0; i < nf; i++) {
-        if (!fs[i]) return set_err(PLOM_ERR_ARG, "null filter %d", i);
-        if (fs[i]->has_run) {
-            int rc = filter_reset_async(c, fs[i]);
-            if (rc != PLOM_OK) return rc;
-        }
+    const int fi = blockIdx.x;
+    if (fi >= nf) return;
+    plom_dfilter *f = &filters[fi];
+    for (int n = threadIdx.x; n < f->N; n += blockDim.x) {
+        f->rflag[n] = 0;
+        f->llike_n[n] = 0.0;
+        f->ess_n[n] = 0.0;
     }
-    /* device array of descriptors: copies of each filter's descriptor, in N-descending order */
-    std::vector<int> order((size_t)nf);
-    for (int i = 0; i < nf; i++) order[(size_t)i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return fs[a]->desc.N > fs[b]->desc.N; });
-    /* groups by (model, precision, variant) */
-    struct group { int model, prec, variant; std::vector<int2> tiles; std::vector<int> prefix; };
+    if (threadIdx.x == 0) {
+        f->cnt[0] = 0u;
+        f->cnt[1] = 0u;
+        for (int k = 0; k < 4; k++) f->scal[k] = 0.0;
+        f->n_all_fail[0] = 0;
+        f->step = 0;
+    }
+}
+
+struct plom_batch {
+    int nf;
+    std::vector<plom_filter *> fs;     /* in N-descending order (row r of the descriptor array) */
+    struct group { int model, prec, variant; std::vector<int2> tiles; int2 *d; };
     std::vector<group> groups;
     std::vector<int2> all_tiles;
-    std::vector<int> all_prefix;
-    int maxN = fs[order[0]]->desc.N;
-    for (int r = 0; r < nf; r++) {
-        plom_filter *f = fs[order[(size_t)r]];
-        group *g = nullptr;
-        for (auto &gg : groups)
-            if (gg.model == f->desc.model_id && gg.prec == f->desc.precision && gg.variant == f->variant) g = &gg;
-        if (!g) { groups.push_back(group{f->desc.model_id, f->desc.precision, f->variant, {}, {}}); g = &groups.back(); }
-        for (int t = 0; t < f->h.T; t++) { g->tiles.push_back(make_int2(r, t)); all_tiles.push_back(make_int2(r, t)); }
-    }
-    /* active tile prefix per interval s: filters are sorted by N, so active ones are a prefix */
-    auto prefix_len = [&](const std::vector<int2> &tl, int s) {
-        int k = 0;
-        while (k < (int)tl.size() && fs[order[(size_t)tl[(size_t)k].x]]->desc.N > s) k++;
-        return k;
-    };
-    /* one device buffer: descriptors, then tile maps */
-    size_t bytes = align256((size_t)nf * sizeof(plom_dfilter)) + align256(all_tiles.size() * sizeof(int2));
-    for (auto &g : groups) bytes += align256(g.tiles.size() * sizeof(int2));
-    char *buf = nullptr;
-    HIPCHK(hipMallocAsync((void **)&buf, bytes, c->stream));
-    std::vector<plom_dfilter> hd((size_t)nf);
-    for (int r = 0; r < nf; r++) hd[(size_t)r] = fs[order[(size_t)r]]->h;
-    plom_dfilter *ddf = (plom_dfilter *)buf;
-    size_t o = align256((size_t)nf * sizeof(plom_dfilter));
-    int2 *d_all = (int2 *)(buf + o);
-    o += align256(all_tiles.size() * sizeof(int2));
-    std::vector<int2 *> d_groups;
-    for (auto &g : groups) { d_groups.push_back((int2 *)(buf + o)); o += align256(g.tiles.size() * sizeof(int2)); }
-    HIPCHK(hipMemcpyAsync(ddf, hd.data(), (size_t)nf * sizeof(plom_dfilter), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(d_all, all_tiles.data(), all_tiles.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
-    for (size_t gi = 0; gi < groups.size(); gi++)
-        HIPCHK(hipMemcpyAsync(d_groups[gi], groups[gi].tiles.data(), groups[gi].tiles.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    int2 *d_all;
+    plom_dfilter *ddf;
+    char *buf;
+    int maxN;
+    hipGraphExec_t graph;
+};
+
+/* active tiles at interval s: filters are sorted by N, so the active ones are a prefix */
+static int prefix_len(const plom_batch *b, const std::vector<int2> &tl, int s)
+{
     int k = 0;
-    for (int s = 0; s < maxN; s++) {
-        if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
-        for (size_t gi = 0; gi < groups.size(); gi++) {
-            const int nb = prefix_len(groups[gi].tiles, s);
-            if (nb > 0) g_models[groups[gi].model].launch_prop(groups[gi].prec, groups[gi].variant, nb, c->stream, ddf, d_groups[gi]);
-        }
-        if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
-        const int nb = prefix_len(all_tiles, s);
-        hipLaunchKernelGGL(k_scan, dim3(nb), dim3(TILE), 0, c->stream, ddf, (const int2 *)d_all);
-    }
-    if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
-    c->kev_used = k;
-    /* final resampling for the filters that keep ancestry or the final cloud */
+    while (k < (int)tl.size() && b->fs[(size_t)tl[(size_t)k].x]->desc.N > s) k++;
+    return k;
+}
+
+static int batch_create(plom_ctx *c, int nf, plom_filter **in, plom_batch **out)
+{
+    plom_batch *b = new plom_batch();
+    b->nf = nf;
+    b->graph = nullptr;
+    b->fs.assign(in, in + nf);
+    std::stable_sort(b->fs.begin(), b->fs.end(), [](plom_filter *x, plom_filter *y) { return x->desc.N > y->desc.N; });
+    b->maxN = b->fs[0]->desc.N;
     for (int r = 0; r < nf; r++) {
-        plom_filter *f = fs[order[(size_t)r]];
-        if (needs_final(f)) {
-            /* tiles of filter r are contiguous in all_tiles */
-            size_t first = 0;
-            while (all_tiles[first].x != r) first++;
-            enqueue_final(c, f, d_all + first, ddf, f->h.T);
+        plom_filter *f = b->fs[(size_t)r];
+        plom_batch::group *g = nullptr;
+        for (auto &gg : b->groups)
+            if (gg.model == f->desc.model_id && gg.prec == f->desc.precision && gg.variant == f->variant) g = &gg;
+        if (!g) {
+            b->groups.push_back(plom_batch::group{f->desc.model_id, f->desc.precision, f->variant, {}, nullptr});
+            g = &b->groups.back();
         }
+        for (int t = 0; t < f->h.T; t++) {
+            g->tiles.push_back(make_int2(r, t));
+            b->all_tiles.push_back(make_int2(r, t));
+        }
+    }
+    size_t bytes = align256((size_t)nf * sizeof(plom_dfilter)) + align256(b->all_tiles.size() * sizeof(int2));
+    for (auto &g : b->groups) bytes += align256(g.tiles.size() * sizeof(int2));
+    hipError_t e = hipMalloc((void **)&b->buf, bytes);
+    if (e != hipSuccess) {
+        delete b;
+        return set_err(PLOM_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    std::vector<plom_dfilter> hd((size_t)nf);
+    for (int r = 0; r < nf; r++) hd[(size_t)r] = b->fs[(size_t)r]->h;
+    b->ddf = (plom_dfilter *)b->buf;
+    size_t o = align256((size_t)nf * sizeof(plom_dfilter));
+    b->d_all = (int2 *)(b->buf + o);
+    o += align256(b->all_tiles.size() * sizeof(int2));
+    for (auto &g : b->groups) { g.d = (int2 *)(b->buf + o); o += align256(g.tiles.size() * sizeof(int2)); }
+    HIPCHK(hipMemcpyAsync(b->ddf, hd.data(), (size_t)nf * sizeof(plom_dfilter), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b->d_all, b->all_tiles.data(), b->all_tiles.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    for (auto &g : b->groups)
+        HIPCHK(hipMemcpyAsync(g.d, g.tiles.data(), g.tiles.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *out = b;
+    return PLOM_OK;
+}
+
+/* the launch sequence of a batch: reset, every interval, final resampling */
+static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing)
+{
+    hipLaunchKernelGGL(k_reset, dim3(b->nf), dim3(256), 0, c->stream, b->ddf, b->nf);
+    int k = 0;
+    for (int s = 0; s < b->maxN; s++) {
+        if (timing && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
+        for (auto &g : b->groups) {
+            const int nb = prefix_len(b, g.tiles, s);
+            if (nb > 0) g_models[g.model].launch_prop(g.prec, g.variant, nb, c->stream, b->ddf, g.d);
+        }
+        if (timing && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
+        const int nb = prefix_len(b, b->all_tiles, s);
+        hipLaunchKernelGGL(k_scan, dim3(nb), dim3(TILE), 0, c->stream, b->ddf, (const int2 *)b->d_all);
+    }
+    if (timing && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
+    c->kev_used = k;
+    size_t first = 0;
+    for (int r = 0; r < b->nf; r++) {
+        plom_filter *f = b->fs[(size_t)r];
+        while (b->all_tiles[first].x != r) first++;
+        if (needs_final(f)) enqueue_final(c, f, b->d_all + first, b->ddf, f->h.T);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    /* bring each filter's device descriptor (step counter) in line with the batch copy */
-    for (int r = 0; r < nf; r++)
-        HIPCHK(hipMemcpyAsync(fs[order[(size_t)r]]->df, ddf + r, sizeof(plom_dfilter), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipFreeAsync(buf, c->stream));
-    for (int i = 0; i < nf; i++) fs[i]->has_run = 1;
     return PLOM_OK;
+}
+
+static int batch_launch(plom_ctx *c, plom_batch *b, int use_graph)
+{
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    if (use_graph) {
+        if (!b->graph) {
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = batch_enqueue(c, b, 0);
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (rc != PLOM_OK) return rc;
+            if (e != hipSuccess) return set_err(PLOM_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+            HIPCHK(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+            HIPCHK(hipGraphDestroy(g));
+        }
+        c->kev_used = 0;
+        HIPCHK(hipGraphLaunch(b->graph, c->stream));
+    } else {
+        int rc = batch_enqueue(c, b, c->ktiming);
+        if (rc != PLOM_OK) return rc;
+    }
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    for (auto f : b->fs) f->has_run = 1;
+    return PLOM_OK;
+}
+
+/* after a batch, each filter's own device descriptor gets the batch copy's state back
+ * (plom_filter_read reads results through the filter's own buffers) */
+static int batch_writeback(plom_ctx *c, plom_batch *b)
+{
+    for (int r = 0; r < b->nf; r++)
+        HIPCHK(hipMemcpyAsync(b->fs[(size_t)r]->df, b->ddf + r, sizeof(plom_dfilter), hipMemcpyDeviceToDevice, c->stream));
+    return PLOM_OK;
+}
+
+static int batch_destroy(plom_ctx *c, plom_batch *b)
+{
+    if (!b) return PLOM_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (b->graph) HIPCHK(hipGraphExecDestroy(b->graph));
+    HIPCHK(hipFree(b->buf));
+    delete b;
+    return PLOM_OK;
+}
+
+extern "C" int plom_batch_create(plom_ctx *c, int nf, plom_filter **fs, plom_batch **out)
+{
+    if (!c || nf <= 0 || !fs || !out) return set_err(PLOM_ERR_ARG, "bad batch arguments");
+    for (int i = 0; i < nf; i++)
+        if (!fs[i]) return set_err(PLOM_ERR_ARG, "null filter %d", i);
+    HIPCHK(hipSetDevice(c->device));
+    return batch_create(c, nf, fs, out);
+}
+
+extern "C" int plom_batch_launch(plom_ctx *c, plom_batch *b, int use_graph)
+{
+    if (!c || !b) return set_err(PLOM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    int rc = batch_launch(c, b, use_graph);
+    if (rc == PLOM_OK) rc = batch_writeback(c, b);
+    return rc;
+}
+
+extern "C" int plom_batch_destroy(plom_ctx *c, plom_batch *b)
+{
+    if (!c) return set_err(PLOM_ERR_ARG, "null context");
+    return batch_destroy(c, b);
+}
+
+extern "C" int plom_batch_run(plom_ctx *c, int nf, plom_filter **fs, int use_graph)
+{
+    if (!c || nf < 0 || (nf > 0 && !fs)) return set_err(PLOM_ERR_ARG, "bad batch arguments");
+    if (nf == 0) return PLOM_OK;
+    plom_batch *b = nullptr;
+    int rc = plom_batch_create(c, nf, fs, &b);
+    if (rc != PLOM_OK) return rc;
+    rc = plom_batch_launch(c, b, use_graph);
+    int rc2 = batch_destroy(c, b);
+    return rc != PLOM_OK ? rc : rc2;
 }
 
 extern "C" int plom_ctx_set_kernel_timing(plom_ctx *c, int on)

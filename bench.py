@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--N", type=int, default=520)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-J", type=int, default=20000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each filter or request "
@@ -110,6 +112,15 @@ def main():
         return 2
     load_package()
     from iip_amd import smc, configs
+
+    if args.copy_calibration:
+        with smc.Context(0) as ctx:
+            gbps, ms = smc.copy_bandwidth_gbps(ctx, 1 << 30, 1)
+        print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30, "GBps": gbps, "ms": ms}))
+        return 0
+
+    with smc.Context(0) as ctx:
+        copy_gbps, _ = smc.copy_bandwidth_gbps(ctx, 1 << 30, 5)
 
     J, N = args.J, args.N
     fp64 = run_gpu(smc, configs, "fp64", args.steps, args.warmup, J, N)
@@ -123,7 +134,8 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": traffic,
                 "kernel": "k_propagate<sirs4,double,full>",
                 "algorithmic_bytes_per_launch": per_launch,
-                "avg_launch_ms": round(fp64["propagate_ms_per_launch"], 5)}
+                "avg_launch_ms": round(fp64["propagate_ms_per_launch"], 5),
+                "measured_copy_GBps": round(copy_gbps, 1)}
 
     cpu = None
     if not args.no_cpu:

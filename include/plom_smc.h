@@ -138,6 +138,30 @@ int plom_batch_launch(plom_ctx *ctx, plom_batch *batch, int use_graph);  /* asyn
 int plom_batch_destroy(plom_ctx *ctx, plom_batch *batch);
 int plom_batch_run(plom_ctx *ctx, int n_filters, plom_filter **filters, int use_graph); /* create + launch + destroy */
 
+/* Served stream (no counterpart in the reference): continuous batching of filter
+ * requests.  plom_server_submit copies the inputs and queues the request;
+ * plom_server_step admits queued requests while their device bytes fit the budget
+ * (FIFO), advances every admitted request by one interval in one batched launch round,
+ * and retires finished ones; plom_server_status gives PLOM_SRV_QUEUED / RUNNING / DONE;
+ * plom_server_result returns log-likelihood, n_all_fail, loglik_n and ess_n of a done
+ * request and releases it.  Results equal plom_smc_run on the same descriptor and inputs
+ * (FP64: bit for bit), whatever else is being served. */
+#define PLOM_SRV_QUEUED 0
+#define PLOM_SRV_RUNNING 1
+#define PLOM_SRV_DONE 2
+typedef struct {
+    int64_t hbm_budget_bytes;  /* 0: 80 % of the free device memory at creation */
+    int32_t max_active;        /* 0: 4096 */
+    int32_t reserved;
+} plom_server_config;
+typedef struct plom_server plom_server;
+int plom_server_create(plom_ctx *ctx, const plom_server_config *cfg, plom_server **out);
+int plom_server_submit(plom_server *srv, const plom_filter_desc *desc, const plom_filter_inputs *in, uint64_t *req_id);
+int plom_server_step(plom_server *srv, int32_t *n_active, int32_t *n_queued, int32_t *n_done);
+int plom_server_status(plom_server *srv, uint64_t req_id);
+int plom_server_result(plom_server *srv, uint64_t req_id, plom_filter_outputs *out);
+int plom_server_destroy(plom_server *srv);
+
 int plom_sync(plom_ctx *ctx);
 
 /* Timing of the last plom_filter_run / plom_batch_run from HIP events on the context
@@ -149,6 +173,8 @@ int plom_last_timing(plom_ctx *ctx, double *total_ms, double *propagate_ms, doub
 
 /* device self-test of the shared arithmetic: out[i*6+..] = exp, log, Phi, sin, cos, sqrt of in[i] */
 int plom_selftest_math(plom_ctx *ctx, int n, const double *in, double *out);
+/* plain device copy of nbytes (read nbytes + write nbytes), repeated: bandwidth reference */
+int plom_selftest_copy(plom_ctx *ctx, int64_t nbytes, int reps, double *ms_per_copy);
 /* device self-test of the streams: the first n normals of particle j at interval n0 */
 int plom_selftest_normals(plom_ctx *ctx, uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j,
                           int n, double *out);

@@ -37,6 +37,7 @@ class Emitter:
         self.m = model
         self.p = "plom_%s" % model.name.lower()
         self.used_pars = set()
+        self.time_funcs = None
 
     # ---------------------------------------------------------------- symbols
     def xidx(self, state, cac):
@@ -63,6 +64,16 @@ class Emitter:
             raise E.__class__  # unreachable: symbols are checked at model build
 
         def call(fn, args, node):
+            if fn in ("sin", "cos", "exp", "log", "sqrt", "heaviside", "ramp", "slowstep", "pow") and \
+                    E.symbols(node) <= {"t", "M_PI", "ONE_YEAR"} and not any(c.fn == "mix" for c in E.calls(node)) \
+                    and hasattr(self, "time_funcs") and self.time_funcs is not None:
+                simple = {"sin": "plom_sin", "cos": "plom_cos", "exp": "plom_exp", "log": "plom_log",
+                          "sqrt": "plom_sqrt", "heaviside": "plom_heaviside", "ramp": "plom_ramp",
+                          "slowstep": "plom_slowstep", "pow": "plom_pow"}
+                text = "%s%s(%s)" % (simple[fn], "" if real == "double" else "f", ", ".join(args))
+                if text not in self.time_funcs:
+                    self.time_funcs[text] = "tf%d" % len(self.time_funcs)
+                return self.time_funcs[text]
             if fn == "correct_rate":
                 # skipped in SDE/ODE code (Ccoder.py:55-56, generator_C with no_correct_rate)
                 return "(%s)" % args[0]
@@ -120,8 +131,11 @@ class Emitter:
         for u in range(m.N_TS_INC_UNIQUE):
             L.append("    X[%d] = %s; /* reset_inc */" % (m.inc_offset + u, zero))
         L.append("    double t = t0;")
-        L.append("    while (t < t1) {")
+        L.append("    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));")
+        L.append("    for (int k = 0; k < K; k++) {")
         L.append("        const %s tr = (%s)t; (void)tr;" % (Rt, Rt))
+        tf_at = len(L)
+        self.time_funcs = {}
         # States read across populations (inside mix()) keep their old value until every
         # population is done; the others are written back right after their population's
         # block.  Every right-hand side reads only old values either way, as in the
@@ -228,6 +242,11 @@ class Emitter:
         L.append("        t += dt;")
         L.append("    }")
         L.append("}")
+        # time-only special functions (e.g. the seasonal forcing), once per substep
+        tfs = ["        const %s %s = %s;" % (Rt, name, text) for text, name in
+               sorted(self.time_funcs.items(), key=lambda kv: int(kv[1][2:]))]
+        L[tf_at:tf_at] = tfs
+        self.time_funcs = None
         uni = "PLOM_UNIFORM64" if real == "double" else "PLOM_UNIFORM32"
         binds = ["    const %s P%d = %s(pv[%d]);" % (Rt, k, uni, k) for k in sorted(self.used_pars)]
         L[bind_at:bind_at] = binds

@@ -102,7 +102,9 @@ PLOM_HD double plom_log_core(double x, int e)
     uint64_t u = plom_d2u(x);
     e += (int)(u >> 52) - 1023;
     double m = plom_u2d((u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
-    if (m > 0x1.6a09e667f3bcdp+0) { m = m * 0.5; e += 1; }
+    const int big = m > 0x1.6a09e667f3bcdp+0;
+    m = big ? m * 0.5 : m;
+    e += big;
     double f = m - 1.0;
     double s = f / (2.0 + f);
     double z = s * s;
@@ -167,6 +169,17 @@ PLOM_HD double plom_cos_kernel(double r)
     return w + (((1.0 - w) - hz) + z * (z * p));
 }
 
+/* sin/cos of r + q pi/2 from sin r, cos r without branches: odd q swaps the pair,
+ * q in {2,3} negates sin, q in {1,2} negates cos (sign-bit flips are exact) */
+PLOM_HD void plom_quadrant(int q, double sr, double cr, double *s, double *c)
+{
+    const int odd = q & 1;
+    double a = odd ? cr : sr;
+    double b = odd ? sr : cr;
+    *s = plom_u2d(plom_d2u(a) ^ ((uint64_t)(q & 2) << 62));
+    *c = plom_u2d(plom_d2u(b) ^ ((uint64_t)((q + 1) & 2) << 62));
+}
+
 #define PLOM_PIO2_1 0x1.921fb54400000p+0
 #define PLOM_PIO2_2 0x1.0b4611a600000p-34
 #define PLOM_PIO2_3 0x1.3198a2e037073p-69
@@ -176,22 +189,16 @@ PLOM_HD double plom_cos_kernel(double r)
 PLOM_HD void plom_sincos(double x, double *s, double *c)
 {
     /* outside |x| <= 2^20 the quadrant count would overflow an int (undefined and
-     * different between x86 and gfx950): NaN on both sides instead */
-    if (!(plom_fabs(x) <= 0x1p20)) {
-        *s = PLOM_NAN;
-        *c = PLOM_NAN;
-        return;
-    }
-    double kd = plom_rint(x * PLOM_2_OVER_PI);
-    double r = x - kd * PLOM_PIO2_1;
+     * different between x86 and gfx950): NaN on both sides instead, without a branch */
+    const int ok = plom_fabs(x) <= 0x1p20;
+    const double xs = ok ? x : 0.0;
+    double kd = plom_rint(xs * PLOM_2_OVER_PI);
+    double r = xs - kd * PLOM_PIO2_1;
     r = r - kd * PLOM_PIO2_2;
     r = r - kd * PLOM_PIO2_3;
-    int q = ((int)kd) & 3;
-    double sr = plom_sin_kernel(r), cr = plom_cos_kernel(r);
-    if (q == 0) { *s = sr; *c = cr; }
-    else if (q == 1) { *s = cr; *c = -sr; }
-    else if (q == 2) { *s = -sr; *c = -cr; }
-    else { *s = -cr; *c = sr; }
+    plom_quadrant(((int)kd) & 3, plom_sin_kernel(r), plom_cos_kernel(r), s, c);
+    *s = ok ? *s : PLOM_NAN;
+    *c = ok ? *c : PLOM_NAN;
 }
 
 PLOM_HD double plom_sin(double x) { double s, c; plom_sincos(x, &s, &c); return s; }
@@ -206,12 +213,7 @@ PLOM_HD void plom_sincos_2pi(double u, double *s, double *c)
     double qd = plom_rint(4.0 * u);
     double r = u - 0.25 * qd;
     double x = r * PLOM_TWO_PI;
-    int q = ((int)qd) & 3;
-    double sr = plom_sin_kernel(x), cr = plom_cos_kernel(x);
-    if (q == 0) { *s = sr; *c = cr; }
-    else if (q == 1) { *s = cr; *c = -sr; }
-    else if (q == 2) { *s = -sr; *c = -cr; }
-    else { *s = -cr; *c = sr; }
+    plom_quadrant(((int)qd) & 3, plom_sin_kernel(x), plom_cos_kernel(x), s, c);
 }
 
 /* ---------------------------------------------------------------- powers */

@@ -32,10 +32,25 @@ static __device__ inline __attribute__((always_inline)) float plom_uniform32(flo
 }
 #define PLOM_UNIFORM64(x) plom_uniform64(x)
 #define PLOM_UNIFORM32(x) plom_uniform32(x)
+#define PLOM_UNIFORM_INT(x) ((int)__builtin_amdgcn_readfirstlane((uint32_t)(x)))
 #else
 #define PLOM_UNIFORM64(x) (x)
 #define PLOM_UNIFORM32(x) (x)
+#define PLOM_UNIFORM_INT(x) (x)
 #endif
+
+/* number of Euler substeps of an interval: the iterations of
+ * t = t0; while (t < t1) t += dt;  (prediction_util.c:275-287, t accumulated in double) */
+PLOM_HD int plom_substeps(double t0, double t1, double dt)
+{
+    int K = 0;
+    double t = t0;
+    while (t < t1) {
+        t += dt;
+        K++;
+    }
+    return K;
+}
 
 #define PLOM_NO_DEM_STO 1u
 #define PLOM_NO_ENV_STO 2u

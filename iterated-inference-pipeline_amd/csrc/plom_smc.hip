@@ -135,17 +135,6 @@ __device__ inline uint64_t scan256(uint64_t v, uint64_t *lds)
     return v + off;
 }
 
-__device__ inline uint32_t arrive(uint32_t *cnt)
-{
-    __shared__ uint32_t s_old;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_old = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    __threadfence();
-    return s_old;
-}
-
 /* systematic resampling of interval n: ancestor of output slot k */
 __device__ inline int select_ancestor(const plom_dfilter *f, int k, int n)
 {
@@ -293,51 +282,64 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
     const double s = tree256(w, lds);
     const double s2 = tree256(w * w, lds);
     if (threadIdx.x == 0) { f->tmax[tile] = mt; f->tsum[tile] = s; f->tsum2[tile] = s2; }
-    if (arrive(&f->cnt[0]) == (uint32_t)(f->T - 1)) {
-        finalize_weights(f, n, lds);
-        if (threadIdx.x == 0) f->cnt[0] = 0u;
-    }
+}
+
+/* one block per filter: the tiles' statistics -> loglik_n, ESS, resampling flag */
+__global__ __launch_bounds__(TILE) void k_weights(plom_dfilter *filters, const int *fidx)
+{
+    __shared__ double lds[8];
+    const int fi = fidx ? fidx[blockIdx.x] : (int)blockIdx.x;
+    plom_dfilter *f = &filters[fi];
+    const int n = f->step;
+    if (n >= f->N) return;
+    if ((f->flags & PLOM_FLAG_NO_FILTER) || f->nonan[n] == 0) return;
+    finalize_weights(f, n, lds);
 }
 
 __global__ __launch_bounds__(TILE) void k_scan(plom_dfilter *filters, const int2 *tiles)
 {
     __shared__ uint64_t lds[8];
-    __shared__ uint64_t s_carry;
     int fi = 0, tile = blockIdx.x;
     if (tiles) { const int2 tt = tiles[blockIdx.x]; fi = tt.x; tile = tt.y; }
     plom_dfilter *f = &filters[fi];
     const int n = f->step;
     if (n >= f->N) return;
     const bool active = !(f->flags & PLOM_FLAG_NO_FILTER) && f->nonan[n] > 0 && f->rflag[n];
+    if (!active) return;
+    const double m = f->scal[0], S = f->scal[1];
+    const int j = tile * TILE + threadIdx.x;
+    const double l = f->logw[j];
+    uint64_t q = 0;
+    if (l != -PLOM_INF) q = (uint64_t)((plom_exp(l - m) / S) * 0x1p62);
+    const uint64_t c = scan256(q, lds);
+    f->cum[j] = c;
+    if (threadIdx.x == TILE - 1) f->ttot[tile] = c;
+}
+
+/* one block per filter: inclusive prefix of the tile totals, then the next interval */
+__global__ __launch_bounds__(TILE) void k_advance(plom_dfilter *filters, const int *fidx)
+{
+    __shared__ uint64_t lds[8];
+    __shared__ uint64_t s_carry;
+    const int fi = fidx ? fidx[blockIdx.x] : (int)blockIdx.x;
+    plom_dfilter *f = &filters[fi];
+    const int n = f->step;
+    if (n >= f->N) return;
+    const bool active = !(f->flags & PLOM_FLAG_NO_FILTER) && f->nonan[n] > 0 && f->rflag[n];
     if (active) {
-        const double m = f->scal[0], S = f->scal[1];
-        const int j = tile * TILE + threadIdx.x;
-        const double l = f->logw[j];
-        uint64_t q = 0;
-        if (l != -PLOM_INF) q = (uint64_t)((plom_exp(l - m) / S) * 0x1p62);
-        const uint64_t c = scan256(q, lds);
-        f->cum[j] = c;
-        if (threadIdx.x == TILE - 1) f->ttot[tile] = c;
-    }
-    if (arrive(&f->cnt[1]) == (uint32_t)(f->T - 1)) {
-        if (active) {
-            if (threadIdx.x == 0) s_carry = 0;
+        if (threadIdx.x == 0) s_carry = 0;
+        __syncthreads();
+        for (int base = 0; base < f->T; base += TILE) {
+            const int t = base + threadIdx.x;
+            const uint64_t v = t < f->T ? f->ttot[t] : 0ull;
+            const uint64_t c = scan256(v, lds) + s_carry;
+            if (t < f->T) f->tincl[t] = c;
             __syncthreads();
-            for (int base = 0; base < f->T; base += TILE) {
-                const int t = base + threadIdx.x;
-                const uint64_t v = t < f->T ? f->ttot[t] : 0ull;
-                const uint64_t c = scan256(v, lds) + s_carry;
-                if (t < f->T) f->tincl[t] = c;
-                __syncthreads();
-                if (threadIdx.x == TILE - 1) s_carry = c;
-                __syncthreads();
-            }
-        }
-        if (threadIdx.x == 0) {
-            f->step = n + 1;
-            f->cnt[1] = 0u;
+            if (threadIdx.x == TILE - 1) s_carry = c;
+            __syncthreads();
         }
     }
+    if (threadIdx.x == 0) f->step = n + 1;
 }
 
 /* resampling of the last interval: ancestry row N-1 and the resampled final cloud */
@@ -369,6 +371,14 @@ __global__ void k_selftest_math(int n, const double *in, double *out)
     out[i * 6 + 3] = plom_sin(x);
     out[i * 6 + 4] = plom_cos(x);
     out[i * 6 + 5] = plom_sqrt(x);
+}
+
+/* plain copy (16-byte vectors, grid-stride): the bandwidth reference and the calibration
+ * of the FETCH_SIZE / WRITE_SIZE counters */
+__global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 __global__ void k_selftest_normals(uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int n, double *out)
@@ -474,10 +484,14 @@ struct plom_filter {
     int variant;
     char *dev;              /* one allocation holding everything */
     size_t dev_bytes;
+    int async_alloc;        /* allocated with hipMallocAsync (server) */
     plom_dfilter *df;       /* device copy of the descriptor */
     plom_dfilter h;         /* host copy (device pointers) */
-    hipGraphExec_t graph;   /* captured launch sequence of plom_filter_run */
     int has_run;
+    /* host staging of the converted inputs: must outlive the asynchronous uploads */
+    std::vector<int32_t> nonan;
+    std::vector<float> pvf;
+    std::vector<char> x0b;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -586,8 +600,6 @@ extern "C" int plom_sync(plom_ctx *c)
 /* reset the per-run device state of a filter (enqueued on the context stream) */
 static int filter_reset_async(plom_ctx *c, plom_filter *f)
 {
-    const layout *Lp = nullptr;
-    (void)Lp;
     plom_dfilter *h = &f->h;
     HIPCHK(hipMemsetAsync(h->cnt, 0, 2 * sizeof(uint32_t), c->stream));
     HIPCHK(hipMemsetAsync(h->scal, 0, 4 * sizeof(double), c->stream));
@@ -602,7 +614,9 @@ static int filter_reset_async(plom_ctx *c, plom_filter *f)
     return PLOM_OK;
 }
 
-extern "C" int plom_filter_create(plom_ctx *c, const plom_filter_desc *d, const plom_filter_inputs *in, plom_filter **out)
+/* create a filter; with async the allocation is stream-ordered and nothing waits (the
+ * caller keeps `in` alive until the stream has passed the uploads) */
+static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom_filter_inputs *in, int async, plom_filter **out)
 {
     if (!c || !out || !in) return set_err(PLOM_ERR_ARG, "null argument");
     plom_model_info mi;
@@ -614,15 +628,16 @@ extern "C" int plom_filter_create(plom_ctx *c, const plom_filter_desc *d, const 
     HIPCHK(hipSetDevice(c->device));
     layout L;
     make_layout(d, &mi, &L);
-    plom_filter *f = (plom_filter *)calloc(1, sizeof(plom_filter));
+    plom_filter *f = new (std::nothrow) plom_filter();
     if (!f) return set_err(PLOM_ERR_NOMEM, "host allocation failed");
     f->desc = *d;
     f->info = mi;
     f->variant = (int)(d->noises_off & 3u);
     f->dev_bytes = L.total;
-    hipError_t e = hipMalloc((void **)&f->dev, L.total);
+    f->async_alloc = async;
+    hipError_t e = async ? hipMallocAsync((void **)&f->dev, L.total, c->stream) : hipMalloc((void **)&f->dev, L.total);
     if (e != hipSuccess) {
-        free(f);
+        delete f;
         return set_err(PLOM_ERR_NOMEM, "hipMalloc(%zu): %s", L.total, hipGetErrorString(e));
     }
     char *b = f->dev;
@@ -650,37 +665,43 @@ extern "C" int plom_filter_create(plom_ctx *c, const plom_filter_desc *d, const 
     h->ancestry = (d->flags & PLOM_FLAG_STORE_ANCESTRY) ? (int32_t *)(b + L.off_anc) : nullptr;
     f->df = (plom_dfilter *)(b + L.off_df);
 
-    /* host-side staging of the inputs */
     const int S = mi.size_proj;
-    std::vector<int32_t> nonan((size_t)d->N);
+    f->nonan.assign((size_t)d->N, 0);
     for (int n = 0; n < d->N; n++) {
         int k = 0;
         for (int ts = 0; ts < mi.n_ts; ts++) k += !(in->data[(size_t)n * mi.n_ts + ts] != in->data[(size_t)n * mi.n_ts + ts]);
-        nonan[(size_t)n] = k;
+        f->nonan[(size_t)n] = k;
     }
-    std::vector<float> pvf((size_t)mi.pv_size);
-    for (int i = 0; i < mi.pv_size; i++) pvf[(size_t)i] = (float)in->pv[i];
-    std::vector<char> x0b((size_t)S * 8);
+    f->pvf.assign((size_t)mi.pv_size, 0.f);
+    for (int i = 0; i < mi.pv_size; i++) f->pvf[(size_t)i] = (float)in->pv[i];
+    f->x0b.assign((size_t)S * 8, 0);
     if (d->precision == PLOM_FP32) {
-        for (int k = 0; k < S; k++) ((float *)x0b.data())[k] = (float)in->x0[k];
+        for (int k = 0; k < S; k++) ((float *)f->x0b.data())[k] = (float)in->x0[k];
     } else {
-        memcpy(x0b.data(), in->x0, (size_t)S * 8);
+        memcpy(f->x0b.data(), in->x0, (size_t)S * 8);
     }
     HIPCHK(hipMemcpyAsync(b + L.off_times, in->times, (size_t)(d->N + 1) * 8, hipMemcpyHostToDevice, c->stream));
     if (d->N > 0) {
         HIPCHK(hipMemcpyAsync(b + L.off_data, in->data, (size_t)d->N * mi.n_ts * 8, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(b + L.off_nonan, nonan.data(), (size_t)d->N * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(b + L.off_nonan, f->nonan.data(), (size_t)d->N * 4, hipMemcpyHostToDevice, c->stream));
     }
     HIPCHK(hipMemcpyAsync(b + L.off_pv, in->pv, (size_t)mi.pv_size * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(b + L.off_pvf, pvf.data(), (size_t)mi.pv_size * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(b + L.off_x0, x0b.data(), (size_t)S * 8, hipMemcpyHostToDevice, c->stream));
-    /* padding lanes of the last tile read log-weight -inf and zero increments */
-    HIPCHK(hipMemsetAsync(b + L.off_logw, 0, (size_t)Jp * 8, c->stream));
+    HIPCHK(hipMemcpyAsync(b + L.off_pvf, f->pvf.data(), (size_t)mi.pv_size * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b + L.off_x0, f->x0b.data(), (size_t)S * 8, hipMemcpyHostToDevice, c->stream));
     rc = filter_reset_async(c, f);
-    if (rc != PLOM_OK) { (void)hipFree(f->dev); free(f); return rc; }
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (rc != PLOM_OK) {
+        if (async) (void)hipFreeAsync(f->dev, c->stream); else (void)hipFree(f->dev);
+        delete f;
+        return rc;
+    }
+    if (!async) HIPCHK(hipStreamSynchronize(c->stream));
     *out = f;
     return PLOM_OK;
+}
+
+extern "C" int plom_filter_create(plom_ctx *c, const plom_filter_desc *d, const plom_filter_inputs *in, plom_filter **out)
+{
+    return filter_create_impl(c, d, in, 0, out);
 }
 
 extern "C" int plom_filter_reset(plom_ctx *c, plom_filter *f)
@@ -694,10 +715,19 @@ extern "C" int plom_filter_destroy(plom_ctx *c, plom_filter *f)
 {
     if (!f) return PLOM_OK;
     if (c) HIPCHK(hipStreamSynchronize(c->stream));
-    if (f->graph) HIPCHK(hipGraphExecDestroy(f->graph));
-    HIPCHK(hipFree(f->dev));
-    free(f);
+    if (f->async_alloc && c) HIPCHK(hipFreeAsync(f->dev, c->stream));
+    else HIPCHK(hipFree(f->dev));
+    delete f;
     return PLOM_OK;
+}
+
+/* weights, fixed-point scan and tile prefix of one interval: K2 (one block per filter),
+ * K3 (one block per tile), K4 (one block per filter) */
+static void enqueue_resample(hipStream_t s, plom_dfilter *df, const int2 *tiles, int ntiles, const int *fidx, int nfilters)
+{
+    hipLaunchKernelGGL(k_weights, dim3(nfilters), dim3(TILE), 0, s, df, fidx);
+    hipLaunchKernelGGL(k_scan, dim3(ntiles), dim3(TILE), 0, s, df, tiles);
+    hipLaunchKernelGGL(k_advance, dim3(nfilters), dim3(TILE), 0, s, df, fidx);
 }
 
 static void enqueue_final(plom_ctx *c, plom_filter *f, const int2 *tiles, plom_dfilter *df, int nblocks)
@@ -729,7 +759,7 @@ extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
         if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
         me->launch_prop(f->desc.precision, f->variant, T, c->stream, f->df, nullptr);
         if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
-        hipLaunchKernelGGL(k_scan, dim3(T), dim3(TILE), 0, c->stream, f->df, (const int2 *)nullptr);
+        enqueue_resample(c->stream, f->df, nullptr, T, nullptr, 1);
     }
     if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
     c->kev_used = k;
@@ -894,7 +924,9 @@ static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing)
         }
         if (timing && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
         const int nb = prefix_len(b, b->all_tiles, s);
-        hipLaunchKernelGGL(k_scan, dim3(nb), dim3(TILE), 0, c->stream, b->ddf, (const int2 *)b->d_all);
+        int nfa = 0;
+        while (nfa < b->nf && b->fs[(size_t)nfa]->desc.N > s) nfa++;
+        enqueue_resample(c->stream, b->ddf, (const int2 *)b->d_all, nb, nullptr, nfa);
     }
     if (timing && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
     c->kev_used = k;
@@ -988,6 +1020,275 @@ extern "C" int plom_batch_run(plom_ctx *c, int nf, plom_filter **fs, int use_gra
     return rc != PLOM_OK ? rc : rc2;
 }
 
+/* ------------------------------------------------------------------ server */
+/* Continuous batching: requests wait in a FIFO on the host; at every step the server
+ * admits the queue head while its device bytes fit the HBM budget (and the slot limit),
+ * then advances every active request by one observation interval with one batched
+ * launch round (one propagate launch per (model, precision, variant) group, one scan
+ * launch), and retires the requests whose last interval was just enqueued (a HIP event
+ * marks their completion).  A request's result depends only on its descriptor and
+ * inputs (its random key is its own stream id), never on the mix around it. */
+#define SRV_QUEUED 0
+#define SRV_RUNNING 1
+#define SRV_DONE 2
+
+struct srv_request {
+    uint64_t id;
+    plom_filter_desc desc;
+    std::vector<double> times, data, pv, x0;
+    plom_filter *f = nullptr;
+    int slot = -1;
+    int progress = 0;          /* intervals enqueued */
+    int state = SRV_QUEUED;
+    hipEvent_t done_ev = nullptr;
+    int64_t bytes = 0;
+};
+
+struct plom_server {
+    plom_ctx *c;
+    int64_t budget, in_use;
+    int max_active;
+    uint64_t next_id;
+    std::vector<srv_request *> queue;          /* FIFO */
+    std::vector<srv_request *> active;
+    std::vector<srv_request *> finished;       /* waiting for their event / result */
+    std::vector<srv_request *> done;
+    int capacity;                              /* descriptor slots on the device */
+    plom_dfilter *d_slots;
+    std::vector<int> free_slots;
+    int2 *d_tiles;                             /* tile maps: all tiles, then per group */
+    size_t tiles_cap;
+    int *d_slots_active;                       /* slot index of every active request */
+    std::vector<int> h_active[2];
+    int dirty;                                 /* active set changed: rebuild the maps */
+    struct group { int model, prec, variant; size_t off, n; };
+    std::vector<group> groups;
+    size_t n_all;
+    std::vector<int2> h_tiles[2];              /* double-buffered host copies of the maps */
+    int hbuf;
+    int64_t steps;
+};
+
+static int srv_rebuild_maps(plom_server *S)
+{
+    plom_ctx *c = S->c;
+    S->hbuf ^= 1;
+    std::vector<int2> &h = S->h_tiles[S->hbuf];
+    h.clear();
+    S->groups.clear();
+    for (auto r : S->active)
+        for (int t = 0; t < r->f->h.T; t++) h.push_back(make_int2(r->slot, t));
+    S->n_all = h.size();
+    for (auto r : S->active) {
+        plom_server::group *g = nullptr;
+        for (auto &gg : S->groups)
+            if (gg.model == r->desc.model_id && gg.prec == r->desc.precision && gg.variant == r->f->variant) g = &gg;
+        if (!g) { S->groups.push_back(plom_server::group{r->desc.model_id, r->desc.precision, r->f->variant, 0, 0}); }
+    }
+    for (auto &g : S->groups) {
+        g.off = h.size();
+        for (auto r : S->active)
+            if (g.model == r->desc.model_id && g.prec == r->desc.precision && g.variant == r->f->variant)
+                for (int t = 0; t < r->f->h.T; t++) h.push_back(make_int2(r->slot, t));
+        g.n = h.size() - g.off;
+    }
+    if (h.size() > S->tiles_cap) {
+        if (S->d_tiles) HIPCHK(hipFreeAsync(S->d_tiles, c->stream));
+        S->tiles_cap = h.size() * 2 + 1024;
+        HIPCHK(hipMallocAsync((void **)&S->d_tiles, S->tiles_cap * sizeof(int2), c->stream));
+    }
+    if (!h.empty()) HIPCHK(hipMemcpyAsync(S->d_tiles, h.data(), h.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    std::vector<int> &ha = S->h_active[S->hbuf];
+    ha.clear();
+    for (auto r : S->active) ha.push_back(r->slot);
+    if (!ha.empty()) HIPCHK(hipMemcpyAsync(S->d_slots_active, ha.data(), ha.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    S->dirty = 0;
+    return PLOM_OK;
+}
+
+extern "C" int plom_server_create(plom_ctx *c, const plom_server_config *cfg, plom_server **out)
+{
+    if (!c || !out) return set_err(PLOM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    plom_server *S = new (std::nothrow) plom_server();
+    if (!S) return set_err(PLOM_ERR_NOMEM, "host allocation failed");
+    S->c = c;
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    S->budget = (cfg && cfg->hbm_budget_bytes > 0) ? cfg->hbm_budget_bytes : (int64_t)(0.8 * (double)free_b);
+    S->max_active = (cfg && cfg->max_active > 0) ? cfg->max_active : 4096;
+    S->capacity = S->max_active;
+    S->in_use = 0;
+    S->next_id = 1;
+    S->d_tiles = nullptr;
+    S->tiles_cap = 0;
+    S->dirty = 1;
+    S->hbuf = 0;
+    S->steps = 0;
+    hipError_t e = hipMalloc((void **)&S->d_slots, (size_t)S->capacity * sizeof(plom_dfilter));
+    if (e != hipSuccess) { delete S; return set_err(PLOM_ERR_NOMEM, "hipMalloc slots: %s", hipGetErrorString(e)); }
+    e = hipMalloc((void **)&S->d_slots_active, (size_t)S->capacity * sizeof(int));
+    if (e != hipSuccess) { (void)hipFree(S->d_slots); delete S; return set_err(PLOM_ERR_NOMEM, "hipMalloc slots: %s", hipGetErrorString(e)); }
+    for (int i = S->capacity - 1; i >= 0; i--) S->free_slots.push_back(i);
+    *out = S;
+    return PLOM_OK;
+}
+
+extern "C" int plom_server_submit(plom_server *S, const plom_filter_desc *d, const plom_filter_inputs *in, uint64_t *req_id)
+{
+    if (!S || !d || !in || !req_id) return set_err(PLOM_ERR_ARG, "null argument");
+    plom_model_info mi;
+    int rc = check_desc(d, &mi);
+    if (rc != PLOM_OK) return rc;
+    if (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_FINAL_STATE))
+        return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods (no ancestry / final state)");
+    if (!in->times || !in->pv || !in->x0 || (d->N > 0 && !in->data)) return set_err(PLOM_ERR_ARG, "null input array");
+    layout L;
+    make_layout(d, &mi, &L);
+    if ((int64_t)L.total > S->budget)
+        return set_err(PLOM_ERR_NOMEM, "request needs %zu device bytes, more than the server budget %lld", L.total, (long long)S->budget);
+    srv_request *r = new (std::nothrow) srv_request();
+    if (!r) return set_err(PLOM_ERR_NOMEM, "host allocation failed");
+    r->id = S->next_id++;
+    r->desc = *d;
+    r->times.assign(in->times, in->times + d->N + 1);
+    r->data.assign(in->data, in->data + (size_t)d->N * mi.n_ts);
+    r->pv.assign(in->pv, in->pv + mi.pv_size);
+    r->x0.assign(in->x0, in->x0 + mi.size_proj);
+    r->bytes = (int64_t)L.total;
+    S->queue.push_back(r);
+    *req_id = r->id;
+    return PLOM_OK;
+}
+
+/* admit, advance every active request one interval, retire; returns counts */
+extern "C" int plom_server_step(plom_server *S, int32_t *n_active, int32_t *n_queued, int32_t *n_done)
+{
+    if (!S) return set_err(PLOM_ERR_ARG, "null server");
+    plom_ctx *c = S->c;
+    HIPCHK(hipSetDevice(c->device));
+    /* completions of earlier steps */
+    for (size_t i = 0; i < S->finished.size();) {
+        srv_request *r = S->finished[i];
+        hipError_t q = hipEventQuery(r->done_ev);
+        if (q == hipSuccess) {
+            r->state = SRV_DONE;
+            S->done.push_back(r);
+            S->finished.erase(S->finished.begin() + (long)i);
+        } else if (q == hipErrorNotReady) {
+            i++;
+        } else {
+            return set_err(PLOM_ERR_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+        }
+    }
+    /* admission by free HBM, FIFO */
+    while (!S->queue.empty() && (int)S->active.size() < S->max_active) {
+        srv_request *r = S->queue.front();
+        if (S->in_use + r->bytes > S->budget) break;
+        plom_filter_inputs in = {r->times.data(), r->data.data(), r->pv.data(), r->x0.data()};
+        int rc = filter_create_impl(c, &r->desc, &in, 1, &r->f);
+        if (rc != PLOM_OK) return rc;
+        r->slot = S->free_slots.back();
+        S->free_slots.pop_back();
+        HIPCHK(hipMemcpyAsync(S->d_slots + r->slot, &r->f->h, sizeof(plom_dfilter), hipMemcpyHostToDevice, c->stream));
+        S->in_use += r->bytes;
+        r->state = SRV_RUNNING;
+        S->active.push_back(r);
+        S->queue.erase(S->queue.begin());
+        S->dirty = 1;
+    }
+    if (S->dirty) {
+        int rc = srv_rebuild_maps(S);
+        if (rc != PLOM_OK) return rc;
+    }
+    /* one interval for every active request */
+    if (!S->active.empty()) {
+        for (auto &g : S->groups)
+            g_models[g.model].launch_prop(g.prec, g.variant, (int)g.n, c->stream, S->d_slots, S->d_tiles + g.off);
+        enqueue_resample(c->stream, S->d_slots, (const int2 *)S->d_tiles, (int)S->n_all, S->d_slots_active, (int)S->active.size());
+        HIPCHK(hipGetLastError());
+        S->steps++;
+        for (size_t i = 0; i < S->active.size();) {
+            srv_request *r = S->active[i];
+            r->progress++;
+            if (r->progress >= r->desc.N) {
+                /* the slot descriptor carries the interval counter: copy it back, mark done */
+                HIPCHK(hipMemcpyAsync(r->f->df, S->d_slots + r->slot, sizeof(plom_dfilter), hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming));
+                HIPCHK(hipEventRecord(r->done_ev, c->stream));
+                r->f->has_run = 1;
+                S->free_slots.push_back(r->slot);
+                S->in_use -= r->bytes;
+                S->finished.push_back(r);
+                S->active.erase(S->active.begin() + (long)i);
+                S->dirty = 1;
+            } else {
+                i++;
+            }
+        }
+    }
+    if (n_active) *n_active = (int32_t)S->active.size();
+    if (n_queued) *n_queued = (int32_t)S->queue.size();
+    if (n_done) *n_done = (int32_t)S->done.size();
+    return PLOM_OK;
+}
+
+static srv_request *srv_find(plom_server *S, uint64_t id, std::vector<srv_request *> **where)
+{
+    std::vector<srv_request *> *lists[] = {&S->queue, &S->active, &S->finished, &S->done};
+    for (auto L : lists)
+        for (auto r : *L)
+            if (r->id == id) { if (where) *where = L; return r; }
+    return nullptr;
+}
+
+extern "C" int plom_server_status(plom_server *S, uint64_t id)
+{
+    if (!S) return set_err(PLOM_ERR_ARG, "null server");
+    srv_request *r = srv_find(S, id, nullptr);
+    if (!r) return set_err(PLOM_ERR_ARG, "unknown request %llu", (unsigned long long)id);
+    if (r->state == SRV_RUNNING && r->done_ev && hipEventQuery(r->done_ev) == hipSuccess) return SRV_DONE;
+    return r->state;
+}
+
+extern "C" int plom_server_result(plom_server *S, uint64_t id, plom_filter_outputs *o)
+{
+    if (!S || !o) return set_err(PLOM_ERR_ARG, "null argument");
+    std::vector<srv_request *> *where = nullptr;
+    srv_request *r = srv_find(S, id, &where);
+    if (!r) return set_err(PLOM_ERR_ARG, "unknown request %llu", (unsigned long long)id);
+    if (!r->done_ev) return set_err(PLOM_ERR_STATE, "request %llu has not finished", (unsigned long long)id);
+    HIPCHK(hipEventSynchronize(r->done_ev));
+    int rc = plom_filter_read(S->c, r->f, o);
+    if (rc != PLOM_OK) return rc;
+    HIPCHK(hipEventDestroy(r->done_ev));
+    HIPCHK(hipFreeAsync(r->f->dev, S->c->stream));
+    delete r->f;
+    where->erase(std::find(where->begin(), where->end(), r));
+    delete r;
+    return PLOM_OK;
+}
+
+extern "C" int plom_server_destroy(plom_server *S)
+{
+    if (!S) return PLOM_OK;
+    plom_ctx *c = S->c;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<srv_request *> *lists[] = {&S->queue, &S->active, &S->finished, &S->done};
+    for (auto L : lists)
+        for (auto r : *L) {
+            if (r->done_ev) (void)hipEventDestroy(r->done_ev);
+            if (r->f) { (void)hipFreeAsync(r->f->dev, c->stream); delete r->f; }
+            delete r;
+        }
+    if (S->d_tiles) HIPCHK(hipFreeAsync(S->d_tiles, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(S->d_slots));
+    HIPCHK(hipFree(S->d_slots_active));
+    delete S;
+    return PLOM_OK;
+}
+
 extern "C" int plom_ctx_set_kernel_timing(plom_ctx *c, int on)
 {
     if (!c) return set_err(PLOM_ERR_ARG, "null context");
@@ -1034,6 +1335,30 @@ extern "C" int plom_selftest_math(plom_ctx *c, int n, const double *in, double *
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(out, d + n, (size_t)n * 6 * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipFree(d));
+    return PLOM_OK;
+}
+
+extern "C" int plom_selftest_copy(plom_ctx *c, int64_t nbytes, int reps, double *ms_per_copy)
+{
+    if (!c || nbytes < 16 || reps < 1) return set_err(PLOM_ERR_ARG, "bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n = (size_t)nbytes / 16;
+    double2 *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc((void **)&a, n * 16));
+    HIPCHK(hipMalloc((void **)&b, n * 16));
+    HIPCHK(hipMemsetAsync(a, 0, n * 16, c->stream));
+    const int blocks = 256 * 8 * 4;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, c->stream, (const double2 *)a, b, n);
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, c->stream, (const double2 *)a, b, n);
+    HIPCHK(hipEventRecord(c->ev[3], c->stream));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+    if (ms_per_copy) *ms_per_copy = ms / reps;
+    HIPCHK(hipFree(a));
+    HIPCHK(hipFree(b));
     return PLOM_OK;
 }
 

@@ -22,7 +22,8 @@ PLOM_HD void plom_sir_interval_full_f64(double *X, const double *pv, plom_nstrea
     const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
         double s2 = 0.0;
         const double r0_0 = ((P0 * X[1]) / P4);
@@ -51,7 +52,8 @@ PLOM_HD void plom_sir_interval_no_dem_sto_f64(double *X, const double *pv, plom_
     const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
         double s2 = 0.0;
         const double r0_0 = ((P0 * X[1]) / P4);
@@ -76,7 +78,8 @@ PLOM_HD void plom_sir_interval_no_env_sto_f64(double *X, const double *pv, plom_
     const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
         double s2 = 0.0;
         const double r0_0 = ((P0 * X[1]) / P4);
@@ -105,7 +108,8 @@ PLOM_HD void plom_sir_interval_no_dem_sto_no_env_sto_f64(double *X, const double
     const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
         double s2 = 0.0;
         const double r0_0 = ((P0 * X[1]) / P4);
@@ -164,7 +168,8 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_f
     const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
         float s2 = 0.0f;
         const float r0_0 = ((P0 * X[1]) / P4);
@@ -193,7 +198,8 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
         float s2 = 0.0f;
         const float r0_0 = ((P0 * X[1]) / P4);
@@ -218,7 +224,8 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
         float s2 = 0.0f;
         const float r0_0 = ((P0 * X[1]) / P4);
@@ -247,7 +254,8 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
         float s2 = 0.0f;
         const float r0_0 = ((P0 * X[1]) / P4);

@@ -62,13 +62,15 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
     X[10] = 0.0; /* reset_inc */
     X[11] = 0.0; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
         double za0, zb0;
@@ -83,7 +85,7 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[4]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
         double za2, zb2;
@@ -98,7 +100,7 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[5]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
         double za4, zb4;
@@ -113,7 +115,7 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[6]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
         double za6, zb6;
@@ -193,13 +195,15 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
     X[10] = 0.0; /* reset_inc */
     X[11] = 0.0; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
         double za0, zb0;
@@ -209,7 +213,7 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
         const double w0_1 = sqdt * zb0;
@@ -217,7 +221,7 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
         double za1, zb1;
@@ -227,7 +231,7 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
         const double w0_3 = sqdt * zb1;
@@ -296,13 +300,15 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
     X[10] = 0.0; /* reset_inc */
     X[11] = 0.0; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
         double za0, zb0;
@@ -316,7 +322,7 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[4]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
         const double d0_1 = sqdt * zb1;
@@ -328,7 +334,7 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[5]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt((r0_1*X[1]))*d0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
         double za3, zb3;
@@ -342,7 +348,7 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[6]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt((r0_2*X[2]))*d0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
         const double d0_3 = sqdt * zb4;
@@ -415,34 +421,36 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_no_env_sto_f64(double *X, const doub
     X[10] = 0.0; /* reset_inc */
     X[11] = 0.0; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
         const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr;
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr;
         s8 += ( + (r0_0*X[0]))*dtr;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
         const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr;
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr;
         s9 += ( + (r0_1*X[1]))*dtr;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
         const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr;
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr;
         s10 += ( + (r0_2*X[2]))*dtr;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
         const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr;
@@ -578,13 +586,15 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     X[10] = 0.0f; /* reset_inc */
     X[11] = 0.0f; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const float tf0 = plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = (((P0 * (1.0f + (P4 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const float r0_0 = (((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
         const float r1_0 = P8;
         const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
         float za0, zb0;
@@ -599,7 +609,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - __builtin_sqrtf((r1_0*X[4]))*d1_0+ __builtin_sqrtf((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + __builtin_sqrtf((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0f) ? 0.0f : f0;
-        const float r0_1 = (((P1 * (1.0f + (P5 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const float r0_1 = (((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
         const float r1_1 = P9;
         const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
         float za2, zb2;
@@ -614,7 +624,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - __builtin_sqrtf((r1_1*X[5]))*d1_1+ __builtin_sqrtf((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + __builtin_sqrtf((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0f) ? 0.0f : f1;
-        const float r0_2 = (((P2 * (1.0f + (P6 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const float r0_2 = (((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
         const float r1_2 = P10;
         const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
         float za4, zb4;
@@ -629,7 +639,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - __builtin_sqrtf((r1_2*X[6]))*d1_2+ __builtin_sqrtf((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + __builtin_sqrtf((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0f) ? 0.0f : f2;
-        const float r0_3 = (((P3 * (1.0f + (P7 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const float r0_3 = (((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
         const float r1_3 = P11;
         const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
         float za6, zb6;
@@ -709,13 +719,15 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     X[10] = 0.0f; /* reset_inc */
     X[11] = 0.0f; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const float tf0 = plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = (((P0 * (1.0f + (P4 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const float r0_0 = (((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
         const float r1_0 = P8;
         const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
         float za0, zb0;
@@ -725,7 +737,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0f) ? 0.0f : f0;
-        const float r0_1 = (((P1 * (1.0f + (P5 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const float r0_1 = (((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
         const float r1_1 = P9;
         const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
         const float w0_1 = sqdt * zb0;
@@ -733,7 +745,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0f) ? 0.0f : f1;
-        const float r0_2 = (((P2 * (1.0f + (P6 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const float r0_2 = (((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
         const float r1_2 = P10;
         const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
         float za1, zb1;
@@ -743,7 +755,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0f) ? 0.0f : f2;
-        const float r0_3 = (((P3 * (1.0f + (P7 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const float r0_3 = (((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
         const float r1_3 = P11;
         const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
         const float w0_3 = sqdt * zb1;
@@ -812,13 +824,15 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     X[10] = 0.0f; /* reset_inc */
     X[11] = 0.0f; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const float tf0 = plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = (((P0 * (1.0f + (P4 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const float r0_0 = (((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
         const float r1_0 = P8;
         const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
         float za0, zb0;
@@ -832,7 +846,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - __builtin_sqrtf((r1_0*X[4]))*d1_0+ __builtin_sqrtf((r0_0*X[0]))*d0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + __builtin_sqrtf((r0_0*X[0]))*d0_0;
         X[0] = (f0 < 0.0f) ? 0.0f : f0;
-        const float r0_1 = (((P1 * (1.0f + (P5 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const float r0_1 = (((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
         const float r1_1 = P9;
         const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
         const float d0_1 = sqdt * zb1;
@@ -844,7 +858,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - __builtin_sqrtf((r1_1*X[5]))*d1_1+ __builtin_sqrtf((r0_1*X[1]))*d0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + __builtin_sqrtf((r0_1*X[1]))*d0_1;
         X[1] = (f1 < 0.0f) ? 0.0f : f1;
-        const float r0_2 = (((P2 * (1.0f + (P6 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const float r0_2 = (((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
         const float r1_2 = P10;
         const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
         float za3, zb3;
@@ -858,7 +872,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - __builtin_sqrtf((r1_2*X[6]))*d1_2+ __builtin_sqrtf((r0_2*X[2]))*d0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + __builtin_sqrtf((r0_2*X[2]))*d0_2;
         X[2] = (f2 < 0.0f) ? 0.0f : f2;
-        const float r0_3 = (((P3 * (1.0f + (P7 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const float r0_3 = (((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
         const float r1_3 = P11;
         const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
         const float d0_3 = sqdt * zb4;
@@ -931,34 +945,36 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     X[10] = 0.0f; /* reset_inc */
     X[11] = 0.0f; /* reset_inc */
     double t = t0;
-    while (t < t1) {
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const float tf0 = plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = (((P0 * (1.0f + (P4 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const float r0_0 = (((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
         const float r1_0 = P8;
         const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
         const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr;
         const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr;
         s8 += ( + (r0_0*X[0]))*dtr;
         X[0] = (f0 < 0.0f) ? 0.0f : f0;
-        const float r0_1 = (((P1 * (1.0f + (P5 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const float r0_1 = (((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
         const float r1_1 = P9;
         const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
         const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr;
         const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr;
         s9 += ( + (r0_1*X[1]))*dtr;
         X[1] = (f1 < 0.0f) ? 0.0f : f1;
-        const float r0_2 = (((P2 * (1.0f + (P6 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const float r0_2 = (((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
         const float r1_2 = P10;
         const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
         const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr;
         const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr;
         s10 += ( + (r0_2*X[2]))*dtr;
         X[2] = (f2 < 0.0f) ? 0.0f : f2;
-        const float r0_3 = (((P3 * (1.0f + (P7 * plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f))))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const float r0_3 = (((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
         const float r1_3 = P11;
         const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
         const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr;

@@ -53,7 +53,8 @@ EXPORTS = [
     "plom_model_name", "plom_model_info_get", "plom_smc_run", "plom_filter_create", "plom_filter_reset",
     "plom_filter_run", "plom_filter_read", "plom_filter_destroy", "plom_filter_device_bytes",
     "plom_batch_run", "plom_sync", "plom_ctx_set_kernel_timing", "plom_last_timing", "plom_selftest_math",
-    "plom_selftest_normals",
+    "plom_selftest_normals", "plom_selftest_copy", "plom_batch_create", "plom_batch_launch",
+    "plom_batch_destroy",
 ]
 
 _lib = None
@@ -93,6 +94,10 @@ def lib():
     L.plom_last_timing.argtypes = [C.c_void_p, dp, dp, dp, C.POINTER(C.c_int32)]
     L.plom_selftest_math.argtypes = [C.c_void_p, C.c_int, dp, dp]
     L.plom_selftest_normals.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
+    L.plom_selftest_copy.argtypes = [C.c_void_p, C.c_int64, C.c_int, dp]
+    L.plom_batch_create.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
+    L.plom_batch_launch.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+    L.plom_batch_destroy.argtypes = [C.c_void_p, C.c_void_p]
     _lib = L
     return L
 
@@ -302,6 +307,33 @@ class Filter:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """A batch plan (plom_batch_create): the filters advance together, one batched launch
+    round per interval; with use_graph the launch sequence is captured once and replayed."""
+
+    def __init__(self, ctx, filters):
+        self.ctx = ctx
+        self.filters = list(filters)
+        arr = (C.c_void_p * len(self.filters))(*[f.h for f in self.filters])
+        self.h = C.c_void_p()
+        check(lib().plom_batch_create(ctx.h, len(self.filters), arr, C.byref(self.h)))
+
+    def launch(self, use_graph=False):
+        check(lib().plom_batch_launch(self.ctx.h, self.h, 1 if use_graph else 0))
+
+    def close(self):
+        if self.h:
+            check(lib().plom_batch_destroy(self.ctx.h, self.h))
+            self.h = C.c_void_p()
+
+
+def copy_bandwidth_gbps(ctx, nbytes=1 << 30, reps=5):
+    """Measured device copy bandwidth (read + write bytes / time)."""
+    ms = C.c_double()
+    check(lib().plom_selftest_copy(ctx.h, nbytes, reps, C.byref(ms)))
+    return 2 * nbytes / (ms.value * 1e-3) / 1e9, ms.value
 
 
 def batch_run(ctx, filters, use_graph=False):

@@ -77,3 +77,27 @@ def test_fp32_mean_loglik_within_mc_error(ctx):
     se = np.sqrt(a.var(ddof=1) / R + b.var(ddof=1) / R)
     print("fp64 mean %.4f sd %.4f | fp32 mean %.4f sd %.4f | se %.4f" % (a.mean(), a.std(), b.mean(), b.std(), se))
     assert abs(a.mean() - b.mean()) < 3 * se
+
+
+def test_batch_graph_replays(ctx):
+    """A captured batch replays with the same results as the eager launch, replay after
+    replay, with host work and eager launches in between."""
+    fs = [make(ctx, *c)[0] for c in CASES]
+    smc.batch_run(ctx, fs)
+    eager = [f.read(ancestry=True) for f in fs]
+    b = smc.Batch(ctx, fs)
+    for rep in range(3):
+        print("graph replay", rep, flush=True)
+        b.launch(use_graph=True)
+        got = [f.read(ancestry=True) for f in fs]
+        for e, g in zip(eager, got):
+            assert e.loglik == g.loglik
+            np.testing.assert_array_equal(e.ancestry, g.ancestry)
+            np.testing.assert_array_equal(e.ess_n, g.ess_n)
+        fs[0].run()  # an eager single-filter run between replays
+        assert fs[0].read().loglik == eager[0].loglik
+    b.launch(use_graph=False)
+    assert [f.read().loglik for f in fs] == [e.loglik for e in eager]
+    b.close()
+    for f in fs:
+        f.close()

@@ -144,8 +144,9 @@ int plom_batch_run(plom_ctx *ctx, int n_filters, plom_filter **filters, int use_
  * (FIFO), advances every admitted request by one interval in one batched launch round,
  * and retires finished ones; plom_server_status gives PLOM_SRV_QUEUED / RUNNING / DONE;
  * plom_server_result returns log-likelihood, n_all_fail, loglik_n and ess_n of a done
- * request and releases it.  Results equal plom_smc_run on the same descriptor and inputs
- * (FP64: bit for bit), whatever else is being served. */
+ * request and releases its device memory (a finished request counts against the budget
+ * until then).  Results equal plom_smc_run on the same descriptor and inputs (FP64: bit
+ * for bit), whatever else is being served. */
 #define PLOM_SRV_QUEUED 0
 #define PLOM_SRV_RUNNING 1
 #define PLOM_SRV_DONE 2

@@ -1218,7 +1218,6 @@ extern "C" int plom_server_step(plom_server *S, int32_t *n_active, int32_t *n_qu
                 HIPCHK(hipEventRecord(r->done_ev, c->stream));
                 r->f->has_run = 1;
                 S->free_slots.push_back(r->slot);
-                S->in_use -= r->bytes;
                 S->finished.push_back(r);
                 S->active.erase(S->active.begin() + (long)i);
                 S->dirty = 1;
@@ -1263,6 +1262,7 @@ extern "C" int plom_server_result(plom_server *S, uint64_t id, plom_filter_outpu
     if (rc != PLOM_OK) return rc;
     HIPCHK(hipEventDestroy(r->done_ev));
     HIPCHK(hipFreeAsync(r->f->dev, S->c->stream));
+    S->in_use -= r->bytes; /* the request's device memory counts until its result is taken */
     delete r->f;
     where->erase(std::find(where->begin(), where->end(), r));
     delete r;

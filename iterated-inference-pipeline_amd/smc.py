@@ -54,8 +54,15 @@ EXPORTS = [
     "plom_filter_run", "plom_filter_read", "plom_filter_destroy", "plom_filter_device_bytes",
     "plom_batch_run", "plom_sync", "plom_ctx_set_kernel_timing", "plom_last_timing", "plom_selftest_math",
     "plom_selftest_normals", "plom_selftest_copy", "plom_batch_create", "plom_batch_launch",
-    "plom_batch_destroy",
+    "plom_batch_destroy", "plom_server_create", "plom_server_submit", "plom_server_step",
+    "plom_server_status", "plom_server_result", "plom_server_destroy",
 ]
+
+SRV_QUEUED, SRV_RUNNING, SRV_DONE = 0, 1, 2
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active", C.c_int32), ("reserved", C.c_int32)]
 
 _lib = None
 
@@ -98,6 +105,12 @@ def lib():
     L.plom_batch_create.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
     L.plom_batch_launch.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
     L.plom_batch_destroy.argtypes = [C.c_void_p, C.c_void_p]
+    L.plom_server_create.argtypes = [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]
+    L.plom_server_submit.argtypes = [C.c_void_p, C.POINTER(FilterDesc), C.POINTER(FilterInputs), C.POINTER(C.c_uint64)]
+    L.plom_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.plom_server_status.argtypes = [C.c_void_p, C.c_uint64]
+    L.plom_server_result.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(FilterOutputs)]
+    L.plom_server_destroy.argtypes = [C.c_void_p]
     _lib = L
     return L
 
@@ -326,6 +339,55 @@ class Batch:
     def close(self):
         if self.h:
             check(lib().plom_batch_destroy(self.ctx.h, self.h))
+            self.h = C.c_void_p()
+
+
+class Server:
+    """The served stream (plom_server_*): continuous batching with admission by free HBM."""
+
+    def __init__(self, ctx, hbm_budget_bytes=0, max_active=0):
+        self.ctx = ctx
+        cfg = ServerConfig(int(hbm_budget_bytes), int(max_active), 0)
+        self.h = C.c_void_p()
+        check(lib().plom_server_create(ctx.h, C.byref(cfg), C.byref(self.h)))
+        self.N = {}
+
+    def submit(self, model, J, times, data, pv, x0, precision="fp64", noises_off=0, seed=1, stream=0,
+               dt=0.25, like_min=1e-17):
+        times = np.ascontiguousarray(times, dtype=np.float64)
+        N = len(times) - 1
+        data = np.ascontiguousarray(data, dtype=np.float64).reshape(N, model.n_ts)
+        pv = np.ascontiguousarray(pv, dtype=np.float64)
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        d = _desc(model, J, N, precision, noises_off, 0, seed, stream, dt, like_min)
+        inp = FilterInputs(dptr(times), dptr(data), dptr(pv), dptr(x0))
+        rid = C.c_uint64()
+        check(lib().plom_server_submit(self.h, C.byref(d), C.byref(inp), C.byref(rid)))
+        self.N[rid.value] = N
+        return rid.value
+
+    def step(self):
+        a, q, d = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().plom_server_step(self.h, C.byref(a), C.byref(q), C.byref(d)))
+        return a.value, q.value, d.value
+
+    def status(self, rid):
+        r = lib().plom_server_status(self.h, rid)
+        if r < 0:
+            check(r)
+        return r
+
+    def result(self, rid):
+        N = self.N.pop(rid)
+        ll_n = np.zeros(N)
+        ess = np.zeros(N)
+        out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), None, None, None)
+        check(lib().plom_server_result(self.h, rid, C.byref(out)))
+        return FilterResult(out.loglik, out.n_all_fail, ll_n, ess)
+
+    def close(self):
+        if self.h:
+            check(lib().plom_server_destroy(self.h))
             self.h = C.c_void_p()
 
 

@@ -1,0 +1,94 @@
+"""The served stream: continuous batching with admission by free HBM.
+
+Every request's result equals the same request run alone (bit for bit in FP64),
+whatever arrives before, beside or after it; admission respects the memory budget."""
+import numpy as np
+import pytest
+
+from iip_amd import configs, smc
+from iip_amd.smc import Model
+
+pytestmark = pytest.mark.gpu
+
+REQS = [("sir", 700, 52, 11), ("sirs4", 2000, 30, 12), ("sir", 5000, 12, 13), ("sirs4", 300, 45, 14),
+        ("seir10", 800, 20, 15), ("sir", 257, 52, 16), ("sirs4", 9000, 8, 17), ("sir", 1500, 25, 18)]
+
+
+def inputs(name, N, scale=1.0):
+    m = Model.get(name)
+    times, data = m.dataset(N)
+    p = dict(configs.TRUTH[name])
+    p["beta"] = p["beta"] * scale
+    return m, times, data, m.pack(p), m.x0(configs.X0_PROPS[name])
+
+
+def alone(ctx, name, J, N, stream, precision="fp64"):
+    m, times, data, pv, x0 = inputs(name, N)
+    return smc.run(ctx, m, J, times, data, pv, x0, precision=precision, seed=5, stream=stream)
+
+
+def drain(srv, ids, max_steps=100000):
+    for _ in range(max_steps):
+        srv.step()
+        if all(srv.status(i) == smc.SRV_DONE for i in ids):
+            return
+    raise AssertionError("server did not finish")
+
+
+def test_served_equals_alone(ctx):
+    srv = smc.Server(ctx)
+    ids = []
+    for name, J, N, stream in REQS[:5]:
+        m, times, data, pv, x0 = inputs(name, N)
+        ids.append(srv.submit(m, J, times, data, pv, x0, seed=5, stream=stream))
+    # a few intervals in, more requests arrive and join the running batch
+    for _ in range(3):
+        srv.step()
+    for name, J, N, stream in REQS[5:]:
+        m, times, data, pv, x0 = inputs(name, N)
+        ids.append(srv.submit(m, J, times, data, pv, x0, seed=5, stream=stream))
+    drain(srv, ids)
+    for rid, (name, J, N, stream) in zip(ids, REQS):
+        print("request", name, J, N, flush=True)
+        got = srv.result(rid)
+        ref = alone(ctx, name, J, N, stream)
+        assert got.loglik == ref.loglik
+        np.testing.assert_array_equal(got.loglik_n, ref.loglik_n)
+        np.testing.assert_array_equal(got.ess_n, ref.ess_n)
+    srv.close()
+
+
+def test_admission_by_memory(ctx):
+    m, times, data, pv, x0 = inputs("sirs4", 30)
+    one = smc.Filter(ctx, m, 4000, times, data, pv, x0).device_bytes()
+    srv = smc.Server(ctx, hbm_budget_bytes=int(2.5 * one))
+    ids = [srv.submit(m, 4000, times, data, pv, x0, seed=5, stream=100 + k) for k in range(6)]
+    peak = 0
+    results = {}
+    for _ in range(10000):
+        a, q, d = srv.step()
+        peak = max(peak, a)
+        for i in ids:
+            if i not in results and srv.status(i) == smc.SRV_DONE:
+                results[i] = srv.result(i)   # frees its memory: the next one can be admitted
+        if len(results) == len(ids):
+            break
+    assert peak == 2 and len(results) == 6
+    ref = smc.run(ctx, m, 4000, times, data, pv, x0, seed=5, stream=103)
+    assert results[ids[3]].loglik == ref.loglik
+    with pytest.raises(smc.PlomError):
+        srv.submit(m, 40000, times, data, pv, x0)
+    srv.close()
+
+
+def test_served_fp32_mix(ctx):
+    """FP32 requests served beside FP64 ones give the FP32 result alone."""
+    srv = smc.Server(ctx)
+    m, times, data, pv, x0 = inputs("sirs4", 20)
+    a = srv.submit(m, 3000, times, data, pv, x0, precision="fp32", seed=5, stream=7)
+    b = srv.submit(m, 3000, times, data, pv, x0, precision="fp64", seed=5, stream=7)
+    drain(srv, [a, b])
+    ra, rb = srv.result(a), srv.result(b)
+    assert ra.loglik == alone(ctx, "sirs4", 3000, 20, 7, precision="fp32").loglik
+    assert rb.loglik == alone(ctx, "sirs4", 3000, 20, 7).loglik
+    srv.close()

@@ -753,6 +753,7 @@ extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
     }
     const model_entry *me = &g_models[f->desc.model_id];
     const int T = f->h.T;
+    (void)hipGetLastError(); /* a not-ready query or an unused event earlier must not be reported as a launch failure */
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     int k = 0;
     for (int n = 0; n < f->desc.N; n++) {
@@ -776,8 +777,6 @@ extern "C" int plom_filter_read(plom_ctx *c, plom_filter *f, plom_filter_outputs
     if (!f->has_run) return set_err(PLOM_ERR_STATE, "filter has not run");
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->last_total_ms = ms;
     const plom_dfilter *h = &f->h;
     const size_t N = (size_t)f->desc.N, J = (size_t)f->desc.J;
     double scal[4];
@@ -914,6 +913,7 @@ static int batch_create(plom_ctx *c, int nf, plom_filter **in, plom_batch **out)
 /* the launch sequence of a batch: reset, every interval, final resampling */
 static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing)
 {
+    (void)hipGetLastError(); /* a not-ready query or an unused event earlier must not be reported as a launch failure */
     hipLaunchKernelGGL(k_reset, dim3(b->nf), dim3(256), 0, c->stream, b->ddf, b->nf);
     int k = 0;
     for (int s = 0; s < b->maxN; s++) {
@@ -1171,6 +1171,7 @@ extern "C" int plom_server_step(plom_server *S, int32_t *n_active, int32_t *n_qu
     for (size_t i = 0; i < S->finished.size();) {
         srv_request *r = S->finished[i];
         hipError_t q = hipEventQuery(r->done_ev);
+        if (q == hipErrorNotReady) (void)hipGetLastError();
         if (q == hipSuccess) {
             r->state = SRV_DONE;
             S->done.push_back(r);
@@ -1202,6 +1203,7 @@ extern "C" int plom_server_step(plom_server *S, int32_t *n_active, int32_t *n_qu
         if (rc != PLOM_OK) return rc;
     }
     /* one interval for every active request */
+    (void)hipGetLastError();
     if (!S->active.empty()) {
         for (auto &g : S->groups)
             g_models[g.model].launch_prop(g.prec, g.variant, (int)g.n, c->stream, S->d_slots, S->d_tiles + g.off);
@@ -1246,7 +1248,11 @@ extern "C" int plom_server_status(plom_server *S, uint64_t id)
     if (!S) return set_err(PLOM_ERR_ARG, "null server");
     srv_request *r = srv_find(S, id, nullptr);
     if (!r) return set_err(PLOM_ERR_ARG, "unknown request %llu", (unsigned long long)id);
-    if (r->state == SRV_RUNNING && r->done_ev && hipEventQuery(r->done_ev) == hipSuccess) return SRV_DONE;
+    if (r->state == SRV_RUNNING && r->done_ev) {
+        hipError_t q = hipEventQuery(r->done_ev);
+        if (q == hipSuccess) return SRV_DONE;
+        (void)hipGetLastError();
+    }
     return r->state;
 }
 

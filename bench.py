@@ -1,25 +1,40 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X PLOM SMC filter on one GPU.
+"""Benchmark of the MI355X PLOM SMC filter on one GPU: batched and served vs plain.
 
-Workload (one "step" = one full filter over the series): SIRS with seasonal forcing
-(amplitude 0.3, period 365 d) and multiplicative white noise on beta (sigma 0.1),
-4 coupled populations of 5e5, N = 520 weekly intervals, J = 100000 particles,
+Headline workload (one "step" = one batched pass over all filters): the pMCMC batch, 256
+likelihood evaluations of the SIRS model (seasonal forcing amplitude 0.3 / period 365 d,
+white noise on beta sigma 0.1, 4 coupled populations of 5e5, weekly incidence), with
+parameter sets drawn around the truth, J ~ U{1000..20000} and N ~ U{104..520} per filter,
 dt = 0.25 d (28 Euler substeps per interval), systematic resampling every interval.
-Inputs are resident in HBM before the timed region; the filter runs in FP64 (the
-oracle-exact mode).  `value` is particle-steps/s (J * N per step).
+All 256 filters advance together (one batched launch round per interval), replayed from
+a captured hipGraph.  It runs in FP64, where every log-likelihood is bit-identical to the
+filter run alone (and to the CPU oracle), so the error against FP64 is exactly 0.
+`value` is particle-steps/s (sum of J * N over the filters / time per pass), inputs
+resident in HBM.
 
-Extra legs on the same line: the FP32-propagation run of the same workload, the
-roofline of the dominant kernel (k_propagate) from HIP events around every launch on
-the library's stream, and a CPU baseline: the C oracle (this project's restatement of
-the reference filter) timed on the host cores on a bounded sample.
+Legs on the same JSON line:
+  plain     the same 256 filters run one at a time (one filter per launch sequence,
+            result read before the next one starts), FP64: the baseline of the speedup
+  batched   the headline numbers, the FP32-propagation pass of the same batch and its
+            deviation from FP64
+  served    a Poisson stream (200 requests/s) of SIR / SIRS requests, J log-uniform in
+            [500, 5e4], N ~ U{52..520}, served with continuous batching and admission by
+            free HBM: sustained particle-steps/s and p50 / p99 latency (submit -> result)
+  single    one large filter (SIRS, J = 1e5, N = 520), FP64 and FP32
+  roofline  of the dominant kernel (k_propagate of the batch), from HIP events around
+            every propagate launch on the library's stream
+  cpu_baseline  the C oracle (this project's restatement of the reference filter) on a
+            bounded sample of the headline workload, timed on the host cores
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs a,b,...] [--no-cpu]
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "tests"))
@@ -28,7 +43,12 @@ sys.path.insert(0, HERE)
 from pkgload import load_package  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+METRIC = "particle-steps/s & p50/p99 loglik latency, batched+served vs plain SMC"
+LEGS = ("batched", "plain", "served", "single", "cpu")
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
 def bytes_per_particle_step(S, real_bytes):
@@ -49,13 +69,127 @@ def read_traffic(name):
     return None
 
 
-def run_gpu(smc, configs, precision, steps, warmup, J, N):
-    model = smc.Model.get("sirs4")
-    times, data = model.dataset(N)
-    pv = model.pack(configs.TRUTH["sirs4"])
-    x0 = model.x0(configs.X0_PROPS["sirs4"])
-    ctx = smc.Context(0)
-    f = smc.Filter(ctx, model, J, times, data, pv, x0, precision=precision, seed=configs.FILTER_SEED, stream=0)
+class Inputs:
+    """Datasets, parameter vectors and initial states, built once per (model, N)."""
+
+    def __init__(self, smc, configs):
+        self.smc, self.configs = smc, configs
+        self.models, self.data, self.x0 = {}, {}, {}
+
+    def model(self, name):
+        if name not in self.models:
+            self.models[name] = self.smc.Model.get(name)
+            self.x0[name] = self.models[name].x0(self.configs.X0_PROPS[name])
+        return self.models[name]
+
+    def get(self, name, N, params):
+        m = self.model(name)
+        if (name, N) not in self.data:
+            self.data[(name, N)] = m.dataset(N)
+        times, data = self.data[(name, N)]
+        return m, times, data, m.pack(params), self.x0[name]
+
+
+def make_filters(smc, ctx, inp, draws, precision):
+    fs = []
+    for d in draws:
+        m, times, data, pv, x0 = inp.get(d["model"], d["N"], d["params"])
+        fs.append(smc.Filter(ctx, m, d["J"], times, data, pv, x0, precision=precision,
+                             seed=inp.configs.FILTER_SEED, stream=d["stream"]))
+    return fs
+
+
+def batched_leg(smc, ctx, filters, steps, warmup):
+    """All filters advance together; the launch sequence is captured once as a hipGraph
+    (first launch) and replayed."""
+    b = smc.Batch(ctx, filters)
+    b.launch(use_graph=True)              # capture + first replay
+    for _ in range(warmup):
+        b.launch(use_graph=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.launch(use_graph=True)
+    ctx.sync()
+    wall = (time.perf_counter() - t0) / steps
+    ll = np.array([f.read().loglik for f in filters])
+    # per-launch timing pass (events around every launch), outside the timed region
+    ctx.set_kernel_timing(True)
+    b.launch(use_graph=False)
+    total_ms, prop_ms, scan_ms, nint = ctx.last_timing()
+    ctx.set_kernel_timing(False)
+    b.close()
+    return {"wall_s": wall, "loglik": ll, "prop_ms": prop_ms, "scan_ms": scan_ms, "total_ms": total_ms,
+            "launches": nint}
+
+
+def plain_leg(filters):
+    """One filter at a time: its launch sequence, then its result, then the next filter."""
+    t0 = time.perf_counter()
+    ll = []
+    for f in filters:
+        f.run()
+        ll.append(f.read().loglik)
+    return {"wall_s": time.perf_counter() - t0, "loglik": np.array(ll)}
+
+
+def served_leg(smc, ctx, inp, configs, seconds, rate, check_n=16):
+    reqs = configs.served_mix(rate=rate, duration_s=seconds)
+    prepared = [inp.get(r["model"], r["N"], r["params"]) for r in reqs]
+    srv = smc.Server(ctx)
+    # warm the server's pools and every kernel it launches (not measured)
+    warm = []
+    for k in range(min(4, len(reqs))):
+        m, times, data, pv, x0 = prepared[k]
+        warm.append(srv.submit(m, reqs[k]["J"], times, data, pv, x0, seed=configs.FILTER_SEED, stream=reqs[k]["stream"]))
+    while len(srv.done()) < len(warm):
+        srv.step()
+    for rid in warm:
+        srv.result(rid)
+    sub_t, lat, res = {}, {}, {}
+    i, peak_active = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        now = time.perf_counter() - t0
+        while i < len(reqs) and reqs[i]["t"] <= now:
+            m, times, data, pv, x0 = prepared[i]
+            rid = srv.submit(m, reqs[i]["J"], times, data, pv, x0, seed=configs.FILTER_SEED, stream=reqs[i]["stream"])
+            sub_t[rid] = (time.perf_counter() - t0, i)
+            i += 1
+        a, q, d = srv.step()
+        peak_active = max(peak_active, a)
+        if d:
+            for rid in srv.done():
+                t_done = time.perf_counter() - t0
+                ts, k = sub_t.pop(rid)
+                lat[k] = t_done - ts
+                res[k] = srv.result(rid).loglik
+        if i == len(reqs) and not sub_t:
+            break
+        if a == 0 and q == 0 and d == 0 and not sub_t and i < len(reqs):
+            time.sleep(max(0.0, min(0.001, reqs[i]["t"] - (time.perf_counter() - t0))))
+    makespan = time.perf_counter() - t0
+    ps = sum(r["J"] * r["N"] for r in reqs)
+    lat_ms = np.array([lat[k] for k in range(len(reqs))]) * 1e3
+    # a sample of requests run alone: identical results (FP64)
+    rng = np.random.default_rng(7)
+    sample = rng.choice(len(reqs), size=min(check_n, len(reqs)), replace=False)
+    max_rel = 0.0
+    for k in sample:
+        m, times, data, pv, x0 = prepared[k]
+        alone = smc.run(ctx, m, reqs[k]["J"], times, data, pv, x0, seed=configs.FILTER_SEED, stream=reqs[k]["stream"]).loglik
+        max_rel = max(max_rel, abs(res[k] - alone) / abs(alone))
+    srv.close()
+    return {"requests": len(reqs), "offered_particle_steps_per_s": ps / seconds, "makespan_s": makespan,
+            "particle_steps": ps, "particle_steps_per_s": ps / makespan,
+            "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+            "mean_ms": float(lat_ms.mean()), "max_ms": float(lat_ms.max()), "peak_active": peak_active,
+            "checked_alone": len(sample), "max_rel_err_vs_alone": max_rel}
+
+
+def single_leg(smc, ctx, inp, configs, precision, steps, warmup, J, N):
+    m, times, data, pv, x0 = inp.get("sirs4", N, configs.TRUTH["sirs4"])
+    f = smc.Filter(ctx, m, J, times, data, pv, x0, precision=precision, seed=configs.FILTER_SEED, stream=0)
     for _ in range(warmup):
         f.run()
     ctx.sync()
@@ -63,53 +197,51 @@ def run_gpu(smc, configs, precision, steps, warmup, J, N):
     for _ in range(steps):
         f.run()
     ctx.sync()
-    t1 = time.perf_counter()
+    wall = (time.perf_counter() - t0) / steps
     res = f.read()
-    # per-launch timing pass (events around every launch), not part of the timed region
     ctx.set_kernel_timing(True)
     f.run()
-    total_ms, prop_ms, scan_ms, nint = ctx.last_timing()
+    _, prop_ms, _, nint = ctx.last_timing()
     ctx.set_kernel_timing(False)
     f.close()
-    ctx.close()
-    wall = (t1 - t0) / steps
-    return {
-        "ms_per_step": wall * 1e3,
-        "particle_steps_per_s": J * N / wall,
-        "loglik": res.loglik,
-        "n_all_fail": res.n_all_fail,
-        "propagate_ms_per_launch": prop_ms / max(nint, 1),
-        "scan_ms_per_launch": scan_ms / max(nint, 1),
-        "launches": nint,
-        "S": model.S,
-    }
+    return {"value": round(J * N / wall, 1), "ms_per_filter": round(wall * 1e3, 3), "loglik": res.loglik,
+            "propagate_ms_per_launch": round(prop_ms / max(nint, 1), 5)}
 
 
-def cpu_baseline(configs, smc, J, N, threads):
+def cpu_baseline(smc, inp, configs, draws, threads):
     from oracle import oracle
-    model = smc.Model.get("sirs4")
-    times, data = model.dataset(N)
-    r = oracle.smc(model.id, J, times, data, model.pack(configs.TRUTH["sirs4"]),
-                   model.x0(configs.X0_PROPS["sirs4"]), seed=configs.FILTER_SEED, nthreads=threads)
-    return J * N / r["elapsed_s"], r["elapsed_s"]
+    secs, ps = 0.0, 0
+    for d in draws:
+        m, times, data, pv, x0 = inp.get(d["model"], d["N"], d["params"])
+        r = oracle.smc(m.id, d["J"], times, data, pv, x0, seed=configs.FILTER_SEED, stream=d["stream"], nthreads=threads)
+        secs += r["elapsed_s"]
+        ps += d["J"] * d["N"]
+    return ps / secs, secs, ps
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--J", type=int, default=100000)
-    ap.add_argument("--N", type=int, default=520)
+    ap.add_argument("--legs", default=",".join(LEGS), help="comma list of " + ",".join(LEGS))
+    ap.add_argument("--filters", type=int, default=256)
+    ap.add_argument("--served-seconds", type=float, default=30.0)
+    ap.add_argument("--served-rate", type=float, default=200.0)
+    ap.add_argument("--single-J", type=int, default=100000)
+    ap.add_argument("--single-N", type=int, default=520)
+    ap.add_argument("--cpu-filters", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-J", type=int, default=20000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (each filter or request "
-              "runs on one device; see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (each filter or request runs "
+              "on one device). Run with --gpus 1.", file=sys.stderr)
         return 2
+    legs = set(x for x in args.legs.split(",") if x)
+    if args.no_cpu:
+        legs.discard("cpu")
     load_package()
     from iip_amd import smc, configs
 
@@ -119,54 +251,95 @@ def main():
         print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30, "GBps": gbps, "ms": ms}))
         return 0
 
-    with smc.Context(0) as ctx:
-        copy_gbps, _ = smc.copy_bandwidth_gbps(ctx, 1 << 30, 5)
+    ctx = smc.Context(0)
+    inp = Inputs(smc, configs)
+    copy_gbps, _ = smc.copy_bandwidth_gbps(ctx, 1 << 30, 5)
+    draws = configs.pmcmc_batch(args.filters)
+    ps_batch = sum(d["J"] * d["N"] for d in draws)
+    S = inp.model("sirs4").S
+    out = {"metric": METRIC, "value": None, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic: series simulated from each model at its truth (seed 20261203); parameter sets, "
+                   "sizes and arrivals drawn from fixed seeds",
+           "config": {"workload": "pmcmc_batch: %d SIRS 4x5e5 filters (forcing 0.3/365 d, env noise 0.1), "
+                                  "J~U{1000..20000}, N~U{104..520}, dt 0.25 d, FP64, one batched pass per "
+                                  "step replayed from a hipGraph" % args.filters,
+                      "filters": args.filters, "particle_steps_per_step": ps_batch, "parallelism": "1 GPU"}}
 
-    J, N = args.J, args.N
-    fp64 = run_gpu(smc, configs, "fp64", args.steps, args.warmup, J, N)
-    fp32 = run_gpu(smc, configs, "fp32", args.steps, args.warmup, J, N)
-
-    B = bytes_per_particle_step(fp64["S"], 8)
-    per_launch = B * J
-    achieved = per_launch / (fp64["propagate_ms_per_launch"] * 1e-3) / 1e9
-    traffic = read_traffic("k_propagate_sirs4_fp64_J100000.md")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": traffic,
-                "kernel": "k_propagate<sirs4,double,full>",
-                "algorithmic_bytes_per_launch": per_launch,
-                "avg_launch_ms": round(fp64["propagate_ms_per_launch"], 5),
-                "measured_copy_GBps": round(copy_gbps, 1)}
-
-    cpu = None
-    if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
-        rate, secs = cpu_baseline(configs, smc, args.cpu_J, N, threads)
-        cpu = {"value": round(rate, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": "oracle/smc_oracle.c (canonical FP64) on the same workload with J=%d instead of %d "
-                         "(particle-steps/s is linear in J), N=%d, %.1f s on %d threads" % (args.cpu_J, J, N, secs, threads)}
-
-    out = {
-        "metric": "particle-steps/s & p50/p99 loglik latency, batched+served vs plain SMC",
-        "value": round(fp64["particle_steps_per_s"], 1),
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(fp64["ms_per_step"], 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (series simulated from the model at the benchmark truth, seed 20261203)",
-        "config": {"workload": "SIRS 4x5e5 coupled, seasonal forcing 0.3/365d, env noise 0.1; weekly incidence; "
-                               "J=%d, N=%d, dt=0.25 d, FP64" % (J, N), "J": J, "N": N, "dt": 0.25,
-                   "particle_steps_per_step": J * N, "parallelism": "1 GPU"},
-        "loglik": fp64["loglik"],
-        "fp32": {"value": round(fp32["particle_steps_per_s"], 1), "ms_per_step": round(fp32["ms_per_step"], 3),
-                 "loglik": fp32["loglik"], "avg_launch_ms": round(fp32["propagate_ms_per_launch"], 5)},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
+    fp64 = None
+    if "batched" in legs or "plain" in legs:
+        log("creating %d FP64 filters" % args.filters)
+        f64 = make_filters(smc, ctx, inp, draws, "fp64")
+    if "batched" in legs:
+        log("batched FP64")
+        fp64 = batched_leg(smc, ctx, f64, args.steps, args.warmup)
+        out["value"] = round(ps_batch / fp64["wall_s"], 1)
+        out["ms_per_step"] = round(fp64["wall_s"] * 1e3, 3)
+        log("batched FP32")
+        f32 = make_filters(smc, ctx, inp, draws, "fp32")
+        fp32 = batched_leg(smc, ctx, f32, args.steps, args.warmup)
+        for f in f32:
+            f.close()
+        rel32 = np.abs(fp32["loglik"] - fp64["loglik"]) / np.abs(fp64["loglik"])
+        out["batched"] = {
+            "fp64": {"value": out["value"], "ms_per_pass": out["ms_per_step"], "graph": True,
+                     "propagate_ms": round(fp64["prop_ms"], 3), "weights_scan_ms": round(fp64["scan_ms"], 3),
+                     "launch_rounds": fp64["launches"]},
+            "fp32": {"value": round(ps_batch / fp32["wall_s"], 1), "ms_per_pass": round(fp32["wall_s"] * 1e3, 3),
+                     "mean_rel_dev_vs_fp64": float(rel32.mean()), "max_rel_dev_vs_fp64": float(rel32.max()),
+                     "note": "FP32 propagation with FP64 weights: a different Monte Carlo estimate, not an "
+                             "approximation of the FP64 one; the tests check its mean over replicates "
+                             "against FP64 within Monte Carlo error"}}
+        B = bytes_per_particle_step(S, 8)
+        per_launch = B * ps_batch / max(fp64["launches"], 1)
+        avg_ms = fp64["prop_ms"] / max(fp64["launches"], 1)
+        achieved = B * ps_batch / (fp64["prop_ms"] * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBPS, 5),
+                           "traffic": read_traffic("k_propagate_sirs4_fp64_pmcmc_batch.md"),
+                           "kernel": "k_propagate<sirs4,double,full> (batched)",
+                           "algorithmic_bytes_per_particle_step": B,
+                           "algorithmic_bytes_per_launch": round(per_launch, 1),
+                           "avg_launch_ms": round(avg_ms, 5), "launches": fp64["launches"],
+                           "measured_copy_GBps": round(copy_gbps, 1)}
+    if "plain" in legs:
+        log("plain FP64 (one filter at a time)")
+        pl = plain_leg(f64)
+        out["plain"] = {"value": round(ps_batch / pl["wall_s"], 1), "s_per_pass": round(pl["wall_s"], 3),
+                        "what": "the same %d FP64 filters, one filter per launch sequence, result read before "
+                                "the next starts" % args.filters}
+        if fp64 is not None:
+            rel = np.abs(fp64["loglik"] - pl["loglik"]) / np.abs(pl["loglik"])
+            out["batched"]["fp64"]["speedup_vs_plain"] = round(out["value"] / out["plain"]["value"], 2)
+            out["batched"]["fp64"]["max_rel_err_vs_plain"] = float(rel.max())
+    if "batched" in legs or "plain" in legs:
+        for f in f64:
+            f.close()
+    if "served" in legs:
+        log("served stream, %.0f s at %.0f requests/s" % (args.served_seconds, args.served_rate))
+        sv = served_leg(smc, ctx, inp, configs, args.served_seconds, args.served_rate)
+        sv = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in sv.items()}
+        sv["workload"] = ("served_mix: Poisson %.0f req/s for %.0f s, 50/50 SIR / SIRS4, J log-uniform [500, 5e4], "
+                          "N~U{52..520}, FP64" % (args.served_rate, args.served_seconds))
+        if "plain" in out:
+            sv["vs_plain"] = round(sv["particle_steps_per_s"] / out["plain"]["value"], 2)
+        out["served"] = sv
+    if "single" in legs:
+        log("single large filter")
+        out["single"] = {"workload": "SIRS 4x5e5, J=%d, N=%d" % (args.single_J, args.single_N),
+                         "fp64": single_leg(smc, ctx, inp, configs, "fp64", args.steps, args.warmup, args.single_J, args.single_N),
+                         "fp32": single_leg(smc, ctx, inp, configs, "fp32", args.steps, args.warmup, args.single_J, args.single_N)}
+    ctx.close()
+    if "cpu" in legs:
+        threads = 16
+        sample = draws[:args.cpu_filters]
+        log("CPU oracle on %d filters, %d threads" % (len(sample), threads))
+        rate, secs, ps = cpu_baseline(smc, inp, configs, sample, threads)
+        out["cpu_baseline"] = {"value": round(rate, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
+                               "sample": "oracle/smc_oracle.c (canonical FP64) on the first %d filters of the "
+                                         "pmcmc_batch workload (%d particle-steps), %.1f s on %d threads"
+                                         % (len(sample), ps, secs, threads)}
     print(json.dumps(out))
     return 0
 

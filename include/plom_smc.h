@@ -151,15 +151,17 @@ int plom_batch_run(plom_ctx *ctx, int n_filters, plom_filter **filters, int use_
 #define PLOM_SRV_RUNNING 1
 #define PLOM_SRV_DONE 2
 typedef struct {
-    int64_t hbm_budget_bytes;  /* 0: 80 % of the free device memory at creation */
-    int32_t max_active;        /* 0: 4096 */
-    int32_t reserved;
+    int64_t hbm_budget_bytes;    /* 0: 80 % of the free device memory at creation */
+    int32_t max_active;          /* 0: 4096 */
+    int32_t max_inflight_steps;  /* 0: 4; plom_server_step blocks while that many steps are queued */
 } plom_server_config;
 typedef struct plom_server plom_server;
 int plom_server_create(plom_ctx *ctx, const plom_server_config *cfg, plom_server **out);
 int plom_server_submit(plom_server *srv, const plom_filter_desc *desc, const plom_filter_inputs *in, uint64_t *req_id);
 int plom_server_step(plom_server *srv, int32_t *n_active, int32_t *n_queued, int32_t *n_done);
 int plom_server_status(plom_server *srv, uint64_t req_id);
+/* ids of the done requests whose results have not been taken, in completion order */
+int plom_server_done(plom_server *srv, uint64_t *ids, int32_t cap, int32_t *n);
 int plom_server_result(plom_server *srv, uint64_t req_id, plom_filter_outputs *out);
 int plom_server_destroy(plom_server *srv);
 

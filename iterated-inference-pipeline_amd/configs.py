@@ -1,13 +1,16 @@
 """The benchmark workloads: models, parameters, initial states and sizes.
 
-C1  SIR, 1 x 1e6, beta 0.5/d, gamma 0.2/d, rep 0.6, phi 0.1, N = 52 weeks, J = 1000, FP64
-C2  SIRS with seasonal forcing (amplitude 0.3, period 365 d) and white noise on beta
-    (sigma 0.1), 4 coupled populations of 5e5, N = 520 weeks, J = 1e5, FP64 vs FP32
-C3  256 C2-model filters, parameters drawn around the C2 truth, J ~ U{1000..20000},
-    N ~ U{104..520}
-C4  served stream of SIR / SIRS requests, J log-uniform in [500, 5e4], N ~ U{52..520}
-C5  SEIR (waning immunity) with 10 gravity-coupled populations of 1e6, N = 1040 weeks,
-    J = 1e6, ancestry kept for --full
+sir_j1000_n52        SIR, 1 x 1e6, beta 0.5/d, gamma 0.2/d, rep 0.6, phi 0.1, N = 52 weeks,
+                     J = 1000, FP64 (the CPU-runnable case)
+sirs4_j100000_n520   SIRS with seasonal forcing (amplitude 0.3, period 365 d) and white noise
+                     on beta (sigma 0.1), 4 coupled populations of 5e5, N = 520 weeks,
+                     J = 1e5, FP64 and FP32 propagation
+pmcmc_batch          256 SIRS-model filters (pMCMC likelihood evaluations), parameters drawn
+                     around the SIRS truth, J ~ U{1000..20000}, N ~ U{104..520}
+served_mix           Poisson stream (200 requests/s) of SIR / SIRS requests, J log-uniform in
+                     [500, 5e4], N ~ U{52..520}
+seir10_j1000000_n1040  SEIR (waning immunity) with 10 gravity-coupled populations of 1e6,
+                     N = 1040 weeks, J = 1e6, ancestry kept for --full
 
 Choices the workload names leave open (initial proportions, waning and recovery rates,
 coupling matrices, seeds) are fixed here once and used by tests and bench alike.
@@ -31,24 +34,51 @@ X0_PROPS = {
 
 DATA_N = {"sir": 520, "sirs4": 520, "seir10": 1040}
 
-CONFIGS = {
-    "C1": {"model": "sir", "J": 1000, "N": 52, "precision": "fp64"},
-    "C2": {"model": "sirs4", "J": 100000, "N": 520, "precision": "fp64"},
-    "C2_fp32": {"model": "sirs4", "J": 100000, "N": 520, "precision": "fp32"},
-    "C5": {"model": "seir10", "J": 1000000, "N": 1040, "precision": "fp64"},
+WORKLOADS = {
+    "sir_j1000_n52": {"model": "sir", "J": 1000, "N": 52, "precision": "fp64"},
+    "sirs4_j100000_n520": {"model": "sirs4", "J": 100000, "N": 520, "precision": "fp64"},
+    "sirs4_j100000_n520_fp32": {"model": "sirs4", "J": 100000, "N": 520, "precision": "fp32"},
+    "seir10_j1000000_n1040": {"model": "seir10", "J": 1000000, "N": 1040, "precision": "fp64"},
 }
 
+# parameters a proposal moves (multiplicatively, on the log scale)
+PROPOSED = {"sir": ("beta", "gamma", "rep"), "sirs4": ("beta", "gamma", "sto", "rep")}
 
-def c3_draws(n_filters=256, seed=3, size_seed=4):
-    """Parameter sets around the C2 truth (5 % sd on the log scale) and uneven sizes."""
+
+def proposal(model, rng, sd=0.05):
+    """A parameter set around the truth of `model`: sd on the log scale."""
+    p = dict(TRUTH[model])
+    for k in PROPOSED[model]:
+        p[k] = float(p[k] * np.exp(sd * rng.standard_normal()))
+    return p
+
+
+def pmcmc_batch(n_filters=256, seed=3, size_seed=4):
+    """The batched pMCMC workload: parameter sets around the SIRS truth (5 % sd on the log
+    scale) and uneven sizes J ~ U{1000..20000}, N ~ U{104..520}."""
     rng = np.random.default_rng(seed)
     srng = np.random.default_rng(size_seed)
     out = []
     for i in range(n_filters):
-        p = dict(TRUTH["sirs4"])
-        for k in ("beta", "gamma", "sto", "rep"):
-            p[k] = float(p[k] * np.exp(0.05 * rng.standard_normal()))
+        p = proposal("sirs4", rng)
         J = int(srng.integers(1000, 20001))
         N = int(srng.integers(104, 521))
-        out.append({"params": p, "J": J, "N": N, "stream": 1000 + i})
+        out.append({"model": "sirs4", "params": p, "J": J, "N": N, "stream": 1000 + i})
     return out
+
+
+def served_mix(rate=200.0, duration_s=60.0, seed=5):
+    """The served workload: Poisson arrivals at `rate` per second over `duration_s`, half SIR
+    and half SIRS requests, J log-uniform in [500, 50000], N ~ U{52..520}, parameters
+    around each model's truth.  Returns requests sorted by arrival time (seconds)."""
+    rng = np.random.default_rng(seed)
+    t, out, i = 0.0, [], 0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration_s:
+            return out
+        model = "sir" if rng.random() < 0.5 else "sirs4"
+        J = int(round(float(np.exp(rng.uniform(np.log(500.0), np.log(50000.0))))))
+        N = int(rng.integers(52, 521))
+        out.append({"t": t, "model": model, "params": proposal(model, rng), "J": J, "N": N, "stream": 100000 + i})
+        i += 1

@@ -17,13 +17,11 @@
  *              log(1+f) = f - (f^2/2 - s (f^2/2 + R(s^2))), R the atanh series
  *              sum_{k>=1} 2 s^{2k}/(2k+1) (11 terms, truncation < 1e-18).
  *   plom_sin/cos  three-part Cody-Waite reduction by pi/2, Taylor kernels on |r| <= pi/4.
- *   plom_sincos_2pi  sin/cos of 2 pi u for u in [0,1), reduced exactly in turns
- *              (Box-Muller angle).
  *   plom_Phi   standard normal CDF: Chebyshev interpolants of (Phi(x)-1/2)/x in x^2 on
  *              |x|<=1 and of the Mills ratio t*Phi(-t)/phi(t) in 1/t on four bands,
  *              evaluated by the Clenshaw recurrence, with an exactly split t^2 inside
  *              exp(-t^2/2).
- * Accuracy (checked against libm/scipy/mpmath in tests/test_math_oracle.py): exp, log,
+ * Accuracy (checked against libm/scipy/mpmath in tests/test_math_rng.py): exp, log,
  * sin, cos within 2 ulp; Phi within 1e-15 relative for Phi >= 1e-300.
  */
 #ifndef PLOM_MATH_H
@@ -203,18 +201,6 @@ PLOM_HD void plom_sincos(double x, double *s, double *c)
 
 PLOM_HD double plom_sin(double x) { double s, c; plom_sincos(x, &s, &c); return s; }
 PLOM_HD double plom_cos(double x) { double s, c; plom_sincos(x, &s, &c); return c; }
-
-#define PLOM_TWO_PI 0x1.921fb54442d18p+2
-
-/* sin(2 pi u), cos(2 pi u) for u in [0, 1); u - q/4 is exact, so the only rounding in
- * the argument is the final product with 2 pi. */
-PLOM_HD void plom_sincos_2pi(double u, double *s, double *c)
-{
-    double qd = plom_rint(4.0 * u);
-    double r = u - 0.25 * qd;
-    double x = r * PLOM_TWO_PI;
-    plom_quadrant(((int)qd) & 3, plom_sin_kernel(x), plom_cos_kernel(x), s, c);
-}
 
 /* ---------------------------------------------------------------- powers */
 /* x^n for a small integer exponent, by repeated products in a fixed order */

@@ -23,14 +23,16 @@
  * Uniforms are (x + 1/2) 2^-32 for a 32-bit x: never 0 or 1, exact in double.
  * Gaussians: Box-Muller pairs on two uniforms; the generated step code takes the two
  * values of a pair in draw order and drops the second value of a pair left open at the
- * end of an Euler substep, so the pairing is fixed at code-generation time.  FP64 uses
- * plom_log_core/plom_sqrt/plom_sincos_2pi; the FP32 stream (device only) uses the gfx950
+ * end of an Euler substep, so the pairing is fixed at code-generation time.  FP64 works
+ * on the two 32-bit words with small lookup tables (plom_bm_m2log, plom_bm_sincos: no
+ * division, no branch) and IEEE sqrt; the FP32 stream (device only) uses the gfx950
  * transcendental instructions.
  */
 #ifndef PLOM_RNG_H
 #define PLOM_RNG_H
 
 #include "plom_math.h"
+#include "plom_bm_tables.h"
 
 #define PLOM_PURPOSE_PROPAGATE 0u
 #define PLOM_PURPOSE_RESAMPLE 1u
@@ -88,28 +90,85 @@ PLOM_HD void plom_xoshiro_seed(plom_xoshiro128 *r, const uint32_t w[4])
     if ((w[0] | w[1] | w[2] | w[3]) == 0u) r->s0 = 1u; /* the all-zero state is a fixed point */
 }
 
-PLOM_HD double plom_u01_from32(uint32_t x) { return ((double)x + 0.5) * 0x1p-32; }
-
 /* ---------------------------------------------------------------- FP64 Gaussians */
-typedef plom_xoshiro128 plom_nstream64;
+/* The lookup tables of the transform (see gen_bm_tables.py).  Kernels copy them to LDS
+ * and hand the stream a pointer to that copy; the host uses this array directly. */
+#if defined(__HIPCC__)
+static __constant__ const double plom_bm_tab[PLOM_BM_TAB_SIZE] = PLOM_BM_TAB_VALUES;
+#else
+static const double plom_bm_tab[PLOM_BM_TAB_SIZE] = PLOM_BM_TAB_VALUES;
+#endif
 
-PLOM_HD void plom_nstream64_init(plom_nstream64 *ns, uint32_t seed, uint32_t stream, uint32_t n, uint32_t j)
+typedef struct { plom_xoshiro128 r; const double *tab; } plom_nstream64;
+
+PLOM_HD void plom_nstream64_init(plom_nstream64 *ns, uint32_t seed, uint32_t stream, uint32_t n, uint32_t j,
+                                 const double *tab)
 {
     uint32_t w[4];
     plom_draw128(seed, stream, n, j, PLOM_PURPOSE_PROPAGATE, w);
-    plom_xoshiro_seed(ns, w);
+    plom_xoshiro_seed(&ns->r, w);
+    ns->tab = tab;
 }
 
-/* one Box-Muller pair: z0 = r cos(2 pi u2), z1 = r sin(2 pi u2), r = sqrt(-2 log u1) */
-PLOM_HD void plom_normal_pair64(plom_nstream64 *ns, double *z0, double *z1)
+/* -2 log u for u = (x + 1/2) 2^-32: u = 2^e m, table entry k from the top fraction bits
+ * of m, log u = e ln2 + L_k + log1p(m invc_k - 1) with the degree-7 Taylor polynomial of
+ * log1p on |z| < 2^-8 (truncation < 7e-21).  Entries k >= 64 hold L_k - ln2 and count
+ * one more power of two, so that log u near u = 1 carries no cancellation. */
+PLOM_HD double plom_bm_m2log(uint32_t x, const double *tab)
 {
-    double u1 = plom_u01_from32(plom_xoshiro_next(ns));
-    double u2 = plom_u01_from32(plom_xoshiro_next(ns));
-    double rad = plom_sqrt(-2.0 * plom_log_core(u1, 0));
+    const double u = plom_fma((double)x, 0x1p-32, 0x1p-33); /* exact */
+    const uint64_t b = plom_d2u(u);
+    const uint32_t hi = (uint32_t)(b >> 32);
+    const uint32_t k = (hi >> (20 - PLOM_BM_LOG_BITS)) & ((1u << PLOM_BM_LOG_BITS) - 1u);
+    const int e = (int)(hi >> 20) - 1023 + (int)(k >> (PLOM_BM_LOG_BITS - 1));
+    const double m = plom_u2d((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
+    const double z = plom_fma(m, tab[2 * k], -1.0);
+    double q = plom_fma(z, 0x1.2492492492492p-3, -0x1.5555555555555p-3);
+    q = plom_fma(q, z, 0x1.999999999999ap-3);
+    q = plom_fma(q, z, -0x1.0000000000000p-2);
+    q = plom_fma(q, z, 0x1.5555555555555p-2);
+    q = plom_fma(q, z, -0x1.0000000000000p-1);
+    const double p = plom_fma(z * z, q, z);
+    const double de = (double)e;
+    return -2.0 * (plom_fma(de, PLOM_LN2_HI, tab[2 * k + 1]) + plom_fma(de, PLOM_LN2_LO, p));
+}
+
+/* sin and cos of 2 pi u for u = (y + 1/2) 2^-32: the angle is a_k + d with a_k =
+ * 2 pi (k + 1/2)/256 (k = top 8 bits of y, tabulated) and d = 2 pi (r + 1/2) 2^-32,
+ * r = (y mod 2^24) - 2^23, |d| <= pi/256; Taylor polynomials of sin d (to d^7) and
+ * cos d - 1 (to d^6), truncation < 2e-20. */
+#define PLOM_BM_DANG 0x1.921fb54442d18p-30 /* 2 pi 2^-32 */
+PLOM_HD void plom_bm_sincos(uint32_t y, const double *tab, double *s, double *c)
+{
+    const uint32_t k = y >> (32 - PLOM_BM_SC_BITS);
+    const int r = (int)(y & ((1u << (32 - PLOM_BM_SC_BITS)) - 1u)) - (1 << (31 - PLOM_BM_SC_BITS));
+    const double d = plom_fma((double)r, PLOM_BM_DANG, 0.5 * PLOM_BM_DANG); /* (r + 1/2) 2 pi 2^-32 */
+    const double sa = tab[PLOM_BM_SC_OFFSET + 2 * k], ca = tab[PLOM_BM_SC_OFFSET + 2 * k + 1];
+    const double d2 = d * d;
+    const double ps = plom_fma(plom_fma(d2, -0x1.a01a01a01a01ap-13, 0x1.1111111111111p-7), d2, -0x1.5555555555555p-3);
+    const double sd = plom_fma(d * d2, ps, d);
+    const double pc = plom_fma(plom_fma(d2, -0x1.6c16c16c16c17p-10, 0x1.5555555555555p-5), d2, -0x1.0000000000000p-1);
+    const double cm1 = d2 * pc;
+    *s = sa + plom_fma(ca, sd, sa * cm1);
+    *c = ca + plom_fma(-sa, sd, ca * cm1);
+}
+
+/* one Box-Muller pair from the words x, y: z0 = r cos(2 pi u2), z1 = r sin(2 pi u2),
+ * r = sqrt(-2 log u1) */
+PLOM_HD void plom_bm_pair(uint32_t x, uint32_t y, const double *tab, double *z0, double *z1)
+{
+    const double rad = plom_sqrt(plom_bm_m2log(x, tab));
     double s, c;
-    plom_sincos_2pi(u2, &s, &c);
+    plom_bm_sincos(y, tab, &s, &c);
     *z0 = rad * c;
     *z1 = rad * s;
+}
+
+PLOM_HD void plom_normal_pair64(plom_nstream64 *ns, double *z0, double *z1)
+{
+    const uint32_t x = plom_xoshiro_next(&ns->r);
+    const uint32_t y = plom_xoshiro_next(&ns->r);
+    plom_bm_pair(x, y, ns->tab, z0, z1);
 }
 
 /* the resampling uniform of interval n as the integer numerator 2x+1 of U = (2x+1)/2^33 */

@@ -229,6 +229,12 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
     plom_dfilter *f = &filters[fi];
     const int n = f->step;
     if (n >= f->N) return;
+    /* the Box-Muller tables in LDS (FP64 only; the block returns above as a whole) */
+    __shared__ double s_bm[PLOM_BM_TAB_SIZE];
+    if constexpr (sizeof(R) == 8) {
+        for (int i = threadIdx.x; i < PLOM_BM_TAB_SIZE; i += TILE) s_bm[i] = plom_bm_tab[i];
+        __syncthreads();
+    }
     const int J = f->J, Jp = f->Jp;
     const int j = tile * TILE + threadIdx.x;
     const bool valid = j < J;
@@ -252,7 +258,7 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
     if (valid) {
         if constexpr (sizeof(R) == 8) {
             plom_nstream64 ns;
-            plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j);
+            plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j, s_bm);
             M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt);
         } else {
             plom_nstream32 ns;
@@ -385,7 +391,7 @@ __global__ void k_selftest_normals(uint32_t seed, uint32_t stream, uint32_t n0, 
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     plom_nstream64 ns;
-    plom_nstream64_init(&ns, seed, stream, n0, j);
+    plom_nstream64_init(&ns, seed, stream, n0, j, plom_bm_tab);
     for (int i = 0; i + 1 < n; i += 2) plom_normal_pair64(&ns, &out[i], &out[i + 1]);
     if (n & 1) {
         double spare;
@@ -459,6 +465,7 @@ extern "C" int plom_model_info_get(int id, plom_model_info *out)
 struct plom_ctx {
     int device;
     hipStream_t stream;
+    hipStream_t io;                 /* result reads that must not wait for queued work */
     hipEvent_t ev[4];
     double last_total_ms, last_prop_ms, last_scan_ms;
     int ktiming;                    /* bracket every launch with events */
@@ -488,18 +495,18 @@ struct plom_filter {
     plom_dfilter *df;       /* device copy of the descriptor */
     plom_dfilter h;         /* host copy (device pointers) */
     int has_run;
-    /* host staging of the converted inputs: must outlive the asynchronous uploads */
-    std::vector<int32_t> nonan;
-    std::vector<float> pvf;
-    std::vector<char> x0b;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+/* One device allocation per filter.  The work buffers come first; the tail from off_img
+ * on (descriptor, counters, per-interval outputs, inputs) is written from one host image
+ * in a single copy, so that creating a filter is one upload (from pinned memory when the
+ * server creates it) and starts from zeroed counters without memset calls. */
 struct layout {
-    size_t off_X0, off_X1, off_logw, off_cum, off_tmax, off_tsum, off_tsum2, off_ttot, off_tincl,
-        off_cnt, off_scal, off_naf, off_llike_n, off_ess_n, off_rflag, off_nonan, off_times, off_data,
-        off_pv, off_pvf, off_x0, off_anc, off_df, total;
+    size_t off_X0, off_X1, off_logw, off_cum, off_tmax, off_tsum, off_tsum2, off_ttot, off_tincl, off_anc,
+        off_img, off_df, off_cnt, off_scal, off_naf, off_llike_n, off_ess_n, off_rflag, off_nonan, off_times,
+        off_data, off_pv, off_pvf, off_x0, total;
 };
 
 static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, layout *L)
@@ -519,6 +526,9 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     PUT(off_tsum2, T * 8);
     PUT(off_ttot, T * 8);
     PUT(off_tincl, T * 8);
+    PUT(off_anc, (d->flags & PLOM_FLAG_STORE_ANCESTRY) ? N * J * 4 : 0);
+    L->off_img = o;
+    PUT(off_df, sizeof(plom_dfilter));
     PUT(off_cnt, 2 * 4);
     PUT(off_scal, 4 * 8);
     PUT(off_naf, 4);
@@ -531,8 +541,6 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     PUT(off_pv, (size_t)mi->pv_size * 8);
     PUT(off_pvf, (size_t)mi->pv_size * 4);
     PUT(off_x0, S * 8);
-    PUT(off_anc, (d->flags & PLOM_FLAG_STORE_ANCESTRY) ? N * J * 4 : 0);
-    PUT(off_df, sizeof(plom_dfilter));
 #undef PUT
     L->total = o;
     return PLOM_OK;
@@ -570,6 +578,8 @@ extern "C" int plom_ctx_create(int device, plom_ctx **out)
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { free(c); return set_err(PLOM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+    e = hipStreamCreateWithFlags(&c->io, hipStreamNonBlocking);
+    if (e != hipSuccess) { (void)hipStreamDestroy(c->stream); free(c); return set_err(PLOM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
     for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&c->ev[i]));
     c->ktiming = 0;
     c->kev_used = 0;
@@ -585,6 +595,7 @@ extern "C" int plom_ctx_destroy(plom_ctx *c)
     for (int i = 0; i < 4; i++) HIPCHK(hipEventDestroy(c->ev[i]));
     for (auto e : c->kev) HIPCHK(hipEventDestroy(e));
     HIPCHK(hipStreamDestroy(c->stream));
+    HIPCHK(hipStreamDestroy(c->io));
     c->~plom_ctx();
     free(c);
     return PLOM_OK;
@@ -597,26 +608,44 @@ extern "C" int plom_sync(plom_ctx *c)
     return PLOM_OK;
 }
 
-/* reset the per-run device state of a filter (enqueued on the context stream) */
+/* back to interval 0 for every filter of a batch (a kernel, so that it can sit inside a
+ * captured graph: memset nodes are not used in graphs) */
+__global__ void k_reset(plom_dfilter *filters, int nf)
+{
+    const int fi = blockIdx.x;
+    if (fi >= nf) return;
+    plom_dfilter *f = &filters[fi];
+    for (int n = threadIdx.x; n < f->N; n += blockDim.x) {
+        f->rflag[n] = 0;
+        f->llike_n[n] = 0.0;
+        f->ess_n[n] = 0.0;
+    }
+    if (threadIdx.x == 0) {
+        f->cnt[0] = 0u;
+        f->cnt[1] = 0u;
+        for (int k = 0; k < 4; k++) f->scal[k] = 0.0;
+        f->n_all_fail[0] = 0;
+        f->step = 0;
+    }
+}
+
+/* back to interval 0 (enqueued on the context stream) */
 static int filter_reset_async(plom_ctx *c, plom_filter *f)
 {
-    plom_dfilter *h = &f->h;
-    HIPCHK(hipMemsetAsync(h->cnt, 0, 2 * sizeof(uint32_t), c->stream));
-    HIPCHK(hipMemsetAsync(h->scal, 0, 4 * sizeof(double), c->stream));
-    HIPCHK(hipMemsetAsync(h->n_all_fail, 0, sizeof(int32_t), c->stream));
-    if (h->N > 0) {
-        HIPCHK(hipMemsetAsync(h->rflag, 0, (size_t)h->N * sizeof(int32_t), c->stream));
-        HIPCHK(hipMemsetAsync(h->llike_n, 0, (size_t)h->N * sizeof(double), c->stream));
-        HIPCHK(hipMemsetAsync(h->ess_n, 0, (size_t)h->N * sizeof(double), c->stream));
-    }
-    /* the step counter lives inside the device descriptor */
-    HIPCHK(hipMemcpyAsync(f->df, h, sizeof(plom_dfilter), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, c->stream, f->df, 1);
+    HIPCHK(hipGetLastError());
     return PLOM_OK;
 }
 
-/* create a filter; with async the allocation is stream-ordered and nothing waits (the
- * caller keeps `in` alive until the stream has passed the uploads) */
-static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom_filter_inputs *in, int async, plom_filter **out)
+/* bytes of the host image of a filter (device layout from off_img to the end) */
+static size_t image_bytes(const layout *L) { return L->total - L->off_img; }
+
+/* Create a filter.  The descriptor, zeroed counters and converted inputs are written into
+ * a host image that mirrors the device tail of the allocation, then uploaded in one copy.
+ * With `pinned` (at least image_bytes, page-locked, kept by the caller until the stream
+ * has passed the upload) the allocation and the copy are stream-ordered and nothing
+ * waits; without it the call returns when the filter is on the device. */
+static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom_filter_inputs *in, char *pinned, plom_filter **out)
 {
     if (!c || !out || !in) return set_err(PLOM_ERR_ARG, "null argument");
     plom_model_info mi;
@@ -628,6 +657,15 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     HIPCHK(hipSetDevice(c->device));
     layout L;
     make_layout(d, &mi, &L);
+    const int async = pinned != nullptr;
+    std::vector<char> pageable;
+    char *img = pinned;
+    if (!img) {
+        pageable.assign(image_bytes(&L), 0);
+        img = pageable.data();
+    } else {
+        memset(img, 0, image_bytes(&L));
+    }
     plom_filter *f = new (std::nothrow) plom_filter();
     if (!f) return set_err(PLOM_ERR_NOMEM, "host allocation failed");
     f->desc = *d;
@@ -665,43 +703,41 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     h->ancestry = (d->flags & PLOM_FLAG_STORE_ANCESTRY) ? (int32_t *)(b + L.off_anc) : nullptr;
     f->df = (plom_dfilter *)(b + L.off_df);
 
+    /* host image: descriptor, zeroed counters and outputs, inputs */
     const int S = mi.size_proj;
-    f->nonan.assign((size_t)d->N, 0);
+    auto at = [&](size_t off) { return img + (off - L.off_img); };  /* device offset -> image byte */
+    memcpy(at(L.off_df), h, sizeof(plom_dfilter));
+    int32_t *nonan = (int32_t *)at(L.off_nonan);
     for (int n = 0; n < d->N; n++) {
         int k = 0;
         for (int ts = 0; ts < mi.n_ts; ts++) k += !(in->data[(size_t)n * mi.n_ts + ts] != in->data[(size_t)n * mi.n_ts + ts]);
-        f->nonan[(size_t)n] = k;
+        nonan[n] = k;
     }
-    f->pvf.assign((size_t)mi.pv_size, 0.f);
-    for (int i = 0; i < mi.pv_size; i++) f->pvf[(size_t)i] = (float)in->pv[i];
-    f->x0b.assign((size_t)S * 8, 0);
+    memcpy(at(L.off_times), in->times, (size_t)(d->N + 1) * 8);
+    if (d->N > 0) memcpy(at(L.off_data), in->data, (size_t)d->N * mi.n_ts * 8);
+    memcpy(at(L.off_pv), in->pv, (size_t)mi.pv_size * 8);
+    float *pvf = (float *)at(L.off_pvf);
+    for (int i = 0; i < mi.pv_size; i++) pvf[i] = (float)in->pv[i];
     if (d->precision == PLOM_FP32) {
-        for (int k = 0; k < S; k++) ((float *)f->x0b.data())[k] = (float)in->x0[k];
+        float *x0 = (float *)at(L.off_x0);
+        for (int k = 0; k < S; k++) x0[k] = (float)in->x0[k];
     } else {
-        memcpy(f->x0b.data(), in->x0, (size_t)S * 8);
+        memcpy(at(L.off_x0), in->x0, (size_t)S * 8);
     }
-    HIPCHK(hipMemcpyAsync(b + L.off_times, in->times, (size_t)(d->N + 1) * 8, hipMemcpyHostToDevice, c->stream));
-    if (d->N > 0) {
-        HIPCHK(hipMemcpyAsync(b + L.off_data, in->data, (size_t)d->N * mi.n_ts * 8, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(b + L.off_nonan, f->nonan.data(), (size_t)d->N * 4, hipMemcpyHostToDevice, c->stream));
-    }
-    HIPCHK(hipMemcpyAsync(b + L.off_pv, in->pv, (size_t)mi.pv_size * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(b + L.off_pvf, f->pvf.data(), (size_t)mi.pv_size * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(b + L.off_x0, f->x0b.data(), (size_t)S * 8, hipMemcpyHostToDevice, c->stream));
-    rc = filter_reset_async(c, f);
-    if (rc != PLOM_OK) {
+    e = hipMemcpyAsync(b + L.off_img, img, image_bytes(&L), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && !async) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
         if (async) (void)hipFreeAsync(f->dev, c->stream); else (void)hipFree(f->dev);
         delete f;
-        return rc;
+        return set_err(PLOM_ERR_HIP, "filter upload: %s", hipGetErrorString(e));
     }
-    if (!async) HIPCHK(hipStreamSynchronize(c->stream));
     *out = f;
     return PLOM_OK;
 }
 
 extern "C" int plom_filter_create(plom_ctx *c, const plom_filter_desc *d, const plom_filter_inputs *in, plom_filter **out)
 {
-    return filter_create_impl(c, d, in, 0, out);
+    return filter_create_impl(c, d, in, nullptr, out);
 }
 
 extern "C" int plom_filter_reset(plom_ctx *c, plom_filter *f)
@@ -771,44 +807,51 @@ extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
     return PLOM_OK;
 }
 
+/* copy a finished filter's results to the host on stream `s` (the caller has made sure
+ * that the filter's work is complete) */
+static int filter_read_on(plom_ctx *c, plom_filter *f, plom_filter_outputs *o, hipStream_t s)
+{
+    const plom_dfilter *h = &f->h;
+    const size_t N = (size_t)f->desc.N, J = (size_t)f->desc.J;
+    if (o->ancestry && N && !h->ancestry) return set_err(PLOM_ERR_STATE, "ancestry requested but PLOM_FLAG_STORE_ANCESTRY not set");
+    if (o->final_state && (!(f->desc.flags & PLOM_FLAG_FINAL_STATE) || N == 0))
+        return set_err(PLOM_ERR_STATE, "final state requested but PLOM_FLAG_FINAL_STATE not set (or N == 0)");
+    double scal[4];
+    int32_t naf = 0;
+    HIPCHK(hipMemcpyAsync(scal, h->scal, sizeof scal, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&naf, h->n_all_fail, sizeof naf, hipMemcpyDeviceToHost, s));
+    if (o->loglik_n && N) HIPCHK(hipMemcpyAsync(o->loglik_n, h->llike_n, N * 8, hipMemcpyDeviceToHost, s));
+    if (o->ess_n && N) HIPCHK(hipMemcpyAsync(o->ess_n, h->ess_n, N * 8, hipMemcpyDeviceToHost, s));
+    if (o->ancestry && N) HIPCHK(hipMemcpyAsync(o->ancestry, h->ancestry, N * J * 4, hipMemcpyDeviceToHost, s));
+    if (o->final_logw) HIPCHK(hipMemcpyAsync(o->final_logw, h->logw, J * 8, hipMemcpyDeviceToHost, s));
+    std::vector<float> tmp;
+    if (o->final_state) {
+        const int S = f->info.size_proj;
+        const void *src = h->X[(N + 1) & 1];
+        if (f->desc.precision == PLOM_FP32) {
+            tmp.resize((size_t)S * J);
+            for (int k = 0; k < S; k++)
+                HIPCHK(hipMemcpyAsync(tmp.data() + (size_t)k * J, (const float *)src + (size_t)k * h->Jp, J * 4, hipMemcpyDeviceToHost, s));
+        } else {
+            for (int k = 0; k < S; k++)
+                HIPCHK(hipMemcpyAsync(o->final_state + (size_t)k * J, (const double *)src + (size_t)k * h->Jp, J * 8, hipMemcpyDeviceToHost, s));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    if (!tmp.empty())
+        for (size_t i = 0; i < tmp.size(); i++) o->final_state[i] = tmp[i];
+    o->loglik = scal[3];
+    o->n_all_fail = naf;
+    return PLOM_OK;
+}
+
 extern "C" int plom_filter_read(plom_ctx *c, plom_filter *f, plom_filter_outputs *o)
 {
     if (!c || !f || !o) return set_err(PLOM_ERR_ARG, "null argument");
     if (!f->has_run) return set_err(PLOM_ERR_STATE, "filter has not run");
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
-    const plom_dfilter *h = &f->h;
-    const size_t N = (size_t)f->desc.N, J = (size_t)f->desc.J;
-    double scal[4];
-    int32_t naf = 0;
-    HIPCHK(hipMemcpy(scal, h->scal, sizeof scal, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(&naf, h->n_all_fail, sizeof naf, hipMemcpyDeviceToHost));
-    o->loglik = scal[3];
-    o->n_all_fail = naf;
-    if (o->loglik_n && N) HIPCHK(hipMemcpy(o->loglik_n, h->llike_n, N * 8, hipMemcpyDeviceToHost));
-    if (o->ess_n && N) HIPCHK(hipMemcpy(o->ess_n, h->ess_n, N * 8, hipMemcpyDeviceToHost));
-    if (o->ancestry && N) {
-        if (!h->ancestry) return set_err(PLOM_ERR_STATE, "ancestry requested but PLOM_FLAG_STORE_ANCESTRY not set");
-        HIPCHK(hipMemcpy(o->ancestry, h->ancestry, N * J * 4, hipMemcpyDeviceToHost));
-    }
-    if (o->final_logw) HIPCHK(hipMemcpy(o->final_logw, h->logw, J * 8, hipMemcpyDeviceToHost));
-    if (o->final_state) {
-        if (!(f->desc.flags & PLOM_FLAG_FINAL_STATE) || N == 0)
-            return set_err(PLOM_ERR_STATE, "final state requested but PLOM_FLAG_FINAL_STATE not set (or N == 0)");
-        const int S = f->info.size_proj;
-        const void *src = h->X[(N + 1) & 1];
-        if (f->desc.precision == PLOM_FP32) {
-            std::vector<float> tmp(J);
-            for (int k = 0; k < S; k++) {
-                HIPCHK(hipMemcpy(tmp.data(), (const float *)src + (size_t)k * h->Jp, J * 4, hipMemcpyDeviceToHost));
-                for (size_t j = 0; j < J; j++) o->final_state[(size_t)k * J + j] = tmp[j];
-            }
-        } else {
-            for (int k = 0; k < S; k++)
-                HIPCHK(hipMemcpy(o->final_state + (size_t)k * J, (const double *)src + (size_t)k * h->Jp, J * 8, hipMemcpyDeviceToHost));
-        }
-    }
-    return PLOM_OK;
+    return filter_read_on(c, f, o, c->stream);
 }
 
 extern "C" int plom_smc_run(plom_ctx *c, const plom_filter_desc *d, const plom_filter_inputs *in, plom_filter_outputs *o)
@@ -823,27 +866,6 @@ extern "C" int plom_smc_run(plom_ctx *c, const plom_filter_desc *d, const plom_f
 }
 
 /* ------------------------------------------------------------------ batches */
-/* back to interval 0 for every filter of a batch (a kernel, so that it can sit inside a
- * captured graph: memset nodes are not used in graphs) */
-__global__ void k_reset(plom_dfilter *filters, int nf)
-{
-    const int fi = blockIdx.x;
-    if (fi >= nf) return;
-    plom_dfilter *f = &filters[fi];
-    for (int n = threadIdx.x; n < f->N; n += blockDim.x) {
-        f->rflag[n] = 0;
-        f->llike_n[n] = 0.0;
-        f->ess_n[n] = 0.0;
-    }
-    if (threadIdx.x == 0) {
-        f->cnt[0] = 0u;
-        f->cnt[1] = 0u;
-        for (int k = 0; k < 4; k++) f->scal[k] = 0.0;
-        f->n_all_fail[0] = 0;
-        f->step = 0;
-    }
-}
-
 struct plom_batch {
     int nf;
     std::vector<plom_filter *> fs;     /* in N-descending order (row r of the descriptor array) */
@@ -1024,13 +1046,54 @@ extern "C" int plom_batch_run(plom_ctx *c, int nf, plom_filter **fs, int use_gra
 /* Continuous batching: requests wait in a FIFO on the host; at every step the server
  * admits the queue head while its device bytes fit the HBM budget (and the slot limit),
  * then advances every active request by one observation interval with one batched
- * launch round (one propagate launch per (model, precision, variant) group, one scan
- * launch), and retires the requests whose last interval was just enqueued (a HIP event
- * marks their completion).  A request's result depends only on its descriptor and
- * inputs (its random key is its own stream id), never on the mix around it. */
+ * launch round (one propagate launch per (model, precision, variant) group, then the
+ * weight / scan / advance launches over all of them), and retires the requests whose last
+ * interval was just enqueued (a HIP event marks their completion).  A request's result
+ * depends only on its descriptor and inputs (its random key is its own stream id), never
+ * on the mix around it.
+ *
+ * Nothing in a step waits for the device except the in-flight bound: step s waits for
+ * step s - max_inflight to finish, so the host stays at most that many intervals ahead
+ * and a new request is admitted within that many intervals.  Host-to-device uploads come
+ * from a pool of page-locked blocks, released once the step that used them is known to
+ * be complete. */
 #define SRV_QUEUED 0
 #define SRV_RUNNING 1
 #define SRV_DONE 2
+
+/* page-locked host blocks in power-of-two size classes, reused, freed at the end */
+struct pinned_pool {
+    std::vector<std::vector<char *>> free_by_class;
+    std::vector<std::pair<char *, int>> all;
+    static int size_class(size_t bytes)
+    {
+        int k = 12;
+        while (((size_t)1 << k) < bytes) k++;
+        return k;
+    }
+    int get(size_t bytes, char **out, int *cls)
+    {
+        const int k = size_class(bytes);
+        if ((int)free_by_class.size() <= k) free_by_class.resize((size_t)k + 1);
+        if (!free_by_class[(size_t)k].empty()) {
+            *out = free_by_class[(size_t)k].back();
+            free_by_class[(size_t)k].pop_back();
+        } else {
+            hipError_t e = hipHostMalloc((void **)out, (size_t)1 << k, hipHostMallocDefault);
+            if (e != hipSuccess) return set_err(PLOM_ERR_NOMEM, "hipHostMalloc(%zu): %s", (size_t)1 << k, hipGetErrorString(e));
+            all.push_back({*out, k});
+        }
+        *cls = k;
+        return PLOM_OK;
+    }
+    void put(char *p, int k) { free_by_class[(size_t)k].push_back(p); }
+    void release_all()
+    {
+        for (auto &pk : all) (void)hipHostFree(pk.first);
+        all.clear();
+        free_by_class.clear();
+    }
+};
 
 struct srv_request {
     uint64_t id;
@@ -1051,39 +1114,60 @@ struct plom_server {
     uint64_t next_id;
     std::vector<srv_request *> queue;          /* FIFO */
     std::vector<srv_request *> active;
-    std::vector<srv_request *> finished;       /* waiting for their event / result */
-    std::vector<srv_request *> done;
+    std::vector<srv_request *> finished;       /* waiting for their event */
+    std::vector<srv_request *> done;           /* complete, result not taken yet */
     int capacity;                              /* descriptor slots on the device */
     plom_dfilter *d_slots;
     std::vector<int> free_slots;
     int2 *d_tiles;                             /* tile maps: all tiles, then per group */
     size_t tiles_cap;
     int *d_slots_active;                       /* slot index of every active request */
-    std::vector<int> h_active[2];
     int dirty;                                 /* active set changed: rebuild the maps */
     struct group { int model, prec, variant; size_t off, n; };
     std::vector<group> groups;
     size_t n_all;
-    std::vector<int2> h_tiles[2];              /* double-buffered host copies of the maps */
-    int hbuf;
     int64_t steps;
+    int depth;                                 /* max steps in flight */
+    std::vector<hipEvent_t> step_ev;           /* ring of depth events */
+    std::vector<std::vector<std::pair<char *, int>>> step_release;  /* pinned blocks per ring slot */
+    pinned_pool pool;
 };
+
+/* step s may reuse ring slot s % depth once the step that used it last is complete */
+static int srv_ring_wait(plom_server *S, int64_t s)
+{
+    const size_t k = (size_t)(s % S->depth);
+    if (s >= S->depth) HIPCHK(hipEventSynchronize(S->step_ev[k]));
+    for (auto &pk : S->step_release[k]) S->pool.put(pk.first, pk.second);
+    S->step_release[k].clear();
+    return PLOM_OK;
+}
+
+static int srv_stage(plom_server *S, const void *src, size_t bytes, void *dst)
+{
+    char *p;
+    int k;
+    int rc = S->pool.get(bytes, &p, &k);
+    if (rc != PLOM_OK) return rc;
+    memcpy(p, src, bytes);
+    S->step_release[(size_t)(S->steps % S->depth)].push_back({p, k});
+    HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, S->c->stream));
+    return PLOM_OK;
+}
 
 static int srv_rebuild_maps(plom_server *S)
 {
     plom_ctx *c = S->c;
-    S->hbuf ^= 1;
-    std::vector<int2> &h = S->h_tiles[S->hbuf];
-    h.clear();
+    std::vector<int2> h;
     S->groups.clear();
     for (auto r : S->active)
         for (int t = 0; t < r->f->h.T; t++) h.push_back(make_int2(r->slot, t));
     S->n_all = h.size();
     for (auto r : S->active) {
-        plom_server::group *g = nullptr;
+        bool seen = false;
         for (auto &gg : S->groups)
-            if (gg.model == r->desc.model_id && gg.prec == r->desc.precision && gg.variant == r->f->variant) g = &gg;
-        if (!g) { S->groups.push_back(plom_server::group{r->desc.model_id, r->desc.precision, r->f->variant, 0, 0}); }
+            if (gg.model == r->desc.model_id && gg.prec == r->desc.precision && gg.variant == r->f->variant) seen = true;
+        if (!seen) S->groups.push_back(plom_server::group{r->desc.model_id, r->desc.precision, r->f->variant, 0, 0});
     }
     for (auto &g : S->groups) {
         g.off = h.size();
@@ -1097,11 +1181,16 @@ static int srv_rebuild_maps(plom_server *S)
         S->tiles_cap = h.size() * 2 + 1024;
         HIPCHK(hipMallocAsync((void **)&S->d_tiles, S->tiles_cap * sizeof(int2), c->stream));
     }
-    if (!h.empty()) HIPCHK(hipMemcpyAsync(S->d_tiles, h.data(), h.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
-    std::vector<int> &ha = S->h_active[S->hbuf];
-    ha.clear();
+    if (!h.empty()) {
+        int rc = srv_stage(S, h.data(), h.size() * sizeof(int2), S->d_tiles);
+        if (rc != PLOM_OK) return rc;
+    }
+    std::vector<int> ha;
     for (auto r : S->active) ha.push_back(r->slot);
-    if (!ha.empty()) HIPCHK(hipMemcpyAsync(S->d_slots_active, ha.data(), ha.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (!ha.empty()) {
+        int rc = srv_stage(S, ha.data(), ha.size() * sizeof(int), S->d_slots_active);
+        if (rc != PLOM_OK) return rc;
+    }
     S->dirty = 0;
     return PLOM_OK;
 }
@@ -1109,6 +1198,7 @@ static int srv_rebuild_maps(plom_server *S)
 extern "C" int plom_server_create(plom_ctx *c, const plom_server_config *cfg, plom_server **out)
 {
     if (!c || !out) return set_err(PLOM_ERR_ARG, "null argument");
+    if (cfg && (cfg->max_active < 0 || cfg->max_inflight_steps < 0)) return set_err(PLOM_ERR_ARG, "negative server limit");
     HIPCHK(hipSetDevice(c->device));
     plom_server *S = new (std::nothrow) plom_server();
     if (!S) return set_err(PLOM_ERR_NOMEM, "host allocation failed");
@@ -1117,14 +1207,20 @@ extern "C" int plom_server_create(plom_ctx *c, const plom_server_config *cfg, pl
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
     S->budget = (cfg && cfg->hbm_budget_bytes > 0) ? cfg->hbm_budget_bytes : (int64_t)(0.8 * (double)free_b);
     S->max_active = (cfg && cfg->max_active > 0) ? cfg->max_active : 4096;
+    S->depth = (cfg && cfg->max_inflight_steps > 0) ? cfg->max_inflight_steps : 4;
     S->capacity = S->max_active;
     S->in_use = 0;
     S->next_id = 1;
     S->d_tiles = nullptr;
     S->tiles_cap = 0;
     S->dirty = 1;
-    S->hbuf = 0;
     S->steps = 0;
+    S->step_release.resize((size_t)S->depth);
+    for (int i = 0; i < S->depth; i++) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        S->step_ev.push_back(e);
+    }
     hipError_t e = hipMalloc((void **)&S->d_slots, (size_t)S->capacity * sizeof(plom_dfilter));
     if (e != hipSuccess) { delete S; return set_err(PLOM_ERR_NOMEM, "hipMalloc slots: %s", hipGetErrorString(e)); }
     e = hipMalloc((void **)&S->d_slots_active, (size_t)S->capacity * sizeof(int));
@@ -1143,6 +1239,8 @@ extern "C" int plom_server_submit(plom_server *S, const plom_filter_desc *d, con
     if (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_FINAL_STATE))
         return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods (no ancestry / final state)");
     if (!in->times || !in->pv || !in->x0 || (d->N > 0 && !in->data)) return set_err(PLOM_ERR_ARG, "null input array");
+    for (int n = 0; n < d->N; n++)
+        if (!(in->times[n + 1] > in->times[n])) return set_err(PLOM_ERR_ARG, "times must increase (n=%d)", n);
     layout L;
     make_layout(d, &mi, &L);
     if ((int64_t)L.total > S->budget)
@@ -1152,12 +1250,32 @@ extern "C" int plom_server_submit(plom_server *S, const plom_filter_desc *d, con
     r->id = S->next_id++;
     r->desc = *d;
     r->times.assign(in->times, in->times + d->N + 1);
-    r->data.assign(in->data, in->data + (size_t)d->N * mi.n_ts);
+    if (d->N > 0) r->data.assign(in->data, in->data + (size_t)d->N * mi.n_ts);
     r->pv.assign(in->pv, in->pv + mi.pv_size);
     r->x0.assign(in->x0, in->x0 + mi.size_proj);
     r->bytes = (int64_t)L.total;
     S->queue.push_back(r);
     *req_id = r->id;
+    return PLOM_OK;
+}
+
+/* move the finished requests whose completion event has passed to `done` */
+static int srv_poll(plom_server *S)
+{
+    for (size_t i = 0; i < S->finished.size();) {
+        srv_request *r = S->finished[i];
+        hipError_t q = hipEventQuery(r->done_ev);
+        if (q == hipSuccess) {
+            r->state = SRV_DONE;
+            S->done.push_back(r);
+            S->finished.erase(S->finished.begin() + (long)i);
+        } else if (q == hipErrorNotReady) {
+            (void)hipGetLastError();
+            i++;
+        } else {
+            return set_err(PLOM_ERR_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+        }
+    }
     return PLOM_OK;
 }
 
@@ -1167,31 +1285,31 @@ extern "C" int plom_server_step(plom_server *S, int32_t *n_active, int32_t *n_qu
     if (!S) return set_err(PLOM_ERR_ARG, "null server");
     plom_ctx *c = S->c;
     HIPCHK(hipSetDevice(c->device));
-    /* completions of earlier steps */
-    for (size_t i = 0; i < S->finished.size();) {
-        srv_request *r = S->finished[i];
-        hipError_t q = hipEventQuery(r->done_ev);
-        if (q == hipErrorNotReady) (void)hipGetLastError();
-        if (q == hipSuccess) {
-            r->state = SRV_DONE;
-            S->done.push_back(r);
-            S->finished.erase(S->finished.begin() + (long)i);
-        } else if (q == hipErrorNotReady) {
-            i++;
-        } else {
-            return set_err(PLOM_ERR_HIP, "hipEventQuery: %s", hipGetErrorString(q));
-        }
+    int rc = srv_poll(S);
+    if (rc != PLOM_OK) return rc;
+    if (!S->queue.empty() || !S->active.empty()) {
+        rc = srv_ring_wait(S, S->steps);
+        if (rc != PLOM_OK) return rc;
     }
     /* admission by free HBM, FIFO */
     while (!S->queue.empty() && (int)S->active.size() < S->max_active) {
         srv_request *r = S->queue.front();
         if (S->in_use + r->bytes > S->budget) break;
+        plom_model_info mi = g_models[r->desc.model_id].info;
+        layout L;
+        make_layout(&r->desc, &mi, &L);
+        char *img;
+        int k;
+        rc = S->pool.get(image_bytes(&L), &img, &k);
+        if (rc != PLOM_OK) return rc;
+        S->step_release[(size_t)(S->steps % S->depth)].push_back({img, k});
         plom_filter_inputs in = {r->times.data(), r->data.data(), r->pv.data(), r->x0.data()};
-        int rc = filter_create_impl(c, &r->desc, &in, 1, &r->f);
+        rc = filter_create_impl(c, &r->desc, &in, img, &r->f);
         if (rc != PLOM_OK) return rc;
         r->slot = S->free_slots.back();
         S->free_slots.pop_back();
-        HIPCHK(hipMemcpyAsync(S->d_slots + r->slot, &r->f->h, sizeof(plom_dfilter), hipMemcpyHostToDevice, c->stream));
+        /* the slot descriptor starts as the filter's own (uploaded just before) */
+        HIPCHK(hipMemcpyAsync(S->d_slots + r->slot, r->f->df, sizeof(plom_dfilter), hipMemcpyDeviceToDevice, c->stream));
         S->in_use += r->bytes;
         r->state = SRV_RUNNING;
         S->active.push_back(r);
@@ -1199,7 +1317,7 @@ extern "C" int plom_server_step(plom_server *S, int32_t *n_active, int32_t *n_qu
         S->dirty = 1;
     }
     if (S->dirty) {
-        int rc = srv_rebuild_maps(S);
+        rc = srv_rebuild_maps(S);
         if (rc != PLOM_OK) return rc;
     }
     /* one interval for every active request */
@@ -1209,7 +1327,6 @@ extern "C" int plom_server_step(plom_server *S, int32_t *n_active, int32_t *n_qu
             g_models[g.model].launch_prop(g.prec, g.variant, (int)g.n, c->stream, S->d_slots, S->d_tiles + g.off);
         enqueue_resample(c->stream, S->d_slots, (const int2 *)S->d_tiles, (int)S->n_all, S->d_slots_active, (int)S->active.size());
         HIPCHK(hipGetLastError());
-        S->steps++;
         for (size_t i = 0; i < S->active.size();) {
             srv_request *r = S->active[i];
             r->progress++;
@@ -1227,6 +1344,8 @@ extern "C" int plom_server_step(plom_server *S, int32_t *n_active, int32_t *n_qu
                 i++;
             }
         }
+        HIPCHK(hipEventRecord(S->step_ev[(size_t)(S->steps % S->depth)], c->stream));
+        S->steps++;
     }
     if (n_active) *n_active = (int32_t)S->active.size();
     if (n_queued) *n_queued = (int32_t)S->queue.size();
@@ -1246,25 +1365,40 @@ static srv_request *srv_find(plom_server *S, uint64_t id, std::vector<srv_reques
 extern "C" int plom_server_status(plom_server *S, uint64_t id)
 {
     if (!S) return set_err(PLOM_ERR_ARG, "null server");
+    int rc = srv_poll(S);
+    if (rc != PLOM_OK) return rc;
     srv_request *r = srv_find(S, id, nullptr);
     if (!r) return set_err(PLOM_ERR_ARG, "unknown request %llu", (unsigned long long)id);
-    if (r->state == SRV_RUNNING && r->done_ev) {
-        hipError_t q = hipEventQuery(r->done_ev);
-        if (q == hipSuccess) return SRV_DONE;
-        (void)hipGetLastError();
-    }
     return r->state;
+}
+
+extern "C" int plom_server_done(plom_server *S, uint64_t *ids, int32_t cap, int32_t *n)
+{
+    if (!S || !n || cap < 0 || (cap > 0 && !ids)) return set_err(PLOM_ERR_ARG, "bad arguments");
+    int rc = srv_poll(S);
+    if (rc != PLOM_OK) return rc;
+    int32_t k = 0;
+    for (auto r : S->done) {
+        if (k >= cap) break;
+        ids[k++] = r->id;
+    }
+    *n = k;
+    return PLOM_OK;
 }
 
 extern "C" int plom_server_result(plom_server *S, uint64_t id, plom_filter_outputs *o)
 {
     if (!S || !o) return set_err(PLOM_ERR_ARG, "null argument");
+    if (o->ancestry || o->final_state || o->final_logw)
+        return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods (no ancestry / final state)");
     std::vector<srv_request *> *where = nullptr;
     srv_request *r = srv_find(S, id, &where);
     if (!r) return set_err(PLOM_ERR_ARG, "unknown request %llu", (unsigned long long)id);
     if (!r->done_ev) return set_err(PLOM_ERR_STATE, "request %llu has not finished", (unsigned long long)id);
+    HIPCHK(hipSetDevice(S->c->device));
     HIPCHK(hipEventSynchronize(r->done_ev));
-    int rc = plom_filter_read(S->c, r->f, o);
+    /* the request's work is complete: read on the side stream, past the queued steps */
+    int rc = filter_read_on(S->c, r->f, o, S->c->io);
     if (rc != PLOM_OK) return rc;
     HIPCHK(hipEventDestroy(r->done_ev));
     HIPCHK(hipFreeAsync(r->f->dev, S->c->stream));
@@ -1289,6 +1423,8 @@ extern "C" int plom_server_destroy(plom_server *S)
         }
     if (S->d_tiles) HIPCHK(hipFreeAsync(S->d_tiles, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto e : S->step_ev) HIPCHK(hipEventDestroy(e));
+    S->pool.release_all();
     HIPCHK(hipFree(S->d_slots));
     HIPCHK(hipFree(S->d_slots_active));
     delete S;

@@ -55,14 +55,14 @@ EXPORTS = [
     "plom_batch_run", "plom_sync", "plom_ctx_set_kernel_timing", "plom_last_timing", "plom_selftest_math",
     "plom_selftest_normals", "plom_selftest_copy", "plom_batch_create", "plom_batch_launch",
     "plom_batch_destroy", "plom_server_create", "plom_server_submit", "plom_server_step",
-    "plom_server_status", "plom_server_result", "plom_server_destroy",
+    "plom_server_status", "plom_server_done", "plom_server_result", "plom_server_destroy",
 ]
 
 SRV_QUEUED, SRV_RUNNING, SRV_DONE = 0, 1, 2
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active", C.c_int32), ("reserved", C.c_int32)]
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active", C.c_int32), ("max_inflight_steps", C.c_int32)]
 
 _lib = None
 
@@ -109,6 +109,7 @@ def lib():
     L.plom_server_submit.argtypes = [C.c_void_p, C.POINTER(FilterDesc), C.POINTER(FilterInputs), C.POINTER(C.c_uint64)]
     L.plom_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.plom_server_status.argtypes = [C.c_void_p, C.c_uint64]
+    L.plom_server_done.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_int32, C.POINTER(C.c_int32)]
     L.plom_server_result.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(FilterOutputs)]
     L.plom_server_destroy.argtypes = [C.c_void_p]
     _lib = L
@@ -345,9 +346,10 @@ class Batch:
 class Server:
     """The served stream (plom_server_*): continuous batching with admission by free HBM."""
 
-    def __init__(self, ctx, hbm_budget_bytes=0, max_active=0):
+    def __init__(self, ctx, hbm_budget_bytes=0, max_active=0, max_inflight_steps=0):
         self.ctx = ctx
-        cfg = ServerConfig(int(hbm_budget_bytes), int(max_active), 0)
+        cfg = ServerConfig(int(hbm_budget_bytes), int(max_active), int(max_inflight_steps))
+        self._ids = (C.c_uint64 * 4096)()
         self.h = C.c_void_p()
         check(lib().plom_server_create(ctx.h, C.byref(cfg), C.byref(self.h)))
         self.N = {}
@@ -376,6 +378,12 @@ class Server:
         if r < 0:
             check(r)
         return r
+
+    def done(self):
+        """Ids of the finished requests whose results have not been taken (completion order)."""
+        n = C.c_int32()
+        check(lib().plom_server_done(self.h, self._ids, len(self._ids), C.byref(n)))
+        return list(self._ids[:n.value])
 
     def result(self, rid):
         N = self.N.pop(rid)

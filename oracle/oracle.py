@@ -34,6 +34,7 @@ def lib():
         L.oracle_math.argtypes = [C.c_int, dp, dp]
         L.oracle_normals.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
         L.oracle_philox.argtypes = [C.POINTER(C.c_uint32)] * 3
+        L.oracle_bm_pairs.argtypes = [C.c_int, C.POINTER(C.c_uint32), dp]
         L.oracle_xoshiro.argtypes = [C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_uint32)]
         L.oracle_model_name.restype = C.c_char_p
         _lib = L
@@ -108,6 +109,14 @@ def math(x):
 def normals(seed, stream, n0, j, n):
     out = np.zeros(n)
     lib().oracle_normals(seed, stream, n0, j, n, _dp(out))
+    return out
+
+
+def bm_pairs(xy):
+    """Box-Muller pairs of the 32-bit word pairs xy[:, 0], xy[:, 1] (shape (n, 2))."""
+    xy = np.ascontiguousarray(xy, dtype=np.uint32).reshape(-1, 2)
+    out = np.zeros(xy.shape)
+    lib().oracle_bm_pairs(len(xy), xy.ctypes.data_as(C.POINTER(C.c_uint32)), _dp(out))
     return out
 
 

@@ -104,7 +104,7 @@ static void propagate_range(prop_job *p)
     for (int j = p->j0; j < p->j1; j++) {
         double *x = p->X + (size_t)j * S;
         plom_nstream64 ns;
-        plom_nstream64_init(&ns, p->seed, p->stream, (uint32_t)p->n, (uint32_t)j);
+        plom_nstream64_init(&ns, p->seed, p->stream, (uint32_t)p->n, (uint32_t)j, plom_bm_tab);
         M->interval(x, p->pv, &ns, p->times[p->n], p->times[p->n + 1], p->dt, p->sqdt, p->noises_off);
         if (p->filter && p->nonan) {
             int nf = 0;
@@ -333,7 +333,7 @@ int oracle_trajectory(int model, int N, uint32_t noises_off, uint32_t seed, uint
     memcpy(out, x0, (size_t)S * sizeof(double));
     for (int n = 0; n < N; n++) {
         plom_nstream64 ns;
-        plom_nstream64_init(&ns, seed, stream, (uint32_t)n, j);
+        plom_nstream64_init(&ns, seed, stream, (uint32_t)n, j, plom_bm_tab);
         M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off);
         memcpy(out + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
     }
@@ -371,12 +371,18 @@ void oracle_math(int n, const double *in, double *out)
 void oracle_normals(uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int n, double *out)
 {
     plom_nstream64 ns;
-    plom_nstream64_init(&ns, seed, stream, n0, j);
+    plom_nstream64_init(&ns, seed, stream, n0, j, plom_bm_tab);
     for (int i = 0; i + 1 < n; i += 2) plom_normal_pair64(&ns, &out[i], &out[i + 1]);
     if (n & 1) {
         double spare;
         plom_normal_pair64(&ns, &out[n - 1], &spare);
     }
+}
+
+/* the Box-Muller transform of given word pairs (xy[2i], xy[2i+1]) -> out[2i], out[2i+1] */
+void oracle_bm_pairs(int n, const uint32_t *xy, double *out)
+{
+    for (int i = 0; i < n; i++) plom_bm_pair(xy[2 * i], xy[2 * i + 1], plom_bm_tab, &out[2 * i], &out[2 * i + 1]);
 }
 
 void oracle_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) { plom_philox4x32_10(ctr, key, out); }

@@ -54,7 +54,7 @@ def test_normals_bitwise(ctx):
 
 
 def test_c1_parity(ctx):
-    """Config C1 exactly: SIR, J = 1000, N = 52, FP64."""
+    """The CPU-runnable SIR workload exactly: J = 1000, N = 52, FP64."""
     compare(ctx, "sir", 1000, 52)
 
 

@@ -92,3 +92,28 @@ def test_served_fp32_mix(ctx):
     assert ra.loglik == alone(ctx, "sirs4", 3000, 20, 7, precision="fp32").loglik
     assert rb.loglik == alone(ctx, "sirs4", 3000, 20, 7).loglik
     srv.close()
+
+
+def test_done_list_and_inflight_bound(ctx):
+    """With one step in flight, requests admitted at different times still give their
+    results alone; plom_server_done lists them in completion order (shorter N first)."""
+    srv = smc.Server(ctx, max_inflight_steps=1)
+    order = [("sir", 1200, 40, 21), ("sir", 900, 10, 22), ("sirs4", 600, 25, 23)]
+    ids = []
+    for k, (name, J, N, stream) in enumerate(order):
+        m, times, data, pv, x0 = inputs(name, N)
+        ids.append(srv.submit(m, J, times, data, pv, x0, seed=5, stream=stream))
+        srv.step()
+    finished = []
+    for _ in range(1000):
+        srv.step()
+        for rid in srv.done():
+            if rid not in finished:
+                finished.append(rid)
+        if len(finished) == len(ids):
+            break
+    assert finished == [ids[1], ids[2], ids[0]]
+    for rid, (name, J, N, stream) in zip(ids, order):
+        assert srv.result(rid).loglik == alone(ctx, name, J, N, stream).loglik
+    assert srv.done() == []
+    srv.close()

@@ -95,3 +95,62 @@ def test_sincos_domain():
     """sin/cos are defined on |x| <= 2^20 and NaN outside (the same on host and device)."""
     r = oracle.math(np.array([2.0 ** 20, -2.0 ** 20, 2.0 ** 20 * 1.0000001, 1e300]))
     assert np.isfinite(r[:2, 3:5]).all() and np.isnan(r[2:, 3:5]).all()
+
+
+def bm_exact(x, y):
+    """Box-Muller of the words x, y in 40-digit arithmetic."""
+    u1 = (mp.mpf(int(x)) + mp.mpf(1) / 2) / 2 ** 32
+    u2 = (mp.mpf(int(y)) + mp.mpf(1) / 2) / 2 ** 32
+    r = mp.sqrt(-2 * mp.log(u1))
+    return r, r * mp.cos(2 * mp.pi * u2), r * mp.sin(2 * mp.pi * u2)
+
+
+def test_box_muller_transform_against_mpmath():
+    """The table-driven FP64 transform (plom_bm_pair) against the exact transform: random
+    words, the extreme words (u1 -> 2^-33 and u1 -> 1), and every table boundary of both
+    words.  The error bound is 1e-15 absolute per unit of radius (a few ulps)."""
+    rng = np.random.default_rng(11)
+    xs = list(rng.integers(0, 2 ** 32, 3000, dtype=np.uint64))
+    ys = list(rng.integers(0, 2 ** 32, 3000, dtype=np.uint64))
+    edges = [0, 1, 2, 2 ** 31 - 1, 2 ** 31, 2 ** 32 - 2, 2 ** 32 - 1]
+    # log table boundaries: u = 2^e (1 + k/128) and the k = 64 switch, at several exponents
+    for e in (1, 5, 17, 31):
+        for k in (0, 1, 63, 64, 65, 127):
+            v = (2 ** e) * (128 + k) // 128
+            edges += [v - 1, v, v + 1]
+    # angle table boundaries: y = k 2^24 (+-1) and the centre of each entry
+    sc = []
+    for k in (0, 1, 63, 64, 127, 128, 191, 192, 255):
+        sc += [k * 2 ** 24, k * 2 ** 24 + 2 ** 23, (k + 1) * 2 ** 24 - 1]
+    xs += edges + [int(rng.integers(0, 2 ** 32)) for _ in sc]
+    ys += [int(rng.integers(0, 2 ** 32)) for _ in edges] + sc
+    xy = np.array([[int(a) % 2 ** 32, int(b) % 2 ** 32] for a, b in zip(xs, ys)], dtype=np.uint32)
+    z = oracle.bm_pairs(xy)
+    worst = 0.0
+    for (x, y), (z0, z1) in zip(xy, z):
+        r, e0, e1 = bm_exact(x, y)
+        tol = 1e-15 * max(1.0, float(r))
+        worst = max(worst, abs(z0 - float(e0)) / tol, abs(z1 - float(e1)) / tol)
+    assert worst <= 1.0, worst
+
+
+def test_box_muller_near_one_keeps_absolute_accuracy():
+    """u1 -> 1 gives a tiny radius; log u1 there must not lose absolute accuracy (the
+    table's k >= 64 entries hold log c_k - ln 2 for that reason)."""
+    xy = np.array([[2 ** 32 - 1 - d, 2 ** 30] for d in range(0, 4000, 7)], dtype=np.uint32)
+    z = oracle.bm_pairs(xy)
+    for (x, y), (z0, z1) in zip(xy, z):
+        r, e0, e1 = bm_exact(x, y)
+        # within 2 ulps of the radius (4.5e-16 relative), also where r ~ 1e-5
+        assert abs(z0 - float(e0)) <= 4.5e-16 * float(r) + 1e-22
+        assert abs(z1 - float(e1)) <= 4.5e-16 * float(r) + 1e-22
+
+
+def test_normal_streams_are_the_transform_of_xoshiro_words():
+    """oracle.normals(seed, stream, n, j) = Box-Muller of the xoshiro128** words seeded by
+    Philox(ctr = (j, n, 0, 0), key = (seed, stream)), pairs in order."""
+    seed, stream, n, j = 9, 4, 17, 300
+    w = oracle.philox([j, n, 0, 0], [seed, stream])
+    words, _ = oracle.xoshiro(w, 40)
+    z = oracle.normals(seed, stream, n, j, 40)
+    np.testing.assert_array_equal(z, oracle.bm_pairs(np.array(words, dtype=np.uint32).reshape(-1, 2)).ravel())

@@ -53,7 +53,7 @@ def test_normals_bitwise(ctx):
                                       oracle.normals(seed, stream, n0, j, 257))
 
 
-def test_c1_parity(ctx):
+def test_sir_j1000_n52_parity(ctx):
     """The CPU-runnable SIR workload exactly: J = 1000, N = 52, FP64."""
     compare(ctx, "sir", 1000, 52)
 

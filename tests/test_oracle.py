@@ -133,7 +133,7 @@ def test_one_particle_is_one_path():
     np.testing.assert_allclose(r["loglik_n"], expect, rtol=1e-13, atol=1e-12)
 
 
-def test_filter_canonical_vs_reference_c1():
+def test_filter_canonical_vs_reference_sir_j1000_n52():
     m, times, data, pv, x0 = setup("sir", 52)
     a = oracle.smc(m.id, 1000, times, data, pv, x0, ancestry=True)
     b = oracle.smc(m.id, 1000, times, data, pv, x0, ancestry=True, mode=oracle.REFERENCE)

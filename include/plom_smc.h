@@ -174,7 +174,8 @@ int plom_sync(plom_ctx *ctx);
 int plom_ctx_set_kernel_timing(plom_ctx *ctx, int on);
 int plom_last_timing(plom_ctx *ctx, double *total_ms, double *propagate_ms, double *scan_ms, int32_t *n_intervals);
 
-/* device self-test of the shared arithmetic: out[i*6+..] = exp, log, Phi, sin, cos, sqrt of in[i] */
+/* device self-test of the shared arithmetic: out[i*7+..] = exp, log, Phi, sin, cos, sqrt,
+ * sqrt_mid (the Box-Muller radius square root) of in[i] */
 int plom_selftest_math(plom_ctx *ctx, int n, const double *in, double *out);
 /* plain device copy of nbytes (read nbytes + write nbytes), repeated: bandwidth reference */
 int plom_selftest_copy(plom_ctx *ctx, int64_t nbytes, int reps, double *ms_per_copy);

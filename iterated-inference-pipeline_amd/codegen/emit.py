@@ -104,7 +104,7 @@ class Emitter:
 
     def eq_parts(self, items, cac, real, dem, env):
         """items: list of (sign, term, reaction). Returns (eq, dem, env) strings."""
-        sq = "plom_sqrt" if real == "double" else "__builtin_sqrtf"
+        sq = "plom_sqrt" if real == "double" else "plom_fsqrtf"
         eq = "".join(" %s (%s)" % (s, t) for s, t, _ in items)
         d = "".join("%s %s((%s))*d%d_%d" % (s, sq, t, r["ind_dem_sto"], cac) for s, t, r in items) if dem else ""
         e = ""
@@ -114,6 +114,14 @@ class Emitter:
                     j = int(r["white_noise"]["name"].split("__")[1])
                     e += "%s (%s)*%s*w%d_%d" % (s, t, self.sd_text(r["white_noise"]["sd"], cac, real), j, cac)
         return eq, d, e
+
+    @staticmethod
+    def clip(idx, real):
+        """X = max(f, 0) after a step (core_template.c:328-336); the FP64 form is the
+        reference's comparison, the FP32 form the hardware max."""
+        if real == "double":
+            return "        X[%d] = (f%d < 0.0) ? 0.0 : f%d;" % (idx, idx, idx)
+        return "        X[%d] = plom_clipf(f%d);" % (idx, idx)
 
     def interval(self, variant, dem, env, real):
         m = self.m
@@ -230,15 +238,15 @@ class Emitter:
                     contrib[(idx, c)] = True
             # y_pred -> X with X >= 0 (core_template.c:328-336)
             for idx in local:
-                L.append("        X[%d] = (f%d < %s) ? %s : f%d;" % (idx, idx, zero, zero, idx))
+                L.append(self.clip(idx, real))
         for idx in deferred:
-            L.append("        X[%d] = (f%d < %s) ? %s : f%d;" % (idx, idx, zero, zero, idx))
+            L.append(self.clip(idx, real))
         for idx, cl, lst in incs:
             if cl != sorted(cl):
                 for c in cl:
                     L.append("        s%d += q%d_%d;" % (idx, idx, c))
             L.append("        const %s f%d = X[%d] + s%d;" % (Rt, idx, idx, idx))
-            L.append("        X[%d] = (f%d < %s) ? %s : f%d;" % (idx, idx, zero, zero, idx))
+            L.append(self.clip(idx, real))
         L.append("        t += dt;")
         L.append("    }")
         L.append("}")

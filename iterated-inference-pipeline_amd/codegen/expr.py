@@ -247,6 +247,8 @@ class CEmitter:
                 if isinstance(b, Num) and b.value == int(b.value) and abs(b.value) <= 16:
                     return "plom_powi%s(%s, %d)" % (self.sfx, self.emit(node.a), int(b.value))
                 return "plom_pow%s(%s, %s)" % (self.sfx, self.emit(node.a), self.emit(node.b))
+            if node.op == "/" and self.real == "float":
+                return "plom_fdivf(%s, %s)" % (self.emit(node.a), self.emit(node.b))
             return "(%s %s %s)" % (self.emit(node.a), node.op, self.emit(node.b))
         if isinstance(node, Call):
             args = [self.emit(a) for a in node.args]

@@ -46,6 +46,31 @@ PLOM_HD uint64_t plom_d2u(double x) { union { double d; uint64_t u; } v; v.d = x
 PLOM_HD double plom_u2d(uint64_t u) { union { double d; uint64_t u; } v; v.u = u; return v.d; }
 PLOM_HD double plom_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 PLOM_HD double plom_sqrt(double x) { return __builtin_sqrt(x); }
+
+/* IEEE sqrt of the Box-Muller radius at a lower cost on the device.  gfx950 has no
+ * correctly rounded f64 sqrt instruction; the compiler refines v_rsq_f64 by
+ * Newton-Raphson steps (the core below) and wraps that in a range scaling (inputs below
+ * 2^-767) and a pass-through of 0 and +inf.  For x in [2^-767, 2^1023] the scaling is by
+ * 2^0, so the core alone is the same instruction sequence with the same correctly rounded
+ * result.  plom_sqrt_mid takes only such x: -2 log u lies in [2.3e-10, 46].  The host
+ * computes IEEE sqrt. */
+#if defined(__HIP_DEVICE_COMPILE__)
+PLOM_HD double plom_sqrt_mid(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = 0.5 * y;
+    const double r = plom_fma(-h, g, 0.5);
+    g = plom_fma(g, r, g);
+    h = plom_fma(h, r, h);
+    double d = plom_fma(-g, g, x);
+    g = plom_fma(d, h, g);
+    d = plom_fma(-g, g, x);
+    return plom_fma(d, h, g);
+}
+#else
+PLOM_HD double plom_sqrt_mid(double x) { return __builtin_sqrt(x); }
+#endif
 PLOM_HD double plom_fabs(double x) { return plom_u2d(plom_d2u(x) & 0x7fffffffffffffffULL); }
 PLOM_HD int plom_isnan(double x) { return x != x; }
 /* 2^k for k in [-1022, 1023] */

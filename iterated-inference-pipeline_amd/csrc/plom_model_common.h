@@ -97,6 +97,13 @@ PLOM_DF float plom_powf(float x, float y) { return powf(x, y); }
 PLOM_DF float plom_heavisidef(float x) { return (x < 0.0f) ? 0.0f : 1.0f; }
 PLOM_DF float plom_rampf(float x) { return (x >= 0.0f) ? x : 0.0f; }
 PLOM_DF float plom_slowstepf(float x, float d) { return (x >= 0.0f) ? ((x >= d) ? d : x) : 0.0f; }
+/* FP32 propagation is its own Monte Carlo estimate (checked against FP64 statistically),
+ * so its noise square roots, divisions and clipping use the single-instruction hardware
+ * forms (v_sqrt_f32 and v_rcp_f32 within 1 ulp, v_max_f32) instead of the correctly
+ * rounded expansions */
+PLOM_DF float plom_fsqrtf(float x) { return __builtin_amdgcn_sqrtf(x); }
+PLOM_DF float plom_fdivf(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+PLOM_DF float plom_clipf(float f) { return __builtin_fmaxf(f, 0.0f); }
 #endif
 
 #endif /* PLOM_MODEL_COMMON_H */

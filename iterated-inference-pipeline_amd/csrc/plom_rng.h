@@ -157,7 +157,7 @@ PLOM_HD void plom_bm_sincos(uint32_t y, const double *tab, double *s, double *c)
  * r = sqrt(-2 log u1) */
 PLOM_HD void plom_bm_pair(uint32_t x, uint32_t y, const double *tab, double *z0, double *z1)
 {
-    const double rad = plom_sqrt(plom_bm_m2log(x, tab));
+    const double rad = plom_sqrt_mid(plom_bm_m2log(x, tab));
     double s, c;
     plom_bm_sincos(y, tab, &s, &c);
     *z0 = rad * c;
@@ -202,7 +202,7 @@ __device__ inline __attribute__((always_inline)) void plom_normal_pair32(plom_ns
     float u1 = plom_u01f(plom_xoshiro_next(ns));
     float u2 = plom_u01f(plom_xoshiro_next(ns));
     /* -2 ln u1 = -2 ln2 log2(u1); v_log_f32 is log2, v_sin/v_cos_f32 take turns */
-    float rad = __builtin_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+    float rad = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
     *z0 = rad * __builtin_amdgcn_cosf(u2);
     *z1 = rad * __builtin_amdgcn_sinf(u2);
 }

@@ -371,12 +371,13 @@ __global__ void k_selftest_math(int n, const double *in, double *out)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double x = in[i];
-    out[i * 6 + 0] = plom_exp(x);
-    out[i * 6 + 1] = plom_log(x);
-    out[i * 6 + 2] = plom_Phi(x);
-    out[i * 6 + 3] = plom_sin(x);
-    out[i * 6 + 4] = plom_cos(x);
-    out[i * 6 + 5] = plom_sqrt(x);
+    out[i * 7 + 0] = plom_exp(x);
+    out[i * 7 + 1] = plom_log(x);
+    out[i * 7 + 2] = plom_Phi(x);
+    out[i * 7 + 3] = plom_sin(x);
+    out[i * 7 + 4] = plom_cos(x);
+    out[i * 7 + 5] = plom_sqrt(x);
+    out[i * 7 + 6] = plom_sqrt_mid(x);
 }
 
 /* plain copy (16-byte vectors, grid-stride): the bandwidth reference and the calibration
@@ -1470,12 +1471,12 @@ extern "C" int plom_selftest_math(plom_ctx *c, int n, const double *in, double *
     if (n == 0) return PLOM_OK;
     HIPCHK(hipSetDevice(c->device));
     double *d = nullptr;
-    HIPCHK(hipMalloc((void **)&d, (size_t)n * 7 * 8));
+    HIPCHK(hipMalloc((void **)&d, (size_t)n * 8 * 8));
     HIPCHK(hipMemcpy(d, in, (size_t)n * 8, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_selftest_math, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, d, d + n);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(out, d + n, (size_t)n * 6 * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out, d + n, (size_t)n * 7 * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipFree(d));
     return PLOM_OK;
 }

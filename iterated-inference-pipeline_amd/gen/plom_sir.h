@@ -172,19 +172,19 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_f
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
         float s2 = 0.0f;
-        const float r0_0 = ((P0 * X[1]) / P4);
+        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
         const float r1_0 = P1;
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
         const float d0_0 = sqdt * za0;
         const float d1_0 = sqdt * zb0;
-        const float f0 = X[0] + ( - (r0_0*X[0]))*dtr + - __builtin_sqrtf((r0_0*X[0]))*d0_0;
-        const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - __builtin_sqrtf((r1_0*X[1]))*d1_0+ __builtin_sqrtf((r0_0*X[0]))*d0_0;
-        s2 += ( + (r0_0*X[0]))*dtr + + __builtin_sqrtf((r0_0*X[0]))*d0_0;
-        X[0] = (f0 < 0.0f) ? 0.0f : f0;
-        X[1] = (f1 < 0.0f) ? 0.0f : f1;
+        const float f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_fsqrtf((r0_0*X[0]))*d0_0;
+        const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_fsqrtf((r1_0*X[1]))*d1_0+ plom_fsqrtf((r0_0*X[0]))*d0_0;
+        s2 += ( + (r0_0*X[0]))*dtr + + plom_fsqrtf((r0_0*X[0]))*d0_0;
+        X[0] = plom_clipf(f0);
+        X[1] = plom_clipf(f1);
         const float f2 = X[2] + s2;
-        X[2] = (f2 < 0.0f) ? 0.0f : f2;
+        X[2] = plom_clipf(f2);
         t += dt;
     }
 }
@@ -202,15 +202,15 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
         float s2 = 0.0f;
-        const float r0_0 = ((P0 * X[1]) / P4);
+        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
         const float r1_0 = P1;
         const float f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
         s2 += ( + (r0_0*X[0]))*dtr;
-        X[0] = (f0 < 0.0f) ? 0.0f : f0;
-        X[1] = (f1 < 0.0f) ? 0.0f : f1;
+        X[0] = plom_clipf(f0);
+        X[1] = plom_clipf(f1);
         const float f2 = X[2] + s2;
-        X[2] = (f2 < 0.0f) ? 0.0f : f2;
+        X[2] = plom_clipf(f2);
         t += dt;
     }
 }
@@ -228,19 +228,19 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
         float s2 = 0.0f;
-        const float r0_0 = ((P0 * X[1]) / P4);
+        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
         const float r1_0 = P1;
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
         const float d0_0 = sqdt * za0;
         const float d1_0 = sqdt * zb0;
-        const float f0 = X[0] + ( - (r0_0*X[0]))*dtr + - __builtin_sqrtf((r0_0*X[0]))*d0_0;
-        const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - __builtin_sqrtf((r1_0*X[1]))*d1_0+ __builtin_sqrtf((r0_0*X[0]))*d0_0;
-        s2 += ( + (r0_0*X[0]))*dtr + + __builtin_sqrtf((r0_0*X[0]))*d0_0;
-        X[0] = (f0 < 0.0f) ? 0.0f : f0;
-        X[1] = (f1 < 0.0f) ? 0.0f : f1;
+        const float f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_fsqrtf((r0_0*X[0]))*d0_0;
+        const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_fsqrtf((r1_0*X[1]))*d1_0+ plom_fsqrtf((r0_0*X[0]))*d0_0;
+        s2 += ( + (r0_0*X[0]))*dtr + + plom_fsqrtf((r0_0*X[0]))*d0_0;
+        X[0] = plom_clipf(f0);
+        X[1] = plom_clipf(f1);
         const float f2 = X[2] + s2;
-        X[2] = (f2 < 0.0f) ? 0.0f : f2;
+        X[2] = plom_clipf(f2);
         t += dt;
     }
 }
@@ -258,15 +258,15 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
         float s2 = 0.0f;
-        const float r0_0 = ((P0 * X[1]) / P4);
+        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
         const float r1_0 = P1;
         const float f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
         s2 += ( + (r0_0*X[0]))*dtr;
-        X[0] = (f0 < 0.0f) ? 0.0f : f0;
-        X[1] = (f1 < 0.0f) ? 0.0f : f1;
+        X[0] = plom_clipf(f0);
+        X[1] = plom_clipf(f1);
         const float f2 = X[2] + s2;
-        X[2] = (f2 < 0.0f) ? 0.0f : f2;
+        X[2] = plom_clipf(f2);
         t += dt;
     }
 }

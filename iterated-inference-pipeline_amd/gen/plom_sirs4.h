@@ -589,12 +589,12 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
-        const float tf0 = plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f));
+        const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = (((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), P28);
         const float r1_0 = P8;
         const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
         float za0, zb0;
@@ -605,11 +605,11 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         plom_normal_pair32(ns, &za1, &zb1);
         const float d2_0 = sqdt * za1;
         const float w0_0 = sqdt * zb1;
-        const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - __builtin_sqrtf((r0_0*X[0]))*d0_0+ __builtin_sqrtf((r2_0))*d2_0 + - (r0_0*X[0])*P16*w0_0;
-        const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - __builtin_sqrtf((r1_0*X[4]))*d1_0+ __builtin_sqrtf((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
-        s8 += ( + (r0_0*X[0]))*dtr + + __builtin_sqrtf((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
-        X[0] = (f0 < 0.0f) ? 0.0f : f0;
-        const float r0_1 = (((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - plom_fsqrtf((r0_0*X[0]))*d0_0+ plom_fsqrtf((r2_0))*d2_0 + - (r0_0*X[0])*P16*w0_0;
+        const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_fsqrtf((r1_0*X[4]))*d1_0+ plom_fsqrtf((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
+        s8 += ( + (r0_0*X[0]))*dtr + + plom_fsqrtf((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
+        X[0] = plom_clipf(f0);
+        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), P29);
         const float r1_1 = P9;
         const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
         float za2, zb2;
@@ -620,11 +620,11 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         plom_normal_pair32(ns, &za3, &zb3);
         const float d2_1 = sqdt * za3;
         const float w0_1 = sqdt * zb3;
-        const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - __builtin_sqrtf((r0_1*X[1]))*d0_1+ __builtin_sqrtf((r2_1))*d2_1 + - (r0_1*X[1])*P17*w0_1;
-        const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - __builtin_sqrtf((r1_1*X[5]))*d1_1+ __builtin_sqrtf((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
-        s9 += ( + (r0_1*X[1]))*dtr + + __builtin_sqrtf((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
-        X[1] = (f1 < 0.0f) ? 0.0f : f1;
-        const float r0_2 = (((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - plom_fsqrtf((r0_1*X[1]))*d0_1+ plom_fsqrtf((r2_1))*d2_1 + - (r0_1*X[1])*P17*w0_1;
+        const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_fsqrtf((r1_1*X[5]))*d1_1+ plom_fsqrtf((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
+        s9 += ( + (r0_1*X[1]))*dtr + + plom_fsqrtf((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
+        X[1] = plom_clipf(f1);
+        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), P30);
         const float r1_2 = P10;
         const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
         float za4, zb4;
@@ -635,11 +635,11 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         plom_normal_pair32(ns, &za5, &zb5);
         const float d2_2 = sqdt * za5;
         const float w0_2 = sqdt * zb5;
-        const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - __builtin_sqrtf((r0_2*X[2]))*d0_2+ __builtin_sqrtf((r2_2))*d2_2 + - (r0_2*X[2])*P18*w0_2;
-        const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - __builtin_sqrtf((r1_2*X[6]))*d1_2+ __builtin_sqrtf((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
-        s10 += ( + (r0_2*X[2]))*dtr + + __builtin_sqrtf((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
-        X[2] = (f2 < 0.0f) ? 0.0f : f2;
-        const float r0_3 = (((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - plom_fsqrtf((r0_2*X[2]))*d0_2+ plom_fsqrtf((r2_2))*d2_2 + - (r0_2*X[2])*P18*w0_2;
+        const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_fsqrtf((r1_2*X[6]))*d1_2+ plom_fsqrtf((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
+        s10 += ( + (r0_2*X[2]))*dtr + + plom_fsqrtf((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
+        X[2] = plom_clipf(f2);
+        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), P31);
         const float r1_3 = P11;
         const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
         float za6, zb6;
@@ -650,22 +650,22 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         plom_normal_pair32(ns, &za7, &zb7);
         const float d2_3 = sqdt * za7;
         const float w0_3 = sqdt * zb7;
-        const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - __builtin_sqrtf((r0_3*X[3]))*d0_3+ __builtin_sqrtf((r2_3))*d2_3 + - (r0_3*X[3])*P19*w0_3;
-        const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - __builtin_sqrtf((r1_3*X[7]))*d1_3+ __builtin_sqrtf((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
-        s11 += ( + (r0_3*X[3]))*dtr + + __builtin_sqrtf((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
-        X[3] = (f3 < 0.0f) ? 0.0f : f3;
-        X[4] = (f4 < 0.0f) ? 0.0f : f4;
-        X[5] = (f5 < 0.0f) ? 0.0f : f5;
-        X[6] = (f6 < 0.0f) ? 0.0f : f6;
-        X[7] = (f7 < 0.0f) ? 0.0f : f7;
+        const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - plom_fsqrtf((r0_3*X[3]))*d0_3+ plom_fsqrtf((r2_3))*d2_3 + - (r0_3*X[3])*P19*w0_3;
+        const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - plom_fsqrtf((r1_3*X[7]))*d1_3+ plom_fsqrtf((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
+        s11 += ( + (r0_3*X[3]))*dtr + + plom_fsqrtf((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
+        X[3] = plom_clipf(f3);
+        X[4] = plom_clipf(f4);
+        X[5] = plom_clipf(f5);
+        X[6] = plom_clipf(f6);
+        X[7] = plom_clipf(f7);
         const float f8 = X[8] + s8;
-        X[8] = (f8 < 0.0f) ? 0.0f : f8;
+        X[8] = plom_clipf(f8);
         const float f9 = X[9] + s9;
-        X[9] = (f9 < 0.0f) ? 0.0f : f9;
+        X[9] = plom_clipf(f9);
         const float f10 = X[10] + s10;
-        X[10] = (f10 < 0.0f) ? 0.0f : f10;
+        X[10] = plom_clipf(f10);
         const float f11 = X[11] + s11;
-        X[11] = (f11 < 0.0f) ? 0.0f : f11;
+        X[11] = plom_clipf(f11);
         t += dt;
     }
 }
@@ -722,12 +722,12 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
-        const float tf0 = plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f));
+        const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = (((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), P28);
         const float r1_0 = P8;
         const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
         float za0, zb0;
@@ -736,16 +736,16 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - (r0_0*X[0])*P16*w0_0;
         const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
-        X[0] = (f0 < 0.0f) ? 0.0f : f0;
-        const float r0_1 = (((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        X[0] = plom_clipf(f0);
+        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), P29);
         const float r1_1 = P9;
         const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
         const float w0_1 = sqdt * zb0;
         const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - (r0_1*X[1])*P17*w0_1;
         const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
-        X[1] = (f1 < 0.0f) ? 0.0f : f1;
-        const float r0_2 = (((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        X[1] = plom_clipf(f1);
+        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), P30);
         const float r1_2 = P10;
         const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
         float za1, zb1;
@@ -754,27 +754,27 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - (r0_2*X[2])*P18*w0_2;
         const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
-        X[2] = (f2 < 0.0f) ? 0.0f : f2;
-        const float r0_3 = (((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        X[2] = plom_clipf(f2);
+        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), P31);
         const float r1_3 = P11;
         const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
         const float w0_3 = sqdt * zb1;
         const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - (r0_3*X[3])*P19*w0_3;
         const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + + (r0_3*X[3])*P19*w0_3;
         s11 += ( + (r0_3*X[3]))*dtr + + (r0_3*X[3])*P19*w0_3;
-        X[3] = (f3 < 0.0f) ? 0.0f : f3;
-        X[4] = (f4 < 0.0f) ? 0.0f : f4;
-        X[5] = (f5 < 0.0f) ? 0.0f : f5;
-        X[6] = (f6 < 0.0f) ? 0.0f : f6;
-        X[7] = (f7 < 0.0f) ? 0.0f : f7;
+        X[3] = plom_clipf(f3);
+        X[4] = plom_clipf(f4);
+        X[5] = plom_clipf(f5);
+        X[6] = plom_clipf(f6);
+        X[7] = plom_clipf(f7);
         const float f8 = X[8] + s8;
-        X[8] = (f8 < 0.0f) ? 0.0f : f8;
+        X[8] = plom_clipf(f8);
         const float f9 = X[9] + s9;
-        X[9] = (f9 < 0.0f) ? 0.0f : f9;
+        X[9] = plom_clipf(f9);
         const float f10 = X[10] + s10;
-        X[10] = (f10 < 0.0f) ? 0.0f : f10;
+        X[10] = plom_clipf(f10);
         const float f11 = X[11] + s11;
-        X[11] = (f11 < 0.0f) ? 0.0f : f11;
+        X[11] = plom_clipf(f11);
         t += dt;
     }
 }
@@ -827,12 +827,12 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
-        const float tf0 = plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f));
+        const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = (((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), P28);
         const float r1_0 = P8;
         const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
         float za0, zb0;
@@ -842,11 +842,11 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         float za1, zb1;
         plom_normal_pair32(ns, &za1, &zb1);
         const float d2_0 = sqdt * za1;
-        const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - __builtin_sqrtf((r0_0*X[0]))*d0_0+ __builtin_sqrtf((r2_0))*d2_0;
-        const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - __builtin_sqrtf((r1_0*X[4]))*d1_0+ __builtin_sqrtf((r0_0*X[0]))*d0_0;
-        s8 += ( + (r0_0*X[0]))*dtr + + __builtin_sqrtf((r0_0*X[0]))*d0_0;
-        X[0] = (f0 < 0.0f) ? 0.0f : f0;
-        const float r0_1 = (((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - plom_fsqrtf((r0_0*X[0]))*d0_0+ plom_fsqrtf((r2_0))*d2_0;
+        const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_fsqrtf((r1_0*X[4]))*d1_0+ plom_fsqrtf((r0_0*X[0]))*d0_0;
+        s8 += ( + (r0_0*X[0]))*dtr + + plom_fsqrtf((r0_0*X[0]))*d0_0;
+        X[0] = plom_clipf(f0);
+        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), P29);
         const float r1_1 = P9;
         const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
         const float d0_1 = sqdt * zb1;
@@ -854,11 +854,11 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         plom_normal_pair32(ns, &za2, &zb2);
         const float d1_1 = sqdt * za2;
         const float d2_1 = sqdt * zb2;
-        const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - __builtin_sqrtf((r0_1*X[1]))*d0_1+ __builtin_sqrtf((r2_1))*d2_1;
-        const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - __builtin_sqrtf((r1_1*X[5]))*d1_1+ __builtin_sqrtf((r0_1*X[1]))*d0_1;
-        s9 += ( + (r0_1*X[1]))*dtr + + __builtin_sqrtf((r0_1*X[1]))*d0_1;
-        X[1] = (f1 < 0.0f) ? 0.0f : f1;
-        const float r0_2 = (((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - plom_fsqrtf((r0_1*X[1]))*d0_1+ plom_fsqrtf((r2_1))*d2_1;
+        const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_fsqrtf((r1_1*X[5]))*d1_1+ plom_fsqrtf((r0_1*X[1]))*d0_1;
+        s9 += ( + (r0_1*X[1]))*dtr + + plom_fsqrtf((r0_1*X[1]))*d0_1;
+        X[1] = plom_clipf(f1);
+        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), P30);
         const float r1_2 = P10;
         const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
         float za3, zb3;
@@ -868,11 +868,11 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         float za4, zb4;
         plom_normal_pair32(ns, &za4, &zb4);
         const float d2_2 = sqdt * za4;
-        const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - __builtin_sqrtf((r0_2*X[2]))*d0_2+ __builtin_sqrtf((r2_2))*d2_2;
-        const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - __builtin_sqrtf((r1_2*X[6]))*d1_2+ __builtin_sqrtf((r0_2*X[2]))*d0_2;
-        s10 += ( + (r0_2*X[2]))*dtr + + __builtin_sqrtf((r0_2*X[2]))*d0_2;
-        X[2] = (f2 < 0.0f) ? 0.0f : f2;
-        const float r0_3 = (((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - plom_fsqrtf((r0_2*X[2]))*d0_2+ plom_fsqrtf((r2_2))*d2_2;
+        const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_fsqrtf((r1_2*X[6]))*d1_2+ plom_fsqrtf((r0_2*X[2]))*d0_2;
+        s10 += ( + (r0_2*X[2]))*dtr + + plom_fsqrtf((r0_2*X[2]))*d0_2;
+        X[2] = plom_clipf(f2);
+        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), P31);
         const float r1_3 = P11;
         const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
         const float d0_3 = sqdt * zb4;
@@ -880,22 +880,22 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         plom_normal_pair32(ns, &za5, &zb5);
         const float d1_3 = sqdt * za5;
         const float d2_3 = sqdt * zb5;
-        const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - __builtin_sqrtf((r0_3*X[3]))*d0_3+ __builtin_sqrtf((r2_3))*d2_3;
-        const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - __builtin_sqrtf((r1_3*X[7]))*d1_3+ __builtin_sqrtf((r0_3*X[3]))*d0_3;
-        s11 += ( + (r0_3*X[3]))*dtr + + __builtin_sqrtf((r0_3*X[3]))*d0_3;
-        X[3] = (f3 < 0.0f) ? 0.0f : f3;
-        X[4] = (f4 < 0.0f) ? 0.0f : f4;
-        X[5] = (f5 < 0.0f) ? 0.0f : f5;
-        X[6] = (f6 < 0.0f) ? 0.0f : f6;
-        X[7] = (f7 < 0.0f) ? 0.0f : f7;
+        const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - plom_fsqrtf((r0_3*X[3]))*d0_3+ plom_fsqrtf((r2_3))*d2_3;
+        const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - plom_fsqrtf((r1_3*X[7]))*d1_3+ plom_fsqrtf((r0_3*X[3]))*d0_3;
+        s11 += ( + (r0_3*X[3]))*dtr + + plom_fsqrtf((r0_3*X[3]))*d0_3;
+        X[3] = plom_clipf(f3);
+        X[4] = plom_clipf(f4);
+        X[5] = plom_clipf(f5);
+        X[6] = plom_clipf(f6);
+        X[7] = plom_clipf(f7);
         const float f8 = X[8] + s8;
-        X[8] = (f8 < 0.0f) ? 0.0f : f8;
+        X[8] = plom_clipf(f8);
         const float f9 = X[9] + s9;
-        X[9] = (f9 < 0.0f) ? 0.0f : f9;
+        X[9] = plom_clipf(f9);
         const float f10 = X[10] + s10;
-        X[10] = (f10 < 0.0f) ? 0.0f : f10;
+        X[10] = plom_clipf(f10);
         const float f11 = X[11] + s11;
-        X[11] = (f11 < 0.0f) ? 0.0f : f11;
+        X[11] = plom_clipf(f11);
         t += dt;
     }
 }
@@ -948,51 +948,51 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
-        const float tf0 = plom_sinf((((2.0f * 3.141592653589793f) * tr) / 365.0f));
+        const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = (((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), P28);
         const float r1_0 = P8;
         const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
         const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr;
         const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr;
         s8 += ( + (r0_0*X[0]))*dtr;
-        X[0] = (f0 < 0.0f) ? 0.0f : f0;
-        const float r0_1 = (((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        X[0] = plom_clipf(f0);
+        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), P29);
         const float r1_1 = P9;
         const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
         const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr;
         const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr;
         s9 += ( + (r0_1*X[1]))*dtr;
-        X[1] = (f1 < 0.0f) ? 0.0f : f1;
-        const float r0_2 = (((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        X[1] = plom_clipf(f1);
+        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), P30);
         const float r1_2 = P10;
         const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
         const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr;
         const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr;
         s10 += ( + (r0_2*X[2]))*dtr;
-        X[2] = (f2 < 0.0f) ? 0.0f : f2;
-        const float r0_3 = (((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        X[2] = plom_clipf(f2);
+        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), P31);
         const float r1_3 = P11;
         const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
         const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr;
         const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr;
         s11 += ( + (r0_3*X[3]))*dtr;
-        X[3] = (f3 < 0.0f) ? 0.0f : f3;
-        X[4] = (f4 < 0.0f) ? 0.0f : f4;
-        X[5] = (f5 < 0.0f) ? 0.0f : f5;
-        X[6] = (f6 < 0.0f) ? 0.0f : f6;
-        X[7] = (f7 < 0.0f) ? 0.0f : f7;
+        X[3] = plom_clipf(f3);
+        X[4] = plom_clipf(f4);
+        X[5] = plom_clipf(f5);
+        X[6] = plom_clipf(f6);
+        X[7] = plom_clipf(f7);
         const float f8 = X[8] + s8;
-        X[8] = (f8 < 0.0f) ? 0.0f : f8;
+        X[8] = plom_clipf(f8);
         const float f9 = X[9] + s9;
-        X[9] = (f9 < 0.0f) ? 0.0f : f9;
+        X[9] = plom_clipf(f9);
         const float f10 = X[10] + s10;
-        X[10] = (f10 < 0.0f) ? 0.0f : f10;
+        X[10] = plom_clipf(f10);
         const float f11 = X[11] + s11;
-        X[11] = (f11 < 0.0f) ? 0.0f : f11;
+        X[11] = plom_clipf(f11);
         t += dt;
     }
 }

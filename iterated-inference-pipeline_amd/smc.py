@@ -427,7 +427,7 @@ def run(ctx, model, J, times, data, pv, x0, ancestry=False, final_state=False, *
 
 def selftest_math(ctx, x):
     x = np.ascontiguousarray(x, dtype=np.float64)
-    out = np.zeros((x.size, 6))
+    out = np.zeros((x.size, 7))
     check(lib().plom_selftest_math(ctx.h, x.size, dptr(x), dptr(out)))
     return out
 

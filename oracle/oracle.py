@@ -101,7 +101,7 @@ def obs_loglik(model_id, x, y, pv, t=0.0, like_min=1e-17):
 
 def math(x):
     x = np.ascontiguousarray(x, dtype=np.float64)
-    out = np.zeros((x.size, 6))
+    out = np.zeros((x.size, 7))
     lib().oracle_math(x.size, _dp(x), _dp(out))
     return out
 

@@ -358,12 +358,13 @@ void oracle_math(int n, const double *in, double *out)
 {
     for (int i = 0; i < n; i++) {
         const double x = in[i];
-        out[i * 6 + 0] = plom_exp(x);
-        out[i * 6 + 1] = plom_log(x);
-        out[i * 6 + 2] = plom_Phi(x);
-        out[i * 6 + 3] = plom_sin(x);
-        out[i * 6 + 4] = plom_cos(x);
-        out[i * 6 + 5] = plom_sqrt(x);
+        out[i * 7 + 0] = plom_exp(x);
+        out[i * 7 + 1] = plom_log(x);
+        out[i * 7 + 2] = plom_Phi(x);
+        out[i * 7 + 3] = plom_sin(x);
+        out[i * 7 + 4] = plom_cos(x);
+        out[i * 7 + 5] = plom_sqrt(x);
+        out[i * 7 + 6] = plom_sqrt_mid(x);
     }
 }
 

@@ -99,10 +99,16 @@ def make_filters(smc, ctx, inp, draws, precision):
     return fs
 
 
-def batched_leg(smc, ctx, filters, steps, warmup):
+def batched_leg(smc, ctx, filters, steps, warmup, lanes=0):
     """All filters advance together; the launch sequence is captured once as a hipGraph
-    (first launch) and replayed."""
+    (first launch) and replayed.  lanes = 0: the autotuner picks the number of lanes
+    (concurrent chains) from the first intervals."""
     b = smc.Batch(ctx, filters)
+    tuned = None
+    if lanes:
+        b.configure(lanes)
+    else:
+        lanes, tuned = b.autotune(rounds=32)
     b.launch(use_graph=True)              # capture + first replay
     for _ in range(warmup):
         b.launch(use_graph=True)
@@ -113,14 +119,16 @@ def batched_leg(smc, ctx, filters, steps, warmup):
     ctx.sync()
     wall = (time.perf_counter() - t0) / steps
     ll = np.array([f.read().loglik for f in filters])
-    # per-launch timing pass (events around every launch), outside the timed region
+    # per-launch timing pass (events around every launch), outside the timed region, with
+    # one lane so that no two launches overlap
+    b.configure(1)
     ctx.set_kernel_timing(True)
     b.launch(use_graph=False)
-    total_ms, prop_ms, scan_ms, nint = ctx.last_timing()
+    total_ms, prop_ms, scan_ms, nint, nprop = ctx.last_timing()
     ctx.set_kernel_timing(False)
     b.close()
     return {"wall_s": wall, "loglik": ll, "prop_ms": prop_ms, "scan_ms": scan_ms, "total_ms": total_ms,
-            "launches": nint}
+            "launches": nprop, "rounds": nint, "lanes": lanes, "autotune_ms": tuned}
 
 
 def plain_leg(filters):
@@ -201,7 +209,7 @@ def single_leg(smc, ctx, inp, configs, precision, steps, warmup, J, N):
     res = f.read()
     ctx.set_kernel_timing(True)
     f.run()
-    _, prop_ms, _, nint = ctx.last_timing()
+    _, prop_ms, _, nint, _ = ctx.last_timing()
     ctx.set_kernel_timing(False)
     f.close()
     return {"value": round(J * N / wall, 1), "ms_per_filter": round(wall * 1e3, 3), "loglik": res.loglik,
@@ -226,6 +234,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--legs", default=",".join(LEGS), help="comma list of " + ",".join(LEGS))
     ap.add_argument("--filters", type=int, default=256)
+    ap.add_argument("--lanes", type=int, default=0, help="concurrent chains of the batch (0: autotuned)")
     ap.add_argument("--served-seconds", type=float, default=30.0)
     ap.add_argument("--served-rate", type=float, default=200.0)
     ap.add_argument("--single-J", type=int, default=100000)
@@ -273,20 +282,22 @@ def main():
         f64 = make_filters(smc, ctx, inp, draws, "fp64")
     if "batched" in legs:
         log("batched FP64")
-        fp64 = batched_leg(smc, ctx, f64, args.steps, args.warmup)
+        fp64 = batched_leg(smc, ctx, f64, args.steps, args.warmup, args.lanes)
         out["value"] = round(ps_batch / fp64["wall_s"], 1)
         out["ms_per_step"] = round(fp64["wall_s"] * 1e3, 3)
         log("batched FP32")
         f32 = make_filters(smc, ctx, inp, draws, "fp32")
-        fp32 = batched_leg(smc, ctx, f32, args.steps, args.warmup)
+        fp32 = batched_leg(smc, ctx, f32, args.steps, args.warmup, args.lanes)
         for f in f32:
             f.close()
         rel32 = np.abs(fp32["loglik"] - fp64["loglik"]) / np.abs(fp64["loglik"])
         out["batched"] = {
             "fp64": {"value": out["value"], "ms_per_pass": out["ms_per_step"], "graph": True,
+                     "lanes": fp64["lanes"], "autotune_ms_32_rounds": fp64["autotune_ms"],
                      "propagate_ms": round(fp64["prop_ms"], 3), "weights_scan_ms": round(fp64["scan_ms"], 3),
-                     "launch_rounds": fp64["launches"]},
+                     "launch_rounds": fp64["rounds"], "propagate_launches": fp64["launches"]},
             "fp32": {"value": round(ps_batch / fp32["wall_s"], 1), "ms_per_pass": round(fp32["wall_s"] * 1e3, 3),
+                     "lanes": fp32["lanes"],
                      "mean_rel_dev_vs_fp64": float(rel32.mean()), "max_rel_dev_vs_fp64": float(rel32.max()),
                      "note": "FP32 propagation with FP64 weights: a different Monte Carlo estimate, not an "
                              "approximation of the FP64 one; the tests check its mean over replicates "

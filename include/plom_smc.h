@@ -136,6 +136,17 @@ typedef struct plom_batch plom_batch;
 int plom_batch_create(plom_ctx *ctx, int n_filters, plom_filter **filters, plom_batch **out);
 int plom_batch_launch(plom_ctx *ctx, plom_batch *batch, int use_graph);  /* asynchronous */
 int plom_batch_destroy(plom_ctx *ctx, plom_batch *batch);
+/* Lanes: the plan's filters split round-robin (by descending N) into `lanes` chains, each
+ * advanced on a stream of its own, forked from and joined back into the context stream
+ * (also inside the captured graph), so that one chain fills the SIMDs another leaves idle.
+ * Results do not depend on the number of lanes.  plom_batch_configure sets it (drops a
+ * captured graph); plom_batch_autotune times `rounds` launch rounds spread over the pass
+ * (the grids of intervals 0, maxN/rounds, 2 maxN/rounds, ...) with 1, 2, 4 and 8 lanes
+ * (at most one per filter), keeps the fastest and reports it (`ms_out`: 4 timings, 0 for
+ * a skipped candidate).  The next launch starts the filters from interval 0 again. */
+#define PLOM_MAX_LANES 8
+int plom_batch_configure(plom_ctx *ctx, plom_batch *batch, int lanes);
+int plom_batch_autotune(plom_ctx *ctx, plom_batch *batch, int rounds, int32_t *lanes_out, double *ms_out);
 int plom_batch_run(plom_ctx *ctx, int n_filters, plom_filter **filters, int use_graph); /* create + launch + destroy */
 
 /* Served stream (no counterpart in the reference): continuous batching of filter
@@ -170,9 +181,13 @@ int plom_sync(plom_ctx *ctx);
 /* Timing of the last plom_filter_run / plom_batch_run from HIP events on the context
  * stream.  total_ms always; with kernel timing on (plom_ctx_set_kernel_timing), events
  * also bracket every launch and propagate_ms / scan_ms are the summed durations of the
- * propagate and scan launches, n_intervals the number of propagate launch rounds. */
+ * propagate and scan launches, n_intervals the number of launch rounds and
+ * n_propagate_launches the number of propagate kernels they held (a batch launches one
+ * per (model, precision, variant) group and lane in a round; with timing on, lanes run
+ * one after another, so the launches do not overlap). */
 int plom_ctx_set_kernel_timing(plom_ctx *ctx, int on);
-int plom_last_timing(plom_ctx *ctx, double *total_ms, double *propagate_ms, double *scan_ms, int32_t *n_intervals);
+int plom_last_timing(plom_ctx *ctx, double *total_ms, double *propagate_ms, double *scan_ms, int32_t *n_intervals,
+                     int32_t *n_propagate_launches);
 
 /* device self-test of the shared arithmetic: out[i*7+..] = exp, log, Phi, sin, cos, sqrt,
  * sqrt_mid (the Box-Muller radius square root) of in[i] */

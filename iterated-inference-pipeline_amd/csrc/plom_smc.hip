@@ -472,6 +472,10 @@ struct plom_ctx {
     int ktiming;                    /* bracket every launch with events */
     std::vector<hipEvent_t> kev;    /* pool: 2 per interval + 1 */
     int kev_used;                   /* events recorded by the last run */
+    int kprop_launches;             /* propagate kernel launches of the last run */
+    std::vector<hipStream_t> lane;  /* batch lane streams, created on first use ([0] unused: lane 0 is `stream`) */
+    std::vector<hipEvent_t> join_ev;
+    hipEvent_t fork_ev;
 };
 
 /* event k of the per-launch timing pool (grown on demand) */
@@ -595,6 +599,12 @@ extern "C" int plom_ctx_destroy(plom_ctx *c)
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int i = 0; i < 4; i++) HIPCHK(hipEventDestroy(c->ev[i]));
     for (auto e : c->kev) HIPCHK(hipEventDestroy(e));
+    for (size_t l = 0; l < c->lane.size(); l++) {
+        HIPCHK(hipStreamSynchronize(c->lane[l]));
+        HIPCHK(hipStreamDestroy(c->lane[l]));
+        HIPCHK(hipEventDestroy(c->join_ev[l]));
+    }
+    if (c->fork_ev) HIPCHK(hipEventDestroy(c->fork_ev));
     HIPCHK(hipStreamDestroy(c->stream));
     HIPCHK(hipStreamDestroy(c->io));
     c->~plom_ctx();
@@ -793,9 +803,11 @@ extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
     (void)hipGetLastError(); /* a not-ready query or an unused event earlier must not be reported as a launch failure */
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     int k = 0;
+    c->kprop_launches = 0;
     for (int n = 0; n < f->desc.N; n++) {
         if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
         me->launch_prop(f->desc.precision, f->variant, T, c->stream, f->df, nullptr);
+        c->kprop_launches++;
         if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
         enqueue_resample(c->stream, f->df, nullptr, T, nullptr, 1);
     }
@@ -867,97 +879,182 @@ extern "C" int plom_smc_run(plom_ctx *c, const plom_filter_desc *d, const plom_f
 }
 
 /* ------------------------------------------------------------------ batches */
-struct plom_batch {
-    int nf;
-    std::vector<plom_filter *> fs;     /* in N-descending order (row r of the descriptor array) */
+/* A batch plan splits its filters into `lanes` chains (round-robin over the N-descending
+ * order, so the chains carry similar work).  Each chain advances its filters together, one
+ * launch round per interval, on a stream of its own; the chains fork from and join back
+ * into the context stream, also inside a captured graph.  Two or more chains let one
+ * chain's blocks fill the SIMDs that another leaves idle at the end of a propagate launch
+ * and run beside its small weight / scan / advance launches.  Filters never interact, so
+ * the results do not depend on the number of chains. */
+struct batch_chain {
+    std::vector<plom_filter *> fs;     /* N-descending: row r of the chain's descriptor array */
     struct group { int model, prec, variant; std::vector<int2> tiles; int2 *d; };
     std::vector<group> groups;
     std::vector<int2> all_tiles;
-    int2 *d_all;
-    plom_dfilter *ddf;
-    char *buf;
-    int maxN;
+    int2 *d_all = nullptr;
+    plom_dfilter *ddf = nullptr;
+    char *buf = nullptr;
+    int maxN = 0;
+};
+
+struct plom_batch {
+    int nf;
+    std::vector<plom_filter *> fs;     /* all filters, N-descending */
+    int lanes;
+    std::vector<batch_chain> chains;
     hipGraphExec_t graph;
 };
 
-/* active tiles at interval s: filters are sorted by N, so the active ones are a prefix */
-static int prefix_len(const plom_batch *b, const std::vector<int2> &tl, int s)
+/* active tiles at interval s: a chain's filters are sorted by N, so the active ones are a prefix */
+static int prefix_len(const batch_chain *ch, const std::vector<int2> &tl, int s)
 {
     int k = 0;
-    while (k < (int)tl.size() && b->fs[(size_t)tl[(size_t)k].x]->desc.N > s) k++;
+    while (k < (int)tl.size() && ch->fs[(size_t)tl[(size_t)k].x]->desc.N > s) k++;
     return k;
 }
 
-static int batch_create(plom_ctx *c, int nf, plom_filter **in, plom_batch **out)
+static int chain_build(plom_ctx *c, batch_chain *ch)
 {
-    plom_batch *b = new plom_batch();
-    b->nf = nf;
-    b->graph = nullptr;
-    b->fs.assign(in, in + nf);
-    std::stable_sort(b->fs.begin(), b->fs.end(), [](plom_filter *x, plom_filter *y) { return x->desc.N > y->desc.N; });
-    b->maxN = b->fs[0]->desc.N;
+    const int nf = (int)ch->fs.size();
+    ch->maxN = ch->fs[0]->desc.N;
     for (int r = 0; r < nf; r++) {
-        plom_filter *f = b->fs[(size_t)r];
-        plom_batch::group *g = nullptr;
-        for (auto &gg : b->groups)
+        plom_filter *f = ch->fs[(size_t)r];
+        batch_chain::group *g = nullptr;
+        for (auto &gg : ch->groups)
             if (gg.model == f->desc.model_id && gg.prec == f->desc.precision && gg.variant == f->variant) g = &gg;
         if (!g) {
-            b->groups.push_back(plom_batch::group{f->desc.model_id, f->desc.precision, f->variant, {}, nullptr});
-            g = &b->groups.back();
+            ch->groups.push_back(batch_chain::group{f->desc.model_id, f->desc.precision, f->variant, {}, nullptr});
+            g = &ch->groups.back();
         }
         for (int t = 0; t < f->h.T; t++) {
             g->tiles.push_back(make_int2(r, t));
-            b->all_tiles.push_back(make_int2(r, t));
+            ch->all_tiles.push_back(make_int2(r, t));
         }
     }
-    size_t bytes = align256((size_t)nf * sizeof(plom_dfilter)) + align256(b->all_tiles.size() * sizeof(int2));
-    for (auto &g : b->groups) bytes += align256(g.tiles.size() * sizeof(int2));
-    hipError_t e = hipMalloc((void **)&b->buf, bytes);
+    size_t bytes = align256((size_t)nf * sizeof(plom_dfilter)) + align256(ch->all_tiles.size() * sizeof(int2));
+    for (auto &g : ch->groups) bytes += align256(g.tiles.size() * sizeof(int2));
+    hipError_t e = hipMalloc((void **)&ch->buf, bytes);
     if (e != hipSuccess) {
-        delete b;
+        ch->buf = nullptr;
         return set_err(PLOM_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
     }
     std::vector<plom_dfilter> hd((size_t)nf);
-    for (int r = 0; r < nf; r++) hd[(size_t)r] = b->fs[(size_t)r]->h;
-    b->ddf = (plom_dfilter *)b->buf;
+    for (int r = 0; r < nf; r++) hd[(size_t)r] = ch->fs[(size_t)r]->h;
+    ch->ddf = (plom_dfilter *)ch->buf;
     size_t o = align256((size_t)nf * sizeof(plom_dfilter));
-    b->d_all = (int2 *)(b->buf + o);
-    o += align256(b->all_tiles.size() * sizeof(int2));
-    for (auto &g : b->groups) { g.d = (int2 *)(b->buf + o); o += align256(g.tiles.size() * sizeof(int2)); }
-    HIPCHK(hipMemcpyAsync(b->ddf, hd.data(), (size_t)nf * sizeof(plom_dfilter), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(b->d_all, b->all_tiles.data(), b->all_tiles.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
-    for (auto &g : b->groups)
+    ch->d_all = (int2 *)(ch->buf + o);
+    o += align256(ch->all_tiles.size() * sizeof(int2));
+    for (auto &g : ch->groups) { g.d = (int2 *)(ch->buf + o); o += align256(g.tiles.size() * sizeof(int2)); }
+    HIPCHK(hipMemcpyAsync(ch->ddf, hd.data(), (size_t)nf * sizeof(plom_dfilter), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(ch->d_all, ch->all_tiles.data(), ch->all_tiles.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    for (auto &g : ch->groups)
         HIPCHK(hipMemcpyAsync(g.d, g.tiles.data(), g.tiles.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    *out = b;
     return PLOM_OK;
 }
 
-/* the launch sequence of a batch: reset, every interval, final resampling */
-static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing)
+static void chain_reset(batch_chain *ch, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reset, dim3((unsigned)ch->fs.size()), dim3(256), 0, st, ch->ddf, (int)ch->fs.size());
+}
+
+static int chain_propagate(batch_chain *ch, int s, hipStream_t st)
+{
+    int launches = 0;
+    for (auto &g : ch->groups) {
+        const int nb = prefix_len(ch, g.tiles, s);
+        if (nb > 0) { g_models[g.model].launch_prop(g.prec, g.variant, nb, st, ch->ddf, g.d); launches++; }
+    }
+    return launches;
+}
+
+static void chain_resample(batch_chain *ch, int s, hipStream_t st)
+{
+    const int nb = prefix_len(ch, ch->all_tiles, s);
+    int nfa = 0;
+    while (nfa < (int)ch->fs.size() && ch->fs[(size_t)nfa]->desc.N > s) nfa++;
+    enqueue_resample(st, ch->ddf, (const int2 *)ch->d_all, nb, nullptr, nfa);
+}
+
+static void chain_final(plom_ctx *c, batch_chain *ch, hipStream_t st)
+{
+    size_t first = 0;
+    for (int r = 0; r < (int)ch->fs.size(); r++) {
+        plom_filter *f = ch->fs[(size_t)r];
+        while (ch->all_tiles[first].x != r) first++;
+        if (!needs_final(f)) continue;
+        if (f->desc.precision == PLOM_FP32)
+            hipLaunchKernelGGL((k_final<float>), dim3(f->h.T), dim3(TILE), 0, st, ch->ddf, ch->d_all + first, f->info.size_proj);
+        else
+            hipLaunchKernelGGL((k_final<double>), dim3(f->h.T), dim3(TILE), 0, st, ch->ddf, ch->d_all + first, f->info.size_proj);
+    }
+    (void)c;
+}
+
+/* streams and events of the extra lanes (lane 0 is the context stream) */
+static int ctx_lanes(plom_ctx *c, int lanes)
+{
+    if (!c->fork_ev) HIPCHK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+    while ((int)c->lane.size() < lanes) {
+        hipStream_t st;
+        hipEvent_t ev;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        c->lane.push_back(st);
+        c->join_ev.push_back(ev);
+    }
+    return PLOM_OK;
+}
+
+/* the launch sequence of a batch (reset, every interval, final resampling); with
+ * rounds >= 0 (the autotuner) only `rounds` launch rounds whose grids are those of
+ * intervals spread evenly over the pass (a round costs what its active tiles cost, the
+ * interval the device is at does not matter) and no final resampling; with timing (or one
+ * lane) every chain runs on the context stream, interleaved round by round, between
+ * per-round events */
+static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing, int rounds)
 {
     (void)hipGetLastError(); /* a not-ready query or an unused event earlier must not be reported as a launch failure */
-    hipLaunchKernelGGL(k_reset, dim3(b->nf), dim3(256), 0, c->stream, b->ddf, b->nf);
-    int k = 0;
-    for (int s = 0; s < b->maxN; s++) {
-        if (timing && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
-        for (auto &g : b->groups) {
-            const int nb = prefix_len(b, g.tiles, s);
-            if (nb > 0) g_models[g.model].launch_prop(g.prec, g.variant, nb, c->stream, b->ddf, g.d);
+    const int maxN = b->fs[0]->desc.N;
+    const int R = rounds >= 0 && rounds < maxN ? rounds : maxN;
+    const int stride = R < maxN ? maxN / R : 1;
+    c->kprop_launches = 0;
+    if (timing || b->lanes == 1) {
+        for (auto &ch : b->chains) chain_reset(&ch, c->stream);
+        int k = 0;
+        for (int i = 0; i < R; i++) {
+            const int s = i * stride;
+            if (timing && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
+            for (auto &ch : b->chains)
+                if (s < ch.maxN) c->kprop_launches += chain_propagate(&ch, s, c->stream);
+            if (timing && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
+            for (auto &ch : b->chains)
+                if (s < ch.maxN) chain_resample(&ch, s, c->stream);
         }
         if (timing && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
-        const int nb = prefix_len(b, b->all_tiles, s);
-        int nfa = 0;
-        while (nfa < b->nf && b->fs[(size_t)nfa]->desc.N > s) nfa++;
-        enqueue_resample(c->stream, b->ddf, (const int2 *)b->d_all, nb, nullptr, nfa);
-    }
-    if (timing && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
-    c->kev_used = k;
-    size_t first = 0;
-    for (int r = 0; r < b->nf; r++) {
-        plom_filter *f = b->fs[(size_t)r];
-        while (b->all_tiles[first].x != r) first++;
-        if (needs_final(f)) enqueue_final(c, f, b->d_all + first, b->ddf, f->h.T);
+        c->kev_used = timing ? k : 0;
+        if (R == maxN)
+            for (auto &ch : b->chains) chain_final(c, &ch, c->stream);
+    } else {
+        int rc = ctx_lanes(c, b->lanes);
+        if (rc != PLOM_OK) return rc;
+        HIPCHK(hipEventRecord(c->fork_ev, c->stream));
+        for (int l = 1; l < b->lanes; l++) HIPCHK(hipStreamWaitEvent(c->lane[(size_t)l], c->fork_ev, 0));
+        for (int l = 0; l < b->lanes; l++) {
+            batch_chain *ch = &b->chains[(size_t)l];
+            hipStream_t st = l == 0 ? c->stream : c->lane[(size_t)l];
+            chain_reset(ch, st);
+            for (int i = 0; i < R && i * stride < ch->maxN; i++) {
+                chain_propagate(ch, i * stride, st);
+                chain_resample(ch, i * stride, st);
+            }
+            if (R == maxN) chain_final(c, ch, st);
+        }
+        for (int l = 1; l < b->lanes; l++) {
+            HIPCHK(hipEventRecord(c->join_ev[(size_t)l], c->lane[(size_t)l]));
+            HIPCHK(hipStreamWaitEvent(c->stream, c->join_ev[(size_t)l], 0));
+        }
+        c->kev_used = 0;
     }
     HIPCHK(hipGetLastError());
     return PLOM_OK;
@@ -970,7 +1067,7 @@ static int batch_launch(plom_ctx *c, plom_batch *b, int use_graph)
         if (!b->graph) {
             hipGraph_t g;
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int rc = batch_enqueue(c, b, 0);
+            int rc = batch_enqueue(c, b, 0, -1);
             hipError_t e = hipStreamEndCapture(c->stream, &g);
             if (rc != PLOM_OK) return rc;
             if (e != hipSuccess) return set_err(PLOM_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
@@ -980,7 +1077,7 @@ static int batch_launch(plom_ctx *c, plom_batch *b, int use_graph)
         c->kev_used = 0;
         HIPCHK(hipGraphLaunch(b->graph, c->stream));
     } else {
-        int rc = batch_enqueue(c, b, c->ktiming);
+        int rc = batch_enqueue(c, b, c->ktiming, -1);
         if (rc != PLOM_OK) return rc;
     }
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
@@ -992,19 +1089,44 @@ static int batch_launch(plom_ctx *c, plom_batch *b, int use_graph)
  * (plom_filter_read reads results through the filter's own buffers) */
 static int batch_writeback(plom_ctx *c, plom_batch *b)
 {
-    for (int r = 0; r < b->nf; r++)
-        HIPCHK(hipMemcpyAsync(b->fs[(size_t)r]->df, b->ddf + r, sizeof(plom_dfilter), hipMemcpyDeviceToDevice, c->stream));
+    for (auto &ch : b->chains)
+        for (size_t r = 0; r < ch.fs.size(); r++)
+            HIPCHK(hipMemcpyAsync(ch.fs[r]->df, ch.ddf + r, sizeof(plom_dfilter), hipMemcpyDeviceToDevice, c->stream));
+    return PLOM_OK;
+}
+
+static int batch_free_chains(plom_ctx *c, plom_batch *b)
+{
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (b->graph) HIPCHK(hipGraphExecDestroy(b->graph));
+    b->graph = nullptr;
+    for (auto &ch : b->chains)
+        if (ch.buf) HIPCHK(hipFree(ch.buf));
+    b->chains.clear();
+    return PLOM_OK;
+}
+
+static int batch_set_lanes(plom_ctx *c, plom_batch *b, int lanes)
+{
+    int rc = batch_free_chains(c, b);
+    if (rc != PLOM_OK) return rc;
+    if (lanes > b->nf) lanes = b->nf;
+    b->lanes = lanes;
+    b->chains.resize((size_t)lanes);
+    for (int r = 0; r < b->nf; r++) b->chains[(size_t)(r % lanes)].fs.push_back(b->fs[(size_t)r]);
+    for (auto &ch : b->chains) {
+        rc = chain_build(c, &ch);
+        if (rc != PLOM_OK) return rc;
+    }
     return PLOM_OK;
 }
 
 static int batch_destroy(plom_ctx *c, plom_batch *b)
 {
     if (!b) return PLOM_OK;
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (b->graph) HIPCHK(hipGraphExecDestroy(b->graph));
-    HIPCHK(hipFree(b->buf));
+    int rc = batch_free_chains(c, b);
     delete b;
-    return PLOM_OK;
+    return rc;
 }
 
 extern "C" int plom_batch_create(plom_ctx *c, int nf, plom_filter **fs, plom_batch **out)
@@ -1013,7 +1135,53 @@ extern "C" int plom_batch_create(plom_ctx *c, int nf, plom_filter **fs, plom_bat
     for (int i = 0; i < nf; i++)
         if (!fs[i]) return set_err(PLOM_ERR_ARG, "null filter %d", i);
     HIPCHK(hipSetDevice(c->device));
-    return batch_create(c, nf, fs, out);
+    plom_batch *b = new plom_batch();
+    b->nf = nf;
+    b->graph = nullptr;
+    b->fs.assign(fs, fs + nf);
+    std::stable_sort(b->fs.begin(), b->fs.end(), [](plom_filter *x, plom_filter *y) { return x->desc.N > y->desc.N; });
+    int rc = batch_set_lanes(c, b, 1);
+    if (rc != PLOM_OK) { (void)batch_destroy(c, b); return rc; }
+    *out = b;
+    return PLOM_OK;
+}
+
+extern "C" int plom_batch_configure(plom_ctx *c, plom_batch *b, int lanes)
+{
+    if (!c || !b) return set_err(PLOM_ERR_ARG, "null argument");
+    if (lanes < 1 || lanes > PLOM_MAX_LANES) return set_err(PLOM_ERR_ARG, "lanes=%d outside [1, %d]", lanes, PLOM_MAX_LANES);
+    HIPCHK(hipSetDevice(c->device));
+    return batch_set_lanes(c, b, lanes);
+}
+
+extern "C" int plom_batch_autotune(plom_ctx *c, plom_batch *b, int rounds, int32_t *lanes_out, double *ms_out)
+{
+    if (!c || !b) return set_err(PLOM_ERR_ARG, "null argument");
+    if (rounds < 1) return set_err(PLOM_ERR_ARG, "rounds=%d < 1", rounds);
+    HIPCHK(hipSetDevice(c->device));
+    static const int cand[4] = {1, 2, 4, 8};
+    int best = 1;
+    double best_ms = 0.0;
+    for (int i = 0; i < 4; i++) {
+        if (ms_out) ms_out[i] = 0.0;
+        const int lanes = cand[i];
+        if (lanes > b->nf) continue;
+        int rc = batch_set_lanes(c, b, lanes);
+        if (rc != PLOM_OK) return rc;
+        rc = batch_enqueue(c, b, 0, rounds);           /* warm: first use of the lanes */
+        if (rc != PLOM_OK) return rc;
+        HIPCHK(hipEventRecord(c->ev[2], c->stream));
+        rc = batch_enqueue(c, b, 0, rounds);
+        if (rc != PLOM_OK) return rc;
+        HIPCHK(hipEventRecord(c->ev[3], c->stream));
+        HIPCHK(hipEventSynchronize(c->ev[3]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+        if (ms_out) ms_out[i] = ms;
+        if (i == 0 || ms < best_ms) { best = lanes; best_ms = ms; }
+    }
+    if (lanes_out) *lanes_out = best;
+    return batch_set_lanes(c, b, best);
 }
 
 extern "C" int plom_batch_launch(plom_ctx *c, plom_batch *b, int use_graph)
@@ -1439,7 +1607,8 @@ extern "C" int plom_ctx_set_kernel_timing(plom_ctx *c, int on)
     return PLOM_OK;
 }
 
-extern "C" int plom_last_timing(plom_ctx *c, double *total_ms, double *prop_ms, double *scan_ms, int32_t *n_intervals)
+extern "C" int plom_last_timing(plom_ctx *c, double *total_ms, double *prop_ms, double *scan_ms, int32_t *n_intervals,
+                                int32_t *n_prop_launches)
 {
     if (!c) return set_err(PLOM_ERR_ARG, "null context");
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1462,6 +1631,7 @@ extern "C" int plom_last_timing(plom_ctx *c, double *total_ms, double *prop_ms, 
     if (prop_ms) *prop_ms = c->kev_used ? p : -1.0;
     if (scan_ms) *scan_ms = c->kev_used ? q : -1.0;
     if (n_intervals) *n_intervals = nint;
+    if (n_prop_launches) *n_prop_launches = c->kev_used ? c->kprop_launches : 0;
     return PLOM_OK;
 }
 

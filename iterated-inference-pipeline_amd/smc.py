@@ -54,7 +54,7 @@ EXPORTS = [
     "plom_filter_run", "plom_filter_read", "plom_filter_destroy", "plom_filter_device_bytes",
     "plom_batch_run", "plom_sync", "plom_ctx_set_kernel_timing", "plom_last_timing", "plom_selftest_math",
     "plom_selftest_normals", "plom_selftest_copy", "plom_batch_create", "plom_batch_launch",
-    "plom_batch_destroy", "plom_server_create", "plom_server_submit", "plom_server_step",
+    "plom_batch_destroy", "plom_batch_configure", "plom_batch_autotune", "plom_server_create", "plom_server_submit", "plom_server_step",
     "plom_server_status", "plom_server_done", "plom_server_result", "plom_server_destroy",
 ]
 
@@ -98,13 +98,15 @@ def lib():
     L.plom_batch_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.c_int]
     L.plom_sync.argtypes = [C.c_void_p]
     L.plom_ctx_set_kernel_timing.argtypes = [C.c_void_p, C.c_int]
-    L.plom_last_timing.argtypes = [C.c_void_p, dp, dp, dp, C.POINTER(C.c_int32)]
+    L.plom_last_timing.argtypes = [C.c_void_p, dp, dp, dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.plom_selftest_math.argtypes = [C.c_void_p, C.c_int, dp, dp]
     L.plom_selftest_normals.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
     L.plom_selftest_copy.argtypes = [C.c_void_p, C.c_int64, C.c_int, dp]
     L.plom_batch_create.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
     L.plom_batch_launch.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
     L.plom_batch_destroy.argtypes = [C.c_void_p, C.c_void_p]
+    L.plom_batch_configure.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+    L.plom_batch_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
     L.plom_server_create.argtypes = [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]
     L.plom_server_submit.argtypes = [C.c_void_p, C.POINTER(FilterDesc), C.POINTER(FilterInputs), C.POINTER(C.c_uint64)]
     L.plom_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -154,11 +156,11 @@ class Context:
         check(lib().plom_ctx_set_kernel_timing(self.h, 1 if on else 0))
 
     def last_timing(self):
-        """(total_ms, propagate_ms, scan_ms, n_intervals) of the last run."""
+        """(total_ms, propagate_ms, scan_ms, n_intervals, n_propagate_launches) of the last run."""
         t, p, q = C.c_double(), C.c_double(), C.c_double()
-        n = C.c_int32()
-        check(lib().plom_last_timing(self.h, C.byref(t), C.byref(p), C.byref(q), C.byref(n)))
-        return t.value, p.value, q.value, n.value
+        n, nl = C.c_int32(), C.c_int32()
+        check(lib().plom_last_timing(self.h, C.byref(t), C.byref(p), C.byref(q), C.byref(n), C.byref(nl)))
+        return t.value, p.value, q.value, n.value, nl.value
 
 
 # ---------------------------------------------------------------------- models
@@ -336,6 +338,17 @@ class Batch:
 
     def launch(self, use_graph=False):
         check(lib().plom_batch_launch(self.ctx.h, self.h, 1 if use_graph else 0))
+
+    def configure(self, lanes):
+        check(lib().plom_batch_configure(self.ctx.h, self.h, int(lanes)))
+
+    def autotune(self, rounds=16):
+        """Pick the number of lanes (1, 2, 4, 8) by timing the first `rounds` intervals;
+        returns (chosen lanes, {lanes: ms})."""
+        lanes = C.c_int32()
+        ms = (C.c_double * 4)()
+        check(lib().plom_batch_autotune(self.ctx.h, self.h, int(rounds), C.byref(lanes), ms))
+        return lanes.value, {k: ms[i] for i, k in enumerate((1, 2, 4, 8)) if ms[i] > 0}
 
     def close(self):
         if self.h:

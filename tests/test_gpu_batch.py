@@ -101,3 +101,43 @@ def test_batch_graph_replays(ctx):
     b.close()
     for f in fs:
         f.close()
+
+
+@pytest.mark.parametrize("lanes", [2, 3, 8])
+def test_batch_lanes_equal_one_lane(ctx, lanes):
+    """Chains on their own streams (forked and joined, eagerly and inside a captured
+    graph, replayed twice) give the one-lane results, final clouds and ancestry included."""
+    fs = [make(ctx, *c)[0] for c in CASES]
+    smc.batch_run(ctx, fs)
+    ref = [f.read(ancestry=True, final_state=True) for f in fs]
+    b = smc.Batch(ctx, fs)
+    b.configure(lanes)
+    for use_graph in (False, True, True):
+        print("lanes", lanes, "graph", use_graph, flush=True)
+        b.launch(use_graph=use_graph)
+        got = [f.read(ancestry=True, final_state=True) for f in fs]
+        for r, g in zip(ref, got):
+            assert r.loglik == g.loglik
+            np.testing.assert_array_equal(r.ancestry, g.ancestry)
+            np.testing.assert_array_equal(r.final_state, g.final_state)
+            np.testing.assert_array_equal(r.loglik_n, g.loglik_n)
+    b.close()
+    for f in fs:
+        f.close()
+
+
+def test_batch_autotune_keeps_results(ctx):
+    """The autotuner times 1/2/4/8 lanes on launch rounds spread over the pass, keeps one,
+    and the plan then runs to the same results."""
+    fs = [make(ctx, *c)[0] for c in CASES]
+    smc.batch_run(ctx, fs)
+    ref = [f.read().loglik for f in fs]
+    b = smc.Batch(ctx, fs)
+    lanes, ms = b.autotune(rounds=4)
+    print("autotune", lanes, ms, flush=True)
+    assert lanes in (1, 2, 4) and set(ms) == {1, 2, 4} and all(v > 0 for v in ms.values())
+    b.launch(use_graph=True)
+    assert [f.read().loglik for f in fs] == ref
+    b.close()
+    for f in fs:
+        f.close()

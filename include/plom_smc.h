@@ -55,6 +55,10 @@ extern "C" {
 #define PLOM_FLAG_NO_FILTER 1u        /* -t/--no_filter: propagate only (C/smc/main.c:177) */
 #define PLOM_FLAG_STORE_ANCESTRY 2u   /* keep select[n][j] for every n (needed by --full) */
 #define PLOM_FLAG_FINAL_STATE 4u      /* keep the resampled final cloud */
+#define PLOM_FLAG_LINEAGE 8u          /* sample one particle of the last interval by its weight and
+                                         return its whole path (pmcmc --full,
+                                         sample_traj_and_print, C/core/print.c:715-784); keeps the
+                                         ancestry on the device */
 
 #define PLOM_MAX_J (1 << 21)          /* particles per filter */
 
@@ -98,6 +102,9 @@ typedef struct {
     int32_t *ancestry;   /* optional [N*J]: select[n][j]; needs PLOM_FLAG_STORE_ANCESTRY */
     double *final_state; /* optional [size_proj*J] SoA resampled final cloud (PLOM_FLAG_FINAL_STATE) */
     double *final_logw;  /* optional [J] log-weights of the last interval, before resampling */
+    double *lineage;     /* optional [(N+1)*size_proj]: the sampled path, row n = its state at
+                            times[n] (row 0 = x0); needs PLOM_FLAG_LINEAGE */
+    int32_t *lineage_slots; /* optional [N]: the particle index of that path at each interval */
 } plom_filter_outputs;
 
 typedef struct plom_ctx plom_ctx;

@@ -171,6 +171,15 @@ PLOM_HD void plom_normal_pair64(plom_nstream64 *ns, double *z0, double *z1)
     plom_bm_pair(x, y, ns->tab, z0, z1);
 }
 
+/* the uniform that picks the final particle of a lineage, as the numerator 2x+1 of
+ * U = (2x+1)/2^33 */
+PLOM_HD uint64_t plom_lineage_num(uint32_t seed, uint32_t stream)
+{
+    uint32_t w[4];
+    plom_draw128(seed, stream, 0u, 0u, PLOM_PURPOSE_FULL, w);
+    return 2ull * (uint64_t)w[0] + 1ull;
+}
+
 /* the resampling uniform of interval n as the integer numerator 2x+1 of U = (2x+1)/2^33 */
 PLOM_HD uint64_t plom_resample_num(uint32_t seed, uint32_t stream, uint32_t n)
 {

@@ -90,6 +90,8 @@ struct plom_dfilter {
     double *llike_n, *ess_n;
     int32_t *rflag;     /* rflag[n] = 1 when interval n ended with a resampling */
     int32_t *ancestry;  /* [N][J] or null */
+    int32_t *lin_slots; /* [N] or null: the sampled lineage's particle index per interval */
+    double *lin_state;  /* [(N+1)][S] or null: its states */
 };
 
 /* ------------------------------------------------------------------ block helpers */
@@ -135,15 +137,12 @@ __device__ inline uint64_t scan256(uint64_t v, uint64_t *lds)
     return v + off;
 }
 
-/* systematic resampling of interval n: ancestor of output slot k */
-__device__ inline int select_ancestor(const plom_dfilter *f, int k, int n)
+/* first particle i with C_i * J2 >= Rk in the fixed-point cumulative weights of the
+ * interval whose scan is in f->cum / f->tincl */
+__device__ inline int select_first(const plom_dfilter *f, plom_u128 Rk, uint64_t J2)
 {
     const int T = f->T;
     const uint64_t *tincl = f->tincl;
-    const uint64_t Q = tincl[T - 1];
-    const uint64_t unum = plom_resample_num(f->seed, f->stream, (uint32_t)n);
-    const plom_u128 Rk = (plom_u128)(((uint64_t)k << 33) + unum) * (plom_u128)Q;
-    const uint64_t J2 = (uint64_t)f->J << 33;
     int lo = 0, hi = T - 1;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
@@ -157,6 +156,16 @@ __device__ inline int select_ancestor(const plom_dfilter *f, int k, int n)
         if ((plom_u128)(P + c[mid]) * J2 >= Rk) b = mid; else a = mid + 1;
     }
     return lo * TILE + a;
+}
+
+/* systematic resampling of interval n: ancestor of output slot k, the first i with
+ * C_i J 2^33 >= (k 2^33 + 2x + 1) Q */
+__device__ inline int select_ancestor(const plom_dfilter *f, int k, int n)
+{
+    const uint64_t Q = f->tincl[f->T - 1];
+    const uint64_t unum = plom_resample_num(f->seed, f->stream, (uint32_t)n);
+    const plom_u128 Rk = (plom_u128)(((uint64_t)k << 33) + unum) * (plom_u128)Q;
+    return select_first(f, Rk, (uint64_t)f->J << 33);
 }
 
 /* global reduction of the tile statistics by the last block of interval n */
@@ -366,6 +375,61 @@ __global__ __launch_bounds__(TILE) void k_final(plom_dfilter *filters, const int
     for (int k = 0; k < S; k++) Xout[(size_t)k * Jp + j] = Xin[(size_t)k * Jp + a];
 }
 
+/* The lineage of one filter (one block of one wave; lane 0 works): the final particle by
+ * inverse CDF on the last interval's cumulative weights (equal weights when that interval
+ * had no data, was not filtered or every particle failed), its ancestors by the stored
+ * ancestry, then its states regenerated from x0 with the same counter-based noise the
+ * filter drew, interval by interval: s[N-1] = first i with C_i 2^33 >= (2x+1) Q;
+ * s[n-1] = ancestry[n-1][s[n]]; state[n+1] = interval_n(state[n], noise(n, s[n])). */
+template <class M, typename R, int V>
+__global__ __launch_bounds__(64) void k_lineage(plom_dfilter *f)
+{
+    __shared__ double s_bm[PLOM_BM_TAB_SIZE];
+    if constexpr (sizeof(R) == 8) {
+        for (int i = threadIdx.x; i < PLOM_BM_TAB_SIZE; i += 64) s_bm[i] = plom_bm_tab[i];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const int N = f->N, J = f->J;
+    if (N == 0 || !f->lin_slots) return;
+    const uint64_t unum = plom_lineage_num(f->seed, f->stream);
+    int j;
+    if (!(f->flags & PLOM_FLAG_NO_FILTER) && f->nonan[N - 1] > 0 && f->rflag[N - 1]) {
+        const uint64_t Q = f->tincl[f->T - 1];
+        j = select_first(f, (plom_u128)unum * (plom_u128)Q, 1ull << 33);
+    } else {
+        j = (int)((unum * (uint64_t)J + (1ull << 33) - 1ull) >> 33) - 1;
+    }
+    j = j < 0 ? 0 : (j >= J ? J - 1 : j);
+    f->lin_slots[N - 1] = j;
+    for (int n = N - 1; n >= 1; n--) {
+        j = f->ancestry[(size_t)(n - 1) * J + j];
+        f->lin_slots[n - 1] = j;
+    }
+    R X[M::S];
+    const R *x0 = (const R *)f->x0;
+#pragma unroll
+    for (int k = 0; k < M::S; k++) {
+        X[k] = x0[k];
+        f->lin_state[k] = (double)X[k];
+    }
+    for (int n = 0; n < N; n++) {
+        const double t0 = f->times[n], t1 = f->times[n + 1];
+        const uint32_t jn = (uint32_t)f->lin_slots[n];
+        if constexpr (sizeof(R) == 8) {
+            plom_nstream64 ns;
+            plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, jn, s_bm);
+            M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt);
+        } else {
+            plom_nstream32 ns;
+            plom_nstream32_init(&ns, f->seed, f->stream, (uint32_t)n, jn);
+            M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt);
+        }
+#pragma unroll
+        for (int k = 0; k < M::S; k++) f->lin_state[(size_t)(n + 1) * M::S + k] = (double)X[k];
+    }
+}
+
 __global__ void k_selftest_math(int n, const double *in, double *out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -405,6 +469,7 @@ struct model_entry {
     const char *name;
     plom_model_info info;
     void (*launch_prop)(int prec, int variant, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles);
+    void (*launch_lineage)(int prec, int variant, hipStream_t s, plom_dfilter *df);
 };
 
 template <class M, typename R>
@@ -425,11 +490,29 @@ static void launch_prop(int prec, int variant, int nblocks, hipStream_t s, plom_
     else launch_prop_r<M, double>(variant, nblocks, s, df, tiles);
 }
 
+template <class M, typename R>
+static void launch_lineage_r(int variant, hipStream_t s, plom_dfilter *df)
+{
+    switch (variant) {
+    case 0: hipLaunchKernelGGL((k_lineage<M, R, 0>), dim3(1), dim3(64), 0, s, df); break;
+    case 1: hipLaunchKernelGGL((k_lineage<M, R, 1>), dim3(1), dim3(64), 0, s, df); break;
+    case 2: hipLaunchKernelGGL((k_lineage<M, R, 2>), dim3(1), dim3(64), 0, s, df); break;
+    default: hipLaunchKernelGGL((k_lineage<M, R, 3>), dim3(1), dim3(64), 0, s, df); break;
+    }
+}
+
+template <class M>
+static void launch_lineage(int prec, int variant, hipStream_t s, plom_dfilter *df)
+{
+    if (prec == PLOM_FP32) launch_lineage_r<M, float>(variant, s, df);
+    else launch_lineage_r<M, double>(variant, s, df);
+}
+
 #define MODEL_ENTRY(lname, UNAME)                                                               \
     { #lname,                                                                                   \
       { UNAME##_SIZE_PROJ, UNAME##_N_TS, UNAME##_N_CAC, UNAME##_N_PAR_SV, UNAME##_INC_OFFSET,   \
         UNAME##_N_TS_INC_UNIQUE, UNAME##_PV_SIZE, UNAME##_N_DEM, UNAME##_N_ENV },               \
-      launch_prop<plom_model_##lname> }
+      launch_prop<plom_model_##lname>, launch_lineage<plom_model_##lname> }
 
 static const model_entry g_models[] = {
     MODEL_ENTRY(sir, PLOM_SIR),
@@ -510,6 +593,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
  * server creates it) and starts from zeroed counters without memset calls. */
 struct layout {
     size_t off_X0, off_X1, off_logw, off_cum, off_tmax, off_tsum, off_tsum2, off_ttot, off_tincl, off_anc,
+        off_lslots, off_lstate,
         off_img, off_df, off_cnt, off_scal, off_naf, off_llike_n, off_ess_n, off_rflag, off_nonan, off_times,
         off_data, off_pv, off_pvf, off_x0, total;
 };
@@ -531,7 +615,9 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     PUT(off_tsum2, T * 8);
     PUT(off_ttot, T * 8);
     PUT(off_tincl, T * 8);
-    PUT(off_anc, (d->flags & PLOM_FLAG_STORE_ANCESTRY) ? N * J * 4 : 0);
+    PUT(off_anc, (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_LINEAGE)) ? N * J * 4 : 0);
+    PUT(off_lslots, (d->flags & PLOM_FLAG_LINEAGE) ? N * 4 : 0);
+    PUT(off_lstate, (d->flags & PLOM_FLAG_LINEAGE) ? (N + 1) * S * 8 : 0);
     L->off_img = o;
     PUT(off_df, sizeof(plom_dfilter));
     PUT(off_cnt, 2 * 4);
@@ -711,7 +797,9 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     h->n_all_fail = (int32_t *)(b + L.off_naf);
     h->llike_n = (double *)(b + L.off_llike_n); h->ess_n = (double *)(b + L.off_ess_n);
     h->rflag = (int32_t *)(b + L.off_rflag);
-    h->ancestry = (d->flags & PLOM_FLAG_STORE_ANCESTRY) ? (int32_t *)(b + L.off_anc) : nullptr;
+    h->ancestry = (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_LINEAGE)) ? (int32_t *)(b + L.off_anc) : nullptr;
+    h->lin_slots = (d->flags & PLOM_FLAG_LINEAGE) ? (int32_t *)(b + L.off_lslots) : nullptr;
+    h->lin_state = (d->flags & PLOM_FLAG_LINEAGE) ? (double *)(b + L.off_lstate) : nullptr;
     f->df = (plom_dfilter *)(b + L.off_df);
 
     /* host image: descriptor, zeroed counters and outputs, inputs */
@@ -814,6 +902,8 @@ extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
     if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
     c->kev_used = k;
     if (needs_final(f)) enqueue_final(c, f, nullptr, f->df, T);
+    if (f->desc.N > 0 && (f->desc.flags & PLOM_FLAG_LINEAGE))
+        me->launch_lineage(f->desc.precision, f->variant, c->stream, f->df);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
     f->has_run = 1;
@@ -837,6 +927,12 @@ static int filter_read_on(plom_ctx *c, plom_filter *f, plom_filter_outputs *o, h
     if (o->ess_n && N) HIPCHK(hipMemcpyAsync(o->ess_n, h->ess_n, N * 8, hipMemcpyDeviceToHost, s));
     if (o->ancestry && N) HIPCHK(hipMemcpyAsync(o->ancestry, h->ancestry, N * J * 4, hipMemcpyDeviceToHost, s));
     if (o->final_logw) HIPCHK(hipMemcpyAsync(o->final_logw, h->logw, J * 8, hipMemcpyDeviceToHost, s));
+    if ((o->lineage || o->lineage_slots) && N) {
+        if (!h->lin_state) return set_err(PLOM_ERR_STATE, "lineage requested but PLOM_FLAG_LINEAGE not set");
+        if (o->lineage)
+            HIPCHK(hipMemcpyAsync(o->lineage, h->lin_state, (N + 1) * (size_t)f->info.size_proj * 8, hipMemcpyDeviceToHost, s));
+        if (o->lineage_slots) HIPCHK(hipMemcpyAsync(o->lineage_slots, h->lin_slots, N * 4, hipMemcpyDeviceToHost, s));
+    }
     std::vector<float> tmp;
     if (o->final_state) {
         const int S = f->info.size_proj;
@@ -982,6 +1078,8 @@ static void chain_final(plom_ctx *c, batch_chain *ch, hipStream_t st)
     for (int r = 0; r < (int)ch->fs.size(); r++) {
         plom_filter *f = ch->fs[(size_t)r];
         while (ch->all_tiles[first].x != r) first++;
+        if (f->desc.N > 0 && (f->desc.flags & PLOM_FLAG_LINEAGE))
+            g_models[f->desc.model_id].launch_lineage(f->desc.precision, f->variant, st, ch->ddf + r);
         if (!needs_final(f)) continue;
         if (f->desc.precision == PLOM_FP32)
             hipLaunchKernelGGL((k_final<float>), dim3(f->h.T), dim3(TILE), 0, st, ch->ddf, ch->d_all + first, f->info.size_proj);
@@ -1406,7 +1504,7 @@ extern "C" int plom_server_submit(plom_server *S, const plom_filter_desc *d, con
     int rc = check_desc(d, &mi);
     if (rc != PLOM_OK) return rc;
     if (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_FINAL_STATE))
-        return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods (no ancestry / final state)");
+        return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods and lineages (no ancestry / final state)");
     if (!in->times || !in->pv || !in->x0 || (d->N > 0 && !in->data)) return set_err(PLOM_ERR_ARG, "null input array");
     for (int n = 0; n < d->N; n++)
         if (!(in->times[n + 1] > in->times[n])) return set_err(PLOM_ERR_ARG, "times must increase (n=%d)", n);
@@ -1501,6 +1599,8 @@ extern "C" int plom_server_step(plom_server *S, int32_t *n_active, int32_t *n_qu
             r->progress++;
             if (r->progress >= r->desc.N) {
                 /* the slot descriptor carries the interval counter: copy it back, mark done */
+                if (r->desc.N > 0 && (r->desc.flags & PLOM_FLAG_LINEAGE))
+                    g_models[r->desc.model_id].launch_lineage(r->desc.precision, r->f->variant, c->stream, S->d_slots + r->slot);
                 HIPCHK(hipMemcpyAsync(r->f->df, S->d_slots + r->slot, sizeof(plom_dfilter), hipMemcpyDeviceToDevice, c->stream));
                 HIPCHK(hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming));
                 HIPCHK(hipEventRecord(r->done_ev, c->stream));
@@ -1559,7 +1659,7 @@ extern "C" int plom_server_result(plom_server *S, uint64_t id, plom_filter_outpu
 {
     if (!S || !o) return set_err(PLOM_ERR_ARG, "null argument");
     if (o->ancestry || o->final_state || o->final_logw)
-        return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods (no ancestry / final state)");
+        return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods and lineages (no ancestry / final state)");
     std::vector<srv_request *> *where = nullptr;
     srv_request *r = srv_find(S, id, &where);
     if (!r) return set_err(PLOM_ERR_ARG, "unknown request %llu", (unsigned long long)id);

@@ -20,7 +20,7 @@ from .codegen import load_model, BUILTIN_MODELS, context_data, load_json
 
 PLOM_FP64, PLOM_FP32 = 0, 1
 NO_DEM_STO, NO_ENV_STO, NO_DRIFT = 1, 2, 4
-FLAG_NO_FILTER, FLAG_STORE_ANCESTRY, FLAG_FINAL_STATE = 1, 2, 4
+FLAG_NO_FILTER, FLAG_STORE_ANCESTRY, FLAG_FINAL_STATE, FLAG_LINEAGE = 1, 2, 4, 8
 MAX_J = 1 << 21
 
 
@@ -45,7 +45,8 @@ class FilterOutputs(C.Structure):
     _fields_ = [("loglik", C.c_double), ("n_all_fail", C.c_int32),
                 ("loglik_n", C.POINTER(C.c_double)), ("ess_n", C.POINTER(C.c_double)),
                 ("ancestry", C.POINTER(C.c_int32)), ("final_state", C.POINTER(C.c_double)),
-                ("final_logw", C.POINTER(C.c_double))]
+                ("final_logw", C.POINTER(C.c_double)), ("lineage", C.POINTER(C.c_double)),
+                ("lineage_slots", C.POINTER(C.c_int32))]
 
 
 EXPORTS = [
@@ -255,7 +256,8 @@ def round_dt(DT):
 
 
 class FilterResult:
-    def __init__(self, loglik, n_all_fail, loglik_n, ess_n, ancestry=None, final_state=None, final_logw=None):
+    def __init__(self, loglik, n_all_fail, loglik_n, ess_n, ancestry=None, final_state=None, final_logw=None,
+                 lineage=None, lineage_slots=None):
         self.loglik = loglik
         self.n_all_fail = n_all_fail
         self.loglik_n = loglik_n
@@ -263,6 +265,8 @@ class FilterResult:
         self.ancestry = ancestry
         self.final_state = final_state
         self.final_logw = final_logw
+        self.lineage = lineage
+        self.lineage_slots = lineage_slots
 
 
 def _desc(model, J, N, precision, noises_off, flags, seed, stream, dt, like_min):
@@ -302,16 +306,18 @@ class Filter:
         check(lib().plom_filter_run(self.ctx.h, self.h))
         return self
 
-    def read(self, ancestry=False, final_state=False, final_logw=False):
+    def read(self, ancestry=False, final_state=False, final_logw=False, lineage=False):
         N, J = self.N, self.J
         ll_n = np.zeros(N)
         ess = np.zeros(N)
         anc = np.zeros((N, J), dtype=np.int32) if ancestry else None
         fs = np.zeros((self.model.S, J)) if final_state else None
         fw = np.zeros(J) if final_logw else None
-        out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), iptr(anc), dptr(fs), dptr(fw))
+        lin = np.zeros((N + 1, self.model.S)) if lineage else None
+        lsl = np.zeros(N, dtype=np.int32) if lineage else None
+        out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), iptr(anc), dptr(fs), dptr(fw), dptr(lin), iptr(lsl))
         check(lib().plom_filter_read(self.ctx.h, self.h, C.byref(out)))
-        return FilterResult(out.loglik, out.n_all_fail, ll_n, ess, anc, fs, fw)
+        return FilterResult(out.loglik, out.n_all_fail, ll_n, ess, anc, fs, fw, lin, lsl)
 
     def close(self):
         if self.h:
@@ -368,17 +374,17 @@ class Server:
         self.N = {}
 
     def submit(self, model, J, times, data, pv, x0, precision="fp64", noises_off=0, seed=1, stream=0,
-               dt=0.25, like_min=1e-17):
+               dt=0.25, like_min=1e-17, lineage=False):
         times = np.ascontiguousarray(times, dtype=np.float64)
         N = len(times) - 1
         data = np.ascontiguousarray(data, dtype=np.float64).reshape(N, model.n_ts)
         pv = np.ascontiguousarray(pv, dtype=np.float64)
         x0 = np.ascontiguousarray(x0, dtype=np.float64)
-        d = _desc(model, J, N, precision, noises_off, 0, seed, stream, dt, like_min)
+        d = _desc(model, J, N, precision, noises_off, FLAG_LINEAGE if lineage else 0, seed, stream, dt, like_min)
         inp = FilterInputs(dptr(times), dptr(data), dptr(pv), dptr(x0))
         rid = C.c_uint64()
         check(lib().plom_server_submit(self.h, C.byref(d), C.byref(inp), C.byref(rid)))
-        self.N[rid.value] = N
+        self.N[rid.value] = (N, model.S if lineage else 0)
         return rid.value
 
     def step(self):
@@ -399,12 +405,14 @@ class Server:
         return list(self._ids[:n.value])
 
     def result(self, rid):
-        N = self.N.pop(rid)
+        N, S = self.N.pop(rid)
         ll_n = np.zeros(N)
         ess = np.zeros(N)
-        out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), None, None, None)
+        lin = np.zeros((N + 1, S)) if S else None
+        lsl = np.zeros(N, dtype=np.int32) if S else None
+        out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), None, None, None, dptr(lin), iptr(lsl))
         check(lib().plom_server_result(self.h, rid, C.byref(out)))
-        return FilterResult(out.loglik, out.n_all_fail, ll_n, ess)
+        return FilterResult(out.loglik, out.n_all_fail, ll_n, ess, lineage=lin, lineage_slots=lsl)
 
     def close(self):
         if self.h:
@@ -424,16 +432,18 @@ def batch_run(ctx, filters, use_graph=False):
     check(lib().plom_batch_run(ctx.h, len(filters), arr, 1 if use_graph else 0))
 
 
-def run(ctx, model, J, times, data, pv, x0, ancestry=False, final_state=False, **kw):
+def run(ctx, model, J, times, data, pv, x0, ancestry=False, final_state=False, lineage=False, **kw):
     flags = kw.pop("flags", 0)
     if ancestry:
         flags |= FLAG_STORE_ANCESTRY
     if final_state:
         flags |= FLAG_FINAL_STATE
+    if lineage:
+        flags |= FLAG_LINEAGE
     f = Filter(ctx, model, J, times, data, pv, x0, flags=flags, **kw)
     try:
         f.run()
-        return f.read(ancestry=ancestry, final_state=final_state, final_logw=True)
+        return f.read(ancestry=ancestry, final_state=final_state, final_logw=True, lineage=lineage)
     finally:
         f.close()
 

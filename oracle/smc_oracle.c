@@ -258,7 +258,8 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
                double dt, double like_min, const double *times, const double *data, const double *pv,
                const double *x0, int mode, int nthreads,
                double *loglik, int32_t *n_all_fail, double *loglik_n, double *ess_n, int32_t *ancestry,
-               double *final_state, double *logw_out, double *elapsed_s)
+               double *final_state, double *logw_out, double *elapsed_s,
+               int32_t *lin_slots, double *lin_state, double *all_states)
 {
     if (model < 0 || model >= N_MODELS || J < 1 || N < 0) return -1;
     const oracle_model *M = &MODELS[model];
@@ -270,8 +271,13 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
     double *work = malloc((size_t)(3 * T + 4 * next_pow2(T) + 2 * TILE + J) * sizeof(double));
     uint64_t *cum = malloc((size_t)J * sizeof(uint64_t));
     uint32_t *select = malloc((size_t)J * sizeof(uint32_t));
-    if (!X || !Xn || !logw || !work || !cum || !select) return -2;
+    const int lineage = lin_slots != NULL && lin_state != NULL && N > 0;
+    int32_t *anc = ancestry;
+    if (lineage && !anc) anc = malloc((size_t)N * J * sizeof(int32_t));
+    if (!X || !Xn || !logw || !work || !cum || !select || (lineage && !anc)) return -2;
     for (int j = 0; j < J; j++) memcpy(X + (size_t)j * S, x0, (size_t)S * sizeof(double));
+    if (all_states) memcpy(all_states, X, (size_t)J * S * sizeof(double));
+    int last_resampled = 0;
     const int filter = !(flags & FLAG_NO_FILTER);
     const double all_fail_llike = plom_log(like_min) * (double)n_ts;
     double llike = 0.0;
@@ -288,6 +294,7 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
         job.seed = seed; job.stream = stream; job.noises_off = noises_off;
         job.dt = dt; job.sqdt = plom_sqrt(dt); job.like_min = like_min; job.mode = mode;
         propagate_all(&job, nthreads);
+        if (all_states) memcpy(all_states + (size_t)(n + 1) * J * S, X, (size_t)J * S * sizeof(double));
         double ln = 0.0, es = 0.0;
         int resampled = 0;
         if (filter && nonan) {
@@ -303,8 +310,9 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
         }
         if (loglik_n) loglik_n[n] = ln;
         if (ess_n) ess_n[n] = es;
-        if (ancestry)
-            for (int j = 0; j < J; j++) ancestry[(size_t)n * J + j] = (int32_t)select[j];
+        last_resampled = resampled;
+        if (anc)
+            for (int j = 0; j < J; j++) anc[(size_t)n * J + j] = (int32_t)select[j];
         /* resample_X: gather the ancestors' states */
         for (int j = 0; j < J; j++) memcpy(Xn + (size_t)j * S, X + (size_t)select[j] * S, (size_t)S * sizeof(double));
         double *tmp = X; X = Xn; Xn = tmp;
@@ -316,6 +324,37 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
             for (int j = 0; j < J; j++) final_state[(size_t)k * J + j] = X[(size_t)j * S + k];
     *loglik = llike;
     *n_all_fail = naf;
+    if (lineage) {
+        /* the final particle: inverse CDF of the last interval's weights at one uniform
+         * (print.c:730-737), in the arithmetic of the mode; equal weights without them */
+        const uint64_t unum = plom_lineage_num(seed, stream);
+        int j = 0;
+        if (last_resampled && mode == ORACLE_CANONICAL) {
+            const uint64_t Q = cum[J - 1];
+            while ((u128)cum[j] << 33 < (u128)unum * Q) j++;
+        } else if (last_resampled) {
+            const double U = (double)unum * 0x1p-33;
+            double cw = work[0];
+            while (cw < U && j < J - 1) cw += work[++j];
+        } else {
+            j = (int)((unum * (uint64_t)J + (1ull << 33) - 1ull) >> 33) - 1;
+        }
+        if (j < 0) j = 0;
+        if (j >= J) j = J - 1;
+        /* its ancestors, then its path regenerated with the same noise */
+        lin_slots[N - 1] = j;
+        for (int n = N - 1; n >= 1; n--) lin_slots[n - 1] = anc[(size_t)(n - 1) * J + lin_slots[n]];
+        double x[256];
+        memcpy(x, x0, (size_t)S * sizeof(double));
+        memcpy(lin_state, x, (size_t)S * sizeof(double));
+        for (int n = 0; n < N; n++) {
+            plom_nstream64 ns;
+            plom_nstream64_init(&ns, seed, stream, (uint32_t)n, (uint32_t)lin_slots[n], plom_bm_tab);
+            M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off);
+            memcpy(lin_state + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
+        }
+    }
+    if (anc && anc != ancestry) free(anc);
     free(X); free(Xn); free(logw); free(work); free(cum); free(select);
     return 0;
 }

@@ -184,3 +184,47 @@ def test_missing_series_skipped():
     assert ll == l0 + l2
     _, lnone, _ = oracle.obs_loglik(m.id, x, np.full(4, np.nan), pv)
     assert lnone == 0.0
+
+
+def lineage_uniform(seed, stream):
+    """U = (2x+1)/2^33 with x the first Philox word for purpose 2 (the lineage draw)."""
+    x = oracle.philox([0, 0, 2, 0], [seed, stream])[0]
+    return (2 * x + 1) / 2.0 ** 33
+
+
+@pytest.mark.parametrize("name,noises_off", [("sirs4", 0), ("sir", 0), ("seir10", 2)])
+def test_lineage_is_a_path_of_the_filter(name, noises_off):
+    """The sampled lineage (pmcmc --full) is one particle's genealogy: its final slot is the
+    inverse CDF of the last weights at the lineage uniform, each earlier slot is the
+    recorded ancestor, and every state equals what the filter held for that particle at
+    that interval (all states kept by brute force)."""
+    m, times, data, pv, x0 = setup(name, 12)
+    J = 96
+    r = oracle.smc(m.id, J, times, data, pv, x0, noises_off=noises_off, seed=4, stream=9, ancestry=True,
+                   logw=True, lineage=True, all_states=True)
+    slots, lin, alls, anc = r["lineage_slots"], r["lineage"], r["all_states"], r["ancestry"]
+    N = len(times) - 1
+    np.testing.assert_array_equal(lin[0], x0)
+    for n in range(N):
+        np.testing.assert_array_equal(lin[n + 1], alls[n + 1, slots[n]])
+    for n in range(1, N):
+        assert slots[n - 1] == anc[n - 1, slots[n]]
+    lw = r["logw"][N - 1]
+    w = np.exp(lw - lw.max())
+    c = np.cumsum(w / w.sum())
+    U = lineage_uniform(4, 9)
+    j = slots[N - 1]
+    assert (j == 0 or c[j - 1] < U + 1e-12) and c[j] >= U - 1e-12
+    # the path differs from the other particles' (it is not a copy of particle 0)
+    assert np.any(alls[N, slots[N - 1]] != alls[N, (slots[N - 1] + 1) % J])
+
+
+def test_lineage_without_filtering_uses_equal_weights():
+    m, times, data, pv, x0 = setup("sir", 8)
+    J = 50
+    r = oracle.smc(m.id, J, times, data, pv, x0, flags=1, seed=6, stream=1, lineage=True, all_states=True)
+    U = lineage_uniform(6, 1)
+    assert r["lineage_slots"][-1] == int(np.ceil(U * J)) - 1
+    # without resampling every ancestor is the particle itself
+    assert np.all(r["lineage_slots"] == r["lineage_slots"][-1])
+    np.testing.assert_array_equal(r["lineage"][1:], r["all_states"][1:, r["lineage_slots"][-1]])

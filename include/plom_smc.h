@@ -55,6 +55,9 @@ extern "C" {
 #define PLOM_FLAG_NO_FILTER 1u        /* -t/--no_filter: propagate only (C/smc/main.c:177) */
 #define PLOM_FLAG_STORE_ANCESTRY 2u   /* keep select[n][j] for every n (needed by --full) */
 #define PLOM_FLAG_FINAL_STATE 4u      /* keep the resampled final cloud */
+#define PLOM_FLAG_HAT 16u             /* per interval: weighted mean and 95 % bounds of every state,
+                                         remainder and observation mean (compute_hat, C/core/hat.c:66-137)
+                                         and the prediction residuals (print.c:632-701) */
 #define PLOM_FLAG_LINEAGE 8u          /* sample one particle of the last interval by its weight and
                                          return its whole path (pmcmc --full,
                                          sample_traj_and_print, C/core/print.c:715-784); keeps the
@@ -72,6 +75,7 @@ typedef struct {
     int32_t pv_size;     /* length of the flat natural-parameter vector */
     int32_t n_dem;       /* demographic noises per population */
     int32_t n_env;       /* environmental noises per population */
+    int32_t n_hat;       /* quantities of a hat row: states, remainders, observation means */
 } plom_model_info;
 
 typedef struct {
@@ -105,6 +109,10 @@ typedef struct {
     double *lineage;     /* optional [(N+1)*size_proj]: the sampled path, row n = its state at
                             times[n] (row 0 = x0); needs PLOM_FLAG_LINEAGE */
     int32_t *lineage_slots; /* optional [N]: the particle index of that path at each interval */
+    double *hat;         /* optional [(N+1)*n_hat*3]: per row (times[0..N]) and quantity: low95,
+                            mean, high95 (PLOM_FLAG_HAT) */
+    double *pred_res;    /* optional [N*n_ts*2]: per interval and series: prediction mean and
+                            residual over the resampled cloud; NaN without data (PLOM_FLAG_HAT) */
 } plom_filter_outputs;
 
 typedef struct plom_ctx plom_ctx;

@@ -311,6 +311,72 @@ class Emitter:
 
         return E.CEmitter(resolve, "double")
 
+    def n_hat(self):
+        m = self.m
+        return m.N_PAR_SV * m.N_CAC + (m.N_CAC if m.remainder else 0) + m.N_TS
+
+    def obs_resolver(self, ts):
+        lay = self.m.layout
+
+        def resolve(name):
+            if name == "x":
+                return "x"
+            if name in lay["obs"]:
+                return "pvd[%d]" % (lay["obs"][name] + ts)
+            if name in lay["fixed_obs"]:
+                return "pvd[%d]" % (lay["fixed_obs"][name] + ts)
+            if name == "t":
+                return "t"
+            if name == "M_PI":
+                return "3.141592653589793"
+            if name == "ONE_YEAR":
+                return "365.0"
+            raise ValueError("symbol %r not allowed in the observation model" % name)
+        return resolve
+
+    def obs_moments(self):
+        """obs_mean / obs_var of every series (core_template.c:369-394)."""
+        m = self.m
+        mean_t, var_t = E.parse(m.obs_model["mean"]), E.parse(m.obs_model["var"])
+        L = ["PLOM_HD void %s_obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var)" % self.p,
+             "{", "    (void)t; (void)pvd;"]
+        for ts in range(m.N_TS):
+            em = E.CEmitter(self.obs_resolver(ts), "double")
+            L.append("    {")
+            L.append("        const double x = obs[%d];" % ts)
+            L.append("        mean[%d] = %s;" % (ts, em.emit(mean_t)))
+            L.append("        var[%d] = %s;" % (ts, em.emit(var_t)))
+            L.append("    }")
+        L.append("}")
+        return "\n".join(L)
+
+    def hat_values(self):
+        """The quantities summarised per interval (hat.c:66-137): the states, the remainder
+        of each population (pop size - sum_SV, when the model has one) and the observation
+        mean of every series."""
+        m, lay = self.m, self.m.layout
+        L = ["PLOM_HD void %s_hat_values(const double *X, const double *pvd, double t, double *out)" % self.p,
+             "{", "    (void)t;"]
+        k = 0
+        for i in range(m.N_PAR_SV * m.N_CAC):
+            L.append("    out[%d] = X[%d];" % (k, i))
+            k += 1
+        if m.remainder:
+            if "N" not in lay["fixed"]:
+                raise ValueError("a model with a remainder needs the population size N as a covariate")
+            for c in range(m.N_CAC):
+                terms = ["X[%d]" % (i * m.N_CAC + c) for i in range(m.N_PAR_SV)]
+                expr = terms[0]
+                for t in terms[1:]:
+                    expr = "(%s + %s)" % (expr, t)
+                L.append("    out[%d] = pvd[%d] - %s;" % (k, lay["fixed"]["N"] + c, expr))
+                k += 1
+        L.append("    double obs[%d], var[%d];" % (m.N_TS, m.N_TS))
+        L.append("    %s_proj2obs_f64(X, pvd, obs);" % self.p)
+        L.append("    %s_obs_moments(obs, pvd, t, out + %d, var);" % (self.p, k))
+        L.append("}")
+        return "\n".join(L)
+
     def loglik(self):
         m, lay = self.m, self.m.layout
         L = ["PLOM_HD double %s_loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor)" % self.p,
@@ -364,13 +430,18 @@ class Emitter:
                 "#define %s_SIZE_PROJ %d" % (P, m.size_proj),
                 "#define %s_PV_SIZE %d" % (P, m.layout["size"]),
                 "#define %s_N_DEM %d" % (P, ndem),
-                "#define %s_N_ENV %d" % (P, nenv), ""]
+                "#define %s_N_ENV %d" % (P, nenv),
+                "#define %s_N_HAT %d" % (P, self.n_hat()), ""]
         for v, dem, env in VARIANTS:
             out.append(self.interval(v, dem, env, "double"))
             out.append("")
         out.append(self.proj2obs("double"))
         out.append("")
         out.append(self.loglik())
+        out.append("")
+        out.append(self.obs_moments())
+        out.append("")
+        out.append(self.hat_values())
         out.append("")
         # host dispatcher on the reference's noises_off bits (prediction_util.c:143-161)
         out.append("PLOM_HD void %s_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off)" % self.p)
@@ -390,8 +461,8 @@ class Emitter:
         out.append("")
         name = m.name.lower()
         out.append("struct plom_model_%s {" % name)
-        out.append("    static constexpr int S = %d, N_TS = %d, INC_OFFSET = %d, N_INC = %d, PV = %d, N_CAC = %d;"
-                   % (m.size_proj, m.N_TS, m.inc_offset, m.N_TS_INC_UNIQUE, m.layout["size"], m.N_CAC))
+        out.append("    static constexpr int S = %d, N_TS = %d, INC_OFFSET = %d, N_INC = %d, PV = %d, N_CAC = %d, N_HAT = %d;"
+                   % (m.size_proj, m.N_TS, m.inc_offset, m.N_TS_INC_UNIQUE, m.layout["size"], m.N_CAC, self.n_hat()))
         for tag, Rt, stream in (("f64", "double", "plom_nstream64"), ("f32", "float", "plom_nstream32")):
             out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt)"
                        % (Rt, Rt, stream, Rt))
@@ -404,6 +475,10 @@ class Emitter:
                        % (Rt, self.p, tag))
         out.append("    static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor)"
                    " { return %s_loglik(obs, y, pvd, t, like_min, n_floor); }" % self.p)
+        out.append("    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, double t, double *out)"
+                   " { %s_hat_values(X, pvd, t, out); }" % self.p)
+        out.append("    static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var)"
+                   " { %s_obs_moments(obs, pvd, t, mean, var); }" % self.p)
         out.append("};")
         out.append("#endif")
         out.append("")

@@ -92,6 +92,12 @@ struct plom_dfilter {
     int32_t *ancestry;  /* [N][J] or null */
     int32_t *lin_slots; /* [N] or null: the sampled lineage's particle index per interval */
     double *lin_state;  /* [(N+1)][S] or null: its states */
+    int32_t n_hat, pad2;
+    uint64_t *hkey;     /* [n_hat][Jp] or null: per tile, the sorted keys of the hat quantities */
+    uint64_t *hpref;    /* [n_hat][Jp]: inclusive prefix of their weights in that order */
+    double *hpart;      /* [n_hat][T]: per tile, tree sum of value x weight */
+    double *hat;        /* [(N+1)][n_hat][3]: low95, mean, high95 */
+    double *pres;       /* [N][n_ts][2]: prediction mean and residual */
 };
 
 /* ------------------------------------------------------------------ block helpers */
@@ -430,6 +436,260 @@ __global__ __launch_bounds__(64) void k_lineage(plom_dfilter *f)
     }
 }
 
+/* ------------------------------------------------------------------ hat and residuals */
+/* The summaries of interval n (launched after its weights and scan, before the next
+ * propagate reads the cloud).  Weights are the fixed-point integers of the resampling,
+ * q_j = floor(exp(l_j - m)/S 2^62) (0 for failed particles), or 1 for every particle when
+ * the interval was not weighted; Q is their total.
+ *   mean     tree sum over tiles of the tile trees of value x q, divided by Q;
+ *   low95    the smallest value v with 40 W(<= v) >= Q, W(<= v) the total weight of the
+ *            particles whose value is at most v;
+ *   high95   the smallest v with 40 W(<= v) >= 39 Q
+ * (hat.c:29-55: the first sorted value at which the cumulated weight reaches 2.5 % /
+ * 97.5 %; exact in integers here, and independent of the order of equal values).
+ * Each tile sorts its values (bitonic, in LDS) with their weights and keeps the inclusive
+ * weight prefix; one block per quantity and bound then bisects the 64-bit key space,
+ * summing the tiles' prefixes exactly. */
+__device__ inline uint64_t hat_key(double x)
+{
+    const uint64_t u = plom_d2u(x);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__device__ inline double hat_unkey(uint64_t k)
+{
+    return plom_u2d((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k);
+}
+
+__device__ inline bool hat_weighted(const plom_dfilter *f, int n)
+{
+    return !(f->flags & PLOM_FLAG_NO_FILTER) && f->nonan[n] > 0 && f->rflag[n];
+}
+
+template <class M, typename R>
+__global__ __launch_bounds__(TILE) void k_hat_tiles(plom_dfilter *f)
+{
+    __shared__ double lds[8];
+    __shared__ uint64_t lds64[8];
+    __shared__ uint64_t s_key[TILE];
+    __shared__ uint64_t s_w[TILE];
+    const int n = f->step - 1;
+    if (n < 0 || n >= f->N || !f->hkey) return;
+    const int J = f->J, Jp = f->Jp, T = f->T, tile = blockIdx.x;
+    const int j = tile * TILE + threadIdx.x;
+    const bool valid = j < J;
+    double v[M::N_HAT];
+    uint64_t q = 0;
+    if (valid) {
+        const R *Xs = (const R *)f->X[(n + 1) & 1];
+        double xs[M::S];
+#pragma unroll
+        for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
+        M::hat_values(xs, f->pv, f->times[n + 1], v);
+        if (hat_weighted(f, n)) {
+            const double l = f->logw[j];
+            if (l != -PLOM_INF) q = (uint64_t)((plom_exp(l - f->scal[0]) / f->scal[1]) * 0x1p62);
+        } else {
+            q = 1;
+        }
+    }
+    for (int k = 0; k < M::N_HAT; k++) {
+        const double part = tree256(valid ? v[k] * (double)q : 0.0, lds);
+        if (threadIdx.x == 0) f->hpart[(size_t)k * T + tile] = part;
+        s_key[threadIdx.x] = valid ? hat_key(v[k]) : ~0ull;
+        s_w[threadIdx.x] = q;
+        __syncthreads();
+        for (int size = 2; size <= TILE; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                const int t = threadIdx.x, p = t ^ stride;
+                if (p > t) {
+                    const bool up = (t & size) == 0;
+                    const uint64_t a = s_key[t], b = s_key[p];
+                    if ((a > b) == up) {
+                        s_key[t] = b; s_key[p] = a;
+                        const uint64_t wa = s_w[t];
+                        s_w[t] = s_w[p]; s_w[p] = wa;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        const uint64_t key = s_key[threadIdx.x];
+        const uint64_t pref = scan256(s_w[threadIdx.x], lds64);
+        f->hkey[(size_t)k * Jp + j] = key;
+        f->hpref[(size_t)k * Jp + j] = pref;
+        __syncthreads();
+    }
+}
+
+__device__ inline uint64_t sum256_u64(uint64_t v, uint64_t *lds)
+{
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1) v += __shfl_xor(v, h, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    const uint64_t r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+    __syncthreads();
+    return r;
+}
+
+/* blocks 3k, 3k+1, 3k+2: mean, low95 and high95 of quantity k */
+__global__ __launch_bounds__(TILE) void k_hat_select(plom_dfilter *f)
+{
+    __shared__ double lds[8];
+    __shared__ uint64_t lds64[8];
+    __shared__ double s_chunk[MAX_TILES / TILE + 1];
+    const int n = f->step - 1;
+    if (n < 0 || n >= f->N || !f->hkey) return;
+    const int k = blockIdx.x / 3, role = blockIdx.x % 3;
+    const int T = f->T, Jp = f->Jp, NH = f->n_hat;
+    const uint64_t Q = hat_weighted(f, n) ? f->tincl[T - 1] : (uint64_t)f->J;
+    double *out = f->hat + ((size_t)(n + 1) * NH + k) * 3;
+    if (role == 0) {
+        const int nch = (T + TILE - 1) / TILE;
+        for (int c = 0; c < nch; c++) {
+            const int t = c * TILE + threadIdx.x;
+            const double v = tree256(t < T ? f->hpart[(size_t)k * T + t] : 0.0, lds);
+            if (threadIdx.x == 0) s_chunk[c] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int P = 1;
+            while (P < nch) P <<= 1;
+            for (int c = nch; c < P; c++) s_chunk[c] = 0.0;
+            for (int h = 1; h < P; h <<= 1)
+                for (int i = 0; i < P; i += 2 * h) s_chunk[i] = s_chunk[i] + s_chunk[i + h];
+            out[1] = s_chunk[0] / (double)Q;
+        }
+        return;
+    }
+    const uint64_t pnum = role == 1 ? 1u : 39u;
+    const uint64_t *keys = f->hkey + (size_t)k * Jp, *pref = f->hpref + (size_t)k * Jp;
+    uint64_t lo = 0, hi = ~0ull;
+    while (lo < hi) {
+        const uint64_t mid = lo + ((hi - lo) >> 1);
+        uint64_t W = 0;
+        for (int t = threadIdx.x; t < T; t += TILE) {
+            const uint64_t *kt = keys + (size_t)t * TILE;
+            int a = 0, b = TILE;                 /* count of keys <= mid in the sorted tile */
+            while (a < b) {
+                const int m = (a + b) >> 1;
+                if (kt[m] <= mid) a = m + 1; else b = m;
+            }
+            if (a) W += pref[(size_t)t * TILE + a - 1];
+        }
+        W = sum256_u64(W, lds64);
+        if ((plom_u128)W * 40u >= (plom_u128)Q * pnum) hi = mid; else lo = mid + 1;
+    }
+    if (threadIdx.x == 0) out[role == 1 ? 0 : 2] = hat_unkey(lo);
+}
+
+/* number of systematic points k in [0, J) with (k 2^33 + u) Q <= C J 2^33, i.e. of
+ * particles whose ancestor index is at most the particle with inclusive prefix C */
+__device__ inline uint64_t hat_count_le(uint64_t C, uint64_t J, uint64_t unum, uint64_t Q)
+{
+    const plom_u128 X = (plom_u128)C * J << 33;
+    const double est = (double)C * (double)J / (double)Q - (double)unum * 0x1p-33;
+    int64_t k = est < 0.0 ? 0 : (int64_t)est + 1;
+    if (k > (int64_t)J) k = (int64_t)J;
+    while (k > 0 && (plom_u128)(((uint64_t)(k - 1) << 33) + unum) * Q > X) k--;
+    while (k < (int64_t)J && (plom_u128)(((uint64_t)k << 33) + unum) * Q <= X) k++;
+    return (uint64_t)k;
+}
+
+/* prediction residuals of interval n over the resampled cloud (print.c:632-701): with o_j
+ * the number of copies of particle j, pred = sum o_j mu_j / J, var_state = sum o_j
+ * (mu_j - pred)^2 / (J - 1), var_obs = sum o_j sigma2_j / J, res = (y - pred) /
+ * sqrt(var_state + var_obs); each sum strided by thread, then a tree over the block */
+template <class M, typename R>
+__global__ __launch_bounds__(TILE) void k_pred_res(plom_dfilter *f)
+{
+    __shared__ double lds[8];
+    const int n = f->step - 1;
+    if (n < 0 || n >= f->N || !f->pres) return;
+    double *out = f->pres + (size_t)n * M::N_TS * 2;
+    if ((f->flags & PLOM_FLAG_NO_FILTER) || f->nonan[n] == 0) {
+        if (threadIdx.x < 2 * M::N_TS) out[threadIdx.x] = PLOM_NAN;
+        return;
+    }
+    const int J = f->J, Jp = f->Jp;
+    const bool res_on = f->rflag[n] != 0;
+    const uint64_t Q = res_on ? f->tincl[f->T - 1] : 0;
+    const uint64_t unum = plom_resample_num(f->seed, f->stream, (uint32_t)n);
+    const R *Xs = (const R *)f->X[(n + 1) & 1];
+    const double t1 = f->times[n + 1];
+    double a[M::N_TS], b[M::N_TS];
+#pragma unroll
+    for (int ts = 0; ts < M::N_TS; ts++) { a[ts] = 0.0; b[ts] = 0.0; }
+    for (int j = threadIdx.x; j < J; j += TILE) {
+        uint64_t o = 1;
+        if (res_on) {
+            const int tile = j / TILE;
+            const uint64_t C = (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j];
+            const uint64_t Cm = j ? ((j % TILE) ? (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j - 1] : f->tincl[tile - 1]) : 0ull;
+            o = hat_count_le(C, (uint64_t)J, unum, Q) - hat_count_le(Cm, (uint64_t)J, unum, Q);
+        }
+        if (!o) continue;
+        double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
+#pragma unroll
+        for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
+        M::proj2obs(xs, f->pv, obs);
+        M::obs_moments(obs, f->pv, t1, mu, var);
+#pragma unroll
+        for (int ts = 0; ts < M::N_TS; ts++) { a[ts] += (double)o * mu[ts]; b[ts] += (double)o * var[ts]; }
+    }
+    double pred[M::N_TS], vobs[M::N_TS], c[M::N_TS];
+#pragma unroll
+    for (int ts = 0; ts < M::N_TS; ts++) {
+        pred[ts] = tree256(a[ts], lds) / (double)J;
+        vobs[ts] = tree256(b[ts], lds) / (double)J;
+        c[ts] = 0.0;
+    }
+    for (int j = threadIdx.x; j < J; j += TILE) {
+        uint64_t o = 1;
+        if (res_on) {
+            const int tile = j / TILE;
+            const uint64_t C = (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j];
+            const uint64_t Cm = j ? ((j % TILE) ? (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j - 1] : f->tincl[tile - 1]) : 0ull;
+            o = hat_count_le(C, (uint64_t)J, unum, Q) - hat_count_le(Cm, (uint64_t)J, unum, Q);
+        }
+        if (!o) continue;
+        double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
+#pragma unroll
+        for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
+        M::proj2obs(xs, f->pv, obs);
+        M::obs_moments(obs, f->pv, t1, mu, var);
+#pragma unroll
+        for (int ts = 0; ts < M::N_TS; ts++) {
+            const double d = mu[ts] - pred[ts];
+            c[ts] += (double)o * (d * d);
+        }
+    }
+#pragma unroll
+    for (int ts = 0; ts < M::N_TS; ts++) {
+        const double vstate = tree256(c[ts], lds) / (double)(J - 1);
+        if (threadIdx.x == 0) {
+            const double y = f->data[(size_t)n * M::N_TS + ts];
+            out[2 * ts] = pred[ts];
+            out[2 * ts + 1] = (y - pred[ts]) / plom_sqrt(vstate + vobs[ts]);
+        }
+    }
+}
+
+/* row 0 of the hat: every particle starts at x0 */
+template <class M, typename R>
+__global__ void k_hat0(plom_dfilter *f)
+{
+    if (threadIdx.x != 0 || !f->hat) return;
+    const R *x0 = (const R *)f->x0;
+    double xs[M::S], v[M::N_HAT];
+#pragma unroll
+    for (int k = 0; k < M::S; k++) xs[k] = (double)x0[k];
+    M::hat_values(xs, f->pv, f->times[0], v);
+    for (int k = 0; k < M::N_HAT; k++) f->hat[3 * k] = f->hat[3 * k + 1] = f->hat[3 * k + 2] = v[k];
+}
+
 __global__ void k_selftest_math(int n, const double *in, double *out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -470,6 +730,8 @@ struct model_entry {
     plom_model_info info;
     void (*launch_prop)(int prec, int variant, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles);
     void (*launch_lineage)(int prec, int variant, hipStream_t s, plom_dfilter *df);
+    void (*launch_hat)(int prec, hipStream_t s, plom_dfilter *df, int ntiles, int n_hat);
+    void (*launch_hat0)(int prec, hipStream_t s, plom_dfilter *df);
 };
 
 template <class M, typename R>
@@ -508,11 +770,32 @@ static void launch_lineage(int prec, int variant, hipStream_t s, plom_dfilter *d
     else launch_lineage_r<M, double>(variant, s, df);
 }
 
+template <class M>
+static void launch_hat(int prec, hipStream_t s, plom_dfilter *df, int ntiles, int n_hat)
+{
+    if (prec == PLOM_FP32) {
+        hipLaunchKernelGGL((k_hat_tiles<M, float>), dim3(ntiles), dim3(TILE), 0, s, df);
+        hipLaunchKernelGGL((k_pred_res<M, float>), dim3(1), dim3(TILE), 0, s, df);
+    } else {
+        hipLaunchKernelGGL((k_hat_tiles<M, double>), dim3(ntiles), dim3(TILE), 0, s, df);
+        hipLaunchKernelGGL((k_pred_res<M, double>), dim3(1), dim3(TILE), 0, s, df);
+    }
+    hipLaunchKernelGGL(k_hat_select, dim3(3 * n_hat), dim3(TILE), 0, s, df);
+}
+
+template <class M>
+static void launch_hat0(int prec, hipStream_t s, plom_dfilter *df)
+{
+    if (prec == PLOM_FP32) hipLaunchKernelGGL((k_hat0<M, float>), dim3(1), dim3(64), 0, s, df);
+    else hipLaunchKernelGGL((k_hat0<M, double>), dim3(1), dim3(64), 0, s, df);
+}
+
 #define MODEL_ENTRY(lname, UNAME)                                                               \
     { #lname,                                                                                   \
       { UNAME##_SIZE_PROJ, UNAME##_N_TS, UNAME##_N_CAC, UNAME##_N_PAR_SV, UNAME##_INC_OFFSET,   \
-        UNAME##_N_TS_INC_UNIQUE, UNAME##_PV_SIZE, UNAME##_N_DEM, UNAME##_N_ENV },               \
-      launch_prop<plom_model_##lname>, launch_lineage<plom_model_##lname> }
+        UNAME##_N_TS_INC_UNIQUE, UNAME##_PV_SIZE, UNAME##_N_DEM, UNAME##_N_ENV, UNAME##_N_HAT },  \
+      launch_prop<plom_model_##lname>, launch_lineage<plom_model_##lname>, launch_hat<plom_model_##lname>, \
+      launch_hat0<plom_model_##lname> }
 
 static const model_entry g_models[] = {
     MODEL_ENTRY(sir, PLOM_SIR),
@@ -593,7 +876,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
  * server creates it) and starts from zeroed counters without memset calls. */
 struct layout {
     size_t off_X0, off_X1, off_logw, off_cum, off_tmax, off_tsum, off_tsum2, off_ttot, off_tincl, off_anc,
-        off_lslots, off_lstate,
+        off_lslots, off_lstate, off_hkey, off_hpref, off_hpart, off_hat, off_pres,
         off_img, off_df, off_cnt, off_scal, off_naf, off_llike_n, off_ess_n, off_rflag, off_nonan, off_times,
         off_data, off_pv, off_pvf, off_x0, total;
 };
@@ -618,6 +901,12 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     PUT(off_anc, (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_LINEAGE)) ? N * J * 4 : 0);
     PUT(off_lslots, (d->flags & PLOM_FLAG_LINEAGE) ? N * 4 : 0);
     PUT(off_lstate, (d->flags & PLOM_FLAG_LINEAGE) ? (N + 1) * S * 8 : 0);
+    const size_t NH = (d->flags & PLOM_FLAG_HAT) ? (size_t)mi->n_hat : 0;
+    PUT(off_hkey, NH * Jp * 8);
+    PUT(off_hpref, NH * Jp * 8);
+    PUT(off_hpart, NH * T * 8);
+    PUT(off_hat, NH * (N + 1) * 3 * 8);
+    PUT(off_pres, NH ? N * (size_t)mi->n_ts * 2 * 8 : 0);
     L->off_img = o;
     PUT(off_df, sizeof(plom_dfilter));
     PUT(off_cnt, 2 * 4);
@@ -800,6 +1089,13 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     h->ancestry = (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_LINEAGE)) ? (int32_t *)(b + L.off_anc) : nullptr;
     h->lin_slots = (d->flags & PLOM_FLAG_LINEAGE) ? (int32_t *)(b + L.off_lslots) : nullptr;
     h->lin_state = (d->flags & PLOM_FLAG_LINEAGE) ? (double *)(b + L.off_lstate) : nullptr;
+    const int hat_on = (d->flags & PLOM_FLAG_HAT) != 0;
+    h->n_hat = hat_on ? mi.n_hat : 0;
+    h->hkey = hat_on ? (uint64_t *)(b + L.off_hkey) : nullptr;
+    h->hpref = hat_on ? (uint64_t *)(b + L.off_hpref) : nullptr;
+    h->hpart = hat_on ? (double *)(b + L.off_hpart) : nullptr;
+    h->hat = hat_on ? (double *)(b + L.off_hat) : nullptr;
+    h->pres = hat_on ? (double *)(b + L.off_pres) : nullptr;
     f->df = (plom_dfilter *)(b + L.off_df);
 
     /* host image: descriptor, zeroed counters and outputs, inputs */
@@ -898,7 +1194,9 @@ extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
         c->kprop_launches++;
         if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
         enqueue_resample(c->stream, f->df, nullptr, T, nullptr, 1);
+        if (f->desc.flags & PLOM_FLAG_HAT) me->launch_hat(f->desc.precision, c->stream, f->df, T, f->info.n_hat);
     }
+    if ((f->desc.flags & PLOM_FLAG_HAT) && f->desc.N > 0) me->launch_hat0(f->desc.precision, c->stream, f->df);
     if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
     c->kev_used = k;
     if (needs_final(f)) enqueue_final(c, f, nullptr, f->df, T);
@@ -927,6 +1225,11 @@ static int filter_read_on(plom_ctx *c, plom_filter *f, plom_filter_outputs *o, h
     if (o->ess_n && N) HIPCHK(hipMemcpyAsync(o->ess_n, h->ess_n, N * 8, hipMemcpyDeviceToHost, s));
     if (o->ancestry && N) HIPCHK(hipMemcpyAsync(o->ancestry, h->ancestry, N * J * 4, hipMemcpyDeviceToHost, s));
     if (o->final_logw) HIPCHK(hipMemcpyAsync(o->final_logw, h->logw, J * 8, hipMemcpyDeviceToHost, s));
+    if ((o->hat || o->pred_res) && N) {
+        if (!h->hat) return set_err(PLOM_ERR_STATE, "hat requested but PLOM_FLAG_HAT not set");
+        if (o->hat) HIPCHK(hipMemcpyAsync(o->hat, h->hat, (N + 1) * (size_t)h->n_hat * 3 * 8, hipMemcpyDeviceToHost, s));
+        if (o->pred_res) HIPCHK(hipMemcpyAsync(o->pred_res, h->pres, N * (size_t)h->n_ts * 2 * 8, hipMemcpyDeviceToHost, s));
+    }
     if ((o->lineage || o->lineage_slots) && N) {
         if (!h->lin_state) return set_err(PLOM_ERR_STATE, "lineage requested but PLOM_FLAG_LINEAGE not set");
         if (o->lineage)
@@ -1070,6 +1373,11 @@ static void chain_resample(batch_chain *ch, int s, hipStream_t st)
     int nfa = 0;
     while (nfa < (int)ch->fs.size() && ch->fs[(size_t)nfa]->desc.N > s) nfa++;
     enqueue_resample(st, ch->ddf, (const int2 *)ch->d_all, nb, nullptr, nfa);
+    for (int r = 0; r < nfa; r++) {
+        plom_filter *f = ch->fs[(size_t)r];
+        if (f->desc.flags & PLOM_FLAG_HAT)
+            g_models[f->desc.model_id].launch_hat(f->desc.precision, st, ch->ddf + r, f->h.T, f->info.n_hat);
+    }
 }
 
 static void chain_final(plom_ctx *c, batch_chain *ch, hipStream_t st)
@@ -1080,6 +1388,8 @@ static void chain_final(plom_ctx *c, batch_chain *ch, hipStream_t st)
         while (ch->all_tiles[first].x != r) first++;
         if (f->desc.N > 0 && (f->desc.flags & PLOM_FLAG_LINEAGE))
             g_models[f->desc.model_id].launch_lineage(f->desc.precision, f->variant, st, ch->ddf + r);
+        if (f->desc.N > 0 && (f->desc.flags & PLOM_FLAG_HAT))
+            g_models[f->desc.model_id].launch_hat0(f->desc.precision, st, ch->ddf + r);
         if (!needs_final(f)) continue;
         if (f->desc.precision == PLOM_FP32)
             hipLaunchKernelGGL((k_final<float>), dim3(f->h.T), dim3(TILE), 0, st, ch->ddf, ch->d_all + first, f->info.size_proj);
@@ -1503,8 +1813,8 @@ extern "C" int plom_server_submit(plom_server *S, const plom_filter_desc *d, con
     plom_model_info mi;
     int rc = check_desc(d, &mi);
     if (rc != PLOM_OK) return rc;
-    if (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_FINAL_STATE))
-        return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods and lineages (no ancestry / final state)");
+    if (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_FINAL_STATE | PLOM_FLAG_HAT))
+        return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods and lineages (no ancestry, final state or hat)");
     if (!in->times || !in->pv || !in->x0 || (d->N > 0 && !in->data)) return set_err(PLOM_ERR_ARG, "null input array");
     for (int n = 0; n < d->N; n++)
         if (!(in->times[n + 1] > in->times[n])) return set_err(PLOM_ERR_ARG, "times must increase (n=%d)", n);

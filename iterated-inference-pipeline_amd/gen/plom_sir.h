@@ -12,6 +12,7 @@
 #define PLOM_SIR_PV_SIZE 5
 #define PLOM_SIR_N_DEM 2
 #define PLOM_SIR_N_ENV 0
+#define PLOM_SIR_N_HAT 4
 
 PLOM_HD void plom_sir_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
 {
@@ -149,6 +150,27 @@ PLOM_HD double plom_sir_loglik(const double *obs, const double *y, const double 
     return loglike;
 }
 
+PLOM_HD void plom_sir_obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var)
+{
+    (void)t; (void)pvd;
+    {
+        const double x = obs[0];
+        mean[0] = (pvd[2] * x);
+        var[0] = (((pvd[2] * (1.0 - pvd[2])) * x) + plom_powi(((pvd[2] * pvd[3]) * x), 2));
+    }
+}
+
+PLOM_HD void plom_sir_hat_values(const double *X, const double *pvd, double t, double *out)
+{
+    (void)t;
+    out[0] = X[0];
+    out[1] = X[1];
+    out[2] = pvd[4] - (X[0] + X[1]);
+    double obs[1], var[1];
+    plom_sir_proj2obs_f64(X, pvd, obs);
+    plom_sir_obs_moments(obs, pvd, t, out + 3, var);
+}
+
 PLOM_HD void plom_sir_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off)
 {
     const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);
@@ -278,7 +300,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_proj2obs_f
 }
 
 struct plom_model_sir {
-    static constexpr int S = 3, N_TS = 1, INC_OFFSET = 2, N_INC = 1, PV = 5, N_CAC = 1;
+    static constexpr int S = 3, N_TS = 1, INC_OFFSET = 2, N_INC = 1, PV = 5, N_CAC = 1, N_HAT = 4;
     template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
     {
         if constexpr (V == 0) plom_sir_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt);
@@ -296,6 +318,8 @@ struct plom_model_sir {
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, double *obs) { plom_sir_proj2obs_f32(X, pvd, obs); }
     static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor) { return plom_sir_loglik(obs, y, pvd, t, like_min, n_floor); }
+    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, double t, double *out) { plom_sir_hat_values(X, pvd, t, out); }
+    static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var) { plom_sir_obs_moments(obs, pvd, t, mean, var); }
 };
 #endif
 

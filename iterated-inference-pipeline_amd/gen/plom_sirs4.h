@@ -12,6 +12,7 @@
 #define PLOM_SIRS4_PV_SIZE 48
 #define PLOM_SIRS4_N_DEM 3
 #define PLOM_SIRS4_N_ENV 1
+#define PLOM_SIRS4_N_HAT 16
 
 PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
 {
@@ -527,6 +528,51 @@ PLOM_HD double plom_sirs4_loglik(const double *obs, const double *y, const doubl
     return loglike;
 }
 
+PLOM_HD void plom_sirs4_obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var)
+{
+    (void)t; (void)pvd;
+    {
+        const double x = obs[0];
+        mean[0] = (pvd[20] * x);
+        var[0] = (((pvd[20] * (1.0 - pvd[20])) * x) + plom_powi(((pvd[20] * pvd[24]) * x), 2));
+    }
+    {
+        const double x = obs[1];
+        mean[1] = (pvd[21] * x);
+        var[1] = (((pvd[21] * (1.0 - pvd[21])) * x) + plom_powi(((pvd[21] * pvd[25]) * x), 2));
+    }
+    {
+        const double x = obs[2];
+        mean[2] = (pvd[22] * x);
+        var[2] = (((pvd[22] * (1.0 - pvd[22])) * x) + plom_powi(((pvd[22] * pvd[26]) * x), 2));
+    }
+    {
+        const double x = obs[3];
+        mean[3] = (pvd[23] * x);
+        var[3] = (((pvd[23] * (1.0 - pvd[23])) * x) + plom_powi(((pvd[23] * pvd[27]) * x), 2));
+    }
+}
+
+PLOM_HD void plom_sirs4_hat_values(const double *X, const double *pvd, double t, double *out)
+{
+    (void)t;
+    out[0] = X[0];
+    out[1] = X[1];
+    out[2] = X[2];
+    out[3] = X[3];
+    out[4] = X[4];
+    out[5] = X[5];
+    out[6] = X[6];
+    out[7] = X[7];
+    out[8] = pvd[28] - (X[0] + X[4]);
+    out[9] = pvd[29] - (X[1] + X[5]);
+    out[10] = pvd[30] - (X[2] + X[6]);
+    out[11] = pvd[31] - (X[3] + X[7]);
+    double obs[4], var[4];
+    plom_sirs4_proj2obs_f64(X, pvd, obs);
+    plom_sirs4_obs_moments(obs, pvd, t, out + 12, var);
+}
+
 PLOM_HD void plom_sirs4_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off)
 {
     const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);
@@ -1007,7 +1053,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_proj2obs
 }
 
 struct plom_model_sirs4 {
-    static constexpr int S = 12, N_TS = 4, INC_OFFSET = 8, N_INC = 4, PV = 48, N_CAC = 4;
+    static constexpr int S = 12, N_TS = 4, INC_OFFSET = 8, N_INC = 4, PV = 48, N_CAC = 4, N_HAT = 16;
     template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
     {
         if constexpr (V == 0) plom_sirs4_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt);
@@ -1025,6 +1071,8 @@ struct plom_model_sirs4 {
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, double *obs) { plom_sirs4_proj2obs_f32(X, pvd, obs); }
     static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor) { return plom_sirs4_loglik(obs, y, pvd, t, like_min, n_floor); }
+    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, double t, double *out) { plom_sirs4_hat_values(X, pvd, t, out); }
+    static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var) { plom_sirs4_obs_moments(obs, pvd, t, mean, var); }
 };
 #endif
 

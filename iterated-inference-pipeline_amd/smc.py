@@ -20,14 +20,14 @@ from .codegen import load_model, BUILTIN_MODELS, context_data, load_json
 
 PLOM_FP64, PLOM_FP32 = 0, 1
 NO_DEM_STO, NO_ENV_STO, NO_DRIFT = 1, 2, 4
-FLAG_NO_FILTER, FLAG_STORE_ANCESTRY, FLAG_FINAL_STATE, FLAG_LINEAGE = 1, 2, 4, 8
+FLAG_NO_FILTER, FLAG_STORE_ANCESTRY, FLAG_FINAL_STATE, FLAG_LINEAGE, FLAG_HAT = 1, 2, 4, 8, 16
 MAX_J = 1 << 21
 
 
 class ModelInfo(C.Structure):
     _fields_ = [("size_proj", C.c_int32), ("n_ts", C.c_int32), ("n_cac", C.c_int32),
                 ("n_par_sv", C.c_int32), ("inc_offset", C.c_int32), ("n_inc", C.c_int32),
-                ("pv_size", C.c_int32), ("n_dem", C.c_int32), ("n_env", C.c_int32)]
+                ("pv_size", C.c_int32), ("n_dem", C.c_int32), ("n_env", C.c_int32), ("n_hat", C.c_int32)]
 
 
 class FilterDesc(C.Structure):
@@ -46,7 +46,8 @@ class FilterOutputs(C.Structure):
                 ("loglik_n", C.POINTER(C.c_double)), ("ess_n", C.POINTER(C.c_double)),
                 ("ancestry", C.POINTER(C.c_int32)), ("final_state", C.POINTER(C.c_double)),
                 ("final_logw", C.POINTER(C.c_double)), ("lineage", C.POINTER(C.c_double)),
-                ("lineage_slots", C.POINTER(C.c_int32))]
+                ("lineage_slots", C.POINTER(C.c_int32)), ("hat", C.POINTER(C.c_double)),
+                ("pred_res", C.POINTER(C.c_double))]
 
 
 EXPORTS = [
@@ -177,6 +178,7 @@ class Model:
         self.S = self.desc.size_proj
         self.n_ts = self.desc.N_TS
         self.n_cac = self.desc.N_CAC
+        self.n_hat = self.desc.N_PAR_SV * self.desc.N_CAC + (self.desc.N_CAC if self.desc.remainder else 0) + self.n_ts
         self.layout = self.desc.layout
         self.context = load_json(os.path.join(PKG_DIR, "examples", name, "context.json"))
 
@@ -306,7 +308,7 @@ class Filter:
         check(lib().plom_filter_run(self.ctx.h, self.h))
         return self
 
-    def read(self, ancestry=False, final_state=False, final_logw=False, lineage=False):
+    def read(self, ancestry=False, final_state=False, final_logw=False, lineage=False, hat=False):
         N, J = self.N, self.J
         ll_n = np.zeros(N)
         ess = np.zeros(N)
@@ -315,9 +317,14 @@ class Filter:
         fw = np.zeros(J) if final_logw else None
         lin = np.zeros((N + 1, self.model.S)) if lineage else None
         lsl = np.zeros(N, dtype=np.int32) if lineage else None
-        out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), iptr(anc), dptr(fs), dptr(fw), dptr(lin), iptr(lsl))
+        ht = np.zeros((N + 1, self.model.n_hat, 3)) if hat else None
+        pr = np.zeros((N, self.model.n_ts, 2)) if hat else None
+        out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), iptr(anc), dptr(fs), dptr(fw), dptr(lin), iptr(lsl),
+                            dptr(ht), dptr(pr))
         check(lib().plom_filter_read(self.ctx.h, self.h, C.byref(out)))
-        return FilterResult(out.loglik, out.n_all_fail, ll_n, ess, anc, fs, fw, lin, lsl)
+        r = FilterResult(out.loglik, out.n_all_fail, ll_n, ess, anc, fs, fw, lin, lsl)
+        r.hat, r.pred_res = ht, pr
+        return r
 
     def close(self):
         if self.h:

@@ -48,19 +48,27 @@ typedef unsigned __int128 u128;
 typedef void (*interval_fn)(double *, const double *, plom_nstream64 *, double, double, double, double, unsigned int);
 typedef void (*proj2obs_fn)(const double *, const double *, double *);
 typedef double (*loglik_fn)(const double *, const double *, const double *, double, double, int *);
+typedef void (*moments_fn)(const double *, const double *, double, double *, double *);
+typedef void (*hat_fn)(const double *, const double *, double, double *);
 
 typedef struct {
     const char *name;
-    int S, n_ts;
+    int S, n_ts, n_hat;
     interval_fn interval;
     proj2obs_fn proj2obs;
     loglik_fn loglik;
+    moments_fn obs_moments;
+    hat_fn hat_values;
 } oracle_model;
 
+#define ORACLE_MODEL(lname, UNAME)                                                                 \
+    { #lname, UNAME##_SIZE_PROJ, UNAME##_N_TS, UNAME##_N_HAT, plom_##lname##_interval_f64,           \
+      plom_##lname##_proj2obs_f64, plom_##lname##_loglik, plom_##lname##_obs_moments, plom_##lname##_hat_values }
+
 static const oracle_model MODELS[] = {
-    {"sir", PLOM_SIR_SIZE_PROJ, PLOM_SIR_N_TS, plom_sir_interval_f64, plom_sir_proj2obs_f64, plom_sir_loglik},
-    {"sirs4", PLOM_SIRS4_SIZE_PROJ, PLOM_SIRS4_N_TS, plom_sirs4_interval_f64, plom_sirs4_proj2obs_f64, plom_sirs4_loglik},
-    {"seir10", PLOM_SEIR10_SIZE_PROJ, PLOM_SEIR10_N_TS, plom_seir10_interval_f64, plom_seir10_proj2obs_f64, plom_seir10_loglik},
+    ORACLE_MODEL(sir, PLOM_SIR),
+    ORACLE_MODEL(sirs4, PLOM_SIRS4),
+    ORACLE_MODEL(seir10, PLOM_SEIR10),
 };
 #define N_MODELS ((int)(sizeof MODELS / sizeof MODELS[0]))
 
@@ -68,6 +76,7 @@ int oracle_model_count(void) { return N_MODELS; }
 const char *oracle_model_name(int m) { return (m >= 0 && m < N_MODELS) ? MODELS[m].name : ""; }
 int oracle_model_size_proj(int m) { return MODELS[m].S; }
 int oracle_model_n_ts(int m) { return MODELS[m].n_ts; }
+int oracle_model_n_hat(int m) { return MODELS[m].n_hat; }
 
 /* ------------------------------------------------------------------ canonical trees */
 /* recursive contiguous pairwise sum of a[0..n), n a power of two */
@@ -253,13 +262,115 @@ static int weight_select_reference(const double *logw, int J, int n, int n_ts, d
     return 1;
 }
 
+/* ------------------------------------------------------------------ hat and residuals */
+/* The per-interval summaries (compute_hat, hat.c:66-137; print_prediction_residuals,
+ * print.c:632-701) in the build's arithmetic: fixed-point weights q_j (1 each when the
+ * interval is not weighted), their total Q; the mean is the tree over tiles of the tile
+ * trees of value x q, over Q; the 95 % bounds are found by sorting the values (ordered as
+ * 64-bit keys) and walking the exact cumulative weights; the residuals use the offspring
+ * counts of the selection, sums strided over 256 slots then a tree. */
+static uint64_t hat_key(double x)
+{
+    uint64_t u;
+    memcpy(&u, &x, 8);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+typedef struct { uint64_t key; uint64_t w; double v; } hat_item;
+
+static int hat_cmp(const void *a, const void *b)
+{
+    const uint64_t x = ((const hat_item *)a)->key, y = ((const hat_item *)b)->key;
+    return x < y ? -1 : (x > y);
+}
+
+static void hat_row(const oracle_model *M, const double *X, int J, const double *pv, double t,
+                    const uint64_t *q, uint64_t Q, double *row, double *vals, hat_item *items, double *buf)
+{
+    const int NH = M->n_hat, S = M->S;
+    const int T = (J + TILE - 1) / TILE, P = next_pow2(T);
+    for (int j = 0; j < J; j++) M->hat_values(X + (size_t)j * S, pv, t, vals + (size_t)j * NH);
+    for (int k = 0; k < NH; k++) {
+        double *tiles = buf + TILE;
+        for (int tt = 0; tt < P; tt++) {
+            tiles[tt] = 0.0;
+            if (tt >= T) continue;
+            for (int i = 0; i < TILE; i++) {
+                const int j = tt * TILE + i;
+                buf[i] = j < J ? vals[(size_t)j * NH + k] * (double)q[j] : 0.0;
+            }
+            tiles[tt] = tree_sum(buf, TILE);
+        }
+        row[3 * k + 1] = tree_sum(tiles, P) / (double)Q;
+        for (int j = 0; j < J; j++) {
+            items[j].v = vals[(size_t)j * NH + k];
+            items[j].key = hat_key(items[j].v);
+            items[j].w = q[j];
+        }
+        qsort(items, (size_t)J, sizeof(hat_item), hat_cmp);
+        for (int b = 0; b < 2; b++) {
+            const uint64_t p = b ? 39u : 1u;
+            uint64_t cum = 0;
+            int i = 0;
+            for (; i < J; i++) {
+                cum += items[i].w;
+                if ((u128)cum * 40u >= (u128)Q * p) break;
+            }
+            row[3 * k + (b ? 2 : 0)] = items[i < J ? i : J - 1].v;
+        }
+    }
+}
+
+static void pred_res_row(const oracle_model *M, const double *X, int J, const double *pv, double t, const double *y,
+                         const uint32_t *select, uint32_t *offspring, double *row)
+{
+    const int S = M->S, NT = M->n_ts;
+    double a[TILE][16], b[TILE][16], c[TILE][16], col[TILE], obs[64], mu[64], var[64];
+    double pred[16], vobs[16];
+    memset(offspring, 0, (size_t)J * sizeof(uint32_t));
+    for (int k = 0; k < J; k++) offspring[select[k]]++;
+    memset(a, 0, sizeof a);
+    memset(b, 0, sizeof b);
+    memset(c, 0, sizeof c);
+    for (int j = 0; j < J; j++) {
+        if (!offspring[j]) continue;
+        M->proj2obs(X + (size_t)j * S, pv, obs);
+        M->obs_moments(obs, pv, t, mu, var);
+        for (int ts = 0; ts < NT; ts++) {
+            a[j % TILE][ts] += (double)offspring[j] * mu[ts];
+            b[j % TILE][ts] += (double)offspring[j] * var[ts];
+        }
+    }
+    for (int ts = 0; ts < NT; ts++) {
+        for (int i = 0; i < TILE; i++) col[i] = a[i][ts];
+        pred[ts] = tree_sum(col, TILE) / (double)J;
+        for (int i = 0; i < TILE; i++) col[i] = b[i][ts];
+        vobs[ts] = tree_sum(col, TILE) / (double)J;
+    }
+    for (int j = 0; j < J; j++) {
+        if (!offspring[j]) continue;
+        M->proj2obs(X + (size_t)j * S, pv, obs);
+        M->obs_moments(obs, pv, t, mu, var);
+        for (int ts = 0; ts < NT; ts++) {
+            const double d = mu[ts] - pred[ts];
+            c[j % TILE][ts] += (double)offspring[j] * (d * d);
+        }
+    }
+    for (int ts = 0; ts < NT; ts++) {
+        for (int i = 0; i < TILE; i++) col[i] = c[i][ts];
+        const double vstate = tree_sum(col, TILE) / (double)(J - 1);
+        row[2 * ts] = pred[ts];
+        row[2 * ts + 1] = (y[ts] - pred[ts]) / plom_sqrt(vstate + vobs[ts]);
+    }
+}
+
 /* ------------------------------------------------------------------ the filter */
 int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uint32_t seed, uint32_t stream,
                double dt, double like_min, const double *times, const double *data, const double *pv,
                const double *x0, int mode, int nthreads,
                double *loglik, int32_t *n_all_fail, double *loglik_n, double *ess_n, int32_t *ancestry,
                double *final_state, double *logw_out, double *elapsed_s,
-               int32_t *lin_slots, double *lin_state, double *all_states)
+               int32_t *lin_slots, double *lin_state, double *all_states, double *hat, double *pred_res)
 {
     if (model < 0 || model >= N_MODELS || J < 1 || N < 0) return -1;
     const oracle_model *M = &MODELS[model];
@@ -278,6 +389,22 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
     for (int j = 0; j < J; j++) memcpy(X + (size_t)j * S, x0, (size_t)S * sizeof(double));
     if (all_states) memcpy(all_states, X, (size_t)J * S * sizeof(double));
     int last_resampled = 0;
+    const int NH = M->n_hat;
+    double *hvals = NULL, *hbuf = NULL;
+    hat_item *hitems = NULL;
+    uint64_t *hq = NULL;
+    uint32_t *offspring = NULL;
+    if (hat) {
+        hvals = malloc((size_t)J * NH * sizeof(double));
+        hbuf = malloc((size_t)(TILE + next_pow2(T)) * sizeof(double));
+        hitems = malloc((size_t)J * sizeof(hat_item));
+        hq = malloc((size_t)J * sizeof(uint64_t));
+        offspring = malloc((size_t)J * sizeof(uint32_t));
+        if (!hvals || !hbuf || !hitems || !hq || !offspring || M->n_ts > 16) return -2;
+        double v0[256];
+        M->hat_values(x0, pv, times[0], v0);
+        for (int k = 0; k < NH; k++) hat[3 * k] = hat[3 * k + 1] = hat[3 * k + 2] = v0[k];
+    }
     const int filter = !(flags & FLAG_NO_FILTER);
     const double all_fail_llike = plom_log(like_min) * (double)n_ts;
     double llike = 0.0;
@@ -311,6 +438,21 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
         if (loglik_n) loglik_n[n] = ln;
         if (ess_n) ess_n[n] = es;
         last_resampled = resampled;
+        if (hat) {
+            uint64_t Q = (uint64_t)J;
+            if (resampled && mode == ORACLE_CANONICAL) {
+                for (int j = 0; j < J; j++) hq[j] = cum[j] - (j ? cum[j - 1] : 0ull);
+                Q = cum[J - 1];
+            } else {
+                for (int j = 0; j < J; j++) hq[j] = 1;
+            }
+            hat_row(M, X, J, pv, times[n + 1], hq, Q, hat + (size_t)(n + 1) * NH * 3, hvals, hitems, hbuf);
+            double *pr = pred_res + (size_t)n * n_ts * 2;
+            if (filter && nonan)
+                pred_res_row(M, X, J, pv, times[n + 1], data + (size_t)n * n_ts, select, offspring, pr);
+            else
+                for (int k = 0; k < 2 * n_ts; k++) pr[k] = NAN;
+        }
         if (anc)
             for (int j = 0; j < J; j++) anc[(size_t)n * J + j] = (int32_t)select[j];
         /* resample_X: gather the ancestors' states */
@@ -355,6 +497,7 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
         }
     }
     if (anc && anc != ancestry) free(anc);
+    free(hvals); free(hbuf); free(hitems); free(hq); free(offspring);
     free(X); free(Xn); free(logw); free(work); free(cum); free(select);
     return 0;
 }

@@ -35,6 +35,7 @@ def test_host_side_queries():
         assert L.plom_model_info_get(i, C.byref(info)) == 0
         m = smc.Model.get(name)
         assert info.size_proj == m.S and info.n_ts == m.n_ts and info.pv_size == m.layout["size"]
+        assert info.n_hat == m.n_hat == (16 if name == "sirs4" else (4 if name == "sir" else 50))
     assert L.plom_model_find(b"nope") < 0
     assert b"unknown model" in L.plom_last_error()
 

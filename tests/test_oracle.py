@@ -228,3 +228,92 @@ def test_lineage_without_filtering_uses_equal_weights():
     # without resampling every ancestor is the particle itself
     assert np.all(r["lineage_slots"] == r["lineage_slots"][-1])
     np.testing.assert_array_equal(r["lineage"][1:], r["all_states"][1:, r["lineage_slots"][-1]])
+
+
+def hat_quantities(m, states, pv):
+    """The hat quantities written from the model files (hat.c:66-137): states, remainder
+    N - sum of the states, and the observation mean rep * x of every series
+    (examples/*/link.json), for states [..., S]."""
+    d = m.desc
+    lay = d.layout
+    out = [states[..., i] for i in range(d.N_PAR_SV * d.N_CAC)]
+    for c in range(d.N_CAC):
+        out.append(pv[lay["fixed"]["N"] + c] - sum(states[..., i * d.N_CAC + c] for i in range(d.N_PAR_SV)))
+    obs = np.stack([states[..., d.inc_offset + u] for u in range(d.N_TS)], axis=-1)  # one series per incidence
+    for ts in range(d.N_TS):
+        out.append(pv[lay["obs"]["rep"] + ts] * obs[..., ts])
+    return np.stack(out, axis=-1)
+
+
+@pytest.mark.parametrize("name", ["sir", "sirs4"])
+def test_hat_against_numpy(name):
+    """Weighted mean (numpy, float weights) and the 2.5 / 97.5 % bounds (sorted values, the
+    first whose cumulated weight reaches the level) of every hat quantity, on the cloud
+    before resampling; unweighted rows where the interval had no data."""
+    m, times, data, pv, x0 = setup(name, 10)
+    data = data.copy()
+    data[4, :] = np.nan
+    J = 500
+    r = oracle.smc(m.id, J, times, data, pv, x0, seed=12, stream=2, logw=True, all_states=True, hat=True)
+    hat, alls = r["hat"], r["all_states"]
+    q0 = hat_quantities(m, x0, pv)
+    np.testing.assert_array_equal(hat[0, :, 1], q0)
+    for n in range(len(times) - 1):
+        vals = hat_quantities(m, alls[n + 1], pv)
+        if np.isnan(data[n]).all():
+            w = np.ones(J)
+        else:
+            lw = r["logw"][n]
+            w = np.where(np.isfinite(lw), np.exp(lw - lw.max()), 0.0)
+        w = w / w.sum()
+        for k in range(vals.shape[1]):
+            v = vals[:, k]
+            assert hat[n + 1, k, 1] == pytest.approx(np.sum(v * w), rel=1e-12, abs=1e-9)
+            order = np.argsort(v, kind="stable")
+            c = np.cumsum(w[order])
+            for col, level in ((0, 0.025), (2, 0.975)):
+                i = int(np.searchsorted(c, level - 1e-12))
+                near = {v[order[max(i - 1, 0)]], v[order[i]], v[order[min(i + 1, J - 1)]]}
+                assert hat[n + 1, k, col] in near
+
+
+def test_hat_deterministic_limit():
+    """Without noise every particle follows the one path: both bounds are its value exactly
+    and the mean is that value to rounding."""
+    m, times, data, pv, x0 = setup("sirs4", 8)
+    r = oracle.smc(m.id, 300, times, data, pv, x0, noises_off=3, hat=True)
+    traj = oracle.trajectory(m.id, times, pv, x0, noises_off=3)
+    for n in range(len(times)):
+        q = hat_quantities(m, traj[n], pv)
+        np.testing.assert_array_equal(r["hat"][n, :, 0], q)
+        np.testing.assert_array_equal(r["hat"][n, :, 2], q)
+        np.testing.assert_allclose(r["hat"][n, :, 1], q, rtol=1e-13)
+
+
+def test_pred_res_against_numpy():
+    """Prediction residuals (print.c:632-701) over the resampled cloud, from the stored
+    states and selection: pred = mean of rep x, var_state its sample variance, var_obs the
+    mean of rep (1 - rep) x + (rep phi x)^2, res = (y - pred) / sqrt(var_state + var_obs);
+    NaN rows where the interval had no data."""
+    m, times, data, pv, x0 = setup("sirs4", 9)
+    data = data.copy()
+    data[3, :] = np.nan
+    J = 400
+    r = oracle.smc(m.id, J, times, data, pv, x0, seed=5, stream=7, ancestry=True, all_states=True, hat=True)
+    d = m.desc
+    lay = d.layout
+    for n in range(len(times) - 1):
+        row = r["pred_res"][n]
+        if np.isnan(data[n]).all():
+            assert np.isnan(row).all()
+            continue
+        cloud = r["all_states"][n + 1][r["ancestry"][n]]
+        for ts in range(d.N_TS):
+            rep, phi = pv[lay["obs"]["rep"] + ts], pv[lay["obs"]["phi"] + ts]
+            x = cloud[:, d.inc_offset + ts]
+            mu = rep * x
+            var = rep * (1.0 - rep) * x + (rep * phi * x) ** 2
+            pred = mu.mean()
+            res = (data[n, ts] - pred) / np.sqrt(mu.var(ddof=1) + var.mean())
+            assert row[ts, 0] == pytest.approx(pred, rel=1e-12)
+            assert row[ts, 1] == pytest.approx(res, rel=1e-9, abs=1e-12)

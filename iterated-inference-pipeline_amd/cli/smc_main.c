@@ -10,8 +10,11 @@
  * "coupling" naming the generated model compiled into the library) and one line of
  * theta JSON on stdin (json.c:359-383).
  * Outputs: the log lines of main.c:266-356 (unless -q/-P), trace_<id>.csv
- * (print.c:189-207, 396-440), and the theta with updated guesses on stdout (-P) or in
- * .done_<id>.json (print.c:33-67).
+ * (print.c:189-207, 396-440), hat_<id>.csv (print.c:151-186, 445-573: time, low95 / mean
+ * / high95 of every state, remainder and observation mean, rows t = 0 and every
+ * interval), pred_res_<id>.csv (print.c:132-143, 632-701: rows for the intervals with
+ * data), and the theta with updated guesses on stdout (-P) or in .done_<id>.json
+ * (print.c:33-67).
  *
  * Parameter handling follows the reference: routers from partitions
  * (build.c:267-475), transform then back-transform with scale, unit multiplier and
@@ -19,8 +22,9 @@
  * (json.c:405-424, proposal.c:87-114), initial state = proportions x population size at
  * t = 0 (prediction_util.c:82-115, build.c:1179-1185).
  *
- * Differences: implementation "psr" and "ode" are not built (only "sde"); hat, pred_res
- * and X outputs are not written yet (-e -d are implied, a note goes to stderr);
+ * Differences: implementation "psr" and "ode" are not built (only "sde"); the X output
+ * of -r is not written (a note goes to stderr); hat and pred_res use this build's
+ * arithmetic (exact integer weights for the 95 % bounds, tree sums);
  * --prior is not supported; -N is accepted and ignored (the GPU replaces threads).
  * Extensions: --fp32 (FP32 propagation), --device <k>, --seed <k> (random key, default
  * 2 + id as build.c:965-971), a full-precision "logV_17g" log line, and --dry_run (print
@@ -170,9 +174,7 @@ static pj *partition_groups(const pj *theta, const char *pid, const pj *order, c
     pj *parts = pj_get(theta, "partition");
     pj *p = pj_get(parts, pid);
     if (p) return pj_get(p, "group");
-    static pj *made[8];
-    static int n_made = 0;
-    pj *groups = NULL;
+    pj *groups = NULL;   /* synthesised groups live until the command exits */
     char text[1 << 16];
     int o = 0;
     if (!strcmp(pid, "variable_population") || !strcmp(pid, "variable_time_series")) {
@@ -191,7 +193,6 @@ static pj *partition_groups(const pj *theta, const char *pid, const pj *order, c
     }
     char err[256];
     groups = pj_parse(text, err, sizeof err);
-    if (n_made < 8) made[n_made++] = groups;
     return groups;
 }
 
@@ -487,8 +488,8 @@ int main(int argc, char *argv[])
         printf("]}\n");
         return 0;
     }
-    if (print_hat || print_pred_res || print_X)
-        fprintf(stderr, "note: hat/pred_res/X outputs are not written by this build yet (as with -e -d)\n");
+    if (print_X) fprintf(stderr, "note: the X output (-r) is not written by this build\n");
+    const int want_hat = print_hat || (print_pred_res && filter);
 
     snprintf(str, sizeof str, "Starting plom-smc with the following options: i = %d, J = %d, LIKE_MIN = %g, n_threads = %d",
              general_id, J, like_min, n_threads);
@@ -505,7 +506,7 @@ int main(int argc, char *argv[])
     d.N = N;
     d.precision = fp32 ? PLOM_FP32 : PLOM_FP64;
     d.noises_off = noises_off;
-    d.flags = filter ? 0u : PLOM_FLAG_NO_FILTER;
+    d.flags = (filter ? 0u : PLOM_FLAG_NO_FILTER) | (want_hat ? PLOM_FLAG_HAT : 0u);
     d.seed = (uint32_t)(seed_opt >= 0 ? seed_opt : 2 + general_id);
     d.stream = 0;
     d.dt = dt;
@@ -513,6 +514,12 @@ int main(int argc, char *argv[])
     plom_filter_inputs in = {times, data, pv, x0};
     plom_filter_outputs out;
     memset(&out, 0, sizeof out);
+    double *ess_n = calloc((size_t)N + 1, sizeof(double));
+    double *hat = want_hat ? calloc((size_t)(N + 1) * mi.n_hat * 3, sizeof(double)) : NULL;
+    double *pres = want_hat ? calloc((size_t)N * N_TS * 2 + 1, sizeof(double)) : NULL;
+    out.ess_n = ess_n;
+    out.hat = hat;
+    out.pred_res = pres;
     if (plom_smc_run(ctx, &d, &in, &out) != PLOM_OK) die(plom_last_error());
     plom_ctx_destroy(ctx);
 
@@ -544,6 +551,61 @@ int main(int argc, char *argv[])
                 for (int g = 0; g < R[i].n_gp; g++) fprintf(ft, "%g,", f_inv(R[i].tr, R[i].mean[g], R[i].min[g], R[i].max[g]));
         fprintf(ft, "%g\n", out.loglik);
         fclose(ft);
+    }
+    /* hat_<id>.csv (header_hat, print.c:151-186; print_p_hat, print.c:445-573) */
+    const char *remainder = pj_string(pj_get(settings, "remainder"));
+    const pj *psz = pj_get(settings, "POP_SIZE_EQ_SUM_SV");
+    const int has_rem = !(psz && psz->type == PJ_TRUE) && remainder;
+    if (print_hat && hat) {
+        snprintf(str, sizeof str, "%shat_%d.csv", path, general_id);
+        FILE *fh = fopen(str, "w");
+        if (!fh) die("cannot write the hat file");
+        fprintf(fh, "time,");
+        for (int i = 0; i < n_sv; i++)
+            for (int cac = 0; cac < N_CAC; cac++) {
+                const char *nm = pj_string(pj_at(par_sv, i)), *cn = pj_string(pj_at(cac_id, cac));
+                fprintf(fh, "low95:%s:%s,%s:%s,high95:%s:%s,", nm, cn, nm, cn, nm, cn);
+            }
+        if (has_rem)
+            for (int cac = 0; cac < N_CAC; cac++) {
+                const char *cn = pj_string(pj_at(cac_id, cac));
+                fprintf(fh, "low95:%s:%s,%s:%s,high95:%s:%s,", remainder, cn, remainder, cn, remainder, cn);
+            }
+        for (int ts = 0; ts < N_TS; ts++) {
+            const char *tn = pj_string(pj_at(ts_id, ts));
+            fprintf(fh, "low95:%s,%s,high95:%s%s", tn, tn, tn, ts < N_TS - 1 ? "," : "");
+        }
+        fprintf(fh, "\n");
+        for (int n = 0; n <= N; n++) {
+            const double *row = hat + (size_t)n * mi.n_hat * 3;
+            fprintf(fh, "%g,", times[n]);
+            for (int k = 0; k < mi.n_hat; k++)
+                fprintf(fh, "%g,%g,%g%s", row[3 * k], row[3 * k + 1], row[3 * k + 2], k < mi.n_hat - 1 ? "," : "");
+            fprintf(fh, "\n");
+        }
+        fclose(fh);
+    }
+    /* pred_res_<id>.csv (header_prediction_residuals, print.c:132-143; rows only where the
+     * interval has data, print.c:632-701 as called from smc.c:455-457) */
+    if (print_pred_res && pres && filter) {
+        snprintf(str, sizeof str, "%spred_res_%d.csv", path, general_id);
+        FILE *fp = fopen(str, "w");
+        if (!fp) die("cannot write the pred_res file");
+        fprintf(fp, "time,");
+        for (int ts = 0; ts < N_TS; ts++) {
+            const char *tn = pj_string(pj_at(ts_id, ts));
+            fprintf(fp, "mean:%s,res:%s,", tn, tn);
+        }
+        fprintf(fp, "ess\n");
+        for (int n = 0; n < N; n++) {
+            int nonan = 0;
+            for (int ts = 0; ts < N_TS; ts++) nonan += !isnan(data[n * N_TS + ts]);
+            if (!nonan) continue;
+            fprintf(fp, "%g,", times[n + 1]);
+            for (int ts = 0; ts < N_TS; ts++) fprintf(fp, "%g,%g,", pres[((size_t)n * N_TS + ts) * 2], pres[((size_t)n * N_TS + ts) * 2 + 1]);
+            fprintf(fp, "%g\n", ess_n[n]);
+        }
+        fclose(fp);
     }
     /* plom_print_done (print.c:33-67): guesses replaced by f_inv(mean) */
     for (int i = 0; i < n_par; i++) {

@@ -97,3 +97,38 @@ def test_cli_run_matches_api(tmp_path, ctx):
     assert r.returncode == 0 and not r.stdout.endswith("\n")
     echoed = json.loads(r.stdout)
     assert echoed["parameter"]["beta"]["group"]["all"]["guess"]["value"] == pytest.approx(0.5, rel=1e-15)
+
+
+@pytest.mark.gpu
+def test_cli_hat_and_pred_res_files(tmp_path, ctx):
+    """By default `smc` writes hat_<id>.csv (rows t = 0 and every interval; low95, mean,
+    high95 of the states, the remainder and the observation means; header_hat,
+    print.c:151-186) and pred_res_<id>.csv (rows for the intervals with data;
+    print.c:132-143), with the values of the library's hat / residual outputs."""
+    d = model_dir(tmp_path, "sir")
+    theta = os.path.join(PKG, "examples", "sir", "theta.json")
+    dry = json.loads(run_cli(d, ["sde", "-J", "2000", "-o", "30", "--dry_run"], theta).stdout)
+    r = run_cli(d, ["sde", "-J", "2000", "-o", "30", "-i", "3"], theta)
+    assert r.returncode == 0, r.stderr
+    hat = open(d / "hat_3.csv").read().splitlines()
+    assert hat[0] == ("time,low95:S:city1__all,S:city1__all,high95:S:city1__all,"
+                      "low95:I:city1__all,I:city1__all,high95:I:city1__all,"
+                      "low95:R:city1__all,R:city1__all,high95:R:city1__all,"
+                      "low95:city1__CDC__inc,city1__CDC__inc,high95:city1__CDC__inc")
+    assert len(hat) == 1 + 31
+    pres = open(d / "pred_res_3.csv").read().splitlines()
+    assert pres[0] == "time,mean:city1__CDC__inc,res:city1__CDC__inc,ess"
+    assert len(pres) == 1 + 30
+    m = smc.Model.get("sir")
+    times, data = m.dataset(30)
+    f = smc.Filter(ctx, m, 2000, times, data, np.array(dry["pv"]), np.array(dry["x0"]), flags=smc.FLAG_HAT,
+                   seed=2 + 3, stream=0)
+    f.run()
+    g = f.read(hat=True)
+    f.close()
+    rows = np.array([[float(x) for x in line.split(",")] for line in hat[1:]])
+    np.testing.assert_allclose(rows[:, 0], times, rtol=1e-6)
+    np.testing.assert_allclose(rows[:, 1:], g.hat.reshape(31, -1), rtol=1e-5, atol=1e-9)
+    prow = np.array([[float(x) for x in line.split(",")] for line in pres[1:]])
+    np.testing.assert_allclose(prow[:, 1:-1], g.pred_res.reshape(30, -1), rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(prow[:, -1], g.ess_n, rtol=1e-5)

@@ -44,7 +44,7 @@ from pkgload import load_package  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
 METRIC = "particle-steps/s & p50/p99 loglik latency, batched+served vs plain SMC"
-LEGS = ("batched", "plain", "served", "single", "cpu")
+LEGS = ("batched", "plain", "served", "single", "memory", "cpu")
 
 
 def log(msg):
@@ -141,7 +141,9 @@ def plain_leg(filters):
     return {"wall_s": time.perf_counter() - t0, "loglik": np.array(ll)}
 
 
-def served_leg(smc, ctx, inp, configs, seconds, rate, check_n=16):
+def served_leg(smc, ctx, inp, configs, seconds, rate, check_n=16, heavy=None):
+    """heavy: an extra request (dict with model, J, N, stream) submitted with the first
+    arrival, with lineage, served alongside the stream."""
     reqs = configs.served_mix(rate=rate, duration_s=seconds)
     prepared = [inp.get(r["model"], r["N"], r["params"]) for r in reqs]
     srv = smc.Server(ctx)
@@ -156,7 +158,12 @@ def served_leg(smc, ctx, inp, configs, seconds, rate, check_n=16):
         srv.result(rid)
     sub_t, lat, res = {}, {}, {}
     i, peak_active = 0, 0
+    heavy_id, heavy_out = None, {}
     t0 = time.perf_counter()
+    if heavy is not None:
+        m, times, data, pv, x0 = inp.get(heavy["model"], heavy["N"], configs.TRUTH[heavy["model"]])
+        heavy_id = srv.submit(m, heavy["J"], times, data, pv, x0, seed=configs.FILTER_SEED, stream=heavy["stream"],
+                              lineage=True)
     while True:
         now = time.perf_counter() - t0
         while i < len(reqs) and reqs[i]["t"] <= now:
@@ -169,10 +176,14 @@ def served_leg(smc, ctx, inp, configs, seconds, rate, check_n=16):
         if d:
             for rid in srv.done():
                 t_done = time.perf_counter() - t0
+                if rid == heavy_id:
+                    r = srv.result(rid)
+                    heavy_out = {"latency_s": t_done, "loglik": r.loglik, "lineage_rows": int(r.lineage.shape[0])}
+                    continue
                 ts, k = sub_t.pop(rid)
                 lat[k] = t_done - ts
                 res[k] = srv.result(rid).loglik
-        if i == len(reqs) and not sub_t:
+        if i == len(reqs) and not sub_t and (heavy_id is None or heavy_out):
             break
         if a == 0 and q == 0 and d == 0 and not sub_t and i < len(reqs):
             time.sleep(max(0.0, min(0.001, reqs[i]["t"] - (time.perf_counter() - t0))))
@@ -188,11 +199,33 @@ def served_leg(smc, ctx, inp, configs, seconds, rate, check_n=16):
         alone = smc.run(ctx, m, reqs[k]["J"], times, data, pv, x0, seed=configs.FILTER_SEED, stream=reqs[k]["stream"]).loglik
         max_rel = max(max_rel, abs(res[k] - alone) / abs(alone))
     srv.close()
-    return {"requests": len(reqs), "offered_particle_steps_per_s": ps / seconds, "makespan_s": makespan,
+    out = {"requests": len(reqs), "offered_particle_steps_per_s": ps / seconds, "makespan_s": makespan,
             "particle_steps": ps, "particle_steps_per_s": ps / makespan,
             "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
             "mean_ms": float(lat_ms.mean()), "max_ms": float(lat_ms.max()), "peak_active": peak_active,
             "checked_alone": len(sample), "max_rel_err_vs_alone": max_rel}
+    if heavy is not None:
+        out["heavy"] = heavy_out
+    return out
+
+
+def memory_leg(smc, ctx, inp, configs, J, N):
+    """The memory-heavy filter: SEIR, 10 gravity-coupled populations of 1e6 (S = 40), one
+    run in FP64 keeping the ancestry [N][J] on the device and returning one lineage."""
+    m, times, data, pv, x0 = inp.get("seir10", N, configs.TRUTH["seir10"])
+    f = smc.Filter(ctx, m, J, times, data, pv, x0, flags=smc.FLAG_LINEAGE, seed=configs.FILTER_SEED, stream=999)
+    dev = f.device_bytes()
+    f.run()                              # warm
+    ctx.sync()
+    t0 = time.perf_counter()
+    f.run()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    r = f.read(lineage=True)
+    f.close()
+    return {"workload": "seir10_j%d_n%d: SEIR 10x1e6 gravity-coupled, env noise, ancestry kept, lineage, FP64" % (J, N),
+            "value": round(J * N / wall, 1), "s_per_filter": round(wall, 3), "device_bytes": dev,
+            "loglik": r.loglik, "lineage_final_slot": int(r.lineage_slots[-1])}
 
 
 def single_leg(smc, ctx, inp, configs, precision, steps, warmup, J, N):
@@ -240,6 +273,8 @@ def main():
     ap.add_argument("--single-J", type=int, default=100000)
     ap.add_argument("--single-N", type=int, default=520)
     ap.add_argument("--cpu-filters", type=int, default=8)
+    ap.add_argument("--memory-J", type=int, default=1000000)
+    ap.add_argument("--memory-N", type=int, default=1040)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
@@ -336,6 +371,19 @@ def main():
         if "plain" in out:
             sv["vs_plain"] = round(sv["particle_steps_per_s"] / out["plain"]["value"], 2)
         out["served"] = sv
+    if "memory" in legs:
+        log("memory-heavy filter alone")
+        mem = memory_leg(smc, ctx, inp, configs, args.memory_J, args.memory_N)
+        log("memory-heavy filter served alongside the stream")
+        sv = served_leg(smc, ctx, inp, configs, args.served_seconds, args.served_rate, check_n=4,
+                        heavy={"model": "seir10", "J": args.memory_J, "N": args.memory_N, "stream": 999})
+        mem["served_alongside"] = {"heavy_latency_s": round(sv["heavy"]["latency_s"], 3),
+                                   "heavy_loglik_equals_alone": sv["heavy"]["loglik"] == mem["loglik"],
+                                   "p50_ms": round(sv["p50_ms"], 3), "p99_ms": round(sv["p99_ms"], 3),
+                                   "particle_steps_per_s": round((sv["particle_steps"] + args.memory_J * args.memory_N)
+                                                                 / sv["makespan_s"], 1),
+                                   "max_rel_err_vs_alone": sv["max_rel_err_vs_alone"]}
+        out["memory_heavy"] = mem
     if "single" in legs:
         log("single large filter")
         out["single"] = {"workload": "SIRS 4x5e5, J=%d, N=%d" % (args.single_J, args.single_N),

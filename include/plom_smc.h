@@ -169,9 +169,11 @@ int plom_batch_run(plom_ctx *ctx, int n_filters, plom_filter **filters, int use_
  * plom_server_step admits queued requests while their device bytes fit the budget
  * (FIFO), advances every admitted request by one interval in one batched launch round,
  * and retires finished ones; plom_server_status gives PLOM_SRV_QUEUED / RUNNING / DONE;
- * plom_server_result returns log-likelihood, n_all_fail, loglik_n and ess_n of a done
- * request and releases its device memory (a finished request counts against the budget
- * until then).  Results equal plom_smc_run on the same descriptor and inputs (FP64: bit
+ * plom_server_result returns log-likelihood, n_all_fail, loglik_n and ess_n (and the
+ * lineage of a PLOM_FLAG_LINEAGE request) of a done request and releases its device memory
+ * (a finished request counts against the budget until then).  Requests whose work per
+ * interval reaches the heavy threshold step in a lane of their own, concurrently with the
+ * others, so that they do not hold back the small requests.  Results equal plom_smc_run on the same descriptor and inputs (FP64: bit
  * for bit), whatever else is being served. */
 #define PLOM_SRV_QUEUED 0
 #define PLOM_SRV_RUNNING 1
@@ -179,7 +181,10 @@ int plom_batch_run(plom_ctx *ctx, int n_filters, plom_filter **filters, int use_
 typedef struct {
     int64_t hbm_budget_bytes;    /* 0: 80 % of the free device memory at creation */
     int32_t max_active;          /* 0: 4096 */
-    int32_t max_inflight_steps;  /* 0: 4; plom_server_step blocks while that many steps are queued */
+    int32_t max_inflight_steps;  /* 0: 4; per lane: a lane with that many steps queued waits */
+    int64_t heavy_threshold;     /* J * size_proj from which a request runs in the heavy lane
+                                    (its own stream, stepping independently of the small
+                                    requests); 0: 4e6; negative: one lane for all */
 } plom_server_config;
 typedef struct plom_server plom_server;
 int plom_server_create(plom_ctx *ctx, const plom_server_config *cfg, plom_server **out);

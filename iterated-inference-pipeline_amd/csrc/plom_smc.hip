@@ -1031,8 +1031,10 @@ static size_t image_bytes(const layout *L) { return L->total - L->off_img; }
  * With `pinned` (at least image_bytes, page-locked, kept by the caller until the stream
  * has passed the upload) the allocation and the copy are stream-ordered and nothing
  * waits; without it the call returns when the filter is on the device. */
-static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom_filter_inputs *in, char *pinned, plom_filter **out)
+static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom_filter_inputs *in, char *pinned,
+                              plom_filter **out, hipStream_t st = nullptr)
 {
+    if (!st && c) st = c->stream;
     if (!c || !out || !in) return set_err(PLOM_ERR_ARG, "null argument");
     plom_model_info mi;
     int rc = check_desc(d, &mi);
@@ -1059,7 +1061,7 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     f->variant = (int)(d->noises_off & 3u);
     f->dev_bytes = L.total;
     f->async_alloc = async;
-    hipError_t e = async ? hipMallocAsync((void **)&f->dev, L.total, c->stream) : hipMalloc((void **)&f->dev, L.total);
+    hipError_t e = async ? hipMallocAsync((void **)&f->dev, L.total, st) : hipMalloc((void **)&f->dev, L.total);
     if (e != hipSuccess) {
         delete f;
         return set_err(PLOM_ERR_NOMEM, "hipMalloc(%zu): %s", L.total, hipGetErrorString(e));
@@ -1119,10 +1121,10 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     } else {
         memcpy(at(L.off_x0), in->x0, (size_t)S * 8);
     }
-    e = hipMemcpyAsync(b + L.off_img, img, image_bytes(&L), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess && !async) e = hipStreamSynchronize(c->stream);
+    e = hipMemcpyAsync(b + L.off_img, img, image_bytes(&L), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !async) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
-        if (async) (void)hipFreeAsync(f->dev, c->stream); else (void)hipFree(f->dev);
+        if (async) (void)hipFreeAsync(f->dev, st); else (void)hipFree(f->dev);
         delete f;
         return set_err(PLOM_ERR_HIP, "filter upload: %s", hipGetErrorString(e));
     }
@@ -1622,21 +1624,24 @@ extern "C" int plom_batch_run(plom_ctx *c, int nf, plom_filter **fs, int use_gra
 /* ------------------------------------------------------------------ server */
 /* Continuous batching: requests wait in a FIFO on the host; at every step the server
  * admits the queue head while its device bytes fit the HBM budget (and the slot limit),
- * then advances every active request by one observation interval with one batched
+ * then advances the requests of each lane by one observation interval with one batched
  * launch round (one propagate launch per (model, precision, variant) group, then the
  * weight / scan / advance launches over all of them), and retires the requests whose last
  * interval was just enqueued (a HIP event marks their completion).  A request's result
  * depends only on its descriptor and inputs (its random key is its own stream id), never
  * on the mix around it.
  *
- * Nothing in a step waits for the device except the in-flight bound: step s waits for
- * step s - max_inflight to finish, so the host stays at most that many intervals ahead
- * and a new request is admitted within that many intervals.  Host-to-device uploads come
- * from a pool of page-locked blocks, released once the step that used them is known to
- * be complete. */
+ * Two lanes: requests whose work per interval (J x size_proj) reaches the heavy threshold
+ * run in lane 1, on a stream of their own, so that the short intervals of the small
+ * requests (lane 0, the context stream) do not wait behind theirs; the two lanes run
+ * concurrently on the device.  Each lane keeps at most max_inflight_steps steps queued;
+ * a step skips a lane whose window is full and blocks only when no lane can advance.
+ * Host-to-device uploads come from a pool of page-locked blocks, released once the lane
+ * step that used them is known to be complete. */
 #define SRV_QUEUED 0
 #define SRV_RUNNING 1
 #define SRV_DONE 2
+#define SRV_LANES 2
 
 /* page-locked host blocks in power-of-two size classes, reused, freed at the end */
 struct pinned_pool {
@@ -1678,98 +1683,133 @@ struct srv_request {
     std::vector<double> times, data, pv, x0;
     plom_filter *f = nullptr;
     int slot = -1;
+    int lane = 0;
     int progress = 0;          /* intervals enqueued */
     int state = SRV_QUEUED;
     hipEvent_t done_ev = nullptr;
     int64_t bytes = 0;
 };
 
+struct srv_lane {
+    hipStream_t stream = nullptr;
+    std::vector<srv_request *> active;
+    int2 *d_tiles = nullptr;                   /* tile maps: all tiles, then per group */
+    size_t tiles_cap = 0;
+    int *d_slots_active = nullptr;             /* slot index of every active request */
+    int dirty = 1;                             /* active set changed: rebuild the maps */
+    struct group { int model, prec, variant; size_t off, n; };
+    std::vector<group> groups;
+    size_t n_all = 0;
+    int64_t steps = 0;
+    std::vector<hipEvent_t> step_ev;           /* ring of max_inflight_steps events */
+    std::vector<std::vector<std::pair<char *, int>>> step_release;  /* pinned blocks per ring slot */
+};
+
 struct plom_server {
     plom_ctx *c;
-    int64_t budget, in_use;
+    int64_t budget, in_use, heavy;
     int max_active;
     uint64_t next_id;
     std::vector<srv_request *> queue;          /* FIFO */
-    std::vector<srv_request *> active;
     std::vector<srv_request *> finished;       /* waiting for their event */
     std::vector<srv_request *> done;           /* complete, result not taken yet */
     int capacity;                              /* descriptor slots on the device */
     plom_dfilter *d_slots;
     std::vector<int> free_slots;
-    int2 *d_tiles;                             /* tile maps: all tiles, then per group */
-    size_t tiles_cap;
-    int *d_slots_active;                       /* slot index of every active request */
-    int dirty;                                 /* active set changed: rebuild the maps */
-    struct group { int model, prec, variant; size_t off, n; };
-    std::vector<group> groups;
-    size_t n_all;
-    int64_t steps;
-    int depth;                                 /* max steps in flight */
-    std::vector<hipEvent_t> step_ev;           /* ring of depth events */
-    std::vector<std::vector<std::pair<char *, int>>> step_release;  /* pinned blocks per ring slot */
+    int depth;                                 /* max steps in flight per lane */
+    srv_lane lanes[SRV_LANES];
     pinned_pool pool;
 };
 
-/* step s may reuse ring slot s % depth once the step that used it last is complete */
-static int srv_ring_wait(plom_server *S, int64_t s)
+static int srv_n_active(const plom_server *S)
 {
-    const size_t k = (size_t)(s % S->depth);
-    if (s >= S->depth) HIPCHK(hipEventSynchronize(S->step_ev[k]));
-    for (auto &pk : S->step_release[k]) S->pool.put(pk.first, pk.second);
-    S->step_release[k].clear();
-    return PLOM_OK;
+    int n = 0;
+    for (int l = 0; l < SRV_LANES; l++) n += (int)S->lanes[l].active.size();
+    return n;
 }
 
-static int srv_stage(plom_server *S, const void *src, size_t bytes, void *dst)
+/* the lane may start its next step: its ring slot is free (the step that used it last
+ * has completed); then the slot's pinned blocks go back to the pool */
+static int srv_lane_ready(plom_server *S, srv_lane *L, int block)
+{
+    const size_t k = (size_t)(L->steps % S->depth);
+    if (L->steps >= S->depth) {
+        hipEvent_t e = L->step_ev[k];
+        if (block) {
+            HIPCHK(hipEventSynchronize(e));
+        } else {
+            hipError_t q = hipEventQuery(e);
+            if (q == hipErrorNotReady) { (void)hipGetLastError(); return 0; }
+            if (q != hipSuccess) return set_err(PLOM_ERR_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+        }
+    }
+    for (auto &pk : L->step_release[k]) S->pool.put(pk.first, pk.second);
+    L->step_release[k].clear();
+    return 1;
+}
+
+static int srv_stage(plom_server *S, srv_lane *L, const void *src, size_t bytes, void *dst)
 {
     char *p;
     int k;
     int rc = S->pool.get(bytes, &p, &k);
     if (rc != PLOM_OK) return rc;
     memcpy(p, src, bytes);
-    S->step_release[(size_t)(S->steps % S->depth)].push_back({p, k});
-    HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, S->c->stream));
+    L->step_release[(size_t)(L->steps % S->depth)].push_back({p, k});
+    HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, L->stream));
     return PLOM_OK;
 }
 
-static int srv_rebuild_maps(plom_server *S)
+static int srv_rebuild_maps(plom_server *S, srv_lane *L)
 {
-    plom_ctx *c = S->c;
     std::vector<int2> h;
-    S->groups.clear();
-    for (auto r : S->active)
+    L->groups.clear();
+    for (auto r : L->active)
         for (int t = 0; t < r->f->h.T; t++) h.push_back(make_int2(r->slot, t));
-    S->n_all = h.size();
-    for (auto r : S->active) {
+    L->n_all = h.size();
+    for (auto r : L->active) {
         bool seen = false;
-        for (auto &gg : S->groups)
+        for (auto &gg : L->groups)
             if (gg.model == r->desc.model_id && gg.prec == r->desc.precision && gg.variant == r->f->variant) seen = true;
-        if (!seen) S->groups.push_back(plom_server::group{r->desc.model_id, r->desc.precision, r->f->variant, 0, 0});
+        if (!seen) L->groups.push_back(srv_lane::group{r->desc.model_id, r->desc.precision, r->f->variant, 0, 0});
     }
-    for (auto &g : S->groups) {
+    for (auto &g : L->groups) {
         g.off = h.size();
-        for (auto r : S->active)
+        for (auto r : L->active)
             if (g.model == r->desc.model_id && g.prec == r->desc.precision && g.variant == r->f->variant)
                 for (int t = 0; t < r->f->h.T; t++) h.push_back(make_int2(r->slot, t));
         g.n = h.size() - g.off;
     }
-    if (h.size() > S->tiles_cap) {
-        if (S->d_tiles) HIPCHK(hipFreeAsync(S->d_tiles, c->stream));
-        S->tiles_cap = h.size() * 2 + 1024;
-        HIPCHK(hipMallocAsync((void **)&S->d_tiles, S->tiles_cap * sizeof(int2), c->stream));
+    if (h.size() > L->tiles_cap) {
+        if (L->d_tiles) HIPCHK(hipFreeAsync(L->d_tiles, L->stream));
+        L->tiles_cap = h.size() * 2 + 1024;
+        HIPCHK(hipMallocAsync((void **)&L->d_tiles, L->tiles_cap * sizeof(int2), L->stream));
     }
     if (!h.empty()) {
-        int rc = srv_stage(S, h.data(), h.size() * sizeof(int2), S->d_tiles);
+        int rc = srv_stage(S, L, h.data(), h.size() * sizeof(int2), L->d_tiles);
         if (rc != PLOM_OK) return rc;
     }
     std::vector<int> ha;
-    for (auto r : S->active) ha.push_back(r->slot);
+    for (auto r : L->active) ha.push_back(r->slot);
     if (!ha.empty()) {
-        int rc = srv_stage(S, ha.data(), ha.size() * sizeof(int), S->d_slots_active);
+        int rc = srv_stage(S, L, ha.data(), ha.size() * sizeof(int), L->d_slots_active);
         if (rc != PLOM_OK) return rc;
     }
-    S->dirty = 0;
+    L->dirty = 0;
     return PLOM_OK;
+}
+
+static void srv_free(plom_server *S)
+{
+    for (int l = 0; l < SRV_LANES; l++) {
+        srv_lane *L = &S->lanes[l];
+        for (auto e : L->step_ev) (void)hipEventDestroy(e);
+        if (L->d_slots_active) (void)hipFree(L->d_slots_active);
+        if (l > 0 && L->stream) (void)hipStreamDestroy(L->stream);
+    }
+    if (S->d_slots) (void)hipFree(S->d_slots);
+    S->pool.release_all();
+    delete S;
 }
 
 extern "C" int plom_server_create(plom_ctx *c, const plom_server_config *cfg, plom_server **out)
@@ -1785,23 +1825,31 @@ extern "C" int plom_server_create(plom_ctx *c, const plom_server_config *cfg, pl
     S->budget = (cfg && cfg->hbm_budget_bytes > 0) ? cfg->hbm_budget_bytes : (int64_t)(0.8 * (double)free_b);
     S->max_active = (cfg && cfg->max_active > 0) ? cfg->max_active : 4096;
     S->depth = (cfg && cfg->max_inflight_steps > 0) ? cfg->max_inflight_steps : 4;
+    S->heavy = (cfg && cfg->heavy_threshold != 0) ? cfg->heavy_threshold : (int64_t)4000000;
     S->capacity = S->max_active;
     S->in_use = 0;
     S->next_id = 1;
-    S->d_tiles = nullptr;
-    S->tiles_cap = 0;
-    S->dirty = 1;
-    S->steps = 0;
-    S->step_release.resize((size_t)S->depth);
-    for (int i = 0; i < S->depth; i++) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        S->step_ev.push_back(e);
-    }
+    S->d_slots = nullptr;
     hipError_t e = hipMalloc((void **)&S->d_slots, (size_t)S->capacity * sizeof(plom_dfilter));
-    if (e != hipSuccess) { delete S; return set_err(PLOM_ERR_NOMEM, "hipMalloc slots: %s", hipGetErrorString(e)); }
-    e = hipMalloc((void **)&S->d_slots_active, (size_t)S->capacity * sizeof(int));
-    if (e != hipSuccess) { (void)hipFree(S->d_slots); delete S; return set_err(PLOM_ERR_NOMEM, "hipMalloc slots: %s", hipGetErrorString(e)); }
+    if (e != hipSuccess) { srv_free(S); return set_err(PLOM_ERR_NOMEM, "hipMalloc slots: %s", hipGetErrorString(e)); }
+    for (int l = 0; l < SRV_LANES; l++) {
+        srv_lane *L = &S->lanes[l];
+        if (l == 0) {
+            L->stream = c->stream;
+        } else {
+            e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking);
+            if (e != hipSuccess) { L->stream = nullptr; srv_free(S); return set_err(PLOM_ERR_HIP, "heavy lane stream: %s", hipGetErrorString(e)); }
+        }
+        e = hipMalloc((void **)&L->d_slots_active, (size_t)S->capacity * sizeof(int));
+        if (e != hipSuccess) { L->d_slots_active = nullptr; srv_free(S); return set_err(PLOM_ERR_NOMEM, "hipMalloc slots: %s", hipGetErrorString(e)); }
+        L->step_release.resize((size_t)S->depth);
+        for (int i = 0; i < S->depth; i++) {
+            hipEvent_t ev;
+            e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+            if (e != hipSuccess) { srv_free(S); return set_err(PLOM_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e)); }
+            L->step_ev.push_back(ev);
+        }
+    }
     for (int i = S->capacity - 1; i >= 0; i--) S->free_slots.push_back(i);
     *out = S;
     return PLOM_OK;
@@ -1831,6 +1879,7 @@ extern "C" int plom_server_submit(plom_server *S, const plom_filter_desc *d, con
     r->pv.assign(in->pv, in->pv + mi.pv_size);
     r->x0.assign(in->x0, in->x0 + mi.size_proj);
     r->bytes = (int64_t)L.total;
+    r->lane = (S->heavy > 0 && (int64_t)d->J * mi.size_proj >= S->heavy) ? 1 : 0;
     S->queue.push_back(r);
     *req_id = r->id;
     return PLOM_OK;
@@ -1856,7 +1905,45 @@ static int srv_poll(plom_server *S)
     return PLOM_OK;
 }
 
-/* admit, advance every active request one interval, retire; returns counts */
+/* one interval for every active request of a lane whose ring has room */
+static int srv_lane_step(plom_server *S, srv_lane *L)
+{
+    plom_ctx *c = S->c;
+    if (L->dirty) {
+        int rc = srv_rebuild_maps(S, L);
+        if (rc != PLOM_OK) return rc;
+    }
+    (void)hipGetLastError();
+    for (auto &g : L->groups)
+        g_models[g.model].launch_prop(g.prec, g.variant, (int)g.n, L->stream, S->d_slots, L->d_tiles + g.off);
+    enqueue_resample(L->stream, S->d_slots, (const int2 *)L->d_tiles, (int)L->n_all, L->d_slots_active, (int)L->active.size());
+    HIPCHK(hipGetLastError());
+    for (size_t i = 0; i < L->active.size();) {
+        srv_request *r = L->active[i];
+        r->progress++;
+        if (r->progress >= r->desc.N) {
+            /* the slot descriptor carries the interval counter: copy it back, mark done */
+            if (r->desc.N > 0 && (r->desc.flags & PLOM_FLAG_LINEAGE))
+                g_models[r->desc.model_id].launch_lineage(r->desc.precision, r->f->variant, L->stream, S->d_slots + r->slot);
+            HIPCHK(hipMemcpyAsync(r->f->df, S->d_slots + r->slot, sizeof(plom_dfilter), hipMemcpyDeviceToDevice, L->stream));
+            HIPCHK(hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming));
+            HIPCHK(hipEventRecord(r->done_ev, L->stream));
+            r->f->has_run = 1;
+            S->free_slots.push_back(r->slot);
+            S->finished.push_back(r);
+            L->active.erase(L->active.begin() + (long)i);
+            L->dirty = 1;
+        } else {
+            i++;
+        }
+    }
+    HIPCHK(hipEventRecord(L->step_ev[(size_t)(L->steps % S->depth)], L->stream));
+    L->steps++;
+    (void)c;
+    return PLOM_OK;
+}
+
+/* admit, advance every lane that can start a step by one interval, retire; returns counts */
 extern "C" int plom_server_step(plom_server *S, int32_t *n_active, int32_t *n_queued, int32_t *n_done)
 {
     if (!S) return set_err(PLOM_ERR_ARG, "null server");
@@ -1864,69 +1951,65 @@ extern "C" int plom_server_step(plom_server *S, int32_t *n_active, int32_t *n_qu
     HIPCHK(hipSetDevice(c->device));
     int rc = srv_poll(S);
     if (rc != PLOM_OK) return rc;
-    if (!S->queue.empty() || !S->active.empty()) {
-        rc = srv_ring_wait(S, S->steps);
-        if (rc != PLOM_OK) return rc;
+    /* which lanes can take a step now (a lane with queued work counts as busy) */
+    int ready[SRV_LANES];
+    for (int l = 0; l < SRV_LANES; l++) {
+        ready[l] = 0;
+        srv_lane *L = &S->lanes[l];
+        bool wants = !L->active.empty();
+        for (auto r : S->queue) wants = wants || r->lane == l;
+        if (!wants) continue;
+        rc = srv_lane_ready(S, L, 0);
+        if (rc < 0) return rc;
+        ready[l] = rc;
     }
-    /* admission by free HBM, FIFO */
-    while (!S->queue.empty() && (int)S->active.size() < S->max_active) {
+    if (!ready[0] && !ready[1]) {
+        /* nothing can start: wait for the lane that will free a slot first (the light one
+         * if it is busy) */
+        for (int l = 0; l < SRV_LANES; l++) {
+            srv_lane *L = &S->lanes[l];
+            bool wants = !L->active.empty();
+            for (auto r : S->queue) wants = wants || r->lane == l;
+            if (!wants) continue;
+            rc = srv_lane_ready(S, L, 1);
+            if (rc < 0) return rc;
+            ready[l] = 1;
+            break;
+        }
+    }
+    /* admission by free HBM, FIFO: the queue head blocks the queue */
+    while (!S->queue.empty() && srv_n_active(S) < S->max_active) {
         srv_request *r = S->queue.front();
-        if (S->in_use + r->bytes > S->budget) break;
+        if (S->in_use + r->bytes > S->budget || !ready[r->lane]) break;
+        srv_lane *L = &S->lanes[r->lane];
         plom_model_info mi = g_models[r->desc.model_id].info;
-        layout L;
-        make_layout(&r->desc, &mi, &L);
+        layout Lo;
+        make_layout(&r->desc, &mi, &Lo);
         char *img;
         int k;
-        rc = S->pool.get(image_bytes(&L), &img, &k);
+        rc = S->pool.get(image_bytes(&Lo), &img, &k);
         if (rc != PLOM_OK) return rc;
-        S->step_release[(size_t)(S->steps % S->depth)].push_back({img, k});
+        L->step_release[(size_t)(L->steps % S->depth)].push_back({img, k});
         plom_filter_inputs in = {r->times.data(), r->data.data(), r->pv.data(), r->x0.data()};
-        rc = filter_create_impl(c, &r->desc, &in, img, &r->f);
+        rc = filter_create_impl(c, &r->desc, &in, img, &r->f, L->stream);
         if (rc != PLOM_OK) return rc;
         r->slot = S->free_slots.back();
         S->free_slots.pop_back();
         /* the slot descriptor starts as the filter's own (uploaded just before) */
-        HIPCHK(hipMemcpyAsync(S->d_slots + r->slot, r->f->df, sizeof(plom_dfilter), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(S->d_slots + r->slot, r->f->df, sizeof(plom_dfilter), hipMemcpyDeviceToDevice, L->stream));
         S->in_use += r->bytes;
         r->state = SRV_RUNNING;
-        S->active.push_back(r);
+        L->active.push_back(r);
         S->queue.erase(S->queue.begin());
-        S->dirty = 1;
+        L->dirty = 1;
     }
-    if (S->dirty) {
-        rc = srv_rebuild_maps(S);
+    for (int l = 0; l < SRV_LANES; l++) {
+        srv_lane *L = &S->lanes[l];
+        if (!ready[l] || L->active.empty()) continue;
+        rc = srv_lane_step(S, L);
         if (rc != PLOM_OK) return rc;
     }
-    /* one interval for every active request */
-    (void)hipGetLastError();
-    if (!S->active.empty()) {
-        for (auto &g : S->groups)
-            g_models[g.model].launch_prop(g.prec, g.variant, (int)g.n, c->stream, S->d_slots, S->d_tiles + g.off);
-        enqueue_resample(c->stream, S->d_slots, (const int2 *)S->d_tiles, (int)S->n_all, S->d_slots_active, (int)S->active.size());
-        HIPCHK(hipGetLastError());
-        for (size_t i = 0; i < S->active.size();) {
-            srv_request *r = S->active[i];
-            r->progress++;
-            if (r->progress >= r->desc.N) {
-                /* the slot descriptor carries the interval counter: copy it back, mark done */
-                if (r->desc.N > 0 && (r->desc.flags & PLOM_FLAG_LINEAGE))
-                    g_models[r->desc.model_id].launch_lineage(r->desc.precision, r->f->variant, c->stream, S->d_slots + r->slot);
-                HIPCHK(hipMemcpyAsync(r->f->df, S->d_slots + r->slot, sizeof(plom_dfilter), hipMemcpyDeviceToDevice, c->stream));
-                HIPCHK(hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming));
-                HIPCHK(hipEventRecord(r->done_ev, c->stream));
-                r->f->has_run = 1;
-                S->free_slots.push_back(r->slot);
-                S->finished.push_back(r);
-                S->active.erase(S->active.begin() + (long)i);
-                S->dirty = 1;
-            } else {
-                i++;
-            }
-        }
-        HIPCHK(hipEventRecord(S->step_ev[(size_t)(S->steps % S->depth)], c->stream));
-        S->steps++;
-    }
-    if (n_active) *n_active = (int32_t)S->active.size();
+    if (n_active) *n_active = (int32_t)srv_n_active(S);
     if (n_queued) *n_queued = (int32_t)S->queue.size();
     if (n_done) *n_done = (int32_t)S->done.size();
     return PLOM_OK;
@@ -1934,7 +2017,7 @@ extern "C" int plom_server_step(plom_server *S, int32_t *n_active, int32_t *n_qu
 
 static srv_request *srv_find(plom_server *S, uint64_t id, std::vector<srv_request *> **where)
 {
-    std::vector<srv_request *> *lists[] = {&S->queue, &S->active, &S->finished, &S->done};
+    std::vector<srv_request *> *lists[] = {&S->queue, &S->lanes[0].active, &S->lanes[1].active, &S->finished, &S->done};
     for (auto L : lists)
         for (auto r : *L)
             if (r->id == id) { if (where) *where = L; return r; }
@@ -1968,8 +2051,8 @@ extern "C" int plom_server_done(plom_server *S, uint64_t *ids, int32_t cap, int3
 extern "C" int plom_server_result(plom_server *S, uint64_t id, plom_filter_outputs *o)
 {
     if (!S || !o) return set_err(PLOM_ERR_ARG, "null argument");
-    if (o->ancestry || o->final_state || o->final_logw)
-        return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods and lineages (no ancestry / final state)");
+    if (o->ancestry || o->final_state || o->final_logw || o->hat || o->pred_res)
+        return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods and lineages (no ancestry, final state or hat)");
     std::vector<srv_request *> *where = nullptr;
     srv_request *r = srv_find(S, id, &where);
     if (!r) return set_err(PLOM_ERR_ARG, "unknown request %llu", (unsigned long long)id);
@@ -1980,7 +2063,7 @@ extern "C" int plom_server_result(plom_server *S, uint64_t id, plom_filter_outpu
     int rc = filter_read_on(S->c, r->f, o, S->c->io);
     if (rc != PLOM_OK) return rc;
     HIPCHK(hipEventDestroy(r->done_ev));
-    HIPCHK(hipFreeAsync(r->f->dev, S->c->stream));
+    HIPCHK(hipFreeAsync(r->f->dev, S->lanes[r->lane].stream));
     S->in_use -= r->bytes; /* the request's device memory counts until its result is taken */
     delete r->f;
     where->erase(std::find(where->begin(), where->end(), r));
@@ -1992,21 +2075,22 @@ extern "C" int plom_server_destroy(plom_server *S)
 {
     if (!S) return PLOM_OK;
     plom_ctx *c = S->c;
-    HIPCHK(hipStreamSynchronize(c->stream));
-    std::vector<srv_request *> *lists[] = {&S->queue, &S->active, &S->finished, &S->done};
+    for (int l = 0; l < SRV_LANES; l++)
+        if (S->lanes[l].stream) HIPCHK(hipStreamSynchronize(S->lanes[l].stream));
+    std::vector<srv_request *> *lists[] = {&S->queue, &S->lanes[0].active, &S->lanes[1].active, &S->finished, &S->done};
     for (auto L : lists)
         for (auto r : *L) {
             if (r->done_ev) (void)hipEventDestroy(r->done_ev);
-            if (r->f) { (void)hipFreeAsync(r->f->dev, c->stream); delete r->f; }
+            if (r->f) { (void)hipFreeAsync(r->f->dev, S->lanes[r->lane].stream); delete r->f; }
             delete r;
         }
-    if (S->d_tiles) HIPCHK(hipFreeAsync(S->d_tiles, c->stream));
+    for (int l = 0; l < SRV_LANES; l++) {
+        srv_lane *L = &S->lanes[l];
+        if (L->d_tiles) HIPCHK(hipFreeAsync(L->d_tiles, L->stream));
+        HIPCHK(hipStreamSynchronize(L->stream));
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (auto e : S->step_ev) HIPCHK(hipEventDestroy(e));
-    S->pool.release_all();
-    HIPCHK(hipFree(S->d_slots));
-    HIPCHK(hipFree(S->d_slots_active));
-    delete S;
+    srv_free(S);
     return PLOM_OK;
 }
 

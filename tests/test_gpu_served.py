@@ -117,3 +117,28 @@ def test_done_list_and_inflight_bound(ctx):
         assert srv.result(rid).loglik == alone(ctx, name, J, N, stream).loglik
     assert srv.done() == []
     srv.close()
+
+
+def test_heavy_lane_equals_alone(ctx):
+    """A request at or above the heavy threshold steps in its own lane beside the small
+    ones; every result still equals its run alone."""
+    srv = smc.Server(ctx, heavy_threshold=30000)   # J * size_proj: sirs4 (S = 12) with J >= 2500
+    reqs = [("sirs4", 6000, 40, 41), ("sir", 800, 30, 42), ("sir", 1500, 12, 43), ("sirs4", 700, 20, 44),
+            ("sirs4", 3000, 10, 45)]
+    ids = []
+    for name, J, N, stream in reqs:
+        m, times, data, pv, x0 = inputs(name, N)
+        ids.append(srv.submit(m, J, times, data, pv, x0, seed=5, stream=stream))
+    got = {}
+    for _ in range(20000):
+        srv.step()
+        for rid in srv.done():
+            got[rid] = srv.result(rid)
+        if len(got) == len(ids):
+            break
+    assert len(got) == len(ids)
+    for rid, (name, J, N, stream) in zip(ids, reqs):
+        ref = alone(ctx, name, J, N, stream)
+        assert got[rid].loglik == ref.loglik
+        np.testing.assert_array_equal(got[rid].loglik_n, ref.loglik_n)
+    srv.close()

@@ -109,6 +109,8 @@ typedef struct {
     double *lineage;     /* optional [(N+1)*size_proj]: the sampled path, row n = its state at
                             times[n] (row 0 = x0); needs PLOM_FLAG_LINEAGE */
     int32_t *lineage_slots; /* optional [N]: the particle index of that path at each interval */
+    double *lineage_obs; /* optional [(N+1)*n_ts*2]: per row and series, the observation mean along
+                            the path and one observation drawn from the observation model */
     double *hat;         /* optional [(N+1)*n_hat*3]: per row (times[0..N]) and quantity: low95,
                             mean, high95 (PLOM_FLAG_HAT) */
     double *pred_res;    /* optional [N*n_ts*2]: per interval and series: prediction mean and
@@ -136,6 +138,11 @@ int plom_smc_run(plom_ctx *ctx, const plom_filter_desc *desc, const plom_filter_
 int plom_filter_create(plom_ctx *ctx, const plom_filter_desc *desc, const plom_filter_inputs *in,
                        plom_filter **out);
 int plom_filter_reset(plom_ctx *ctx, plom_filter *f);  /* back to interval 0, same inputs */
+/* new natural parameters [pv_size] and initial state [size_proj] (either may be NULL: kept)
+ * and a new random key, in place (no reallocation; batch plans holding the filter pick the
+ * change up at their next launch, graphs included): one pMCMC proposal (pmcmc.c:205-214) */
+int plom_filter_update(plom_ctx *ctx, plom_filter *f, const double *pv, const double *x0, uint32_t seed,
+                       uint32_t stream);
 int plom_filter_run(plom_ctx *ctx, plom_filter *f);    /* enqueue every interval */
 int plom_filter_read(plom_ctx *ctx, plom_filter *f, plom_filter_outputs *out); /* waits */
 int plom_filter_destroy(plom_ctx *ctx, plom_filter *f);

@@ -19,6 +19,7 @@
  *   PURPOSE_RESAMPLE gives the one systematic-resampling uniform of interval n
  *   (the role of gsl_ran_flat(calc[0]) in smc.c:100).
  *   PURPOSE_FULL gives the uniform that picks the final particle of a --full lineage.
+ *   PURPOSE_OBS gives the Gaussians of the observations drawn along a lineage.
  *
  * Uniforms are (x + 1/2) 2^-32 for a 32-bit x: never 0 or 1, exact in double.
  * Gaussians: Box-Muller pairs on two uniforms; the generated step code takes the two
@@ -37,6 +38,7 @@
 #define PLOM_PURPOSE_PROPAGATE 0u
 #define PLOM_PURPOSE_RESAMPLE 1u
 #define PLOM_PURPOSE_FULL 2u
+#define PLOM_PURPOSE_OBS 3u
 
 PLOM_HD uint32_t plom_mulhi32(uint32_t a, uint32_t b)
 {
@@ -178,6 +180,27 @@ PLOM_HD uint64_t plom_lineage_num(uint32_t seed, uint32_t stream)
     uint32_t w[4];
     plom_draw128(seed, stream, 0u, 0u, PLOM_PURPOSE_FULL, w);
     return 2ull * (uint64_t)w[0] + 1ull;
+}
+
+/* the observation row n of a lineage: out[2 ts] = mean, out[2 ts + 1] = max(mean + sd z, 0)
+ * with z the Box-Muller values of Philox(ctr = (0, n, 3, 0), key = (seed, stream)), two
+ * series per 128-bit draw (ctr word 3 counts the draws) */
+PLOM_HD void plom_lineage_obs(uint32_t seed, uint32_t stream, uint32_t n, int n_ts, const double *mean,
+                              const double *var, const double *tab, double *out)
+{
+    for (int ts = 0; ts < n_ts; ts += 2) {
+        uint32_t w[4];
+        const uint32_t ctr[4] = { 0u, n, PLOM_PURPOSE_OBS, (uint32_t)(ts / 2) };
+        const uint32_t key[2] = { seed, stream };
+        plom_philox4x32_10(ctr, key, w);
+        double z[2];
+        plom_bm_pair(w[0], w[1], tab, &z[0], &z[1]);
+        for (int k = 0; k < 2 && ts + k < n_ts; k++) {
+            const double y = mean[ts + k] + plom_sqrt(var[ts + k]) * z[k];
+            out[2 * (ts + k)] = mean[ts + k];
+            out[2 * (ts + k) + 1] = (y > 0.0) ? y : 0.0;
+        }
+    }
 }
 
 /* the resampling uniform of interval n as the integer numerator 2x+1 of U = (2x+1)/2^33 */

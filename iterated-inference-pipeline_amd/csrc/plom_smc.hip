@@ -92,6 +92,7 @@ struct plom_dfilter {
     int32_t *ancestry;  /* [N][J] or null */
     int32_t *lin_slots; /* [N] or null: the sampled lineage's particle index per interval */
     double *lin_state;  /* [(N+1)][S] or null: its states */
+    double *lin_obs;    /* [(N+1)][n_ts][2] or null: its observation mean and one drawn observation */
     int32_t n_hat, pad2;
     uint64_t *hkey;     /* [n_hat][Jp] or null: per tile, the sorted keys of the hat quantities */
     uint64_t *hpref;    /* [n_hat][Jp]: inclusive prefix of their weights in that order */
@@ -433,6 +434,16 @@ __global__ __launch_bounds__(64) void k_lineage(plom_dfilter *f)
         }
 #pragma unroll
         for (int k = 0; k < M::S; k++) f->lin_state[(size_t)(n + 1) * M::S + k] = (double)X[k];
+    }
+    /* observation mean and one drawn observation per row (print_p_X, print.c:312-341:
+     * obs_mean and observation(), core_template.c:369-412: mean + sd N(0,1), clipped at 0) */
+    for (int n = 0; n <= N; n++) {
+        double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
+#pragma unroll
+        for (int k = 0; k < M::S; k++) xs[k] = f->lin_state[(size_t)n * M::S + k];
+        M::proj2obs(xs, f->pv, obs);
+        M::obs_moments(obs, f->pv, f->times[n], mu, var);
+        plom_lineage_obs(f->seed, f->stream, (uint32_t)n, M::N_TS, mu, var, plom_bm_tab, f->lin_obs + (size_t)n * M::N_TS * 2);
     }
 }
 
@@ -866,6 +877,8 @@ struct plom_filter {
     plom_dfilter *df;       /* device copy of the descriptor */
     plom_dfilter h;         /* host copy (device pointers) */
     int has_run;
+    int dirty;              /* descriptor changed on the device since batch plans copied it */
+    size_t off_pv, off_pvf, off_x0;   /* image offsets of the parameter and initial-state inputs */
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -876,7 +889,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
  * server creates it) and starts from zeroed counters without memset calls. */
 struct layout {
     size_t off_X0, off_X1, off_logw, off_cum, off_tmax, off_tsum, off_tsum2, off_ttot, off_tincl, off_anc,
-        off_lslots, off_lstate, off_hkey, off_hpref, off_hpart, off_hat, off_pres,
+        off_lslots, off_lstate, off_lobs, off_hkey, off_hpref, off_hpart, off_hat, off_pres,
         off_img, off_df, off_cnt, off_scal, off_naf, off_llike_n, off_ess_n, off_rflag, off_nonan, off_times,
         off_data, off_pv, off_pvf, off_x0, total;
 };
@@ -901,6 +914,7 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     PUT(off_anc, (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_LINEAGE)) ? N * J * 4 : 0);
     PUT(off_lslots, (d->flags & PLOM_FLAG_LINEAGE) ? N * 4 : 0);
     PUT(off_lstate, (d->flags & PLOM_FLAG_LINEAGE) ? (N + 1) * S * 8 : 0);
+    PUT(off_lobs, (d->flags & PLOM_FLAG_LINEAGE) ? (N + 1) * (size_t)mi->n_ts * 2 * 8 : 0);
     const size_t NH = (d->flags & PLOM_FLAG_HAT) ? (size_t)mi->n_hat : 0;
     PUT(off_hkey, NH * Jp * 8);
     PUT(off_hpref, NH * Jp * 8);
@@ -1091,6 +1105,7 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     h->ancestry = (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_LINEAGE)) ? (int32_t *)(b + L.off_anc) : nullptr;
     h->lin_slots = (d->flags & PLOM_FLAG_LINEAGE) ? (int32_t *)(b + L.off_lslots) : nullptr;
     h->lin_state = (d->flags & PLOM_FLAG_LINEAGE) ? (double *)(b + L.off_lstate) : nullptr;
+    h->lin_obs = (d->flags & PLOM_FLAG_LINEAGE) ? (double *)(b + L.off_lobs) : nullptr;
     const int hat_on = (d->flags & PLOM_FLAG_HAT) != 0;
     h->n_hat = hat_on ? mi.n_hat : 0;
     h->hkey = hat_on ? (uint64_t *)(b + L.off_hkey) : nullptr;
@@ -1099,6 +1114,7 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     h->hat = hat_on ? (double *)(b + L.off_hat) : nullptr;
     h->pres = hat_on ? (double *)(b + L.off_pres) : nullptr;
     f->df = (plom_dfilter *)(b + L.off_df);
+    f->off_pv = L.off_pv; f->off_pvf = L.off_pvf; f->off_x0 = L.off_x0;
 
     /* host image: descriptor, zeroed counters and outputs, inputs */
     const int S = mi.size_proj;
@@ -1142,6 +1158,44 @@ extern "C" int plom_filter_reset(plom_ctx *c, plom_filter *f)
     if (!c || !f) return set_err(PLOM_ERR_ARG, "null argument");
     f->has_run = 0;
     return filter_reset_async(c, f);
+}
+
+extern "C" int plom_filter_update(plom_ctx *c, plom_filter *f, const double *pv, const double *x0, uint32_t seed,
+                                  uint32_t stream)
+{
+    if (!c || !f) return set_err(PLOM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const int S = f->info.size_proj, P = f->info.pv_size;
+    /* pv, pvf and x0 are consecutive regions of the image */
+    std::vector<char> img(f->off_x0 + (size_t)S * 8 - f->off_pv, 0);
+    char *base = img.data();
+    if (!pv || !x0) {
+        /* keep what is on the device for the part not given */
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(base, f->dev + f->off_pv, img.size(), hipMemcpyDeviceToHost));
+    }
+    if (pv) {
+        memcpy(base, pv, (size_t)P * 8);
+        float *pvf = (float *)(base + (f->off_pvf - f->off_pv));
+        for (int i = 0; i < P; i++) pvf[i] = (float)pv[i];
+    }
+    if (x0) {
+        char *xd = base + (f->off_x0 - f->off_pv);
+        if (f->desc.precision == PLOM_FP32) {
+            for (int k = 0; k < S; k++) ((float *)xd)[k] = (float)x0[k];
+        } else {
+            memcpy(xd, x0, (size_t)S * 8);
+        }
+    }
+    HIPCHK(hipMemcpyAsync(f->dev + f->off_pv, base, img.size(), hipMemcpyHostToDevice, c->stream));
+    f->desc.seed = seed;
+    f->desc.stream = stream;
+    f->h.seed = seed;
+    f->h.stream = stream;
+    HIPCHK(hipMemcpyAsync(f->df, &f->h, sizeof(plom_dfilter), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    f->dirty = 1;   /* has_run stays: the next run starts with a reset */
+    return PLOM_OK;
 }
 
 extern "C" int plom_filter_destroy(plom_ctx *c, plom_filter *f)
@@ -1232,8 +1286,10 @@ static int filter_read_on(plom_ctx *c, plom_filter *f, plom_filter_outputs *o, h
         if (o->hat) HIPCHK(hipMemcpyAsync(o->hat, h->hat, (N + 1) * (size_t)h->n_hat * 3 * 8, hipMemcpyDeviceToHost, s));
         if (o->pred_res) HIPCHK(hipMemcpyAsync(o->pred_res, h->pres, N * (size_t)h->n_ts * 2 * 8, hipMemcpyDeviceToHost, s));
     }
-    if ((o->lineage || o->lineage_slots) && N) {
+    if ((o->lineage || o->lineage_slots || o->lineage_obs) && N) {
         if (!h->lin_state) return set_err(PLOM_ERR_STATE, "lineage requested but PLOM_FLAG_LINEAGE not set");
+        if (o->lineage_obs)
+            HIPCHK(hipMemcpyAsync(o->lineage_obs, h->lin_obs, (N + 1) * (size_t)h->n_ts * 2 * 8, hipMemcpyDeviceToHost, s));
         if (o->lineage)
             HIPCHK(hipMemcpyAsync(o->lineage, h->lin_state, (N + 1) * (size_t)f->info.size_proj * 8, hipMemcpyDeviceToHost, s));
         if (o->lineage_slots) HIPCHK(hipMemcpyAsync(o->lineage_slots, h->lin_slots, N * 4, hipMemcpyDeviceToHost, s));
@@ -1472,6 +1528,12 @@ static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing, int rounds)
 
 static int batch_launch(plom_ctx *c, plom_batch *b, int use_graph)
 {
+    /* descriptors changed by plom_filter_update since the plan copied them */
+    for (auto &ch : b->chains)
+        for (size_t r = 0; r < ch.fs.size(); r++)
+            if (ch.fs[r]->dirty)
+                HIPCHK(hipMemcpyAsync(ch.ddf + r, ch.fs[r]->df, sizeof(plom_dfilter), hipMemcpyDeviceToDevice, c->stream));
+    for (auto f : b->fs) f->dirty = 0;
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     if (use_graph) {
         if (!b->graph) {

@@ -46,13 +46,14 @@ class FilterOutputs(C.Structure):
                 ("loglik_n", C.POINTER(C.c_double)), ("ess_n", C.POINTER(C.c_double)),
                 ("ancestry", C.POINTER(C.c_int32)), ("final_state", C.POINTER(C.c_double)),
                 ("final_logw", C.POINTER(C.c_double)), ("lineage", C.POINTER(C.c_double)),
-                ("lineage_slots", C.POINTER(C.c_int32)), ("hat", C.POINTER(C.c_double)),
-                ("pred_res", C.POINTER(C.c_double))]
+                ("lineage_slots", C.POINTER(C.c_int32)), ("lineage_obs", C.POINTER(C.c_double)),
+                ("hat", C.POINTER(C.c_double)), ("pred_res", C.POINTER(C.c_double))]
 
 
 EXPORTS = [
     "plom_last_error", "plom_ctx_create", "plom_ctx_destroy", "plom_model_count", "plom_model_find",
     "plom_model_name", "plom_model_info_get", "plom_smc_run", "plom_filter_create", "plom_filter_reset",
+    "plom_filter_update",
     "plom_filter_run", "plom_filter_read", "plom_filter_destroy", "plom_filter_device_bytes",
     "plom_batch_run", "plom_sync", "plom_ctx_set_kernel_timing", "plom_last_timing", "plom_selftest_math",
     "plom_selftest_normals", "plom_selftest_copy", "plom_batch_create", "plom_batch_launch",
@@ -93,6 +94,8 @@ def lib():
     L.plom_smc_run.argtypes = [C.c_void_p, C.POINTER(FilterDesc), C.POINTER(FilterInputs), C.POINTER(FilterOutputs)]
     L.plom_filter_create.argtypes = [C.c_void_p, C.POINTER(FilterDesc), C.POINTER(FilterInputs), C.POINTER(C.c_void_p)]
     L.plom_filter_reset.argtypes = [C.c_void_p, C.c_void_p]
+    L.plom_filter_update.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.c_uint32, C.c_uint32]
     L.plom_filter_run.argtypes = [C.c_void_p, C.c_void_p]
     L.plom_filter_read.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(FilterOutputs)]
     L.plom_filter_destroy.argtypes = [C.c_void_p, C.c_void_p]
@@ -320,12 +323,24 @@ class Filter:
         lsl = np.zeros(N, dtype=np.int32) if lineage else None
         ht = np.zeros((N + 1, self.model.n_hat, 3)) if hat else None
         pr = np.zeros((N, self.model.n_ts, 2)) if hat else None
+        lob = np.zeros((N + 1, self.model.n_ts, 2)) if lineage else None
         out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), iptr(anc), dptr(fs), dptr(fw), dptr(lin), iptr(lsl),
-                            dptr(ht), dptr(pr))
+                            dptr(lob), dptr(ht), dptr(pr))
         check(lib().plom_filter_read(self.ctx.h, self.h, C.byref(out)))
         r = FilterResult(out.loglik, out.n_all_fail, ll_n, ess, anc, fs, fw, lin, lsl)
-        r.hat, r.pred_res = ht, pr
+        r.hat, r.pred_res, r.lineage_obs = ht, pr, lob
         return r
+
+    def update(self, pv=None, x0=None, seed=1, stream=0):
+        """New parameters / initial state / random key in place (one pMCMC proposal)."""
+        pv = None if pv is None else np.ascontiguousarray(pv, dtype=np.float64)
+        x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+        if pv is not None and pv.shape != (self.model.layout["size"],):
+            raise ValueError("pv must have %d values" % self.model.layout["size"])
+        if x0 is not None and x0.shape != (self.model.S,):
+            raise ValueError("x0 must have %d values" % self.model.S)
+        check(lib().plom_filter_update(self.ctx.h, self.h, dptr(pv), dptr(x0), int(seed) & 0xffffffff,
+                                       int(stream) & 0xffffffff))
 
     def close(self):
         if self.h:
@@ -418,7 +433,7 @@ class Server:
         ess = np.zeros(N)
         lin = np.zeros((N + 1, S)) if S else None
         lsl = np.zeros(N, dtype=np.int32) if S else None
-        out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), None, None, None, dptr(lin), iptr(lsl))
+        out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), None, None, None, dptr(lin), iptr(lsl), None, None, None)
         check(lib().plom_server_result(self.h, rid, C.byref(out)))
         return FilterResult(out.loglik, out.n_all_fail, ll_n, ess, lineage=lin, lineage_slots=lsl)
 

@@ -27,7 +27,7 @@ def lib():
         ip = C.POINTER(C.c_int32)
         L.oracle_smc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                  C.c_double, C.c_double, dp, dp, dp, dp, C.c_int, C.c_int,
-                                 dp, ip, dp, dp, ip, dp, dp, dp, ip, dp, dp, dp, dp]
+                                 dp, ip, dp, dp, ip, dp, dp, dp, ip, dp, dp, dp, dp, dp]
         L.oracle_trajectory.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                         C.c_double, dp, dp, dp, dp]
         L.oracle_obs_loglik.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, dp, ip]
@@ -74,16 +74,17 @@ def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=
     NH = lib().oracle_model_n_hat(model_id)
     n_ts = lib().oracle_model_n_ts(model_id)
     ht = np.zeros((N + 1, NH, 3)) if hat else None
+    lob = np.zeros((N + 1, n_ts, 2)) if lineage else None
     pr = np.zeros((N, n_ts, 2)) if hat else None
     el = C.c_double()
     rc = lib().oracle_smc(model_id, J, N, noises_off, flags, seed, stream, dt, like_min, _dp(times), _dp(data),
                           _dp(pv), _dp(x0), mode, nthreads, C.byref(ll), C.byref(naf), _dp(ll_n), _dp(ess),
-                          _ip(anc), _dp(fs), _dp(lw), C.byref(el), _ip(lsl), _dp(lst), _dp(alls), _dp(ht), _dp(pr))
+                          _ip(anc), _dp(fs), _dp(lw), C.byref(el), _ip(lsl), _dp(lst), _dp(alls), _dp(ht), _dp(pr), _dp(lob))
     if rc != 0:
         raise RuntimeError("oracle_smc failed: %d" % rc)
     return {"loglik": ll.value, "n_all_fail": naf.value, "loglik_n": ll_n, "ess_n": ess,
             "ancestry": anc, "final_state": fs, "logw": lw, "elapsed_s": el.value,
-            "lineage_slots": lsl, "lineage": lst, "all_states": alls, "hat": ht, "pred_res": pr}
+            "lineage_slots": lsl, "lineage": lst, "lineage_obs": lob, "all_states": alls, "hat": ht, "pred_res": pr}
 
 
 def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=0.25):

@@ -370,7 +370,8 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
                const double *x0, int mode, int nthreads,
                double *loglik, int32_t *n_all_fail, double *loglik_n, double *ess_n, int32_t *ancestry,
                double *final_state, double *logw_out, double *elapsed_s,
-               int32_t *lin_slots, double *lin_state, double *all_states, double *hat, double *pred_res)
+               int32_t *lin_slots, double *lin_state, double *all_states, double *hat, double *pred_res,
+               double *lin_obs)
 {
     if (model < 0 || model >= N_MODELS || J < 1 || N < 0) return -1;
     const oracle_model *M = &MODELS[model];
@@ -495,6 +496,13 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
             M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off);
             memcpy(lin_state + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
         }
+        if (lin_obs)
+            for (int n = 0; n <= N; n++) {
+                double obs[64], mu[64], var[64];
+                M->proj2obs(lin_state + (size_t)n * S, pv, obs);
+                M->obs_moments(obs, pv, times[n], mu, var);
+                plom_lineage_obs(seed, stream, (uint32_t)n, n_ts, mu, var, plom_bm_tab, lin_obs + (size_t)n * n_ts * 2);
+            }
     }
     if (anc && anc != ancestry) free(anc);
     free(hvals); free(hbuf); free(hitems); free(hq); free(offspring);

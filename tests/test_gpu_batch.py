@@ -47,6 +47,52 @@ def test_batch_equals_alone_and_oracle(ctx):
         f.close()
 
 
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_update_between_graph_replays(ctx, lanes):
+    """plom_filter_update (new parameters, initial state and key) between replays of a
+    captured batch: each replay equals fresh filters run alone with those inputs, for the
+    updated filters and the untouched ones alike."""
+    rng = np.random.default_rng(21)
+    cases = [("sirs4", 1200, 20, 1), ("sir", 900, 30, 2), ("sirs4", 700, 15, 3)]
+    fs = [make(ctx, *c, flags=smc.FLAG_STORE_ANCESTRY | smc.FLAG_LINEAGE)[0] for c in cases]
+    b = smc.Batch(ctx, fs)
+    b.configure(lanes)
+    cur = [(configs.TRUTH[n], 7, st) for n, _, _, st in cases]
+    for it in range(3):
+        if it:
+            for k in (0, 2) if it == 1 else (1,):
+                name = cases[k][0]
+                p = configs.proposal(name, rng)
+                x0 = Model.get(name).x0(configs.X0_PROPS[name]) * (1.0 + 0.01 * it)
+                st = 500 + 10 * it + k
+                fs[k].update(Model.get(name).pack(p), x0, seed=7, stream=st)
+                cur[k] = (p, 7, st, x0)
+        print("replay", it, "lanes", lanes, flush=True)
+        b.launch(use_graph=True)
+        for k, (name, J, N, _) in enumerate(cases):
+            m = Model.get(name)
+            times, data = m.dataset(N)
+            p, seed, st = cur[k][:3]
+            x0 = cur[k][3] if len(cur[k]) > 3 else m.x0(configs.X0_PROPS[name])
+            a = smc.run(ctx, m, J, times, data, m.pack(p), x0, seed=seed, stream=st, lineage=True)
+            r = fs[k].read(lineage=True)
+            assert r.loglik == a.loglik, (it, k)
+            np.testing.assert_array_equal(r.loglik_n, a.loglik_n)
+            np.testing.assert_array_equal(r.lineage, a.lineage)
+    b.close()
+    for f in fs:
+        f.close()
+
+
+def test_update_rejects_bad_sizes_and_running_filters(ctx):
+    f, (m, times, data, pv, x0) = make(ctx, "sir", 100, 5, 1)
+    with pytest.raises(ValueError):
+        f.update(pv[:-1], x0, seed=1, stream=1)
+    with pytest.raises(ValueError):
+        f.update(pv, x0[:-1], seed=1, stream=1)
+    f.close()
+
+
 def test_batch_repeat_and_reset(ctx):
     """A batch can be run again (filters reset to interval 0) with the same results."""
     fs = [make(ctx, *c)[0] for c in CASES[:3]]

@@ -29,6 +29,7 @@ def test_lineage_equals_oracle(ctx, name, J, N, flags):
     o = oracle.smc(m.id, J, times, data, pv, x0, flags=flags, seed=3, stream=11, lineage=True, nthreads=8)
     np.testing.assert_array_equal(g.lineage_slots, o["lineage_slots"])
     np.testing.assert_array_equal(g.lineage, o["lineage"])
+    np.testing.assert_array_equal(g.lineage_obs, o["lineage_obs"])
     assert g.loglik == o["loglik"]
 
 
@@ -60,6 +61,7 @@ def test_lineage_batched_equals_alone(ctx, lanes):
         r = f.read(lineage=True)
         np.testing.assert_array_equal(r.lineage_slots, a.lineage_slots)
         np.testing.assert_array_equal(r.lineage, a.lineage)
+        np.testing.assert_array_equal(r.lineage_obs, a.lineage_obs)
     b.close()
     for f in fs:
         f.close()

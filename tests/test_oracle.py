@@ -230,6 +230,38 @@ def test_lineage_without_filtering_uses_equal_weights():
     np.testing.assert_array_equal(r["lineage"][1:], r["all_states"][1:, r["lineage_slots"][-1]])
 
 
+@pytest.mark.parametrize("name", ["sir", "sirs4"])
+def test_lineage_observations(name):
+    """Observation rows of a lineage (print_p_X, print.c:312-341): per row and series the
+    mean rep x of the lineage state's incidence and one draw max(mean + sd z, 0) with
+    sd^2 = rep (1 - rep) x + (rep phi x)^2 (examples/*/link.json) and z the Box-Muller
+    values (40-digit mpmath) of Philox(ctr = (0, n, 3, ts // 2), key = (seed, stream))."""
+    m, times, data, pv, x0 = setup(name, 10)
+    r = oracle.smc(m.id, 64, times, data, pv, x0, seed=8, stream=5, lineage=True)
+    lin, lob = r["lineage"], r["lineage_obs"]
+    d, lay = m.desc, m.desc.layout
+    for n in range(len(times)):
+        for ts in range(d.N_TS):
+            x = lin[n, d.inc_offset + ts]
+            rep, phi = pv[lay["obs"]["rep"] + ts], pv[lay["obs"]["phi"] + ts]
+            mu = rep * x
+            sd = np.sqrt(rep * (1 - rep) * x + (rep * phi * x) ** 2)
+            w = oracle.philox([0, n, 3, ts // 2], [8, 5])
+            _, zc, zs = bm_exact(w[0], w[1])
+            z = float(zc if ts % 2 == 0 else zs)
+            assert lob[n, ts, 0] == pytest.approx(mu, rel=1e-14, abs=1e-300)
+            assert lob[n, ts, 1] == pytest.approx(max(mu + sd * z, 0.0), rel=1e-12, abs=1e-9)
+    assert np.all(lob[..., 1] >= 0)
+
+
+def bm_exact(x, y):
+    """Box-Muller of the words x, y in 60-digit arithmetic: radius, cos and sin values."""
+    u1 = (mp.mpf(int(x)) + mp.mpf(1) / 2) / 2 ** 32
+    u2 = (mp.mpf(int(y)) + mp.mpf(1) / 2) / 2 ** 32
+    r = mp.sqrt(-2 * mp.log(u1))
+    return r, r * mp.cos(2 * mp.pi * u2), r * mp.sin(2 * mp.pi * u2)
+
+
 def hat_quantities(m, states, pv):
     """The hat quantities written from the model files (hat.c:66-137): states, remainder
     N - sum of the states, and the observation mean rep * x of every series

@@ -48,12 +48,16 @@ def build_cli():
     cli = os.path.join(PKG, "cli")
     if not os.path.isdir(cli):
         return None
-    srcs = sorted(os.path.join(cli, f) for f in os.listdir(cli) if f.endswith(".c"))
-    out = os.path.join(cli, "smc")
-    cmd = ["gcc", "-O2", "-std=gnu99", "-Wall", "-ffp-contract=off", "-I" + INC, "-I" + CSRC, "-I" + GEN] + srcs + \
-          ["-L" + CSRC, "-lplom_smc", "-Wl,-rpath,$ORIGIN/../csrc", "-lm", "-o", out]
-    run(cmd)
-    return out
+    shared = [os.path.join(cli, f) for f in ("plom_cli.c", "pjson.c")]
+    outs = []
+    for name in ("smc", "pmcmc"):
+        out = os.path.join(cli, name)
+        cmd = ["gcc", "-O2", "-std=gnu99", "-Wall", "-ffp-contract=off", "-I" + INC, "-I" + CSRC, "-I" + GEN,
+               os.path.join(cli, name + "_main.c")] + shared + \
+              ["-L" + CSRC, "-lplom_smc", "-Wl,-rpath,$ORIGIN/../csrc", "-lm", "-o", out]
+        run(cmd)
+        outs.append(out)
+    return outs
 
 
 def build_all():

@@ -132,3 +132,70 @@ def test_cli_hat_and_pred_res_files(tmp_path, ctx):
     prow = np.array([[float(x) for x in line.split(",")] for line in pres[1:]])
     np.testing.assert_allclose(prow[:, 1:-1], g.pred_res.reshape(30, -1), rtol=1e-5, atol=1e-9)
     np.testing.assert_allclose(prow[:, -1], g.ess_n, rtol=1e-5)
+
+
+# ---------------------------------------------------------------------- pmcmc (host side)
+PMCMC = os.path.join(PKG, "cli", "pmcmc")
+
+
+def run_pmcmc(d, args, theta_path):
+    with open(theta_path) as f:
+        return subprocess.run([PMCMC] + args, cwd=str(d), stdin=f, capture_output=True, text=True, timeout=600)
+
+
+def test_pmcmc_dry_run_best(tmp_path):
+    """build_best / load_best (build.c:1504-1601, load.c:304-406) under a Bayesian method:
+    uniform priors move on the logit scale of [min, max] (transform.c:332-350), the initial
+    covariance is diag(sd_transf^2), every parameter with sd_transf > 0 is estimated, the
+    covariance switch defaults to 5 n^2 (mcmc_util.c:47-53); the filter inputs equal the
+    smc command's up to the rounding of the other transform."""
+    d = model_dir(tmp_path, "sir")
+    r = run_pmcmc(d, ["sde", "-J", "10", "-o", "20", "--dry_run"], os.path.join(PKG, "examples", "sir", "theta.json"))
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    theta = json.load(open(os.path.join(PKG, "examples", "sir", "theta.json")))["parameter"]
+    names = ["S", "I", "beta", "gamma", "rep", "phi"]
+    assert [c["name"] for c in out["components"]] == [n + ":all" for n in names]
+    n = len(names)
+    for o, nm in enumerate(names):
+        g = theta[nm]["group"]["all"]
+        lo, hi, guess, sd = g["min"]["value"], g["max"]["value"], g["guess"]["value"], g["sd_transf"]["value"]
+        assert out["components"][o]["tr"] == "logit_ab" and out["components"][o]["prior"] == "flat"
+        assert out["theta"][o] == pytest.approx(math.log((guess - lo) / (hi - guess)), rel=1e-15)
+        assert out["var"][o * n + o] == sd * sd
+    assert sum(abs(v) for i, v in enumerate(out["var"]) if i % (n + 1)) == 0.0
+    assert out["estimated"] == list(range(n)) and out["m_switch"] == 5 * n * n
+    ref = json.loads(run_cli(d, ["sde", "-J", "10", "-o", "20", "--dry_run"], os.path.join(PKG, "examples", "sir", "theta.json")).stdout)
+    np.testing.assert_allclose(out["pv"], ref["pv"], rtol=1e-14)
+    np.testing.assert_allclose(out["x0"], ref["x0"], rtol=1e-14)
+
+
+def test_pmcmc_follow_and_covariance(tmp_path):
+    """A follower (build.c:1540-1589) is not estimated, has a zero jump size and copies its
+    leader; a "covariance" in theta (load.c:413-470) replaces diag(sd_transf^2)."""
+    d = model_dir(tmp_path, "sir")
+    theta = json.load(open(os.path.join(PKG, "examples", "sir", "theta.json")))
+    theta["parameter"]["phi"] = {"follow": "rep"}
+    p = tmp_path / "follow.json"
+    p.write_text(json.dumps(theta) + "\n")
+    out = json.loads(run_pmcmc(d, ["sde", "-J", "10", "--dry_run"], str(p)).stdout)
+    assert out["estimated"] == [0, 1, 2, 3, 4]
+    assert out["components"][5]["follow"] == 4 and out["theta"][5] == out["theta"][4]
+    assert out["var"][5 * 6 + 5] == 0.0
+    theta = json.load(open(os.path.join(PKG, "examples", "sir", "theta.json")))
+    cov = np.diag([0.01, 0.02, 0.03, 0.04, 0.05, 0.06])
+    cov[2, 3] = cov[3, 2] = 0.005
+    theta["covariance"] = [["S:all", "I:all", "beta:all", "gamma:all", "rep:all", "phi:all"]] + cov.tolist()
+    p.write_text(json.dumps(theta) + "\n")
+    out = json.loads(run_pmcmc(d, ["sde", "-J", "10", "--dry_run"], str(p)).stdout)
+    np.testing.assert_array_equal(np.array(out["var"]).reshape(6, 6), cov)
+
+
+def test_pmcmc_usage_errors(tmp_path):
+    d = model_dir(tmp_path, "sir")
+    th = os.path.join(PKG, "examples", "sir", "theta.json")
+    assert run_pmcmc(d, ["psr", "-J", "10"], th).returncode == 1
+    r = run_pmcmc(d, ["sde", "-J", "10", "-P", "--chains", "2"], th)
+    assert r.returncode != 0 and "one chain" in r.stderr
+    r = run_pmcmc(d, ["sde", "-J", "10", "--chains", "0"], th)
+    assert r.returncode != 0

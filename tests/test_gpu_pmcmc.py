@@ -107,17 +107,26 @@ def test_pmcmc_chains_equal_oracle(tmp_path, mode):
             assert done["parameter"][name]["group"]["all"]["guess"]["value"] == pytest.approx(v, rel=1e-15)
 
 
-def test_pmcmc_first_iteration_is_the_smc_filter(tmp_path):
-    """Iteration 0 of chain 0 is the `smc` filter of the same theta and id (key 2 + id)."""
+def test_pmcmc_first_iteration_is_the_smc_filter(tmp_path, ctx):
+    """Iteration 0 of chain 0 is the filter of the initial theta with the key (2 + id, 0),
+    the `smc` command's key: bit for bit with the inputs of the Bayesian transforms, and
+    to their rounding against `smc` (which moves uniform-prior parameters on their own
+    transforms instead of logit_ab)."""
     d = model_dir(tmp_path, "sir")
-    r = run_pmcmc(d, ["sde", "-J", "1000", "-o", "30", "-M", "1", "--trace_17g", "-i", "2", "-n", "0"])
+    args = ["sde", "-J", "1000", "-o", "30", "-M", "1", "--trace_17g", "-i", "2", "-n", "0"]
+    dry = json.loads(run_pmcmc(d, args + ["--dry_run"]).stdout)
+    r = run_pmcmc(d, args)
     assert r.returncode == 0, r.stderr
     _, rows = read_csv(tmp_path / "sir" / "trace_2.csv")
+    m = smc.Model.get("sir")
+    times, data = m.dataset(30)
+    g = smc.run(ctx, m, 1000, times, data, np.array(dry["pv"]), np.array(dry["x0"]), seed=4, stream=0)
+    assert rows[0][-1] == g.loglik
     smc_bin = os.path.join(PKG, "cli", "smc")
     import subprocess
     with open(THETA) as f:
         s = subprocess.run([smc_bin, "sde", "-J", "1000", "-o", "30", "-i", "2", "-e", "-d", "-b"], cwd=str(d), stdin=f,
                            capture_output=True, text=True, timeout=600)
     ll = float([l for l in s.stdout.splitlines() if l.startswith("logV_17g:")][0].split(":")[1])
-    assert rows[0][-1] == ll
+    assert rows[0][-1] == pytest.approx(ll, rel=1e-12)
     assert not os.path.exists(tmp_path / "sir" / "X_2.csv")

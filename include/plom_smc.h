@@ -27,6 +27,12 @@
  *                           admission by free HBM.
  *   plom_model_*            the generated per-model API (C/core/plom.h:896-913,
  *                           C/templates/core_template.c) compiled into the library.
+ *   PLOM_FLAG_MIF,          one pass of iterated filtering (C/mif/mif.c:128-200): every
+ *   plom_filter_mif_start   particle carries its own parameter vector theta, resampled with
+ *                           its state and moved by a random walk after every observation
+ *                           (resample_and_mut_theta_mif, C/mif/methods.c:261-367); the pass
+ *                           returns the weighted mean and the variance of theta per
+ *                           observation (mean_var_theta_theoretical_mif, methods.c:146-205).
  */
 #ifndef PLOM_SMC_H
 #define PLOM_SMC_H
@@ -63,6 +69,10 @@ extern "C" {
                                          sample_traj_and_print, C/core/print.c:715-784); keeps the
                                          ancestry on the device */
 
+#define PLOM_FLAG_MIF 32u             /* per-particle parameters (iterated filtering): needs
+                                         plom_filter_inputs.mif and plom_filter_mif_start before
+                                         every run; not with NO_FILTER, HAT or LINEAGE */
+
 #define PLOM_MAX_J (1 << 21)          /* particles per filter */
 
 typedef struct {
@@ -76,6 +86,8 @@ typedef struct {
     int32_t n_dem;       /* demographic noises per population */
     int32_t n_env;       /* environmental noises per population */
     int32_t n_hat;       /* quantities of a hat row: states, remainders, observation means */
+    int32_t n_pp;        /* leading slots of pv that PLOM_FLAG_MIF makes per particle: the process
+                            parameters [par][n_cac] then the observation parameters [par][n_ts] */
 } plom_model_info;
 
 typedef struct {
@@ -89,13 +101,32 @@ typedef struct {
     uint32_t stream;     /* random key, second word: one per filter / request */
     double dt;           /* Euler step, already 1/round(1/DT) (C/core/build.c:1298-1304) */
     double like_min;     /* LIKE_MIN (C/smc/main.c:83) */
+    int32_t n_theta;     /* PLOM_FLAG_MIF: components of the per-particle parameter vector theta */
+    int32_t pad;
 } plom_filter_desc;
+
+/* How a particle's theta (transformed scale, one value per parameter and group, as
+ * p_best->mean) gives the per-particle slots of pv (back_transform_theta2par,
+ * C/core/transform.c:462-514) and how it moves (PLOM_FLAG_MIF).  Arrays over the
+ * components have n_theta entries. */
+typedef struct {
+    const int32_t *pv_src;     /* [n_pp]: the component whose natural value fills the slot, or -1
+                                  (the slot keeps pv) */
+    const int32_t *tr;         /* 0 identity, 1 log, 2 logit, 3 logit_ab (on [min, max]) */
+    const int32_t *scale;      /* 0 none, 1 10^x, 2 10^-x (applied after the inverse transform) */
+    const int32_t *duration;   /* 1: the value is a duration, the rate is 1/(1e-5 + value) */
+    const double *min, *max;   /* bounds of logit_ab */
+    const double *multiplier;  /* unit multiplier */
+    const double *sd;          /* random-walk standard deviation per unit of mut_fac (0: the
+                                  component is resampled but not moved) */
+} plom_mif_map;
 
 typedef struct {
     const double *times; /* [N+1] days since date_0 (settings data.times) */
     const double *data;  /* [N*n_ts] observations, NaN = missing */
     const double *pv;    /* [pv_size] natural parameters per population / series */
     const double *x0;    /* [size_proj] initial state (incidence slots ignored) */
+    const plom_mif_map *mif; /* PLOM_FLAG_MIF only (copied at creation) */
 } plom_filter_inputs;
 
 typedef struct {
@@ -115,6 +146,10 @@ typedef struct {
                             mean, high95 (PLOM_FLAG_HAT) */
     double *pred_res;    /* optional [N*n_ts*2]: per interval and series: prediction mean and
                             residual over the resampled cloud; NaN without data (PLOM_FLAG_HAT) */
+    double *mif_mean;    /* optional [(N+1)*n_theta] (PLOM_FLAG_MIF): row n+1 = the weighted mean of
+                            the particles' theta at interval n, NaN without data; row 0 NaN */
+    double *mif_var;     /* optional [(N+1)*n_theta]: the same rows, unweighted variance over the
+                            particles (divisor J-1) */
 } plom_filter_outputs;
 
 typedef struct plom_ctx plom_ctx;
@@ -143,6 +178,12 @@ int plom_filter_reset(plom_ctx *ctx, plom_filter *f);  /* back to interval 0, sa
  * change up at their next launch, graphs included): one pMCMC proposal (pmcmc.c:205-214) */
 int plom_filter_update(plom_ctx *ctx, plom_filter *f, const double *pv, const double *x0, uint32_t seed,
                        uint32_t stream);
+/* PLOM_FLAG_MIF: the particles' starting parameters theta [n_theta*J] (component-major) and
+ * initial states x0 [size_proj*J] (state-major), the random-walk factor mut_fac [N] applied
+ * after each interval with data (FREEZE*sqrt(delta_t), C/mif/mif.c:183), and the random key
+ * of the pass.  A pass consumes them: call before every run (or batch launch). */
+int plom_filter_mif_start(plom_ctx *ctx, plom_filter *f, const double *theta, const double *x0,
+                          const double *mut_fac, uint32_t seed, uint32_t stream);
 int plom_filter_run(plom_ctx *ctx, plom_filter *f);    /* enqueue every interval */
 int plom_filter_read(plom_ctx *ctx, plom_filter *f, plom_filter_outputs *out); /* waits */
 int plom_filter_destroy(plom_ctx *ctx, plom_filter *f);

@@ -11,6 +11,10 @@ model_builder/Ccoder.py:444-505, selected by get_f_pred, C/core/prediction_util.
   plom_<m>_loglik                   sum over non-missing series of log(discretised-normal
                                     likelihood), sanitised (core_template.c:342-367,
                                     C/core/likelihood.c:56-92); always FP64
+  ..._pp variants (device only)     the same functions with per-particle process and
+                                    observation parameters (iterated filtering): slots
+                                    below NPP come from the lane's own array lp, the
+                                    covariates and the coupling matrix from pv
 
 Everything is unrolled over populations and states so that, on the GPU, a particle's
 whole state lives in registers.  Arithmetic is printed in the reference's term order
@@ -123,17 +127,22 @@ class Emitter:
             return "        X[%d] = (f%d < 0.0) ? 0.0 : f%d;" % (idx, idx, idx)
         return "        X[%d] = plom_clipf(f%d);" % (idx, idx)
 
-    def interval(self, variant, dem, env, real):
+    def interval(self, variant, dem, env, real, pp=False):
         m = self.m
         Rt, sfx, tag, stream, normal = _R(real)
         zero = "0.0" if real == "double" else "0.0f"
-        qual = "PLOM_HD" if real == "double" else "static __device__ inline __attribute__((always_inline))"
+        dev = "static __device__ inline __attribute__((always_inline))"
+        qual = "PLOM_HD" if real == "double" and not pp else dev
         L = []
-        L.append("%s void %s_interval_%s_%s(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt)"
-                 % (qual, self.p, variant, tag, Rt, Rt, stream, Rt))
+        if pp:
+            L.append("%s void %s_interval_%s_pp_%s(%s *X, const %s *pv, const %s *lp, %s *ns, double t0, double t1, double dt, %s sqdt)"
+                     % (qual, self.p, variant, tag, Rt, Rt, Rt, stream, Rt))
+        else:
+            L.append("%s void %s_interval_%s_%s(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt)"
+                     % (qual, self.p, variant, tag, Rt, Rt, stream, Rt))
         L.append("{")
         L.append("    const %s dtr = (%s)dt;" % (Rt, Rt))
-        L.append("    (void)sqdt; (void)ns; (void)pv;")
+        L.append("    (void)sqdt; (void)ns; (void)pv;%s" % (" (void)lp;" if pp else ""))
         self.used_pars = set()
         bind_at = len(L)
         for u in range(m.N_TS_INC_UNIQUE):
@@ -256,17 +265,25 @@ class Emitter:
         L[tf_at:tf_at] = tfs
         self.time_funcs = None
         uni = "PLOM_UNIFORM64" if real == "double" else "PLOM_UNIFORM32"
-        binds = ["    const %s P%d = %s(pv[%d]);" % (Rt, k, uni, k) for k in sorted(self.used_pars)]
+        npp = m.layout["npp"]
+        binds = [("    const %s P%d = lp[%d];" % (Rt, k, k)) if pp and k < npp else
+                 ("    const %s P%d = %s(pv[%d]);" % (Rt, k, uni, k)) for k in sorted(self.used_pars)]
         L[bind_at:bind_at] = binds
         return "\n".join(L)
 
     # ---------------------------------------------------------------- observations
-    def proj2obs(self, real):
+    def proj2obs(self, real, pp=False):
         m = self.m
         Rt, sfx, tag = _R(real)[:3]
-        qual = "PLOM_HD" if real == "double" else "static __device__ inline __attribute__((always_inline))"
-        L = ["%s void %s_proj2obs_%s(const %s *X, const double *pvd, double *obs)" % (qual, self.p, tag, Rt), "{",
-             "    (void)pvd; (void)X;"]
+        dev = "static __device__ inline __attribute__((always_inline))"
+        qual = "PLOM_HD" if real == "double" and not pp else dev
+        if pp:
+            L = ["%s void %s_proj2obs_pp_%s(const %s *X, const double *pvd, const double *lpd, double *obs)"
+                 % (qual, self.p, tag, Rt), "{", "    (void)pvd; (void)lpd; (void)X;"]
+        else:
+            L = ["%s void %s_proj2obs_%s(const %s *X, const double *pvd, double *obs)" % (qual, self.p, tag, Rt), "{",
+                 "    (void)pvd; (void)X;"]
+
         ind = 0
         u = 0
         for o, lst in enumerate(m.obs_var_def):
@@ -288,7 +305,7 @@ class Emitter:
                 L.append("        double sum_prev = 0.0;")
                 for (ci, ai) in info["cac"][tsu]:
                     c = ci * m.N_AC + ai
-                    em = self._prev_emitter(c)
+                    em = self._prev_emitter(c, pp)
                     L.append("        sum_prev += %s;" % em.emit(tree))
                 for _ in range(info["n_stream"][tsu]):
                     L.append("        obs[%d] = sum_prev;" % ind)
@@ -297,7 +314,7 @@ class Emitter:
         L.append("}")
         return "\n".join(L)
 
-    def _prev_emitter(self, cac):
+    def _prev_emitter(self, cac, pp=False):
         m, lay = self.m, self.m.layout
 
         def resolve(name):
@@ -306,7 +323,7 @@ class Emitter:
             if name in lay["fixed"]:
                 return "pvd[%d]" % (lay["fixed"][name] + cac)
             if name in lay["proc"]:
-                return "pvd[%d]" % (lay["proc"][name] + cac)
+                return "%s[%d]" % ("lpd" if pp else "pvd", lay["proc"][name] + cac)
             raise ValueError("symbol %r not allowed in a prevalence definition" % name)
 
         return E.CEmitter(resolve, "double")
@@ -377,10 +394,15 @@ class Emitter:
         L.append("}")
         return "\n".join(L)
 
-    def loglik(self):
+    def loglik(self, pp=False):
         m, lay = self.m, self.m.layout
-        L = ["PLOM_HD double %s_loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor)" % self.p,
-             "{", "    double loglike = 0.0;", "    int nf = 0;", "    (void)t; (void)pvd;"]
+        if pp:
+            L = ["static __device__ inline __attribute__((always_inline)) double %s_loglik_pp(const double *obs, const double *y, "
+                 "const double *pvd, const double *lpd, double t, double like_min, int *n_floor)" % self.p,
+                 "{", "    double loglike = 0.0;", "    int nf = 0;", "    (void)t; (void)pvd; (void)lpd;"]
+        else:
+            L = ["PLOM_HD double %s_loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor)" % self.p,
+                 "{", "    double loglike = 0.0;", "    int nf = 0;", "    (void)t; (void)pvd;"]
         mean_t = E.parse(m.obs_model["mean"])
         var_t = E.parse(m.obs_model["var"])
         for ts in range(m.N_TS):
@@ -388,7 +410,7 @@ class Emitter:
                 if name == "x":
                     return "x"
                 if name in lay["obs"]:
-                    return "pvd[%d]" % (lay["obs"][name] + ts)
+                    return "%s[%d]" % ("lpd" if pp else "pvd", lay["obs"][name] + ts)
                 if name in lay["fixed_obs"]:
                     return "pvd[%d]" % (lay["fixed_obs"][name] + ts)
                 if name == "t":
@@ -431,7 +453,8 @@ class Emitter:
                 "#define %s_PV_SIZE %d" % (P, m.layout["size"]),
                 "#define %s_N_DEM %d" % (P, ndem),
                 "#define %s_N_ENV %d" % (P, nenv),
-                "#define %s_N_HAT %d" % (P, self.n_hat()), ""]
+                "#define %s_N_HAT %d" % (P, self.n_hat()),
+                "#define %s_N_PP %d" % (P, m.layout["npp"]), ""]
         for v, dem, env in VARIANTS:
             out.append(self.interval(v, dem, env, "double"))
             out.append("")
@@ -459,10 +482,19 @@ class Emitter:
             out.append("")
         out.append(self.proj2obs("float"))
         out.append("")
+        for real in ("double", "float"):
+            for v, dem, env in VARIANTS:
+                out.append(self.interval(v, dem, env, real, pp=True))
+                out.append("")
+            out.append(self.proj2obs(real, pp=True))
+            out.append("")
+        out.append(self.loglik(pp=True))
+        out.append("")
         name = m.name.lower()
         out.append("struct plom_model_%s {" % name)
-        out.append("    static constexpr int S = %d, N_TS = %d, INC_OFFSET = %d, N_INC = %d, PV = %d, N_CAC = %d, N_HAT = %d;"
-                   % (m.size_proj, m.N_TS, m.inc_offset, m.N_TS_INC_UNIQUE, m.layout["size"], m.N_CAC, self.n_hat()))
+        out.append("    static constexpr int S = %d, N_TS = %d, INC_OFFSET = %d, N_INC = %d, PV = %d, N_CAC = %d, N_HAT = %d, NPP = %d;"
+                   % (m.size_proj, m.N_TS, m.inc_offset, m.N_TS_INC_UNIQUE, m.layout["size"], m.N_CAC, self.n_hat(),
+                      m.layout["npp"]))
         for tag, Rt, stream in (("f64", "double", "plom_nstream64"), ("f32", "float", "plom_nstream32")):
             out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt)"
                        % (Rt, Rt, stream, Rt))
@@ -471,10 +503,21 @@ class Emitter:
                 kw = "if" if vi == 0 else "else if"
                 out.append("        %s constexpr (V == %d) %s_interval_%s_%s(X, pv, ns, t0, t1, dt, sqdt);" % (kw, vi, self.p, v, tag))
             out.append("    }")
+            out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(%s *X, const %s *pv, const %s *lp, %s *ns, double t0, double t1, double dt, %s sqdt)"
+                       % (Rt, Rt, Rt, stream, Rt))
+            out.append("    {")
+            for vi, (v, dem, env) in enumerate(VARIANTS):
+                kw = "if" if vi == 0 else "else if"
+                out.append("        %s constexpr (V == %d) %s_interval_%s_pp_%s(X, pv, lp, ns, t0, t1, dt, sqdt);" % (kw, vi, self.p, v, tag))
+            out.append("    }")
             out.append("    static __device__ inline __attribute__((always_inline)) void proj2obs(const %s *X, const double *pvd, double *obs) { %s_proj2obs_%s(X, pvd, obs); }"
+                       % (Rt, self.p, tag))
+            out.append("    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const %s *X, const double *pvd, const double *lpd, double *obs) { %s_proj2obs_pp_%s(X, pvd, lpd, obs); }"
                        % (Rt, self.p, tag))
         out.append("    static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor)"
                    " { return %s_loglik(obs, y, pvd, t, like_min, n_floor); }" % self.p)
+        out.append("    static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, double t, double like_min, int *n_floor)"
+                   " { return %s_loglik_pp(obs, y, pvd, lpd, t, like_min, n_floor); }" % self.p)
         out.append("    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, double t, double *out)"
                    " { %s_hat_values(X, pvd, t, out); }" % self.p)
         out.append("    static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var)"

@@ -258,6 +258,7 @@ class PlomModel:
         for p in self.par_fixed_obs:
             lay["fixed_obs"][p] = off
             off += self.N_TS
+        lay["npp"] = len(self.par_proc) * self.N_CAC + len(self.par_obs) * self.N_TS  # the per-particle slots
         lay["coupling"] = off if self.coupling is not None else None
         if self.coupling is not None:
             off += self.N_CAC * self.N_CAC

@@ -42,6 +42,7 @@
 #include "plom_smc.h"
 #include "plom_math.h"
 #include "plom_rng.h"
+#include "plom_mif.h"
 #include "plom_models.h"
 
 typedef unsigned __int128 plom_u128;
@@ -99,6 +100,15 @@ struct plom_dfilter {
     double *hpart;      /* [n_hat][T]: per tile, tree sum of value x weight */
     double *hat;        /* [(N+1)][n_hat][3]: low95, mean, high95 */
     double *pres;       /* [N][n_ts][2]: prediction mean and residual */
+    /* PLOM_FLAG_MIF: per-particle parameters */
+    int32_t n_theta, n_pp;
+    const int32_t *pm_src;          /* [n_pp] component of each per-particle pv slot, -1 = pv */
+    const int32_t *pm_tr, *pm_sc, *pm_dur;      /* [n_theta] */
+    const double *pm_min, *pm_max, *pm_mult, *pm_sd;   /* [n_theta] */
+    const double *mut_fac;          /* [N] random-walk factor after interval n (with data) */
+    double *th[2];                  /* [n_theta][Jp] theta of the particles, ping-pong like X */
+    double *mpw, *mps, *mpm, *mpq;  /* tile partials: [T] sum w, [n_theta][T] sum w theta, mean, M2 */
+    double *mmean, *mvar;           /* [(N+1)][n_theta] weighted mean and variance of theta */
 };
 
 /* ------------------------------------------------------------------ block helpers */
@@ -236,7 +246,41 @@ __device__ void finalize_weights(plom_dfilter *f, int n, double *lds)
 }
 
 /* ------------------------------------------------------------------ kernels */
-template <class M, typename R, int V>
+/* PLOM_FLAG_MIF: theta of particle j from its ancestor a, moved by the random walk when
+ * interval n-1 had data (resample_and_mut_theta_mif, C/mif/methods.c:261-367: every
+ * component with a jump size moves by mut_fac[n-1] * sd * N(0,1)); written to the output
+ * table; then the per-particle slots of pv from it (back_transform_theta2par) */
+template <int NPP>
+__device__ inline void mif_particle(const plom_dfilter *f, int n, int j, int a, const double *tab, double *lp)
+{
+    const int Jp = f->Jp, nth = f->n_theta;
+    const double *Tin = f->th[n & 1];
+    double *Tout = f->th[(n + 1) & 1];
+    const double fac = (n > 0 && f->nonan[n - 1] > 0) ? f->mut_fac[n - 1] : 0.0;
+    for (int o = 0; o < nth; o += 2) {
+        const double sd0 = fac * f->pm_sd[o];
+        const double sd1 = (o + 1 < nth) ? fac * f->pm_sd[o + 1] : 0.0;
+        double z0 = 0.0, z1 = 0.0;
+        if (sd0 != 0.0 || sd1 != 0.0) plom_mif_normals(f->seed, f->stream, (uint32_t)n, (uint32_t)j, (uint32_t)(o >> 1), tab, &z0, &z1);
+        double v = Tin[(size_t)o * Jp + a];
+        if (sd0 != 0.0) v = v + sd0 * z0;
+        Tout[(size_t)o * Jp + j] = v;
+        if (o + 1 < nth) {
+            double w = Tin[(size_t)(o + 1) * Jp + a];
+            if (sd1 != 0.0) w = w + sd1 * z1;
+            Tout[(size_t)(o + 1) * Jp + j] = w;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NPP; s++) {
+        const int o = f->pm_src[s];
+        lp[s] = o < 0 ? f->pv[s]
+                      : plom_mif_natural(f->pm_tr[o], f->pm_sc[o], f->pm_dur[o], Tout[(size_t)o * Jp + j], f->pm_min[o],
+                                         f->pm_max[o], f->pm_mult[o]);
+    }
+}
+
+template <class M, typename R, int V, bool MIF>
 __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const int2 *tiles)
 {
     __shared__ double lds[8];
@@ -256,18 +300,29 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
     const bool valid = j < J;
 
     R X[M::S];
-    if (n == 0) {
+    int a = j;
+    if (n == 0 && !MIF) {
         const R *x0 = (const R *)f->x0;
 #pragma unroll
         for (int k = 0; k < M::S; k++) X[k] = x0[k];
     } else {
-        int a = j;
-        if (valid && f->rflag[n - 1]) a = select_ancestor(f, j, n - 1);
-        if (valid && f->ancestry) f->ancestry[(size_t)(n - 1) * J + j] = a;
+        /* (with MIF, interval 0 starts from the particles' own initial states in X[0]) */
+        if (n > 0) {
+            if (valid && f->rflag[n - 1]) a = select_ancestor(f, j, n - 1);
+            if (valid && f->ancestry) f->ancestry[(size_t)(n - 1) * J + j] = a;
+        }
         const R *Xin = (const R *)f->X[n & 1];
         if (!valid) a = 0;
 #pragma unroll
         for (int k = 0; k < M::S; k++) X[k] = Xin[(size_t)k * Jp + a];
+    }
+    /* per-particle slots of pv (MIF only; the array is unused otherwise) */
+    double lp[MIF ? M::NPP : 1];
+    if constexpr (MIF) {
+        if (valid) {
+            if constexpr (sizeof(R) == 8) mif_particle<M::NPP>(f, n, j, a, s_bm, lp);
+            else mif_particle<M::NPP>(f, n, j, a, plom_bm_tab, lp);
+        }
     }
 
     const double t0 = f->times[n], t1 = f->times[n + 1];
@@ -275,11 +330,19 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
         if constexpr (sizeof(R) == 8) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j, s_bm);
-            M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt);
+            if constexpr (MIF) M::template interval_pp<V>((double *)X, f->pv, lp, &ns, t0, t1, f->dt, f->sqdt);
+            else M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt);
         } else {
             plom_nstream32 ns;
             plom_nstream32_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j);
-            M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt);
+            if constexpr (MIF) {
+                float lpf[M::NPP];
+#pragma unroll
+                for (int k = 0; k < M::NPP; k++) lpf[k] = (float)lp[k];
+                M::template interval_pp<V>((float *)X, f->pvf, lpf, &ns, t0, t1, f->dt, (float)f->sqdt);
+            } else {
+                M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt);
+            }
         }
         R *Xout = (R *)f->X[(n + 1) & 1];
 #pragma unroll
@@ -291,9 +354,14 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
     double l = -PLOM_INF;
     if (valid) {
         double obs[M::N_TS];
-        M::proj2obs(X, f->pv, obs);
         int nf = 0;
-        l = M::loglik(obs, f->data + (size_t)n * f->n_ts, f->pv, t1, f->like_min, &nf);
+        if constexpr (MIF) {
+            M::proj2obs_pp(X, f->pv, lp, obs);
+            l = M::loglik_pp(obs, f->data + (size_t)n * f->n_ts, f->pv, lp, t1, f->like_min, &nf);
+        } else {
+            M::proj2obs(X, f->pv, obs);
+            l = M::loglik(obs, f->data + (size_t)n * f->n_ts, f->pv, t1, f->like_min, &nf);
+        }
         /* failure: every series observed and floored at LIKE_MIN (smc.c:44) */
         if (nf == M::N_TS) l = -PLOM_INF;
     }
@@ -316,6 +384,127 @@ __global__ __launch_bounds__(TILE) void k_weights(plom_dfilter *filters, const i
     if (n >= f->N) return;
     if ((f->flags & PLOM_FLAG_NO_FILTER) || f->nonan[n] == 0) return;
     finalize_weights(f, n, lds);
+}
+
+/* PLOM_FLAG_MIF, interval n with data, after k_weights: per tile, the sum of the weights
+ * w_j = exp(l_j - m) (1 each when every particle failed, smc.c:55-70) and, per component,
+ * the sum of w_j theta_j, the tile mean and the sum of squared deviations from it */
+__global__ __launch_bounds__(TILE) void k_mif_tiles(plom_dfilter *filters, const int2 *tiles)
+{
+    __shared__ double lds[8];
+    int fi = 0, tile = blockIdx.x;
+    if (tiles) { const int2 tt = tiles[blockIdx.x]; fi = tt.x; tile = tt.y; }
+    plom_dfilter *f = &filters[fi];
+    const int n = f->step;
+    if (n >= f->N || !(f->flags & PLOM_FLAG_MIF) || f->nonan[n] == 0) return;
+    const int J = f->J, Jp = f->Jp, T = f->T;
+    const int j = tile * TILE + threadIdx.x;
+    const bool valid = j < J;
+    double w = 0.0;
+    if (valid) {
+        if (f->rflag[n]) {
+            const double l = f->logw[j];
+            w = (l == -PLOM_INF) ? 0.0 : plom_exp(l - f->scal[0]);
+        } else {
+            w = 1.0;
+        }
+    }
+    const double sw = tree256(w, lds);
+    const int rem = J - tile * TILE;
+    const double ct = (double)(rem < TILE ? rem : TILE);
+    const double *th = f->th[(n + 1) & 1];
+    for (int o = 0; o < f->n_theta; o++) {
+        const double v = valid ? th[(size_t)o * Jp + j] : 0.0;
+        const double s1 = tree256(valid ? w * v : 0.0, lds);
+        const double mt = tree256(v, lds) / ct;
+        const double d = valid ? v - mt : 0.0;
+        const double q = tree256(d * d, lds);
+        if (threadIdx.x == 0) {
+            f->mps[(size_t)o * T + tile] = s1;
+            f->mpm[(size_t)o * T + tile] = mt;
+            f->mpq[(size_t)o * T + tile] = q;
+        }
+    }
+    if (threadIdx.x == 0) f->mpw[tile] = sw;
+}
+
+/* pairwise tree (contiguous, zero padded to a power of two) over the T tile values of a
+ * filter, in chunks of 256 then over the chunks; every thread gets the total */
+__device__ double tile_tree(const double *v, int T, double *lds, double *chunks)
+{
+    const int nch = (T + TILE - 1) / TILE;
+    for (int c = 0; c < nch; c++) {
+        const int t = c * TILE + threadIdx.x;
+        const double s = tree256(t < T ? v[t] : 0.0, lds);
+        if (threadIdx.x == 0) chunks[c] = s;
+    }
+    __syncthreads();
+    __shared__ double r;
+    if (threadIdx.x == 0) {
+        int P = 1;
+        while (P < nch) P <<= 1;
+        for (int c = nch; c < P; c++) chunks[c] = 0.0;
+        for (int h = 1; h < P; h <<= 1)
+            for (int i = 0; i < P; i += 2 * h) chunks[i] = chunks[i] + chunks[i + h];
+        r = chunks[0];
+    }
+    __syncthreads();
+    const double out = r;
+    __syncthreads();
+    return out;
+}
+
+/* one block per filter: weighted mean sum(w theta)/sum(w) and variance M2/(J-1) of every
+ * component, the (count, mean, M2) of the tiles merged along the same fixed tree */
+__global__ __launch_bounds__(TILE) void k_mif_reduce(plom_dfilter *filters, const int *fidx)
+{
+    __shared__ double lds[8];
+    __shared__ double chunks[MAX_TILES / TILE + 1];
+    __shared__ double sc[TILE], sm[TILE], sq[TILE];
+    __shared__ double cc[MAX_TILES / TILE + 1], cm[MAX_TILES / TILE + 1], cq[MAX_TILES / TILE + 1];
+    const int fi = fidx ? fidx[blockIdx.x] : (int)blockIdx.x;
+    plom_dfilter *f = &filters[fi];
+    const int n = f->step;
+    if (n >= f->N || !(f->flags & PLOM_FLAG_MIF) || f->nonan[n] == 0) return;
+    const int J = f->J, T = f->T, nth = f->n_theta;
+    const double W = tile_tree(f->mpw, T, lds, chunks);
+    const int nch = (T + TILE - 1) / TILE;
+    for (int o = 0; o < nth; o++) {
+        const double SW = tile_tree(f->mps + (size_t)o * T, T, lds, chunks);
+        for (int c = 0; c < nch; c++) {
+            const int t = c * TILE + threadIdx.x;
+            if (t < T) {
+                const int rem = J - t * TILE;
+                sc[threadIdx.x] = (double)(rem < TILE ? rem : TILE);
+                sm[threadIdx.x] = f->mpm[(size_t)o * T + t];
+                sq[threadIdx.x] = f->mpq[(size_t)o * T + t];
+            } else {
+                sc[threadIdx.x] = 0.0; sm[threadIdx.x] = 0.0; sq[threadIdx.x] = 0.0;
+            }
+            __syncthreads();
+            for (int h = 1; h < TILE; h <<= 1) {
+                const int i = threadIdx.x * 2 * h;
+                double c2 = 0.0, m2 = 0.0, q2 = 0.0;
+                if (i < TILE) plom_mif_combine(sc[i], sm[i], sq[i], sc[i + h], sm[i + h], sq[i + h], &c2, &m2, &q2);
+                __syncthreads();
+                if (i < TILE) { sc[i] = c2; sm[i] = m2; sq[i] = q2; }
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) { cc[c] = sc[0]; cm[c] = sm[0]; cq[c] = sq[0]; }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            int P = 1;
+            while (P < nch) P <<= 1;
+            for (int c = nch; c < P; c++) { cc[c] = 0.0; cm[c] = 0.0; cq[c] = 0.0; }
+            for (int h = 1; h < P; h <<= 1)
+                for (int i = 0; i < P; i += 2 * h)
+                    plom_mif_combine(cc[i], cm[i], cq[i], cc[i + h], cm[i + h], cq[i + h], &cc[i], &cm[i], &cq[i]);
+            f->mmean[(size_t)(n + 1) * nth + o] = SW / W;
+            f->mvar[(size_t)(n + 1) * nth + o] = cq[0] / (double)(J - 1);
+        }
+        __syncthreads();
+    }
 }
 
 __global__ __launch_bounds__(TILE) void k_scan(plom_dfilter *filters, const int2 *tiles)
@@ -739,28 +928,33 @@ __global__ void k_selftest_normals(uint32_t seed, uint32_t stream, uint32_t n0, 
 struct model_entry {
     const char *name;
     plom_model_info info;
-    void (*launch_prop)(int prec, int variant, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles);
+    void (*launch_prop)(int prec, int variant, int mif, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles);
     void (*launch_lineage)(int prec, int variant, hipStream_t s, plom_dfilter *df);
     void (*launch_hat)(int prec, hipStream_t s, plom_dfilter *df, int ntiles, int n_hat);
     void (*launch_hat0)(int prec, hipStream_t s, plom_dfilter *df);
 };
 
-template <class M, typename R>
+template <class M, typename R, bool MIF>
 static void launch_prop_r(int variant, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles)
 {
     switch (variant) {
-    case 0: hipLaunchKernelGGL((k_propagate<M, R, 0>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
-    case 1: hipLaunchKernelGGL((k_propagate<M, R, 1>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
-    case 2: hipLaunchKernelGGL((k_propagate<M, R, 2>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
-    default: hipLaunchKernelGGL((k_propagate<M, R, 3>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
+    case 0: hipLaunchKernelGGL((k_propagate<M, R, 0, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
+    case 1: hipLaunchKernelGGL((k_propagate<M, R, 1, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
+    case 2: hipLaunchKernelGGL((k_propagate<M, R, 2, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
+    default: hipLaunchKernelGGL((k_propagate<M, R, 3, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
     }
 }
 
 template <class M>
-static void launch_prop(int prec, int variant, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles)
+static void launch_prop(int prec, int variant, int mif, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles)
 {
-    if (prec == PLOM_FP32) launch_prop_r<M, float>(variant, nblocks, s, df, tiles);
-    else launch_prop_r<M, double>(variant, nblocks, s, df, tiles);
+    if (prec == PLOM_FP32) {
+        if (mif) launch_prop_r<M, float, true>(variant, nblocks, s, df, tiles);
+        else launch_prop_r<M, float, false>(variant, nblocks, s, df, tiles);
+    } else {
+        if (mif) launch_prop_r<M, double, true>(variant, nblocks, s, df, tiles);
+        else launch_prop_r<M, double, false>(variant, nblocks, s, df, tiles);
+    }
 }
 
 template <class M, typename R>
@@ -804,7 +998,8 @@ static void launch_hat0(int prec, hipStream_t s, plom_dfilter *df)
 #define MODEL_ENTRY(lname, UNAME)                                                               \
     { #lname,                                                                                   \
       { UNAME##_SIZE_PROJ, UNAME##_N_TS, UNAME##_N_CAC, UNAME##_N_PAR_SV, UNAME##_INC_OFFSET,   \
-        UNAME##_N_TS_INC_UNIQUE, UNAME##_PV_SIZE, UNAME##_N_DEM, UNAME##_N_ENV, UNAME##_N_HAT },  \
+        UNAME##_N_TS_INC_UNIQUE, UNAME##_PV_SIZE, UNAME##_N_DEM, UNAME##_N_ENV, UNAME##_N_HAT,    \
+        UNAME##_N_PP },                                                                         \
       launch_prop<plom_model_##lname>, launch_lineage<plom_model_##lname>, launch_hat<plom_model_##lname>, \
       launch_hat0<plom_model_##lname> }
 
@@ -879,6 +1074,9 @@ struct plom_filter {
     int has_run;
     int dirty;              /* descriptor changed on the device since batch plans copied it */
     size_t off_pv, off_pvf, off_x0;   /* image offsets of the parameter and initial-state inputs */
+    size_t off_mutfac;      /* PLOM_FLAG_MIF: image offset of mut_fac */
+    int mif_ready;          /* PLOM_FLAG_MIF: a starting cloud is uploaded and not yet consumed */
+    std::vector<int32_t> nonan;       /* per interval: observed series (host copy) */
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -890,8 +1088,10 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 struct layout {
     size_t off_X0, off_X1, off_logw, off_cum, off_tmax, off_tsum, off_tsum2, off_ttot, off_tincl, off_anc,
         off_lslots, off_lstate, off_lobs, off_hkey, off_hpref, off_hpart, off_hat, off_pres,
+        off_th0, off_th1, off_mpw, off_mps, off_mpm, off_mpq, off_mmean, off_mvar,
         off_img, off_df, off_cnt, off_scal, off_naf, off_llike_n, off_ess_n, off_rflag, off_nonan, off_times,
-        off_data, off_pv, off_pvf, off_x0, total;
+        off_data, off_pv, off_pvf, off_x0, off_pm_src, off_pm_tr, off_pm_sc, off_pm_dur, off_pm_min, off_pm_max,
+        off_pm_mult, off_pm_sd, off_mutfac, total;
 };
 
 static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, layout *L)
@@ -921,6 +1121,15 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     PUT(off_hpart, NH * T * 8);
     PUT(off_hat, NH * (N + 1) * 3 * 8);
     PUT(off_pres, NH ? N * (size_t)mi->n_ts * 2 * 8 : 0);
+    const size_t NT = (d->flags & PLOM_FLAG_MIF) ? (size_t)d->n_theta : 0;
+    PUT(off_th0, NT * Jp * 8);
+    PUT(off_th1, NT * Jp * 8);
+    PUT(off_mpw, NT ? T * 8 : 0);
+    PUT(off_mps, NT * T * 8);
+    PUT(off_mpm, NT * T * 8);
+    PUT(off_mpq, NT * T * 8);
+    PUT(off_mmean, NT * (N + 1) * 8);
+    PUT(off_mvar, NT * (N + 1) * 8);
     L->off_img = o;
     PUT(off_df, sizeof(plom_dfilter));
     PUT(off_cnt, 2 * 4);
@@ -935,6 +1144,16 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     PUT(off_pv, (size_t)mi->pv_size * 8);
     PUT(off_pvf, (size_t)mi->pv_size * 4);
     PUT(off_x0, S * 8);
+    const size_t NPP = NT ? (size_t)mi->n_pp : 0;
+    PUT(off_pm_src, NPP * 4);
+    PUT(off_pm_tr, NT * 4);
+    PUT(off_pm_sc, NT * 4);
+    PUT(off_pm_dur, NT * 4);
+    PUT(off_pm_min, NT * 8);
+    PUT(off_pm_max, NT * 8);
+    PUT(off_pm_mult, NT * 8);
+    PUT(off_pm_sd, NT * 8);
+    PUT(off_mutfac, NT ? N * 8 : 0);
 #undef PUT
     L->total = o;
     return PLOM_OK;
@@ -950,6 +1169,11 @@ static int check_desc(const plom_filter_desc *d, plom_model_info *mi)
     if (d->precision != PLOM_FP64 && d->precision != PLOM_FP32) return set_err(PLOM_ERR_ARG, "bad precision %d", d->precision);
     if (!(d->dt > 0.0)) return set_err(PLOM_ERR_ARG, "dt must be > 0");
     if (!(d->like_min > 0.0)) return set_err(PLOM_ERR_ARG, "like_min must be > 0");
+    if (d->flags & PLOM_FLAG_MIF) {
+        if (d->n_theta < 1 || d->n_theta > 4096) return set_err(PLOM_ERR_ARG, "n_theta=%d outside [1, 4096]", d->n_theta);
+        if (d->flags & (PLOM_FLAG_NO_FILTER | PLOM_FLAG_HAT | PLOM_FLAG_LINEAGE))
+            return set_err(PLOM_ERR_ARG, "PLOM_FLAG_MIF does not combine with NO_FILTER, HAT or LINEAGE");
+    }
     return PLOM_OK;
 }
 
@@ -1054,6 +1278,17 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     int rc = check_desc(d, &mi);
     if (rc != PLOM_OK) return rc;
     if (!in->times || !in->pv || !in->x0 || (d->N > 0 && !in->data)) return set_err(PLOM_ERR_ARG, "null input array");
+    const plom_mif_map *mm = in->mif;
+    if (d->flags & PLOM_FLAG_MIF) {
+        if (!mm || !mm->pv_src || !mm->tr || !mm->scale || !mm->duration || !mm->min || !mm->max || !mm->multiplier || !mm->sd)
+            return set_err(PLOM_ERR_ARG, "PLOM_FLAG_MIF needs a complete plom_mif_map");
+        for (int s2 = 0; s2 < mi.n_pp; s2++)
+            if (mm->pv_src[s2] < -1 || mm->pv_src[s2] >= d->n_theta)
+                return set_err(PLOM_ERR_ARG, "mif pv_src[%d]=%d outside [-1, n_theta)", s2, mm->pv_src[s2]);
+        for (int o = 0; o < d->n_theta; o++)
+            if (mm->tr[o] < 0 || mm->tr[o] > 3 || mm->scale[o] < 0 || mm->scale[o] > 2 || !(mm->sd[o] >= 0.0))
+                return set_err(PLOM_ERR_ARG, "mif component %d: bad transform, scale or sd", o);
+    }
     for (int n = 0; n < d->N; n++)
         if (!(in->times[n + 1] > in->times[n])) return set_err(PLOM_ERR_ARG, "times must increase (n=%d)", n);
     HIPCHK(hipSetDevice(c->device));
@@ -1113,18 +1348,51 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     h->hpart = hat_on ? (double *)(b + L.off_hpart) : nullptr;
     h->hat = hat_on ? (double *)(b + L.off_hat) : nullptr;
     h->pres = hat_on ? (double *)(b + L.off_pres) : nullptr;
+    const int mif_on = (d->flags & PLOM_FLAG_MIF) != 0;
+    h->n_theta = mif_on ? d->n_theta : 0;
+    h->n_pp = mif_on ? mi.n_pp : 0;
+    h->pm_src = mif_on ? (const int32_t *)(b + L.off_pm_src) : nullptr;
+    h->pm_tr = mif_on ? (const int32_t *)(b + L.off_pm_tr) : nullptr;
+    h->pm_sc = mif_on ? (const int32_t *)(b + L.off_pm_sc) : nullptr;
+    h->pm_dur = mif_on ? (const int32_t *)(b + L.off_pm_dur) : nullptr;
+    h->pm_min = mif_on ? (const double *)(b + L.off_pm_min) : nullptr;
+    h->pm_max = mif_on ? (const double *)(b + L.off_pm_max) : nullptr;
+    h->pm_mult = mif_on ? (const double *)(b + L.off_pm_mult) : nullptr;
+    h->pm_sd = mif_on ? (const double *)(b + L.off_pm_sd) : nullptr;
+    h->mut_fac = mif_on ? (const double *)(b + L.off_mutfac) : nullptr;
+    h->th[0] = mif_on ? (double *)(b + L.off_th0) : nullptr;
+    h->th[1] = mif_on ? (double *)(b + L.off_th1) : nullptr;
+    h->mpw = mif_on ? (double *)(b + L.off_mpw) : nullptr;
+    h->mps = mif_on ? (double *)(b + L.off_mps) : nullptr;
+    h->mpm = mif_on ? (double *)(b + L.off_mpm) : nullptr;
+    h->mpq = mif_on ? (double *)(b + L.off_mpq) : nullptr;
+    h->mmean = mif_on ? (double *)(b + L.off_mmean) : nullptr;
+    h->mvar = mif_on ? (double *)(b + L.off_mvar) : nullptr;
     f->df = (plom_dfilter *)(b + L.off_df);
     f->off_pv = L.off_pv; f->off_pvf = L.off_pvf; f->off_x0 = L.off_x0;
+    f->off_mutfac = L.off_mutfac;
 
     /* host image: descriptor, zeroed counters and outputs, inputs */
     const int S = mi.size_proj;
     auto at = [&](size_t off) { return img + (off - L.off_img); };  /* device offset -> image byte */
     memcpy(at(L.off_df), h, sizeof(plom_dfilter));
     int32_t *nonan = (int32_t *)at(L.off_nonan);
+    f->nonan.assign((size_t)d->N, 0);
     for (int n = 0; n < d->N; n++) {
         int k = 0;
         for (int ts = 0; ts < mi.n_ts; ts++) k += !(in->data[(size_t)n * mi.n_ts + ts] != in->data[(size_t)n * mi.n_ts + ts]);
         nonan[n] = k;
+        f->nonan[(size_t)n] = k;
+    }
+    if (mif_on) {
+        memcpy(at(L.off_pm_src), mm->pv_src, (size_t)mi.n_pp * 4);
+        memcpy(at(L.off_pm_tr), mm->tr, (size_t)d->n_theta * 4);
+        memcpy(at(L.off_pm_sc), mm->scale, (size_t)d->n_theta * 4);
+        memcpy(at(L.off_pm_dur), mm->duration, (size_t)d->n_theta * 4);
+        memcpy(at(L.off_pm_min), mm->min, (size_t)d->n_theta * 8);
+        memcpy(at(L.off_pm_max), mm->max, (size_t)d->n_theta * 8);
+        memcpy(at(L.off_pm_mult), mm->multiplier, (size_t)d->n_theta * 8);
+        memcpy(at(L.off_pm_sd), mm->sd, (size_t)d->n_theta * 8);
     }
     memcpy(at(L.off_times), in->times, (size_t)(d->N + 1) * 8);
     if (d->N > 0) memcpy(at(L.off_data), in->data, (size_t)d->N * mi.n_ts * 8);
@@ -1198,6 +1466,44 @@ extern "C" int plom_filter_update(plom_ctx *c, plom_filter *f, const double *pv,
     return PLOM_OK;
 }
 
+extern "C" int plom_filter_mif_start(plom_ctx *c, plom_filter *f, const double *theta, const double *x0,
+                                     const double *mut_fac, uint32_t seed, uint32_t stream)
+{
+    if (!c || !f || !theta || !x0 || (f->desc.N > 0 && !mut_fac)) return set_err(PLOM_ERR_ARG, "null argument");
+    if (!(f->desc.flags & PLOM_FLAG_MIF)) return set_err(PLOM_ERR_STATE, "filter was not created with PLOM_FLAG_MIF");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t J = (size_t)f->desc.J, Jp = (size_t)f->h.Jp, S = (size_t)f->info.size_proj, NT = (size_t)f->desc.n_theta;
+    const size_t N = (size_t)f->desc.N;
+    for (size_t n = 0; n < N; n++)
+        if (!(mut_fac[n] >= 0.0)) return set_err(PLOM_ERR_ARG, "mut_fac[%zu] must be >= 0", n);
+    /* theta -> th[0] and x0 -> X[0], padded rows, in one staged copy each */
+    std::vector<double> tb(NT * Jp, 0.0);
+    for (size_t o = 0; o < NT; o++) memcpy(tb.data() + o * Jp, theta + o * J, J * 8);
+    HIPCHK(hipMemcpyAsync(f->h.th[0], tb.data(), tb.size() * 8, hipMemcpyHostToDevice, c->stream));
+    std::vector<double> xb;
+    std::vector<float> xf;
+    if (f->desc.precision == PLOM_FP32) {
+        xf.assign(S * Jp, 0.0f);
+        for (size_t k = 0; k < S; k++)
+            for (size_t j = 0; j < J; j++) xf[k * Jp + j] = (float)x0[k * J + j];
+        HIPCHK(hipMemcpyAsync(f->h.X[0], xf.data(), xf.size() * 4, hipMemcpyHostToDevice, c->stream));
+    } else {
+        xb.assign(S * Jp, 0.0);
+        for (size_t k = 0; k < S; k++) memcpy(xb.data() + k * Jp, x0 + k * J, J * 8);
+        HIPCHK(hipMemcpyAsync(f->h.X[0], xb.data(), xb.size() * 8, hipMemcpyHostToDevice, c->stream));
+    }
+    if (N) HIPCHK(hipMemcpyAsync(f->dev + f->off_mutfac, mut_fac, N * 8, hipMemcpyHostToDevice, c->stream));
+    f->desc.seed = seed;
+    f->desc.stream = stream;
+    f->h.seed = seed;
+    f->h.stream = stream;
+    HIPCHK(hipMemcpyAsync(f->df, &f->h, sizeof(plom_dfilter), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    f->dirty = 1;
+    f->mif_ready = 1;
+    return PLOM_OK;
+}
+
 extern "C" int plom_filter_destroy(plom_ctx *c, plom_filter *f)
 {
     if (!f) return PLOM_OK;
@@ -1210,9 +1516,15 @@ extern "C" int plom_filter_destroy(plom_ctx *c, plom_filter *f)
 
 /* weights, fixed-point scan and tile prefix of one interval: K2 (one block per filter),
  * K3 (one block per tile), K4 (one block per filter) */
-static void enqueue_resample(hipStream_t s, plom_dfilter *df, const int2 *tiles, int ntiles, const int *fidx, int nfilters)
+static void enqueue_resample(hipStream_t s, plom_dfilter *df, const int2 *tiles, int ntiles, const int *fidx, int nfilters,
+                             int mif = 0)
 {
     hipLaunchKernelGGL(k_weights, dim3(nfilters), dim3(TILE), 0, s, df, fidx);
+    if (mif) {
+        /* theta moments of the weighted cloud, before the scan and the interval counter move on */
+        hipLaunchKernelGGL(k_mif_tiles, dim3(ntiles), dim3(TILE), 0, s, df, tiles);
+        hipLaunchKernelGGL(k_mif_reduce, dim3(nfilters), dim3(TILE), 0, s, df, fidx);
+    }
     hipLaunchKernelGGL(k_scan, dim3(ntiles), dim3(TILE), 0, s, df, tiles);
     hipLaunchKernelGGL(k_advance, dim3(nfilters), dim3(TILE), 0, s, df, fidx);
 }
@@ -1240,16 +1552,21 @@ extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
     }
     const model_entry *me = &g_models[f->desc.model_id];
     const int T = f->h.T;
+    const int mif = (f->desc.flags & PLOM_FLAG_MIF) != 0;
+    if (mif) {
+        if (!f->mif_ready) return set_err(PLOM_ERR_STATE, "PLOM_FLAG_MIF: call plom_filter_mif_start before every run");
+        f->mif_ready = 0;
+    }
     (void)hipGetLastError(); /* a not-ready query or an unused event earlier must not be reported as a launch failure */
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     int k = 0;
     c->kprop_launches = 0;
     for (int n = 0; n < f->desc.N; n++) {
         if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
-        me->launch_prop(f->desc.precision, f->variant, T, c->stream, f->df, nullptr);
+        me->launch_prop(f->desc.precision, f->variant, mif, T, c->stream, f->df, nullptr);
         c->kprop_launches++;
         if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
-        enqueue_resample(c->stream, f->df, nullptr, T, nullptr, 1);
+        enqueue_resample(c->stream, f->df, nullptr, T, nullptr, 1, mif);
         if (f->desc.flags & PLOM_FLAG_HAT) me->launch_hat(f->desc.precision, c->stream, f->df, T, f->info.n_hat);
     }
     if ((f->desc.flags & PLOM_FLAG_HAT) && f->desc.N > 0) me->launch_hat0(f->desc.precision, c->stream, f->df);
@@ -1294,6 +1611,10 @@ static int filter_read_on(plom_ctx *c, plom_filter *f, plom_filter_outputs *o, h
             HIPCHK(hipMemcpyAsync(o->lineage, h->lin_state, (N + 1) * (size_t)f->info.size_proj * 8, hipMemcpyDeviceToHost, s));
         if (o->lineage_slots) HIPCHK(hipMemcpyAsync(o->lineage_slots, h->lin_slots, N * 4, hipMemcpyDeviceToHost, s));
     }
+    if ((o->mif_mean || o->mif_var) && !h->mmean) return set_err(PLOM_ERR_STATE, "mif moments requested but PLOM_FLAG_MIF not set");
+    const size_t NT = (size_t)h->n_theta;
+    if (o->mif_mean && N) HIPCHK(hipMemcpyAsync(o->mif_mean + NT, h->mmean + NT, N * NT * 8, hipMemcpyDeviceToHost, s));
+    if (o->mif_var && N) HIPCHK(hipMemcpyAsync(o->mif_var + NT, h->mvar + NT, N * NT * 8, hipMemcpyDeviceToHost, s));
     std::vector<float> tmp;
     if (o->final_state) {
         const int S = f->info.size_proj;
@@ -1310,6 +1631,13 @@ static int filter_read_on(plom_ctx *c, plom_filter *f, plom_filter_outputs *o, h
     HIPCHK(hipStreamSynchronize(s));
     if (!tmp.empty())
         for (size_t i = 0; i < tmp.size(); i++) o->final_state[i] = tmp[i];
+    /* rows without data (and row 0) carry no moments */
+    for (size_t r = 0; r <= N && (o->mif_mean || o->mif_var) && NT; r++)
+        if (r == 0 || f->nonan[r - 1] == 0)
+            for (size_t q = 0; q < NT; q++) {
+                if (o->mif_mean) o->mif_mean[r * NT + q] = NAN;
+                if (o->mif_var) o->mif_var[r * NT + q] = NAN;
+            }
     o->loglik = scal[3];
     o->n_all_fail = naf;
     return PLOM_OK;
@@ -1345,13 +1673,14 @@ extern "C" int plom_smc_run(plom_ctx *c, const plom_filter_desc *d, const plom_f
  * the results do not depend on the number of chains. */
 struct batch_chain {
     std::vector<plom_filter *> fs;     /* N-descending: row r of the chain's descriptor array */
-    struct group { int model, prec, variant; std::vector<int2> tiles; int2 *d; };
+    struct group { int model, prec, variant, mif; std::vector<int2> tiles; int2 *d; };
     std::vector<group> groups;
     std::vector<int2> all_tiles;
     int2 *d_all = nullptr;
     plom_dfilter *ddf = nullptr;
     char *buf = nullptr;
     int maxN = 0;
+    int any_mif = 0;
 };
 
 struct plom_batch {
@@ -1377,10 +1706,12 @@ static int chain_build(plom_ctx *c, batch_chain *ch)
     for (int r = 0; r < nf; r++) {
         plom_filter *f = ch->fs[(size_t)r];
         batch_chain::group *g = nullptr;
+        const int mif = (f->desc.flags & PLOM_FLAG_MIF) != 0;
+        ch->any_mif |= mif;
         for (auto &gg : ch->groups)
-            if (gg.model == f->desc.model_id && gg.prec == f->desc.precision && gg.variant == f->variant) g = &gg;
+            if (gg.model == f->desc.model_id && gg.prec == f->desc.precision && gg.variant == f->variant && gg.mif == mif) g = &gg;
         if (!g) {
-            ch->groups.push_back(batch_chain::group{f->desc.model_id, f->desc.precision, f->variant, {}, nullptr});
+            ch->groups.push_back(batch_chain::group{f->desc.model_id, f->desc.precision, f->variant, mif, {}, nullptr});
             g = &ch->groups.back();
         }
         for (int t = 0; t < f->h.T; t++) {
@@ -1420,7 +1751,7 @@ static int chain_propagate(batch_chain *ch, int s, hipStream_t st)
     int launches = 0;
     for (auto &g : ch->groups) {
         const int nb = prefix_len(ch, g.tiles, s);
-        if (nb > 0) { g_models[g.model].launch_prop(g.prec, g.variant, nb, st, ch->ddf, g.d); launches++; }
+        if (nb > 0) { g_models[g.model].launch_prop(g.prec, g.variant, g.mif, nb, st, ch->ddf, g.d); launches++; }
     }
     return launches;
 }
@@ -1430,7 +1761,7 @@ static void chain_resample(batch_chain *ch, int s, hipStream_t st)
     const int nb = prefix_len(ch, ch->all_tiles, s);
     int nfa = 0;
     while (nfa < (int)ch->fs.size() && ch->fs[(size_t)nfa]->desc.N > s) nfa++;
-    enqueue_resample(st, ch->ddf, (const int2 *)ch->d_all, nb, nullptr, nfa);
+    enqueue_resample(st, ch->ddf, (const int2 *)ch->d_all, nb, nullptr, nfa, ch->any_mif);
     for (int r = 0; r < nfa; r++) {
         plom_filter *f = ch->fs[(size_t)r];
         if (f->desc.flags & PLOM_FLAG_HAT)
@@ -1528,6 +1859,10 @@ static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing, int rounds)
 
 static int batch_launch(plom_ctx *c, plom_batch *b, int use_graph)
 {
+    for (auto f : b->fs)
+        if ((f->desc.flags & PLOM_FLAG_MIF) && !f->mif_ready)
+            return set_err(PLOM_ERR_STATE, "PLOM_FLAG_MIF: call plom_filter_mif_start before every launch");
+    for (auto f : b->fs) f->mif_ready = 0;
     /* descriptors changed by plom_filter_update since the plan copied them */
     for (auto &ch : b->chains)
         for (size_t r = 0; r < ch.fs.size(); r++)
@@ -1630,6 +1965,9 @@ extern "C" int plom_batch_autotune(plom_ctx *c, plom_batch *b, int rounds, int32
 {
     if (!c || !b) return set_err(PLOM_ERR_ARG, "null argument");
     if (rounds < 1) return set_err(PLOM_ERR_ARG, "rounds=%d < 1", rounds);
+    for (auto f : b->fs)
+        if (f->desc.flags & PLOM_FLAG_MIF)
+            return set_err(PLOM_ERR_ARG, "autotuning runs the filters and would consume the MIF starting clouds");
     HIPCHK(hipSetDevice(c->device));
     static const int cand[4] = {1, 2, 4, 8};
     int best = 1;
@@ -1923,8 +2261,8 @@ extern "C" int plom_server_submit(plom_server *S, const plom_filter_desc *d, con
     plom_model_info mi;
     int rc = check_desc(d, &mi);
     if (rc != PLOM_OK) return rc;
-    if (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_FINAL_STATE | PLOM_FLAG_HAT))
-        return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods and lineages (no ancestry, final state or hat)");
+    if (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_FINAL_STATE | PLOM_FLAG_HAT | PLOM_FLAG_MIF))
+        return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods and lineages (no ancestry, final state, hat or MIF)");
     if (!in->times || !in->pv || !in->x0 || (d->N > 0 && !in->data)) return set_err(PLOM_ERR_ARG, "null input array");
     for (int n = 0; n < d->N; n++)
         if (!(in->times[n + 1] > in->times[n])) return set_err(PLOM_ERR_ARG, "times must increase (n=%d)", n);
@@ -1977,7 +2315,7 @@ static int srv_lane_step(plom_server *S, srv_lane *L)
     }
     (void)hipGetLastError();
     for (auto &g : L->groups)
-        g_models[g.model].launch_prop(g.prec, g.variant, (int)g.n, L->stream, S->d_slots, L->d_tiles + g.off);
+        g_models[g.model].launch_prop(g.prec, g.variant, 0, (int)g.n, L->stream, S->d_slots, L->d_tiles + g.off);
     enqueue_resample(L->stream, S->d_slots, (const int2 *)L->d_tiles, (int)L->n_all, L->d_slots_active, (int)L->active.size());
     HIPCHK(hipGetLastError());
     for (size_t i = 0; i < L->active.size();) {

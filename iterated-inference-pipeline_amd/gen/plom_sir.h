@@ -13,6 +13,7 @@
 #define PLOM_SIR_N_DEM 2
 #define PLOM_SIR_N_ENV 0
 #define PLOM_SIR_N_HAT 4
+#define PLOM_SIR_N_PP 4
 
 PLOM_HD void plom_sir_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
 {
@@ -299,8 +300,262 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_proj2obs_f
     obs[0] = (double)X[2];
 }
 
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    const double dtr = (double)dt;
+    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    const double P0 = lp[0];
+    const double P1 = lp[1];
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    X[2] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = (double)t; (void)tr;
+        double s2 = 0.0;
+        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r1_0 = P1;
+        double za0, zb0;
+        plom_normal_pair64(ns, &za0, &zb0);
+        const double d0_0 = sqdt * za0;
+        const double d1_0 = sqdt * zb0;
+        const double f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0;
+        const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[1]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
+        s2 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0;
+        X[0] = (f0 < 0.0) ? 0.0 : f0;
+        X[1] = (f1 < 0.0) ? 0.0 : f1;
+        const double f2 = X[2] + s2;
+        X[2] = (f2 < 0.0) ? 0.0 : f2;
+        t += dt;
+    }
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    const double dtr = (double)dt;
+    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    const double P0 = lp[0];
+    const double P1 = lp[1];
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    X[2] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = (double)t; (void)tr;
+        double s2 = 0.0;
+        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r1_0 = P1;
+        const double f0 = X[0] + ( - (r0_0*X[0]))*dtr;
+        const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
+        s2 += ( + (r0_0*X[0]))*dtr;
+        X[0] = (f0 < 0.0) ? 0.0 : f0;
+        X[1] = (f1 < 0.0) ? 0.0 : f1;
+        const double f2 = X[2] + s2;
+        X[2] = (f2 < 0.0) ? 0.0 : f2;
+        t += dt;
+    }
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    const double dtr = (double)dt;
+    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    const double P0 = lp[0];
+    const double P1 = lp[1];
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    X[2] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = (double)t; (void)tr;
+        double s2 = 0.0;
+        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r1_0 = P1;
+        double za0, zb0;
+        plom_normal_pair64(ns, &za0, &zb0);
+        const double d0_0 = sqdt * za0;
+        const double d1_0 = sqdt * zb0;
+        const double f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0;
+        const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[1]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
+        s2 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0;
+        X[0] = (f0 < 0.0) ? 0.0 : f0;
+        X[1] = (f1 < 0.0) ? 0.0 : f1;
+        const double f2 = X[2] + s2;
+        X[2] = (f2 < 0.0) ? 0.0 : f2;
+        t += dt;
+    }
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    const double dtr = (double)dt;
+    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    const double P0 = lp[0];
+    const double P1 = lp[1];
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    X[2] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = (double)t; (void)tr;
+        double s2 = 0.0;
+        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r1_0 = P1;
+        const double f0 = X[0] + ( - (r0_0*X[0]))*dtr;
+        const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
+        s2 += ( + (r0_0*X[0]))*dtr;
+        X[0] = (f0 < 0.0) ? 0.0 : f0;
+        X[1] = (f1 < 0.0) ? 0.0 : f1;
+        const double f2 = X[2] + s2;
+        X[2] = (f2 < 0.0) ? 0.0 : f2;
+        t += dt;
+    }
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sir_proj2obs_pp_f64(const double *X, const double *pvd, const double *lpd, double *obs)
+{
+    (void)pvd; (void)lpd; (void)X;
+    obs[0] = (double)X[2];
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+{
+    const float dtr = (float)dt;
+    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    const float P0 = lp[0];
+    const float P1 = lp[1];
+    const float P4 = PLOM_UNIFORM32(pv[4]);
+    X[2] = 0.0f; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const float tr = (float)t; (void)tr;
+        float s2 = 0.0f;
+        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
+        const float r1_0 = P1;
+        float za0, zb0;
+        plom_normal_pair32(ns, &za0, &zb0);
+        const float d0_0 = sqdt * za0;
+        const float d1_0 = sqdt * zb0;
+        const float f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_fsqrtf((r0_0*X[0]))*d0_0;
+        const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_fsqrtf((r1_0*X[1]))*d1_0+ plom_fsqrtf((r0_0*X[0]))*d0_0;
+        s2 += ( + (r0_0*X[0]))*dtr + + plom_fsqrtf((r0_0*X[0]))*d0_0;
+        X[0] = plom_clipf(f0);
+        X[1] = plom_clipf(f1);
+        const float f2 = X[2] + s2;
+        X[2] = plom_clipf(f2);
+        t += dt;
+    }
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+{
+    const float dtr = (float)dt;
+    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    const float P0 = lp[0];
+    const float P1 = lp[1];
+    const float P4 = PLOM_UNIFORM32(pv[4]);
+    X[2] = 0.0f; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const float tr = (float)t; (void)tr;
+        float s2 = 0.0f;
+        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
+        const float r1_0 = P1;
+        const float f0 = X[0] + ( - (r0_0*X[0]))*dtr;
+        const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
+        s2 += ( + (r0_0*X[0]))*dtr;
+        X[0] = plom_clipf(f0);
+        X[1] = plom_clipf(f1);
+        const float f2 = X[2] + s2;
+        X[2] = plom_clipf(f2);
+        t += dt;
+    }
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+{
+    const float dtr = (float)dt;
+    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    const float P0 = lp[0];
+    const float P1 = lp[1];
+    const float P4 = PLOM_UNIFORM32(pv[4]);
+    X[2] = 0.0f; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const float tr = (float)t; (void)tr;
+        float s2 = 0.0f;
+        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
+        const float r1_0 = P1;
+        float za0, zb0;
+        plom_normal_pair32(ns, &za0, &zb0);
+        const float d0_0 = sqdt * za0;
+        const float d1_0 = sqdt * zb0;
+        const float f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_fsqrtf((r0_0*X[0]))*d0_0;
+        const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_fsqrtf((r1_0*X[1]))*d1_0+ plom_fsqrtf((r0_0*X[0]))*d0_0;
+        s2 += ( + (r0_0*X[0]))*dtr + + plom_fsqrtf((r0_0*X[0]))*d0_0;
+        X[0] = plom_clipf(f0);
+        X[1] = plom_clipf(f1);
+        const float f2 = X[2] + s2;
+        X[2] = plom_clipf(f2);
+        t += dt;
+    }
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+{
+    const float dtr = (float)dt;
+    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    const float P0 = lp[0];
+    const float P1 = lp[1];
+    const float P4 = PLOM_UNIFORM32(pv[4]);
+    X[2] = 0.0f; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const float tr = (float)t; (void)tr;
+        float s2 = 0.0f;
+        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
+        const float r1_0 = P1;
+        const float f0 = X[0] + ( - (r0_0*X[0]))*dtr;
+        const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
+        s2 += ( + (r0_0*X[0]))*dtr;
+        X[0] = plom_clipf(f0);
+        X[1] = plom_clipf(f1);
+        const float f2 = X[2] + s2;
+        X[2] = plom_clipf(f2);
+        t += dt;
+    }
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sir_proj2obs_pp_f32(const float *X, const double *pvd, const double *lpd, double *obs)
+{
+    (void)pvd; (void)lpd; (void)X;
+    obs[0] = (double)X[2];
+}
+
+static __device__ inline __attribute__((always_inline)) double plom_sir_loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, double t, double like_min, int *n_floor)
+{
+    double loglike = 0.0;
+    int nf = 0;
+    (void)t; (void)pvd; (void)lpd;
+    if (!plom_isnan(y[0])) {
+        const double x = obs[0];
+        const double gsl_mu = (lpd[2] * x);
+        const double gsl_sd = plom_sqrt((((lpd[2] * (1.0 - lpd[2])) * x) + plom_powi(((lpd[2] * lpd[3]) * x), 2)));
+        double like;
+        if (y[0] > 0.0) like = plom_cdf_gaussian_P(y[0] + 0.5 - gsl_mu, gsl_sd) - plom_cdf_gaussian_P(y[0] - 0.5 - gsl_mu, gsl_sd);
+        else like = plom_cdf_gaussian_P(y[0] + 0.5 - gsl_mu, gsl_sd);
+        loglike += plom_log(plom_sanitize_likelihood(like, like_min, &nf));
+    }
+    *n_floor = nf;
+    return loglike;
+}
+
 struct plom_model_sir {
-    static constexpr int S = 3, N_TS = 1, INC_OFFSET = 2, N_INC = 1, PV = 5, N_CAC = 1, N_HAT = 4;
+    static constexpr int S = 3, N_TS = 1, INC_OFFSET = 2, N_INC = 1, PV = 5, N_CAC = 1, N_HAT = 4, NPP = 4;
     template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
     {
         if constexpr (V == 0) plom_sir_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt);
@@ -308,7 +563,15 @@ struct plom_model_sir {
         else if constexpr (V == 2) plom_sir_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
         else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
     }
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+    {
+        if constexpr (V == 0) plom_sir_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 2) plom_sir_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
+    }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, double *obs) { plom_sir_proj2obs_f64(X, pvd, obs); }
+    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, double *obs) { plom_sir_proj2obs_pp_f64(X, pvd, lpd, obs); }
     template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
     {
         if constexpr (V == 0) plom_sir_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt);
@@ -316,8 +579,17 @@ struct plom_model_sir {
         else if constexpr (V == 2) plom_sir_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt);
         else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt);
     }
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+    {
+        if constexpr (V == 0) plom_sir_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 2) plom_sir_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
+    }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, double *obs) { plom_sir_proj2obs_f32(X, pvd, obs); }
+    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, double *obs) { plom_sir_proj2obs_pp_f32(X, pvd, lpd, obs); }
     static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor) { return plom_sir_loglik(obs, y, pvd, t, like_min, n_floor); }
+    static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, double t, double like_min, int *n_floor) { return plom_sir_loglik_pp(obs, y, pvd, lpd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, double t, double *out) { plom_sir_hat_values(X, pvd, t, out); }
     static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var) { plom_sir_obs_moments(obs, pvd, t, mean, var); }
 };

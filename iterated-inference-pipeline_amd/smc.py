@@ -20,25 +20,33 @@ from .codegen import load_model, BUILTIN_MODELS, context_data, load_json
 
 PLOM_FP64, PLOM_FP32 = 0, 1
 NO_DEM_STO, NO_ENV_STO, NO_DRIFT = 1, 2, 4
-FLAG_NO_FILTER, FLAG_STORE_ANCESTRY, FLAG_FINAL_STATE, FLAG_LINEAGE, FLAG_HAT = 1, 2, 4, 8, 16
+FLAG_NO_FILTER, FLAG_STORE_ANCESTRY, FLAG_FINAL_STATE, FLAG_LINEAGE, FLAG_HAT, FLAG_MIF = 1, 2, 4, 8, 16, 32
 MAX_J = 1 << 21
 
 
 class ModelInfo(C.Structure):
     _fields_ = [("size_proj", C.c_int32), ("n_ts", C.c_int32), ("n_cac", C.c_int32),
                 ("n_par_sv", C.c_int32), ("inc_offset", C.c_int32), ("n_inc", C.c_int32),
-                ("pv_size", C.c_int32), ("n_dem", C.c_int32), ("n_env", C.c_int32), ("n_hat", C.c_int32)]
+                ("pv_size", C.c_int32), ("n_dem", C.c_int32), ("n_env", C.c_int32), ("n_hat", C.c_int32),
+                ("n_pp", C.c_int32)]
 
 
 class FilterDesc(C.Structure):
     _fields_ = [("model_id", C.c_int32), ("J", C.c_int32), ("N", C.c_int32), ("precision", C.c_int32),
                 ("noises_off", C.c_uint32), ("flags", C.c_uint32), ("seed", C.c_uint32),
-                ("stream", C.c_uint32), ("dt", C.c_double), ("like_min", C.c_double)]
+                ("stream", C.c_uint32), ("dt", C.c_double), ("like_min", C.c_double), ("n_theta", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class MifMap(C.Structure):
+    _fields_ = [("pv_src", C.POINTER(C.c_int32)), ("tr", C.POINTER(C.c_int32)), ("scale", C.POINTER(C.c_int32)),
+                ("duration", C.POINTER(C.c_int32)), ("min", C.POINTER(C.c_double)), ("max", C.POINTER(C.c_double)),
+                ("multiplier", C.POINTER(C.c_double)), ("sd", C.POINTER(C.c_double))]
 
 
 class FilterInputs(C.Structure):
     _fields_ = [("times", C.POINTER(C.c_double)), ("data", C.POINTER(C.c_double)),
-                ("pv", C.POINTER(C.c_double)), ("x0", C.POINTER(C.c_double))]
+                ("pv", C.POINTER(C.c_double)), ("x0", C.POINTER(C.c_double)), ("mif", C.POINTER(MifMap))]
 
 
 class FilterOutputs(C.Structure):
@@ -47,13 +55,14 @@ class FilterOutputs(C.Structure):
                 ("ancestry", C.POINTER(C.c_int32)), ("final_state", C.POINTER(C.c_double)),
                 ("final_logw", C.POINTER(C.c_double)), ("lineage", C.POINTER(C.c_double)),
                 ("lineage_slots", C.POINTER(C.c_int32)), ("lineage_obs", C.POINTER(C.c_double)),
-                ("hat", C.POINTER(C.c_double)), ("pred_res", C.POINTER(C.c_double))]
+                ("hat", C.POINTER(C.c_double)), ("pred_res", C.POINTER(C.c_double)),
+                ("mif_mean", C.POINTER(C.c_double)), ("mif_var", C.POINTER(C.c_double))]
 
 
 EXPORTS = [
     "plom_last_error", "plom_ctx_create", "plom_ctx_destroy", "plom_model_count", "plom_model_find",
     "plom_model_name", "plom_model_info_get", "plom_smc_run", "plom_filter_create", "plom_filter_reset",
-    "plom_filter_update",
+    "plom_filter_update", "plom_filter_mif_start",
     "plom_filter_run", "plom_filter_read", "plom_filter_destroy", "plom_filter_device_bytes",
     "plom_batch_run", "plom_sync", "plom_ctx_set_kernel_timing", "plom_last_timing", "plom_selftest_math",
     "plom_selftest_normals", "plom_selftest_copy", "plom_batch_create", "plom_batch_launch",
@@ -96,6 +105,7 @@ def lib():
     L.plom_filter_reset.argtypes = [C.c_void_p, C.c_void_p]
     L.plom_filter_update.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.c_uint32, C.c_uint32]
+    L.plom_filter_mif_start.argtypes = [C.c_void_p, C.c_void_p, dp, dp, dp, C.c_uint32, C.c_uint32]
     L.plom_filter_run.argtypes = [C.c_void_p, C.c_void_p]
     L.plom_filter_read.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(FilterOutputs)]
     L.plom_filter_destroy.argtypes = [C.c_void_p, C.c_void_p]
@@ -183,6 +193,7 @@ class Model:
         self.n_ts = self.desc.N_TS
         self.n_cac = self.desc.N_CAC
         self.n_hat = self.desc.N_PAR_SV * self.desc.N_CAC + (self.desc.N_CAC if self.desc.remainder else 0) + self.n_ts
+        self.n_pp = self.desc.layout["npp"]
         self.layout = self.desc.layout
         self.context = load_json(os.path.join(PKG_DIR, "examples", name, "context.json"))
 
@@ -290,7 +301,9 @@ class Filter:
     """A device-resident filter (plom_filter_create / run / read)."""
 
     def __init__(self, ctx, model, J, times, data, pv, x0, precision="fp64", noises_off=0, flags=0,
-                 seed=1, stream=0, dt=0.25, like_min=1e-17):
+                 seed=1, stream=0, dt=0.25, like_min=1e-17, mif=None):
+        """mif (with FLAG_MIF): dict of numpy arrays pv_src [n_pp], tr, scale, duration, min, max,
+        multiplier, sd [n_theta] (plom_mif_map)."""
         self.ctx, self.model = ctx, model
         self.J, self.N = int(J), int(len(times) - 1)
         self.flags = int(flags)
@@ -301,9 +314,35 @@ class Filter:
         if self.pv.size != model.layout["size"] or self.x0.size != model.S:
             raise ValueError("pv / x0 size mismatch for model %s" % model.name)
         self.desc = _desc(model, J, self.N, precision, noises_off, flags, seed, stream, dt, like_min)
-        inp = FilterInputs(dptr(self.times), dptr(self.data), dptr(self.pv), dptr(self.x0))
+        self.n_theta = 0
+        mmap = None
+        if mif is not None:
+            i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+            f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+            self._mif = {k: (i32 if k in ("pv_src", "tr", "scale", "duration") else f64)(mif[k])
+                         for k in ("pv_src", "tr", "scale", "duration", "min", "max", "multiplier", "sd")}
+            self.n_theta = len(self._mif["tr"])
+            if len(self._mif["pv_src"]) != model.n_pp:
+                raise ValueError("pv_src must have n_pp = %d entries" % model.n_pp)
+            self.desc.n_theta = self.n_theta
+            m = self._mif
+            mmap = MifMap(iptr(m["pv_src"]), iptr(m["tr"]), iptr(m["scale"]), iptr(m["duration"]), dptr(m["min"]),
+                          dptr(m["max"]), dptr(m["multiplier"]), dptr(m["sd"]))
+        inp = FilterInputs(dptr(self.times), dptr(self.data), dptr(self.pv), dptr(self.x0),
+                           C.pointer(mmap) if mmap is not None else None)
         self.h = C.c_void_p()
         check(lib().plom_filter_create(ctx.h, C.byref(self.desc), C.byref(inp), C.byref(self.h)))
+
+    def mif_start(self, theta, x0, mut_fac, seed=1, stream=0):
+        """The particles' starting theta [n_theta, J], initial states [S, J] and the random-walk
+        factor [N] of one iterated-filtering pass (plom_filter_mif_start)."""
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        mut_fac = np.ascontiguousarray(mut_fac, dtype=np.float64)
+        if theta.shape != (self.n_theta, self.J) or x0.shape != (self.model.S, self.J) or mut_fac.shape != (self.N,):
+            raise ValueError("theta [n_theta, J], x0 [S, J] and mut_fac [N] expected")
+        check(lib().plom_filter_mif_start(self.ctx.h, self.h, dptr(theta), dptr(x0), dptr(mut_fac),
+                                          int(seed) & 0xffffffff, int(stream) & 0xffffffff))
 
     def device_bytes(self):
         return lib().plom_filter_device_bytes(C.byref(self.desc))
@@ -312,7 +351,7 @@ class Filter:
         check(lib().plom_filter_run(self.ctx.h, self.h))
         return self
 
-    def read(self, ancestry=False, final_state=False, final_logw=False, lineage=False, hat=False):
+    def read(self, ancestry=False, final_state=False, final_logw=False, lineage=False, hat=False, mif=False):
         N, J = self.N, self.J
         ll_n = np.zeros(N)
         ess = np.zeros(N)
@@ -324,11 +363,14 @@ class Filter:
         ht = np.zeros((N + 1, self.model.n_hat, 3)) if hat else None
         pr = np.zeros((N, self.model.n_ts, 2)) if hat else None
         lob = np.zeros((N + 1, self.model.n_ts, 2)) if lineage else None
+        mm = np.zeros((N + 1, self.n_theta)) if mif else None
+        mv = np.zeros((N + 1, self.n_theta)) if mif else None
         out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), iptr(anc), dptr(fs), dptr(fw), dptr(lin), iptr(lsl),
-                            dptr(lob), dptr(ht), dptr(pr))
+                            dptr(lob), dptr(ht), dptr(pr), dptr(mm), dptr(mv))
         check(lib().plom_filter_read(self.ctx.h, self.h, C.byref(out)))
         r = FilterResult(out.loglik, out.n_all_fail, ll_n, ess, anc, fs, fw, lin, lsl)
         r.hat, r.pred_res, r.lineage_obs = ht, pr, lob
+        r.mif_mean, r.mif_var = mm, mv
         return r
 
     def update(self, pv=None, x0=None, seed=1, stream=0):
@@ -404,7 +446,7 @@ class Server:
         pv = np.ascontiguousarray(pv, dtype=np.float64)
         x0 = np.ascontiguousarray(x0, dtype=np.float64)
         d = _desc(model, J, N, precision, noises_off, FLAG_LINEAGE if lineage else 0, seed, stream, dt, like_min)
-        inp = FilterInputs(dptr(times), dptr(data), dptr(pv), dptr(x0))
+        inp = FilterInputs(dptr(times), dptr(data), dptr(pv), dptr(x0), None)
         rid = C.c_uint64()
         check(lib().plom_server_submit(self.h, C.byref(d), C.byref(inp), C.byref(rid)))
         self.N[rid.value] = (N, model.S if lineage else 0)
@@ -433,7 +475,8 @@ class Server:
         ess = np.zeros(N)
         lin = np.zeros((N + 1, S)) if S else None
         lsl = np.zeros(N, dtype=np.int32) if S else None
-        out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), None, None, None, dptr(lin), iptr(lsl), None, None, None)
+        out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), None, None, None, dptr(lin), iptr(lsl), None, None, None,
+                            None, None)
         check(lib().plom_server_result(self.h, rid, C.byref(out)))
         return FilterResult(out.loglik, out.n_all_fail, ll_n, ess, lineage=lin, lineage_slots=lsl)
 

@@ -37,6 +37,12 @@ def lib():
         L.oracle_bm_pairs.argtypes = [C.c_int, C.POINTER(C.c_uint32), dp]
         L.oracle_xoshiro.argtypes = [C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_uint32)]
         L.oracle_model_name.restype = C.c_char_p
+        L.oracle_mif_natural.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.oracle_mif_natural.restype = C.c_double
+        i32p = C.POINTER(C.c_int32)
+        L.oracle_mif_pass.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double,
+                                      C.c_double, dp, dp, dp, C.c_int, i32p, i32p, i32p, i32p, dp, dp, dp, dp, dp,
+                                      dp, dp, C.c_int, dp, i32p, dp, dp, dp, dp, dp, dp, i32p]
         _lib = L
     return _lib
 
@@ -85,6 +91,42 @@ def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=
     return {"loglik": ll.value, "n_all_fail": naf.value, "loglik_n": ll_n, "ess_n": ess,
             "ancestry": anc, "final_state": fs, "logw": lw, "elapsed_s": el.value,
             "lineage_slots": lsl, "lineage": lst, "lineage_obs": lob, "all_states": alls, "hat": ht, "pred_res": pr}
+
+
+def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=0, seed=1, stream=0, dt=0.25,
+             like_min=1e-17, nthreads=1, theta_out=False, logw=False, ancestry=False):
+    """One pass of iterated filtering restated on the CPU (oracle_mif_pass).  mif: dict of
+    pv_src [n_pp], tr, scale, duration, min, max, multiplier, sd [n_theta]; theta0
+    [n_theta, J]; x0 [S, J]; mut_fac [N]."""
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    N = len(times) - 1
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    m = {k: (i32 if k in ("pv_src", "tr", "scale", "duration") else f64)(mif[k])
+         for k in ("pv_src", "tr", "scale", "duration", "min", "max", "multiplier", "sd")}
+    nth = len(m["tr"])
+    theta0, x0, mut_fac = f64(theta0), f64(x0), f64(mut_fac)
+    data, pv = f64(data), f64(pv)
+    ll, naf = C.c_double(), C.c_int32()
+    ll_n, ess = np.zeros(N), np.zeros(N)
+    mm, mv = np.zeros((N + 1, nth)), np.zeros((N + 1, nth))
+    tho = np.zeros((N + 1, nth, J)) if theta_out else None
+    lw = np.zeros((N, J)) if logw else None
+    anc = np.zeros((N, J), dtype=np.int32) if ancestry else None
+    rc = lib().oracle_mif_pass(model_id, J, N, noises_off, seed, stream, dt, like_min, _dp(times), _dp(data), _dp(pv),
+                               nth, _ip(m["pv_src"]), _ip(m["tr"]), _ip(m["scale"]), _ip(m["duration"]), _dp(m["min"]),
+                               _dp(m["max"]), _dp(m["multiplier"]), _dp(m["sd"]), _dp(mut_fac), _dp(theta0), _dp(x0),
+                               nthreads, C.byref(ll), C.byref(naf), _dp(ll_n), _dp(ess), _dp(mm), _dp(mv), _dp(tho),
+                               _dp(lw), _ip(anc))
+    if rc != 0:
+        raise RuntimeError("oracle_mif_pass failed: %d" % rc)
+    return {"loglik": ll.value, "n_all_fail": naf.value, "loglik_n": ll_n, "ess_n": ess, "mif_mean": mm,
+            "mif_var": mv, "theta": tho, "logw": lw, "ancestry": anc}
+
+
+def mif_natural(tr, sc, dur, x, a, b, mult):
+    """plom_mif_natural (csrc/plom_mif.h) through the oracle build."""
+    return lib().oracle_mif_natural(int(tr), int(sc), int(dur), float(x), float(a), float(b), float(mult))
 
 
 def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=0.25):

@@ -12,6 +12,13 @@
  *   weight                  model_builder/C/core/smc.c:27-83
  *   systematic_sampling     model_builder/C/core/smc.c:90-112
  *   resample_X              model_builder/C/core/smc.c:142-161
+ *   one MIF pass            model_builder/C/mif/mif.c:128-200 with per-particle parameters
+ *                           (oracle_mif_pass): the particles' theta resampled with their
+ *                           state and moved by the random walk after every observation
+ *                           (resample_and_mut_theta_mif, C/mif/methods.c:261-367), and the
+ *                           weighted mean / variance of theta per observation
+ *                           (mean_var_theta_theoretical_mif, methods.c:146-205) in the
+ *                           canonical tree order of the GPU (tiles of 256, then tiles)
  * Two arithmetic modes:
  *   ORACLE_CANONICAL (0)    the arithmetic the GPU filter defines (log-domain weights,
  *                           recursive pairwise tree sums over tiles of 256, fixed-point
@@ -36,6 +43,7 @@
 
 #include "plom_math.h"
 #include "plom_rng.h"
+#include "plom_mif.h"
 #include "plom_models.h"
 
 #define ORACLE_CANONICAL 0
@@ -53,7 +61,7 @@ typedef void (*hat_fn)(const double *, const double *, double, double *);
 
 typedef struct {
     const char *name;
-    int S, n_ts, n_hat;
+    int S, n_ts, n_hat, n_pp, pv_size;
     interval_fn interval;
     proj2obs_fn proj2obs;
     loglik_fn loglik;
@@ -62,7 +70,8 @@ typedef struct {
 } oracle_model;
 
 #define ORACLE_MODEL(lname, UNAME)                                                                 \
-    { #lname, UNAME##_SIZE_PROJ, UNAME##_N_TS, UNAME##_N_HAT, plom_##lname##_interval_f64,           \
+    { #lname, UNAME##_SIZE_PROJ, UNAME##_N_TS, UNAME##_N_HAT, UNAME##_N_PP, UNAME##_PV_SIZE,           \
+      plom_##lname##_interval_f64,                                                                   \
       plom_##lname##_proj2obs_f64, plom_##lname##_loglik, plom_##lname##_obs_moments, plom_##lname##_hat_values }
 
 static const oracle_model MODELS[] = {
@@ -77,6 +86,11 @@ const char *oracle_model_name(int m) { return (m >= 0 && m < N_MODELS) ? MODELS[
 int oracle_model_size_proj(int m) { return MODELS[m].S; }
 int oracle_model_n_ts(int m) { return MODELS[m].n_ts; }
 int oracle_model_n_hat(int m) { return MODELS[m].n_hat; }
+int oracle_model_n_pp(int m) { return MODELS[m].n_pp; }
+double oracle_mif_natural(int tr, int sc, int dur, double x, double a, double b, double mult)
+{
+    return plom_mif_natural(tr, sc, dur, x, a, b, mult);
+}
 
 /* ------------------------------------------------------------------ canonical trees */
 /* recursive contiguous pairwise sum of a[0..n), n a power of two */
@@ -99,6 +113,7 @@ typedef struct {
     double *X;          /* [J][S] */
     double *logw;       /* [J] */
     const double *pv, *times, *data;
+    size_t pv_stride;   /* 0: one pv for every particle; else particle j uses pv + j * pv_stride */
     int J, n, nonan, filter, n_ts;
     uint32_t seed, stream, noises_off;
     double dt, sqdt, like_min;
@@ -112,13 +127,14 @@ static void propagate_range(prop_job *p)
     double obs[64];
     for (int j = p->j0; j < p->j1; j++) {
         double *x = p->X + (size_t)j * S;
+        const double *pv = p->pv + (size_t)j * p->pv_stride;
         plom_nstream64 ns;
         plom_nstream64_init(&ns, p->seed, p->stream, (uint32_t)p->n, (uint32_t)j, plom_bm_tab);
-        M->interval(x, p->pv, &ns, p->times[p->n], p->times[p->n + 1], p->dt, p->sqdt, p->noises_off);
+        M->interval(x, pv, &ns, p->times[p->n], p->times[p->n + 1], p->dt, p->sqdt, p->noises_off);
         if (p->filter && p->nonan) {
             int nf = 0;
-            M->proj2obs(x, p->pv, obs);
-            double l = M->loglik(obs, p->data + (size_t)p->n * p->n_ts, p->pv, p->times[p->n + 1], p->like_min, &nf);
+            M->proj2obs(x, pv, obs);
+            double l = M->loglik(obs, p->data + (size_t)p->n * p->n_ts, pv, p->times[p->n + 1], p->like_min, &nf);
             if (p->mode == ORACLE_CANONICAL && nf == p->n_ts) l = -INFINITY;
             p->logw[j] = l;
         }
@@ -507,6 +523,172 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
     if (anc && anc != ancestry) free(anc);
     free(hvals); free(hbuf); free(hitems); free(hq); free(offspring);
     free(X); free(Xn); free(logw); free(work); free(cum); free(select);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ iterated filtering */
+/* recursive contiguous pairwise merge of n (count, mean, M2) triples (n a power of two) */
+static void chan_tree(const double *c, const double *m, const double *q, int n, double *oc, double *om, double *oq)
+{
+    if (n == 1) { *oc = c[0]; *om = m[0]; *oq = q[0]; return; }
+    double ca, ma, qa, cb, mb, qb;
+    chan_tree(c, m, q, n / 2, &ca, &ma, &qa);
+    chan_tree(c + n / 2, m + n / 2, q + n / 2, n / 2, &cb, &mb, &qb);
+    plom_mif_combine(ca, ma, qa, cb, mb, qb, oc, om, oq);
+}
+
+/* One pass of iterated filtering with per-particle parameters.  theta0 [n_theta][J] and
+ * x0 [S][J] are the particles' starting parameters and states; after an interval n-1 with
+ * data, every component with sd > 0 moves by (mut_fac[n-1] * sd) * z (z: plom_mif_normals of
+ * the particle's new slot); the pv slots s < n_pp with pv_src[s] >= 0 are the natural values
+ * of their components.  Outputs as the filter's, plus mif_mean / mif_var rows (n+1) for the
+ * intervals with data and, optionally, every particle's theta per row [(N+1)][n_theta][J]. */
+int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed, uint32_t stream, double dt,
+                    double like_min, const double *times, const double *data, const double *pv, int n_theta,
+                    const int32_t *pv_src, const int32_t *tr, const int32_t *sc, const int32_t *dur, const double *mn,
+                    const double *mx, const double *mult, const double *sd, const double *mut_fac,
+                    const double *theta0, const double *x0, int nthreads, double *loglik, int32_t *n_all_fail,
+                    double *loglik_n, double *ess_n, double *mif_mean, double *mif_var, double *theta_out,
+                    double *logw_out, int32_t *ancestry)
+{
+    if (model < 0 || model >= N_MODELS || J < 1 || N < 0 || n_theta < 1) return -1;
+    const oracle_model *M = &MODELS[model];
+    const int S = M->S, n_ts = M->n_ts, P = M->n_pp;
+    const int T = (J + TILE - 1) / TILE, PT = next_pow2(T);
+    double *X = malloc((size_t)J * S * sizeof(double)), *Xn = malloc((size_t)J * S * sizeof(double));
+    double *TH = malloc((size_t)J * n_theta * sizeof(double)), *THn = malloc((size_t)J * n_theta * sizeof(double));
+    double *logw = malloc((size_t)J * sizeof(double));
+    double *work = malloc((size_t)(3 * T + 4 * PT + 2 * TILE + J) * sizeof(double));
+    uint64_t *cum = malloc((size_t)J * sizeof(uint64_t));
+    uint32_t *select = malloc((size_t)J * sizeof(uint32_t));
+    double *buf = malloc((size_t)TILE * sizeof(double));
+    double *tc = malloc((size_t)PT * 5 * sizeof(double));
+    if (!X || !Xn || !TH || !THn || !logw || !work || !cum || !select || !buf || !tc) return -2;
+    for (int j = 0; j < J; j++) {
+        for (int k = 0; k < S; k++) X[(size_t)j * S + k] = x0[(size_t)k * J + j];
+        for (int o = 0; o < n_theta; o++) TH[(size_t)j * n_theta + o] = theta0[(size_t)o * J + j];
+    }
+    if (theta_out) memcpy(theta_out, theta0, (size_t)n_theta * J * sizeof(double));
+    /* per-particle parameter rows: the caller's pv, the per-particle slots overwritten */
+    const int pvn = M->pv_size;
+    double *PVJ = malloc((size_t)J * pvn * sizeof(double));
+    if (!PVJ) return -2;
+    const double all_fail_llike = plom_log(like_min) * (double)n_ts;
+    double llike = 0.0;
+    int naf = 0;
+    int prev_data = 0;
+    for (int n = 0; n < N; n++) {
+        int nonan = 0;
+        for (int k = 0; k < n_ts; k++) nonan += !isnan(data[(size_t)n * n_ts + k]);
+        /* the random walk after an interval with data, then the natural parameters */
+        const double fac = (n > 0 && prev_data) ? mut_fac[n - 1] : 0.0;
+        for (int j = 0; j < J; j++) {
+            double *th = TH + (size_t)j * n_theta;
+            for (int o = 0; o < n_theta; o += 2) {
+                const double sd0 = fac * sd[o];
+                const double sd1 = (o + 1 < n_theta) ? fac * sd[o + 1] : 0.0;
+                double z0 = 0.0, z1 = 0.0;
+                if (sd0 != 0.0 || sd1 != 0.0)
+                    plom_mif_normals(seed, stream, (uint32_t)n, (uint32_t)j, (uint32_t)(o >> 1), plom_bm_tab, &z0, &z1);
+                if (sd0 != 0.0) th[o] = th[o] + sd0 * z0;
+                if (o + 1 < n_theta && sd1 != 0.0) th[o + 1] = th[o + 1] + sd1 * z1;
+            }
+            double *row = PVJ + (size_t)j * pvn;
+            memcpy(row, pv, (size_t)pvn * sizeof(double));
+            for (int s2 = 0; s2 < P; s2++) {
+                const int o = pv_src[s2];
+                if (o >= 0) row[s2] = plom_mif_natural(tr[o], sc[o], dur[o], th[o], mn[o], mx[o], mult[o]);
+            }
+        }
+        if (theta_out)
+            for (int o = 0; o < n_theta; o++)
+                for (int j = 0; j < J; j++)
+                    theta_out[((size_t)(n + 1) * n_theta + o) * J + j] = TH[(size_t)j * n_theta + o];
+        prop_job job;
+        memset(&job, 0, sizeof job);
+        job.M = M; job.X = X; job.logw = logw; job.pv = PVJ; job.pv_stride = (size_t)pvn; job.times = times;
+        job.data = data; job.J = J; job.n = n; job.nonan = nonan; job.filter = 1; job.n_ts = n_ts;
+        job.seed = seed; job.stream = stream; job.noises_off = noises_off;
+        job.dt = dt; job.sqdt = plom_sqrt(dt); job.like_min = like_min; job.mode = ORACLE_CANONICAL;
+        propagate_all(&job, nthreads);
+        double ln = 0.0, es = 0.0;
+        int resampled = 0;
+        if (nonan) {
+            resampled = weight_select_canonical(logw, J, n, seed, stream, all_fail_llike, &ln, &es, select, work, cum);
+            if (!resampled) naf++;
+            llike += ln;
+            if (logw_out) memcpy(logw_out + (size_t)n * J, logw, (size_t)J * sizeof(double));
+            /* moments of theta: weights exp(l - m) (1 each when every particle failed) */
+            double m = -INFINITY;
+            for (int j = 0; j < J; j++) if (logw[j] > m) m = logw[j];
+            double *tw = tc, *ts1 = tc + PT, *tcn = tc + 2 * PT, *tmn = tc + 3 * PT, *tq = tc + 4 * PT;
+            for (int t = 0; t < PT; t++) tw[t] = 0.0;
+            for (int t = 0; t < T; t++) {
+                for (int i = 0; i < TILE; i++) {
+                    const int j = t * TILE + i;
+                    double w = 0.0;
+                    if (j < J) w = resampled ? (logw[j] == -INFINITY ? 0.0 : plom_exp(logw[j] - m)) : 1.0;
+                    buf[i] = w;
+                }
+                tw[t] = tree_sum(buf, TILE);
+            }
+            const double W = tree_sum(tw, PT);
+            for (int o = 0; o < n_theta; o++) {
+                for (int t = 0; t < PT; t++) { ts1[t] = 0.0; tcn[t] = 0.0; tmn[t] = 0.0; tq[t] = 0.0; }
+                for (int t = 0; t < T; t++) {
+                    const int rem = J - t * TILE;
+                    const double ct = (double)(rem < TILE ? rem : TILE);
+                    for (int i = 0; i < TILE; i++) {
+                        const int j = t * TILE + i;
+                        double w = 0.0;
+                        if (j < J) w = resampled ? (logw[j] == -INFINITY ? 0.0 : plom_exp(logw[j] - m)) : 1.0;
+                        buf[i] = j < J ? w * TH[(size_t)j * n_theta + o] : 0.0;
+                    }
+                    ts1[t] = tree_sum(buf, TILE);
+                    for (int i = 0; i < TILE; i++) {
+                        const int j = t * TILE + i;
+                        buf[i] = j < J ? TH[(size_t)j * n_theta + o] : 0.0;
+                    }
+                    const double mt = tree_sum(buf, TILE) / ct;
+                    for (int i = 0; i < TILE; i++) {
+                        const int j = t * TILE + i;
+                        const double d = j < J ? TH[(size_t)j * n_theta + o] - mt : 0.0;
+                        buf[i] = d * d;
+                    }
+                    tcn[t] = ct;
+                    tmn[t] = mt;
+                    tq[t] = tree_sum(buf, TILE);
+                }
+                double cc, cm, cq;
+                chan_tree(tcn, tmn, tq, PT, &cc, &cm, &cq);
+                mif_mean[(size_t)(n + 1) * n_theta + o] = tree_sum(ts1, PT) / W;
+                mif_var[(size_t)(n + 1) * n_theta + o] = cq / (double)(J - 1);
+            }
+        } else {
+            for (int j = 0; j < J; j++) select[j] = (uint32_t)j;
+            for (int o = 0; o < n_theta; o++) {
+                mif_mean[(size_t)(n + 1) * n_theta + o] = NAN;
+                mif_var[(size_t)(n + 1) * n_theta + o] = NAN;
+            }
+        }
+        if (loglik_n) loglik_n[n] = ln;
+        if (ess_n) ess_n[n] = es;
+        prev_data = nonan > 0;
+        if (ancestry)
+            for (int j = 0; j < J; j++) ancestry[(size_t)n * J + j] = (int32_t)select[j];
+        /* resample states and parameters together */
+        for (int j = 0; j < J; j++) {
+            memcpy(Xn + (size_t)j * S, X + (size_t)select[j] * S, (size_t)S * sizeof(double));
+            memcpy(THn + (size_t)j * n_theta, TH + (size_t)select[j] * n_theta, (size_t)n_theta * sizeof(double));
+        }
+        double *tmp = X; X = Xn; Xn = tmp;
+        tmp = TH; TH = THn; THn = tmp;
+    }
+    for (int o = 0; o < n_theta; o++) { mif_mean[o] = NAN; mif_var[o] = NAN; }
+    *loglik = llike;
+    *n_all_fail = naf;
+    free(X); free(Xn); free(TH); free(THn); free(logw); free(work); free(cum); free(select); free(buf); free(tc);
+    free(PVJ);
     return 0;
 }
 

@@ -1,0 +1,134 @@
+"""Iterated filtering (PLOM_FLAG_MIF: per-particle parameters, their random walk and the
+moments of theta per observation) on the GPU against the CPU oracle (oracle.mif_pass,
+pinned in test_mif_oracle.py): bit-identical FP64 log-likelihoods, ESS and theta moments,
+alone and batched (graph replays, several lanes)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from iip_amd import configs, smc
+from iip_amd.smc import Model
+
+pytestmark = pytest.mark.gpu
+
+
+def mif_setup(name, N, J, seed):
+    """theta = log of every per-particle pv slot (logit for the reporting rates), except
+    slot 1 which keeps pv; two extra components that no slot reads; jump sizes 0.03."""
+    m = Model.get(name)
+    times, data = m.dataset(N)
+    pv = m.pack(configs.TRUTH[name])
+    npp = m.n_pp
+    lay = m.layout
+    rep_slots = set(range(lay["obs"]["rep"], lay["obs"]["rep"] + m.n_ts)) if "rep" in lay["obs"] else set()
+    tr, th0, src = [], [], []
+    for s in range(npp):
+        if s == 1:
+            src.append(-1)
+            continue
+        src.append(len(tr))
+        if s in rep_slots:
+            tr.append(2)
+            th0.append(math.log(pv[s] / (1 - pv[s])))
+        else:
+            tr.append(1)
+            th0.append(math.log(pv[s]))
+    tr += [3, 0]
+    th0 += [0.3, -1.2]
+    nth = len(tr)
+    mp = {"pv_src": src, "tr": tr, "scale": [0] * nth, "duration": [0] * nth, "min": [0.0] * (nth - 2) + [0.1, 0.0],
+          "max": [0.0] * (nth - 2) + [0.9, 0.0], "multiplier": [1.0] * nth, "sd": [0.03] * (nth - 2) + [0.0, 0.05]}
+    rng = np.random.default_rng(seed)
+    theta0 = np.array(th0)[:, None] + rng.normal(0.0, 0.02, (nth, J))
+    x0 = np.repeat(m.x0(configs.X0_PROPS[name])[:, None], J, axis=1) * (1.0 + rng.normal(0.0, 0.01, (1, J)))
+    mut = np.linspace(0.4, 1.0, N)
+    return m, times, data, pv, mp, theta0, x0, mut
+
+
+def check_equal(g, o):
+    assert g.loglik == o["loglik"]
+    assert g.n_all_fail == o["n_all_fail"]
+    np.testing.assert_array_equal(g.loglik_n, o["loglik_n"])
+    np.testing.assert_array_equal(g.ess_n, o["ess_n"])
+    np.testing.assert_array_equal(g.mif_mean, o["mif_mean"])
+    np.testing.assert_array_equal(g.mif_var, o["mif_var"])
+
+
+@pytest.mark.parametrize("name,J,N", [("sir", 1000, 40), ("sir", 257, 12), ("sirs4", 1500, 20), ("seir10", 600, 10),
+                                      ("sirs4", 1, 6)])
+def test_mif_pass_equals_oracle(ctx, name, J, N):
+    m, times, data, pv, mp, theta0, x0, mut = mif_setup(name, N, J, 11)
+    if name == "sir" and J == 257:
+        data = data.copy()
+        data[3] = np.nan
+        data[7] = np.nan
+    print("mif", name, J, N, flush=True)
+    f = smc.Filter(ctx, m, J, times, data, pv, x0[:, 0], flags=smc.FLAG_MIF, mif=mp, seed=1, stream=0)
+    f.mif_start(theta0, x0, mut, seed=6, stream=2)
+    f.run()
+    g = f.read(mif=True)
+    o = oracle.mif_pass(m.id, J, times, data, pv, mp, theta0, x0, mut, seed=6, stream=2, nthreads=8)
+    check_equal(g, o)
+    # a second pass needs a new start
+    with pytest.raises(smc.PlomError):
+        f.run()
+    f.mif_start(theta0, x0, mut * 0.5, seed=6, stream=3)
+    f.run()
+    g2 = f.read(mif=True)
+    o2 = oracle.mif_pass(m.id, J, times, data, pv, mp, theta0, x0, mut * 0.5, seed=6, stream=3, nthreads=8)
+    check_equal(g2, o2)
+    f.close()
+
+
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_mif_batched_equals_alone(ctx, lanes):
+    cases = [("sir", 900, 30, 1), ("sirs4", 700, 15, 2), ("sir", 300, 25, 3)]
+    fs, starts = [], []
+    for name, J, N, st in cases:
+        m, times, data, pv, mp, theta0, x0, mut = mif_setup(name, N, J, st)
+        f = smc.Filter(ctx, m, J, times, data, pv, x0[:, 0], flags=smc.FLAG_MIF, mif=mp)
+        fs.append(f)
+        starts.append((theta0, x0, mut, st, (m, times, data, pv, mp)))
+    b = smc.Batch(ctx, fs)
+    b.configure(lanes)
+    for rep in range(2):
+        for f, (theta0, x0, mut, st, _) in zip(fs, starts):
+            f.mif_start(theta0, x0, mut, seed=20 + rep, stream=st)
+        print("batch replay", rep, flush=True)
+        b.launch(use_graph=True)
+        for f, (theta0, x0, mut, st, (m, times, data, pv, mp)) in zip(fs, starts):
+            o = oracle.mif_pass(m.id, f.J, times, data, pv, mp, theta0, x0, mut, seed=20 + rep, stream=st, nthreads=8)
+            check_equal(f.read(mif=True), o)
+    with pytest.raises(smc.PlomError):
+        b.launch(use_graph=True)
+    b.close()
+    for f in fs:
+        f.close()
+
+
+def test_mif_fp32_close_to_fp64(ctx):
+    m, times, data, pv, mp, theta0, x0, mut = mif_setup("sirs4", 30, 4000, 5)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        f = smc.Filter(ctx, m, 4000, times, data, pv, x0[:, 0], precision=prec, flags=smc.FLAG_MIF, mif=mp)
+        f.mif_start(theta0, x0, mut, seed=3, stream=1)
+        f.run()
+        res[prec] = f.read(mif=True)
+        f.close()
+    # FP32 propagation draws other normals: another Monte Carlo realisation of the same
+    # pass, so the estimates agree within the spread of the parameter cloud
+    assert abs(res["fp32"].loglik - res["fp64"].loglik) < 0.02 * abs(res["fp64"].loglik)
+    ok = ~np.isnan(res["fp64"].mif_mean)
+    spread = np.sqrt(np.maximum(res["fp64"].mif_var[ok], res["fp32"].mif_var[ok]))
+    assert np.all(np.abs(res["fp32"].mif_mean[ok] - res["fp64"].mif_mean[ok]) <= 3.0 * spread + 1e-12)
+
+
+def test_mif_rejects_bad_maps(ctx):
+    m, times, data, pv, mp, theta0, x0, mut = mif_setup("sir", 5, 50, 1)
+    bad = dict(mp, pv_src=[0, 99, 1, 2])
+    with pytest.raises(smc.PlomError):
+        smc.Filter(ctx, m, 50, times, data, pv, x0[:, 0], flags=smc.FLAG_MIF, mif=bad)
+    with pytest.raises(smc.PlomError):
+        smc.Filter(ctx, m, 50, times, data, pv, x0[:, 0], flags=smc.FLAG_MIF | smc.FLAG_HAT, mif=mp)

@@ -50,7 +50,7 @@ def build_cli():
         return None
     shared = [os.path.join(cli, f) for f in ("plom_cli.c", "pjson.c")]
     outs = []
-    for name in ("smc", "pmcmc"):
+    for name in ("smc", "pmcmc", "mif"):
         out = os.path.join(cli, name)
         cmd = ["gcc", "-O2", "-std=gnu99", "-Wall", "-ffp-contract=off", "-I" + INC, "-I" + CSRC, "-I" + GEN,
                os.path.join(cli, name + "_main.c")] + shared + \

@@ -199,3 +199,76 @@ def test_pmcmc_usage_errors(tmp_path):
     assert r.returncode != 0 and "one chain" in r.stderr
     r = run_pmcmc(d, ["sde", "-J", "10", "--chains", "0"], th)
     assert r.returncode != 0
+
+
+# ---------------------------------------------------------------------- mif (host side)
+MIF = os.path.join(PKG, "cli", "mif")
+
+
+def run_mif(d, args, theta_path):
+    with open(theta_path) as f:
+        return subprocess.run([MIF] + args, cwd=str(d), stdin=f, capture_output=True, text=True, timeout=600)
+
+
+def test_mif_dry_run(tmp_path):
+    """build_best + rescale_covariance_mif (methods.c:26-70): process / observation
+    variances scaled by 1/nb_obs; their jump sizes are the random walk's; the initial
+    conditions are resampled only; the fixed lag L = floor(0.75 N_DATA) (build.c:25)."""
+    d = model_dir(tmp_path, "sir")
+    r = run_mif(d, ["sde", "-J", "10", "--dry_run"], os.path.join(PKG, "examples", "sir", "theta.json"))
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    N = out["N"]
+    assert out["L"] == int(math.floor(0.75 * N)) and out["lag_row"] == out["L"]
+    assert out["sv"] == [1, 1, 0, 0, 0, 0] and out["estimated"] == [1] * 6
+    assert out["tr"] == [2, 2, 1, 1, 2, 1]          # the parameters' own transforms (no --prior)
+    assert out["pv_src"] == [2, 3, 4, 5]
+    n = 6
+    var = np.array(out["var"]).reshape(n, n)
+    assert var[0, 0] == 0.01 ** 2 and var[1, 1] == 0.01 ** 2
+    for o, sdt in ((2, 0.02), (3, 0.02), (4, 0.02), (5, 0.02)):
+        assert var[o, o] == sdt * sdt * (1.0 / N)
+        assert out["sd"][o] == math.sqrt(var[o, o])
+    assert out["sd"][0] == 0.0 and out["sd"][1] == 0.0
+    assert out["delta"][:3] == [7.0, 7.0, 7.0]
+    # -f: initial conditions only, nb_obs = L + 1
+    r = run_mif(d, ["sde", "-J", "10", "-f", "--dry_run"], os.path.join(PKG, "examples", "sir", "theta.json"))
+    out = json.loads(r.stdout)
+    assert out["N"] == int(math.floor(0.75 * 520)) + 1 and out["estimated"] == [1, 1, 0, 0, 0, 0]
+
+
+def test_mif_converges_towards_the_truth(tmp_path):
+    """With data simulated at the truth (beta = 0.5), iterated filtering started at
+    beta = 0.35 moves its estimate most of the way to the truth (the driver restatement
+    with the CPU filter oracle)."""
+    from oracle import oracle, mif_oracle
+    d = model_dir(tmp_path, "sir")
+    theta = json.load(open(os.path.join(PKG, "examples", "sir", "theta.json")))
+    for k in ("S", "I", "gamma", "rep", "phi"):
+        theta["parameter"][k]["group"]["all"]["sd_transf"]["value"] = 0.0
+    theta["parameter"]["beta"]["group"]["all"]["guess"]["value"] = 0.35
+    theta["parameter"]["beta"]["group"]["all"]["sd_transf"]["value"] = 0.3
+    p = tmp_path / "t.json"
+    p.write_text(json.dumps(theta) + "\n")
+    N = 52
+    dry = json.loads(run_mif(d, ["sde", "-J", "10", "--dry_run"], str(p)).stdout)
+    dry = dict(dry, N=N, delta=dry["delta"][:N], data_rows=[r for r in dry["data_rows"] if r < N],
+               lag_row=max(r for r in dry["data_rows"] if r <= int(0.75 * N)))
+    m = smc.Model.get("sir")
+    times, data = m.dataset(N)
+    mp = {k: dry[k] for k in ("pv_src", "tr", "scale", "duration", "min", "max", "multiplier", "sd")}
+    sdv = np.array(dry["var"]).reshape(6, 6)
+    mp["sd"] = [math.sqrt(sdv[o, o] * 520 / N) if o == 2 else 0.0 for o in range(6)]
+    dry["var"] = list((sdv * np.where(np.arange(6) >= 2, 520 / N, 1.0)[:, None] * np.where(np.arange(6) >= 2, 1.0, 1.0)).ravel())
+
+    def pass_fn(th, x0, mut, seed, stream):
+        return oracle.mif_pass(m.id, th.shape[1], times, data, np.array(dry["pv"]), mp, th, x0, mut, seed=seed,
+                               stream=stream, nthreads=8)
+
+    def x0_of(th):
+        return [mif_oracle.f_inv(2, th[0], 0, 0) * 1e6, mif_oracle.f_inv(2, th[1], 0, 0) * 1e6, 0.0]
+
+    out = mif_oracle.run(dry, pass_fn, x0_of, lambda th: 0, M=15, J=1000, seed=2, a=0.9, switch=100)
+    beta = [row[1][2] for row in out["trace"]]
+    assert beta[0] == pytest.approx(0.35)
+    assert abs(beta[-1] - 0.5) < 0.35 * abs(0.35 - 0.5), beta

@@ -1,0 +1,71 @@
+"""The `mif` command (iterated filtering, C/mif/mif.c:21-247) on the GPU against the CPU
+restatement of its driver (oracle/mif_oracle.py) driven by the CPU filter oracle with the
+same keys: identical trace rows (17 digits) in both update regimes (m <= SWITCH and
+after), for two replicates run as one batch, and the mif file of -r."""
+import json
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle, mif_oracle
+from iip_amd import smc
+from test_cli import PKG, model_dir
+
+pytestmark = pytest.mark.gpu
+
+MIF = os.path.join(PKG, "cli", "mif")
+THETA = os.path.join(PKG, "examples", "sir", "theta.json")
+
+
+def run_mif(d, args):
+    with open(THETA) as f:
+        return subprocess.run([MIF] + args, cwd=str(d), stdin=f, capture_output=True, text=True, timeout=600)
+
+
+def read_csv(path):
+    lines = open(path).read().splitlines()
+    return lines[0].split(","), [[float(v) for v in line.split(",")] for line in lines[1:]]
+
+
+def test_mif_command_equals_restatement(tmp_path):
+    d = model_dir(tmp_path, "sir")
+    J, M, K = 400, 4, 2
+    args = ["sde", "-J", str(J), "-M", str(M), "-S", "2", "-a", "0.9", "-r", "--trace_17g", "--chains", str(K), "-i", "3"]
+    dry = json.loads(run_mif(d, args + ["--dry_run"]).stdout)
+    print("mif command", flush=True)
+    r = run_mif(d, args)
+    assert r.returncode == 0, r.stderr
+    m = smc.Model.get("sir")
+    N = dry["N"]
+    times, data = m.dataset(N)
+    mp = {k: dry[k] for k in ("pv_src", "tr", "scale", "duration", "min", "max", "multiplier", "sd")}
+    pv = np.array(dry["pv"])
+
+    def pass_fn(th, x0, mut, seed, stream):
+        return oracle.mif_pass(m.id, th.shape[1], times, data, pv, mp, th, x0, mut, seed=seed, stream=stream, nthreads=8)
+
+    def x0_of(th):
+        return [mif_oracle.f_inv(2, th[0], 0, 0) * 1e6, mif_oracle.f_inv(2, th[1], 0, 0) * 1e6, 0.0]
+
+    def constrain(th):
+        return int(mif_oracle.f_inv(2, th[0], 0, 0) + mif_oracle.f_inv(2, th[1], 0, 0) > 1.0)
+
+    for k in range(K):
+        o = mif_oracle.run(dry, pass_fn, x0_of, constrain, M=M, J=J, seed=2 + 3 + k, a=0.9, switch=2, print_traj=True)
+        head, rows = read_csv(tmp_path / "sir" / ("trace_%d.csv" % (3 + k)))
+        assert head == ["index", "S:all", "I:all", "beta:all", "gamma:all", "rep:all", "phi:all", "log_like"]
+        assert len(rows) == M + 1
+        assert math.isnan(rows[0][-1])
+        for got, (idx, vals, ll) in zip(rows, o["trace"]):
+            assert got[0] == idx
+            assert got[1:-1] == vals
+            if idx > 0:
+                assert got[-1] == ll
+        head, mrows = read_csv(tmp_path / "sir" / ("mif_%d.csv" % (3 + k)))
+        assert head[:4] == ["index", "time", "mean:S:all", "var:S:all"] and head[-1] == "ess"
+        assert len(mrows) == M * len(dry["data_rows"])
+        done = json.load(open(tmp_path / "sir" / (".done_%d.json" % (3 + k))))
+        assert done["parameter"]["beta"]["group"]["all"]["guess"]["value"] == pytest.approx(o["trace"][-1][1][2], rel=1e-15)

@@ -1,0 +1,202 @@
+/*
+ * plom_psr.h -- the discrete samplers of the Poisson-system-with-stochastic-rates step
+ * (implementation "psr", step_psr, C/templates/core_template.c:104-209), shared by the GPU
+ * kernels and the CPU oracle (compiled with -ffp-contract=off on both sides, so a particle
+ * draws the same numbers on both).
+ *
+ * Every draw consumes 32-bit words of the particle's xoshiro128** stream (plom_rng.h) in
+ * a fixed order; the arithmetic uses this library's exp / log / sqrt (plom_math.h).
+ *
+ *   plom_unif53        (a >> 6) 2^27 + (b >> 5) + 1/2, times 2^-53: a uniform in (0, 1)
+ *                      from two words.
+ *   plom_lfact         log k!: exact table below 30, Stirling's series above (error
+ *                      below 1e-14).
+ *   plom_gamma_draw    Gamma(a, 1): Marsaglia and Tsang (2000, ACM TOMS 26:363-372) for
+ *                      a >= 1; Gamma(a + 1) U^(1/a) below.
+ *   plom_poisson_draw  Poisson(mu): the multiplication method (Knuth, TAOCP vol. 2,
+ *                      3.4.1) for mu < 10; Hoermann's transformed rejection with squeeze
+ *                      PTRS (1993, Insurance: Math. Econ. 12:39-45) above.
+ *   plom_binom_draw    Binomial(n, p), p <= 1/2 by symmetry: inversion by sequential
+ *                      search (Kachitvichyanukul and Schmeiser 1988, BINV) for n p < 10;
+ *                      Hoermann's BTRS (1993, J. Stat. Comput. Simul. 46:101-110) above.
+ *   plom_multinomial   the sequential binomials of plom_ran_multinomial
+ *                      (C/core/prediction_util.c:323-343).
+ *
+ * The rejection loops are bounded (PLOM_PSR_MAX_TRY attempts; the probability of reaching
+ * the bound is below 1e-60 for every parameter) and then return the distribution's mode,
+ * so that every GPU lane leaves the loop.
+ */
+#ifndef PLOM_PSR_H
+#define PLOM_PSR_H
+
+#include "plom_math.h"
+#include "plom_rng.h"
+
+#define PLOM_PSR_MAX_TRY 64
+
+PLOM_HD double plom_floor(double x) { return __builtin_floor(x); }
+#define PLOM_LN_SQRT_2PI 0.91893853320467274178
+
+PLOM_HD double plom_unif53(plom_nstream64 *ns)
+{
+    const uint32_t a = plom_xoshiro_next(&ns->r) >> 6;
+    const uint32_t b = plom_xoshiro_next(&ns->r) >> 5;
+    return ((double)a * 134217728.0 + (double)b + 0.5) * 0x1p-53;
+}
+
+/* log k! for k = 0..29 (50-digit values rounded to nearest) */
+#define PLOM_LFACT_TAB { \
+    0.0, 0.0, 0.6931471805599453, 1.791759469228055, 3.1780538303479458, \
+    4.787491742782046, 6.579251212010101, 8.525161361065415, 10.60460290274525, 12.801827480081469, \
+    15.104412573075516, 17.502307845873887, 19.987214495661885, 22.552163853123425, 25.19122118273868, \
+    27.89927138384089, 30.671860106080672, 33.50507345013689, 36.39544520803305, 39.339884187199495, \
+    42.335616460753485, 45.38013889847691, 48.47118135183523, 51.60667556776438, 54.78472939811232, \
+    58.00360522298052, 61.261701761002, 64.55753862700634, 67.88974313718154, 71.25703896716801 \
+}
+
+#if defined(__HIPCC__)
+static __constant__ const double plom_lfact_tab[30] = PLOM_LFACT_TAB;
+#else
+static const double plom_lfact_tab[30] = PLOM_LFACT_TAB;
+#endif
+
+PLOM_HD double plom_lfact(double k)
+{
+    if (k < 30.0) return plom_lfact_tab[(int)k];
+    const double x = k + 1.0;
+    const double r = 1.0 / x, r2 = r * r;
+    /* ln Gamma(x) = (x - 1/2) ln x - x + ln sqrt(2 pi) + 1/(12x) - 1/(360x^3) + 1/(1260x^5) */
+    return (x - 0.5) * plom_log(x) - x + PLOM_LN_SQRT_2PI + r * (1.0 / 12.0 - r2 * (1.0 / 360.0 - r2 * (1.0 / 1260.0)));
+}
+
+/* Gamma(a, 1), a > 0 */
+PLOM_HD double plom_gamma_draw(plom_nstream64 *ns, double a)
+{
+    double boost = 1.0;
+    if (a < 1.0) {
+        /* Gamma(a) = Gamma(a + 1) U^(1/a) */
+        boost = plom_exp(plom_log(plom_unif53(ns)) / a);
+        a = a + 1.0;
+    }
+    const double d = a - 1.0 / 3.0;
+    const double c = 1.0 / plom_sqrt(9.0 * d);
+    for (int t = 0; t < PLOM_PSR_MAX_TRY; t++) {
+        double z, z1;
+        plom_normal_pair64(ns, &z, &z1);
+        const double v1 = 1.0 + c * z;
+        if (v1 <= 0.0) continue;
+        const double v = v1 * v1 * v1;
+        const double u = plom_unif53(ns);
+        const double z2 = z * z;
+        if (u < 1.0 - 0.0331 * (z2 * z2)) return (d * v) * boost;
+        if (plom_log(u) < 0.5 * z2 + d * (1.0 - v + plom_log(v))) return (d * v) * boost;
+    }
+    return d * boost;
+}
+
+/* Poisson(mu), mu >= 0, as a double holding an integer */
+PLOM_HD double plom_poisson_draw(plom_nstream64 *ns, double mu)
+{
+    if (!(mu > 0.0)) return 0.0;
+    if (mu < 10.0) {
+        const double L = plom_exp(-mu);
+        double p = plom_unif53(ns);
+        double k = 0.0;
+        /* P(k > 200) < 1e-150 for mu < 10 */
+        while (p > L && k < 200.0) {
+            k = k + 1.0;
+            p = p * plom_unif53(ns);
+        }
+        return k;
+    }
+    const double slam = plom_sqrt(mu);
+    const double loglam = plom_log(mu);
+    const double b = 0.931 + 2.53 * slam;
+    const double a = -0.059 + 0.02483 * b;
+    const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+    const double vr = 0.9277 - 3.6224 / (b - 2.0);
+    for (int t = 0; t < PLOM_PSR_MAX_TRY; t++) {
+        const double U = plom_unif53(ns) - 0.5;
+        const double V = plom_unif53(ns);
+        const double us = 0.5 - plom_fabs(U);
+        const double k = plom_floor((2.0 * a / us + b) * U + mu + 0.43);
+        if (us >= 0.07 && V <= vr) return k;
+        if (k < 0.0 || (us < 0.013 && V > us)) continue;
+        if (plom_log(V) + plom_log(invalpha) - plom_log(a / (us * us) + b) <= -mu + k * loglam - plom_lfact(k)) return k;
+    }
+    return plom_floor(mu);
+}
+
+/* Binomial(n, p) for p <= 1/2 */
+PLOM_HD double plom_binom_half(plom_nstream64 *ns, double n, double p)
+{
+    const double q = 1.0 - p;
+    const double np = n * p;
+    if (np < 10.0) {
+        /* inversion: r = P(X = x) by the recursion r(x+1) = r(x) (a/(x+1) - s) */
+        const double s = p / q;
+        const double a = (n + 1.0) * s;
+        /* q^n = exp(n log(1 - p)); log(1 - p) by its series when p is small */
+        double l1p;
+        if (p < 1e-4) {
+            const double p2 = p * p;
+            l1p = -(p + p2 * (0.5 + p * (1.0 / 3.0 + p * 0.25)));
+        } else {
+            l1p = plom_log(q);
+        }
+        double r = plom_exp(n * l1p);
+        double u = plom_unif53(ns);
+        double x = 0.0;
+        /* P(X > 110) < 1e-60 for n p < 10 */
+        while (u > r && x < n && x < 110.0) {
+            u = u - r;
+            x = x + 1.0;
+            r = r * (a / x - s);
+        }
+        return x;
+    }
+    const double spq = plom_sqrt(np * q);
+    const double b = 1.15 + 2.53 * spq;
+    const double a = -0.0873 + 0.0248 * b + 0.01 * p;
+    const double c = np + 0.5;
+    const double vr = 0.92 - 4.2 / b;
+    const double alpha = (2.83 + 5.1 / b) * spq;
+    const double lpq = plom_log(p / q);
+    const double m = plom_floor((n + 1.0) * p);
+    const double h = plom_lfact(m) + plom_lfact(n - m);
+    for (int t = 0; t < PLOM_PSR_MAX_TRY; t++) {
+        const double u = plom_unif53(ns) - 0.5;
+        double v = plom_unif53(ns);
+        const double us = 0.5 - plom_fabs(u);
+        const double k = plom_floor((2.0 * a / us + b) * u + c);
+        if (k < 0.0 || k > n) continue;
+        if (us >= 0.07 && v <= vr) return k;
+        v = plom_log(v * alpha / (a / (us * us) + b));
+        if (v <= h - plom_lfact(k) - plom_lfact(n - k) + (k - m) * lpq) return k;
+    }
+    return m;
+}
+
+/* Binomial(n, p), n a non-negative integer held in a double */
+PLOM_HD double plom_binom_draw(plom_nstream64 *ns, double n, double p)
+{
+    if (!(n > 0.0) || !(p > 0.0)) return 0.0;
+    if (p >= 1.0) return n;
+    if (p > 0.5) return n - plom_binom_half(ns, n, 1.0 - p);
+    return plom_binom_half(ns, n, p);
+}
+
+/* n[0..K) ~ Multinomial(N, p[0..K)), p summing to 1: the sequential binomials of
+ * plom_ran_multinomial (prediction_util.c:323-343) */
+PLOM_HD void plom_multinomial(plom_nstream64 *ns, int K, double N, const double *p, double *n)
+{
+    double sum_p = 0.0, sum_n = 0.0;
+    for (int k = 0; k < K; k++) {
+        if (p[k] > 0.0) n[k] = plom_binom_draw(ns, N - sum_n, p[k] / (1.0 - sum_p));
+        else n[k] = 0.0;
+        sum_p += p[k];
+        sum_n += n[k];
+    }
+}
+
+#endif

@@ -37,6 +37,8 @@ def lib():
         L.oracle_bm_pairs.argtypes = [C.c_int, C.POINTER(C.c_uint32), dp]
         L.oracle_xoshiro.argtypes = [C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_uint32)]
         L.oracle_model_name.restype = C.c_char_p
+        L.oracle_psr_draws.argtypes = [C.c_int, C.c_double, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                       C.c_int, dp, dp]
         L.oracle_mif_natural.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
         L.oracle_mif_natural.restype = C.c_double
         i32p = C.POINTER(C.c_int32)
@@ -122,6 +124,18 @@ def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=
         raise RuntimeError("oracle_mif_pass failed: %d" % rc)
     return {"loglik": ll.value, "n_all_fail": naf.value, "loglik_n": ll_n, "ess_n": ess, "mif_mean": mm,
             "mif_var": mv, "theta": tho, "logw": lw, "ancestry": anc}
+
+
+PSR_UNIFORM, PSR_GAMMA, PSR_POISSON, PSR_BINOMIAL, PSR_LFACT, PSR_MULTINOMIAL = range(6)
+
+
+def psr_draws(kind, a, count, b=0.0, seed=1, stream=0, n=0, j=0, p=None):
+    """count draws of one PSR sampler (csrc/plom_psr.h) from the stream (seed, stream, n, j);
+    for PSR_MULTINOMIAL, the counts of N = a over the probabilities p."""
+    pp = np.ascontiguousarray(p, dtype=np.float64) if p is not None else None
+    out = np.zeros(len(pp) if pp is not None else count)
+    lib().oracle_psr_draws(kind, float(a), float(b), seed, stream, n, j, len(out), _dp(pp), _dp(out))
+    return out
 
 
 def mif_natural(tr, sc, dur, x, a, b, mult):

@@ -44,6 +44,7 @@
 #include "plom_math.h"
 #include "plom_rng.h"
 #include "plom_mif.h"
+#include "plom_psr.h"
 #include "plom_models.h"
 
 #define ORACLE_CANONICAL 0
@@ -87,6 +88,29 @@ int oracle_model_size_proj(int m) { return MODELS[m].S; }
 int oracle_model_n_ts(int m) { return MODELS[m].n_ts; }
 int oracle_model_n_hat(int m) { return MODELS[m].n_hat; }
 int oracle_model_n_pp(int m) { return MODELS[m].n_pp; }
+/* count draws of one sampler from the particle stream (seed, stream, n, j), in order:
+ * kind 0 uniform, 1 Gamma(a, 1), 2 Poisson(a), 3 Binomial(a, b), 4 log a!; for kind 5 the
+ * multinomial of N = a over the K = count probabilities in p, written to out[0..K) */
+void oracle_psr_draws(int kind, double a, double b, uint32_t seed, uint32_t stream, uint32_t n, uint32_t j, int count,
+                      const double *p, double *out)
+{
+    plom_nstream64 ns;
+    plom_nstream64_init(&ns, seed, stream, n, j, plom_bm_tab);
+    if (kind == 5) {
+        plom_multinomial(&ns, count, a, p, out);
+        return;
+    }
+    for (int i = 0; i < count; i++) {
+        switch (kind) {
+        case 0: out[i] = plom_unif53(&ns); break;
+        case 1: out[i] = plom_gamma_draw(&ns, a); break;
+        case 2: out[i] = plom_poisson_draw(&ns, a); break;
+        case 3: out[i] = plom_binom_draw(&ns, a, b); break;
+        default: out[i] = plom_lfact(a + (double)i); break;
+        }
+    }
+}
+
 double oracle_mif_natural(int tr, int sc, int dur, double x, double a, double b, double mult)
 {
     return plom_mif_natural(tr, sc, dur, x, a, b, mult);

@@ -69,6 +69,10 @@ extern "C" {
                                          sample_traj_and_print, C/core/print.c:715-784); keeps the
                                          ancestry on the device */
 
+#define PLOM_FLAG_PSR 64u             /* implementation "psr" (the reference's default): the Poisson
+                                         system with stochastic rates, an Euler-multinomial step with
+                                         gamma white noise (step_psr, C/templates/core_template.c:104-209);
+                                         FP64 only; of noises_off only PLOM_NO_ENV_STO_BIT applies */
 #define PLOM_FLAG_MIF 32u             /* per-particle parameters (iterated filtering): needs
                                          plom_filter_inputs.mif and plom_filter_mif_start before
                                          every run; not with NO_FILTER, HAT or LINEAGE */

@@ -271,6 +271,140 @@ class Emitter:
         L[bind_at:bind_at] = binds
         return "\n".join(L)
 
+    # ---------------------------------------------------------------- PSR interval
+    def interval_psr(self, env, pp=False):
+        """One observation interval of the Poisson system with stochastic rates (step_psr,
+        core_template.c:104-209; generator Ccoder.py:196-322) for one particle, FP64:
+        per population, the white noises xi = sd^2 Gamma(dt/sd^2)/dt (1 without them), then
+        for every state with exits the probabilities (1 - exp(-sum))(r_i dt/sum) and the
+        stay probability, the multinomial of its (truncated) count, the Poisson flows out of
+        the remainder; then every state = stays + inflows and the incidences += their
+        reactions' counts.  Rates are read from the states at the start of the step for
+        every population (the coupling extension mix() makes them depend on other
+        populations)."""
+        m = self.m
+        variant = "psr_full" if env else "psr_no_env_sto"
+        dev = "static __device__ inline __attribute__((always_inline))"
+        L = []
+        if pp:
+            L.append("%s void %s_interval_%s_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)"
+                     % (dev, self.p, variant))
+        else:
+            L.append("PLOM_HD void %s_interval_%s_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)"
+                     % (self.p, variant))
+        L.append("{")
+        L.append("    (void)sqdt; (void)ns; (void)pv;%s" % (" (void)lp;" if pp else ""))
+        self.used_pars = set()
+        bind_at = len(L)
+        for u in range(m.N_TS_INC_UNIQUE):
+            L.append("    X[%d] = 0.0; /* reset_inc */" % (m.inc_offset + u))
+        L.append("    double t = t0;")
+        L.append("    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));")
+        L.append("    for (int k = 0; k < K; k++) {")
+        L.append("        const double tr = t; (void)tr;")
+        tf_at = len(L)
+        self.time_funcs = {}
+        # rate caches of every population from the states at the start of the step
+        for c in range(m.N_CAC):
+            em = self.rate_emitter(c, "double")
+            for k, tree in enumerate(m.rate_trees):
+                L.append("        const double r%d_%d = %s;" % (k, c, em.emit(tree)))
+        univ = ("U", m.remainder)
+        for c in range(m.N_CAC):
+            # white noises of population c
+            for j, w in enumerate(m.white_noise):
+                if env:
+                    sd = self.sd_text(w["sd"], c, "double")
+                    L.append("        const double sd%d_%d = %s;" % (j, c, sd))
+                    L.append("        const double v%d_%d = sd%d_%d * sd%d_%d;" % (j, c, j, c, j, c))
+                    L.append("        const double xi%d_%d = (v%d_%d * plom_gamma_draw(ns, dt / v%d_%d)) / dt;" % (j, c, j, c, j, c))
+                else:
+                    L.append("        const double xi%d_%d = 1.0;" % (j, c))
+
+            def noisy(r):
+                k = r["ind_cache"]
+                if "white_noise" in r:
+                    jn = int(r["white_noise"]["name"].split("__")[1])
+                    return "(r%d_%d * xi%d_%d)" % (k, c, jn, c)
+                return "r%d_%d" % (k, c)
+            # multinomial of every state with exits
+            for s in m.par_sv:
+                exits = [(ri, r) for ri, r in enumerate(m.proc_model) if r["from"] == s]
+                if not exits:
+                    continue
+                idx = self.xidx(s, c)
+                K = len(exits) + 1
+                terms = ["%s * dt" % noisy(r) for _, r in exits]
+                L.append("        {")
+                L.append("            double p[%d], cnt[%d];" % (K, K))
+                L.append("            const double sum = %s;" % " + ".join(terms))
+                L.append("            if (sum > 0.0) {")
+                L.append("                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);")
+                for i, term in enumerate(terms):
+                    L.append("                p[%d] = one_minus_exp_sum * ((%s) / sum);" % (i, term))
+                L.append("                p[%d] = 1.0%s;" % (K - 1, "".join(" - p[%d]" % i for i in range(K - 1))))
+                L.append("            } else {")
+                for i in range(K - 1):
+                    L.append("                p[%d] = 0.0;" % i)
+                L.append("                p[%d] = 1.0;" % (K - 1))
+                L.append("            }")
+                L.append("            plom_multinomial(ns, %d, plom_floor(X[%d]), p, cnt);" % (K, idx))
+                for i, (ri, r) in enumerate(exits):
+                    L.append("            n%d_%d = cnt[%d];" % (ri, c, i))
+                L.append("            stay%d = cnt[%d];" % (idx, K - 1))
+                L.append("        }")
+            # Poisson flows out of the universe / the remainder
+            for src in univ:
+                if src is None:
+                    continue
+                for ri, r in enumerate(m.proc_model):
+                    if r["from"] == src:
+                        L.append("        n%d_%d = plom_poisson_draw(ns, %s * dt);" % (ri, c, noisy(r)))
+        # declarations of the counts (inserted before the draws)
+        decl = []
+        for c in range(m.N_CAC):
+            for ri, r in enumerate(m.proc_model):
+                decl.append("        double n%d_%d = 0.0;" % (ri, c))
+        for s in m.par_sv:
+            if any(r["from"] == s for r in m.proc_model):
+                for c in range(m.N_CAC):
+                    decl.append("        double stay%d = 0.0;" % self.xidx(s, c))
+        decl.append("        " + " ".join("(void)n%d_%d;" % (ri, c) for c in range(m.N_CAC) for ri in range(len(m.proc_model))))
+        # update of the states: stays (or the old count) plus inflows
+        for c in range(m.N_CAC):
+            for s in m.par_sv:
+                idx = self.xidx(s, c)
+                base = "stay%d" % idx if any(r["from"] == s for r in m.proc_model) else "X[%d]" % idx
+                ins = ["n%d_%d" % (ri, c) for ri, r in enumerate(m.proc_model) if r["to"] == s]
+                L.append("        X[%d] = %s;" % (idx, " + ".join([base] + ins)))
+        # incidences: the counts of their reactions
+        u = 0
+        for o, lst in enumerate(m.obs_var_def):
+            if not m.is_inc[o]:
+                continue
+            info = m.obs2ts[o]
+            for tsu in range(info["n_ts_unique"]):
+                cl = [ci * m.N_AC + ai for (ci, ai) in info["cac"][tsu]]
+                terms = []
+                for c in cl:
+                    for dfn in lst:
+                        for ri in dfn["reactions"]:
+                            terms.append("n%d_%d" % (ri, c))
+                L.append("        X[%d] = X[%d] + (%s);" % (m.inc_offset + u, m.inc_offset + u, " + ".join(terms) if terms else "0.0"))
+                u += 1
+        L.append("        t += dt;")
+        L.append("    }")
+        L.append("}")
+        tfs = ["        const double %s = %s;" % (name, text) for text, name in
+               sorted(self.time_funcs.items(), key=lambda kv: int(kv[1][2:]))]
+        L[tf_at:tf_at] = tfs + decl
+        self.time_funcs = None
+        npp = m.layout["npp"]
+        binds = [("    const double P%d = lp[%d];" % (k, k)) if pp and k < npp else
+                 ("    const double P%d = PLOM_UNIFORM64(pv[%d]);" % (k, k)) for k in sorted(self.used_pars)]
+        L[bind_at:bind_at] = binds
+        return "\n".join(L)
+
     # ---------------------------------------------------------------- observations
     def proj2obs(self, real, pp=False):
         m = self.m
@@ -466,9 +600,18 @@ class Emitter:
         out.append("")
         out.append(self.hat_values())
         out.append("")
-        # host dispatcher on the reference's noises_off bits (prediction_util.c:143-161)
+        for env in (True, False):
+            out.append(self.interval_psr(env))
+            out.append("")
+        # host dispatcher on the reference's noises_off bits (prediction_util.c:143-161),
+        # PLOM_IMPL_PSR selecting the psr implementation (prediction_util.c:162-170)
         out.append("PLOM_HD void %s_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off)" % self.p)
         out.append("{")
+        out.append("    if (noises_off & PLOM_IMPL_PSR) {")
+        out.append("        if (noises_off & PLOM_NO_ENV_STO) %s_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
+        out.append("        else %s_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
+        out.append("        return;")
+        out.append("    }")
         out.append("    const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);")
         out.append("    if (v == 0u) %s_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
         out.append("    else if (v == PLOM_NO_DEM_STO) %s_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
@@ -482,6 +625,9 @@ class Emitter:
             out.append("")
         out.append(self.proj2obs("float"))
         out.append("")
+        for env in (True, False):
+            out.append(self.interval_psr(env, pp=True))
+            out.append("")
         for real in ("double", "float"):
             for v, dem, env in VARIANTS:
                 out.append(self.interval(v, dem, env, real, pp=True))
@@ -502,6 +648,9 @@ class Emitter:
             for vi, (v, dem, env) in enumerate(VARIANTS):
                 kw = "if" if vi == 0 else "else if"
                 out.append("        %s constexpr (V == %d) %s_interval_%s_%s(X, pv, ns, t0, t1, dt, sqdt);" % (kw, vi, self.p, v, tag))
+            if tag == "f64":
+                out.append("        else if constexpr (V == 4) %s_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
+                out.append("        else if constexpr (V == 5) %s_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
             out.append("    }")
             out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(%s *X, const %s *pv, const %s *lp, %s *ns, double t0, double t1, double dt, %s sqdt)"
                        % (Rt, Rt, Rt, stream, Rt))
@@ -509,6 +658,9 @@ class Emitter:
             for vi, (v, dem, env) in enumerate(VARIANTS):
                 kw = "if" if vi == 0 else "else if"
                 out.append("        %s constexpr (V == %d) %s_interval_%s_pp_%s(X, pv, lp, ns, t0, t1, dt, sqdt);" % (kw, vi, self.p, v, tag))
+            if tag == "f64":
+                out.append("        else if constexpr (V == 4) %s_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);" % self.p)
+                out.append("        else if constexpr (V == 5) %s_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);" % self.p)
             out.append("    }")
             out.append("    static __device__ inline __attribute__((always_inline)) void proj2obs(const %s *X, const double *pvd, double *obs) { %s_proj2obs_%s(X, pvd, obs); }"
                        % (Rt, self.p, tag))

@@ -12,6 +12,7 @@
 #define PLOM_MODEL_COMMON_H
 
 #include "plom_rng.h"
+#include "plom_psr.h"
 
 /* PLOM_UNIFORM64/32: a value every lane of the wave holds (the filter's parameters: a
  * wave never spans two filters), moved to scalar registers on the device */
@@ -54,6 +55,8 @@ PLOM_HD int plom_substeps(double t0, double t1, double dt)
 
 #define PLOM_NO_DEM_STO 1u
 #define PLOM_NO_ENV_STO 2u
+/* the host dispatchers' selector of the psr implementation (Euler-multinomial step) */
+#define PLOM_IMPL_PSR 0x100u
 #define PLOM_NO_DRIFT 4u
 
 PLOM_HD double plom_heaviside(double x) { return (x < 0.0) ? 0.0 : 1.0; }

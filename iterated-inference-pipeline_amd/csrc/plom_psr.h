@@ -34,6 +34,14 @@
 
 #define PLOM_PSR_MAX_TRY 64
 
+/* the samplers are real calls on the device: a step holds dozens of draws, and inlining
+ * the rejection loops at every one of them multiplies the code size */
+#if defined(__HIPCC__)
+#define PLOM_HD_CALL static __host__ __device__ __attribute__((noinline))
+#else
+#define PLOM_HD_CALL static
+#endif
+
 PLOM_HD double plom_floor(double x) { return __builtin_floor(x); }
 #define PLOM_LN_SQRT_2PI 0.91893853320467274178
 
@@ -70,7 +78,7 @@ PLOM_HD double plom_lfact(double k)
 }
 
 /* Gamma(a, 1), a > 0 */
-PLOM_HD double plom_gamma_draw(plom_nstream64 *ns, double a)
+PLOM_HD_CALL double plom_gamma_draw(plom_nstream64 *ns, double a)
 {
     double boost = 1.0;
     if (a < 1.0) {
@@ -95,7 +103,7 @@ PLOM_HD double plom_gamma_draw(plom_nstream64 *ns, double a)
 }
 
 /* Poisson(mu), mu >= 0, as a double holding an integer */
-PLOM_HD double plom_poisson_draw(plom_nstream64 *ns, double mu)
+PLOM_HD_CALL double plom_poisson_draw(plom_nstream64 *ns, double mu)
 {
     if (!(mu > 0.0)) return 0.0;
     if (mu < 10.0) {
@@ -178,7 +186,7 @@ PLOM_HD double plom_binom_half(plom_nstream64 *ns, double n, double p)
 }
 
 /* Binomial(n, p), n a non-negative integer held in a double */
-PLOM_HD double plom_binom_draw(plom_nstream64 *ns, double n, double p)
+PLOM_HD_CALL double plom_binom_draw(plom_nstream64 *ns, double n, double p)
 {
     if (!(n > 0.0) || !(p > 0.0)) return 0.0;
     if (p >= 1.0) return n;
@@ -188,7 +196,7 @@ PLOM_HD double plom_binom_draw(plom_nstream64 *ns, double n, double p)
 
 /* n[0..K) ~ Multinomial(N, p[0..K)), p summing to 1: the sequential binomials of
  * plom_ran_multinomial (prediction_util.c:323-343) */
-PLOM_HD void plom_multinomial(plom_nstream64 *ns, int K, double N, const double *p, double *n)
+PLOM_HD_CALL void plom_multinomial(plom_nstream64 *ns, int K, double N, const double *p, double *n)
 {
     double sum_p = 0.0, sum_n = 0.0;
     for (int k = 0; k < K; k++) {

@@ -934,9 +934,14 @@ struct model_entry {
     void (*launch_hat0)(int prec, hipStream_t s, plom_dfilter *df);
 };
 
+/* variants 0-3: SDE noise switches; 4, 5: psr with / without white noise (FP64 only) */
 template <class M, typename R, bool MIF>
 static void launch_prop_r(int variant, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles)
 {
+    if constexpr (sizeof(R) == 8) {
+        if (variant == 4) { hipLaunchKernelGGL((k_propagate<M, R, 4, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); return; }
+        if (variant == 5) { hipLaunchKernelGGL((k_propagate<M, R, 5, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); return; }
+    }
     switch (variant) {
     case 0: hipLaunchKernelGGL((k_propagate<M, R, 0, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
     case 1: hipLaunchKernelGGL((k_propagate<M, R, 1, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
@@ -960,6 +965,10 @@ static void launch_prop(int prec, int variant, int mif, int nblocks, hipStream_t
 template <class M, typename R>
 static void launch_lineage_r(int variant, hipStream_t s, plom_dfilter *df)
 {
+    if constexpr (sizeof(R) == 8) {
+        if (variant == 4) { hipLaunchKernelGGL((k_lineage<M, R, 4>), dim3(1), dim3(64), 0, s, df); return; }
+        if (variant == 5) { hipLaunchKernelGGL((k_lineage<M, R, 5>), dim3(1), dim3(64), 0, s, df); return; }
+    }
     switch (variant) {
     case 0: hipLaunchKernelGGL((k_lineage<M, R, 0>), dim3(1), dim3(64), 0, s, df); break;
     case 1: hipLaunchKernelGGL((k_lineage<M, R, 1>), dim3(1), dim3(64), 0, s, df); break;
@@ -1169,6 +1178,8 @@ static int check_desc(const plom_filter_desc *d, plom_model_info *mi)
     if (d->precision != PLOM_FP64 && d->precision != PLOM_FP32) return set_err(PLOM_ERR_ARG, "bad precision %d", d->precision);
     if (!(d->dt > 0.0)) return set_err(PLOM_ERR_ARG, "dt must be > 0");
     if (!(d->like_min > 0.0)) return set_err(PLOM_ERR_ARG, "like_min must be > 0");
+    if ((d->flags & PLOM_FLAG_PSR) && d->precision != PLOM_FP64)
+        return set_err(PLOM_ERR_ARG, "PLOM_FLAG_PSR (the psr implementation) runs in FP64 only");
     if (d->flags & PLOM_FLAG_MIF) {
         if (d->n_theta < 1 || d->n_theta > 4096) return set_err(PLOM_ERR_ARG, "n_theta=%d outside [1, 4096]", d->n_theta);
         if (d->flags & (PLOM_FLAG_NO_FILTER | PLOM_FLAG_HAT | PLOM_FLAG_LINEAGE))
@@ -1307,7 +1318,7 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     if (!f) return set_err(PLOM_ERR_NOMEM, "host allocation failed");
     f->desc = *d;
     f->info = mi;
-    f->variant = (int)(d->noises_off & 3u);
+    f->variant = (d->flags & PLOM_FLAG_PSR) ? 4 + ((d->noises_off & PLOM_NO_ENV_STO_BIT) ? 1 : 0) : (int)(d->noises_off & 3u);
     f->dev_bytes = L.total;
     f->async_alloc = async;
     hipError_t e = async ? hipMallocAsync((void **)&f->dev, L.total, st) : hipMalloc((void **)&f->dev, L.total);

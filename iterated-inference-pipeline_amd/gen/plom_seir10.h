@@ -1700,8 +1700,1627 @@ PLOM_HD void plom_seir10_hat_values(const double *X, const double *pvd, double t
     plom_seir10_obs_moments(obs, pvd, t, out + 40, var);
 }
 
+PLOM_HD void plom_seir10_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    (void)sqdt; (void)ns; (void)pv;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P2 = PLOM_UNIFORM64(pv[2]);
+    const double P3 = PLOM_UNIFORM64(pv[3]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    const double P5 = PLOM_UNIFORM64(pv[5]);
+    const double P6 = PLOM_UNIFORM64(pv[6]);
+    const double P7 = PLOM_UNIFORM64(pv[7]);
+    const double P8 = PLOM_UNIFORM64(pv[8]);
+    const double P9 = PLOM_UNIFORM64(pv[9]);
+    const double P10 = PLOM_UNIFORM64(pv[10]);
+    const double P11 = PLOM_UNIFORM64(pv[11]);
+    const double P12 = PLOM_UNIFORM64(pv[12]);
+    const double P13 = PLOM_UNIFORM64(pv[13]);
+    const double P14 = PLOM_UNIFORM64(pv[14]);
+    const double P15 = PLOM_UNIFORM64(pv[15]);
+    const double P16 = PLOM_UNIFORM64(pv[16]);
+    const double P17 = PLOM_UNIFORM64(pv[17]);
+    const double P18 = PLOM_UNIFORM64(pv[18]);
+    const double P19 = PLOM_UNIFORM64(pv[19]);
+    const double P20 = PLOM_UNIFORM64(pv[20]);
+    const double P21 = PLOM_UNIFORM64(pv[21]);
+    const double P22 = PLOM_UNIFORM64(pv[22]);
+    const double P23 = PLOM_UNIFORM64(pv[23]);
+    const double P24 = PLOM_UNIFORM64(pv[24]);
+    const double P25 = PLOM_UNIFORM64(pv[25]);
+    const double P26 = PLOM_UNIFORM64(pv[26]);
+    const double P27 = PLOM_UNIFORM64(pv[27]);
+    const double P28 = PLOM_UNIFORM64(pv[28]);
+    const double P29 = PLOM_UNIFORM64(pv[29]);
+    const double P30 = PLOM_UNIFORM64(pv[30]);
+    const double P31 = PLOM_UNIFORM64(pv[31]);
+    const double P32 = PLOM_UNIFORM64(pv[32]);
+    const double P33 = PLOM_UNIFORM64(pv[33]);
+    const double P34 = PLOM_UNIFORM64(pv[34]);
+    const double P35 = PLOM_UNIFORM64(pv[35]);
+    const double P36 = PLOM_UNIFORM64(pv[36]);
+    const double P37 = PLOM_UNIFORM64(pv[37]);
+    const double P38 = PLOM_UNIFORM64(pv[38]);
+    const double P39 = PLOM_UNIFORM64(pv[39]);
+    const double P40 = PLOM_UNIFORM64(pv[40]);
+    const double P41 = PLOM_UNIFORM64(pv[41]);
+    const double P42 = PLOM_UNIFORM64(pv[42]);
+    const double P43 = PLOM_UNIFORM64(pv[43]);
+    const double P44 = PLOM_UNIFORM64(pv[44]);
+    const double P45 = PLOM_UNIFORM64(pv[45]);
+    const double P46 = PLOM_UNIFORM64(pv[46]);
+    const double P47 = PLOM_UNIFORM64(pv[47]);
+    const double P48 = PLOM_UNIFORM64(pv[48]);
+    const double P49 = PLOM_UNIFORM64(pv[49]);
+    const double P70 = PLOM_UNIFORM64(pv[70]);
+    const double P71 = PLOM_UNIFORM64(pv[71]);
+    const double P72 = PLOM_UNIFORM64(pv[72]);
+    const double P73 = PLOM_UNIFORM64(pv[73]);
+    const double P74 = PLOM_UNIFORM64(pv[74]);
+    const double P75 = PLOM_UNIFORM64(pv[75]);
+    const double P76 = PLOM_UNIFORM64(pv[76]);
+    const double P77 = PLOM_UNIFORM64(pv[77]);
+    const double P78 = PLOM_UNIFORM64(pv[78]);
+    const double P79 = PLOM_UNIFORM64(pv[79]);
+    const double P80 = PLOM_UNIFORM64(pv[80]);
+    const double P81 = PLOM_UNIFORM64(pv[81]);
+    const double P82 = PLOM_UNIFORM64(pv[82]);
+    const double P83 = PLOM_UNIFORM64(pv[83]);
+    const double P84 = PLOM_UNIFORM64(pv[84]);
+    const double P85 = PLOM_UNIFORM64(pv[85]);
+    const double P86 = PLOM_UNIFORM64(pv[86]);
+    const double P87 = PLOM_UNIFORM64(pv[87]);
+    const double P88 = PLOM_UNIFORM64(pv[88]);
+    const double P89 = PLOM_UNIFORM64(pv[89]);
+    const double P90 = PLOM_UNIFORM64(pv[90]);
+    const double P91 = PLOM_UNIFORM64(pv[91]);
+    const double P92 = PLOM_UNIFORM64(pv[92]);
+    const double P93 = PLOM_UNIFORM64(pv[93]);
+    const double P94 = PLOM_UNIFORM64(pv[94]);
+    const double P95 = PLOM_UNIFORM64(pv[95]);
+    const double P96 = PLOM_UNIFORM64(pv[96]);
+    const double P97 = PLOM_UNIFORM64(pv[97]);
+    const double P98 = PLOM_UNIFORM64(pv[98]);
+    const double P99 = PLOM_UNIFORM64(pv[99]);
+    const double P100 = PLOM_UNIFORM64(pv[100]);
+    const double P101 = PLOM_UNIFORM64(pv[101]);
+    const double P102 = PLOM_UNIFORM64(pv[102]);
+    const double P103 = PLOM_UNIFORM64(pv[103]);
+    const double P104 = PLOM_UNIFORM64(pv[104]);
+    const double P105 = PLOM_UNIFORM64(pv[105]);
+    const double P106 = PLOM_UNIFORM64(pv[106]);
+    const double P107 = PLOM_UNIFORM64(pv[107]);
+    const double P108 = PLOM_UNIFORM64(pv[108]);
+    const double P109 = PLOM_UNIFORM64(pv[109]);
+    const double P110 = PLOM_UNIFORM64(pv[110]);
+    const double P111 = PLOM_UNIFORM64(pv[111]);
+    const double P112 = PLOM_UNIFORM64(pv[112]);
+    const double P113 = PLOM_UNIFORM64(pv[113]);
+    const double P114 = PLOM_UNIFORM64(pv[114]);
+    const double P115 = PLOM_UNIFORM64(pv[115]);
+    const double P116 = PLOM_UNIFORM64(pv[116]);
+    const double P117 = PLOM_UNIFORM64(pv[117]);
+    const double P118 = PLOM_UNIFORM64(pv[118]);
+    const double P119 = PLOM_UNIFORM64(pv[119]);
+    const double P120 = PLOM_UNIFORM64(pv[120]);
+    const double P121 = PLOM_UNIFORM64(pv[121]);
+    const double P122 = PLOM_UNIFORM64(pv[122]);
+    const double P123 = PLOM_UNIFORM64(pv[123]);
+    const double P124 = PLOM_UNIFORM64(pv[124]);
+    const double P125 = PLOM_UNIFORM64(pv[125]);
+    const double P126 = PLOM_UNIFORM64(pv[126]);
+    const double P127 = PLOM_UNIFORM64(pv[127]);
+    const double P128 = PLOM_UNIFORM64(pv[128]);
+    const double P129 = PLOM_UNIFORM64(pv[129]);
+    const double P130 = PLOM_UNIFORM64(pv[130]);
+    const double P131 = PLOM_UNIFORM64(pv[131]);
+    const double P132 = PLOM_UNIFORM64(pv[132]);
+    const double P133 = PLOM_UNIFORM64(pv[133]);
+    const double P134 = PLOM_UNIFORM64(pv[134]);
+    const double P135 = PLOM_UNIFORM64(pv[135]);
+    const double P136 = PLOM_UNIFORM64(pv[136]);
+    const double P137 = PLOM_UNIFORM64(pv[137]);
+    const double P138 = PLOM_UNIFORM64(pv[138]);
+    const double P139 = PLOM_UNIFORM64(pv[139]);
+    const double P140 = PLOM_UNIFORM64(pv[140]);
+    const double P141 = PLOM_UNIFORM64(pv[141]);
+    const double P142 = PLOM_UNIFORM64(pv[142]);
+    const double P143 = PLOM_UNIFORM64(pv[143]);
+    const double P144 = PLOM_UNIFORM64(pv[144]);
+    const double P145 = PLOM_UNIFORM64(pv[145]);
+    const double P146 = PLOM_UNIFORM64(pv[146]);
+    const double P147 = PLOM_UNIFORM64(pv[147]);
+    const double P148 = PLOM_UNIFORM64(pv[148]);
+    const double P149 = PLOM_UNIFORM64(pv[149]);
+    const double P150 = PLOM_UNIFORM64(pv[150]);
+    const double P151 = PLOM_UNIFORM64(pv[151]);
+    const double P152 = PLOM_UNIFORM64(pv[152]);
+    const double P153 = PLOM_UNIFORM64(pv[153]);
+    const double P154 = PLOM_UNIFORM64(pv[154]);
+    const double P155 = PLOM_UNIFORM64(pv[155]);
+    const double P156 = PLOM_UNIFORM64(pv[156]);
+    const double P157 = PLOM_UNIFORM64(pv[157]);
+    const double P158 = PLOM_UNIFORM64(pv[158]);
+    const double P159 = PLOM_UNIFORM64(pv[159]);
+    const double P160 = PLOM_UNIFORM64(pv[160]);
+    const double P161 = PLOM_UNIFORM64(pv[161]);
+    const double P162 = PLOM_UNIFORM64(pv[162]);
+    const double P163 = PLOM_UNIFORM64(pv[163]);
+    const double P164 = PLOM_UNIFORM64(pv[164]);
+    const double P165 = PLOM_UNIFORM64(pv[165]);
+    const double P166 = PLOM_UNIFORM64(pv[166]);
+    const double P167 = PLOM_UNIFORM64(pv[167]);
+    const double P168 = PLOM_UNIFORM64(pv[168]);
+    const double P169 = PLOM_UNIFORM64(pv[169]);
+    const double P170 = PLOM_UNIFORM64(pv[170]);
+    const double P171 = PLOM_UNIFORM64(pv[171]);
+    const double P172 = PLOM_UNIFORM64(pv[172]);
+    const double P173 = PLOM_UNIFORM64(pv[173]);
+    const double P174 = PLOM_UNIFORM64(pv[174]);
+    const double P175 = PLOM_UNIFORM64(pv[175]);
+    const double P176 = PLOM_UNIFORM64(pv[176]);
+    const double P177 = PLOM_UNIFORM64(pv[177]);
+    const double P178 = PLOM_UNIFORM64(pv[178]);
+    const double P179 = PLOM_UNIFORM64(pv[179]);
+    X[30] = 0.0; /* reset_inc */
+    X[31] = 0.0; /* reset_inc */
+    X[32] = 0.0; /* reset_inc */
+    X[33] = 0.0; /* reset_inc */
+    X[34] = 0.0; /* reset_inc */
+    X[35] = 0.0; /* reset_inc */
+    X[36] = 0.0; /* reset_inc */
+    X[37] = 0.0; /* reset_inc */
+    X[38] = 0.0; /* reset_inc */
+    X[39] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = t; (void)tr;
+        double n0_0 = 0.0;
+        double n1_0 = 0.0;
+        double n2_0 = 0.0;
+        double n3_0 = 0.0;
+        double n0_1 = 0.0;
+        double n1_1 = 0.0;
+        double n2_1 = 0.0;
+        double n3_1 = 0.0;
+        double n0_2 = 0.0;
+        double n1_2 = 0.0;
+        double n2_2 = 0.0;
+        double n3_2 = 0.0;
+        double n0_3 = 0.0;
+        double n1_3 = 0.0;
+        double n2_3 = 0.0;
+        double n3_3 = 0.0;
+        double n0_4 = 0.0;
+        double n1_4 = 0.0;
+        double n2_4 = 0.0;
+        double n3_4 = 0.0;
+        double n0_5 = 0.0;
+        double n1_5 = 0.0;
+        double n2_5 = 0.0;
+        double n3_5 = 0.0;
+        double n0_6 = 0.0;
+        double n1_6 = 0.0;
+        double n2_6 = 0.0;
+        double n3_6 = 0.0;
+        double n0_7 = 0.0;
+        double n1_7 = 0.0;
+        double n2_7 = 0.0;
+        double n3_7 = 0.0;
+        double n0_8 = 0.0;
+        double n1_8 = 0.0;
+        double n2_8 = 0.0;
+        double n3_8 = 0.0;
+        double n0_9 = 0.0;
+        double n1_9 = 0.0;
+        double n2_9 = 0.0;
+        double n3_9 = 0.0;
+        double stay0 = 0.0;
+        double stay1 = 0.0;
+        double stay2 = 0.0;
+        double stay3 = 0.0;
+        double stay4 = 0.0;
+        double stay5 = 0.0;
+        double stay6 = 0.0;
+        double stay7 = 0.0;
+        double stay8 = 0.0;
+        double stay9 = 0.0;
+        double stay10 = 0.0;
+        double stay11 = 0.0;
+        double stay12 = 0.0;
+        double stay13 = 0.0;
+        double stay14 = 0.0;
+        double stay15 = 0.0;
+        double stay16 = 0.0;
+        double stay17 = 0.0;
+        double stay18 = 0.0;
+        double stay19 = 0.0;
+        double stay20 = 0.0;
+        double stay21 = 0.0;
+        double stay22 = 0.0;
+        double stay23 = 0.0;
+        double stay24 = 0.0;
+        double stay25 = 0.0;
+        double stay26 = 0.0;
+        double stay27 = 0.0;
+        double stay28 = 0.0;
+        double stay29 = 0.0;
+        (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n3_2; (void)n0_3; (void)n1_3; (void)n2_3; (void)n3_3; (void)n0_4; (void)n1_4; (void)n2_4; (void)n3_4; (void)n0_5; (void)n1_5; (void)n2_5; (void)n3_5; (void)n0_6; (void)n1_6; (void)n2_6; (void)n3_6; (void)n0_7; (void)n1_7; (void)n2_7; (void)n3_7; (void)n0_8; (void)n1_8; (void)n2_8; (void)n3_8; (void)n0_9; (void)n1_9; (void)n2_9; (void)n3_9;
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r1_0 = P10;
+        const double r2_0 = P20;
+        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r1_1 = P11;
+        const double r2_1 = P21;
+        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r1_2 = P12;
+        const double r2_2 = P22;
+        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r1_3 = P13;
+        const double r2_3 = P23;
+        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r1_4 = P14;
+        const double r2_4 = P24;
+        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r1_5 = P15;
+        const double r2_5 = P25;
+        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r1_6 = P16;
+        const double r2_6 = P26;
+        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r1_7 = P17;
+        const double r2_7 = P27;
+        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r1_8 = P18;
+        const double r2_8 = P28;
+        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r1_9 = P19;
+        const double r2_9 = P29;
+        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double sd0_0 = P40;
+        const double v0_0 = sd0_0 * sd0_0;
+        const double xi0_0 = (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_0 * xi0_0) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_0 * xi0_0) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
+            n0_0 = cnt[0];
+            stay0 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[10]), p, cnt);
+            n1_0 = cnt[0];
+            stay10 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[20]), p, cnt);
+            n2_0 = cnt[0];
+            stay20 = cnt[1];
+        }
+        n3_0 = plom_poisson_draw(ns, r3_0 * dt);
+        const double sd0_1 = P41;
+        const double v0_1 = sd0_1 * sd0_1;
+        const double xi0_1 = (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_1 * xi0_1) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_1 * xi0_1) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
+            n0_1 = cnt[0];
+            stay1 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_1 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_1 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[11]), p, cnt);
+            n1_1 = cnt[0];
+            stay11 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_1 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_1 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[21]), p, cnt);
+            n2_1 = cnt[0];
+            stay21 = cnt[1];
+        }
+        n3_1 = plom_poisson_draw(ns, r3_1 * dt);
+        const double sd0_2 = P42;
+        const double v0_2 = sd0_2 * sd0_2;
+        const double xi0_2 = (v0_2 * plom_gamma_draw(ns, dt / v0_2)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_2 * xi0_2) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_2 * xi0_2) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[2]), p, cnt);
+            n0_2 = cnt[0];
+            stay2 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_2 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_2 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[12]), p, cnt);
+            n1_2 = cnt[0];
+            stay12 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_2 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_2 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[22]), p, cnt);
+            n2_2 = cnt[0];
+            stay22 = cnt[1];
+        }
+        n3_2 = plom_poisson_draw(ns, r3_2 * dt);
+        const double sd0_3 = P43;
+        const double v0_3 = sd0_3 * sd0_3;
+        const double xi0_3 = (v0_3 * plom_gamma_draw(ns, dt / v0_3)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_3 * xi0_3) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_3 * xi0_3) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[3]), p, cnt);
+            n0_3 = cnt[0];
+            stay3 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_3 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_3 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[13]), p, cnt);
+            n1_3 = cnt[0];
+            stay13 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_3 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_3 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[23]), p, cnt);
+            n2_3 = cnt[0];
+            stay23 = cnt[1];
+        }
+        n3_3 = plom_poisson_draw(ns, r3_3 * dt);
+        const double sd0_4 = P44;
+        const double v0_4 = sd0_4 * sd0_4;
+        const double xi0_4 = (v0_4 * plom_gamma_draw(ns, dt / v0_4)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_4 * xi0_4) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_4 * xi0_4) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[4]), p, cnt);
+            n0_4 = cnt[0];
+            stay4 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_4 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_4 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[14]), p, cnt);
+            n1_4 = cnt[0];
+            stay14 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_4 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_4 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[24]), p, cnt);
+            n2_4 = cnt[0];
+            stay24 = cnt[1];
+        }
+        n3_4 = plom_poisson_draw(ns, r3_4 * dt);
+        const double sd0_5 = P45;
+        const double v0_5 = sd0_5 * sd0_5;
+        const double xi0_5 = (v0_5 * plom_gamma_draw(ns, dt / v0_5)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_5 * xi0_5) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_5 * xi0_5) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[5]), p, cnt);
+            n0_5 = cnt[0];
+            stay5 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_5 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_5 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[15]), p, cnt);
+            n1_5 = cnt[0];
+            stay15 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_5 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_5 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[25]), p, cnt);
+            n2_5 = cnt[0];
+            stay25 = cnt[1];
+        }
+        n3_5 = plom_poisson_draw(ns, r3_5 * dt);
+        const double sd0_6 = P46;
+        const double v0_6 = sd0_6 * sd0_6;
+        const double xi0_6 = (v0_6 * plom_gamma_draw(ns, dt / v0_6)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_6 * xi0_6) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_6 * xi0_6) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[6]), p, cnt);
+            n0_6 = cnt[0];
+            stay6 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_6 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_6 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[16]), p, cnt);
+            n1_6 = cnt[0];
+            stay16 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_6 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_6 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[26]), p, cnt);
+            n2_6 = cnt[0];
+            stay26 = cnt[1];
+        }
+        n3_6 = plom_poisson_draw(ns, r3_6 * dt);
+        const double sd0_7 = P47;
+        const double v0_7 = sd0_7 * sd0_7;
+        const double xi0_7 = (v0_7 * plom_gamma_draw(ns, dt / v0_7)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_7 * xi0_7) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_7 * xi0_7) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[7]), p, cnt);
+            n0_7 = cnt[0];
+            stay7 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_7 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_7 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[17]), p, cnt);
+            n1_7 = cnt[0];
+            stay17 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_7 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_7 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[27]), p, cnt);
+            n2_7 = cnt[0];
+            stay27 = cnt[1];
+        }
+        n3_7 = plom_poisson_draw(ns, r3_7 * dt);
+        const double sd0_8 = P48;
+        const double v0_8 = sd0_8 * sd0_8;
+        const double xi0_8 = (v0_8 * plom_gamma_draw(ns, dt / v0_8)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_8 * xi0_8) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_8 * xi0_8) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[8]), p, cnt);
+            n0_8 = cnt[0];
+            stay8 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_8 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_8 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[18]), p, cnt);
+            n1_8 = cnt[0];
+            stay18 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_8 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_8 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[28]), p, cnt);
+            n2_8 = cnt[0];
+            stay28 = cnt[1];
+        }
+        n3_8 = plom_poisson_draw(ns, r3_8 * dt);
+        const double sd0_9 = P49;
+        const double v0_9 = sd0_9 * sd0_9;
+        const double xi0_9 = (v0_9 * plom_gamma_draw(ns, dt / v0_9)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_9 * xi0_9) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_9 * xi0_9) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[9]), p, cnt);
+            n0_9 = cnt[0];
+            stay9 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_9 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_9 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[19]), p, cnt);
+            n1_9 = cnt[0];
+            stay19 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_9 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_9 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[29]), p, cnt);
+            n2_9 = cnt[0];
+            stay29 = cnt[1];
+        }
+        n3_9 = plom_poisson_draw(ns, r3_9 * dt);
+        X[0] = stay0 + n3_0;
+        X[10] = stay10 + n0_0;
+        X[20] = stay20 + n1_0;
+        X[1] = stay1 + n3_1;
+        X[11] = stay11 + n0_1;
+        X[21] = stay21 + n1_1;
+        X[2] = stay2 + n3_2;
+        X[12] = stay12 + n0_2;
+        X[22] = stay22 + n1_2;
+        X[3] = stay3 + n3_3;
+        X[13] = stay13 + n0_3;
+        X[23] = stay23 + n1_3;
+        X[4] = stay4 + n3_4;
+        X[14] = stay14 + n0_4;
+        X[24] = stay24 + n1_4;
+        X[5] = stay5 + n3_5;
+        X[15] = stay15 + n0_5;
+        X[25] = stay25 + n1_5;
+        X[6] = stay6 + n3_6;
+        X[16] = stay16 + n0_6;
+        X[26] = stay26 + n1_6;
+        X[7] = stay7 + n3_7;
+        X[17] = stay17 + n0_7;
+        X[27] = stay27 + n1_7;
+        X[8] = stay8 + n3_8;
+        X[18] = stay18 + n0_8;
+        X[28] = stay28 + n1_8;
+        X[9] = stay9 + n3_9;
+        X[19] = stay19 + n0_9;
+        X[29] = stay29 + n1_9;
+        X[30] = X[30] + (n1_0);
+        X[31] = X[31] + (n1_1);
+        X[32] = X[32] + (n1_2);
+        X[33] = X[33] + (n1_3);
+        X[34] = X[34] + (n1_4);
+        X[35] = X[35] + (n1_5);
+        X[36] = X[36] + (n1_6);
+        X[37] = X[37] + (n1_7);
+        X[38] = X[38] + (n1_8);
+        X[39] = X[39] + (n1_9);
+        t += dt;
+    }
+}
+
+PLOM_HD void plom_seir10_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    (void)sqdt; (void)ns; (void)pv;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P2 = PLOM_UNIFORM64(pv[2]);
+    const double P3 = PLOM_UNIFORM64(pv[3]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    const double P5 = PLOM_UNIFORM64(pv[5]);
+    const double P6 = PLOM_UNIFORM64(pv[6]);
+    const double P7 = PLOM_UNIFORM64(pv[7]);
+    const double P8 = PLOM_UNIFORM64(pv[8]);
+    const double P9 = PLOM_UNIFORM64(pv[9]);
+    const double P10 = PLOM_UNIFORM64(pv[10]);
+    const double P11 = PLOM_UNIFORM64(pv[11]);
+    const double P12 = PLOM_UNIFORM64(pv[12]);
+    const double P13 = PLOM_UNIFORM64(pv[13]);
+    const double P14 = PLOM_UNIFORM64(pv[14]);
+    const double P15 = PLOM_UNIFORM64(pv[15]);
+    const double P16 = PLOM_UNIFORM64(pv[16]);
+    const double P17 = PLOM_UNIFORM64(pv[17]);
+    const double P18 = PLOM_UNIFORM64(pv[18]);
+    const double P19 = PLOM_UNIFORM64(pv[19]);
+    const double P20 = PLOM_UNIFORM64(pv[20]);
+    const double P21 = PLOM_UNIFORM64(pv[21]);
+    const double P22 = PLOM_UNIFORM64(pv[22]);
+    const double P23 = PLOM_UNIFORM64(pv[23]);
+    const double P24 = PLOM_UNIFORM64(pv[24]);
+    const double P25 = PLOM_UNIFORM64(pv[25]);
+    const double P26 = PLOM_UNIFORM64(pv[26]);
+    const double P27 = PLOM_UNIFORM64(pv[27]);
+    const double P28 = PLOM_UNIFORM64(pv[28]);
+    const double P29 = PLOM_UNIFORM64(pv[29]);
+    const double P30 = PLOM_UNIFORM64(pv[30]);
+    const double P31 = PLOM_UNIFORM64(pv[31]);
+    const double P32 = PLOM_UNIFORM64(pv[32]);
+    const double P33 = PLOM_UNIFORM64(pv[33]);
+    const double P34 = PLOM_UNIFORM64(pv[34]);
+    const double P35 = PLOM_UNIFORM64(pv[35]);
+    const double P36 = PLOM_UNIFORM64(pv[36]);
+    const double P37 = PLOM_UNIFORM64(pv[37]);
+    const double P38 = PLOM_UNIFORM64(pv[38]);
+    const double P39 = PLOM_UNIFORM64(pv[39]);
+    const double P70 = PLOM_UNIFORM64(pv[70]);
+    const double P71 = PLOM_UNIFORM64(pv[71]);
+    const double P72 = PLOM_UNIFORM64(pv[72]);
+    const double P73 = PLOM_UNIFORM64(pv[73]);
+    const double P74 = PLOM_UNIFORM64(pv[74]);
+    const double P75 = PLOM_UNIFORM64(pv[75]);
+    const double P76 = PLOM_UNIFORM64(pv[76]);
+    const double P77 = PLOM_UNIFORM64(pv[77]);
+    const double P78 = PLOM_UNIFORM64(pv[78]);
+    const double P79 = PLOM_UNIFORM64(pv[79]);
+    const double P80 = PLOM_UNIFORM64(pv[80]);
+    const double P81 = PLOM_UNIFORM64(pv[81]);
+    const double P82 = PLOM_UNIFORM64(pv[82]);
+    const double P83 = PLOM_UNIFORM64(pv[83]);
+    const double P84 = PLOM_UNIFORM64(pv[84]);
+    const double P85 = PLOM_UNIFORM64(pv[85]);
+    const double P86 = PLOM_UNIFORM64(pv[86]);
+    const double P87 = PLOM_UNIFORM64(pv[87]);
+    const double P88 = PLOM_UNIFORM64(pv[88]);
+    const double P89 = PLOM_UNIFORM64(pv[89]);
+    const double P90 = PLOM_UNIFORM64(pv[90]);
+    const double P91 = PLOM_UNIFORM64(pv[91]);
+    const double P92 = PLOM_UNIFORM64(pv[92]);
+    const double P93 = PLOM_UNIFORM64(pv[93]);
+    const double P94 = PLOM_UNIFORM64(pv[94]);
+    const double P95 = PLOM_UNIFORM64(pv[95]);
+    const double P96 = PLOM_UNIFORM64(pv[96]);
+    const double P97 = PLOM_UNIFORM64(pv[97]);
+    const double P98 = PLOM_UNIFORM64(pv[98]);
+    const double P99 = PLOM_UNIFORM64(pv[99]);
+    const double P100 = PLOM_UNIFORM64(pv[100]);
+    const double P101 = PLOM_UNIFORM64(pv[101]);
+    const double P102 = PLOM_UNIFORM64(pv[102]);
+    const double P103 = PLOM_UNIFORM64(pv[103]);
+    const double P104 = PLOM_UNIFORM64(pv[104]);
+    const double P105 = PLOM_UNIFORM64(pv[105]);
+    const double P106 = PLOM_UNIFORM64(pv[106]);
+    const double P107 = PLOM_UNIFORM64(pv[107]);
+    const double P108 = PLOM_UNIFORM64(pv[108]);
+    const double P109 = PLOM_UNIFORM64(pv[109]);
+    const double P110 = PLOM_UNIFORM64(pv[110]);
+    const double P111 = PLOM_UNIFORM64(pv[111]);
+    const double P112 = PLOM_UNIFORM64(pv[112]);
+    const double P113 = PLOM_UNIFORM64(pv[113]);
+    const double P114 = PLOM_UNIFORM64(pv[114]);
+    const double P115 = PLOM_UNIFORM64(pv[115]);
+    const double P116 = PLOM_UNIFORM64(pv[116]);
+    const double P117 = PLOM_UNIFORM64(pv[117]);
+    const double P118 = PLOM_UNIFORM64(pv[118]);
+    const double P119 = PLOM_UNIFORM64(pv[119]);
+    const double P120 = PLOM_UNIFORM64(pv[120]);
+    const double P121 = PLOM_UNIFORM64(pv[121]);
+    const double P122 = PLOM_UNIFORM64(pv[122]);
+    const double P123 = PLOM_UNIFORM64(pv[123]);
+    const double P124 = PLOM_UNIFORM64(pv[124]);
+    const double P125 = PLOM_UNIFORM64(pv[125]);
+    const double P126 = PLOM_UNIFORM64(pv[126]);
+    const double P127 = PLOM_UNIFORM64(pv[127]);
+    const double P128 = PLOM_UNIFORM64(pv[128]);
+    const double P129 = PLOM_UNIFORM64(pv[129]);
+    const double P130 = PLOM_UNIFORM64(pv[130]);
+    const double P131 = PLOM_UNIFORM64(pv[131]);
+    const double P132 = PLOM_UNIFORM64(pv[132]);
+    const double P133 = PLOM_UNIFORM64(pv[133]);
+    const double P134 = PLOM_UNIFORM64(pv[134]);
+    const double P135 = PLOM_UNIFORM64(pv[135]);
+    const double P136 = PLOM_UNIFORM64(pv[136]);
+    const double P137 = PLOM_UNIFORM64(pv[137]);
+    const double P138 = PLOM_UNIFORM64(pv[138]);
+    const double P139 = PLOM_UNIFORM64(pv[139]);
+    const double P140 = PLOM_UNIFORM64(pv[140]);
+    const double P141 = PLOM_UNIFORM64(pv[141]);
+    const double P142 = PLOM_UNIFORM64(pv[142]);
+    const double P143 = PLOM_UNIFORM64(pv[143]);
+    const double P144 = PLOM_UNIFORM64(pv[144]);
+    const double P145 = PLOM_UNIFORM64(pv[145]);
+    const double P146 = PLOM_UNIFORM64(pv[146]);
+    const double P147 = PLOM_UNIFORM64(pv[147]);
+    const double P148 = PLOM_UNIFORM64(pv[148]);
+    const double P149 = PLOM_UNIFORM64(pv[149]);
+    const double P150 = PLOM_UNIFORM64(pv[150]);
+    const double P151 = PLOM_UNIFORM64(pv[151]);
+    const double P152 = PLOM_UNIFORM64(pv[152]);
+    const double P153 = PLOM_UNIFORM64(pv[153]);
+    const double P154 = PLOM_UNIFORM64(pv[154]);
+    const double P155 = PLOM_UNIFORM64(pv[155]);
+    const double P156 = PLOM_UNIFORM64(pv[156]);
+    const double P157 = PLOM_UNIFORM64(pv[157]);
+    const double P158 = PLOM_UNIFORM64(pv[158]);
+    const double P159 = PLOM_UNIFORM64(pv[159]);
+    const double P160 = PLOM_UNIFORM64(pv[160]);
+    const double P161 = PLOM_UNIFORM64(pv[161]);
+    const double P162 = PLOM_UNIFORM64(pv[162]);
+    const double P163 = PLOM_UNIFORM64(pv[163]);
+    const double P164 = PLOM_UNIFORM64(pv[164]);
+    const double P165 = PLOM_UNIFORM64(pv[165]);
+    const double P166 = PLOM_UNIFORM64(pv[166]);
+    const double P167 = PLOM_UNIFORM64(pv[167]);
+    const double P168 = PLOM_UNIFORM64(pv[168]);
+    const double P169 = PLOM_UNIFORM64(pv[169]);
+    const double P170 = PLOM_UNIFORM64(pv[170]);
+    const double P171 = PLOM_UNIFORM64(pv[171]);
+    const double P172 = PLOM_UNIFORM64(pv[172]);
+    const double P173 = PLOM_UNIFORM64(pv[173]);
+    const double P174 = PLOM_UNIFORM64(pv[174]);
+    const double P175 = PLOM_UNIFORM64(pv[175]);
+    const double P176 = PLOM_UNIFORM64(pv[176]);
+    const double P177 = PLOM_UNIFORM64(pv[177]);
+    const double P178 = PLOM_UNIFORM64(pv[178]);
+    const double P179 = PLOM_UNIFORM64(pv[179]);
+    X[30] = 0.0; /* reset_inc */
+    X[31] = 0.0; /* reset_inc */
+    X[32] = 0.0; /* reset_inc */
+    X[33] = 0.0; /* reset_inc */
+    X[34] = 0.0; /* reset_inc */
+    X[35] = 0.0; /* reset_inc */
+    X[36] = 0.0; /* reset_inc */
+    X[37] = 0.0; /* reset_inc */
+    X[38] = 0.0; /* reset_inc */
+    X[39] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = t; (void)tr;
+        double n0_0 = 0.0;
+        double n1_0 = 0.0;
+        double n2_0 = 0.0;
+        double n3_0 = 0.0;
+        double n0_1 = 0.0;
+        double n1_1 = 0.0;
+        double n2_1 = 0.0;
+        double n3_1 = 0.0;
+        double n0_2 = 0.0;
+        double n1_2 = 0.0;
+        double n2_2 = 0.0;
+        double n3_2 = 0.0;
+        double n0_3 = 0.0;
+        double n1_3 = 0.0;
+        double n2_3 = 0.0;
+        double n3_3 = 0.0;
+        double n0_4 = 0.0;
+        double n1_4 = 0.0;
+        double n2_4 = 0.0;
+        double n3_4 = 0.0;
+        double n0_5 = 0.0;
+        double n1_5 = 0.0;
+        double n2_5 = 0.0;
+        double n3_5 = 0.0;
+        double n0_6 = 0.0;
+        double n1_6 = 0.0;
+        double n2_6 = 0.0;
+        double n3_6 = 0.0;
+        double n0_7 = 0.0;
+        double n1_7 = 0.0;
+        double n2_7 = 0.0;
+        double n3_7 = 0.0;
+        double n0_8 = 0.0;
+        double n1_8 = 0.0;
+        double n2_8 = 0.0;
+        double n3_8 = 0.0;
+        double n0_9 = 0.0;
+        double n1_9 = 0.0;
+        double n2_9 = 0.0;
+        double n3_9 = 0.0;
+        double stay0 = 0.0;
+        double stay1 = 0.0;
+        double stay2 = 0.0;
+        double stay3 = 0.0;
+        double stay4 = 0.0;
+        double stay5 = 0.0;
+        double stay6 = 0.0;
+        double stay7 = 0.0;
+        double stay8 = 0.0;
+        double stay9 = 0.0;
+        double stay10 = 0.0;
+        double stay11 = 0.0;
+        double stay12 = 0.0;
+        double stay13 = 0.0;
+        double stay14 = 0.0;
+        double stay15 = 0.0;
+        double stay16 = 0.0;
+        double stay17 = 0.0;
+        double stay18 = 0.0;
+        double stay19 = 0.0;
+        double stay20 = 0.0;
+        double stay21 = 0.0;
+        double stay22 = 0.0;
+        double stay23 = 0.0;
+        double stay24 = 0.0;
+        double stay25 = 0.0;
+        double stay26 = 0.0;
+        double stay27 = 0.0;
+        double stay28 = 0.0;
+        double stay29 = 0.0;
+        (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n3_2; (void)n0_3; (void)n1_3; (void)n2_3; (void)n3_3; (void)n0_4; (void)n1_4; (void)n2_4; (void)n3_4; (void)n0_5; (void)n1_5; (void)n2_5; (void)n3_5; (void)n0_6; (void)n1_6; (void)n2_6; (void)n3_6; (void)n0_7; (void)n1_7; (void)n2_7; (void)n3_7; (void)n0_8; (void)n1_8; (void)n2_8; (void)n3_8; (void)n0_9; (void)n1_9; (void)n2_9; (void)n3_9;
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r1_0 = P10;
+        const double r2_0 = P20;
+        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r1_1 = P11;
+        const double r2_1 = P21;
+        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r1_2 = P12;
+        const double r2_2 = P22;
+        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r1_3 = P13;
+        const double r2_3 = P23;
+        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r1_4 = P14;
+        const double r2_4 = P24;
+        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r1_5 = P15;
+        const double r2_5 = P25;
+        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r1_6 = P16;
+        const double r2_6 = P26;
+        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r1_7 = P17;
+        const double r2_7 = P27;
+        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r1_8 = P18;
+        const double r2_8 = P28;
+        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r1_9 = P19;
+        const double r2_9 = P29;
+        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double xi0_0 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_0 * xi0_0) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_0 * xi0_0) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
+            n0_0 = cnt[0];
+            stay0 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[10]), p, cnt);
+            n1_0 = cnt[0];
+            stay10 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[20]), p, cnt);
+            n2_0 = cnt[0];
+            stay20 = cnt[1];
+        }
+        n3_0 = plom_poisson_draw(ns, r3_0 * dt);
+        const double xi0_1 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_1 * xi0_1) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_1 * xi0_1) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
+            n0_1 = cnt[0];
+            stay1 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_1 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_1 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[11]), p, cnt);
+            n1_1 = cnt[0];
+            stay11 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_1 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_1 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[21]), p, cnt);
+            n2_1 = cnt[0];
+            stay21 = cnt[1];
+        }
+        n3_1 = plom_poisson_draw(ns, r3_1 * dt);
+        const double xi0_2 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_2 * xi0_2) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_2 * xi0_2) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[2]), p, cnt);
+            n0_2 = cnt[0];
+            stay2 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_2 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_2 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[12]), p, cnt);
+            n1_2 = cnt[0];
+            stay12 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_2 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_2 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[22]), p, cnt);
+            n2_2 = cnt[0];
+            stay22 = cnt[1];
+        }
+        n3_2 = plom_poisson_draw(ns, r3_2 * dt);
+        const double xi0_3 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_3 * xi0_3) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_3 * xi0_3) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[3]), p, cnt);
+            n0_3 = cnt[0];
+            stay3 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_3 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_3 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[13]), p, cnt);
+            n1_3 = cnt[0];
+            stay13 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_3 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_3 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[23]), p, cnt);
+            n2_3 = cnt[0];
+            stay23 = cnt[1];
+        }
+        n3_3 = plom_poisson_draw(ns, r3_3 * dt);
+        const double xi0_4 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_4 * xi0_4) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_4 * xi0_4) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[4]), p, cnt);
+            n0_4 = cnt[0];
+            stay4 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_4 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_4 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[14]), p, cnt);
+            n1_4 = cnt[0];
+            stay14 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_4 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_4 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[24]), p, cnt);
+            n2_4 = cnt[0];
+            stay24 = cnt[1];
+        }
+        n3_4 = plom_poisson_draw(ns, r3_4 * dt);
+        const double xi0_5 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_5 * xi0_5) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_5 * xi0_5) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[5]), p, cnt);
+            n0_5 = cnt[0];
+            stay5 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_5 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_5 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[15]), p, cnt);
+            n1_5 = cnt[0];
+            stay15 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_5 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_5 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[25]), p, cnt);
+            n2_5 = cnt[0];
+            stay25 = cnt[1];
+        }
+        n3_5 = plom_poisson_draw(ns, r3_5 * dt);
+        const double xi0_6 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_6 * xi0_6) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_6 * xi0_6) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[6]), p, cnt);
+            n0_6 = cnt[0];
+            stay6 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_6 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_6 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[16]), p, cnt);
+            n1_6 = cnt[0];
+            stay16 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_6 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_6 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[26]), p, cnt);
+            n2_6 = cnt[0];
+            stay26 = cnt[1];
+        }
+        n3_6 = plom_poisson_draw(ns, r3_6 * dt);
+        const double xi0_7 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_7 * xi0_7) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_7 * xi0_7) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[7]), p, cnt);
+            n0_7 = cnt[0];
+            stay7 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_7 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_7 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[17]), p, cnt);
+            n1_7 = cnt[0];
+            stay17 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_7 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_7 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[27]), p, cnt);
+            n2_7 = cnt[0];
+            stay27 = cnt[1];
+        }
+        n3_7 = plom_poisson_draw(ns, r3_7 * dt);
+        const double xi0_8 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_8 * xi0_8) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_8 * xi0_8) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[8]), p, cnt);
+            n0_8 = cnt[0];
+            stay8 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_8 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_8 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[18]), p, cnt);
+            n1_8 = cnt[0];
+            stay18 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_8 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_8 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[28]), p, cnt);
+            n2_8 = cnt[0];
+            stay28 = cnt[1];
+        }
+        n3_8 = plom_poisson_draw(ns, r3_8 * dt);
+        const double xi0_9 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_9 * xi0_9) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_9 * xi0_9) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[9]), p, cnt);
+            n0_9 = cnt[0];
+            stay9 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_9 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_9 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[19]), p, cnt);
+            n1_9 = cnt[0];
+            stay19 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_9 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_9 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[29]), p, cnt);
+            n2_9 = cnt[0];
+            stay29 = cnt[1];
+        }
+        n3_9 = plom_poisson_draw(ns, r3_9 * dt);
+        X[0] = stay0 + n3_0;
+        X[10] = stay10 + n0_0;
+        X[20] = stay20 + n1_0;
+        X[1] = stay1 + n3_1;
+        X[11] = stay11 + n0_1;
+        X[21] = stay21 + n1_1;
+        X[2] = stay2 + n3_2;
+        X[12] = stay12 + n0_2;
+        X[22] = stay22 + n1_2;
+        X[3] = stay3 + n3_3;
+        X[13] = stay13 + n0_3;
+        X[23] = stay23 + n1_3;
+        X[4] = stay4 + n3_4;
+        X[14] = stay14 + n0_4;
+        X[24] = stay24 + n1_4;
+        X[5] = stay5 + n3_5;
+        X[15] = stay15 + n0_5;
+        X[25] = stay25 + n1_5;
+        X[6] = stay6 + n3_6;
+        X[16] = stay16 + n0_6;
+        X[26] = stay26 + n1_6;
+        X[7] = stay7 + n3_7;
+        X[17] = stay17 + n0_7;
+        X[27] = stay27 + n1_7;
+        X[8] = stay8 + n3_8;
+        X[18] = stay18 + n0_8;
+        X[28] = stay28 + n1_8;
+        X[9] = stay9 + n3_9;
+        X[19] = stay19 + n0_9;
+        X[29] = stay29 + n1_9;
+        X[30] = X[30] + (n1_0);
+        X[31] = X[31] + (n1_1);
+        X[32] = X[32] + (n1_2);
+        X[33] = X[33] + (n1_3);
+        X[34] = X[34] + (n1_4);
+        X[35] = X[35] + (n1_5);
+        X[36] = X[36] + (n1_6);
+        X[37] = X[37] + (n1_7);
+        X[38] = X[38] + (n1_8);
+        X[39] = X[39] + (n1_9);
+        t += dt;
+    }
+}
+
 PLOM_HD void plom_seir10_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off)
 {
+    if (noises_off & PLOM_IMPL_PSR) {
+        if (noises_off & PLOM_NO_ENV_STO) plom_seir10_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
+        else plom_seir10_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);
+        return;
+    }
     const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);
     if (v == 0u) plom_seir10_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt);
     else if (v == PLOM_NO_DEM_STO) plom_seir10_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
@@ -3191,6 +4810,1620 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_proj2ob
     obs[7] = (double)X[37];
     obs[8] = (double)X[38];
     obs[9] = (double)X[39];
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    const double P0 = lp[0];
+    const double P1 = lp[1];
+    const double P2 = lp[2];
+    const double P3 = lp[3];
+    const double P4 = lp[4];
+    const double P5 = lp[5];
+    const double P6 = lp[6];
+    const double P7 = lp[7];
+    const double P8 = lp[8];
+    const double P9 = lp[9];
+    const double P10 = lp[10];
+    const double P11 = lp[11];
+    const double P12 = lp[12];
+    const double P13 = lp[13];
+    const double P14 = lp[14];
+    const double P15 = lp[15];
+    const double P16 = lp[16];
+    const double P17 = lp[17];
+    const double P18 = lp[18];
+    const double P19 = lp[19];
+    const double P20 = lp[20];
+    const double P21 = lp[21];
+    const double P22 = lp[22];
+    const double P23 = lp[23];
+    const double P24 = lp[24];
+    const double P25 = lp[25];
+    const double P26 = lp[26];
+    const double P27 = lp[27];
+    const double P28 = lp[28];
+    const double P29 = lp[29];
+    const double P30 = lp[30];
+    const double P31 = lp[31];
+    const double P32 = lp[32];
+    const double P33 = lp[33];
+    const double P34 = lp[34];
+    const double P35 = lp[35];
+    const double P36 = lp[36];
+    const double P37 = lp[37];
+    const double P38 = lp[38];
+    const double P39 = lp[39];
+    const double P40 = lp[40];
+    const double P41 = lp[41];
+    const double P42 = lp[42];
+    const double P43 = lp[43];
+    const double P44 = lp[44];
+    const double P45 = lp[45];
+    const double P46 = lp[46];
+    const double P47 = lp[47];
+    const double P48 = lp[48];
+    const double P49 = lp[49];
+    const double P70 = PLOM_UNIFORM64(pv[70]);
+    const double P71 = PLOM_UNIFORM64(pv[71]);
+    const double P72 = PLOM_UNIFORM64(pv[72]);
+    const double P73 = PLOM_UNIFORM64(pv[73]);
+    const double P74 = PLOM_UNIFORM64(pv[74]);
+    const double P75 = PLOM_UNIFORM64(pv[75]);
+    const double P76 = PLOM_UNIFORM64(pv[76]);
+    const double P77 = PLOM_UNIFORM64(pv[77]);
+    const double P78 = PLOM_UNIFORM64(pv[78]);
+    const double P79 = PLOM_UNIFORM64(pv[79]);
+    const double P80 = PLOM_UNIFORM64(pv[80]);
+    const double P81 = PLOM_UNIFORM64(pv[81]);
+    const double P82 = PLOM_UNIFORM64(pv[82]);
+    const double P83 = PLOM_UNIFORM64(pv[83]);
+    const double P84 = PLOM_UNIFORM64(pv[84]);
+    const double P85 = PLOM_UNIFORM64(pv[85]);
+    const double P86 = PLOM_UNIFORM64(pv[86]);
+    const double P87 = PLOM_UNIFORM64(pv[87]);
+    const double P88 = PLOM_UNIFORM64(pv[88]);
+    const double P89 = PLOM_UNIFORM64(pv[89]);
+    const double P90 = PLOM_UNIFORM64(pv[90]);
+    const double P91 = PLOM_UNIFORM64(pv[91]);
+    const double P92 = PLOM_UNIFORM64(pv[92]);
+    const double P93 = PLOM_UNIFORM64(pv[93]);
+    const double P94 = PLOM_UNIFORM64(pv[94]);
+    const double P95 = PLOM_UNIFORM64(pv[95]);
+    const double P96 = PLOM_UNIFORM64(pv[96]);
+    const double P97 = PLOM_UNIFORM64(pv[97]);
+    const double P98 = PLOM_UNIFORM64(pv[98]);
+    const double P99 = PLOM_UNIFORM64(pv[99]);
+    const double P100 = PLOM_UNIFORM64(pv[100]);
+    const double P101 = PLOM_UNIFORM64(pv[101]);
+    const double P102 = PLOM_UNIFORM64(pv[102]);
+    const double P103 = PLOM_UNIFORM64(pv[103]);
+    const double P104 = PLOM_UNIFORM64(pv[104]);
+    const double P105 = PLOM_UNIFORM64(pv[105]);
+    const double P106 = PLOM_UNIFORM64(pv[106]);
+    const double P107 = PLOM_UNIFORM64(pv[107]);
+    const double P108 = PLOM_UNIFORM64(pv[108]);
+    const double P109 = PLOM_UNIFORM64(pv[109]);
+    const double P110 = PLOM_UNIFORM64(pv[110]);
+    const double P111 = PLOM_UNIFORM64(pv[111]);
+    const double P112 = PLOM_UNIFORM64(pv[112]);
+    const double P113 = PLOM_UNIFORM64(pv[113]);
+    const double P114 = PLOM_UNIFORM64(pv[114]);
+    const double P115 = PLOM_UNIFORM64(pv[115]);
+    const double P116 = PLOM_UNIFORM64(pv[116]);
+    const double P117 = PLOM_UNIFORM64(pv[117]);
+    const double P118 = PLOM_UNIFORM64(pv[118]);
+    const double P119 = PLOM_UNIFORM64(pv[119]);
+    const double P120 = PLOM_UNIFORM64(pv[120]);
+    const double P121 = PLOM_UNIFORM64(pv[121]);
+    const double P122 = PLOM_UNIFORM64(pv[122]);
+    const double P123 = PLOM_UNIFORM64(pv[123]);
+    const double P124 = PLOM_UNIFORM64(pv[124]);
+    const double P125 = PLOM_UNIFORM64(pv[125]);
+    const double P126 = PLOM_UNIFORM64(pv[126]);
+    const double P127 = PLOM_UNIFORM64(pv[127]);
+    const double P128 = PLOM_UNIFORM64(pv[128]);
+    const double P129 = PLOM_UNIFORM64(pv[129]);
+    const double P130 = PLOM_UNIFORM64(pv[130]);
+    const double P131 = PLOM_UNIFORM64(pv[131]);
+    const double P132 = PLOM_UNIFORM64(pv[132]);
+    const double P133 = PLOM_UNIFORM64(pv[133]);
+    const double P134 = PLOM_UNIFORM64(pv[134]);
+    const double P135 = PLOM_UNIFORM64(pv[135]);
+    const double P136 = PLOM_UNIFORM64(pv[136]);
+    const double P137 = PLOM_UNIFORM64(pv[137]);
+    const double P138 = PLOM_UNIFORM64(pv[138]);
+    const double P139 = PLOM_UNIFORM64(pv[139]);
+    const double P140 = PLOM_UNIFORM64(pv[140]);
+    const double P141 = PLOM_UNIFORM64(pv[141]);
+    const double P142 = PLOM_UNIFORM64(pv[142]);
+    const double P143 = PLOM_UNIFORM64(pv[143]);
+    const double P144 = PLOM_UNIFORM64(pv[144]);
+    const double P145 = PLOM_UNIFORM64(pv[145]);
+    const double P146 = PLOM_UNIFORM64(pv[146]);
+    const double P147 = PLOM_UNIFORM64(pv[147]);
+    const double P148 = PLOM_UNIFORM64(pv[148]);
+    const double P149 = PLOM_UNIFORM64(pv[149]);
+    const double P150 = PLOM_UNIFORM64(pv[150]);
+    const double P151 = PLOM_UNIFORM64(pv[151]);
+    const double P152 = PLOM_UNIFORM64(pv[152]);
+    const double P153 = PLOM_UNIFORM64(pv[153]);
+    const double P154 = PLOM_UNIFORM64(pv[154]);
+    const double P155 = PLOM_UNIFORM64(pv[155]);
+    const double P156 = PLOM_UNIFORM64(pv[156]);
+    const double P157 = PLOM_UNIFORM64(pv[157]);
+    const double P158 = PLOM_UNIFORM64(pv[158]);
+    const double P159 = PLOM_UNIFORM64(pv[159]);
+    const double P160 = PLOM_UNIFORM64(pv[160]);
+    const double P161 = PLOM_UNIFORM64(pv[161]);
+    const double P162 = PLOM_UNIFORM64(pv[162]);
+    const double P163 = PLOM_UNIFORM64(pv[163]);
+    const double P164 = PLOM_UNIFORM64(pv[164]);
+    const double P165 = PLOM_UNIFORM64(pv[165]);
+    const double P166 = PLOM_UNIFORM64(pv[166]);
+    const double P167 = PLOM_UNIFORM64(pv[167]);
+    const double P168 = PLOM_UNIFORM64(pv[168]);
+    const double P169 = PLOM_UNIFORM64(pv[169]);
+    const double P170 = PLOM_UNIFORM64(pv[170]);
+    const double P171 = PLOM_UNIFORM64(pv[171]);
+    const double P172 = PLOM_UNIFORM64(pv[172]);
+    const double P173 = PLOM_UNIFORM64(pv[173]);
+    const double P174 = PLOM_UNIFORM64(pv[174]);
+    const double P175 = PLOM_UNIFORM64(pv[175]);
+    const double P176 = PLOM_UNIFORM64(pv[176]);
+    const double P177 = PLOM_UNIFORM64(pv[177]);
+    const double P178 = PLOM_UNIFORM64(pv[178]);
+    const double P179 = PLOM_UNIFORM64(pv[179]);
+    X[30] = 0.0; /* reset_inc */
+    X[31] = 0.0; /* reset_inc */
+    X[32] = 0.0; /* reset_inc */
+    X[33] = 0.0; /* reset_inc */
+    X[34] = 0.0; /* reset_inc */
+    X[35] = 0.0; /* reset_inc */
+    X[36] = 0.0; /* reset_inc */
+    X[37] = 0.0; /* reset_inc */
+    X[38] = 0.0; /* reset_inc */
+    X[39] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = t; (void)tr;
+        double n0_0 = 0.0;
+        double n1_0 = 0.0;
+        double n2_0 = 0.0;
+        double n3_0 = 0.0;
+        double n0_1 = 0.0;
+        double n1_1 = 0.0;
+        double n2_1 = 0.0;
+        double n3_1 = 0.0;
+        double n0_2 = 0.0;
+        double n1_2 = 0.0;
+        double n2_2 = 0.0;
+        double n3_2 = 0.0;
+        double n0_3 = 0.0;
+        double n1_3 = 0.0;
+        double n2_3 = 0.0;
+        double n3_3 = 0.0;
+        double n0_4 = 0.0;
+        double n1_4 = 0.0;
+        double n2_4 = 0.0;
+        double n3_4 = 0.0;
+        double n0_5 = 0.0;
+        double n1_5 = 0.0;
+        double n2_5 = 0.0;
+        double n3_5 = 0.0;
+        double n0_6 = 0.0;
+        double n1_6 = 0.0;
+        double n2_6 = 0.0;
+        double n3_6 = 0.0;
+        double n0_7 = 0.0;
+        double n1_7 = 0.0;
+        double n2_7 = 0.0;
+        double n3_7 = 0.0;
+        double n0_8 = 0.0;
+        double n1_8 = 0.0;
+        double n2_8 = 0.0;
+        double n3_8 = 0.0;
+        double n0_9 = 0.0;
+        double n1_9 = 0.0;
+        double n2_9 = 0.0;
+        double n3_9 = 0.0;
+        double stay0 = 0.0;
+        double stay1 = 0.0;
+        double stay2 = 0.0;
+        double stay3 = 0.0;
+        double stay4 = 0.0;
+        double stay5 = 0.0;
+        double stay6 = 0.0;
+        double stay7 = 0.0;
+        double stay8 = 0.0;
+        double stay9 = 0.0;
+        double stay10 = 0.0;
+        double stay11 = 0.0;
+        double stay12 = 0.0;
+        double stay13 = 0.0;
+        double stay14 = 0.0;
+        double stay15 = 0.0;
+        double stay16 = 0.0;
+        double stay17 = 0.0;
+        double stay18 = 0.0;
+        double stay19 = 0.0;
+        double stay20 = 0.0;
+        double stay21 = 0.0;
+        double stay22 = 0.0;
+        double stay23 = 0.0;
+        double stay24 = 0.0;
+        double stay25 = 0.0;
+        double stay26 = 0.0;
+        double stay27 = 0.0;
+        double stay28 = 0.0;
+        double stay29 = 0.0;
+        (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n3_2; (void)n0_3; (void)n1_3; (void)n2_3; (void)n3_3; (void)n0_4; (void)n1_4; (void)n2_4; (void)n3_4; (void)n0_5; (void)n1_5; (void)n2_5; (void)n3_5; (void)n0_6; (void)n1_6; (void)n2_6; (void)n3_6; (void)n0_7; (void)n1_7; (void)n2_7; (void)n3_7; (void)n0_8; (void)n1_8; (void)n2_8; (void)n3_8; (void)n0_9; (void)n1_9; (void)n2_9; (void)n3_9;
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r1_0 = P10;
+        const double r2_0 = P20;
+        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r1_1 = P11;
+        const double r2_1 = P21;
+        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r1_2 = P12;
+        const double r2_2 = P22;
+        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r1_3 = P13;
+        const double r2_3 = P23;
+        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r1_4 = P14;
+        const double r2_4 = P24;
+        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r1_5 = P15;
+        const double r2_5 = P25;
+        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r1_6 = P16;
+        const double r2_6 = P26;
+        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r1_7 = P17;
+        const double r2_7 = P27;
+        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r1_8 = P18;
+        const double r2_8 = P28;
+        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r1_9 = P19;
+        const double r2_9 = P29;
+        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double sd0_0 = P40;
+        const double v0_0 = sd0_0 * sd0_0;
+        const double xi0_0 = (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_0 * xi0_0) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_0 * xi0_0) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
+            n0_0 = cnt[0];
+            stay0 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[10]), p, cnt);
+            n1_0 = cnt[0];
+            stay10 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[20]), p, cnt);
+            n2_0 = cnt[0];
+            stay20 = cnt[1];
+        }
+        n3_0 = plom_poisson_draw(ns, r3_0 * dt);
+        const double sd0_1 = P41;
+        const double v0_1 = sd0_1 * sd0_1;
+        const double xi0_1 = (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_1 * xi0_1) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_1 * xi0_1) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
+            n0_1 = cnt[0];
+            stay1 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_1 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_1 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[11]), p, cnt);
+            n1_1 = cnt[0];
+            stay11 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_1 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_1 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[21]), p, cnt);
+            n2_1 = cnt[0];
+            stay21 = cnt[1];
+        }
+        n3_1 = plom_poisson_draw(ns, r3_1 * dt);
+        const double sd0_2 = P42;
+        const double v0_2 = sd0_2 * sd0_2;
+        const double xi0_2 = (v0_2 * plom_gamma_draw(ns, dt / v0_2)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_2 * xi0_2) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_2 * xi0_2) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[2]), p, cnt);
+            n0_2 = cnt[0];
+            stay2 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_2 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_2 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[12]), p, cnt);
+            n1_2 = cnt[0];
+            stay12 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_2 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_2 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[22]), p, cnt);
+            n2_2 = cnt[0];
+            stay22 = cnt[1];
+        }
+        n3_2 = plom_poisson_draw(ns, r3_2 * dt);
+        const double sd0_3 = P43;
+        const double v0_3 = sd0_3 * sd0_3;
+        const double xi0_3 = (v0_3 * plom_gamma_draw(ns, dt / v0_3)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_3 * xi0_3) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_3 * xi0_3) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[3]), p, cnt);
+            n0_3 = cnt[0];
+            stay3 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_3 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_3 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[13]), p, cnt);
+            n1_3 = cnt[0];
+            stay13 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_3 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_3 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[23]), p, cnt);
+            n2_3 = cnt[0];
+            stay23 = cnt[1];
+        }
+        n3_3 = plom_poisson_draw(ns, r3_3 * dt);
+        const double sd0_4 = P44;
+        const double v0_4 = sd0_4 * sd0_4;
+        const double xi0_4 = (v0_4 * plom_gamma_draw(ns, dt / v0_4)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_4 * xi0_4) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_4 * xi0_4) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[4]), p, cnt);
+            n0_4 = cnt[0];
+            stay4 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_4 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_4 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[14]), p, cnt);
+            n1_4 = cnt[0];
+            stay14 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_4 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_4 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[24]), p, cnt);
+            n2_4 = cnt[0];
+            stay24 = cnt[1];
+        }
+        n3_4 = plom_poisson_draw(ns, r3_4 * dt);
+        const double sd0_5 = P45;
+        const double v0_5 = sd0_5 * sd0_5;
+        const double xi0_5 = (v0_5 * plom_gamma_draw(ns, dt / v0_5)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_5 * xi0_5) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_5 * xi0_5) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[5]), p, cnt);
+            n0_5 = cnt[0];
+            stay5 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_5 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_5 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[15]), p, cnt);
+            n1_5 = cnt[0];
+            stay15 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_5 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_5 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[25]), p, cnt);
+            n2_5 = cnt[0];
+            stay25 = cnt[1];
+        }
+        n3_5 = plom_poisson_draw(ns, r3_5 * dt);
+        const double sd0_6 = P46;
+        const double v0_6 = sd0_6 * sd0_6;
+        const double xi0_6 = (v0_6 * plom_gamma_draw(ns, dt / v0_6)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_6 * xi0_6) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_6 * xi0_6) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[6]), p, cnt);
+            n0_6 = cnt[0];
+            stay6 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_6 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_6 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[16]), p, cnt);
+            n1_6 = cnt[0];
+            stay16 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_6 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_6 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[26]), p, cnt);
+            n2_6 = cnt[0];
+            stay26 = cnt[1];
+        }
+        n3_6 = plom_poisson_draw(ns, r3_6 * dt);
+        const double sd0_7 = P47;
+        const double v0_7 = sd0_7 * sd0_7;
+        const double xi0_7 = (v0_7 * plom_gamma_draw(ns, dt / v0_7)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_7 * xi0_7) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_7 * xi0_7) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[7]), p, cnt);
+            n0_7 = cnt[0];
+            stay7 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_7 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_7 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[17]), p, cnt);
+            n1_7 = cnt[0];
+            stay17 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_7 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_7 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[27]), p, cnt);
+            n2_7 = cnt[0];
+            stay27 = cnt[1];
+        }
+        n3_7 = plom_poisson_draw(ns, r3_7 * dt);
+        const double sd0_8 = P48;
+        const double v0_8 = sd0_8 * sd0_8;
+        const double xi0_8 = (v0_8 * plom_gamma_draw(ns, dt / v0_8)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_8 * xi0_8) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_8 * xi0_8) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[8]), p, cnt);
+            n0_8 = cnt[0];
+            stay8 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_8 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_8 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[18]), p, cnt);
+            n1_8 = cnt[0];
+            stay18 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_8 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_8 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[28]), p, cnt);
+            n2_8 = cnt[0];
+            stay28 = cnt[1];
+        }
+        n3_8 = plom_poisson_draw(ns, r3_8 * dt);
+        const double sd0_9 = P49;
+        const double v0_9 = sd0_9 * sd0_9;
+        const double xi0_9 = (v0_9 * plom_gamma_draw(ns, dt / v0_9)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_9 * xi0_9) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_9 * xi0_9) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[9]), p, cnt);
+            n0_9 = cnt[0];
+            stay9 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_9 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_9 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[19]), p, cnt);
+            n1_9 = cnt[0];
+            stay19 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_9 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_9 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[29]), p, cnt);
+            n2_9 = cnt[0];
+            stay29 = cnt[1];
+        }
+        n3_9 = plom_poisson_draw(ns, r3_9 * dt);
+        X[0] = stay0 + n3_0;
+        X[10] = stay10 + n0_0;
+        X[20] = stay20 + n1_0;
+        X[1] = stay1 + n3_1;
+        X[11] = stay11 + n0_1;
+        X[21] = stay21 + n1_1;
+        X[2] = stay2 + n3_2;
+        X[12] = stay12 + n0_2;
+        X[22] = stay22 + n1_2;
+        X[3] = stay3 + n3_3;
+        X[13] = stay13 + n0_3;
+        X[23] = stay23 + n1_3;
+        X[4] = stay4 + n3_4;
+        X[14] = stay14 + n0_4;
+        X[24] = stay24 + n1_4;
+        X[5] = stay5 + n3_5;
+        X[15] = stay15 + n0_5;
+        X[25] = stay25 + n1_5;
+        X[6] = stay6 + n3_6;
+        X[16] = stay16 + n0_6;
+        X[26] = stay26 + n1_6;
+        X[7] = stay7 + n3_7;
+        X[17] = stay17 + n0_7;
+        X[27] = stay27 + n1_7;
+        X[8] = stay8 + n3_8;
+        X[18] = stay18 + n0_8;
+        X[28] = stay28 + n1_8;
+        X[9] = stay9 + n3_9;
+        X[19] = stay19 + n0_9;
+        X[29] = stay29 + n1_9;
+        X[30] = X[30] + (n1_0);
+        X[31] = X[31] + (n1_1);
+        X[32] = X[32] + (n1_2);
+        X[33] = X[33] + (n1_3);
+        X[34] = X[34] + (n1_4);
+        X[35] = X[35] + (n1_5);
+        X[36] = X[36] + (n1_6);
+        X[37] = X[37] + (n1_7);
+        X[38] = X[38] + (n1_8);
+        X[39] = X[39] + (n1_9);
+        t += dt;
+    }
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    const double P0 = lp[0];
+    const double P1 = lp[1];
+    const double P2 = lp[2];
+    const double P3 = lp[3];
+    const double P4 = lp[4];
+    const double P5 = lp[5];
+    const double P6 = lp[6];
+    const double P7 = lp[7];
+    const double P8 = lp[8];
+    const double P9 = lp[9];
+    const double P10 = lp[10];
+    const double P11 = lp[11];
+    const double P12 = lp[12];
+    const double P13 = lp[13];
+    const double P14 = lp[14];
+    const double P15 = lp[15];
+    const double P16 = lp[16];
+    const double P17 = lp[17];
+    const double P18 = lp[18];
+    const double P19 = lp[19];
+    const double P20 = lp[20];
+    const double P21 = lp[21];
+    const double P22 = lp[22];
+    const double P23 = lp[23];
+    const double P24 = lp[24];
+    const double P25 = lp[25];
+    const double P26 = lp[26];
+    const double P27 = lp[27];
+    const double P28 = lp[28];
+    const double P29 = lp[29];
+    const double P30 = lp[30];
+    const double P31 = lp[31];
+    const double P32 = lp[32];
+    const double P33 = lp[33];
+    const double P34 = lp[34];
+    const double P35 = lp[35];
+    const double P36 = lp[36];
+    const double P37 = lp[37];
+    const double P38 = lp[38];
+    const double P39 = lp[39];
+    const double P70 = PLOM_UNIFORM64(pv[70]);
+    const double P71 = PLOM_UNIFORM64(pv[71]);
+    const double P72 = PLOM_UNIFORM64(pv[72]);
+    const double P73 = PLOM_UNIFORM64(pv[73]);
+    const double P74 = PLOM_UNIFORM64(pv[74]);
+    const double P75 = PLOM_UNIFORM64(pv[75]);
+    const double P76 = PLOM_UNIFORM64(pv[76]);
+    const double P77 = PLOM_UNIFORM64(pv[77]);
+    const double P78 = PLOM_UNIFORM64(pv[78]);
+    const double P79 = PLOM_UNIFORM64(pv[79]);
+    const double P80 = PLOM_UNIFORM64(pv[80]);
+    const double P81 = PLOM_UNIFORM64(pv[81]);
+    const double P82 = PLOM_UNIFORM64(pv[82]);
+    const double P83 = PLOM_UNIFORM64(pv[83]);
+    const double P84 = PLOM_UNIFORM64(pv[84]);
+    const double P85 = PLOM_UNIFORM64(pv[85]);
+    const double P86 = PLOM_UNIFORM64(pv[86]);
+    const double P87 = PLOM_UNIFORM64(pv[87]);
+    const double P88 = PLOM_UNIFORM64(pv[88]);
+    const double P89 = PLOM_UNIFORM64(pv[89]);
+    const double P90 = PLOM_UNIFORM64(pv[90]);
+    const double P91 = PLOM_UNIFORM64(pv[91]);
+    const double P92 = PLOM_UNIFORM64(pv[92]);
+    const double P93 = PLOM_UNIFORM64(pv[93]);
+    const double P94 = PLOM_UNIFORM64(pv[94]);
+    const double P95 = PLOM_UNIFORM64(pv[95]);
+    const double P96 = PLOM_UNIFORM64(pv[96]);
+    const double P97 = PLOM_UNIFORM64(pv[97]);
+    const double P98 = PLOM_UNIFORM64(pv[98]);
+    const double P99 = PLOM_UNIFORM64(pv[99]);
+    const double P100 = PLOM_UNIFORM64(pv[100]);
+    const double P101 = PLOM_UNIFORM64(pv[101]);
+    const double P102 = PLOM_UNIFORM64(pv[102]);
+    const double P103 = PLOM_UNIFORM64(pv[103]);
+    const double P104 = PLOM_UNIFORM64(pv[104]);
+    const double P105 = PLOM_UNIFORM64(pv[105]);
+    const double P106 = PLOM_UNIFORM64(pv[106]);
+    const double P107 = PLOM_UNIFORM64(pv[107]);
+    const double P108 = PLOM_UNIFORM64(pv[108]);
+    const double P109 = PLOM_UNIFORM64(pv[109]);
+    const double P110 = PLOM_UNIFORM64(pv[110]);
+    const double P111 = PLOM_UNIFORM64(pv[111]);
+    const double P112 = PLOM_UNIFORM64(pv[112]);
+    const double P113 = PLOM_UNIFORM64(pv[113]);
+    const double P114 = PLOM_UNIFORM64(pv[114]);
+    const double P115 = PLOM_UNIFORM64(pv[115]);
+    const double P116 = PLOM_UNIFORM64(pv[116]);
+    const double P117 = PLOM_UNIFORM64(pv[117]);
+    const double P118 = PLOM_UNIFORM64(pv[118]);
+    const double P119 = PLOM_UNIFORM64(pv[119]);
+    const double P120 = PLOM_UNIFORM64(pv[120]);
+    const double P121 = PLOM_UNIFORM64(pv[121]);
+    const double P122 = PLOM_UNIFORM64(pv[122]);
+    const double P123 = PLOM_UNIFORM64(pv[123]);
+    const double P124 = PLOM_UNIFORM64(pv[124]);
+    const double P125 = PLOM_UNIFORM64(pv[125]);
+    const double P126 = PLOM_UNIFORM64(pv[126]);
+    const double P127 = PLOM_UNIFORM64(pv[127]);
+    const double P128 = PLOM_UNIFORM64(pv[128]);
+    const double P129 = PLOM_UNIFORM64(pv[129]);
+    const double P130 = PLOM_UNIFORM64(pv[130]);
+    const double P131 = PLOM_UNIFORM64(pv[131]);
+    const double P132 = PLOM_UNIFORM64(pv[132]);
+    const double P133 = PLOM_UNIFORM64(pv[133]);
+    const double P134 = PLOM_UNIFORM64(pv[134]);
+    const double P135 = PLOM_UNIFORM64(pv[135]);
+    const double P136 = PLOM_UNIFORM64(pv[136]);
+    const double P137 = PLOM_UNIFORM64(pv[137]);
+    const double P138 = PLOM_UNIFORM64(pv[138]);
+    const double P139 = PLOM_UNIFORM64(pv[139]);
+    const double P140 = PLOM_UNIFORM64(pv[140]);
+    const double P141 = PLOM_UNIFORM64(pv[141]);
+    const double P142 = PLOM_UNIFORM64(pv[142]);
+    const double P143 = PLOM_UNIFORM64(pv[143]);
+    const double P144 = PLOM_UNIFORM64(pv[144]);
+    const double P145 = PLOM_UNIFORM64(pv[145]);
+    const double P146 = PLOM_UNIFORM64(pv[146]);
+    const double P147 = PLOM_UNIFORM64(pv[147]);
+    const double P148 = PLOM_UNIFORM64(pv[148]);
+    const double P149 = PLOM_UNIFORM64(pv[149]);
+    const double P150 = PLOM_UNIFORM64(pv[150]);
+    const double P151 = PLOM_UNIFORM64(pv[151]);
+    const double P152 = PLOM_UNIFORM64(pv[152]);
+    const double P153 = PLOM_UNIFORM64(pv[153]);
+    const double P154 = PLOM_UNIFORM64(pv[154]);
+    const double P155 = PLOM_UNIFORM64(pv[155]);
+    const double P156 = PLOM_UNIFORM64(pv[156]);
+    const double P157 = PLOM_UNIFORM64(pv[157]);
+    const double P158 = PLOM_UNIFORM64(pv[158]);
+    const double P159 = PLOM_UNIFORM64(pv[159]);
+    const double P160 = PLOM_UNIFORM64(pv[160]);
+    const double P161 = PLOM_UNIFORM64(pv[161]);
+    const double P162 = PLOM_UNIFORM64(pv[162]);
+    const double P163 = PLOM_UNIFORM64(pv[163]);
+    const double P164 = PLOM_UNIFORM64(pv[164]);
+    const double P165 = PLOM_UNIFORM64(pv[165]);
+    const double P166 = PLOM_UNIFORM64(pv[166]);
+    const double P167 = PLOM_UNIFORM64(pv[167]);
+    const double P168 = PLOM_UNIFORM64(pv[168]);
+    const double P169 = PLOM_UNIFORM64(pv[169]);
+    const double P170 = PLOM_UNIFORM64(pv[170]);
+    const double P171 = PLOM_UNIFORM64(pv[171]);
+    const double P172 = PLOM_UNIFORM64(pv[172]);
+    const double P173 = PLOM_UNIFORM64(pv[173]);
+    const double P174 = PLOM_UNIFORM64(pv[174]);
+    const double P175 = PLOM_UNIFORM64(pv[175]);
+    const double P176 = PLOM_UNIFORM64(pv[176]);
+    const double P177 = PLOM_UNIFORM64(pv[177]);
+    const double P178 = PLOM_UNIFORM64(pv[178]);
+    const double P179 = PLOM_UNIFORM64(pv[179]);
+    X[30] = 0.0; /* reset_inc */
+    X[31] = 0.0; /* reset_inc */
+    X[32] = 0.0; /* reset_inc */
+    X[33] = 0.0; /* reset_inc */
+    X[34] = 0.0; /* reset_inc */
+    X[35] = 0.0; /* reset_inc */
+    X[36] = 0.0; /* reset_inc */
+    X[37] = 0.0; /* reset_inc */
+    X[38] = 0.0; /* reset_inc */
+    X[39] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = t; (void)tr;
+        double n0_0 = 0.0;
+        double n1_0 = 0.0;
+        double n2_0 = 0.0;
+        double n3_0 = 0.0;
+        double n0_1 = 0.0;
+        double n1_1 = 0.0;
+        double n2_1 = 0.0;
+        double n3_1 = 0.0;
+        double n0_2 = 0.0;
+        double n1_2 = 0.0;
+        double n2_2 = 0.0;
+        double n3_2 = 0.0;
+        double n0_3 = 0.0;
+        double n1_3 = 0.0;
+        double n2_3 = 0.0;
+        double n3_3 = 0.0;
+        double n0_4 = 0.0;
+        double n1_4 = 0.0;
+        double n2_4 = 0.0;
+        double n3_4 = 0.0;
+        double n0_5 = 0.0;
+        double n1_5 = 0.0;
+        double n2_5 = 0.0;
+        double n3_5 = 0.0;
+        double n0_6 = 0.0;
+        double n1_6 = 0.0;
+        double n2_6 = 0.0;
+        double n3_6 = 0.0;
+        double n0_7 = 0.0;
+        double n1_7 = 0.0;
+        double n2_7 = 0.0;
+        double n3_7 = 0.0;
+        double n0_8 = 0.0;
+        double n1_8 = 0.0;
+        double n2_8 = 0.0;
+        double n3_8 = 0.0;
+        double n0_9 = 0.0;
+        double n1_9 = 0.0;
+        double n2_9 = 0.0;
+        double n3_9 = 0.0;
+        double stay0 = 0.0;
+        double stay1 = 0.0;
+        double stay2 = 0.0;
+        double stay3 = 0.0;
+        double stay4 = 0.0;
+        double stay5 = 0.0;
+        double stay6 = 0.0;
+        double stay7 = 0.0;
+        double stay8 = 0.0;
+        double stay9 = 0.0;
+        double stay10 = 0.0;
+        double stay11 = 0.0;
+        double stay12 = 0.0;
+        double stay13 = 0.0;
+        double stay14 = 0.0;
+        double stay15 = 0.0;
+        double stay16 = 0.0;
+        double stay17 = 0.0;
+        double stay18 = 0.0;
+        double stay19 = 0.0;
+        double stay20 = 0.0;
+        double stay21 = 0.0;
+        double stay22 = 0.0;
+        double stay23 = 0.0;
+        double stay24 = 0.0;
+        double stay25 = 0.0;
+        double stay26 = 0.0;
+        double stay27 = 0.0;
+        double stay28 = 0.0;
+        double stay29 = 0.0;
+        (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n3_2; (void)n0_3; (void)n1_3; (void)n2_3; (void)n3_3; (void)n0_4; (void)n1_4; (void)n2_4; (void)n3_4; (void)n0_5; (void)n1_5; (void)n2_5; (void)n3_5; (void)n0_6; (void)n1_6; (void)n2_6; (void)n3_6; (void)n0_7; (void)n1_7; (void)n2_7; (void)n3_7; (void)n0_8; (void)n1_8; (void)n2_8; (void)n3_8; (void)n0_9; (void)n1_9; (void)n2_9; (void)n3_9;
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r1_0 = P10;
+        const double r2_0 = P20;
+        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r1_1 = P11;
+        const double r2_1 = P21;
+        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r1_2 = P12;
+        const double r2_2 = P22;
+        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r1_3 = P13;
+        const double r2_3 = P23;
+        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r1_4 = P14;
+        const double r2_4 = P24;
+        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r1_5 = P15;
+        const double r2_5 = P25;
+        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r1_6 = P16;
+        const double r2_6 = P26;
+        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r1_7 = P17;
+        const double r2_7 = P27;
+        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r1_8 = P18;
+        const double r2_8 = P28;
+        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r1_9 = P19;
+        const double r2_9 = P29;
+        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double xi0_0 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_0 * xi0_0) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_0 * xi0_0) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
+            n0_0 = cnt[0];
+            stay0 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[10]), p, cnt);
+            n1_0 = cnt[0];
+            stay10 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[20]), p, cnt);
+            n2_0 = cnt[0];
+            stay20 = cnt[1];
+        }
+        n3_0 = plom_poisson_draw(ns, r3_0 * dt);
+        const double xi0_1 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_1 * xi0_1) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_1 * xi0_1) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
+            n0_1 = cnt[0];
+            stay1 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_1 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_1 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[11]), p, cnt);
+            n1_1 = cnt[0];
+            stay11 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_1 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_1 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[21]), p, cnt);
+            n2_1 = cnt[0];
+            stay21 = cnt[1];
+        }
+        n3_1 = plom_poisson_draw(ns, r3_1 * dt);
+        const double xi0_2 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_2 * xi0_2) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_2 * xi0_2) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[2]), p, cnt);
+            n0_2 = cnt[0];
+            stay2 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_2 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_2 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[12]), p, cnt);
+            n1_2 = cnt[0];
+            stay12 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_2 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_2 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[22]), p, cnt);
+            n2_2 = cnt[0];
+            stay22 = cnt[1];
+        }
+        n3_2 = plom_poisson_draw(ns, r3_2 * dt);
+        const double xi0_3 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_3 * xi0_3) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_3 * xi0_3) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[3]), p, cnt);
+            n0_3 = cnt[0];
+            stay3 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_3 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_3 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[13]), p, cnt);
+            n1_3 = cnt[0];
+            stay13 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_3 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_3 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[23]), p, cnt);
+            n2_3 = cnt[0];
+            stay23 = cnt[1];
+        }
+        n3_3 = plom_poisson_draw(ns, r3_3 * dt);
+        const double xi0_4 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_4 * xi0_4) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_4 * xi0_4) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[4]), p, cnt);
+            n0_4 = cnt[0];
+            stay4 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_4 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_4 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[14]), p, cnt);
+            n1_4 = cnt[0];
+            stay14 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_4 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_4 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[24]), p, cnt);
+            n2_4 = cnt[0];
+            stay24 = cnt[1];
+        }
+        n3_4 = plom_poisson_draw(ns, r3_4 * dt);
+        const double xi0_5 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_5 * xi0_5) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_5 * xi0_5) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[5]), p, cnt);
+            n0_5 = cnt[0];
+            stay5 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_5 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_5 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[15]), p, cnt);
+            n1_5 = cnt[0];
+            stay15 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_5 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_5 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[25]), p, cnt);
+            n2_5 = cnt[0];
+            stay25 = cnt[1];
+        }
+        n3_5 = plom_poisson_draw(ns, r3_5 * dt);
+        const double xi0_6 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_6 * xi0_6) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_6 * xi0_6) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[6]), p, cnt);
+            n0_6 = cnt[0];
+            stay6 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_6 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_6 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[16]), p, cnt);
+            n1_6 = cnt[0];
+            stay16 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_6 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_6 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[26]), p, cnt);
+            n2_6 = cnt[0];
+            stay26 = cnt[1];
+        }
+        n3_6 = plom_poisson_draw(ns, r3_6 * dt);
+        const double xi0_7 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_7 * xi0_7) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_7 * xi0_7) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[7]), p, cnt);
+            n0_7 = cnt[0];
+            stay7 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_7 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_7 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[17]), p, cnt);
+            n1_7 = cnt[0];
+            stay17 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_7 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_7 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[27]), p, cnt);
+            n2_7 = cnt[0];
+            stay27 = cnt[1];
+        }
+        n3_7 = plom_poisson_draw(ns, r3_7 * dt);
+        const double xi0_8 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_8 * xi0_8) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_8 * xi0_8) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[8]), p, cnt);
+            n0_8 = cnt[0];
+            stay8 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_8 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_8 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[18]), p, cnt);
+            n1_8 = cnt[0];
+            stay18 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_8 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_8 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[28]), p, cnt);
+            n2_8 = cnt[0];
+            stay28 = cnt[1];
+        }
+        n3_8 = plom_poisson_draw(ns, r3_8 * dt);
+        const double xi0_9 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_9 * xi0_9) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_9 * xi0_9) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[9]), p, cnt);
+            n0_9 = cnt[0];
+            stay9 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_9 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_9 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[19]), p, cnt);
+            n1_9 = cnt[0];
+            stay19 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r2_9 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r2_9 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[29]), p, cnt);
+            n2_9 = cnt[0];
+            stay29 = cnt[1];
+        }
+        n3_9 = plom_poisson_draw(ns, r3_9 * dt);
+        X[0] = stay0 + n3_0;
+        X[10] = stay10 + n0_0;
+        X[20] = stay20 + n1_0;
+        X[1] = stay1 + n3_1;
+        X[11] = stay11 + n0_1;
+        X[21] = stay21 + n1_1;
+        X[2] = stay2 + n3_2;
+        X[12] = stay12 + n0_2;
+        X[22] = stay22 + n1_2;
+        X[3] = stay3 + n3_3;
+        X[13] = stay13 + n0_3;
+        X[23] = stay23 + n1_3;
+        X[4] = stay4 + n3_4;
+        X[14] = stay14 + n0_4;
+        X[24] = stay24 + n1_4;
+        X[5] = stay5 + n3_5;
+        X[15] = stay15 + n0_5;
+        X[25] = stay25 + n1_5;
+        X[6] = stay6 + n3_6;
+        X[16] = stay16 + n0_6;
+        X[26] = stay26 + n1_6;
+        X[7] = stay7 + n3_7;
+        X[17] = stay17 + n0_7;
+        X[27] = stay27 + n1_7;
+        X[8] = stay8 + n3_8;
+        X[18] = stay18 + n0_8;
+        X[28] = stay28 + n1_8;
+        X[9] = stay9 + n3_9;
+        X[19] = stay19 + n0_9;
+        X[29] = stay29 + n1_9;
+        X[30] = X[30] + (n1_0);
+        X[31] = X[31] + (n1_1);
+        X[32] = X[32] + (n1_2);
+        X[33] = X[33] + (n1_3);
+        X[34] = X[34] + (n1_4);
+        X[35] = X[35] + (n1_5);
+        X[36] = X[36] + (n1_6);
+        X[37] = X[37] + (n1_7);
+        X[38] = X[38] + (n1_8);
+        X[39] = X[39] + (n1_9);
+        t += dt;
+    }
 }
 
 static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
@@ -6266,6 +9499,8 @@ struct plom_model_seir10 {
         else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
         else if constexpr (V == 2) plom_seir10_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
         else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 4) plom_seir10_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 5) plom_seir10_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
     }
     template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
     {
@@ -6273,6 +9508,8 @@ struct plom_model_seir10 {
         else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
         else if constexpr (V == 2) plom_seir10_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
         else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 4) plom_seir10_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 5) plom_seir10_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, double *obs) { plom_seir10_proj2obs_f64(X, pvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, double *obs) { plom_seir10_proj2obs_pp_f64(X, pvd, lpd, obs); }

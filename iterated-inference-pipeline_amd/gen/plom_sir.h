@@ -172,8 +172,123 @@ PLOM_HD void plom_sir_hat_values(const double *X, const double *pvd, double t, d
     plom_sir_obs_moments(obs, pvd, t, out + 3, var);
 }
 
+PLOM_HD void plom_sir_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    (void)sqdt; (void)ns; (void)pv;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    X[2] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = t; (void)tr;
+        double n0_0 = 0.0;
+        double n1_0 = 0.0;
+        double stay0 = 0.0;
+        double stay1 = 0.0;
+        (void)n0_0; (void)n1_0;
+        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r1_0 = P1;
+        {
+            double p[2], cnt[2];
+            const double sum = r0_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r0_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
+            n0_0 = cnt[0];
+            stay0 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
+            n1_0 = cnt[0];
+            stay1 = cnt[1];
+        }
+        X[0] = stay0;
+        X[1] = stay1 + n0_0;
+        X[2] = X[2] + (n0_0);
+        t += dt;
+    }
+}
+
+PLOM_HD void plom_sir_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    (void)sqdt; (void)ns; (void)pv;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    X[2] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = t; (void)tr;
+        double n0_0 = 0.0;
+        double n1_0 = 0.0;
+        double stay0 = 0.0;
+        double stay1 = 0.0;
+        (void)n0_0; (void)n1_0;
+        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r1_0 = P1;
+        {
+            double p[2], cnt[2];
+            const double sum = r0_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r0_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
+            n0_0 = cnt[0];
+            stay0 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
+            n1_0 = cnt[0];
+            stay1 = cnt[1];
+        }
+        X[0] = stay0;
+        X[1] = stay1 + n0_0;
+        X[2] = X[2] + (n0_0);
+        t += dt;
+    }
+}
+
 PLOM_HD void plom_sir_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off)
 {
+    if (noises_off & PLOM_IMPL_PSR) {
+        if (noises_off & PLOM_NO_ENV_STO) plom_sir_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
+        else plom_sir_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);
+        return;
+    }
     const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);
     if (v == 0u) plom_sir_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt);
     else if (v == PLOM_NO_DEM_STO) plom_sir_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
@@ -298,6 +413,116 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_proj2obs_f
 {
     (void)pvd; (void)X;
     obs[0] = (double)X[2];
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    const double P0 = lp[0];
+    const double P1 = lp[1];
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    X[2] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = t; (void)tr;
+        double n0_0 = 0.0;
+        double n1_0 = 0.0;
+        double stay0 = 0.0;
+        double stay1 = 0.0;
+        (void)n0_0; (void)n1_0;
+        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r1_0 = P1;
+        {
+            double p[2], cnt[2];
+            const double sum = r0_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r0_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
+            n0_0 = cnt[0];
+            stay0 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
+            n1_0 = cnt[0];
+            stay1 = cnt[1];
+        }
+        X[0] = stay0;
+        X[1] = stay1 + n0_0;
+        X[2] = X[2] + (n0_0);
+        t += dt;
+    }
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    const double P0 = lp[0];
+    const double P1 = lp[1];
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    X[2] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = t; (void)tr;
+        double n0_0 = 0.0;
+        double n1_0 = 0.0;
+        double stay0 = 0.0;
+        double stay1 = 0.0;
+        (void)n0_0; (void)n1_0;
+        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r1_0 = P1;
+        {
+            double p[2], cnt[2];
+            const double sum = r0_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r0_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
+            n0_0 = cnt[0];
+            stay0 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
+            n1_0 = cnt[0];
+            stay1 = cnt[1];
+        }
+        X[0] = stay0;
+        X[1] = stay1 + n0_0;
+        X[2] = X[2] + (n0_0);
+        t += dt;
+    }
 }
 
 static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
@@ -562,6 +787,8 @@ struct plom_model_sir {
         else if constexpr (V == 1) plom_sir_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
         else if constexpr (V == 2) plom_sir_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
         else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 4) plom_sir_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 5) plom_sir_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
     }
     template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
     {
@@ -569,6 +796,8 @@ struct plom_model_sir {
         else if constexpr (V == 1) plom_sir_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
         else if constexpr (V == 2) plom_sir_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
         else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 4) plom_sir_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 5) plom_sir_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, double *obs) { plom_sir_proj2obs_f64(X, pvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, double *obs) { plom_sir_proj2obs_pp_f64(X, pvd, lpd, obs); }

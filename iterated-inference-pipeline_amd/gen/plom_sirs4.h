@@ -574,8 +574,475 @@ PLOM_HD void plom_sirs4_hat_values(const double *X, const double *pvd, double t,
     plom_sirs4_obs_moments(obs, pvd, t, out + 12, var);
 }
 
+PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    (void)sqdt; (void)ns; (void)pv;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P2 = PLOM_UNIFORM64(pv[2]);
+    const double P3 = PLOM_UNIFORM64(pv[3]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    const double P5 = PLOM_UNIFORM64(pv[5]);
+    const double P6 = PLOM_UNIFORM64(pv[6]);
+    const double P7 = PLOM_UNIFORM64(pv[7]);
+    const double P8 = PLOM_UNIFORM64(pv[8]);
+    const double P9 = PLOM_UNIFORM64(pv[9]);
+    const double P10 = PLOM_UNIFORM64(pv[10]);
+    const double P11 = PLOM_UNIFORM64(pv[11]);
+    const double P12 = PLOM_UNIFORM64(pv[12]);
+    const double P13 = PLOM_UNIFORM64(pv[13]);
+    const double P14 = PLOM_UNIFORM64(pv[14]);
+    const double P15 = PLOM_UNIFORM64(pv[15]);
+    const double P16 = PLOM_UNIFORM64(pv[16]);
+    const double P17 = PLOM_UNIFORM64(pv[17]);
+    const double P18 = PLOM_UNIFORM64(pv[18]);
+    const double P19 = PLOM_UNIFORM64(pv[19]);
+    const double P28 = PLOM_UNIFORM64(pv[28]);
+    const double P29 = PLOM_UNIFORM64(pv[29]);
+    const double P30 = PLOM_UNIFORM64(pv[30]);
+    const double P31 = PLOM_UNIFORM64(pv[31]);
+    const double P32 = PLOM_UNIFORM64(pv[32]);
+    const double P33 = PLOM_UNIFORM64(pv[33]);
+    const double P34 = PLOM_UNIFORM64(pv[34]);
+    const double P35 = PLOM_UNIFORM64(pv[35]);
+    const double P36 = PLOM_UNIFORM64(pv[36]);
+    const double P37 = PLOM_UNIFORM64(pv[37]);
+    const double P38 = PLOM_UNIFORM64(pv[38]);
+    const double P39 = PLOM_UNIFORM64(pv[39]);
+    const double P40 = PLOM_UNIFORM64(pv[40]);
+    const double P41 = PLOM_UNIFORM64(pv[41]);
+    const double P42 = PLOM_UNIFORM64(pv[42]);
+    const double P43 = PLOM_UNIFORM64(pv[43]);
+    const double P44 = PLOM_UNIFORM64(pv[44]);
+    const double P45 = PLOM_UNIFORM64(pv[45]);
+    const double P46 = PLOM_UNIFORM64(pv[46]);
+    const double P47 = PLOM_UNIFORM64(pv[47]);
+    X[8] = 0.0; /* reset_inc */
+    X[9] = 0.0; /* reset_inc */
+    X[10] = 0.0; /* reset_inc */
+    X[11] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = t; (void)tr;
+        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        double n0_0 = 0.0;
+        double n1_0 = 0.0;
+        double n2_0 = 0.0;
+        double n0_1 = 0.0;
+        double n1_1 = 0.0;
+        double n2_1 = 0.0;
+        double n0_2 = 0.0;
+        double n1_2 = 0.0;
+        double n2_2 = 0.0;
+        double n0_3 = 0.0;
+        double n1_3 = 0.0;
+        double n2_3 = 0.0;
+        double stay0 = 0.0;
+        double stay1 = 0.0;
+        double stay2 = 0.0;
+        double stay3 = 0.0;
+        double stay4 = 0.0;
+        double stay5 = 0.0;
+        double stay6 = 0.0;
+        double stay7 = 0.0;
+        (void)n0_0; (void)n1_0; (void)n2_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n0_3; (void)n1_3; (void)n2_3;
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r1_0 = P8;
+        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r1_1 = P9;
+        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r1_2 = P10;
+        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r1_3 = P11;
+        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double sd0_0 = P16;
+        const double v0_0 = sd0_0 * sd0_0;
+        const double xi0_0 = (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_0 * xi0_0) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_0 * xi0_0) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
+            n0_0 = cnt[0];
+            stay0 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[4]), p, cnt);
+            n1_0 = cnt[0];
+            stay4 = cnt[1];
+        }
+        n2_0 = plom_poisson_draw(ns, r2_0 * dt);
+        const double sd0_1 = P17;
+        const double v0_1 = sd0_1 * sd0_1;
+        const double xi0_1 = (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_1 * xi0_1) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_1 * xi0_1) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
+            n0_1 = cnt[0];
+            stay1 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_1 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_1 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[5]), p, cnt);
+            n1_1 = cnt[0];
+            stay5 = cnt[1];
+        }
+        n2_1 = plom_poisson_draw(ns, r2_1 * dt);
+        const double sd0_2 = P18;
+        const double v0_2 = sd0_2 * sd0_2;
+        const double xi0_2 = (v0_2 * plom_gamma_draw(ns, dt / v0_2)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_2 * xi0_2) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_2 * xi0_2) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[2]), p, cnt);
+            n0_2 = cnt[0];
+            stay2 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_2 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_2 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[6]), p, cnt);
+            n1_2 = cnt[0];
+            stay6 = cnt[1];
+        }
+        n2_2 = plom_poisson_draw(ns, r2_2 * dt);
+        const double sd0_3 = P19;
+        const double v0_3 = sd0_3 * sd0_3;
+        const double xi0_3 = (v0_3 * plom_gamma_draw(ns, dt / v0_3)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_3 * xi0_3) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_3 * xi0_3) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[3]), p, cnt);
+            n0_3 = cnt[0];
+            stay3 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_3 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_3 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[7]), p, cnt);
+            n1_3 = cnt[0];
+            stay7 = cnt[1];
+        }
+        n2_3 = plom_poisson_draw(ns, r2_3 * dt);
+        X[0] = stay0 + n2_0;
+        X[4] = stay4 + n0_0;
+        X[1] = stay1 + n2_1;
+        X[5] = stay5 + n0_1;
+        X[2] = stay2 + n2_2;
+        X[6] = stay6 + n0_2;
+        X[3] = stay3 + n2_3;
+        X[7] = stay7 + n0_3;
+        X[8] = X[8] + (n0_0);
+        X[9] = X[9] + (n0_1);
+        X[10] = X[10] + (n0_2);
+        X[11] = X[11] + (n0_3);
+        t += dt;
+    }
+}
+
+PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    (void)sqdt; (void)ns; (void)pv;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P2 = PLOM_UNIFORM64(pv[2]);
+    const double P3 = PLOM_UNIFORM64(pv[3]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    const double P5 = PLOM_UNIFORM64(pv[5]);
+    const double P6 = PLOM_UNIFORM64(pv[6]);
+    const double P7 = PLOM_UNIFORM64(pv[7]);
+    const double P8 = PLOM_UNIFORM64(pv[8]);
+    const double P9 = PLOM_UNIFORM64(pv[9]);
+    const double P10 = PLOM_UNIFORM64(pv[10]);
+    const double P11 = PLOM_UNIFORM64(pv[11]);
+    const double P12 = PLOM_UNIFORM64(pv[12]);
+    const double P13 = PLOM_UNIFORM64(pv[13]);
+    const double P14 = PLOM_UNIFORM64(pv[14]);
+    const double P15 = PLOM_UNIFORM64(pv[15]);
+    const double P28 = PLOM_UNIFORM64(pv[28]);
+    const double P29 = PLOM_UNIFORM64(pv[29]);
+    const double P30 = PLOM_UNIFORM64(pv[30]);
+    const double P31 = PLOM_UNIFORM64(pv[31]);
+    const double P32 = PLOM_UNIFORM64(pv[32]);
+    const double P33 = PLOM_UNIFORM64(pv[33]);
+    const double P34 = PLOM_UNIFORM64(pv[34]);
+    const double P35 = PLOM_UNIFORM64(pv[35]);
+    const double P36 = PLOM_UNIFORM64(pv[36]);
+    const double P37 = PLOM_UNIFORM64(pv[37]);
+    const double P38 = PLOM_UNIFORM64(pv[38]);
+    const double P39 = PLOM_UNIFORM64(pv[39]);
+    const double P40 = PLOM_UNIFORM64(pv[40]);
+    const double P41 = PLOM_UNIFORM64(pv[41]);
+    const double P42 = PLOM_UNIFORM64(pv[42]);
+    const double P43 = PLOM_UNIFORM64(pv[43]);
+    const double P44 = PLOM_UNIFORM64(pv[44]);
+    const double P45 = PLOM_UNIFORM64(pv[45]);
+    const double P46 = PLOM_UNIFORM64(pv[46]);
+    const double P47 = PLOM_UNIFORM64(pv[47]);
+    X[8] = 0.0; /* reset_inc */
+    X[9] = 0.0; /* reset_inc */
+    X[10] = 0.0; /* reset_inc */
+    X[11] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = t; (void)tr;
+        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        double n0_0 = 0.0;
+        double n1_0 = 0.0;
+        double n2_0 = 0.0;
+        double n0_1 = 0.0;
+        double n1_1 = 0.0;
+        double n2_1 = 0.0;
+        double n0_2 = 0.0;
+        double n1_2 = 0.0;
+        double n2_2 = 0.0;
+        double n0_3 = 0.0;
+        double n1_3 = 0.0;
+        double n2_3 = 0.0;
+        double stay0 = 0.0;
+        double stay1 = 0.0;
+        double stay2 = 0.0;
+        double stay3 = 0.0;
+        double stay4 = 0.0;
+        double stay5 = 0.0;
+        double stay6 = 0.0;
+        double stay7 = 0.0;
+        (void)n0_0; (void)n1_0; (void)n2_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n0_3; (void)n1_3; (void)n2_3;
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r1_0 = P8;
+        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r1_1 = P9;
+        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r1_2 = P10;
+        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r1_3 = P11;
+        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double xi0_0 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_0 * xi0_0) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_0 * xi0_0) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
+            n0_0 = cnt[0];
+            stay0 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[4]), p, cnt);
+            n1_0 = cnt[0];
+            stay4 = cnt[1];
+        }
+        n2_0 = plom_poisson_draw(ns, r2_0 * dt);
+        const double xi0_1 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_1 * xi0_1) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_1 * xi0_1) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
+            n0_1 = cnt[0];
+            stay1 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_1 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_1 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[5]), p, cnt);
+            n1_1 = cnt[0];
+            stay5 = cnt[1];
+        }
+        n2_1 = plom_poisson_draw(ns, r2_1 * dt);
+        const double xi0_2 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_2 * xi0_2) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_2 * xi0_2) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[2]), p, cnt);
+            n0_2 = cnt[0];
+            stay2 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_2 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_2 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[6]), p, cnt);
+            n1_2 = cnt[0];
+            stay6 = cnt[1];
+        }
+        n2_2 = plom_poisson_draw(ns, r2_2 * dt);
+        const double xi0_3 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_3 * xi0_3) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_3 * xi0_3) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[3]), p, cnt);
+            n0_3 = cnt[0];
+            stay3 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_3 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_3 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[7]), p, cnt);
+            n1_3 = cnt[0];
+            stay7 = cnt[1];
+        }
+        n2_3 = plom_poisson_draw(ns, r2_3 * dt);
+        X[0] = stay0 + n2_0;
+        X[4] = stay4 + n0_0;
+        X[1] = stay1 + n2_1;
+        X[5] = stay5 + n0_1;
+        X[2] = stay2 + n2_2;
+        X[6] = stay6 + n0_2;
+        X[3] = stay3 + n2_3;
+        X[7] = stay7 + n0_3;
+        X[8] = X[8] + (n0_0);
+        X[9] = X[9] + (n0_1);
+        X[10] = X[10] + (n0_2);
+        X[11] = X[11] + (n0_3);
+        t += dt;
+    }
+}
+
 PLOM_HD void plom_sirs4_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off)
 {
+    if (noises_off & PLOM_IMPL_PSR) {
+        if (noises_off & PLOM_NO_ENV_STO) plom_sirs4_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
+        else plom_sirs4_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);
+        return;
+    }
     const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);
     if (v == 0u) plom_sirs4_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt);
     else if (v == PLOM_NO_DEM_STO) plom_sirs4_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
@@ -1051,6 +1518,468 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_proj2obs
     obs[1] = (double)X[9];
     obs[2] = (double)X[10];
     obs[3] = (double)X[11];
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    const double P0 = lp[0];
+    const double P1 = lp[1];
+    const double P2 = lp[2];
+    const double P3 = lp[3];
+    const double P4 = lp[4];
+    const double P5 = lp[5];
+    const double P6 = lp[6];
+    const double P7 = lp[7];
+    const double P8 = lp[8];
+    const double P9 = lp[9];
+    const double P10 = lp[10];
+    const double P11 = lp[11];
+    const double P12 = lp[12];
+    const double P13 = lp[13];
+    const double P14 = lp[14];
+    const double P15 = lp[15];
+    const double P16 = lp[16];
+    const double P17 = lp[17];
+    const double P18 = lp[18];
+    const double P19 = lp[19];
+    const double P28 = PLOM_UNIFORM64(pv[28]);
+    const double P29 = PLOM_UNIFORM64(pv[29]);
+    const double P30 = PLOM_UNIFORM64(pv[30]);
+    const double P31 = PLOM_UNIFORM64(pv[31]);
+    const double P32 = PLOM_UNIFORM64(pv[32]);
+    const double P33 = PLOM_UNIFORM64(pv[33]);
+    const double P34 = PLOM_UNIFORM64(pv[34]);
+    const double P35 = PLOM_UNIFORM64(pv[35]);
+    const double P36 = PLOM_UNIFORM64(pv[36]);
+    const double P37 = PLOM_UNIFORM64(pv[37]);
+    const double P38 = PLOM_UNIFORM64(pv[38]);
+    const double P39 = PLOM_UNIFORM64(pv[39]);
+    const double P40 = PLOM_UNIFORM64(pv[40]);
+    const double P41 = PLOM_UNIFORM64(pv[41]);
+    const double P42 = PLOM_UNIFORM64(pv[42]);
+    const double P43 = PLOM_UNIFORM64(pv[43]);
+    const double P44 = PLOM_UNIFORM64(pv[44]);
+    const double P45 = PLOM_UNIFORM64(pv[45]);
+    const double P46 = PLOM_UNIFORM64(pv[46]);
+    const double P47 = PLOM_UNIFORM64(pv[47]);
+    X[8] = 0.0; /* reset_inc */
+    X[9] = 0.0; /* reset_inc */
+    X[10] = 0.0; /* reset_inc */
+    X[11] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = t; (void)tr;
+        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        double n0_0 = 0.0;
+        double n1_0 = 0.0;
+        double n2_0 = 0.0;
+        double n0_1 = 0.0;
+        double n1_1 = 0.0;
+        double n2_1 = 0.0;
+        double n0_2 = 0.0;
+        double n1_2 = 0.0;
+        double n2_2 = 0.0;
+        double n0_3 = 0.0;
+        double n1_3 = 0.0;
+        double n2_3 = 0.0;
+        double stay0 = 0.0;
+        double stay1 = 0.0;
+        double stay2 = 0.0;
+        double stay3 = 0.0;
+        double stay4 = 0.0;
+        double stay5 = 0.0;
+        double stay6 = 0.0;
+        double stay7 = 0.0;
+        (void)n0_0; (void)n1_0; (void)n2_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n0_3; (void)n1_3; (void)n2_3;
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r1_0 = P8;
+        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r1_1 = P9;
+        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r1_2 = P10;
+        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r1_3 = P11;
+        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double sd0_0 = P16;
+        const double v0_0 = sd0_0 * sd0_0;
+        const double xi0_0 = (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_0 * xi0_0) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_0 * xi0_0) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
+            n0_0 = cnt[0];
+            stay0 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[4]), p, cnt);
+            n1_0 = cnt[0];
+            stay4 = cnt[1];
+        }
+        n2_0 = plom_poisson_draw(ns, r2_0 * dt);
+        const double sd0_1 = P17;
+        const double v0_1 = sd0_1 * sd0_1;
+        const double xi0_1 = (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_1 * xi0_1) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_1 * xi0_1) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
+            n0_1 = cnt[0];
+            stay1 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_1 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_1 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[5]), p, cnt);
+            n1_1 = cnt[0];
+            stay5 = cnt[1];
+        }
+        n2_1 = plom_poisson_draw(ns, r2_1 * dt);
+        const double sd0_2 = P18;
+        const double v0_2 = sd0_2 * sd0_2;
+        const double xi0_2 = (v0_2 * plom_gamma_draw(ns, dt / v0_2)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_2 * xi0_2) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_2 * xi0_2) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[2]), p, cnt);
+            n0_2 = cnt[0];
+            stay2 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_2 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_2 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[6]), p, cnt);
+            n1_2 = cnt[0];
+            stay6 = cnt[1];
+        }
+        n2_2 = plom_poisson_draw(ns, r2_2 * dt);
+        const double sd0_3 = P19;
+        const double v0_3 = sd0_3 * sd0_3;
+        const double xi0_3 = (v0_3 * plom_gamma_draw(ns, dt / v0_3)) / dt;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_3 * xi0_3) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_3 * xi0_3) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[3]), p, cnt);
+            n0_3 = cnt[0];
+            stay3 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_3 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_3 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[7]), p, cnt);
+            n1_3 = cnt[0];
+            stay7 = cnt[1];
+        }
+        n2_3 = plom_poisson_draw(ns, r2_3 * dt);
+        X[0] = stay0 + n2_0;
+        X[4] = stay4 + n0_0;
+        X[1] = stay1 + n2_1;
+        X[5] = stay5 + n0_1;
+        X[2] = stay2 + n2_2;
+        X[6] = stay6 + n0_2;
+        X[3] = stay3 + n2_3;
+        X[7] = stay7 + n0_3;
+        X[8] = X[8] + (n0_0);
+        X[9] = X[9] + (n0_1);
+        X[10] = X[10] + (n0_2);
+        X[11] = X[11] + (n0_3);
+        t += dt;
+    }
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+{
+    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    const double P0 = lp[0];
+    const double P1 = lp[1];
+    const double P2 = lp[2];
+    const double P3 = lp[3];
+    const double P4 = lp[4];
+    const double P5 = lp[5];
+    const double P6 = lp[6];
+    const double P7 = lp[7];
+    const double P8 = lp[8];
+    const double P9 = lp[9];
+    const double P10 = lp[10];
+    const double P11 = lp[11];
+    const double P12 = lp[12];
+    const double P13 = lp[13];
+    const double P14 = lp[14];
+    const double P15 = lp[15];
+    const double P28 = PLOM_UNIFORM64(pv[28]);
+    const double P29 = PLOM_UNIFORM64(pv[29]);
+    const double P30 = PLOM_UNIFORM64(pv[30]);
+    const double P31 = PLOM_UNIFORM64(pv[31]);
+    const double P32 = PLOM_UNIFORM64(pv[32]);
+    const double P33 = PLOM_UNIFORM64(pv[33]);
+    const double P34 = PLOM_UNIFORM64(pv[34]);
+    const double P35 = PLOM_UNIFORM64(pv[35]);
+    const double P36 = PLOM_UNIFORM64(pv[36]);
+    const double P37 = PLOM_UNIFORM64(pv[37]);
+    const double P38 = PLOM_UNIFORM64(pv[38]);
+    const double P39 = PLOM_UNIFORM64(pv[39]);
+    const double P40 = PLOM_UNIFORM64(pv[40]);
+    const double P41 = PLOM_UNIFORM64(pv[41]);
+    const double P42 = PLOM_UNIFORM64(pv[42]);
+    const double P43 = PLOM_UNIFORM64(pv[43]);
+    const double P44 = PLOM_UNIFORM64(pv[44]);
+    const double P45 = PLOM_UNIFORM64(pv[45]);
+    const double P46 = PLOM_UNIFORM64(pv[46]);
+    const double P47 = PLOM_UNIFORM64(pv[47]);
+    X[8] = 0.0; /* reset_inc */
+    X[9] = 0.0; /* reset_inc */
+    X[10] = 0.0; /* reset_inc */
+    X[11] = 0.0; /* reset_inc */
+    double t = t0;
+    const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
+    for (int k = 0; k < K; k++) {
+        const double tr = t; (void)tr;
+        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        double n0_0 = 0.0;
+        double n1_0 = 0.0;
+        double n2_0 = 0.0;
+        double n0_1 = 0.0;
+        double n1_1 = 0.0;
+        double n2_1 = 0.0;
+        double n0_2 = 0.0;
+        double n1_2 = 0.0;
+        double n2_2 = 0.0;
+        double n0_3 = 0.0;
+        double n1_3 = 0.0;
+        double n2_3 = 0.0;
+        double stay0 = 0.0;
+        double stay1 = 0.0;
+        double stay2 = 0.0;
+        double stay3 = 0.0;
+        double stay4 = 0.0;
+        double stay5 = 0.0;
+        double stay6 = 0.0;
+        double stay7 = 0.0;
+        (void)n0_0; (void)n1_0; (void)n2_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n0_3; (void)n1_3; (void)n2_3;
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r1_0 = P8;
+        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r1_1 = P9;
+        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r1_2 = P10;
+        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r1_3 = P11;
+        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double xi0_0 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_0 * xi0_0) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_0 * xi0_0) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
+            n0_0 = cnt[0];
+            stay0 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_0 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[4]), p, cnt);
+            n1_0 = cnt[0];
+            stay4 = cnt[1];
+        }
+        n2_0 = plom_poisson_draw(ns, r2_0 * dt);
+        const double xi0_1 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_1 * xi0_1) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_1 * xi0_1) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
+            n0_1 = cnt[0];
+            stay1 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_1 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_1 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[5]), p, cnt);
+            n1_1 = cnt[0];
+            stay5 = cnt[1];
+        }
+        n2_1 = plom_poisson_draw(ns, r2_1 * dt);
+        const double xi0_2 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_2 * xi0_2) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_2 * xi0_2) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[2]), p, cnt);
+            n0_2 = cnt[0];
+            stay2 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_2 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_2 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[6]), p, cnt);
+            n1_2 = cnt[0];
+            stay6 = cnt[1];
+        }
+        n2_2 = plom_poisson_draw(ns, r2_2 * dt);
+        const double xi0_3 = 1.0;
+        {
+            double p[2], cnt[2];
+            const double sum = (r0_3 * xi0_3) * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * (((r0_3 * xi0_3) * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[3]), p, cnt);
+            n0_3 = cnt[0];
+            stay3 = cnt[1];
+        }
+        {
+            double p[2], cnt[2];
+            const double sum = r1_3 * dt;
+            if (sum > 0.0) {
+                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
+                p[0] = one_minus_exp_sum * ((r1_3 * dt) / sum);
+                p[1] = 1.0 - p[0];
+            } else {
+                p[0] = 0.0;
+                p[1] = 1.0;
+            }
+            plom_multinomial(ns, 2, plom_floor(X[7]), p, cnt);
+            n1_3 = cnt[0];
+            stay7 = cnt[1];
+        }
+        n2_3 = plom_poisson_draw(ns, r2_3 * dt);
+        X[0] = stay0 + n2_0;
+        X[4] = stay4 + n0_0;
+        X[1] = stay1 + n2_1;
+        X[5] = stay5 + n0_1;
+        X[2] = stay2 + n2_2;
+        X[6] = stay6 + n0_2;
+        X[3] = stay3 + n2_3;
+        X[7] = stay7 + n0_3;
+        X[8] = X[8] + (n0_0);
+        X[9] = X[9] + (n0_1);
+        X[10] = X[10] + (n0_2);
+        X[11] = X[11] + (n0_3);
+        t += dt;
+    }
 }
 
 static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
@@ -2044,6 +2973,8 @@ struct plom_model_sirs4 {
         else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
         else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
         else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 4) plom_sirs4_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 5) plom_sirs4_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
     }
     template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
     {
@@ -2051,6 +2982,8 @@ struct plom_model_sirs4 {
         else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
         else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
         else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 4) plom_sirs4_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
+        else if constexpr (V == 5) plom_sirs4_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, double *obs) { plom_sirs4_proj2obs_f64(X, pvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, double *obs) { plom_sirs4_proj2obs_pp_f64(X, pvd, lpd, obs); }

@@ -20,7 +20,7 @@ from .codegen import load_model, BUILTIN_MODELS, context_data, load_json
 
 PLOM_FP64, PLOM_FP32 = 0, 1
 NO_DEM_STO, NO_ENV_STO, NO_DRIFT = 1, 2, 4
-FLAG_NO_FILTER, FLAG_STORE_ANCESTRY, FLAG_FINAL_STATE, FLAG_LINEAGE, FLAG_HAT, FLAG_MIF = 1, 2, 4, 8, 16, 32
+FLAG_NO_FILTER, FLAG_STORE_ANCESTRY, FLAG_FINAL_STATE, FLAG_LINEAGE, FLAG_HAT, FLAG_MIF, FLAG_PSR = 1, 2, 4, 8, 16, 32, 64
 MAX_J = 1 << 21
 
 
@@ -439,13 +439,14 @@ class Server:
         self.N = {}
 
     def submit(self, model, J, times, data, pv, x0, precision="fp64", noises_off=0, seed=1, stream=0,
-               dt=0.25, like_min=1e-17, lineage=False):
+               dt=0.25, like_min=1e-17, lineage=False, flags=0):
         times = np.ascontiguousarray(times, dtype=np.float64)
         N = len(times) - 1
         data = np.ascontiguousarray(data, dtype=np.float64).reshape(N, model.n_ts)
         pv = np.ascontiguousarray(pv, dtype=np.float64)
         x0 = np.ascontiguousarray(x0, dtype=np.float64)
-        d = _desc(model, J, N, precision, noises_off, FLAG_LINEAGE if lineage else 0, seed, stream, dt, like_min)
+        d = _desc(model, J, N, precision, noises_off, (FLAG_LINEAGE if lineage else 0) | int(flags), seed, stream, dt,
+                  like_min)
         inp = FilterInputs(dptr(times), dptr(data), dptr(pv), dptr(x0), None)
         rid = C.c_uint64()
         check(lib().plom_server_submit(self.h, C.byref(d), C.byref(inp), C.byref(rid)))

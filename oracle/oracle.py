@@ -57,12 +57,17 @@ def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int32)) if a is not None else None
 
 
+IMPL_PSR = 0x100   # the psr implementation (PLOM_IMPL_PSR of csrc/plom_model_common.h)
+
+
 def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=0, dt=0.25,
         like_min=1e-17, mode=CANONICAL, nthreads=1, ancestry=False, final_state=False, logw=False,
-        lineage=False, all_states=False, hat=False):
+        lineage=False, all_states=False, hat=False, psr=False):
     """The filter restated on the CPU.  lineage: the sampled path (slots [N], states
     [N+1][S]); all_states: every particle's state after every interval, before resampling
-    ([N+1][J][S], for tests on small runs)."""
+    ([N+1][J][S], for tests on small runs); psr: the Euler-multinomial implementation."""
+    if psr:
+        noises_off = noises_off | IMPL_PSR
     times = np.ascontiguousarray(times, dtype=np.float64)
     N = len(times) - 1
     data = np.ascontiguousarray(data, dtype=np.float64)
@@ -96,10 +101,12 @@ def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=
 
 
 def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=0, seed=1, stream=0, dt=0.25,
-             like_min=1e-17, nthreads=1, theta_out=False, logw=False, ancestry=False):
+             like_min=1e-17, nthreads=1, theta_out=False, logw=False, ancestry=False, psr=False):
     """One pass of iterated filtering restated on the CPU (oracle_mif_pass).  mif: dict of
     pv_src [n_pp], tr, scale, duration, min, max, multiplier, sd [n_theta]; theta0
     [n_theta, J]; x0 [S, J]; mut_fac [N]."""
+    if psr:
+        noises_off = noises_off | IMPL_PSR
     times = np.ascontiguousarray(times, dtype=np.float64)
     N = len(times) - 1
     i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
@@ -143,7 +150,9 @@ def mif_natural(tr, sc, dur, x, a, b, mult):
     return lib().oracle_mif_natural(int(tr), int(sc), int(dur), float(x), float(a), float(b), float(mult))
 
 
-def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=0.25):
+def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=0.25, psr=False):
+    if psr:
+        noises_off = noises_off | IMPL_PSR
     times = np.ascontiguousarray(times, dtype=np.float64)
     N = len(times) - 1
     S = lib().oracle_model_size_proj(model_id)

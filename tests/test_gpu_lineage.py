@@ -75,10 +75,11 @@ def test_lineage_served_equals_alone(ctx):
         m, times, data, pv, x0 = setup(name, N)
         ids.append(srv.submit(m, J, times, data, pv, x0, seed=9, stream=stream, lineage=lin))
     got = {}
-    for _ in range(1000):
+    for _ in range(100000):
         srv.step()
         for rid in srv.done():
-            got[rid] = srv.result(rid)
+            if rid not in got:
+                got[rid] = srv.result(rid)
         if len(got) == len(ids):
             break
     for rid, (name, J, N, stream, lin) in zip(ids, reqs):

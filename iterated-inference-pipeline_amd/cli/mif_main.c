@@ -31,7 +31,8 @@
  * the starting parameters come from Philox-4x32-10 keyed (seed_k, 0xfffffffe), counter
  * (j, m, attempt, block), four Box-Muller normals per block.
  *
- * Differences: only the 'sde' implementation; --prior is not supported; a "covariance" in
+ * Implementations 'sde' and 'psr' (the default, FP64, initial states rounded); not 'ode'.
+ * Differences: --prior is not supported; a "covariance" in
  * theta (MVN proposals and mutation) is refused; the fixed-lag update uses the filtered
  * weighted mean of the initial conditions at the last observation up to L (the reference
  * weights the resampled particles with the weights before resampling, methods.c:370-388,
@@ -115,7 +116,7 @@ int main(int argc, char *argv[])
         "                     [-a, --cooling <float>] [-b, --heat <float>] [-L, --lag <float>] [-S, --switch <integer>]\n"
         "                     [-f --ic_only] [-q, --quiet] [-P, --pipe]\n"
         "                     [--chains <K>] [--lanes <k>] [--fp32] [--device <k>] [--seed <k>] [--trace_17g] [-h, --help]\n"
-        "where implementation is 'sde' (this build runs the SDE filter on the GPU)\n";
+        "where implementation is 'sde' or 'psr' (default)\n";
     for (;;) {
         int idx = 0;
         int ch = getopt_long(argc, argv, "rqPhxyzs:v:w:i:J:l:M:a:b:L:S:fp:N:g:", long_options, &idx);
@@ -154,12 +155,8 @@ int main(int argc, char *argv[])
     }
     argc -= optind;
     argv += optind;
-    if (argc == 0 || strcmp(argv[0], "sde")) {
-        fprintf(stderr, "%s", help);
-        if (argc == 0 || !strcmp(argv[0], "psr") || !strcmp(argv[0], "ode"))
-            fprintf(stderr, "error: only the 'sde' implementation is built for the GPU\n");
-        return 1;
-    }
+    const int psr = cli_implementation(argc, argv, help);
+    if (psr && fp32) cli_die("the psr implementation runs in FP64 only (use 'sde' with --fp32)");
     if (prior) cli_die("--prior is not supported by this build");
     if (n_chains < 1 || n_chains > 4096) cli_die("--chains must be in [1, 4096]");
     if (pipe && n_chains > 1) cli_die("-P (pipe) prints one theta: use it with one chain");
@@ -173,6 +170,7 @@ int main(int argc, char *argv[])
     }
     cli_problem P;
     cli_load(&P, 0, -1, dt);
+    P.psr = psr;
     if (pj_get(P.theta, "covariance")) cli_die("a \"covariance\" in theta (MVN proposals) is not supported by mif in this build");
     const int L = (int)floor(prop_L * P.N_DATA);
     if (ic_only) P.N = (L + 1 > P.N_DATA) ? P.N_DATA : L + 1;
@@ -346,7 +344,7 @@ int main(int argc, char *argv[])
         d.N = N;
         d.precision = fp32 ? PLOM_FP32 : PLOM_FP64;
         d.noises_off = noises_off;
-        d.flags = PLOM_FLAG_MIF;
+        d.flags = PLOM_FLAG_MIF | (psr ? PLOM_FLAG_PSR : 0u);
         d.seed = c->seed;
         d.dt = P.dt;
         d.like_min = like_min;
@@ -402,7 +400,11 @@ int main(int argc, char *argv[])
                 for (int o = 0; o < n; o++) c->theta[(size_t)o * J + j] = th[o];
                 for (int i = 0; i < P.n_sv; i++)
                     for (int cac = 0; cac < P.N_CAC; cac++)
-                        c->x0[(size_t)(i * P.N_CAC + cac) * J + j] = cli_natural(&P, i, P.R[i].map[cac], th) * P.fixed[cac];
+                    {
+                        double v = cli_natural(&P, i, P.R[i].map[cac], th) * P.fixed[cac];
+                        if (psr) v = round(v);
+                        c->x0[(size_t)(i * P.N_CAC + cac) * J + j] = v;
+                    }
             }
             if (plom_filter_mif_start(ctx, c->f, c->theta, c->x0, mut, c->seed, (uint32_t)m) != PLOM_OK) cli_die(plom_last_error());
         }

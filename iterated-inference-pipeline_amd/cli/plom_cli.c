@@ -31,6 +31,15 @@ void cli_die(const char *msg)
     exit(EXIT_FAILURE);
 }
 
+int cli_implementation(int argc, char **argv, const char *help)
+{
+    if (argc == 0 || !strcmp(argv[0], "psr")) return 1;
+    if (!strcmp(argv[0], "sde")) return 0;
+    fprintf(stderr, "%s", help);
+    if (!strcmp(argv[0], "ode")) fprintf(stderr, "error: the 'ode' implementation is not built for the GPU\n");
+    exit(1);
+}
+
 /* ------------------------------------------------------------------ transforms */
 double cli_f_tr(cli_tr k, double x, double a, double b)
 {
@@ -465,7 +474,10 @@ void cli_pack(const cli_problem *P, const double *theta, double *pv, double *x0)
     /* initial state: proportions x population size (prediction_util.c:82-115) */
     memset(x0, 0, (size_t)P->mi.size_proj * sizeof(double));
     for (int i = 0; i < P->n_sv; i++)
-        for (int cac = 0; cac < N_CAC; cac++) x0[i * N_CAC + cac] = cli_natural(P, i, P->R[i].map[cac], theta) * P->fixed[cac];
+        for (int cac = 0; cac < N_CAC; cac++) {
+            x0[i * N_CAC + cac] = cli_natural(P, i, P->R[i].map[cac], theta) * P->fixed[cac];
+            if (P->psr) x0[i * N_CAC + cac] = round(x0[i * N_CAC + cac]);   /* counts (prediction_util.c:108-110) */
+        }
 }
 
 /* ------------------------------------------------------------------ writers */

@@ -58,11 +58,17 @@ typedef struct {
     const char *remainder;
     int has_rem;        /* a remainder state is written (not POP_SIZE_EQ_SUM_SV) */
     int *noise_par;     /* [n_par] 1 for the white-noise sd parameters (ind_noise_sd) */
+    int psr;            /* implementation psr: initial states rounded (prop2Xpop_size) */
 } cli_problem;
 
 extern int CLI_QUIET;
 extern int CLI_TRACE_17G;   /* trace values with 17 significant digits instead of %g */
 void cli_log(const char *msg);
+
+/* the implementation argument of the commands (e.g. C/smc/main.c:223-238): 'psr' (also the
+ * default without one) -> 1, 'sde' -> 0; 'ode' is not built; anything else prints the help
+ * and exits with 1 */
+int cli_implementation(int argc, char **argv, const char *help);
 void cli_die(const char *msg);
 
 double cli_f_tr(cli_tr k, double x, double a, double b);

@@ -40,7 +40,8 @@
  * four per block), (m, 2, 0, 0) for the acceptance uniform, (m, 3, block, 0) for the
  * shuffle.  Filters never use stream 0xffffffff (iterations are below 2^31).
  *
- * Differences: only the 'sde' implementation; -Z/-c (zmq workers) and -N are accepted
+ * Implementations 'sde' and 'psr' (the default, FP64, initial states rounded); not 'ode'.
+ * Differences: -Z/-c (zmq workers) and -N are accepted
  * and ignored; per-group transforms are used throughout (the reference takes group 0's
  * f_inv in the trace, the done file and the proposal's change of variable); the proposal
  * density is evaluated in the log domain (same ratio); a non positive definite
@@ -312,7 +313,7 @@ int main(int argc, char *argv[])
         "                [-E --epsilon <int || 50>] [--epsilon_max <float || 50.0>] [--smooth] [--alpha <float || 0.02>]\n"
         "                [-o, --nb_obs <int>] [-q, --quiet] [-P, --pipe]\n"
         "                [--chains <K || 1>] [--lanes <k>] [--fp32] [--device <k>] [--seed <k>] [--trace_17g] [-h, --help]\n"
-        "where implementation is 'sde' (this build runs the SDE filter on the GPU)\n";
+        "where implementation is 'sde' or 'psr' (default)\n";
     for (;;) {
         int idx = 0;
         int ch = getopt_long(argc, argv, "qPhrxyzs:v:w:i:J:l:M:p:c:N:ZS:E:a:f:G:n:o:g:", long_options, &idx);
@@ -355,12 +356,8 @@ int main(int argc, char *argv[])
     MC.smooth = smooth_flag;
     argc -= optind;
     argv += optind;
-    if (argc == 0 || strcmp(argv[0], "sde")) {
-        fprintf(stderr, "%s", help);
-        if (argc == 0 || !strcmp(argv[0], "psr") || !strcmp(argv[0], "ode"))
-            fprintf(stderr, "error: only the 'sde' implementation is built for the GPU\n");
-        return 1;
-    }
+    const int psr = cli_implementation(argc, argv, help);
+    if (psr && fp32) cli_die("the psr implementation runs in FP64 only (use 'sde' with --fp32)");
     if (n_chains < 1 || n_chains > 4096) cli_die("--chains must be in [1, 4096]");
     if (pipe && n_chains > 1) cli_die("-P (pipe) prints one theta: use it with one chain");
     if (M_iter < 1 || M_iter >= (1 << 30)) cli_die("-M must be in [1, 2^30)");
@@ -374,6 +371,7 @@ int main(int argc, char *argv[])
 
     cli_problem P;
     cli_load(&P, 1, nb_obs, dt);
+    P.psr = psr;
     MC.P = &P;
     const int n = P.n_theta, N = P.N, S = P.mi.size_proj, N_TS = P.N_TS;
     MC.n = n;
@@ -554,7 +552,7 @@ int main(int argc, char *argv[])
         d.N = N;
         d.precision = fp32 ? PLOM_FP32 : PLOM_FP64;
         d.noises_off = noises_off;
-        d.flags = print_x ? PLOM_FLAG_LINEAGE : 0u;
+        d.flags = (print_x ? PLOM_FLAG_LINEAGE : 0u) | (psr ? PLOM_FLAG_PSR : 0u);
         d.seed = C[k].seed;
         d.stream = 0;
         d.dt = P.dt;

@@ -19,7 +19,9 @@
  * Parameter handling (routers, transforms, theta remainder, initial state) is the shared
  * host logic of plom_cli.c.
  *
- * Differences: implementation "psr" and "ode" are not built (only "sde"); the X output
+ * Implementations "sde" and "psr" (the default, as main.c:223-238: FP64, the initial state
+ * rounded to counts, prediction_util.c:108-110); "ode" is not built.
+ * Differences: the X output
  * of -r is not written (a note goes to stderr); hat and pred_res use this build's
  * arithmetic (exact integer weights for the 95 % bounds, tree sums);
  * --prior is not supported; -N is accepted and ignored (the GPU replaces threads).
@@ -70,7 +72,7 @@ int main(int argc, char *argv[])
         "                     [-s, --DT <float>] [--eps_abs <float>] [--eps_rel <float>]\n"
         "                     [-l, --LIKE_MIN <float>] [-J <integer>] [--prior]\n"
         "                     [-q, --quiet] [-P, --pipe] [--fp32] [--device <k>] [--seed <k>] [-h, --help]\n"
-        "where implementation is 'sde' (this build runs the SDE filter on the GPU)\n";
+        "where implementation is 'sde' or 'psr' (default)\n";
     for (;;) {
         int idx = 0;
         int ch = getopt_long(argc, argv, "qPxyzs:v:w:p:i:J:l:trbehrN:o:g:d", long_options, &idx);
@@ -105,12 +107,8 @@ int main(int argc, char *argv[])
     }
     argc -= optind;
     argv += optind;
-    if (argc == 0 || strcmp(argv[0], "sde")) {
-        fprintf(stderr, "%s", help);
-        if (argc == 0 || !strcmp(argv[0], "psr") || !strcmp(argv[0], "ode"))
-            fprintf(stderr, "error: only the 'sde' implementation is built for the GPU\n");
-        return 1;
-    }
+    const int psr = cli_implementation(argc, argv, help);
+    if (psr && fp32) cli_die("the psr implementation runs in FP64 only (use 'sde' with --fp32)");
     if (prior) cli_die("--prior is not supported by this build");
 
     char str[2048];
@@ -119,6 +117,7 @@ int main(int argc, char *argv[])
 
     cli_problem P;
     cli_load(&P, 0, nb_obs, dt);
+    P.psr = psr;
     const int N = P.N, N_TS = P.N_TS;
     const plom_model_info mi = P.mi;
     double *theta = calloc((size_t)P.n_theta + 1, sizeof(double));
@@ -157,7 +156,7 @@ int main(int argc, char *argv[])
     d.N = N;
     d.precision = fp32 ? PLOM_FP32 : PLOM_FP64;
     d.noises_off = noises_off;
-    d.flags = (filter ? 0u : PLOM_FLAG_NO_FILTER) | (want_hat ? PLOM_FLAG_HAT : 0u);
+    d.flags = (filter ? 0u : PLOM_FLAG_NO_FILTER) | (want_hat ? PLOM_FLAG_HAT : 0u) | (psr ? PLOM_FLAG_PSR : 0u);
     d.seed = (uint32_t)(seed_opt >= 0 ? seed_opt : 2 + general_id);
     d.stream = 0;
     d.dt = dt;

@@ -58,6 +58,22 @@ def test_dry_run_parameters(tmp_path):
     assert out["N"] == 52 and out["dt"] == 0.25
 
 
+def test_psr_rounds_the_initial_state(tmp_path):
+    """psr (the default implementation) works on counts: prop2Xpop_size rounds the
+    initial state (prediction_util.c:108-110)."""
+    d = model_dir(tmp_path, "sir")
+    th = os.path.join(PKG, "examples", "sir", "theta.json")
+    theta = json.load(open(th))
+    theta["parameter"]["I"]["group"]["all"]["guess"]["value"] = 0.00098765
+    p = tmp_path / "t.json"
+    p.write_text(json.dumps(theta) + "\n")
+    sde = json.loads(run_cli(d, ["sde", "-J", "10", "--dry_run"], str(p)).stdout)
+    psr = json.loads(run_cli(d, ["-J", "10", "--dry_run"], str(p)).stdout)
+    assert sde["x0"][1] != round(sde["x0"][1])
+    assert psr["x0"] == [round(v) for v in sde["x0"]]
+    assert psr["pv"] == sde["pv"]
+
+
 def test_theta_remainder_and_errors(tmp_path):
     d = model_dir(tmp_path, "sir")
     theta = json.load(open(os.path.join(PKG, "examples", "sir", "theta.json")))
@@ -67,8 +83,10 @@ def test_theta_remainder_and_errors(tmp_path):
     p.write_text(json.dumps(theta) + "\n")
     r = run_cli(d, ["sde", "-J", "10", "--dry_run"], str(p))
     assert r.returncode != 0 and "greater than 1.0" in r.stderr
-    r = run_cli(d, ["psr", "-J", "10"], os.path.join(PKG, "examples", "sir", "theta.json"))
-    assert r.returncode == 1
+    r = run_cli(d, ["ode", "-J", "10"], os.path.join(PKG, "examples", "sir", "theta.json"))
+    assert r.returncode == 1 and "ode" in r.stderr
+    r = run_cli(d, ["psr", "-J", "10", "--fp32"], os.path.join(PKG, "examples", "sir", "theta.json"))
+    assert r.returncode != 0 and "FP64" in r.stderr
 
 
 @pytest.mark.gpu
@@ -194,7 +212,7 @@ def test_pmcmc_follow_and_covariance(tmp_path):
 def test_pmcmc_usage_errors(tmp_path):
     d = model_dir(tmp_path, "sir")
     th = os.path.join(PKG, "examples", "sir", "theta.json")
-    assert run_pmcmc(d, ["psr", "-J", "10"], th).returncode == 1
+    assert run_pmcmc(d, ["ode", "-J", "10"], th).returncode == 1
     r = run_pmcmc(d, ["sde", "-J", "10", "-P", "--chains", "2"], th)
     assert r.returncode != 0 and "one chain" in r.stderr
     r = run_pmcmc(d, ["sde", "-J", "10", "--chains", "0"], th)
@@ -272,3 +290,22 @@ def test_mif_converges_towards_the_truth(tmp_path):
     beta = [row[1][2] for row in out["trace"]]
     assert beta[0] == pytest.approx(0.35)
     assert abs(beta[-1] - 0.5) < 0.35 * abs(0.35 - 0.5), beta
+
+
+@pytest.mark.gpu
+def test_cli_psr_matches_api(tmp_path, ctx):
+    """`smc` with no implementation runs psr (the reference's default) and gives the
+    library's PLOM_FLAG_PSR log-likelihood for the dry run's inputs (rounded state)."""
+    from oracle import oracle
+    d = model_dir(tmp_path, "sir")
+    theta = os.path.join(PKG, "examples", "sir", "theta.json")
+    dry = json.loads(run_cli(d, ["-J", "1000", "-o", "40", "--dry_run"], theta).stdout)
+    print("cli psr run", flush=True)
+    r = run_cli(d, ["-J", "1000", "-o", "40", "-e", "-d", "-b"], theta)
+    assert r.returncode == 0, r.stderr
+    ll = float([l for l in r.stdout.splitlines() if l.startswith("logV_17g:")][0].split(":")[1])
+    m = smc.Model.get("sir")
+    times, data = m.dataset(40)
+    g = smc.run(ctx, m, 1000, times, data, np.array(dry["pv"]), np.array(dry["x0"]), flags=smc.FLAG_PSR, seed=2, stream=0)
+    o = oracle.smc(m.id, 1000, times, data, np.array(dry["pv"]), np.array(dry["x0"]), seed=2, stream=0, psr=True)
+    assert ll == g.loglik == o["loglik"]

@@ -21,6 +21,10 @@ Legs on the same JSON line:
             [500, 5e4], N ~ U{52..520}, served with continuous batching and admission by
             free HBM: sustained particle-steps/s and p50 / p99 latency (submit -> result)
   single    one large filter (SIRS, J = 1e5, N = 520), FP64 and FP32
+  psr       the same filter with the psr implementation (Euler-multinomial step, FP64),
+            and the C oracle's psr on a bounded sample of it
+  mif       one iterated-filtering pass of the same filter with per-particle parameters
+            (28 per-particle slots, random walk after every observation, theta moments)
   roofline  of the dominant kernel (k_propagate of the batch), from HIP events around
             every propagate launch on the library's stream
   cpu_baseline  the C oracle (this project's restatement of the reference filter) on a
@@ -44,7 +48,7 @@ from pkgload import load_package  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
 METRIC = "particle-steps/s & p50/p99 loglik latency, batched+served vs plain SMC"
-LEGS = ("batched", "plain", "served", "single", "memory", "cpu")
+LEGS = ("batched", "plain", "served", "single", "psr", "mif", "memory", "cpu")
 
 
 def log(msg):
@@ -249,6 +253,65 @@ def single_leg(smc, ctx, inp, configs, precision, steps, warmup, J, N):
             "propagate_ms_per_launch": round(prop_ms / max(nint, 1), 5)}
 
 
+def psr_leg(smc, ctx, inp, configs, steps, warmup, J, N, cpu, cpu_J, cpu_N):
+    m, times, data, pv, x0 = inp.get("sirs4", N, configs.TRUTH["sirs4"])
+    x0 = np.round(x0)
+    f = smc.Filter(ctx, m, J, times, data, pv, x0, flags=smc.FLAG_PSR, seed=configs.FILTER_SEED, stream=0)
+    for _ in range(warmup):
+        f.run()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f.run()
+    ctx.sync()
+    wall = (time.perf_counter() - t0) / steps
+    res = f.read()
+    f.close()
+    out = {"value": round(J * N / wall, 1), "ms_per_filter": round(wall * 1e3, 3), "loglik": res.loglik}
+    if cpu:
+        from oracle import oracle
+        mc, tc, dc, pvc, x0c = inp.get("sirs4", cpu_N, configs.TRUTH["sirs4"])
+        r = oracle.smc(mc.id, cpu_J, tc, dc, pvc, np.round(x0c), seed=configs.FILTER_SEED, stream=0, psr=True,
+                       nthreads=16)
+        out["cpu_oracle"] = {"value": round(cpu_J * cpu_N / r["elapsed_s"], 1), "cores": 16,
+                             "sample": "SIRS psr, J=%d, N=%d" % (cpu_J, cpu_N)}
+    return out
+
+
+def mif_leg(smc, ctx, inp, configs, steps, warmup, J, N):
+    """theta = log of every per-particle pv slot (logit of the reporting rates), jump sizes
+    0.02, the walk factor of a weekly interval at the first iteration"""
+    m, times, data, pv, x0 = inp.get("sirs4", N, configs.TRUTH["sirs4"])
+    lay = m.layout
+    rep = set(range(lay["obs"]["rep"], lay["obs"]["rep"] + m.n_ts))
+    tr = [2 if k in rep else 1 for k in range(m.n_pp)]
+    th = np.array([np.log(pv[k] / (1 - pv[k])) if k in rep else np.log(pv[k]) for k in range(m.n_pp)])
+    nth = m.n_pp
+    mp = {"pv_src": list(range(nth)), "tr": tr, "scale": [0] * nth, "duration": [0] * nth, "min": [0.0] * nth,
+          "max": [0.0] * nth, "multiplier": [1.0] * nth, "sd": [0.02] * nth}
+    rng = np.random.default_rng(7)
+    theta = th[:, None] + rng.normal(0.0, 0.02, (nth, J))
+    xs = np.repeat(x0[:, None], J, axis=1)
+    mut = np.full(N, np.sqrt(7.0))
+    f = smc.Filter(ctx, m, J, times, data, pv, x0, flags=smc.FLAG_MIF, mif=mp, seed=configs.FILTER_SEED)
+    for _ in range(warmup):
+        f.mif_start(theta, xs, mut, seed=1, stream=0)
+        f.run()
+    ctx.sync()
+    wall = 0.0
+    for it in range(steps):
+        f.mif_start(theta, xs, mut, seed=1, stream=it)   # the upload of the starting cloud is not timed
+        t0 = time.perf_counter()
+        f.run()
+        ctx.sync()
+        wall += time.perf_counter() - t0
+    wall /= steps
+    res = f.read(mif=True)
+    f.close()
+    return {"value": round(J * N / wall, 1), "ms_per_pass": round(wall * 1e3, 3), "loglik": res.loglik,
+            "n_theta": nth}
+
+
 def cpu_baseline(smc, inp, configs, draws, threads):
     from oracle import oracle
     secs, ps = 0.0, 0
@@ -389,6 +452,15 @@ def main():
         out["single"] = {"workload": "SIRS 4x5e5, J=%d, N=%d" % (args.single_J, args.single_N),
                          "fp64": single_leg(smc, ctx, inp, configs, "fp64", args.steps, args.warmup, args.single_J, args.single_N),
                          "fp32": single_leg(smc, ctx, inp, configs, "fp32", args.steps, args.warmup, args.single_J, args.single_N)}
+    if "psr" in legs:
+        log("psr filter")
+        out["psr"] = dict(workload="SIRS 4x5e5 psr (Euler-multinomial), J=%d, N=%d, FP64" % (args.single_J, args.single_N),
+                          **psr_leg(smc, ctx, inp, configs, args.steps, args.warmup, args.single_J, args.single_N,
+                                    "cpu" in legs, 2000, 104))
+    if "mif" in legs:
+        log("iterated-filtering pass")
+        out["mif"] = dict(workload="SIRS 4x5e5, J=%d, N=%d, FP64, per-particle parameters" % (args.single_J, args.single_N),
+                          **mif_leg(smc, ctx, inp, configs, args.steps, args.warmup, args.single_J, args.single_N))
     ctx.close()
     if "cpu" in legs:
         threads = 16

@@ -388,18 +388,31 @@ __global__ __launch_bounds__(TILE) void k_weights(plom_dfilter *filters, const i
 
 /* PLOM_FLAG_MIF, interval n with data, after k_weights: per tile, the sum of the weights
  * w_j = exp(l_j - m) (1 each when every particle failed, smc.c:55-70) and, per component,
- * the sum of w_j theta_j, the tile mean and the sum of squared deviations from it */
+ * the sum of w_j theta_j, the tile mean and the sum of squared deviations from it.  Every
+ * sum is the tree of tree256 (a butterfly within each wave, then (w0 + w1) + (w2 + w3));
+ * the components go through in chunks of 64 with one LDS exchange per pass. */
+#define MIF_CHUNK 64
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1) v = v + __shfl_xor(v, h, 64);
+    return v;
+}
+
 __global__ __launch_bounds__(TILE) void k_mif_tiles(plom_dfilter *filters, const int2 *tiles)
 {
     __shared__ double lds[8];
+    __shared__ double red[MIF_CHUNK][4][2];
+    __shared__ double smean[MIF_CHUNK];
     int fi = 0, tile = blockIdx.x;
     if (tiles) { const int2 tt = tiles[blockIdx.x]; fi = tt.x; tile = tt.y; }
     plom_dfilter *f = &filters[fi];
     const int n = f->step;
     if (n >= f->N || !(f->flags & PLOM_FLAG_MIF) || f->nonan[n] == 0) return;
-    const int J = f->J, Jp = f->Jp, T = f->T;
+    const int J = f->J, Jp = f->Jp, T = f->T, nth = f->n_theta;
     const int j = tile * TILE + threadIdx.x;
     const bool valid = j < J;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     double w = 0.0;
     if (valid) {
         if (f->rflag[n]) {
@@ -413,17 +426,35 @@ __global__ __launch_bounds__(TILE) void k_mif_tiles(plom_dfilter *filters, const
     const int rem = J - tile * TILE;
     const double ct = (double)(rem < TILE ? rem : TILE);
     const double *th = f->th[(n + 1) & 1];
-    for (int o = 0; o < f->n_theta; o++) {
-        const double v = valid ? th[(size_t)o * Jp + j] : 0.0;
-        const double s1 = tree256(valid ? w * v : 0.0, lds);
-        const double mt = tree256(v, lds) / ct;
-        const double d = valid ? v - mt : 0.0;
-        const double q = tree256(d * d, lds);
-        if (threadIdx.x == 0) {
-            f->mps[(size_t)o * T + tile] = s1;
-            f->mpm[(size_t)o * T + tile] = mt;
-            f->mpq[(size_t)o * T + tile] = q;
+    for (int o0 = 0; o0 < nth; o0 += MIF_CHUNK) {
+        const int nc = nth - o0 < MIF_CHUNK ? nth - o0 : MIF_CHUNK;
+        for (int c = 0; c < nc; c++) {
+            const double v = valid ? th[(size_t)(o0 + c) * Jp + j] : 0.0;
+            const double a = wave_sum(valid ? w * v : 0.0);
+            const double b = wave_sum(v);
+            if (lane == 0) { red[c][wv][0] = a; red[c][wv][1] = b; }
         }
+        __syncthreads();
+        if (threadIdx.x < nc) {
+            const int c = threadIdx.x;
+            const double s1 = (red[c][0][0] + red[c][1][0]) + (red[c][2][0] + red[c][3][0]);
+            const double mt = ((red[c][0][1] + red[c][1][1]) + (red[c][2][1] + red[c][3][1])) / ct;
+            smean[c] = mt;
+            f->mps[(size_t)(o0 + c) * T + tile] = s1;
+            f->mpm[(size_t)(o0 + c) * T + tile] = mt;
+        }
+        __syncthreads();
+        for (int c = 0; c < nc; c++) {
+            const double d = valid ? th[(size_t)(o0 + c) * Jp + j] - smean[c] : 0.0;
+            const double q = wave_sum(d * d);
+            if (lane == 0) red[c][wv][0] = q;
+        }
+        __syncthreads();
+        if (threadIdx.x < nc) {
+            const int c = threadIdx.x;
+            f->mpq[(size_t)(o0 + c) * T + tile] = (red[c][0][0] + red[c][1][0]) + (red[c][2][0] + red[c][3][0]);
+        }
+        __syncthreads();
     }
     if (threadIdx.x == 0) f->mpw[tile] = sw;
 }
@@ -454,8 +485,8 @@ __device__ double tile_tree(const double *v, int T, double *lds, double *chunks)
     return out;
 }
 
-/* one block per filter: weighted mean sum(w theta)/sum(w) and variance M2/(J-1) of every
- * component, the (count, mean, M2) of the tiles merged along the same fixed tree */
+/* one block per (filter, component): weighted mean sum(w theta)/sum(w) and variance
+ * M2/(J-1), the (count, mean, M2) of the tiles merged along the same fixed tree */
 __global__ __launch_bounds__(TILE) void k_mif_reduce(plom_dfilter *filters, const int *fidx)
 {
     __shared__ double lds[8];
@@ -467,9 +498,11 @@ __global__ __launch_bounds__(TILE) void k_mif_reduce(plom_dfilter *filters, cons
     const int n = f->step;
     if (n >= f->N || !(f->flags & PLOM_FLAG_MIF) || f->nonan[n] == 0) return;
     const int J = f->J, T = f->T, nth = f->n_theta;
+    const int o = blockIdx.y;
+    if (o >= nth) return;
     const double W = tile_tree(f->mpw, T, lds, chunks);
     const int nch = (T + TILE - 1) / TILE;
-    for (int o = 0; o < nth; o++) {
+    {
         const double SW = tile_tree(f->mps + (size_t)o * T, T, lds, chunks);
         for (int c = 0; c < nch; c++) {
             const int t = c * TILE + threadIdx.x;
@@ -1528,13 +1561,14 @@ extern "C" int plom_filter_destroy(plom_ctx *c, plom_filter *f)
 /* weights, fixed-point scan and tile prefix of one interval: K2 (one block per filter),
  * K3 (one block per tile), K4 (one block per filter) */
 static void enqueue_resample(hipStream_t s, plom_dfilter *df, const int2 *tiles, int ntiles, const int *fidx, int nfilters,
-                             int mif = 0)
+                             int mif_theta = 0)
 {
     hipLaunchKernelGGL(k_weights, dim3(nfilters), dim3(TILE), 0, s, df, fidx);
-    if (mif) {
-        /* theta moments of the weighted cloud, before the scan and the interval counter move on */
+    if (mif_theta > 0) {
+        /* theta moments of the weighted cloud, before the scan and the interval counter move
+         * on; mif_theta is the largest n_theta of the MIF filters */
         hipLaunchKernelGGL(k_mif_tiles, dim3(ntiles), dim3(TILE), 0, s, df, tiles);
-        hipLaunchKernelGGL(k_mif_reduce, dim3(nfilters), dim3(TILE), 0, s, df, fidx);
+        hipLaunchKernelGGL(k_mif_reduce, dim3(nfilters, mif_theta), dim3(TILE), 0, s, df, fidx);
     }
     hipLaunchKernelGGL(k_scan, dim3(ntiles), dim3(TILE), 0, s, df, tiles);
     hipLaunchKernelGGL(k_advance, dim3(nfilters), dim3(TILE), 0, s, df, fidx);
@@ -1577,7 +1611,7 @@ extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
         me->launch_prop(f->desc.precision, f->variant, mif, T, c->stream, f->df, nullptr);
         c->kprop_launches++;
         if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
-        enqueue_resample(c->stream, f->df, nullptr, T, nullptr, 1, mif);
+        enqueue_resample(c->stream, f->df, nullptr, T, nullptr, 1, mif ? f->desc.n_theta : 0);
         if (f->desc.flags & PLOM_FLAG_HAT) me->launch_hat(f->desc.precision, c->stream, f->df, T, f->info.n_hat);
     }
     if ((f->desc.flags & PLOM_FLAG_HAT) && f->desc.N > 0) me->launch_hat0(f->desc.precision, c->stream, f->df);
@@ -1691,7 +1725,7 @@ struct batch_chain {
     plom_dfilter *ddf = nullptr;
     char *buf = nullptr;
     int maxN = 0;
-    int any_mif = 0;
+    int any_mif = 0;                   /* largest n_theta of the chain's MIF filters (0: none) */
 };
 
 struct plom_batch {
@@ -1718,7 +1752,7 @@ static int chain_build(plom_ctx *c, batch_chain *ch)
         plom_filter *f = ch->fs[(size_t)r];
         batch_chain::group *g = nullptr;
         const int mif = (f->desc.flags & PLOM_FLAG_MIF) != 0;
-        ch->any_mif |= mif;
+        if (mif && f->desc.n_theta > ch->any_mif) ch->any_mif = f->desc.n_theta;
         for (auto &gg : ch->groups)
             if (gg.model == f->desc.model_id && gg.prec == f->desc.precision && gg.variant == f->variant && gg.mif == mif) g = &gg;
         if (!g) {

@@ -92,6 +92,10 @@ typedef struct {
     int32_t n_hat;       /* quantities of a hat row: states, remainders, observation means */
     int32_t n_pp;        /* leading slots of pv that PLOM_FLAG_MIF makes per particle: the process
                             parameters [par][n_cac] then the observation parameters [par][n_ts] */
+    int32_t fix_off;     /* first slot of pv holding a covariate (metadata: population size, birth
+                            rate, proportion ...): [covariate][n_cac] then those of the observation
+                            model [covariate][n_ts] */
+    int32_t n_fix;       /* number of those slots (plom_filter_inputs.cov has n_fix series) */
 } plom_model_info;
 
 typedef struct {
@@ -125,12 +129,28 @@ typedef struct {
                                   component is resampled but not moved) */
 } plom_mif_map;
 
+/* A covariate as a function of time, for one slot of pv (one covariate of one population or
+ * series), as the reference's spline of par_fixed_values (build_p_calc,
+ * C/core/build.c:1075-1211, evaluated by the generated code, model_builder/Ccoder.py:39-43):
+ * n >= 4 knots: natural cubic spline; 2 or 3: linear; 1: constant.  The filter evaluates it at
+ * the start of every Euler substep (rates) and at every observation time (observation model,
+ * remainders); every such time must lie within [x[0], x[n-1]] when n > 1. */
+typedef struct {
+    int32_t n;
+    int32_t pad;
+    const double *x;     /* [n] strictly increasing, days since date_0 */
+    const double *y;     /* [n] values, in the data's units */
+} plom_covariate;
+
 typedef struct {
     const double *times; /* [N+1] days since date_0 (settings data.times) */
     const double *data;  /* [N*n_ts] observations, NaN = missing */
     const double *pv;    /* [pv_size] natural parameters per population / series */
     const double *x0;    /* [size_proj] initial state (incidence slots ignored) */
     const plom_mif_map *mif; /* PLOM_FLAG_MIF only (copied at creation) */
+    const plom_covariate *cov; /* optional [n_fix]: time-varying covariates, evaluated at creation
+                                  (they then replace the covariate slots of pv); NULL: the
+                                  covariates are the constants in pv */
 } plom_filter_inputs;
 
 typedef struct {
@@ -168,6 +188,11 @@ int plom_model_count(void);
 int plom_model_find(const char *name);                 /* model id, or PLOM_ERR_MODEL */
 int plom_model_name(int model_id, char *buf, int len);
 int plom_model_info_get(int model_id, plom_model_info *out);
+
+/* the value of a covariate at the times t[0..nt) (the population size at t = 0 for the initial
+ * state, pop_size_t0 of C/core/build.c:1179-1185, and at the observation times for the
+ * remainders written by the commands); PLOM_ERR_ARG for a time outside its knots */
+int plom_covariate_eval(const plom_covariate *cov, int nt, const double *t, double *out);
 
 /* one filter, blocking: upload, run, read back, free */
 int plom_smc_run(plom_ctx *ctx, const plom_filter_desc *desc, const plom_filter_inputs *in,

@@ -126,7 +126,8 @@ int main(int argc, char *argv[])
         case 'y': noises_off |= PLOM_NO_ENV_STO_BIT; break;
         case 'z': noises_off |= PLOM_NO_DRIFT_BIT; break;
         case 's': dt = atof(optarg); break;
-        case 'v': case 'w': case 'g': break;
+        case 'v': case 'w': break;
+        case 'g': CLI_FREEZE = atof(optarg); break;
         case 'h': printf("%s", help); return 1;
         case 'p': snprintf(path, sizeof path, "%s", optarg); break;
         case 'N': n_threads = atoi(optarg); break;
@@ -349,7 +350,7 @@ int main(int argc, char *argv[])
         d.dt = P.dt;
         d.like_min = like_min;
         d.n_theta = n;
-        plom_filter_inputs in = {P.times, P.data, pv, x0one, &mmap};
+        plom_filter_inputs in = {P.times, P.data, pv, x0one, &mmap, P.has_cov ? P.cov : NULL};
         if (plom_filter_create(ctx, &d, &in, &fs[k]) != PLOM_OK) cli_die(plom_last_error());
         c->f = fs[k];
     }

@@ -297,6 +297,88 @@ double cli_natural(const cli_problem *P, int i, int g, const double *theta)
 }
 
 /* ------------------------------------------------------------------ loading */
+double CLI_FREEZE = -1.0;
+
+/* The series of covariate slot s: par_fixed_values[name] entry "id" == key (NULL if absent):
+ * x in days since date_0, y times the unit multiplier (get_multiplier, transform.c:267-301,
+ * data unit = days); frozen (-g) at its value at min(freeze, last knot) over its knots
+ * (build.c:1188-1205). */
+static void load_series(cli_problem *P, int s, const pj *list, const char *key)
+{
+    plom_covariate *c = &P->cov[s];
+    const int q = index_of_series(list, key);
+    if (q < 0) {
+        /* no series for this population / series: the slot is never read */
+        c->n = 1;
+        c->x = calloc(1, sizeof(double));
+        c->y = calloc(1, sizeof(double));
+        return;
+    }
+    const pj *ser = pj_at(list, q), *jx = pj_get(ser, "x"), *jy = pj_get(ser, "y");
+    const int n = pj_size(jy);
+    if (n < 1 || pj_size(jx) != n) cli_die("a par_fixed_values series needs as many x as y values");
+    double *x = calloc((size_t)n, sizeof(double)), *y = calloc((size_t)n, sizeof(double));
+    double mult = 1.0;
+    const char *unit = pj_string(pj_get(ser, "unit")), *type = pj_string(pj_get(ser, "type"));
+    if (unit) {
+        if (type && strcmp(type, "rate_as_duration")) cli_die("not a valid type");
+        mult = type ? u_duration(unit, "D") : u_duration("D", unit);
+    }
+    for (int i = 0; i < n; i++) {
+        x[i] = pj_number(pj_at(jx, i), NULL);
+        y[i] = pj_number(pj_at(jy, i), NULL) * mult;
+    }
+    c->n = n;
+    c->x = x;
+    c->y = y;
+    if (CLI_FREEZE >= 0.0 && n > 1) {
+        double t = CLI_FREEZE < x[n - 1] ? CLI_FREEZE : x[n - 1], v;
+        if (plom_covariate_eval(c, 1, &t, &v) != PLOM_OK) cli_die(plom_last_error());
+        double *fx = calloc(2, sizeof(double)), *fy = calloc(2, sizeof(double));
+        fx[0] = x[0];
+        fx[1] = x[n - 1];
+        fy[0] = fy[1] = v;
+        c->n = 2;
+        c->x = fx;
+        c->y = fy;
+    }
+    for (int i = 1; i < c->n; i++)
+        if (c->y[i] != c->y[0]) P->has_cov = 1;
+}
+
+/* slots: [covariate][N_CAC] then [observation covariate][N_TS] (an observation covariate
+ * without a series of its own for a time series takes its first population's); the values
+ * at t = 0 go into pv (pop_size_t0, build.c:1179-1185, for N) */
+static void cli_load_covariates(cli_problem *P, const pj *pfv, const pj *par_fixed, const pj *pfo)
+{
+    const int N_CAC = P->N_CAC, N_TS = P->N_TS;
+    const int n_fix = P->n_fixed * N_CAC + pj_size(pfo) * N_TS;
+    if (n_fix != P->mi.n_fix) cli_die("settings covariates differ from the compiled model");
+    P->cov = calloc((size_t)n_fix + 1, sizeof(plom_covariate));
+    P->cov0 = calloc((size_t)n_fix + 1, sizeof(double));
+    int s = 0;
+    for (int k = 0; k < P->n_fixed; k++) {
+        const pj *list = pj_get(pfv, pj_string(pj_at(par_fixed, k)));
+        for (int cac = 0; cac < N_CAC; cac++) load_series(P, s++, list, pj_string(pj_at(P->cac_id, cac)));
+    }
+    for (int q = 0; q < pj_size(pfo); q++) {
+        const pj *list = pj_get(pfv, pj_string(pj_at(pfo, q)));
+        int k = index_of(par_fixed, pj_string(pj_at(pfo, q)));
+        for (int ts = 0; ts < N_TS; ts++) {
+            if (index_of_series(list, pj_string(pj_at(P->ts_id, ts))) >= 0 || k < 0) {
+                load_series(P, s++, list, pj_string(pj_at(P->ts_id, ts)));
+            } else {
+                P->cov[s] = P->cov[k * N_CAC];
+                s++;
+            }
+        }
+    }
+    const double t0 = 0.0;
+    for (int i = 0; i < n_fix; i++)
+        if (plom_covariate_eval(&P->cov[i], 1, &t0, &P->cov0[i]) != PLOM_OK) cli_die(plom_last_error());
+    for (int i = 0; i < P->n_fixed * N_CAC; i++) P->fixed[i] = P->cov0[i];
+}
+
 void cli_load(cli_problem *P, int is_bayesian, int nb_obs, double dt)
 {
     char err[512], str[1024];
@@ -373,21 +455,10 @@ void cli_load(cli_problem *P, int is_bayesian, int nb_obs, double dt)
         if (k >= 0 && k < P->n_par) P->noise_par[k] = 1;
     }
 
-    /* covariates: constant per population (value at t = 0), as pop_size_t0 (build.c:1179-1185) */
-    pj *pfv = pj_get(sdata, "par_fixed_values");
+    /* covariates: one series per covariate slot of pv (build_p_calc, build.c:1075-1211) */
     P->n_fixed = pj_size(par_fixed);
-    P->fixed = calloc((size_t)(P->n_fixed * N_CAC + 1), sizeof(double));
-    for (int k = 0; k < P->n_fixed; k++) {
-        pj *series = pj_get(pfv, pj_string(pj_at(par_fixed, k)));
-        for (int cac = 0; cac < N_CAC; cac++) {
-            pj *s = pj_at(series, index_of_series(series, pj_string(pj_at(P->cac_id, cac))));
-            pj *y = pj_get(s, "y");
-            double y0 = pj_number(pj_at(y, 0), NULL);
-            for (int q = 1; q < pj_size(y); q++)
-                if (pj_number(pj_at(y, q), NULL) != y0) cli_die("time-varying covariates are not supported by this build yet");
-            P->fixed[k * N_CAC + cac] = y0;
-        }
-    }
+    P->fixed = calloc((size_t)P->n_fixed * N_CAC + 1, sizeof(double));
+    cli_load_covariates(P, pj_get(sdata, "par_fixed_values"), par_fixed, pj_get(settings, "par_fixed_obs"));
 
     /* data and times (build.c:662-881) */
     P->N = (nb_obs > 0 && nb_obs < P->N_DATA) ? nb_obs : P->N_DATA;
@@ -407,6 +478,14 @@ void cli_load(cli_problem *P, int is_bayesian, int nb_obs, double dt)
     P->remainder = pj_string(pj_get(settings, "remainder"));
     const pj *psz = pj_get(settings, "POP_SIZE_EQ_SUM_SV");
     P->has_rem = !(psz && psz->type == PJ_TRUE) && P->remainder;
+    /* the population sizes at the observation times (remainders of the X file) */
+    P->pop_t = calloc((size_t)(N + 1) * N_CAC + 1, sizeof(double));
+    double *col = calloc((size_t)N + 2, sizeof(double));
+    for (int cac = 0; P->n_fixed > 0 && cac < N_CAC; cac++) {
+        if (plom_covariate_eval(&P->cov[cac], N + 1, P->times, col) != PLOM_OK) cli_die(plom_last_error());
+        for (int n = 0; n <= N; n++) P->pop_t[n * N_CAC + cac] = col[n];
+    }
+    free(col);
 }
 
 void cli_theta_from_guess(const cli_problem *P, double *theta)
@@ -452,18 +531,7 @@ void cli_pack(const cli_problem *P, const double *theta, double *pv, double *x0)
         for (int cac = 0; cac < N_CAC; cac++) pv[o++] = cli_natural(P, i, P->R[i].map[cac], theta);
     for (int i = P->n_sv + P->n_proc; i < P->n_par; i++)
         for (int ts = 0; ts < N_TS; ts++) pv[o++] = cli_natural(P, i, P->R[i].map[ts], theta);
-    for (int k = 0; k < P->n_fixed * N_CAC; k++) pv[o++] = P->fixed[k];
-    pj *pfo = pj_get(P->settings, "par_fixed_obs");
-    pj *par_fixed = pj_get(pj_get(P->settings, "orders"), "par_fixed");
-    pj *pfv = pj_get(pj_get(P->settings, "data"), "par_fixed_values");
-    for (int q = 0; q < pj_size(pfo); q++) {
-        int k = index_of(par_fixed, pj_string(pj_at(pfo, q)));
-        pj *series = pj_get(pfv, pj_string(pj_at(pfo, q)));
-        for (int ts = 0; ts < N_TS; ts++) {
-            pj *s = pj_at(series, index_of_series(series, pj_string(pj_at(P->ts_id, ts))));
-            pv[o++] = s ? pj_number(pj_at(pj_get(s, "y"), 0), NULL) : P->fixed[k * N_CAC];
-        }
-    }
+    for (int k = 0; k < P->mi.n_fix; k++) pv[o++] = P->cov0[k];   /* covariates at t = 0 */
     pj *coupling = pj_get(pj_get(P->settings, "coupling"), "matrix");
     for (int a = 0; a < pj_size(coupling); a++)
         for (int b = 0; b < pj_size(pj_at(coupling, a)); b++) pv[o++] = pj_number(pj_at(pj_at(coupling, a), b), NULL);
@@ -540,7 +608,7 @@ void cli_x_rows(const cli_problem *P, FILE *f, int index, const double *lineage,
             for (int cac = 0; cac < N_CAC; cac++) {
                 double s = 0.0;
                 for (int i = 0; i < P->n_sv; i++) s += x[i * N_CAC + cac];
-                fprintf(f, "%g,", P->fixed[cac] - s);
+                fprintf(f, "%g,", P->pop_t[r * N_CAC + cac] - s);
             }
         for (int ts = 0; ts < N_TS; ts++) fprintf(f, "%g,", ob[2 * ts]);
         for (int ts = 0; ts < N_TS; ts++) fprintf(f, "%g%s", ob[2 * ts + 1], ts < N_TS - 1 ? "," : "");

@@ -51,7 +51,11 @@ typedef struct {
     int ind_theta_remainder;
     pj *par_sv, *cac_id, *ts_id;
     int n_fixed;
-    double *fixed;      /* [n_fixed][N_CAC] constant covariates; k = 0 is the population size */
+    double *fixed;      /* [n_fixed][N_CAC] covariates at t = 0; k = 0 is the population size */
+    plom_covariate *cov; /* [n_fix] the covariate series of the pv slots (plom_covariate) */
+    double *cov0;       /* [n_fix] their values at t = 0 */
+    int has_cov;        /* some covariate changes with time: the filters take cov */
+    double *pop_t;      /* [N + 1][N_CAC] population sizes at the observation times */
     double *times;      /* [N + 1] */
     double *data;       /* [N][N_TS], NaN = missing */
     double dt;
@@ -62,6 +66,7 @@ typedef struct {
 } cli_problem;
 
 extern int CLI_QUIET;
+extern double CLI_FREEZE;   /* -g/--freeze_forcing: covariates constant from that time on (-1: off) */
 extern int CLI_TRACE_17G;   /* trace values with 17 significant digits instead of %g */
 void cli_log(const char *msg);
 

@@ -324,7 +324,8 @@ int main(int argc, char *argv[])
         case 'y': noises_off |= PLOM_NO_ENV_STO_BIT; break;
         case 'z': noises_off |= PLOM_NO_DRIFT_BIT; break;
         case 's': dt = atof(optarg); break;
-        case 'v': case 'w': case 'g': case 'c': case 'Z': break;
+        case 'v': case 'w': case 'c': case 'Z': break;
+        case 'g': CLI_FREEZE = atof(optarg); break;
         case 'o': nb_obs = atoi(optarg); break;
         case 'a': MC.a = atof(optarg); break;
         case 'S': MC.m_switch = atoi(optarg); break;
@@ -557,7 +558,7 @@ int main(int argc, char *argv[])
         d.stream = 0;
         d.dt = P.dt;
         d.like_min = LIKE_MIN;
-        plom_filter_inputs in = {P.times, P.data, C[k].pv, C[k].x0};
+        plom_filter_inputs in = {P.times, P.data, C[k].pv, C[k].x0, NULL, P.has_cov ? P.cov : NULL};
         if (plom_filter_create(ctx, &d, &in, &fs[k]) != PLOM_OK) cli_die(plom_last_error());
         C[k].f = fs[k];
     }

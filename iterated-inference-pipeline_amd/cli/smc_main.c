@@ -82,7 +82,8 @@ int main(int argc, char *argv[])
         case 'y': noises_off |= PLOM_NO_ENV_STO_BIT; break;
         case 'z': noises_off |= PLOM_NO_DRIFT_BIT; break;
         case 's': dt = atof(optarg); break;
-        case 'v': case 'w': case 'g': break;
+        case 'v': case 'w': break;
+        case 'g': CLI_FREEZE = atof(optarg); break;
         case 'o': nb_obs = atoi(optarg); break;
         case 'h': printf("%s", help); return 1;
         case 'p': snprintf(path, sizeof path, "%s", optarg); break;
@@ -161,7 +162,7 @@ int main(int argc, char *argv[])
     d.stream = 0;
     d.dt = dt;
     d.like_min = like_min;
-    plom_filter_inputs in = {times, data, pv, x0};
+    plom_filter_inputs in = {times, data, pv, x0, NULL, P.has_cov ? P.cov : NULL};
     plom_filter_outputs out;
     memset(&out, 0, sizeof out);
     double *ess_n = calloc((size_t)N + 1, sizeof(double));

@@ -58,7 +58,7 @@ class Emitter:
             if name in lay["proc"]:
                 return self.par(lay["proc"][name] + cac)
             if name in lay["fixed"]:
-                return self.par(lay["fixed"][name] + cac)
+                return self.cov(lay["fixed"][name] + cac)
             if name == "t":
                 return "tr"
             if name == "M_PI":
@@ -95,6 +95,25 @@ class Emitter:
         """Name of parameter pv[k] inside an interval function (bound once, uniform)."""
         self.used_pars.add(k)
         return "P%d" % k
+
+    def cov(self, k):
+        """Name of the covariate in slot k of pv inside an interval function: its value at
+        the start of the substep, read from the row of the covariate table (uniform)."""
+        i = k - self.m.layout["fix_off"]
+        self.used_cov.add(i)
+        return "C%d" % i
+
+    def cov_loads(self, real):
+        """The substep's covariate values (table row cv + k cs; cs = 0 repeats one row)."""
+        if not self.used_cov:
+            return []
+        L = ["        const double *cvk = cv + (size_t)k * (size_t)cs;"]
+        for i in sorted(self.used_cov):
+            if real == "double":
+                L.append("        const double C%d = plom_cov_ld(cvk + %d);" % (i, i))
+            else:
+                L.append("        const float C%d = (float)plom_cov_ld(cvk + %d);" % (i, i))
+        return L
 
     # ---------------------------------------------------------------- interval
     def term(self, reaction, cac, from_state):
@@ -135,15 +154,16 @@ class Emitter:
         qual = "PLOM_HD" if real == "double" and not pp else dev
         L = []
         if pp:
-            L.append("%s void %s_interval_%s_pp_%s(%s *X, const %s *pv, const %s *lp, %s *ns, double t0, double t1, double dt, %s sqdt)"
+            L.append("%s void %s_interval_%s_pp_%s(%s *X, const %s *pv, const %s *lp, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs)"
                      % (qual, self.p, variant, tag, Rt, Rt, Rt, stream, Rt))
         else:
-            L.append("%s void %s_interval_%s_%s(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt)"
+            L.append("%s void %s_interval_%s_%s(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs)"
                      % (qual, self.p, variant, tag, Rt, Rt, stream, Rt))
         L.append("{")
         L.append("    const %s dtr = (%s)dt;" % (Rt, Rt))
-        L.append("    (void)sqdt; (void)ns; (void)pv;%s" % (" (void)lp;" if pp else ""))
+        L.append("    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;%s" % (" (void)lp;" if pp else ""))
         self.used_pars = set()
+        self.used_cov = set()
         bind_at = len(L)
         for u in range(m.N_TS_INC_UNIQUE):
             L.append("    X[%d] = %s; /* reset_inc */" % (m.inc_offset + u, zero))
@@ -262,7 +282,7 @@ class Emitter:
         # time-only special functions (e.g. the seasonal forcing), once per substep
         tfs = ["        const %s %s = %s;" % (Rt, name, text) for text, name in
                sorted(self.time_funcs.items(), key=lambda kv: int(kv[1][2:]))]
-        L[tf_at:tf_at] = tfs
+        L[tf_at:tf_at] = self.cov_loads(real) + tfs
         self.time_funcs = None
         uni = "PLOM_UNIFORM64" if real == "double" else "PLOM_UNIFORM32"
         npp = m.layout["npp"]
@@ -287,14 +307,15 @@ class Emitter:
         dev = "static __device__ inline __attribute__((always_inline))"
         L = []
         if pp:
-            L.append("%s void %s_interval_%s_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)"
+            L.append("%s void %s_interval_%s_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)"
                      % (dev, self.p, variant))
         else:
-            L.append("PLOM_HD void %s_interval_%s_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)"
+            L.append("PLOM_HD void %s_interval_%s_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)"
                      % (self.p, variant))
         L.append("{")
-        L.append("    (void)sqdt; (void)ns; (void)pv;%s" % (" (void)lp;" if pp else ""))
+        L.append("    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;%s" % (" (void)lp;" if pp else ""))
         self.used_pars = set()
+        self.used_cov = set()
         bind_at = len(L)
         for u in range(m.N_TS_INC_UNIQUE):
             L.append("    X[%d] = 0.0; /* reset_inc */" % (m.inc_offset + u))
@@ -397,7 +418,7 @@ class Emitter:
         L.append("}")
         tfs = ["        const double %s = %s;" % (name, text) for text, name in
                sorted(self.time_funcs.items(), key=lambda kv: int(kv[1][2:]))]
-        L[tf_at:tf_at] = tfs + decl
+        L[tf_at:tf_at] = self.cov_loads("double") + tfs + decl
         self.time_funcs = None
         npp = m.layout["npp"]
         binds = [("    const double P%d = lp[%d];" % (k, k)) if pp and k < npp else
@@ -412,11 +433,11 @@ class Emitter:
         dev = "static __device__ inline __attribute__((always_inline))"
         qual = "PLOM_HD" if real == "double" and not pp else dev
         if pp:
-            L = ["%s void %s_proj2obs_pp_%s(const %s *X, const double *pvd, const double *lpd, double *obs)"
-                 % (qual, self.p, tag, Rt), "{", "    (void)pvd; (void)lpd; (void)X;"]
+            L = ["%s void %s_proj2obs_pp_%s(const %s *X, const double *pvd, const double *lpd, const double *cvd, double *obs)"
+                 % (qual, self.p, tag, Rt), "{", "    (void)pvd; (void)lpd; (void)cvd; (void)X;"]
         else:
-            L = ["%s void %s_proj2obs_%s(const %s *X, const double *pvd, double *obs)" % (qual, self.p, tag, Rt), "{",
-                 "    (void)pvd; (void)X;"]
+            L = ["%s void %s_proj2obs_%s(const %s *X, const double *pvd, const double *cvd, double *obs)" % (qual, self.p, tag, Rt), "{",
+                 "    (void)pvd; (void)cvd; (void)X;"]
 
         ind = 0
         u = 0
@@ -455,7 +476,7 @@ class Emitter:
             if name in m.par_sv:
                 return "(double)X[%d]" % self.xidx(name, cac)
             if name in lay["fixed"]:
-                return "pvd[%d]" % (lay["fixed"][name] + cac)
+                return "cvd[%d]" % (lay["fixed"][name] + cac - lay["fix_off"])
             if name in lay["proc"]:
                 return "%s[%d]" % ("lpd" if pp else "pvd", lay["proc"][name] + cac)
             raise ValueError("symbol %r not allowed in a prevalence definition" % name)
@@ -475,7 +496,7 @@ class Emitter:
             if name in lay["obs"]:
                 return "pvd[%d]" % (lay["obs"][name] + ts)
             if name in lay["fixed_obs"]:
-                return "pvd[%d]" % (lay["fixed_obs"][name] + ts)
+                return "cvd[%d]" % (lay["fixed_obs"][name] + ts - lay["fix_off"])
             if name == "t":
                 return "t"
             if name == "M_PI":
@@ -489,8 +510,8 @@ class Emitter:
         """obs_mean / obs_var of every series (core_template.c:369-394)."""
         m = self.m
         mean_t, var_t = E.parse(m.obs_model["mean"]), E.parse(m.obs_model["var"])
-        L = ["PLOM_HD void %s_obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var)" % self.p,
-             "{", "    (void)t; (void)pvd;"]
+        L = ["PLOM_HD void %s_obs_moments(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var)" % self.p,
+             "{", "    (void)t; (void)pvd; (void)cvd;"]
         for ts in range(m.N_TS):
             em = E.CEmitter(self.obs_resolver(ts), "double")
             L.append("    {")
@@ -506,8 +527,8 @@ class Emitter:
         of each population (pop size - sum_SV, when the model has one) and the observation
         mean of every series."""
         m, lay = self.m, self.m.layout
-        L = ["PLOM_HD void %s_hat_values(const double *X, const double *pvd, double t, double *out)" % self.p,
-             "{", "    (void)t;"]
+        L = ["PLOM_HD void %s_hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out)" % self.p,
+             "{", "    (void)t; (void)pvd; (void)cvd;"]
         k = 0
         for i in range(m.N_PAR_SV * m.N_CAC):
             L.append("    out[%d] = X[%d];" % (k, i))
@@ -520,11 +541,11 @@ class Emitter:
                 expr = terms[0]
                 for t in terms[1:]:
                     expr = "(%s + %s)" % (expr, t)
-                L.append("    out[%d] = pvd[%d] - %s;" % (k, lay["fixed"]["N"] + c, expr))
+                L.append("    out[%d] = cvd[%d] - %s;" % (k, lay["fixed"]["N"] + c - lay["fix_off"], expr))
                 k += 1
         L.append("    double obs[%d], var[%d];" % (m.N_TS, m.N_TS))
-        L.append("    %s_proj2obs_f64(X, pvd, obs);" % self.p)
-        L.append("    %s_obs_moments(obs, pvd, t, out + %d, var);" % (self.p, k))
+        L.append("    %s_proj2obs_f64(X, pvd, cvd, obs);" % self.p)
+        L.append("    %s_obs_moments(obs, pvd, cvd, t, out + %d, var);" % (self.p, k))
         L.append("}")
         return "\n".join(L)
 
@@ -532,11 +553,11 @@ class Emitter:
         m, lay = self.m, self.m.layout
         if pp:
             L = ["static __device__ inline __attribute__((always_inline)) double %s_loglik_pp(const double *obs, const double *y, "
-                 "const double *pvd, const double *lpd, double t, double like_min, int *n_floor)" % self.p,
-                 "{", "    double loglike = 0.0;", "    int nf = 0;", "    (void)t; (void)pvd; (void)lpd;"]
+                 "const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor)" % self.p,
+                 "{", "    double loglike = 0.0;", "    int nf = 0;", "    (void)t; (void)pvd; (void)lpd; (void)cvd;"]
         else:
-            L = ["PLOM_HD double %s_loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor)" % self.p,
-                 "{", "    double loglike = 0.0;", "    int nf = 0;", "    (void)t; (void)pvd;"]
+            L = ["PLOM_HD double %s_loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor)" % self.p,
+                 "{", "    double loglike = 0.0;", "    int nf = 0;", "    (void)t; (void)pvd; (void)cvd;"]
         mean_t = E.parse(m.obs_model["mean"])
         var_t = E.parse(m.obs_model["var"])
         for ts in range(m.N_TS):
@@ -546,7 +567,7 @@ class Emitter:
                 if name in lay["obs"]:
                     return "%s[%d]" % ("lpd" if pp else "pvd", lay["obs"][name] + ts)
                 if name in lay["fixed_obs"]:
-                    return "pvd[%d]" % (lay["fixed_obs"][name] + ts)
+                    return "cvd[%d]" % (lay["fixed_obs"][name] + ts - lay["fix_off"])
                 if name == "t":
                     return "t"
                 if name == "M_PI":
@@ -588,7 +609,9 @@ class Emitter:
                 "#define %s_N_DEM %d" % (P, ndem),
                 "#define %s_N_ENV %d" % (P, nenv),
                 "#define %s_N_HAT %d" % (P, self.n_hat()),
-                "#define %s_N_PP %d" % (P, m.layout["npp"]), ""]
+                "#define %s_N_PP %d" % (P, m.layout["npp"]),
+                "#define %s_FIX_OFF %d" % (P, m.layout["fix_off"]),
+                "#define %s_N_FIX %d" % (P, m.layout["n_fix"]), ""]
         for v, dem, env in VARIANTS:
             out.append(self.interval(v, dem, env, "double"))
             out.append("")
@@ -605,18 +628,18 @@ class Emitter:
             out.append("")
         # host dispatcher on the reference's noises_off bits (prediction_util.c:143-161),
         # PLOM_IMPL_PSR selecting the psr implementation (prediction_util.c:162-170)
-        out.append("PLOM_HD void %s_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off)" % self.p)
+        out.append("PLOM_HD void %s_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs)" % self.p)
         out.append("{")
         out.append("    if (noises_off & PLOM_IMPL_PSR) {")
-        out.append("        if (noises_off & PLOM_NO_ENV_STO) %s_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
-        out.append("        else %s_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
+        out.append("        if (noises_off & PLOM_NO_ENV_STO) %s_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
+        out.append("        else %s_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
         out.append("        return;")
         out.append("    }")
         out.append("    const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);")
-        out.append("    if (v == 0u) %s_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
-        out.append("    else if (v == PLOM_NO_DEM_STO) %s_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
-        out.append("    else if (v == PLOM_NO_ENV_STO) %s_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
-        out.append("    else %s_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
+        out.append("    if (v == 0u) %s_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
+        out.append("    else if (v == PLOM_NO_DEM_STO) %s_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
+        out.append("    else if (v == PLOM_NO_ENV_STO) %s_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
+        out.append("    else %s_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
         out.append("}")
         out.append("")
         out.append("#if defined(__HIPCC__)")
@@ -638,42 +661,42 @@ class Emitter:
         out.append("")
         name = m.name.lower()
         out.append("struct plom_model_%s {" % name)
-        out.append("    static constexpr int S = %d, N_TS = %d, INC_OFFSET = %d, N_INC = %d, PV = %d, N_CAC = %d, N_HAT = %d, NPP = %d;"
+        out.append("    static constexpr int S = %d, N_TS = %d, INC_OFFSET = %d, N_INC = %d, PV = %d, N_CAC = %d, N_HAT = %d, NPP = %d, FIX_OFF = %d, N_FIX = %d;"
                    % (m.size_proj, m.N_TS, m.inc_offset, m.N_TS_INC_UNIQUE, m.layout["size"], m.N_CAC, self.n_hat(),
-                      m.layout["npp"]))
+                      m.layout["npp"], m.layout["fix_off"], m.layout["n_fix"]))
         for tag, Rt, stream in (("f64", "double", "plom_nstream64"), ("f32", "float", "plom_nstream32")):
-            out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt)"
+            out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs)"
                        % (Rt, Rt, stream, Rt))
             out.append("    {")
             for vi, (v, dem, env) in enumerate(VARIANTS):
                 kw = "if" if vi == 0 else "else if"
-                out.append("        %s constexpr (V == %d) %s_interval_%s_%s(X, pv, ns, t0, t1, dt, sqdt);" % (kw, vi, self.p, v, tag))
+                out.append("        %s constexpr (V == %d) %s_interval_%s_%s(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % (kw, vi, self.p, v, tag))
             if tag == "f64":
-                out.append("        else if constexpr (V == 4) %s_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
-                out.append("        else if constexpr (V == 5) %s_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);" % self.p)
+                out.append("        else if constexpr (V == 4) %s_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
+                out.append("        else if constexpr (V == 5) %s_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
             out.append("    }")
-            out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(%s *X, const %s *pv, const %s *lp, %s *ns, double t0, double t1, double dt, %s sqdt)"
+            out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(%s *X, const %s *pv, const %s *lp, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs)"
                        % (Rt, Rt, Rt, stream, Rt))
             out.append("    {")
             for vi, (v, dem, env) in enumerate(VARIANTS):
                 kw = "if" if vi == 0 else "else if"
-                out.append("        %s constexpr (V == %d) %s_interval_%s_pp_%s(X, pv, lp, ns, t0, t1, dt, sqdt);" % (kw, vi, self.p, v, tag))
+                out.append("        %s constexpr (V == %d) %s_interval_%s_pp_%s(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);" % (kw, vi, self.p, v, tag))
             if tag == "f64":
-                out.append("        else if constexpr (V == 4) %s_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);" % self.p)
-                out.append("        else if constexpr (V == 5) %s_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);" % self.p)
+                out.append("        else if constexpr (V == 4) %s_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
+                out.append("        else if constexpr (V == 5) %s_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
             out.append("    }")
-            out.append("    static __device__ inline __attribute__((always_inline)) void proj2obs(const %s *X, const double *pvd, double *obs) { %s_proj2obs_%s(X, pvd, obs); }"
+            out.append("    static __device__ inline __attribute__((always_inline)) void proj2obs(const %s *X, const double *pvd, const double *cvd, double *obs) { %s_proj2obs_%s(X, pvd, cvd, obs); }"
                        % (Rt, self.p, tag))
-            out.append("    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const %s *X, const double *pvd, const double *lpd, double *obs) { %s_proj2obs_pp_%s(X, pvd, lpd, obs); }"
+            out.append("    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const %s *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { %s_proj2obs_pp_%s(X, pvd, lpd, cvd, obs); }"
                        % (Rt, self.p, tag))
-        out.append("    static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor)"
-                   " { return %s_loglik(obs, y, pvd, t, like_min, n_floor); }" % self.p)
-        out.append("    static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, double t, double like_min, int *n_floor)"
-                   " { return %s_loglik_pp(obs, y, pvd, lpd, t, like_min, n_floor); }" % self.p)
-        out.append("    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, double t, double *out)"
-                   " { %s_hat_values(X, pvd, t, out); }" % self.p)
-        out.append("    static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var)"
-                   " { %s_obs_moments(obs, pvd, t, mean, var); }" % self.p)
+        out.append("    static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor)"
+                   " { return %s_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }" % self.p)
+        out.append("    static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor)"
+                   " { return %s_loglik_pp(obs, y, pvd, lpd, cvd, t, like_min, n_floor); }" % self.p)
+        out.append("    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out)"
+                   " { %s_hat_values(X, pvd, cvd, t, out); }" % self.p)
+        out.append("    static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var)"
+                   " { %s_obs_moments(obs, pvd, cvd, t, mean, var); }" % self.p)
         out.append("};")
         out.append("#endif")
         out.append("")

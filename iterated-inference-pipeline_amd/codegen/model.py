@@ -259,6 +259,8 @@ class PlomModel:
             lay["fixed_obs"][p] = off
             off += self.N_TS
         lay["npp"] = len(self.par_proc) * self.N_CAC + len(self.par_obs) * self.N_TS  # the per-particle slots
+        lay["fix_off"] = lay["npp"]  # the covariate slots follow them
+        lay["n_fix"] = len(self.par_fixed) * self.N_CAC + len(self.par_fixed_obs) * self.N_TS
         lay["coupling"] = off if self.coupling is not None else None
         if self.coupling is not None:
             off += self.N_CAC * self.N_CAC

@@ -11,6 +11,11 @@ served_mix           Poisson stream (200 requests/s) of SIR / SIRS requests, J l
                      [500, 5e4], N ~ U{52..520}
 seir10_j1000000_n1040  SEIR (waning immunity) with 10 gravity-coupled populations of 1e6,
                      N = 1040 weeks, J = 1e6, ancestry kept for --full
+noise                the SIR model with births, deaths and white noise of the reference's
+                     example (model_builder/example/noise), 2 cities, 4 series (three
+                     incidences, one prevalence), with time-varying covariates: population
+                     sizes, birth and death rates (per week) and proportions under
+                     surveillance, N = 104 weeks
 
 Choices the workload names leave open (initial proportions, waning and recovery rates,
 coupling matrices, seeds) are fixed here once and used by tests and bench alike.
@@ -24,15 +29,17 @@ TRUTH = {
     "sir": {"beta": 0.5, "gamma": 0.2, "rep": 0.6, "phi": 0.1},
     "sirs4": {"beta": 0.5, "e": 0.3, "gamma": 0.2, "omega": 1.0 / 365.0, "sto": 0.1, "rep": 0.6, "phi": 0.1},
     "seir10": {"beta": 0.6, "eps": 0.4, "gamma": 0.25, "omega": 1.0 / 730.0, "sto": 0.1, "rep": 0.6, "phi": 0.1},
+    "noise": {"r0": 1.8, "v": 0.1, "sto": 0.1, "rep": 0.6, "phi": 0.1},
 }
 
 X0_PROPS = {
     "sir": {"S": 0.999, "I": 0.001},
     "sirs4": {"S": [0.45, 0.46, 0.44, 0.47], "I": [0.002, 0.001, 0.003, 0.0015]},
     "seir10": {"S": [0.42 + 0.005 * k for k in range(10)], "E": [0.0005] * 10, "I": [0.001 + 0.0002 * k for k in range(10)]},
+    "noise": {"S": [0.7, 0.65], "I": [0.001, 0.0005]},
 }
 
-DATA_N = {"sir": 520, "sirs4": 520, "seir10": 1040}
+DATA_N = {"sir": 520, "sirs4": 520, "seir10": 1040, "noise": 104}
 
 WORKLOADS = {
     "sir_j1000_n52": {"model": "sir", "J": 1000, "N": 52, "precision": "fp64"},

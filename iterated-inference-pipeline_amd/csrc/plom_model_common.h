@@ -40,6 +40,19 @@ static __device__ inline __attribute__((always_inline)) float plom_uniform32(flo
 #define PLOM_UNIFORM_INT(x) (x)
 #endif
 
+/* one covariate value of the current substep (every lane reads the same address: a scalar
+ * load through the constant address space on the device) */
+#if defined(__HIP_DEVICE_COMPILE__)
+static __device__ inline __attribute__((always_inline)) double plom_cov_ld(const double *p)
+{
+    return *(const __attribute__((address_space(4))) double *)p;
+}
+#elif defined(__HIPCC__)
+static __host__ __device__ inline double plom_cov_ld(const double *p) { return *p; }
+#else
+static inline double plom_cov_ld(const double *p) { return *p; }
+#endif
+
 /* number of Euler substeps of an interval: the iterations of
  * t = t0; while (t < t1) t += dt;  (prediction_util.c:275-287, t accumulated in double) */
 PLOM_HD int plom_substeps(double t0, double t1, double dt)

@@ -41,6 +41,7 @@
 
 #include "plom_smc.h"
 #include "plom_math.h"
+#include "plom_cov.h"
 #include "plom_rng.h"
 #include "plom_mif.h"
 #include "plom_models.h"
@@ -81,6 +82,9 @@ struct plom_dfilter {
     const void *x0;
     const double *pv;
     const float *pvf;
+    const double *cov;  /* covariate table [N kmax + N + 1][cov_cs] (plom_cov.h), or the covariate
+                           slots of pv with cov_cs = 0 (constant covariates) */
+    int32_t cov_cs, cov_kmax;
     const double *times, *data;
     const int32_t *nonan;
     double *logw, *tmax, *tsum, *tsum2;
@@ -110,6 +114,23 @@ struct plom_dfilter {
     double *mpw, *mps, *mpm, *mpq;  /* tile partials: [T] sum w, [n_theta][T] sum w theta, mean, M2 */
     double *mmean, *mvar;           /* [(N+1)][n_theta] weighted mean and variance of theta */
 };
+
+/* ------------------------------------------------------------------ covariate rows */
+/* the rows of interval n's substeps (the same in every lane: moved to scalar registers so
+ * that the generated code reads the values with scalar loads) */
+__device__ inline const double *cov_steps(const plom_dfilter *f, int n)
+{
+    const uint64_t u = (uint64_t)(f->cov + (size_t)n * (size_t)f->cov_kmax * (size_t)f->cov_cs);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+    return (const double *)(((uint64_t)hi << 32) | lo);
+}
+
+/* the row of observation time times[r], 0 <= r <= N */
+__device__ inline const double *cov_at(const plom_dfilter *f, int r)
+{
+    return f->cov + ((size_t)f->N * (size_t)f->cov_kmax + (size_t)r) * (size_t)f->cov_cs;
+}
 
 /* ------------------------------------------------------------------ block helpers */
 /* recursive contiguous pairwise sum of the 256 values of a block (every lane gets it) */
@@ -330,8 +351,8 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
         if constexpr (sizeof(R) == 8) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j, s_bm);
-            if constexpr (MIF) M::template interval_pp<V>((double *)X, f->pv, lp, &ns, t0, t1, f->dt, f->sqdt);
-            else M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt);
+            if constexpr (MIF) M::template interval_pp<V>((double *)X, f->pv, lp, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs);
+            else M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs);
         } else {
             plom_nstream32 ns;
             plom_nstream32_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j);
@@ -339,9 +360,9 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
                 float lpf[M::NPP];
 #pragma unroll
                 for (int k = 0; k < M::NPP; k++) lpf[k] = (float)lp[k];
-                M::template interval_pp<V>((float *)X, f->pvf, lpf, &ns, t0, t1, f->dt, (float)f->sqdt);
+                M::template interval_pp<V>((float *)X, f->pvf, lpf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs);
             } else {
-                M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt);
+                M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs);
             }
         }
         R *Xout = (R *)f->X[(n + 1) & 1];
@@ -355,12 +376,13 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
     if (valid) {
         double obs[M::N_TS];
         int nf = 0;
+        const double *cvd = cov_at(f, n + 1);
         if constexpr (MIF) {
-            M::proj2obs_pp(X, f->pv, lp, obs);
-            l = M::loglik_pp(obs, f->data + (size_t)n * f->n_ts, f->pv, lp, t1, f->like_min, &nf);
+            M::proj2obs_pp(X, f->pv, lp, cvd, obs);
+            l = M::loglik_pp(obs, f->data + (size_t)n * f->n_ts, f->pv, lp, cvd, t1, f->like_min, &nf);
         } else {
-            M::proj2obs(X, f->pv, obs);
-            l = M::loglik(obs, f->data + (size_t)n * f->n_ts, f->pv, t1, f->like_min, &nf);
+            M::proj2obs(X, f->pv, cvd, obs);
+            l = M::loglik(obs, f->data + (size_t)n * f->n_ts, f->pv, cvd, t1, f->like_min, &nf);
         }
         /* failure: every series observed and floored at LIKE_MIN (smc.c:44) */
         if (nf == M::N_TS) l = -PLOM_INF;
@@ -648,11 +670,11 @@ __global__ __launch_bounds__(64) void k_lineage(plom_dfilter *f)
         if constexpr (sizeof(R) == 8) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, jn, s_bm);
-            M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt);
+            M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs);
         } else {
             plom_nstream32 ns;
             plom_nstream32_init(&ns, f->seed, f->stream, (uint32_t)n, jn);
-            M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt);
+            M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs);
         }
 #pragma unroll
         for (int k = 0; k < M::S; k++) f->lin_state[(size_t)(n + 1) * M::S + k] = (double)X[k];
@@ -663,8 +685,8 @@ __global__ __launch_bounds__(64) void k_lineage(plom_dfilter *f)
         double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
 #pragma unroll
         for (int k = 0; k < M::S; k++) xs[k] = f->lin_state[(size_t)n * M::S + k];
-        M::proj2obs(xs, f->pv, obs);
-        M::obs_moments(obs, f->pv, f->times[n], mu, var);
+        M::proj2obs(xs, f->pv, cov_at(f, n), obs);
+        M::obs_moments(obs, f->pv, cov_at(f, n), f->times[n], mu, var);
         plom_lineage_obs(f->seed, f->stream, (uint32_t)n, M::N_TS, mu, var, plom_bm_tab, f->lin_obs + (size_t)n * M::N_TS * 2);
     }
 }
@@ -718,7 +740,7 @@ __global__ __launch_bounds__(TILE) void k_hat_tiles(plom_dfilter *f)
         double xs[M::S];
 #pragma unroll
         for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
-        M::hat_values(xs, f->pv, f->times[n + 1], v);
+        M::hat_values(xs, f->pv, cov_at(f, n + 1), f->times[n + 1], v);
         if (hat_weighted(f, n)) {
             const double l = f->logw[j];
             if (l != -PLOM_INF) q = (uint64_t)((plom_exp(l - f->scal[0]) / f->scal[1]) * 0x1p62);
@@ -867,8 +889,8 @@ __global__ __launch_bounds__(TILE) void k_pred_res(plom_dfilter *f)
         double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
 #pragma unroll
         for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
-        M::proj2obs(xs, f->pv, obs);
-        M::obs_moments(obs, f->pv, t1, mu, var);
+        M::proj2obs(xs, f->pv, cov_at(f, n + 1), obs);
+        M::obs_moments(obs, f->pv, cov_at(f, n + 1), t1, mu, var);
 #pragma unroll
         for (int ts = 0; ts < M::N_TS; ts++) { a[ts] += (double)o * mu[ts]; b[ts] += (double)o * var[ts]; }
     }
@@ -891,8 +913,8 @@ __global__ __launch_bounds__(TILE) void k_pred_res(plom_dfilter *f)
         double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
 #pragma unroll
         for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
-        M::proj2obs(xs, f->pv, obs);
-        M::obs_moments(obs, f->pv, t1, mu, var);
+        M::proj2obs(xs, f->pv, cov_at(f, n + 1), obs);
+        M::obs_moments(obs, f->pv, cov_at(f, n + 1), t1, mu, var);
 #pragma unroll
         for (int ts = 0; ts < M::N_TS; ts++) {
             const double d = mu[ts] - pred[ts];
@@ -919,7 +941,7 @@ __global__ void k_hat0(plom_dfilter *f)
     double xs[M::S], v[M::N_HAT];
 #pragma unroll
     for (int k = 0; k < M::S; k++) xs[k] = (double)x0[k];
-    M::hat_values(xs, f->pv, f->times[0], v);
+    M::hat_values(xs, f->pv, cov_at(f, 0), f->times[0], v);
     for (int k = 0; k < M::N_HAT; k++) f->hat[3 * k] = f->hat[3 * k + 1] = f->hat[3 * k + 2] = v[k];
 }
 
@@ -1041,7 +1063,7 @@ static void launch_hat0(int prec, hipStream_t s, plom_dfilter *df)
     { #lname,                                                                                   \
       { UNAME##_SIZE_PROJ, UNAME##_N_TS, UNAME##_N_CAC, UNAME##_N_PAR_SV, UNAME##_INC_OFFSET,   \
         UNAME##_N_TS_INC_UNIQUE, UNAME##_PV_SIZE, UNAME##_N_DEM, UNAME##_N_ENV, UNAME##_N_HAT,    \
-        UNAME##_N_PP },                                                                         \
+        UNAME##_N_PP, UNAME##_FIX_OFF, UNAME##_N_FIX },                                                                      \
       launch_prop<plom_model_##lname>, launch_lineage<plom_model_##lname>, launch_hat<plom_model_##lname>, \
       launch_hat0<plom_model_##lname> }
 
@@ -1049,6 +1071,7 @@ static const model_entry g_models[] = {
     MODEL_ENTRY(sir, PLOM_SIR),
     MODEL_ENTRY(sirs4, PLOM_SIRS4),
     MODEL_ENTRY(seir10, PLOM_SEIR10),
+    MODEL_ENTRY(noise, PLOM_NOISE),
 };
 static const int g_n_models = (int)(sizeof g_models / sizeof g_models[0]);
 
@@ -1133,10 +1156,11 @@ struct layout {
         off_th0, off_th1, off_mpw, off_mps, off_mpm, off_mpq, off_mmean, off_mvar,
         off_img, off_df, off_cnt, off_scal, off_naf, off_llike_n, off_ess_n, off_rflag, off_nonan, off_times,
         off_data, off_pv, off_pvf, off_x0, off_pm_src, off_pm_tr, off_pm_sc, off_pm_dur, off_pm_min, off_pm_max,
-        off_pm_mult, off_pm_sd, off_mutfac, total;
+        off_pm_mult, off_pm_sd, off_mutfac, off_cov, total;
 };
 
-static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, layout *L)
+/* cov_vals: values of the covariate table (0 without time-varying covariates) */
+static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, layout *L, size_t cov_vals = 0)
 {
     const size_t J = (size_t)d->J, N = (size_t)d->N;
     const size_t Jp = (J + TILE - 1) / TILE * TILE, T = Jp / TILE;
@@ -1196,6 +1220,7 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     PUT(off_pm_mult, NT * 8);
     PUT(off_pm_sd, NT * 8);
     PUT(off_mutfac, NT ? N * 8 : 0);
+    PUT(off_cov, cov_vals * 8);
 #undef PUT
     L->total = o;
     return PLOM_OK;
@@ -1217,6 +1242,22 @@ static int check_desc(const plom_filter_desc *d, plom_model_info *mi)
         if (d->n_theta < 1 || d->n_theta > 4096) return set_err(PLOM_ERR_ARG, "n_theta=%d outside [1, 4096]", d->n_theta);
         if (d->flags & (PLOM_FLAG_NO_FILTER | PLOM_FLAG_HAT | PLOM_FLAG_LINEAGE))
             return set_err(PLOM_ERR_ARG, "PLOM_FLAG_MIF does not combine with NO_FILTER, HAT or LINEAGE");
+    }
+    return PLOM_OK;
+}
+
+extern "C" int plom_covariate_eval(const plom_covariate *cov, int nt, const double *t, double *out)
+{
+    if (!cov || nt < 0 || (nt > 0 && (!t || !out))) return set_err(PLOM_ERR_ARG, "null argument");
+    char msg[256];
+    if (plom_cov_check(cov, 1, msg, sizeof msg)) return set_err(PLOM_ERR_ARG, "%s", msg);
+    std::vector<double> c((size_t)cov->n + 1);
+    if (plom_spline_prepare(cov->n, cov->x, cov->y, c.data())) return set_err(PLOM_ERR_NOMEM, "host allocation failed");
+    for (int i = 0; i < nt; i++) {
+        if (cov->n > 1 && !(t[i] >= cov->x[0] && t[i] <= cov->x[cov->n - 1]))
+            return set_err(PLOM_ERR_ARG, "time %.17g outside the covariate's knots [%.17g, %.17g]", t[i], cov->x[0],
+                           cov->x[cov->n - 1]);
+        out[i] = plom_spline_eval(cov->n, cov->x, cov->y, c.data(), t[i]);
     }
     return PLOM_OK;
 }
@@ -1335,9 +1376,20 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     }
     for (int n = 0; n < d->N; n++)
         if (!(in->times[n + 1] > in->times[n])) return set_err(PLOM_ERR_ARG, "times must increase (n=%d)", n);
+    /* time-varying covariates: their values at every substep and observation time */
+    std::vector<double> covtab;
+    int cov_kmax = 0;
+    const int cov_on = in->cov != nullptr && mi.n_fix > 0;
+    if (cov_on) {
+        cov_kmax = plom_cov_kmax(d->N, in->times, d->dt);
+        covtab.resize(plom_cov_rows(d->N, cov_kmax) * (size_t)mi.n_fix);
+        char msg[256];
+        if (plom_cov_table(in->cov, mi.n_fix, d->N, in->times, d->dt, cov_kmax, covtab.data(), msg, sizeof msg))
+            return set_err(PLOM_ERR_ARG, "%s", msg);
+    }
     HIPCHK(hipSetDevice(c->device));
     layout L;
-    make_layout(d, &mi, &L);
+    make_layout(d, &mi, &L, covtab.size());
     const int async = pinned != nullptr;
     std::vector<char> pageable;
     char *img = pinned;
@@ -1412,6 +1464,15 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     h->mpq = mif_on ? (double *)(b + L.off_mpq) : nullptr;
     h->mmean = mif_on ? (double *)(b + L.off_mmean) : nullptr;
     h->mvar = mif_on ? (double *)(b + L.off_mvar) : nullptr;
+    if (cov_on) {
+        h->cov = (const double *)(b + L.off_cov);
+        h->cov_cs = mi.n_fix;
+        h->cov_kmax = cov_kmax;
+    } else {
+        h->cov = h->pv + mi.fix_off;   /* one row, repeated: the constants in pv */
+        h->cov_cs = 0;
+        h->cov_kmax = 0;
+    }
     f->df = (plom_dfilter *)(b + L.off_df);
     f->off_pv = L.off_pv; f->off_pvf = L.off_pvf; f->off_x0 = L.off_x0;
     f->off_mutfac = L.off_mutfac;
@@ -1441,6 +1502,7 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     memcpy(at(L.off_times), in->times, (size_t)(d->N + 1) * 8);
     if (d->N > 0) memcpy(at(L.off_data), in->data, (size_t)d->N * mi.n_ts * 8);
     memcpy(at(L.off_pv), in->pv, (size_t)mi.pv_size * 8);
+    if (cov_on) memcpy(at(L.off_cov), covtab.data(), covtab.size() * 8);
     float *pvf = (float *)at(L.off_pvf);
     for (int i = 0; i < mi.pv_size; i++) pvf[i] = (float)in->pv[i];
     if (d->precision == PLOM_FP32) {
@@ -2309,6 +2371,8 @@ extern "C" int plom_server_submit(plom_server *S, const plom_filter_desc *d, con
     if (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_FINAL_STATE | PLOM_FLAG_HAT | PLOM_FLAG_MIF))
         return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods and lineages (no ancestry, final state, hat or MIF)");
     if (!in->times || !in->pv || !in->x0 || (d->N > 0 && !in->data)) return set_err(PLOM_ERR_ARG, "null input array");
+    if (in->cov && mi.n_fix > 0)
+        return set_err(PLOM_ERR_ARG, "served requests take the covariates as constants in pv (no time-varying covariates)");
     for (int n = 0; n < d->N; n++)
         if (!(in->times[n + 1] > in->times[n])) return set_err(PLOM_ERR_ARG, "times must increase (n=%d)", n);
     layout L;

@@ -4,5 +4,6 @@
 #include "plom_sir.h"
 #include "plom_sirs4.h"
 #include "plom_seir10.h"
-#define PLOM_N_MODELS 3
+#include "plom_noise.h"
+#define PLOM_N_MODELS 4
 #endif

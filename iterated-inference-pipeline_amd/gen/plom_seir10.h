@@ -14,11 +14,13 @@
 #define PLOM_SEIR10_N_ENV 1
 #define PLOM_SEIR10_N_HAT 50
 #define PLOM_SEIR10_N_PP 70
+#define PLOM_SEIR10_FIX_OFF 70
+#define PLOM_SEIR10_N_FIX 10
 
-PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -69,16 +71,6 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
     const double P47 = PLOM_UNIFORM64(pv[47]);
     const double P48 = PLOM_UNIFORM64(pv[48]);
     const double P49 = PLOM_UNIFORM64(pv[49]);
-    const double P70 = PLOM_UNIFORM64(pv[70]);
-    const double P71 = PLOM_UNIFORM64(pv[71]);
-    const double P72 = PLOM_UNIFORM64(pv[72]);
-    const double P73 = PLOM_UNIFORM64(pv[73]);
-    const double P74 = PLOM_UNIFORM64(pv[74]);
-    const double P75 = PLOM_UNIFORM64(pv[75]);
-    const double P76 = PLOM_UNIFORM64(pv[76]);
-    const double P77 = PLOM_UNIFORM64(pv[77]);
-    const double P78 = PLOM_UNIFORM64(pv[78]);
-    const double P79 = PLOM_UNIFORM64(pv[79]);
     const double P80 = PLOM_UNIFORM64(pv[80]);
     const double P81 = PLOM_UNIFORM64(pv[81]);
     const double P82 = PLOM_UNIFORM64(pv[82]);
@@ -193,6 +185,17 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
+        const double C4 = plom_cov_ld(cvk + 4);
+        const double C5 = plom_cov_ld(cvk + 5);
+        const double C6 = plom_cov_ld(cvk + 6);
+        const double C7 = plom_cov_ld(cvk + 7);
+        const double C8 = plom_cov_ld(cvk + 8);
+        const double C9 = plom_cov_ld(cvk + 9);
         double s30 = 0.0;
         double s31 = 0.0;
         double s32 = 0.0;
@@ -203,10 +206,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         double s37 = 0.0;
         double s38 = 0.0;
         double s39 = 0.0;
-        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
         const double r1_0 = P10;
         const double r2_0 = P20;
-        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const double r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
         const double d0_0 = sqdt * za0;
@@ -224,10 +227,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         s30 += ( + (r1_0*X[10]))*dtr + + plom_sqrt((r1_0*X[10]))*d1_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
-        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
         const double r1_1 = P11;
         const double r2_1 = P21;
-        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const double r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         const double d0_1 = sqdt * zb2;
         double za3, zb3;
         plom_normal_pair64(ns, &za3, &zb3);
@@ -243,10 +246,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         s31 += ( + (r1_1*X[11]))*dtr + + plom_sqrt((r1_1*X[11]))*d1_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
-        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
         const double r1_2 = P12;
         const double r2_2 = P22;
-        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const double r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         double za5, zb5;
         plom_normal_pair64(ns, &za5, &zb5);
         const double d0_2 = sqdt * za5;
@@ -264,10 +267,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         s32 += ( + (r1_2*X[12]))*dtr + + plom_sqrt((r1_2*X[12]))*d1_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         X[12] = (f12 < 0.0) ? 0.0 : f12;
-        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
         const double r1_3 = P13;
         const double r2_3 = P23;
-        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const double r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         const double d0_3 = sqdt * zb7;
         double za8, zb8;
         plom_normal_pair64(ns, &za8, &zb8);
@@ -283,10 +286,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         s33 += ( + (r1_3*X[13]))*dtr + + plom_sqrt((r1_3*X[13]))*d1_3;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[13] = (f13 < 0.0) ? 0.0 : f13;
-        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
         const double r1_4 = P14;
         const double r2_4 = P24;
-        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const double r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         double za10, zb10;
         plom_normal_pair64(ns, &za10, &zb10);
         const double d0_4 = sqdt * za10;
@@ -304,10 +307,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         s34 += ( + (r1_4*X[14]))*dtr + + plom_sqrt((r1_4*X[14]))*d1_4;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[14] = (f14 < 0.0) ? 0.0 : f14;
-        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
         const double r1_5 = P15;
         const double r2_5 = P25;
-        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const double r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         const double d0_5 = sqdt * zb12;
         double za13, zb13;
         plom_normal_pair64(ns, &za13, &zb13);
@@ -323,10 +326,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         s35 += ( + (r1_5*X[15]))*dtr + + plom_sqrt((r1_5*X[15]))*d1_5;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[15] = (f15 < 0.0) ? 0.0 : f15;
-        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
         const double r1_6 = P16;
         const double r2_6 = P26;
-        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const double r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         double za15, zb15;
         plom_normal_pair64(ns, &za15, &zb15);
         const double d0_6 = sqdt * za15;
@@ -344,10 +347,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         s36 += ( + (r1_6*X[16]))*dtr + + plom_sqrt((r1_6*X[16]))*d1_6;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[16] = (f16 < 0.0) ? 0.0 : f16;
-        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
         const double r1_7 = P17;
         const double r2_7 = P27;
-        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const double r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         const double d0_7 = sqdt * zb17;
         double za18, zb18;
         plom_normal_pair64(ns, &za18, &zb18);
@@ -363,10 +366,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         s37 += ( + (r1_7*X[17]))*dtr + + plom_sqrt((r1_7*X[17]))*d1_7;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
         X[17] = (f17 < 0.0) ? 0.0 : f17;
-        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
         const double r1_8 = P18;
         const double r2_8 = P28;
-        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const double r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         double za20, zb20;
         plom_normal_pair64(ns, &za20, &zb20);
         const double d0_8 = sqdt * za20;
@@ -384,10 +387,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         s38 += ( + (r1_8*X[18]))*dtr + + plom_sqrt((r1_8*X[18]))*d1_8;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
         X[18] = (f18 < 0.0) ? 0.0 : f18;
-        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
         const double r1_9 = P19;
         const double r2_9 = P29;
-        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const double d0_9 = sqdt * zb22;
         double za23, zb23;
         plom_normal_pair64(ns, &za23, &zb23);
@@ -437,10 +440,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
     }
 }
 
-PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -491,16 +494,6 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, pl
     const double P47 = PLOM_UNIFORM64(pv[47]);
     const double P48 = PLOM_UNIFORM64(pv[48]);
     const double P49 = PLOM_UNIFORM64(pv[49]);
-    const double P70 = PLOM_UNIFORM64(pv[70]);
-    const double P71 = PLOM_UNIFORM64(pv[71]);
-    const double P72 = PLOM_UNIFORM64(pv[72]);
-    const double P73 = PLOM_UNIFORM64(pv[73]);
-    const double P74 = PLOM_UNIFORM64(pv[74]);
-    const double P75 = PLOM_UNIFORM64(pv[75]);
-    const double P76 = PLOM_UNIFORM64(pv[76]);
-    const double P77 = PLOM_UNIFORM64(pv[77]);
-    const double P78 = PLOM_UNIFORM64(pv[78]);
-    const double P79 = PLOM_UNIFORM64(pv[79]);
     const double P80 = PLOM_UNIFORM64(pv[80]);
     const double P81 = PLOM_UNIFORM64(pv[81]);
     const double P82 = PLOM_UNIFORM64(pv[82]);
@@ -615,6 +608,17 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, pl
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
+        const double C4 = plom_cov_ld(cvk + 4);
+        const double C5 = plom_cov_ld(cvk + 5);
+        const double C6 = plom_cov_ld(cvk + 6);
+        const double C7 = plom_cov_ld(cvk + 7);
+        const double C8 = plom_cov_ld(cvk + 8);
+        const double C9 = plom_cov_ld(cvk + 9);
         double s30 = 0.0;
         double s31 = 0.0;
         double s32 = 0.0;
@@ -625,10 +629,10 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, pl
         double s37 = 0.0;
         double s38 = 0.0;
         double s39 = 0.0;
-        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
         const double r1_0 = P10;
         const double r2_0 = P20;
-        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const double r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
         const double w0_0 = sqdt * za0;
@@ -638,10 +642,10 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, pl
         s30 += ( + (r1_0*X[10]))*dtr;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
-        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
         const double r1_1 = P11;
         const double r2_1 = P21;
-        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const double r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         const double w0_1 = sqdt * zb0;
         const double f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr + - (r0_1*X[1])*P41*w0_1;
         const double f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P41*w0_1;
@@ -649,10 +653,10 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, pl
         s31 += ( + (r1_1*X[11]))*dtr;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
-        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
         const double r1_2 = P12;
         const double r2_2 = P22;
-        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const double r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         double za1, zb1;
         plom_normal_pair64(ns, &za1, &zb1);
         const double w0_2 = sqdt * za1;
@@ -662,10 +666,10 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, pl
         s32 += ( + (r1_2*X[12]))*dtr;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         X[12] = (f12 < 0.0) ? 0.0 : f12;
-        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
         const double r1_3 = P13;
         const double r2_3 = P23;
-        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const double r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         const double w0_3 = sqdt * zb1;
         const double f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr + - (r0_3*X[3])*P43*w0_3;
         const double f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr + + (r0_3*X[3])*P43*w0_3;
@@ -673,10 +677,10 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, pl
         s33 += ( + (r1_3*X[13]))*dtr;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[13] = (f13 < 0.0) ? 0.0 : f13;
-        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
         const double r1_4 = P14;
         const double r2_4 = P24;
-        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const double r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
         const double w0_4 = sqdt * za2;
@@ -686,10 +690,10 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, pl
         s34 += ( + (r1_4*X[14]))*dtr;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[14] = (f14 < 0.0) ? 0.0 : f14;
-        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
         const double r1_5 = P15;
         const double r2_5 = P25;
-        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const double r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         const double w0_5 = sqdt * zb2;
         const double f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr + - (r0_5*X[5])*P45*w0_5;
         const double f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr + + (r0_5*X[5])*P45*w0_5;
@@ -697,10 +701,10 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, pl
         s35 += ( + (r1_5*X[15]))*dtr;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[15] = (f15 < 0.0) ? 0.0 : f15;
-        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
         const double r1_6 = P16;
         const double r2_6 = P26;
-        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const double r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         double za3, zb3;
         plom_normal_pair64(ns, &za3, &zb3);
         const double w0_6 = sqdt * za3;
@@ -710,10 +714,10 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, pl
         s36 += ( + (r1_6*X[16]))*dtr;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[16] = (f16 < 0.0) ? 0.0 : f16;
-        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
         const double r1_7 = P17;
         const double r2_7 = P27;
-        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const double r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         const double w0_7 = sqdt * zb3;
         const double f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr + - (r0_7*X[7])*P47*w0_7;
         const double f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr + + (r0_7*X[7])*P47*w0_7;
@@ -721,10 +725,10 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, pl
         s37 += ( + (r1_7*X[17]))*dtr;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
         X[17] = (f17 < 0.0) ? 0.0 : f17;
-        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
         const double r1_8 = P18;
         const double r2_8 = P28;
-        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const double r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         double za4, zb4;
         plom_normal_pair64(ns, &za4, &zb4);
         const double w0_8 = sqdt * za4;
@@ -734,10 +738,10 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, pl
         s38 += ( + (r1_8*X[18]))*dtr;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
         X[18] = (f18 < 0.0) ? 0.0 : f18;
-        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
         const double r1_9 = P19;
         const double r2_9 = P29;
-        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const double w0_9 = sqdt * zb4;
         const double f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr + - (r0_9*X[9])*P49*w0_9;
         const double f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr + + (r0_9*X[9])*P49*w0_9;
@@ -779,10 +783,10 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, pl
     }
 }
 
-PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -823,16 +827,6 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
     const double P37 = PLOM_UNIFORM64(pv[37]);
     const double P38 = PLOM_UNIFORM64(pv[38]);
     const double P39 = PLOM_UNIFORM64(pv[39]);
-    const double P70 = PLOM_UNIFORM64(pv[70]);
-    const double P71 = PLOM_UNIFORM64(pv[71]);
-    const double P72 = PLOM_UNIFORM64(pv[72]);
-    const double P73 = PLOM_UNIFORM64(pv[73]);
-    const double P74 = PLOM_UNIFORM64(pv[74]);
-    const double P75 = PLOM_UNIFORM64(pv[75]);
-    const double P76 = PLOM_UNIFORM64(pv[76]);
-    const double P77 = PLOM_UNIFORM64(pv[77]);
-    const double P78 = PLOM_UNIFORM64(pv[78]);
-    const double P79 = PLOM_UNIFORM64(pv[79]);
     const double P80 = PLOM_UNIFORM64(pv[80]);
     const double P81 = PLOM_UNIFORM64(pv[81]);
     const double P82 = PLOM_UNIFORM64(pv[82]);
@@ -947,6 +941,17 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
+        const double C4 = plom_cov_ld(cvk + 4);
+        const double C5 = plom_cov_ld(cvk + 5);
+        const double C6 = plom_cov_ld(cvk + 6);
+        const double C7 = plom_cov_ld(cvk + 7);
+        const double C8 = plom_cov_ld(cvk + 8);
+        const double C9 = plom_cov_ld(cvk + 9);
         double s30 = 0.0;
         double s31 = 0.0;
         double s32 = 0.0;
@@ -957,10 +962,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         double s37 = 0.0;
         double s38 = 0.0;
         double s39 = 0.0;
-        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
         const double r1_0 = P10;
         const double r2_0 = P20;
-        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const double r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
         const double d0_0 = sqdt * za0;
@@ -975,10 +980,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         s30 += ( + (r1_0*X[10]))*dtr + + plom_sqrt((r1_0*X[10]))*d1_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
-        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
         const double r1_1 = P11;
         const double r2_1 = P21;
-        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const double r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
         const double d0_1 = sqdt * za2;
@@ -993,10 +998,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         s31 += ( + (r1_1*X[11]))*dtr + + plom_sqrt((r1_1*X[11]))*d1_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
-        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
         const double r1_2 = P12;
         const double r2_2 = P22;
-        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const double r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         double za4, zb4;
         plom_normal_pair64(ns, &za4, &zb4);
         const double d0_2 = sqdt * za4;
@@ -1011,10 +1016,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         s32 += ( + (r1_2*X[12]))*dtr + + plom_sqrt((r1_2*X[12]))*d1_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         X[12] = (f12 < 0.0) ? 0.0 : f12;
-        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
         const double r1_3 = P13;
         const double r2_3 = P23;
-        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const double r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         double za6, zb6;
         plom_normal_pair64(ns, &za6, &zb6);
         const double d0_3 = sqdt * za6;
@@ -1029,10 +1034,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         s33 += ( + (r1_3*X[13]))*dtr + + plom_sqrt((r1_3*X[13]))*d1_3;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[13] = (f13 < 0.0) ? 0.0 : f13;
-        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
         const double r1_4 = P14;
         const double r2_4 = P24;
-        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const double r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         double za8, zb8;
         plom_normal_pair64(ns, &za8, &zb8);
         const double d0_4 = sqdt * za8;
@@ -1047,10 +1052,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         s34 += ( + (r1_4*X[14]))*dtr + + plom_sqrt((r1_4*X[14]))*d1_4;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[14] = (f14 < 0.0) ? 0.0 : f14;
-        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
         const double r1_5 = P15;
         const double r2_5 = P25;
-        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const double r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         double za10, zb10;
         plom_normal_pair64(ns, &za10, &zb10);
         const double d0_5 = sqdt * za10;
@@ -1065,10 +1070,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         s35 += ( + (r1_5*X[15]))*dtr + + plom_sqrt((r1_5*X[15]))*d1_5;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[15] = (f15 < 0.0) ? 0.0 : f15;
-        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
         const double r1_6 = P16;
         const double r2_6 = P26;
-        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const double r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         double za12, zb12;
         plom_normal_pair64(ns, &za12, &zb12);
         const double d0_6 = sqdt * za12;
@@ -1083,10 +1088,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         s36 += ( + (r1_6*X[16]))*dtr + + plom_sqrt((r1_6*X[16]))*d1_6;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[16] = (f16 < 0.0) ? 0.0 : f16;
-        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
         const double r1_7 = P17;
         const double r2_7 = P27;
-        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const double r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         double za14, zb14;
         plom_normal_pair64(ns, &za14, &zb14);
         const double d0_7 = sqdt * za14;
@@ -1101,10 +1106,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         s37 += ( + (r1_7*X[17]))*dtr + + plom_sqrt((r1_7*X[17]))*d1_7;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
         X[17] = (f17 < 0.0) ? 0.0 : f17;
-        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
         const double r1_8 = P18;
         const double r2_8 = P28;
-        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const double r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         double za16, zb16;
         plom_normal_pair64(ns, &za16, &zb16);
         const double d0_8 = sqdt * za16;
@@ -1119,10 +1124,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         s38 += ( + (r1_8*X[18]))*dtr + + plom_sqrt((r1_8*X[18]))*d1_8;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
         X[18] = (f18 < 0.0) ? 0.0 : f18;
-        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
         const double r1_9 = P19;
         const double r2_9 = P29;
-        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         double za18, zb18;
         plom_normal_pair64(ns, &za18, &zb18);
         const double d0_9 = sqdt * za18;
@@ -1171,10 +1176,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
     }
 }
 
-PLOM_HD void plom_seir10_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_seir10_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -1215,16 +1220,6 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_no_env_sto_f64(double *X, const dou
     const double P37 = PLOM_UNIFORM64(pv[37]);
     const double P38 = PLOM_UNIFORM64(pv[38]);
     const double P39 = PLOM_UNIFORM64(pv[39]);
-    const double P70 = PLOM_UNIFORM64(pv[70]);
-    const double P71 = PLOM_UNIFORM64(pv[71]);
-    const double P72 = PLOM_UNIFORM64(pv[72]);
-    const double P73 = PLOM_UNIFORM64(pv[73]);
-    const double P74 = PLOM_UNIFORM64(pv[74]);
-    const double P75 = PLOM_UNIFORM64(pv[75]);
-    const double P76 = PLOM_UNIFORM64(pv[76]);
-    const double P77 = PLOM_UNIFORM64(pv[77]);
-    const double P78 = PLOM_UNIFORM64(pv[78]);
-    const double P79 = PLOM_UNIFORM64(pv[79]);
     const double P80 = PLOM_UNIFORM64(pv[80]);
     const double P81 = PLOM_UNIFORM64(pv[81]);
     const double P82 = PLOM_UNIFORM64(pv[82]);
@@ -1339,6 +1334,17 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_no_env_sto_f64(double *X, const dou
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
+        const double C4 = plom_cov_ld(cvk + 4);
+        const double C5 = plom_cov_ld(cvk + 5);
+        const double C6 = plom_cov_ld(cvk + 6);
+        const double C7 = plom_cov_ld(cvk + 7);
+        const double C8 = plom_cov_ld(cvk + 8);
+        const double C9 = plom_cov_ld(cvk + 9);
         double s30 = 0.0;
         double s31 = 0.0;
         double s32 = 0.0;
@@ -1349,100 +1355,100 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_no_env_sto_f64(double *X, const dou
         double s37 = 0.0;
         double s38 = 0.0;
         double s39 = 0.0;
-        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
         const double r1_0 = P10;
         const double r2_0 = P20;
-        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const double r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         const double f0 = X[0] + ( - (r0_0*X[0]) + (r3_0))*dtr;
         const double f10 = X[10] + ( - (r1_0*X[10]) + (r0_0*X[0]))*dtr;
         const double f20 = X[20] + ( - (r2_0*X[20]) + (r1_0*X[10]))*dtr;
         s30 += ( + (r1_0*X[10]))*dtr;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
-        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
         const double r1_1 = P11;
         const double r2_1 = P21;
-        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const double r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         const double f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr;
         const double f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr;
         const double f21 = X[21] + ( - (r2_1*X[21]) + (r1_1*X[11]))*dtr;
         s31 += ( + (r1_1*X[11]))*dtr;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
-        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
         const double r1_2 = P12;
         const double r2_2 = P22;
-        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const double r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         const double f2 = X[2] + ( - (r0_2*X[2]) + (r3_2))*dtr;
         const double f12 = X[12] + ( - (r1_2*X[12]) + (r0_2*X[2]))*dtr;
         const double f22 = X[22] + ( - (r2_2*X[22]) + (r1_2*X[12]))*dtr;
         s32 += ( + (r1_2*X[12]))*dtr;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         X[12] = (f12 < 0.0) ? 0.0 : f12;
-        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
         const double r1_3 = P13;
         const double r2_3 = P23;
-        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const double r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         const double f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr;
         const double f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr;
         const double f23 = X[23] + ( - (r2_3*X[23]) + (r1_3*X[13]))*dtr;
         s33 += ( + (r1_3*X[13]))*dtr;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[13] = (f13 < 0.0) ? 0.0 : f13;
-        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
         const double r1_4 = P14;
         const double r2_4 = P24;
-        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const double r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         const double f4 = X[4] + ( - (r0_4*X[4]) + (r3_4))*dtr;
         const double f14 = X[14] + ( - (r1_4*X[14]) + (r0_4*X[4]))*dtr;
         const double f24 = X[24] + ( - (r2_4*X[24]) + (r1_4*X[14]))*dtr;
         s34 += ( + (r1_4*X[14]))*dtr;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[14] = (f14 < 0.0) ? 0.0 : f14;
-        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
         const double r1_5 = P15;
         const double r2_5 = P25;
-        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const double r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         const double f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr;
         const double f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr;
         const double f25 = X[25] + ( - (r2_5*X[25]) + (r1_5*X[15]))*dtr;
         s35 += ( + (r1_5*X[15]))*dtr;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[15] = (f15 < 0.0) ? 0.0 : f15;
-        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
         const double r1_6 = P16;
         const double r2_6 = P26;
-        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const double r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         const double f6 = X[6] + ( - (r0_6*X[6]) + (r3_6))*dtr;
         const double f16 = X[16] + ( - (r1_6*X[16]) + (r0_6*X[6]))*dtr;
         const double f26 = X[26] + ( - (r2_6*X[26]) + (r1_6*X[16]))*dtr;
         s36 += ( + (r1_6*X[16]))*dtr;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[16] = (f16 < 0.0) ? 0.0 : f16;
-        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
         const double r1_7 = P17;
         const double r2_7 = P27;
-        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const double r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         const double f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr;
         const double f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr;
         const double f27 = X[27] + ( - (r2_7*X[27]) + (r1_7*X[17]))*dtr;
         s37 += ( + (r1_7*X[17]))*dtr;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
         X[17] = (f17 < 0.0) ? 0.0 : f17;
-        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
         const double r1_8 = P18;
         const double r2_8 = P28;
-        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const double r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         const double f8 = X[8] + ( - (r0_8*X[8]) + (r3_8))*dtr;
         const double f18 = X[18] + ( - (r1_8*X[18]) + (r0_8*X[8]))*dtr;
         const double f28 = X[28] + ( - (r2_8*X[28]) + (r1_8*X[18]))*dtr;
         s38 += ( + (r1_8*X[18]))*dtr;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
         X[18] = (f18 < 0.0) ? 0.0 : f18;
-        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
         const double r1_9 = P19;
         const double r2_9 = P29;
-        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const double f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr;
         const double f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr;
         const double f29 = X[29] + ( - (r2_9*X[29]) + (r1_9*X[19]))*dtr;
@@ -1483,9 +1489,9 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_no_env_sto_f64(double *X, const dou
     }
 }
 
-PLOM_HD void plom_seir10_proj2obs_f64(const double *X, const double *pvd, double *obs)
+PLOM_HD void plom_seir10_proj2obs_f64(const double *X, const double *pvd, const double *cvd, double *obs)
 {
-    (void)pvd; (void)X;
+    (void)pvd; (void)cvd; (void)X;
     obs[0] = (double)X[30];
     obs[1] = (double)X[31];
     obs[2] = (double)X[32];
@@ -1498,11 +1504,11 @@ PLOM_HD void plom_seir10_proj2obs_f64(const double *X, const double *pvd, double
     obs[9] = (double)X[39];
 }
 
-PLOM_HD double plom_seir10_loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor)
+PLOM_HD double plom_seir10_loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor)
 {
     double loglike = 0.0;
     int nf = 0;
-    (void)t; (void)pvd;
+    (void)t; (void)pvd; (void)cvd;
     if (!plom_isnan(y[0])) {
         const double x = obs[0];
         const double gsl_mu = (pvd[50] * x);
@@ -1597,9 +1603,9 @@ PLOM_HD double plom_seir10_loglik(const double *obs, const double *y, const doub
     return loglike;
 }
 
-PLOM_HD void plom_seir10_obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var)
+PLOM_HD void plom_seir10_obs_moments(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var)
 {
-    (void)t; (void)pvd;
+    (void)t; (void)pvd; (void)cvd;
     {
         const double x = obs[0];
         mean[0] = (pvd[50] * x);
@@ -1652,9 +1658,9 @@ PLOM_HD void plom_seir10_obs_moments(const double *obs, const double *pvd, doubl
     }
 }
 
-PLOM_HD void plom_seir10_hat_values(const double *X, const double *pvd, double t, double *out)
+PLOM_HD void plom_seir10_hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out)
 {
-    (void)t;
+    (void)t; (void)pvd; (void)cvd;
     out[0] = X[0];
     out[1] = X[1];
     out[2] = X[2];
@@ -1685,24 +1691,24 @@ PLOM_HD void plom_seir10_hat_values(const double *X, const double *pvd, double t
     out[27] = X[27];
     out[28] = X[28];
     out[29] = X[29];
-    out[30] = pvd[70] - ((X[0] + X[10]) + X[20]);
-    out[31] = pvd[71] - ((X[1] + X[11]) + X[21]);
-    out[32] = pvd[72] - ((X[2] + X[12]) + X[22]);
-    out[33] = pvd[73] - ((X[3] + X[13]) + X[23]);
-    out[34] = pvd[74] - ((X[4] + X[14]) + X[24]);
-    out[35] = pvd[75] - ((X[5] + X[15]) + X[25]);
-    out[36] = pvd[76] - ((X[6] + X[16]) + X[26]);
-    out[37] = pvd[77] - ((X[7] + X[17]) + X[27]);
-    out[38] = pvd[78] - ((X[8] + X[18]) + X[28]);
-    out[39] = pvd[79] - ((X[9] + X[19]) + X[29]);
+    out[30] = cvd[0] - ((X[0] + X[10]) + X[20]);
+    out[31] = cvd[1] - ((X[1] + X[11]) + X[21]);
+    out[32] = cvd[2] - ((X[2] + X[12]) + X[22]);
+    out[33] = cvd[3] - ((X[3] + X[13]) + X[23]);
+    out[34] = cvd[4] - ((X[4] + X[14]) + X[24]);
+    out[35] = cvd[5] - ((X[5] + X[15]) + X[25]);
+    out[36] = cvd[6] - ((X[6] + X[16]) + X[26]);
+    out[37] = cvd[7] - ((X[7] + X[17]) + X[27]);
+    out[38] = cvd[8] - ((X[8] + X[18]) + X[28]);
+    out[39] = cvd[9] - ((X[9] + X[19]) + X[29]);
     double obs[10], var[10];
-    plom_seir10_proj2obs_f64(X, pvd, obs);
-    plom_seir10_obs_moments(obs, pvd, t, out + 40, var);
+    plom_seir10_proj2obs_f64(X, pvd, cvd, obs);
+    plom_seir10_obs_moments(obs, pvd, cvd, t, out + 40, var);
 }
 
-PLOM_HD void plom_seir10_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_seir10_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -1753,16 +1759,6 @@ PLOM_HD void plom_seir10_interval_psr_full_f64(double *X, const double *pv, plom
     const double P47 = PLOM_UNIFORM64(pv[47]);
     const double P48 = PLOM_UNIFORM64(pv[48]);
     const double P49 = PLOM_UNIFORM64(pv[49]);
-    const double P70 = PLOM_UNIFORM64(pv[70]);
-    const double P71 = PLOM_UNIFORM64(pv[71]);
-    const double P72 = PLOM_UNIFORM64(pv[72]);
-    const double P73 = PLOM_UNIFORM64(pv[73]);
-    const double P74 = PLOM_UNIFORM64(pv[74]);
-    const double P75 = PLOM_UNIFORM64(pv[75]);
-    const double P76 = PLOM_UNIFORM64(pv[76]);
-    const double P77 = PLOM_UNIFORM64(pv[77]);
-    const double P78 = PLOM_UNIFORM64(pv[78]);
-    const double P79 = PLOM_UNIFORM64(pv[79]);
     const double P80 = PLOM_UNIFORM64(pv[80]);
     const double P81 = PLOM_UNIFORM64(pv[81]);
     const double P82 = PLOM_UNIFORM64(pv[82]);
@@ -1877,6 +1873,17 @@ PLOM_HD void plom_seir10_interval_psr_full_f64(double *X, const double *pv, plom
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
+        const double C4 = plom_cov_ld(cvk + 4);
+        const double C5 = plom_cov_ld(cvk + 5);
+        const double C6 = plom_cov_ld(cvk + 6);
+        const double C7 = plom_cov_ld(cvk + 7);
+        const double C8 = plom_cov_ld(cvk + 8);
+        const double C9 = plom_cov_ld(cvk + 9);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -1948,46 +1955,46 @@ PLOM_HD void plom_seir10_interval_psr_full_f64(double *X, const double *pv, plom
         double stay28 = 0.0;
         double stay29 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n3_2; (void)n0_3; (void)n1_3; (void)n2_3; (void)n3_3; (void)n0_4; (void)n1_4; (void)n2_4; (void)n3_4; (void)n0_5; (void)n1_5; (void)n2_5; (void)n3_5; (void)n0_6; (void)n1_6; (void)n2_6; (void)n3_6; (void)n0_7; (void)n1_7; (void)n2_7; (void)n3_7; (void)n0_8; (void)n1_8; (void)n2_8; (void)n3_8; (void)n0_9; (void)n1_9; (void)n2_9; (void)n3_9;
-        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
         const double r1_0 = P10;
         const double r2_0 = P20;
-        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
-        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
         const double r1_1 = P11;
         const double r2_1 = P21;
-        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
-        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
         const double r1_2 = P12;
         const double r2_2 = P22;
-        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
-        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
         const double r1_3 = P13;
         const double r2_3 = P23;
-        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
-        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
         const double r1_4 = P14;
         const double r2_4 = P24;
-        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
-        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
         const double r1_5 = P15;
         const double r2_5 = P25;
-        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
-        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
         const double r1_6 = P16;
         const double r2_6 = P26;
-        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
-        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
         const double r1_7 = P17;
         const double r2_7 = P27;
-        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
-        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
         const double r1_8 = P18;
         const double r2_8 = P28;
-        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
-        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
         const double r1_9 = P19;
         const double r2_9 = P29;
-        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const double sd0_0 = P40;
         const double v0_0 = sd0_0 * sd0_0;
         const double xi0_0 = (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt;
@@ -2522,9 +2529,9 @@ PLOM_HD void plom_seir10_interval_psr_full_f64(double *X, const double *pv, plom
     }
 }
 
-PLOM_HD void plom_seir10_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_seir10_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -2565,16 +2572,6 @@ PLOM_HD void plom_seir10_interval_psr_no_env_sto_f64(double *X, const double *pv
     const double P37 = PLOM_UNIFORM64(pv[37]);
     const double P38 = PLOM_UNIFORM64(pv[38]);
     const double P39 = PLOM_UNIFORM64(pv[39]);
-    const double P70 = PLOM_UNIFORM64(pv[70]);
-    const double P71 = PLOM_UNIFORM64(pv[71]);
-    const double P72 = PLOM_UNIFORM64(pv[72]);
-    const double P73 = PLOM_UNIFORM64(pv[73]);
-    const double P74 = PLOM_UNIFORM64(pv[74]);
-    const double P75 = PLOM_UNIFORM64(pv[75]);
-    const double P76 = PLOM_UNIFORM64(pv[76]);
-    const double P77 = PLOM_UNIFORM64(pv[77]);
-    const double P78 = PLOM_UNIFORM64(pv[78]);
-    const double P79 = PLOM_UNIFORM64(pv[79]);
     const double P80 = PLOM_UNIFORM64(pv[80]);
     const double P81 = PLOM_UNIFORM64(pv[81]);
     const double P82 = PLOM_UNIFORM64(pv[82]);
@@ -2689,6 +2686,17 @@ PLOM_HD void plom_seir10_interval_psr_no_env_sto_f64(double *X, const double *pv
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
+        const double C4 = plom_cov_ld(cvk + 4);
+        const double C5 = plom_cov_ld(cvk + 5);
+        const double C6 = plom_cov_ld(cvk + 6);
+        const double C7 = plom_cov_ld(cvk + 7);
+        const double C8 = plom_cov_ld(cvk + 8);
+        const double C9 = plom_cov_ld(cvk + 9);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -2760,46 +2768,46 @@ PLOM_HD void plom_seir10_interval_psr_no_env_sto_f64(double *X, const double *pv
         double stay28 = 0.0;
         double stay29 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n3_2; (void)n0_3; (void)n1_3; (void)n2_3; (void)n3_3; (void)n0_4; (void)n1_4; (void)n2_4; (void)n3_4; (void)n0_5; (void)n1_5; (void)n2_5; (void)n3_5; (void)n0_6; (void)n1_6; (void)n2_6; (void)n3_6; (void)n0_7; (void)n1_7; (void)n2_7; (void)n3_7; (void)n0_8; (void)n1_8; (void)n2_8; (void)n3_8; (void)n0_9; (void)n1_9; (void)n2_9; (void)n3_9;
-        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
         const double r1_0 = P10;
         const double r2_0 = P20;
-        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
-        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
         const double r1_1 = P11;
         const double r2_1 = P21;
-        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
-        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
         const double r1_2 = P12;
         const double r2_2 = P22;
-        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
-        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
         const double r1_3 = P13;
         const double r2_3 = P23;
-        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
-        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
         const double r1_4 = P14;
         const double r2_4 = P24;
-        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
-        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
         const double r1_5 = P15;
         const double r2_5 = P25;
-        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
-        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
         const double r1_6 = P16;
         const double r2_6 = P26;
-        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
-        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
         const double r1_7 = P17;
         const double r2_7 = P27;
-        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
-        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
         const double r1_8 = P18;
         const double r2_8 = P28;
-        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
-        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
         const double r1_9 = P19;
         const double r2_9 = P29;
-        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const double xi0_0 = 1.0;
         {
             double p[2], cnt[2];
@@ -3314,25 +3322,25 @@ PLOM_HD void plom_seir10_interval_psr_no_env_sto_f64(double *X, const double *pv
     }
 }
 
-PLOM_HD void plom_seir10_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off)
+PLOM_HD void plom_seir10_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs)
 {
     if (noises_off & PLOM_IMPL_PSR) {
-        if (noises_off & PLOM_NO_ENV_STO) plom_seir10_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else plom_seir10_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);
+        if (noises_off & PLOM_NO_ENV_STO) plom_seir10_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else plom_seir10_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
         return;
     }
     const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);
-    if (v == 0u) plom_seir10_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt);
-    else if (v == PLOM_NO_DEM_STO) plom_seir10_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-    else if (v == PLOM_NO_ENV_STO) plom_seir10_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-    else plom_seir10_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
+    if (v == 0u) plom_seir10_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+    else if (v == PLOM_NO_DEM_STO) plom_seir10_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+    else if (v == PLOM_NO_ENV_STO) plom_seir10_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+    else plom_seir10_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
 }
 
 #if defined(__HIPCC__)
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -3383,16 +3391,6 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const float P47 = PLOM_UNIFORM32(pv[47]);
     const float P48 = PLOM_UNIFORM32(pv[48]);
     const float P49 = PLOM_UNIFORM32(pv[49]);
-    const float P70 = PLOM_UNIFORM32(pv[70]);
-    const float P71 = PLOM_UNIFORM32(pv[71]);
-    const float P72 = PLOM_UNIFORM32(pv[72]);
-    const float P73 = PLOM_UNIFORM32(pv[73]);
-    const float P74 = PLOM_UNIFORM32(pv[74]);
-    const float P75 = PLOM_UNIFORM32(pv[75]);
-    const float P76 = PLOM_UNIFORM32(pv[76]);
-    const float P77 = PLOM_UNIFORM32(pv[77]);
-    const float P78 = PLOM_UNIFORM32(pv[78]);
-    const float P79 = PLOM_UNIFORM32(pv[79]);
     const float P80 = PLOM_UNIFORM32(pv[80]);
     const float P81 = PLOM_UNIFORM32(pv[81]);
     const float P82 = PLOM_UNIFORM32(pv[82]);
@@ -3507,6 +3505,17 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
+        const float C4 = (float)plom_cov_ld(cvk + 4);
+        const float C5 = (float)plom_cov_ld(cvk + 5);
+        const float C6 = (float)plom_cov_ld(cvk + 6);
+        const float C7 = (float)plom_cov_ld(cvk + 7);
+        const float C8 = (float)plom_cov_ld(cvk + 8);
+        const float C9 = (float)plom_cov_ld(cvk + 9);
         float s30 = 0.0f;
         float s31 = 0.0f;
         float s32 = 0.0f;
@@ -3517,10 +3526,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         float s37 = 0.0f;
         float s38 = 0.0f;
         float s39 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), P70);
+        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), C0);
         const float r1_0 = P10;
         const float r2_0 = P20;
-        const float r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const float r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
         const float d0_0 = sqdt * za0;
@@ -3538,10 +3547,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s30 += ( + (r1_0*X[10]))*dtr + + plom_fsqrtf((r1_0*X[10]))*d1_0;
         X[0] = plom_clipf(f0);
         X[10] = plom_clipf(f10);
-        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), P71);
+        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), C1);
         const float r1_1 = P11;
         const float r2_1 = P21;
-        const float r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const float r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         const float d0_1 = sqdt * zb2;
         float za3, zb3;
         plom_normal_pair32(ns, &za3, &zb3);
@@ -3557,10 +3566,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s31 += ( + (r1_1*X[11]))*dtr + + plom_fsqrtf((r1_1*X[11]))*d1_1;
         X[1] = plom_clipf(f1);
         X[11] = plom_clipf(f11);
-        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), P72);
+        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), C2);
         const float r1_2 = P12;
         const float r2_2 = P22;
-        const float r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const float r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         float za5, zb5;
         plom_normal_pair32(ns, &za5, &zb5);
         const float d0_2 = sqdt * za5;
@@ -3578,10 +3587,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s32 += ( + (r1_2*X[12]))*dtr + + plom_fsqrtf((r1_2*X[12]))*d1_2;
         X[2] = plom_clipf(f2);
         X[12] = plom_clipf(f12);
-        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), P73);
+        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), C3);
         const float r1_3 = P13;
         const float r2_3 = P23;
-        const float r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const float r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         const float d0_3 = sqdt * zb7;
         float za8, zb8;
         plom_normal_pair32(ns, &za8, &zb8);
@@ -3597,10 +3606,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s33 += ( + (r1_3*X[13]))*dtr + + plom_fsqrtf((r1_3*X[13]))*d1_3;
         X[3] = plom_clipf(f3);
         X[13] = plom_clipf(f13);
-        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), P74);
+        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), C4);
         const float r1_4 = P14;
         const float r2_4 = P24;
-        const float r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const float r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         float za10, zb10;
         plom_normal_pair32(ns, &za10, &zb10);
         const float d0_4 = sqdt * za10;
@@ -3618,10 +3627,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s34 += ( + (r1_4*X[14]))*dtr + + plom_fsqrtf((r1_4*X[14]))*d1_4;
         X[4] = plom_clipf(f4);
         X[14] = plom_clipf(f14);
-        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), P75);
+        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), C5);
         const float r1_5 = P15;
         const float r2_5 = P25;
-        const float r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const float r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         const float d0_5 = sqdt * zb12;
         float za13, zb13;
         plom_normal_pair32(ns, &za13, &zb13);
@@ -3637,10 +3646,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s35 += ( + (r1_5*X[15]))*dtr + + plom_fsqrtf((r1_5*X[15]))*d1_5;
         X[5] = plom_clipf(f5);
         X[15] = plom_clipf(f15);
-        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), P76);
+        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), C6);
         const float r1_6 = P16;
         const float r2_6 = P26;
-        const float r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const float r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         float za15, zb15;
         plom_normal_pair32(ns, &za15, &zb15);
         const float d0_6 = sqdt * za15;
@@ -3658,10 +3667,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s36 += ( + (r1_6*X[16]))*dtr + + plom_fsqrtf((r1_6*X[16]))*d1_6;
         X[6] = plom_clipf(f6);
         X[16] = plom_clipf(f16);
-        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), P77);
+        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), C7);
         const float r1_7 = P17;
         const float r2_7 = P27;
-        const float r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const float r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         const float d0_7 = sqdt * zb17;
         float za18, zb18;
         plom_normal_pair32(ns, &za18, &zb18);
@@ -3677,10 +3686,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s37 += ( + (r1_7*X[17]))*dtr + + plom_fsqrtf((r1_7*X[17]))*d1_7;
         X[7] = plom_clipf(f7);
         X[17] = plom_clipf(f17);
-        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), P78);
+        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), C8);
         const float r1_8 = P18;
         const float r2_8 = P28;
-        const float r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const float r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         float za20, zb20;
         plom_normal_pair32(ns, &za20, &zb20);
         const float d0_8 = sqdt * za20;
@@ -3698,10 +3707,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s38 += ( + (r1_8*X[18]))*dtr + + plom_fsqrtf((r1_8*X[18]))*d1_8;
         X[8] = plom_clipf(f8);
         X[18] = plom_clipf(f18);
-        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), P79);
+        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), C9);
         const float r1_9 = P19;
         const float r2_9 = P29;
-        const float r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const float r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const float d0_9 = sqdt * zb22;
         float za23, zb23;
         plom_normal_pair32(ns, &za23, &zb23);
@@ -3751,10 +3760,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -3805,16 +3814,6 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const float P47 = PLOM_UNIFORM32(pv[47]);
     const float P48 = PLOM_UNIFORM32(pv[48]);
     const float P49 = PLOM_UNIFORM32(pv[49]);
-    const float P70 = PLOM_UNIFORM32(pv[70]);
-    const float P71 = PLOM_UNIFORM32(pv[71]);
-    const float P72 = PLOM_UNIFORM32(pv[72]);
-    const float P73 = PLOM_UNIFORM32(pv[73]);
-    const float P74 = PLOM_UNIFORM32(pv[74]);
-    const float P75 = PLOM_UNIFORM32(pv[75]);
-    const float P76 = PLOM_UNIFORM32(pv[76]);
-    const float P77 = PLOM_UNIFORM32(pv[77]);
-    const float P78 = PLOM_UNIFORM32(pv[78]);
-    const float P79 = PLOM_UNIFORM32(pv[79]);
     const float P80 = PLOM_UNIFORM32(pv[80]);
     const float P81 = PLOM_UNIFORM32(pv[81]);
     const float P82 = PLOM_UNIFORM32(pv[82]);
@@ -3929,6 +3928,17 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
+        const float C4 = (float)plom_cov_ld(cvk + 4);
+        const float C5 = (float)plom_cov_ld(cvk + 5);
+        const float C6 = (float)plom_cov_ld(cvk + 6);
+        const float C7 = (float)plom_cov_ld(cvk + 7);
+        const float C8 = (float)plom_cov_ld(cvk + 8);
+        const float C9 = (float)plom_cov_ld(cvk + 9);
         float s30 = 0.0f;
         float s31 = 0.0f;
         float s32 = 0.0f;
@@ -3939,10 +3949,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         float s37 = 0.0f;
         float s38 = 0.0f;
         float s39 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), P70);
+        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), C0);
         const float r1_0 = P10;
         const float r2_0 = P20;
-        const float r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const float r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
         const float w0_0 = sqdt * za0;
@@ -3952,10 +3962,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s30 += ( + (r1_0*X[10]))*dtr;
         X[0] = plom_clipf(f0);
         X[10] = plom_clipf(f10);
-        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), P71);
+        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), C1);
         const float r1_1 = P11;
         const float r2_1 = P21;
-        const float r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const float r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         const float w0_1 = sqdt * zb0;
         const float f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr + - (r0_1*X[1])*P41*w0_1;
         const float f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P41*w0_1;
@@ -3963,10 +3973,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s31 += ( + (r1_1*X[11]))*dtr;
         X[1] = plom_clipf(f1);
         X[11] = plom_clipf(f11);
-        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), P72);
+        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), C2);
         const float r1_2 = P12;
         const float r2_2 = P22;
-        const float r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const float r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         float za1, zb1;
         plom_normal_pair32(ns, &za1, &zb1);
         const float w0_2 = sqdt * za1;
@@ -3976,10 +3986,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s32 += ( + (r1_2*X[12]))*dtr;
         X[2] = plom_clipf(f2);
         X[12] = plom_clipf(f12);
-        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), P73);
+        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), C3);
         const float r1_3 = P13;
         const float r2_3 = P23;
-        const float r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const float r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         const float w0_3 = sqdt * zb1;
         const float f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr + - (r0_3*X[3])*P43*w0_3;
         const float f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr + + (r0_3*X[3])*P43*w0_3;
@@ -3987,10 +3997,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s33 += ( + (r1_3*X[13]))*dtr;
         X[3] = plom_clipf(f3);
         X[13] = plom_clipf(f13);
-        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), P74);
+        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), C4);
         const float r1_4 = P14;
         const float r2_4 = P24;
-        const float r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const float r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         float za2, zb2;
         plom_normal_pair32(ns, &za2, &zb2);
         const float w0_4 = sqdt * za2;
@@ -4000,10 +4010,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s34 += ( + (r1_4*X[14]))*dtr;
         X[4] = plom_clipf(f4);
         X[14] = plom_clipf(f14);
-        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), P75);
+        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), C5);
         const float r1_5 = P15;
         const float r2_5 = P25;
-        const float r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const float r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         const float w0_5 = sqdt * zb2;
         const float f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr + - (r0_5*X[5])*P45*w0_5;
         const float f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr + + (r0_5*X[5])*P45*w0_5;
@@ -4011,10 +4021,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s35 += ( + (r1_5*X[15]))*dtr;
         X[5] = plom_clipf(f5);
         X[15] = plom_clipf(f15);
-        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), P76);
+        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), C6);
         const float r1_6 = P16;
         const float r2_6 = P26;
-        const float r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const float r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         float za3, zb3;
         plom_normal_pair32(ns, &za3, &zb3);
         const float w0_6 = sqdt * za3;
@@ -4024,10 +4034,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s36 += ( + (r1_6*X[16]))*dtr;
         X[6] = plom_clipf(f6);
         X[16] = plom_clipf(f16);
-        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), P77);
+        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), C7);
         const float r1_7 = P17;
         const float r2_7 = P27;
-        const float r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const float r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         const float w0_7 = sqdt * zb3;
         const float f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr + - (r0_7*X[7])*P47*w0_7;
         const float f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr + + (r0_7*X[7])*P47*w0_7;
@@ -4035,10 +4045,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s37 += ( + (r1_7*X[17]))*dtr;
         X[7] = plom_clipf(f7);
         X[17] = plom_clipf(f17);
-        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), P78);
+        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), C8);
         const float r1_8 = P18;
         const float r2_8 = P28;
-        const float r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const float r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         float za4, zb4;
         plom_normal_pair32(ns, &za4, &zb4);
         const float w0_8 = sqdt * za4;
@@ -4048,10 +4058,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s38 += ( + (r1_8*X[18]))*dtr;
         X[8] = plom_clipf(f8);
         X[18] = plom_clipf(f18);
-        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), P79);
+        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), C9);
         const float r1_9 = P19;
         const float r2_9 = P29;
-        const float r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const float r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const float w0_9 = sqdt * zb4;
         const float f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr + - (r0_9*X[9])*P49*w0_9;
         const float f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr + + (r0_9*X[9])*P49*w0_9;
@@ -4093,10 +4103,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -4137,16 +4147,6 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const float P37 = PLOM_UNIFORM32(pv[37]);
     const float P38 = PLOM_UNIFORM32(pv[38]);
     const float P39 = PLOM_UNIFORM32(pv[39]);
-    const float P70 = PLOM_UNIFORM32(pv[70]);
-    const float P71 = PLOM_UNIFORM32(pv[71]);
-    const float P72 = PLOM_UNIFORM32(pv[72]);
-    const float P73 = PLOM_UNIFORM32(pv[73]);
-    const float P74 = PLOM_UNIFORM32(pv[74]);
-    const float P75 = PLOM_UNIFORM32(pv[75]);
-    const float P76 = PLOM_UNIFORM32(pv[76]);
-    const float P77 = PLOM_UNIFORM32(pv[77]);
-    const float P78 = PLOM_UNIFORM32(pv[78]);
-    const float P79 = PLOM_UNIFORM32(pv[79]);
     const float P80 = PLOM_UNIFORM32(pv[80]);
     const float P81 = PLOM_UNIFORM32(pv[81]);
     const float P82 = PLOM_UNIFORM32(pv[82]);
@@ -4261,6 +4261,17 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
+        const float C4 = (float)plom_cov_ld(cvk + 4);
+        const float C5 = (float)plom_cov_ld(cvk + 5);
+        const float C6 = (float)plom_cov_ld(cvk + 6);
+        const float C7 = (float)plom_cov_ld(cvk + 7);
+        const float C8 = (float)plom_cov_ld(cvk + 8);
+        const float C9 = (float)plom_cov_ld(cvk + 9);
         float s30 = 0.0f;
         float s31 = 0.0f;
         float s32 = 0.0f;
@@ -4271,10 +4282,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         float s37 = 0.0f;
         float s38 = 0.0f;
         float s39 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), P70);
+        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), C0);
         const float r1_0 = P10;
         const float r2_0 = P20;
-        const float r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const float r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
         const float d0_0 = sqdt * za0;
@@ -4289,10 +4300,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s30 += ( + (r1_0*X[10]))*dtr + + plom_fsqrtf((r1_0*X[10]))*d1_0;
         X[0] = plom_clipf(f0);
         X[10] = plom_clipf(f10);
-        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), P71);
+        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), C1);
         const float r1_1 = P11;
         const float r2_1 = P21;
-        const float r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const float r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         float za2, zb2;
         plom_normal_pair32(ns, &za2, &zb2);
         const float d0_1 = sqdt * za2;
@@ -4307,10 +4318,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s31 += ( + (r1_1*X[11]))*dtr + + plom_fsqrtf((r1_1*X[11]))*d1_1;
         X[1] = plom_clipf(f1);
         X[11] = plom_clipf(f11);
-        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), P72);
+        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), C2);
         const float r1_2 = P12;
         const float r2_2 = P22;
-        const float r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const float r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         float za4, zb4;
         plom_normal_pair32(ns, &za4, &zb4);
         const float d0_2 = sqdt * za4;
@@ -4325,10 +4336,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s32 += ( + (r1_2*X[12]))*dtr + + plom_fsqrtf((r1_2*X[12]))*d1_2;
         X[2] = plom_clipf(f2);
         X[12] = plom_clipf(f12);
-        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), P73);
+        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), C3);
         const float r1_3 = P13;
         const float r2_3 = P23;
-        const float r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const float r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         float za6, zb6;
         plom_normal_pair32(ns, &za6, &zb6);
         const float d0_3 = sqdt * za6;
@@ -4343,10 +4354,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s33 += ( + (r1_3*X[13]))*dtr + + plom_fsqrtf((r1_3*X[13]))*d1_3;
         X[3] = plom_clipf(f3);
         X[13] = plom_clipf(f13);
-        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), P74);
+        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), C4);
         const float r1_4 = P14;
         const float r2_4 = P24;
-        const float r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const float r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         float za8, zb8;
         plom_normal_pair32(ns, &za8, &zb8);
         const float d0_4 = sqdt * za8;
@@ -4361,10 +4372,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s34 += ( + (r1_4*X[14]))*dtr + + plom_fsqrtf((r1_4*X[14]))*d1_4;
         X[4] = plom_clipf(f4);
         X[14] = plom_clipf(f14);
-        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), P75);
+        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), C5);
         const float r1_5 = P15;
         const float r2_5 = P25;
-        const float r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const float r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         float za10, zb10;
         plom_normal_pair32(ns, &za10, &zb10);
         const float d0_5 = sqdt * za10;
@@ -4379,10 +4390,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s35 += ( + (r1_5*X[15]))*dtr + + plom_fsqrtf((r1_5*X[15]))*d1_5;
         X[5] = plom_clipf(f5);
         X[15] = plom_clipf(f15);
-        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), P76);
+        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), C6);
         const float r1_6 = P16;
         const float r2_6 = P26;
-        const float r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const float r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         float za12, zb12;
         plom_normal_pair32(ns, &za12, &zb12);
         const float d0_6 = sqdt * za12;
@@ -4397,10 +4408,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s36 += ( + (r1_6*X[16]))*dtr + + plom_fsqrtf((r1_6*X[16]))*d1_6;
         X[6] = plom_clipf(f6);
         X[16] = plom_clipf(f16);
-        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), P77);
+        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), C7);
         const float r1_7 = P17;
         const float r2_7 = P27;
-        const float r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const float r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         float za14, zb14;
         plom_normal_pair32(ns, &za14, &zb14);
         const float d0_7 = sqdt * za14;
@@ -4415,10 +4426,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s37 += ( + (r1_7*X[17]))*dtr + + plom_fsqrtf((r1_7*X[17]))*d1_7;
         X[7] = plom_clipf(f7);
         X[17] = plom_clipf(f17);
-        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), P78);
+        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), C8);
         const float r1_8 = P18;
         const float r2_8 = P28;
-        const float r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const float r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         float za16, zb16;
         plom_normal_pair32(ns, &za16, &zb16);
         const float d0_8 = sqdt * za16;
@@ -4433,10 +4444,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s38 += ( + (r1_8*X[18]))*dtr + + plom_fsqrtf((r1_8*X[18]))*d1_8;
         X[8] = plom_clipf(f8);
         X[18] = plom_clipf(f18);
-        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), P79);
+        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), C9);
         const float r1_9 = P19;
         const float r2_9 = P29;
-        const float r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const float r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         float za18, zb18;
         plom_normal_pair32(ns, &za18, &zb18);
         const float d0_9 = sqdt * za18;
@@ -4485,10 +4496,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -4529,16 +4540,6 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const float P37 = PLOM_UNIFORM32(pv[37]);
     const float P38 = PLOM_UNIFORM32(pv[38]);
     const float P39 = PLOM_UNIFORM32(pv[39]);
-    const float P70 = PLOM_UNIFORM32(pv[70]);
-    const float P71 = PLOM_UNIFORM32(pv[71]);
-    const float P72 = PLOM_UNIFORM32(pv[72]);
-    const float P73 = PLOM_UNIFORM32(pv[73]);
-    const float P74 = PLOM_UNIFORM32(pv[74]);
-    const float P75 = PLOM_UNIFORM32(pv[75]);
-    const float P76 = PLOM_UNIFORM32(pv[76]);
-    const float P77 = PLOM_UNIFORM32(pv[77]);
-    const float P78 = PLOM_UNIFORM32(pv[78]);
-    const float P79 = PLOM_UNIFORM32(pv[79]);
     const float P80 = PLOM_UNIFORM32(pv[80]);
     const float P81 = PLOM_UNIFORM32(pv[81]);
     const float P82 = PLOM_UNIFORM32(pv[82]);
@@ -4653,6 +4654,17 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
+        const float C4 = (float)plom_cov_ld(cvk + 4);
+        const float C5 = (float)plom_cov_ld(cvk + 5);
+        const float C6 = (float)plom_cov_ld(cvk + 6);
+        const float C7 = (float)plom_cov_ld(cvk + 7);
+        const float C8 = (float)plom_cov_ld(cvk + 8);
+        const float C9 = (float)plom_cov_ld(cvk + 9);
         float s30 = 0.0f;
         float s31 = 0.0f;
         float s32 = 0.0f;
@@ -4663,100 +4675,100 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         float s37 = 0.0f;
         float s38 = 0.0f;
         float s39 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), P70);
+        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), C0);
         const float r1_0 = P10;
         const float r2_0 = P20;
-        const float r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const float r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         const float f0 = X[0] + ( - (r0_0*X[0]) + (r3_0))*dtr;
         const float f10 = X[10] + ( - (r1_0*X[10]) + (r0_0*X[0]))*dtr;
         const float f20 = X[20] + ( - (r2_0*X[20]) + (r1_0*X[10]))*dtr;
         s30 += ( + (r1_0*X[10]))*dtr;
         X[0] = plom_clipf(f0);
         X[10] = plom_clipf(f10);
-        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), P71);
+        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), C1);
         const float r1_1 = P11;
         const float r2_1 = P21;
-        const float r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const float r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         const float f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr;
         const float f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr;
         const float f21 = X[21] + ( - (r2_1*X[21]) + (r1_1*X[11]))*dtr;
         s31 += ( + (r1_1*X[11]))*dtr;
         X[1] = plom_clipf(f1);
         X[11] = plom_clipf(f11);
-        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), P72);
+        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), C2);
         const float r1_2 = P12;
         const float r2_2 = P22;
-        const float r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const float r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         const float f2 = X[2] + ( - (r0_2*X[2]) + (r3_2))*dtr;
         const float f12 = X[12] + ( - (r1_2*X[12]) + (r0_2*X[2]))*dtr;
         const float f22 = X[22] + ( - (r2_2*X[22]) + (r1_2*X[12]))*dtr;
         s32 += ( + (r1_2*X[12]))*dtr;
         X[2] = plom_clipf(f2);
         X[12] = plom_clipf(f12);
-        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), P73);
+        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), C3);
         const float r1_3 = P13;
         const float r2_3 = P23;
-        const float r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const float r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         const float f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr;
         const float f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr;
         const float f23 = X[23] + ( - (r2_3*X[23]) + (r1_3*X[13]))*dtr;
         s33 += ( + (r1_3*X[13]))*dtr;
         X[3] = plom_clipf(f3);
         X[13] = plom_clipf(f13);
-        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), P74);
+        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), C4);
         const float r1_4 = P14;
         const float r2_4 = P24;
-        const float r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const float r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         const float f4 = X[4] + ( - (r0_4*X[4]) + (r3_4))*dtr;
         const float f14 = X[14] + ( - (r1_4*X[14]) + (r0_4*X[4]))*dtr;
         const float f24 = X[24] + ( - (r2_4*X[24]) + (r1_4*X[14]))*dtr;
         s34 += ( + (r1_4*X[14]))*dtr;
         X[4] = plom_clipf(f4);
         X[14] = plom_clipf(f14);
-        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), P75);
+        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), C5);
         const float r1_5 = P15;
         const float r2_5 = P25;
-        const float r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const float r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         const float f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr;
         const float f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr;
         const float f25 = X[25] + ( - (r2_5*X[25]) + (r1_5*X[15]))*dtr;
         s35 += ( + (r1_5*X[15]))*dtr;
         X[5] = plom_clipf(f5);
         X[15] = plom_clipf(f15);
-        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), P76);
+        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), C6);
         const float r1_6 = P16;
         const float r2_6 = P26;
-        const float r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const float r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         const float f6 = X[6] + ( - (r0_6*X[6]) + (r3_6))*dtr;
         const float f16 = X[16] + ( - (r1_6*X[16]) + (r0_6*X[6]))*dtr;
         const float f26 = X[26] + ( - (r2_6*X[26]) + (r1_6*X[16]))*dtr;
         s36 += ( + (r1_6*X[16]))*dtr;
         X[6] = plom_clipf(f6);
         X[16] = plom_clipf(f16);
-        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), P77);
+        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), C7);
         const float r1_7 = P17;
         const float r2_7 = P27;
-        const float r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const float r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         const float f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr;
         const float f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr;
         const float f27 = X[27] + ( - (r2_7*X[27]) + (r1_7*X[17]))*dtr;
         s37 += ( + (r1_7*X[17]))*dtr;
         X[7] = plom_clipf(f7);
         X[17] = plom_clipf(f17);
-        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), P78);
+        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), C8);
         const float r1_8 = P18;
         const float r2_8 = P28;
-        const float r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const float r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         const float f8 = X[8] + ( - (r0_8*X[8]) + (r3_8))*dtr;
         const float f18 = X[18] + ( - (r1_8*X[18]) + (r0_8*X[8]))*dtr;
         const float f28 = X[28] + ( - (r2_8*X[28]) + (r1_8*X[18]))*dtr;
         s38 += ( + (r1_8*X[18]))*dtr;
         X[8] = plom_clipf(f8);
         X[18] = plom_clipf(f18);
-        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), P79);
+        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), C9);
         const float r1_9 = P19;
         const float r2_9 = P29;
-        const float r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const float r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const float f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr;
         const float f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr;
         const float f29 = X[29] + ( - (r2_9*X[29]) + (r1_9*X[19]))*dtr;
@@ -4797,9 +4809,9 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_proj2obs_f32(const float *X, const double *pvd, double *obs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_proj2obs_f32(const float *X, const double *pvd, const double *cvd, double *obs)
 {
-    (void)pvd; (void)X;
+    (void)pvd; (void)cvd; (void)X;
     obs[0] = (double)X[30];
     obs[1] = (double)X[31];
     obs[2] = (double)X[32];
@@ -4812,9 +4824,9 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_proj2ob
     obs[9] = (double)X[39];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -4865,16 +4877,6 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const double P47 = lp[47];
     const double P48 = lp[48];
     const double P49 = lp[49];
-    const double P70 = PLOM_UNIFORM64(pv[70]);
-    const double P71 = PLOM_UNIFORM64(pv[71]);
-    const double P72 = PLOM_UNIFORM64(pv[72]);
-    const double P73 = PLOM_UNIFORM64(pv[73]);
-    const double P74 = PLOM_UNIFORM64(pv[74]);
-    const double P75 = PLOM_UNIFORM64(pv[75]);
-    const double P76 = PLOM_UNIFORM64(pv[76]);
-    const double P77 = PLOM_UNIFORM64(pv[77]);
-    const double P78 = PLOM_UNIFORM64(pv[78]);
-    const double P79 = PLOM_UNIFORM64(pv[79]);
     const double P80 = PLOM_UNIFORM64(pv[80]);
     const double P81 = PLOM_UNIFORM64(pv[81]);
     const double P82 = PLOM_UNIFORM64(pv[82]);
@@ -4989,6 +4991,17 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
+        const double C4 = plom_cov_ld(cvk + 4);
+        const double C5 = plom_cov_ld(cvk + 5);
+        const double C6 = plom_cov_ld(cvk + 6);
+        const double C7 = plom_cov_ld(cvk + 7);
+        const double C8 = plom_cov_ld(cvk + 8);
+        const double C9 = plom_cov_ld(cvk + 9);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -5060,46 +5073,46 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         double stay28 = 0.0;
         double stay29 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n3_2; (void)n0_3; (void)n1_3; (void)n2_3; (void)n3_3; (void)n0_4; (void)n1_4; (void)n2_4; (void)n3_4; (void)n0_5; (void)n1_5; (void)n2_5; (void)n3_5; (void)n0_6; (void)n1_6; (void)n2_6; (void)n3_6; (void)n0_7; (void)n1_7; (void)n2_7; (void)n3_7; (void)n0_8; (void)n1_8; (void)n2_8; (void)n3_8; (void)n0_9; (void)n1_9; (void)n2_9; (void)n3_9;
-        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
         const double r1_0 = P10;
         const double r2_0 = P20;
-        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
-        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
         const double r1_1 = P11;
         const double r2_1 = P21;
-        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
-        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
         const double r1_2 = P12;
         const double r2_2 = P22;
-        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
-        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
         const double r1_3 = P13;
         const double r2_3 = P23;
-        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
-        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
         const double r1_4 = P14;
         const double r2_4 = P24;
-        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
-        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
         const double r1_5 = P15;
         const double r2_5 = P25;
-        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
-        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
         const double r1_6 = P16;
         const double r2_6 = P26;
-        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
-        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
         const double r1_7 = P17;
         const double r2_7 = P27;
-        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
-        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
         const double r1_8 = P18;
         const double r2_8 = P28;
-        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
-        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
         const double r1_9 = P19;
         const double r2_9 = P29;
-        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const double sd0_0 = P40;
         const double v0_0 = sd0_0 * sd0_0;
         const double xi0_0 = (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt;
@@ -5634,9 +5647,9 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -5677,16 +5690,6 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const double P37 = lp[37];
     const double P38 = lp[38];
     const double P39 = lp[39];
-    const double P70 = PLOM_UNIFORM64(pv[70]);
-    const double P71 = PLOM_UNIFORM64(pv[71]);
-    const double P72 = PLOM_UNIFORM64(pv[72]);
-    const double P73 = PLOM_UNIFORM64(pv[73]);
-    const double P74 = PLOM_UNIFORM64(pv[74]);
-    const double P75 = PLOM_UNIFORM64(pv[75]);
-    const double P76 = PLOM_UNIFORM64(pv[76]);
-    const double P77 = PLOM_UNIFORM64(pv[77]);
-    const double P78 = PLOM_UNIFORM64(pv[78]);
-    const double P79 = PLOM_UNIFORM64(pv[79]);
     const double P80 = PLOM_UNIFORM64(pv[80]);
     const double P81 = PLOM_UNIFORM64(pv[81]);
     const double P82 = PLOM_UNIFORM64(pv[82]);
@@ -5801,6 +5804,17 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
+        const double C4 = plom_cov_ld(cvk + 4);
+        const double C5 = plom_cov_ld(cvk + 5);
+        const double C6 = plom_cov_ld(cvk + 6);
+        const double C7 = plom_cov_ld(cvk + 7);
+        const double C8 = plom_cov_ld(cvk + 8);
+        const double C9 = plom_cov_ld(cvk + 9);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -5872,46 +5886,46 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         double stay28 = 0.0;
         double stay29 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n3_2; (void)n0_3; (void)n1_3; (void)n2_3; (void)n3_3; (void)n0_4; (void)n1_4; (void)n2_4; (void)n3_4; (void)n0_5; (void)n1_5; (void)n2_5; (void)n3_5; (void)n0_6; (void)n1_6; (void)n2_6; (void)n3_6; (void)n0_7; (void)n1_7; (void)n2_7; (void)n3_7; (void)n0_8; (void)n1_8; (void)n2_8; (void)n3_8; (void)n0_9; (void)n1_9; (void)n2_9; (void)n3_9;
-        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
         const double r1_0 = P10;
         const double r2_0 = P20;
-        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
-        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
         const double r1_1 = P11;
         const double r2_1 = P21;
-        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
-        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
         const double r1_2 = P12;
         const double r2_2 = P22;
-        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
-        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
         const double r1_3 = P13;
         const double r2_3 = P23;
-        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
-        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
         const double r1_4 = P14;
         const double r2_4 = P24;
-        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
-        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
         const double r1_5 = P15;
         const double r2_5 = P25;
-        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
-        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
         const double r1_6 = P16;
         const double r2_6 = P26;
-        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
-        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
         const double r1_7 = P17;
         const double r2_7 = P27;
-        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
-        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
         const double r1_8 = P18;
         const double r2_8 = P28;
-        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
-        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
         const double r1_9 = P19;
         const double r2_9 = P29;
-        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const double xi0_0 = 1.0;
         {
             double p[2], cnt[2];
@@ -6426,10 +6440,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -6480,16 +6494,6 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const double P47 = lp[47];
     const double P48 = lp[48];
     const double P49 = lp[49];
-    const double P70 = PLOM_UNIFORM64(pv[70]);
-    const double P71 = PLOM_UNIFORM64(pv[71]);
-    const double P72 = PLOM_UNIFORM64(pv[72]);
-    const double P73 = PLOM_UNIFORM64(pv[73]);
-    const double P74 = PLOM_UNIFORM64(pv[74]);
-    const double P75 = PLOM_UNIFORM64(pv[75]);
-    const double P76 = PLOM_UNIFORM64(pv[76]);
-    const double P77 = PLOM_UNIFORM64(pv[77]);
-    const double P78 = PLOM_UNIFORM64(pv[78]);
-    const double P79 = PLOM_UNIFORM64(pv[79]);
     const double P80 = PLOM_UNIFORM64(pv[80]);
     const double P81 = PLOM_UNIFORM64(pv[81]);
     const double P82 = PLOM_UNIFORM64(pv[82]);
@@ -6604,6 +6608,17 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
+        const double C4 = plom_cov_ld(cvk + 4);
+        const double C5 = plom_cov_ld(cvk + 5);
+        const double C6 = plom_cov_ld(cvk + 6);
+        const double C7 = plom_cov_ld(cvk + 7);
+        const double C8 = plom_cov_ld(cvk + 8);
+        const double C9 = plom_cov_ld(cvk + 9);
         double s30 = 0.0;
         double s31 = 0.0;
         double s32 = 0.0;
@@ -6614,10 +6629,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         double s37 = 0.0;
         double s38 = 0.0;
         double s39 = 0.0;
-        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
         const double r1_0 = P10;
         const double r2_0 = P20;
-        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const double r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
         const double d0_0 = sqdt * za0;
@@ -6635,10 +6650,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s30 += ( + (r1_0*X[10]))*dtr + + plom_sqrt((r1_0*X[10]))*d1_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
-        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
         const double r1_1 = P11;
         const double r2_1 = P21;
-        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const double r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         const double d0_1 = sqdt * zb2;
         double za3, zb3;
         plom_normal_pair64(ns, &za3, &zb3);
@@ -6654,10 +6669,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s31 += ( + (r1_1*X[11]))*dtr + + plom_sqrt((r1_1*X[11]))*d1_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
-        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
         const double r1_2 = P12;
         const double r2_2 = P22;
-        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const double r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         double za5, zb5;
         plom_normal_pair64(ns, &za5, &zb5);
         const double d0_2 = sqdt * za5;
@@ -6675,10 +6690,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s32 += ( + (r1_2*X[12]))*dtr + + plom_sqrt((r1_2*X[12]))*d1_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         X[12] = (f12 < 0.0) ? 0.0 : f12;
-        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
         const double r1_3 = P13;
         const double r2_3 = P23;
-        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const double r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         const double d0_3 = sqdt * zb7;
         double za8, zb8;
         plom_normal_pair64(ns, &za8, &zb8);
@@ -6694,10 +6709,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s33 += ( + (r1_3*X[13]))*dtr + + plom_sqrt((r1_3*X[13]))*d1_3;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[13] = (f13 < 0.0) ? 0.0 : f13;
-        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
         const double r1_4 = P14;
         const double r2_4 = P24;
-        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const double r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         double za10, zb10;
         plom_normal_pair64(ns, &za10, &zb10);
         const double d0_4 = sqdt * za10;
@@ -6715,10 +6730,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s34 += ( + (r1_4*X[14]))*dtr + + plom_sqrt((r1_4*X[14]))*d1_4;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[14] = (f14 < 0.0) ? 0.0 : f14;
-        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
         const double r1_5 = P15;
         const double r2_5 = P25;
-        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const double r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         const double d0_5 = sqdt * zb12;
         double za13, zb13;
         plom_normal_pair64(ns, &za13, &zb13);
@@ -6734,10 +6749,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s35 += ( + (r1_5*X[15]))*dtr + + plom_sqrt((r1_5*X[15]))*d1_5;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[15] = (f15 < 0.0) ? 0.0 : f15;
-        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
         const double r1_6 = P16;
         const double r2_6 = P26;
-        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const double r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         double za15, zb15;
         plom_normal_pair64(ns, &za15, &zb15);
         const double d0_6 = sqdt * za15;
@@ -6755,10 +6770,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s36 += ( + (r1_6*X[16]))*dtr + + plom_sqrt((r1_6*X[16]))*d1_6;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[16] = (f16 < 0.0) ? 0.0 : f16;
-        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
         const double r1_7 = P17;
         const double r2_7 = P27;
-        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const double r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         const double d0_7 = sqdt * zb17;
         double za18, zb18;
         plom_normal_pair64(ns, &za18, &zb18);
@@ -6774,10 +6789,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s37 += ( + (r1_7*X[17]))*dtr + + plom_sqrt((r1_7*X[17]))*d1_7;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
         X[17] = (f17 < 0.0) ? 0.0 : f17;
-        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
         const double r1_8 = P18;
         const double r2_8 = P28;
-        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const double r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         double za20, zb20;
         plom_normal_pair64(ns, &za20, &zb20);
         const double d0_8 = sqdt * za20;
@@ -6795,10 +6810,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s38 += ( + (r1_8*X[18]))*dtr + + plom_sqrt((r1_8*X[18]))*d1_8;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
         X[18] = (f18 < 0.0) ? 0.0 : f18;
-        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
         const double r1_9 = P19;
         const double r2_9 = P29;
-        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const double d0_9 = sqdt * zb22;
         double za23, zb23;
         plom_normal_pair64(ns, &za23, &zb23);
@@ -6848,10 +6863,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -6902,16 +6917,6 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const double P47 = lp[47];
     const double P48 = lp[48];
     const double P49 = lp[49];
-    const double P70 = PLOM_UNIFORM64(pv[70]);
-    const double P71 = PLOM_UNIFORM64(pv[71]);
-    const double P72 = PLOM_UNIFORM64(pv[72]);
-    const double P73 = PLOM_UNIFORM64(pv[73]);
-    const double P74 = PLOM_UNIFORM64(pv[74]);
-    const double P75 = PLOM_UNIFORM64(pv[75]);
-    const double P76 = PLOM_UNIFORM64(pv[76]);
-    const double P77 = PLOM_UNIFORM64(pv[77]);
-    const double P78 = PLOM_UNIFORM64(pv[78]);
-    const double P79 = PLOM_UNIFORM64(pv[79]);
     const double P80 = PLOM_UNIFORM64(pv[80]);
     const double P81 = PLOM_UNIFORM64(pv[81]);
     const double P82 = PLOM_UNIFORM64(pv[82]);
@@ -7026,6 +7031,17 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
+        const double C4 = plom_cov_ld(cvk + 4);
+        const double C5 = plom_cov_ld(cvk + 5);
+        const double C6 = plom_cov_ld(cvk + 6);
+        const double C7 = plom_cov_ld(cvk + 7);
+        const double C8 = plom_cov_ld(cvk + 8);
+        const double C9 = plom_cov_ld(cvk + 9);
         double s30 = 0.0;
         double s31 = 0.0;
         double s32 = 0.0;
@@ -7036,10 +7052,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         double s37 = 0.0;
         double s38 = 0.0;
         double s39 = 0.0;
-        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
         const double r1_0 = P10;
         const double r2_0 = P20;
-        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const double r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
         const double w0_0 = sqdt * za0;
@@ -7049,10 +7065,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s30 += ( + (r1_0*X[10]))*dtr;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
-        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
         const double r1_1 = P11;
         const double r2_1 = P21;
-        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const double r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         const double w0_1 = sqdt * zb0;
         const double f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr + - (r0_1*X[1])*P41*w0_1;
         const double f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P41*w0_1;
@@ -7060,10 +7076,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s31 += ( + (r1_1*X[11]))*dtr;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
-        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
         const double r1_2 = P12;
         const double r2_2 = P22;
-        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const double r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         double za1, zb1;
         plom_normal_pair64(ns, &za1, &zb1);
         const double w0_2 = sqdt * za1;
@@ -7073,10 +7089,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s32 += ( + (r1_2*X[12]))*dtr;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         X[12] = (f12 < 0.0) ? 0.0 : f12;
-        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
         const double r1_3 = P13;
         const double r2_3 = P23;
-        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const double r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         const double w0_3 = sqdt * zb1;
         const double f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr + - (r0_3*X[3])*P43*w0_3;
         const double f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr + + (r0_3*X[3])*P43*w0_3;
@@ -7084,10 +7100,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s33 += ( + (r1_3*X[13]))*dtr;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[13] = (f13 < 0.0) ? 0.0 : f13;
-        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
         const double r1_4 = P14;
         const double r2_4 = P24;
-        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const double r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
         const double w0_4 = sqdt * za2;
@@ -7097,10 +7113,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s34 += ( + (r1_4*X[14]))*dtr;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[14] = (f14 < 0.0) ? 0.0 : f14;
-        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
         const double r1_5 = P15;
         const double r2_5 = P25;
-        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const double r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         const double w0_5 = sqdt * zb2;
         const double f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr + - (r0_5*X[5])*P45*w0_5;
         const double f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr + + (r0_5*X[5])*P45*w0_5;
@@ -7108,10 +7124,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s35 += ( + (r1_5*X[15]))*dtr;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[15] = (f15 < 0.0) ? 0.0 : f15;
-        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
         const double r1_6 = P16;
         const double r2_6 = P26;
-        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const double r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         double za3, zb3;
         plom_normal_pair64(ns, &za3, &zb3);
         const double w0_6 = sqdt * za3;
@@ -7121,10 +7137,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s36 += ( + (r1_6*X[16]))*dtr;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[16] = (f16 < 0.0) ? 0.0 : f16;
-        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
         const double r1_7 = P17;
         const double r2_7 = P27;
-        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const double r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         const double w0_7 = sqdt * zb3;
         const double f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr + - (r0_7*X[7])*P47*w0_7;
         const double f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr + + (r0_7*X[7])*P47*w0_7;
@@ -7132,10 +7148,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s37 += ( + (r1_7*X[17]))*dtr;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
         X[17] = (f17 < 0.0) ? 0.0 : f17;
-        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
         const double r1_8 = P18;
         const double r2_8 = P28;
-        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const double r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         double za4, zb4;
         plom_normal_pair64(ns, &za4, &zb4);
         const double w0_8 = sqdt * za4;
@@ -7145,10 +7161,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s38 += ( + (r1_8*X[18]))*dtr;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
         X[18] = (f18 < 0.0) ? 0.0 : f18;
-        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
         const double r1_9 = P19;
         const double r2_9 = P29;
-        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const double w0_9 = sqdt * zb4;
         const double f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr + - (r0_9*X[9])*P49*w0_9;
         const double f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr + + (r0_9*X[9])*P49*w0_9;
@@ -7190,10 +7206,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -7234,16 +7250,6 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const double P37 = lp[37];
     const double P38 = lp[38];
     const double P39 = lp[39];
-    const double P70 = PLOM_UNIFORM64(pv[70]);
-    const double P71 = PLOM_UNIFORM64(pv[71]);
-    const double P72 = PLOM_UNIFORM64(pv[72]);
-    const double P73 = PLOM_UNIFORM64(pv[73]);
-    const double P74 = PLOM_UNIFORM64(pv[74]);
-    const double P75 = PLOM_UNIFORM64(pv[75]);
-    const double P76 = PLOM_UNIFORM64(pv[76]);
-    const double P77 = PLOM_UNIFORM64(pv[77]);
-    const double P78 = PLOM_UNIFORM64(pv[78]);
-    const double P79 = PLOM_UNIFORM64(pv[79]);
     const double P80 = PLOM_UNIFORM64(pv[80]);
     const double P81 = PLOM_UNIFORM64(pv[81]);
     const double P82 = PLOM_UNIFORM64(pv[82]);
@@ -7358,6 +7364,17 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
+        const double C4 = plom_cov_ld(cvk + 4);
+        const double C5 = plom_cov_ld(cvk + 5);
+        const double C6 = plom_cov_ld(cvk + 6);
+        const double C7 = plom_cov_ld(cvk + 7);
+        const double C8 = plom_cov_ld(cvk + 8);
+        const double C9 = plom_cov_ld(cvk + 9);
         double s30 = 0.0;
         double s31 = 0.0;
         double s32 = 0.0;
@@ -7368,10 +7385,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         double s37 = 0.0;
         double s38 = 0.0;
         double s39 = 0.0;
-        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
         const double r1_0 = P10;
         const double r2_0 = P20;
-        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const double r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
         const double d0_0 = sqdt * za0;
@@ -7386,10 +7403,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s30 += ( + (r1_0*X[10]))*dtr + + plom_sqrt((r1_0*X[10]))*d1_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
-        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
         const double r1_1 = P11;
         const double r2_1 = P21;
-        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const double r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
         const double d0_1 = sqdt * za2;
@@ -7404,10 +7421,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s31 += ( + (r1_1*X[11]))*dtr + + plom_sqrt((r1_1*X[11]))*d1_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
-        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
         const double r1_2 = P12;
         const double r2_2 = P22;
-        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const double r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         double za4, zb4;
         plom_normal_pair64(ns, &za4, &zb4);
         const double d0_2 = sqdt * za4;
@@ -7422,10 +7439,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s32 += ( + (r1_2*X[12]))*dtr + + plom_sqrt((r1_2*X[12]))*d1_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         X[12] = (f12 < 0.0) ? 0.0 : f12;
-        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
         const double r1_3 = P13;
         const double r2_3 = P23;
-        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const double r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         double za6, zb6;
         plom_normal_pair64(ns, &za6, &zb6);
         const double d0_3 = sqdt * za6;
@@ -7440,10 +7457,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s33 += ( + (r1_3*X[13]))*dtr + + plom_sqrt((r1_3*X[13]))*d1_3;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[13] = (f13 < 0.0) ? 0.0 : f13;
-        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
         const double r1_4 = P14;
         const double r2_4 = P24;
-        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const double r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         double za8, zb8;
         plom_normal_pair64(ns, &za8, &zb8);
         const double d0_4 = sqdt * za8;
@@ -7458,10 +7475,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s34 += ( + (r1_4*X[14]))*dtr + + plom_sqrt((r1_4*X[14]))*d1_4;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[14] = (f14 < 0.0) ? 0.0 : f14;
-        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
         const double r1_5 = P15;
         const double r2_5 = P25;
-        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const double r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         double za10, zb10;
         plom_normal_pair64(ns, &za10, &zb10);
         const double d0_5 = sqdt * za10;
@@ -7476,10 +7493,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s35 += ( + (r1_5*X[15]))*dtr + + plom_sqrt((r1_5*X[15]))*d1_5;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[15] = (f15 < 0.0) ? 0.0 : f15;
-        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
         const double r1_6 = P16;
         const double r2_6 = P26;
-        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const double r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         double za12, zb12;
         plom_normal_pair64(ns, &za12, &zb12);
         const double d0_6 = sqdt * za12;
@@ -7494,10 +7511,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s36 += ( + (r1_6*X[16]))*dtr + + plom_sqrt((r1_6*X[16]))*d1_6;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[16] = (f16 < 0.0) ? 0.0 : f16;
-        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
         const double r1_7 = P17;
         const double r2_7 = P27;
-        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const double r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         double za14, zb14;
         plom_normal_pair64(ns, &za14, &zb14);
         const double d0_7 = sqdt * za14;
@@ -7512,10 +7529,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s37 += ( + (r1_7*X[17]))*dtr + + plom_sqrt((r1_7*X[17]))*d1_7;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
         X[17] = (f17 < 0.0) ? 0.0 : f17;
-        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
         const double r1_8 = P18;
         const double r2_8 = P28;
-        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const double r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         double za16, zb16;
         plom_normal_pair64(ns, &za16, &zb16);
         const double d0_8 = sqdt * za16;
@@ -7530,10 +7547,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s38 += ( + (r1_8*X[18]))*dtr + + plom_sqrt((r1_8*X[18]))*d1_8;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
         X[18] = (f18 < 0.0) ? 0.0 : f18;
-        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
         const double r1_9 = P19;
         const double r2_9 = P29;
-        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         double za18, zb18;
         plom_normal_pair64(ns, &za18, &zb18);
         const double d0_9 = sqdt * za18;
@@ -7582,10 +7599,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -7626,16 +7643,6 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const double P37 = lp[37];
     const double P38 = lp[38];
     const double P39 = lp[39];
-    const double P70 = PLOM_UNIFORM64(pv[70]);
-    const double P71 = PLOM_UNIFORM64(pv[71]);
-    const double P72 = PLOM_UNIFORM64(pv[72]);
-    const double P73 = PLOM_UNIFORM64(pv[73]);
-    const double P74 = PLOM_UNIFORM64(pv[74]);
-    const double P75 = PLOM_UNIFORM64(pv[75]);
-    const double P76 = PLOM_UNIFORM64(pv[76]);
-    const double P77 = PLOM_UNIFORM64(pv[77]);
-    const double P78 = PLOM_UNIFORM64(pv[78]);
-    const double P79 = PLOM_UNIFORM64(pv[79]);
     const double P80 = PLOM_UNIFORM64(pv[80]);
     const double P81 = PLOM_UNIFORM64(pv[81]);
     const double P82 = PLOM_UNIFORM64(pv[82]);
@@ -7750,6 +7757,17 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
+        const double C4 = plom_cov_ld(cvk + 4);
+        const double C5 = plom_cov_ld(cvk + 5);
+        const double C6 = plom_cov_ld(cvk + 6);
+        const double C7 = plom_cov_ld(cvk + 7);
+        const double C8 = plom_cov_ld(cvk + 8);
+        const double C9 = plom_cov_ld(cvk + 9);
         double s30 = 0.0;
         double s31 = 0.0;
         double s32 = 0.0;
@@ -7760,100 +7778,100 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         double s37 = 0.0;
         double s38 = 0.0;
         double s39 = 0.0;
-        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / P70);
+        const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
         const double r1_0 = P10;
         const double r2_0 = P20;
-        const double r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const double r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         const double f0 = X[0] + ( - (r0_0*X[0]) + (r3_0))*dtr;
         const double f10 = X[10] + ( - (r1_0*X[10]) + (r0_0*X[0]))*dtr;
         const double f20 = X[20] + ( - (r2_0*X[20]) + (r1_0*X[10]))*dtr;
         s30 += ( + (r1_0*X[10]))*dtr;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
-        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / P71);
+        const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
         const double r1_1 = P11;
         const double r2_1 = P21;
-        const double r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const double r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         const double f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr;
         const double f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr;
         const double f21 = X[21] + ( - (r2_1*X[21]) + (r1_1*X[11]))*dtr;
         s31 += ( + (r1_1*X[11]))*dtr;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
-        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / P72);
+        const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
         const double r1_2 = P12;
         const double r2_2 = P22;
-        const double r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const double r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         const double f2 = X[2] + ( - (r0_2*X[2]) + (r3_2))*dtr;
         const double f12 = X[12] + ( - (r1_2*X[12]) + (r0_2*X[2]))*dtr;
         const double f22 = X[22] + ( - (r2_2*X[22]) + (r1_2*X[12]))*dtr;
         s32 += ( + (r1_2*X[12]))*dtr;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         X[12] = (f12 < 0.0) ? 0.0 : f12;
-        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / P73);
+        const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
         const double r1_3 = P13;
         const double r2_3 = P23;
-        const double r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const double r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         const double f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr;
         const double f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr;
         const double f23 = X[23] + ( - (r2_3*X[23]) + (r1_3*X[13]))*dtr;
         s33 += ( + (r1_3*X[13]))*dtr;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[13] = (f13 < 0.0) ? 0.0 : f13;
-        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / P74);
+        const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
         const double r1_4 = P14;
         const double r2_4 = P24;
-        const double r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const double r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         const double f4 = X[4] + ( - (r0_4*X[4]) + (r3_4))*dtr;
         const double f14 = X[14] + ( - (r1_4*X[14]) + (r0_4*X[4]))*dtr;
         const double f24 = X[24] + ( - (r2_4*X[24]) + (r1_4*X[14]))*dtr;
         s34 += ( + (r1_4*X[14]))*dtr;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[14] = (f14 < 0.0) ? 0.0 : f14;
-        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / P75);
+        const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
         const double r1_5 = P15;
         const double r2_5 = P25;
-        const double r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const double r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         const double f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr;
         const double f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr;
         const double f25 = X[25] + ( - (r2_5*X[25]) + (r1_5*X[15]))*dtr;
         s35 += ( + (r1_5*X[15]))*dtr;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[15] = (f15 < 0.0) ? 0.0 : f15;
-        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / P76);
+        const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
         const double r1_6 = P16;
         const double r2_6 = P26;
-        const double r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const double r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         const double f6 = X[6] + ( - (r0_6*X[6]) + (r3_6))*dtr;
         const double f16 = X[16] + ( - (r1_6*X[16]) + (r0_6*X[6]))*dtr;
         const double f26 = X[26] + ( - (r2_6*X[26]) + (r1_6*X[16]))*dtr;
         s36 += ( + (r1_6*X[16]))*dtr;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[16] = (f16 < 0.0) ? 0.0 : f16;
-        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / P77);
+        const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
         const double r1_7 = P17;
         const double r2_7 = P27;
-        const double r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const double r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         const double f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr;
         const double f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr;
         const double f27 = X[27] + ( - (r2_7*X[27]) + (r1_7*X[17]))*dtr;
         s37 += ( + (r1_7*X[17]))*dtr;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
         X[17] = (f17 < 0.0) ? 0.0 : f17;
-        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / P78);
+        const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
         const double r1_8 = P18;
         const double r2_8 = P28;
-        const double r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const double r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         const double f8 = X[8] + ( - (r0_8*X[8]) + (r3_8))*dtr;
         const double f18 = X[18] + ( - (r1_8*X[18]) + (r0_8*X[8]))*dtr;
         const double f28 = X[28] + ( - (r2_8*X[28]) + (r1_8*X[18]))*dtr;
         s38 += ( + (r1_8*X[18]))*dtr;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
         X[18] = (f18 < 0.0) ? 0.0 : f18;
-        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / P79);
+        const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
         const double r1_9 = P19;
         const double r2_9 = P29;
-        const double r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const double r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const double f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr;
         const double f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr;
         const double f29 = X[29] + ( - (r2_9*X[29]) + (r1_9*X[19]))*dtr;
@@ -7894,9 +7912,9 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_proj2obs_pp_f64(const double *X, const double *pvd, const double *lpd, double *obs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_proj2obs_pp_f64(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs)
 {
-    (void)pvd; (void)lpd; (void)X;
+    (void)pvd; (void)lpd; (void)cvd; (void)X;
     obs[0] = (double)X[30];
     obs[1] = (double)X[31];
     obs[2] = (double)X[32];
@@ -7909,10 +7927,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_proj2ob
     obs[9] = (double)X[39];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -7963,16 +7981,6 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const float P47 = lp[47];
     const float P48 = lp[48];
     const float P49 = lp[49];
-    const float P70 = PLOM_UNIFORM32(pv[70]);
-    const float P71 = PLOM_UNIFORM32(pv[71]);
-    const float P72 = PLOM_UNIFORM32(pv[72]);
-    const float P73 = PLOM_UNIFORM32(pv[73]);
-    const float P74 = PLOM_UNIFORM32(pv[74]);
-    const float P75 = PLOM_UNIFORM32(pv[75]);
-    const float P76 = PLOM_UNIFORM32(pv[76]);
-    const float P77 = PLOM_UNIFORM32(pv[77]);
-    const float P78 = PLOM_UNIFORM32(pv[78]);
-    const float P79 = PLOM_UNIFORM32(pv[79]);
     const float P80 = PLOM_UNIFORM32(pv[80]);
     const float P81 = PLOM_UNIFORM32(pv[81]);
     const float P82 = PLOM_UNIFORM32(pv[82]);
@@ -8087,6 +8095,17 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
+        const float C4 = (float)plom_cov_ld(cvk + 4);
+        const float C5 = (float)plom_cov_ld(cvk + 5);
+        const float C6 = (float)plom_cov_ld(cvk + 6);
+        const float C7 = (float)plom_cov_ld(cvk + 7);
+        const float C8 = (float)plom_cov_ld(cvk + 8);
+        const float C9 = (float)plom_cov_ld(cvk + 9);
         float s30 = 0.0f;
         float s31 = 0.0f;
         float s32 = 0.0f;
@@ -8097,10 +8116,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         float s37 = 0.0f;
         float s38 = 0.0f;
         float s39 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), P70);
+        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), C0);
         const float r1_0 = P10;
         const float r2_0 = P20;
-        const float r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const float r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
         const float d0_0 = sqdt * za0;
@@ -8118,10 +8137,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s30 += ( + (r1_0*X[10]))*dtr + + plom_fsqrtf((r1_0*X[10]))*d1_0;
         X[0] = plom_clipf(f0);
         X[10] = plom_clipf(f10);
-        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), P71);
+        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), C1);
         const float r1_1 = P11;
         const float r2_1 = P21;
-        const float r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const float r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         const float d0_1 = sqdt * zb2;
         float za3, zb3;
         plom_normal_pair32(ns, &za3, &zb3);
@@ -8137,10 +8156,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s31 += ( + (r1_1*X[11]))*dtr + + plom_fsqrtf((r1_1*X[11]))*d1_1;
         X[1] = plom_clipf(f1);
         X[11] = plom_clipf(f11);
-        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), P72);
+        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), C2);
         const float r1_2 = P12;
         const float r2_2 = P22;
-        const float r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const float r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         float za5, zb5;
         plom_normal_pair32(ns, &za5, &zb5);
         const float d0_2 = sqdt * za5;
@@ -8158,10 +8177,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s32 += ( + (r1_2*X[12]))*dtr + + plom_fsqrtf((r1_2*X[12]))*d1_2;
         X[2] = plom_clipf(f2);
         X[12] = plom_clipf(f12);
-        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), P73);
+        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), C3);
         const float r1_3 = P13;
         const float r2_3 = P23;
-        const float r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const float r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         const float d0_3 = sqdt * zb7;
         float za8, zb8;
         plom_normal_pair32(ns, &za8, &zb8);
@@ -8177,10 +8196,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s33 += ( + (r1_3*X[13]))*dtr + + plom_fsqrtf((r1_3*X[13]))*d1_3;
         X[3] = plom_clipf(f3);
         X[13] = plom_clipf(f13);
-        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), P74);
+        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), C4);
         const float r1_4 = P14;
         const float r2_4 = P24;
-        const float r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const float r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         float za10, zb10;
         plom_normal_pair32(ns, &za10, &zb10);
         const float d0_4 = sqdt * za10;
@@ -8198,10 +8217,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s34 += ( + (r1_4*X[14]))*dtr + + plom_fsqrtf((r1_4*X[14]))*d1_4;
         X[4] = plom_clipf(f4);
         X[14] = plom_clipf(f14);
-        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), P75);
+        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), C5);
         const float r1_5 = P15;
         const float r2_5 = P25;
-        const float r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const float r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         const float d0_5 = sqdt * zb12;
         float za13, zb13;
         plom_normal_pair32(ns, &za13, &zb13);
@@ -8217,10 +8236,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s35 += ( + (r1_5*X[15]))*dtr + + plom_fsqrtf((r1_5*X[15]))*d1_5;
         X[5] = plom_clipf(f5);
         X[15] = plom_clipf(f15);
-        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), P76);
+        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), C6);
         const float r1_6 = P16;
         const float r2_6 = P26;
-        const float r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const float r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         float za15, zb15;
         plom_normal_pair32(ns, &za15, &zb15);
         const float d0_6 = sqdt * za15;
@@ -8238,10 +8257,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s36 += ( + (r1_6*X[16]))*dtr + + plom_fsqrtf((r1_6*X[16]))*d1_6;
         X[6] = plom_clipf(f6);
         X[16] = plom_clipf(f16);
-        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), P77);
+        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), C7);
         const float r1_7 = P17;
         const float r2_7 = P27;
-        const float r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const float r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         const float d0_7 = sqdt * zb17;
         float za18, zb18;
         plom_normal_pair32(ns, &za18, &zb18);
@@ -8257,10 +8276,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s37 += ( + (r1_7*X[17]))*dtr + + plom_fsqrtf((r1_7*X[17]))*d1_7;
         X[7] = plom_clipf(f7);
         X[17] = plom_clipf(f17);
-        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), P78);
+        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), C8);
         const float r1_8 = P18;
         const float r2_8 = P28;
-        const float r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const float r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         float za20, zb20;
         plom_normal_pair32(ns, &za20, &zb20);
         const float d0_8 = sqdt * za20;
@@ -8278,10 +8297,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s38 += ( + (r1_8*X[18]))*dtr + + plom_fsqrtf((r1_8*X[18]))*d1_8;
         X[8] = plom_clipf(f8);
         X[18] = plom_clipf(f18);
-        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), P79);
+        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), C9);
         const float r1_9 = P19;
         const float r2_9 = P29;
-        const float r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const float r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const float d0_9 = sqdt * zb22;
         float za23, zb23;
         plom_normal_pair32(ns, &za23, &zb23);
@@ -8331,10 +8350,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -8385,16 +8404,6 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const float P47 = lp[47];
     const float P48 = lp[48];
     const float P49 = lp[49];
-    const float P70 = PLOM_UNIFORM32(pv[70]);
-    const float P71 = PLOM_UNIFORM32(pv[71]);
-    const float P72 = PLOM_UNIFORM32(pv[72]);
-    const float P73 = PLOM_UNIFORM32(pv[73]);
-    const float P74 = PLOM_UNIFORM32(pv[74]);
-    const float P75 = PLOM_UNIFORM32(pv[75]);
-    const float P76 = PLOM_UNIFORM32(pv[76]);
-    const float P77 = PLOM_UNIFORM32(pv[77]);
-    const float P78 = PLOM_UNIFORM32(pv[78]);
-    const float P79 = PLOM_UNIFORM32(pv[79]);
     const float P80 = PLOM_UNIFORM32(pv[80]);
     const float P81 = PLOM_UNIFORM32(pv[81]);
     const float P82 = PLOM_UNIFORM32(pv[82]);
@@ -8509,6 +8518,17 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
+        const float C4 = (float)plom_cov_ld(cvk + 4);
+        const float C5 = (float)plom_cov_ld(cvk + 5);
+        const float C6 = (float)plom_cov_ld(cvk + 6);
+        const float C7 = (float)plom_cov_ld(cvk + 7);
+        const float C8 = (float)plom_cov_ld(cvk + 8);
+        const float C9 = (float)plom_cov_ld(cvk + 9);
         float s30 = 0.0f;
         float s31 = 0.0f;
         float s32 = 0.0f;
@@ -8519,10 +8539,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         float s37 = 0.0f;
         float s38 = 0.0f;
         float s39 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), P70);
+        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), C0);
         const float r1_0 = P10;
         const float r2_0 = P20;
-        const float r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const float r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
         const float w0_0 = sqdt * za0;
@@ -8532,10 +8552,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s30 += ( + (r1_0*X[10]))*dtr;
         X[0] = plom_clipf(f0);
         X[10] = plom_clipf(f10);
-        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), P71);
+        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), C1);
         const float r1_1 = P11;
         const float r2_1 = P21;
-        const float r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const float r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         const float w0_1 = sqdt * zb0;
         const float f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr + - (r0_1*X[1])*P41*w0_1;
         const float f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P41*w0_1;
@@ -8543,10 +8563,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s31 += ( + (r1_1*X[11]))*dtr;
         X[1] = plom_clipf(f1);
         X[11] = plom_clipf(f11);
-        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), P72);
+        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), C2);
         const float r1_2 = P12;
         const float r2_2 = P22;
-        const float r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const float r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         float za1, zb1;
         plom_normal_pair32(ns, &za1, &zb1);
         const float w0_2 = sqdt * za1;
@@ -8556,10 +8576,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s32 += ( + (r1_2*X[12]))*dtr;
         X[2] = plom_clipf(f2);
         X[12] = plom_clipf(f12);
-        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), P73);
+        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), C3);
         const float r1_3 = P13;
         const float r2_3 = P23;
-        const float r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const float r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         const float w0_3 = sqdt * zb1;
         const float f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr + - (r0_3*X[3])*P43*w0_3;
         const float f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr + + (r0_3*X[3])*P43*w0_3;
@@ -8567,10 +8587,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s33 += ( + (r1_3*X[13]))*dtr;
         X[3] = plom_clipf(f3);
         X[13] = plom_clipf(f13);
-        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), P74);
+        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), C4);
         const float r1_4 = P14;
         const float r2_4 = P24;
-        const float r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const float r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         float za2, zb2;
         plom_normal_pair32(ns, &za2, &zb2);
         const float w0_4 = sqdt * za2;
@@ -8580,10 +8600,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s34 += ( + (r1_4*X[14]))*dtr;
         X[4] = plom_clipf(f4);
         X[14] = plom_clipf(f14);
-        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), P75);
+        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), C5);
         const float r1_5 = P15;
         const float r2_5 = P25;
-        const float r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const float r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         const float w0_5 = sqdt * zb2;
         const float f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr + - (r0_5*X[5])*P45*w0_5;
         const float f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr + + (r0_5*X[5])*P45*w0_5;
@@ -8591,10 +8611,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s35 += ( + (r1_5*X[15]))*dtr;
         X[5] = plom_clipf(f5);
         X[15] = plom_clipf(f15);
-        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), P76);
+        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), C6);
         const float r1_6 = P16;
         const float r2_6 = P26;
-        const float r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const float r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         float za3, zb3;
         plom_normal_pair32(ns, &za3, &zb3);
         const float w0_6 = sqdt * za3;
@@ -8604,10 +8624,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s36 += ( + (r1_6*X[16]))*dtr;
         X[6] = plom_clipf(f6);
         X[16] = plom_clipf(f16);
-        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), P77);
+        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), C7);
         const float r1_7 = P17;
         const float r2_7 = P27;
-        const float r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const float r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         const float w0_7 = sqdt * zb3;
         const float f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr + - (r0_7*X[7])*P47*w0_7;
         const float f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr + + (r0_7*X[7])*P47*w0_7;
@@ -8615,10 +8635,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s37 += ( + (r1_7*X[17]))*dtr;
         X[7] = plom_clipf(f7);
         X[17] = plom_clipf(f17);
-        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), P78);
+        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), C8);
         const float r1_8 = P18;
         const float r2_8 = P28;
-        const float r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const float r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         float za4, zb4;
         plom_normal_pair32(ns, &za4, &zb4);
         const float w0_8 = sqdt * za4;
@@ -8628,10 +8648,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s38 += ( + (r1_8*X[18]))*dtr;
         X[8] = plom_clipf(f8);
         X[18] = plom_clipf(f18);
-        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), P79);
+        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), C9);
         const float r1_9 = P19;
         const float r2_9 = P29;
-        const float r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const float r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const float w0_9 = sqdt * zb4;
         const float f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr + - (r0_9*X[9])*P49*w0_9;
         const float f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr + + (r0_9*X[9])*P49*w0_9;
@@ -8673,10 +8693,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -8717,16 +8737,6 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const float P37 = lp[37];
     const float P38 = lp[38];
     const float P39 = lp[39];
-    const float P70 = PLOM_UNIFORM32(pv[70]);
-    const float P71 = PLOM_UNIFORM32(pv[71]);
-    const float P72 = PLOM_UNIFORM32(pv[72]);
-    const float P73 = PLOM_UNIFORM32(pv[73]);
-    const float P74 = PLOM_UNIFORM32(pv[74]);
-    const float P75 = PLOM_UNIFORM32(pv[75]);
-    const float P76 = PLOM_UNIFORM32(pv[76]);
-    const float P77 = PLOM_UNIFORM32(pv[77]);
-    const float P78 = PLOM_UNIFORM32(pv[78]);
-    const float P79 = PLOM_UNIFORM32(pv[79]);
     const float P80 = PLOM_UNIFORM32(pv[80]);
     const float P81 = PLOM_UNIFORM32(pv[81]);
     const float P82 = PLOM_UNIFORM32(pv[82]);
@@ -8841,6 +8851,17 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
+        const float C4 = (float)plom_cov_ld(cvk + 4);
+        const float C5 = (float)plom_cov_ld(cvk + 5);
+        const float C6 = (float)plom_cov_ld(cvk + 6);
+        const float C7 = (float)plom_cov_ld(cvk + 7);
+        const float C8 = (float)plom_cov_ld(cvk + 8);
+        const float C9 = (float)plom_cov_ld(cvk + 9);
         float s30 = 0.0f;
         float s31 = 0.0f;
         float s32 = 0.0f;
@@ -8851,10 +8872,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         float s37 = 0.0f;
         float s38 = 0.0f;
         float s39 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), P70);
+        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), C0);
         const float r1_0 = P10;
         const float r2_0 = P20;
-        const float r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const float r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
         const float d0_0 = sqdt * za0;
@@ -8869,10 +8890,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s30 += ( + (r1_0*X[10]))*dtr + + plom_fsqrtf((r1_0*X[10]))*d1_0;
         X[0] = plom_clipf(f0);
         X[10] = plom_clipf(f10);
-        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), P71);
+        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), C1);
         const float r1_1 = P11;
         const float r2_1 = P21;
-        const float r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const float r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         float za2, zb2;
         plom_normal_pair32(ns, &za2, &zb2);
         const float d0_1 = sqdt * za2;
@@ -8887,10 +8908,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s31 += ( + (r1_1*X[11]))*dtr + + plom_fsqrtf((r1_1*X[11]))*d1_1;
         X[1] = plom_clipf(f1);
         X[11] = plom_clipf(f11);
-        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), P72);
+        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), C2);
         const float r1_2 = P12;
         const float r2_2 = P22;
-        const float r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const float r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         float za4, zb4;
         plom_normal_pair32(ns, &za4, &zb4);
         const float d0_2 = sqdt * za4;
@@ -8905,10 +8926,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s32 += ( + (r1_2*X[12]))*dtr + + plom_fsqrtf((r1_2*X[12]))*d1_2;
         X[2] = plom_clipf(f2);
         X[12] = plom_clipf(f12);
-        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), P73);
+        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), C3);
         const float r1_3 = P13;
         const float r2_3 = P23;
-        const float r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const float r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         float za6, zb6;
         plom_normal_pair32(ns, &za6, &zb6);
         const float d0_3 = sqdt * za6;
@@ -8923,10 +8944,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s33 += ( + (r1_3*X[13]))*dtr + + plom_fsqrtf((r1_3*X[13]))*d1_3;
         X[3] = plom_clipf(f3);
         X[13] = plom_clipf(f13);
-        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), P74);
+        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), C4);
         const float r1_4 = P14;
         const float r2_4 = P24;
-        const float r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const float r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         float za8, zb8;
         plom_normal_pair32(ns, &za8, &zb8);
         const float d0_4 = sqdt * za8;
@@ -8941,10 +8962,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s34 += ( + (r1_4*X[14]))*dtr + + plom_fsqrtf((r1_4*X[14]))*d1_4;
         X[4] = plom_clipf(f4);
         X[14] = plom_clipf(f14);
-        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), P75);
+        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), C5);
         const float r1_5 = P15;
         const float r2_5 = P25;
-        const float r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const float r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         float za10, zb10;
         plom_normal_pair32(ns, &za10, &zb10);
         const float d0_5 = sqdt * za10;
@@ -8959,10 +8980,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s35 += ( + (r1_5*X[15]))*dtr + + plom_fsqrtf((r1_5*X[15]))*d1_5;
         X[5] = plom_clipf(f5);
         X[15] = plom_clipf(f15);
-        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), P76);
+        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), C6);
         const float r1_6 = P16;
         const float r2_6 = P26;
-        const float r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const float r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         float za12, zb12;
         plom_normal_pair32(ns, &za12, &zb12);
         const float d0_6 = sqdt * za12;
@@ -8977,10 +8998,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s36 += ( + (r1_6*X[16]))*dtr + + plom_fsqrtf((r1_6*X[16]))*d1_6;
         X[6] = plom_clipf(f6);
         X[16] = plom_clipf(f16);
-        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), P77);
+        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), C7);
         const float r1_7 = P17;
         const float r2_7 = P27;
-        const float r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const float r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         float za14, zb14;
         plom_normal_pair32(ns, &za14, &zb14);
         const float d0_7 = sqdt * za14;
@@ -8995,10 +9016,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s37 += ( + (r1_7*X[17]))*dtr + + plom_fsqrtf((r1_7*X[17]))*d1_7;
         X[7] = plom_clipf(f7);
         X[17] = plom_clipf(f17);
-        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), P78);
+        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), C8);
         const float r1_8 = P18;
         const float r2_8 = P28;
-        const float r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const float r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         float za16, zb16;
         plom_normal_pair32(ns, &za16, &zb16);
         const float d0_8 = sqdt * za16;
@@ -9013,10 +9034,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         s38 += ( + (r1_8*X[18]))*dtr + + plom_fsqrtf((r1_8*X[18]))*d1_8;
         X[8] = plom_clipf(f8);
         X[18] = plom_clipf(f18);
-        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), P79);
+        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), C9);
         const float r1_9 = P19;
         const float r2_9 = P29;
-        const float r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const float r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         float za18, zb18;
         plom_normal_pair32(ns, &za18, &zb18);
         const float d0_9 = sqdt * za18;
@@ -9065,10 +9086,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -9109,16 +9130,6 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const float P37 = lp[37];
     const float P38 = lp[38];
     const float P39 = lp[39];
-    const float P70 = PLOM_UNIFORM32(pv[70]);
-    const float P71 = PLOM_UNIFORM32(pv[71]);
-    const float P72 = PLOM_UNIFORM32(pv[72]);
-    const float P73 = PLOM_UNIFORM32(pv[73]);
-    const float P74 = PLOM_UNIFORM32(pv[74]);
-    const float P75 = PLOM_UNIFORM32(pv[75]);
-    const float P76 = PLOM_UNIFORM32(pv[76]);
-    const float P77 = PLOM_UNIFORM32(pv[77]);
-    const float P78 = PLOM_UNIFORM32(pv[78]);
-    const float P79 = PLOM_UNIFORM32(pv[79]);
     const float P80 = PLOM_UNIFORM32(pv[80]);
     const float P81 = PLOM_UNIFORM32(pv[81]);
     const float P82 = PLOM_UNIFORM32(pv[82]);
@@ -9233,6 +9244,17 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
+        const float C4 = (float)plom_cov_ld(cvk + 4);
+        const float C5 = (float)plom_cov_ld(cvk + 5);
+        const float C6 = (float)plom_cov_ld(cvk + 6);
+        const float C7 = (float)plom_cov_ld(cvk + 7);
+        const float C8 = (float)plom_cov_ld(cvk + 8);
+        const float C9 = (float)plom_cov_ld(cvk + 9);
         float s30 = 0.0f;
         float s31 = 0.0f;
         float s32 = 0.0f;
@@ -9243,100 +9265,100 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         float s37 = 0.0f;
         float s38 = 0.0f;
         float s39 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), P70);
+        const float r0_0 = plom_fdivf((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])), C0);
         const float r1_0 = P10;
         const float r2_0 = P20;
-        const float r3_0 = (P30 * (((P70 - X[0]) - X[10]) - X[20]));
+        const float r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
         const float f0 = X[0] + ( - (r0_0*X[0]) + (r3_0))*dtr;
         const float f10 = X[10] + ( - (r1_0*X[10]) + (r0_0*X[0]))*dtr;
         const float f20 = X[20] + ( - (r2_0*X[20]) + (r1_0*X[10]))*dtr;
         s30 += ( + (r1_0*X[10]))*dtr;
         X[0] = plom_clipf(f0);
         X[10] = plom_clipf(f10);
-        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), P71);
+        const float r0_1 = plom_fdivf((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])), C1);
         const float r1_1 = P11;
         const float r2_1 = P21;
-        const float r3_1 = (P31 * (((P71 - X[1]) - X[11]) - X[21]));
+        const float r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
         const float f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr;
         const float f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr;
         const float f21 = X[21] + ( - (r2_1*X[21]) + (r1_1*X[11]))*dtr;
         s31 += ( + (r1_1*X[11]))*dtr;
         X[1] = plom_clipf(f1);
         X[11] = plom_clipf(f11);
-        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), P72);
+        const float r0_2 = plom_fdivf((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])), C2);
         const float r1_2 = P12;
         const float r2_2 = P22;
-        const float r3_2 = (P32 * (((P72 - X[2]) - X[12]) - X[22]));
+        const float r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
         const float f2 = X[2] + ( - (r0_2*X[2]) + (r3_2))*dtr;
         const float f12 = X[12] + ( - (r1_2*X[12]) + (r0_2*X[2]))*dtr;
         const float f22 = X[22] + ( - (r2_2*X[22]) + (r1_2*X[12]))*dtr;
         s32 += ( + (r1_2*X[12]))*dtr;
         X[2] = plom_clipf(f2);
         X[12] = plom_clipf(f12);
-        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), P73);
+        const float r0_3 = plom_fdivf((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])), C3);
         const float r1_3 = P13;
         const float r2_3 = P23;
-        const float r3_3 = (P33 * (((P73 - X[3]) - X[13]) - X[23]));
+        const float r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
         const float f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr;
         const float f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr;
         const float f23 = X[23] + ( - (r2_3*X[23]) + (r1_3*X[13]))*dtr;
         s33 += ( + (r1_3*X[13]))*dtr;
         X[3] = plom_clipf(f3);
         X[13] = plom_clipf(f13);
-        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), P74);
+        const float r0_4 = plom_fdivf((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])), C4);
         const float r1_4 = P14;
         const float r2_4 = P24;
-        const float r3_4 = (P34 * (((P74 - X[4]) - X[14]) - X[24]));
+        const float r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
         const float f4 = X[4] + ( - (r0_4*X[4]) + (r3_4))*dtr;
         const float f14 = X[14] + ( - (r1_4*X[14]) + (r0_4*X[4]))*dtr;
         const float f24 = X[24] + ( - (r2_4*X[24]) + (r1_4*X[14]))*dtr;
         s34 += ( + (r1_4*X[14]))*dtr;
         X[4] = plom_clipf(f4);
         X[14] = plom_clipf(f14);
-        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), P75);
+        const float r0_5 = plom_fdivf((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])), C5);
         const float r1_5 = P15;
         const float r2_5 = P25;
-        const float r3_5 = (P35 * (((P75 - X[5]) - X[15]) - X[25]));
+        const float r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
         const float f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr;
         const float f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr;
         const float f25 = X[25] + ( - (r2_5*X[25]) + (r1_5*X[15]))*dtr;
         s35 += ( + (r1_5*X[15]))*dtr;
         X[5] = plom_clipf(f5);
         X[15] = plom_clipf(f15);
-        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), P76);
+        const float r0_6 = plom_fdivf((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])), C6);
         const float r1_6 = P16;
         const float r2_6 = P26;
-        const float r3_6 = (P36 * (((P76 - X[6]) - X[16]) - X[26]));
+        const float r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
         const float f6 = X[6] + ( - (r0_6*X[6]) + (r3_6))*dtr;
         const float f16 = X[16] + ( - (r1_6*X[16]) + (r0_6*X[6]))*dtr;
         const float f26 = X[26] + ( - (r2_6*X[26]) + (r1_6*X[16]))*dtr;
         s36 += ( + (r1_6*X[16]))*dtr;
         X[6] = plom_clipf(f6);
         X[16] = plom_clipf(f16);
-        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), P77);
+        const float r0_7 = plom_fdivf((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])), C7);
         const float r1_7 = P17;
         const float r2_7 = P27;
-        const float r3_7 = (P37 * (((P77 - X[7]) - X[17]) - X[27]));
+        const float r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
         const float f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr;
         const float f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr;
         const float f27 = X[27] + ( - (r2_7*X[27]) + (r1_7*X[17]))*dtr;
         s37 += ( + (r1_7*X[17]))*dtr;
         X[7] = plom_clipf(f7);
         X[17] = plom_clipf(f17);
-        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), P78);
+        const float r0_8 = plom_fdivf((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])), C8);
         const float r1_8 = P18;
         const float r2_8 = P28;
-        const float r3_8 = (P38 * (((P78 - X[8]) - X[18]) - X[28]));
+        const float r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
         const float f8 = X[8] + ( - (r0_8*X[8]) + (r3_8))*dtr;
         const float f18 = X[18] + ( - (r1_8*X[18]) + (r0_8*X[8]))*dtr;
         const float f28 = X[28] + ( - (r2_8*X[28]) + (r1_8*X[18]))*dtr;
         s38 += ( + (r1_8*X[18]))*dtr;
         X[8] = plom_clipf(f8);
         X[18] = plom_clipf(f18);
-        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), P79);
+        const float r0_9 = plom_fdivf((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])), C9);
         const float r1_9 = P19;
         const float r2_9 = P29;
-        const float r3_9 = (P39 * (((P79 - X[9]) - X[19]) - X[29]));
+        const float r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
         const float f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr;
         const float f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr;
         const float f29 = X[29] + ( - (r2_9*X[29]) + (r1_9*X[19]))*dtr;
@@ -9377,9 +9399,9 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_proj2obs_pp_f32(const float *X, const double *pvd, const double *lpd, double *obs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_proj2obs_pp_f32(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs)
 {
-    (void)pvd; (void)lpd; (void)X;
+    (void)pvd; (void)lpd; (void)cvd; (void)X;
     obs[0] = (double)X[30];
     obs[1] = (double)X[31];
     obs[2] = (double)X[32];
@@ -9392,11 +9414,11 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_proj2ob
     obs[9] = (double)X[39];
 }
 
-static __device__ inline __attribute__((always_inline)) double plom_seir10_loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, double t, double like_min, int *n_floor)
+static __device__ inline __attribute__((always_inline)) double plom_seir10_loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor)
 {
     double loglike = 0.0;
     int nf = 0;
-    (void)t; (void)pvd; (void)lpd;
+    (void)t; (void)pvd; (void)lpd; (void)cvd;
     if (!plom_isnan(y[0])) {
         const double x = obs[0];
         const double gsl_mu = (lpd[50] * x);
@@ -9492,47 +9514,47 @@ static __device__ inline __attribute__((always_inline)) double plom_seir10_logli
 }
 
 struct plom_model_seir10 {
-    static constexpr int S = 40, N_TS = 10, INC_OFFSET = 30, N_INC = 10, PV = 180, N_CAC = 10, N_HAT = 50, NPP = 70;
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+    static constexpr int S = 40, N_TS = 10, INC_OFFSET = 30, N_INC = 10, PV = 180, N_CAC = 10, N_HAT = 50, NPP = 70, FIX_OFF = 70, N_FIX = 10;
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
     {
-        if constexpr (V == 0) plom_seir10_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 4) plom_seir10_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 5) plom_seir10_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
+        if constexpr (V == 0) plom_seir10_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 4) plom_seir10_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 5) plom_seir10_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
     {
-        if constexpr (V == 0) plom_seir10_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 4) plom_seir10_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 5) plom_seir10_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
+        if constexpr (V == 0) plom_seir10_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 4) plom_seir10_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 5) plom_seir10_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
     }
-    static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, double *obs) { plom_seir10_proj2obs_f64(X, pvd, obs); }
-    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, double *obs) { plom_seir10_proj2obs_pp_f64(X, pvd, lpd, obs); }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+    static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, const double *cvd, double *obs) { plom_seir10_proj2obs_f64(X, pvd, cvd, obs); }
+    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_seir10_proj2obs_pp_f64(X, pvd, lpd, cvd, obs); }
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
     {
-        if constexpr (V == 0) plom_seir10_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt);
+        if constexpr (V == 0) plom_seir10_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
     {
-        if constexpr (V == 0) plom_seir10_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
+        if constexpr (V == 0) plom_seir10_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
     }
-    static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, double *obs) { plom_seir10_proj2obs_f32(X, pvd, obs); }
-    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, double *obs) { plom_seir10_proj2obs_pp_f32(X, pvd, lpd, obs); }
-    static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor) { return plom_seir10_loglik(obs, y, pvd, t, like_min, n_floor); }
-    static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, double t, double like_min, int *n_floor) { return plom_seir10_loglik_pp(obs, y, pvd, lpd, t, like_min, n_floor); }
-    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, double t, double *out) { plom_seir10_hat_values(X, pvd, t, out); }
-    static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var) { plom_seir10_obs_moments(obs, pvd, t, mean, var); }
+    static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_seir10_proj2obs_f32(X, pvd, cvd, obs); }
+    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_seir10_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
+    static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor) { return plom_seir10_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }
+    static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor) { return plom_seir10_loglik_pp(obs, y, pvd, lpd, cvd, t, like_min, n_floor); }
+    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out) { plom_seir10_hat_values(X, pvd, cvd, t, out); }
+    static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var) { plom_seir10_obs_moments(obs, pvd, cvd, t, mean, var); }
 };
 #endif
 

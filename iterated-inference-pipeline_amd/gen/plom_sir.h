@@ -14,21 +14,24 @@
 #define PLOM_SIR_N_ENV 0
 #define PLOM_SIR_N_HAT 4
 #define PLOM_SIR_N_PP 4
+#define PLOM_SIR_FIX_OFF 4
+#define PLOM_SIR_N_FIX 1
 
-PLOM_HD void plom_sir_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_sir_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
-    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
@@ -45,20 +48,21 @@ PLOM_HD void plom_sir_interval_full_f64(double *X, const double *pv, plom_nstrea
     }
 }
 
-PLOM_HD void plom_sir_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_sir_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
-    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         const double f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
@@ -71,20 +75,21 @@ PLOM_HD void plom_sir_interval_no_dem_sto_f64(double *X, const double *pv, plom_
     }
 }
 
-PLOM_HD void plom_sir_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_sir_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
-    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
@@ -101,20 +106,21 @@ PLOM_HD void plom_sir_interval_no_env_sto_f64(double *X, const double *pv, plom_
     }
 }
 
-PLOM_HD void plom_sir_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_sir_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
-    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         const double f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
@@ -127,17 +133,17 @@ PLOM_HD void plom_sir_interval_no_dem_sto_no_env_sto_f64(double *X, const double
     }
 }
 
-PLOM_HD void plom_sir_proj2obs_f64(const double *X, const double *pvd, double *obs)
+PLOM_HD void plom_sir_proj2obs_f64(const double *X, const double *pvd, const double *cvd, double *obs)
 {
-    (void)pvd; (void)X;
+    (void)pvd; (void)cvd; (void)X;
     obs[0] = (double)X[2];
 }
 
-PLOM_HD double plom_sir_loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor)
+PLOM_HD double plom_sir_loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor)
 {
     double loglike = 0.0;
     int nf = 0;
-    (void)t; (void)pvd;
+    (void)t; (void)pvd; (void)cvd;
     if (!plom_isnan(y[0])) {
         const double x = obs[0];
         const double gsl_mu = (pvd[2] * x);
@@ -151,9 +157,9 @@ PLOM_HD double plom_sir_loglik(const double *obs, const double *y, const double 
     return loglike;
 }
 
-PLOM_HD void plom_sir_obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var)
+PLOM_HD void plom_sir_obs_moments(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var)
 {
-    (void)t; (void)pvd;
+    (void)t; (void)pvd; (void)cvd;
     {
         const double x = obs[0];
         mean[0] = (pvd[2] * x);
@@ -161,34 +167,35 @@ PLOM_HD void plom_sir_obs_moments(const double *obs, const double *pvd, double t
     }
 }
 
-PLOM_HD void plom_sir_hat_values(const double *X, const double *pvd, double t, double *out)
+PLOM_HD void plom_sir_hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out)
 {
-    (void)t;
+    (void)t; (void)pvd; (void)cvd;
     out[0] = X[0];
     out[1] = X[1];
-    out[2] = pvd[4] - (X[0] + X[1]);
+    out[2] = cvd[0] - (X[0] + X[1]);
     double obs[1], var[1];
-    plom_sir_proj2obs_f64(X, pvd, obs);
-    plom_sir_obs_moments(obs, pvd, t, out + 3, var);
+    plom_sir_proj2obs_f64(X, pvd, cvd, obs);
+    plom_sir_obs_moments(obs, pvd, cvd, t, out + 3, var);
 }
 
-PLOM_HD void plom_sir_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_sir_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
-    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double stay0 = 0.0;
         double stay1 = 0.0;
         (void)n0_0; (void)n1_0;
-        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         {
             double p[2], cnt[2];
@@ -227,23 +234,24 @@ PLOM_HD void plom_sir_interval_psr_full_f64(double *X, const double *pv, plom_ns
     }
 }
 
-PLOM_HD void plom_sir_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_sir_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
-    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double stay0 = 0.0;
         double stay1 = 0.0;
         (void)n0_0; (void)n1_0;
-        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         {
             double p[2], cnt[2];
@@ -282,35 +290,36 @@ PLOM_HD void plom_sir_interval_psr_no_env_sto_f64(double *X, const double *pv, p
     }
 }
 
-PLOM_HD void plom_sir_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off)
+PLOM_HD void plom_sir_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs)
 {
     if (noises_off & PLOM_IMPL_PSR) {
-        if (noises_off & PLOM_NO_ENV_STO) plom_sir_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else plom_sir_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);
+        if (noises_off & PLOM_NO_ENV_STO) plom_sir_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else plom_sir_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
         return;
     }
     const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);
-    if (v == 0u) plom_sir_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt);
-    else if (v == PLOM_NO_DEM_STO) plom_sir_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-    else if (v == PLOM_NO_ENV_STO) plom_sir_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-    else plom_sir_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
+    if (v == 0u) plom_sir_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+    else if (v == PLOM_NO_DEM_STO) plom_sir_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+    else if (v == PLOM_NO_ENV_STO) plom_sir_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+    else plom_sir_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
 }
 
 #if defined(__HIPCC__)
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
-    const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
         float s2 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
+        const float r0_0 = plom_fdivf((P0 * X[1]), C0);
         const float r1_0 = P1;
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
@@ -327,20 +336,21 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_f
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
-    const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
         float s2 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
+        const float r0_0 = plom_fdivf((P0 * X[1]), C0);
         const float r1_0 = P1;
         const float f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
@@ -353,20 +363,21 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
-    const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
         float s2 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
+        const float r0_0 = plom_fdivf((P0 * X[1]), C0);
         const float r1_0 = P1;
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
@@ -383,20 +394,21 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
-    const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
         float s2 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
+        const float r0_0 = plom_fdivf((P0 * X[1]), C0);
         const float r1_0 = P1;
         const float f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
@@ -409,29 +421,30 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_proj2obs_f32(const float *X, const double *pvd, double *obs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_proj2obs_f32(const float *X, const double *pvd, const double *cvd, double *obs)
 {
-    (void)pvd; (void)X;
+    (void)pvd; (void)cvd; (void)X;
     obs[0] = (double)X[2];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
-    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double stay0 = 0.0;
         double stay1 = 0.0;
         (void)n0_0; (void)n1_0;
-        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         {
             double p[2], cnt[2];
@@ -470,23 +483,24 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_p
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
-    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double stay0 = 0.0;
         double stay1 = 0.0;
         (void)n0_0; (void)n1_0;
-        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         {
             double p[2], cnt[2];
@@ -525,20 +539,21 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_p
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
-    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
@@ -555,20 +570,21 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_f
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
-    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         const double f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
@@ -581,20 +597,21 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
-    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
@@ -611,20 +628,21 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
-    const double P4 = PLOM_UNIFORM64(pv[4]);
     X[2] = 0.0; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / P4);
+        const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         const double f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
@@ -637,26 +655,27 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_proj2obs_pp_f64(const double *X, const double *pvd, const double *lpd, double *obs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_proj2obs_pp_f64(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs)
 {
-    (void)pvd; (void)lpd; (void)X;
+    (void)pvd; (void)lpd; (void)cvd; (void)X;
     obs[0] = (double)X[2];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
-    const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
         float s2 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
+        const float r0_0 = plom_fdivf((P0 * X[1]), C0);
         const float r1_0 = P1;
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
@@ -673,20 +692,21 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_f
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
-    const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
         float s2 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
+        const float r0_0 = plom_fdivf((P0 * X[1]), C0);
         const float r1_0 = P1;
         const float f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
@@ -699,20 +719,21 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
-    const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
         float s2 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
+        const float r0_0 = plom_fdivf((P0 * X[1]), C0);
         const float r1_0 = P1;
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
@@ -729,20 +750,21 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
-    const float P4 = PLOM_UNIFORM32(pv[4]);
     X[2] = 0.0f; /* reset_inc */
     double t = t0;
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
         float s2 = 0.0f;
-        const float r0_0 = plom_fdivf((P0 * X[1]), P4);
+        const float r0_0 = plom_fdivf((P0 * X[1]), C0);
         const float r1_0 = P1;
         const float f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const float f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
@@ -755,17 +777,17 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_proj2obs_pp_f32(const float *X, const double *pvd, const double *lpd, double *obs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_proj2obs_pp_f32(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs)
 {
-    (void)pvd; (void)lpd; (void)X;
+    (void)pvd; (void)lpd; (void)cvd; (void)X;
     obs[0] = (double)X[2];
 }
 
-static __device__ inline __attribute__((always_inline)) double plom_sir_loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, double t, double like_min, int *n_floor)
+static __device__ inline __attribute__((always_inline)) double plom_sir_loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor)
 {
     double loglike = 0.0;
     int nf = 0;
-    (void)t; (void)pvd; (void)lpd;
+    (void)t; (void)pvd; (void)lpd; (void)cvd;
     if (!plom_isnan(y[0])) {
         const double x = obs[0];
         const double gsl_mu = (lpd[2] * x);
@@ -780,47 +802,47 @@ static __device__ inline __attribute__((always_inline)) double plom_sir_loglik_p
 }
 
 struct plom_model_sir {
-    static constexpr int S = 3, N_TS = 1, INC_OFFSET = 2, N_INC = 1, PV = 5, N_CAC = 1, N_HAT = 4, NPP = 4;
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+    static constexpr int S = 3, N_TS = 1, INC_OFFSET = 2, N_INC = 1, PV = 5, N_CAC = 1, N_HAT = 4, NPP = 4, FIX_OFF = 4, N_FIX = 1;
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
     {
-        if constexpr (V == 0) plom_sir_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 2) plom_sir_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 4) plom_sir_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 5) plom_sir_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
+        if constexpr (V == 0) plom_sir_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 2) plom_sir_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 4) plom_sir_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 5) plom_sir_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
     {
-        if constexpr (V == 0) plom_sir_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 2) plom_sir_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 4) plom_sir_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 5) plom_sir_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
+        if constexpr (V == 0) plom_sir_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 2) plom_sir_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 4) plom_sir_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 5) plom_sir_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
     }
-    static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, double *obs) { plom_sir_proj2obs_f64(X, pvd, obs); }
-    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, double *obs) { plom_sir_proj2obs_pp_f64(X, pvd, lpd, obs); }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+    static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, const double *cvd, double *obs) { plom_sir_proj2obs_f64(X, pvd, cvd, obs); }
+    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sir_proj2obs_pp_f64(X, pvd, lpd, cvd, obs); }
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
     {
-        if constexpr (V == 0) plom_sir_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 2) plom_sir_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt);
+        if constexpr (V == 0) plom_sir_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 2) plom_sir_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
     {
-        if constexpr (V == 0) plom_sir_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 2) plom_sir_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
+        if constexpr (V == 0) plom_sir_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 2) plom_sir_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
     }
-    static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, double *obs) { plom_sir_proj2obs_f32(X, pvd, obs); }
-    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, double *obs) { plom_sir_proj2obs_pp_f32(X, pvd, lpd, obs); }
-    static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor) { return plom_sir_loglik(obs, y, pvd, t, like_min, n_floor); }
-    static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, double t, double like_min, int *n_floor) { return plom_sir_loglik_pp(obs, y, pvd, lpd, t, like_min, n_floor); }
-    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, double t, double *out) { plom_sir_hat_values(X, pvd, t, out); }
-    static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var) { plom_sir_obs_moments(obs, pvd, t, mean, var); }
+    static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_sir_proj2obs_f32(X, pvd, cvd, obs); }
+    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sir_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
+    static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor) { return plom_sir_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }
+    static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor) { return plom_sir_loglik_pp(obs, y, pvd, lpd, cvd, t, like_min, n_floor); }
+    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out) { plom_sir_hat_values(X, pvd, cvd, t, out); }
+    static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var) { plom_sir_obs_moments(obs, pvd, cvd, t, mean, var); }
 };
 #endif
 

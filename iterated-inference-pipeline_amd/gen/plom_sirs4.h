@@ -14,11 +14,13 @@
 #define PLOM_SIRS4_N_ENV 1
 #define PLOM_SIRS4_N_HAT 16
 #define PLOM_SIRS4_N_PP 28
+#define PLOM_SIRS4_FIX_OFF 28
+#define PLOM_SIRS4_N_FIX 4
 
-PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -39,10 +41,6 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
     const double P17 = PLOM_UNIFORM64(pv[17]);
     const double P18 = PLOM_UNIFORM64(pv[18]);
     const double P19 = PLOM_UNIFORM64(pv[19]);
-    const double P28 = PLOM_UNIFORM64(pv[28]);
-    const double P29 = PLOM_UNIFORM64(pv[29]);
-    const double P30 = PLOM_UNIFORM64(pv[30]);
-    const double P31 = PLOM_UNIFORM64(pv[31]);
     const double P32 = PLOM_UNIFORM64(pv[32]);
     const double P33 = PLOM_UNIFORM64(pv[33]);
     const double P34 = PLOM_UNIFORM64(pv[34]);
@@ -67,14 +65,19 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
         const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
         const double r1_0 = P8;
-        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
         const double d0_0 = sqdt * za0;
@@ -87,9 +90,9 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[4]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
-        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
         const double d0_1 = sqdt * za2;
@@ -102,9 +105,9 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[5]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
-        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         double za4, zb4;
         plom_normal_pair64(ns, &za4, &zb4);
         const double d0_2 = sqdt * za4;
@@ -117,9 +120,9 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[6]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
-        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         double za6, zb6;
         plom_normal_pair64(ns, &za6, &zb6);
         const double d0_3 = sqdt * za6;
@@ -148,10 +151,10 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
     }
 }
 
-PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -172,10 +175,6 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
     const double P17 = PLOM_UNIFORM64(pv[17]);
     const double P18 = PLOM_UNIFORM64(pv[18]);
     const double P19 = PLOM_UNIFORM64(pv[19]);
-    const double P28 = PLOM_UNIFORM64(pv[28]);
-    const double P29 = PLOM_UNIFORM64(pv[29]);
-    const double P30 = PLOM_UNIFORM64(pv[30]);
-    const double P31 = PLOM_UNIFORM64(pv[31]);
     const double P32 = PLOM_UNIFORM64(pv[32]);
     const double P33 = PLOM_UNIFORM64(pv[33]);
     const double P34 = PLOM_UNIFORM64(pv[34]);
@@ -200,14 +199,19 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
         const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
         const double r1_0 = P8;
-        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
         const double w0_0 = sqdt * za0;
@@ -215,17 +219,17 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
-        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const double w0_1 = sqdt * zb0;
         const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - (r0_1*X[1])*P17*w0_1;
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
-        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         double za1, zb1;
         plom_normal_pair64(ns, &za1, &zb1);
         const double w0_2 = sqdt * za1;
@@ -233,9 +237,9 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
-        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double w0_3 = sqdt * zb1;
         const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - (r0_3*X[3])*P19*w0_3;
         const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + + (r0_3*X[3])*P19*w0_3;
@@ -257,10 +261,10 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
     }
 }
 
-PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -277,10 +281,6 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
     const double P13 = PLOM_UNIFORM64(pv[13]);
     const double P14 = PLOM_UNIFORM64(pv[14]);
     const double P15 = PLOM_UNIFORM64(pv[15]);
-    const double P28 = PLOM_UNIFORM64(pv[28]);
-    const double P29 = PLOM_UNIFORM64(pv[29]);
-    const double P30 = PLOM_UNIFORM64(pv[30]);
-    const double P31 = PLOM_UNIFORM64(pv[31]);
     const double P32 = PLOM_UNIFORM64(pv[32]);
     const double P33 = PLOM_UNIFORM64(pv[33]);
     const double P34 = PLOM_UNIFORM64(pv[34]);
@@ -305,14 +305,19 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
         const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
         const double r1_0 = P8;
-        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
         const double d0_0 = sqdt * za0;
@@ -324,9 +329,9 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[4]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
-        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const double d0_1 = sqdt * zb1;
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
@@ -336,9 +341,9 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[5]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt((r0_1*X[1]))*d0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
-        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         double za3, zb3;
         plom_normal_pair64(ns, &za3, &zb3);
         const double d0_2 = sqdt * za3;
@@ -350,9 +355,9 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[6]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt((r0_2*X[2]))*d0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
-        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double d0_3 = sqdt * zb4;
         double za5, zb5;
         plom_normal_pair64(ns, &za5, &zb5);
@@ -378,10 +383,10 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
     }
 }
 
-PLOM_HD void plom_sirs4_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_sirs4_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -398,10 +403,6 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_no_env_sto_f64(double *X, const doub
     const double P13 = PLOM_UNIFORM64(pv[13]);
     const double P14 = PLOM_UNIFORM64(pv[14]);
     const double P15 = PLOM_UNIFORM64(pv[15]);
-    const double P28 = PLOM_UNIFORM64(pv[28]);
-    const double P29 = PLOM_UNIFORM64(pv[29]);
-    const double P30 = PLOM_UNIFORM64(pv[30]);
-    const double P31 = PLOM_UNIFORM64(pv[31]);
     const double P32 = PLOM_UNIFORM64(pv[32]);
     const double P33 = PLOM_UNIFORM64(pv[33]);
     const double P34 = PLOM_UNIFORM64(pv[34]);
@@ -426,35 +427,40 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_no_env_sto_f64(double *X, const doub
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
         const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
         const double r1_0 = P8;
-        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr;
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr;
         s8 += ( + (r0_0*X[0]))*dtr;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
-        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr;
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr;
         s9 += ( + (r0_1*X[1]))*dtr;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
-        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr;
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr;
         s10 += ( + (r0_2*X[2]))*dtr;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
-        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr;
         const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr;
         s11 += ( + (r0_3*X[3]))*dtr;
@@ -475,20 +481,20 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_no_env_sto_f64(double *X, const doub
     }
 }
 
-PLOM_HD void plom_sirs4_proj2obs_f64(const double *X, const double *pvd, double *obs)
+PLOM_HD void plom_sirs4_proj2obs_f64(const double *X, const double *pvd, const double *cvd, double *obs)
 {
-    (void)pvd; (void)X;
+    (void)pvd; (void)cvd; (void)X;
     obs[0] = (double)X[8];
     obs[1] = (double)X[9];
     obs[2] = (double)X[10];
     obs[3] = (double)X[11];
 }
 
-PLOM_HD double plom_sirs4_loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor)
+PLOM_HD double plom_sirs4_loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor)
 {
     double loglike = 0.0;
     int nf = 0;
-    (void)t; (void)pvd;
+    (void)t; (void)pvd; (void)cvd;
     if (!plom_isnan(y[0])) {
         const double x = obs[0];
         const double gsl_mu = (pvd[20] * x);
@@ -529,9 +535,9 @@ PLOM_HD double plom_sirs4_loglik(const double *obs, const double *y, const doubl
     return loglike;
 }
 
-PLOM_HD void plom_sirs4_obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var)
+PLOM_HD void plom_sirs4_obs_moments(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var)
 {
-    (void)t; (void)pvd;
+    (void)t; (void)pvd; (void)cvd;
     {
         const double x = obs[0];
         mean[0] = (pvd[20] * x);
@@ -554,9 +560,9 @@ PLOM_HD void plom_sirs4_obs_moments(const double *obs, const double *pvd, double
     }
 }
 
-PLOM_HD void plom_sirs4_hat_values(const double *X, const double *pvd, double t, double *out)
+PLOM_HD void plom_sirs4_hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out)
 {
-    (void)t;
+    (void)t; (void)pvd; (void)cvd;
     out[0] = X[0];
     out[1] = X[1];
     out[2] = X[2];
@@ -565,18 +571,18 @@ PLOM_HD void plom_sirs4_hat_values(const double *X, const double *pvd, double t,
     out[5] = X[5];
     out[6] = X[6];
     out[7] = X[7];
-    out[8] = pvd[28] - (X[0] + X[4]);
-    out[9] = pvd[29] - (X[1] + X[5]);
-    out[10] = pvd[30] - (X[2] + X[6]);
-    out[11] = pvd[31] - (X[3] + X[7]);
+    out[8] = cvd[0] - (X[0] + X[4]);
+    out[9] = cvd[1] - (X[1] + X[5]);
+    out[10] = cvd[2] - (X[2] + X[6]);
+    out[11] = cvd[3] - (X[3] + X[7]);
     double obs[4], var[4];
-    plom_sirs4_proj2obs_f64(X, pvd, obs);
-    plom_sirs4_obs_moments(obs, pvd, t, out + 12, var);
+    plom_sirs4_proj2obs_f64(X, pvd, cvd, obs);
+    plom_sirs4_obs_moments(obs, pvd, cvd, t, out + 12, var);
 }
 
-PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -597,10 +603,6 @@ PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_
     const double P17 = PLOM_UNIFORM64(pv[17]);
     const double P18 = PLOM_UNIFORM64(pv[18]);
     const double P19 = PLOM_UNIFORM64(pv[19]);
-    const double P28 = PLOM_UNIFORM64(pv[28]);
-    const double P29 = PLOM_UNIFORM64(pv[29]);
-    const double P30 = PLOM_UNIFORM64(pv[30]);
-    const double P31 = PLOM_UNIFORM64(pv[31]);
     const double P32 = PLOM_UNIFORM64(pv[32]);
     const double P33 = PLOM_UNIFORM64(pv[33]);
     const double P34 = PLOM_UNIFORM64(pv[34]);
@@ -625,6 +627,11 @@ PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
         const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double n0_0 = 0.0;
         double n1_0 = 0.0;
@@ -647,18 +654,18 @@ PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_
         double stay6 = 0.0;
         double stay7 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n0_3; (void)n1_3; (void)n2_3;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
         const double r1_0 = P8;
-        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
-        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
-        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
-        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double sd0_0 = P16;
         const double v0_0 = sd0_0 * sd0_0;
         const double xi0_0 = (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt;
@@ -811,9 +818,9 @@ PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_
     }
 }
 
-PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -830,10 +837,6 @@ PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv,
     const double P13 = PLOM_UNIFORM64(pv[13]);
     const double P14 = PLOM_UNIFORM64(pv[14]);
     const double P15 = PLOM_UNIFORM64(pv[15]);
-    const double P28 = PLOM_UNIFORM64(pv[28]);
-    const double P29 = PLOM_UNIFORM64(pv[29]);
-    const double P30 = PLOM_UNIFORM64(pv[30]);
-    const double P31 = PLOM_UNIFORM64(pv[31]);
     const double P32 = PLOM_UNIFORM64(pv[32]);
     const double P33 = PLOM_UNIFORM64(pv[33]);
     const double P34 = PLOM_UNIFORM64(pv[34]);
@@ -858,6 +861,11 @@ PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv,
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
         const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double n0_0 = 0.0;
         double n1_0 = 0.0;
@@ -880,18 +888,18 @@ PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv,
         double stay6 = 0.0;
         double stay7 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n0_3; (void)n1_3; (void)n2_3;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
         const double r1_0 = P8;
-        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
-        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
-        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
-        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double xi0_0 = 1.0;
         {
             double p[2], cnt[2];
@@ -1036,25 +1044,25 @@ PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv,
     }
 }
 
-PLOM_HD void plom_sirs4_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off)
+PLOM_HD void plom_sirs4_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs)
 {
     if (noises_off & PLOM_IMPL_PSR) {
-        if (noises_off & PLOM_NO_ENV_STO) plom_sirs4_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else plom_sirs4_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);
+        if (noises_off & PLOM_NO_ENV_STO) plom_sirs4_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else plom_sirs4_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
         return;
     }
     const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);
-    if (v == 0u) plom_sirs4_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt);
-    else if (v == PLOM_NO_DEM_STO) plom_sirs4_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-    else if (v == PLOM_NO_ENV_STO) plom_sirs4_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-    else plom_sirs4_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
+    if (v == 0u) plom_sirs4_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+    else if (v == PLOM_NO_DEM_STO) plom_sirs4_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+    else if (v == PLOM_NO_ENV_STO) plom_sirs4_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+    else plom_sirs4_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
 }
 
 #if defined(__HIPCC__)
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -1075,10 +1083,6 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const float P17 = PLOM_UNIFORM32(pv[17]);
     const float P18 = PLOM_UNIFORM32(pv[18]);
     const float P19 = PLOM_UNIFORM32(pv[19]);
-    const float P28 = PLOM_UNIFORM32(pv[28]);
-    const float P29 = PLOM_UNIFORM32(pv[29]);
-    const float P30 = PLOM_UNIFORM32(pv[30]);
-    const float P31 = PLOM_UNIFORM32(pv[31]);
     const float P32 = PLOM_UNIFORM32(pv[32]);
     const float P33 = PLOM_UNIFORM32(pv[33]);
     const float P34 = PLOM_UNIFORM32(pv[34]);
@@ -1103,14 +1107,19 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
         const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), P28);
+        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0);
         const float r1_0 = P8;
-        const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const float r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
         const float d0_0 = sqdt * za0;
@@ -1123,9 +1132,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_fsqrtf((r1_0*X[4]))*d1_0+ plom_fsqrtf((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + plom_fsqrtf((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         X[0] = plom_clipf(f0);
-        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), P29);
+        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1);
         const float r1_1 = P9;
-        const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const float r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         float za2, zb2;
         plom_normal_pair32(ns, &za2, &zb2);
         const float d0_1 = sqdt * za2;
@@ -1138,9 +1147,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_fsqrtf((r1_1*X[5]))*d1_1+ plom_fsqrtf((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + plom_fsqrtf((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         X[1] = plom_clipf(f1);
-        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), P30);
+        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2);
         const float r1_2 = P10;
-        const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const float r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         float za4, zb4;
         plom_normal_pair32(ns, &za4, &zb4);
         const float d0_2 = sqdt * za4;
@@ -1153,9 +1162,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_fsqrtf((r1_2*X[6]))*d1_2+ plom_fsqrtf((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + plom_fsqrtf((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         X[2] = plom_clipf(f2);
-        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), P31);
+        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3);
         const float r1_3 = P11;
-        const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const float r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         float za6, zb6;
         plom_normal_pair32(ns, &za6, &zb6);
         const float d0_3 = sqdt * za6;
@@ -1184,10 +1193,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -1208,10 +1217,6 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const float P17 = PLOM_UNIFORM32(pv[17]);
     const float P18 = PLOM_UNIFORM32(pv[18]);
     const float P19 = PLOM_UNIFORM32(pv[19]);
-    const float P28 = PLOM_UNIFORM32(pv[28]);
-    const float P29 = PLOM_UNIFORM32(pv[29]);
-    const float P30 = PLOM_UNIFORM32(pv[30]);
-    const float P31 = PLOM_UNIFORM32(pv[31]);
     const float P32 = PLOM_UNIFORM32(pv[32]);
     const float P33 = PLOM_UNIFORM32(pv[33]);
     const float P34 = PLOM_UNIFORM32(pv[34]);
@@ -1236,14 +1241,19 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
         const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), P28);
+        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0);
         const float r1_0 = P8;
-        const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const float r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
         const float w0_0 = sqdt * za0;
@@ -1251,17 +1261,17 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         X[0] = plom_clipf(f0);
-        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), P29);
+        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1);
         const float r1_1 = P9;
-        const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const float r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const float w0_1 = sqdt * zb0;
         const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - (r0_1*X[1])*P17*w0_1;
         const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         X[1] = plom_clipf(f1);
-        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), P30);
+        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2);
         const float r1_2 = P10;
-        const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const float r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         float za1, zb1;
         plom_normal_pair32(ns, &za1, &zb1);
         const float w0_2 = sqdt * za1;
@@ -1269,9 +1279,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         X[2] = plom_clipf(f2);
-        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), P31);
+        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3);
         const float r1_3 = P11;
-        const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const float r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const float w0_3 = sqdt * zb1;
         const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - (r0_3*X[3])*P19*w0_3;
         const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + + (r0_3*X[3])*P19*w0_3;
@@ -1293,10 +1303,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -1313,10 +1323,6 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const float P13 = PLOM_UNIFORM32(pv[13]);
     const float P14 = PLOM_UNIFORM32(pv[14]);
     const float P15 = PLOM_UNIFORM32(pv[15]);
-    const float P28 = PLOM_UNIFORM32(pv[28]);
-    const float P29 = PLOM_UNIFORM32(pv[29]);
-    const float P30 = PLOM_UNIFORM32(pv[30]);
-    const float P31 = PLOM_UNIFORM32(pv[31]);
     const float P32 = PLOM_UNIFORM32(pv[32]);
     const float P33 = PLOM_UNIFORM32(pv[33]);
     const float P34 = PLOM_UNIFORM32(pv[34]);
@@ -1341,14 +1347,19 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
         const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), P28);
+        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0);
         const float r1_0 = P8;
-        const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const float r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
         const float d0_0 = sqdt * za0;
@@ -1360,9 +1371,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_fsqrtf((r1_0*X[4]))*d1_0+ plom_fsqrtf((r0_0*X[0]))*d0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + plom_fsqrtf((r0_0*X[0]))*d0_0;
         X[0] = plom_clipf(f0);
-        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), P29);
+        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1);
         const float r1_1 = P9;
-        const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const float r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const float d0_1 = sqdt * zb1;
         float za2, zb2;
         plom_normal_pair32(ns, &za2, &zb2);
@@ -1372,9 +1383,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_fsqrtf((r1_1*X[5]))*d1_1+ plom_fsqrtf((r0_1*X[1]))*d0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + plom_fsqrtf((r0_1*X[1]))*d0_1;
         X[1] = plom_clipf(f1);
-        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), P30);
+        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2);
         const float r1_2 = P10;
-        const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const float r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         float za3, zb3;
         plom_normal_pair32(ns, &za3, &zb3);
         const float d0_2 = sqdt * za3;
@@ -1386,9 +1397,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_fsqrtf((r1_2*X[6]))*d1_2+ plom_fsqrtf((r0_2*X[2]))*d0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + plom_fsqrtf((r0_2*X[2]))*d0_2;
         X[2] = plom_clipf(f2);
-        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), P31);
+        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3);
         const float r1_3 = P11;
-        const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const float r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const float d0_3 = sqdt * zb4;
         float za5, zb5;
         plom_normal_pair32(ns, &za5, &zb5);
@@ -1414,10 +1425,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -1434,10 +1445,6 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const float P13 = PLOM_UNIFORM32(pv[13]);
     const float P14 = PLOM_UNIFORM32(pv[14]);
     const float P15 = PLOM_UNIFORM32(pv[15]);
-    const float P28 = PLOM_UNIFORM32(pv[28]);
-    const float P29 = PLOM_UNIFORM32(pv[29]);
-    const float P30 = PLOM_UNIFORM32(pv[30]);
-    const float P31 = PLOM_UNIFORM32(pv[31]);
     const float P32 = PLOM_UNIFORM32(pv[32]);
     const float P33 = PLOM_UNIFORM32(pv[33]);
     const float P34 = PLOM_UNIFORM32(pv[34]);
@@ -1462,35 +1469,40 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
         const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), P28);
+        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0);
         const float r1_0 = P8;
-        const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const float r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr;
         const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr;
         s8 += ( + (r0_0*X[0]))*dtr;
         X[0] = plom_clipf(f0);
-        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), P29);
+        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1);
         const float r1_1 = P9;
-        const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const float r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr;
         const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr;
         s9 += ( + (r0_1*X[1]))*dtr;
         X[1] = plom_clipf(f1);
-        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), P30);
+        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2);
         const float r1_2 = P10;
-        const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const float r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr;
         const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr;
         s10 += ( + (r0_2*X[2]))*dtr;
         X[2] = plom_clipf(f2);
-        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), P31);
+        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3);
         const float r1_3 = P11;
-        const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const float r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr;
         const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr;
         s11 += ( + (r0_3*X[3]))*dtr;
@@ -1511,18 +1523,18 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_proj2obs_f32(const float *X, const double *pvd, double *obs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_proj2obs_f32(const float *X, const double *pvd, const double *cvd, double *obs)
 {
-    (void)pvd; (void)X;
+    (void)pvd; (void)cvd; (void)X;
     obs[0] = (double)X[8];
     obs[1] = (double)X[9];
     obs[2] = (double)X[10];
     obs[3] = (double)X[11];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -1543,10 +1555,6 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const double P17 = lp[17];
     const double P18 = lp[18];
     const double P19 = lp[19];
-    const double P28 = PLOM_UNIFORM64(pv[28]);
-    const double P29 = PLOM_UNIFORM64(pv[29]);
-    const double P30 = PLOM_UNIFORM64(pv[30]);
-    const double P31 = PLOM_UNIFORM64(pv[31]);
     const double P32 = PLOM_UNIFORM64(pv[32]);
     const double P33 = PLOM_UNIFORM64(pv[33]);
     const double P34 = PLOM_UNIFORM64(pv[34]);
@@ -1571,6 +1579,11 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
         const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double n0_0 = 0.0;
         double n1_0 = 0.0;
@@ -1593,18 +1606,18 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         double stay6 = 0.0;
         double stay7 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n0_3; (void)n1_3; (void)n2_3;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
         const double r1_0 = P8;
-        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
-        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
-        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
-        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double sd0_0 = P16;
         const double v0_0 = sd0_0 * sd0_0;
         const double xi0_0 = (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt;
@@ -1757,9 +1770,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -1776,10 +1789,6 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const double P13 = lp[13];
     const double P14 = lp[14];
     const double P15 = lp[15];
-    const double P28 = PLOM_UNIFORM64(pv[28]);
-    const double P29 = PLOM_UNIFORM64(pv[29]);
-    const double P30 = PLOM_UNIFORM64(pv[30]);
-    const double P31 = PLOM_UNIFORM64(pv[31]);
     const double P32 = PLOM_UNIFORM64(pv[32]);
     const double P33 = PLOM_UNIFORM64(pv[33]);
     const double P34 = PLOM_UNIFORM64(pv[34]);
@@ -1804,6 +1813,11 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
         const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double n0_0 = 0.0;
         double n1_0 = 0.0;
@@ -1826,18 +1840,18 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         double stay6 = 0.0;
         double stay7 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n0_3; (void)n1_3; (void)n2_3;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
         const double r1_0 = P8;
-        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
-        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
-        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
-        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double xi0_0 = 1.0;
         {
             double p[2], cnt[2];
@@ -1982,10 +1996,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -2006,10 +2020,6 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const double P17 = lp[17];
     const double P18 = lp[18];
     const double P19 = lp[19];
-    const double P28 = PLOM_UNIFORM64(pv[28]);
-    const double P29 = PLOM_UNIFORM64(pv[29]);
-    const double P30 = PLOM_UNIFORM64(pv[30]);
-    const double P31 = PLOM_UNIFORM64(pv[31]);
     const double P32 = PLOM_UNIFORM64(pv[32]);
     const double P33 = PLOM_UNIFORM64(pv[33]);
     const double P34 = PLOM_UNIFORM64(pv[34]);
@@ -2034,14 +2044,19 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
         const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
         const double r1_0 = P8;
-        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
         const double d0_0 = sqdt * za0;
@@ -2054,9 +2069,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[4]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
-        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
         const double d0_1 = sqdt * za2;
@@ -2069,9 +2084,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[5]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
-        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         double za4, zb4;
         plom_normal_pair64(ns, &za4, &zb4);
         const double d0_2 = sqdt * za4;
@@ -2084,9 +2099,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[6]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
-        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         double za6, zb6;
         plom_normal_pair64(ns, &za6, &zb6);
         const double d0_3 = sqdt * za6;
@@ -2115,10 +2130,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -2139,10 +2154,6 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const double P17 = lp[17];
     const double P18 = lp[18];
     const double P19 = lp[19];
-    const double P28 = PLOM_UNIFORM64(pv[28]);
-    const double P29 = PLOM_UNIFORM64(pv[29]);
-    const double P30 = PLOM_UNIFORM64(pv[30]);
-    const double P31 = PLOM_UNIFORM64(pv[31]);
     const double P32 = PLOM_UNIFORM64(pv[32]);
     const double P33 = PLOM_UNIFORM64(pv[33]);
     const double P34 = PLOM_UNIFORM64(pv[34]);
@@ -2167,14 +2178,19 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
         const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
         const double r1_0 = P8;
-        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
         const double w0_0 = sqdt * za0;
@@ -2182,17 +2198,17 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
-        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const double w0_1 = sqdt * zb0;
         const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - (r0_1*X[1])*P17*w0_1;
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
-        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         double za1, zb1;
         plom_normal_pair64(ns, &za1, &zb1);
         const double w0_2 = sqdt * za1;
@@ -2200,9 +2216,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
-        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double w0_3 = sqdt * zb1;
         const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - (r0_3*X[3])*P19*w0_3;
         const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + + (r0_3*X[3])*P19*w0_3;
@@ -2224,10 +2240,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -2244,10 +2260,6 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const double P13 = lp[13];
     const double P14 = lp[14];
     const double P15 = lp[15];
-    const double P28 = PLOM_UNIFORM64(pv[28]);
-    const double P29 = PLOM_UNIFORM64(pv[29]);
-    const double P30 = PLOM_UNIFORM64(pv[30]);
-    const double P31 = PLOM_UNIFORM64(pv[31]);
     const double P32 = PLOM_UNIFORM64(pv[32]);
     const double P33 = PLOM_UNIFORM64(pv[33]);
     const double P34 = PLOM_UNIFORM64(pv[34]);
@@ -2272,14 +2284,19 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
         const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
         const double r1_0 = P8;
-        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
         const double d0_0 = sqdt * za0;
@@ -2291,9 +2308,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[4]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
-        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const double d0_1 = sqdt * zb1;
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
@@ -2303,9 +2320,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[5]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt((r0_1*X[1]))*d0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
-        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         double za3, zb3;
         plom_normal_pair64(ns, &za3, &zb3);
         const double d0_2 = sqdt * za3;
@@ -2317,9 +2334,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[6]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt((r0_2*X[2]))*d0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
-        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double d0_3 = sqdt * zb4;
         double za5, zb5;
         plom_normal_pair64(ns, &za5, &zb5);
@@ -2345,10 +2362,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -2365,10 +2382,6 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const double P13 = lp[13];
     const double P14 = lp[14];
     const double P15 = lp[15];
-    const double P28 = PLOM_UNIFORM64(pv[28]);
-    const double P29 = PLOM_UNIFORM64(pv[29]);
-    const double P30 = PLOM_UNIFORM64(pv[30]);
-    const double P31 = PLOM_UNIFORM64(pv[31]);
     const double P32 = PLOM_UNIFORM64(pv[32]);
     const double P33 = PLOM_UNIFORM64(pv[33]);
     const double P34 = PLOM_UNIFORM64(pv[34]);
@@ -2393,35 +2406,40 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const double tr = (double)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const double C0 = plom_cov_ld(cvk + 0);
+        const double C1 = plom_cov_ld(cvk + 1);
+        const double C2 = plom_cov_ld(cvk + 2);
+        const double C3 = plom_cov_ld(cvk + 3);
         const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / P28);
+        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
         const double r1_0 = P8;
-        const double r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr;
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr;
         s8 += ( + (r0_0*X[0]))*dtr;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / P29);
+        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
-        const double r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr;
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr;
         s9 += ( + (r0_1*X[1]))*dtr;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / P30);
+        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
-        const double r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr;
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr;
         s10 += ( + (r0_2*X[2]))*dtr;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / P31);
+        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
-        const double r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr;
         const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr;
         s11 += ( + (r0_3*X[3]))*dtr;
@@ -2442,19 +2460,19 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_proj2obs_pp_f64(const double *X, const double *pvd, const double *lpd, double *obs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_proj2obs_pp_f64(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs)
 {
-    (void)pvd; (void)lpd; (void)X;
+    (void)pvd; (void)lpd; (void)cvd; (void)X;
     obs[0] = (double)X[8];
     obs[1] = (double)X[9];
     obs[2] = (double)X[10];
     obs[3] = (double)X[11];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -2475,10 +2493,6 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const float P17 = lp[17];
     const float P18 = lp[18];
     const float P19 = lp[19];
-    const float P28 = PLOM_UNIFORM32(pv[28]);
-    const float P29 = PLOM_UNIFORM32(pv[29]);
-    const float P30 = PLOM_UNIFORM32(pv[30]);
-    const float P31 = PLOM_UNIFORM32(pv[31]);
     const float P32 = PLOM_UNIFORM32(pv[32]);
     const float P33 = PLOM_UNIFORM32(pv[33]);
     const float P34 = PLOM_UNIFORM32(pv[34]);
@@ -2503,14 +2517,19 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
         const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), P28);
+        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0);
         const float r1_0 = P8;
-        const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const float r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
         const float d0_0 = sqdt * za0;
@@ -2523,9 +2542,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_fsqrtf((r1_0*X[4]))*d1_0+ plom_fsqrtf((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + plom_fsqrtf((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         X[0] = plom_clipf(f0);
-        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), P29);
+        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1);
         const float r1_1 = P9;
-        const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const float r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         float za2, zb2;
         plom_normal_pair32(ns, &za2, &zb2);
         const float d0_1 = sqdt * za2;
@@ -2538,9 +2557,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_fsqrtf((r1_1*X[5]))*d1_1+ plom_fsqrtf((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + plom_fsqrtf((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         X[1] = plom_clipf(f1);
-        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), P30);
+        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2);
         const float r1_2 = P10;
-        const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const float r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         float za4, zb4;
         plom_normal_pair32(ns, &za4, &zb4);
         const float d0_2 = sqdt * za4;
@@ -2553,9 +2572,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_fsqrtf((r1_2*X[6]))*d1_2+ plom_fsqrtf((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + plom_fsqrtf((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         X[2] = plom_clipf(f2);
-        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), P31);
+        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3);
         const float r1_3 = P11;
-        const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const float r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         float za6, zb6;
         plom_normal_pair32(ns, &za6, &zb6);
         const float d0_3 = sqdt * za6;
@@ -2584,10 +2603,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -2608,10 +2627,6 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const float P17 = lp[17];
     const float P18 = lp[18];
     const float P19 = lp[19];
-    const float P28 = PLOM_UNIFORM32(pv[28]);
-    const float P29 = PLOM_UNIFORM32(pv[29]);
-    const float P30 = PLOM_UNIFORM32(pv[30]);
-    const float P31 = PLOM_UNIFORM32(pv[31]);
     const float P32 = PLOM_UNIFORM32(pv[32]);
     const float P33 = PLOM_UNIFORM32(pv[33]);
     const float P34 = PLOM_UNIFORM32(pv[34]);
@@ -2636,14 +2651,19 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
         const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), P28);
+        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0);
         const float r1_0 = P8;
-        const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const float r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
         const float w0_0 = sqdt * za0;
@@ -2651,17 +2671,17 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         X[0] = plom_clipf(f0);
-        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), P29);
+        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1);
         const float r1_1 = P9;
-        const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const float r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const float w0_1 = sqdt * zb0;
         const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - (r0_1*X[1])*P17*w0_1;
         const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         X[1] = plom_clipf(f1);
-        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), P30);
+        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2);
         const float r1_2 = P10;
-        const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const float r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         float za1, zb1;
         plom_normal_pair32(ns, &za1, &zb1);
         const float w0_2 = sqdt * za1;
@@ -2669,9 +2689,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         X[2] = plom_clipf(f2);
-        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), P31);
+        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3);
         const float r1_3 = P11;
-        const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const float r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const float w0_3 = sqdt * zb1;
         const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - (r0_3*X[3])*P19*w0_3;
         const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + + (r0_3*X[3])*P19*w0_3;
@@ -2693,10 +2713,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -2713,10 +2733,6 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const float P13 = lp[13];
     const float P14 = lp[14];
     const float P15 = lp[15];
-    const float P28 = PLOM_UNIFORM32(pv[28]);
-    const float P29 = PLOM_UNIFORM32(pv[29]);
-    const float P30 = PLOM_UNIFORM32(pv[30]);
-    const float P31 = PLOM_UNIFORM32(pv[31]);
     const float P32 = PLOM_UNIFORM32(pv[32]);
     const float P33 = PLOM_UNIFORM32(pv[33]);
     const float P34 = PLOM_UNIFORM32(pv[34]);
@@ -2741,14 +2757,19 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
         const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), P28);
+        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0);
         const float r1_0 = P8;
-        const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const float r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         float za0, zb0;
         plom_normal_pair32(ns, &za0, &zb0);
         const float d0_0 = sqdt * za0;
@@ -2760,9 +2781,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_fsqrtf((r1_0*X[4]))*d1_0+ plom_fsqrtf((r0_0*X[0]))*d0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + plom_fsqrtf((r0_0*X[0]))*d0_0;
         X[0] = plom_clipf(f0);
-        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), P29);
+        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1);
         const float r1_1 = P9;
-        const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const float r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const float d0_1 = sqdt * zb1;
         float za2, zb2;
         plom_normal_pair32(ns, &za2, &zb2);
@@ -2772,9 +2793,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_fsqrtf((r1_1*X[5]))*d1_1+ plom_fsqrtf((r0_1*X[1]))*d0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + plom_fsqrtf((r0_1*X[1]))*d0_1;
         X[1] = plom_clipf(f1);
-        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), P30);
+        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2);
         const float r1_2 = P10;
-        const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const float r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         float za3, zb3;
         plom_normal_pair32(ns, &za3, &zb3);
         const float d0_2 = sqdt * za3;
@@ -2786,9 +2807,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_fsqrtf((r1_2*X[6]))*d1_2+ plom_fsqrtf((r0_2*X[2]))*d0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + plom_fsqrtf((r0_2*X[2]))*d0_2;
         X[2] = plom_clipf(f2);
-        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), P31);
+        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3);
         const float r1_3 = P11;
-        const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const float r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const float d0_3 = sqdt * zb4;
         float za5, zb5;
         plom_normal_pair32(ns, &za5, &zb5);
@@ -2814,10 +2835,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -2834,10 +2855,6 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const float P13 = lp[13];
     const float P14 = lp[14];
     const float P15 = lp[15];
-    const float P28 = PLOM_UNIFORM32(pv[28]);
-    const float P29 = PLOM_UNIFORM32(pv[29]);
-    const float P30 = PLOM_UNIFORM32(pv[30]);
-    const float P31 = PLOM_UNIFORM32(pv[31]);
     const float P32 = PLOM_UNIFORM32(pv[32]);
     const float P33 = PLOM_UNIFORM32(pv[33]);
     const float P34 = PLOM_UNIFORM32(pv[34]);
@@ -2862,35 +2879,40 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     const int K = PLOM_UNIFORM_INT(plom_substeps(t0, t1, dt));
     for (int k = 0; k < K; k++) {
         const float tr = (float)t; (void)tr;
+        const double *cvk = cv + (size_t)k * (size_t)cs;
+        const float C0 = (float)plom_cov_ld(cvk + 0);
+        const float C1 = (float)plom_cov_ld(cvk + 1);
+        const float C2 = (float)plom_cov_ld(cvk + 2);
+        const float C3 = (float)plom_cov_ld(cvk + 3);
         const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
         float s11 = 0.0f;
-        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), P28);
+        const float r0_0 = plom_fdivf(((P0 * (1.0f + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0);
         const float r1_0 = P8;
-        const float r2_0 = (P12 * ((P28 - X[0]) - X[4]));
+        const float r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         const float f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr;
         const float f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr;
         s8 += ( + (r0_0*X[0]))*dtr;
         X[0] = plom_clipf(f0);
-        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), P29);
+        const float r0_1 = plom_fdivf(((P1 * (1.0f + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1);
         const float r1_1 = P9;
-        const float r2_1 = (P13 * ((P29 - X[1]) - X[5]));
+        const float r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const float f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr;
         const float f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr;
         s9 += ( + (r0_1*X[1]))*dtr;
         X[1] = plom_clipf(f1);
-        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), P30);
+        const float r0_2 = plom_fdivf(((P2 * (1.0f + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2);
         const float r1_2 = P10;
-        const float r2_2 = (P14 * ((P30 - X[2]) - X[6]));
+        const float r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         const float f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr;
         const float f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr;
         s10 += ( + (r0_2*X[2]))*dtr;
         X[2] = plom_clipf(f2);
-        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), P31);
+        const float r0_3 = plom_fdivf(((P3 * (1.0f + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3);
         const float r1_3 = P11;
-        const float r2_3 = (P15 * ((P31 - X[3]) - X[7]));
+        const float r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const float f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr;
         const float f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr;
         s11 += ( + (r0_3*X[3]))*dtr;
@@ -2911,20 +2933,20 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_proj2obs_pp_f32(const float *X, const double *pvd, const double *lpd, double *obs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_proj2obs_pp_f32(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs)
 {
-    (void)pvd; (void)lpd; (void)X;
+    (void)pvd; (void)lpd; (void)cvd; (void)X;
     obs[0] = (double)X[8];
     obs[1] = (double)X[9];
     obs[2] = (double)X[10];
     obs[3] = (double)X[11];
 }
 
-static __device__ inline __attribute__((always_inline)) double plom_sirs4_loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, double t, double like_min, int *n_floor)
+static __device__ inline __attribute__((always_inline)) double plom_sirs4_loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor)
 {
     double loglike = 0.0;
     int nf = 0;
-    (void)t; (void)pvd; (void)lpd;
+    (void)t; (void)pvd; (void)lpd; (void)cvd;
     if (!plom_isnan(y[0])) {
         const double x = obs[0];
         const double gsl_mu = (lpd[20] * x);
@@ -2966,47 +2988,47 @@ static __device__ inline __attribute__((always_inline)) double plom_sirs4_loglik
 }
 
 struct plom_model_sirs4 {
-    static constexpr int S = 12, N_TS = 4, INC_OFFSET = 8, N_INC = 4, PV = 48, N_CAC = 4, N_HAT = 16, NPP = 28;
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+    static constexpr int S = 12, N_TS = 4, INC_OFFSET = 8, N_INC = 4, PV = 48, N_CAC = 4, N_HAT = 16, NPP = 28, FIX_OFF = 28, N_FIX = 4;
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
     {
-        if constexpr (V == 0) plom_sirs4_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 4) plom_sirs4_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 5) plom_sirs4_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt);
+        if constexpr (V == 0) plom_sirs4_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 4) plom_sirs4_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 5) plom_sirs4_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
     {
-        if constexpr (V == 0) plom_sirs4_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 4) plom_sirs4_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 5) plom_sirs4_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt);
+        if constexpr (V == 0) plom_sirs4_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 4) plom_sirs4_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 5) plom_sirs4_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
     }
-    static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, double *obs) { plom_sirs4_proj2obs_f64(X, pvd, obs); }
-    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, double *obs) { plom_sirs4_proj2obs_pp_f64(X, pvd, lpd, obs); }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+    static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, const double *cvd, double *obs) { plom_sirs4_proj2obs_f64(X, pvd, cvd, obs); }
+    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sirs4_proj2obs_pp_f64(X, pvd, lpd, cvd, obs); }
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
     {
-        if constexpr (V == 0) plom_sirs4_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt);
+        if constexpr (V == 0) plom_sirs4_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
     {
-        if constexpr (V == 0) plom_sirs4_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
-        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt);
+        if constexpr (V == 0) plom_sirs4_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
     }
-    static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, double *obs) { plom_sirs4_proj2obs_f32(X, pvd, obs); }
-    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, double *obs) { plom_sirs4_proj2obs_pp_f32(X, pvd, lpd, obs); }
-    static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, double t, double like_min, int *n_floor) { return plom_sirs4_loglik(obs, y, pvd, t, like_min, n_floor); }
-    static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, double t, double like_min, int *n_floor) { return plom_sirs4_loglik_pp(obs, y, pvd, lpd, t, like_min, n_floor); }
-    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, double t, double *out) { plom_sirs4_hat_values(X, pvd, t, out); }
-    static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, double t, double *mean, double *var) { plom_sirs4_obs_moments(obs, pvd, t, mean, var); }
+    static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_sirs4_proj2obs_f32(X, pvd, cvd, obs); }
+    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sirs4_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
+    static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor) { return plom_sirs4_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }
+    static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor) { return plom_sirs4_loglik_pp(obs, y, pvd, lpd, cvd, t, like_min, n_floor); }
+    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out) { plom_sirs4_hat_values(X, pvd, cvd, t, out); }
+    static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var) { plom_sirs4_obs_moments(obs, pvd, cvd, t, mean, var); }
 };
 #endif
 

@@ -10,6 +10,7 @@ The library must be built (iterated-inference-pipeline_amd/build.py); there is n
 fallback: loading fails loudly if libplom_smc.so is missing.
 """
 import ctypes as C
+import datetime
 import math
 import os
 
@@ -28,7 +29,7 @@ class ModelInfo(C.Structure):
     _fields_ = [("size_proj", C.c_int32), ("n_ts", C.c_int32), ("n_cac", C.c_int32),
                 ("n_par_sv", C.c_int32), ("inc_offset", C.c_int32), ("n_inc", C.c_int32),
                 ("pv_size", C.c_int32), ("n_dem", C.c_int32), ("n_env", C.c_int32), ("n_hat", C.c_int32),
-                ("n_pp", C.c_int32)]
+                ("n_pp", C.c_int32), ("fix_off", C.c_int32), ("n_fix", C.c_int32)]
 
 
 class FilterDesc(C.Structure):
@@ -44,9 +45,14 @@ class MifMap(C.Structure):
                 ("multiplier", C.POINTER(C.c_double)), ("sd", C.POINTER(C.c_double))]
 
 
+class Covariate(C.Structure):
+    _fields_ = [("n", C.c_int32), ("pad", C.c_int32), ("x", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double))]
+
+
 class FilterInputs(C.Structure):
     _fields_ = [("times", C.POINTER(C.c_double)), ("data", C.POINTER(C.c_double)),
-                ("pv", C.POINTER(C.c_double)), ("x0", C.POINTER(C.c_double)), ("mif", C.POINTER(MifMap))]
+                ("pv", C.POINTER(C.c_double)), ("x0", C.POINTER(C.c_double)), ("mif", C.POINTER(MifMap)),
+                ("cov", C.POINTER(Covariate))]
 
 
 class FilterOutputs(C.Structure):
@@ -62,7 +68,7 @@ class FilterOutputs(C.Structure):
 EXPORTS = [
     "plom_last_error", "plom_ctx_create", "plom_ctx_destroy", "plom_model_count", "plom_model_find",
     "plom_model_name", "plom_model_info_get", "plom_smc_run", "plom_filter_create", "plom_filter_reset",
-    "plom_filter_update", "plom_filter_mif_start",
+    "plom_filter_update", "plom_filter_mif_start", "plom_covariate_eval",
     "plom_filter_run", "plom_filter_read", "plom_filter_destroy", "plom_filter_device_bytes",
     "plom_batch_run", "plom_sync", "plom_ctx_set_kernel_timing", "plom_last_timing", "plom_selftest_math",
     "plom_selftest_normals", "plom_selftest_copy", "plom_batch_create", "plom_batch_launch",
@@ -194,6 +200,8 @@ class Model:
         self.n_cac = self.desc.N_CAC
         self.n_hat = self.desc.N_PAR_SV * self.desc.N_CAC + (self.desc.N_CAC if self.desc.remainder else 0) + self.n_ts
         self.n_pp = self.desc.layout["npp"]
+        self.n_fix = self.desc.layout["n_fix"]
+        self.fix_off = self.desc.layout["fix_off"]
         self.layout = self.desc.layout
         self.context = load_json(os.path.join(PKG_DIR, "examples", name, "context.json"))
 
@@ -203,18 +211,58 @@ class Model:
             cls._cache[name] = cls(name)
         return cls._cache[name]
 
+    def covariates(self):
+        """The covariates of the context as one (x, y) knot pair per covariate slot of pv (pv
+        order: every covariate per population, then the covariates of the observation model
+        per series), x in days since the first data date, y in the data's units (unit
+        multiplier of get_multiplier, C/core/transform.c:267-301, data unit = days).  A
+        population (series) without a series of its own gets the constant 0 (the slot is
+        not read); an observation covariate falls back to its first population's series."""
+        m = self.desc
+        src = self.context["data"]["source"]
+        date_0 = datetime.date.fromisoformat(src[self.context["time_series"][0]["id"]]["t0"])
+        meta = {x["id"]: x for x in self.context.get("metadata", [])}
+
+        def series(name, key):
+            md = meta[name]
+            s = md["source"].get(key)
+            if s is None:
+                return None
+            xs, ys = [], []
+            for d, v in s["value"]:
+                if v is not None:
+                    xs.append(float((datetime.date.fromisoformat(d) - date_0).days))
+                    ys.append(float(v))
+            mult = 1.0
+            if "unit" in md:
+                mult = _U_DURATION[md["unit"]]["D"] if md.get("type") == "rate_as_duration" else _U_DURATION["D"][md["unit"]]
+            return (np.asarray(xs), np.asarray(ys) * mult)
+
+        out = []
+        for p in m.par_fixed:
+            for cac in m.cac_id:
+                s = series(p, cac)
+                out.append(s if s is not None else (np.zeros(1), np.zeros(1)))
+        for p in m.par_fixed_obs:
+            for t, cacs in zip(m.ts_id, m.ts_cacs()):
+                s = series(p, t)
+                if s is None:
+                    s = series(p, m.cac_id[cacs[0]])
+                out.append(s if s is not None else (np.zeros(1), np.zeros(1)))
+        return out
+
+    def time_varying(self):
+        """True when some covariate of the context changes with time."""
+        return any(len(set(y.tolist())) > 1 for _, y in self.covariates())
+
     def fixed_values(self):
-        """Constant covariate values per population (first point of each metadata series)."""
+        """Covariate values per population at t = 0 (pop_size_t0, C/core/build.c:1179-1185)."""
+        cov = self.covariates()
         out = {}
-        for meta in self.context.get("metadata", []):
-            vals = []
-            for cac in self.desc.cac_id:
-                series = meta["source"][cac]["value"]
-                ys = [v[1] for v in series if v[1] is not None]
-                if len(set(ys)) > 1:
-                    raise ValueError("time-varying covariate %s is not supported yet" % meta["id"])
-                vals.append(float(ys[0]))
-            out[meta["id"]] = vals
+        k = 0
+        for p in self.desc.par_fixed:
+            out[p] = [float(covariate_eval(x, y, [0.0])[0]) for x, y in cov[k:k + self.n_cac]]
+            k += self.n_cac
         return out
 
     def pack(self, params):
@@ -235,10 +283,10 @@ class Model:
         fixed = self.fixed_values()
         for p, off in lay["fixed"].items():
             put(off, m.N_CAC, params.get(p, fixed[p]))
+        cov = self.covariates()
         for p, off in lay["fixed_obs"].items():
-            # observation-level covariates follow the populations of each series
-            ts_c = m.ts_cacs()
-            put(off, m.N_TS, [fixed[p][c[0]] for c in ts_c])
+            put(off, m.N_TS, [float(covariate_eval(x, y, [0.0])[0]) for x, y in cov[off - lay["fix_off"]:
+                                                                                    off - lay["fix_off"] + m.N_TS]])
         if lay["coupling"] is not None:
             pv[lay["coupling"]:lay["coupling"] + m.N_CAC ** 2] = np.asarray(m.coupling, dtype=np.float64).ravel()
         return pv
@@ -263,6 +311,27 @@ class Model:
         d = np.array([[np.nan if v is None else float(v) for v in row] for row in data[:N]], dtype=np.float64)
         d = d.reshape(N, self.n_ts)
         return np.asarray(times[:N + 1], dtype=np.float64), d
+
+
+# duration conversion factors u_par -> u_data (C/core/transform.c:185-257)
+_U = ["D", "W", "B", "M", "Y"]
+_UT = [[1.0, 1.0 / 7.0, 1.0 / 14.0, 12.0 / 365.0, 1.0 / 365.0],
+       [7.0, 1.0, 0.5, 84.0 / 365.0, 7.0 / 365.0],
+       [14.0, 2.0, 1.0, 84.0 / 365.0, 7.0 / 365.0],
+       [365.0 / 12.0, 365.0 / 84.0, 365.0 / 168.0, 1.0, 1.0 / 12.0],
+       [365.0, 365.0 / 7.0, 365.0 / 14.0, 12.0, 1.0]]
+_U_DURATION = {a: {b: _UT[i][j] for j, b in enumerate(_U)} for i, a in enumerate(_U)}
+
+
+def covariate_eval(x, y, t):
+    """plom_covariate_eval: the covariate through knots (x, y) at the times t."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    out = np.zeros(len(t))
+    cv = Covariate(len(x), 0, dptr(x), dptr(y))
+    check(lib().plom_covariate_eval(C.byref(cv), len(t), dptr(t), dptr(out)))
+    return out
 
 
 def round_dt(DT):
@@ -297,13 +366,30 @@ def _desc(model, J, N, precision, noises_off, flags, seed, stream, dt, like_min)
     return d
 
 
+def _cov_array(model, cov):
+    """(pointer to plom_covariate[n_fix], the arrays it points into) or (None, None)."""
+    if cov is None:
+        return None, None
+    if len(cov) != model.n_fix:
+        raise ValueError("cov must hold n_fix = %d series" % model.n_fix)
+    keep = [(np.ascontiguousarray(x, dtype=np.float64), np.ascontiguousarray(y, dtype=np.float64)) for x, y in cov]
+    for x, y in keep:
+        if x.shape != y.shape or x.ndim != 1:
+            raise ValueError("a covariate needs x and y of one equal length")
+    arr = (Covariate * max(model.n_fix, 1))()
+    for i, (x, y) in enumerate(keep):
+        arr[i] = Covariate(len(x), 0, dptr(x), dptr(y))
+    return C.cast(arr, C.POINTER(Covariate)), (arr, keep)
+
+
 class Filter:
     """A device-resident filter (plom_filter_create / run / read)."""
 
     def __init__(self, ctx, model, J, times, data, pv, x0, precision="fp64", noises_off=0, flags=0,
-                 seed=1, stream=0, dt=0.25, like_min=1e-17, mif=None):
+                 seed=1, stream=0, dt=0.25, like_min=1e-17, mif=None, cov=None):
         """mif (with FLAG_MIF): dict of numpy arrays pv_src [n_pp], tr, scale, duration, min, max,
-        multiplier, sd [n_theta] (plom_mif_map)."""
+        multiplier, sd [n_theta] (plom_mif_map).  cov: time-varying covariates, one (x, y) knot
+        pair per covariate slot of pv (plom_covariate; model.n_fix of them)."""
         self.ctx, self.model = ctx, model
         self.J, self.N = int(J), int(len(times) - 1)
         self.flags = int(flags)
@@ -328,8 +414,9 @@ class Filter:
             m = self._mif
             mmap = MifMap(iptr(m["pv_src"]), iptr(m["tr"]), iptr(m["scale"]), iptr(m["duration"]), dptr(m["min"]),
                           dptr(m["max"]), dptr(m["multiplier"]), dptr(m["sd"]))
+        covs, self._cov = _cov_array(model, cov)
         inp = FilterInputs(dptr(self.times), dptr(self.data), dptr(self.pv), dptr(self.x0),
-                           C.pointer(mmap) if mmap is not None else None)
+                           C.pointer(mmap) if mmap is not None else None, covs)
         self.h = C.c_void_p()
         check(lib().plom_filter_create(ctx.h, C.byref(self.desc), C.byref(inp), C.byref(self.h)))
 
@@ -439,7 +526,7 @@ class Server:
         self.N = {}
 
     def submit(self, model, J, times, data, pv, x0, precision="fp64", noises_off=0, seed=1, stream=0,
-               dt=0.25, like_min=1e-17, lineage=False, flags=0):
+               dt=0.25, like_min=1e-17, lineage=False, flags=0, cov=None):
         times = np.ascontiguousarray(times, dtype=np.float64)
         N = len(times) - 1
         data = np.ascontiguousarray(data, dtype=np.float64).reshape(N, model.n_ts)
@@ -447,7 +534,8 @@ class Server:
         x0 = np.ascontiguousarray(x0, dtype=np.float64)
         d = _desc(model, J, N, precision, noises_off, (FLAG_LINEAGE if lineage else 0) | int(flags), seed, stream, dt,
                   like_min)
-        inp = FilterInputs(dptr(times), dptr(data), dptr(pv), dptr(x0), None)
+        covs, keep = _cov_array(model, cov)
+        inp = FilterInputs(dptr(times), dptr(data), dptr(pv), dptr(x0), None, covs)
         rid = C.c_uint64()
         check(lib().plom_server_submit(self.h, C.byref(d), C.byref(inp), C.byref(rid)))
         self.N[rid.value] = (N, model.S if lineage else 0)

@@ -27,9 +27,11 @@ def lib():
         ip = C.POINTER(C.c_int32)
         L.oracle_smc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                  C.c_double, C.c_double, dp, dp, dp, dp, C.c_int, C.c_int,
-                                 dp, ip, dp, dp, ip, dp, dp, dp, ip, dp, dp, dp, dp, dp]
+                                 dp, ip, dp, dp, ip, dp, dp, dp, ip, dp, dp, dp, dp, dp, ip, dp, dp]
         L.oracle_trajectory.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-                                        C.c_double, dp, dp, dp, dp]
+                                        C.c_double, dp, dp, dp, dp, ip, dp, dp]
+        L.oracle_cov_table.argtypes = [C.c_int, C.c_int, dp, C.c_double, ip, dp, dp, dp, ip]
+        L.oracle_spline.argtypes = [C.c_int, dp, dp, C.c_int, dp, dp]
         L.oracle_obs_loglik.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, dp, ip]
         L.oracle_math.argtypes = [C.c_int, dp, dp]
         L.oracle_normals.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
@@ -44,7 +46,7 @@ def lib():
         i32p = C.POINTER(C.c_int32)
         L.oracle_mif_pass.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double,
                                       C.c_double, dp, dp, dp, C.c_int, i32p, i32p, i32p, i32p, dp, dp, dp, dp, dp,
-                                      dp, dp, C.c_int, dp, i32p, dp, dp, dp, dp, dp, dp, i32p]
+                                      dp, dp, C.c_int, dp, i32p, dp, dp, dp, dp, dp, dp, i32p, i32p, dp, dp]
         _lib = L
     return _lib
 
@@ -60,12 +62,59 @@ def _ip(a):
 IMPL_PSR = 0x100   # the psr implementation (PLOM_IMPL_PSR of csrc/plom_model_common.h)
 
 
+def _cov(cov):
+    """cov: one (x, y) knot pair per covariate slot of pv, or None -> (n, x, y) arrays."""
+    if cov is None:
+        return None, None, None
+    n = np.ascontiguousarray([len(x) for x, _ in cov], dtype=np.int32)
+    xs = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x, _ in cov]), dtype=np.float64)
+    ys = np.ascontiguousarray(np.concatenate([np.asarray(y, dtype=np.float64) for _, y in cov]), dtype=np.float64)
+    return n, xs, ys
+
+
+_keep = []
+
+
+def _cov_args(cov):
+    n, xs, ys = _cov(cov)
+    _keep[:] = [n, xs, ys]
+    return _ip(n), _dp(xs), _dp(ys)
+
+
+def cov_table(model_id, times, dt, cov):
+    """The covariate table of a filter (csrc/plom_cov.h): (table [rows][n_fix], kmax)."""
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    N = len(times) - 1
+    nfix = lib().oracle_model_n_fix(model_id)
+    kmax_bound = int(np.max(np.diff(times)) / dt) + 2 if N else 0
+    out = np.zeros(((N * kmax_bound + N + 1), max(nfix, 1)))
+    km = C.c_int32()
+    n, xs, ys = _cov(cov)
+    rc = lib().oracle_cov_table(model_id, N, _dp(times), dt, _ip(n), _dp(xs), _dp(ys), _dp(out), C.byref(km))
+    if rc != 0:
+        raise ValueError("covariate table: %d" % rc)
+    rows = N * km.value + N + 1
+    return out.reshape(-1)[:rows * nfix].reshape(rows, nfix), km.value
+
+
+def spline(x, y, t):
+    """The interpolating spline of csrc/plom_cov.h through (x, y), at the times t."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    out = np.zeros(len(t))
+    if lib().oracle_spline(len(x), _dp(x), _dp(y), len(t), _dp(t), _dp(out)) != 0:
+        raise RuntimeError("oracle_spline failed")
+    return out
+
+
 def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=0, dt=0.25,
         like_min=1e-17, mode=CANONICAL, nthreads=1, ancestry=False, final_state=False, logw=False,
-        lineage=False, all_states=False, hat=False, psr=False):
+        lineage=False, all_states=False, hat=False, psr=False, cov=None):
     """The filter restated on the CPU.  lineage: the sampled path (slots [N], states
     [N+1][S]); all_states: every particle's state after every interval, before resampling
-    ([N+1][J][S], for tests on small runs); psr: the Euler-multinomial implementation."""
+    ([N+1][J][S], for tests on small runs); psr: the Euler-multinomial implementation;
+    cov: time-varying covariates, one (x, y) knot pair per covariate slot of pv."""
     if psr:
         noises_off = noises_off | IMPL_PSR
     times = np.ascontiguousarray(times, dtype=np.float64)
@@ -92,7 +141,8 @@ def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=
     el = C.c_double()
     rc = lib().oracle_smc(model_id, J, N, noises_off, flags, seed, stream, dt, like_min, _dp(times), _dp(data),
                           _dp(pv), _dp(x0), mode, nthreads, C.byref(ll), C.byref(naf), _dp(ll_n), _dp(ess),
-                          _ip(anc), _dp(fs), _dp(lw), C.byref(el), _ip(lsl), _dp(lst), _dp(alls), _dp(ht), _dp(pr), _dp(lob))
+                          _ip(anc), _dp(fs), _dp(lw), C.byref(el), _ip(lsl), _dp(lst), _dp(alls), _dp(ht), _dp(pr), _dp(lob),
+                          *_cov_args(cov))
     if rc != 0:
         raise RuntimeError("oracle_smc failed: %d" % rc)
     return {"loglik": ll.value, "n_all_fail": naf.value, "loglik_n": ll_n, "ess_n": ess,
@@ -101,7 +151,7 @@ def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=
 
 
 def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=0, seed=1, stream=0, dt=0.25,
-             like_min=1e-17, nthreads=1, theta_out=False, logw=False, ancestry=False, psr=False):
+             like_min=1e-17, nthreads=1, theta_out=False, logw=False, ancestry=False, psr=False, cov=None):
     """One pass of iterated filtering restated on the CPU (oracle_mif_pass).  mif: dict of
     pv_src [n_pp], tr, scale, duration, min, max, multiplier, sd [n_theta]; theta0
     [n_theta, J]; x0 [S, J]; mut_fac [N]."""
@@ -126,7 +176,7 @@ def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=
                                nth, _ip(m["pv_src"]), _ip(m["tr"]), _ip(m["scale"]), _ip(m["duration"]), _dp(m["min"]),
                                _dp(m["max"]), _dp(m["multiplier"]), _dp(m["sd"]), _dp(mut_fac), _dp(theta0), _dp(x0),
                                nthreads, C.byref(ll), C.byref(naf), _dp(ll_n), _dp(ess), _dp(mm), _dp(mv), _dp(tho),
-                               _dp(lw), _ip(anc))
+                               _dp(lw), _ip(anc), *_cov_args(cov))
     if rc != 0:
         raise RuntimeError("oracle_mif_pass failed: %d" % rc)
     return {"loglik": ll.value, "n_all_fail": naf.value, "loglik_n": ll_n, "ess_n": ess, "mif_mean": mm,
@@ -150,7 +200,7 @@ def mif_natural(tr, sc, dur, x, a, b, mult):
     return lib().oracle_mif_natural(int(tr), int(sc), int(dur), float(x), float(a), float(b), float(mult))
 
 
-def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=0.25, psr=False):
+def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=0.25, psr=False, cov=None):
     if psr:
         noises_off = noises_off | IMPL_PSR
     times = np.ascontiguousarray(times, dtype=np.float64)
@@ -159,7 +209,7 @@ def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=
     out = np.zeros((N + 1, S))
     rc = lib().oracle_trajectory(model_id, N, noises_off, seed, stream, j, dt, _dp(times),
                                  _dp(np.ascontiguousarray(pv, dtype=np.float64)),
-                                 _dp(np.ascontiguousarray(x0, dtype=np.float64)), _dp(out))
+                                 _dp(np.ascontiguousarray(x0, dtype=np.float64)), _dp(out), *_cov_args(cov))
     if rc != 0:
         raise RuntimeError("oracle_trajectory failed")
     return out

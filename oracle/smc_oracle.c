@@ -45,6 +45,7 @@
 #include "plom_rng.h"
 #include "plom_mif.h"
 #include "plom_psr.h"
+#include "plom_cov.h"
 #include "plom_models.h"
 
 #define ORACLE_CANONICAL 0
@@ -54,15 +55,16 @@
 
 typedef unsigned __int128 u128;
 
-typedef void (*interval_fn)(double *, const double *, plom_nstream64 *, double, double, double, double, unsigned int);
-typedef void (*proj2obs_fn)(const double *, const double *, double *);
-typedef double (*loglik_fn)(const double *, const double *, const double *, double, double, int *);
-typedef void (*moments_fn)(const double *, const double *, double, double *, double *);
-typedef void (*hat_fn)(const double *, const double *, double, double *);
+typedef void (*interval_fn)(double *, const double *, plom_nstream64 *, double, double, double, double, unsigned int,
+                            const double *, int);
+typedef void (*proj2obs_fn)(const double *, const double *, const double *, double *);
+typedef double (*loglik_fn)(const double *, const double *, const double *, const double *, double, double, int *);
+typedef void (*moments_fn)(const double *, const double *, const double *, double, double *, double *);
+typedef void (*hat_fn)(const double *, const double *, const double *, double, double *);
 
 typedef struct {
     const char *name;
-    int S, n_ts, n_hat, n_pp, pv_size;
+    int S, n_ts, n_hat, n_pp, pv_size, fix_off, n_fix;
     interval_fn interval;
     proj2obs_fn proj2obs;
     loglik_fn loglik;
@@ -72,6 +74,7 @@ typedef struct {
 
 #define ORACLE_MODEL(lname, UNAME)                                                                 \
     { #lname, UNAME##_SIZE_PROJ, UNAME##_N_TS, UNAME##_N_HAT, UNAME##_N_PP, UNAME##_PV_SIZE,           \
+      UNAME##_FIX_OFF, UNAME##_N_FIX,                                                                  \
       plom_##lname##_interval_f64,                                                                   \
       plom_##lname##_proj2obs_f64, plom_##lname##_loglik, plom_##lname##_obs_moments, plom_##lname##_hat_values }
 
@@ -79,6 +82,7 @@ static const oracle_model MODELS[] = {
     ORACLE_MODEL(sir, PLOM_SIR),
     ORACLE_MODEL(sirs4, PLOM_SIRS4),
     ORACLE_MODEL(seir10, PLOM_SEIR10),
+    ORACLE_MODEL(noise, PLOM_NOISE),
 };
 #define N_MODELS ((int)(sizeof MODELS / sizeof MODELS[0]))
 
@@ -116,6 +120,80 @@ double oracle_mif_natural(int tr, int sc, int dur, double x, double a, double b,
     return plom_mif_natural(tr, sc, dur, x, a, b, mult);
 }
 
+/* ------------------------------------------------------------------ covariates */
+/* The covariate table of a filter (plom_cov.h): the rows of interval n's substeps and the
+ * row of observation time times[r]; without time-varying covariates, the covariate slots of
+ * pv repeated (cs = 0). */
+typedef struct {
+    const double *tab;
+    int cs, kmax, N;
+    double *owned;
+} cov_view;
+
+static const double *cv_steps(const cov_view *c, int n) { return c->tab + (size_t)n * (size_t)c->kmax * (size_t)c->cs; }
+static const double *cv_at(const cov_view *c, int r)
+{
+    return c->tab + ((size_t)c->N * (size_t)c->kmax + (size_t)r) * (size_t)c->cs;
+}
+
+/* cov_n [n_fix] knot counts, cov_x / cov_y the knots of every series one after another;
+ * cov_n NULL: the constants of pv.  0, or -3 when a time falls outside a series' knots. */
+static int cov_build(const oracle_model *M, int N, const double *times, double dt, const int32_t *cov_n,
+                     const double *cov_x, const double *cov_y, const double *pv, cov_view *out)
+{
+    memset(out, 0, sizeof *out);
+    out->N = N;
+    if (!cov_n || M->n_fix == 0) {
+        out->tab = pv + M->fix_off;
+        return 0;
+    }
+    plom_covariate *cv = malloc(sizeof(plom_covariate) * (size_t)M->n_fix);
+    if (!cv) return -2;
+    size_t o = 0;
+    for (int i = 0; i < M->n_fix; i++) {
+        cv[i].n = cov_n[i];
+        cv[i].pad = 0;
+        cv[i].x = cov_x + o;
+        cv[i].y = cov_y + o;
+        o += (size_t)cov_n[i];
+    }
+    out->kmax = plom_cov_kmax(N, times, dt);
+    out->cs = M->n_fix;
+    out->owned = malloc(sizeof(double) * plom_cov_rows(N, out->kmax) * (size_t)M->n_fix);
+    char msg[256];
+    const int rc = out->owned ? plom_cov_table(cv, M->n_fix, N, times, dt, out->kmax, out->owned, msg, sizeof msg) : -2;
+    free(cv);
+    out->tab = out->owned;
+    return rc == 0 ? 0 : -3;
+}
+
+/* the table itself (for tests): out[plom_cov_rows(N, kmax) * n_fix], *kmax_out */
+int oracle_cov_table(int model, int N, const double *times, double dt, const int32_t *cov_n, const double *cov_x,
+                     const double *cov_y, double *out, int32_t *kmax_out)
+{
+    if (model < 0 || model >= N_MODELS) return -1;
+    cov_view c;
+    const int rc = cov_build(&MODELS[model], N, times, dt, cov_n, cov_x, cov_y, NULL, &c);
+    if (rc) { free(c.owned); return rc; }
+    *kmax_out = c.kmax;
+    memcpy(out, c.tab, sizeof(double) * plom_cov_rows(N, c.kmax) * (size_t)MODELS[model].n_fix);
+    free(c.owned);
+    return 0;
+}
+
+/* one spline (plom_cov.h) at the times t[0..nt) (for tests) */
+int oracle_spline(int n, const double *x, const double *y, int nt, const double *t, double *out)
+{
+    double *c = malloc(sizeof(double) * (size_t)(n + 1));
+    if (!c || plom_spline_prepare(n, x, y, c)) { free(c); return -2; }
+    for (int i = 0; i < nt; i++) out[i] = plom_spline_eval(n, x, y, c, t[i]);
+    free(c);
+    return 0;
+}
+
+int oracle_model_n_fix(int m) { return MODELS[m].n_fix; }
+int oracle_model_fix_off(int m) { return MODELS[m].fix_off; }
+
 /* ------------------------------------------------------------------ canonical trees */
 /* recursive contiguous pairwise sum of a[0..n), n a power of two */
 static double tree_sum(const double *a, int n)
@@ -138,6 +216,7 @@ typedef struct {
     double *logw;       /* [J] */
     const double *pv, *times, *data;
     size_t pv_stride;   /* 0: one pv for every particle; else particle j uses pv + j * pv_stride */
+    const cov_view *cv;
     int J, n, nonan, filter, n_ts;
     uint32_t seed, stream, noises_off;
     double dt, sqdt, like_min;
@@ -154,11 +233,13 @@ static void propagate_range(prop_job *p)
         const double *pv = p->pv + (size_t)j * p->pv_stride;
         plom_nstream64 ns;
         plom_nstream64_init(&ns, p->seed, p->stream, (uint32_t)p->n, (uint32_t)j, plom_bm_tab);
-        M->interval(x, pv, &ns, p->times[p->n], p->times[p->n + 1], p->dt, p->sqdt, p->noises_off);
+        M->interval(x, pv, &ns, p->times[p->n], p->times[p->n + 1], p->dt, p->sqdt, p->noises_off, cv_steps(p->cv, p->n),
+                    p->cv->cs);
         if (p->filter && p->nonan) {
             int nf = 0;
-            M->proj2obs(x, pv, obs);
-            double l = M->loglik(obs, p->data + (size_t)p->n * p->n_ts, pv, p->times[p->n + 1], p->like_min, &nf);
+            const double *cvd = cv_at(p->cv, p->n + 1);
+            M->proj2obs(x, pv, cvd, obs);
+            double l = M->loglik(obs, p->data + (size_t)p->n * p->n_ts, pv, cvd, p->times[p->n + 1], p->like_min, &nf);
             if (p->mode == ORACLE_CANONICAL && nf == p->n_ts) l = -INFINITY;
             p->logw[j] = l;
         }
@@ -324,12 +405,12 @@ static int hat_cmp(const void *a, const void *b)
     return x < y ? -1 : (x > y);
 }
 
-static void hat_row(const oracle_model *M, const double *X, int J, const double *pv, double t,
+static void hat_row(const oracle_model *M, const double *X, int J, const double *pv, const double *cvd, double t,
                     const uint64_t *q, uint64_t Q, double *row, double *vals, hat_item *items, double *buf)
 {
     const int NH = M->n_hat, S = M->S;
     const int T = (J + TILE - 1) / TILE, P = next_pow2(T);
-    for (int j = 0; j < J; j++) M->hat_values(X + (size_t)j * S, pv, t, vals + (size_t)j * NH);
+    for (int j = 0; j < J; j++) M->hat_values(X + (size_t)j * S, pv, cvd, t, vals + (size_t)j * NH);
     for (int k = 0; k < NH; k++) {
         double *tiles = buf + TILE;
         for (int tt = 0; tt < P; tt++) {
@@ -361,7 +442,8 @@ static void hat_row(const oracle_model *M, const double *X, int J, const double 
     }
 }
 
-static void pred_res_row(const oracle_model *M, const double *X, int J, const double *pv, double t, const double *y,
+static void pred_res_row(const oracle_model *M, const double *X, int J, const double *pv, const double *cvd, double t,
+                         const double *y,
                          const uint32_t *select, uint32_t *offspring, double *row)
 {
     const int S = M->S, NT = M->n_ts;
@@ -374,8 +456,8 @@ static void pred_res_row(const oracle_model *M, const double *X, int J, const do
     memset(c, 0, sizeof c);
     for (int j = 0; j < J; j++) {
         if (!offspring[j]) continue;
-        M->proj2obs(X + (size_t)j * S, pv, obs);
-        M->obs_moments(obs, pv, t, mu, var);
+        M->proj2obs(X + (size_t)j * S, pv, cvd, obs);
+        M->obs_moments(obs, pv, cvd, t, mu, var);
         for (int ts = 0; ts < NT; ts++) {
             a[j % TILE][ts] += (double)offspring[j] * mu[ts];
             b[j % TILE][ts] += (double)offspring[j] * var[ts];
@@ -389,8 +471,8 @@ static void pred_res_row(const oracle_model *M, const double *X, int J, const do
     }
     for (int j = 0; j < J; j++) {
         if (!offspring[j]) continue;
-        M->proj2obs(X + (size_t)j * S, pv, obs);
-        M->obs_moments(obs, pv, t, mu, var);
+        M->proj2obs(X + (size_t)j * S, pv, cvd, obs);
+        M->obs_moments(obs, pv, cvd, t, mu, var);
         for (int ts = 0; ts < NT; ts++) {
             const double d = mu[ts] - pred[ts];
             c[j % TILE][ts] += (double)offspring[j] * (d * d);
@@ -411,10 +493,13 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
                double *loglik, int32_t *n_all_fail, double *loglik_n, double *ess_n, int32_t *ancestry,
                double *final_state, double *logw_out, double *elapsed_s,
                int32_t *lin_slots, double *lin_state, double *all_states, double *hat, double *pred_res,
-               double *lin_obs)
+               double *lin_obs, const int32_t *cov_n, const double *cov_x, const double *cov_y)
 {
     if (model < 0 || model >= N_MODELS || J < 1 || N < 0) return -1;
     const oracle_model *M = &MODELS[model];
+    cov_view cv;
+    const int crc = cov_build(M, N, times, dt, cov_n, cov_x, cov_y, pv, &cv);
+    if (crc) { free(cv.owned); return crc; }
     const int S = M->S, n_ts = M->n_ts;
     const int T = (J + TILE - 1) / TILE;
     double *X = malloc((size_t)J * S * sizeof(double));
@@ -443,7 +528,7 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
         offspring = malloc((size_t)J * sizeof(uint32_t));
         if (!hvals || !hbuf || !hitems || !hq || !offspring || M->n_ts > 16) return -2;
         double v0[256];
-        M->hat_values(x0, pv, times[0], v0);
+        M->hat_values(x0, pv, cv_at(&cv, 0), times[0], v0);
         for (int k = 0; k < NH; k++) hat[3 * k] = hat[3 * k + 1] = hat[3 * k + 2] = v0[k];
     }
     const int filter = !(flags & FLAG_NO_FILTER);
@@ -457,7 +542,7 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
         for (int k = 0; k < n_ts; k++) nonan += !isnan(data[(size_t)n * n_ts + k]);
         prop_job job;
         memset(&job, 0, sizeof job);
-        job.M = M; job.X = X; job.logw = logw; job.pv = pv; job.times = times; job.data = data;
+        job.M = M; job.X = X; job.logw = logw; job.pv = pv; job.cv = &cv; job.times = times; job.data = data;
         job.J = J; job.n = n; job.nonan = nonan; job.filter = filter; job.n_ts = n_ts;
         job.seed = seed; job.stream = stream; job.noises_off = noises_off;
         job.dt = dt; job.sqdt = plom_sqrt(dt); job.like_min = like_min; job.mode = mode;
@@ -487,10 +572,11 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
             } else {
                 for (int j = 0; j < J; j++) hq[j] = 1;
             }
-            hat_row(M, X, J, pv, times[n + 1], hq, Q, hat + (size_t)(n + 1) * NH * 3, hvals, hitems, hbuf);
+            hat_row(M, X, J, pv, cv_at(&cv, n + 1), times[n + 1], hq, Q, hat + (size_t)(n + 1) * NH * 3, hvals, hitems,
+                    hbuf);
             double *pr = pred_res + (size_t)n * n_ts * 2;
             if (filter && nonan)
-                pred_res_row(M, X, J, pv, times[n + 1], data + (size_t)n * n_ts, select, offspring, pr);
+                pred_res_row(M, X, J, pv, cv_at(&cv, n + 1), times[n + 1], data + (size_t)n * n_ts, select, offspring, pr);
             else
                 for (int k = 0; k < 2 * n_ts; k++) pr[k] = NAN;
         }
@@ -533,20 +619,21 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
         for (int n = 0; n < N; n++) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, seed, stream, (uint32_t)n, (uint32_t)lin_slots[n], plom_bm_tab);
-            M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off);
+            M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off, cv_steps(&cv, n), cv.cs);
             memcpy(lin_state + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
         }
         if (lin_obs)
             for (int n = 0; n <= N; n++) {
                 double obs[64], mu[64], var[64];
-                M->proj2obs(lin_state + (size_t)n * S, pv, obs);
-                M->obs_moments(obs, pv, times[n], mu, var);
+                M->proj2obs(lin_state + (size_t)n * S, pv, cv_at(&cv, n), obs);
+                M->obs_moments(obs, pv, cv_at(&cv, n), times[n], mu, var);
                 plom_lineage_obs(seed, stream, (uint32_t)n, n_ts, mu, var, plom_bm_tab, lin_obs + (size_t)n * n_ts * 2);
             }
     }
     if (anc && anc != ancestry) free(anc);
     free(hvals); free(hbuf); free(hitems); free(hq); free(offspring);
     free(X); free(Xn); free(logw); free(work); free(cum); free(select);
+    free(cv.owned);
     return 0;
 }
 
@@ -573,10 +660,14 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
                     const double *mx, const double *mult, const double *sd, const double *mut_fac,
                     const double *theta0, const double *x0, int nthreads, double *loglik, int32_t *n_all_fail,
                     double *loglik_n, double *ess_n, double *mif_mean, double *mif_var, double *theta_out,
-                    double *logw_out, int32_t *ancestry)
+                    double *logw_out, int32_t *ancestry, const int32_t *cov_n, const double *cov_x,
+                    const double *cov_y)
 {
     if (model < 0 || model >= N_MODELS || J < 1 || N < 0 || n_theta < 1) return -1;
     const oracle_model *M = &MODELS[model];
+    cov_view cv;
+    const int crc = cov_build(M, N, times, dt, cov_n, cov_x, cov_y, pv, &cv);
+    if (crc) { free(cv.owned); return crc; }
     const int S = M->S, n_ts = M->n_ts, P = M->n_pp;
     const int T = (J + TILE - 1) / TILE, PT = next_pow2(T);
     double *X = malloc((size_t)J * S * sizeof(double)), *Xn = malloc((size_t)J * S * sizeof(double));
@@ -630,7 +721,7 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
                     theta_out[((size_t)(n + 1) * n_theta + o) * J + j] = TH[(size_t)j * n_theta + o];
         prop_job job;
         memset(&job, 0, sizeof job);
-        job.M = M; job.X = X; job.logw = logw; job.pv = PVJ; job.pv_stride = (size_t)pvn; job.times = times;
+        job.M = M; job.X = X; job.logw = logw; job.pv = PVJ; job.pv_stride = (size_t)pvn; job.cv = &cv; job.times = times;
         job.data = data; job.J = J; job.n = n; job.nonan = nonan; job.filter = 1; job.n_ts = n_ts;
         job.seed = seed; job.stream = stream; job.noises_off = noises_off;
         job.dt = dt; job.sqdt = plom_sqrt(dt); job.like_min = like_min; job.mode = ORACLE_CANONICAL;
@@ -713,16 +804,21 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
     *n_all_fail = naf;
     free(X); free(Xn); free(TH); free(THn); free(logw); free(work); free(cum); free(select); free(buf); free(tc);
     free(PVJ);
+    free(cv.owned);
     return 0;
 }
 
 /* one particle through N intervals, no filtering: the state at every observation time,
  * out[(n+1)*S + k]; out[0..S) = x0 (with incidences reset) */
 int oracle_trajectory(int model, int N, uint32_t noises_off, uint32_t seed, uint32_t stream, uint32_t j,
-                      double dt, const double *times, const double *pv, const double *x0, double *out)
+                      double dt, const double *times, const double *pv, const double *x0, double *out,
+                      const int32_t *cov_n, const double *cov_x, const double *cov_y)
 {
     if (model < 0 || model >= N_MODELS) return -1;
     const oracle_model *M = &MODELS[model];
+    cov_view cv;
+    const int crc = cov_build(M, N, times, dt, cov_n, cov_x, cov_y, pv, &cv);
+    if (crc) { free(cv.owned); return crc; }
     const int S = M->S;
     double x[256];
     memcpy(x, x0, (size_t)S * sizeof(double));
@@ -730,9 +826,10 @@ int oracle_trajectory(int model, int N, uint32_t noises_off, uint32_t seed, uint
     for (int n = 0; n < N; n++) {
         plom_nstream64 ns;
         plom_nstream64_init(&ns, seed, stream, (uint32_t)n, j, plom_bm_tab);
-        M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off);
+        M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off, cv_steps(&cv, n), cv.cs);
         memcpy(out + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
     }
+    free(cv.owned);
     return 0;
 }
 
@@ -743,8 +840,8 @@ int oracle_obs_loglik(int model, const double *x, const double *y, const double 
     if (model < 0 || model >= N_MODELS) return -1;
     const oracle_model *M = &MODELS[model];
     int nf = 0;
-    M->proj2obs(x, pv, obs);
-    *loglik = M->loglik(obs, y, pv, t, like_min, &nf);
+    M->proj2obs(x, pv, pv + M->fix_off, obs);
+    *loglik = M->loglik(obs, y, pv, pv + M->fix_off, t, like_min, &nf);
     *n_floor = nf;
     return 0;
 }

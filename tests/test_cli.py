@@ -309,3 +309,43 @@ def test_cli_psr_matches_api(tmp_path, ctx):
     g = smc.run(ctx, m, 1000, times, data, np.array(dry["pv"]), np.array(dry["x0"]), flags=smc.FLAG_PSR, seed=2, stream=0)
     o = oracle.smc(m.id, 1000, times, data, np.array(dry["pv"]), np.array(dry["x0"]), seed=2, stream=0, psr=True)
     assert ll == g.loglik == o["loglik"]
+
+
+def test_noise_dry_run_covariates(tmp_path):
+    """The noise example's covariates enter pv at their t = 0 values, rates per week turned
+    into rates per day (get_multiplier, transform.c:267-301); the initial state uses the
+    population size at t = 0 (build.c:1179-1185)."""
+    d = model_dir(tmp_path, "noise")
+    r = run_cli(d, ["sde", "-J", "10", "--dry_run"], os.path.join(PKG, "examples", "noise", "theta.json"))
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    m = smc.Model.get("noise")
+    np.testing.assert_array_equal(out["pv"][m.fix_off:], m.pack(configs.TRUTH["noise"])[m.fix_off:])
+    mu_b = m.layout["fixed"]["mu_b"]
+    assert out["pv"][mu_b] == 0.00027 * (1.0 / 7.0)
+    assert out["x0"][0] == pytest.approx(0.7 * 1e6, rel=1e-15) and out["x0"][1] == pytest.approx(0.65 * 6e5, rel=1e-15)
+
+
+@pytest.mark.gpu
+def test_cli_noise_covariates_match_api(tmp_path, ctx):
+    """`smc sde` on the noise example (time-varying N, mu_b, mu_d, prop) gives the Python
+    API's log-likelihood with the model's covariate series; with -g (freeze_forcing) every
+    covariate is constant at its value at that time (build.c:1188-1205)."""
+    from oracle import oracle
+    d = model_dir(tmp_path, "noise")
+    theta = os.path.join(PKG, "examples", "noise", "theta.json")
+    dry = json.loads(run_cli(d, ["sde", "-J", "2000", "--dry_run"], theta).stdout)
+    m = smc.Model.get("noise")
+    times, data = m.dataset()
+    pv, x0 = np.array(dry["pv"]), np.array(dry["x0"])
+    cov = m.covariates()
+    for extra, cv in ((["-g", "100"], [(np.array([x[0], x[-1]]), np.full(2, smc.covariate_eval(x, y, [min(100.0, x[-1])])[0]))
+                                       if len(x) > 1 else (x, y) for x, y in cov]), ([], cov)):
+        print("cli noise", extra, flush=True)
+        r = run_cli(d, ["sde", "-J", "2000", "-b", "-e", "-d"] + extra, theta)
+        assert r.returncode == 0, r.stderr
+        ll_cli = float([l for l in r.stdout.splitlines() if l.startswith("logV_17g:")][0].split(":")[1])
+        g = smc.run(ctx, m, 2000, times, data, pv, x0, seed=2, stream=0, cov=cv)
+        assert ll_cli == g.loglik
+    o = oracle.smc(m.id, 2000, times, data, pv, x0, seed=2, stream=0, cov=cov, nthreads=8)
+    assert ll_cli == o["loglik"]

@@ -331,6 +331,10 @@ static void load_series(cli_problem *P, int s, const pj *list, const char *key)
     c->n = n;
     c->x = x;
     c->y = y;
+    /* the value at t = 0 before any freeze: the population size of the initial state
+     * (pop_size_t0 is set before the freeze, build.c:1179-1205) */
+    const double t0 = 0.0;
+    if (plom_covariate_eval(c, 1, &t0, &P->cov_raw0[s]) != PLOM_OK) cli_die(plom_last_error());
     if (CLI_FREEZE >= 0.0 && n > 1) {
         double t = CLI_FREEZE < x[n - 1] ? CLI_FREEZE : x[n - 1], v;
         if (plom_covariate_eval(c, 1, &t, &v) != PLOM_OK) cli_die(plom_last_error());
@@ -348,7 +352,8 @@ static void load_series(cli_problem *P, int s, const pj *list, const char *key)
 
 /* slots: [covariate][N_CAC] then [observation covariate][N_TS] (an observation covariate
  * without a series of its own for a time series takes its first population's); the values
- * at t = 0 go into pv (pop_size_t0, build.c:1179-1185, for N) */
+ * at t = 0 of the (frozen) series go into pv, those of the series as given make the
+ * population sizes of the initial state (pop_size_t0, build.c:1179-1185) */
 static void cli_load_covariates(cli_problem *P, const pj *pfv, const pj *par_fixed, const pj *pfo)
 {
     const int N_CAC = P->N_CAC, N_TS = P->N_TS;
@@ -356,6 +361,7 @@ static void cli_load_covariates(cli_problem *P, const pj *pfv, const pj *par_fix
     if (n_fix != P->mi.n_fix) cli_die("settings covariates differ from the compiled model");
     P->cov = calloc((size_t)n_fix + 1, sizeof(plom_covariate));
     P->cov0 = calloc((size_t)n_fix + 1, sizeof(double));
+    P->cov_raw0 = calloc((size_t)n_fix + 1, sizeof(double));
     int s = 0;
     for (int k = 0; k < P->n_fixed; k++) {
         const pj *list = pj_get(pfv, pj_string(pj_at(par_fixed, k)));
@@ -369,6 +375,7 @@ static void cli_load_covariates(cli_problem *P, const pj *pfv, const pj *par_fix
                 load_series(P, s++, list, pj_string(pj_at(P->ts_id, ts)));
             } else {
                 P->cov[s] = P->cov[k * N_CAC];
+                P->cov_raw0[s] = P->cov_raw0[k * N_CAC];
                 s++;
             }
         }
@@ -376,7 +383,7 @@ static void cli_load_covariates(cli_problem *P, const pj *pfv, const pj *par_fix
     const double t0 = 0.0;
     for (int i = 0; i < n_fix; i++)
         if (plom_covariate_eval(&P->cov[i], 1, &t0, &P->cov0[i]) != PLOM_OK) cli_die(plom_last_error());
-    for (int i = 0; i < P->n_fixed * N_CAC; i++) P->fixed[i] = P->cov0[i];
+    for (int i = 0; i < P->n_fixed * N_CAC; i++) P->fixed[i] = P->cov_raw0[i];
 }
 
 void cli_load(cli_problem *P, int is_bayesian, int nb_obs, double dt)

@@ -54,6 +54,7 @@ typedef struct {
     double *fixed;      /* [n_fixed][N_CAC] covariates at t = 0; k = 0 is the population size */
     plom_covariate *cov; /* [n_fix] the covariate series of the pv slots (plom_covariate) */
     double *cov0;       /* [n_fix] their values at t = 0 */
+    double *cov_raw0;   /* [n_fix] the values at t = 0 before -g freezes the series */
     int has_cov;        /* some covariate changes with time: the filters take cov */
     double *pop_t;      /* [N + 1][N_CAC] population sizes at the observation times */
     double *times;      /* [N + 1] */

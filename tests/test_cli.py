@@ -334,13 +334,16 @@ def test_cli_noise_covariates_match_api(tmp_path, ctx):
     from oracle import oracle
     d = model_dir(tmp_path, "noise")
     theta = os.path.join(PKG, "examples", "noise", "theta.json")
-    dry = json.loads(run_cli(d, ["sde", "-J", "2000", "--dry_run"], theta).stdout)
     m = smc.Model.get("noise")
     times, data = m.dataset()
-    pv, x0 = np.array(dry["pv"]), np.array(dry["x0"])
     cov = m.covariates()
-    for extra, cv in ((["-g", "100"], [(np.array([x[0], x[-1]]), np.full(2, smc.covariate_eval(x, y, [min(100.0, x[-1])])[0]))
-                                       if len(x) > 1 else (x, y) for x, y in cov]), ([], cov)):
+    frozen = [(np.array([x[0], x[-1]]), np.full(2, smc.covariate_eval(x, y, [min(100.0, x[-1])])[0]))
+              if len(x) > 1 else (x, y) for x, y in cov]
+    for extra, cv in ((["-g", "100"], frozen), ([], cov)):
+        dry = json.loads(run_cli(d, ["sde", "-J", "2000", "--dry_run"] + extra, theta).stdout)
+        pv, x0 = np.array(dry["pv"]), np.array(dry["x0"])
+        # the initial state uses N(0) of the series as given, frozen or not (build.c:1179-1205)
+        assert x0[0] == pytest.approx(0.7 * 1e6, rel=1e-15)
         print("cli noise", extra, flush=True)
         r = run_cli(d, ["sde", "-J", "2000", "-b", "-e", "-d"] + extra, theta)
         assert r.returncode == 0, r.stderr

@@ -73,6 +73,10 @@ extern "C" {
                                          system with stochastic rates, an Euler-multinomial step with
                                          gamma white noise (step_psr, C/templates/core_template.c:104-209);
                                          FP64 only; of noises_off only PLOM_NO_ENV_STO_BIT applies */
+#define PLOM_FLAG_ODE 128u            /* implementation "ode": the rate equations integrated without noise by
+                                         adaptive Runge-Kutta-Fehlberg 4(5) under the error control
+                                         eps_abs / eps_rel (f_prediction_ode, C/core/prediction_util.c:176-193);
+                                         FP64 only; noises_off is ignored; covariates constant (pv) */
 #define PLOM_FLAG_MIF 32u             /* per-particle parameters (iterated filtering): needs
                                          plom_filter_inputs.mif and plom_filter_mif_start before
                                          every run; not with NO_FILTER, HAT or LINEAGE */
@@ -111,6 +115,8 @@ typedef struct {
     double like_min;     /* LIKE_MIN (C/smc/main.c:83) */
     int32_t n_theta;     /* PLOM_FLAG_MIF: components of the per-particle parameter vector theta */
     int32_t pad;
+    double eps_abs;      /* PLOM_FLAG_ODE: absolute and relative error targets of the integration; */
+    double eps_rel;      /* both 0: the reference's defaults 1e-6 and 1e-3 (C/core/plom.h:90-91) */
 } plom_filter_desc;
 
 /* How a particle's theta (transformed scale, one value per parameter and group, as

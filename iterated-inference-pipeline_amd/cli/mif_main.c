@@ -31,7 +31,8 @@
  * the starting parameters come from Philox-4x32-10 keyed (seed_k, 0xfffffffe), counter
  * (j, m, attempt, block), four Box-Muller normals per block.
  *
- * Implementations 'sde' and 'psr' (the default, FP64, initial states rounded); not 'ode'.
+ * Implementations 'sde', 'psr' (the default, FP64, initial states rounded) and 'ode' (FP64,
+ * adaptive RKF45 under --eps_abs / --eps_rel, every noise off).
  * Differences: --prior is not supported; a "covariance" in
  * theta (MVN proposals and mutation) is refused; the fixed-lag update uses the filtered
  * weighted mean of the initial conditions at the last observation up to L (the reference
@@ -90,6 +91,7 @@ int main(int argc, char *argv[])
     long seed_opt = -1;
     double dt = 0.0, like_min = 1e-17, mif_a = 0.975, mif_b = 2.0, prop_L = 0.75;
     unsigned int noises_off = 0;
+    double eps_abs = 1e-6, eps_rel = 1e-3;   /* PLOM_EPS_ABS / PLOM_EPS_REL (C/core/plom.h:90-91) */
     char path[1024] = "./";
     static struct option long_options[] = {
         {"prior", no_argument, 0, 1000}, {"no_dem_sto", no_argument, 0, 'x'},
@@ -116,7 +118,7 @@ int main(int argc, char *argv[])
         "                     [-a, --cooling <float>] [-b, --heat <float>] [-L, --lag <float>] [-S, --switch <integer>]\n"
         "                     [-f --ic_only] [-q, --quiet] [-P, --pipe]\n"
         "                     [--chains <K>] [--lanes <k>] [--fp32] [--device <k>] [--seed <k>] [--trace_17g] [-h, --help]\n"
-        "where implementation is 'sde' or 'psr' (default)\n";
+        "where implementation is 'ode', 'sde' or 'psr' (default)\n";
     for (;;) {
         int idx = 0;
         int ch = getopt_long(argc, argv, "rqPhxyzs:v:w:i:J:l:M:a:b:L:S:fp:N:g:", long_options, &idx);
@@ -126,7 +128,8 @@ int main(int argc, char *argv[])
         case 'y': noises_off |= PLOM_NO_ENV_STO_BIT; break;
         case 'z': noises_off |= PLOM_NO_DRIFT_BIT; break;
         case 's': dt = atof(optarg); break;
-        case 'v': case 'w': break;
+        case 'v': eps_abs = atof(optarg); break;
+        case 'w': eps_rel = atof(optarg); break;
         case 'g': CLI_FREEZE = atof(optarg); break;
         case 'h': printf("%s", help); return 1;
         case 'p': snprintf(path, sizeof path, "%s", optarg); break;
@@ -156,8 +159,10 @@ int main(int argc, char *argv[])
     }
     argc -= optind;
     argv += optind;
-    const int psr = cli_implementation(argc, argv, help);
-    if (psr && fp32) cli_die("the psr implementation runs in FP64 only (use 'sde' with --fp32)");
+    const int impl = cli_implementation(argc, argv, help);
+    uint32_t impl_flags = 0;
+    cli_apply_implementation(impl, fp32, &noises_off, &impl_flags);
+    const int psr = impl == CLI_PSR;
     if (prior) cli_die("--prior is not supported by this build");
     if (n_chains < 1 || n_chains > 4096) cli_die("--chains must be in [1, 4096]");
     if (pipe && n_chains > 1) cli_die("-P (pipe) prints one theta: use it with one chain");
@@ -345,7 +350,9 @@ int main(int argc, char *argv[])
         d.N = N;
         d.precision = fp32 ? PLOM_FP32 : PLOM_FP64;
         d.noises_off = noises_off;
-        d.flags = PLOM_FLAG_MIF | (psr ? PLOM_FLAG_PSR : 0u);
+        d.eps_abs = eps_abs;
+        d.eps_rel = eps_rel;
+        d.flags = PLOM_FLAG_MIF | impl_flags;
         d.seed = c->seed;
         d.dt = P.dt;
         d.like_min = like_min;

@@ -33,11 +33,22 @@ void cli_die(const char *msg)
 
 int cli_implementation(int argc, char **argv, const char *help)
 {
-    if (argc == 0 || !strcmp(argv[0], "psr")) return 1;
-    if (!strcmp(argv[0], "sde")) return 0;
+    if (argc == 0 || !strcmp(argv[0], "psr")) return CLI_PSR;
+    if (!strcmp(argv[0], "sde")) return CLI_SDE;
+    if (!strcmp(argv[0], "ode")) return CLI_ODE;
     fprintf(stderr, "%s", help);
-    if (!strcmp(argv[0], "ode")) fprintf(stderr, "error: the 'ode' implementation is not built for the GPU\n");
     exit(1);
+}
+
+void cli_apply_implementation(int impl, int fp32, unsigned int *noises_off, uint32_t *flags)
+{
+    if (impl != CLI_SDE && fp32) cli_die("the psr and ode implementations run in FP64 only (use 'sde' with --fp32)");
+    if (impl == CLI_PSR) *flags |= PLOM_FLAG_PSR;
+    if (impl == CLI_ODE) {
+        *flags |= PLOM_FLAG_ODE;
+        /* 'ode' turns every noise off (C/smc/main.c:227-229) */
+        *noises_off |= PLOM_NO_DEM_STO_BIT | PLOM_NO_ENV_STO_BIT | PLOM_NO_DRIFT_BIT;
+    }
 }
 
 /* ------------------------------------------------------------------ transforms */

@@ -71,10 +71,15 @@ extern double CLI_FREEZE;   /* -g/--freeze_forcing: covariates constant from tha
 extern int CLI_TRACE_17G;   /* trace values with 17 significant digits instead of %g */
 void cli_log(const char *msg);
 
-/* the implementation argument of the commands (e.g. C/smc/main.c:223-238): 'psr' (also the
- * default without one) -> 1, 'sde' -> 0; 'ode' is not built; anything else prints the help
- * and exits with 1 */
+/* the implementation argument of the commands (C/smc/main.c:223-238): 'psr' (also the
+ * default without one), 'sde' or 'ode'; anything else prints the help and exits with 1 */
+#define CLI_SDE 0
+#define CLI_PSR 1
+#define CLI_ODE 2
 int cli_implementation(int argc, char **argv, const char *help);
+/* the implementation's flags (PLOM_FLAG_PSR / PLOM_FLAG_ODE); 'ode' also turns every noise
+ * off; psr and ode refuse --fp32 */
+void cli_apply_implementation(int impl, int fp32, unsigned int *noises_off, uint32_t *flags);
 void cli_die(const char *msg);
 
 double cli_f_tr(cli_tr k, double x, double a, double b);

@@ -40,7 +40,8 @@
  * four per block), (m, 2, 0, 0) for the acceptance uniform, (m, 3, block, 0) for the
  * shuffle.  Filters never use stream 0xffffffff (iterations are below 2^31).
  *
- * Implementations 'sde' and 'psr' (the default, FP64, initial states rounded); not 'ode'.
+ * Implementations 'sde', 'psr' (the default, FP64, initial states rounded) and 'ode' (FP64,
+ * adaptive RKF45 under --eps_abs / --eps_rel, every noise off).
  * Differences: -Z/-c (zmq workers) and -N are accepted
  * and ignored; per-group transforms are used throughout (the reference takes group 0's
  * f_inv in the trace, the done file and the proposal's change of variable); the proposal
@@ -275,6 +276,7 @@ int main(int argc, char *argv[])
     long seed_opt = -1;
     double dt = 0.0;
     unsigned int noises_off = 0;
+    double eps_abs = 1e-6, eps_rel = 1e-3;   /* PLOM_EPS_ABS / PLOM_EPS_REL (C/core/plom.h:90-91) */
     char path[1024] = "./";
     mcmc_t MC;
     memset(&MC, 0, sizeof MC);
@@ -313,7 +315,7 @@ int main(int argc, char *argv[])
         "                [-E --epsilon <int || 50>] [--epsilon_max <float || 50.0>] [--smooth] [--alpha <float || 0.02>]\n"
         "                [-o, --nb_obs <int>] [-q, --quiet] [-P, --pipe]\n"
         "                [--chains <K || 1>] [--lanes <k>] [--fp32] [--device <k>] [--seed <k>] [--trace_17g] [-h, --help]\n"
-        "where implementation is 'sde' or 'psr' (default)\n";
+        "where implementation is 'ode', 'sde' or 'psr' (default)\n";
     for (;;) {
         int idx = 0;
         int ch = getopt_long(argc, argv, "qPhrxyzs:v:w:i:J:l:M:p:c:N:ZS:E:a:f:G:n:o:g:", long_options, &idx);
@@ -324,7 +326,9 @@ int main(int argc, char *argv[])
         case 'y': noises_off |= PLOM_NO_ENV_STO_BIT; break;
         case 'z': noises_off |= PLOM_NO_DRIFT_BIT; break;
         case 's': dt = atof(optarg); break;
-        case 'v': case 'w': case 'c': case 'Z': break;
+        case 'v': eps_abs = atof(optarg); break;
+        case 'w': eps_rel = atof(optarg); break;
+        case 'c': case 'Z': break;
         case 'g': CLI_FREEZE = atof(optarg); break;
         case 'o': nb_obs = atoi(optarg); break;
         case 'a': MC.a = atof(optarg); break;
@@ -357,8 +361,10 @@ int main(int argc, char *argv[])
     MC.smooth = smooth_flag;
     argc -= optind;
     argv += optind;
-    const int psr = cli_implementation(argc, argv, help);
-    if (psr && fp32) cli_die("the psr implementation runs in FP64 only (use 'sde' with --fp32)");
+    const int impl = cli_implementation(argc, argv, help);
+    uint32_t impl_flags = 0;
+    cli_apply_implementation(impl, fp32, &noises_off, &impl_flags);
+    const int psr = impl == CLI_PSR;
     if (n_chains < 1 || n_chains > 4096) cli_die("--chains must be in [1, 4096]");
     if (pipe && n_chains > 1) cli_die("-P (pipe) prints one theta: use it with one chain");
     if (M_iter < 1 || M_iter >= (1 << 30)) cli_die("-M must be in [1, 2^30)");
@@ -553,7 +559,9 @@ int main(int argc, char *argv[])
         d.N = N;
         d.precision = fp32 ? PLOM_FP32 : PLOM_FP64;
         d.noises_off = noises_off;
-        d.flags = (print_x ? PLOM_FLAG_LINEAGE : 0u) | (psr ? PLOM_FLAG_PSR : 0u);
+        d.eps_abs = eps_abs;
+        d.eps_rel = eps_rel;
+        d.flags = (print_x ? PLOM_FLAG_LINEAGE : 0u) | impl_flags;
         d.seed = C[k].seed;
         d.stream = 0;
         d.dt = P.dt;

@@ -19,8 +19,9 @@
  * Parameter handling (routers, transforms, theta remainder, initial state) is the shared
  * host logic of plom_cli.c.
  *
- * Implementations "sde" and "psr" (the default, as main.c:223-238: FP64, the initial state
- * rounded to counts, prediction_util.c:108-110); "ode" is not built.
+ * Implementations "sde", "psr" (the default, as main.c:223-238: FP64, the initial state
+ * rounded to counts, prediction_util.c:108-110) and "ode" (FP64, adaptive RKF45 under
+ * --eps_abs / --eps_rel, every noise off).
  * Differences: the X output
  * of -r is not written (a note goes to stderr); hat and pred_res use this build's
  * arithmetic (exact integer weights for the 95 % bounds, tree sums);
@@ -47,6 +48,7 @@ int main(int argc, char *argv[])
     long seed_opt = -1;
     double dt = 0.0, like_min = 1e-17;
     unsigned int noises_off = 0;
+    double eps_abs = 1e-6, eps_rel = 1e-3;   /* PLOM_EPS_ABS / PLOM_EPS_REL (C/core/plom.h:90-91) */
     char path[1024] = "./";
     static struct option long_options[] = {
         {"traj", no_argument, 0, 'r'}, {"no_dem_sto", no_argument, 0, 'x'},
@@ -72,7 +74,7 @@ int main(int argc, char *argv[])
         "                     [-s, --DT <float>] [--eps_abs <float>] [--eps_rel <float>]\n"
         "                     [-l, --LIKE_MIN <float>] [-J <integer>] [--prior]\n"
         "                     [-q, --quiet] [-P, --pipe] [--fp32] [--device <k>] [--seed <k>] [-h, --help]\n"
-        "where implementation is 'sde' or 'psr' (default)\n";
+        "where implementation is 'ode', 'sde' or 'psr' (default)\n";
     for (;;) {
         int idx = 0;
         int ch = getopt_long(argc, argv, "qPxyzs:v:w:p:i:J:l:trbehrN:o:g:d", long_options, &idx);
@@ -82,7 +84,8 @@ int main(int argc, char *argv[])
         case 'y': noises_off |= PLOM_NO_ENV_STO_BIT; break;
         case 'z': noises_off |= PLOM_NO_DRIFT_BIT; break;
         case 's': dt = atof(optarg); break;
-        case 'v': case 'w': break;
+        case 'v': eps_abs = atof(optarg); break;
+        case 'w': eps_rel = atof(optarg); break;
         case 'g': CLI_FREEZE = atof(optarg); break;
         case 'o': nb_obs = atoi(optarg); break;
         case 'h': printf("%s", help); return 1;
@@ -108,8 +111,10 @@ int main(int argc, char *argv[])
     }
     argc -= optind;
     argv += optind;
-    const int psr = cli_implementation(argc, argv, help);
-    if (psr && fp32) cli_die("the psr implementation runs in FP64 only (use 'sde' with --fp32)");
+    const int impl = cli_implementation(argc, argv, help);
+    uint32_t impl_flags = 0;
+    cli_apply_implementation(impl, fp32, &noises_off, &impl_flags);
+    const int psr = impl == CLI_PSR;
     if (prior) cli_die("--prior is not supported by this build");
 
     char str[2048];
@@ -157,7 +162,9 @@ int main(int argc, char *argv[])
     d.N = N;
     d.precision = fp32 ? PLOM_FP32 : PLOM_FP64;
     d.noises_off = noises_off;
-    d.flags = (filter ? 0u : PLOM_FLAG_NO_FILTER) | (want_hat ? PLOM_FLAG_HAT : 0u) | (psr ? PLOM_FLAG_PSR : 0u);
+    d.eps_abs = eps_abs;
+    d.eps_rel = eps_rel;
+    d.flags = (filter ? 0u : PLOM_FLAG_NO_FILTER) | (want_hat ? PLOM_FLAG_HAT : 0u) | impl_flags;
     d.seed = (uint32_t)(seed_opt >= 0 ? seed_opt : 2 + general_id);
     d.stream = 0;
     d.dt = dt;

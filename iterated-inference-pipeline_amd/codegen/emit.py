@@ -426,6 +426,88 @@ class Emitter:
         L[bind_at:bind_at] = binds
         return "\n".join(L)
 
+    # ---------------------------------------------------------------- ODE
+    def ode_rhs(self, pp=False):
+        """dy/dt of the states and of the observed incidences (the derivative form of
+        step_ode, core_template.c:215-339 with noises_off == 'ode': no noise, no clipping);
+        the incidences sum their reactions over the populations of their series, in the
+        series' population order.  Covariates are the constants of cv (no time-varying
+        table with this implementation)."""
+        m = self.m
+        dev = "static __device__ inline __attribute__((always_inline))"
+        name = "%s_ode_rhs%s" % (self.p, "_pp" if pp else "")
+        L = ["%s void %s(double t, const double *X, const double *pv, const double *lp, const double *cv, double *f)"
+             % (dev if pp else "PLOM_HD", name), "{", "    (void)pv; (void)lp; (void)cv;",
+             "    const double tr = t; (void)tr;"]
+        self.used_pars = set()
+        self.used_cov = set()
+        self.time_funcs = {}
+        body = []
+        for c in range(m.N_CAC):
+            em = self.rate_emitter(c, "double")
+            for k, tree in enumerate(m.rate_trees):
+                body.append("    const double r%d_%d = %s;" % (k, c, em.emit(tree)))
+        for c in range(m.N_CAC):
+            for s in m.par_sv:
+                items = [("-", self.term(r, c, r["from"]), r) for r in m.proc_model if r["from"] == s]
+                items += [("+", self.term(r, c, r["from"]), r) for r in m.proc_model if r["to"] == s]
+                eq, _, _ = self.eq_parts(items, c, "double", False, False)
+                body.append("    f[%d] = %s;" % (self.xidx(s, c), eq if eq else "0.0"))
+        u = 0
+        for o, lst in enumerate(m.obs_var_def):
+            if not m.is_inc[o]:
+                continue
+            info = m.obs2ts[o]
+            for tsu in range(info["n_ts_unique"]):
+                idx = m.inc_offset + u
+                body.append("    {")
+                body.append("        double sum_inc = 0.0;")
+                for (ci, ai) in info["cac"][tsu]:
+                    c = ci * m.N_AC + ai
+                    items = []
+                    for dfn in lst:
+                        for ri in dfn["reactions"]:
+                            r = m.proc_model[ri]
+                            items.append(("+", self.term(r, c, dfn["from"]), r))
+                    eq, _, _ = self.eq_parts(items, c, "double", False, False)
+                    body.append("        sum_inc += %s;" % eq)
+                body.append("        f[%d] = sum_inc;" % idx)
+                body.append("    }")
+                u += 1
+        npp = m.layout["npp"]
+        binds = [("    const double P%d = lp[%d];" % (k, k)) if pp and k < npp else
+                 ("    const double P%d = PLOM_UNIFORM64(pv[%d]);" % (k, k)) for k in sorted(self.used_pars)]
+        covs = ["    const double C%d = plom_cov_ld(cv + %d);" % (i, i) for i in sorted(self.used_cov)]
+        tfs = ["    const double %s = %s;" % (name_, text) for text, name_ in
+               sorted(self.time_funcs.items(), key=lambda kv: int(kv[1][2:]))]
+        self.time_funcs = None
+        L += binds + covs + tfs + body + ["}"]
+        return "\n".join(L)
+
+    def interval_ode(self, pp=False):
+        """One interval of the ode implementation: reset the incidences, then the adaptive
+        RKF45 integration of ode_rhs (csrc/plom_ode.h) from t0 to t1, first step dt."""
+        m = self.m
+        dev = "static __device__ inline __attribute__((always_inline))"
+        sfx = "_pp" if pp else ""
+        L = ["#define PLOM_ODE_FN %s_ode_integrate%s" % (self.p, sfx),
+             "#define PLOM_ODE_RHS %s_ode_rhs%s" % (self.p, sfx),
+             "#define PLOM_ODE_N %d" % m.size_proj,
+             "#define PLOM_ODE_QUAL %s" % (dev if pp else "PLOM_HD"),
+             '#include "plom_ode_impl.h"', ""]
+        if pp:
+            L.append("%s int %s_interval_ode_pp_f64(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)"
+                     % (dev, self.p))
+        else:
+            L.append("PLOM_HD int %s_interval_ode_f64(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)"
+                     % self.p)
+        L.append("{")
+        for u in range(m.N_TS_INC_UNIQUE):
+            L.append("    X[%d] = 0.0; /* reset_inc */" % (m.inc_offset + u))
+        L.append("    return %s_ode_integrate%s(X, pv, %s, cv, t0, t1, dt, eps_abs, eps_rel);" % (self.p, sfx, "lp" if pp else "0"))
+        L.append("}")
+        return "\n".join(L)
+
     # ---------------------------------------------------------------- observations
     def proj2obs(self, real, pp=False):
         m = self.m
@@ -626,10 +708,19 @@ class Emitter:
         for env in (True, False):
             out.append(self.interval_psr(env))
             out.append("")
+        out.append(self.ode_rhs())
+        out.append("")
+        out.append(self.interval_ode())
+        out.append("")
         # host dispatcher on the reference's noises_off bits (prediction_util.c:143-161),
-        # PLOM_IMPL_PSR selecting the psr implementation (prediction_util.c:162-170)
-        out.append("PLOM_HD void %s_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs)" % self.p)
+        # PLOM_IMPL_PSR / PLOM_IMPL_ODE selecting the psr / ode implementations
+        # (prediction_util.c:139-170)
+        out.append("PLOM_HD void %s_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel)" % self.p)
         out.append("{")
+        out.append("    if (noises_off & PLOM_IMPL_ODE) {")
+        out.append("        (void)%s_interval_ode_f64(X, pv, cv, t0, t1, dt, eps_abs, eps_rel);" % self.p)
+        out.append("        return;")
+        out.append("    }")
         out.append("    if (noises_off & PLOM_IMPL_PSR) {")
         out.append("        if (noises_off & PLOM_NO_ENV_STO) %s_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
         out.append("        else %s_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
@@ -651,6 +742,10 @@ class Emitter:
         for env in (True, False):
             out.append(self.interval_psr(env, pp=True))
             out.append("")
+        out.append(self.ode_rhs(pp=True))
+        out.append("")
+        out.append(self.interval_ode(pp=True))
+        out.append("")
         for real in ("double", "float"):
             for v, dem, env in VARIANTS:
                 out.append(self.interval(v, dem, env, real, pp=True))
@@ -689,6 +784,10 @@ class Emitter:
                        % (Rt, self.p, tag))
             out.append("    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const %s *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { %s_proj2obs_pp_%s(X, pvd, lpd, cvd, obs); }"
                        % (Rt, self.p, tag))
+        out.append("    static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double ea, double er)"
+                   " { return %s_interval_ode_f64(X, pv, cv, t0, t1, dt, ea, er); }" % self.p)
+        out.append("    static __device__ inline __attribute__((always_inline)) int interval_ode_pp(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double ea, double er)"
+                   " { return %s_interval_ode_pp_f64(X, pv, lp, cv, t0, t1, dt, ea, er); }" % self.p)
         out.append("    static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor)"
                    " { return %s_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }" % self.p)
         out.append("    static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor)"

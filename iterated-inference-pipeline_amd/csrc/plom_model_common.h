@@ -70,6 +70,8 @@ PLOM_HD int plom_substeps(double t0, double t1, double dt)
 #define PLOM_NO_ENV_STO 2u
 /* the host dispatchers' selector of the psr implementation (Euler-multinomial step) */
 #define PLOM_IMPL_PSR 0x100u
+/* the host dispatchers' selector of the ode implementation (adaptive RKF45) */
+#define PLOM_IMPL_ODE 0x200u
 #define PLOM_NO_DRIFT 4u
 
 PLOM_HD double plom_heaviside(double x) { return (x < 0.0) ? 0.0 : 1.0; }

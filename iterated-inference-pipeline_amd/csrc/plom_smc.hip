@@ -42,6 +42,7 @@
 #include "plom_smc.h"
 #include "plom_math.h"
 #include "plom_cov.h"
+#include "plom_ode.h"
 #include "plom_rng.h"
 #include "plom_mif.h"
 #include "plom_models.h"
@@ -78,6 +79,7 @@ struct plom_dfilter {
     int32_t J, Jp, T, N, S, n_ts, flags, step;
     uint32_t seed, stream, prec, pad;
     double dt, sqdt, like_min, all_fail_llike;
+    double eps_abs, eps_rel;   /* PLOM_FLAG_ODE: error control of the RKF45 integration */
     void *X[2];
     const void *x0;
     const double *pv;
@@ -348,7 +350,11 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
 
     const double t0 = f->times[n], t1 = f->times[n + 1];
     if (valid) {
-        if constexpr (sizeof(R) == 8) {
+        if constexpr (V == 6) {
+            /* the ode implementation: deterministic, no random numbers */
+            if constexpr (MIF) M::interval_ode_pp((double *)X, f->pv, lp, f->cov, t0, t1, f->dt, f->eps_abs, f->eps_rel);
+            else M::interval_ode((double *)X, f->pv, f->cov, t0, t1, f->dt, f->eps_abs, f->eps_rel);
+        } else if constexpr (sizeof(R) == 8) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j, s_bm);
             if constexpr (MIF) M::template interval_pp<V>((double *)X, f->pv, lp, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs);
@@ -667,7 +673,9 @@ __global__ __launch_bounds__(64) void k_lineage(plom_dfilter *f)
     for (int n = 0; n < N; n++) {
         const double t0 = f->times[n], t1 = f->times[n + 1];
         const uint32_t jn = (uint32_t)f->lin_slots[n];
-        if constexpr (sizeof(R) == 8) {
+        if constexpr (V == 6) {
+            M::interval_ode((double *)X, f->pv, f->cov, t0, t1, f->dt, f->eps_abs, f->eps_rel);
+        } else if constexpr (sizeof(R) == 8) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, jn, s_bm);
             M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs);
@@ -996,6 +1004,7 @@ static void launch_prop_r(int variant, int nblocks, hipStream_t s, plom_dfilter 
     if constexpr (sizeof(R) == 8) {
         if (variant == 4) { hipLaunchKernelGGL((k_propagate<M, R, 4, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); return; }
         if (variant == 5) { hipLaunchKernelGGL((k_propagate<M, R, 5, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); return; }
+        if (variant == 6) { hipLaunchKernelGGL((k_propagate<M, R, 6, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); return; }
     }
     switch (variant) {
     case 0: hipLaunchKernelGGL((k_propagate<M, R, 0, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
@@ -1023,6 +1032,7 @@ static void launch_lineage_r(int variant, hipStream_t s, plom_dfilter *df)
     if constexpr (sizeof(R) == 8) {
         if (variant == 4) { hipLaunchKernelGGL((k_lineage<M, R, 4>), dim3(1), dim3(64), 0, s, df); return; }
         if (variant == 5) { hipLaunchKernelGGL((k_lineage<M, R, 5>), dim3(1), dim3(64), 0, s, df); return; }
+        if (variant == 6) { hipLaunchKernelGGL((k_lineage<M, R, 6>), dim3(1), dim3(64), 0, s, df); return; }
     }
     switch (variant) {
     case 0: hipLaunchKernelGGL((k_lineage<M, R, 0>), dim3(1), dim3(64), 0, s, df); break;
@@ -1238,6 +1248,12 @@ static int check_desc(const plom_filter_desc *d, plom_model_info *mi)
     if (!(d->like_min > 0.0)) return set_err(PLOM_ERR_ARG, "like_min must be > 0");
     if ((d->flags & PLOM_FLAG_PSR) && d->precision != PLOM_FP64)
         return set_err(PLOM_ERR_ARG, "PLOM_FLAG_PSR (the psr implementation) runs in FP64 only");
+    if (d->flags & PLOM_FLAG_ODE) {
+        if (d->precision != PLOM_FP64) return set_err(PLOM_ERR_ARG, "PLOM_FLAG_ODE (the ode implementation) runs in FP64 only");
+        if (d->flags & PLOM_FLAG_PSR) return set_err(PLOM_ERR_ARG, "PLOM_FLAG_ODE and PLOM_FLAG_PSR exclude each other");
+        if (d->eps_abs < 0.0 || d->eps_rel < 0.0 || !(d->eps_abs + d->eps_rel >= 0.0))
+            return set_err(PLOM_ERR_ARG, "eps_abs and eps_rel must be >= 0");
+    }
     if (d->flags & PLOM_FLAG_MIF) {
         if (d->n_theta < 1 || d->n_theta > 4096) return set_err(PLOM_ERR_ARG, "n_theta=%d outside [1, 4096]", d->n_theta);
         if (d->flags & (PLOM_FLAG_NO_FILTER | PLOM_FLAG_HAT | PLOM_FLAG_LINEAGE))
@@ -1380,6 +1396,8 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     std::vector<double> covtab;
     int cov_kmax = 0;
     const int cov_on = in->cov != nullptr && mi.n_fix > 0;
+    if (cov_on && (d->flags & PLOM_FLAG_ODE))
+        return set_err(PLOM_ERR_ARG, "the ode implementation takes the covariates as constants in pv (no time-varying covariates)");
     if (cov_on) {
         cov_kmax = plom_cov_kmax(d->N, in->times, d->dt);
         covtab.resize(plom_cov_rows(d->N, cov_kmax) * (size_t)mi.n_fix);
@@ -1403,7 +1421,9 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     if (!f) return set_err(PLOM_ERR_NOMEM, "host allocation failed");
     f->desc = *d;
     f->info = mi;
-    f->variant = (d->flags & PLOM_FLAG_PSR) ? 4 + ((d->noises_off & PLOM_NO_ENV_STO_BIT) ? 1 : 0) : (int)(d->noises_off & 3u);
+    f->variant = (d->flags & PLOM_FLAG_ODE) ? 6
+                 : (d->flags & PLOM_FLAG_PSR) ? 4 + ((d->noises_off & PLOM_NO_ENV_STO_BIT) ? 1 : 0)
+                 : (int)(d->noises_off & 3u);
     f->dev_bytes = L.total;
     f->async_alloc = async;
     hipError_t e = async ? hipMallocAsync((void **)&f->dev, L.total, st) : hipMalloc((void **)&f->dev, L.total);
@@ -1420,6 +1440,8 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     h->dt = d->dt;
     h->sqdt = plom_sqrt(d->dt);
     h->like_min = d->like_min;
+    h->eps_abs = (d->eps_abs > 0.0 || d->eps_rel > 0.0) ? d->eps_abs : PLOM_ODE_EPS_ABS;
+    h->eps_rel = (d->eps_abs > 0.0 || d->eps_rel > 0.0) ? d->eps_rel : PLOM_ODE_EPS_REL;
     h->all_fail_llike = plom_log(d->like_min) * (double)mi.n_ts;
     h->X[0] = b + L.off_X0; h->X[1] = b + L.off_X1;
     h->x0 = b + L.off_x0;

@@ -630,8 +630,64 @@ PLOM_HD void plom_noise_interval_psr_no_env_sto_f64(double *X, const double *pv,
     }
 }
 
-PLOM_HD void plom_noise_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs)
+PLOM_HD void plom_noise_ode_rhs(double t, const double *X, const double *pv, const double *lp, const double *cv, double *f)
 {
+    (void)pv; (void)lp; (void)cv;
+    const double tr = t; (void)tr;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P2 = PLOM_UNIFORM64(pv[2]);
+    const double P3 = PLOM_UNIFORM64(pv[3]);
+    const double C0 = plom_cov_ld(cv + 0);
+    const double C1 = plom_cov_ld(cv + 1);
+    const double C4 = plom_cov_ld(cv + 4);
+    const double C5 = plom_cov_ld(cv + 5);
+    const double C6 = plom_cov_ld(cv + 6);
+    const double C7 = plom_cov_ld(cv + 7);
+    const double r0_0 = (C4 * C0);
+    const double r1_0 = (((P0 / C0) * P2) * X[2]);
+    const double r2_0 = (P2);
+    const double r3_0 = C6;
+    const double r0_1 = (C5 * C1);
+    const double r1_1 = (((P1 / C1) * P3) * X[3]);
+    const double r2_1 = (P3);
+    const double r3_1 = C7;
+    f[0] =  - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0);
+    f[2] =  - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]);
+    f[1] =  - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1);
+    f[3] =  - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]);
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r2_0*X[2]) + (r3_0*X[2]);
+        sum_inc +=  + (r2_1*X[3]) + (r3_1*X[3]);
+        f[4] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_1*X[1]);
+        f[5] = sum_inc;
+    }
+}
+
+#define PLOM_ODE_FN plom_noise_ode_integrate
+#define PLOM_ODE_RHS plom_noise_ode_rhs
+#define PLOM_ODE_N 6
+#define PLOM_ODE_QUAL PLOM_HD
+#include "plom_ode_impl.h"
+
+PLOM_HD int plom_noise_interval_ode_f64(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)
+{
+    X[4] = 0.0; /* reset_inc */
+    X[5] = 0.0; /* reset_inc */
+    return plom_noise_ode_integrate(X, pv, 0, cv, t0, t1, dt, eps_abs, eps_rel);
+}
+
+PLOM_HD void plom_noise_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel)
+{
+    if (noises_off & PLOM_IMPL_ODE) {
+        (void)plom_noise_interval_ode_f64(X, pv, cv, t0, t1, dt, eps_abs, eps_rel);
+        return;
+    }
     if (noises_off & PLOM_IMPL_PSR) {
         if (noises_off & PLOM_NO_ENV_STO) plom_noise_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
         else plom_noise_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
@@ -1172,6 +1228,58 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         X[5] = X[5] + (n1_1);
         t += dt;
     }
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_noise_ode_rhs_pp(double t, const double *X, const double *pv, const double *lp, const double *cv, double *f)
+{
+    (void)pv; (void)lp; (void)cv;
+    const double tr = t; (void)tr;
+    const double P0 = lp[0];
+    const double P1 = lp[1];
+    const double P2 = lp[2];
+    const double P3 = lp[3];
+    const double C0 = plom_cov_ld(cv + 0);
+    const double C1 = plom_cov_ld(cv + 1);
+    const double C4 = plom_cov_ld(cv + 4);
+    const double C5 = plom_cov_ld(cv + 5);
+    const double C6 = plom_cov_ld(cv + 6);
+    const double C7 = plom_cov_ld(cv + 7);
+    const double r0_0 = (C4 * C0);
+    const double r1_0 = (((P0 / C0) * P2) * X[2]);
+    const double r2_0 = (P2);
+    const double r3_0 = C6;
+    const double r0_1 = (C5 * C1);
+    const double r1_1 = (((P1 / C1) * P3) * X[3]);
+    const double r2_1 = (P3);
+    const double r3_1 = C7;
+    f[0] =  - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0);
+    f[2] =  - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]);
+    f[1] =  - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1);
+    f[3] =  - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]);
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r2_0*X[2]) + (r3_0*X[2]);
+        sum_inc +=  + (r2_1*X[3]) + (r3_1*X[3]);
+        f[4] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_1*X[1]);
+        f[5] = sum_inc;
+    }
+}
+
+#define PLOM_ODE_FN plom_noise_ode_integrate_pp
+#define PLOM_ODE_RHS plom_noise_ode_rhs_pp
+#define PLOM_ODE_N 6
+#define PLOM_ODE_QUAL static __device__ inline __attribute__((always_inline))
+#include "plom_ode_impl.h"
+
+static __device__ inline __attribute__((always_inline)) int plom_noise_interval_ode_pp_f64(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)
+{
+    X[4] = 0.0; /* reset_inc */
+    X[5] = 0.0; /* reset_inc */
+    return plom_noise_ode_integrate_pp(X, pv, lp, cv, t0, t1, dt, eps_abs, eps_rel);
 }
 
 static __device__ inline __attribute__((always_inline)) void plom_noise_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
@@ -1787,6 +1895,8 @@ struct plom_model_noise {
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_noise_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_noise_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
+    static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_noise_interval_ode_f64(X, pv, cv, t0, t1, dt, ea, er); }
+    static __device__ inline __attribute__((always_inline)) int interval_ode_pp(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_noise_interval_ode_pp_f64(X, pv, lp, cv, t0, t1, dt, ea, er); }
     static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor) { return plom_noise_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor) { return plom_noise_loglik_pp(obs, y, pvd, lpd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out) { plom_noise_hat_values(X, pvd, cvd, t, out); }

@@ -3322,8 +3322,309 @@ PLOM_HD void plom_seir10_interval_psr_no_env_sto_f64(double *X, const double *pv
     }
 }
 
-PLOM_HD void plom_seir10_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs)
+PLOM_HD void plom_seir10_ode_rhs(double t, const double *X, const double *pv, const double *lp, const double *cv, double *f)
 {
+    (void)pv; (void)lp; (void)cv;
+    const double tr = t; (void)tr;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P2 = PLOM_UNIFORM64(pv[2]);
+    const double P3 = PLOM_UNIFORM64(pv[3]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    const double P5 = PLOM_UNIFORM64(pv[5]);
+    const double P6 = PLOM_UNIFORM64(pv[6]);
+    const double P7 = PLOM_UNIFORM64(pv[7]);
+    const double P8 = PLOM_UNIFORM64(pv[8]);
+    const double P9 = PLOM_UNIFORM64(pv[9]);
+    const double P10 = PLOM_UNIFORM64(pv[10]);
+    const double P11 = PLOM_UNIFORM64(pv[11]);
+    const double P12 = PLOM_UNIFORM64(pv[12]);
+    const double P13 = PLOM_UNIFORM64(pv[13]);
+    const double P14 = PLOM_UNIFORM64(pv[14]);
+    const double P15 = PLOM_UNIFORM64(pv[15]);
+    const double P16 = PLOM_UNIFORM64(pv[16]);
+    const double P17 = PLOM_UNIFORM64(pv[17]);
+    const double P18 = PLOM_UNIFORM64(pv[18]);
+    const double P19 = PLOM_UNIFORM64(pv[19]);
+    const double P20 = PLOM_UNIFORM64(pv[20]);
+    const double P21 = PLOM_UNIFORM64(pv[21]);
+    const double P22 = PLOM_UNIFORM64(pv[22]);
+    const double P23 = PLOM_UNIFORM64(pv[23]);
+    const double P24 = PLOM_UNIFORM64(pv[24]);
+    const double P25 = PLOM_UNIFORM64(pv[25]);
+    const double P26 = PLOM_UNIFORM64(pv[26]);
+    const double P27 = PLOM_UNIFORM64(pv[27]);
+    const double P28 = PLOM_UNIFORM64(pv[28]);
+    const double P29 = PLOM_UNIFORM64(pv[29]);
+    const double P30 = PLOM_UNIFORM64(pv[30]);
+    const double P31 = PLOM_UNIFORM64(pv[31]);
+    const double P32 = PLOM_UNIFORM64(pv[32]);
+    const double P33 = PLOM_UNIFORM64(pv[33]);
+    const double P34 = PLOM_UNIFORM64(pv[34]);
+    const double P35 = PLOM_UNIFORM64(pv[35]);
+    const double P36 = PLOM_UNIFORM64(pv[36]);
+    const double P37 = PLOM_UNIFORM64(pv[37]);
+    const double P38 = PLOM_UNIFORM64(pv[38]);
+    const double P39 = PLOM_UNIFORM64(pv[39]);
+    const double P80 = PLOM_UNIFORM64(pv[80]);
+    const double P81 = PLOM_UNIFORM64(pv[81]);
+    const double P82 = PLOM_UNIFORM64(pv[82]);
+    const double P83 = PLOM_UNIFORM64(pv[83]);
+    const double P84 = PLOM_UNIFORM64(pv[84]);
+    const double P85 = PLOM_UNIFORM64(pv[85]);
+    const double P86 = PLOM_UNIFORM64(pv[86]);
+    const double P87 = PLOM_UNIFORM64(pv[87]);
+    const double P88 = PLOM_UNIFORM64(pv[88]);
+    const double P89 = PLOM_UNIFORM64(pv[89]);
+    const double P90 = PLOM_UNIFORM64(pv[90]);
+    const double P91 = PLOM_UNIFORM64(pv[91]);
+    const double P92 = PLOM_UNIFORM64(pv[92]);
+    const double P93 = PLOM_UNIFORM64(pv[93]);
+    const double P94 = PLOM_UNIFORM64(pv[94]);
+    const double P95 = PLOM_UNIFORM64(pv[95]);
+    const double P96 = PLOM_UNIFORM64(pv[96]);
+    const double P97 = PLOM_UNIFORM64(pv[97]);
+    const double P98 = PLOM_UNIFORM64(pv[98]);
+    const double P99 = PLOM_UNIFORM64(pv[99]);
+    const double P100 = PLOM_UNIFORM64(pv[100]);
+    const double P101 = PLOM_UNIFORM64(pv[101]);
+    const double P102 = PLOM_UNIFORM64(pv[102]);
+    const double P103 = PLOM_UNIFORM64(pv[103]);
+    const double P104 = PLOM_UNIFORM64(pv[104]);
+    const double P105 = PLOM_UNIFORM64(pv[105]);
+    const double P106 = PLOM_UNIFORM64(pv[106]);
+    const double P107 = PLOM_UNIFORM64(pv[107]);
+    const double P108 = PLOM_UNIFORM64(pv[108]);
+    const double P109 = PLOM_UNIFORM64(pv[109]);
+    const double P110 = PLOM_UNIFORM64(pv[110]);
+    const double P111 = PLOM_UNIFORM64(pv[111]);
+    const double P112 = PLOM_UNIFORM64(pv[112]);
+    const double P113 = PLOM_UNIFORM64(pv[113]);
+    const double P114 = PLOM_UNIFORM64(pv[114]);
+    const double P115 = PLOM_UNIFORM64(pv[115]);
+    const double P116 = PLOM_UNIFORM64(pv[116]);
+    const double P117 = PLOM_UNIFORM64(pv[117]);
+    const double P118 = PLOM_UNIFORM64(pv[118]);
+    const double P119 = PLOM_UNIFORM64(pv[119]);
+    const double P120 = PLOM_UNIFORM64(pv[120]);
+    const double P121 = PLOM_UNIFORM64(pv[121]);
+    const double P122 = PLOM_UNIFORM64(pv[122]);
+    const double P123 = PLOM_UNIFORM64(pv[123]);
+    const double P124 = PLOM_UNIFORM64(pv[124]);
+    const double P125 = PLOM_UNIFORM64(pv[125]);
+    const double P126 = PLOM_UNIFORM64(pv[126]);
+    const double P127 = PLOM_UNIFORM64(pv[127]);
+    const double P128 = PLOM_UNIFORM64(pv[128]);
+    const double P129 = PLOM_UNIFORM64(pv[129]);
+    const double P130 = PLOM_UNIFORM64(pv[130]);
+    const double P131 = PLOM_UNIFORM64(pv[131]);
+    const double P132 = PLOM_UNIFORM64(pv[132]);
+    const double P133 = PLOM_UNIFORM64(pv[133]);
+    const double P134 = PLOM_UNIFORM64(pv[134]);
+    const double P135 = PLOM_UNIFORM64(pv[135]);
+    const double P136 = PLOM_UNIFORM64(pv[136]);
+    const double P137 = PLOM_UNIFORM64(pv[137]);
+    const double P138 = PLOM_UNIFORM64(pv[138]);
+    const double P139 = PLOM_UNIFORM64(pv[139]);
+    const double P140 = PLOM_UNIFORM64(pv[140]);
+    const double P141 = PLOM_UNIFORM64(pv[141]);
+    const double P142 = PLOM_UNIFORM64(pv[142]);
+    const double P143 = PLOM_UNIFORM64(pv[143]);
+    const double P144 = PLOM_UNIFORM64(pv[144]);
+    const double P145 = PLOM_UNIFORM64(pv[145]);
+    const double P146 = PLOM_UNIFORM64(pv[146]);
+    const double P147 = PLOM_UNIFORM64(pv[147]);
+    const double P148 = PLOM_UNIFORM64(pv[148]);
+    const double P149 = PLOM_UNIFORM64(pv[149]);
+    const double P150 = PLOM_UNIFORM64(pv[150]);
+    const double P151 = PLOM_UNIFORM64(pv[151]);
+    const double P152 = PLOM_UNIFORM64(pv[152]);
+    const double P153 = PLOM_UNIFORM64(pv[153]);
+    const double P154 = PLOM_UNIFORM64(pv[154]);
+    const double P155 = PLOM_UNIFORM64(pv[155]);
+    const double P156 = PLOM_UNIFORM64(pv[156]);
+    const double P157 = PLOM_UNIFORM64(pv[157]);
+    const double P158 = PLOM_UNIFORM64(pv[158]);
+    const double P159 = PLOM_UNIFORM64(pv[159]);
+    const double P160 = PLOM_UNIFORM64(pv[160]);
+    const double P161 = PLOM_UNIFORM64(pv[161]);
+    const double P162 = PLOM_UNIFORM64(pv[162]);
+    const double P163 = PLOM_UNIFORM64(pv[163]);
+    const double P164 = PLOM_UNIFORM64(pv[164]);
+    const double P165 = PLOM_UNIFORM64(pv[165]);
+    const double P166 = PLOM_UNIFORM64(pv[166]);
+    const double P167 = PLOM_UNIFORM64(pv[167]);
+    const double P168 = PLOM_UNIFORM64(pv[168]);
+    const double P169 = PLOM_UNIFORM64(pv[169]);
+    const double P170 = PLOM_UNIFORM64(pv[170]);
+    const double P171 = PLOM_UNIFORM64(pv[171]);
+    const double P172 = PLOM_UNIFORM64(pv[172]);
+    const double P173 = PLOM_UNIFORM64(pv[173]);
+    const double P174 = PLOM_UNIFORM64(pv[174]);
+    const double P175 = PLOM_UNIFORM64(pv[175]);
+    const double P176 = PLOM_UNIFORM64(pv[176]);
+    const double P177 = PLOM_UNIFORM64(pv[177]);
+    const double P178 = PLOM_UNIFORM64(pv[178]);
+    const double P179 = PLOM_UNIFORM64(pv[179]);
+    const double C0 = plom_cov_ld(cv + 0);
+    const double C1 = plom_cov_ld(cv + 1);
+    const double C2 = plom_cov_ld(cv + 2);
+    const double C3 = plom_cov_ld(cv + 3);
+    const double C4 = plom_cov_ld(cv + 4);
+    const double C5 = plom_cov_ld(cv + 5);
+    const double C6 = plom_cov_ld(cv + 6);
+    const double C7 = plom_cov_ld(cv + 7);
+    const double C8 = plom_cov_ld(cv + 8);
+    const double C9 = plom_cov_ld(cv + 9);
+    const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
+    const double r1_0 = P10;
+    const double r2_0 = P20;
+    const double r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
+    const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
+    const double r1_1 = P11;
+    const double r2_1 = P21;
+    const double r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
+    const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
+    const double r1_2 = P12;
+    const double r2_2 = P22;
+    const double r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
+    const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
+    const double r1_3 = P13;
+    const double r2_3 = P23;
+    const double r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
+    const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
+    const double r1_4 = P14;
+    const double r2_4 = P24;
+    const double r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
+    const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
+    const double r1_5 = P15;
+    const double r2_5 = P25;
+    const double r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
+    const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
+    const double r1_6 = P16;
+    const double r2_6 = P26;
+    const double r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
+    const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
+    const double r1_7 = P17;
+    const double r2_7 = P27;
+    const double r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
+    const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
+    const double r1_8 = P18;
+    const double r2_8 = P28;
+    const double r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
+    const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
+    const double r1_9 = P19;
+    const double r2_9 = P29;
+    const double r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
+    f[0] =  - (r0_0*X[0]) + (r3_0);
+    f[10] =  - (r1_0*X[10]) + (r0_0*X[0]);
+    f[20] =  - (r2_0*X[20]) + (r1_0*X[10]);
+    f[1] =  - (r0_1*X[1]) + (r3_1);
+    f[11] =  - (r1_1*X[11]) + (r0_1*X[1]);
+    f[21] =  - (r2_1*X[21]) + (r1_1*X[11]);
+    f[2] =  - (r0_2*X[2]) + (r3_2);
+    f[12] =  - (r1_2*X[12]) + (r0_2*X[2]);
+    f[22] =  - (r2_2*X[22]) + (r1_2*X[12]);
+    f[3] =  - (r0_3*X[3]) + (r3_3);
+    f[13] =  - (r1_3*X[13]) + (r0_3*X[3]);
+    f[23] =  - (r2_3*X[23]) + (r1_3*X[13]);
+    f[4] =  - (r0_4*X[4]) + (r3_4);
+    f[14] =  - (r1_4*X[14]) + (r0_4*X[4]);
+    f[24] =  - (r2_4*X[24]) + (r1_4*X[14]);
+    f[5] =  - (r0_5*X[5]) + (r3_5);
+    f[15] =  - (r1_5*X[15]) + (r0_5*X[5]);
+    f[25] =  - (r2_5*X[25]) + (r1_5*X[15]);
+    f[6] =  - (r0_6*X[6]) + (r3_6);
+    f[16] =  - (r1_6*X[16]) + (r0_6*X[6]);
+    f[26] =  - (r2_6*X[26]) + (r1_6*X[16]);
+    f[7] =  - (r0_7*X[7]) + (r3_7);
+    f[17] =  - (r1_7*X[17]) + (r0_7*X[7]);
+    f[27] =  - (r2_7*X[27]) + (r1_7*X[17]);
+    f[8] =  - (r0_8*X[8]) + (r3_8);
+    f[18] =  - (r1_8*X[18]) + (r0_8*X[8]);
+    f[28] =  - (r2_8*X[28]) + (r1_8*X[18]);
+    f[9] =  - (r0_9*X[9]) + (r3_9);
+    f[19] =  - (r1_9*X[19]) + (r0_9*X[9]);
+    f[29] =  - (r2_9*X[29]) + (r1_9*X[19]);
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_0*X[10]);
+        f[30] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_1*X[11]);
+        f[31] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_2*X[12]);
+        f[32] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_3*X[13]);
+        f[33] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_4*X[14]);
+        f[34] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_5*X[15]);
+        f[35] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_6*X[16]);
+        f[36] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_7*X[17]);
+        f[37] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_8*X[18]);
+        f[38] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_9*X[19]);
+        f[39] = sum_inc;
+    }
+}
+
+#define PLOM_ODE_FN plom_seir10_ode_integrate
+#define PLOM_ODE_RHS plom_seir10_ode_rhs
+#define PLOM_ODE_N 40
+#define PLOM_ODE_QUAL PLOM_HD
+#include "plom_ode_impl.h"
+
+PLOM_HD int plom_seir10_interval_ode_f64(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)
+{
+    X[30] = 0.0; /* reset_inc */
+    X[31] = 0.0; /* reset_inc */
+    X[32] = 0.0; /* reset_inc */
+    X[33] = 0.0; /* reset_inc */
+    X[34] = 0.0; /* reset_inc */
+    X[35] = 0.0; /* reset_inc */
+    X[36] = 0.0; /* reset_inc */
+    X[37] = 0.0; /* reset_inc */
+    X[38] = 0.0; /* reset_inc */
+    X[39] = 0.0; /* reset_inc */
+    return plom_seir10_ode_integrate(X, pv, 0, cv, t0, t1, dt, eps_abs, eps_rel);
+}
+
+PLOM_HD void plom_seir10_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel)
+{
+    if (noises_off & PLOM_IMPL_ODE) {
+        (void)plom_seir10_interval_ode_f64(X, pv, cv, t0, t1, dt, eps_abs, eps_rel);
+        return;
+    }
     if (noises_off & PLOM_IMPL_PSR) {
         if (noises_off & PLOM_NO_ENV_STO) plom_seir10_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
         else plom_seir10_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
@@ -6440,6 +6741,303 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
+static __device__ inline __attribute__((always_inline)) void plom_seir10_ode_rhs_pp(double t, const double *X, const double *pv, const double *lp, const double *cv, double *f)
+{
+    (void)pv; (void)lp; (void)cv;
+    const double tr = t; (void)tr;
+    const double P0 = lp[0];
+    const double P1 = lp[1];
+    const double P2 = lp[2];
+    const double P3 = lp[3];
+    const double P4 = lp[4];
+    const double P5 = lp[5];
+    const double P6 = lp[6];
+    const double P7 = lp[7];
+    const double P8 = lp[8];
+    const double P9 = lp[9];
+    const double P10 = lp[10];
+    const double P11 = lp[11];
+    const double P12 = lp[12];
+    const double P13 = lp[13];
+    const double P14 = lp[14];
+    const double P15 = lp[15];
+    const double P16 = lp[16];
+    const double P17 = lp[17];
+    const double P18 = lp[18];
+    const double P19 = lp[19];
+    const double P20 = lp[20];
+    const double P21 = lp[21];
+    const double P22 = lp[22];
+    const double P23 = lp[23];
+    const double P24 = lp[24];
+    const double P25 = lp[25];
+    const double P26 = lp[26];
+    const double P27 = lp[27];
+    const double P28 = lp[28];
+    const double P29 = lp[29];
+    const double P30 = lp[30];
+    const double P31 = lp[31];
+    const double P32 = lp[32];
+    const double P33 = lp[33];
+    const double P34 = lp[34];
+    const double P35 = lp[35];
+    const double P36 = lp[36];
+    const double P37 = lp[37];
+    const double P38 = lp[38];
+    const double P39 = lp[39];
+    const double P80 = PLOM_UNIFORM64(pv[80]);
+    const double P81 = PLOM_UNIFORM64(pv[81]);
+    const double P82 = PLOM_UNIFORM64(pv[82]);
+    const double P83 = PLOM_UNIFORM64(pv[83]);
+    const double P84 = PLOM_UNIFORM64(pv[84]);
+    const double P85 = PLOM_UNIFORM64(pv[85]);
+    const double P86 = PLOM_UNIFORM64(pv[86]);
+    const double P87 = PLOM_UNIFORM64(pv[87]);
+    const double P88 = PLOM_UNIFORM64(pv[88]);
+    const double P89 = PLOM_UNIFORM64(pv[89]);
+    const double P90 = PLOM_UNIFORM64(pv[90]);
+    const double P91 = PLOM_UNIFORM64(pv[91]);
+    const double P92 = PLOM_UNIFORM64(pv[92]);
+    const double P93 = PLOM_UNIFORM64(pv[93]);
+    const double P94 = PLOM_UNIFORM64(pv[94]);
+    const double P95 = PLOM_UNIFORM64(pv[95]);
+    const double P96 = PLOM_UNIFORM64(pv[96]);
+    const double P97 = PLOM_UNIFORM64(pv[97]);
+    const double P98 = PLOM_UNIFORM64(pv[98]);
+    const double P99 = PLOM_UNIFORM64(pv[99]);
+    const double P100 = PLOM_UNIFORM64(pv[100]);
+    const double P101 = PLOM_UNIFORM64(pv[101]);
+    const double P102 = PLOM_UNIFORM64(pv[102]);
+    const double P103 = PLOM_UNIFORM64(pv[103]);
+    const double P104 = PLOM_UNIFORM64(pv[104]);
+    const double P105 = PLOM_UNIFORM64(pv[105]);
+    const double P106 = PLOM_UNIFORM64(pv[106]);
+    const double P107 = PLOM_UNIFORM64(pv[107]);
+    const double P108 = PLOM_UNIFORM64(pv[108]);
+    const double P109 = PLOM_UNIFORM64(pv[109]);
+    const double P110 = PLOM_UNIFORM64(pv[110]);
+    const double P111 = PLOM_UNIFORM64(pv[111]);
+    const double P112 = PLOM_UNIFORM64(pv[112]);
+    const double P113 = PLOM_UNIFORM64(pv[113]);
+    const double P114 = PLOM_UNIFORM64(pv[114]);
+    const double P115 = PLOM_UNIFORM64(pv[115]);
+    const double P116 = PLOM_UNIFORM64(pv[116]);
+    const double P117 = PLOM_UNIFORM64(pv[117]);
+    const double P118 = PLOM_UNIFORM64(pv[118]);
+    const double P119 = PLOM_UNIFORM64(pv[119]);
+    const double P120 = PLOM_UNIFORM64(pv[120]);
+    const double P121 = PLOM_UNIFORM64(pv[121]);
+    const double P122 = PLOM_UNIFORM64(pv[122]);
+    const double P123 = PLOM_UNIFORM64(pv[123]);
+    const double P124 = PLOM_UNIFORM64(pv[124]);
+    const double P125 = PLOM_UNIFORM64(pv[125]);
+    const double P126 = PLOM_UNIFORM64(pv[126]);
+    const double P127 = PLOM_UNIFORM64(pv[127]);
+    const double P128 = PLOM_UNIFORM64(pv[128]);
+    const double P129 = PLOM_UNIFORM64(pv[129]);
+    const double P130 = PLOM_UNIFORM64(pv[130]);
+    const double P131 = PLOM_UNIFORM64(pv[131]);
+    const double P132 = PLOM_UNIFORM64(pv[132]);
+    const double P133 = PLOM_UNIFORM64(pv[133]);
+    const double P134 = PLOM_UNIFORM64(pv[134]);
+    const double P135 = PLOM_UNIFORM64(pv[135]);
+    const double P136 = PLOM_UNIFORM64(pv[136]);
+    const double P137 = PLOM_UNIFORM64(pv[137]);
+    const double P138 = PLOM_UNIFORM64(pv[138]);
+    const double P139 = PLOM_UNIFORM64(pv[139]);
+    const double P140 = PLOM_UNIFORM64(pv[140]);
+    const double P141 = PLOM_UNIFORM64(pv[141]);
+    const double P142 = PLOM_UNIFORM64(pv[142]);
+    const double P143 = PLOM_UNIFORM64(pv[143]);
+    const double P144 = PLOM_UNIFORM64(pv[144]);
+    const double P145 = PLOM_UNIFORM64(pv[145]);
+    const double P146 = PLOM_UNIFORM64(pv[146]);
+    const double P147 = PLOM_UNIFORM64(pv[147]);
+    const double P148 = PLOM_UNIFORM64(pv[148]);
+    const double P149 = PLOM_UNIFORM64(pv[149]);
+    const double P150 = PLOM_UNIFORM64(pv[150]);
+    const double P151 = PLOM_UNIFORM64(pv[151]);
+    const double P152 = PLOM_UNIFORM64(pv[152]);
+    const double P153 = PLOM_UNIFORM64(pv[153]);
+    const double P154 = PLOM_UNIFORM64(pv[154]);
+    const double P155 = PLOM_UNIFORM64(pv[155]);
+    const double P156 = PLOM_UNIFORM64(pv[156]);
+    const double P157 = PLOM_UNIFORM64(pv[157]);
+    const double P158 = PLOM_UNIFORM64(pv[158]);
+    const double P159 = PLOM_UNIFORM64(pv[159]);
+    const double P160 = PLOM_UNIFORM64(pv[160]);
+    const double P161 = PLOM_UNIFORM64(pv[161]);
+    const double P162 = PLOM_UNIFORM64(pv[162]);
+    const double P163 = PLOM_UNIFORM64(pv[163]);
+    const double P164 = PLOM_UNIFORM64(pv[164]);
+    const double P165 = PLOM_UNIFORM64(pv[165]);
+    const double P166 = PLOM_UNIFORM64(pv[166]);
+    const double P167 = PLOM_UNIFORM64(pv[167]);
+    const double P168 = PLOM_UNIFORM64(pv[168]);
+    const double P169 = PLOM_UNIFORM64(pv[169]);
+    const double P170 = PLOM_UNIFORM64(pv[170]);
+    const double P171 = PLOM_UNIFORM64(pv[171]);
+    const double P172 = PLOM_UNIFORM64(pv[172]);
+    const double P173 = PLOM_UNIFORM64(pv[173]);
+    const double P174 = PLOM_UNIFORM64(pv[174]);
+    const double P175 = PLOM_UNIFORM64(pv[175]);
+    const double P176 = PLOM_UNIFORM64(pv[176]);
+    const double P177 = PLOM_UNIFORM64(pv[177]);
+    const double P178 = PLOM_UNIFORM64(pv[178]);
+    const double P179 = PLOM_UNIFORM64(pv[179]);
+    const double C0 = plom_cov_ld(cv + 0);
+    const double C1 = plom_cov_ld(cv + 1);
+    const double C2 = plom_cov_ld(cv + 2);
+    const double C3 = plom_cov_ld(cv + 3);
+    const double C4 = plom_cov_ld(cv + 4);
+    const double C5 = plom_cov_ld(cv + 5);
+    const double C6 = plom_cov_ld(cv + 6);
+    const double C7 = plom_cov_ld(cv + 7);
+    const double C8 = plom_cov_ld(cv + 8);
+    const double C9 = plom_cov_ld(cv + 9);
+    const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
+    const double r1_0 = P10;
+    const double r2_0 = P20;
+    const double r3_0 = (P30 * (((C0 - X[0]) - X[10]) - X[20]));
+    const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
+    const double r1_1 = P11;
+    const double r2_1 = P21;
+    const double r3_1 = (P31 * (((C1 - X[1]) - X[11]) - X[21]));
+    const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
+    const double r1_2 = P12;
+    const double r2_2 = P22;
+    const double r3_2 = (P32 * (((C2 - X[2]) - X[12]) - X[22]));
+    const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
+    const double r1_3 = P13;
+    const double r2_3 = P23;
+    const double r3_3 = (P33 * (((C3 - X[3]) - X[13]) - X[23]));
+    const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
+    const double r1_4 = P14;
+    const double r2_4 = P24;
+    const double r3_4 = (P34 * (((C4 - X[4]) - X[14]) - X[24]));
+    const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
+    const double r1_5 = P15;
+    const double r2_5 = P25;
+    const double r3_5 = (P35 * (((C5 - X[5]) - X[15]) - X[25]));
+    const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
+    const double r1_6 = P16;
+    const double r2_6 = P26;
+    const double r3_6 = (P36 * (((C6 - X[6]) - X[16]) - X[26]));
+    const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
+    const double r1_7 = P17;
+    const double r2_7 = P27;
+    const double r3_7 = (P37 * (((C7 - X[7]) - X[17]) - X[27]));
+    const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
+    const double r1_8 = P18;
+    const double r2_8 = P28;
+    const double r3_8 = (P38 * (((C8 - X[8]) - X[18]) - X[28]));
+    const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
+    const double r1_9 = P19;
+    const double r2_9 = P29;
+    const double r3_9 = (P39 * (((C9 - X[9]) - X[19]) - X[29]));
+    f[0] =  - (r0_0*X[0]) + (r3_0);
+    f[10] =  - (r1_0*X[10]) + (r0_0*X[0]);
+    f[20] =  - (r2_0*X[20]) + (r1_0*X[10]);
+    f[1] =  - (r0_1*X[1]) + (r3_1);
+    f[11] =  - (r1_1*X[11]) + (r0_1*X[1]);
+    f[21] =  - (r2_1*X[21]) + (r1_1*X[11]);
+    f[2] =  - (r0_2*X[2]) + (r3_2);
+    f[12] =  - (r1_2*X[12]) + (r0_2*X[2]);
+    f[22] =  - (r2_2*X[22]) + (r1_2*X[12]);
+    f[3] =  - (r0_3*X[3]) + (r3_3);
+    f[13] =  - (r1_3*X[13]) + (r0_3*X[3]);
+    f[23] =  - (r2_3*X[23]) + (r1_3*X[13]);
+    f[4] =  - (r0_4*X[4]) + (r3_4);
+    f[14] =  - (r1_4*X[14]) + (r0_4*X[4]);
+    f[24] =  - (r2_4*X[24]) + (r1_4*X[14]);
+    f[5] =  - (r0_5*X[5]) + (r3_5);
+    f[15] =  - (r1_5*X[15]) + (r0_5*X[5]);
+    f[25] =  - (r2_5*X[25]) + (r1_5*X[15]);
+    f[6] =  - (r0_6*X[6]) + (r3_6);
+    f[16] =  - (r1_6*X[16]) + (r0_6*X[6]);
+    f[26] =  - (r2_6*X[26]) + (r1_6*X[16]);
+    f[7] =  - (r0_7*X[7]) + (r3_7);
+    f[17] =  - (r1_7*X[17]) + (r0_7*X[7]);
+    f[27] =  - (r2_7*X[27]) + (r1_7*X[17]);
+    f[8] =  - (r0_8*X[8]) + (r3_8);
+    f[18] =  - (r1_8*X[18]) + (r0_8*X[8]);
+    f[28] =  - (r2_8*X[28]) + (r1_8*X[18]);
+    f[9] =  - (r0_9*X[9]) + (r3_9);
+    f[19] =  - (r1_9*X[19]) + (r0_9*X[9]);
+    f[29] =  - (r2_9*X[29]) + (r1_9*X[19]);
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_0*X[10]);
+        f[30] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_1*X[11]);
+        f[31] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_2*X[12]);
+        f[32] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_3*X[13]);
+        f[33] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_4*X[14]);
+        f[34] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_5*X[15]);
+        f[35] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_6*X[16]);
+        f[36] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_7*X[17]);
+        f[37] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_8*X[18]);
+        f[38] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r1_9*X[19]);
+        f[39] = sum_inc;
+    }
+}
+
+#define PLOM_ODE_FN plom_seir10_ode_integrate_pp
+#define PLOM_ODE_RHS plom_seir10_ode_rhs_pp
+#define PLOM_ODE_N 40
+#define PLOM_ODE_QUAL static __device__ inline __attribute__((always_inline))
+#include "plom_ode_impl.h"
+
+static __device__ inline __attribute__((always_inline)) int plom_seir10_interval_ode_pp_f64(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)
+{
+    X[30] = 0.0; /* reset_inc */
+    X[31] = 0.0; /* reset_inc */
+    X[32] = 0.0; /* reset_inc */
+    X[33] = 0.0; /* reset_inc */
+    X[34] = 0.0; /* reset_inc */
+    X[35] = 0.0; /* reset_inc */
+    X[36] = 0.0; /* reset_inc */
+    X[37] = 0.0; /* reset_inc */
+    X[38] = 0.0; /* reset_inc */
+    X[39] = 0.0; /* reset_inc */
+    return plom_seir10_ode_integrate_pp(X, pv, lp, cv, t0, t1, dt, eps_abs, eps_rel);
+}
+
 static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
 {
     const double dtr = (double)dt;
@@ -9551,6 +10149,8 @@ struct plom_model_seir10 {
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_seir10_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_seir10_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
+    static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_seir10_interval_ode_f64(X, pv, cv, t0, t1, dt, ea, er); }
+    static __device__ inline __attribute__((always_inline)) int interval_ode_pp(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_seir10_interval_ode_pp_f64(X, pv, lp, cv, t0, t1, dt, ea, er); }
     static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor) { return plom_seir10_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor) { return plom_seir10_loglik_pp(obs, y, pvd, lpd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out) { plom_seir10_hat_values(X, pvd, cvd, t, out); }

@@ -290,8 +290,42 @@ PLOM_HD void plom_sir_interval_psr_no_env_sto_f64(double *X, const double *pv, p
     }
 }
 
-PLOM_HD void plom_sir_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs)
+PLOM_HD void plom_sir_ode_rhs(double t, const double *X, const double *pv, const double *lp, const double *cv, double *f)
 {
+    (void)pv; (void)lp; (void)cv;
+    const double tr = t; (void)tr;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double C0 = plom_cov_ld(cv + 0);
+    const double r0_0 = ((P0 * X[1]) / C0);
+    const double r1_0 = P1;
+    f[0] =  - (r0_0*X[0]);
+    f[1] =  - (r1_0*X[1]) + (r0_0*X[0]);
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r0_0*X[0]);
+        f[2] = sum_inc;
+    }
+}
+
+#define PLOM_ODE_FN plom_sir_ode_integrate
+#define PLOM_ODE_RHS plom_sir_ode_rhs
+#define PLOM_ODE_N 3
+#define PLOM_ODE_QUAL PLOM_HD
+#include "plom_ode_impl.h"
+
+PLOM_HD int plom_sir_interval_ode_f64(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)
+{
+    X[2] = 0.0; /* reset_inc */
+    return plom_sir_ode_integrate(X, pv, 0, cv, t0, t1, dt, eps_abs, eps_rel);
+}
+
+PLOM_HD void plom_sir_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel)
+{
+    if (noises_off & PLOM_IMPL_ODE) {
+        (void)plom_sir_interval_ode_f64(X, pv, cv, t0, t1, dt, eps_abs, eps_rel);
+        return;
+    }
     if (noises_off & PLOM_IMPL_PSR) {
         if (noises_off & PLOM_NO_ENV_STO) plom_sir_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
         else plom_sir_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
@@ -537,6 +571,36 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_p
         X[2] = X[2] + (n0_0);
         t += dt;
     }
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sir_ode_rhs_pp(double t, const double *X, const double *pv, const double *lp, const double *cv, double *f)
+{
+    (void)pv; (void)lp; (void)cv;
+    const double tr = t; (void)tr;
+    const double P0 = lp[0];
+    const double P1 = lp[1];
+    const double C0 = plom_cov_ld(cv + 0);
+    const double r0_0 = ((P0 * X[1]) / C0);
+    const double r1_0 = P1;
+    f[0] =  - (r0_0*X[0]);
+    f[1] =  - (r1_0*X[1]) + (r0_0*X[0]);
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r0_0*X[0]);
+        f[2] = sum_inc;
+    }
+}
+
+#define PLOM_ODE_FN plom_sir_ode_integrate_pp
+#define PLOM_ODE_RHS plom_sir_ode_rhs_pp
+#define PLOM_ODE_N 3
+#define PLOM_ODE_QUAL static __device__ inline __attribute__((always_inline))
+#include "plom_ode_impl.h"
+
+static __device__ inline __attribute__((always_inline)) int plom_sir_interval_ode_pp_f64(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)
+{
+    X[2] = 0.0; /* reset_inc */
+    return plom_sir_ode_integrate_pp(X, pv, lp, cv, t0, t1, dt, eps_abs, eps_rel);
 }
 
 static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
@@ -839,6 +903,8 @@ struct plom_model_sir {
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_sir_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sir_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
+    static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_sir_interval_ode_f64(X, pv, cv, t0, t1, dt, ea, er); }
+    static __device__ inline __attribute__((always_inline)) int interval_ode_pp(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_sir_interval_ode_pp_f64(X, pv, lp, cv, t0, t1, dt, ea, er); }
     static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor) { return plom_sir_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor) { return plom_sir_loglik_pp(obs, y, pvd, lpd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out) { plom_sir_hat_values(X, pvd, cvd, t, out); }

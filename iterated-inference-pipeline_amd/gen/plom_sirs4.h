@@ -1044,8 +1044,110 @@ PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv,
     }
 }
 
-PLOM_HD void plom_sirs4_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs)
+PLOM_HD void plom_sirs4_ode_rhs(double t, const double *X, const double *pv, const double *lp, const double *cv, double *f)
 {
+    (void)pv; (void)lp; (void)cv;
+    const double tr = t; (void)tr;
+    const double P0 = PLOM_UNIFORM64(pv[0]);
+    const double P1 = PLOM_UNIFORM64(pv[1]);
+    const double P2 = PLOM_UNIFORM64(pv[2]);
+    const double P3 = PLOM_UNIFORM64(pv[3]);
+    const double P4 = PLOM_UNIFORM64(pv[4]);
+    const double P5 = PLOM_UNIFORM64(pv[5]);
+    const double P6 = PLOM_UNIFORM64(pv[6]);
+    const double P7 = PLOM_UNIFORM64(pv[7]);
+    const double P8 = PLOM_UNIFORM64(pv[8]);
+    const double P9 = PLOM_UNIFORM64(pv[9]);
+    const double P10 = PLOM_UNIFORM64(pv[10]);
+    const double P11 = PLOM_UNIFORM64(pv[11]);
+    const double P12 = PLOM_UNIFORM64(pv[12]);
+    const double P13 = PLOM_UNIFORM64(pv[13]);
+    const double P14 = PLOM_UNIFORM64(pv[14]);
+    const double P15 = PLOM_UNIFORM64(pv[15]);
+    const double P32 = PLOM_UNIFORM64(pv[32]);
+    const double P33 = PLOM_UNIFORM64(pv[33]);
+    const double P34 = PLOM_UNIFORM64(pv[34]);
+    const double P35 = PLOM_UNIFORM64(pv[35]);
+    const double P36 = PLOM_UNIFORM64(pv[36]);
+    const double P37 = PLOM_UNIFORM64(pv[37]);
+    const double P38 = PLOM_UNIFORM64(pv[38]);
+    const double P39 = PLOM_UNIFORM64(pv[39]);
+    const double P40 = PLOM_UNIFORM64(pv[40]);
+    const double P41 = PLOM_UNIFORM64(pv[41]);
+    const double P42 = PLOM_UNIFORM64(pv[42]);
+    const double P43 = PLOM_UNIFORM64(pv[43]);
+    const double P44 = PLOM_UNIFORM64(pv[44]);
+    const double P45 = PLOM_UNIFORM64(pv[45]);
+    const double P46 = PLOM_UNIFORM64(pv[46]);
+    const double P47 = PLOM_UNIFORM64(pv[47]);
+    const double C0 = plom_cov_ld(cv + 0);
+    const double C1 = plom_cov_ld(cv + 1);
+    const double C2 = plom_cov_ld(cv + 2);
+    const double C3 = plom_cov_ld(cv + 3);
+    const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+    const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
+    const double r1_0 = P8;
+    const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
+    const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
+    const double r1_1 = P9;
+    const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
+    const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
+    const double r1_2 = P10;
+    const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
+    const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
+    const double r1_3 = P11;
+    const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
+    f[0] =  - (r0_0*X[0]) + (r2_0);
+    f[4] =  - (r1_0*X[4]) + (r0_0*X[0]);
+    f[1] =  - (r0_1*X[1]) + (r2_1);
+    f[5] =  - (r1_1*X[5]) + (r0_1*X[1]);
+    f[2] =  - (r0_2*X[2]) + (r2_2);
+    f[6] =  - (r1_2*X[6]) + (r0_2*X[2]);
+    f[3] =  - (r0_3*X[3]) + (r2_3);
+    f[7] =  - (r1_3*X[7]) + (r0_3*X[3]);
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r0_0*X[0]);
+        f[8] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r0_1*X[1]);
+        f[9] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r0_2*X[2]);
+        f[10] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r0_3*X[3]);
+        f[11] = sum_inc;
+    }
+}
+
+#define PLOM_ODE_FN plom_sirs4_ode_integrate
+#define PLOM_ODE_RHS plom_sirs4_ode_rhs
+#define PLOM_ODE_N 12
+#define PLOM_ODE_QUAL PLOM_HD
+#include "plom_ode_impl.h"
+
+PLOM_HD int plom_sirs4_interval_ode_f64(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)
+{
+    X[8] = 0.0; /* reset_inc */
+    X[9] = 0.0; /* reset_inc */
+    X[10] = 0.0; /* reset_inc */
+    X[11] = 0.0; /* reset_inc */
+    return plom_sirs4_ode_integrate(X, pv, 0, cv, t0, t1, dt, eps_abs, eps_rel);
+}
+
+PLOM_HD void plom_sirs4_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel)
+{
+    if (noises_off & PLOM_IMPL_ODE) {
+        (void)plom_sirs4_interval_ode_f64(X, pv, cv, t0, t1, dt, eps_abs, eps_rel);
+        return;
+    }
     if (noises_off & PLOM_IMPL_PSR) {
         if (noises_off & PLOM_NO_ENV_STO) plom_sirs4_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
         else plom_sirs4_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
@@ -1994,6 +2096,104 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         X[11] = X[11] + (n0_3);
         t += dt;
     }
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_ode_rhs_pp(double t, const double *X, const double *pv, const double *lp, const double *cv, double *f)
+{
+    (void)pv; (void)lp; (void)cv;
+    const double tr = t; (void)tr;
+    const double P0 = lp[0];
+    const double P1 = lp[1];
+    const double P2 = lp[2];
+    const double P3 = lp[3];
+    const double P4 = lp[4];
+    const double P5 = lp[5];
+    const double P6 = lp[6];
+    const double P7 = lp[7];
+    const double P8 = lp[8];
+    const double P9 = lp[9];
+    const double P10 = lp[10];
+    const double P11 = lp[11];
+    const double P12 = lp[12];
+    const double P13 = lp[13];
+    const double P14 = lp[14];
+    const double P15 = lp[15];
+    const double P32 = PLOM_UNIFORM64(pv[32]);
+    const double P33 = PLOM_UNIFORM64(pv[33]);
+    const double P34 = PLOM_UNIFORM64(pv[34]);
+    const double P35 = PLOM_UNIFORM64(pv[35]);
+    const double P36 = PLOM_UNIFORM64(pv[36]);
+    const double P37 = PLOM_UNIFORM64(pv[37]);
+    const double P38 = PLOM_UNIFORM64(pv[38]);
+    const double P39 = PLOM_UNIFORM64(pv[39]);
+    const double P40 = PLOM_UNIFORM64(pv[40]);
+    const double P41 = PLOM_UNIFORM64(pv[41]);
+    const double P42 = PLOM_UNIFORM64(pv[42]);
+    const double P43 = PLOM_UNIFORM64(pv[43]);
+    const double P44 = PLOM_UNIFORM64(pv[44]);
+    const double P45 = PLOM_UNIFORM64(pv[45]);
+    const double P46 = PLOM_UNIFORM64(pv[46]);
+    const double P47 = PLOM_UNIFORM64(pv[47]);
+    const double C0 = plom_cov_ld(cv + 0);
+    const double C1 = plom_cov_ld(cv + 1);
+    const double C2 = plom_cov_ld(cv + 2);
+    const double C3 = plom_cov_ld(cv + 3);
+    const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+    const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
+    const double r1_0 = P8;
+    const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
+    const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
+    const double r1_1 = P9;
+    const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
+    const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
+    const double r1_2 = P10;
+    const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
+    const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
+    const double r1_3 = P11;
+    const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
+    f[0] =  - (r0_0*X[0]) + (r2_0);
+    f[4] =  - (r1_0*X[4]) + (r0_0*X[0]);
+    f[1] =  - (r0_1*X[1]) + (r2_1);
+    f[5] =  - (r1_1*X[5]) + (r0_1*X[1]);
+    f[2] =  - (r0_2*X[2]) + (r2_2);
+    f[6] =  - (r1_2*X[6]) + (r0_2*X[2]);
+    f[3] =  - (r0_3*X[3]) + (r2_3);
+    f[7] =  - (r1_3*X[7]) + (r0_3*X[3]);
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r0_0*X[0]);
+        f[8] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r0_1*X[1]);
+        f[9] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r0_2*X[2]);
+        f[10] = sum_inc;
+    }
+    {
+        double sum_inc = 0.0;
+        sum_inc +=  + (r0_3*X[3]);
+        f[11] = sum_inc;
+    }
+}
+
+#define PLOM_ODE_FN plom_sirs4_ode_integrate_pp
+#define PLOM_ODE_RHS plom_sirs4_ode_rhs_pp
+#define PLOM_ODE_N 12
+#define PLOM_ODE_QUAL static __device__ inline __attribute__((always_inline))
+#include "plom_ode_impl.h"
+
+static __device__ inline __attribute__((always_inline)) int plom_sirs4_interval_ode_pp_f64(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)
+{
+    X[8] = 0.0; /* reset_inc */
+    X[9] = 0.0; /* reset_inc */
+    X[10] = 0.0; /* reset_inc */
+    X[11] = 0.0; /* reset_inc */
+    return plom_sirs4_ode_integrate_pp(X, pv, lp, cv, t0, t1, dt, eps_abs, eps_rel);
 }
 
 static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
@@ -3025,6 +3225,8 @@ struct plom_model_sirs4 {
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_sirs4_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sirs4_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
+    static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_sirs4_interval_ode_f64(X, pv, cv, t0, t1, dt, ea, er); }
+    static __device__ inline __attribute__((always_inline)) int interval_ode_pp(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_sirs4_interval_ode_pp_f64(X, pv, lp, cv, t0, t1, dt, ea, er); }
     static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor) { return plom_sirs4_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor) { return plom_sirs4_loglik_pp(obs, y, pvd, lpd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out) { plom_sirs4_hat_values(X, pvd, cvd, t, out); }

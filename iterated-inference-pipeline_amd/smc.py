@@ -22,6 +22,7 @@ from .codegen import load_model, BUILTIN_MODELS, context_data, load_json
 PLOM_FP64, PLOM_FP32 = 0, 1
 NO_DEM_STO, NO_ENV_STO, NO_DRIFT = 1, 2, 4
 FLAG_NO_FILTER, FLAG_STORE_ANCESTRY, FLAG_FINAL_STATE, FLAG_LINEAGE, FLAG_HAT, FLAG_MIF, FLAG_PSR = 1, 2, 4, 8, 16, 32, 64
+FLAG_ODE = 128
 MAX_J = 1 << 21
 
 
@@ -36,7 +37,7 @@ class FilterDesc(C.Structure):
     _fields_ = [("model_id", C.c_int32), ("J", C.c_int32), ("N", C.c_int32), ("precision", C.c_int32),
                 ("noises_off", C.c_uint32), ("flags", C.c_uint32), ("seed", C.c_uint32),
                 ("stream", C.c_uint32), ("dt", C.c_double), ("like_min", C.c_double), ("n_theta", C.c_int32),
-                ("pad", C.c_int32)]
+                ("pad", C.c_int32), ("eps_abs", C.c_double), ("eps_rel", C.c_double)]
 
 
 class MifMap(C.Structure):
@@ -355,8 +356,9 @@ class FilterResult:
         self.lineage_slots = lineage_slots
 
 
-def _desc(model, J, N, precision, noises_off, flags, seed, stream, dt, like_min):
+def _desc(model, J, N, precision, noises_off, flags, seed, stream, dt, like_min, eps_abs=0.0, eps_rel=0.0):
     d = FilterDesc()
+    d.eps_abs, d.eps_rel = float(eps_abs), float(eps_rel)
     d.model_id = model.id
     d.J, d.N = int(J), int(N)
     d.precision = PLOM_FP32 if precision in ("fp32", PLOM_FP32) else PLOM_FP64
@@ -386,7 +388,7 @@ class Filter:
     """A device-resident filter (plom_filter_create / run / read)."""
 
     def __init__(self, ctx, model, J, times, data, pv, x0, precision="fp64", noises_off=0, flags=0,
-                 seed=1, stream=0, dt=0.25, like_min=1e-17, mif=None, cov=None):
+                 seed=1, stream=0, dt=0.25, like_min=1e-17, mif=None, cov=None, eps_abs=0.0, eps_rel=0.0):
         """mif (with FLAG_MIF): dict of numpy arrays pv_src [n_pp], tr, scale, duration, min, max,
         multiplier, sd [n_theta] (plom_mif_map).  cov: time-varying covariates, one (x, y) knot
         pair per covariate slot of pv (plom_covariate; model.n_fix of them)."""
@@ -399,7 +401,7 @@ class Filter:
         self.x0 = np.ascontiguousarray(x0, dtype=np.float64)
         if self.pv.size != model.layout["size"] or self.x0.size != model.S:
             raise ValueError("pv / x0 size mismatch for model %s" % model.name)
-        self.desc = _desc(model, J, self.N, precision, noises_off, flags, seed, stream, dt, like_min)
+        self.desc = _desc(model, J, self.N, precision, noises_off, flags, seed, stream, dt, like_min, eps_abs, eps_rel)
         self.n_theta = 0
         mmap = None
         if mif is not None:

@@ -27,9 +27,10 @@ def lib():
         ip = C.POINTER(C.c_int32)
         L.oracle_smc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                  C.c_double, C.c_double, dp, dp, dp, dp, C.c_int, C.c_int,
-                                 dp, ip, dp, dp, ip, dp, dp, dp, ip, dp, dp, dp, dp, dp, ip, dp, dp]
+                                 dp, ip, dp, dp, ip, dp, dp, dp, ip, dp, dp, dp, dp, dp, ip, dp, dp, C.c_double, C.c_double]
         L.oracle_trajectory.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-                                        C.c_double, dp, dp, dp, dp, ip, dp, dp]
+                                        C.c_double, dp, dp, dp, dp, ip, dp, dp, C.c_double, C.c_double]
+        L.oracle_ode_test.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, ip]
         L.oracle_cov_table.argtypes = [C.c_int, C.c_int, dp, C.c_double, ip, dp, dp, dp, ip]
         L.oracle_spline.argtypes = [C.c_int, dp, dp, C.c_int, dp, dp]
         L.oracle_obs_loglik.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, dp, ip]
@@ -46,7 +47,8 @@ def lib():
         i32p = C.POINTER(C.c_int32)
         L.oracle_mif_pass.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double,
                                       C.c_double, dp, dp, dp, C.c_int, i32p, i32p, i32p, i32p, dp, dp, dp, dp, dp,
-                                      dp, dp, C.c_int, dp, i32p, dp, dp, dp, dp, dp, dp, i32p, i32p, dp, dp]
+                                      dp, dp, C.c_int, dp, i32p, dp, dp, dp, dp, dp, dp, i32p, i32p, dp, dp, C.c_double,
+                                      C.c_double]
         _lib = L
     return _lib
 
@@ -60,6 +62,28 @@ def _ip(a):
 
 
 IMPL_PSR = 0x100   # the psr implementation (PLOM_IMPL_PSR of csrc/plom_model_common.h)
+IMPL_ODE = 0x200   # the ode implementation (PLOM_IMPL_ODE)
+
+
+def _impl(noises_off, psr, ode):
+    return noises_off | (IMPL_PSR if psr else 0) | (IMPL_ODE if ode else 0)
+
+
+def _eps(eps_abs, eps_rel):
+    """Both 0: the reference's defaults (1e-6, 1e-3), as the library takes them."""
+    return (eps_abs, eps_rel) if (eps_abs > 0 or eps_rel > 0) else (1e-6, 1e-3)
+
+
+def ode_test(kind, p, y, t0, t1, h, eps_abs, eps_rel):
+    """The RKF45 integrator of csrc/plom_ode_impl.h on an analytic system (see oracle_ode_test):
+    (y at t1, right-hand side evaluations, status)."""
+    pp = np.zeros(5)
+    pp[:len(p)] = p
+    yy = np.zeros(2)
+    yy[:len(y)] = y
+    n = C.c_int32()
+    rc = lib().oracle_ode_test(kind, _dp(pp), _dp(yy), t0, t1, h, eps_abs, eps_rel, C.byref(n))
+    return yy[:len(y)], n.value, rc
 
 
 def _cov(cov):
@@ -110,13 +134,13 @@ def spline(x, y, t):
 
 def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=0, dt=0.25,
         like_min=1e-17, mode=CANONICAL, nthreads=1, ancestry=False, final_state=False, logw=False,
-        lineage=False, all_states=False, hat=False, psr=False, cov=None):
+        lineage=False, all_states=False, hat=False, psr=False, cov=None, ode=False, eps_abs=0.0, eps_rel=0.0):
     """The filter restated on the CPU.  lineage: the sampled path (slots [N], states
     [N+1][S]); all_states: every particle's state after every interval, before resampling
     ([N+1][J][S], for tests on small runs); psr: the Euler-multinomial implementation;
-    cov: time-varying covariates, one (x, y) knot pair per covariate slot of pv."""
-    if psr:
-        noises_off = noises_off | IMPL_PSR
+    cov: time-varying covariates, one (x, y) knot pair per covariate slot of pv; ode: the
+    adaptive RKF45 implementation under eps_abs / eps_rel."""
+    noises_off = _impl(noises_off, psr, ode)
     times = np.ascontiguousarray(times, dtype=np.float64)
     N = len(times) - 1
     data = np.ascontiguousarray(data, dtype=np.float64)
@@ -142,7 +166,7 @@ def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=
     rc = lib().oracle_smc(model_id, J, N, noises_off, flags, seed, stream, dt, like_min, _dp(times), _dp(data),
                           _dp(pv), _dp(x0), mode, nthreads, C.byref(ll), C.byref(naf), _dp(ll_n), _dp(ess),
                           _ip(anc), _dp(fs), _dp(lw), C.byref(el), _ip(lsl), _dp(lst), _dp(alls), _dp(ht), _dp(pr), _dp(lob),
-                          *_cov_args(cov))
+                          *_cov_args(cov), *_eps(eps_abs, eps_rel))
     if rc != 0:
         raise RuntimeError("oracle_smc failed: %d" % rc)
     return {"loglik": ll.value, "n_all_fail": naf.value, "loglik_n": ll_n, "ess_n": ess,
@@ -151,12 +175,12 @@ def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=
 
 
 def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=0, seed=1, stream=0, dt=0.25,
-             like_min=1e-17, nthreads=1, theta_out=False, logw=False, ancestry=False, psr=False, cov=None):
+             like_min=1e-17, nthreads=1, theta_out=False, logw=False, ancestry=False, psr=False, cov=None, ode=False,
+             eps_abs=0.0, eps_rel=0.0):
     """One pass of iterated filtering restated on the CPU (oracle_mif_pass).  mif: dict of
     pv_src [n_pp], tr, scale, duration, min, max, multiplier, sd [n_theta]; theta0
     [n_theta, J]; x0 [S, J]; mut_fac [N]."""
-    if psr:
-        noises_off = noises_off | IMPL_PSR
+    noises_off = _impl(noises_off, psr, ode)
     times = np.ascontiguousarray(times, dtype=np.float64)
     N = len(times) - 1
     i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
@@ -176,7 +200,7 @@ def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=
                                nth, _ip(m["pv_src"]), _ip(m["tr"]), _ip(m["scale"]), _ip(m["duration"]), _dp(m["min"]),
                                _dp(m["max"]), _dp(m["multiplier"]), _dp(m["sd"]), _dp(mut_fac), _dp(theta0), _dp(x0),
                                nthreads, C.byref(ll), C.byref(naf), _dp(ll_n), _dp(ess), _dp(mm), _dp(mv), _dp(tho),
-                               _dp(lw), _ip(anc), *_cov_args(cov))
+                               _dp(lw), _ip(anc), *_cov_args(cov), *_eps(eps_abs, eps_rel))
     if rc != 0:
         raise RuntimeError("oracle_mif_pass failed: %d" % rc)
     return {"loglik": ll.value, "n_all_fail": naf.value, "loglik_n": ll_n, "ess_n": ess, "mif_mean": mm,
@@ -200,16 +224,17 @@ def mif_natural(tr, sc, dur, x, a, b, mult):
     return lib().oracle_mif_natural(int(tr), int(sc), int(dur), float(x), float(a), float(b), float(mult))
 
 
-def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=0.25, psr=False, cov=None):
-    if psr:
-        noises_off = noises_off | IMPL_PSR
+def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=0.25, psr=False, cov=None, ode=False,
+               eps_abs=0.0, eps_rel=0.0):
+    noises_off = _impl(noises_off, psr, ode)
     times = np.ascontiguousarray(times, dtype=np.float64)
     N = len(times) - 1
     S = lib().oracle_model_size_proj(model_id)
     out = np.zeros((N + 1, S))
     rc = lib().oracle_trajectory(model_id, N, noises_off, seed, stream, j, dt, _dp(times),
                                  _dp(np.ascontiguousarray(pv, dtype=np.float64)),
-                                 _dp(np.ascontiguousarray(x0, dtype=np.float64)), _dp(out), *_cov_args(cov))
+                                 _dp(np.ascontiguousarray(x0, dtype=np.float64)), _dp(out), *_cov_args(cov),
+                                 *_eps(eps_abs, eps_rel))
     if rc != 0:
         raise RuntimeError("oracle_trajectory failed")
     return out

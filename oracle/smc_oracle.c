@@ -56,7 +56,7 @@
 typedef unsigned __int128 u128;
 
 typedef void (*interval_fn)(double *, const double *, plom_nstream64 *, double, double, double, double, unsigned int,
-                            const double *, int);
+                            const double *, int, double, double);
 typedef void (*proj2obs_fn)(const double *, const double *, const double *, double *);
 typedef double (*loglik_fn)(const double *, const double *, const double *, const double *, double, double, int *);
 typedef void (*moments_fn)(const double *, const double *, const double *, double, double *, double *);
@@ -219,7 +219,7 @@ typedef struct {
     const cov_view *cv;
     int J, n, nonan, filter, n_ts;
     uint32_t seed, stream, noises_off;
-    double dt, sqdt, like_min;
+    double dt, sqdt, like_min, eps_abs, eps_rel;
     int j0, j1, mode;
 } prop_job;
 
@@ -234,7 +234,7 @@ static void propagate_range(prop_job *p)
         plom_nstream64 ns;
         plom_nstream64_init(&ns, p->seed, p->stream, (uint32_t)p->n, (uint32_t)j, plom_bm_tab);
         M->interval(x, pv, &ns, p->times[p->n], p->times[p->n + 1], p->dt, p->sqdt, p->noises_off, cv_steps(p->cv, p->n),
-                    p->cv->cs);
+                    p->cv->cs, p->eps_abs, p->eps_rel);
         if (p->filter && p->nonan) {
             int nf = 0;
             const double *cvd = cv_at(p->cv, p->n + 1);
@@ -493,7 +493,8 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
                double *loglik, int32_t *n_all_fail, double *loglik_n, double *ess_n, int32_t *ancestry,
                double *final_state, double *logw_out, double *elapsed_s,
                int32_t *lin_slots, double *lin_state, double *all_states, double *hat, double *pred_res,
-               double *lin_obs, const int32_t *cov_n, const double *cov_x, const double *cov_y)
+               double *lin_obs, const int32_t *cov_n, const double *cov_x, const double *cov_y, double eps_abs,
+               double eps_rel)
 {
     if (model < 0 || model >= N_MODELS || J < 1 || N < 0) return -1;
     const oracle_model *M = &MODELS[model];
@@ -546,6 +547,7 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
         job.J = J; job.n = n; job.nonan = nonan; job.filter = filter; job.n_ts = n_ts;
         job.seed = seed; job.stream = stream; job.noises_off = noises_off;
         job.dt = dt; job.sqdt = plom_sqrt(dt); job.like_min = like_min; job.mode = mode;
+        job.eps_abs = eps_abs; job.eps_rel = eps_rel;
         propagate_all(&job, nthreads);
         if (all_states) memcpy(all_states + (size_t)(n + 1) * J * S, X, (size_t)J * S * sizeof(double));
         double ln = 0.0, es = 0.0;
@@ -619,7 +621,8 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
         for (int n = 0; n < N; n++) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, seed, stream, (uint32_t)n, (uint32_t)lin_slots[n], plom_bm_tab);
-            M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off, cv_steps(&cv, n), cv.cs);
+            M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off, cv_steps(&cv, n), cv.cs, eps_abs,
+                        eps_rel);
             memcpy(lin_state + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
         }
         if (lin_obs)
@@ -661,7 +664,7 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
                     const double *theta0, const double *x0, int nthreads, double *loglik, int32_t *n_all_fail,
                     double *loglik_n, double *ess_n, double *mif_mean, double *mif_var, double *theta_out,
                     double *logw_out, int32_t *ancestry, const int32_t *cov_n, const double *cov_x,
-                    const double *cov_y)
+                    const double *cov_y, double eps_abs, double eps_rel)
 {
     if (model < 0 || model >= N_MODELS || J < 1 || N < 0 || n_theta < 1) return -1;
     const oracle_model *M = &MODELS[model];
@@ -725,6 +728,7 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
         job.data = data; job.J = J; job.n = n; job.nonan = nonan; job.filter = 1; job.n_ts = n_ts;
         job.seed = seed; job.stream = stream; job.noises_off = noises_off;
         job.dt = dt; job.sqdt = plom_sqrt(dt); job.like_min = like_min; job.mode = ORACLE_CANONICAL;
+        job.eps_abs = eps_abs; job.eps_rel = eps_rel;
         propagate_all(&job, nthreads);
         double ln = 0.0, es = 0.0;
         int resampled = 0;
@@ -812,7 +816,7 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
  * out[(n+1)*S + k]; out[0..S) = x0 (with incidences reset) */
 int oracle_trajectory(int model, int N, uint32_t noises_off, uint32_t seed, uint32_t stream, uint32_t j,
                       double dt, const double *times, const double *pv, const double *x0, double *out,
-                      const int32_t *cov_n, const double *cov_x, const double *cov_y)
+                      const int32_t *cov_n, const double *cov_x, const double *cov_y, double eps_abs, double eps_rel)
 {
     if (model < 0 || model >= N_MODELS) return -1;
     const oracle_model *M = &MODELS[model];
@@ -826,7 +830,8 @@ int oracle_trajectory(int model, int N, uint32_t noises_off, uint32_t seed, uint
     for (int n = 0; n < N; n++) {
         plom_nstream64 ns;
         plom_nstream64_init(&ns, seed, stream, (uint32_t)n, j, plom_bm_tab);
-        M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off, cv_steps(&cv, n), cv.cs);
+        M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off, cv_steps(&cv, n), cv.cs, eps_abs,
+                    eps_rel);
         memcpy(out + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
     }
     free(cv.owned);
@@ -844,6 +849,54 @@ int oracle_obs_loglik(int model, const double *x, const double *y, const double 
     *loglik = M->loglik(obs, y, pv, pv + M->fix_off, t, like_min, &nf);
     *n_floor = nf;
     return 0;
+}
+
+/* ------------------------------------------------------------------ the RKF45 integrator */
+/* analytic right-hand sides for the tests of csrc/plom_ode_impl.h:
+ *   kind 0   y' = p0 y (dimension 1)
+ *   kind 1   y' = p0 + p1 t + p2 t^2 + p3 t^3 + p4 t^4 (dimension 1)
+ *   kind 2   y0' = y1, y1' = -p0 y0 (harmonic oscillator, dimension 2) */
+static int ode_kind;
+static void ode_test_rhs(double t, const double *y, const double *pv, const double *lp, const double *cv, double *f)
+{
+    (void)lp; (void)cv;
+    if (ode_kind == 0) {
+        f[0] = pv[0] * y[0];
+        f[1] = 0.0;
+    } else if (ode_kind == 1) {
+        f[0] = pv[0] + t * (pv[1] + t * (pv[2] + t * (pv[3] + t * pv[4])));
+        f[1] = 0.0;
+    } else {
+        f[0] = y[1];
+        f[1] = -pv[0] * y[0];
+    }
+}
+
+static int ode_test_count;
+static void ode_test_rhs_counted(double t, const double *y, const double *pv, const double *lp, const double *cv,
+                                 double *f)
+{
+    ode_test_count++;
+    ode_test_rhs(t, y, pv, lp, cv, f);
+}
+
+#define PLOM_ODE_FN ode_test_integrate
+#define PLOM_ODE_RHS ode_test_rhs_counted
+#define PLOM_ODE_N 2
+#define PLOM_ODE_QUAL static
+#include "plom_ode_impl.h"
+
+/* integrate a test system (kind above, parameters p[5]) over [t0, t1] from y (2 values; the
+ * second is carried along for the 1-d kinds), first step h; returns the right-hand side
+ * evaluations in *n_eval, and 0 or -1 (step bound) */
+int oracle_ode_test(int kind, const double *p, double *y, double t0, double t1, double h, double eps_abs, double eps_rel,
+                    int32_t *n_eval)
+{
+    ode_kind = kind;
+    ode_test_count = 0;
+    const int rc = ode_test_integrate(y, p, NULL, NULL, t0, t1, h, eps_abs, eps_rel);
+    *n_eval = ode_test_count;
+    return rc;
 }
 
 /* shared scalar arithmetic, host build (compared with the device build bit for bit) */

@@ -83,8 +83,10 @@ def test_theta_remainder_and_errors(tmp_path):
     p.write_text(json.dumps(theta) + "\n")
     r = run_cli(d, ["sde", "-J", "10", "--dry_run"], str(p))
     assert r.returncode != 0 and "greater than 1.0" in r.stderr
-    r = run_cli(d, ["ode", "-J", "10"], os.path.join(PKG, "examples", "sir", "theta.json"))
-    assert r.returncode == 1 and "ode" in r.stderr
+    r = run_cli(d, ["ode", "-J", "10", "--fp32"], os.path.join(PKG, "examples", "sir", "theta.json"))
+    assert r.returncode != 0 and "FP64" in r.stderr
+    r = run_cli(d, ["euler", "-J", "10"], os.path.join(PKG, "examples", "sir", "theta.json"))
+    assert r.returncode == 1 and "implementation is 'ode', 'sde' or 'psr'" in r.stderr
     r = run_cli(d, ["psr", "-J", "10", "--fp32"], os.path.join(PKG, "examples", "sir", "theta.json"))
     assert r.returncode != 0 and "FP64" in r.stderr
 
@@ -352,3 +354,22 @@ def test_cli_noise_covariates_match_api(tmp_path, ctx):
         assert ll_cli == g.loglik
     o = oracle.smc(m.id, 2000, times, data, pv, x0, seed=2, stream=0, cov=cov, nthreads=8)
     assert ll_cli == o["loglik"]
+
+
+@pytest.mark.gpu
+def test_cli_ode_matches_api(tmp_path, ctx):
+    """`smc ode` integrates without noise under --eps_abs / --eps_rel (defaults 1e-6 / 1e-3,
+    C/core/plom.h:90-91) and gives the API's log-likelihood for one particle."""
+    d = model_dir(tmp_path, "sir")
+    theta = os.path.join(PKG, "examples", "sir", "theta.json")
+    dry = json.loads(run_cli(d, ["ode", "-J", "1", "--dry_run"], theta).stdout)
+    m = smc.Model.get("sir")
+    times, data = m.dataset()
+    pv, x0 = np.array(dry["pv"]), np.array(dry["x0"])
+    for extra, eps in (([], (1e-6, 1e-3)), (["--eps_rel", "1e-5", "--eps_abs", "1e-8"], (1e-8, 1e-5))):
+        print("cli ode", extra, flush=True)
+        r = run_cli(d, ["ode", "-J", "1", "-b", "-e", "-d"] + extra, theta)
+        assert r.returncode == 0, r.stderr
+        ll_cli = float([l for l in r.stdout.splitlines() if l.startswith("logV_17g:")][0].split(":")[1])
+        g = smc.run(ctx, m, 1, times, data, pv, x0, flags=smc.FLAG_ODE, eps_abs=eps[0], eps_rel=eps[1], seed=2)
+        assert ll_cli == g.loglik

@@ -100,6 +100,9 @@ typedef struct {
                             rate, proportion ...): [covariate][n_cac] then those of the observation
                             model [covariate][n_ts] */
     int32_t n_fix;       /* number of those slots (plom_filter_inputs.cov has n_fix series) */
+    int32_t n_drift;     /* drifting process parameters (diffusions); their states sit at drift_offset,
+                            one per parameter and population: [n_drift][n_cac] */
+    int32_t drift_offset;
 } plom_model_info;
 
 typedef struct {
@@ -148,6 +151,26 @@ typedef struct {
     const double *y;     /* [n] values, in the data's units */
 } plom_covariate;
 
+/* The drifting parameters of a model with diffusions (compute_drift, C/core/drift.c:27-42;
+ * back-transform in step_sde_*, C/templates/core_template.c:251-271), per drifting
+ * parameter i and population c at index i * n_cac + c.  The drift state of (i, c) lives in
+ * x0 / the state at drift_offset + i * n_cac + c, on the parameter's transformed scale (its
+ * theta value, theta_driftIC2Xdrift, C/core/prediction_util.c:119-131); populations of one
+ * group start equal and move together.  After every Euler substep the state of group g
+ * moves by (pv[vol_slot[i * n_cac + g]] sqrt(dt)) z with one standard normal z per group, in group
+ * order; the rates read the natural value of the state. */
+typedef struct {
+    const int32_t *group;      /* [n_drift * n_cac] group of each population, 0 <= g < n_cac */
+    const int32_t *tr;         /* [n_drift * n_cac] 0 identity, 1 log, 2 logit, 3 logit_ab */
+    const int32_t *scale;      /* 0 none, 1 10^x, 2 10^-x */
+    const int32_t *duration;   /* 1: a duration, the rate is 1/(1e-5 + value) */
+    const double *min, *max;   /* bounds of logit_ab */
+    const double *multiplier;  /* unit multiplier */
+    const int32_t *vol_slot;   /* [n_drift * n_cac]: entry i * n_cac + g is the pv slot holding the natural
+                                  volatility of group g (natural[vol][g]; -1: none), so that a new pv
+                                  (plom_filter_update) moves the volatility too */
+} plom_drift_map;
+
 typedef struct {
     const double *times; /* [N+1] days since date_0 (settings data.times) */
     const double *data;  /* [N*n_ts] observations, NaN = missing */
@@ -157,6 +180,9 @@ typedef struct {
     const plom_covariate *cov; /* optional [n_fix]: time-varying covariates, evaluated at creation
                                   (they then replace the covariate slots of pv); NULL: the
                                   covariates are the constants in pv */
+    const plom_drift_map *drift; /* models with diffusions, unless noises_off has PLOM_NO_DRIFT_BIT or
+                                    the implementation is ode (then the parameters keep their pv
+                                    values); not with PLOM_FLAG_MIF */
 } plom_filter_inputs;
 
 typedef struct {

@@ -177,6 +177,8 @@ int main(int argc, char *argv[])
     cli_problem P;
     cli_load(&P, 0, -1, dt);
     P.psr = psr;
+    P.drift_on = P.n_drift > 0 && !(noises_off & PLOM_NO_DRIFT_BIT) && !(impl_flags & PLOM_FLAG_ODE);
+    if (P.drift_on) cli_die("mif with drifting parameters is not supported by this build (use --no_diff)");
     if (pj_get(P.theta, "covariance")) cli_die("a \"covariance\" in theta (MVN proposals) is not supported by mif in this build");
     const int L = (int)floor(prop_L * P.N_DATA);
     if (ic_only) P.N = (L + 1 > P.N_DATA) ? P.N_DATA : L + 1;
@@ -357,7 +359,8 @@ int main(int argc, char *argv[])
         d.dt = P.dt;
         d.like_min = like_min;
         d.n_theta = n;
-        plom_filter_inputs in = {P.times, P.data, pv, x0one, &mmap, P.has_cov ? P.cov : NULL};
+        plom_filter_inputs in = {P.times, P.data, pv, x0one, &mmap, P.has_cov ? P.cov : NULL,
+                                  P.drift_on ? &P.dmap : NULL};
         if (plom_filter_create(ctx, &d, &in, &fs[k]) != PLOM_OK) cli_die(plom_last_error());
         c->f = fs[k];
     }

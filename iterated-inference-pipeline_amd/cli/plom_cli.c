@@ -466,6 +466,45 @@ void cli_load(cli_problem *P, int is_bayesian, int nb_obs, double dt)
         }
         P->R[i].follow = j;
     }
+    /* drifting parameters: the groups and transforms of the parameter's router; the
+     * volatility of group g is natural[vol][g] (drift.c:38-40), i.e. the pv slot of the first
+     * population in group g of the volatility's router */
+    pj *jdr = pj_get(settings, "drift");
+    pj *jdp = pj_get(jdr, "ind_par_Xdrift_applied"), *jdv = pj_get(jdr, "ind_volatility_Xdrift");
+    P->n_drift = pj_size(jdp);
+    if (P->n_drift != P->mi.n_drift || pj_size(jdv) != P->n_drift) cli_die("settings drift differs from the compiled model");
+    if (P->n_drift) {
+        const size_t nd = (size_t)P->n_drift * N_CAC;
+        int32_t *grp = calloc(nd, 4), *tr = calloc(nd, 4), *sc = calloc(nd, 4), *dur = calloc(nd, 4), *vs = calloc(nd, 4);
+        double *mn = calloc(nd, 8), *mx = calloc(nd, 8), *mult = calloc(nd, 8);
+        P->drift_par = calloc((size_t)P->n_drift, sizeof(int));
+        for (int i = 0; i < P->n_drift; i++) {
+            const int ip = (int)pj_number(pj_at(jdp, i), NULL), iv = (int)pj_number(pj_at(jdv, i), NULL);
+            if (ip < P->n_sv || ip >= P->n_sv + P->n_proc || iv < P->n_sv || iv >= P->n_sv + P->n_proc)
+                cli_die("a drifting parameter and its volatility must be process parameters");
+            P->drift_par[i] = ip;
+            const cli_router *r = &P->R[ip], *rv = &P->R[iv];
+            for (int c = 0; c < N_CAC; c++) {
+                const int g = r->map[c], q = i * N_CAC + c;
+                grp[q] = g;
+                tr[q] = (int32_t)r->tr[g];
+                sc[q] = (int32_t)r->sc;
+                dur[q] = r->is_duration;
+                mn[q] = r->min[g];
+                mx[q] = r->max[g];
+                mult[q] = r->multiplier;
+                vs[q] = -1;
+            }
+            for (int g = 0; g < N_CAC; g++)
+                for (int c = 0; c < N_CAC; c++)
+                    if (rv->map[c] == g) {
+                        vs[i * N_CAC + g] = (iv - P->n_sv) * N_CAC + c;
+                        break;
+                    }
+        }
+        P->dmap.group = grp; P->dmap.tr = tr; P->dmap.scale = sc; P->dmap.duration = dur;
+        P->dmap.min = mn; P->dmap.max = mx; P->dmap.multiplier = mult; P->dmap.vol_slot = vs;
+    }
     P->noise_par = calloc((size_t)P->n_par, sizeof(int));
     pj *ins = pj_get(settings, "ind_noise_sd");
     for (int q = 0; q < pj_size(ins); q++) {
@@ -564,6 +603,12 @@ void cli_pack(const cli_problem *P, const double *theta, double *pv, double *x0)
             x0[i * N_CAC + cac] = cli_natural(P, i, P->R[i].map[cac], theta) * P->fixed[cac];
             if (P->psr) x0[i * N_CAC + cac] = round(x0[i * N_CAC + cac]);   /* counts (prediction_util.c:108-110) */
         }
+    /* drift states: the drifting parameter's theta (theta_driftIC2Xdrift, prediction_util.c:119-131) */
+    for (int i = 0; i < P->n_drift; i++)
+        for (int cac = 0; cac < N_CAC; cac++) {
+            const int ip = P->drift_par[i];
+            x0[P->mi.drift_offset + i * N_CAC + cac] = theta[P->offset[ip] + P->R[ip].map[cac]];
+        }
 }
 
 /* ------------------------------------------------------------------ writers */
@@ -609,6 +654,11 @@ void cli_x_header(const cli_problem *P, FILE *f)
     if (P->has_rem)
         for (int cac = 0; cac < P->N_CAC; cac++) fprintf(f, "%s:%s,", P->remainder, pj_string(pj_at(P->cac_id, cac)));
     for (int ts = 0; ts < P->N_TS; ts++) fprintf(f, "obs_mean:%s,", pj_string(pj_at(P->ts_id, ts)));
+    /* drift columns (header_X, print.c:115-122) */
+    for (int i = 0; P->drift_on && i < P->n_drift; i++) {
+        const cli_router *r = &P->R[P->drift_par[i]];
+        for (int g = 0; g < r->n_gp; g++) fprintf(f, "drift:%s:%s,", r->name, r->group_name[g]);
+    }
     for (int ts = 0; ts < P->N_TS; ts++)
         fprintf(f, "obs_real:%s%s", pj_string(pj_at(P->ts_id, ts)), ts < P->N_TS - 1 ? "," : "");
     fprintf(f, "\n");
@@ -629,6 +679,16 @@ void cli_x_rows(const cli_problem *P, FILE *f, int index, const double *lineage,
                 fprintf(f, "%g,", P->pop_t[r * N_CAC + cac] - s);
             }
         for (int ts = 0; ts < N_TS; ts++) fprintf(f, "%g,", ob[2 * ts]);
+        /* drift: f_inv of the state of the group's first population (print_X, print.c:321-327) */
+        for (int i = 0; P->drift_on && i < P->n_drift; i++) {
+            const cli_router *rr = &P->R[P->drift_par[i]];
+            for (int g = 0; g < rr->n_gp; g++)
+                for (int c = 0; c < N_CAC; c++)
+                    if (rr->map[c] == g) {
+                        fprintf(f, "%g,", cli_f_inv(rr->tr[g], x[P->mi.drift_offset + i * N_CAC + c], rr->min[g], rr->max[g]));
+                        break;
+                    }
+        }
         for (int ts = 0; ts < N_TS; ts++) fprintf(f, "%g%s", ob[2 * ts + 1], ts < N_TS - 1 ? "," : "");
         fprintf(f, "\n");
     }

@@ -57,6 +57,10 @@ typedef struct {
     double *cov_raw0;   /* [n_fix] the values at t = 0 before -g freezes the series */
     int has_cov;        /* some covariate changes with time: the filters take cov */
     double *pop_t;      /* [N + 1][N_CAC] population sizes at the observation times */
+    int n_drift;        /* drifting parameters (settings drift, build_drift, build.c:609-646) */
+    int *drift_par;     /* [n_drift] router index of each drifting parameter */
+    plom_drift_map dmap; /* per parameter and population: group, back-transform, volatility slot */
+    int drift_on;       /* the drift runs (set by the command: not with --no_diff or ode) */
     double *times;      /* [N + 1] */
     double *data;       /* [N][N_TS], NaN = missing */
     double dt;

@@ -379,6 +379,7 @@ int main(int argc, char *argv[])
     cli_problem P;
     cli_load(&P, 1, nb_obs, dt);
     P.psr = psr;
+    P.drift_on = P.n_drift > 0 && !(noises_off & PLOM_NO_DRIFT_BIT) && !(impl_flags & PLOM_FLAG_ODE);
     MC.P = &P;
     const int n = P.n_theta, N = P.N, S = P.mi.size_proj, N_TS = P.N_TS;
     MC.n = n;
@@ -566,7 +567,8 @@ int main(int argc, char *argv[])
         d.stream = 0;
         d.dt = P.dt;
         d.like_min = LIKE_MIN;
-        plom_filter_inputs in = {P.times, P.data, C[k].pv, C[k].x0, NULL, P.has_cov ? P.cov : NULL};
+        plom_filter_inputs in = {P.times, P.data, C[k].pv, C[k].x0, NULL, P.has_cov ? P.cov : NULL,
+                                  P.drift_on ? &P.dmap : NULL};
         if (plom_filter_create(ctx, &d, &in, &fs[k]) != PLOM_OK) cli_die(plom_last_error());
         C[k].f = fs[k];
     }

@@ -124,6 +124,7 @@ int main(int argc, char *argv[])
     cli_problem P;
     cli_load(&P, 0, nb_obs, dt);
     P.psr = psr;
+    P.drift_on = P.n_drift > 0 && !(noises_off & PLOM_NO_DRIFT_BIT) && !(impl_flags & PLOM_FLAG_ODE);
     const int N = P.N, N_TS = P.N_TS;
     const plom_model_info mi = P.mi;
     double *theta = calloc((size_t)P.n_theta + 1, sizeof(double));
@@ -169,7 +170,8 @@ int main(int argc, char *argv[])
     d.stream = 0;
     d.dt = dt;
     d.like_min = like_min;
-    plom_filter_inputs in = {times, data, pv, x0, NULL, P.has_cov ? P.cov : NULL};
+    plom_filter_inputs in = {times, data, pv, x0, NULL, P.has_cov ? P.cov : NULL,
+                              P.drift_on ? &P.dmap : NULL};
     plom_filter_outputs out;
     memset(&out, 0, sizeof out);
     double *ess_n = calloc((size_t)N + 1, sizeof(double));
@@ -221,12 +223,31 @@ int main(int argc, char *argv[])
             const char *tn = pj_string(pj_at(ts_id, ts));
             fprintf(fh, "low95:%s,%s,high95:%s%s", tn, tn, tn, ts < N_TS - 1 ? "," : "");
         }
+        /* drift: per group of each drifting parameter (header_hat, print.c:175-183) */
+        for (int i = 0; P.drift_on && i < P.n_drift; i++) {
+            const cli_router *r = &P.R[P.drift_par[i]];
+            for (int g = 0; g < r->n_gp; g++)
+                fprintf(fh, ",low95:diff:%s:%s,diff:%s:%s,high95:diff:%s:%s", r->name, r->group_name[g], r->name,
+                        r->group_name[g], r->name, r->group_name[g]);
+        }
         fprintf(fh, "\n");
+        const int nh_base = mi.n_hat - mi.n_drift * N_CAC;
         for (int n = 0; n <= N; n++) {
             const double *row = hat + (size_t)n * mi.n_hat * 3;
             fprintf(fh, "%g,", times[n]);
-            for (int k = 0; k < mi.n_hat; k++)
-                fprintf(fh, "%g,%g,%g%s", row[3 * k], row[3 * k + 1], row[3 * k + 2], k < mi.n_hat - 1 ? "," : "");
+            for (int k = 0; k < nh_base; k++)
+                fprintf(fh, "%g,%g,%g%s", row[3 * k], row[3 * k + 1], row[3 * k + 2], k < nh_base - 1 ? "," : "");
+            /* the drift entry of a group is that of its first population */
+            for (int i = 0; P.drift_on && i < P.n_drift; i++) {
+                const cli_router *r = &P.R[P.drift_par[i]];
+                for (int g = 0; g < r->n_gp; g++)
+                    for (int c = 0; c < N_CAC; c++)
+                        if (r->map[c] == g) {
+                            const int k = nh_base + i * N_CAC + c;
+                            fprintf(fh, ",%g,%g,%g", row[3 * k], row[3 * k + 1], row[3 * k + 2]);
+                            break;
+                        }
+            }
             fprintf(fh, "\n");
         }
         fclose(fh);

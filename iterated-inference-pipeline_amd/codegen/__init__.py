@@ -15,7 +15,7 @@ EXAMPLES = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__
 GEN_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gen")
 
 # models compiled into the library, in model-id order
-BUILTIN_MODELS = ["sir", "sirs4", "seir10", "noise"]
+BUILTIN_MODELS = ["sir", "sirs4", "seir10", "noise", "drift"]
 
 
 def load_json(path):

@@ -42,6 +42,8 @@ class Emitter:
         self.p = "plom_%s" % model.name.lower()
         self.used_pars = set()
         self.time_funcs = None
+        self.drift_mode = "pv"
+        self.used_drift = set()
 
     # ---------------------------------------------------------------- symbols
     def xidx(self, state, cac):
@@ -55,6 +57,8 @@ class Emitter:
         def resolve(name):
             if name in m.par_sv:
                 return "X[%d]" % self.xidx(name, cac)
+            if name in m.drift_par_proc and self.drift_mode == "state":
+                return self.drift_name(m.drift_par_proc.index(name), cac)
             if name in lay["proc"]:
                 return self.par(lay["proc"][name] + cac)
             if name in lay["fixed"]:
@@ -95,6 +99,66 @@ class Emitter:
         """Name of parameter pv[k] inside an interval function (bound once, uniform)."""
         self.used_pars.add(k)
         return "P%d" % k
+
+    def drift_name(self, i, cac):
+        """Value of drifting parameter i in population cac inside an interval function: the
+        back-transformed drift state at the start of the substep, or the pv value without
+        drift (core_template.c:128-152, 251-271)."""
+        self.used_drift.add((i, cac))
+        self.par(self.m.layout["proc"][self.m.drift_par_proc[i]] + cac)
+        return "D%d_%d" % (i, cac)
+
+    def drift_loads(self, real):
+        if not self.used_drift:
+            return []
+        m = self.m
+        Rt = "double" if real == "double" else "float"
+        cast = "" if real == "double" else "(float)"
+        L = ["        %s %s;" % (Rt, ", ".join("D%d_%d" % ic for ic in sorted(self.used_drift))),
+             "        if (dm) {"]
+        for i, c in sorted(self.used_drift):
+            q = i * m.N_CAC + c
+            L.append("            D%d_%d = %splom_mif_natural(dm->tr[%d], dm->sc[%d], dm->dur[%d], (double)X[%d], dm->min[%d], dm->max[%d], dm->mult[%d]);"
+                     % (i, c, cast, q, q, q, m.drift_offset + q, q, q, q))
+        L.append("        } else {")
+        for i, c in sorted(self.used_drift):
+            L.append("            D%d_%d = P%d;" % (i, c, m.layout["proc"][m.drift_par_proc[i]] + c))
+        L.append("        }")
+        return L
+
+    def drift_update(self, real):
+        """compute_drift (C/core/drift.c:27-42) after the substep: x += (vol_g sqrt(dt)) z_g for
+        every group g of every drifting parameter (vol_g the pv slot the map names), z_g in group
+        order from the particle's stream (Box-Muller pairs); every population of group g moves
+        by the same amount."""
+        m = self.m
+        if not m.N_DRIFT:
+            return []
+        Rt, _, _, _, normal = _R(real)
+        zero = "0.0" if real == "double" else "0.0f"
+        L = ["        if (dm) {"]
+        for i in range(m.N_DRIFT):
+            L += ["            {",
+                  "                const int ng = dm->ng[%d];" % i,
+                  "                %s zb = %s;" % (Rt, zero),
+                  "                for (int g = 0; g < ng; g++) {",
+                  "                    %s z;" % Rt,
+                  "                    if ((g & 1) == 0) {",
+                  "                        %s za;" % Rt,
+                  "                        %s(ns, &za, &zb);" % normal,
+                  "                        z = za;",
+                  "                    } else {",
+                  "                        z = zb;",
+                  "                    }",
+                  "                    const int vs = dm->vol_slot[%d + g];" % (i * m.N_CAC),
+                  "                    const %s vol = vs >= 0 ? pv[vs] : %s;" % (Rt, zero),
+                  "                    const %s inc = (vol * sqdt) * z;" % Rt]
+            for c in range(m.N_CAC):
+                q = m.drift_offset + i * m.N_CAC + c
+                L.append("                    if (dm->group[%d] == g) X[%d] = X[%d] + inc;" % (i * m.N_CAC + c, q, q))
+            L += ["                }", "            }"]
+        L.append("        }")
+        return L
 
     def cov(self, k):
         """Name of the covariate in slot k of pv inside an interval function: its value at
@@ -154,16 +218,18 @@ class Emitter:
         qual = "PLOM_HD" if real == "double" and not pp else dev
         L = []
         if pp:
-            L.append("%s void %s_interval_%s_pp_%s(%s *X, const %s *pv, const %s *lp, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs)"
+            L.append("%s void %s_interval_%s_pp_%s(%s *X, const %s *pv, const %s *lp, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs, const plom_drift_view *dm)"
                      % (qual, self.p, variant, tag, Rt, Rt, Rt, stream, Rt))
         else:
-            L.append("%s void %s_interval_%s_%s(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs)"
+            L.append("%s void %s_interval_%s_%s(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs, const plom_drift_view *dm)"
                      % (qual, self.p, variant, tag, Rt, Rt, stream, Rt))
         L.append("{")
         L.append("    const %s dtr = (%s)dt;" % (Rt, Rt))
-        L.append("    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;%s" % (" (void)lp;" if pp else ""))
+        L.append("    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;%s" % (" (void)lp;" if pp else ""))
         self.used_pars = set()
         self.used_cov = set()
+        self.used_drift = set()
+        self.drift_mode = "state"
         bind_at = len(L)
         for u in range(m.N_TS_INC_UNIQUE):
             L.append("    X[%d] = %s; /* reset_inc */" % (m.inc_offset + u, zero))
@@ -276,13 +342,15 @@ class Emitter:
                     L.append("        s%d += q%d_%d;" % (idx, idx, c))
             L.append("        const %s f%d = X[%d] + s%d;" % (Rt, idx, idx, idx))
             L.append(self.clip(idx, real))
+        L += self.drift_update(real)
         L.append("        t += dt;")
         L.append("    }")
         L.append("}")
         # time-only special functions (e.g. the seasonal forcing), once per substep
         tfs = ["        const %s %s = %s;" % (Rt, name, text) for text, name in
                sorted(self.time_funcs.items(), key=lambda kv: int(kv[1][2:]))]
-        L[tf_at:tf_at] = self.cov_loads(real) + tfs
+        L[tf_at:tf_at] = self.cov_loads(real) + self.drift_loads(real) + tfs
+        self.drift_mode = "pv"
         self.time_funcs = None
         uni = "PLOM_UNIFORM64" if real == "double" else "PLOM_UNIFORM32"
         npp = m.layout["npp"]
@@ -307,15 +375,17 @@ class Emitter:
         dev = "static __device__ inline __attribute__((always_inline))"
         L = []
         if pp:
-            L.append("%s void %s_interval_%s_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)"
+            L.append("%s void %s_interval_%s_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)"
                      % (dev, self.p, variant))
         else:
-            L.append("PLOM_HD void %s_interval_%s_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)"
+            L.append("PLOM_HD void %s_interval_%s_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)"
                      % (self.p, variant))
         L.append("{")
-        L.append("    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;%s" % (" (void)lp;" if pp else ""))
+        L.append("    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;%s" % (" (void)lp;" if pp else ""))
         self.used_pars = set()
         self.used_cov = set()
+        self.used_drift = set()
+        self.drift_mode = "state"
         bind_at = len(L)
         for u in range(m.N_TS_INC_UNIQUE):
             L.append("    X[%d] = 0.0; /* reset_inc */" % (m.inc_offset + u))
@@ -413,12 +483,14 @@ class Emitter:
                             terms.append("n%d_%d" % (ri, c))
                 L.append("        X[%d] = X[%d] + (%s);" % (m.inc_offset + u, m.inc_offset + u, " + ".join(terms) if terms else "0.0"))
                 u += 1
+        L += self.drift_update("double")
         L.append("        t += dt;")
         L.append("    }")
         L.append("}")
         tfs = ["        const double %s = %s;" % (name, text) for text, name in
                sorted(self.time_funcs.items(), key=lambda kv: int(kv[1][2:]))]
-        L[tf_at:tf_at] = self.cov_loads("double") + tfs + decl
+        L[tf_at:tf_at] = self.cov_loads("double") + self.drift_loads("double") + tfs + decl
+        self.drift_mode = "pv"
         self.time_funcs = None
         npp = m.layout["npp"]
         binds = [("    const double P%d = lp[%d];" % (k, k)) if pp and k < npp else
@@ -453,6 +525,8 @@ class Emitter:
                 items += [("+", self.term(r, c, r["from"]), r) for r in m.proc_model if r["to"] == s]
                 eq, _, _ = self.eq_parts(items, c, "double", False, False)
                 body.append("    f[%d] = %s;" % (self.xidx(s, c), eq if eq else "0.0"))
+        for q in range(m.N_DRIFT * m.N_CAC):
+            body.append("    f[%d] = 0.0; /* drift: constant (the drifting parameters are their pv values) */" % (m.drift_offset + q))
         u = 0
         for o, lst in enumerate(m.obs_var_def):
             if not m.is_inc[o]:
@@ -567,7 +641,7 @@ class Emitter:
 
     def n_hat(self):
         m = self.m
-        return m.N_PAR_SV * m.N_CAC + (m.N_CAC if m.remainder else 0) + m.N_TS
+        return m.N_PAR_SV * m.N_CAC + (m.N_CAC if m.remainder else 0) + m.N_TS + m.N_DRIFT * m.N_CAC
 
     def obs_resolver(self, ts):
         lay = self.m.layout
@@ -609,8 +683,8 @@ class Emitter:
         of each population (pop size - sum_SV, when the model has one) and the observation
         mean of every series."""
         m, lay = self.m, self.m.layout
-        L = ["PLOM_HD void %s_hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out)" % self.p,
-             "{", "    (void)t; (void)pvd; (void)cvd;"]
+        L = ["PLOM_HD void %s_hat_values(const double *X, const double *pvd, const double *cvd, const plom_drift_view *dm, double t, double *out)" % self.p,
+             "{", "    (void)t; (void)pvd; (void)cvd; (void)dm;"]
         k = 0
         for i in range(m.N_PAR_SV * m.N_CAC):
             L.append("    out[%d] = X[%d];" % (k, i))
@@ -628,6 +702,12 @@ class Emitter:
         L.append("    double obs[%d], var[%d];" % (m.N_TS, m.N_TS))
         L.append("    %s_proj2obs_f64(X, pvd, cvd, obs);" % self.p)
         L.append("    %s_obs_moments(obs, pvd, cvd, t, out + %d, var);" % (self.p, k))
+        k += m.N_TS
+        # drifting parameters on their scale of estimation (f_inv, hat.c:141-168)
+        for q in range(m.N_DRIFT * m.N_CAC):
+            L.append("    out[%d] = dm ? plom_mif_f_inv(dm->tr[%d], X[%d], dm->min[%d], dm->max[%d]) : PLOM_NAN;"
+                     % (k, q, m.drift_offset + q, q, q))
+            k += 1
         L.append("}")
         return "\n".join(L)
 
@@ -693,7 +773,9 @@ class Emitter:
                 "#define %s_N_HAT %d" % (P, self.n_hat()),
                 "#define %s_N_PP %d" % (P, m.layout["npp"]),
                 "#define %s_FIX_OFF %d" % (P, m.layout["fix_off"]),
-                "#define %s_N_FIX %d" % (P, m.layout["n_fix"]), ""]
+                "#define %s_N_FIX %d" % (P, m.layout["n_fix"]),
+                "#define %s_N_DRIFT %d" % (P, m.N_DRIFT),
+                "#define %s_DRIFT_OFFSET %d" % (P, m.drift_offset), ""]
         for v, dem, env in VARIANTS:
             out.append(self.interval(v, dem, env, "double"))
             out.append("")
@@ -715,22 +797,22 @@ class Emitter:
         # host dispatcher on the reference's noises_off bits (prediction_util.c:143-161),
         # PLOM_IMPL_PSR / PLOM_IMPL_ODE selecting the psr / ode implementations
         # (prediction_util.c:139-170)
-        out.append("PLOM_HD void %s_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel)" % self.p)
+        out.append("PLOM_HD void %s_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel, const plom_drift_view *dm)" % self.p)
         out.append("{")
         out.append("    if (noises_off & PLOM_IMPL_ODE) {")
         out.append("        (void)%s_interval_ode_f64(X, pv, cv, t0, t1, dt, eps_abs, eps_rel);" % self.p)
         out.append("        return;")
         out.append("    }")
         out.append("    if (noises_off & PLOM_IMPL_PSR) {")
-        out.append("        if (noises_off & PLOM_NO_ENV_STO) %s_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
-        out.append("        else %s_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
+        out.append("        if (noises_off & PLOM_NO_ENV_STO) %s_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
+        out.append("        else %s_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
         out.append("        return;")
         out.append("    }")
         out.append("    const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);")
-        out.append("    if (v == 0u) %s_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
-        out.append("    else if (v == PLOM_NO_DEM_STO) %s_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
-        out.append("    else if (v == PLOM_NO_ENV_STO) %s_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
-        out.append("    else %s_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
+        out.append("    if (v == 0u) %s_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
+        out.append("    else if (v == PLOM_NO_DEM_STO) %s_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
+        out.append("    else if (v == PLOM_NO_ENV_STO) %s_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
+        out.append("    else %s_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
         out.append("}")
         out.append("")
         out.append("#if defined(__HIPCC__)")
@@ -756,29 +838,29 @@ class Emitter:
         out.append("")
         name = m.name.lower()
         out.append("struct plom_model_%s {" % name)
-        out.append("    static constexpr int S = %d, N_TS = %d, INC_OFFSET = %d, N_INC = %d, PV = %d, N_CAC = %d, N_HAT = %d, NPP = %d, FIX_OFF = %d, N_FIX = %d;"
+        out.append("    static constexpr int S = %d, N_TS = %d, INC_OFFSET = %d, N_INC = %d, PV = %d, N_CAC = %d, N_HAT = %d, NPP = %d, FIX_OFF = %d, N_FIX = %d, N_DRIFT = %d;"
                    % (m.size_proj, m.N_TS, m.inc_offset, m.N_TS_INC_UNIQUE, m.layout["size"], m.N_CAC, self.n_hat(),
-                      m.layout["npp"], m.layout["fix_off"], m.layout["n_fix"]))
+                      m.layout["npp"], m.layout["fix_off"], m.layout["n_fix"], m.N_DRIFT))
         for tag, Rt, stream in (("f64", "double", "plom_nstream64"), ("f32", "float", "plom_nstream32")):
-            out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs)"
+            out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs, const plom_drift_view *dm)"
                        % (Rt, Rt, stream, Rt))
             out.append("    {")
             for vi, (v, dem, env) in enumerate(VARIANTS):
                 kw = "if" if vi == 0 else "else if"
-                out.append("        %s constexpr (V == %d) %s_interval_%s_%s(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % (kw, vi, self.p, v, tag))
+                out.append("        %s constexpr (V == %d) %s_interval_%s_%s(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % (kw, vi, self.p, v, tag))
             if tag == "f64":
-                out.append("        else if constexpr (V == 4) %s_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
-                out.append("        else if constexpr (V == 5) %s_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
+                out.append("        else if constexpr (V == 4) %s_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
+                out.append("        else if constexpr (V == 5) %s_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
             out.append("    }")
-            out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(%s *X, const %s *pv, const %s *lp, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs)"
+            out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(%s *X, const %s *pv, const %s *lp, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs, const plom_drift_view *dm)"
                        % (Rt, Rt, Rt, stream, Rt))
             out.append("    {")
             for vi, (v, dem, env) in enumerate(VARIANTS):
                 kw = "if" if vi == 0 else "else if"
-                out.append("        %s constexpr (V == %d) %s_interval_%s_pp_%s(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);" % (kw, vi, self.p, v, tag))
+                out.append("        %s constexpr (V == %d) %s_interval_%s_pp_%s(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);" % (kw, vi, self.p, v, tag))
             if tag == "f64":
-                out.append("        else if constexpr (V == 4) %s_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
-                out.append("        else if constexpr (V == 5) %s_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);" % self.p)
+                out.append("        else if constexpr (V == 4) %s_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
+                out.append("        else if constexpr (V == 5) %s_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
             out.append("    }")
             out.append("    static __device__ inline __attribute__((always_inline)) void proj2obs(const %s *X, const double *pvd, const double *cvd, double *obs) { %s_proj2obs_%s(X, pvd, cvd, obs); }"
                        % (Rt, self.p, tag))
@@ -792,8 +874,8 @@ class Emitter:
                    " { return %s_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }" % self.p)
         out.append("    static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor)"
                    " { return %s_loglik_pp(obs, y, pvd, lpd, cvd, t, like_min, n_floor); }" % self.p)
-        out.append("    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out)"
-                   " { %s_hat_values(X, pvd, cvd, t, out); }" % self.p)
+        out.append("    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, const plom_drift_view *dm, double t, double *out)"
+                   " { %s_hat_values(X, pvd, cvd, dm, t, out); }" % self.p)
         out.append("    static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var)"
                    " { %s_obs_moments(obs, pvd, cvd, t, mean, var); }" % self.p)
         out.append("};")

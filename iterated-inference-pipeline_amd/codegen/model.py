@@ -170,8 +170,11 @@ class PlomModel:
             raise ModelError("incidence time series do not map one-to-one onto observed incidences")
 
         self.N_PAR_SV = len(self.par_sv)
-        self.N_DRIFT = 0
-        self.inc_offset = self.N_PAR_SV * self.N_CAC + self.N_DRIFT
+        # drifting parameters (diffusions, Cmodel.py:84-108): one state slot per population for
+        # each, after the states (populations of one group of the parameter share one path)
+        self.N_DRIFT = len(self.drift_par_proc)
+        self.drift_offset = self.N_PAR_SV * self.N_CAC
+        self.inc_offset = self.drift_offset + self.N_DRIFT * self.N_CAC
         self.size_proj = self.inc_offset + self.N_TS_INC_UNIQUE
 
         coupling = context.get("coupling")
@@ -201,8 +204,9 @@ class PlomModel:
 
     def check_codegen(self):
         """Raise ModelError for what the code generator does not support."""
-        if self.drift_par_proc:
-            raise ModelError("diffusions on parameters (drift) are not supported by this build yet")
+        for p, v in zip(self.drift_par_proc, self.vol_par_proc):
+            if p not in self.par_proc or v not in self.par_proc:
+                raise ModelError("a diffusion needs a process parameter and a process volatility (%s, %s)" % (p, v))
         self._check_symbols()
 
     # ------------------------------------------------------------------ helpers
@@ -296,7 +300,8 @@ class PlomModel:
                 "dates": dates if dates is not None else [],
                 "times": times if times is not None else [0],
             },
-            "drift": {"ind_par_Xdrift_applied": [], "ind_volatility_Xdrift": []},
+            "drift": {"ind_par_Xdrift_applied": [all_order.index(x) for x in self.drift_par_proc],
+                      "ind_volatility_Xdrift": [all_order.index(x) for x in self.vol_par_proc]},
             "ind_noise_sd": sorted(set(all_order.index(w["sd"]) for w in self.white_noise if w["sd"] in all_order)),
             "remainder": self.remainder,
             "date_0": str(date_0) if date_0 is not None else None,
@@ -307,11 +312,11 @@ class PlomModel:
                 "N_TS": self.N_TS, "N_TS_INC": self.N_TS_INC, "N_TS_INC_UNIQUE": self.N_TS_INC_UNIQUE,
                 "N_DATA": len(data) if data is not None else 0,
                 "N_OBS_ALL": len(self.obs_var_def), "N_OBS_INC": self.N_OBS_INC, "N_OBS_PREV": self.N_OBS_PREV,
-                "N_DRIFT": 0, "IS_SCHOOL_TERMS": 0,
+                "N_DRIFT": self.N_DRIFT, "IS_SCHOOL_TERMS": 0,
             },
             "orders": {
                 "par_sv": self.par_sv, "par_proc": self.par_proc, "par_fixed": self.par_fixed,
-                "par_obs": self.par_obs, "ts_id": self.ts_id, "cac_id": self.cac_id, "drift_var": [],
+                "par_obs": self.par_obs, "ts_id": self.ts_id, "cac_id": self.cac_id, "drift_var": self.drift_var,
             },
             # extension consumed by this build's C host: the model the kernels were generated for
             "model": self.name,

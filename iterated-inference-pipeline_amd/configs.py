@@ -16,6 +16,8 @@ noise                the SIR model with births, deaths and white noise of the re
                      incidences, one prevalence), with time-varying covariates: population
                      sizes, birth and death rates (per week) and proportions under
                      surveillance, N = 104 weeks
+drift                the reference's example model_builder/example/drift: the noise model without
+                     white noise and with a diffusing r0 (volatility vol), same context
 
 Choices the workload names leave open (initial proportions, waning and recovery rates,
 coupling matrices, seeds) are fixed here once and used by tests and bench alike.
@@ -30,6 +32,7 @@ TRUTH = {
     "sirs4": {"beta": 0.5, "e": 0.3, "gamma": 0.2, "omega": 1.0 / 365.0, "sto": 0.1, "rep": 0.6, "phi": 0.1},
     "seir10": {"beta": 0.6, "eps": 0.4, "gamma": 0.25, "omega": 1.0 / 730.0, "sto": 0.1, "rep": 0.6, "phi": 0.1},
     "noise": {"r0": 1.8, "v": 0.1, "sto": 0.1, "rep": 0.6, "phi": 0.1},
+    "drift": {"r0": 1.8, "v": 0.1, "vol": 0.05, "rep": 0.6, "phi": 0.1},
 }
 
 X0_PROPS = {
@@ -37,9 +40,10 @@ X0_PROPS = {
     "sirs4": {"S": [0.45, 0.46, 0.44, 0.47], "I": [0.002, 0.001, 0.003, 0.0015]},
     "seir10": {"S": [0.42 + 0.005 * k for k in range(10)], "E": [0.0005] * 10, "I": [0.001 + 0.0002 * k for k in range(10)]},
     "noise": {"S": [0.7, 0.65], "I": [0.001, 0.0005]},
+    "drift": {"S": [0.7, 0.65], "I": [0.001, 0.0005]},
 }
 
-DATA_N = {"sir": 520, "sirs4": 520, "seir10": 1040, "noise": 104}
+DATA_N = {"sir": 520, "sirs4": 520, "seir10": 1040, "noise": 104, "drift": 104}
 
 WORKLOADS = {
     "sir_j1000_n52": {"model": "sir", "J": 1000, "N": 52, "precision": "fp64"},
@@ -47,6 +51,32 @@ WORKLOADS = {
     "sirs4_j100000_n520_fp32": {"model": "sirs4", "J": 100000, "N": 520, "precision": "fp32"},
     "seir10_j1000000_n1040": {"model": "seir10", "J": 1000000, "N": 1040, "precision": "fp64"},
 }
+
+def drift_setup(model, x0, params, groups=None):
+    """The drift map of a model with diffusions and its initial state: every drifting
+    parameter on the log scale (as the example's theta), one group per population unless
+    groups [n_drift][n_cac] says otherwise; the volatility of group g is the pv slot of the
+    first population of that group; the drift states start at log of the parameter's
+    natural value."""
+    import math
+    m = model.desc
+    n = m.N_DRIFT * m.N_CAC
+    g = np.asarray(groups if groups is not None else [list(range(m.N_CAC))] * m.N_DRIFT, dtype=np.int32).ravel()
+    vols = np.full(n, -1, dtype=np.int32)
+    for i, v in enumerate(m.vol_par_proc):
+        for gi in range(m.N_CAC):
+            cacs = [c for c in range(m.N_CAC) if g[i * m.N_CAC + c] == gi]
+            if cacs:
+                vols[i * m.N_CAC + gi] = m.layout["proc"][v] + cacs[0]
+    drift = {"group": g, "tr": np.ones(n, dtype=np.int32), "scale": np.zeros(n, dtype=np.int32),
+             "duration": np.zeros(n, dtype=np.int32), "min": np.zeros(n), "max": np.zeros(n),
+             "multiplier": np.ones(n), "vol_slot": vols}
+    x = np.array(x0, dtype=np.float64)
+    for i, p in enumerate(m.drift_par_proc):
+        for c in range(m.N_CAC):
+            x[m.drift_offset + i * m.N_CAC + c] = math.log(params[p])
+    return x, drift
+
 
 # parameters a proposal moves (multiplicatively, on the log scale)
 PROPOSED = {"sir": ("beta", "gamma", "rep"), "sirs4": ("beta", "gamma", "sto", "rep")}

@@ -13,6 +13,18 @@
 
 #include "plom_rng.h"
 #include "plom_psr.h"
+#include "plom_mif.h"
+
+/* The drifting parameters of a filter (diffusions, C/core/drift.c:27-42), per drifting
+ * parameter i and population c (index i * n_cac + c): the group of the population (its
+ * populations share one path), the back-transform of the drift state to the natural value
+ * (as plom_mif_natural), and the pv slot of the natural volatility of group g in
+ * vol_slot[i * n_cac + g] (-1: none); ng[i] is the number of groups.  A null view: no drift
+ * (the parameters keep their pv values). */
+typedef struct {
+    const int32_t *group, *ng, *tr, *sc, *dur, *vol_slot;
+    const double *min, *max, *mult;
+} plom_drift_view;
 
 /* PLOM_UNIFORM64/32: a value every lane of the wave holds (the filter's parameters: a
  * wave never spans two filters), moved to scalar registers on the device */

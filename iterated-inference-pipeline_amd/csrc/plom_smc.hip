@@ -87,6 +87,8 @@ struct plom_dfilter {
     const double *cov;  /* covariate table [N kmax + N + 1][cov_cs] (plom_cov.h), or the covariate
                            slots of pv with cov_cs = 0 (constant covariates) */
     int32_t cov_cs, cov_kmax;
+    plom_drift_view dv; /* drifting parameters (used when drift_on) */
+    int32_t drift_on, pad3;
     const double *times, *data;
     const int32_t *nonan;
     double *logw, *tmax, *tsum, *tsum2;
@@ -357,8 +359,8 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
         } else if constexpr (sizeof(R) == 8) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j, s_bm);
-            if constexpr (MIF) M::template interval_pp<V>((double *)X, f->pv, lp, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs);
-            else M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs);
+            if constexpr (MIF) M::template interval_pp<V>((double *)X, f->pv, lp, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs, nullptr);
+            else M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs, f->drift_on ? &f->dv : nullptr);
         } else {
             plom_nstream32 ns;
             plom_nstream32_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j);
@@ -366,9 +368,10 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
                 float lpf[M::NPP];
 #pragma unroll
                 for (int k = 0; k < M::NPP; k++) lpf[k] = (float)lp[k];
-                M::template interval_pp<V>((float *)X, f->pvf, lpf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs);
+                M::template interval_pp<V>((float *)X, f->pvf, lpf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs, nullptr);
             } else {
-                M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs);
+                M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs,
+                                        f->drift_on ? &f->dv : nullptr);
             }
         }
         R *Xout = (R *)f->X[(n + 1) & 1];
@@ -678,11 +681,13 @@ __global__ __launch_bounds__(64) void k_lineage(plom_dfilter *f)
         } else if constexpr (sizeof(R) == 8) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, jn, s_bm);
-            M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs);
+            M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs,
+                                    f->drift_on ? &f->dv : nullptr);
         } else {
             plom_nstream32 ns;
             plom_nstream32_init(&ns, f->seed, f->stream, (uint32_t)n, jn);
-            M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs);
+            M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs,
+                                    f->drift_on ? &f->dv : nullptr);
         }
 #pragma unroll
         for (int k = 0; k < M::S; k++) f->lin_state[(size_t)(n + 1) * M::S + k] = (double)X[k];
@@ -748,7 +753,7 @@ __global__ __launch_bounds__(TILE) void k_hat_tiles(plom_dfilter *f)
         double xs[M::S];
 #pragma unroll
         for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
-        M::hat_values(xs, f->pv, cov_at(f, n + 1), f->times[n + 1], v);
+        M::hat_values(xs, f->pv, cov_at(f, n + 1), f->drift_on ? &f->dv : nullptr, f->times[n + 1], v);
         if (hat_weighted(f, n)) {
             const double l = f->logw[j];
             if (l != -PLOM_INF) q = (uint64_t)((plom_exp(l - f->scal[0]) / f->scal[1]) * 0x1p62);
@@ -949,7 +954,7 @@ __global__ void k_hat0(plom_dfilter *f)
     double xs[M::S], v[M::N_HAT];
 #pragma unroll
     for (int k = 0; k < M::S; k++) xs[k] = (double)x0[k];
-    M::hat_values(xs, f->pv, cov_at(f, 0), f->times[0], v);
+    M::hat_values(xs, f->pv, cov_at(f, 0), f->drift_on ? &f->dv : nullptr, f->times[0], v);
     for (int k = 0; k < M::N_HAT; k++) f->hat[3 * k] = f->hat[3 * k + 1] = f->hat[3 * k + 2] = v[k];
 }
 
@@ -1073,7 +1078,7 @@ static void launch_hat0(int prec, hipStream_t s, plom_dfilter *df)
     { #lname,                                                                                   \
       { UNAME##_SIZE_PROJ, UNAME##_N_TS, UNAME##_N_CAC, UNAME##_N_PAR_SV, UNAME##_INC_OFFSET,   \
         UNAME##_N_TS_INC_UNIQUE, UNAME##_PV_SIZE, UNAME##_N_DEM, UNAME##_N_ENV, UNAME##_N_HAT,    \
-        UNAME##_N_PP, UNAME##_FIX_OFF, UNAME##_N_FIX },                                                                      \
+        UNAME##_N_PP, UNAME##_FIX_OFF, UNAME##_N_FIX, UNAME##_N_DRIFT, UNAME##_DRIFT_OFFSET },                                                                      \
       launch_prop<plom_model_##lname>, launch_lineage<plom_model_##lname>, launch_hat<plom_model_##lname>, \
       launch_hat0<plom_model_##lname> }
 
@@ -1082,6 +1087,7 @@ static const model_entry g_models[] = {
     MODEL_ENTRY(sirs4, PLOM_SIRS4),
     MODEL_ENTRY(seir10, PLOM_SEIR10),
     MODEL_ENTRY(noise, PLOM_NOISE),
+    MODEL_ENTRY(drift, PLOM_DRIFT),
 };
 static const int g_n_models = (int)(sizeof g_models / sizeof g_models[0]);
 
@@ -1166,7 +1172,8 @@ struct layout {
         off_th0, off_th1, off_mpw, off_mps, off_mpm, off_mpq, off_mmean, off_mvar,
         off_img, off_df, off_cnt, off_scal, off_naf, off_llike_n, off_ess_n, off_rflag, off_nonan, off_times,
         off_data, off_pv, off_pvf, off_x0, off_pm_src, off_pm_tr, off_pm_sc, off_pm_dur, off_pm_min, off_pm_max,
-        off_pm_mult, off_pm_sd, off_mutfac, off_cov, total;
+        off_pm_mult, off_pm_sd, off_mutfac, off_cov, off_dm_group, off_dm_ng, off_dm_tr, off_dm_sc, off_dm_dur,
+        off_dm_min, off_dm_max, off_dm_mult, off_dm_vol, total;   /* off_dm_vol: int32 vol slots */
 };
 
 /* cov_vals: values of the covariate table (0 without time-varying covariates) */
@@ -1231,6 +1238,16 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     PUT(off_pm_sd, NT * 8);
     PUT(off_mutfac, NT ? N * 8 : 0);
     PUT(off_cov, cov_vals * 8);
+    const size_t ND = (size_t)mi->n_drift * (size_t)mi->n_cac;
+    PUT(off_dm_group, ND * 4);
+    PUT(off_dm_ng, (size_t)mi->n_drift * 4);
+    PUT(off_dm_tr, ND * 4);
+    PUT(off_dm_sc, ND * 4);
+    PUT(off_dm_dur, ND * 4);
+    PUT(off_dm_min, ND * 8);
+    PUT(off_dm_max, ND * 8);
+    PUT(off_dm_mult, ND * 8);
+    PUT(off_dm_vol, ND * 4);
 #undef PUT
     L->total = o;
     return PLOM_OK;
@@ -1396,6 +1413,27 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     std::vector<double> covtab;
     int cov_kmax = 0;
     const int cov_on = in->cov != nullptr && mi.n_fix > 0;
+    /* drifting parameters */
+    const int drift_on = mi.n_drift > 0 && !(d->noises_off & PLOM_NO_DRIFT_BIT) && !(d->flags & PLOM_FLAG_ODE);
+    const plom_drift_map *dmap = in->drift;
+    const size_t ND = (size_t)mi.n_drift * (size_t)mi.n_cac;
+    std::vector<int32_t> dng((size_t)mi.n_drift + 1, 0);
+    if (drift_on) {
+        if (d->flags & PLOM_FLAG_MIF) return set_err(PLOM_ERR_ARG, "drifting parameters do not combine with PLOM_FLAG_MIF");
+        if (!dmap || !dmap->group || !dmap->tr || !dmap->scale || !dmap->duration || !dmap->min || !dmap->max ||
+            !dmap->multiplier || !dmap->vol_slot)
+            return set_err(PLOM_ERR_ARG, "this model has drifting parameters: it needs a complete plom_drift_map "
+                                         "(or PLOM_NO_DRIFT_BIT in noises_off)");
+        for (size_t q = 0; q < ND; q++) {
+            if (dmap->group[q] < 0 || dmap->group[q] >= mi.n_cac) return set_err(PLOM_ERR_ARG, "drift group[%zu] out of range", q);
+            if (dmap->tr[q] < 0 || dmap->tr[q] > 3 || dmap->scale[q] < 0 || dmap->scale[q] > 2)
+                return set_err(PLOM_ERR_ARG, "drift %zu: bad transform or scale", q);
+            if (dmap->vol_slot[q] < -1 || dmap->vol_slot[q] >= mi.n_pp)
+                return set_err(PLOM_ERR_ARG, "drift vol_slot[%zu]=%d is not a process parameter slot of pv", q, dmap->vol_slot[q]);
+            const size_t i = q / (size_t)mi.n_cac;
+            if (dmap->group[q] + 1 > dng[i]) dng[i] = dmap->group[q] + 1;
+        }
+    }
     if (cov_on && (d->flags & PLOM_FLAG_ODE))
         return set_err(PLOM_ERR_ARG, "the ode implementation takes the covariates as constants in pv (no time-varying covariates)");
     if (cov_on) {
@@ -1486,6 +1524,18 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     h->mpq = mif_on ? (double *)(b + L.off_mpq) : nullptr;
     h->mmean = mif_on ? (double *)(b + L.off_mmean) : nullptr;
     h->mvar = mif_on ? (double *)(b + L.off_mvar) : nullptr;
+    h->drift_on = drift_on;
+    if (drift_on) {
+        h->dv.group = (const int32_t *)(b + L.off_dm_group);
+        h->dv.ng = (const int32_t *)(b + L.off_dm_ng);
+        h->dv.tr = (const int32_t *)(b + L.off_dm_tr);
+        h->dv.sc = (const int32_t *)(b + L.off_dm_sc);
+        h->dv.dur = (const int32_t *)(b + L.off_dm_dur);
+        h->dv.min = (const double *)(b + L.off_dm_min);
+        h->dv.max = (const double *)(b + L.off_dm_max);
+        h->dv.mult = (const double *)(b + L.off_dm_mult);
+        h->dv.vol_slot = (const int32_t *)(b + L.off_dm_vol);
+    }
     if (cov_on) {
         h->cov = (const double *)(b + L.off_cov);
         h->cov_cs = mi.n_fix;
@@ -1525,6 +1575,17 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     if (d->N > 0) memcpy(at(L.off_data), in->data, (size_t)d->N * mi.n_ts * 8);
     memcpy(at(L.off_pv), in->pv, (size_t)mi.pv_size * 8);
     if (cov_on) memcpy(at(L.off_cov), covtab.data(), covtab.size() * 8);
+    if (drift_on) {
+        memcpy(at(L.off_dm_group), dmap->group, ND * 4);
+        memcpy(at(L.off_dm_ng), dng.data(), (size_t)mi.n_drift * 4);
+        memcpy(at(L.off_dm_tr), dmap->tr, ND * 4);
+        memcpy(at(L.off_dm_sc), dmap->scale, ND * 4);
+        memcpy(at(L.off_dm_dur), dmap->duration, ND * 4);
+        memcpy(at(L.off_dm_min), dmap->min, ND * 8);
+        memcpy(at(L.off_dm_max), dmap->max, ND * 8);
+        memcpy(at(L.off_dm_mult), dmap->multiplier, ND * 8);
+        memcpy(at(L.off_dm_vol), dmap->vol_slot, ND * 4);
+    }
     float *pvf = (float *)at(L.off_pvf);
     for (int i = 0; i < mi.pv_size; i++) pvf[i] = (float)in->pv[i];
     if (d->precision == PLOM_FP32) {
@@ -2395,6 +2456,8 @@ extern "C" int plom_server_submit(plom_server *S, const plom_filter_desc *d, con
     if (!in->times || !in->pv || !in->x0 || (d->N > 0 && !in->data)) return set_err(PLOM_ERR_ARG, "null input array");
     if (in->cov && mi.n_fix > 0)
         return set_err(PLOM_ERR_ARG, "served requests take the covariates as constants in pv (no time-varying covariates)");
+    if (mi.n_drift > 0 && !(d->noises_off & PLOM_NO_DRIFT_BIT) && !(d->flags & PLOM_FLAG_ODE))
+        return set_err(PLOM_ERR_ARG, "served requests run without drifting parameters (set PLOM_NO_DRIFT_BIT)");
     for (int n = 0; n < d->N; n++)
         if (!(in->times[n + 1] > in->times[n])) return set_err(PLOM_ERR_ARG, "times must increase (n=%d)", n);
     layout L;

@@ -5,5 +5,6 @@
 #include "plom_sirs4.h"
 #include "plom_seir10.h"
 #include "plom_noise.h"
-#define PLOM_N_MODELS 4
+#include "plom_drift.h"
+#define PLOM_N_MODELS 5
 #endif

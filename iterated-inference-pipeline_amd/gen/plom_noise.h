@@ -16,11 +16,13 @@
 #define PLOM_NOISE_N_PP 14
 #define PLOM_NOISE_FIX_OFF 14
 #define PLOM_NOISE_N_FIX 12
+#define PLOM_NOISE_N_DRIFT 0
+#define PLOM_NOISE_DRIFT_OFFSET 4
 
-PLOM_HD void plom_noise_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_noise_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -93,10 +95,10 @@ PLOM_HD void plom_noise_interval_full_f64(double *X, const double *pv, plom_nstr
     }
 }
 
-PLOM_HD void plom_noise_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_noise_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -149,10 +151,10 @@ PLOM_HD void plom_noise_interval_no_dem_sto_f64(double *X, const double *pv, plo
     }
 }
 
-PLOM_HD void plom_noise_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_noise_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -219,10 +221,10 @@ PLOM_HD void plom_noise_interval_no_env_sto_f64(double *X, const double *pv, plo
     }
 }
 
-PLOM_HD void plom_noise_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_noise_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -352,9 +354,9 @@ PLOM_HD void plom_noise_obs_moments(const double *obs, const double *pvd, const 
     }
 }
 
-PLOM_HD void plom_noise_hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out)
+PLOM_HD void plom_noise_hat_values(const double *X, const double *pvd, const double *cvd, const plom_drift_view *dm, double t, double *out)
 {
-    (void)t; (void)pvd; (void)cvd;
+    (void)t; (void)pvd; (void)cvd; (void)dm;
     out[0] = X[0];
     out[1] = X[1];
     out[2] = X[2];
@@ -366,9 +368,9 @@ PLOM_HD void plom_noise_hat_values(const double *X, const double *pvd, const dou
     plom_noise_obs_moments(obs, pvd, cvd, t, out + 6, var);
 }
 
-PLOM_HD void plom_noise_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_noise_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -501,9 +503,9 @@ PLOM_HD void plom_noise_interval_psr_full_f64(double *X, const double *pv, plom_
     }
 }
 
-PLOM_HD void plom_noise_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_noise_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -682,29 +684,29 @@ PLOM_HD int plom_noise_interval_ode_f64(double *X, const double *pv, const doubl
     return plom_noise_ode_integrate(X, pv, 0, cv, t0, t1, dt, eps_abs, eps_rel);
 }
 
-PLOM_HD void plom_noise_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel)
+PLOM_HD void plom_noise_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel, const plom_drift_view *dm)
 {
     if (noises_off & PLOM_IMPL_ODE) {
         (void)plom_noise_interval_ode_f64(X, pv, cv, t0, t1, dt, eps_abs, eps_rel);
         return;
     }
     if (noises_off & PLOM_IMPL_PSR) {
-        if (noises_off & PLOM_NO_ENV_STO) plom_noise_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else plom_noise_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        if (noises_off & PLOM_NO_ENV_STO) plom_noise_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else plom_noise_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
         return;
     }
     const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);
-    if (v == 0u) plom_noise_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-    else if (v == PLOM_NO_DEM_STO) plom_noise_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-    else if (v == PLOM_NO_ENV_STO) plom_noise_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-    else plom_noise_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+    if (v == 0u) plom_noise_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+    else if (v == PLOM_NO_DEM_STO) plom_noise_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+    else if (v == PLOM_NO_ENV_STO) plom_noise_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+    else plom_noise_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
 }
 
 #if defined(__HIPCC__)
-static __device__ inline __attribute__((always_inline)) void plom_noise_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_noise_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -777,10 +779,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -833,10 +835,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -903,10 +905,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -966,9 +968,9 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_proj2obs
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_noise_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_noise_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -1101,9 +1103,9 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_noise_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_noise_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -1282,10 +1284,10 @@ static __device__ inline __attribute__((always_inline)) int plom_noise_interval_
     return plom_noise_ode_integrate_pp(X, pv, lp, cv, t0, t1, dt, eps_abs, eps_rel);
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_noise_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_noise_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -1358,10 +1360,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -1414,10 +1416,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -1484,10 +1486,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -1547,10 +1549,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_proj2obs
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_noise_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_noise_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -1623,10 +1625,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -1679,10 +1681,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -1749,10 +1751,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_noise_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -1858,40 +1860,40 @@ static __device__ inline __attribute__((always_inline)) double plom_noise_loglik
 }
 
 struct plom_model_noise {
-    static constexpr int S = 6, N_TS = 4, INC_OFFSET = 4, N_INC = 2, PV = 26, N_CAC = 2, N_HAT = 10, NPP = 14, FIX_OFF = 14, N_FIX = 12;
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+    static constexpr int S = 6, N_TS = 4, INC_OFFSET = 4, N_INC = 2, PV = 26, N_CAC = 2, N_HAT = 10, NPP = 14, FIX_OFF = 14, N_FIX = 12, N_DRIFT = 0;
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_noise_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_noise_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_noise_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_noise_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 4) plom_noise_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 5) plom_noise_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_noise_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_noise_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_noise_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_noise_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 4) plom_noise_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 5) plom_noise_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_noise_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_noise_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_noise_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_noise_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 4) plom_noise_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 5) plom_noise_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_noise_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_noise_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_noise_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_noise_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 4) plom_noise_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 5) plom_noise_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, const double *cvd, double *obs) { plom_noise_proj2obs_f64(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_noise_proj2obs_pp_f64(X, pvd, lpd, cvd, obs); }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_noise_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_noise_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_noise_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_noise_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_noise_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_noise_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_noise_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_noise_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_noise_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_noise_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_noise_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_noise_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_noise_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_noise_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_noise_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_noise_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_noise_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_noise_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
@@ -1899,7 +1901,7 @@ struct plom_model_noise {
     static __device__ inline __attribute__((always_inline)) int interval_ode_pp(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_noise_interval_ode_pp_f64(X, pv, lp, cv, t0, t1, dt, ea, er); }
     static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor) { return plom_noise_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor) { return plom_noise_loglik_pp(obs, y, pvd, lpd, cvd, t, like_min, n_floor); }
-    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out) { plom_noise_hat_values(X, pvd, cvd, t, out); }
+    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, const plom_drift_view *dm, double t, double *out) { plom_noise_hat_values(X, pvd, cvd, dm, t, out); }
     static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var) { plom_noise_obs_moments(obs, pvd, cvd, t, mean, var); }
 };
 #endif

@@ -16,11 +16,13 @@
 #define PLOM_SEIR10_N_PP 70
 #define PLOM_SEIR10_FIX_OFF 70
 #define PLOM_SEIR10_N_FIX 10
+#define PLOM_SEIR10_N_DRIFT 0
+#define PLOM_SEIR10_DRIFT_OFFSET 30
 
-PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -440,10 +442,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
     }
 }
 
-PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -783,10 +785,10 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, pl
     }
 }
 
-PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -1176,10 +1178,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
     }
 }
 
-PLOM_HD void plom_seir10_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_seir10_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -1658,9 +1660,9 @@ PLOM_HD void plom_seir10_obs_moments(const double *obs, const double *pvd, const
     }
 }
 
-PLOM_HD void plom_seir10_hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out)
+PLOM_HD void plom_seir10_hat_values(const double *X, const double *pvd, const double *cvd, const plom_drift_view *dm, double t, double *out)
 {
-    (void)t; (void)pvd; (void)cvd;
+    (void)t; (void)pvd; (void)cvd; (void)dm;
     out[0] = X[0];
     out[1] = X[1];
     out[2] = X[2];
@@ -1706,9 +1708,9 @@ PLOM_HD void plom_seir10_hat_values(const double *X, const double *pvd, const do
     plom_seir10_obs_moments(obs, pvd, cvd, t, out + 40, var);
 }
 
-PLOM_HD void plom_seir10_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_seir10_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -2529,9 +2531,9 @@ PLOM_HD void plom_seir10_interval_psr_full_f64(double *X, const double *pv, plom
     }
 }
 
-PLOM_HD void plom_seir10_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_seir10_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -3619,29 +3621,29 @@ PLOM_HD int plom_seir10_interval_ode_f64(double *X, const double *pv, const doub
     return plom_seir10_ode_integrate(X, pv, 0, cv, t0, t1, dt, eps_abs, eps_rel);
 }
 
-PLOM_HD void plom_seir10_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel)
+PLOM_HD void plom_seir10_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel, const plom_drift_view *dm)
 {
     if (noises_off & PLOM_IMPL_ODE) {
         (void)plom_seir10_interval_ode_f64(X, pv, cv, t0, t1, dt, eps_abs, eps_rel);
         return;
     }
     if (noises_off & PLOM_IMPL_PSR) {
-        if (noises_off & PLOM_NO_ENV_STO) plom_seir10_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else plom_seir10_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        if (noises_off & PLOM_NO_ENV_STO) plom_seir10_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else plom_seir10_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
         return;
     }
     const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);
-    if (v == 0u) plom_seir10_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-    else if (v == PLOM_NO_DEM_STO) plom_seir10_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-    else if (v == PLOM_NO_ENV_STO) plom_seir10_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-    else plom_seir10_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+    if (v == 0u) plom_seir10_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+    else if (v == PLOM_NO_DEM_STO) plom_seir10_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+    else if (v == PLOM_NO_ENV_STO) plom_seir10_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+    else plom_seir10_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
 }
 
 #if defined(__HIPCC__)
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -4061,10 +4063,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -4404,10 +4406,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -4797,10 +4799,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -5125,9 +5127,9 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_proj2ob
     obs[9] = (double)X[39];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -5948,9 +5950,9 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -7038,10 +7040,10 @@ static __device__ inline __attribute__((always_inline)) int plom_seir10_interval
     return plom_seir10_ode_integrate_pp(X, pv, lp, cv, t0, t1, dt, eps_abs, eps_rel);
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -7461,10 +7463,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -7804,10 +7806,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -8197,10 +8199,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -8525,10 +8527,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_proj2ob
     obs[9] = (double)X[39];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -8948,10 +8950,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -9291,10 +9293,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -9684,10 +9686,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -10112,40 +10114,40 @@ static __device__ inline __attribute__((always_inline)) double plom_seir10_logli
 }
 
 struct plom_model_seir10 {
-    static constexpr int S = 40, N_TS = 10, INC_OFFSET = 30, N_INC = 10, PV = 180, N_CAC = 10, N_HAT = 50, NPP = 70, FIX_OFF = 70, N_FIX = 10;
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+    static constexpr int S = 40, N_TS = 10, INC_OFFSET = 30, N_INC = 10, PV = 180, N_CAC = 10, N_HAT = 50, NPP = 70, FIX_OFF = 70, N_FIX = 10, N_DRIFT = 0;
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_seir10_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 4) plom_seir10_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 5) plom_seir10_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_seir10_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 4) plom_seir10_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 5) plom_seir10_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_seir10_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 4) plom_seir10_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 5) plom_seir10_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_seir10_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 4) plom_seir10_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 5) plom_seir10_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, const double *cvd, double *obs) { plom_seir10_proj2obs_f64(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_seir10_proj2obs_pp_f64(X, pvd, lpd, cvd, obs); }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_seir10_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_seir10_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_seir10_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_seir10_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_seir10_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_seir10_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
@@ -10153,7 +10155,7 @@ struct plom_model_seir10 {
     static __device__ inline __attribute__((always_inline)) int interval_ode_pp(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_seir10_interval_ode_pp_f64(X, pv, lp, cv, t0, t1, dt, ea, er); }
     static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor) { return plom_seir10_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor) { return plom_seir10_loglik_pp(obs, y, pvd, lpd, cvd, t, like_min, n_floor); }
-    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out) { plom_seir10_hat_values(X, pvd, cvd, t, out); }
+    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, const plom_drift_view *dm, double t, double *out) { plom_seir10_hat_values(X, pvd, cvd, dm, t, out); }
     static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var) { plom_seir10_obs_moments(obs, pvd, cvd, t, mean, var); }
 };
 #endif

@@ -16,11 +16,13 @@
 #define PLOM_SIR_N_PP 4
 #define PLOM_SIR_FIX_OFF 4
 #define PLOM_SIR_N_FIX 1
+#define PLOM_SIR_N_DRIFT 0
+#define PLOM_SIR_DRIFT_OFFSET 2
 
-PLOM_HD void plom_sir_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_sir_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     X[2] = 0.0; /* reset_inc */
@@ -48,10 +50,10 @@ PLOM_HD void plom_sir_interval_full_f64(double *X, const double *pv, plom_nstrea
     }
 }
 
-PLOM_HD void plom_sir_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_sir_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     X[2] = 0.0; /* reset_inc */
@@ -75,10 +77,10 @@ PLOM_HD void plom_sir_interval_no_dem_sto_f64(double *X, const double *pv, plom_
     }
 }
 
-PLOM_HD void plom_sir_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_sir_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     X[2] = 0.0; /* reset_inc */
@@ -106,10 +108,10 @@ PLOM_HD void plom_sir_interval_no_env_sto_f64(double *X, const double *pv, plom_
     }
 }
 
-PLOM_HD void plom_sir_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_sir_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     X[2] = 0.0; /* reset_inc */
@@ -167,9 +169,9 @@ PLOM_HD void plom_sir_obs_moments(const double *obs, const double *pvd, const do
     }
 }
 
-PLOM_HD void plom_sir_hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out)
+PLOM_HD void plom_sir_hat_values(const double *X, const double *pvd, const double *cvd, const plom_drift_view *dm, double t, double *out)
 {
-    (void)t; (void)pvd; (void)cvd;
+    (void)t; (void)pvd; (void)cvd; (void)dm;
     out[0] = X[0];
     out[1] = X[1];
     out[2] = cvd[0] - (X[0] + X[1]);
@@ -178,9 +180,9 @@ PLOM_HD void plom_sir_hat_values(const double *X, const double *pvd, const doubl
     plom_sir_obs_moments(obs, pvd, cvd, t, out + 3, var);
 }
 
-PLOM_HD void plom_sir_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_sir_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     X[2] = 0.0; /* reset_inc */
@@ -234,9 +236,9 @@ PLOM_HD void plom_sir_interval_psr_full_f64(double *X, const double *pv, plom_ns
     }
 }
 
-PLOM_HD void plom_sir_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_sir_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     X[2] = 0.0; /* reset_inc */
@@ -320,29 +322,29 @@ PLOM_HD int plom_sir_interval_ode_f64(double *X, const double *pv, const double 
     return plom_sir_ode_integrate(X, pv, 0, cv, t0, t1, dt, eps_abs, eps_rel);
 }
 
-PLOM_HD void plom_sir_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel)
+PLOM_HD void plom_sir_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel, const plom_drift_view *dm)
 {
     if (noises_off & PLOM_IMPL_ODE) {
         (void)plom_sir_interval_ode_f64(X, pv, cv, t0, t1, dt, eps_abs, eps_rel);
         return;
     }
     if (noises_off & PLOM_IMPL_PSR) {
-        if (noises_off & PLOM_NO_ENV_STO) plom_sir_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else plom_sir_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        if (noises_off & PLOM_NO_ENV_STO) plom_sir_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else plom_sir_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
         return;
     }
     const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);
-    if (v == 0u) plom_sir_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-    else if (v == PLOM_NO_DEM_STO) plom_sir_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-    else if (v == PLOM_NO_ENV_STO) plom_sir_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-    else plom_sir_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+    if (v == 0u) plom_sir_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+    else if (v == PLOM_NO_DEM_STO) plom_sir_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+    else if (v == PLOM_NO_ENV_STO) plom_sir_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+    else plom_sir_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
 }
 
 #if defined(__HIPCC__)
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     X[2] = 0.0f; /* reset_inc */
@@ -370,10 +372,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_f
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     X[2] = 0.0f; /* reset_inc */
@@ -397,10 +399,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     X[2] = 0.0f; /* reset_inc */
@@ -428,10 +430,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     X[2] = 0.0f; /* reset_inc */
@@ -461,9 +463,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_proj2obs_f
     obs[0] = (double)X[2];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     X[2] = 0.0; /* reset_inc */
@@ -517,9 +519,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_p
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     X[2] = 0.0; /* reset_inc */
@@ -603,10 +605,10 @@ static __device__ inline __attribute__((always_inline)) int plom_sir_interval_od
     return plom_sir_ode_integrate_pp(X, pv, lp, cv, t0, t1, dt, eps_abs, eps_rel);
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     X[2] = 0.0; /* reset_inc */
@@ -634,10 +636,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_f
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     X[2] = 0.0; /* reset_inc */
@@ -661,10 +663,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     X[2] = 0.0; /* reset_inc */
@@ -692,10 +694,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     X[2] = 0.0; /* reset_inc */
@@ -725,10 +727,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_proj2obs_p
     obs[0] = (double)X[2];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     X[2] = 0.0f; /* reset_inc */
@@ -756,10 +758,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_f
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     X[2] = 0.0f; /* reset_inc */
@@ -783,10 +785,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     X[2] = 0.0f; /* reset_inc */
@@ -814,10 +816,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sir_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     X[2] = 0.0f; /* reset_inc */
@@ -866,40 +868,40 @@ static __device__ inline __attribute__((always_inline)) double plom_sir_loglik_p
 }
 
 struct plom_model_sir {
-    static constexpr int S = 3, N_TS = 1, INC_OFFSET = 2, N_INC = 1, PV = 5, N_CAC = 1, N_HAT = 4, NPP = 4, FIX_OFF = 4, N_FIX = 1;
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+    static constexpr int S = 3, N_TS = 1, INC_OFFSET = 2, N_INC = 1, PV = 5, N_CAC = 1, N_HAT = 4, NPP = 4, FIX_OFF = 4, N_FIX = 1, N_DRIFT = 0;
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_sir_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_sir_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 4) plom_sir_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 5) plom_sir_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_sir_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_sir_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 4) plom_sir_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 5) plom_sir_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_sir_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_sir_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 4) plom_sir_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 5) plom_sir_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_sir_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_sir_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 4) plom_sir_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 5) plom_sir_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, const double *cvd, double *obs) { plom_sir_proj2obs_f64(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sir_proj2obs_pp_f64(X, pvd, lpd, cvd, obs); }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_sir_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_sir_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_sir_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_sir_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_sir_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_sir_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_sir_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_sir_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_sir_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_sir_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sir_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
@@ -907,7 +909,7 @@ struct plom_model_sir {
     static __device__ inline __attribute__((always_inline)) int interval_ode_pp(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_sir_interval_ode_pp_f64(X, pv, lp, cv, t0, t1, dt, ea, er); }
     static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor) { return plom_sir_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor) { return plom_sir_loglik_pp(obs, y, pvd, lpd, cvd, t, like_min, n_floor); }
-    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out) { plom_sir_hat_values(X, pvd, cvd, t, out); }
+    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, const plom_drift_view *dm, double t, double *out) { plom_sir_hat_values(X, pvd, cvd, dm, t, out); }
     static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var) { plom_sir_obs_moments(obs, pvd, cvd, t, mean, var); }
 };
 #endif

@@ -16,11 +16,13 @@
 #define PLOM_SIRS4_N_PP 28
 #define PLOM_SIRS4_FIX_OFF 28
 #define PLOM_SIRS4_N_FIX 4
+#define PLOM_SIRS4_N_DRIFT 0
+#define PLOM_SIRS4_DRIFT_OFFSET 8
 
-PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -151,10 +153,10 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
     }
 }
 
-PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -261,10 +263,10 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
     }
 }
 
-PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -383,10 +385,10 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
     }
 }
 
-PLOM_HD void plom_sirs4_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_sirs4_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -560,9 +562,9 @@ PLOM_HD void plom_sirs4_obs_moments(const double *obs, const double *pvd, const 
     }
 }
 
-PLOM_HD void plom_sirs4_hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out)
+PLOM_HD void plom_sirs4_hat_values(const double *X, const double *pvd, const double *cvd, const plom_drift_view *dm, double t, double *out)
 {
-    (void)t; (void)pvd; (void)cvd;
+    (void)t; (void)pvd; (void)cvd; (void)dm;
     out[0] = X[0];
     out[1] = X[1];
     out[2] = X[2];
@@ -580,9 +582,9 @@ PLOM_HD void plom_sirs4_hat_values(const double *X, const double *pvd, const dou
     plom_sirs4_obs_moments(obs, pvd, cvd, t, out + 12, var);
 }
 
-PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -818,9 +820,9 @@ PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_
     }
 }
 
-PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -1142,29 +1144,29 @@ PLOM_HD int plom_sirs4_interval_ode_f64(double *X, const double *pv, const doubl
     return plom_sirs4_ode_integrate(X, pv, 0, cv, t0, t1, dt, eps_abs, eps_rel);
 }
 
-PLOM_HD void plom_sirs4_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel)
+PLOM_HD void plom_sirs4_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel, const plom_drift_view *dm)
 {
     if (noises_off & PLOM_IMPL_ODE) {
         (void)plom_sirs4_interval_ode_f64(X, pv, cv, t0, t1, dt, eps_abs, eps_rel);
         return;
     }
     if (noises_off & PLOM_IMPL_PSR) {
-        if (noises_off & PLOM_NO_ENV_STO) plom_sirs4_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else plom_sirs4_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        if (noises_off & PLOM_NO_ENV_STO) plom_sirs4_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else plom_sirs4_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
         return;
     }
     const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);
-    if (v == 0u) plom_sirs4_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-    else if (v == PLOM_NO_DEM_STO) plom_sirs4_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-    else if (v == PLOM_NO_ENV_STO) plom_sirs4_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-    else plom_sirs4_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+    if (v == 0u) plom_sirs4_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+    else if (v == PLOM_NO_DEM_STO) plom_sirs4_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+    else if (v == PLOM_NO_ENV_STO) plom_sirs4_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+    else plom_sirs4_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
 }
 
 #if defined(__HIPCC__)
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -1295,10 +1297,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -1405,10 +1407,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -1527,10 +1529,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -1634,9 +1636,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_proj2obs
     obs[3] = (double)X[11];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -1872,9 +1874,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -2196,10 +2198,10 @@ static __device__ inline __attribute__((always_inline)) int plom_sirs4_interval_
     return plom_sirs4_ode_integrate_pp(X, pv, lp, cv, t0, t1, dt, eps_abs, eps_rel);
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -2330,10 +2332,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -2440,10 +2442,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -2562,10 +2564,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -2669,10 +2671,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_proj2obs
     obs[3] = (double)X[11];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -2803,10 +2805,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -2913,10 +2915,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -3035,10 +3037,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -3188,40 +3190,40 @@ static __device__ inline __attribute__((always_inline)) double plom_sirs4_loglik
 }
 
 struct plom_model_sirs4 {
-    static constexpr int S = 12, N_TS = 4, INC_OFFSET = 8, N_INC = 4, PV = 48, N_CAC = 4, N_HAT = 16, NPP = 28, FIX_OFF = 28, N_FIX = 4;
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+    static constexpr int S = 12, N_TS = 4, INC_OFFSET = 8, N_INC = 4, PV = 48, N_CAC = 4, N_HAT = 16, NPP = 28, FIX_OFF = 28, N_FIX = 4, N_DRIFT = 0;
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_sirs4_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 4) plom_sirs4_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 5) plom_sirs4_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_sirs4_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 4) plom_sirs4_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 5) plom_sirs4_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_sirs4_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 4) plom_sirs4_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 5) plom_sirs4_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_sirs4_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 4) plom_sirs4_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 5) plom_sirs4_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, const double *cvd, double *obs) { plom_sirs4_proj2obs_f64(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sirs4_proj2obs_pp_f64(X, pvd, lpd, cvd, obs); }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_sirs4_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_sirs4_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
     {
-        if constexpr (V == 0) plom_sirs4_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
-        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs);
+        if constexpr (V == 0) plom_sirs4_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_sirs4_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sirs4_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
@@ -3229,7 +3231,7 @@ struct plom_model_sirs4 {
     static __device__ inline __attribute__((always_inline)) int interval_ode_pp(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_sirs4_interval_ode_pp_f64(X, pv, lp, cv, t0, t1, dt, ea, er); }
     static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor) { return plom_sirs4_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor) { return plom_sirs4_loglik_pp(obs, y, pvd, lpd, cvd, t, like_min, n_floor); }
-    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, double t, double *out) { plom_sirs4_hat_values(X, pvd, cvd, t, out); }
+    static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, const plom_drift_view *dm, double t, double *out) { plom_sirs4_hat_values(X, pvd, cvd, dm, t, out); }
     static __device__ inline __attribute__((always_inline)) void obs_moments(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var) { plom_sirs4_obs_moments(obs, pvd, cvd, t, mean, var); }
 };
 #endif

@@ -30,7 +30,8 @@ class ModelInfo(C.Structure):
     _fields_ = [("size_proj", C.c_int32), ("n_ts", C.c_int32), ("n_cac", C.c_int32),
                 ("n_par_sv", C.c_int32), ("inc_offset", C.c_int32), ("n_inc", C.c_int32),
                 ("pv_size", C.c_int32), ("n_dem", C.c_int32), ("n_env", C.c_int32), ("n_hat", C.c_int32),
-                ("n_pp", C.c_int32), ("fix_off", C.c_int32), ("n_fix", C.c_int32)]
+                ("n_pp", C.c_int32), ("fix_off", C.c_int32), ("n_fix", C.c_int32), ("n_drift", C.c_int32),
+                ("drift_offset", C.c_int32)]
 
 
 class FilterDesc(C.Structure):
@@ -50,10 +51,16 @@ class Covariate(C.Structure):
     _fields_ = [("n", C.c_int32), ("pad", C.c_int32), ("x", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double))]
 
 
+class DriftMap(C.Structure):
+    _fields_ = [("group", C.POINTER(C.c_int32)), ("tr", C.POINTER(C.c_int32)), ("scale", C.POINTER(C.c_int32)),
+                ("duration", C.POINTER(C.c_int32)), ("min", C.POINTER(C.c_double)), ("max", C.POINTER(C.c_double)),
+                ("multiplier", C.POINTER(C.c_double)), ("vol_slot", C.POINTER(C.c_int32))]
+
+
 class FilterInputs(C.Structure):
     _fields_ = [("times", C.POINTER(C.c_double)), ("data", C.POINTER(C.c_double)),
                 ("pv", C.POINTER(C.c_double)), ("x0", C.POINTER(C.c_double)), ("mif", C.POINTER(MifMap)),
-                ("cov", C.POINTER(Covariate))]
+                ("cov", C.POINTER(Covariate)), ("drift", C.POINTER(DriftMap))]
 
 
 class FilterOutputs(C.Structure):
@@ -199,7 +206,9 @@ class Model:
         self.S = self.desc.size_proj
         self.n_ts = self.desc.N_TS
         self.n_cac = self.desc.N_CAC
-        self.n_hat = self.desc.N_PAR_SV * self.desc.N_CAC + (self.desc.N_CAC if self.desc.remainder else 0) + self.n_ts
+        self.n_drift = self.desc.N_DRIFT
+        self.n_hat = self.desc.N_PAR_SV * self.desc.N_CAC + (self.desc.N_CAC if self.desc.remainder else 0) + self.n_ts \
+            + self.n_drift * self.n_cac
         self.n_pp = self.desc.layout["npp"]
         self.n_fix = self.desc.layout["n_fix"]
         self.fix_off = self.desc.layout["fix_off"]
@@ -368,6 +377,23 @@ def _desc(model, J, N, precision, noises_off, flags, seed, stream, dt, like_min,
     return d
 
 
+def _drift_map(model, drift):
+    """(pointer to plom_drift_map, the arrays it points into) or (None, None); drift: dict of
+    group, tr, scale, duration, min, max, multiplier, vol_slot, each [n_drift * n_cac]."""
+    if drift is None:
+        return None, None
+    n = model.n_drift * model.n_cac
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    keep = {k: (i32 if k in ("group", "tr", "scale", "duration", "vol_slot") else f64)(drift[k])
+            for k in ("group", "tr", "scale", "duration", "min", "max", "multiplier", "vol_slot")}
+    if any(v.size != n for v in keep.values()):
+        raise ValueError("every drift map array needs n_drift * n_cac = %d entries" % n)
+    dm = DriftMap(iptr(keep["group"]), iptr(keep["tr"]), iptr(keep["scale"]), iptr(keep["duration"]), dptr(keep["min"]),
+                  dptr(keep["max"]), dptr(keep["multiplier"]), iptr(keep["vol_slot"]))
+    return C.pointer(dm), (dm, keep)
+
+
 def _cov_array(model, cov):
     """(pointer to plom_covariate[n_fix], the arrays it points into) or (None, None)."""
     if cov is None:
@@ -388,7 +414,7 @@ class Filter:
     """A device-resident filter (plom_filter_create / run / read)."""
 
     def __init__(self, ctx, model, J, times, data, pv, x0, precision="fp64", noises_off=0, flags=0,
-                 seed=1, stream=0, dt=0.25, like_min=1e-17, mif=None, cov=None, eps_abs=0.0, eps_rel=0.0):
+                 seed=1, stream=0, dt=0.25, like_min=1e-17, mif=None, cov=None, eps_abs=0.0, eps_rel=0.0, drift=None):
         """mif (with FLAG_MIF): dict of numpy arrays pv_src [n_pp], tr, scale, duration, min, max,
         multiplier, sd [n_theta] (plom_mif_map).  cov: time-varying covariates, one (x, y) knot
         pair per covariate slot of pv (plom_covariate; model.n_fix of them)."""
@@ -417,8 +443,9 @@ class Filter:
             mmap = MifMap(iptr(m["pv_src"]), iptr(m["tr"]), iptr(m["scale"]), iptr(m["duration"]), dptr(m["min"]),
                           dptr(m["max"]), dptr(m["multiplier"]), dptr(m["sd"]))
         covs, self._cov = _cov_array(model, cov)
+        dmap, self._drift = _drift_map(model, drift)
         inp = FilterInputs(dptr(self.times), dptr(self.data), dptr(self.pv), dptr(self.x0),
-                           C.pointer(mmap) if mmap is not None else None, covs)
+                           C.pointer(mmap) if mmap is not None else None, covs, dmap)
         self.h = C.c_void_p()
         check(lib().plom_filter_create(ctx.h, C.byref(self.desc), C.byref(inp), C.byref(self.h)))
 

@@ -27,9 +27,11 @@ def lib():
         ip = C.POINTER(C.c_int32)
         L.oracle_smc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                  C.c_double, C.c_double, dp, dp, dp, dp, C.c_int, C.c_int,
-                                 dp, ip, dp, dp, ip, dp, dp, dp, ip, dp, dp, dp, dp, dp, ip, dp, dp, C.c_double, C.c_double]
+                                 dp, ip, dp, dp, ip, dp, dp, dp, ip, dp, dp, dp, dp, dp, ip, dp, dp, C.c_double, C.c_double,
+                                 ip, ip, ip, ip, dp, dp, dp, ip]
         L.oracle_trajectory.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-                                        C.c_double, dp, dp, dp, dp, ip, dp, dp, C.c_double, C.c_double]
+                                        C.c_double, dp, dp, dp, dp, ip, dp, dp, C.c_double, C.c_double,
+                                        ip, ip, ip, ip, dp, dp, dp, ip]
         L.oracle_ode_test.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, ip]
         L.oracle_cov_table.argtypes = [C.c_int, C.c_int, dp, C.c_double, ip, dp, dp, dp, ip]
         L.oracle_spline.argtypes = [C.c_int, dp, dp, C.c_int, dp, dp]
@@ -67,6 +69,22 @@ IMPL_ODE = 0x200   # the ode implementation (PLOM_IMPL_ODE)
 
 def _impl(noises_off, psr, ode):
     return noises_off | (IMPL_PSR if psr else 0) | (IMPL_ODE if ode else 0)
+
+
+_keep_drift = []
+
+
+def _drift_args(drift):
+    """drift: dict of group, tr, scale, duration [n_drift * n_cac] ints and min, max,
+    multiplier doubles and vol_slot ints (plom_drift_map), or None."""
+    if drift is None:
+        return (None,) * 8
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    arr = [i32(drift["group"]), i32(drift["tr"]), i32(drift["scale"]), i32(drift["duration"]), f64(drift["min"]),
+           f64(drift["max"]), f64(drift["multiplier"]), i32(drift["vol_slot"])]
+    _keep_drift[:] = arr
+    return tuple(_ip(a) for a in arr[:4]) + tuple(_dp(a) for a in arr[4:7]) + (_ip(arr[7]),)
 
 
 def _eps(eps_abs, eps_rel):
@@ -134,7 +152,8 @@ def spline(x, y, t):
 
 def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=0, dt=0.25,
         like_min=1e-17, mode=CANONICAL, nthreads=1, ancestry=False, final_state=False, logw=False,
-        lineage=False, all_states=False, hat=False, psr=False, cov=None, ode=False, eps_abs=0.0, eps_rel=0.0):
+        lineage=False, all_states=False, hat=False, psr=False, cov=None, ode=False, eps_abs=0.0, eps_rel=0.0,
+        drift=None):
     """The filter restated on the CPU.  lineage: the sampled path (slots [N], states
     [N+1][S]); all_states: every particle's state after every interval, before resampling
     ([N+1][J][S], for tests on small runs); psr: the Euler-multinomial implementation;
@@ -166,7 +185,7 @@ def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=
     rc = lib().oracle_smc(model_id, J, N, noises_off, flags, seed, stream, dt, like_min, _dp(times), _dp(data),
                           _dp(pv), _dp(x0), mode, nthreads, C.byref(ll), C.byref(naf), _dp(ll_n), _dp(ess),
                           _ip(anc), _dp(fs), _dp(lw), C.byref(el), _ip(lsl), _dp(lst), _dp(alls), _dp(ht), _dp(pr), _dp(lob),
-                          *_cov_args(cov), *_eps(eps_abs, eps_rel))
+                          *_cov_args(cov), *_eps(eps_abs, eps_rel), *_drift_args(drift))
     if rc != 0:
         raise RuntimeError("oracle_smc failed: %d" % rc)
     return {"loglik": ll.value, "n_all_fail": naf.value, "loglik_n": ll_n, "ess_n": ess,
@@ -225,7 +244,7 @@ def mif_natural(tr, sc, dur, x, a, b, mult):
 
 
 def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=0.25, psr=False, cov=None, ode=False,
-               eps_abs=0.0, eps_rel=0.0):
+               eps_abs=0.0, eps_rel=0.0, drift=None):
     noises_off = _impl(noises_off, psr, ode)
     times = np.ascontiguousarray(times, dtype=np.float64)
     N = len(times) - 1
@@ -234,7 +253,7 @@ def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=
     rc = lib().oracle_trajectory(model_id, N, noises_off, seed, stream, j, dt, _dp(times),
                                  _dp(np.ascontiguousarray(pv, dtype=np.float64)),
                                  _dp(np.ascontiguousarray(x0, dtype=np.float64)), _dp(out), *_cov_args(cov),
-                                 *_eps(eps_abs, eps_rel))
+                                 *_eps(eps_abs, eps_rel), *_drift_args(drift))
     if rc != 0:
         raise RuntimeError("oracle_trajectory failed")
     return out

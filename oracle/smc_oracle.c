@@ -56,15 +56,15 @@
 typedef unsigned __int128 u128;
 
 typedef void (*interval_fn)(double *, const double *, plom_nstream64 *, double, double, double, double, unsigned int,
-                            const double *, int, double, double);
+                            const double *, int, double, double, const plom_drift_view *);
 typedef void (*proj2obs_fn)(const double *, const double *, const double *, double *);
 typedef double (*loglik_fn)(const double *, const double *, const double *, const double *, double, double, int *);
 typedef void (*moments_fn)(const double *, const double *, const double *, double, double *, double *);
-typedef void (*hat_fn)(const double *, const double *, const double *, double, double *);
+typedef void (*hat_fn)(const double *, const double *, const double *, const plom_drift_view *, double, double *);
 
 typedef struct {
     const char *name;
-    int S, n_ts, n_hat, n_pp, pv_size, fix_off, n_fix;
+    int S, n_ts, n_hat, n_pp, pv_size, fix_off, n_fix, n_drift, n_cac;
     interval_fn interval;
     proj2obs_fn proj2obs;
     loglik_fn loglik;
@@ -74,7 +74,7 @@ typedef struct {
 
 #define ORACLE_MODEL(lname, UNAME)                                                                 \
     { #lname, UNAME##_SIZE_PROJ, UNAME##_N_TS, UNAME##_N_HAT, UNAME##_N_PP, UNAME##_PV_SIZE,           \
-      UNAME##_FIX_OFF, UNAME##_N_FIX,                                                                  \
+      UNAME##_FIX_OFF, UNAME##_N_FIX, UNAME##_N_DRIFT, UNAME##_N_CAC,                                  \
       plom_##lname##_interval_f64,                                                                   \
       plom_##lname##_proj2obs_f64, plom_##lname##_loglik, plom_##lname##_obs_moments, plom_##lname##_hat_values }
 
@@ -83,6 +83,7 @@ static const oracle_model MODELS[] = {
     ORACLE_MODEL(sirs4, PLOM_SIRS4),
     ORACLE_MODEL(seir10, PLOM_SEIR10),
     ORACLE_MODEL(noise, PLOM_NOISE),
+    ORACLE_MODEL(drift, PLOM_DRIFT),
 };
 #define N_MODELS ((int)(sizeof MODELS / sizeof MODELS[0]))
 
@@ -192,6 +193,32 @@ int oracle_spline(int n, const double *x, const double *y, int nt, const double 
 }
 
 int oracle_model_n_fix(int m) { return MODELS[m].n_fix; }
+
+/* ------------------------------------------------------------------ drifting parameters */
+/* the view of a drift map given as flat arrays [n_drift * n_cac] (group NULL: no drift) */
+typedef struct {
+    plom_drift_view v;
+    int32_t ng[64];
+    int on;
+} drift_arg;
+
+static void drift_build(const oracle_model *M, const int32_t *group, const int32_t *tr, const int32_t *sc,
+                        const int32_t *dur, const double *mn, const double *mx, const double *mult, const int32_t *vol,
+                        drift_arg *out)
+{
+    memset(out, 0, sizeof *out);
+    if (!group || M->n_drift == 0) return;
+    for (int i = 0; i < M->n_drift && i < 64; i++) {
+        out->ng[i] = 0;
+        for (int c = 0; c < M->n_cac; c++)
+            if (group[i * M->n_cac + c] + 1 > out->ng[i]) out->ng[i] = group[i * M->n_cac + c] + 1;
+    }
+    out->v.group = group; out->v.ng = out->ng; out->v.tr = tr; out->v.sc = sc; out->v.dur = dur;
+    out->v.min = mn; out->v.max = mx; out->v.mult = mult; out->v.vol_slot = vol;
+    out->on = 1;
+}
+
+static const plom_drift_view *drift_view(const drift_arg *d) { return d->on ? &d->v : NULL; }
 int oracle_model_fix_off(int m) { return MODELS[m].fix_off; }
 
 /* ------------------------------------------------------------------ canonical trees */
@@ -217,6 +244,7 @@ typedef struct {
     const double *pv, *times, *data;
     size_t pv_stride;   /* 0: one pv for every particle; else particle j uses pv + j * pv_stride */
     const cov_view *cv;
+    const plom_drift_view *dm;
     int J, n, nonan, filter, n_ts;
     uint32_t seed, stream, noises_off;
     double dt, sqdt, like_min, eps_abs, eps_rel;
@@ -234,7 +262,7 @@ static void propagate_range(prop_job *p)
         plom_nstream64 ns;
         plom_nstream64_init(&ns, p->seed, p->stream, (uint32_t)p->n, (uint32_t)j, plom_bm_tab);
         M->interval(x, pv, &ns, p->times[p->n], p->times[p->n + 1], p->dt, p->sqdt, p->noises_off, cv_steps(p->cv, p->n),
-                    p->cv->cs, p->eps_abs, p->eps_rel);
+                    p->cv->cs, p->eps_abs, p->eps_rel, p->dm);
         if (p->filter && p->nonan) {
             int nf = 0;
             const double *cvd = cv_at(p->cv, p->n + 1);
@@ -405,12 +433,13 @@ static int hat_cmp(const void *a, const void *b)
     return x < y ? -1 : (x > y);
 }
 
-static void hat_row(const oracle_model *M, const double *X, int J, const double *pv, const double *cvd, double t,
-                    const uint64_t *q, uint64_t Q, double *row, double *vals, hat_item *items, double *buf)
+static void hat_row(const oracle_model *M, const double *X, int J, const double *pv, const double *cvd,
+                    const plom_drift_view *dm, double t, const uint64_t *q, uint64_t Q, double *row, double *vals,
+                    hat_item *items, double *buf)
 {
     const int NH = M->n_hat, S = M->S;
     const int T = (J + TILE - 1) / TILE, P = next_pow2(T);
-    for (int j = 0; j < J; j++) M->hat_values(X + (size_t)j * S, pv, cvd, t, vals + (size_t)j * NH);
+    for (int j = 0; j < J; j++) M->hat_values(X + (size_t)j * S, pv, cvd, dm, t, vals + (size_t)j * NH);
     for (int k = 0; k < NH; k++) {
         double *tiles = buf + TILE;
         for (int tt = 0; tt < P; tt++) {
@@ -494,10 +523,15 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
                double *final_state, double *logw_out, double *elapsed_s,
                int32_t *lin_slots, double *lin_state, double *all_states, double *hat, double *pred_res,
                double *lin_obs, const int32_t *cov_n, const double *cov_x, const double *cov_y, double eps_abs,
-               double eps_rel)
+               double eps_rel, const int32_t *dm_group, const int32_t *dm_tr, const int32_t *dm_sc,
+               const int32_t *dm_dur, const double *dm_min, const double *dm_max, const double *dm_mult,
+               const int32_t *dm_vol)
 {
     if (model < 0 || model >= N_MODELS || J < 1 || N < 0) return -1;
     const oracle_model *M = &MODELS[model];
+    drift_arg da;
+    drift_build(M, dm_group, dm_tr, dm_sc, dm_dur, dm_min, dm_max, dm_mult, dm_vol, &da);
+    const plom_drift_view *dm = drift_view(&da);
     cov_view cv;
     const int crc = cov_build(M, N, times, dt, cov_n, cov_x, cov_y, pv, &cv);
     if (crc) { free(cv.owned); return crc; }
@@ -529,7 +563,7 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
         offspring = malloc((size_t)J * sizeof(uint32_t));
         if (!hvals || !hbuf || !hitems || !hq || !offspring || M->n_ts > 16) return -2;
         double v0[256];
-        M->hat_values(x0, pv, cv_at(&cv, 0), times[0], v0);
+        M->hat_values(x0, pv, cv_at(&cv, 0), dm, times[0], v0);
         for (int k = 0; k < NH; k++) hat[3 * k] = hat[3 * k + 1] = hat[3 * k + 2] = v0[k];
     }
     const int filter = !(flags & FLAG_NO_FILTER);
@@ -543,7 +577,7 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
         for (int k = 0; k < n_ts; k++) nonan += !isnan(data[(size_t)n * n_ts + k]);
         prop_job job;
         memset(&job, 0, sizeof job);
-        job.M = M; job.X = X; job.logw = logw; job.pv = pv; job.cv = &cv; job.times = times; job.data = data;
+        job.M = M; job.X = X; job.logw = logw; job.pv = pv; job.cv = &cv; job.dm = dm; job.times = times; job.data = data;
         job.J = J; job.n = n; job.nonan = nonan; job.filter = filter; job.n_ts = n_ts;
         job.seed = seed; job.stream = stream; job.noises_off = noises_off;
         job.dt = dt; job.sqdt = plom_sqrt(dt); job.like_min = like_min; job.mode = mode;
@@ -574,8 +608,8 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
             } else {
                 for (int j = 0; j < J; j++) hq[j] = 1;
             }
-            hat_row(M, X, J, pv, cv_at(&cv, n + 1), times[n + 1], hq, Q, hat + (size_t)(n + 1) * NH * 3, hvals, hitems,
-                    hbuf);
+            hat_row(M, X, J, pv, cv_at(&cv, n + 1), dm, times[n + 1], hq, Q, hat + (size_t)(n + 1) * NH * 3, hvals,
+                    hitems, hbuf);
             double *pr = pred_res + (size_t)n * n_ts * 2;
             if (filter && nonan)
                 pred_res_row(M, X, J, pv, cv_at(&cv, n + 1), times[n + 1], data + (size_t)n * n_ts, select, offspring, pr);
@@ -622,7 +656,7 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
             plom_nstream64 ns;
             plom_nstream64_init(&ns, seed, stream, (uint32_t)n, (uint32_t)lin_slots[n], plom_bm_tab);
             M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off, cv_steps(&cv, n), cv.cs, eps_abs,
-                        eps_rel);
+                        eps_rel, dm);
             memcpy(lin_state + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
         }
         if (lin_obs)
@@ -816,10 +850,14 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
  * out[(n+1)*S + k]; out[0..S) = x0 (with incidences reset) */
 int oracle_trajectory(int model, int N, uint32_t noises_off, uint32_t seed, uint32_t stream, uint32_t j,
                       double dt, const double *times, const double *pv, const double *x0, double *out,
-                      const int32_t *cov_n, const double *cov_x, const double *cov_y, double eps_abs, double eps_rel)
+                      const int32_t *cov_n, const double *cov_x, const double *cov_y, double eps_abs, double eps_rel,
+                      const int32_t *dm_group, const int32_t *dm_tr, const int32_t *dm_sc, const int32_t *dm_dur,
+                      const double *dm_min, const double *dm_max, const double *dm_mult, const int32_t *dm_vol)
 {
     if (model < 0 || model >= N_MODELS) return -1;
     const oracle_model *M = &MODELS[model];
+    drift_arg da;
+    drift_build(M, dm_group, dm_tr, dm_sc, dm_dur, dm_min, dm_max, dm_mult, dm_vol, &da);
     cov_view cv;
     const int crc = cov_build(M, N, times, dt, cov_n, cov_x, cov_y, pv, &cv);
     if (crc) { free(cv.owned); return crc; }
@@ -831,7 +869,7 @@ int oracle_trajectory(int model, int N, uint32_t noises_off, uint32_t seed, uint
         plom_nstream64 ns;
         plom_nstream64_init(&ns, seed, stream, (uint32_t)n, j, plom_bm_tab);
         M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off, cv_steps(&cv, n), cv.cs, eps_abs,
-                    eps_rel);
+                    eps_rel, drift_view(&da));
         memcpy(out + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
     }
     free(cv.owned);

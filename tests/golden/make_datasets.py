@@ -91,14 +91,18 @@ def main():
     if not os.path.exists(path):
         with open(path, "w") as f:
             json.dump(noise_context(), f, indent=1)
-    for name in ("sir", "sirs4", "seir10", "noise"):
+    for name in ("sir", "sirs4", "seir10", "noise", "drift"):
         model = Model.get(name)
         N = configs.DATA_N[name]
         times = np.arange(N + 1, dtype=np.float64) * 7.0
         pv = model.pack(configs.TRUTH[name])
         x0 = model.x0(configs.X0_PROPS[name])
         cov = model.covariates() if model.time_varying() else None
-        traj = oracle.trajectory(model.id, times, pv, x0, seed=configs.DATA_SEED, stream=0xDA7A, dt=0.25, cov=cov)
+        drift = None
+        if model.n_drift:
+            x0, drift = configs.drift_setup(model, x0, configs.TRUTH[name])
+        traj = oracle.trajectory(model.id, times, pv, x0, seed=configs.DATA_SEED, stream=0xDA7A, dt=0.25, cov=cov,
+                                 drift=drift)
         ctab, kmax = oracle.cov_table(model.id, times, 0.25, cov) if cov is not None else (None, 0)
         obs = np.zeros(model.n_ts)
         data = np.zeros((N, model.n_ts))
@@ -119,7 +123,7 @@ def main():
         for k, ts in enumerate(model.desc.ts_id):
             ctx["data"]["source"][ts] = {"t0": t0.isoformat(),
                                          "value": [[d, float(v)] for d, v in zip(dates, data[:, k])]}
-        for meta in ctx.get("metadata", []) if name != "noise" else []:
+        for meta in ctx.get("metadata", []) if name not in ("noise", "drift") else []:
             for cac, src in meta["source"].items():
                 v = src["value"][0][1]
                 src["value"] = [[t0.isoformat(), v], [dates[len(dates) // 2], v], [dates[-1], v]]

@@ -28,15 +28,17 @@ def test_library_exports_every_declared_symbol():
 
 def test_host_side_queries():
     L = smc.lib()
-    assert L.plom_model_count() == 4
-    for i, name in enumerate(["sir", "sirs4", "seir10", "noise"]):
+    assert L.plom_model_count() == 5
+    for i, name in enumerate(["sir", "sirs4", "seir10", "noise", "drift"]):
         assert L.plom_model_find(name.encode()) == i
         info = smc.ModelInfo()
         assert L.plom_model_info_get(i, C.byref(info)) == 0
         m = smc.Model.get(name)
         assert info.size_proj == m.S and info.n_ts == m.n_ts and info.pv_size == m.layout["size"]
-        assert info.n_hat == m.n_hat == {"sir": 4, "sirs4": 16, "seir10": 50, "noise": 10}[name]
-        assert info.fix_off == m.fix_off and info.n_fix == m.n_fix == {"sir": 1, "sirs4": 4, "seir10": 10, "noise": 12}[name]
+        assert info.n_hat == m.n_hat == {"sir": 4, "sirs4": 16, "seir10": 50, "noise": 10, "drift": 12}[name]
+        assert info.fix_off == m.fix_off and info.n_fix == m.n_fix == {"sir": 1, "sirs4": 4, "seir10": 10, "noise": 12,
+                                                                       "drift": 12}[name]
+        assert info.n_drift == m.n_drift == (1 if name == "drift" else 0)
     assert L.plom_model_find(b"nope") < 0
     assert b"unknown model" in L.plom_last_error()
 

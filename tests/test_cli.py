@@ -373,3 +373,47 @@ def test_cli_ode_matches_api(tmp_path, ctx):
         ll_cli = float([l for l in r.stdout.splitlines() if l.startswith("logV_17g:")][0].split(":")[1])
         g = smc.run(ctx, m, 1, times, data, pv, x0, flags=smc.FLAG_ODE, eps_abs=eps[0], eps_rel=eps[1], seed=2)
         assert ll_cli == g.loglik
+
+
+def test_drift_dry_run(tmp_path):
+    """The drift example: the drift states start at the drifting parameter's theta
+    (theta_driftIC2Xdrift, prediction_util.c:119-131), one per population of its groups."""
+    d = model_dir(tmp_path, "drift")
+    r = run_cli(d, ["sde", "-J", "10", "--dry_run"], os.path.join(PKG, "examples", "drift", "theta.json"))
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    m = smc.Model.get("drift")
+    off = m.desc.drift_offset
+    assert out["x0"][off:off + 2] == [math.log(1.8), math.log(1.8)]
+    vol = m.layout["proc"]["vol"]
+    assert out["pv"][vol:vol + 2] == pytest.approx([0.05, 0.05], rel=1e-15)
+    r = run_cli(d, ["-J", "10", "--dry_run"], os.path.join(PKG, "examples", "drift", "theta.json"))
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_drift_matches_api(tmp_path, ctx):
+    """`smc sde` on the drift example equals the API with the same drift map (one group per
+    city, r0 on the log scale, volatility of the city's vol); hat carries one drift column
+    triple per group; --no_diff keeps r0 at its pv value."""
+    from iip_amd import configs
+    d = model_dir(tmp_path, "drift")
+    theta = os.path.join(PKG, "examples", "drift", "theta.json")
+    dry = json.loads(run_cli(d, ["sde", "-J", "1500", "--dry_run"], theta).stdout)
+    m = smc.Model.get("drift")
+    times, data = m.dataset()
+    pv, x0 = np.array(dry["pv"]), np.array(dry["x0"])
+    _, drift = configs.drift_setup(m, x0, configs.TRUTH["drift"])
+    print("cli drift", flush=True)
+    r = run_cli(d, ["sde", "-J", "1500", "-b", "-d"], theta)
+    assert r.returncode == 0, r.stderr
+    ll_cli = float([l for l in r.stdout.splitlines() if l.startswith("logV_17g:")][0].split(":")[1])
+    g = smc.run(ctx, m, 1500, times, data, pv, x0, seed=2, stream=0, cov=m.covariates(), drift=drift)
+    assert ll_cli == g.loglik
+    head = open(d / "hat_0.csv").readline().strip()
+    assert head.endswith("low95:diff:r0:city1__all,diff:r0:city1__all,high95:diff:r0:city1__all,"
+                         "low95:diff:r0:city2__all,diff:r0:city2__all,high95:diff:r0:city2__all")
+    r = run_cli(d, ["sde", "-J", "1500", "-b", "-d", "-e", "--no_diff"], theta)
+    ll_nd = float([l for l in r.stdout.splitlines() if l.startswith("logV_17g:")][0].split(":")[1])
+    g2 = smc.run(ctx, m, 1500, times, data, pv, x0, seed=2, stream=0, cov=m.covariates(), noises_off=smc.NO_DRIFT)
+    assert ll_nd == g2.loglik != ll_cli

@@ -77,6 +77,9 @@ extern "C" {
                                          adaptive Runge-Kutta-Fehlberg 4(5) under the error control
                                          eps_abs / eps_rel (f_prediction_ode, C/core/prediction_util.c:176-193);
                                          FP64 only; noises_off is ignored; covariates constant (pv) */
+#define PLOM_FLAG_ALL_STATES 256u     /* keep every particle's state after every interval (smc -r, print_X,
+                                         C/core/print.c:360-392): plom_filter_outputs.all_states /
+                                         all_obs; keeps the ancestry */
 #define PLOM_FLAG_MIF 32u             /* per-particle parameters (iterated filtering): needs
                                          plom_filter_inputs.mif and plom_filter_mif_start before
                                          every run; not with NO_FILTER, HAT or LINEAGE */
@@ -206,6 +209,12 @@ typedef struct {
                             the particles' theta at interval n, NaN without data; row 0 NaN */
     double *mif_var;     /* optional [(N+1)*n_theta]: the same rows, unweighted variance over the
                             particles (divisor J-1) */
+    double *all_states;  /* optional [(N+1)*J*size_proj] (PLOM_FLAG_ALL_STATES): row n, particle j = the
+                            cloud after interval n-1 as resampling leaves it (resample_X,
+                            C/core/smc.c:142-161; row 0 = x0) */
+    double *all_obs;     /* optional [(N+1)*J*n_ts*2]: per row, particle and series, the observation
+                            mean of that state at times[n] and one observation drawn from the
+                            observation model (obs_mean / observation, core_template.c:369-412) */
 } plom_filter_outputs;
 
 typedef struct plom_ctx plom_ctx;

@@ -666,10 +666,14 @@ void cli_x_header(const cli_problem *P, FILE *f)
 
 void cli_x_rows(const cli_problem *P, FILE *f, int index, const double *lineage, const double *lineage_obs)
 {
-    const int S = P->mi.size_proj, N_CAC = P->N_CAC, N_TS = P->N_TS;
-    for (int r = P->N; r >= 0; r--) {
-        const double *x = lineage + (size_t)r * S;
-        const double *ob = lineage_obs + (size_t)r * N_TS * 2;
+    const int S = P->mi.size_proj, N_TS = P->N_TS;
+    for (int r = P->N; r >= 0; r--) cli_x_row(P, f, index, r, lineage + (size_t)r * S, lineage_obs + (size_t)r * N_TS * 2);
+}
+
+void cli_x_row(const cli_problem *P, FILE *f, int index, int r, const double *x, const double *ob)
+{
+    const int N_CAC = P->N_CAC, N_TS = P->N_TS;
+    {
         fprintf(f, "%d,%g,", index, P->times[r]);
         for (int k = 0; k < P->n_sv * N_CAC; k++) fprintf(f, "%g,", x[k]);
         if (P->has_rem)

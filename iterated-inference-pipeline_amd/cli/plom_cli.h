@@ -121,6 +121,9 @@ void cli_trace_row(const cli_problem *P, FILE *f, int index, const double *theta
  * last row first, as sample_traj_and_print (print.c:715-784) */
 void cli_x_header(const cli_problem *P, FILE *f);
 void cli_x_rows(const cli_problem *P, FILE *f, int index, const double *lineage, const double *lineage_obs);
+/* one X row (print_p_X, print.c:271-348): index, time of row r, states, remainders, observation
+ * means ob[2 ts], drifting parameters, drawn observations ob[2 ts + 1] */
+void cli_x_row(const cli_problem *P, FILE *f, int index, int r, const double *x, const double *ob);
 
 /* plom_print_done (print.c:33-67): guesses replaced by f_inv(theta), then the theta JSON
  * on stdout (pipe, no trailing newline) or in <path>.done_<id>.json */

@@ -22,8 +22,7 @@
  * Implementations "sde", "psr" (the default, as main.c:223-238: FP64, the initial state
  * rounded to counts, prediction_util.c:108-110) and "ode" (FP64, adaptive RKF45 under
  * --eps_abs / --eps_rel, every noise off).
- * Differences: the X output
- * of -r is not written (a note goes to stderr); hat and pred_res use this build's
+ * Differences: hat, pred_res and the X output of -r use this build's
  * arithmetic (exact integer weights for the 95 % bounds, tree sums);
  * --prior is not supported; -N is accepted and ignored (the GPU replaces threads).
  * Extensions: --fp32 (FP32 propagation), --device <k>, --seed <k> (random key, default
@@ -145,7 +144,6 @@ int main(int argc, char *argv[])
         printf("]}\n");
         return 0;
     }
-    if (print_X) fprintf(stderr, "note: the X output (-r) is not written by this build\n");
     const int want_hat = print_hat || (print_pred_res && filter);
 
     snprintf(str, sizeof str, "Starting plom-smc with the following options: i = %d, J = %d, LIKE_MIN = %g, n_threads = %d",
@@ -165,7 +163,8 @@ int main(int argc, char *argv[])
     d.noises_off = noises_off;
     d.eps_abs = eps_abs;
     d.eps_rel = eps_rel;
-    d.flags = (filter ? 0u : PLOM_FLAG_NO_FILTER) | (want_hat ? PLOM_FLAG_HAT : 0u) | impl_flags;
+    d.flags = (filter ? 0u : PLOM_FLAG_NO_FILTER) | (want_hat ? PLOM_FLAG_HAT : 0u) | (print_X ? PLOM_FLAG_ALL_STATES : 0u) |
+              impl_flags;
     d.seed = (uint32_t)(seed_opt >= 0 ? seed_opt : 2 + general_id);
     d.stream = 0;
     d.dt = dt;
@@ -180,6 +179,12 @@ int main(int argc, char *argv[])
     out.ess_n = ess_n;
     out.hat = hat;
     out.pred_res = pres;
+    /* -r: every particle after every interval (print_X, smc.c:463-465) */
+    double *xall = print_X ? calloc((size_t)(N + 1) * J * mi.size_proj + 1, sizeof(double)) : NULL;
+    double *xobs = print_X ? calloc((size_t)(N + 1) * J * N_TS * 2 + 1, sizeof(double)) : NULL;
+    if (print_X && (!xall || !xobs)) cli_die("not enough host memory for the X output (-r)");
+    out.all_states = xall;
+    out.all_obs = xobs;
     if (plom_smc_run(ctx, &d, &in, &out) != PLOM_OK) cli_die(plom_last_error());
     plom_ctx_destroy(ctx);
 
@@ -202,6 +207,16 @@ int main(int argc, char *argv[])
         cli_trace_header(&P, ft);
         cli_trace_row(&P, ft, 0, theta, out.loglik);
         fclose(ft);
+    }
+    /* X_<id>.csv (header_X, print.c:86-129; print_X, print.c:360-392): row 0 then every interval,
+     * every particle of the resampled cloud */
+    if (print_X) {
+        FILE *fx = cli_fopen(path, "X", general_id);
+        cli_x_header(&P, fx);
+        for (int n = 0; n <= N; n++)
+            for (int j = 0; j < J; j++)
+                cli_x_row(&P, fx, j, n, xall + ((size_t)n * J + j) * mi.size_proj, xobs + ((size_t)n * J + j) * N_TS * 2);
+        fclose(fx);
     }
     /* hat_<id>.csv (header_hat, print.c:151-186; print_p_hat, print.c:445-573) */
     const pj *par_sv = P.par_sv, *cac_id = P.cac_id, *ts_id = P.ts_id;

@@ -182,15 +182,16 @@ PLOM_HD uint64_t plom_lineage_num(uint32_t seed, uint32_t stream)
     return 2ull * (uint64_t)w[0] + 1ull;
 }
 
-/* the observation row n of a lineage: out[2 ts] = mean, out[2 ts + 1] = max(mean + sd z, 0)
- * with z the Box-Muller values of Philox(ctr = (0, n, 3, 0), key = (seed, stream)), two
- * series per 128-bit draw (ctr word 3 counts the draws) */
-PLOM_HD void plom_lineage_obs(uint32_t seed, uint32_t stream, uint32_t n, int n_ts, const double *mean,
-                              const double *var, const double *tab, double *out)
+/* one observation row n of slot s: out[2 ts] = mean, out[2 ts + 1] = max(mean + sd z, 0) with
+ * z the Box-Muller values of Philox(ctr = (s, n, 3, 0), key = (seed, stream)), two series per
+ * 128-bit draw (ctr word 3 counts the draws); slot 0 is the lineage, slot j + 1 particle j of
+ * the stored clouds */
+PLOM_HD void plom_obs_draw(uint32_t seed, uint32_t stream, uint32_t n, uint32_t slot, int n_ts, const double *mean,
+                           const double *var, const double *tab, double *out)
 {
     for (int ts = 0; ts < n_ts; ts += 2) {
         uint32_t w[4];
-        const uint32_t ctr[4] = { 0u, n, PLOM_PURPOSE_OBS, (uint32_t)(ts / 2) };
+        const uint32_t ctr[4] = { slot, n, PLOM_PURPOSE_OBS, (uint32_t)(ts / 2) };
         const uint32_t key[2] = { seed, stream };
         plom_philox4x32_10(ctr, key, w);
         double z[2];
@@ -201,6 +202,13 @@ PLOM_HD void plom_lineage_obs(uint32_t seed, uint32_t stream, uint32_t n, int n_
             out[2 * (ts + k) + 1] = (y > 0.0) ? y : 0.0;
         }
     }
+}
+
+/* the observation row n of a lineage (slot 0) */
+PLOM_HD void plom_lineage_obs(uint32_t seed, uint32_t stream, uint32_t n, int n_ts, const double *mean,
+                              const double *var, const double *tab, double *out)
+{
+    plom_obs_draw(seed, stream, n, 0u, n_ts, mean, var, tab, out);
 }
 
 /* the resampling uniform of interval n as the integer numerator 2x+1 of U = (2x+1)/2^33 */

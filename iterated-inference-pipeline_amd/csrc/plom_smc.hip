@@ -117,6 +117,7 @@ struct plom_dfilter {
     double *th[2];                  /* [n_theta][Jp] theta of the particles, ping-pong like X */
     double *mpw, *mps, *mpm, *mpq;  /* tile partials: [T] sum w, [n_theta][T] sum w theta, mean, M2 */
     double *mmean, *mvar;           /* [(N+1)][n_theta] weighted mean and variance of theta */
+    void *allX;                     /* PLOM_FLAG_ALL_STATES: [(N+1)][S][Jp] every propagated cloud */
 };
 
 /* ------------------------------------------------------------------ covariate rows */
@@ -341,6 +342,11 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
 #pragma unroll
         for (int k = 0; k < M::S; k++) X[k] = Xin[(size_t)k * Jp + a];
     }
+    if (n == 0 && f->allX && valid) {
+        R *A = (R *)f->allX;
+#pragma unroll
+        for (int k = 0; k < M::S; k++) A[(size_t)k * Jp + j] = X[k];
+    }
     /* per-particle slots of pv (MIF only; the array is unused otherwise) */
     double lp[MIF ? M::NPP : 1];
     if constexpr (MIF) {
@@ -377,6 +383,11 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
         R *Xout = (R *)f->X[(n + 1) & 1];
 #pragma unroll
         for (int k = 0; k < M::S; k++) Xout[(size_t)k * Jp + j] = X[k];
+        if (f->allX) {
+            R *A = (R *)f->allX + (size_t)(n + 1) * M::S * Jp;
+#pragma unroll
+            for (int k = 0; k < M::S; k++) A[(size_t)k * Jp + j] = X[k];
+        }
     }
 
     if ((f->flags & PLOM_FLAG_NO_FILTER) || f->nonan[n] == 0) return;
@@ -1000,7 +1011,13 @@ struct model_entry {
     void (*launch_lineage)(int prec, int variant, hipStream_t s, plom_dfilter *df);
     void (*launch_hat)(int prec, hipStream_t s, plom_dfilter *df, int ntiles, int n_hat);
     void (*launch_hat0)(int prec, hipStream_t s, plom_dfilter *df);
+    /* host builds of the generated observation functions (stored clouds, read on the host) */
+    void (*proj2obs_h)(const double *X, const double *pvd, const double *cvd, double *obs);
+    void (*moments_h)(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var);
 };
+
+/* the Box-Muller table for host code (the kernels read the __constant__ one) */
+static const double g_bm_tab_host[PLOM_BM_TAB_SIZE] = PLOM_BM_TAB_VALUES;
 
 /* variants 0-3: SDE noise switches; 4, 5: psr with / without white noise (FP64 only) */
 template <class M, typename R, bool MIF>
@@ -1080,7 +1097,7 @@ static void launch_hat0(int prec, hipStream_t s, plom_dfilter *df)
         UNAME##_N_TS_INC_UNIQUE, UNAME##_PV_SIZE, UNAME##_N_DEM, UNAME##_N_ENV, UNAME##_N_HAT,    \
         UNAME##_N_PP, UNAME##_FIX_OFF, UNAME##_N_FIX, UNAME##_N_DRIFT, UNAME##_DRIFT_OFFSET },                                                                      \
       launch_prop<plom_model_##lname>, launch_lineage<plom_model_##lname>, launch_hat<plom_model_##lname>, \
-      launch_hat0<plom_model_##lname> }
+      launch_hat0<plom_model_##lname>, plom_##lname##_proj2obs_f64, plom_##lname##_obs_moments }
 
 static const model_entry g_models[] = {
     MODEL_ENTRY(sir, PLOM_SIR),
@@ -1158,6 +1175,8 @@ struct plom_filter {
     size_t off_mutfac;      /* PLOM_FLAG_MIF: image offset of mut_fac */
     int mif_ready;          /* PLOM_FLAG_MIF: a starting cloud is uploaded and not yet consumed */
     std::vector<int32_t> nonan;       /* per interval: observed series (host copy) */
+    std::vector<double> pv_h, times_h, cov_h;   /* host copies for the stored clouds' observations */
+    int cov_kmax_h;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1173,7 +1192,7 @@ struct layout {
         off_img, off_df, off_cnt, off_scal, off_naf, off_llike_n, off_ess_n, off_rflag, off_nonan, off_times,
         off_data, off_pv, off_pvf, off_x0, off_pm_src, off_pm_tr, off_pm_sc, off_pm_dur, off_pm_min, off_pm_max,
         off_pm_mult, off_pm_sd, off_mutfac, off_cov, off_dm_group, off_dm_ng, off_dm_tr, off_dm_sc, off_dm_dur,
-        off_dm_min, off_dm_max, off_dm_mult, off_dm_vol, total;   /* off_dm_vol: int32 vol slots */
+        off_dm_min, off_dm_max, off_dm_mult, off_dm_vol, off_allX, total;   /* off_dm_vol: int32 vol slots */
 };
 
 /* cov_vals: values of the covariate table (0 without time-varying covariates) */
@@ -1194,7 +1213,8 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     PUT(off_tsum2, T * 8);
     PUT(off_ttot, T * 8);
     PUT(off_tincl, T * 8);
-    PUT(off_anc, (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_LINEAGE)) ? N * J * 4 : 0);
+    PUT(off_anc, (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_LINEAGE | PLOM_FLAG_ALL_STATES)) ? N * J * 4 : 0);
+    PUT(off_allX, (d->flags & PLOM_FLAG_ALL_STATES) ? (N + 1) * S * Jp * rs : 0);
     PUT(off_lslots, (d->flags & PLOM_FLAG_LINEAGE) ? N * 4 : 0);
     PUT(off_lstate, (d->flags & PLOM_FLAG_LINEAGE) ? (N + 1) * S * 8 : 0);
     PUT(off_lobs, (d->flags & PLOM_FLAG_LINEAGE) ? (N + 1) * (size_t)mi->n_ts * 2 * 8 : 0);
@@ -1493,7 +1513,8 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     h->n_all_fail = (int32_t *)(b + L.off_naf);
     h->llike_n = (double *)(b + L.off_llike_n); h->ess_n = (double *)(b + L.off_ess_n);
     h->rflag = (int32_t *)(b + L.off_rflag);
-    h->ancestry = (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_LINEAGE)) ? (int32_t *)(b + L.off_anc) : nullptr;
+    h->ancestry = (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_LINEAGE | PLOM_FLAG_ALL_STATES)) ? (int32_t *)(b + L.off_anc) : nullptr;
+    h->allX = (d->flags & PLOM_FLAG_ALL_STATES) ? (void *)(b + L.off_allX) : nullptr;
     h->lin_slots = (d->flags & PLOM_FLAG_LINEAGE) ? (int32_t *)(b + L.off_lslots) : nullptr;
     h->lin_state = (d->flags & PLOM_FLAG_LINEAGE) ? (double *)(b + L.off_lstate) : nullptr;
     h->lin_obs = (d->flags & PLOM_FLAG_LINEAGE) ? (double *)(b + L.off_lobs) : nullptr;
@@ -1547,6 +1568,12 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     }
     f->df = (plom_dfilter *)(b + L.off_df);
     f->off_pv = L.off_pv; f->off_pvf = L.off_pvf; f->off_x0 = L.off_x0;
+    if (d->flags & PLOM_FLAG_ALL_STATES) {
+        f->pv_h.assign(in->pv, in->pv + mi.pv_size);
+        f->times_h.assign(in->times, in->times + d->N + 1);
+        if (cov_on) f->cov_h = covtab;
+        f->cov_kmax_h = cov_kmax;
+    }
     f->off_mutfac = L.off_mutfac;
 
     /* host image: descriptor, zeroed counters and outputs, inputs */
@@ -1633,6 +1660,7 @@ extern "C" int plom_filter_update(plom_ctx *c, plom_filter *f, const double *pv,
     }
     if (pv) {
         memcpy(base, pv, (size_t)P * 8);
+        if (!f->pv_h.empty()) f->pv_h.assign(pv, pv + P);
         float *pvf = (float *)(base + (f->off_pvf - f->off_pv));
         for (int i = 0; i < P; i++) pvf[i] = (float)pv[i];
     }
@@ -1729,7 +1757,7 @@ static void enqueue_final(plom_ctx *c, plom_filter *f, const int2 *tiles, plom_d
 
 static int needs_final(const plom_filter *f)
 {
-    return f->desc.N > 0 && (f->desc.flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_FINAL_STATE));
+    return f->desc.N > 0 && (f->desc.flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_FINAL_STATE | PLOM_FLAG_ALL_STATES));
 }
 
 extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
@@ -1818,9 +1846,53 @@ static int filter_read_on(plom_ctx *c, plom_filter *f, plom_filter_outputs *o, h
                 HIPCHK(hipMemcpyAsync(o->final_state + (size_t)k * J, (const double *)src + (size_t)k * h->Jp, J * 8, hipMemcpyDeviceToHost, s));
         }
     }
+    /* the stored clouds: the propagated cloud of every interval and the ancestry */
+    const int want_all = o->all_states || o->all_obs;
+    std::vector<double> allX;
+    std::vector<float> allXf;
+    std::vector<int32_t> anc;
+    const size_t S = (size_t)f->info.size_proj, Jp = (size_t)h->Jp;
+    if (want_all) {
+        if (!h->allX) return set_err(PLOM_ERR_STATE, "all states requested but PLOM_FLAG_ALL_STATES not set");
+        if (f->desc.precision == PLOM_FP32) {
+            allXf.resize((N + 1) * S * Jp);
+            HIPCHK(hipMemcpyAsync(allXf.data(), h->allX, allXf.size() * 4, hipMemcpyDeviceToHost, s));
+        } else {
+            allX.resize((N + 1) * S * Jp);
+            HIPCHK(hipMemcpyAsync(allX.data(), h->allX, allX.size() * 8, hipMemcpyDeviceToHost, s));
+        }
+        anc.resize(N * J + 1);
+        if (N) HIPCHK(hipMemcpyAsync(anc.data(), h->ancestry, N * J * 4, hipMemcpyDeviceToHost, s));
+    }
     HIPCHK(hipStreamSynchronize(s));
     if (!tmp.empty())
         for (size_t i = 0; i < tmp.size(); i++) o->final_state[i] = tmp[i];
+    if (want_all) {
+        /* row n = the cloud after interval n-1 as resampling leaves it: particle j holds its
+         * ancestor's state (identity where the interval was not resampled) */
+        const model_entry *me = &g_models[f->desc.model_id];
+        const int NTS = f->info.n_ts;
+        std::vector<double> x(S), obs((size_t)NTS + 1), mean((size_t)NTS + 1), var((size_t)NTS + 1);
+        for (size_t n = 0; n <= N; n++) {
+            const double *cvd = f->cov_h.empty()
+                ? f->pv_h.data() + f->info.fix_off
+                : f->cov_h.data() + (N * (size_t)f->cov_kmax_h + n) * (size_t)f->info.n_fix;
+            for (size_t j = 0; j < J; j++) {
+                const size_t src = n ? (size_t)anc[(n - 1) * J + j] : j;
+                for (size_t k = 0; k < S; k++) {
+                    const size_t at = (n * S + k) * Jp + src;
+                    x[k] = allX.empty() ? (double)allXf[at] : allX[at];
+                }
+                if (o->all_states) memcpy(o->all_states + (n * J + j) * S, x.data(), S * 8);
+                if (o->all_obs) {
+                    me->proj2obs_h(x.data(), f->pv_h.data(), cvd, obs.data());
+                    me->moments_h(obs.data(), f->pv_h.data(), cvd, f->times_h[n], mean.data(), var.data());
+                    plom_obs_draw(f->desc.seed, f->desc.stream, (uint32_t)n, (uint32_t)(j + 1), NTS, mean.data(), var.data(),
+                                  g_bm_tab_host, o->all_obs + (n * J + j) * (size_t)NTS * 2);
+                }
+            }
+        }
+    }
     /* rows without data (and row 0) carry no moments */
     for (size_t r = 0; r <= N && (o->mif_mean || o->mif_var) && NT; r++)
         if (r == 0 || f->nonan[r - 1] == 0)
@@ -2451,7 +2523,7 @@ extern "C" int plom_server_submit(plom_server *S, const plom_filter_desc *d, con
     plom_model_info mi;
     int rc = check_desc(d, &mi);
     if (rc != PLOM_OK) return rc;
-    if (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_FINAL_STATE | PLOM_FLAG_HAT | PLOM_FLAG_MIF))
+    if (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_FINAL_STATE | PLOM_FLAG_HAT | PLOM_FLAG_MIF | PLOM_FLAG_ALL_STATES))
         return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods and lineages (no ancestry, final state, hat or MIF)");
     if (!in->times || !in->pv || !in->x0 || (d->N > 0 && !in->data)) return set_err(PLOM_ERR_ARG, "null input array");
     if (in->cov && mi.n_fix > 0)

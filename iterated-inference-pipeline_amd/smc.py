@@ -23,6 +23,7 @@ PLOM_FP64, PLOM_FP32 = 0, 1
 NO_DEM_STO, NO_ENV_STO, NO_DRIFT = 1, 2, 4
 FLAG_NO_FILTER, FLAG_STORE_ANCESTRY, FLAG_FINAL_STATE, FLAG_LINEAGE, FLAG_HAT, FLAG_MIF, FLAG_PSR = 1, 2, 4, 8, 16, 32, 64
 FLAG_ODE = 128
+FLAG_ALL_STATES = 256
 MAX_J = 1 << 21
 
 
@@ -70,7 +71,8 @@ class FilterOutputs(C.Structure):
                 ("final_logw", C.POINTER(C.c_double)), ("lineage", C.POINTER(C.c_double)),
                 ("lineage_slots", C.POINTER(C.c_int32)), ("lineage_obs", C.POINTER(C.c_double)),
                 ("hat", C.POINTER(C.c_double)), ("pred_res", C.POINTER(C.c_double)),
-                ("mif_mean", C.POINTER(C.c_double)), ("mif_var", C.POINTER(C.c_double))]
+                ("mif_mean", C.POINTER(C.c_double)), ("mif_var", C.POINTER(C.c_double)),
+                ("all_states", C.POINTER(C.c_double)), ("all_obs", C.POINTER(C.c_double))]
 
 
 EXPORTS = [
@@ -467,7 +469,9 @@ class Filter:
         check(lib().plom_filter_run(self.ctx.h, self.h))
         return self
 
-    def read(self, ancestry=False, final_state=False, final_logw=False, lineage=False, hat=False, mif=False):
+    def read(self, ancestry=False, final_state=False, final_logw=False, lineage=False, hat=False, mif=False,
+             all_states=False):
+        """all_states (FLAG_ALL_STATES): r.all_states [N+1, J, S] and r.all_obs [N+1, J, n_ts, 2]."""
         N, J = self.N, self.J
         ll_n = np.zeros(N)
         ess = np.zeros(N)
@@ -481,12 +485,15 @@ class Filter:
         lob = np.zeros((N + 1, self.model.n_ts, 2)) if lineage else None
         mm = np.zeros((N + 1, self.n_theta)) if mif else None
         mv = np.zeros((N + 1, self.n_theta)) if mif else None
+        als = np.zeros((N + 1, J, self.model.S)) if all_states else None
+        alo = np.zeros((N + 1, J, self.model.n_ts, 2)) if all_states else None
         out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), iptr(anc), dptr(fs), dptr(fw), dptr(lin), iptr(lsl),
-                            dptr(lob), dptr(ht), dptr(pr), dptr(mm), dptr(mv))
+                            dptr(lob), dptr(ht), dptr(pr), dptr(mm), dptr(mv), dptr(als), dptr(alo))
         check(lib().plom_filter_read(self.ctx.h, self.h, C.byref(out)))
         r = FilterResult(out.loglik, out.n_all_fail, ll_n, ess, anc, fs, fw, lin, lsl)
         r.hat, r.pred_res, r.lineage_obs = ht, pr, lob
         r.mif_mean, r.mif_var = mm, mv
+        r.all_states, r.all_obs = als, alo
         return r
 
     def update(self, pv=None, x0=None, seed=1, stream=0):

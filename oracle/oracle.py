@@ -35,6 +35,7 @@ def lib():
         L.oracle_ode_test.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, ip]
         L.oracle_cov_table.argtypes = [C.c_int, C.c_int, dp, C.c_double, ip, dp, dp, dp, ip]
         L.oracle_spline.argtypes = [C.c_int, dp, dp, C.c_int, dp, dp]
+        L.oracle_obs_draw.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, dp]
         L.oracle_obs_loglik.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, dp, ip]
         L.oracle_math.argtypes = [C.c_int, dp, dp]
         L.oracle_normals.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
@@ -257,6 +258,17 @@ def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=
     if rc != 0:
         raise RuntimeError("oracle_trajectory failed")
     return out
+
+
+def obs_draw(model_id, x, pv, t, seed, stream, n, slot, cvd=None):
+    """(observation means [n_ts], drawn observations [n_ts]) of state x (plom_obs_draw)."""
+    n_ts = lib().oracle_model_n_ts(model_id)
+    out = np.zeros(2 * n_ts)
+    lib().oracle_obs_draw(model_id, _dp(np.ascontiguousarray(x, dtype=np.float64)),
+                          _dp(np.ascontiguousarray(pv, dtype=np.float64)),
+                          _dp(np.ascontiguousarray(cvd, dtype=np.float64)) if cvd is not None else None, t, seed, stream,
+                          n, slot, _dp(out))
+    return out[0::2], out[1::2]
 
 
 def obs_loglik(model_id, x, y, pv, t=0.0, like_min=1e-17):

@@ -937,6 +937,21 @@ int oracle_ode_test(int kind, const double *p, double *y, double t0, double t1, 
     return rc;
 }
 
+/* the observation mean of state x at time t and one observation drawn for (row n, slot)
+ * (plom_obs_draw); cvd: the covariates at t (NULL: those of pv) */
+int oracle_obs_draw(int model, const double *x, const double *pv, const double *cvd, double t, uint32_t seed,
+                    uint32_t stream, uint32_t n, uint32_t slot, double *out)
+{
+    if (model < 0 || model >= N_MODELS) return -1;
+    const oracle_model *M = &MODELS[model];
+    double obs[64], mu[64], var[64];
+    const double *cv = cvd ? cvd : pv + M->fix_off;
+    M->proj2obs(x, pv, cv, obs);
+    M->obs_moments(obs, pv, cv, t, mu, var);
+    plom_obs_draw(seed, stream, n, slot, M->n_ts, mu, var, plom_bm_tab, out);
+    return 0;
+}
+
 /* shared scalar arithmetic, host build (compared with the device build bit for bit) */
 void oracle_math(int n, const double *in, double *out)
 {

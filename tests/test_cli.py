@@ -417,3 +417,30 @@ def test_cli_drift_matches_api(tmp_path, ctx):
     ll_nd = float([l for l in r.stdout.splitlines() if l.startswith("logV_17g:")][0].split(":")[1])
     g2 = smc.run(ctx, m, 1500, times, data, pv, x0, seed=2, stream=0, cov=m.covariates(), noises_off=smc.NO_DRIFT)
     assert ll_nd == g2.loglik != ll_cli
+
+
+@pytest.mark.gpu
+def test_cli_x_file(tmp_path, ctx):
+    """`smc -r` writes X_<id>.csv: the header of header_X and (N+1) J rows, particle index and
+    time first, the API's stored clouds in %g."""
+    d = model_dir(tmp_path, "sir")
+    theta = os.path.join(PKG, "examples", "sir", "theta.json")
+    dry = json.loads(run_cli(d, ["sde", "-J", "50", "-o", "12", "--dry_run"], theta).stdout)
+    print("cli -r", flush=True)
+    r = run_cli(d, ["sde", "-J", "50", "-o", "12", "-r", "-b", "-e", "-d"], theta)
+    assert r.returncode == 0, r.stderr
+    lines = open(d / "X_0.csv").read().splitlines()
+    assert lines[0] == "index,time,S:city1__all,I:city1__all,R:city1__all,obs_mean:city1__CDC__inc,obs_real:city1__CDC__inc"
+    assert len(lines) == 1 + 13 * 50
+    m = smc.Model.get("sir")
+    times, data = m.dataset(12)
+    f = smc.Filter(ctx, m, 50, times, data, np.array(dry["pv"]), np.array(dry["x0"]), flags=smc.FLAG_ALL_STATES, seed=2)
+    f.run()
+    g = f.read(all_states=True)
+    f.close()
+    rows = np.array([[float(v) for v in l.split(",")] for l in lines[1:]])
+    np.testing.assert_array_equal(rows[:, 0], np.tile(np.arange(50), 13))
+    np.testing.assert_array_equal(rows[:, 1], np.repeat(times, 50))
+    np.testing.assert_allclose(rows[:, 2:4], g.all_states[:, :, :2].reshape(-1, 2), rtol=1e-5)
+    np.testing.assert_allclose(rows[:, 5], g.all_obs[:, :, 0, 0].ravel(), rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(rows[:, 6], g.all_obs[:, :, 0, 1].ravel(), rtol=1e-5, atol=1e-5)

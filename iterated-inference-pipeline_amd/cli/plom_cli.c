@@ -297,6 +297,73 @@ static void build_router(cli_router *r, const char *key, pj *theta, const pj *or
     }
 }
 
+/* ------------------------------------------------------------------ priors */
+static double flat_pdf(double x, double a, double b)
+{
+    return (x < b && x >= a) ? 1.0 / (b - a) : 0.0;
+}
+
+double cli_gauss_pdf(double u, double sd)
+{
+    return exp(-u * u / (2.0 * sd * sd)) / (sqrt(2.0 * M_PI) * sd);
+}
+
+/* normal_prior (metropolis_hastings_prior.c:95-100): mean (min+max)/2, sd (max-min)/4 */
+static double normal_prior(double x, double a, double b)
+{
+    return cli_gauss_pdf(x - 0.5 * (b + a), (b - a) / 4.0);
+}
+
+/* pseudo_unif_prior (metropolis_hastings_prior.c:111-132): flat on [min+d, max-d],
+ * quadratic shoulders of width d = 0.01 (max-min), integrating to 1 */
+static double pseudo_unif_prior(double x, double a, double b)
+{
+    const double d = 0.01 * (b - a);
+    const double h = 1.0 / (b - a - 2.0 / 3.0 * d);
+    const double c2 = -h / (d * d);
+    const double c1 = -2.0 * (a + d) * c2;
+    const double c0 = h + c2 * (a + d) * (a + d);
+    if (x <= a || x >= b) return 0.0;
+    if (x < a + d) return (c2 * x + c1) * x + c0;
+    if (x < b - d) return h;
+    const double y = x - (b - a - 2.0 * d);
+    return (c2 * y + c1) * y + c0;
+}
+
+double cli_prior_pdf(cli_prior kind, double x, double a, double b)
+{
+    if (kind == PRIOR_NORMAL) return normal_prior(x, a, b);
+    if (kind == PRIOR_PSEUDO_UNIFORM) return pseudo_unif_prior(x, a, b);
+    return flat_pdf(x, a, b);
+}
+
+/* log_prob_prior (metropolis_hastings_prior.c:59-92) over the parameters with sd_transf > 0
+ * (neither followers nor the theta remainder, load_best): sum of log prior(f_inv(theta)),
+ * each density floored at like_min (NaN, infinite or negative -> like_min and 1 returned) */
+int cli_log_prior(const cli_problem *P, const double *theta, double like_min, double *out)
+{
+    int err = 0;
+    double Lp = 0.0;
+    for (int i = 0; i < P->n_par; i++) {
+        const cli_router *r = &P->R[i];
+        if (r->is_remainder || r->follow >= 0) continue;
+        for (int g = 0; g < r->n_gp; g++) {
+            if (!(r->sd[g] > 0.0)) continue;
+            const double x = cli_f_inv(r->tr[g], theta[P->offset[i] + g], r->min[g], r->max[g]);
+            double p = cli_prior_pdf(r->prior[g], x, r->min[g], r->max[g]);
+            if (isnan(p) || isinf(p) || p < 0.0) {
+                err = 1;
+                p = like_min;
+            } else if (p <= like_min) {
+                p = like_min;
+            }
+            Lp += log(p);
+        }
+    }
+    *out = Lp;
+    return err;
+}
+
 double cli_natural(const cli_problem *P, int i, int g, const double *theta)
 {
     const cli_router *r = &P->R[i];

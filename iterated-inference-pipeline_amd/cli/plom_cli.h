@@ -106,6 +106,13 @@ void cli_follow(const cli_problem *P, double *theta);
 
 /* back_transform_x (transform.c:462-480) of group g of parameter i */
 double cli_natural(const cli_problem *P, int i, int g, const double *theta);
+/* prior densities on the natural scale: flat on [a, b], normal_prior, pseudo_unif_prior
+ * (metropolis_hastings_prior.c:95-132) */
+double cli_prior_pdf(cli_prior kind, double x, double a, double b);
+/* the N(0, sd^2) density at u */
+double cli_gauss_pdf(double u, double sd);
+/* log prior of theta over the estimated parameters (sd_transf > 0); 1 on a numerical failure */
+int cli_log_prior(const cli_problem *P, const double *theta, double like_min, double *out);
 
 /* the parameter vector in the generated layout and the initial state */
 void cli_pack(const cli_problem *P, const double *theta, double *pv, double *x0);

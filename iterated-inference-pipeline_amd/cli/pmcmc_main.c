@@ -126,38 +126,6 @@ static void chain_normals(const chain_t *c, int m, int attempt, int n, double *z
 }
 
 /* ------------------------------------------------------------------ priors and densities */
-static double flat_pdf(double x, double a, double b)
-{
-    return (x < b && x >= a) ? 1.0 / (b - a) : 0.0;
-}
-
-static double gauss_pdf(double u, double sd)
-{
-    return exp(-u * u / (2.0 * sd * sd)) / (sqrt(2.0 * M_PI) * sd);
-}
-
-/* normal_prior (metropolis_hastings_prior.c:95-100): mean (min+max)/2, sd (max-min)/4 */
-static double normal_prior(double x, double a, double b)
-{
-    return gauss_pdf(x - 0.5 * (b + a), (b - a) / 4.0);
-}
-
-/* pseudo_unif_prior (metropolis_hastings_prior.c:111-132): flat on [min+d, max-d],
- * quadratic shoulders of width d = 0.01 (max-min), integrating to 1 */
-static double pseudo_unif_prior(double x, double a, double b)
-{
-    const double d = 0.01 * (b - a);
-    const double h = 1.0 / (b - a - 2.0 / 3.0 * d);
-    const double c2 = -h / (d * d);
-    const double c1 = -2.0 * (a + d) * c2;
-    const double c0 = h + c2 * (a + d) * (a + d);
-    if (x <= a || x >= b) return 0.0;
-    if (x < a + d) return (c2 * x + c1) * x + c0;
-    if (x < b - d) return h;
-    const double y = x - (b - a - 2.0 * d);
-    return (c2 * y + c1) * y + c0;
-}
-
 /* log_prob_prior: returns 0 on success, 1 on a numerical failure (the value is still summed) */
 static int log_prob_prior(const mcmc_t *M, const double *theta, double *out)
 {
@@ -166,10 +134,7 @@ static int log_prob_prior(const mcmc_t *M, const double *theta, double *out)
     for (int k = 0; k < M->n; k++) {
         if (!M->is_estimated[k]) continue;
         const double x = cli_f_inv(M->tr[k], theta[k], M->pmin[k], M->pmax[k]);
-        double p;
-        if (M->prior[k] == PRIOR_NORMAL) p = normal_prior(x, M->pmin[k], M->pmax[k]);
-        else if (M->prior[k] == PRIOR_PSEUDO_UNIFORM) p = pseudo_unif_prior(x, M->pmin[k], M->pmax[k]);
-        else p = flat_pdf(x, M->pmin[k], M->pmax[k]);
+        double p = cli_prior_pdf(M->prior[k], x, M->pmin[k], M->pmax[k]);
         if (isnan(p) || isinf(p) || p < 0.0) {
             err = 1;
             p = LIKE_MIN;
@@ -227,7 +192,7 @@ static int log_prob_proposal(const mcmc_t *M, const double *a, const double *b, 
         if (!M->is_estimated[k]) continue;
         const double der = cli_f_inv_der(M->tr[k], a[k], M->pmin[k], M->pmax[k]);
         if (!M->full) {
-            const double p = gauss_pdf(a[k] - b[k], sd_fac * sqrt(var[k * M->n + k])) / der;
+            const double p = cli_gauss_pdf(a[k] - b[k], sd_fac * sqrt(var[k * M->n + k])) / der;
             if (isnan(p) || isinf(p) || p < 0.0) return 1;
             Lp += log(p);
         } else {

@@ -24,7 +24,7 @@
  * --eps_abs / --eps_rel, every noise off).
  * Differences: hat, pred_res and the X output of -r use this build's
  * arithmetic (exact integer weights for the 95 % bounds, tree sums);
- * --prior is not supported; -N is accepted and ignored (the GPU replaces threads).
+ * -N is accepted and ignored (the GPU replaces threads).
  * Extensions: --fp32 (FP32 propagation), --device <k>, --seed <k> (random key, default
  * 2 + id as build.c:965-971), a full-precision "logV_17g" log line, and --dry_run (print
  * the natural parameter vector and initial state as JSON and stop before the GPU).
@@ -114,7 +114,6 @@ int main(int argc, char *argv[])
     uint32_t impl_flags = 0;
     cli_apply_implementation(impl, fp32, &noises_off, &impl_flags);
     const int psr = impl == CLI_PSR;
-    if (prior) cli_die("--prior is not supported by this build");
 
     char str[2048];
     snprintf(str, sizeof str, "%s.done_%d.json", path, general_id);
@@ -187,6 +186,12 @@ int main(int argc, char *argv[])
     out.all_obs = xobs;
     if (plom_smc_run(ctx, &d, &in, &out) != PLOM_OK) cli_die(plom_last_error());
     plom_ctx_destroy(ctx);
+    /* --prior: the log prior of theta joins the log-likelihood (smc/main.c:335-345) */
+    if (prior) {
+        double lp;
+        cli_log_prior(&P, theta, like_min, &lp);
+        out.loglik += lp;
+    }
 
     clock_gettime(CLOCK_MONOTONIC, &t1);
     double secs = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);

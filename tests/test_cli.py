@@ -444,3 +444,21 @@ def test_cli_x_file(tmp_path, ctx):
     np.testing.assert_allclose(rows[:, 2:4], g.all_states[:, :, :2].reshape(-1, 2), rtol=1e-5)
     np.testing.assert_allclose(rows[:, 5], g.all_obs[:, :, 0, 0].ravel(), rtol=1e-5, atol=1e-5)
     np.testing.assert_allclose(rows[:, 6], g.all_obs[:, :, 0, 1].ravel(), rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.gpu
+def test_cli_prior(tmp_path, ctx):
+    """--prior adds log prior(theta) to the log-likelihood (smc/main.c:335-345): with uniform
+    priors, the sum of -log(max - min) over the parameters with sd_transf > 0."""
+    d = model_dir(tmp_path, "sir")
+    theta = os.path.join(PKG, "examples", "sir", "theta.json")
+    th = json.load(open(theta))
+    lp = sum(-math.log(g["max"]["value"] - g["min"]["value"]) for p in th["parameter"].values()
+             for g in p["group"].values() if g["sd_transf"]["value"] > 0)
+    lls = []
+    for extra in ([], ["--prior"]):
+        print("cli prior", extra, flush=True)
+        r = run_cli(d, ["sde", "-J", "500", "-o", "30", "-b", "-e", "-d"] + extra, theta)
+        assert r.returncode == 0, r.stderr
+        lls.append(float([l for l in r.stdout.splitlines() if l.startswith("logV_17g:")][0].split(":")[1]))
+    assert lls[1] == pytest.approx(lls[0] + lp, rel=1e-13)

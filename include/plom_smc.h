@@ -139,6 +139,11 @@ typedef struct {
     const double *multiplier;  /* unit multiplier */
     const double *sd;          /* random-walk standard deviation per unit of mut_fac (0: the
                                   component is resampled but not moved) */
+    const double *chol;        /* optional [n_theta * n_theta], n_theta <= 64: a lower-triangular factor L
+                                  (row-major) of the random walk's covariance per unit of mut_fac^2
+                                  (a "covariance" in theta, resample_and_mut_theta_mif with is_mvn,
+                                  C/mif/methods.c:275-330): theta_o moves by mut_fac sum_{q<=o}
+                                  L[o][q] z_q instead of mut_fac sd_o z_o; rows of zeros do not move */
 } plom_mif_map;
 
 /* A covariate as a function of time, for one slot of pv (one covariate of one population or

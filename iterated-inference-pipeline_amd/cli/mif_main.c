@@ -33,8 +33,10 @@
  *
  * Implementations 'sde', 'psr' (the default, FP64, initial states rounded) and 'ode' (FP64,
  * adaptive RKF45 under --eps_abs / --eps_rel, every noise off).
- * Differences: --prior is not supported; a "covariance" in
- * theta (MVN proposals and mutation) is refused; the fixed-lag update uses the filtered
+ * A "covariance" in theta makes the starting proposals mean + (b FREEZE) Lv z and the random
+ * walk mut_fac L z (Lv: Cholesky factor of the rescaled covariance of the estimated
+ * components; L: its rows of the process / observation components in component space).
+ * Differences: --prior is not supported; the fixed-lag update uses the filtered
  * weighted mean of the initial conditions at the last observation up to L (the reference
  * weights the resampled particles with the weights before resampling, methods.c:370-388,
  * and also re-averages the components that are not estimated); -N is ignored.
@@ -51,6 +53,7 @@
 #include <unistd.h>
 
 #include "plom_cli.h"
+#include "plom_mif.h"
 #include "plom_rng.h"
 
 #define START_STREAM 0xfffffffeu
@@ -179,7 +182,6 @@ int main(int argc, char *argv[])
     P.psr = psr;
     P.drift_on = P.n_drift > 0 && !(noises_off & PLOM_NO_DRIFT_BIT) && !(impl_flags & PLOM_FLAG_ODE);
     if (P.drift_on) cli_die("mif with drifting parameters is not supported by this build (use --no_diff)");
-    if (pj_get(P.theta, "covariance")) cli_die("a \"covariance\" in theta (MVN proposals) is not supported by mif in this build");
     const int L = (int)floor(prop_L * P.N_DATA);
     if (ic_only) P.N = (L + 1 > P.N_DATA) ? P.N_DATA : L + 1;
     if (L > P.N) {
@@ -195,25 +197,31 @@ int main(int argc, char *argv[])
     cli_theta_from_guess(&P, theta0);
     double *var = calloc((size_t)n * n + 1, sizeof(double));
     int *est = calloc((size_t)n + 1, sizeof(int)), *is_sv = calloc((size_t)n + 1, sizeof(int));
+    /* a "covariance" in theta makes the proposals and the random walk multivariate normal
+     * (mif.c:84-121, methods.c:275-330) */
+    const int is_mvn = cli_load_covariance(&P, var);
     for (int i = 0; i < P.n_par; i++)
         for (int g = 0; g < P.R[i].n_gp; g++) {
             const int o = P.offset[i] + g;
-            const double sdv = (P.R[i].follow >= 0 || P.R[i].is_remainder) ? 0.0 : P.R[i].sd[g];
-            var[o * n + o] = sdv > 0.0 ? sdv * sdv : 0.0;
-            est[o] = sdv > 0.0;
+            const int free_par = P.R[i].follow < 0 && !P.R[i].is_remainder;
+            if (!is_mvn) {
+                const double sdv = free_par ? P.R[i].sd[g] : 0.0;
+                var[o * n + o] = sdv > 0.0 ? sdv * sdv : 0.0;
+            }
+            est[o] = free_par && var[o * n + o] > 0.0;
             is_sv[o] = i < P.n_sv;
         }
+    if (ic_only)
+        for (int o = 0; o < n; o++)
+            if (!is_sv[o]) est[o] = 0;
     if (noises_off & PLOM_NO_ENV_STO_BIT)
         for (int i = 0; i < P.n_par; i++)
             if (P.noise_par[i])
-                for (int g = 0; g < P.R[i].n_gp; g++) {
-                    const int o = P.offset[i] + g;
-                    var[o * n + o] = 0.0;
-                    est[o] = 0;
-                }
-    if (ic_only)
-        for (int o = 0; o < n; o++)
-            if (!is_sv[o]) { var[o * n + o] = 0.0; est[o] = 0; }
+                for (int g = 0; g < P.R[i].n_gp; g++) est[P.offset[i] + g] = 0;
+    /* components not estimated take no part in the covariance */
+    for (int o = 0; o < n; o++)
+        if (!est[o])
+            for (int k = 0; k < n; k++) var[o * n + k] = var[k * n + o] = 0.0;
     for (int r = 0; r < n; r++)
         for (int c2 = 0; c2 < n; c2++) {
             if (!is_sv[r] && !is_sv[c2]) var[r * n + c2] *= 1.0 / (double)N;
@@ -243,7 +251,29 @@ int main(int argc, char *argv[])
             mmul[o] = P.R[i].multiplier;
             msd[o] = (est[o] && !is_sv[o]) ? sqrt(var[o * n + o]) : 0.0;
         }
-    plom_mif_map mmap = {pv_src, mtr, msc, mdur, mmin, mmax, mmul, msd};
+    /* multivariate walk: the Cholesky factor Lv of var restricted to the estimated components
+     * (ke), for the proposals; the walk's factor in component space keeps the rows of the
+     * process / observation components (the initial conditions are only resampled,
+     * methods.c:320-328) */
+    int ne = 0;
+    int *ke = calloc((size_t)n + 1, sizeof(int));
+    for (int o = 0; o < n; o++)
+        if (est[o]) ke[ne++] = o;
+    double *Lv = NULL, *mchol = NULL;
+    if (is_mvn) {
+        if (n > PLOM_MIF_MVN_MAX) cli_die("a covariance in theta takes at most 64 components here");
+        double *sub = calloc((size_t)ne * ne + 1, sizeof(double));
+        Lv = calloc((size_t)ne * ne + 1, sizeof(double));
+        for (int a = 0; a < ne; a++)
+            for (int b = 0; b < ne; b++) sub[a * ne + b] = var[ke[a] * n + ke[b]];
+        if (ne && cli_cholesky(sub, ne, 1.0, Lv)) cli_die("Covariance matrix is not positive definite");
+        free(sub);
+        mchol = calloc((size_t)n * n + 1, sizeof(double));
+        for (int a = 0; a < ne; a++)
+            if (!is_sv[ke[a]])
+                for (int b = 0; b <= a; b++) mchol[ke[a] * n + ke[b]] = Lv[a * ne + b];
+    }
+    plom_mif_map mmap = {pv_src, mtr, msc, mdur, mmin, mmax, mmul, msd, mchol};
 
     /* observations with data, delta_t since the previous one (mif.c:140-150) */
     int *data_rows = calloc((size_t)N + 1, sizeof(int));
@@ -294,6 +324,10 @@ int main(int argc, char *argv[])
         for (int o = 0; o < n; o++) printf("%s%.17g", o ? ", " : "", mmul[o]);
         printf("], \"sd\": [");
         for (int o = 0; o < n; o++) printf("%s%.17g", o ? ", " : "", msd[o]);
+        printf("], \"chol\": [");
+        for (int o = 0; mchol && o < n * n; o++) printf("%s%.17g", o ? ", " : "", mchol[o]);
+        printf("], \"chol_est\": [");
+        for (int o = 0; Lv && o < ne * ne; o++) printf("%s%.17g", o ? ", " : "", Lv[o]);
         printf("], \"delta\": [");
         for (int r = 0; r < N; r++) printf("%s%.17g", r ? ", " : "", delta[r]);
         printf("], \"data_rows\": [");
@@ -370,6 +404,7 @@ int main(int argc, char *argv[])
     if (plom_batch_configure(ctx, batch, lanes) != PLOM_OK) cli_die(plom_last_error());
     double *mut = calloc((size_t)N + 1, sizeof(double));
     double *th = calloc((size_t)n + 4, sizeof(double));
+    double *zs = calloc((size_t)n + 4, sizeof(double));
 
     for (int m = 1; m <= M_iter; m++) {
         struct timespec t_it;
@@ -390,20 +425,25 @@ int main(int argc, char *argv[])
                 for (int attempt = 0;; attempt++) {
                     if (attempt > 1000000) cli_die("no proposal satisfies the initial-condition constraint");
                     memcpy(th, c->mean, (size_t)n * sizeof(double));
-                    int q = 0;
-                    double z[4];
-                    for (int o = 0; o < n; o++) {
-                        if (!est[o]) continue;
-                        if ((q & 3) == 0) {
-                            uint32_t w[4];
-                            const uint32_t ctr[4] = {(uint32_t)j, (uint32_t)m, (uint32_t)attempt, (uint32_t)(q >> 2)};
-                            const uint32_t key[2] = {c->seed, START_STREAM};
-                            plom_philox4x32_10(ctr, key, w);
-                            plom_bm_pair(w[0], w[1], plom_bm_tab, &z[0], &z[1]);
-                            plom_bm_pair(w[2], w[3], plom_bm_tab, &z[2], &z[3]);
+                    /* ne normals, four per Philox draw (ctr = (j, m, attempt, block)) */
+                    for (int q = 0; q < ne; q += 4) {
+                        uint32_t w[4];
+                        const uint32_t ctr[4] = {(uint32_t)j, (uint32_t)m, (uint32_t)attempt, (uint32_t)(q >> 2)};
+                        const uint32_t key[2] = {c->seed, START_STREAM};
+                        plom_philox4x32_10(ctr, key, w);
+                        plom_bm_pair(w[0], w[1], plom_bm_tab, &zs[q], &zs[q + 1]);
+                        plom_bm_pair(w[2], w[3], plom_bm_tab, &zs[q + 2], &zs[q + 3]);
+                    }
+                    for (int q = 0; q < ne; q++) {
+                        const int o = ke[q];
+                        if (is_mvn) {
+                            /* ran_proposal_chol (proposal.c:40-69): mean + (b FREEZE) Lv z */
+                            double acc = 0.0;
+                            for (int p = 0; p <= q; p++) acc = acc + Lv[q * ne + p] * zs[p];
+                            th[o] = c->mean[o] + sd_fac * acc;
+                        } else {
+                            th[o] = c->mean[o] + sd_fac * sqrt(var[o * n + o]) * zs[q];
                         }
-                        th[o] = c->mean[o] + sd_fac * sqrt(var[o * n + o]) * z[q & 3];
-                        q++;
                     }
                     if (cli_theta_remainder(&P, th) == 0) break;
                 }

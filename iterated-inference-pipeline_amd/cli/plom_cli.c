@@ -364,6 +364,57 @@ int cli_log_prior(const cli_problem *P, const double *theta, double like_min, do
     return err;
 }
 
+/* Cholesky factor (lower, row-major) of the n x n matrix s A; returns 0 when positive definite */
+int cli_cholesky(const double *A, int n, double s, double *L)
+{
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j <= i; j++) {
+            double v = s * A[i * n + j];
+            for (int k = 0; k < j; k++) v -= L[i * n + k] * L[j * n + k];
+            if (i == j) {
+                if (!(v > 0.0)) return 1;
+                L[i * n + i] = sqrt(v);
+            } else {
+                L[i * n + j] = v / L[j * n + j];
+            }
+        }
+    for (int i = 0; i < n; i++)
+        for (int j = i + 1; j < n; j++) L[i * n + j] = 0.0;
+    return 0;
+}
+
+/* load_covariance (load.c:413-470): theta's "covariance" into var [n_theta][n_theta], row 0
+ * being the header and the theta remainder padded with 0; 0 (var untouched) without one */
+int cli_load_covariance(const cli_problem *P, double *var)
+{
+    pj *cov = pj_get(P->theta, "covariance");
+    if (!cov) return 0;
+    const int n = P->n_theta;
+    int ri = 1;
+    for (int i = 0; i < P->n_par; i++)
+        for (int g = 0; g < P->R[i].n_gp; g++) {
+            const int o = P->offset[i] + g;
+            pj *row = NULL;
+            if (i != P->ind_theta_remainder) {
+                row = pj_at(cov, ri++);
+                if (!row || row->type != PJ_ARRAY) cli_die("covariance row is not an array");
+            }
+            int ck = 0;
+            for (int ii = 0; ii < P->n_par; ii++)
+                for (int gg = 0; gg < P->R[ii].n_gp; gg++) {
+                    const int oo = P->offset[ii] + gg;
+                    if (i != P->ind_theta_remainder && ii != P->ind_theta_remainder) {
+                        pj *v = pj_at(row, ck++);
+                        if (!pj_is_number(v)) cli_die("covariance entry is not a number");
+                        var[o * n + oo] = v->num;
+                    } else {
+                        var[o * n + oo] = 0.0;
+                    }
+                }
+        }
+    return 1;
+}
+
 double cli_natural(const cli_problem *P, int i, int g, const double *theta)
 {
     const cli_router *r = &P->R[i];

@@ -109,6 +109,10 @@ double cli_natural(const cli_problem *P, int i, int g, const double *theta);
 /* prior densities on the natural scale: flat on [a, b], normal_prior, pseudo_unif_prior
  * (metropolis_hastings_prior.c:95-132) */
 double cli_prior_pdf(cli_prior kind, double x, double a, double b);
+/* Cholesky factor (lower, row-major) of the n x n matrix s A; 0 when positive definite */
+int cli_cholesky(const double *A, int n, double s, double *L);
+/* theta's "covariance" into var [n_theta][n_theta] (load_covariance, load.c:413-470); 0 without one */
+int cli_load_covariance(const cli_problem *P, double *var);
 /* the N(0, sd^2) density at u */
 double cli_gauss_pdf(double u, double sd);
 /* log prior of theta over the estimated parameters (sd_transf > 0); 1 on a numerical failure */

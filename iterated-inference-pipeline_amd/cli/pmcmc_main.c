@@ -147,25 +147,6 @@ static int log_prob_prior(const mcmc_t *M, const double *theta, double *out)
     return err;
 }
 
-/* Cholesky factor (lower, row-major) of the n x n matrix s A; returns 0 when positive definite */
-static int cholesky(const double *A, int n, double s, double *L)
-{
-    for (int i = 0; i < n; i++)
-        for (int j = 0; j <= i; j++) {
-            double v = s * A[i * n + j];
-            for (int k = 0; k < j; k++) v -= L[i * n + k] * L[j * n + k];
-            if (i == j) {
-                if (!(v > 0.0)) return 1;
-                L[i * n + i] = sqrt(v);
-            } else {
-                L[i * n + j] = v / L[j * n + j];
-            }
-        }
-    for (int i = 0; i < n; i++)
-        for (int j = i + 1; j < n; j++) L[i * n + j] = 0.0;
-    return 0;
-}
-
 /* log q(a | b) on the natural scale: the transformed-scale density divided by the
  * derivative of f_inv at a (log_prob_proposal, proposal.c:120-188).  With mvn, Lc is the
  * Cholesky factor of sd_fac^2 var restricted to the estimated components. */
@@ -373,32 +354,7 @@ int main(int argc, char *argv[])
             is_follower[o] = P.R[i].follow >= 0;
             MC.var0[o * n + o] = (P.R[i].follow >= 0 || P.R[i].is_remainder) ? 0.0 : P.R[i].sd[g];
         }
-    pj *cov = pj_get(P.theta, "covariance");
-    if (cov) {
-        /* load_covariance (load.c:413-470): row 0 is the header; the remainder is padded with 0 */
-        int ri = 1;
-        for (int i = 0; i < P.n_par; i++)
-            for (int g = 0; g < P.R[i].n_gp; g++) {
-                const int o = P.offset[i] + g;
-                pj *row = NULL;
-                if (i != P.ind_theta_remainder) {
-                    row = pj_at(cov, ri++);
-                    if (!row || row->type != PJ_ARRAY) cli_die("covariance row is not an array");
-                }
-                int ck = 0;
-                for (int ii = 0; ii < P.n_par; ii++)
-                    for (int gg = 0; gg < P.R[ii].n_gp; gg++) {
-                        const int oo = P.offset[ii] + gg;
-                        if (i != P.ind_theta_remainder && ii != P.ind_theta_remainder) {
-                            pj *v = pj_at(row, ck++);
-                            if (!pj_is_number(v)) cli_die("covariance entry is not a number");
-                            MC.var0[o * n + oo] = v->num;
-                        } else {
-                            MC.var0[o * n + oo] = 0.0;
-                        }
-                    }
-            }
-    } else {
+    if (!cli_load_covariance(&P, MC.var0)) {
         for (int o = 0; o < n; o++)
             if (MC.var0[o * n + o] > 0.0) MC.var0[o * n + o] *= MC.var0[o * n + o];
     }
@@ -587,13 +543,13 @@ int main(int argc, char *argv[])
                     if (m * c->global_ar >= MC.m_switch) var = c->var_sampling;
                     for (int i = 0; i < nt; i++)
                         for (int q = 0; q < nt; q++) sub[i * nt + q] = var[MC.tbe0[i] * n + MC.tbe0[q]];
-                    if (cholesky(sub, nt, sd_fac * sd_fac, Lk[k])) {
+                    if (cli_cholesky(sub, nt, sd_fac * sd_fac, Lk[k])) {
                         if (!warned_pd) fprintf(stderr, "Covariance matrix is not positive definite: the initial covariance is used\n");
                         warned_pd = 1;
                         var = MC.var0;
                         for (int i = 0; i < nt; i++)
                             for (int q = 0; q < nt; q++) sub[i * nt + q] = var[MC.tbe0[i] * n + MC.tbe0[q]];
-                        if (cholesky(sub, nt, sd_fac * sd_fac, Lk[k])) cli_die("the initial covariance is not positive definite");
+                        if (cli_cholesky(sub, nt, sd_fac * sd_fac, Lk[k])) cli_die("the initial covariance is not positive definite");
                     }
                 } else if (nt > 0 && c->has_cycled) {
                     /* gsl_ran_shuffle: Fisher-Yates, j uniform in [0, i] */

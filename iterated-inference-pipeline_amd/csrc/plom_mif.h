@@ -22,6 +22,7 @@
 #include "plom_rng.h"
 
 #define PLOM_PURPOSE_MUTATE 4u
+#define PLOM_MIF_MVN_MAX 64     /* components of a correlated random walk */
 #define PLOM_MIF_LN10 2.302585092994046
 
 PLOM_HD double plom_mif_f_inv(int tr, double x, double a, double b)

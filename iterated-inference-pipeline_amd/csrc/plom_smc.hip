@@ -114,6 +114,7 @@ struct plom_dfilter {
     const int32_t *pm_tr, *pm_sc, *pm_dur;      /* [n_theta] */
     const double *pm_min, *pm_max, *pm_mult, *pm_sd;   /* [n_theta] */
     const double *mut_fac;          /* [N] random-walk factor after interval n (with data) */
+    const double *pm_chol;          /* [n_theta][n_theta] lower factor of the walk's covariance, or null */
     double *th[2];                  /* [n_theta][Jp] theta of the particles, ping-pong like X */
     double *mpw, *mps, *mpm, *mpq;  /* tile partials: [T] sum w, [n_theta][T] sum w theta, mean, M2 */
     double *mmean, *mvar;           /* [(N+1)][n_theta] weighted mean and variance of theta */
@@ -283,6 +284,25 @@ __device__ inline void mif_particle(const plom_dfilter *f, int n, int j, int a, 
     const double *Tin = f->th[n & 1];
     double *Tout = f->th[(n + 1) & 1];
     const double fac = (n > 0 && f->nonan[n - 1] > 0) ? f->mut_fac[n - 1] : 0.0;
+    if (f->pm_chol) {
+        /* correlated walk: theta_o += fac sum_{q<=o} L[o][q] z_q, z from the same counters */
+        double z[PLOM_MIF_MVN_MAX];
+        for (int o = 0; o < nth; o += 2) {
+            double z0 = 0.0, z1 = 0.0;
+            if (fac != 0.0) plom_mif_normals(f->seed, f->stream, (uint32_t)n, (uint32_t)j, (uint32_t)(o >> 1), tab, &z0, &z1);
+            z[o] = z0;
+            if (o + 1 < nth) z[o + 1] = z1;
+        }
+        for (int o = 0; o < nth; o++) {
+            double v = Tin[(size_t)o * Jp + a];
+            if (fac != 0.0) {
+                double acc = 0.0;
+                for (int q = 0; q <= o; q++) acc = acc + f->pm_chol[(size_t)o * nth + q] * z[q];
+                v = v + fac * acc;
+            }
+            Tout[(size_t)o * Jp + j] = v;
+        }
+    } else
     for (int o = 0; o < nth; o += 2) {
         const double sd0 = fac * f->pm_sd[o];
         const double sd1 = (o + 1 < nth) ? fac * f->pm_sd[o + 1] : 0.0;
@@ -1192,11 +1212,13 @@ struct layout {
         off_img, off_df, off_cnt, off_scal, off_naf, off_llike_n, off_ess_n, off_rflag, off_nonan, off_times,
         off_data, off_pv, off_pvf, off_x0, off_pm_src, off_pm_tr, off_pm_sc, off_pm_dur, off_pm_min, off_pm_max,
         off_pm_mult, off_pm_sd, off_mutfac, off_cov, off_dm_group, off_dm_ng, off_dm_tr, off_dm_sc, off_dm_dur,
-        off_dm_min, off_dm_max, off_dm_mult, off_dm_vol, off_allX, total;   /* off_dm_vol: int32 vol slots */
+        off_dm_min, off_dm_max, off_dm_mult, off_dm_vol, off_allX, off_pm_chol, total;   /* off_dm_vol: int32 vol slots */
 };
 
-/* cov_vals: values of the covariate table (0 without time-varying covariates) */
-static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, layout *L, size_t cov_vals = 0)
+/* cov_vals: values of the covariate table (0 without time-varying covariates); chol_vals: of
+ * the MIF walk's covariance factor (0 without one) */
+static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, layout *L, size_t cov_vals = 0,
+                       size_t chol_vals = 0)
 {
     const size_t J = (size_t)d->J, N = (size_t)d->N;
     const size_t Jp = (J + TILE - 1) / TILE * TILE, T = Jp / TILE;
@@ -1257,6 +1279,7 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     PUT(off_pm_mult, NT * 8);
     PUT(off_pm_sd, NT * 8);
     PUT(off_mutfac, NT ? N * 8 : 0);
+    PUT(off_pm_chol, chol_vals * 8);
     PUT(off_cov, cov_vals * 8);
     const size_t ND = (size_t)mi->n_drift * (size_t)mi->n_cac;
     PUT(off_dm_group, ND * 4);
@@ -1426,6 +1449,8 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
         for (int o = 0; o < d->n_theta; o++)
             if (mm->tr[o] < 0 || mm->tr[o] > 3 || mm->scale[o] < 0 || mm->scale[o] > 2 || !(mm->sd[o] >= 0.0))
                 return set_err(PLOM_ERR_ARG, "mif component %d: bad transform, scale or sd", o);
+        if (mm->chol && d->n_theta > PLOM_MIF_MVN_MAX)
+            return set_err(PLOM_ERR_ARG, "a correlated random walk (mif chol) takes at most %d components", PLOM_MIF_MVN_MAX);
     }
     for (int n = 0; n < d->N; n++)
         if (!(in->times[n + 1] > in->times[n])) return set_err(PLOM_ERR_ARG, "times must increase (n=%d)", n);
@@ -1465,7 +1490,8 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     }
     HIPCHK(hipSetDevice(c->device));
     layout L;
-    make_layout(d, &mi, &L, covtab.size());
+    const size_t chol_vals = (d->flags & PLOM_FLAG_MIF) && mm && mm->chol ? (size_t)d->n_theta * d->n_theta : 0;
+    make_layout(d, &mi, &L, covtab.size(), chol_vals);
     const int async = pinned != nullptr;
     std::vector<char> pageable;
     char *img = pinned;
@@ -1537,6 +1563,7 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     h->pm_mult = mif_on ? (const double *)(b + L.off_pm_mult) : nullptr;
     h->pm_sd = mif_on ? (const double *)(b + L.off_pm_sd) : nullptr;
     h->mut_fac = mif_on ? (const double *)(b + L.off_mutfac) : nullptr;
+    h->pm_chol = (mif_on && mm->chol) ? (const double *)(b + L.off_pm_chol) : nullptr;
     h->th[0] = mif_on ? (double *)(b + L.off_th0) : nullptr;
     h->th[1] = mif_on ? (double *)(b + L.off_th1) : nullptr;
     h->mpw = mif_on ? (double *)(b + L.off_mpw) : nullptr;
@@ -1597,6 +1624,7 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
         memcpy(at(L.off_pm_max), mm->max, (size_t)d->n_theta * 8);
         memcpy(at(L.off_pm_mult), mm->multiplier, (size_t)d->n_theta * 8);
         memcpy(at(L.off_pm_sd), mm->sd, (size_t)d->n_theta * 8);
+        if (mm->chol) memcpy(at(L.off_pm_chol), mm->chol, (size_t)d->n_theta * d->n_theta * 8);
     }
     memcpy(at(L.off_times), in->times, (size_t)(d->N + 1) * 8);
     if (d->N > 0) memcpy(at(L.off_data), in->data, (size_t)d->N * mi.n_ts * 8);

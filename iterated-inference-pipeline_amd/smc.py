@@ -45,7 +45,7 @@ class FilterDesc(C.Structure):
 class MifMap(C.Structure):
     _fields_ = [("pv_src", C.POINTER(C.c_int32)), ("tr", C.POINTER(C.c_int32)), ("scale", C.POINTER(C.c_int32)),
                 ("duration", C.POINTER(C.c_int32)), ("min", C.POINTER(C.c_double)), ("max", C.POINTER(C.c_double)),
-                ("multiplier", C.POINTER(C.c_double)), ("sd", C.POINTER(C.c_double))]
+                ("multiplier", C.POINTER(C.c_double)), ("sd", C.POINTER(C.c_double)), ("chol", C.POINTER(C.c_double))]
 
 
 class Covariate(C.Structure):
@@ -442,8 +442,9 @@ class Filter:
                 raise ValueError("pv_src must have n_pp = %d entries" % model.n_pp)
             self.desc.n_theta = self.n_theta
             m = self._mif
+            m["chol"] = f64(mif["chol"]).reshape(-1) if mif.get("chol") is not None else None
             mmap = MifMap(iptr(m["pv_src"]), iptr(m["tr"]), iptr(m["scale"]), iptr(m["duration"]), dptr(m["min"]),
-                          dptr(m["max"]), dptr(m["multiplier"]), dptr(m["sd"]))
+                          dptr(m["max"]), dptr(m["multiplier"]), dptr(m["sd"]), dptr(m["chol"]))
         covs, self._cov = _cov_array(model, cov)
         dmap, self._drift = _drift_map(model, drift)
         inp = FilterInputs(dptr(self.times), dptr(self.data), dptr(self.pv), dptr(self.x0),

@@ -8,8 +8,10 @@ It restates, in plain Python, the iteration loop of the reference
 * fill_theta_bart_and_Vt_mif (methods.c:114-138): every row of the filtered means starts
   at the current theta, Vt[0] = (b FREEZE)^2 var_kk, FREEZE = a^(m-1);
 * the particles' starting parameters (ran_proposal, proposal.c:71-84): mean +
-  (b FREEZE) sqrt(var_kk) z for every estimated component, redrawn until the initial
-  proportions fit (proposal.c:87-114), then followers copy their leaders;
+  (b FREEZE) sqrt(var_kk) z for every estimated component, or with a covariance in theta
+  (ran_proposal_chol, proposal.c:40-69) mean + (b FREEZE) Lv z, Lv the Cholesky factor of
+  the estimated components' covariance; redrawn until the initial proportions fit
+  (proposal.c:87-114), then followers copy their leaders;
 * one pass with per-particle parameters, given as a callable (the CPU filter oracle,
   oracle.mif_pass, in the tests);
 * per observation with data: the filtered mean, and the variance plus delta_t FREEZE^2
@@ -74,6 +76,9 @@ def run(dry, pass_fn, x0_of, constrain, M, J, seed, a=0.975, b=2.0, switch=5, pr
     mean = list(dry["theta"])
     est_list = [o for o in range(n) if est[o]]
     nblocks = (len(est_list) + 3) // 4
+    ne = len(est_list)
+    Lv = np.array(dry.get("chol_est") or [], dtype=float)
+    Lv = Lv.reshape(ne, ne) if Lv.size else None
     rows = [r for r in range(N) if not math.isnan(delta[r]) and r in dry["data_rows"]]
     trace = [(0, [f_inv(tr[o], mean[o], mn[o], mx[o]) for o in keep], float("nan"))]
     for m in range(1, M + 1):
@@ -92,7 +97,13 @@ def run(dry, pass_fn, x0_of, constrain, M, J, seed, a=0.975, b=2.0, switch=5, pr
                 th = list(mean)
                 z = start_normals(seed, j, m, attempt, nblocks)
                 for q, o in enumerate(est_list):
-                    th[o] = mean[o] + sd_fac * math.sqrt(var[o, o]) * z[q]
+                    if Lv is None:
+                        th[o] = mean[o] + sd_fac * math.sqrt(var[o, o]) * z[q]
+                    else:
+                        acc = 0.0
+                        for p_ in range(q + 1):
+                            acc = acc + Lv[q, p_] * z[p_]
+                        th[o] = mean[o] + sd_fac * acc
                 if constrain(th) == 0:
                     break
                 attempt += 1

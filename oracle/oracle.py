@@ -51,7 +51,7 @@ def lib():
         L.oracle_mif_pass.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double,
                                       C.c_double, dp, dp, dp, C.c_int, i32p, i32p, i32p, i32p, dp, dp, dp, dp, dp,
                                       dp, dp, C.c_int, dp, i32p, dp, dp, dp, dp, dp, dp, i32p, i32p, dp, dp, C.c_double,
-                                      C.c_double]
+                                      C.c_double, dp]
         _lib = L
     return _lib
 
@@ -198,8 +198,8 @@ def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=
              like_min=1e-17, nthreads=1, theta_out=False, logw=False, ancestry=False, psr=False, cov=None, ode=False,
              eps_abs=0.0, eps_rel=0.0):
     """One pass of iterated filtering restated on the CPU (oracle_mif_pass).  mif: dict of
-    pv_src [n_pp], tr, scale, duration, min, max, multiplier, sd [n_theta]; theta0
-    [n_theta, J]; x0 [S, J]; mut_fac [N]."""
+    pv_src [n_pp], tr, scale, duration, min, max, multiplier, sd [n_theta] and optionally chol
+    [n_theta, n_theta] (a correlated walk); theta0 [n_theta, J]; x0 [S, J]; mut_fac [N]."""
     noises_off = _impl(noises_off, psr, ode)
     times = np.ascontiguousarray(times, dtype=np.float64)
     N = len(times) - 1
@@ -208,6 +208,7 @@ def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=
     m = {k: (i32 if k in ("pv_src", "tr", "scale", "duration") else f64)(mif[k])
          for k in ("pv_src", "tr", "scale", "duration", "min", "max", "multiplier", "sd")}
     nth = len(m["tr"])
+    chol = f64(mif["chol"]).reshape(-1) if mif.get("chol") is not None else None
     theta0, x0, mut_fac = f64(theta0), f64(x0), f64(mut_fac)
     data, pv = f64(data), f64(pv)
     ll, naf = C.c_double(), C.c_int32()
@@ -220,7 +221,7 @@ def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=
                                nth, _ip(m["pv_src"]), _ip(m["tr"]), _ip(m["scale"]), _ip(m["duration"]), _dp(m["min"]),
                                _dp(m["max"]), _dp(m["multiplier"]), _dp(m["sd"]), _dp(mut_fac), _dp(theta0), _dp(x0),
                                nthreads, C.byref(ll), C.byref(naf), _dp(ll_n), _dp(ess), _dp(mm), _dp(mv), _dp(tho),
-                               _dp(lw), _ip(anc), *_cov_args(cov), *_eps(eps_abs, eps_rel))
+                               _dp(lw), _ip(anc), *_cov_args(cov), *_eps(eps_abs, eps_rel), _dp(chol))
     if rc != 0:
         raise RuntimeError("oracle_mif_pass failed: %d" % rc)
     return {"loglik": ll.value, "n_all_fail": naf.value, "loglik_n": ll_n, "ess_n": ess, "mif_mean": mm,

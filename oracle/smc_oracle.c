@@ -698,9 +698,10 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
                     const double *theta0, const double *x0, int nthreads, double *loglik, int32_t *n_all_fail,
                     double *loglik_n, double *ess_n, double *mif_mean, double *mif_var, double *theta_out,
                     double *logw_out, int32_t *ancestry, const int32_t *cov_n, const double *cov_x,
-                    const double *cov_y, double eps_abs, double eps_rel)
+                    const double *cov_y, double eps_abs, double eps_rel, const double *chol)
 {
     if (model < 0 || model >= N_MODELS || J < 1 || N < 0 || n_theta < 1) return -1;
+    if (chol && n_theta > PLOM_MIF_MVN_MAX) return -1;
     const oracle_model *M = &MODELS[model];
     cov_view cv;
     const int crc = cov_build(M, N, times, dt, cov_n, cov_x, cov_y, pv, &cv);
@@ -736,6 +737,22 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
         const double fac = (n > 0 && prev_data) ? mut_fac[n - 1] : 0.0;
         for (int j = 0; j < J; j++) {
             double *th = TH + (size_t)j * n_theta;
+            if (chol) {
+                /* correlated walk: th_o += fac sum_{q<=o} L[o][q] z_q */
+                double z[PLOM_MIF_MVN_MAX];
+                for (int o = 0; o < n_theta; o += 2) {
+                    double z0 = 0.0, z1 = 0.0;
+                    if (fac != 0.0)
+                        plom_mif_normals(seed, stream, (uint32_t)n, (uint32_t)j, (uint32_t)(o >> 1), plom_bm_tab, &z0, &z1);
+                    z[o] = z0;
+                    if (o + 1 < n_theta) z[o + 1] = z1;
+                }
+                for (int o = 0; o < n_theta && fac != 0.0; o++) {
+                    double acc = 0.0;
+                    for (int q = 0; q <= o; q++) acc = acc + chol[(size_t)o * n_theta + q] * z[q];
+                    th[o] = th[o] + fac * acc;
+                }
+            } else
             for (int o = 0; o < n_theta; o += 2) {
                 const double sd0 = fac * sd[o];
                 const double sd1 = (o + 1 < n_theta) ? fac * sd[o + 1] : 0.0;

@@ -132,3 +132,23 @@ def test_mif_rejects_bad_maps(ctx):
         smc.Filter(ctx, m, 50, times, data, pv, x0[:, 0], flags=smc.FLAG_MIF, mif=bad)
     with pytest.raises(smc.PlomError):
         smc.Filter(ctx, m, 50, times, data, pv, x0[:, 0], flags=smc.FLAG_MIF | smc.FLAG_HAT, mif=mp)
+
+
+@pytest.mark.parametrize("name,J,N", [("sir", 1000, 30), ("sirs4", 700, 15)])
+def test_correlated_walk_equals_oracle(ctx, name, J, N):
+    """A covariance factor L (mif map chol): every component moves by mut_fac (L z)_o."""
+    m, times, data, pv, mp, theta0, x0, mut = mif_setup(name, N, J, 11)
+    nth = len(mp["tr"])
+    rng = np.random.default_rng(nth)
+    L = np.tril(rng.normal(0.0, 0.01, (nth, nth)))
+    L[np.arange(nth), np.arange(nth)] = np.abs(L[np.arange(nth), np.arange(nth)]) + 0.02
+    L[nth - 2] = 0.0                                   # a component that is only resampled
+    mp = dict(mp, chol=L)
+    print("mif chol", name, J, N, flush=True)
+    f = smc.Filter(ctx, m, J, times, data, pv, x0[:, 0], flags=smc.FLAG_MIF, mif=mp)
+    f.mif_start(theta0, x0, mut, seed=6, stream=2)
+    f.run()
+    g = f.read(mif=True)
+    f.close()
+    o = oracle.mif_pass(m.id, J, times, data, pv, mp, theta0, x0, mut, seed=6, stream=2, nthreads=8)
+    check_equal(g, o)

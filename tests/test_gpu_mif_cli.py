@@ -69,3 +69,51 @@ def test_mif_command_equals_restatement(tmp_path):
         assert len(mrows) == M * len(dry["data_rows"])
         done = json.load(open(tmp_path / "sir" / (".done_%d.json" % (3 + k))))
         assert done["parameter"]["beta"]["group"]["all"]["guess"]["value"] == pytest.approx(o["trace"][-1][1][2], rel=1e-15)
+
+
+def test_mif_command_with_a_covariance(tmp_path):
+    """A "covariance" in theta: multivariate starting proposals and a correlated random walk
+    (mif.c:84-121, methods.c:275-330); the command's trace equals the restatement's."""
+    d = model_dir(tmp_path, "sir")
+    th = json.load(open(THETA))
+    names = ["S", "I", "beta", "gamma", "rep", "phi"]
+    sd = [0.01, 0.01, 0.02, 0.02, 0.02, 0.02]
+    cov = [[sd[a] * sd[b] * (1.0 if a == b else (0.6 if {a, b} == {2, 3} else (0.3 if {a, b} == {2, 4} else 0.0)))
+            for b in range(6)] for a in range(6)]
+    th["covariance"] = [names] + cov
+    tpath = tmp_path / "theta_cov.json"
+    tpath.write_text(json.dumps(th) + "\n")
+    J, M = 300, 3
+    args = ["sde", "-J", str(J), "-M", str(M), "-S", "1", "--trace_17g", "-i", "0"]
+
+    def run(a):
+        with open(tpath) as f:
+            return subprocess.run([MIF] + a, cwd=str(d), stdin=f, capture_output=True, text=True, timeout=600)
+    dry = json.loads(run(args + ["--dry_run"]).stdout)
+    assert len(dry["chol"]) == 36 and dry["chol"][2 * 6 + 3] == 0.0 and dry["chol"][3 * 6 + 2] != 0.0
+    print("mif command mvn", flush=True)
+    r = run(args)
+    assert r.returncode == 0, r.stderr
+    m = smc.Model.get("sir")
+    times, data = m.dataset(dry["N"])
+    mp = {k: dry[k] for k in ("pv_src", "tr", "scale", "duration", "min", "max", "multiplier", "sd")}
+    mp["chol"] = np.array(dry["chol"]).reshape(6, 6)
+    pv = np.array(dry["pv"])
+
+    def pass_fn(thp, x0, mut, seed, stream):
+        return oracle.mif_pass(m.id, thp.shape[1], times, data, pv, mp, thp, x0, mut, seed=seed, stream=stream, nthreads=8)
+
+    def x0_of(t):
+        return [mif_oracle.f_inv(2, t[0], 0, 0) * 1e6, mif_oracle.f_inv(2, t[1], 0, 0) * 1e6, 0.0]
+
+    def constrain(t):
+        return int(mif_oracle.f_inv(2, t[0], 0, 0) + mif_oracle.f_inv(2, t[1], 0, 0) > 1.0)
+
+    o = mif_oracle.run(dry, pass_fn, x0_of, constrain, M=M, J=J, seed=2, switch=1)
+    head, rows = read_csv(tmp_path / "sir" / "trace_0.csv")
+    assert len(rows) == M + 1
+    for (idx, vals, ll), row in zip(o["trace"], rows):
+        assert row[0] == idx
+        np.testing.assert_array_equal(row[1:-1], vals)
+        if idx:
+            assert row[-1] == ll

@@ -156,3 +156,41 @@ def test_random_walk_and_resampling_of_theta():
                 rr = mp.sqrt(-2 * mp.log(u1))
                 z = float(rr * (mp.cos(2 * mp.pi * u2) if o % 2 == 0 else mp.sin(2 * mp.pi * u2)))
                 assert th[n + 1][o, j] == pytest.approx(prev[o, j] + (fac[n - 1] * sd[o]) * z, rel=0, abs=1e-14)
+
+
+def test_correlated_walk_with_a_covariance_factor():
+    """With a lower-triangular factor L (a "covariance" in theta, methods.c:275-330): after an
+    interval with data, theta_o = ancestor's + mut_fac sum_{q<=o} L[o][q] z_q, the z from the
+    same Philox counters as the diagonal walk (mpmath Box-Muller here); rows of zeros (the
+    initial conditions S, I here) do not move."""
+    m = Model.get("sir")
+    times, data = m.dataset(6)
+    J = 200
+    theta0, x0 = random_start(m, J, 7)
+    rng = np.random.default_rng(3)
+    L = np.zeros((6, 6))
+    for o in range(2, 6):
+        L[o, 2:o + 1] = rng.normal(0.0, 0.02, o - 1)
+        L[o, o] = abs(L[o, o]) + 0.01
+    mp_map = dict(SIR_MAP, chol=L)
+    fac = np.linspace(0.6, 1.1, 6)
+    pv = natural_pv(m, sir_theta())
+    r = oracle.mif_pass(m.id, J, times, data, pv, mp_map, theta0, x0, fac, seed=4, stream=2, theta_out=True,
+                        ancestry=True)
+    th, anc = r["theta"], r["ancestry"]
+    for n in range(1, 6):
+        prev = th[n][:, anc[n - 1]]
+        for j in range(0, J, 23):
+            z = []
+            for b in range(3):
+                w = oracle.philox([j, n, 4, b], [4, 2])
+                u1 = (mp.mpf(int(w[0])) + mp.mpf(1) / 2) / 2 ** 32
+                u2 = (mp.mpf(int(w[1])) + mp.mpf(1) / 2) / 2 ** 32
+                rr = mp.sqrt(-2 * mp.log(u1))
+                z += [float(rr * mp.cos(2 * mp.pi * u2)), float(rr * mp.sin(2 * mp.pi * u2))]
+            for o in range(6):
+                want = prev[o, j] + fac[n - 1] * sum(L[o, q] * z[q] for q in range(o + 1))
+                if o < 2:
+                    assert th[n + 1][o, j] == prev[o, j]
+                else:
+                    assert th[n + 1][o, j] == pytest.approx(want, rel=0, abs=1e-14)

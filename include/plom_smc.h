@@ -144,7 +144,23 @@ typedef struct {
                                   (a "covariance" in theta, resample_and_mut_theta_mif with is_mvn,
                                   C/mif/methods.c:275-330): theta_o moves by mut_fac sum_{q<=o}
                                   L[o][q] z_q instead of mut_fac sd_o z_o; rows of zeros do not move */
+    /* optional (all three or none): the prior patch of the weights (mif --prior,
+     * patch_likelihood_prior, C/mif/methods.c:469-523).  At an interval n with data, particle
+     * j's log-weight gains sum over o with prior[o] >= 0 and n < prior_until[o] of
+     * prior_pow[o] log(max(p_o, like_min)), p_o the density of f_inv(theta_o) (its natural
+     * value before scale, multiplier and duration) under prior[o] on [min[o], max[o]]; the
+     * particle fails when the patched weight is at or below like_min^n_ts.  The reference
+     * uses prior_pow 1/nb_obs for the process / observation parameters at every interval and
+     * 1/L for the initial conditions at intervals n < L. */
+    const int32_t *prior;      /* [n_theta] PLOM_PRIOR_*: -1 none, 0 uniform, 1 normal, 2 pseudo-uniform */
+    const double *prior_pow;   /* [n_theta] exponent of the density */
+    const int32_t *prior_until;/* [n_theta] the term applies at intervals n < prior_until[o] */
 } plom_mif_map;
+
+#define PLOM_PRIOR_NONE (-1)
+#define PLOM_PRIOR_UNIFORM 0
+#define PLOM_PRIOR_NORMAL 1
+#define PLOM_PRIOR_PSEUDO_UNIFORM 2
 
 /* A covariate as a function of time, for one slot of pv (one covariate of one population or
  * series), as the reference's spline of par_fixed_values (build_p_calc,

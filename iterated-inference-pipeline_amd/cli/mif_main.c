@@ -36,7 +36,12 @@
  * A "covariance" in theta makes the starting proposals mean + (b FREEZE) Lv z and the random
  * walk mut_fac L z (Lv: Cholesky factor of the rescaled covariance of the estimated
  * components; L: its rows of the process / observation components in component space).
- * Differences: --prior is not supported; the fixed-lag update uses the filtered
+ * --prior (patch_likelihood_prior, methods.c:469-523; is_bayesian, mif/build.c:45): uniform and
+ * pseudo-uniform priors move on the logit_ab scale, and at every interval with data the
+ * weights take the factor prior^(1/nb_obs) per estimated process / observation component and
+ * prior^(1/L) per estimated initial condition while n < L (plom_mif_map.prior*; in the log
+ * domain on the device).
+ * Differences: the fixed-lag update uses the filtered
  * weighted mean of the initial conditions at the last observation up to L (the reference
  * weights the resampled particles with the weights before resampling, methods.c:370-388,
  * and also re-averages the components that are not estimated); -N is ignored.
@@ -119,7 +124,7 @@ int main(int argc, char *argv[])
         "                     [-s, --DT <float>] [-r, --traj] [-p, --path <path>] [-i, --id <integer>]\n"
         "                     [-l, --LIKE_MIN <float>] [-J <integer>] [-M, --iter <integer>]\n"
         "                     [-a, --cooling <float>] [-b, --heat <float>] [-L, --lag <float>] [-S, --switch <integer>]\n"
-        "                     [-f --ic_only] [-q, --quiet] [-P, --pipe]\n"
+        "                     [-f --ic_only] [-q, --quiet] [-P, --pipe] [--prior]\n"
         "                     [--chains <K>] [--lanes <k>] [--fp32] [--device <k>] [--seed <k>] [--trace_17g] [-h, --help]\n"
         "where implementation is 'ode', 'sde' or 'psr' (default)\n";
     for (;;) {
@@ -166,7 +171,6 @@ int main(int argc, char *argv[])
     uint32_t impl_flags = 0;
     cli_apply_implementation(impl, fp32, &noises_off, &impl_flags);
     const int psr = impl == CLI_PSR;
-    if (prior) cli_die("--prior is not supported by this build");
     if (n_chains < 1 || n_chains > 4096) cli_die("--chains must be in [1, 4096]");
     if (pipe && n_chains > 1) cli_die("-P (pipe) prints one theta: use it with one chain");
     if (J < 2) cli_die("-J must be at least 2 (the variance of theta over the particles)");
@@ -178,7 +182,7 @@ int main(int argc, char *argv[])
         unlink(str);
     }
     cli_problem P;
-    cli_load(&P, 0, -1, dt);
+    cli_load(&P, prior, -1, dt);
     P.psr = psr;
     P.drift_on = P.n_drift > 0 && !(noises_off & PLOM_NO_DRIFT_BIT) && !(impl_flags & PLOM_FLAG_ODE);
     if (P.drift_on) cli_die("mif with drifting parameters is not supported by this build (use --no_diff)");
@@ -273,7 +277,33 @@ int main(int argc, char *argv[])
             if (!is_sv[ke[a]])
                 for (int b = 0; b <= a; b++) mchol[ke[a] * n + ke[b]] = Lv[a * ne + b];
     }
-    plom_mif_map mmap = {pv_src, mtr, msc, mdur, mmin, mmax, mmul, msd, mchol};
+    /* --prior: per estimated component its prior, the exponent and the intervals it applies
+     * to (patch_likelihood_prior, methods.c:469-523): 1/nb_obs at every interval for the
+     * process / observation parameters, 1/L at intervals n < L for the initial conditions */
+    int32_t *mprior = NULL, *muntil = NULL;
+    double *mppow = NULL;
+    if (prior) {
+        mprior = calloc((size_t)n + 1, sizeof(int32_t));
+        muntil = calloc((size_t)n + 1, sizeof(int32_t));
+        mppow = calloc((size_t)n + 1, sizeof(double));
+        for (int i = 0; i < P.n_par; i++)
+            for (int g = 0; g < P.R[i].n_gp; g++) {
+                const int o = P.offset[i] + g;
+                mprior[o] = PLOM_PRIOR_NONE;
+                if (!est[o]) continue;
+                const cli_prior k = P.R[i].prior[g];
+                mprior[o] = k == PRIOR_NORMAL ? PLOM_PRIOR_NORMAL
+                            : (k == PRIOR_PSEUDO_UNIFORM ? PLOM_PRIOR_PSEUDO_UNIFORM : PLOM_PRIOR_UNIFORM);
+                if (is_sv[o]) {
+                    muntil[o] = L;
+                    mppow[o] = L > 0 ? 1.0 / (double)L : 0.0;
+                } else {
+                    muntil[o] = N;
+                    mppow[o] = 1.0 / (double)N;
+                }
+            }
+    }
+    plom_mif_map mmap = {pv_src, mtr, msc, mdur, mmin, mmax, mmul, msd, mchol, mprior, mppow, muntil};
 
     /* observations with data, delta_t since the previous one (mif.c:140-150) */
     int *data_rows = calloc((size_t)N + 1, sizeof(int));
@@ -328,6 +358,12 @@ int main(int argc, char *argv[])
         for (int o = 0; mchol && o < n * n; o++) printf("%s%.17g", o ? ", " : "", mchol[o]);
         printf("], \"chol_est\": [");
         for (int o = 0; Lv && o < ne * ne; o++) printf("%s%.17g", o ? ", " : "", Lv[o]);
+        printf("], \"prior\": [");
+        for (int o = 0; mprior && o < n; o++) printf("%s%d", o ? ", " : "", mprior[o]);
+        printf("], \"prior_pow\": [");
+        for (int o = 0; mppow && o < n; o++) printf("%s%.17g", o ? ", " : "", mppow[o]);
+        printf("], \"prior_until\": [");
+        for (int o = 0; muntil && o < n; o++) printf("%s%d", o ? ", " : "", muntil[o]);
         printf("], \"delta\": [");
         for (int r = 0; r < N; r++) printf("%s%.17g", r ? ", " : "", delta[r]);
         printf("], \"data_rows\": [");

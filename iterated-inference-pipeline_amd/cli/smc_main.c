@@ -120,7 +120,9 @@ int main(int argc, char *argv[])
     unlink(str);
 
     cli_problem P;
-    cli_load(&P, 0, nb_obs, dt);
+    /* with --prior, parameters with a uniform or pseudo-uniform prior use the logit_ab
+     * transform (is_bayesian, smc/main.c:246) */
+    cli_load(&P, prior, nb_obs, dt);
     P.psr = psr;
     P.drift_on = P.n_drift > 0 && !(noises_off & PLOM_NO_DRIFT_BIT) && !(impl_flags & PLOM_FLAG_ODE);
     const int N = P.N, N_TS = P.N_TS;

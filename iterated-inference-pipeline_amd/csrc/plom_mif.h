@@ -14,6 +14,18 @@
  *                      disjoint sets of values (Chan, Golub and LeVeque 1983): the merge
  *                      step of the fixed-shape tree over tiles that gives the variance of
  *                      theta (the reference's one-pass variance, methods.c:168-181).
+ *   plom_mif_prior_pdf the prior densities on the natural scale: uniform on [a, b)
+ *                      (gsl_ran_flat_pdf), normal with mean (a+b)/2 and sd (b-a)/4
+ *                      (normal_prior, C/core/metropolis_hastings_prior.c:93-98) and the
+ *                      pseudo-uniform density with quadratic shoulders of width
+ *                      d = (b-a)/100 (pseudo_unif_prior, same file:109-132).
+ *   plom_mif_prior_term  e log(max(p, LIKE_MIN)) for one component (NaN, infinite or
+ *                      negative densities count as LIKE_MIN, sanitize_likelihood,
+ *                      C/core/likelihood.c:77-91): the log of the factor prior^e by which
+ *                      mif --prior multiplies a particle's weight (patch_likelihood_prior,
+ *                      C/mif/methods.c:469-523).
+ *   plom_mif_prior_patch  the patched log-weight, or -inf for a failure (weight(),
+ *                      C/core/smc.c:44: a weight at or below LIKE_MIN^N_TS).
  */
 #ifndef PLOM_MIF_H
 #define PLOM_MIF_H
@@ -66,6 +78,53 @@ PLOM_HD void plom_mif_normals(uint32_t seed, uint32_t stream, uint32_t n, uint32
     const uint32_t key[2] = { seed, stream };
     plom_philox4x32_10(ctr, key, w);
     plom_bm_pair(w[0], w[1], tab, z0, z1);
+}
+
+#ifndef PLOM_PRIOR_NONE          /* (also in include/plom_smc.h) */
+#define PLOM_PRIOR_NONE (-1)
+#define PLOM_PRIOR_UNIFORM 0
+#define PLOM_PRIOR_NORMAL 1
+#define PLOM_PRIOR_PSEUDO_UNIFORM 2
+#endif
+#define PLOM_MIF_INV_SQRT_2PI 0.3989422804014327
+
+PLOM_HD double plom_mif_prior_pdf(int kind, double x, double a, double b)
+{
+    if (kind == PLOM_PRIOR_NORMAL) {
+        const double sd = (b - a) / 4.0;
+        const double u = (x - 0.5 * (a + b)) / sd;
+        return PLOM_MIF_INV_SQRT_2PI / sd * plom_exp(-0.5 * (u * u));
+    }
+    if (kind == PLOM_PRIOR_PSEUDO_UNIFORM) {
+        /* flat at h on [a+d, b-d]; on the shoulders the parabola c2 (x-a-d)^2 + h, which is 0
+         * at a and b (mirrored on the right) and has the flat part's value and slope at the
+         * joins; h makes the density integrate to 1 */
+        if (x <= a || x >= b) return 0.0;
+        const double d = 0.01 * (b - a);
+        const double h = 1.0 / ((b - a) - (2.0 / 3.0) * d);
+        double u;
+        if (x < a + d) u = x - (a + d);
+        else if (x < b - d) return h;
+        else u = x - (b - d);
+        return h - (h / (d * d)) * (u * u);
+    }
+    return (x >= a && x < b) ? 1.0 / (b - a) : 0.0;
+}
+
+PLOM_HD double plom_mif_prior_term(int kind, double e, int tr, double theta, double a, double b, double like_min)
+{
+    double p = plom_mif_prior_pdf(kind, plom_mif_f_inv(tr, theta, a, b), a, b);
+    if (!(p > like_min) || p == PLOM_INF) p = like_min;
+    return e * plom_log(p);
+}
+
+/* l: the particle's log-likelihood (the floored sum when every series is observed and
+ * floored: all_floored); pt: its summed prior terms; all_fail_llike: N_TS log LIKE_MIN */
+PLOM_HD double plom_mif_prior_patch(double l, int all_floored, double pt, double all_fail_llike)
+{
+    if (all_floored) return pt > 0.0 ? l + pt : -PLOM_INF;
+    const double lp = l + pt;
+    return lp <= all_fail_llike ? -PLOM_INF : lp;
 }
 
 /* (c, mean, m2) of a followed by b; an empty side (c == 0) leaves the other unchanged */

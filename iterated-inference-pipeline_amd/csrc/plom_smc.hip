@@ -115,6 +115,8 @@ struct plom_dfilter {
     const double *pm_min, *pm_max, *pm_mult, *pm_sd;   /* [n_theta] */
     const double *mut_fac;          /* [N] random-walk factor after interval n (with data) */
     const double *pm_chol;          /* [n_theta][n_theta] lower factor of the walk's covariance, or null */
+    const int32_t *pm_prior, *pm_puntil;   /* [n_theta] prior patch of the weights, or null */
+    const double *pm_ppow;                 /* [n_theta] */
     double *th[2];                  /* [n_theta][Jp] theta of the particles, ping-pong like X */
     double *mpw, *mps, *mpm, *mpq;  /* tile partials: [T] sum w, [n_theta][T] sum w theta, mean, M2 */
     double *mmean, *mvar;           /* [(N+1)][n_theta] weighted mean and variance of theta */
@@ -424,8 +426,21 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
             M::proj2obs(X, f->pv, cvd, obs);
             l = M::loglik(obs, f->data + (size_t)n * f->n_ts, f->pv, cvd, t1, f->like_min, &nf);
         }
-        /* failure: every series observed and floored at LIKE_MIN (smc.c:44) */
-        if (nf == M::N_TS) l = -PLOM_INF;
+        if (MIF && f->pm_prior) {
+            /* mif --prior: the prior terms of the particle's theta for this interval (the
+             * table holds what mif_particle wrote for it), then the failure test on the
+             * patched weight */
+            const double *th = f->th[(n + 1) & 1];
+            double pt = 0.0;
+            for (int o = 0; o < f->n_theta; o++)
+                if (f->pm_prior[o] != PLOM_PRIOR_NONE && n < f->pm_puntil[o])
+                    pt = pt + plom_mif_prior_term(f->pm_prior[o], f->pm_ppow[o], f->pm_tr[o], th[(size_t)o * Jp + j],
+                                                  f->pm_min[o], f->pm_max[o], f->like_min);
+            l = plom_mif_prior_patch(l, nf == M::N_TS, pt, f->all_fail_llike);
+        } else if (nf == M::N_TS) {
+            /* failure: every series observed and floored at LIKE_MIN (smc.c:44) */
+            l = -PLOM_INF;
+        }
     }
     f->logw[j] = l;
     const double mt = max256(l, lds);
@@ -1212,13 +1227,14 @@ struct layout {
         off_img, off_df, off_cnt, off_scal, off_naf, off_llike_n, off_ess_n, off_rflag, off_nonan, off_times,
         off_data, off_pv, off_pvf, off_x0, off_pm_src, off_pm_tr, off_pm_sc, off_pm_dur, off_pm_min, off_pm_max,
         off_pm_mult, off_pm_sd, off_mutfac, off_cov, off_dm_group, off_dm_ng, off_dm_tr, off_dm_sc, off_dm_dur,
-        off_dm_min, off_dm_max, off_dm_mult, off_dm_vol, off_allX, off_pm_chol, total;   /* off_dm_vol: int32 vol slots */
+        off_dm_min, off_dm_max, off_dm_mult, off_dm_vol, off_allX, off_pm_chol, off_pm_prior, off_pm_ppow,
+        off_pm_puntil, total;   /* off_dm_vol: int32 vol slots */
 };
 
 /* cov_vals: values of the covariate table (0 without time-varying covariates); chol_vals: of
- * the MIF walk's covariance factor (0 without one) */
+ * the MIF walk's covariance factor (0 without one); prior_on: the MIF prior patch */
 static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, layout *L, size_t cov_vals = 0,
-                       size_t chol_vals = 0)
+                       size_t chol_vals = 0, int prior_on = 0)
 {
     const size_t J = (size_t)d->J, N = (size_t)d->N;
     const size_t Jp = (J + TILE - 1) / TILE * TILE, T = Jp / TILE;
@@ -1280,6 +1296,9 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     PUT(off_pm_sd, NT * 8);
     PUT(off_mutfac, NT ? N * 8 : 0);
     PUT(off_pm_chol, chol_vals * 8);
+    PUT(off_pm_prior, prior_on ? NT * 4 : 0);
+    PUT(off_pm_ppow, prior_on ? NT * 8 : 0);
+    PUT(off_pm_puntil, prior_on ? NT * 4 : 0);
     PUT(off_cov, cov_vals * 8);
     const size_t ND = (size_t)mi->n_drift * (size_t)mi->n_cac;
     PUT(off_dm_group, ND * 4);
@@ -1451,6 +1470,15 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
                 return set_err(PLOM_ERR_ARG, "mif component %d: bad transform, scale or sd", o);
         if (mm->chol && d->n_theta > PLOM_MIF_MVN_MAX)
             return set_err(PLOM_ERR_ARG, "a correlated random walk (mif chol) takes at most %d components", PLOM_MIF_MVN_MAX);
+        if ((mm->prior != nullptr) != (mm->prior_pow != nullptr) || (mm->prior != nullptr) != (mm->prior_until != nullptr))
+            return set_err(PLOM_ERR_ARG, "mif prior, prior_pow and prior_until go together");
+        for (int o = 0; mm->prior && o < d->n_theta; o++) {
+            if (mm->prior[o] < PLOM_PRIOR_NONE || mm->prior[o] > PLOM_PRIOR_PSEUDO_UNIFORM)
+                return set_err(PLOM_ERR_ARG, "mif component %d: unknown prior %d", o, mm->prior[o]);
+            if (mm->prior[o] != PLOM_PRIOR_NONE && (!(mm->max[o] > mm->min[o]) || !(mm->prior_pow[o] >= 0.0) ||
+                                                    mm->prior_pow[o] == PLOM_INF))
+                return set_err(PLOM_ERR_ARG, "mif component %d: a prior needs min < max and a finite prior_pow >= 0", o);
+        }
     }
     for (int n = 0; n < d->N; n++)
         if (!(in->times[n + 1] > in->times[n])) return set_err(PLOM_ERR_ARG, "times must increase (n=%d)", n);
@@ -1491,7 +1519,8 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     HIPCHK(hipSetDevice(c->device));
     layout L;
     const size_t chol_vals = (d->flags & PLOM_FLAG_MIF) && mm && mm->chol ? (size_t)d->n_theta * d->n_theta : 0;
-    make_layout(d, &mi, &L, covtab.size(), chol_vals);
+    const int prior_on = (d->flags & PLOM_FLAG_MIF) && mm && mm->prior;
+    make_layout(d, &mi, &L, covtab.size(), chol_vals, prior_on);
     const int async = pinned != nullptr;
     std::vector<char> pageable;
     char *img = pinned;
@@ -1564,6 +1593,9 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     h->pm_sd = mif_on ? (const double *)(b + L.off_pm_sd) : nullptr;
     h->mut_fac = mif_on ? (const double *)(b + L.off_mutfac) : nullptr;
     h->pm_chol = (mif_on && mm->chol) ? (const double *)(b + L.off_pm_chol) : nullptr;
+    h->pm_prior = prior_on ? (const int32_t *)(b + L.off_pm_prior) : nullptr;
+    h->pm_ppow = prior_on ? (const double *)(b + L.off_pm_ppow) : nullptr;
+    h->pm_puntil = prior_on ? (const int32_t *)(b + L.off_pm_puntil) : nullptr;
     h->th[0] = mif_on ? (double *)(b + L.off_th0) : nullptr;
     h->th[1] = mif_on ? (double *)(b + L.off_th1) : nullptr;
     h->mpw = mif_on ? (double *)(b + L.off_mpw) : nullptr;
@@ -1625,6 +1657,11 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
         memcpy(at(L.off_pm_mult), mm->multiplier, (size_t)d->n_theta * 8);
         memcpy(at(L.off_pm_sd), mm->sd, (size_t)d->n_theta * 8);
         if (mm->chol) memcpy(at(L.off_pm_chol), mm->chol, (size_t)d->n_theta * d->n_theta * 8);
+        if (prior_on) {
+            memcpy(at(L.off_pm_prior), mm->prior, (size_t)d->n_theta * 4);
+            memcpy(at(L.off_pm_ppow), mm->prior_pow, (size_t)d->n_theta * 8);
+            memcpy(at(L.off_pm_puntil), mm->prior_until, (size_t)d->n_theta * 4);
+        }
     }
     memcpy(at(L.off_times), in->times, (size_t)(d->N + 1) * 8);
     if (d->N > 0) memcpy(at(L.off_data), in->data, (size_t)d->N * mi.n_ts * 8);

@@ -45,7 +45,12 @@ class FilterDesc(C.Structure):
 class MifMap(C.Structure):
     _fields_ = [("pv_src", C.POINTER(C.c_int32)), ("tr", C.POINTER(C.c_int32)), ("scale", C.POINTER(C.c_int32)),
                 ("duration", C.POINTER(C.c_int32)), ("min", C.POINTER(C.c_double)), ("max", C.POINTER(C.c_double)),
-                ("multiplier", C.POINTER(C.c_double)), ("sd", C.POINTER(C.c_double)), ("chol", C.POINTER(C.c_double))]
+                ("multiplier", C.POINTER(C.c_double)), ("sd", C.POINTER(C.c_double)), ("chol", C.POINTER(C.c_double)),
+                ("prior", C.POINTER(C.c_int32)), ("prior_pow", C.POINTER(C.c_double)),
+                ("prior_until", C.POINTER(C.c_int32))]
+
+
+PRIOR_NONE, PRIOR_UNIFORM, PRIOR_NORMAL, PRIOR_PSEUDO_UNIFORM = -1, 0, 1, 2   # plom_mif_map.prior
 
 
 class Covariate(C.Structure):
@@ -443,8 +448,14 @@ class Filter:
             self.desc.n_theta = self.n_theta
             m = self._mif
             m["chol"] = f64(mif["chol"]).reshape(-1) if mif.get("chol") is not None else None
+            has_prior = mif.get("prior") is not None
+            for k, cv in (("prior", i32), ("prior_pow", f64), ("prior_until", i32)):
+                m[k] = cv(mif[k]) if has_prior else None
+                if has_prior and m[k].shape != (self.n_theta,):
+                    raise ValueError("mif %s must have n_theta = %d entries" % (k, self.n_theta))
             mmap = MifMap(iptr(m["pv_src"]), iptr(m["tr"]), iptr(m["scale"]), iptr(m["duration"]), dptr(m["min"]),
-                          dptr(m["max"]), dptr(m["multiplier"]), dptr(m["sd"]), dptr(m["chol"]))
+                          dptr(m["max"]), dptr(m["multiplier"]), dptr(m["sd"]), dptr(m["chol"]), iptr(m["prior"]),
+                          dptr(m["prior_pow"]), iptr(m["prior_until"]))
         covs, self._cov = _cov_array(model, cov)
         dmap, self._drift = _drift_map(model, drift)
         inp = FilterInputs(dptr(self.times), dptr(self.data), dptr(self.pv), dptr(self.x0),

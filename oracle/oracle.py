@@ -51,7 +51,12 @@ def lib():
         L.oracle_mif_pass.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double,
                                       C.c_double, dp, dp, dp, C.c_int, i32p, i32p, i32p, i32p, dp, dp, dp, dp, dp,
                                       dp, dp, C.c_int, dp, i32p, dp, dp, dp, dp, dp, dp, i32p, i32p, dp, dp, C.c_double,
-                                      C.c_double, dp]
+                                      C.c_double, dp, i32p, dp, i32p]
+        L.oracle_mif_prior_pdf.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+        L.oracle_mif_prior_pdf.restype = C.c_double
+        L.oracle_mif_prior_term.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double,
+                                            C.c_double]
+        L.oracle_mif_prior_term.restype = C.c_double
         _lib = L
     return _lib
 
@@ -198,8 +203,9 @@ def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=
              like_min=1e-17, nthreads=1, theta_out=False, logw=False, ancestry=False, psr=False, cov=None, ode=False,
              eps_abs=0.0, eps_rel=0.0):
     """One pass of iterated filtering restated on the CPU (oracle_mif_pass).  mif: dict of
-    pv_src [n_pp], tr, scale, duration, min, max, multiplier, sd [n_theta] and optionally chol
-    [n_theta, n_theta] (a correlated walk); theta0 [n_theta, J]; x0 [S, J]; mut_fac [N]."""
+    pv_src [n_pp], tr, scale, duration, min, max, multiplier, sd [n_theta], optionally chol
+    [n_theta, n_theta] (a correlated walk) and optionally prior, prior_pow, prior_until
+    [n_theta] (the prior patch of the weights); theta0 [n_theta, J]; x0 [S, J]; mut_fac [N]."""
     noises_off = _impl(noises_off, psr, ode)
     times = np.ascontiguousarray(times, dtype=np.float64)
     N = len(times) - 1
@@ -209,6 +215,8 @@ def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=
          for k in ("pv_src", "tr", "scale", "duration", "min", "max", "multiplier", "sd")}
     nth = len(m["tr"])
     chol = f64(mif["chol"]).reshape(-1) if mif.get("chol") is not None else None
+    pr = [i32(mif["prior"]), f64(mif["prior_pow"]), i32(mif["prior_until"])] if mif.get("prior") is not None \
+        else [None, None, None]
     theta0, x0, mut_fac = f64(theta0), f64(x0), f64(mut_fac)
     data, pv = f64(data), f64(pv)
     ll, naf = C.c_double(), C.c_int32()
@@ -221,7 +229,8 @@ def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=
                                nth, _ip(m["pv_src"]), _ip(m["tr"]), _ip(m["scale"]), _ip(m["duration"]), _dp(m["min"]),
                                _dp(m["max"]), _dp(m["multiplier"]), _dp(m["sd"]), _dp(mut_fac), _dp(theta0), _dp(x0),
                                nthreads, C.byref(ll), C.byref(naf), _dp(ll_n), _dp(ess), _dp(mm), _dp(mv), _dp(tho),
-                               _dp(lw), _ip(anc), *_cov_args(cov), *_eps(eps_abs, eps_rel), _dp(chol))
+                               _dp(lw), _ip(anc), *_cov_args(cov), *_eps(eps_abs, eps_rel), _dp(chol), _ip(pr[0]),
+                               _dp(pr[1]), _ip(pr[2]))
     if rc != 0:
         raise RuntimeError("oracle_mif_pass failed: %d" % rc)
     return {"loglik": ll.value, "n_all_fail": naf.value, "loglik_n": ll_n, "ess_n": ess, "mif_mean": mm,
@@ -243,6 +252,19 @@ def psr_draws(kind, a, count, b=0.0, seed=1, stream=0, n=0, j=0, p=None):
 def mif_natural(tr, sc, dur, x, a, b, mult):
     """plom_mif_natural (csrc/plom_mif.h) through the oracle build."""
     return lib().oracle_mif_natural(int(tr), int(sc), int(dur), float(x), float(a), float(b), float(mult))
+
+
+PRIOR_NONE, PRIOR_UNIFORM, PRIOR_NORMAL, PRIOR_PSEUDO_UNIFORM = -1, 0, 1, 2
+
+
+def mif_prior_pdf(kind, x, a, b):
+    """plom_mif_prior_pdf (csrc/plom_mif.h) through the oracle build."""
+    return lib().oracle_mif_prior_pdf(int(kind), float(x), float(a), float(b))
+
+
+def mif_prior_term(kind, e, tr, theta, a, b, like_min=1e-17):
+    """plom_mif_prior_term: e log(max(prior(f_inv(theta)), like_min))."""
+    return lib().oracle_mif_prior_term(int(kind), float(e), int(tr), float(theta), float(a), float(b), float(like_min))
 
 
 def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=0.25, psr=False, cov=None, ode=False,

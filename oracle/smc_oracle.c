@@ -18,7 +18,9 @@
  *                           (resample_and_mut_theta_mif, C/mif/methods.c:261-367), and the
  *                           weighted mean / variance of theta per observation
  *                           (mean_var_theta_theoretical_mif, methods.c:146-205) in the
- *                           canonical tree order of the GPU (tiles of 256, then tiles)
+ *                           canonical tree order of the GPU (tiles of 256, then tiles);
+ *                           with a prior, the weights patched by prior^e per component
+ *                           (patch_likelihood_prior, methods.c:469-523) in the log domain
  * Two arithmetic modes:
  *   ORACLE_CANONICAL (0)    the arithmetic the GPU filter defines (log-domain weights,
  *                           recursive pairwise tree sums over tiles of 256, fixed-point
@@ -119,6 +121,13 @@ void oracle_psr_draws(int kind, double a, double b, uint32_t seed, uint32_t stre
 double oracle_mif_natural(int tr, int sc, int dur, double x, double a, double b, double mult)
 {
     return plom_mif_natural(tr, sc, dur, x, a, b, mult);
+}
+
+double oracle_mif_prior_pdf(int kind, double x, double a, double b) { return plom_mif_prior_pdf(kind, x, a, b); }
+
+double oracle_mif_prior_term(int kind, double e, int tr, double theta, double a, double b, double like_min)
+{
+    return plom_mif_prior_term(kind, e, tr, theta, a, b, like_min);
 }
 
 /* ------------------------------------------------------------------ covariates */
@@ -249,6 +258,8 @@ typedef struct {
     uint32_t seed, stream, noises_off;
     double dt, sqdt, like_min, eps_abs, eps_rel;
     int j0, j1, mode;
+    const double *lprior;   /* [J] or NULL: summed prior terms of the particles (mif --prior) */
+    double all_fail_llike;  /* n_ts log(like_min), with lprior */
 } prop_job;
 
 static void propagate_range(prop_job *p)
@@ -268,7 +279,15 @@ static void propagate_range(prop_job *p)
             const double *cvd = cv_at(p->cv, p->n + 1);
             M->proj2obs(x, pv, cvd, obs);
             double l = M->loglik(obs, p->data + (size_t)p->n * p->n_ts, pv, cvd, p->times[p->n + 1], p->like_min, &nf);
-            if (p->mode == ORACLE_CANONICAL && nf == p->n_ts) l = -INFINITY;
+            if (p->lprior) {
+                /* patch_likelihood_prior (C/mif/methods.c:469-523) in the log domain: the
+                 * weight exp(l) prior^e fails at or below like_min^n_ts (smc.c:44) */
+                const double pt = p->lprior[j];
+                if (nf == p->n_ts) l = pt > 0.0 ? l + pt : -INFINITY;
+                else l = (l + pt <= p->all_fail_llike) ? -INFINITY : l + pt;
+            } else if (p->mode == ORACLE_CANONICAL && nf == p->n_ts) {
+                l = -INFINITY;
+            }
             p->logw[j] = l;
         }
     }
@@ -698,7 +717,8 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
                     const double *theta0, const double *x0, int nthreads, double *loglik, int32_t *n_all_fail,
                     double *loglik_n, double *ess_n, double *mif_mean, double *mif_var, double *theta_out,
                     double *logw_out, int32_t *ancestry, const int32_t *cov_n, const double *cov_x,
-                    const double *cov_y, double eps_abs, double eps_rel, const double *chol)
+                    const double *cov_y, double eps_abs, double eps_rel, const double *chol, const int32_t *prior,
+                    const double *prior_pow, const int32_t *prior_until)
 {
     if (model < 0 || model >= N_MODELS || J < 1 || N < 0 || n_theta < 1) return -1;
     if (chol && n_theta > PLOM_MIF_MVN_MAX) return -1;
@@ -725,7 +745,8 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
     /* per-particle parameter rows: the caller's pv, the per-particle slots overwritten */
     const int pvn = M->pv_size;
     double *PVJ = malloc((size_t)J * pvn * sizeof(double));
-    if (!PVJ) return -2;
+    double *LP = prior ? malloc((size_t)J * sizeof(double)) : NULL;
+    if (!PVJ || (prior && !LP)) return -2;
     const double all_fail_llike = plom_log(like_min) * (double)n_ts;
     double llike = 0.0;
     int naf = 0;
@@ -768,6 +789,14 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
                 const int o = pv_src[s2];
                 if (o >= 0) row[s2] = plom_mif_natural(tr[o], sc[o], dur[o], th[o], mn[o], mx[o], mult[o]);
             }
+            /* mif --prior: the particle's prior terms at this interval, in component order */
+            if (prior) {
+                double pt = 0.0;
+                for (int o = 0; o < n_theta; o++)
+                    if (prior[o] >= 0 && n < prior_until[o])
+                        pt = pt + plom_mif_prior_term(prior[o], prior_pow[o], tr[o], th[o], mn[o], mx[o], like_min);
+                LP[j] = pt;
+            }
         }
         if (theta_out)
             for (int o = 0; o < n_theta; o++)
@@ -780,6 +809,7 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
         job.seed = seed; job.stream = stream; job.noises_off = noises_off;
         job.dt = dt; job.sqdt = plom_sqrt(dt); job.like_min = like_min; job.mode = ORACLE_CANONICAL;
         job.eps_abs = eps_abs; job.eps_rel = eps_rel;
+        job.lprior = LP; job.all_fail_llike = all_fail_llike;
         propagate_all(&job, nthreads);
         double ln = 0.0, es = 0.0;
         int resampled = 0;
@@ -859,6 +889,7 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
     *n_all_fail = naf;
     free(X); free(Xn); free(TH); free(THn); free(logw); free(work); free(cum); free(select); free(buf); free(tc);
     free(PVJ);
+    free(LP);
     free(cv.owned);
     return 0;
 }

@@ -294,6 +294,76 @@ def test_mif_converges_towards_the_truth(tmp_path):
     assert abs(beta[-1] - 0.5) < 0.35 * abs(0.35 - 0.5), beta
 
 
+def test_mif_prior_dry_run(tmp_path):
+    """mif --prior (patch_likelihood_prior, methods.c:469-523; is_bayesian, mif/build.c:45):
+    uniform and pseudo-uniform priors move on the logit_ab scale of [min, max]
+    (transform.c:332-350), a normal prior keeps the parameter's transform; every estimated
+    component carries its prior with exponent 1/nb_obs at every interval (process /
+    observation) or 1/L at intervals n < L (initial conditions); components not estimated
+    carry none."""
+    d = model_dir(tmp_path, "sir")
+    theta = json.load(open(os.path.join(PKG, "examples", "sir", "theta.json")))
+    theta["parameter"]["beta"]["group"]["all"]["prior"]["value"] = "normal"
+    theta["parameter"]["rep"]["group"]["all"]["prior"]["value"] = "pseudo_uniform"
+    theta["parameter"]["phi"]["group"]["all"]["sd_transf"]["value"] = 0.0
+    p = tmp_path / "t.json"
+    p.write_text(json.dumps(theta) + "\n")
+    out = json.loads(run_mif(d, ["sde", "-J", "10", "--prior", "--dry_run"], str(p)).stdout)
+    N, L = out["N"], out["L"]
+    assert out["tr"] == [3, 3, 1, 3, 3, 3]
+    assert out["prior"] == [0, 0, 1, 0, 2, -1]
+    assert out["prior_pow"] == [1.0 / L, 1.0 / L, 1.0 / N, 1.0 / N, 1.0 / N, 0.0]
+    assert out["prior_until"] == [L, L, N, N, N, 0]
+    # theta on the logit_ab scale: f_inv gives back the guesses
+    from oracle import mif_oracle
+    for o, g in ((0, 0.999), (3, 5.0), (4, 0.6)):
+        assert mif_oracle.f_inv(3, out["theta"][o], out["min"][o], out["max"][o]) == pytest.approx(g, rel=1e-14)
+    plain = json.loads(run_mif(d, ["sde", "-J", "10", "--dry_run"], str(p)).stdout)
+    assert plain["prior"] == [] and plain["tr"] == [2, 2, 1, 1, 2, 1]
+
+
+def test_mif_prior_pulls_towards_the_prior(tmp_path):
+    """With a narrow normal prior on beta (mean 0.4, sd 0.01) the driver restatement with
+    the CPU filter oracle ends nearer 0.4 than without it, from the same start and keys."""
+    from oracle import oracle, mif_oracle
+    d = model_dir(tmp_path, "sir")
+    theta = json.load(open(os.path.join(PKG, "examples", "sir", "theta.json")))
+    for k in ("S", "I", "gamma", "rep", "phi"):
+        theta["parameter"][k]["group"]["all"]["sd_transf"]["value"] = 0.0
+    g = theta["parameter"]["beta"]["group"]["all"]
+    g["guess"]["value"], g["sd_transf"]["value"] = 0.45, 0.3
+    g["prior"]["value"], g["min"]["value"], g["max"]["value"] = "normal", 0.38, 0.42
+    p = tmp_path / "t.json"
+    p.write_text(json.dumps(theta) + "\n")
+    N = 52
+    m = smc.Model.get("sir")
+    times, data = m.dataset(N)
+    ends = []
+    for extra in ([], ["--prior"]):
+        dry = json.loads(run_mif(d, ["sde", "-J", "10", "--dry_run"] + extra, str(p)).stdout)
+        dry = dict(dry, N=N, delta=dry["delta"][:N], data_rows=[r for r in dry["data_rows"] if r < N],
+                   lag_row=max(r for r in dry["data_rows"] if r <= int(0.75 * N)))
+        mp = {k: dry[k] for k in ("pv_src", "tr", "scale", "duration", "min", "max", "multiplier", "sd")}
+        sdv = np.array(dry["var"]).reshape(6, 6)
+        mp["sd"] = [math.sqrt(sdv[o, o] * 520 / N) if o == 2 else 0.0 for o in range(6)]
+        if extra:
+            mp.update(prior=dry["prior"], prior_pow=[1.0 / N if o == 2 else 0.0 for o in range(6)],
+                      prior_until=[N] * 6)
+            assert dry["prior"][2] == 1
+        dry["var"] = list((sdv * np.where(np.arange(6) >= 2, 520 / N, 1.0)[:, None]).ravel())
+
+        def pass_fn(th, x0, mut, seed, stream):
+            return oracle.mif_pass(m.id, th.shape[1], times, data, np.array(dry["pv"]), mp, th, x0, mut, seed=seed,
+                                   stream=stream, nthreads=8)
+
+        def x0_of(th):
+            return [mif_oracle.f_inv(dry["tr"][o], th[o], dry["min"][o], dry["max"][o]) * 1e6 for o in (0, 1)] + [0.0]
+
+        out = mif_oracle.run(dry, pass_fn, x0_of, lambda th: 0, M=12, J=1000, seed=2, a=0.9, switch=100)
+        ends.append(out["trace"][-1][1][2])
+    assert abs(ends[1] - 0.4) < 0.5 * abs(ends[0] - 0.4), ends
+
+
 @pytest.mark.gpu
 def test_cli_psr_matches_api(tmp_path, ctx):
     """`smc` with no implementation runs psr (the reference's default) and gives the

@@ -117,3 +117,50 @@ def test_mif_command_with_a_covariance(tmp_path):
         np.testing.assert_array_equal(row[1:-1], vals)
         if idx:
             assert row[-1] == ll
+
+
+def test_mif_command_with_a_prior(tmp_path):
+    """mif --prior: logit_ab for the uniform and pseudo-uniform priors, the prior patch of the
+    weights (patch_likelihood_prior, methods.c:469-523) on the device; the command's trace
+    equals the restatement's with the same map."""
+    d = model_dir(tmp_path, "sir")
+    th = json.load(open(THETA))
+    th["parameter"]["beta"]["group"]["all"]["prior"]["value"] = "normal"
+    th["parameter"]["rep"]["group"]["all"]["prior"]["value"] = "pseudo_uniform"
+    tpath = tmp_path / "theta_prior.json"
+    tpath.write_text(json.dumps(th) + "\n")
+    J, M = 400, 3
+    args = ["sde", "-J", str(J), "-M", str(M), "-S", "1", "--trace_17g", "-i", "0", "--prior"]
+
+    def run(a):
+        with open(tpath) as f:
+            return subprocess.run([MIF] + a, cwd=str(d), stdin=f, capture_output=True, text=True, timeout=600)
+    dry = json.loads(run(args + ["--dry_run"]).stdout)
+    assert dry["prior"] == [0, 0, 1, 0, 2, 0] and dry["tr"] == [3, 3, 1, 3, 3, 3]
+    print("mif command prior", flush=True)
+    r = run(args)
+    assert r.returncode == 0, r.stderr
+    m = smc.Model.get("sir")
+    times, data = m.dataset(dry["N"])
+    mp = {k: dry[k] for k in ("pv_src", "tr", "scale", "duration", "min", "max", "multiplier", "sd", "prior",
+                              "prior_pow", "prior_until")}
+    pv = np.array(dry["pv"])
+    tr, mn, mx = dry["tr"], dry["min"], dry["max"]
+
+    def pass_fn(thp, x0, mut, seed, stream):
+        return oracle.mif_pass(m.id, thp.shape[1], times, data, pv, mp, thp, x0, mut, seed=seed, stream=stream, nthreads=8)
+
+    def x0_of(t):
+        return [mif_oracle.f_inv(tr[o], t[o], mn[o], mx[o]) * 1e6 for o in (0, 1)] + [0.0]
+
+    def constrain(t):
+        return int(mif_oracle.f_inv(tr[0], t[0], mn[0], mx[0]) + mif_oracle.f_inv(tr[1], t[1], mn[1], mx[1]) > 1.0)
+
+    o = mif_oracle.run(dry, pass_fn, x0_of, constrain, M=M, J=J, seed=2, switch=1)
+    head, rows = read_csv(tmp_path / "sir" / "trace_0.csv")
+    assert len(rows) == M + 1
+    for (idx, vals, ll), row in zip(o["trace"], rows):
+        assert row[0] == idx
+        np.testing.assert_array_equal(row[1:-1], vals)
+        if idx:
+            assert row[-1] == ll

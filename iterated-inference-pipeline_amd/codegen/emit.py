@@ -503,13 +503,13 @@ class Emitter:
         """dy/dt of the states and of the observed incidences (the derivative form of
         step_ode, core_template.c:215-339 with noises_off == 'ode': no noise, no clipping);
         the incidences sum their reactions over the populations of their series, in the
-        series' population order.  Covariates are the constants of cv (no time-varying
-        table with this implementation)."""
+        series' population order.  Covariates: the splines of sp at t (the reference's
+        gsl_spline_eval at the integrator's t), or the constants of cv when sp is null."""
         m = self.m
         dev = "static __device__ inline __attribute__((always_inline))"
         name = "%s_ode_rhs%s" % (self.p, "_pp" if pp else "")
-        L = ["%s void %s(double t, const double *X, const double *pv, const double *lp, const double *cv, double *f)"
-             % (dev if pp else "PLOM_HD", name), "{", "    (void)pv; (void)lp; (void)cv;",
+        L = ["%s void %s(double t, const double *X, const double *pv, const double *lp, const double *cv, const plom_spline_set *sp, double *f)"
+             % (dev if pp else "PLOM_HD", name), "{", "    (void)pv; (void)lp; (void)cv; (void)sp;",
              "    const double tr = t; (void)tr;"]
         self.used_pars = set()
         self.used_cov = set()
@@ -551,7 +551,8 @@ class Emitter:
         npp = m.layout["npp"]
         binds = [("    const double P%d = lp[%d];" % (k, k)) if pp and k < npp else
                  ("    const double P%d = PLOM_UNIFORM64(pv[%d]);" % (k, k)) for k in sorted(self.used_pars)]
-        covs = ["    const double C%d = plom_cov_ld(cv + %d);" % (i, i) for i in sorted(self.used_cov)]
+        covs = ["    const double C%d = sp ? plom_spline_at(sp, %d, t) : plom_cov_ld(cv + %d);" % (i, i, i)
+                for i in sorted(self.used_cov)]
         tfs = ["    const double %s = %s;" % (name_, text) for text, name_ in
                sorted(self.time_funcs.items(), key=lambda kv: int(kv[1][2:]))]
         self.time_funcs = None
@@ -570,15 +571,15 @@ class Emitter:
              "#define PLOM_ODE_QUAL %s" % (dev if pp else "PLOM_HD"),
              '#include "plom_ode_impl.h"', ""]
         if pp:
-            L.append("%s int %s_interval_ode_pp_f64(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)"
+            L.append("%s int %s_interval_ode_pp_f64(double *X, const double *pv, const double *lp, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double eps_abs, double eps_rel)"
                      % (dev, self.p))
         else:
-            L.append("PLOM_HD int %s_interval_ode_f64(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)"
+            L.append("PLOM_HD int %s_interval_ode_f64(double *X, const double *pv, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double eps_abs, double eps_rel)"
                      % self.p)
         L.append("{")
         for u in range(m.N_TS_INC_UNIQUE):
             L.append("    X[%d] = 0.0; /* reset_inc */" % (m.inc_offset + u))
-        L.append("    return %s_ode_integrate%s(X, pv, %s, cv, t0, t1, dt, eps_abs, eps_rel);" % (self.p, sfx, "lp" if pp else "0"))
+        L.append("    return %s_ode_integrate%s(X, pv, %s, cv, sp, t0, t1, dt, eps_abs, eps_rel);" % (self.p, sfx, "lp" if pp else "0"))
         L.append("}")
         return "\n".join(L)
 
@@ -797,10 +798,10 @@ class Emitter:
         # host dispatcher on the reference's noises_off bits (prediction_util.c:143-161),
         # PLOM_IMPL_PSR / PLOM_IMPL_ODE selecting the psr / ode implementations
         # (prediction_util.c:139-170)
-        out.append("PLOM_HD void %s_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel, const plom_drift_view *dm)" % self.p)
+        out.append("PLOM_HD void %s_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel, const plom_drift_view *dm, const plom_spline_set *sp)" % self.p)
         out.append("{")
         out.append("    if (noises_off & PLOM_IMPL_ODE) {")
-        out.append("        (void)%s_interval_ode_f64(X, pv, cv, t0, t1, dt, eps_abs, eps_rel);" % self.p)
+        out.append("        (void)%s_interval_ode_f64(X, pv, cv, sp, t0, t1, dt, eps_abs, eps_rel);" % self.p)
         out.append("        return;")
         out.append("    }")
         out.append("    if (noises_off & PLOM_IMPL_PSR) {")
@@ -866,10 +867,10 @@ class Emitter:
                        % (Rt, self.p, tag))
             out.append("    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const %s *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { %s_proj2obs_pp_%s(X, pvd, lpd, cvd, obs); }"
                        % (Rt, self.p, tag))
-        out.append("    static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double ea, double er)"
-                   " { return %s_interval_ode_f64(X, pv, cv, t0, t1, dt, ea, er); }" % self.p)
-        out.append("    static __device__ inline __attribute__((always_inline)) int interval_ode_pp(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double ea, double er)"
-                   " { return %s_interval_ode_pp_f64(X, pv, lp, cv, t0, t1, dt, ea, er); }" % self.p)
+        out.append("    static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double ea, double er)"
+                   " { return %s_interval_ode_f64(X, pv, cv, sp, t0, t1, dt, ea, er); }" % self.p)
+        out.append("    static __device__ inline __attribute__((always_inline)) int interval_ode_pp(double *X, const double *pv, const double *lp, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double ea, double er)"
+                   " { return %s_interval_ode_pp_f64(X, pv, lp, cv, sp, t0, t1, dt, ea, er); }" % self.p)
         out.append("    static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor)"
                    " { return %s_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }" % self.p)
         out.append("    static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor)"

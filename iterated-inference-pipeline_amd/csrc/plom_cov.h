@@ -21,7 +21,10 @@
  *                        t = x_{n-1}): y_i + dx (b_i + dx (c_i + dx d_i)) with
  *                        b_i = (y_{i+1} - y_i)/h_i - h_i (c_{i+1} + 2 c_i)/3 and
  *                        d_i = (c_{i+1} - c_i)/(3 h_i); linear interpolation for 2 or 3 knots;
- *                        the value itself for one knot.
+ *                        the value itself for one knot (plom_spline_value of
+ *                        plom_model_common.h, which the device evaluates too).
+ *   plom_spline_pack     the knots and coefficients of every covariate in one array
+ *                        (plom_spline_set), for the ode implementation.
  *   plom_cov_table       the table [rows][n_fix] of a filter (layout below).
  *
  * Table layout (n_fix = the model's fixed slots of pv, in pv order: every covariate per
@@ -74,23 +77,42 @@ static int plom_spline_prepare(int n, const double *x, const double *y, double *
     return 0;
 }
 
+static int plom_cov_check(const plom_covariate *cov, int n_fix, char *err, size_t errlen);
+
 /* the value at t, x[0] <= t <= x[n-1] (checked by the caller) */
 static double plom_spline_eval(int n, const double *x, const double *y, const double *c, double t)
 {
-    if (n == 1) return y[0];
-    int lo = 0, hi = n - 1;
-    while (hi > lo + 1) {
-        const int mid = (lo + hi) / 2;
-        if (x[mid] > t) hi = mid;
-        else lo = mid;
+    return plom_spline_value(n, x, y, c, t);
+}
+
+/* the splines of n_fix covariates packed for the device (plom_spline_set): n_out[n_fix],
+ * off_out[n_fix], val_out[3 * total knots]; 0, or -1 on a bad series / no memory */
+static int plom_spline_pack(const plom_covariate *cov, int n_fix, int32_t *n_out, int32_t *off_out, double *val_out,
+                            char *err, size_t errlen)
+{
+    if (plom_cov_check(cov, n_fix, err, errlen)) return -1;
+    size_t o = 0;
+    for (int i = 0; i < n_fix; i++) {
+        const int n = cov[i].n;
+        n_out[i] = n;
+        off_out[i] = (int32_t)o;
+        memcpy(val_out + o, cov[i].x, sizeof(double) * (size_t)n);
+        memcpy(val_out + o + n, cov[i].y, sizeof(double) * (size_t)n);
+        for (int q = 0; q < n; q++) val_out[o + 2 * (size_t)n + q] = 0.0;
+        if (plom_spline_prepare(n, cov[i].x, cov[i].y, val_out + o + 2 * (size_t)n)) {
+            snprintf(err, errlen, "covariate %d: out of host memory", i);
+            return -1;
+        }
+        o += 3 * (size_t)n;
     }
-    const double h = x[lo + 1] - x[lo];
-    const double dx = t - x[lo];
-    const double dy = y[lo + 1] - y[lo];
-    if (n < 4) return y[lo] + dx * (dy / h);
-    const double b = dy / h - h * (c[lo + 1] + 2.0 * c[lo]) / 3.0;
-    const double d = (c[lo + 1] - c[lo]) / (3.0 * h);
-    return y[lo] + dx * (b + dx * (c[lo] + dx * d));
+    return 0;
+}
+
+static size_t plom_spline_vals(const plom_covariate *cov, int n_fix)
+{
+    size_t v = 0;
+    for (int i = 0; i < n_fix; i++) v += 3 * (size_t)cov[i].n;
+    return v;
 }
 
 /* substep count of the longest interval */

@@ -65,6 +65,45 @@ static __host__ __device__ inline double plom_cov_ld(const double *p) { return *
 static inline double plom_cov_ld(const double *p) { return *p; }
 #endif
 
+/* Time-varying covariates as splines, for the ode implementation, whose integrator asks for
+ * them at any time (the reference's step_ode evaluates gsl_spline_eval at the integrator's
+ * t, model_builder/Ccoder.py:39-43).  Covariate i (a fixed slot of pv, in pv order) has
+ * n[i] knots; its knot times, values and natural-spline coefficients (plom_cov.h) are
+ * val[off[i] + q], val[off[i] + n[i] + q] and val[off[i] + 2 n[i] + q]. */
+typedef struct {
+    const int32_t *n, *off;
+    const double *val;
+} plom_spline_set;
+
+/* one spline at t: on the segment x_i <= t < x_{i+1} (the first or last segment outside the
+ * knots) y_i + dx (b_i + dx (c_i + dx d_i)) with b_i = (y_{i+1} - y_i)/h_i - h_i (c_{i+1} +
+ * 2 c_i)/3 and d_i = (c_{i+1} - c_i)/(3 h_i); linear interpolation for 2 or 3 knots; the value
+ * itself for one knot */
+PLOM_HD double plom_spline_value(int n, const double *x, const double *y, const double *c, double t)
+{
+    if (n == 1) return y[0];
+    int lo = 0, hi = n - 1;
+    while (hi > lo + 1) {
+        const int mid = (lo + hi) / 2;
+        if (x[mid] > t) hi = mid;
+        else lo = mid;
+    }
+    const double h = x[lo + 1] - x[lo];
+    const double dx = t - x[lo];
+    const double dy = y[lo + 1] - y[lo];
+    if (n < 4) return y[lo] + dx * (dy / h);
+    const double b = dy / h - h * (c[lo + 1] + 2.0 * c[lo]) / 3.0;
+    const double d = (c[lo + 1] - c[lo]) / (3.0 * h);
+    return y[lo] + dx * (b + dx * (c[lo] + dx * d));
+}
+
+PLOM_HD double plom_spline_at(const plom_spline_set *sp, int i, double t)
+{
+    const int n = sp->n[i];
+    const double *b = sp->val + sp->off[i];
+    return plom_spline_value(n, b, b + n, b + 2 * n, t);
+}
+
 /* number of Euler substeps of an interval: the iterations of
  * t = t0; while (t < t1) t += dt;  (prediction_util.c:275-287, t accumulated in double) */
 PLOM_HD int plom_substeps(double t0, double t1, double dt)

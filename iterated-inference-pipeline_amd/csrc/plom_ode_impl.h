@@ -1,12 +1,14 @@
 /*
  * plom_ode_impl.h -- defines one adaptive RKF45 interval function (method and contract in
  * plom_ode.h).  Included once per right-hand side, with PLOM_ODE_FN, PLOM_ODE_RHS,
- * PLOM_ODE_N and PLOM_ODE_QUAL defined; undefines them at the end.
+ * PLOM_ODE_N and PLOM_ODE_QUAL defined; undefines them at the end.  The right-hand side is
+ * called as PLOM_ODE_RHS(t, y, pv, lp, cv, sp, dydt): sp, when not null, gives the
+ * covariates as splines of t, else cv holds their constant values.
  */
 #include "plom_ode.h"
 
-PLOM_ODE_QUAL int PLOM_ODE_FN(double *y, const double *pv, const double *lp, const double *cv, double t0, double t1,
-                              double h, double eps_abs, double eps_rel)
+PLOM_ODE_QUAL int PLOM_ODE_FN(double *y, const double *pv, const double *lp, const double *cv, const plom_spline_set *sp,
+                              double t0, double t1, double h, double eps_abs, double eps_rel)
 {
     double k1[PLOM_ODE_N], k2[PLOM_ODE_N], k3[PLOM_ODE_N], k4[PLOM_ODE_N], k5[PLOM_ODE_N], k6[PLOM_ODE_N];
     double yt[PLOM_ODE_N], yn[PLOM_ODE_N];
@@ -14,7 +16,7 @@ PLOM_ODE_QUAL int PLOM_ODE_FN(double *y, const double *pv, const double *lp, con
     int steps = 0;
     while (t < t1) {
         /* derivative at the start of the step (used by every try from this state) */
-        PLOM_ODE_RHS(t, y, pv, lp, cv, k1);
+        PLOM_ODE_RHS(t, y, pv, lp, cv, sp, k1);
         for (;;) {
             if (++steps > PLOM_ODE_MAX_STEPS) {
                 for (int i = 0; i < PLOM_ODE_N; i++) y[i] = PLOM_NAN;
@@ -27,19 +29,19 @@ PLOM_ODE_QUAL int PLOM_ODE_FN(double *y, const double *pv, const double *lp, con
                 last = 1;
             }
             for (int i = 0; i < PLOM_ODE_N; i++) yt[i] = y[i] + hs * (PLOM_RKF_A21 * k1[i]);
-            PLOM_ODE_RHS(t + PLOM_RKF_C2 * hs, yt, pv, lp, cv, k2);
+            PLOM_ODE_RHS(t + PLOM_RKF_C2 * hs, yt, pv, lp, cv, sp, k2);
             for (int i = 0; i < PLOM_ODE_N; i++) yt[i] = y[i] + hs * (PLOM_RKF_A31 * k1[i] + PLOM_RKF_A32 * k2[i]);
-            PLOM_ODE_RHS(t + PLOM_RKF_C3 * hs, yt, pv, lp, cv, k3);
+            PLOM_ODE_RHS(t + PLOM_RKF_C3 * hs, yt, pv, lp, cv, sp, k3);
             for (int i = 0; i < PLOM_ODE_N; i++)
                 yt[i] = y[i] + hs * (PLOM_RKF_A41 * k1[i] + PLOM_RKF_A42 * k2[i] + PLOM_RKF_A43 * k3[i]);
-            PLOM_ODE_RHS(t + PLOM_RKF_C4 * hs, yt, pv, lp, cv, k4);
+            PLOM_ODE_RHS(t + PLOM_RKF_C4 * hs, yt, pv, lp, cv, sp, k4);
             for (int i = 0; i < PLOM_ODE_N; i++)
                 yt[i] = y[i] + hs * (PLOM_RKF_A51 * k1[i] + PLOM_RKF_A52 * k2[i] + PLOM_RKF_A53 * k3[i] + PLOM_RKF_A54 * k4[i]);
-            PLOM_ODE_RHS(t + hs, yt, pv, lp, cv, k5);
+            PLOM_ODE_RHS(t + hs, yt, pv, lp, cv, sp, k5);
             for (int i = 0; i < PLOM_ODE_N; i++)
                 yt[i] = y[i] + hs * (PLOM_RKF_A61 * k1[i] + PLOM_RKF_A62 * k2[i] + PLOM_RKF_A63 * k3[i] +
                                      PLOM_RKF_A64 * k4[i] + PLOM_RKF_A65 * k5[i]);
-            PLOM_ODE_RHS(t + PLOM_RKF_C6 * hs, yt, pv, lp, cv, k6);
+            PLOM_ODE_RHS(t + PLOM_RKF_C6 * hs, yt, pv, lp, cv, sp, k6);
             /* fifth-order solution, error estimate and the largest error ratio */
             double rmax = PLOM_DBL_MIN;
             for (int i = 0; i < PLOM_ODE_N; i++) {

@@ -121,6 +121,10 @@ struct plom_dfilter {
     double *mpw, *mps, *mpm, *mpq;  /* tile partials: [T] sum w, [n_theta][T] sum w theta, mean, M2 */
     double *mmean, *mvar;           /* [(N+1)][n_theta] weighted mean and variance of theta */
     void *allX;                     /* PLOM_FLAG_ALL_STATES: [(N+1)][S][Jp] every propagated cloud */
+    /* the ode implementation with time-varying covariates: their splines, evaluated at the
+     * integrator's times (spl_on = 0: the covariates are constants) */
+    plom_spline_set spl;
+    int32_t spl_on, pad_spl;
 };
 
 /* ------------------------------------------------------------------ covariate rows */
@@ -382,8 +386,9 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
     if (valid) {
         if constexpr (V == 6) {
             /* the ode implementation: deterministic, no random numbers */
-            if constexpr (MIF) M::interval_ode_pp((double *)X, f->pv, lp, f->cov, t0, t1, f->dt, f->eps_abs, f->eps_rel);
-            else M::interval_ode((double *)X, f->pv, f->cov, t0, t1, f->dt, f->eps_abs, f->eps_rel);
+            const plom_spline_set *sp = f->spl_on ? &f->spl : nullptr;
+            if constexpr (MIF) M::interval_ode_pp((double *)X, f->pv, lp, f->cov, sp, t0, t1, f->dt, f->eps_abs, f->eps_rel);
+            else M::interval_ode((double *)X, f->pv, f->cov, sp, t0, t1, f->dt, f->eps_abs, f->eps_rel);
         } else if constexpr (sizeof(R) == 8) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j, s_bm);
@@ -723,7 +728,8 @@ __global__ __launch_bounds__(64) void k_lineage(plom_dfilter *f)
         const double t0 = f->times[n], t1 = f->times[n + 1];
         const uint32_t jn = (uint32_t)f->lin_slots[n];
         if constexpr (V == 6) {
-            M::interval_ode((double *)X, f->pv, f->cov, t0, t1, f->dt, f->eps_abs, f->eps_rel);
+            M::interval_ode((double *)X, f->pv, f->cov, f->spl_on ? &f->spl : nullptr, t0, t1, f->dt, f->eps_abs,
+                            f->eps_rel);
         } else if constexpr (sizeof(R) == 8) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, jn, s_bm);
@@ -1228,13 +1234,14 @@ struct layout {
         off_data, off_pv, off_pvf, off_x0, off_pm_src, off_pm_tr, off_pm_sc, off_pm_dur, off_pm_min, off_pm_max,
         off_pm_mult, off_pm_sd, off_mutfac, off_cov, off_dm_group, off_dm_ng, off_dm_tr, off_dm_sc, off_dm_dur,
         off_dm_min, off_dm_max, off_dm_mult, off_dm_vol, off_allX, off_pm_chol, off_pm_prior, off_pm_ppow,
-        off_pm_puntil, total;   /* off_dm_vol: int32 vol slots */
+        off_pm_puntil, off_spl_n, off_spl_off, off_spl_val, total;   /* off_dm_vol: int32 vol slots */
 };
 
 /* cov_vals: values of the covariate table (0 without time-varying covariates); chol_vals: of
- * the MIF walk's covariance factor (0 without one); prior_on: the MIF prior patch */
+ * the MIF walk's covariance factor (0 without one); prior_on: the MIF prior patch; spl_vals:
+ * values of the covariate splines (the ode implementation with time-varying covariates) */
 static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, layout *L, size_t cov_vals = 0,
-                       size_t chol_vals = 0, int prior_on = 0)
+                       size_t chol_vals = 0, int prior_on = 0, size_t spl_vals = 0)
 {
     const size_t J = (size_t)d->J, N = (size_t)d->N;
     const size_t Jp = (J + TILE - 1) / TILE * TILE, T = Jp / TILE;
@@ -1299,6 +1306,9 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     PUT(off_pm_prior, prior_on ? NT * 4 : 0);
     PUT(off_pm_ppow, prior_on ? NT * 8 : 0);
     PUT(off_pm_puntil, prior_on ? NT * 4 : 0);
+    PUT(off_spl_n, spl_vals ? (size_t)mi->n_fix * 4 : 0);
+    PUT(off_spl_off, spl_vals ? (size_t)mi->n_fix * 4 : 0);
+    PUT(off_spl_val, spl_vals * 8);
     PUT(off_cov, cov_vals * 8);
     const size_t ND = (size_t)mi->n_drift * (size_t)mi->n_cac;
     PUT(off_dm_group, ND * 4);
@@ -1507,20 +1517,28 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
             if (dmap->group[q] + 1 > dng[i]) dng[i] = dmap->group[q] + 1;
         }
     }
-    if (cov_on && (d->flags & PLOM_FLAG_ODE))
-        return set_err(PLOM_ERR_ARG, "the ode implementation takes the covariates as constants in pv (no time-varying covariates)");
+    /* the ode implementation evaluates the covariates' splines itself, at its own times */
+    std::vector<int32_t> spl_n, spl_off;
+    std::vector<double> spl_val;
     if (cov_on) {
         cov_kmax = plom_cov_kmax(d->N, in->times, d->dt);
         covtab.resize(plom_cov_rows(d->N, cov_kmax) * (size_t)mi.n_fix);
         char msg[256];
         if (plom_cov_table(in->cov, mi.n_fix, d->N, in->times, d->dt, cov_kmax, covtab.data(), msg, sizeof msg))
             return set_err(PLOM_ERR_ARG, "%s", msg);
+        if (d->flags & PLOM_FLAG_ODE) {
+            spl_n.resize((size_t)mi.n_fix);
+            spl_off.resize((size_t)mi.n_fix);
+            spl_val.resize(plom_spline_vals(in->cov, mi.n_fix));
+            if (plom_spline_pack(in->cov, mi.n_fix, spl_n.data(), spl_off.data(), spl_val.data(), msg, sizeof msg))
+                return set_err(PLOM_ERR_ARG, "%s", msg);
+        }
     }
     HIPCHK(hipSetDevice(c->device));
     layout L;
     const size_t chol_vals = (d->flags & PLOM_FLAG_MIF) && mm && mm->chol ? (size_t)d->n_theta * d->n_theta : 0;
     const int prior_on = (d->flags & PLOM_FLAG_MIF) && mm && mm->prior;
-    make_layout(d, &mi, &L, covtab.size(), chol_vals, prior_on);
+    make_layout(d, &mi, &L, covtab.size(), chol_vals, prior_on, spl_val.size());
     const int async = pinned != nullptr;
     std::vector<char> pageable;
     char *img = pinned;
@@ -1616,6 +1634,12 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
         h->dv.mult = (const double *)(b + L.off_dm_mult);
         h->dv.vol_slot = (const int32_t *)(b + L.off_dm_vol);
     }
+    if (!spl_val.empty()) {
+        h->spl.n = (const int32_t *)(b + L.off_spl_n);
+        h->spl.off = (const int32_t *)(b + L.off_spl_off);
+        h->spl.val = (const double *)(b + L.off_spl_val);
+        h->spl_on = 1;
+    }
     if (cov_on) {
         h->cov = (const double *)(b + L.off_cov);
         h->cov_cs = mi.n_fix;
@@ -1667,6 +1691,11 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     if (d->N > 0) memcpy(at(L.off_data), in->data, (size_t)d->N * mi.n_ts * 8);
     memcpy(at(L.off_pv), in->pv, (size_t)mi.pv_size * 8);
     if (cov_on) memcpy(at(L.off_cov), covtab.data(), covtab.size() * 8);
+    if (!spl_val.empty()) {
+        memcpy(at(L.off_spl_n), spl_n.data(), spl_n.size() * 4);
+        memcpy(at(L.off_spl_off), spl_off.data(), spl_off.size() * 4);
+        memcpy(at(L.off_spl_val), spl_val.data(), spl_val.size() * 8);
+    }
     if (drift_on) {
         memcpy(at(L.off_dm_group), dmap->group, ND * 4);
         memcpy(at(L.off_dm_ng), dng.data(), (size_t)mi.n_drift * 4);

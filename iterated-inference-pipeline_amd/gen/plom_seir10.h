@@ -3324,9 +3324,9 @@ PLOM_HD void plom_seir10_interval_psr_no_env_sto_f64(double *X, const double *pv
     }
 }
 
-PLOM_HD void plom_seir10_ode_rhs(double t, const double *X, const double *pv, const double *lp, const double *cv, double *f)
+PLOM_HD void plom_seir10_ode_rhs(double t, const double *X, const double *pv, const double *lp, const double *cv, const plom_spline_set *sp, double *f)
 {
-    (void)pv; (void)lp; (void)cv;
+    (void)pv; (void)lp; (void)cv; (void)sp;
     const double tr = t; (void)tr;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
@@ -3468,16 +3468,16 @@ PLOM_HD void plom_seir10_ode_rhs(double t, const double *X, const double *pv, co
     const double P177 = PLOM_UNIFORM64(pv[177]);
     const double P178 = PLOM_UNIFORM64(pv[178]);
     const double P179 = PLOM_UNIFORM64(pv[179]);
-    const double C0 = plom_cov_ld(cv + 0);
-    const double C1 = plom_cov_ld(cv + 1);
-    const double C2 = plom_cov_ld(cv + 2);
-    const double C3 = plom_cov_ld(cv + 3);
-    const double C4 = plom_cov_ld(cv + 4);
-    const double C5 = plom_cov_ld(cv + 5);
-    const double C6 = plom_cov_ld(cv + 6);
-    const double C7 = plom_cov_ld(cv + 7);
-    const double C8 = plom_cov_ld(cv + 8);
-    const double C9 = plom_cov_ld(cv + 9);
+    const double C0 = sp ? plom_spline_at(sp, 0, t) : plom_cov_ld(cv + 0);
+    const double C1 = sp ? plom_spline_at(sp, 1, t) : plom_cov_ld(cv + 1);
+    const double C2 = sp ? plom_spline_at(sp, 2, t) : plom_cov_ld(cv + 2);
+    const double C3 = sp ? plom_spline_at(sp, 3, t) : plom_cov_ld(cv + 3);
+    const double C4 = sp ? plom_spline_at(sp, 4, t) : plom_cov_ld(cv + 4);
+    const double C5 = sp ? plom_spline_at(sp, 5, t) : plom_cov_ld(cv + 5);
+    const double C6 = sp ? plom_spline_at(sp, 6, t) : plom_cov_ld(cv + 6);
+    const double C7 = sp ? plom_spline_at(sp, 7, t) : plom_cov_ld(cv + 7);
+    const double C8 = sp ? plom_spline_at(sp, 8, t) : plom_cov_ld(cv + 8);
+    const double C9 = sp ? plom_spline_at(sp, 9, t) : plom_cov_ld(cv + 9);
     const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
     const double r1_0 = P10;
     const double r2_0 = P20;
@@ -3606,7 +3606,7 @@ PLOM_HD void plom_seir10_ode_rhs(double t, const double *X, const double *pv, co
 #define PLOM_ODE_QUAL PLOM_HD
 #include "plom_ode_impl.h"
 
-PLOM_HD int plom_seir10_interval_ode_f64(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)
+PLOM_HD int plom_seir10_interval_ode_f64(double *X, const double *pv, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double eps_abs, double eps_rel)
 {
     X[30] = 0.0; /* reset_inc */
     X[31] = 0.0; /* reset_inc */
@@ -3618,13 +3618,13 @@ PLOM_HD int plom_seir10_interval_ode_f64(double *X, const double *pv, const doub
     X[37] = 0.0; /* reset_inc */
     X[38] = 0.0; /* reset_inc */
     X[39] = 0.0; /* reset_inc */
-    return plom_seir10_ode_integrate(X, pv, 0, cv, t0, t1, dt, eps_abs, eps_rel);
+    return plom_seir10_ode_integrate(X, pv, 0, cv, sp, t0, t1, dt, eps_abs, eps_rel);
 }
 
-PLOM_HD void plom_seir10_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel, const plom_drift_view *dm)
+PLOM_HD void plom_seir10_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel, const plom_drift_view *dm, const plom_spline_set *sp)
 {
     if (noises_off & PLOM_IMPL_ODE) {
-        (void)plom_seir10_interval_ode_f64(X, pv, cv, t0, t1, dt, eps_abs, eps_rel);
+        (void)plom_seir10_interval_ode_f64(X, pv, cv, sp, t0, t1, dt, eps_abs, eps_rel);
         return;
     }
     if (noises_off & PLOM_IMPL_PSR) {
@@ -6743,9 +6743,9 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_ode_rhs_pp(double t, const double *X, const double *pv, const double *lp, const double *cv, double *f)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_ode_rhs_pp(double t, const double *X, const double *pv, const double *lp, const double *cv, const plom_spline_set *sp, double *f)
 {
-    (void)pv; (void)lp; (void)cv;
+    (void)pv; (void)lp; (void)cv; (void)sp;
     const double tr = t; (void)tr;
     const double P0 = lp[0];
     const double P1 = lp[1];
@@ -6887,16 +6887,16 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_ode_rhs
     const double P177 = PLOM_UNIFORM64(pv[177]);
     const double P178 = PLOM_UNIFORM64(pv[178]);
     const double P179 = PLOM_UNIFORM64(pv[179]);
-    const double C0 = plom_cov_ld(cv + 0);
-    const double C1 = plom_cov_ld(cv + 1);
-    const double C2 = plom_cov_ld(cv + 2);
-    const double C3 = plom_cov_ld(cv + 3);
-    const double C4 = plom_cov_ld(cv + 4);
-    const double C5 = plom_cov_ld(cv + 5);
-    const double C6 = plom_cov_ld(cv + 6);
-    const double C7 = plom_cov_ld(cv + 7);
-    const double C8 = plom_cov_ld(cv + 8);
-    const double C9 = plom_cov_ld(cv + 9);
+    const double C0 = sp ? plom_spline_at(sp, 0, t) : plom_cov_ld(cv + 0);
+    const double C1 = sp ? plom_spline_at(sp, 1, t) : plom_cov_ld(cv + 1);
+    const double C2 = sp ? plom_spline_at(sp, 2, t) : plom_cov_ld(cv + 2);
+    const double C3 = sp ? plom_spline_at(sp, 3, t) : plom_cov_ld(cv + 3);
+    const double C4 = sp ? plom_spline_at(sp, 4, t) : plom_cov_ld(cv + 4);
+    const double C5 = sp ? plom_spline_at(sp, 5, t) : plom_cov_ld(cv + 5);
+    const double C6 = sp ? plom_spline_at(sp, 6, t) : plom_cov_ld(cv + 6);
+    const double C7 = sp ? plom_spline_at(sp, 7, t) : plom_cov_ld(cv + 7);
+    const double C8 = sp ? plom_spline_at(sp, 8, t) : plom_cov_ld(cv + 8);
+    const double C9 = sp ? plom_spline_at(sp, 9, t) : plom_cov_ld(cv + 9);
     const double r0_0 = ((P0 * (P80 * X[20] + P81 * X[21] + P82 * X[22] + P83 * X[23] + P84 * X[24] + P85 * X[25] + P86 * X[26] + P87 * X[27] + P88 * X[28] + P89 * X[29])) / C0);
     const double r1_0 = P10;
     const double r2_0 = P20;
@@ -7025,7 +7025,7 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_ode_rhs
 #define PLOM_ODE_QUAL static __device__ inline __attribute__((always_inline))
 #include "plom_ode_impl.h"
 
-static __device__ inline __attribute__((always_inline)) int plom_seir10_interval_ode_pp_f64(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)
+static __device__ inline __attribute__((always_inline)) int plom_seir10_interval_ode_pp_f64(double *X, const double *pv, const double *lp, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double eps_abs, double eps_rel)
 {
     X[30] = 0.0; /* reset_inc */
     X[31] = 0.0; /* reset_inc */
@@ -7037,7 +7037,7 @@ static __device__ inline __attribute__((always_inline)) int plom_seir10_interval
     X[37] = 0.0; /* reset_inc */
     X[38] = 0.0; /* reset_inc */
     X[39] = 0.0; /* reset_inc */
-    return plom_seir10_ode_integrate_pp(X, pv, lp, cv, t0, t1, dt, eps_abs, eps_rel);
+    return plom_seir10_ode_integrate_pp(X, pv, lp, cv, sp, t0, t1, dt, eps_abs, eps_rel);
 }
 
 static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
@@ -10151,8 +10151,8 @@ struct plom_model_seir10 {
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_seir10_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_seir10_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
-    static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_seir10_interval_ode_f64(X, pv, cv, t0, t1, dt, ea, er); }
-    static __device__ inline __attribute__((always_inline)) int interval_ode_pp(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_seir10_interval_ode_pp_f64(X, pv, lp, cv, t0, t1, dt, ea, er); }
+    static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double ea, double er) { return plom_seir10_interval_ode_f64(X, pv, cv, sp, t0, t1, dt, ea, er); }
+    static __device__ inline __attribute__((always_inline)) int interval_ode_pp(double *X, const double *pv, const double *lp, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double ea, double er) { return plom_seir10_interval_ode_pp_f64(X, pv, lp, cv, sp, t0, t1, dt, ea, er); }
     static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor) { return plom_seir10_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor) { return plom_seir10_loglik_pp(obs, y, pvd, lpd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, const plom_drift_view *dm, double t, double *out) { plom_seir10_hat_values(X, pvd, cvd, dm, t, out); }

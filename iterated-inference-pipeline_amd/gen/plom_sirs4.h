@@ -1046,9 +1046,9 @@ PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv,
     }
 }
 
-PLOM_HD void plom_sirs4_ode_rhs(double t, const double *X, const double *pv, const double *lp, const double *cv, double *f)
+PLOM_HD void plom_sirs4_ode_rhs(double t, const double *X, const double *pv, const double *lp, const double *cv, const plom_spline_set *sp, double *f)
 {
-    (void)pv; (void)lp; (void)cv;
+    (void)pv; (void)lp; (void)cv; (void)sp;
     const double tr = t; (void)tr;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
@@ -1082,10 +1082,10 @@ PLOM_HD void plom_sirs4_ode_rhs(double t, const double *X, const double *pv, con
     const double P45 = PLOM_UNIFORM64(pv[45]);
     const double P46 = PLOM_UNIFORM64(pv[46]);
     const double P47 = PLOM_UNIFORM64(pv[47]);
-    const double C0 = plom_cov_ld(cv + 0);
-    const double C1 = plom_cov_ld(cv + 1);
-    const double C2 = plom_cov_ld(cv + 2);
-    const double C3 = plom_cov_ld(cv + 3);
+    const double C0 = sp ? plom_spline_at(sp, 0, t) : plom_cov_ld(cv + 0);
+    const double C1 = sp ? plom_spline_at(sp, 1, t) : plom_cov_ld(cv + 1);
+    const double C2 = sp ? plom_spline_at(sp, 2, t) : plom_cov_ld(cv + 2);
+    const double C3 = sp ? plom_spline_at(sp, 3, t) : plom_cov_ld(cv + 3);
     const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
     const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
     const double r1_0 = P8;
@@ -1135,19 +1135,19 @@ PLOM_HD void plom_sirs4_ode_rhs(double t, const double *X, const double *pv, con
 #define PLOM_ODE_QUAL PLOM_HD
 #include "plom_ode_impl.h"
 
-PLOM_HD int plom_sirs4_interval_ode_f64(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)
+PLOM_HD int plom_sirs4_interval_ode_f64(double *X, const double *pv, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double eps_abs, double eps_rel)
 {
     X[8] = 0.0; /* reset_inc */
     X[9] = 0.0; /* reset_inc */
     X[10] = 0.0; /* reset_inc */
     X[11] = 0.0; /* reset_inc */
-    return plom_sirs4_ode_integrate(X, pv, 0, cv, t0, t1, dt, eps_abs, eps_rel);
+    return plom_sirs4_ode_integrate(X, pv, 0, cv, sp, t0, t1, dt, eps_abs, eps_rel);
 }
 
-PLOM_HD void plom_sirs4_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel, const plom_drift_view *dm)
+PLOM_HD void plom_sirs4_interval_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, unsigned int noises_off, const double *cv, int cs, double eps_abs, double eps_rel, const plom_drift_view *dm, const plom_spline_set *sp)
 {
     if (noises_off & PLOM_IMPL_ODE) {
-        (void)plom_sirs4_interval_ode_f64(X, pv, cv, t0, t1, dt, eps_abs, eps_rel);
+        (void)plom_sirs4_interval_ode_f64(X, pv, cv, sp, t0, t1, dt, eps_abs, eps_rel);
         return;
     }
     if (noises_off & PLOM_IMPL_PSR) {
@@ -2100,9 +2100,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_ode_rhs_pp(double t, const double *X, const double *pv, const double *lp, const double *cv, double *f)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_ode_rhs_pp(double t, const double *X, const double *pv, const double *lp, const double *cv, const plom_spline_set *sp, double *f)
 {
-    (void)pv; (void)lp; (void)cv;
+    (void)pv; (void)lp; (void)cv; (void)sp;
     const double tr = t; (void)tr;
     const double P0 = lp[0];
     const double P1 = lp[1];
@@ -2136,10 +2136,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_ode_rhs_
     const double P45 = PLOM_UNIFORM64(pv[45]);
     const double P46 = PLOM_UNIFORM64(pv[46]);
     const double P47 = PLOM_UNIFORM64(pv[47]);
-    const double C0 = plom_cov_ld(cv + 0);
-    const double C1 = plom_cov_ld(cv + 1);
-    const double C2 = plom_cov_ld(cv + 2);
-    const double C3 = plom_cov_ld(cv + 3);
+    const double C0 = sp ? plom_spline_at(sp, 0, t) : plom_cov_ld(cv + 0);
+    const double C1 = sp ? plom_spline_at(sp, 1, t) : plom_cov_ld(cv + 1);
+    const double C2 = sp ? plom_spline_at(sp, 2, t) : plom_cov_ld(cv + 2);
+    const double C3 = sp ? plom_spline_at(sp, 3, t) : plom_cov_ld(cv + 3);
     const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
     const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
     const double r1_0 = P8;
@@ -2189,13 +2189,13 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_ode_rhs_
 #define PLOM_ODE_QUAL static __device__ inline __attribute__((always_inline))
 #include "plom_ode_impl.h"
 
-static __device__ inline __attribute__((always_inline)) int plom_sirs4_interval_ode_pp_f64(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double eps_abs, double eps_rel)
+static __device__ inline __attribute__((always_inline)) int plom_sirs4_interval_ode_pp_f64(double *X, const double *pv, const double *lp, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double eps_abs, double eps_rel)
 {
     X[8] = 0.0; /* reset_inc */
     X[9] = 0.0; /* reset_inc */
     X[10] = 0.0; /* reset_inc */
     X[11] = 0.0; /* reset_inc */
-    return plom_sirs4_ode_integrate_pp(X, pv, lp, cv, t0, t1, dt, eps_abs, eps_rel);
+    return plom_sirs4_ode_integrate_pp(X, pv, lp, cv, sp, t0, t1, dt, eps_abs, eps_rel);
 }
 
 static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
@@ -3227,8 +3227,8 @@ struct plom_model_sirs4 {
     }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_sirs4_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sirs4_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
-    static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_sirs4_interval_ode_f64(X, pv, cv, t0, t1, dt, ea, er); }
-    static __device__ inline __attribute__((always_inline)) int interval_ode_pp(double *X, const double *pv, const double *lp, const double *cv, double t0, double t1, double dt, double ea, double er) { return plom_sirs4_interval_ode_pp_f64(X, pv, lp, cv, t0, t1, dt, ea, er); }
+    static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double ea, double er) { return plom_sirs4_interval_ode_f64(X, pv, cv, sp, t0, t1, dt, ea, er); }
+    static __device__ inline __attribute__((always_inline)) int interval_ode_pp(double *X, const double *pv, const double *lp, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double ea, double er) { return plom_sirs4_interval_ode_pp_f64(X, pv, lp, cv, sp, t0, t1, dt, ea, er); }
     static __device__ inline __attribute__((always_inline)) double loglik(const double *obs, const double *y, const double *pvd, const double *cvd, double t, double like_min, int *n_floor) { return plom_sirs4_loglik(obs, y, pvd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) double loglik_pp(const double *obs, const double *y, const double *pvd, const double *lpd, const double *cvd, double t, double like_min, int *n_floor) { return plom_sirs4_loglik_pp(obs, y, pvd, lpd, cvd, t, like_min, n_floor); }
     static __device__ inline __attribute__((always_inline)) void hat_values(const double *X, const double *pvd, const double *cvd, const plom_drift_view *dm, double t, double *out) { plom_sirs4_hat_values(X, pvd, cvd, dm, t, out); }

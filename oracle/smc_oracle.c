@@ -58,7 +58,7 @@
 typedef unsigned __int128 u128;
 
 typedef void (*interval_fn)(double *, const double *, plom_nstream64 *, double, double, double, double, unsigned int,
-                            const double *, int, double, double, const plom_drift_view *);
+                            const double *, int, double, double, const plom_drift_view *, const plom_spline_set *);
 typedef void (*proj2obs_fn)(const double *, const double *, const double *, double *);
 typedef double (*loglik_fn)(const double *, const double *, const double *, const double *, double, double, int *);
 typedef void (*moments_fn)(const double *, const double *, const double *, double, double *, double *);
@@ -138,7 +138,25 @@ typedef struct {
     const double *tab;
     int cs, kmax, N;
     double *owned;
+    /* the splines themselves (plom_spline_pack), for the ode implementation */
+    plom_spline_set spl;
+    int32_t *spl_n, *spl_off;
+    double *spl_val;
 } cov_view;
+
+static void cov_free(cov_view *c)
+{
+    free(c->owned);
+    free(c->spl_n);
+    free(c->spl_off);
+    free(c->spl_val);
+}
+
+/* the splines for an ode run, NULL without time-varying covariates */
+static const plom_spline_set *cv_splines(const cov_view *c, uint32_t noises_off)
+{
+    return (c->spl_val && (noises_off & PLOM_IMPL_ODE)) ? &c->spl : NULL;
+}
 
 static const double *cv_steps(const cov_view *c, int n) { return c->tab + (size_t)n * (size_t)c->kmax * (size_t)c->cs; }
 static const double *cv_at(const cov_view *c, int r)
@@ -171,7 +189,17 @@ static int cov_build(const oracle_model *M, int N, const double *times, double d
     out->cs = M->n_fix;
     out->owned = malloc(sizeof(double) * plom_cov_rows(N, out->kmax) * (size_t)M->n_fix);
     char msg[256];
-    const int rc = out->owned ? plom_cov_table(cv, M->n_fix, N, times, dt, out->kmax, out->owned, msg, sizeof msg) : -2;
+    int rc = out->owned ? plom_cov_table(cv, M->n_fix, N, times, dt, out->kmax, out->owned, msg, sizeof msg) : -2;
+    if (rc == 0) {
+        out->spl_n = malloc(sizeof(int32_t) * (size_t)M->n_fix);
+        out->spl_off = malloc(sizeof(int32_t) * (size_t)M->n_fix);
+        out->spl_val = malloc(sizeof(double) * (plom_spline_vals(cv, M->n_fix) + 1));
+        rc = (out->spl_n && out->spl_off && out->spl_val)
+                 ? plom_spline_pack(cv, M->n_fix, out->spl_n, out->spl_off, out->spl_val, msg, sizeof msg) : -2;
+        out->spl.n = out->spl_n;
+        out->spl.off = out->spl_off;
+        out->spl.val = out->spl_val;
+    }
     free(cv);
     out->tab = out->owned;
     return rc == 0 ? 0 : -3;
@@ -184,10 +212,10 @@ int oracle_cov_table(int model, int N, const double *times, double dt, const int
     if (model < 0 || model >= N_MODELS) return -1;
     cov_view c;
     const int rc = cov_build(&MODELS[model], N, times, dt, cov_n, cov_x, cov_y, NULL, &c);
-    if (rc) { free(c.owned); return rc; }
+    if (rc) { cov_free(&c); return rc; }
     *kmax_out = c.kmax;
     memcpy(out, c.tab, sizeof(double) * plom_cov_rows(N, c.kmax) * (size_t)MODELS[model].n_fix);
-    free(c.owned);
+    cov_free(&c);
     return 0;
 }
 
@@ -273,7 +301,7 @@ static void propagate_range(prop_job *p)
         plom_nstream64 ns;
         plom_nstream64_init(&ns, p->seed, p->stream, (uint32_t)p->n, (uint32_t)j, plom_bm_tab);
         M->interval(x, pv, &ns, p->times[p->n], p->times[p->n + 1], p->dt, p->sqdt, p->noises_off, cv_steps(p->cv, p->n),
-                    p->cv->cs, p->eps_abs, p->eps_rel, p->dm);
+                    p->cv->cs, p->eps_abs, p->eps_rel, p->dm, cv_splines(p->cv, p->noises_off));
         if (p->filter && p->nonan) {
             int nf = 0;
             const double *cvd = cv_at(p->cv, p->n + 1);
@@ -553,7 +581,7 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
     const plom_drift_view *dm = drift_view(&da);
     cov_view cv;
     const int crc = cov_build(M, N, times, dt, cov_n, cov_x, cov_y, pv, &cv);
-    if (crc) { free(cv.owned); return crc; }
+    if (crc) { cov_free(&cv); return crc; }
     const int S = M->S, n_ts = M->n_ts;
     const int T = (J + TILE - 1) / TILE;
     double *X = malloc((size_t)J * S * sizeof(double));
@@ -675,7 +703,7 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
             plom_nstream64 ns;
             plom_nstream64_init(&ns, seed, stream, (uint32_t)n, (uint32_t)lin_slots[n], plom_bm_tab);
             M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off, cv_steps(&cv, n), cv.cs, eps_abs,
-                        eps_rel, dm);
+                        eps_rel, dm, cv_splines(&cv, noises_off));
             memcpy(lin_state + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
         }
         if (lin_obs)
@@ -689,7 +717,7 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
     if (anc && anc != ancestry) free(anc);
     free(hvals); free(hbuf); free(hitems); free(hq); free(offspring);
     free(X); free(Xn); free(logw); free(work); free(cum); free(select);
-    free(cv.owned);
+    cov_free(&cv);
     return 0;
 }
 
@@ -725,7 +753,7 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
     const oracle_model *M = &MODELS[model];
     cov_view cv;
     const int crc = cov_build(M, N, times, dt, cov_n, cov_x, cov_y, pv, &cv);
-    if (crc) { free(cv.owned); return crc; }
+    if (crc) { cov_free(&cv); return crc; }
     const int S = M->S, n_ts = M->n_ts, P = M->n_pp;
     const int T = (J + TILE - 1) / TILE, PT = next_pow2(T);
     double *X = malloc((size_t)J * S * sizeof(double)), *Xn = malloc((size_t)J * S * sizeof(double));
@@ -890,7 +918,7 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
     free(X); free(Xn); free(TH); free(THn); free(logw); free(work); free(cum); free(select); free(buf); free(tc);
     free(PVJ);
     free(LP);
-    free(cv.owned);
+    cov_free(&cv);
     return 0;
 }
 
@@ -908,7 +936,7 @@ int oracle_trajectory(int model, int N, uint32_t noises_off, uint32_t seed, uint
     drift_build(M, dm_group, dm_tr, dm_sc, dm_dur, dm_min, dm_max, dm_mult, dm_vol, &da);
     cov_view cv;
     const int crc = cov_build(M, N, times, dt, cov_n, cov_x, cov_y, pv, &cv);
-    if (crc) { free(cv.owned); return crc; }
+    if (crc) { cov_free(&cv); return crc; }
     const int S = M->S;
     double x[256];
     memcpy(x, x0, (size_t)S * sizeof(double));
@@ -917,10 +945,10 @@ int oracle_trajectory(int model, int N, uint32_t noises_off, uint32_t seed, uint
         plom_nstream64 ns;
         plom_nstream64_init(&ns, seed, stream, (uint32_t)n, j, plom_bm_tab);
         M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off, cv_steps(&cv, n), cv.cs, eps_abs,
-                    eps_rel, drift_view(&da));
+                    eps_rel, drift_view(&da), cv_splines(&cv, noises_off));
         memcpy(out + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
     }
-    free(cv.owned);
+    cov_free(&cv);
     return 0;
 }
 
@@ -960,8 +988,9 @@ static void ode_test_rhs(double t, const double *y, const double *pv, const doub
 
 static int ode_test_count;
 static void ode_test_rhs_counted(double t, const double *y, const double *pv, const double *lp, const double *cv,
-                                 double *f)
+                                 const plom_spline_set *sp, double *f)
 {
+    (void)sp;
     ode_test_count++;
     ode_test_rhs(t, y, pv, lp, cv, f);
 }
@@ -980,7 +1009,7 @@ int oracle_ode_test(int kind, const double *p, double *y, double t0, double t1, 
 {
     ode_kind = kind;
     ode_test_count = 0;
-    const int rc = ode_test_integrate(y, p, NULL, NULL, t0, t1, h, eps_abs, eps_rel);
+    const int rc = ode_test_integrate(y, p, NULL, NULL, NULL, t0, t1, h, eps_abs, eps_rel);
     *n_eval = ode_test_count;
     return rc;
 }

@@ -2,7 +2,8 @@
 the GPU against the CPU oracle, which is pinned in test_ode.py: bit-identical FP64
 log-likelihoods, ESS, hat and lineages; per-particle parameters (iterated filtering), where
 every lane integrates its own system with its own step sizes; batched with stochastic
-filters; the refusals (FP32, PSR, time-varying covariates)."""
+filters; time-varying covariates evaluated as splines at the integrator's times (alone,
+iterated filtering, batched, and the `smc ode` command); the refusals (FP32, PSR)."""
 import numpy as np
 import pytest
 
@@ -84,5 +85,79 @@ def test_ode_refusals(ctx):
         smc.Filter(ctx, m, 10, times, data, pv, x0, precision="fp32", flags=smc.FLAG_ODE)
     with pytest.raises(smc.PlomError):
         smc.Filter(ctx, m, 10, times, data, pv, x0, flags=smc.FLAG_ODE | smc.FLAG_PSR)
-    with pytest.raises(smc.PlomError):
-        smc.Filter(ctx, m, 10, times, data, pv, x0, flags=smc.FLAG_ODE, cov=m.covariates())
+
+
+@pytest.mark.parametrize("J,N,eps", [(257, 104, (0.0, 0.0)), (1, 60, (1e-8, 1e-8))])
+def test_ode_with_covariates_equals_oracle(ctx, J, N, eps):
+    m, times, data, pv, x0 = setup("noise", N)
+    cov = m.covariates()
+    print("ode cov", J, N, eps, flush=True)
+    f = smc.Filter(ctx, m, J, times, data, pv, x0, flags=smc.FLAG_ODE | smc.FLAG_HAT | smc.FLAG_LINEAGE,
+                   eps_abs=eps[0], eps_rel=eps[1], seed=3, cov=cov)
+    f.run()
+    g = f.read(hat=True, lineage=True)
+    f.close()
+    o = oracle.smc(m.id, J, times, data, pv, x0, ode=True, eps_abs=eps[0], eps_rel=eps[1], seed=3, hat=True,
+                   lineage=True, nthreads=8, cov=cov)
+    np.testing.assert_array_equal(g.loglik_n, o["loglik_n"])
+    np.testing.assert_array_equal(g.hat, o["hat"])
+    np.testing.assert_array_equal(g.lineage, o["lineage"])
+    assert g.loglik == o["loglik"]
+    # the splines matter: constant covariates give another path
+    assert g.loglik != oracle.smc(m.id, J, times, data, pv, x0, ode=True, eps_abs=eps[0], eps_rel=eps[1], seed=3)["loglik"]
+
+
+def test_ode_mif_pass_with_covariates_equals_oracle(ctx):
+    m, times, data, pv, mp, theta0, x0, mut = mif_setup("noise", 30, 700, 2)
+    cov = m.covariates()
+    print("ode mif cov", flush=True)
+    f = smc.Filter(ctx, m, 700, times, data, pv, x0[:, 0], flags=smc.FLAG_MIF | smc.FLAG_ODE, mif=mp, cov=cov)
+    f.mif_start(theta0, x0, mut, seed=4, stream=9)
+    f.run()
+    g = f.read(mif=True)
+    f.close()
+    o = oracle.mif_pass(m.id, 700, times, data, pv, mp, theta0, x0, mut, seed=4, stream=9, nthreads=8, ode=True,
+                        cov=cov)
+    check_equal(g, o)
+
+
+def test_ode_with_covariates_batched(ctx):
+    m, times, data, pv, x0 = setup("noise", 50)
+    cov = m.covariates()
+    cases = [(smc.FLAG_ODE, cov, 300), (0, cov, 1200), (smc.FLAG_ODE, None, 200)]
+    fs, alone = [], []
+    for k, (flags, cv, J) in enumerate(cases):
+        print("ode cov batch", k, flush=True)
+        fs.append(smc.Filter(ctx, m, J, times, data, pv, x0, flags=flags, seed=5, stream=k, cov=cv))
+        alone.append(oracle.smc(m.id, J, times, data, pv, x0, ode=bool(flags), seed=5, stream=k, cov=cv, nthreads=8))
+    b = smc.Batch(ctx, fs)
+    b.configure(1)
+    b.launch(use_graph=True)
+    for f, o in zip(fs, alone):
+        r = f.read()
+        np.testing.assert_array_equal(r.loglik_n, o["loglik_n"])
+    b.close()
+    for f in fs:
+        f.close()
+
+
+def test_smc_ode_command_with_covariates(tmp_path, ctx):
+    """`smc ode` on the noise example (time-varying N, mu_b, mu_d, prop): the command's
+    log-likelihood is the library's and the oracle's with the model's covariate series."""
+    import json
+    import os
+    from test_cli import PKG, model_dir, run_cli
+    d = model_dir(tmp_path, "noise")
+    theta = os.path.join(PKG, "examples", "noise", "theta.json")
+    m = Model.get("noise")
+    times, data = m.dataset()
+    cov = m.covariates()
+    dry = json.loads(run_cli(d, ["ode", "-J", "1", "--dry_run"], theta).stdout)
+    pv, x0 = np.array(dry["pv"]), np.array(dry["x0"])
+    print("smc ode cov command", flush=True)
+    r = run_cli(d, ["ode", "-J", "1", "-b", "-e", "-d"], theta)
+    assert r.returncode == 0, r.stderr
+    ll = float([line for line in r.stdout.splitlines() if line.startswith("logV_17g:")][0].split(":")[1])
+    g = smc.run(ctx, m, 1, times, data, pv, x0, flags=smc.FLAG_ODE, seed=2, stream=0, cov=cov)
+    o = oracle.smc(m.id, 1, times, data, pv, x0, ode=True, seed=2, stream=0, cov=cov)
+    assert ll == g.loglik == o["loglik"]

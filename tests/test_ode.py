@@ -11,13 +11,19 @@ the CPU:
   on the same equations (written out here from the model files), incidences included; at
   the reference's default tolerances (1e-6, 1e-3) within 1e-3 relative;
 * the filter: every particle follows the same path, so ESS = J and the per-interval
-  log-likelihood is the one-path likelihood, exactly.
+  log-likelihood is the one-path likelihood, exactly;
+* time-varying covariates: the right-hand side evaluates the covariates' splines at the
+  integrator's own times (the reference's step_ode calls gsl_spline_eval at t); with tight
+  tolerances the noise model's path matches scipy's DOP853 with scipy's natural cubic
+  splines in the right-hand side, and differs from the path with the covariates frozen at
+  their t = 0 values.
 """
 import math
 
 import numpy as np
 import pytest
 from scipy.integrate import solve_ivp
+from scipy.interpolate import CubicSpline
 
 from oracle import oracle
 from iip_amd import configs
@@ -140,3 +146,79 @@ def test_filter_is_the_one_path_likelihood():
         obs, ll, nf = oracle.obs_loglik(m.id, path[n + 1], data[n], pv, times[n + 1])
         assert o["loglik_n"][n] == ll
         np.testing.assert_array_equal(o["hat"][n + 1, :, 0], o["hat"][n + 1, :, 2])   # no spread
+
+
+def spline_fns(cov):
+    """scipy's version of each covariate: natural cubic spline (4 knots or more), linear (2 or
+    3), constant (1)"""
+    out = []
+    for x, y in cov:
+        if len(x) >= 4:
+            out.append(CubicSpline(x, y, bc_type="natural"))
+        elif len(x) > 1:
+            out.append(lambda t, x=x, y=y: np.interp(t, x, y))
+        else:
+            out.append(lambda t, y=y: y[0])
+    return out
+
+
+def noise_rhs_cov(pv, m, cov):
+    """noise_rhs with N, mu_b and mu_d of each city read from their splines at t"""
+    lay, fo = m.layout, m.layout["fix_off"]
+    r0, v = pv[lay["proc"]["r0"]], pv[lay["proc"]["v"]]
+    sp = spline_fns(cov)
+    c = lambda name, k, t: float(sp[lay["fixed"][name] + k - fo](t))
+
+    def f(t, y):
+        S, I = y[0:2], y[2:4]
+        out = np.zeros(6)
+        for k in (0, 1):
+            N, mb, md = c("N", k, t), c("mu_b", k, t), c("mu_d", k, t)
+            inf = r0 / N * v * I[k] * S[k]
+            out[k] = mb * N - inf - md * S[k]
+            out[2 + k] = inf - v * I[k] - md * I[k]
+            out[4] += v * I[k] + md * I[k]
+            if k == 1:
+                out[5] = inf
+        return out
+    return f
+
+
+def test_covariates_are_splines_of_the_integrators_time():
+    m = Model.get("noise")
+    N = 60
+    times, _ = m.dataset(N)
+    pv, x0 = m.pack(configs.TRUTH["noise"]), m.x0(configs.X0_PROPS["noise"])
+    cov = m.covariates()
+    assert any(len(x) >= 4 and np.ptp(y) > 0 for x, y in cov)
+    f = noise_rhs_cov(pv, m, cov)
+    ref = [x0.copy()]
+    y = x0.copy()
+    for n in range(N):
+        y[len(y) - m.desc.N_TS_INC_UNIQUE:] = 0.0
+        sol = solve_ivp(f, (times[n], times[n + 1]), y, method="DOP853", rtol=1e-13, atol=1e-10)
+        y = sol.y[:, -1].copy()
+        ref.append(y.copy())
+    ref = np.array(ref)
+    tight = oracle.trajectory(m.id, times, pv, x0, ode=True, eps_abs=1e-10, eps_rel=1e-11, cov=cov)
+    scale = np.abs(ref).max(axis=0) + 1.0
+    assert np.all(np.abs(tight[1:] - ref[1:]) <= 2e-8 * scale), np.abs(tight[1:] - ref[1:]).max(axis=0) / scale
+    frozen = oracle.trajectory(m.id, times, pv, x0, ode=True, eps_abs=1e-10, eps_rel=1e-11)
+    assert np.abs(frozen[1:] - ref[1:]).max() > 1e3 * 2e-8 * scale.max()
+
+
+def test_filter_with_covariates_is_the_one_path_likelihood():
+    m = Model.get("noise")
+    times, data = m.dataset(40)
+    pv, x0 = m.pack(configs.TRUTH["noise"]), m.x0(configs.X0_PROPS["noise"])
+    cov = m.covariates()
+    o = oracle.smc(m.id, 200, times, data, pv, x0, ode=True, nthreads=4, cov=cov)
+    path = oracle.trajectory(m.id, times, pv, x0, ode=True, cov=cov)
+    tab, kmax = oracle.cov_table(m.id, times, 0.25, cov)
+    np.testing.assert_array_equal(o["ess_n"], 200.0)
+    for n in range(40):
+        cvd = tab[40 * kmax + n + 1]
+        pvn = pv.copy()
+        pvn[m.fix_off:] = cvd
+        obs, ll, nf = oracle.obs_loglik(m.id, path[n + 1], data[n], pvn, times[n + 1])
+        assert o["loglik_n"][n] == ll

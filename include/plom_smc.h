@@ -76,7 +76,8 @@ extern "C" {
 #define PLOM_FLAG_ODE 128u            /* implementation "ode": the rate equations integrated without noise by
                                          adaptive Runge-Kutta-Fehlberg 4(5) under the error control
                                          eps_abs / eps_rel (f_prediction_ode, C/core/prediction_util.c:176-193);
-                                         FP64 only; noises_off is ignored; covariates constant (pv) */
+                                         FP64 only; noises_off is ignored; time-varying covariates are
+                                         evaluated as splines at the integrator's times */
 #define PLOM_FLAG_ALL_STATES 256u     /* keep every particle's state after every interval (smc -r, print_X,
                                          C/core/print.c:360-392): plom_filter_outputs.all_states /
                                          all_obs; keeps the ancestry */
@@ -203,7 +204,8 @@ typedef struct {
     const plom_mif_map *mif; /* PLOM_FLAG_MIF only (copied at creation) */
     const plom_covariate *cov; /* optional [n_fix]: time-varying covariates, evaluated at creation
                                   (they then replace the covariate slots of pv); NULL: the
-                                  covariates are the constants in pv */
+                                  covariates are the constants in pv; a served request's are
+                                  copied at submission */
     const plom_drift_map *drift; /* models with diffusions, unless noises_off has PLOM_NO_DRIFT_BIT or
                                     the implementation is ode (then the parameters keep their pv
                                     values); not with PLOM_FLAG_MIF */
@@ -312,8 +314,10 @@ int plom_batch_run(plom_ctx *ctx, int n_filters, plom_filter **filters, int use_
  * lineage of a PLOM_FLAG_LINEAGE request) of a done request and releases its device memory
  * (a finished request counts against the budget until then).  Requests whose work per
  * interval reaches the heavy threshold step in a lane of their own, concurrently with the
- * others, so that they do not hold back the small requests.  Results equal plom_smc_run on the same descriptor and inputs (FP64: bit
- * for bit), whatever else is being served. */
+ * others, so that they do not hold back the small requests.  Requests take time-varying
+ * covariates and drift maps like filters (checked and copied at submission).  Results equal
+ * plom_smc_run on the same descriptor and inputs (FP64: bit for bit), whatever else is being
+ * served. */
 #define PLOM_SRV_QUEUED 0
 #define PLOM_SRV_RUNNING 1
 #define PLOM_SRV_DONE 2

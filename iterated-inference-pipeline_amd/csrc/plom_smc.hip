@@ -1459,6 +1459,60 @@ static size_t image_bytes(const layout *L) { return L->total - L->off_img; }
  * With `pinned` (at least image_bytes, page-locked, kept by the caller until the stream
  * has passed the upload) the allocation and the copy are stream-ordered and nothing
  * waits; without it the call returns when the filter is on the device. */
+/* Validation and host-side preparation of a filter's time-varying covariates (their table
+ * and, for the ode implementation, their splines) and drift map; shared by the filter and
+ * the server, which checks a request when it is submitted. */
+struct input_prep {
+    std::vector<double> covtab;             /* plom_cov_table, or empty */
+    int cov_kmax = 0, cov_on = 0, drift_on = 0;
+    std::vector<int32_t> dng;               /* groups per drifting parameter */
+    std::vector<int32_t> spl_n, spl_off;    /* plom_spline_pack (ode with covariates) */
+    std::vector<double> spl_val;
+};
+
+static int prepare_inputs(const plom_filter_desc *d, const plom_model_info &mi, const plom_filter_inputs *in,
+                          input_prep *p)
+{
+    p->cov_on = in->cov != nullptr && mi.n_fix > 0;
+    p->drift_on = mi.n_drift > 0 && !(d->noises_off & PLOM_NO_DRIFT_BIT) && !(d->flags & PLOM_FLAG_ODE);
+    const plom_drift_map *dmap = in->drift;
+    const size_t ND = (size_t)mi.n_drift * (size_t)mi.n_cac;
+    p->dng.assign((size_t)mi.n_drift + 1, 0);
+    if (p->drift_on) {
+        if (d->flags & PLOM_FLAG_MIF) return set_err(PLOM_ERR_ARG, "drifting parameters do not combine with PLOM_FLAG_MIF");
+        if (!dmap || !dmap->group || !dmap->tr || !dmap->scale || !dmap->duration || !dmap->min || !dmap->max ||
+            !dmap->multiplier || !dmap->vol_slot)
+            return set_err(PLOM_ERR_ARG, "this model has drifting parameters: it needs a complete plom_drift_map "
+                                         "(or PLOM_NO_DRIFT_BIT in noises_off)");
+        for (size_t q = 0; q < ND; q++) {
+            if (dmap->group[q] < 0 || dmap->group[q] >= mi.n_cac) return set_err(PLOM_ERR_ARG, "drift group[%zu] out of range", q);
+            if (dmap->tr[q] < 0 || dmap->tr[q] > 3 || dmap->scale[q] < 0 || dmap->scale[q] > 2)
+                return set_err(PLOM_ERR_ARG, "drift %zu: bad transform or scale", q);
+            if (dmap->vol_slot[q] < -1 || dmap->vol_slot[q] >= mi.n_pp)
+                return set_err(PLOM_ERR_ARG, "drift vol_slot[%zu]=%d is not a process parameter slot of pv", q, dmap->vol_slot[q]);
+            const size_t i = q / (size_t)mi.n_cac;
+            if (dmap->group[q] + 1 > p->dng[i]) p->dng[i] = dmap->group[q] + 1;
+        }
+    }
+    /* time-varying covariates: their values at every substep and observation time; the ode
+     * implementation also evaluates their splines itself, at its own times */
+    if (p->cov_on) {
+        p->cov_kmax = plom_cov_kmax(d->N, in->times, d->dt);
+        p->covtab.resize(plom_cov_rows(d->N, p->cov_kmax) * (size_t)mi.n_fix);
+        char msg[256];
+        if (plom_cov_table(in->cov, mi.n_fix, d->N, in->times, d->dt, p->cov_kmax, p->covtab.data(), msg, sizeof msg))
+            return set_err(PLOM_ERR_ARG, "%s", msg);
+        if (d->flags & PLOM_FLAG_ODE) {
+            p->spl_n.resize((size_t)mi.n_fix);
+            p->spl_off.resize((size_t)mi.n_fix);
+            p->spl_val.resize(plom_spline_vals(in->cov, mi.n_fix));
+            if (plom_spline_pack(in->cov, mi.n_fix, p->spl_n.data(), p->spl_off.data(), p->spl_val.data(), msg, sizeof msg))
+                return set_err(PLOM_ERR_ARG, "%s", msg);
+        }
+    }
+    return PLOM_OK;
+}
+
 static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom_filter_inputs *in, char *pinned,
                               plom_filter **out, hipStream_t st = nullptr)
 {
@@ -1492,48 +1546,15 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     }
     for (int n = 0; n < d->N; n++)
         if (!(in->times[n + 1] > in->times[n])) return set_err(PLOM_ERR_ARG, "times must increase (n=%d)", n);
-    /* time-varying covariates: their values at every substep and observation time */
-    std::vector<double> covtab;
-    int cov_kmax = 0;
-    const int cov_on = in->cov != nullptr && mi.n_fix > 0;
-    /* drifting parameters */
-    const int drift_on = mi.n_drift > 0 && !(d->noises_off & PLOM_NO_DRIFT_BIT) && !(d->flags & PLOM_FLAG_ODE);
+    input_prep prep;
+    rc = prepare_inputs(d, mi, in, &prep);
+    if (rc != PLOM_OK) return rc;
+    const std::vector<double> &covtab = prep.covtab;
+    const int cov_kmax = prep.cov_kmax, cov_on = prep.cov_on, drift_on = prep.drift_on;
     const plom_drift_map *dmap = in->drift;
     const size_t ND = (size_t)mi.n_drift * (size_t)mi.n_cac;
-    std::vector<int32_t> dng((size_t)mi.n_drift + 1, 0);
-    if (drift_on) {
-        if (d->flags & PLOM_FLAG_MIF) return set_err(PLOM_ERR_ARG, "drifting parameters do not combine with PLOM_FLAG_MIF");
-        if (!dmap || !dmap->group || !dmap->tr || !dmap->scale || !dmap->duration || !dmap->min || !dmap->max ||
-            !dmap->multiplier || !dmap->vol_slot)
-            return set_err(PLOM_ERR_ARG, "this model has drifting parameters: it needs a complete plom_drift_map "
-                                         "(or PLOM_NO_DRIFT_BIT in noises_off)");
-        for (size_t q = 0; q < ND; q++) {
-            if (dmap->group[q] < 0 || dmap->group[q] >= mi.n_cac) return set_err(PLOM_ERR_ARG, "drift group[%zu] out of range", q);
-            if (dmap->tr[q] < 0 || dmap->tr[q] > 3 || dmap->scale[q] < 0 || dmap->scale[q] > 2)
-                return set_err(PLOM_ERR_ARG, "drift %zu: bad transform or scale", q);
-            if (dmap->vol_slot[q] < -1 || dmap->vol_slot[q] >= mi.n_pp)
-                return set_err(PLOM_ERR_ARG, "drift vol_slot[%zu]=%d is not a process parameter slot of pv", q, dmap->vol_slot[q]);
-            const size_t i = q / (size_t)mi.n_cac;
-            if (dmap->group[q] + 1 > dng[i]) dng[i] = dmap->group[q] + 1;
-        }
-    }
-    /* the ode implementation evaluates the covariates' splines itself, at its own times */
-    std::vector<int32_t> spl_n, spl_off;
-    std::vector<double> spl_val;
-    if (cov_on) {
-        cov_kmax = plom_cov_kmax(d->N, in->times, d->dt);
-        covtab.resize(plom_cov_rows(d->N, cov_kmax) * (size_t)mi.n_fix);
-        char msg[256];
-        if (plom_cov_table(in->cov, mi.n_fix, d->N, in->times, d->dt, cov_kmax, covtab.data(), msg, sizeof msg))
-            return set_err(PLOM_ERR_ARG, "%s", msg);
-        if (d->flags & PLOM_FLAG_ODE) {
-            spl_n.resize((size_t)mi.n_fix);
-            spl_off.resize((size_t)mi.n_fix);
-            spl_val.resize(plom_spline_vals(in->cov, mi.n_fix));
-            if (plom_spline_pack(in->cov, mi.n_fix, spl_n.data(), spl_off.data(), spl_val.data(), msg, sizeof msg))
-                return set_err(PLOM_ERR_ARG, "%s", msg);
-        }
-    }
+    const std::vector<int32_t> &dng = prep.dng, &spl_n = prep.spl_n, &spl_off = prep.spl_off;
+    const std::vector<double> &spl_val = prep.spl_val;
     HIPCHK(hipSetDevice(c->device));
     layout L;
     const size_t chol_vals = (d->flags & PLOM_FLAG_MIF) && mm && mm->chol ? (size_t)d->n_theta * d->n_theta : 0;
@@ -2437,6 +2458,14 @@ struct srv_request {
     uint64_t id;
     plom_filter_desc desc;
     std::vector<double> times, data, pv, x0;
+    /* time-varying covariates and the drift map, copied at submission */
+    std::vector<plom_covariate> cov;
+    std::vector<double> cov_x, cov_y;
+    std::vector<int32_t> dm_i;      /* group, tr, scale, duration, vol_slot */
+    std::vector<double> dm_d;       /* min, max, multiplier */
+    plom_drift_map dmap = {};
+    int has_drift = 0;
+    size_t cov_vals = 0, spl_vals = 0;  /* sizes of the covariate table and splines in the layout */
     plom_filter *f = nullptr;
     int slot = -1;
     int lane = 0;
@@ -2620,14 +2649,14 @@ extern "C" int plom_server_submit(plom_server *S, const plom_filter_desc *d, con
     if (d->flags & (PLOM_FLAG_STORE_ANCESTRY | PLOM_FLAG_FINAL_STATE | PLOM_FLAG_HAT | PLOM_FLAG_MIF | PLOM_FLAG_ALL_STATES))
         return set_err(PLOM_ERR_ARG, "served requests return log-likelihoods and lineages (no ancestry, final state, hat or MIF)");
     if (!in->times || !in->pv || !in->x0 || (d->N > 0 && !in->data)) return set_err(PLOM_ERR_ARG, "null input array");
-    if (in->cov && mi.n_fix > 0)
-        return set_err(PLOM_ERR_ARG, "served requests take the covariates as constants in pv (no time-varying covariates)");
-    if (mi.n_drift > 0 && !(d->noises_off & PLOM_NO_DRIFT_BIT) && !(d->flags & PLOM_FLAG_ODE))
-        return set_err(PLOM_ERR_ARG, "served requests run without drifting parameters (set PLOM_NO_DRIFT_BIT)");
     for (int n = 0; n < d->N; n++)
         if (!(in->times[n + 1] > in->times[n])) return set_err(PLOM_ERR_ARG, "times must increase (n=%d)", n);
+    /* the covariates and the drift map are checked now, so that admission cannot fail on them */
+    input_prep prep;
+    rc = prepare_inputs(d, mi, in, &prep);
+    if (rc != PLOM_OK) return rc;
     layout L;
-    make_layout(d, &mi, &L);
+    make_layout(d, &mi, &L, prep.covtab.size(), 0, 0, prep.spl_val.size());
     if ((int64_t)L.total > S->budget)
         return set_err(PLOM_ERR_NOMEM, "request needs %zu device bytes, more than the server budget %lld", L.total, (long long)S->budget);
     srv_request *r = new (std::nothrow) srv_request();
@@ -2638,6 +2667,35 @@ extern "C" int plom_server_submit(plom_server *S, const plom_filter_desc *d, con
     if (d->N > 0) r->data.assign(in->data, in->data + (size_t)d->N * mi.n_ts);
     r->pv.assign(in->pv, in->pv + mi.pv_size);
     r->x0.assign(in->x0, in->x0 + mi.size_proj);
+    r->cov_vals = prep.covtab.size();
+    r->spl_vals = prep.spl_val.size();
+    if (prep.cov_on) {
+        for (int i = 0; i < mi.n_fix; i++) {
+            r->cov_x.insert(r->cov_x.end(), in->cov[i].x, in->cov[i].x + in->cov[i].n);
+            r->cov_y.insert(r->cov_y.end(), in->cov[i].y, in->cov[i].y + in->cov[i].n);
+        }
+        r->cov.resize((size_t)mi.n_fix);
+        size_t o = 0;
+        for (int i = 0; i < mi.n_fix; i++) {
+            r->cov[(size_t)i] = {in->cov[i].n, 0, r->cov_x.data() + o, r->cov_y.data() + o};
+            o += (size_t)in->cov[i].n;
+        }
+    }
+    if (prep.drift_on) {
+        const size_t ND = (size_t)mi.n_drift * (size_t)mi.n_cac;
+        const plom_drift_map *m = in->drift;
+        for (const int32_t *a : {m->group, m->tr, m->scale, m->duration, m->vol_slot}) r->dm_i.insert(r->dm_i.end(), a, a + ND);
+        for (const double *a : {m->min, m->max, m->multiplier}) r->dm_d.insert(r->dm_d.end(), a, a + ND);
+        r->dmap.group = r->dm_i.data();
+        r->dmap.tr = r->dm_i.data() + ND;
+        r->dmap.scale = r->dm_i.data() + 2 * ND;
+        r->dmap.duration = r->dm_i.data() + 3 * ND;
+        r->dmap.vol_slot = r->dm_i.data() + 4 * ND;
+        r->dmap.min = r->dm_d.data();
+        r->dmap.max = r->dm_d.data() + ND;
+        r->dmap.multiplier = r->dm_d.data() + 2 * ND;
+        r->has_drift = 1;
+    }
     r->bytes = (int64_t)L.total;
     r->lane = (S->heavy > 0 && (int64_t)d->J * mi.size_proj >= S->heavy) ? 1 : 0;
     S->queue.push_back(r);
@@ -2744,13 +2802,14 @@ extern "C" int plom_server_step(plom_server *S, int32_t *n_active, int32_t *n_qu
         srv_lane *L = &S->lanes[r->lane];
         plom_model_info mi = g_models[r->desc.model_id].info;
         layout Lo;
-        make_layout(&r->desc, &mi, &Lo);
+        make_layout(&r->desc, &mi, &Lo, r->cov_vals, 0, 0, r->spl_vals);
         char *img;
         int k;
         rc = S->pool.get(image_bytes(&Lo), &img, &k);
         if (rc != PLOM_OK) return rc;
         L->step_release[(size_t)(L->steps % S->depth)].push_back({img, k});
-        plom_filter_inputs in = {r->times.data(), r->data.data(), r->pv.data(), r->x0.data()};
+        plom_filter_inputs in = {r->times.data(), r->data.data(), r->pv.data(), r->x0.data(), nullptr,
+                                 r->cov.empty() ? nullptr : r->cov.data(), r->has_drift ? &r->dmap : nullptr};
         rc = filter_create_impl(c, &r->desc, &in, img, &r->f, L->stream);
         if (rc != PLOM_OK) return rc;
         r->slot = S->free_slots.back();

@@ -574,16 +574,19 @@ class Server:
         self.N = {}
 
     def submit(self, model, J, times, data, pv, x0, precision="fp64", noises_off=0, seed=1, stream=0,
-               dt=0.25, like_min=1e-17, lineage=False, flags=0, cov=None):
+               dt=0.25, like_min=1e-17, lineage=False, flags=0, cov=None, drift=None, eps_abs=0.0, eps_rel=0.0):
+        """Queue a request (plom_server_submit): the inputs of smc.Filter, copied by the
+        library, covariates and drift map included."""
         times = np.ascontiguousarray(times, dtype=np.float64)
         N = len(times) - 1
         data = np.ascontiguousarray(data, dtype=np.float64).reshape(N, model.n_ts)
         pv = np.ascontiguousarray(pv, dtype=np.float64)
         x0 = np.ascontiguousarray(x0, dtype=np.float64)
         d = _desc(model, J, N, precision, noises_off, (FLAG_LINEAGE if lineage else 0) | int(flags), seed, stream, dt,
-                  like_min)
+                  like_min, eps_abs, eps_rel)
         covs, keep = _cov_array(model, cov)
-        inp = FilterInputs(dptr(times), dptr(data), dptr(pv), dptr(x0), None, covs)
+        dmap, keep_d = _drift_map(model, drift)
+        inp = FilterInputs(dptr(times), dptr(data), dptr(pv), dptr(x0), None, covs, dmap)
         rid = C.c_uint64()
         check(lib().plom_server_submit(self.h, C.byref(d), C.byref(inp), C.byref(rid)))
         self.N[rid.value] = (N, model.S if lineage else 0)

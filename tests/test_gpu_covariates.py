@@ -118,11 +118,44 @@ def test_fp32_table_path(ctx):
     assert abs(np.mean(l32) - np.mean(l64)) < 4 * se + 0.05, (l32, l64)
 
 
-def test_server_refuses_time_varying_covariates(ctx):
-    m, times, data, pv, x0, cov = setup(10)
+def test_served_requests_with_covariates_equal_alone(ctx):
+    """Served requests take time-varying covariates (sde, psr and ode) and drift maps,
+    copied at submission; each result equals the same filter run alone, bit for bit, mixed
+    with requests without them.  Covariates that do not cover the times are refused at
+    submission."""
+    from iip_amd import configs
+    m, times, data, pv, x0, cov = setup(60)
+    dm = Model.get("drift")
+    dtimes, ddata = dm.dataset(40)
+    dpv = dm.pack(configs.TRUTH["drift"])
+    dx0, drift = configs.drift_setup(dm, dm.x0(configs.X0_PROPS["drift"]), configs.TRUTH["drift"])
+    s = Model.get("sirs4")
+    st, sd = s.dataset(30)
+    reqs = [dict(model=m, J=900, times=times, data=data, pv=pv, x0=x0, cov=cov),
+            dict(model=s, J=2000, times=st, data=sd, pv=s.pack(configs.TRUTH["sirs4"]), x0=s.x0(configs.X0_PROPS["sirs4"])),
+            dict(model=m, J=300, times=times, data=data, pv=pv, x0=x0, cov=cov, flags=smc.FLAG_ODE),
+            dict(model=m, J=700, times=times[:31], data=data[:30], pv=pv, x0=np.round(x0), cov=cov, flags=smc.FLAG_PSR),
+            dict(model=dm, J=1200, times=dtimes, data=ddata, pv=dpv, x0=dx0, cov=dm.covariates(), drift=drift),
+            dict(model=m, J=500, times=times, data=data, pv=pv, x0=x0)]
     srv = smc.Server(ctx)
     try:
+        ids = []
+        for k, r in enumerate(reqs):
+            print("served cov", k, r["model"].name, r.get("flags", 0), flush=True)
+            ids.append(srv.submit(r["model"], r["J"], r["times"], r["data"], r["pv"], r["x0"], seed=4, stream=k,
+                                  flags=r.get("flags", 0), cov=r.get("cov"), drift=r.get("drift")))
+        for _ in range(100000):
+            srv.step()
+            if all(srv.status(i) == smc.SRV_DONE for i in ids):
+                break
+        for k, (r, i) in enumerate(zip(reqs, ids)):
+            got = srv.result(i)
+            a = smc.run(ctx, r["model"], r["J"], r["times"], r["data"], r["pv"], r["x0"], seed=4, stream=k,
+                        flags=r.get("flags", 0), cov=r.get("cov"), drift=r.get("drift"))
+            np.testing.assert_array_equal(got.loglik_n, a.loglik_n)
+            assert got.loglik == a.loglik
+        short = [(x[:2], y[:2]) if len(x) > 1 else (x, y) for x, y in cov]
         with pytest.raises(smc.PlomError):
-            srv.submit(m, 100, times, data, pv, x0, cov=cov)
+            srv.submit(m, 100, times, data, pv, x0, cov=short)
     finally:
         srv.close()

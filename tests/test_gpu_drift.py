@@ -2,7 +2,7 @@
 their natural values) against the CPU oracle, which is pinned in test_drift.py: FP64
 bit-exact for the SDE (one group per population, and one shared group) and psr
 implementations, hat and lineages, batched with models without drift; PLOM_NO_DRIFT_BIT
-keeps the pv values; the refusals (MIF, served requests, a missing map)."""
+keeps the pv values; the refusals (MIF, a missing map)."""
 import numpy as np
 import pytest
 

@@ -208,7 +208,8 @@ typedef struct {
                                   copied at submission */
     const plom_drift_map *drift; /* models with diffusions, unless noises_off has PLOM_NO_DRIFT_BIT or
                                     the implementation is ode (then the parameters keep their pv
-                                    values); not with PLOM_FLAG_MIF */
+                                    values); with PLOM_FLAG_MIF the volatilities are the particle's
+                                    own values of their pv slots */
 } plom_filter_inputs;
 
 typedef struct {

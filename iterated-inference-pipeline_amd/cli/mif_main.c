@@ -185,7 +185,13 @@ int main(int argc, char *argv[])
     cli_load(&P, prior, -1, dt);
     P.psr = psr;
     P.drift_on = P.n_drift > 0 && !(noises_off & PLOM_NO_DRIFT_BIT) && !(impl_flags & PLOM_FLAG_ODE);
-    if (P.drift_on) cli_die("mif with drifting parameters is not supported by this build (use --no_diff)");
+    /* drifting parameters: fitted by fixed-lag smoothing like the initial conditions
+     * (p_it_par_sv_and_drift, build.c:745-746); the particles' drift states start at their
+     * theta (theta_driftIC2Xdrift, prediction_util.c:119-131) and move with the particle's own
+     * volatility */
+    int *fls_par = calloc((size_t)P.n_par + 1, sizeof(int));
+    for (int i = 0; i < P.n_sv; i++) fls_par[i] = 1;
+    for (int i = 0; P.drift_on && i < P.n_drift; i++) fls_par[P.drift_par[i]] = 1;
     const int L = (int)floor(prop_L * P.N_DATA);
     if (ic_only) P.N = (L + 1 > P.N_DATA) ? P.N_DATA : L + 1;
     if (L > P.N) {
@@ -213,7 +219,7 @@ int main(int argc, char *argv[])
                 var[o * n + o] = sdv > 0.0 ? sdv * sdv : 0.0;
             }
             est[o] = free_par && var[o * n + o] > 0.0;
-            is_sv[o] = i < P.n_sv;
+            is_sv[o] = fls_par[i];
         }
     if (ic_only)
         for (int o = 0; o < n; o++)
@@ -364,7 +370,31 @@ int main(int argc, char *argv[])
         for (int o = 0; mppow && o < n; o++) printf("%s%.17g", o ? ", " : "", mppow[o]);
         printf("], \"prior_until\": [");
         for (int o = 0; muntil && o < n; o++) printf("%s%d", o ? ", " : "", muntil[o]);
-        printf("], \"delta\": [");
+        printf("], \"pop0\": [");
+        for (int cac = 0; cac < P.N_CAC; cac++) printf("%s%.17g", cac ? ", " : "", P.fixed[cac]);
+        printf("], \"drift\": ");
+        if (P.drift_on) {
+            const int ND = P.n_drift * P.N_CAC;
+            const int32_t *ia[5] = {P.dmap.group, P.dmap.tr, P.dmap.scale, P.dmap.duration, P.dmap.vol_slot};
+            const char *in[5] = {"group", "tr", "scale", "duration", "vol_slot"};
+            const double *da[3] = {P.dmap.min, P.dmap.max, P.dmap.multiplier};
+            const char *dn[3] = {"min", "max", "multiplier"};
+            printf("{");
+            for (int a = 0; a < 5; a++) {
+                printf("%s\"%s\": [", a ? ", " : "", in[a]);
+                for (int q = 0; q < ND; q++) printf("%s%d", q ? ", " : "", ia[a][q]);
+                printf("]");
+            }
+            for (int a = 0; a < 3; a++) {
+                printf(", \"%s\": [", dn[a]);
+                for (int q = 0; q < ND; q++) printf("%s%.17g", q ? ", " : "", da[a][q]);
+                printf("]");
+            }
+            printf("}");
+        } else {
+            printf("null");
+        }
+        printf(", \"delta\": [");
         for (int r = 0; r < N; r++) printf("%s%.17g", r ? ", " : "", delta[r]);
         printf("], \"data_rows\": [");
         for (int q = 0; q < n_data; q++) printf("%s%d", q ? ", " : "", data_rows[q]);
@@ -491,6 +521,11 @@ int main(int argc, char *argv[])
                         double v = cli_natural(&P, i, P.R[i].map[cac], th) * P.fixed[cac];
                         if (psr) v = round(v);
                         c->x0[(size_t)(i * P.N_CAC + cac) * J + j] = v;
+                    }
+                for (int i = 0; P.drift_on && i < P.n_drift; i++)
+                    for (int cac = 0; cac < P.N_CAC; cac++) {
+                        const int ip = P.drift_par[i];
+                        c->x0[(size_t)(P.mi.drift_offset + i * P.N_CAC + cac) * J + j] = th[P.offset[ip] + P.R[ip].map[cac]];
                     }
             }
             if (plom_filter_mif_start(ctx, c->f, c->theta, c->x0, mut, c->seed, (uint32_t)m) != PLOM_OK) cli_die(plom_last_error());

@@ -126,11 +126,13 @@ class Emitter:
         L.append("        }")
         return L
 
-    def drift_update(self, real):
+    def drift_update(self, real, pp=False):
         """compute_drift (C/core/drift.c:27-42) after the substep: x += (vol_g sqrt(dt)) z_g for
-        every group g of every drifting parameter (vol_g the pv slot the map names), z_g in group
-        order from the particle's stream (Box-Muller pairs); every population of group g moves
-        by the same amount."""
+        every group g of every drifting parameter (vol_g the pv slot the map names; with
+        per-particle parameters, the particle's own value of that slot, picked from lp by an
+        unrolled comparison so that lp stays in registers), z_g in group order from the
+        particle's stream (Box-Muller pairs); every population of group g moves by the same
+        amount."""
         m = self.m
         if not m.N_DRIFT:
             return []
@@ -150,9 +152,13 @@ class Emitter:
                   "                    } else {",
                   "                        z = zb;",
                   "                    }",
-                  "                    const int vs = dm->vol_slot[%d + g];" % (i * m.N_CAC),
-                  "                    const %s vol = vs >= 0 ? pv[vs] : %s;" % (Rt, zero),
-                  "                    const %s inc = (vol * sqdt) * z;" % Rt]
+                  "                    const int vs = dm->vol_slot[%d + g];" % (i * m.N_CAC)]
+            if pp:
+                L.append("                    %s vol = %s;" % (Rt, zero))
+                L += ["                    if (vs == %d) vol = lp[%d];" % (k, k) for k in range(m.layout["npp"])]
+            else:
+                L.append("                    const %s vol = vs >= 0 ? pv[vs] : %s;" % (Rt, zero))
+            L += ["                    const %s inc = (vol * sqdt) * z;" % Rt]
             for c in range(m.N_CAC):
                 q = m.drift_offset + i * m.N_CAC + c
                 L.append("                    if (dm->group[%d] == g) X[%d] = X[%d] + inc;" % (i * m.N_CAC + c, q, q))
@@ -342,7 +348,7 @@ class Emitter:
                     L.append("        s%d += q%d_%d;" % (idx, idx, c))
             L.append("        const %s f%d = X[%d] + s%d;" % (Rt, idx, idx, idx))
             L.append(self.clip(idx, real))
-        L += self.drift_update(real)
+        L += self.drift_update(real, pp)
         L.append("        t += dt;")
         L.append("    }")
         L.append("}")
@@ -483,7 +489,7 @@ class Emitter:
                             terms.append("n%d_%d" % (ri, c))
                 L.append("        X[%d] = X[%d] + (%s);" % (m.inc_offset + u, m.inc_offset + u, " + ".join(terms) if terms else "0.0"))
                 u += 1
-        L += self.drift_update("double")
+        L += self.drift_update("double", pp)
         L.append("        t += dt;")
         L.append("    }")
         L.append("}")

@@ -392,7 +392,7 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
         } else if constexpr (sizeof(R) == 8) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j, s_bm);
-            if constexpr (MIF) M::template interval_pp<V>((double *)X, f->pv, lp, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs, nullptr);
+            if constexpr (MIF) M::template interval_pp<V>((double *)X, f->pv, lp, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs, f->drift_on ? &f->dv : nullptr);
             else M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs, f->drift_on ? &f->dv : nullptr);
         } else {
             plom_nstream32 ns;
@@ -401,7 +401,8 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
                 float lpf[M::NPP];
 #pragma unroll
                 for (int k = 0; k < M::NPP; k++) lpf[k] = (float)lp[k];
-                M::template interval_pp<V>((float *)X, f->pvf, lpf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs, nullptr);
+                M::template interval_pp<V>((float *)X, f->pvf, lpf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs,
+                                           f->drift_on ? &f->dv : nullptr);
             } else {
                 M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs,
                                         f->drift_on ? &f->dv : nullptr);
@@ -1479,7 +1480,6 @@ static int prepare_inputs(const plom_filter_desc *d, const plom_model_info &mi, 
     const size_t ND = (size_t)mi.n_drift * (size_t)mi.n_cac;
     p->dng.assign((size_t)mi.n_drift + 1, 0);
     if (p->drift_on) {
-        if (d->flags & PLOM_FLAG_MIF) return set_err(PLOM_ERR_ARG, "drifting parameters do not combine with PLOM_FLAG_MIF");
         if (!dmap || !dmap->group || !dmap->tr || !dmap->scale || !dmap->duration || !dmap->min || !dmap->max ||
             !dmap->multiplier || !dmap->vol_slot)
             return set_err(PLOM_ERR_ARG, "this model has drifting parameters: it needs a complete plom_drift_map "

@@ -1373,7 +1373,21 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
                         z = zb;
                     }
                     const int vs = dm->vol_slot[0 + g];
-                    const double vol = vs >= 0 ? pv[vs] : 0.0;
+                    double vol = 0.0;
+                    if (vs == 0) vol = lp[0];
+                    if (vs == 1) vol = lp[1];
+                    if (vs == 2) vol = lp[2];
+                    if (vs == 3) vol = lp[3];
+                    if (vs == 4) vol = lp[4];
+                    if (vs == 5) vol = lp[5];
+                    if (vs == 6) vol = lp[6];
+                    if (vs == 7) vol = lp[7];
+                    if (vs == 8) vol = lp[8];
+                    if (vs == 9) vol = lp[9];
+                    if (vs == 10) vol = lp[10];
+                    if (vs == 11) vol = lp[11];
+                    if (vs == 12) vol = lp[12];
+                    if (vs == 13) vol = lp[13];
                     const double inc = (vol * sqdt) * z;
                     if (dm->group[0] == g) X[4] = X[4] + inc;
                     if (dm->group[1] == g) X[5] = X[5] + inc;
@@ -1529,7 +1543,21 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
                         z = zb;
                     }
                     const int vs = dm->vol_slot[0 + g];
-                    const double vol = vs >= 0 ? pv[vs] : 0.0;
+                    double vol = 0.0;
+                    if (vs == 0) vol = lp[0];
+                    if (vs == 1) vol = lp[1];
+                    if (vs == 2) vol = lp[2];
+                    if (vs == 3) vol = lp[3];
+                    if (vs == 4) vol = lp[4];
+                    if (vs == 5) vol = lp[5];
+                    if (vs == 6) vol = lp[6];
+                    if (vs == 7) vol = lp[7];
+                    if (vs == 8) vol = lp[8];
+                    if (vs == 9) vol = lp[9];
+                    if (vs == 10) vol = lp[10];
+                    if (vs == 11) vol = lp[11];
+                    if (vs == 12) vol = lp[12];
+                    if (vs == 13) vol = lp[13];
                     const double inc = (vol * sqdt) * z;
                     if (dm->group[0] == g) X[4] = X[4] + inc;
                     if (dm->group[1] == g) X[5] = X[5] + inc;
@@ -1682,7 +1710,21 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
                         z = zb;
                     }
                     const int vs = dm->vol_slot[0 + g];
-                    const double vol = vs >= 0 ? pv[vs] : 0.0;
+                    double vol = 0.0;
+                    if (vs == 0) vol = lp[0];
+                    if (vs == 1) vol = lp[1];
+                    if (vs == 2) vol = lp[2];
+                    if (vs == 3) vol = lp[3];
+                    if (vs == 4) vol = lp[4];
+                    if (vs == 5) vol = lp[5];
+                    if (vs == 6) vol = lp[6];
+                    if (vs == 7) vol = lp[7];
+                    if (vs == 8) vol = lp[8];
+                    if (vs == 9) vol = lp[9];
+                    if (vs == 10) vol = lp[10];
+                    if (vs == 11) vol = lp[11];
+                    if (vs == 12) vol = lp[12];
+                    if (vs == 13) vol = lp[13];
                     const double inc = (vol * sqdt) * z;
                     if (dm->group[0] == g) X[4] = X[4] + inc;
                     if (dm->group[1] == g) X[5] = X[5] + inc;
@@ -1761,7 +1803,21 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
                         z = zb;
                     }
                     const int vs = dm->vol_slot[0 + g];
-                    const double vol = vs >= 0 ? pv[vs] : 0.0;
+                    double vol = 0.0;
+                    if (vs == 0) vol = lp[0];
+                    if (vs == 1) vol = lp[1];
+                    if (vs == 2) vol = lp[2];
+                    if (vs == 3) vol = lp[3];
+                    if (vs == 4) vol = lp[4];
+                    if (vs == 5) vol = lp[5];
+                    if (vs == 6) vol = lp[6];
+                    if (vs == 7) vol = lp[7];
+                    if (vs == 8) vol = lp[8];
+                    if (vs == 9) vol = lp[9];
+                    if (vs == 10) vol = lp[10];
+                    if (vs == 11) vol = lp[11];
+                    if (vs == 12) vol = lp[12];
+                    if (vs == 13) vol = lp[13];
                     const double inc = (vol * sqdt) * z;
                     if (dm->group[0] == g) X[4] = X[4] + inc;
                     if (dm->group[1] == g) X[5] = X[5] + inc;
@@ -1860,7 +1916,21 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
                         z = zb;
                     }
                     const int vs = dm->vol_slot[0 + g];
-                    const double vol = vs >= 0 ? pv[vs] : 0.0;
+                    double vol = 0.0;
+                    if (vs == 0) vol = lp[0];
+                    if (vs == 1) vol = lp[1];
+                    if (vs == 2) vol = lp[2];
+                    if (vs == 3) vol = lp[3];
+                    if (vs == 4) vol = lp[4];
+                    if (vs == 5) vol = lp[5];
+                    if (vs == 6) vol = lp[6];
+                    if (vs == 7) vol = lp[7];
+                    if (vs == 8) vol = lp[8];
+                    if (vs == 9) vol = lp[9];
+                    if (vs == 10) vol = lp[10];
+                    if (vs == 11) vol = lp[11];
+                    if (vs == 12) vol = lp[12];
+                    if (vs == 13) vol = lp[13];
                     const double inc = (vol * sqdt) * z;
                     if (dm->group[0] == g) X[4] = X[4] + inc;
                     if (dm->group[1] == g) X[5] = X[5] + inc;
@@ -1939,7 +2009,21 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
                         z = zb;
                     }
                     const int vs = dm->vol_slot[0 + g];
-                    const double vol = vs >= 0 ? pv[vs] : 0.0;
+                    double vol = 0.0;
+                    if (vs == 0) vol = lp[0];
+                    if (vs == 1) vol = lp[1];
+                    if (vs == 2) vol = lp[2];
+                    if (vs == 3) vol = lp[3];
+                    if (vs == 4) vol = lp[4];
+                    if (vs == 5) vol = lp[5];
+                    if (vs == 6) vol = lp[6];
+                    if (vs == 7) vol = lp[7];
+                    if (vs == 8) vol = lp[8];
+                    if (vs == 9) vol = lp[9];
+                    if (vs == 10) vol = lp[10];
+                    if (vs == 11) vol = lp[11];
+                    if (vs == 12) vol = lp[12];
+                    if (vs == 13) vol = lp[13];
                     const double inc = (vol * sqdt) * z;
                     if (dm->group[0] == g) X[4] = X[4] + inc;
                     if (dm->group[1] == g) X[5] = X[5] + inc;
@@ -2051,7 +2135,21 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
                         z = zb;
                     }
                     const int vs = dm->vol_slot[0 + g];
-                    const float vol = vs >= 0 ? pv[vs] : 0.0f;
+                    float vol = 0.0f;
+                    if (vs == 0) vol = lp[0];
+                    if (vs == 1) vol = lp[1];
+                    if (vs == 2) vol = lp[2];
+                    if (vs == 3) vol = lp[3];
+                    if (vs == 4) vol = lp[4];
+                    if (vs == 5) vol = lp[5];
+                    if (vs == 6) vol = lp[6];
+                    if (vs == 7) vol = lp[7];
+                    if (vs == 8) vol = lp[8];
+                    if (vs == 9) vol = lp[9];
+                    if (vs == 10) vol = lp[10];
+                    if (vs == 11) vol = lp[11];
+                    if (vs == 12) vol = lp[12];
+                    if (vs == 13) vol = lp[13];
                     const float inc = (vol * sqdt) * z;
                     if (dm->group[0] == g) X[4] = X[4] + inc;
                     if (dm->group[1] == g) X[5] = X[5] + inc;
@@ -2130,7 +2228,21 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
                         z = zb;
                     }
                     const int vs = dm->vol_slot[0 + g];
-                    const float vol = vs >= 0 ? pv[vs] : 0.0f;
+                    float vol = 0.0f;
+                    if (vs == 0) vol = lp[0];
+                    if (vs == 1) vol = lp[1];
+                    if (vs == 2) vol = lp[2];
+                    if (vs == 3) vol = lp[3];
+                    if (vs == 4) vol = lp[4];
+                    if (vs == 5) vol = lp[5];
+                    if (vs == 6) vol = lp[6];
+                    if (vs == 7) vol = lp[7];
+                    if (vs == 8) vol = lp[8];
+                    if (vs == 9) vol = lp[9];
+                    if (vs == 10) vol = lp[10];
+                    if (vs == 11) vol = lp[11];
+                    if (vs == 12) vol = lp[12];
+                    if (vs == 13) vol = lp[13];
                     const float inc = (vol * sqdt) * z;
                     if (dm->group[0] == g) X[4] = X[4] + inc;
                     if (dm->group[1] == g) X[5] = X[5] + inc;
@@ -2229,7 +2341,21 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
                         z = zb;
                     }
                     const int vs = dm->vol_slot[0 + g];
-                    const float vol = vs >= 0 ? pv[vs] : 0.0f;
+                    float vol = 0.0f;
+                    if (vs == 0) vol = lp[0];
+                    if (vs == 1) vol = lp[1];
+                    if (vs == 2) vol = lp[2];
+                    if (vs == 3) vol = lp[3];
+                    if (vs == 4) vol = lp[4];
+                    if (vs == 5) vol = lp[5];
+                    if (vs == 6) vol = lp[6];
+                    if (vs == 7) vol = lp[7];
+                    if (vs == 8) vol = lp[8];
+                    if (vs == 9) vol = lp[9];
+                    if (vs == 10) vol = lp[10];
+                    if (vs == 11) vol = lp[11];
+                    if (vs == 12) vol = lp[12];
+                    if (vs == 13) vol = lp[13];
                     const float inc = (vol * sqdt) * z;
                     if (dm->group[0] == g) X[4] = X[4] + inc;
                     if (dm->group[1] == g) X[5] = X[5] + inc;
@@ -2308,7 +2434,21 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
                         z = zb;
                     }
                     const int vs = dm->vol_slot[0 + g];
-                    const float vol = vs >= 0 ? pv[vs] : 0.0f;
+                    float vol = 0.0f;
+                    if (vs == 0) vol = lp[0];
+                    if (vs == 1) vol = lp[1];
+                    if (vs == 2) vol = lp[2];
+                    if (vs == 3) vol = lp[3];
+                    if (vs == 4) vol = lp[4];
+                    if (vs == 5) vol = lp[5];
+                    if (vs == 6) vol = lp[6];
+                    if (vs == 7) vol = lp[7];
+                    if (vs == 8) vol = lp[8];
+                    if (vs == 9) vol = lp[9];
+                    if (vs == 10) vol = lp[10];
+                    if (vs == 11) vol = lp[11];
+                    if (vs == 12) vol = lp[12];
+                    if (vs == 13) vol = lp[13];
                     const float inc = (vol * sqdt) * z;
                     if (dm->group[0] == g) X[4] = X[4] + inc;
                     if (dm->group[1] == g) X[5] = X[5] + inc;

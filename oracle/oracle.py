@@ -51,7 +51,7 @@ def lib():
         L.oracle_mif_pass.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double,
                                       C.c_double, dp, dp, dp, C.c_int, i32p, i32p, i32p, i32p, dp, dp, dp, dp, dp,
                                       dp, dp, C.c_int, dp, i32p, dp, dp, dp, dp, dp, dp, i32p, i32p, dp, dp, C.c_double,
-                                      C.c_double, dp, i32p, dp, i32p]
+                                      C.c_double, dp, i32p, dp, i32p, ip, ip, ip, ip, dp, dp, dp, ip]
         L.oracle_mif_prior_pdf.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
         L.oracle_mif_prior_pdf.restype = C.c_double
         L.oracle_mif_prior_term.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double,
@@ -201,11 +201,13 @@ def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=
 
 def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=0, seed=1, stream=0, dt=0.25,
              like_min=1e-17, nthreads=1, theta_out=False, logw=False, ancestry=False, psr=False, cov=None, ode=False,
-             eps_abs=0.0, eps_rel=0.0):
+             eps_abs=0.0, eps_rel=0.0, drift=None):
     """One pass of iterated filtering restated on the CPU (oracle_mif_pass).  mif: dict of
     pv_src [n_pp], tr, scale, duration, min, max, multiplier, sd [n_theta], optionally chol
     [n_theta, n_theta] (a correlated walk) and optionally prior, prior_pow, prior_until
-    [n_theta] (the prior patch of the weights); theta0 [n_theta, J]; x0 [S, J]; mut_fac [N]."""
+    [n_theta] (the prior patch of the weights); theta0 [n_theta, J]; x0 [S, J]; mut_fac [N];
+    drift: the drift map of a model with diffusions (the volatilities are read from each
+    particle's own parameters)."""
     noises_off = _impl(noises_off, psr, ode)
     times = np.ascontiguousarray(times, dtype=np.float64)
     N = len(times) - 1
@@ -230,7 +232,7 @@ def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=
                                _dp(m["max"]), _dp(m["multiplier"]), _dp(m["sd"]), _dp(mut_fac), _dp(theta0), _dp(x0),
                                nthreads, C.byref(ll), C.byref(naf), _dp(ll_n), _dp(ess), _dp(mm), _dp(mv), _dp(tho),
                                _dp(lw), _ip(anc), *_cov_args(cov), *_eps(eps_abs, eps_rel), _dp(chol), _ip(pr[0]),
-                               _dp(pr[1]), _ip(pr[2]))
+                               _dp(pr[1]), _ip(pr[2]), *_drift_args(drift))
     if rc != 0:
         raise RuntimeError("oracle_mif_pass failed: %d" % rc)
     return {"loglik": ll.value, "n_all_fail": naf.value, "loglik_n": ll_n, "ess_n": ess, "mif_mean": mm,

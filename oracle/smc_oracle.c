@@ -746,11 +746,16 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
                     double *loglik_n, double *ess_n, double *mif_mean, double *mif_var, double *theta_out,
                     double *logw_out, int32_t *ancestry, const int32_t *cov_n, const double *cov_x,
                     const double *cov_y, double eps_abs, double eps_rel, const double *chol, const int32_t *prior,
-                    const double *prior_pow, const int32_t *prior_until)
+                    const double *prior_pow, const int32_t *prior_until, const int32_t *dm_group, const int32_t *dm_tr,
+                    const int32_t *dm_sc, const int32_t *dm_dur, const double *dm_min, const double *dm_max,
+                    const double *dm_mult, const int32_t *dm_vol)
 {
     if (model < 0 || model >= N_MODELS || J < 1 || N < 0 || n_theta < 1) return -1;
     if (chol && n_theta > PLOM_MIF_MVN_MAX) return -1;
     const oracle_model *M = &MODELS[model];
+    /* drifting parameters: every particle's volatility is its own pv row's slot */
+    drift_arg da;
+    drift_build(M, dm_group, dm_tr, dm_sc, dm_dur, dm_min, dm_max, dm_mult, dm_vol, &da);
     cov_view cv;
     const int crc = cov_build(M, N, times, dt, cov_n, cov_x, cov_y, pv, &cv);
     if (crc) { cov_free(&cv); return crc; }
@@ -838,6 +843,7 @@ int oracle_mif_pass(int model, int J, int N, uint32_t noises_off, uint32_t seed,
         job.dt = dt; job.sqdt = plom_sqrt(dt); job.like_min = like_min; job.mode = ORACLE_CANONICAL;
         job.eps_abs = eps_abs; job.eps_rel = eps_rel;
         job.lprior = LP; job.all_fail_llike = all_fail_llike;
+        job.dm = drift_view(&da);
         propagate_all(&job, nthreads);
         double ln = 0.0, es = 0.0;
         int resampled = 0;

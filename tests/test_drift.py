@@ -118,3 +118,68 @@ def test_hat_reports_the_natural_parameter_and_no_drift_keeps_pv():
     p2 = oracle.trajectory(m.id, times, pv, x0, noises_off=3, seed=2, stream=1, j=0)
     np.testing.assert_array_equal(p2[:, off:off + 2], np.repeat(x0[None, off:off + 2], 16, axis=0))
     assert a["loglik"] != o["loglik"]
+
+
+# iterated filtering with drift: theta = r0 of each city (the drift states' initial values,
+# fitted by fixed-lag smoothing: no walk), v, vol of each city (the particle's own
+# volatility), rep, phi
+DRIFT_MAP = {"pv_src": [-1, -1, 2, 2, 3, 4, 5, 5, 5, 5, 6, 6, 6, 6], "tr": [1, 1, 1, 1, 1, 2, 1], "scale": [0] * 7,
+             "duration": [0] * 7, "min": [0.0] * 7, "max": [0.0] * 7, "multiplier": [1.0] * 7,
+             "sd": [0.0, 0.0, 0.02, 0.05, 0.05, 0.02, 0.02]}
+
+
+def mif_drift_setup(N, J, seed, spread=0.0):
+    m, times, data, pv, x0, drift = setup(N)
+    t = configs.TRUTH["drift"]
+    th = np.array([math.log(t["r0"]), math.log(t["r0"]) + 0.1, math.log(t["v"]), math.log(t["vol"]),
+                   math.log(t["vol"]) + 0.3, math.log(t["rep"] / (1 - t["rep"])), math.log(t["phi"])])
+    rng = np.random.default_rng(seed)
+    theta0 = th[:, None] + spread * rng.normal(0.0, 1.0, (7, J))
+    X0 = np.repeat(x0[:, None], J, axis=1)
+    off = m.desc.drift_offset
+    X0[off:off + 2] = theta0[0:2]
+    return m, times, data, pv, drift, theta0, X0
+
+
+def natural_pv(pv, theta):
+    out = pv.copy()
+    for s, o in enumerate(DRIFT_MAP["pv_src"]):
+        if o >= 0:
+            out[s] = oracle.mif_natural(DRIFT_MAP["tr"][o], 0, 0, theta[o], 0.0, 0.0, 1.0)
+    return out
+
+
+def test_mif_pass_with_drift_and_equal_parameters_is_the_filter():
+    """Equal theta in every particle, no walk: the pass is the drift filter of those
+    parameters, bit for bit (the drift states start at theta's r0)."""
+    N, J = 30, 600
+    m, times, data, pv, drift, theta0, X0 = mif_drift_setup(N, J, 1)
+    pv_eq = natural_pv(pv, theta0[:, 0])
+    r = oracle.mif_pass(m.id, J, times, data, pv_eq, dict(DRIFT_MAP, sd=[0.0] * 7), theta0, X0, np.full(N, 0.7),
+                        seed=5, stream=3, nthreads=8, drift=drift, cov=m.covariates())
+    o = oracle.smc(m.id, J, times, data, pv_eq, X0[:, 0], seed=5, stream=3, nthreads=8, drift=drift, cov=m.covariates())
+    assert r["loglik"] == o["loglik"]
+    np.testing.assert_array_equal(r["loglik_n"], o["loglik_n"])
+    np.testing.assert_array_equal(r["ess_n"], o["ess_n"])
+
+
+def test_mif_drift_uses_each_particles_volatility():
+    """Different theta per particle, only the drift's noise: at the first interval every
+    particle's log-weight is the likelihood of its own path (its volatility, its initial
+    drift state, its stream)."""
+    N, J = 6, 40
+    m, times, data, pv, drift, theta0, X0 = mif_drift_setup(N, J, 2, spread=0.3)
+    cov = m.covariates()
+    r = oracle.mif_pass(m.id, J, times, data, pv, DRIFT_MAP, theta0, X0, np.full(N, 0.7), noises_off=3, seed=7,
+                        stream=2, drift=drift, cov=cov, logw=True)
+    tab, kmax = oracle.cov_table(m.id, times, 0.25, cov)
+    for j in range(J):
+        pvj = natural_pv(pv, theta0[:, j])
+        path = oracle.trajectory(m.id, times[:2], pvj, X0[:, j], noises_off=3, seed=7, stream=2, j=j, drift=drift,
+                                 cov=cov)
+        pvo = pvj.copy()
+        pvo[m.fix_off:] = tab[N * kmax + 1]
+        obs, ll, nf = oracle.obs_loglik(m.id, path[1], data[0], pvo, times[1])
+        assert r["logw"][0][j] == (-math.inf if nf == m.n_ts else ll), j   # every series floored: failure
+    # the volatility matters: the drift states of two particles with different vol differ
+    assert len(set(np.round(r["logw"][0], 12))) > J // 2

@@ -104,3 +104,76 @@ def test_drift_refusals(ctx):
         assert r.loglik == o["loglik"]
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("J,N,noises_off", [(1500, 40, 0), (300, 20, 3)])
+def test_mif_pass_with_drift_equals_oracle(ctx, J, N, noises_off):
+    """Iterated filtering with drifting parameters: the drift states start at the particles'
+    theta and move with each particle's own volatility (a walked component)."""
+    from test_drift import DRIFT_MAP, mif_drift_setup
+    from test_gpu_mif import check_equal
+    m, times, data, pv, drift, theta0, X0 = mif_drift_setup(N, J, 3, spread=0.05)
+    mut = np.linspace(0.5, 1.0, N)
+    cov = m.covariates()
+    print("mif drift", J, N, noises_off, flush=True)
+    f = smc.Filter(ctx, m, J, times, data, pv, X0[:, 0], noises_off=noises_off, flags=smc.FLAG_MIF, mif=DRIFT_MAP,
+                   cov=cov, drift=drift)
+    f.mif_start(theta0, X0, mut, seed=6, stream=2)
+    f.run()
+    g = f.read(mif=True)
+    f.close()
+    o = oracle.mif_pass(m.id, J, times, data, pv, DRIFT_MAP, theta0, X0, mut, noises_off=noises_off, seed=6, stream=2,
+                        nthreads=8, drift=drift, cov=cov)
+    check_equal(g, o)
+
+
+def test_mif_command_with_drift(tmp_path):
+    """`mif` on the drift example: r0's groups are fitted by fixed-lag smoothing (drift
+    states from theta), vol by the random walk; the command's trace equals the restatement's
+    (oracle/mif_oracle.py driving the CPU pass with the dry run's map)."""
+    import json
+    import os
+    import subprocess
+    from oracle import mif_oracle
+    from test_cli import PKG, model_dir
+    from test_gpu_mif_cli import MIF, read_csv
+    d = model_dir(tmp_path, "drift")
+    theta = os.path.join(PKG, "examples", "drift", "theta.json")
+    J, M = 300, 3
+    args = ["sde", "-J", str(J), "-M", str(M), "-S", "1", "--trace_17g", "-i", "0"]
+
+    def run(a):
+        with open(theta) as fh:
+            return subprocess.run([MIF] + a, cwd=str(d), stdin=fh, capture_output=True, text=True, timeout=600)
+    dry = json.loads(run(args + ["--dry_run"]).stdout)
+    assert dry["drift"] is not None and dry["sv"][4] == 1 and dry["sd"][4] == 0.0
+    print("mif command drift", flush=True)
+    r = run(args)
+    assert r.returncode == 0, r.stderr
+    m = Model.get("drift")
+    times, data = m.dataset(dry["N"])
+    mp = {k: dry[k] for k in ("pv_src", "tr", "scale", "duration", "min", "max", "multiplier", "sd")}
+    pv, pop0, tr, mn, mx = np.array(dry["pv"]), dry["pop0"], dry["tr"], dry["min"], dry["max"]
+    cov = m.covariates()
+
+    def pass_fn(th, x0, mut, seed, stream):
+        return oracle.mif_pass(m.id, th.shape[1], times, data, pv, mp, th, x0, mut, seed=seed, stream=stream,
+                               nthreads=8, drift=dry["drift"], cov=cov)
+
+    def x0_of(t):
+        return ([mif_oracle.f_inv(tr[o], t[o], mn[o], mx[o]) * pop0[o % 2] for o in range(4)] + [t[4], t[5]] +
+                [0.0] * (m.S - 6))
+
+    def constrain(t):
+        return sum(int(mif_oracle.f_inv(tr[c], t[c], mn[c], mx[c]) + mif_oracle.f_inv(tr[2 + c], t[2 + c], mn[2 + c],
+                                                                                      mx[2 + c]) > 1.0) for c in (0, 1))
+
+    np.testing.assert_array_equal(x0_of(dry["theta"]), dry["x0"])
+    o = mif_oracle.run(dry, pass_fn, x0_of, constrain, M=M, J=J, seed=2, switch=1)
+    head, rows = read_csv(tmp_path / "drift" / "trace_0.csv")
+    assert len(rows) == M + 1
+    for (idx, vals, ll), row in zip(o["trace"], rows):
+        assert row[0] == idx
+        np.testing.assert_array_equal(row[1:-1], vals)
+        if idx:
+            assert row[-1] == ll

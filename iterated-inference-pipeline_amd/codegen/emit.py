@@ -224,14 +224,14 @@ class Emitter:
         qual = "PLOM_HD" if real == "double" and not pp else dev
         L = []
         if pp:
-            L.append("%s void %s_interval_%s_pp_%s(%s *X, const %s *pv, const %s *lp, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs, const plom_drift_view *dm)"
-                     % (qual, self.p, variant, tag, Rt, Rt, Rt, stream, Rt))
+            L.append("%s void %s_interval_%s_pp_%s(%s *X, const %s *pv, const %s *lp, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs, const plom_drift_view *dm, const %s *tft)"
+                     % (qual, self.p, variant, tag, Rt, Rt, Rt, stream, Rt, Rt))
         else:
-            L.append("%s void %s_interval_%s_%s(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs, const plom_drift_view *dm)"
-                     % (qual, self.p, variant, tag, Rt, Rt, stream, Rt))
+            L.append("%s void %s_interval_%s_%s(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs, const plom_drift_view *dm, const %s *tft)"
+                     % (qual, self.p, variant, tag, Rt, Rt, stream, Rt, Rt))
         L.append("{")
         L.append("    const %s dtr = (%s)dt;" % (Rt, Rt))
-        L.append("    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;%s" % (" (void)lp;" if pp else ""))
+        L.append("    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;%s" % (" (void)lp;" if pp else ""))
         self.used_pars = set()
         self.used_cov = set()
         self.used_drift = set()
@@ -353,10 +353,10 @@ class Emitter:
         L.append("    }")
         L.append("}")
         # time-only special functions (e.g. the seasonal forcing), once per substep
-        tfs = ["        const %s %s = %s;" % (Rt, name, text) for text, name in
-               sorted(self.time_funcs.items(), key=lambda kv: int(kv[1][2:]))]
+        tfs = self.tf_lines(real)
         L[tf_at:tf_at] = self.cov_loads(real) + self.drift_loads(real) + tfs
         self.drift_mode = "pv"
+        self.last_time_funcs = self.time_funcs
         self.time_funcs = None
         uni = "PLOM_UNIFORM64" if real == "double" else "PLOM_UNIFORM32"
         npp = m.layout["npp"]
@@ -381,13 +381,13 @@ class Emitter:
         dev = "static __device__ inline __attribute__((always_inline))"
         L = []
         if pp:
-            L.append("%s void %s_interval_%s_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)"
+            L.append("%s void %s_interval_%s_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)"
                      % (dev, self.p, variant))
         else:
-            L.append("PLOM_HD void %s_interval_%s_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)"
+            L.append("PLOM_HD void %s_interval_%s_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)"
                      % (self.p, variant))
         L.append("{")
-        L.append("    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;%s" % (" (void)lp;" if pp else ""))
+        L.append("    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;%s" % (" (void)lp;" if pp else ""))
         self.used_pars = set()
         self.used_cov = set()
         self.used_drift = set()
@@ -493,8 +493,7 @@ class Emitter:
         L.append("        t += dt;")
         L.append("    }")
         L.append("}")
-        tfs = ["        const double %s = %s;" % (name, text) for text, name in
-               sorted(self.time_funcs.items(), key=lambda kv: int(kv[1][2:]))]
+        tfs = self.tf_lines("double")
         L[tf_at:tf_at] = self.cov_loads("double") + self.drift_loads("double") + tfs + decl
         self.drift_mode = "pv"
         self.time_funcs = None
@@ -761,6 +760,38 @@ class Emitter:
         return "\n".join(L)
 
     # ---------------------------------------------------------------- header
+    def tf_lines(self, real):
+        """The time-only functions of a substep: from the caller's table tft[k][NTF] when it
+        is given (one evaluation per substep time for the whole block, bit-identical: the
+        same function of the same t), else evaluated here."""
+        Rt = "double" if real == "double" else "float"
+        lst = self.tf_list.get(real, []) if getattr(self, "tf_list", None) else []
+        out = []
+        for text, name in sorted(self.time_funcs.items(), key=lambda kv: int(kv[1][2:])):
+            if text in lst:
+                out.append("        const %s %s = tft ? tft[k * %d + %d] : %s;" % (Rt, name, len(lst), lst.index(text), text))
+            else:
+                out.append("        const %s %s = %s;" % (Rt, name, text))
+        return out
+
+    def collect_time_funcs(self):
+        """The model's list of time-only functions, per precision, from its full SDE step."""
+        self.tf_list = {}
+        for real in ("double", "float"):
+            self.interval("full", True, True, real)
+            self.tf_list[real] = [t for t, _ in sorted(self.last_time_funcs.items(), key=lambda kv: int(kv[1][2:]))]
+        if len(self.tf_list["double"]) != len(self.tf_list["float"]):
+            raise ValueError("time functions differ between precisions")
+
+    def time_funcs_fn(self, real):
+        Rt, tag = ("double", "f64") if real == "double" else ("float", "f32")
+        qual = "PLOM_HD" if real == "double" else "static __device__ inline __attribute__((always_inline))"
+        L = ["%s void %s_time_funcs_%s(%s tr, %s *out)" % (qual, self.p, tag, Rt, Rt), "{", "    (void)tr; (void)out;"]
+        for i, text in enumerate(self.tf_list[real]):
+            L.append("    out[%d] = %s;" % (i, text))
+        L.append("}")
+        return "\n".join(L)
+
     def header(self):
         m = self.m
         P = self.p.upper()
@@ -783,6 +814,9 @@ class Emitter:
                 "#define %s_N_FIX %d" % (P, m.layout["n_fix"]),
                 "#define %s_N_DRIFT %d" % (P, m.N_DRIFT),
                 "#define %s_DRIFT_OFFSET %d" % (P, m.drift_offset), ""]
+        self.collect_time_funcs()
+        out.append(self.time_funcs_fn("double"))
+        out.append("")
         for v, dem, env in VARIANTS:
             out.append(self.interval(v, dem, env, "double"))
             out.append("")
@@ -811,18 +845,20 @@ class Emitter:
         out.append("        return;")
         out.append("    }")
         out.append("    if (noises_off & PLOM_IMPL_PSR) {")
-        out.append("        if (noises_off & PLOM_NO_ENV_STO) %s_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
-        out.append("        else %s_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
+        out.append("        if (noises_off & PLOM_NO_ENV_STO) %s_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);" % self.p)
+        out.append("        else %s_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);" % self.p)
         out.append("        return;")
         out.append("    }")
         out.append("    const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);")
-        out.append("    if (v == 0u) %s_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
-        out.append("    else if (v == PLOM_NO_DEM_STO) %s_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
-        out.append("    else if (v == PLOM_NO_ENV_STO) %s_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
-        out.append("    else %s_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
+        out.append("    if (v == 0u) %s_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);" % self.p)
+        out.append("    else if (v == PLOM_NO_DEM_STO) %s_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);" % self.p)
+        out.append("    else if (v == PLOM_NO_ENV_STO) %s_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);" % self.p)
+        out.append("    else %s_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);" % self.p)
         out.append("}")
         out.append("")
         out.append("#if defined(__HIPCC__)")
+        out.append(self.time_funcs_fn("float"))
+        out.append("")
         for v, dem, env in VARIANTS:
             out.append(self.interval(v, dem, env, "float"))
             out.append("")
@@ -848,27 +884,30 @@ class Emitter:
         out.append("    static constexpr int S = %d, N_TS = %d, INC_OFFSET = %d, N_INC = %d, PV = %d, N_CAC = %d, N_HAT = %d, NPP = %d, FIX_OFF = %d, N_FIX = %d, N_DRIFT = %d;"
                    % (m.size_proj, m.N_TS, m.inc_offset, m.N_TS_INC_UNIQUE, m.layout["size"], m.N_CAC, self.n_hat(),
                       m.layout["npp"], m.layout["fix_off"], m.layout["n_fix"], m.N_DRIFT))
+        out.append("    static constexpr int NTF = %d;  /* time-only functions of a substep */" % len(self.tf_list["double"]))
         for tag, Rt, stream in (("f64", "double", "plom_nstream64"), ("f32", "float", "plom_nstream32")):
-            out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs, const plom_drift_view *dm)"
-                       % (Rt, Rt, stream, Rt))
+            out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs, const plom_drift_view *dm, const %s *tft)"
+                       % (Rt, Rt, stream, Rt, Rt))
             out.append("    {")
             for vi, (v, dem, env) in enumerate(VARIANTS):
                 kw = "if" if vi == 0 else "else if"
-                out.append("        %s constexpr (V == %d) %s_interval_%s_%s(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % (kw, vi, self.p, v, tag))
+                out.append("        %s constexpr (V == %d) %s_interval_%s_%s(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);" % (kw, vi, self.p, v, tag))
             if tag == "f64":
-                out.append("        else if constexpr (V == 4) %s_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
-                out.append("        else if constexpr (V == 5) %s_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
+                out.append("        else if constexpr (V == 4) %s_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);" % self.p)
+                out.append("        else if constexpr (V == 5) %s_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);" % self.p)
             out.append("    }")
-            out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(%s *X, const %s *pv, const %s *lp, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs, const plom_drift_view *dm)"
-                       % (Rt, Rt, Rt, stream, Rt))
+            out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(%s *X, const %s *pv, const %s *lp, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs, const plom_drift_view *dm, const %s *tft)"
+                       % (Rt, Rt, Rt, stream, Rt, Rt))
             out.append("    {")
             for vi, (v, dem, env) in enumerate(VARIANTS):
                 kw = "if" if vi == 0 else "else if"
-                out.append("        %s constexpr (V == %d) %s_interval_%s_pp_%s(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);" % (kw, vi, self.p, v, tag))
+                out.append("        %s constexpr (V == %d) %s_interval_%s_pp_%s(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);" % (kw, vi, self.p, v, tag))
             if tag == "f64":
-                out.append("        else if constexpr (V == 4) %s_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
-                out.append("        else if constexpr (V == 5) %s_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);" % self.p)
+                out.append("        else if constexpr (V == 4) %s_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);" % self.p)
+                out.append("        else if constexpr (V == 5) %s_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);" % self.p)
             out.append("    }")
+            out.append("    static __device__ inline __attribute__((always_inline)) void time_funcs(%s tr, %s *out) { %s_time_funcs_%s(tr, out); }"
+                       % (Rt, Rt, self.p, tag))
             out.append("    static __device__ inline __attribute__((always_inline)) void proj2obs(const %s *X, const double *pvd, const double *cvd, double *obs) { %s_proj2obs_%s(X, pvd, cvd, obs); }"
                        % (Rt, self.p, tag))
             out.append("    static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const %s *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { %s_proj2obs_pp_%s(X, pvd, lpd, cvd, obs); }"

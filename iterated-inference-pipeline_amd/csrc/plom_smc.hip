@@ -50,6 +50,7 @@
 typedef unsigned __int128 plom_u128;
 
 #define TILE 256
+#define PLOM_TF_CAP 512   /* LDS entries of the per-interval time-function table */
 #define MAX_TILES (PLOM_MAX_J / TILE)
 
 /* ------------------------------------------------------------------ errors */
@@ -383,6 +384,25 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
     }
 
     const double t0 = f->times[n], t1 = f->times[n + 1];
+    /* the time-only functions of the interval's substeps (the seasonal forcing ...): the same
+     * for every particle of the block, so each substep time is evaluated once, by one
+     * thread, into LDS; the generated step reads the table (identical values: the same
+     * function of the same accumulated t), or evaluates them itself when they do not fit */
+    const R *tft = nullptr;
+    if constexpr (M::NTF > 0 && V != 6) {
+        __shared__ double s_tf_raw[PLOM_TF_CAP];
+        R *s_tf = (R *)s_tf_raw;
+        const int K = plom_substeps(t0, t1, f->dt);
+        if (K * M::NTF <= PLOM_TF_CAP) {
+            for (int k = threadIdx.x; k < K; k += TILE) {
+                double t = t0;
+                for (int q = 0; q < k; q++) t += f->dt;
+                M::time_funcs((R)t, s_tf + k * M::NTF);
+            }
+            __syncthreads();
+            tft = s_tf;
+        }
+    }
     if (valid) {
         if constexpr (V == 6) {
             /* the ode implementation: deterministic, no random numbers */
@@ -392,8 +412,8 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
         } else if constexpr (sizeof(R) == 8) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j, s_bm);
-            if constexpr (MIF) M::template interval_pp<V>((double *)X, f->pv, lp, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs, f->drift_on ? &f->dv : nullptr);
-            else M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs, f->drift_on ? &f->dv : nullptr);
+            if constexpr (MIF) M::template interval_pp<V>((double *)X, f->pv, lp, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs, f->drift_on ? &f->dv : nullptr, (const double *)tft);
+            else M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs, f->drift_on ? &f->dv : nullptr, (const double *)tft);
         } else {
             plom_nstream32 ns;
             plom_nstream32_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j);
@@ -402,10 +422,10 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
 #pragma unroll
                 for (int k = 0; k < M::NPP; k++) lpf[k] = (float)lp[k];
                 M::template interval_pp<V>((float *)X, f->pvf, lpf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs,
-                                           f->drift_on ? &f->dv : nullptr);
+                                           f->drift_on ? &f->dv : nullptr, (const float *)tft);
             } else {
                 M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs,
-                                        f->drift_on ? &f->dv : nullptr);
+                                        f->drift_on ? &f->dv : nullptr, (const float *)tft);
             }
         }
         R *Xout = (R *)f->X[(n + 1) & 1];
@@ -735,12 +755,12 @@ __global__ __launch_bounds__(64) void k_lineage(plom_dfilter *f)
             plom_nstream64 ns;
             plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, jn, s_bm);
             M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs,
-                                    f->drift_on ? &f->dv : nullptr);
+                                    f->drift_on ? &f->dv : nullptr, (const double *)nullptr);
         } else {
             plom_nstream32 ns;
             plom_nstream32_init(&ns, f->seed, f->stream, (uint32_t)n, jn);
             M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs,
-                                    f->drift_on ? &f->dv : nullptr);
+                                    f->drift_on ? &f->dv : nullptr, (const float *)nullptr);
         }
 #pragma unroll
         for (int k = 0; k < M::S; k++) f->lin_state[(size_t)(n + 1) * M::S + k] = (double)X[k];

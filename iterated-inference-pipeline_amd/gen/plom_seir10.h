@@ -19,10 +19,15 @@
 #define PLOM_SEIR10_N_DRIFT 0
 #define PLOM_SEIR10_DRIFT_OFFSET 30
 
-PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+PLOM_HD void plom_seir10_time_funcs_f64(double tr, double *out)
+{
+    (void)tr; (void)out;
+}
+
+PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -442,10 +447,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
     }
 }
 
-PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -785,10 +790,10 @@ PLOM_HD void plom_seir10_interval_no_dem_sto_f64(double *X, const double *pv, pl
     }
 }
 
-PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -1178,10 +1183,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
     }
 }
 
-PLOM_HD void plom_seir10_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+PLOM_HD void plom_seir10_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -1708,9 +1713,9 @@ PLOM_HD void plom_seir10_hat_values(const double *X, const double *pvd, const do
     plom_seir10_obs_moments(obs, pvd, cvd, t, out + 40, var);
 }
 
-PLOM_HD void plom_seir10_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+PLOM_HD void plom_seir10_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -2531,9 +2536,9 @@ PLOM_HD void plom_seir10_interval_psr_full_f64(double *X, const double *pv, plom
     }
 }
 
-PLOM_HD void plom_seir10_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+PLOM_HD void plom_seir10_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -3628,22 +3633,27 @@ PLOM_HD void plom_seir10_interval_f64(double *X, const double *pv, plom_nstream6
         return;
     }
     if (noises_off & PLOM_IMPL_PSR) {
-        if (noises_off & PLOM_NO_ENV_STO) plom_seir10_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else plom_seir10_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        if (noises_off & PLOM_NO_ENV_STO) plom_seir10_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);
+        else plom_seir10_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);
         return;
     }
     const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);
-    if (v == 0u) plom_seir10_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-    else if (v == PLOM_NO_DEM_STO) plom_seir10_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-    else if (v == PLOM_NO_ENV_STO) plom_seir10_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-    else plom_seir10_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+    if (v == 0u) plom_seir10_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);
+    else if (v == PLOM_NO_DEM_STO) plom_seir10_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);
+    else if (v == PLOM_NO_ENV_STO) plom_seir10_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);
+    else plom_seir10_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);
 }
 
 #if defined(__HIPCC__)
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_time_funcs_f32(float tr, float *out)
+{
+    (void)tr; (void)out;
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -4063,10 +4073,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -4406,10 +4416,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -4799,10 +4809,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -5127,9 +5137,9 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_proj2ob
     obs[9] = (double)X[39];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -5950,9 +5960,9 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -7040,10 +7050,10 @@ static __device__ inline __attribute__((always_inline)) int plom_seir10_interval
     return plom_seir10_ode_integrate_pp(X, pv, lp, cv, sp, t0, t1, dt, eps_abs, eps_rel);
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -7463,10 +7473,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -7806,10 +7816,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -8199,10 +8209,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -8527,10 +8537,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_proj2ob
     obs[9] = (double)X[39];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -8950,10 +8960,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -9293,10 +9303,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -9686,10 +9696,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_seir10_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -10115,40 +10125,43 @@ static __device__ inline __attribute__((always_inline)) double plom_seir10_logli
 
 struct plom_model_seir10 {
     static constexpr int S = 40, N_TS = 10, INC_OFFSET = 30, N_INC = 10, PV = 180, N_CAC = 10, N_HAT = 50, NPP = 70, FIX_OFF = 70, N_FIX = 10, N_DRIFT = 0;
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+    static constexpr int NTF = 0;  /* time-only functions of a substep */
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
     {
-        if constexpr (V == 0) plom_seir10_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 4) plom_seir10_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 5) plom_seir10_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        if constexpr (V == 0) plom_seir10_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 4) plom_seir10_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 5) plom_seir10_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
     {
-        if constexpr (V == 0) plom_seir10_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 4) plom_seir10_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 5) plom_seir10_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        if constexpr (V == 0) plom_seir10_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 4) plom_seir10_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 5) plom_seir10_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
+    static __device__ inline __attribute__((always_inline)) void time_funcs(double tr, double *out) { plom_seir10_time_funcs_f64(tr, out); }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, const double *cvd, double *obs) { plom_seir10_proj2obs_f64(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_seir10_proj2obs_pp_f64(X, pvd, lpd, cvd, obs); }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
     {
-        if constexpr (V == 0) plom_seir10_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        if constexpr (V == 0) plom_seir10_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
     {
-        if constexpr (V == 0) plom_seir10_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        if constexpr (V == 0) plom_seir10_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 1) plom_seir10_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 2) plom_seir10_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 3) plom_seir10_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
+    static __device__ inline __attribute__((always_inline)) void time_funcs(float tr, float *out) { plom_seir10_time_funcs_f32(tr, out); }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_seir10_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_seir10_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double ea, double er) { return plom_seir10_interval_ode_f64(X, pv, cv, sp, t0, t1, dt, ea, er); }

@@ -19,10 +19,16 @@
 #define PLOM_SIRS4_N_DRIFT 0
 #define PLOM_SIRS4_DRIFT_OFFSET 8
 
-PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+PLOM_HD void plom_sirs4_time_funcs_f64(double tr, double *out)
+{
+    (void)tr; (void)out;
+    out[0] = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+}
+
+PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -72,7 +78,7 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
@@ -153,10 +159,10 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
     }
 }
 
-PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -206,7 +212,7 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
@@ -263,10 +269,10 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
     }
 }
 
-PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -312,7 +318,7 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
@@ -385,10 +391,10 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
     }
 }
 
-PLOM_HD void plom_sirs4_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+PLOM_HD void plom_sirs4_interval_no_dem_sto_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -434,7 +440,7 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_no_env_sto_f64(double *X, const doub
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
@@ -582,9 +588,9 @@ PLOM_HD void plom_sirs4_hat_values(const double *X, const double *pvd, const dou
     plom_sirs4_obs_moments(obs, pvd, cvd, t, out + 12, var);
 }
 
-PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -634,7 +640,7 @@ PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -820,9 +826,9 @@ PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_
     }
 }
 
-PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const double P0 = PLOM_UNIFORM64(pv[0]);
     const double P1 = PLOM_UNIFORM64(pv[1]);
     const double P2 = PLOM_UNIFORM64(pv[2]);
@@ -868,7 +874,7 @@ PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv,
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -1151,22 +1157,28 @@ PLOM_HD void plom_sirs4_interval_f64(double *X, const double *pv, plom_nstream64
         return;
     }
     if (noises_off & PLOM_IMPL_PSR) {
-        if (noises_off & PLOM_NO_ENV_STO) plom_sirs4_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else plom_sirs4_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        if (noises_off & PLOM_NO_ENV_STO) plom_sirs4_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);
+        else plom_sirs4_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);
         return;
     }
     const unsigned int v = noises_off & (PLOM_NO_DEM_STO | PLOM_NO_ENV_STO);
-    if (v == 0u) plom_sirs4_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-    else if (v == PLOM_NO_DEM_STO) plom_sirs4_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-    else if (v == PLOM_NO_ENV_STO) plom_sirs4_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-    else plom_sirs4_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+    if (v == 0u) plom_sirs4_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);
+    else if (v == PLOM_NO_DEM_STO) plom_sirs4_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);
+    else if (v == PLOM_NO_ENV_STO) plom_sirs4_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);
+    else plom_sirs4_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, 0);
 }
 
 #if defined(__HIPCC__)
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_time_funcs_f32(float tr, float *out)
+{
+    (void)tr; (void)out;
+    out[0] = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+}
+
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -1216,7 +1228,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -1297,10 +1309,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -1350,7 +1362,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -1407,10 +1419,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -1456,7 +1468,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -1529,10 +1541,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft;
     const float P0 = PLOM_UNIFORM32(pv[0]);
     const float P1 = PLOM_UNIFORM32(pv[1]);
     const float P2 = PLOM_UNIFORM32(pv[2]);
@@ -1578,7 +1590,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -1636,9 +1648,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_proj2obs
     obs[3] = (double)X[11];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_psr_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -1688,7 +1700,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -1874,9 +1886,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_psr_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -1922,7 +1934,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -2198,10 +2210,10 @@ static __device__ inline __attribute__((always_inline)) int plom_sirs4_interval_
     return plom_sirs4_ode_integrate_pp(X, pv, lp, cv, sp, t0, t1, dt, eps_abs, eps_rel);
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -2251,7 +2263,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
@@ -2332,10 +2344,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -2385,7 +2397,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
@@ -2442,10 +2454,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -2491,7 +2503,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
@@ -2564,10 +2576,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f64(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
 {
     const double dtr = (double)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const double P0 = lp[0];
     const double P1 = lp[1];
     const double P2 = lp[2];
@@ -2613,7 +2625,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
@@ -2671,10 +2683,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_proj2obs
     obs[3] = (double)X[11];
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -2724,7 +2736,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -2805,10 +2817,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -2858,7 +2870,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -2915,10 +2927,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -2964,7 +2976,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -3037,10 +3049,10 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
     }
 }
 
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f32(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
 {
     const float dtr = (float)dt;
-    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)lp;
+    (void)sqdt; (void)ns; (void)pv; (void)cv; (void)cs; (void)dm; (void)tft; (void)lp;
     const float P0 = lp[0];
     const float P1 = lp[1];
     const float P2 = lp[2];
@@ -3086,7 +3098,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -3191,40 +3203,43 @@ static __device__ inline __attribute__((always_inline)) double plom_sirs4_loglik
 
 struct plom_model_sirs4 {
     static constexpr int S = 12, N_TS = 4, INC_OFFSET = 8, N_INC = 4, PV = 48, N_CAC = 4, N_HAT = 16, NPP = 28, FIX_OFF = 28, N_FIX = 4, N_DRIFT = 0;
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+    static constexpr int NTF = 1;  /* time-only functions of a substep */
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
     {
-        if constexpr (V == 0) plom_sirs4_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 4) plom_sirs4_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 5) plom_sirs4_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        if constexpr (V == 0) plom_sirs4_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 4) plom_sirs4_interval_psr_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 5) plom_sirs4_interval_psr_no_env_sto_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(double *X, const double *pv, const double *lp, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
     {
-        if constexpr (V == 0) plom_sirs4_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 4) plom_sirs4_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 5) plom_sirs4_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        if constexpr (V == 0) plom_sirs4_interval_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 4) plom_sirs4_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 5) plom_sirs4_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
+    static __device__ inline __attribute__((always_inline)) void time_funcs(double tr, double *out) { plom_sirs4_time_funcs_f64(tr, out); }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, const double *cvd, double *obs) { plom_sirs4_proj2obs_f64(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sirs4_proj2obs_pp_f64(X, pvd, lpd, cvd, obs); }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
     {
-        if constexpr (V == 0) plom_sirs4_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        if constexpr (V == 0) plom_sirs4_interval_full_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_f32(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
-    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm)
+    template <int V> static __device__ inline __attribute__((always_inline)) void interval_pp(float *X, const float *pv, const float *lp, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
     {
-        if constexpr (V == 0) plom_sirs4_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
-        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm);
+        if constexpr (V == 0) plom_sirs4_interval_full_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 1) plom_sirs4_interval_no_dem_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
+        else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
+    static __device__ inline __attribute__((always_inline)) void time_funcs(float tr, float *out) { plom_sirs4_time_funcs_f32(tr, out); }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_sirs4_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sirs4_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double ea, double er) { return plom_sirs4_interval_ode_f64(X, pv, cv, sp, t0, t1, dt, ea, er); }

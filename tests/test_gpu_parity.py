@@ -123,3 +123,21 @@ def test_streams_and_seeds(ctx):
     a, _ = compare(ctx, "sir", 700, 10, seed=5, stream=9)
     b, _ = compare(ctx, "sir", 700, 10, seed=5, stream=10)
     assert a.loglik != b.loglik
+
+
+@pytest.mark.parametrize("flags", [0, smc.FLAG_PSR])
+def test_long_intervals_evaluate_the_forcing_inline(ctx, flags):
+    """Intervals of more than 512 substeps do not fit the block's time-function table: the
+    step evaluates the seasonal forcing itself.  Long and weekly intervals mixed, FP64 equal
+    to the oracle (the table and the inline evaluation give the same values)."""
+    m = Model.get("sirs4")
+    _, data = m.dataset(8)
+    times = np.array([0.0, 7.0, 14.0, 150.0, 157.0, 300.0, 307.0, 314.0, 321.0])
+    pv, x0 = m.pack(configs.TRUTH["sirs4"]), m.x0(configs.X0_PROPS["sirs4"])
+    if flags & smc.FLAG_PSR:
+        x0 = np.round(x0)
+    print("long intervals", flags, flush=True)
+    g = smc.run(ctx, m, 1000, times, data, pv, x0, flags=flags, seed=3, stream=1)
+    o = oracle.smc(m.id, 1000, times, data, pv, x0, seed=3, stream=1, nthreads=8, psr=bool(flags & smc.FLAG_PSR))
+    np.testing.assert_array_equal(g.loglik_n, o["loglik_n"])
+    assert g.loglik == o["loglik"]

@@ -106,24 +106,27 @@ def test_drift_refusals(ctx):
         srv.close()
 
 
-@pytest.mark.parametrize("J,N,noises_off", [(1500, 40, 0), (300, 20, 3)])
-def test_mif_pass_with_drift_equals_oracle(ctx, J, N, noises_off):
+@pytest.mark.parametrize("J,N,noises_off,flags", [(1500, 40, 0, 0), (300, 20, 3, 0), (500, 20, 0, smc.FLAG_PSR)])
+def test_mif_pass_with_drift_equals_oracle(ctx, J, N, noises_off, flags):
     """Iterated filtering with drifting parameters: the drift states start at the particles'
     theta and move with each particle's own volatility (a walked component)."""
     from test_drift import DRIFT_MAP, mif_drift_setup
     from test_gpu_mif import check_equal
     m, times, data, pv, drift, theta0, X0 = mif_drift_setup(N, J, 3, spread=0.05)
+    if flags & smc.FLAG_PSR:
+        off = m.desc.drift_offset
+        X0[:off] = np.round(X0[:off])
     mut = np.linspace(0.5, 1.0, N)
     cov = m.covariates()
-    print("mif drift", J, N, noises_off, flush=True)
-    f = smc.Filter(ctx, m, J, times, data, pv, X0[:, 0], noises_off=noises_off, flags=smc.FLAG_MIF, mif=DRIFT_MAP,
-                   cov=cov, drift=drift)
+    print("mif drift", J, N, noises_off, flags, flush=True)
+    f = smc.Filter(ctx, m, J, times, data, pv, X0[:, 0], noises_off=noises_off, flags=smc.FLAG_MIF | flags,
+                   mif=DRIFT_MAP, cov=cov, drift=drift)
     f.mif_start(theta0, X0, mut, seed=6, stream=2)
     f.run()
     g = f.read(mif=True)
     f.close()
     o = oracle.mif_pass(m.id, J, times, data, pv, DRIFT_MAP, theta0, X0, mut, noises_off=noises_off, seed=6, stream=2,
-                        nthreads=8, drift=drift, cov=cov)
+                        nthreads=8, drift=drift, cov=cov, psr=bool(flags & smc.FLAG_PSR))
     check_equal(g, o)
 
 

@@ -2731,6 +2731,7 @@ static int srv_poll(plom_server *S)
         hipError_t q = hipEventQuery(r->done_ev);
         if (q == hipSuccess) {
             r->state = SRV_DONE;
+            S->free_slots.push_back(r->slot);
             S->done.push_back(r);
             S->finished.erase(S->finished.begin() + (long)i);
         } else if (q == hipErrorNotReady) {
@@ -2767,7 +2768,10 @@ static int srv_lane_step(plom_server *S, srv_lane *L)
             HIPCHK(hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming));
             HIPCHK(hipEventRecord(r->done_ev, L->stream));
             r->f->has_run = 1;
-            S->free_slots.push_back(r->slot);
+            /* the slot goes back to the free list only when done_ev has passed (srv_poll):
+             * the lane's stream still has this request's last steps, lineage and copy-back
+             * queued, and they read the slot; a request admitted into it meanwhile on the
+             * other lane's stream would overwrite the slot under them */
             S->finished.push_back(r);
             L->active.erase(L->active.begin() + (long)i);
             L->dirty = 1;
@@ -2816,7 +2820,7 @@ extern "C" int plom_server_step(plom_server *S, int32_t *n_active, int32_t *n_qu
         }
     }
     /* admission by free HBM, FIFO: the queue head blocks the queue */
-    while (!S->queue.empty() && srv_n_active(S) < S->max_active) {
+    while (!S->queue.empty() && srv_n_active(S) < S->max_active && !S->free_slots.empty()) {
         srv_request *r = S->queue.front();
         if (S->in_use + r->bytes > S->budget || !ready[r->lane]) break;
         srv_lane *L = &S->lanes[r->lane];

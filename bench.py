@@ -41,12 +41,12 @@ import time
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, os.path.join(HERE, "tests"))
 sys.path.insert(0, HERE)
 
 from pkgload import load_package  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 METRIC = "particle-steps/s & p50/p99 loglik latency, batched+served vs plain SMC"
 LEGS = ("batched", "plain", "served", "single", "psr", "mif", "memory", "cpu")
 
@@ -61,16 +61,29 @@ def bytes_per_particle_step(S, real_bytes):
     return 2 * S * real_bytes + 20
 
 
-def read_traffic(name):
-    """HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by
-    the copy-kernel calibration, as committed in profiles/<name> ('traffic_bytes_per_launch: N')."""
+def read_profile_value(name, key):
+    """A figure measured by rocprofv3 and committed in profiles/<name> as a line 'key: value'
+    (traffic_bytes_per_launch: FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel
+    calibration; valu_issue_frac: SQ_INSTS_VALU x 4 cycles over the SIMD cycles of the
+    dispatch; fp64_flops_per_particle_step: from the SQ_INSTS_VALU_*_F64 counters)."""
     path = os.path.join(HERE, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":")[1])
     return None
+
+
+def cpu_model():
+    """The host CPU's model name (what lscpu reports), from /proc/cpuinfo."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 class Inputs:
@@ -123,16 +136,18 @@ def batched_leg(smc, ctx, filters, steps, warmup, lanes=0):
     ctx.sync()
     wall = (time.perf_counter() - t0) / steps
     ll = np.array([f.read().loglik for f in filters])
-    # per-launch timing pass (events around every launch), outside the timed region, with
-    # one lane so that no two launches overlap
-    b.configure(1)
+    # per-launch timing pass, outside the timed region, in the timed configuration (the
+    # same lanes): every lane brackets its launches with events on its own stream, so a
+    # launch's duration is the one it has beside the other lanes' launches
     ctx.set_kernel_timing(True)
     b.launch(use_graph=False)
     total_ms, prop_ms, scan_ms, nint, nprop = ctx.last_timing()
+    lane_prop, lane_scan = ctx.last_timing_lanes()
     ctx.set_kernel_timing(False)
     b.close()
     return {"wall_s": wall, "loglik": ll, "prop_ms": prop_ms, "scan_ms": scan_ms, "total_ms": total_ms,
-            "launches": nprop, "rounds": nint, "lanes": lanes, "autotune_ms": tuned}
+            "launches": nprop, "rounds": nint, "lanes": lanes, "autotune_ms": tuned,
+            "lane_prop_ms": lane_prop, "lane_scan_ms": lane_scan}
 
 
 def plain_leg(filters):
@@ -272,8 +287,8 @@ def psr_leg(smc, ctx, inp, configs, steps, warmup, J, N, cpu, cpu_J, cpu_N):
         from oracle import oracle
         mc, tc, dc, pvc, x0c = inp.get("sirs4", cpu_N, configs.TRUTH["sirs4"])
         r = oracle.smc(mc.id, cpu_J, tc, dc, pvc, np.round(x0c), seed=configs.FILTER_SEED, stream=0, psr=True,
-                       nthreads=16)
-        out["cpu_oracle"] = {"value": round(cpu_J * cpu_N / r["elapsed_s"], 1), "cores": 16,
+                       nthreads=min(16, os.cpu_count() or 1))
+        out["cpu_oracle"] = {"value": round(cpu_J * cpu_N / r["elapsed_s"], 1), "cores": min(16, os.cpu_count() or 1),
                              "sample": "SIRS psr, J=%d, N=%d" % (cpu_J, cpu_N)}
     return out
 
@@ -312,12 +327,14 @@ def mif_leg(smc, ctx, inp, configs, steps, warmup, J, N):
             "n_theta": nth}
 
 
-def cpu_baseline(smc, inp, configs, draws, threads):
+def cpu_baseline(smc, inp, configs, draws, threads, reference=False):
     from oracle import oracle
     secs, ps = 0.0, 0
+    mode = oracle.REFERENCE if reference else oracle.CANONICAL
     for d in draws:
         m, times, data, pv, x0 = inp.get(d["model"], d["N"], d["params"])
-        r = oracle.smc(m.id, d["J"], times, data, pv, x0, seed=configs.FILTER_SEED, stream=d["stream"], nthreads=threads)
+        r = oracle.smc(m.id, d["J"], times, data, pv, x0, seed=configs.FILTER_SEED, stream=d["stream"], nthreads=threads,
+                       mode=mode)
         secs += r["elapsed_s"]
         ps += d["J"] * d["N"]
     return ps / secs, secs, ps
@@ -401,17 +418,31 @@ def main():
                              "approximation of the FP64 one; the tests check its mean over replicates "
                              "against FP64 within Monte Carlo error"}}
         B = bytes_per_particle_step(S, 8)
-        per_launch = B * ps_batch / max(fp64["launches"], 1)
-        avg_ms = fp64["prop_ms"] / max(fp64["launches"], 1)
-        achieved = B * ps_batch / (fp64["prop_ms"] * 1e-3) / 1e9
-        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBPS, 5),
-                           "traffic": read_traffic("k_propagate_sirs4_fp64_pmcmc_batch.md"),
-                           "kernel": "k_propagate<sirs4,double,full> (batched)",
-                           "algorithmic_bytes_per_particle_step": B,
-                           "algorithmic_bytes_per_launch": round(per_launch, 1),
-                           "avg_launch_ms": round(avg_ms, 5), "launches": fp64["launches"],
-                           "measured_copy_GBps": round(copy_gbps, 1)}
+        nl = max(fp64["launches"], 1)
+        per_launch = B * ps_batch / nl
+        avg_ms = fp64["prop_ms"] / nl
+        achieved = per_launch / (avg_ms * 1e-3) / 1e9
+        prof = "k_propagate_sirs4_fp64_pmcmc_batch.md"
+        valu = read_profile_value(prof, "valu_issue_frac")
+        fpps = read_profile_value(prof, "fp64_flops_per_particle_step")
+        rl = {"bound": "valu", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+              "frac": round(achieved / HBM_PEAK_GBPS, 5),
+              "traffic": read_profile_value(prof, "traffic_bytes_per_launch"),
+              "kernel": "k_propagate<sirs4,double,full> (batched, %d lanes)" % fp64["lanes"],
+              "binding": "VALU issue: the kernel's arithmetic per byte is about 100x the HBM ridge, so the "
+                         "frac above is its HBM share and valu_issue_frac its binding limit",
+              "valu_issue_frac": valu,
+              "algorithmic_bytes_per_particle_step": B,
+              "algorithmic_bytes_per_launch": round(per_launch, 1),
+              "avg_launch_ms": round(avg_ms, 5), "launches": fp64["launches"],
+              "propagate_ms_per_pass_by_lane": [round(x, 3) for x in fp64["lane_prop_ms"]],
+              "pass_wall_GBps": round(B * ps_batch / fp64["wall_s"] / 1e9, 2),
+              "measured_copy_GBps": round(copy_gbps, 1)}
+        if fpps:
+            tf = fpps * ps_batch / nl / (avg_ms * 1e-3) / 1e12
+            rl["fp64"] = {"flops_per_particle_step": fpps, "achieved_TFLOPs": round(tf, 3),
+                          "peak_TFLOPs": FP64_PEAK_TFLOPS, "frac": round(tf / FP64_PEAK_TFLOPS, 4)}
+        out["roofline"] = rl
     if "plain" in legs:
         log("plain FP64 (one filter at a time)")
         pl = plain_leg(f64)
@@ -463,14 +494,24 @@ def main():
                           **mif_leg(smc, ctx, inp, configs, args.steps, args.warmup, args.single_J, args.single_N))
     ctx.close()
     if "cpu" in legs:
-        threads = 16
+        host = os.cpu_count() or 1
+        threads = min(16, host)        # the CPU share of one GPU on the measuring host
         sample = draws[:args.cpu_filters]
         log("CPU oracle on %d filters, %d threads" % (len(sample), threads))
         rate, secs, ps = cpu_baseline(smc, inp, configs, sample, threads)
+        ref_sample = sample[:2]
+        log("CPU oracle, the reference's sequential arithmetic, on %d filters" % len(ref_sample))
+        rate_ref, secs_ref, ps_ref = cpu_baseline(smc, inp, configs, ref_sample, threads, reference=True)
         out["cpu_baseline"] = {"value": round(rate, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
+                               "host_cpus": host, "cpu_model": cpu_model(),
                                "sample": "oracle/smc_oracle.c (canonical FP64) on the first %d filters of the "
                                          "pmcmc_batch workload (%d particle-steps), %.1f s on %d threads"
-                                         % (len(sample), ps, secs, threads)}
+                                         % (len(sample), ps, secs, threads),
+                               "reference_arithmetic": {
+                                   "value": round(rate_ref, 1), "cores": threads,
+                                   "sample": "the same oracle in the reference's sequential weight / resampling "
+                                             "arithmetic (smc.c:27-112 as written) on the first %d filters "
+                                             "(%d particle-steps), %.1f s" % (len(ref_sample), ps_ref, secs_ref)}}
     print(json.dumps(out))
     return 0
 

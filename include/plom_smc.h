@@ -342,16 +342,22 @@ int plom_server_destroy(plom_server *srv);
 
 int plom_sync(plom_ctx *ctx);
 
-/* Timing of the last plom_filter_run / plom_batch_run from HIP events on the context
- * stream.  total_ms always; with kernel timing on (plom_ctx_set_kernel_timing), events
- * also bracket every launch and propagate_ms / scan_ms are the summed durations of the
- * propagate and scan launches, n_intervals the number of launch rounds and
- * n_propagate_launches the number of propagate kernels they held (a batch launches one
- * per (model, precision, variant) group and lane in a round; with timing on, lanes run
- * one after another, so the launches do not overlap). */
+/* Timing of the last plom_filter_run / plom_batch_launch (eager) from HIP events.
+ * total_ms always (context stream); with kernel timing on (plom_ctx_set_kernel_timing),
+ * events also bracket the launches of every round: propagate_ms / scan_ms are the summed
+ * durations of the propagate launches and of the weight / scan / advance launches,
+ * n_intervals the number of launch rounds and n_propagate_launches the number of
+ * propagate kernels they held (a batch launches one per (model, precision, variant)
+ * group and lane in a round).  A batch on several lanes records each lane's events on
+ * that lane's stream, so the durations are those of the launches as they run beside the
+ * other lanes' launches; plom_last_timing_lanes gives the sums per lane.  No host
+ * counterpart in the reference (its timing is the pmcmc/Tests.py wall clock). */
 int plom_ctx_set_kernel_timing(plom_ctx *ctx, int on);
 int plom_last_timing(plom_ctx *ctx, double *total_ms, double *propagate_ms, double *scan_ms, int32_t *n_intervals,
                      int32_t *n_propagate_launches);
+/* per lane l < cap of the last timed run: summed propagate and weight / scan durations;
+ * n_lanes = lanes that recorded events */
+int plom_last_timing_lanes(plom_ctx *ctx, double *propagate_ms, double *scan_ms, int32_t cap, int32_t *n_lanes);
 
 /* device self-test of the shared arithmetic: out[i*7+..] = exp, log, Phi, sin, cos, sqrt,
  * sqrt_mid (the Box-Muller radius square root) of in[i] */

@@ -1202,24 +1202,43 @@ struct plom_ctx {
     hipEvent_t ev[4];
     double last_total_ms, last_prop_ms, last_scan_ms;
     int ktiming;                    /* bracket every launch with events */
-    std::vector<hipEvent_t> kev;    /* pool: 2 per interval + 1 */
-    int kev_used;                   /* events recorded by the last run */
+    std::vector<hipEvent_t> kev;    /* pool: 3 per launch round and lane */
+    struct kspan { int kind, lane, a, b; };   /* kind 0: propagate launches, 1: weights / scan */
+    std::vector<kspan> kspans;      /* event pairs of the last run (timing on) */
+    int kspan_rounds;               /* launch rounds they cover */
     int kprop_launches;             /* propagate kernel launches of the last run */
     std::vector<hipStream_t> lane;  /* batch lane streams, created on first use ([0] unused: lane 0 is `stream`) */
     std::vector<hipEvent_t> join_ev;
     hipEvent_t fork_ev;
 };
 
-/* event k of the per-launch timing pool (grown on demand) */
-static int kev_record(plom_ctx *c, int k)
+/* event k of the per-launch timing pool (grown on demand), recorded on stream s */
+static int kev_record(plom_ctx *c, int k, hipStream_t s)
 {
     while ((int)c->kev.size() <= k) {
         hipEvent_t e;
         HIPCHK(hipEventCreate(&e));
         c->kev.push_back(e);
     }
-    HIPCHK(hipEventRecord(c->kev[(size_t)k], c->stream));
+    HIPCHK(hipEventRecord(c->kev[(size_t)k], s));
     return PLOM_OK;
+}
+
+/* timing on: an event before the launches of a span; returns its pool index (or -1) */
+static int kspan_open(plom_ctx *c, int *k, hipStream_t s)
+{
+    if (!c->ktiming) return -1;
+    if (kev_record(c, *k, s) != PLOM_OK) return -2;
+    return (*k)++;
+}
+
+/* timing on: an event after the launches of the span opened at `a` */
+static int kspan_close(plom_ctx *c, int *k, hipStream_t s, int a, int kind, int lane)
+{
+    if (!c->ktiming) return -1;
+    if (a < 0 || kev_record(c, *k, s) != PLOM_OK) return -2;
+    c->kspans.push_back(plom_ctx::kspan{kind, lane, a, *k});
+    return (*k)++;
 }
 
 struct plom_filter {
@@ -1411,7 +1430,7 @@ extern "C" int plom_ctx_create(int device, plom_ctx **out)
     if (e != hipSuccess) { (void)hipStreamDestroy(c->stream); free(c); return set_err(PLOM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
     for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&c->ev[i]));
     c->ktiming = 0;
-    c->kev_used = 0;
+    c->kspan_rounds = 0;
     *out = c;
     return PLOM_OK;
 }
@@ -1914,17 +1933,18 @@ extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     int k = 0;
     c->kprop_launches = 0;
+    c->kspans.clear();
+    c->kspan_rounds = c->ktiming ? f->desc.N : 0;
     for (int n = 0; n < f->desc.N; n++) {
-        if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
+        const int a = kspan_open(c, &k, c->stream);
         me->launch_prop(f->desc.precision, f->variant, mif, T, c->stream, f->df, nullptr);
         c->kprop_launches++;
-        if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
+        const int b = kspan_close(c, &k, c->stream, a, 0, 0);
         enqueue_resample(c->stream, f->df, nullptr, T, nullptr, 1, mif ? f->desc.n_theta : 0);
+        if (kspan_close(c, &k, c->stream, b, 1, 0) == -2) return PLOM_ERR_HIP;
         if (f->desc.flags & PLOM_FLAG_HAT) me->launch_hat(f->desc.precision, c->stream, f->df, T, f->info.n_hat);
     }
     if ((f->desc.flags & PLOM_FLAG_HAT) && f->desc.N > 0) me->launch_hat0(f->desc.precision, c->stream, f->df);
-    if (c->ktiming && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
-    c->kev_used = k;
     if (needs_final(f)) enqueue_final(c, f, nullptr, f->df, T);
     if (f->desc.N > 0 && (f->desc.flags & PLOM_FLAG_LINEAGE))
         me->launch_lineage(f->desc.precision, f->variant, c->stream, f->df);
@@ -2213,20 +2233,23 @@ static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing, int rounds)
     const int R = rounds >= 0 && rounds < maxN ? rounds : maxN;
     const int stride = R < maxN ? maxN / R : 1;
     c->kprop_launches = 0;
-    if (timing || b->lanes == 1) {
+    c->kspans.clear();
+    c->kspan_rounds = timing ? R : 0;
+    const int ktiming = c->ktiming;
+    c->ktiming = timing;
+    int k = 0;
+    if (b->lanes == 1) {
         for (auto &ch : b->chains) chain_reset(&ch, c->stream);
-        int k = 0;
         for (int i = 0; i < R; i++) {
             const int s = i * stride;
-            if (timing && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
+            const int a = kspan_open(c, &k, c->stream);
             for (auto &ch : b->chains)
                 if (s < ch.maxN) c->kprop_launches += chain_propagate(&ch, s, c->stream);
-            if (timing && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
+            const int m = kspan_close(c, &k, c->stream, a, 0, 0);
             for (auto &ch : b->chains)
                 if (s < ch.maxN) chain_resample(&ch, s, c->stream);
+            if (kspan_close(c, &k, c->stream, m, 1, 0) == -2) { c->ktiming = ktiming; return PLOM_ERR_HIP; }
         }
-        if (timing && kev_record(c, k++) != PLOM_OK) return PLOM_ERR_HIP;
-        c->kev_used = timing ? k : 0;
         if (R == maxN)
             for (auto &ch : b->chains) chain_final(c, &ch, c->stream);
     } else {
@@ -2238,9 +2261,14 @@ static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing, int rounds)
             batch_chain *ch = &b->chains[(size_t)l];
             hipStream_t st = l == 0 ? c->stream : c->lane[(size_t)l];
             chain_reset(ch, st);
+            /* with timing on, every chain brackets its own launches with events on its own
+             * stream: the durations are those of the concurrent launches */
             for (int i = 0; i < R && i * stride < ch->maxN; i++) {
-                chain_propagate(ch, i * stride, st);
+                const int a = kspan_open(c, &k, st);
+                c->kprop_launches += chain_propagate(ch, i * stride, st);
+                const int m = kspan_close(c, &k, st, a, 0, l);
                 chain_resample(ch, i * stride, st);
+                if (kspan_close(c, &k, st, m, 1, l) == -2) { c->ktiming = ktiming; return PLOM_ERR_HIP; }
             }
             if (R == maxN) chain_final(c, ch, st);
         }
@@ -2248,8 +2276,8 @@ static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing, int rounds)
             HIPCHK(hipEventRecord(c->join_ev[(size_t)l], c->lane[(size_t)l]));
             HIPCHK(hipStreamWaitEvent(c->stream, c->join_ev[(size_t)l], 0));
         }
-        c->kev_used = 0;
     }
+    c->ktiming = ktiming;
     HIPCHK(hipGetLastError());
     return PLOM_OK;
 }
@@ -2278,7 +2306,8 @@ static int batch_launch(plom_ctx *c, plom_batch *b, int use_graph)
             HIPCHK(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
             HIPCHK(hipGraphDestroy(g));
         }
-        c->kev_used = 0;
+        c->kspans.clear();
+        c->kspan_rounds = 0;
         HIPCHK(hipGraphLaunch(b->graph, c->stream));
     } else {
         int rc = batch_enqueue(c, b, c->ktiming, -1);
@@ -2902,6 +2931,15 @@ extern "C" int plom_server_result(plom_server *S, uint64_t id, plom_filter_outpu
     if (!r->done_ev) return set_err(PLOM_ERR_STATE, "request %llu has not finished", (unsigned long long)id);
     HIPCHK(hipSetDevice(S->c->device));
     HIPCHK(hipEventSynchronize(r->done_ev));
+    if (where == &S->finished) {
+        /* taken before a poll moved it to `done`: its completion event has now passed, so
+         * its slot goes back to the free list here (srv_poll will never see it again) */
+        r->state = SRV_DONE;
+        S->free_slots.push_back(r->slot);
+        where->erase(std::find(where->begin(), where->end(), r));
+        S->done.push_back(r);
+        where = &S->done;
+    }
     /* the request's work is complete: read on the side stream, past the queued steps */
     int rc = filter_read_on(S->c, r->f, o, S->c->io);
     if (rc != PLOM_OK) return rc;
@@ -2953,22 +2991,34 @@ extern "C" int plom_last_timing(plom_ctx *c, double *total_ms, double *prop_ms, 
     HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     c->last_total_ms = ms;
     double p = 0.0, q = 0.0;
-    int nint = 0;
-    /* events: e[2n] before the propagate launches of interval n, e[2n+1] before its scan,
-     * e[2n+2] after it */
-    for (int n = 0; 2 * n + 2 < c->kev_used; n++) {
-        float a = 0.f, b = 0.f;
-        HIPCHK(hipEventElapsedTime(&a, c->kev[(size_t)(2 * n)], c->kev[(size_t)(2 * n + 1)]));
-        HIPCHK(hipEventElapsedTime(&b, c->kev[(size_t)(2 * n + 1)], c->kev[(size_t)(2 * n + 2)]));
-        p += a;
-        q += b;
-        nint++;
+    for (const auto &sp : c->kspans) {
+        float a = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, c->kev[(size_t)sp.a], c->kev[(size_t)sp.b]));
+        (sp.kind == 0 ? p : q) += a;
     }
+    const int timed = !c->kspans.empty();
     if (total_ms) *total_ms = ms;
-    if (prop_ms) *prop_ms = c->kev_used ? p : -1.0;
-    if (scan_ms) *scan_ms = c->kev_used ? q : -1.0;
-    if (n_intervals) *n_intervals = nint;
-    if (n_prop_launches) *n_prop_launches = c->kev_used ? c->kprop_launches : 0;
+    if (prop_ms) *prop_ms = timed ? p : -1.0;
+    if (scan_ms) *scan_ms = timed ? q : -1.0;
+    if (n_intervals) *n_intervals = timed ? c->kspan_rounds : 0;
+    if (n_prop_launches) *n_prop_launches = timed ? c->kprop_launches : 0;
+    return PLOM_OK;
+}
+
+extern "C" int plom_last_timing_lanes(plom_ctx *c, double *prop_ms, double *scan_ms, int32_t cap, int32_t *n_lanes)
+{
+    if (!c || !n_lanes || cap < 0 || (cap > 0 && (!prop_ms || !scan_ms))) return set_err(PLOM_ERR_ARG, "bad arguments");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int nl = 0;
+    for (const auto &sp : c->kspans) nl = sp.lane + 1 > nl ? sp.lane + 1 : nl;
+    for (int l = 0; l < cap; l++) { prop_ms[l] = 0.0; scan_ms[l] = 0.0; }
+    for (const auto &sp : c->kspans) {
+        if (sp.lane >= cap) continue;
+        float a = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, c->kev[(size_t)sp.a], c->kev[(size_t)sp.b]));
+        (sp.kind == 0 ? prop_ms : scan_ms)[sp.lane] += a;
+    }
+    *n_lanes = nl;
     return PLOM_OK;
 }
 

@@ -85,13 +85,14 @@ EXPORTS = [
     "plom_model_name", "plom_model_info_get", "plom_smc_run", "plom_filter_create", "plom_filter_reset",
     "plom_filter_update", "plom_filter_mif_start", "plom_covariate_eval",
     "plom_filter_run", "plom_filter_read", "plom_filter_destroy", "plom_filter_device_bytes",
-    "plom_batch_run", "plom_sync", "plom_ctx_set_kernel_timing", "plom_last_timing", "plom_selftest_math",
+    "plom_batch_run", "plom_sync", "plom_ctx_set_kernel_timing", "plom_last_timing", "plom_last_timing_lanes", "plom_selftest_math",
     "plom_selftest_normals", "plom_selftest_copy", "plom_batch_create", "plom_batch_launch",
     "plom_batch_destroy", "plom_batch_configure", "plom_batch_autotune", "plom_server_create", "plom_server_submit", "plom_server_step",
     "plom_server_status", "plom_server_done", "plom_server_result", "plom_server_destroy",
 ]
 
 SRV_QUEUED, SRV_RUNNING, SRV_DONE = 0, 1, 2
+ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_MODEL, ERR_STATE = -1, -2, -3, -4, -5
 
 
 class ServerConfig(C.Structure):
@@ -102,7 +103,11 @@ _lib = None
 
 
 class PlomError(RuntimeError):
-    pass
+    """A library call returned a status other than PLOM_OK (`code`, the PLOM_ERR_* value)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def lib():
@@ -136,6 +141,7 @@ def lib():
     L.plom_sync.argtypes = [C.c_void_p]
     L.plom_ctx_set_kernel_timing.argtypes = [C.c_void_p, C.c_int]
     L.plom_last_timing.argtypes = [C.c_void_p, dp, dp, dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.plom_last_timing_lanes.argtypes = [C.c_void_p, dp, dp, C.c_int32, C.POINTER(C.c_int32)]
     L.plom_selftest_math.argtypes = [C.c_void_p, C.c_int, dp, dp]
     L.plom_selftest_normals.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
     L.plom_selftest_copy.argtypes = [C.c_void_p, C.c_int64, C.c_int, dp]
@@ -157,7 +163,7 @@ def lib():
 
 def check(rc):
     if rc != 0:
-        raise PlomError("plom error %d: %s" % (rc, lib().plom_last_error().decode(errors="replace")))
+        raise PlomError("plom error %d: %s" % (rc, lib().plom_last_error().decode(errors="replace")), rc)
 
 
 def dptr(a):
@@ -198,6 +204,13 @@ class Context:
         n, nl = C.c_int32(), C.c_int32()
         check(lib().plom_last_timing(self.h, C.byref(t), C.byref(p), C.byref(q), C.byref(n), C.byref(nl)))
         return t.value, p.value, q.value, n.value, nl.value
+
+    def last_timing_lanes(self, cap=8):
+        """Per lane of the last timed run: (propagate_ms, scan_ms) lists."""
+        p, q = np.zeros(cap), np.zeros(cap)
+        n = C.c_int32()
+        check(lib().plom_last_timing_lanes(self.h, dptr(p), dptr(q), cap, C.byref(n)))
+        return list(p[:n.value]), list(q[:n.value])
 
 
 # ---------------------------------------------------------------------- models
@@ -610,7 +623,9 @@ class Server:
         return list(self._ids[:n.value])
 
     def result(self, rid):
-        N, S = self.N.pop(rid)
+        """The request's result (plom_server_result), which also frees its device memory;
+        PlomError (PLOM_ERR_STATE) while the request has not finished."""
+        N, S = self.N[rid]
         ll_n = np.zeros(N)
         ess = np.zeros(N)
         lin = np.zeros((N + 1, S)) if S else None
@@ -618,6 +633,7 @@ class Server:
         out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), None, None, None, dptr(lin), iptr(lsl), None, None, None,
                             None, None)
         check(lib().plom_server_result(self.h, rid, C.byref(out)))
+        del self.N[rid]
         return FilterResult(out.loglik, out.n_all_fail, ll_n, ess, lineage=lin, lineage_slots=lsl)
 
     def close(self):

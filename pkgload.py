@@ -4,7 +4,7 @@ import importlib.util
 import os
 import sys
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "iterated-inference-pipeline_amd")
 
 

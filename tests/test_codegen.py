@@ -81,7 +81,7 @@ def test_parser_precedence():
 def test_generated_headers_up_to_date():
     """The committed gen/ headers are what the generator produces from examples/."""
     gen = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "iterated-inference-pipeline_amd", "gen")
-    for name in ("sir", "sirs4", "seir10"):
+    for name in ("sir", "sirs4", "seir10", "noise", "drift"):
         text = emit_header(load_model(name))
         assert open(os.path.join(gen, "plom_%s.h" % name)).read() == text
 

@@ -142,3 +142,82 @@ def test_heavy_lane_equals_alone(ctx):
         assert got[rid].loglik == ref.loglik
         np.testing.assert_array_equal(got[rid].loglik_n, ref.loglik_n)
     srv.close()
+
+
+def test_result_straight_after_step_frees_the_slot(ctx):
+    """plom_server_result called right after the step that enqueued a request's last
+    interval (no status or done call in between, so the request is still on the server's
+    finished list): its slot goes back to the free list there.  With one slot, every later
+    request is still admitted, and each result equals its run alone."""
+    srv = smc.Server(ctx, max_active=1)
+    reqs = [("sir", 500, 6, 61), ("sir", 700, 5, 62), ("sirs4", 300, 4, 63), ("sir", 900, 3, 64)]
+    ids = []
+    for name, J, N, stream in reqs:
+        m, times, data, pv, x0 = inputs(name, N)
+        ids.append(srv.submit(m, J, times, data, pv, x0, seed=5, stream=stream))
+    got, early = {}, 0
+    for _ in range(5000):
+        srv.step()
+        for rid in ids:
+            if rid in got:
+                continue
+            try:
+                got[rid] = srv.result(rid)
+                early += 1
+            except smc.PlomError as e:
+                assert e.code == smc.ERR_STATE
+        if len(got) == len(ids):
+            break
+    assert len(got) == len(ids), "a slot was lost: later requests were never admitted"
+    assert early == len(ids)
+    for rid, (name, J, N, stream) in zip(ids, reqs):
+        ref = alone(ctx, name, J, N, stream)
+        assert got[rid].loglik == ref.loglik
+        np.testing.assert_array_equal(got[rid].loglik_n, ref.loglik_n)
+    srv.close()
+
+
+def test_heavy_slot_reused_by_the_light_lane(ctx):
+    """Regression test of the cross-lane slot reuse: two slots; a heavy request with a
+    lineage steps on lane 1 beside a long light request on lane 0, and light requests wait
+    in the queue.  When the heavy request finishes, the next light request is admitted on
+    lane 0 into the slot it frees, only once lane 1's queued steps, lineage launch and
+    copy-back of that slot have completed.  Every result (the heavy lineage included)
+    equals its run alone and the CPU oracle."""
+    from oracle import oracle
+    srv = smc.Server(ctx, heavy_threshold=30000, max_active=2)   # sirs4 (S = 12) with J >= 2500 is heavy
+    heavy = ("sirs4", 6000, 12, 71)
+    light = [("sir", 800, 52, 72), ("sir", 1500, 9, 73), ("sirs4", 700, 7, 74), ("sir", 2500, 5, 75)]
+    m, times, data, pv, x0 = inputs(heavy[0], heavy[2])
+    hid = srv.submit(m, heavy[1], times, data, pv, x0, seed=5, stream=heavy[3], lineage=True)
+    ids = []
+    for name, J, N, stream in light:
+        m, times, data, pv, x0 = inputs(name, N)
+        ids.append(srv.submit(m, J, times, data, pv, x0, seed=5, stream=stream))
+    got, order = {}, []
+    for _ in range(20000):
+        srv.step()
+        for rid in srv.done():
+            got[rid] = srv.result(rid)
+            order.append(rid)
+        if len(got) == len(ids) + 1:
+            break
+    assert len(got) == len(ids) + 1
+    # two slots: when the heavy request finishes, one light request holds the other slot,
+    # so any further light request that finishes after it was admitted after the heavy
+    # slot was freed, into that slot unless the other one had just been freed too
+    assert len(order) - 1 - order.index(hid) >= 2
+    name, J, N, stream = heavy
+    m, times, data, pv, x0 = inputs(name, N)
+    ref = smc.run(ctx, m, J, times, data, pv, x0, seed=5, stream=stream, lineage=True)
+    o = oracle.smc(m.id, J, times, data, pv, x0, seed=5, stream=stream, lineage=True, nthreads=8)
+    assert got[hid].loglik == ref.loglik == o["loglik"]
+    np.testing.assert_array_equal(got[hid].lineage, ref.lineage)
+    np.testing.assert_array_equal(got[hid].lineage, o["lineage"])
+    np.testing.assert_array_equal(got[hid].lineage_slots, o["lineage_slots"])
+    for rid, (name, J, N, stream) in zip(ids, light):
+        m, times, data, pv, x0 = inputs(name, N)
+        o = oracle.smc(m.id, J, times, data, pv, x0, seed=5, stream=stream, nthreads=8)
+        assert got[rid].loglik == alone(ctx, name, J, N, stream).loglik == o["loglik"]
+        np.testing.assert_array_equal(got[rid].loglik_n, o["loglik_n"])
+    srv.close()

@@ -249,10 +249,20 @@ const char *plom_last_error(void);
 int plom_ctx_create(int device, plom_ctx **out);
 int plom_ctx_destroy(plom_ctx *ctx);
 
+/* The generated per-model API (C/core/plom.h:896-913, C/templates/core_template.c): the
+ * models compiled into the library, then the models registered with plom_model_load. */
 int plom_model_count(void);
 int plom_model_find(const char *name);                 /* model id, or PLOM_ERR_MODEL */
 int plom_model_name(int model_id, char *buf, int len);
 int plom_model_info_get(int model_id, plom_model_info *out);
+/* Register a user's model from its model library (built by `build.py --model <dir>` or
+ * `plom_build.py --lib`: the generated code of that model compiled on its own, the role of
+ * the reference's per-model build, scripts/pmbuilder:50-105 -> Builder.py:283-351 ->
+ * C/templates/Makefile:28-29).  *model_id is the id to put in plom_filter_desc; loading a
+ * library of an already registered model name gives that id again.  PLOM_ERR_MODEL if the
+ * library cannot be loaded or was compiled against another version of this library.  The
+ * library stays loaded until the process ends. */
+int plom_model_load(const char *path, int32_t *model_id);
 
 /* the value of a covariate at the times t[0..nt) (the population size at t = 0 for the initial
  * state, pop_size_t0 of C/core/build.c:1179-1185, and at the observation times for the
@@ -359,14 +369,18 @@ int plom_last_timing(plom_ctx *ctx, double *total_ms, double *propagate_ms, doub
  * n_lanes = lanes that recorded events */
 int plom_last_timing_lanes(plom_ctx *ctx, double *propagate_ms, double *scan_ms, int32_t cap, int32_t *n_lanes);
 
-/* device self-test of the shared arithmetic: out[i*7+..] = exp, log, Phi, sin, cos, sqrt,
- * sqrt_mid (the Box-Muller radius square root) of in[i] */
+/* device self-test of the shared arithmetic: out[i*8+..] = exp, log, Phi, sin, cos, sqrt,
+ * sqrt_mid (the Box-Muller radius square root), sqrt_rate (the demographic noise square
+ * root) of in[i] */
 int plom_selftest_math(plom_ctx *ctx, int n, const double *in, double *out);
 /* plain device copy of nbytes (read nbytes + write nbytes), repeated: bandwidth reference */
 int plom_selftest_copy(plom_ctx *ctx, int64_t nbytes, int reps, double *ms_per_copy);
 /* device self-test of the streams: the first n normals of particle j at interval n0 */
 int plom_selftest_normals(plom_ctx *ctx, uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j,
                           int n, double *out);
+/* the same Gaussians drawn by the FP32 stream (the same words, the transform in FP32),
+ * widened to double */
+int plom_selftest_normals32(plom_ctx *ctx, uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int n, double *out);
 
 #ifdef __cplusplus
 }

@@ -535,6 +535,13 @@ void cli_load(cli_problem *P, int is_bayesian, int nb_obs, double dt)
     P->N_DATA = (int)pj_number(pj_get(cst, "N_DATA"), NULL);
     P->model_name = pj_string(pj_get(settings, "model"));
     if (!P->model_name) cli_die(".settings.json has no 'model' (write it with this build's model builder)");
+    /* a model outside the library: the model library plom_build.py --lib wrote beside the
+     * settings (a path relative to the current directory, where .settings.json is read) */
+    const char *mlib = pj_string(pj_get(settings, "model_library"));
+    if (mlib) {
+        int32_t mid = -1;
+        if (plom_model_load(mlib, &mid) != PLOM_OK) cli_die(plom_last_error());
+    }
     P->model_id = plom_model_find(P->model_name);
     if (P->model_id < 0) cli_die(plom_last_error());
     plom_model_info_get(P->model_id, &P->mi);

@@ -31,9 +31,16 @@ def load_model(name_or_dir):
     return PlomModel(c, p, l, name=p.get("name", os.path.basename(d)))
 
 
+def _write_if_changed(path, text):
+    if not os.path.exists(path) or open(path).read() != text:
+        with open(path, "w") as f:
+            f.write(text)
+
+
 def generate_all(out_dir=GEN_DIR, names=BUILTIN_MODELS):
-    """Write gen/plom_<name>.h for every built-in model and gen/plom_models.h that
-    includes them; returns the list of written paths."""
+    """Write gen/plom_<name>.h for every built-in model, gen/plom_models.h that includes
+    them (the CPU oracle) and gen/plom_model_list.h, the list the library registers its
+    built-in models from (one translation unit per model); returns the written paths."""
     os.makedirs(out_dir, exist_ok=True)
     written = []
     for n in names:
@@ -51,9 +58,12 @@ def generate_all(out_dir=GEN_DIR, names=BUILTIN_MODELS):
     inc.append("#define PLOM_N_MODELS %d" % len(names))
     inc.append("#endif")
     path = os.path.join(out_dir, "plom_models.h")
-    text = "\n".join(inc) + "\n"
-    if not os.path.exists(path) or open(path).read() != text:
-        with open(path, "w") as f:
-            f.write(text)
+    _write_if_changed(path, "\n".join(inc) + "\n")
+    written.append(path)
+    lst = ["/* Generated: the models compiled into libplom_smc, in model-id order. Do not edit. */",
+           "#ifndef PLOM_MODEL_LIST_H", "#define PLOM_MODEL_LIST_H",
+           "#define PLOM_BUILTIN_MODELS(X) " + " ".join("X(%s)" % n for n in names), "#endif"]
+    path = os.path.join(out_dir, "plom_model_list.h")
+    _write_if_changed(path, "\n".join(lst) + "\n")
     written.append(path)
     return written

@@ -197,7 +197,7 @@ class Emitter:
 
     def eq_parts(self, items, cac, real, dem, env):
         """items: list of (sign, term, reaction). Returns (eq, dem, env) strings."""
-        sq = "plom_sqrt" if real == "double" else "plom_fsqrtf"
+        sq = "plom_sqrt_rate" if real == "double" else "plom_fsqrtf"
         eq = "".join(" %s (%s)" % (s, t) for s, t, _ in items)
         d = "".join("%s %s((%s))*d%d_%d" % (s, sq, t, r["ind_dem_sto"], cac) for s, t, r in items) if dem else ""
         e = ""

@@ -1,0 +1,766 @@
+/*
+ * plom_device.h -- the model-dependent device code of the SMC filter (gfx950): the device
+ * filter descriptor, the per-particle kernels templated on a generated model (propagate,
+ * lineage, hat tiles, prediction residuals) and their host launchers.
+ *
+ * Every model is compiled in a translation unit of its own (plom_model.hip), which
+ * includes this header and one generated model header and exports one model_entry; the
+ * library (plom_smc.hip) holds the model-independent kernels (weights, scan, advance,
+ * final resampling, hat selection) and the host side, and reaches a model's kernels only
+ * through its model_entry.  A model outside the library is compiled the same way into a
+ * shared library of its own and registered at run time (plom_model_load).
+ */
+#ifndef PLOM_DEVICE_H
+#define PLOM_DEVICE_H
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "plom_smc.h"
+#include "plom_math.h"
+#include "plom_cov.h"
+#include "plom_ode.h"
+#include "plom_rng.h"
+#include "plom_mif.h"
+
+typedef unsigned __int128 plom_u128;
+
+#define TILE 256
+#define PLOM_TF_CAP 512   /* LDS entries of the per-interval time-function table */
+#define MAX_TILES (PLOM_MAX_J / TILE)
+
+/* ------------------------------------------------------------------ device filter */
+struct plom_dfilter {
+    int32_t J, Jp, T, N, S, n_ts, flags, step;
+    uint32_t seed, stream, prec, pad;
+    double dt, sqdt, like_min, all_fail_llike;
+    double eps_abs, eps_rel;   /* PLOM_FLAG_ODE: error control of the RKF45 integration */
+    void *X[2];
+    const void *x0;
+    const double *pv;
+    const float *pvf;
+    const double *cov;  /* covariate table [N kmax + N + 1][cov_cs] (plom_cov.h), or the covariate
+                           slots of pv with cov_cs = 0 (constant covariates) */
+    int32_t cov_cs, cov_kmax;
+    plom_drift_view dv; /* drifting parameters (used when drift_on) */
+    int32_t drift_on, pad3;
+    const double *times, *data;
+    const int32_t *nonan;
+    double *logw, *tmax, *tsum, *tsum2;
+    uint64_t *cum, *ttot, *tincl;
+    uint32_t *cnt;      /* [0] propagate arrivals, [1] scan arrivals */
+    double *scal;       /* m, S, S2, loglik */
+    int32_t *n_all_fail;
+    double *llike_n, *ess_n;
+    int32_t *rflag;     /* rflag[n] = 1 when interval n ended with a resampling */
+    int32_t *ancestry;  /* [N][J] or null */
+    int32_t *lin_slots; /* [N] or null: the sampled lineage's particle index per interval */
+    double *lin_state;  /* [(N+1)][S] or null: its states */
+    double *lin_obs;    /* [(N+1)][n_ts][2] or null: its observation mean and one drawn observation */
+    int32_t n_hat, pad2;
+    uint64_t *hkey;     /* [n_hat][Jp] or null: per tile, the sorted keys of the hat quantities */
+    uint64_t *hpref;    /* [n_hat][Jp]: inclusive prefix of their weights in that order */
+    double *hpart;      /* [n_hat][T]: per tile, tree sum of value x weight */
+    double *hat;        /* [(N+1)][n_hat][3]: low95, mean, high95 */
+    double *pres;       /* [N][n_ts][2]: prediction mean and residual */
+    /* PLOM_FLAG_MIF: per-particle parameters */
+    int32_t n_theta, n_pp;
+    const int32_t *pm_src;          /* [n_pp] component of each per-particle pv slot, -1 = pv */
+    const int32_t *pm_tr, *pm_sc, *pm_dur;      /* [n_theta] */
+    const double *pm_min, *pm_max, *pm_mult, *pm_sd;   /* [n_theta] */
+    const double *mut_fac;          /* [N] random-walk factor after interval n (with data) */
+    const double *pm_chol;          /* [n_theta][n_theta] lower factor of the walk's covariance, or null */
+    const int32_t *pm_prior, *pm_puntil;   /* [n_theta] prior patch of the weights, or null */
+    const double *pm_ppow;                 /* [n_theta] */
+    double *th[2];                  /* [n_theta][Jp] theta of the particles, ping-pong like X */
+    double *mpw, *mps, *mpm, *mpq;  /* tile partials: [T] sum w, [n_theta][T] sum w theta, mean, M2 */
+    double *mmean, *mvar;           /* [(N+1)][n_theta] weighted mean and variance of theta */
+    void *allX;                     /* PLOM_FLAG_ALL_STATES: [(N+1)][S][Jp] every propagated cloud */
+    /* the ode implementation with time-varying covariates: their splines, evaluated at the
+     * integrator's times (spl_on = 0: the covariates are constants) */
+    plom_spline_set spl;
+    int32_t spl_on, pad_spl;
+};
+
+/* ------------------------------------------------------------------ covariate rows */
+/* the rows of interval n's substeps (the same in every lane: moved to scalar registers so
+ * that the generated code reads the values with scalar loads) */
+__device__ inline const double *cov_steps(const plom_dfilter *f, int n)
+{
+    const uint64_t u = (uint64_t)(f->cov + (size_t)n * (size_t)f->cov_kmax * (size_t)f->cov_cs);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+    return (const double *)(((uint64_t)hi << 32) | lo);
+}
+
+/* the row of observation time times[r], 0 <= r <= N */
+__device__ inline const double *cov_at(const plom_dfilter *f, int r)
+{
+    return f->cov + ((size_t)f->N * (size_t)f->cov_kmax + (size_t)r) * (size_t)f->cov_cs;
+}
+
+/* ------------------------------------------------------------------ block helpers */
+/* recursive contiguous pairwise sum of the 256 values of a block (every lane gets it) */
+__device__ inline double tree256(double v, double *lds)
+{
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1) v = v + __shfl_xor(v, h, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    double r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+    __syncthreads();
+    return r;
+}
+
+__device__ inline double max256(double v, double *lds)
+{
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1) v = fmax(v, __shfl_xor(v, h, 64));
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    double r = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+    __syncthreads();
+    return r;
+}
+
+/* inclusive scan of 256 uint64 values (exact: any order gives the same integers) */
+__device__ inline uint64_t scan256(uint64_t v, uint64_t *lds)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1) {
+        uint64_t o = __shfl_up(v, h, 64);
+        if (lane >= h) v += o;
+    }
+    if (lane == 63) lds[w] = v;
+    __syncthreads();
+    uint64_t off = 0;
+    for (int k = 0; k < w; k++) off += lds[k];
+    __syncthreads();
+    return v + off;
+}
+
+/* first particle i with C_i * J2 >= Rk in the fixed-point cumulative weights of the
+ * interval whose scan is in f->cum / f->tincl */
+__device__ inline int select_first(const plom_dfilter *f, plom_u128 Rk, uint64_t J2)
+{
+    const int T = f->T;
+    const uint64_t *tincl = f->tincl;
+    int lo = 0, hi = T - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if ((plom_u128)tincl[mid] * J2 >= Rk) hi = mid; else lo = mid + 1;
+    }
+    const uint64_t P = lo ? tincl[lo - 1] : 0ull;
+    const uint64_t *c = f->cum + (size_t)lo * TILE;
+    int a = 0, b = TILE - 1;
+    while (a < b) {
+        int mid = (a + b) >> 1;
+        if ((plom_u128)(P + c[mid]) * J2 >= Rk) b = mid; else a = mid + 1;
+    }
+    return lo * TILE + a;
+}
+
+/* systematic resampling of interval n: ancestor of output slot k, the first i with
+ * C_i J 2^33 >= (k 2^33 + 2x + 1) Q */
+__device__ inline int select_ancestor(const plom_dfilter *f, int k, int n)
+{
+    const uint64_t Q = f->tincl[f->T - 1];
+    const uint64_t unum = plom_resample_num(f->seed, f->stream, (uint32_t)n);
+    const plom_u128 Rk = (plom_u128)(((uint64_t)k << 33) + unum) * (plom_u128)Q;
+    return select_first(f, Rk, (uint64_t)f->J << 33);
+}
+
+
+/* ------------------------------------------------------------------ kernels */
+/* PLOM_FLAG_MIF: theta of particle j from its ancestor a, moved by the random walk when
+ * interval n-1 had data (resample_and_mut_theta_mif, C/mif/methods.c:261-367: every
+ * component with a jump size moves by mut_fac[n-1] * sd * N(0,1)); written to the output
+ * table; then the per-particle slots of pv from it (back_transform_theta2par) */
+template <int NPP>
+__device__ inline void mif_particle(const plom_dfilter *f, int n, int j, int a, const double *tab, double *lp)
+{
+    const int Jp = f->Jp, nth = f->n_theta;
+    const double *Tin = f->th[n & 1];
+    double *Tout = f->th[(n + 1) & 1];
+    const double fac = (n > 0 && f->nonan[n - 1] > 0) ? f->mut_fac[n - 1] : 0.0;
+    if (f->pm_chol) {
+        /* correlated walk: theta_o += fac sum_{q<=o} L[o][q] z_q, z from the same counters */
+        double z[PLOM_MIF_MVN_MAX];
+        for (int o = 0; o < nth; o += 2) {
+            double z0 = 0.0, z1 = 0.0;
+            if (fac != 0.0) plom_mif_normals(f->seed, f->stream, (uint32_t)n, (uint32_t)j, (uint32_t)(o >> 1), tab, &z0, &z1);
+            z[o] = z0;
+            if (o + 1 < nth) z[o + 1] = z1;
+        }
+        for (int o = 0; o < nth; o++) {
+            double v = Tin[(size_t)o * Jp + a];
+            if (fac != 0.0) {
+                double acc = 0.0;
+                for (int q = 0; q <= o; q++) acc = acc + f->pm_chol[(size_t)o * nth + q] * z[q];
+                v = v + fac * acc;
+            }
+            Tout[(size_t)o * Jp + j] = v;
+        }
+    } else
+    for (int o = 0; o < nth; o += 2) {
+        const double sd0 = fac * f->pm_sd[o];
+        const double sd1 = (o + 1 < nth) ? fac * f->pm_sd[o + 1] : 0.0;
+        double z0 = 0.0, z1 = 0.0;
+        if (sd0 != 0.0 || sd1 != 0.0) plom_mif_normals(f->seed, f->stream, (uint32_t)n, (uint32_t)j, (uint32_t)(o >> 1), tab, &z0, &z1);
+        double v = Tin[(size_t)o * Jp + a];
+        if (sd0 != 0.0) v = v + sd0 * z0;
+        Tout[(size_t)o * Jp + j] = v;
+        if (o + 1 < nth) {
+            double w = Tin[(size_t)(o + 1) * Jp + a];
+            if (sd1 != 0.0) w = w + sd1 * z1;
+            Tout[(size_t)(o + 1) * Jp + j] = w;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NPP; s++) {
+        const int o = f->pm_src[s];
+        lp[s] = o < 0 ? f->pv[s]
+                      : plom_mif_natural(f->pm_tr[o], f->pm_sc[o], f->pm_dur[o], Tout[(size_t)o * Jp + j], f->pm_min[o],
+                                         f->pm_max[o], f->pm_mult[o]);
+    }
+}
+
+template <class M, typename R, int V, bool MIF>
+__global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const int2 *tiles)
+{
+    __shared__ double lds[8];
+    int fi = 0, tile = blockIdx.x;
+    if (tiles) { const int2 tt = tiles[blockIdx.x]; fi = tt.x; tile = tt.y; }
+    plom_dfilter *f = &filters[fi];
+    const int n = f->step;
+    if (n >= f->N) return;
+    /* in LDS (FP64 only; the block returns above as a whole): the coefficient table of the
+     * stream's Gaussians (not for the ode implementation, which draws none) and, with MIF,
+     * the Box-Muller tables of the random walk */
+    __shared__ double s_icdf[PLOM_ICDF_TAB_SIZE];
+    __shared__ float s_icdf32[PLOM_ICDF_TAB_SIZE];
+    __shared__ double s_bm[PLOM_BM_TAB_SIZE];
+    if constexpr (sizeof(R) == 8 && V != 6) {
+        for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += TILE) s_icdf[i] = plom_icdf_tab[i];
+    }
+    if constexpr (sizeof(R) == 4) {
+        for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += TILE) s_icdf32[i] = (float)plom_icdf_tab[i];
+    }
+    if constexpr (sizeof(R) == 8 && MIF) {
+        for (int i = threadIdx.x; i < PLOM_BM_TAB_SIZE; i += TILE) s_bm[i] = plom_bm_tab[i];
+    }
+    if constexpr (sizeof(R) == 4 || V != 6 || MIF) __syncthreads();
+    const int J = f->J, Jp = f->Jp;
+    const int j = tile * TILE + threadIdx.x;
+    const bool valid = j < J;
+
+    R X[M::S];
+    int a = j;
+    if (n == 0 && !MIF) {
+        const R *x0 = (const R *)f->x0;
+#pragma unroll
+        for (int k = 0; k < M::S; k++) X[k] = x0[k];
+    } else {
+        /* (with MIF, interval 0 starts from the particles' own initial states in X[0]) */
+        if (n > 0) {
+            if (valid && f->rflag[n - 1]) a = select_ancestor(f, j, n - 1);
+            if (valid && f->ancestry) f->ancestry[(size_t)(n - 1) * J + j] = a;
+        }
+        const R *Xin = (const R *)f->X[n & 1];
+        if (!valid) a = 0;
+#pragma unroll
+        for (int k = 0; k < M::S; k++) X[k] = Xin[(size_t)k * Jp + a];
+    }
+    if (n == 0 && f->allX && valid) {
+        R *A = (R *)f->allX;
+#pragma unroll
+        for (int k = 0; k < M::S; k++) A[(size_t)k * Jp + j] = X[k];
+    }
+    /* per-particle slots of pv (MIF only; the array is unused otherwise) */
+    double lp[MIF ? M::NPP : 1];
+    if constexpr (MIF) {
+        if (valid) {
+            if constexpr (sizeof(R) == 8) mif_particle<M::NPP>(f, n, j, a, s_bm, lp);
+            else mif_particle<M::NPP>(f, n, j, a, plom_bm_tab, lp);
+        }
+    }
+
+    const double t0 = f->times[n], t1 = f->times[n + 1];
+    /* the time-only functions of the interval's substeps (the seasonal forcing ...): the same
+     * for every particle of the block, so each substep time is evaluated once, by one
+     * thread, into LDS; the generated step reads the table (identical values: the same
+     * function of the same accumulated t), or evaluates them itself when they do not fit */
+    const R *tft = nullptr;
+    if constexpr (M::NTF > 0 && V != 6) {
+        __shared__ double s_tf_raw[PLOM_TF_CAP];
+        R *s_tf = (R *)s_tf_raw;
+        const int K = plom_substeps(t0, t1, f->dt);
+        if (K * M::NTF <= PLOM_TF_CAP) {
+            for (int k = threadIdx.x; k < K; k += TILE) {
+                double t = t0;
+                for (int q = 0; q < k; q++) t += f->dt;
+                M::time_funcs((R)t, s_tf + k * M::NTF);
+            }
+            __syncthreads();
+            tft = s_tf;
+        }
+    }
+    if (valid) {
+        if constexpr (V == 6) {
+            /* the ode implementation: deterministic, no random numbers */
+            const plom_spline_set *sp = f->spl_on ? &f->spl : nullptr;
+            if constexpr (MIF) M::interval_ode_pp((double *)X, f->pv, lp, f->cov, sp, t0, t1, f->dt, f->eps_abs, f->eps_rel);
+            else M::interval_ode((double *)X, f->pv, f->cov, sp, t0, t1, f->dt, f->eps_abs, f->eps_rel);
+        } else if constexpr (sizeof(R) == 8) {
+            plom_nstream64 ns;
+            plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j, s_icdf);
+            if constexpr (MIF) M::template interval_pp<V>((double *)X, f->pv, lp, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs, f->drift_on ? &f->dv : nullptr, (const double *)tft);
+            else M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs, f->drift_on ? &f->dv : nullptr, (const double *)tft);
+        } else {
+            plom_nstream32 ns;
+            plom_nstream32_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j, s_icdf32);
+            if constexpr (MIF) {
+                float lpf[M::NPP];
+#pragma unroll
+                for (int k = 0; k < M::NPP; k++) lpf[k] = (float)lp[k];
+                M::template interval_pp<V>((float *)X, f->pvf, lpf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs,
+                                           f->drift_on ? &f->dv : nullptr, (const float *)tft);
+            } else {
+                M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs,
+                                        f->drift_on ? &f->dv : nullptr, (const float *)tft);
+            }
+        }
+        R *Xout = (R *)f->X[(n + 1) & 1];
+#pragma unroll
+        for (int k = 0; k < M::S; k++) Xout[(size_t)k * Jp + j] = X[k];
+        if (f->allX) {
+            R *A = (R *)f->allX + (size_t)(n + 1) * M::S * Jp;
+#pragma unroll
+            for (int k = 0; k < M::S; k++) A[(size_t)k * Jp + j] = X[k];
+        }
+    }
+
+    if ((f->flags & PLOM_FLAG_NO_FILTER) || f->nonan[n] == 0) return;
+
+    double l = -PLOM_INF;
+    if (valid) {
+        double obs[M::N_TS];
+        int nf = 0;
+        const double *cvd = cov_at(f, n + 1);
+        if constexpr (MIF) {
+            M::proj2obs_pp(X, f->pv, lp, cvd, obs);
+            l = M::loglik_pp(obs, f->data + (size_t)n * f->n_ts, f->pv, lp, cvd, t1, f->like_min, &nf);
+        } else {
+            M::proj2obs(X, f->pv, cvd, obs);
+            l = M::loglik(obs, f->data + (size_t)n * f->n_ts, f->pv, cvd, t1, f->like_min, &nf);
+        }
+        if (MIF && f->pm_prior) {
+            /* mif --prior: the prior terms of the particle's theta for this interval (the
+             * table holds what mif_particle wrote for it), then the failure test on the
+             * patched weight */
+            const double *th = f->th[(n + 1) & 1];
+            double pt = 0.0;
+            for (int o = 0; o < f->n_theta; o++)
+                if (f->pm_prior[o] != PLOM_PRIOR_NONE && n < f->pm_puntil[o])
+                    pt = pt + plom_mif_prior_term(f->pm_prior[o], f->pm_ppow[o], f->pm_tr[o], th[(size_t)o * Jp + j],
+                                                  f->pm_min[o], f->pm_max[o], f->like_min);
+            l = plom_mif_prior_patch(l, nf == M::N_TS, pt, f->all_fail_llike);
+        } else if (nf == M::N_TS) {
+            /* failure: every series observed and floored at LIKE_MIN (smc.c:44) */
+            l = -PLOM_INF;
+        }
+    }
+    f->logw[j] = l;
+    const double mt = max256(l, lds);
+    double w = 0.0;
+    if (mt != -PLOM_INF && l != -PLOM_INF) w = plom_exp(l - mt);
+    const double s = tree256(w, lds);
+    const double s2 = tree256(w * w, lds);
+    if (threadIdx.x == 0) { f->tmax[tile] = mt; f->tsum[tile] = s; f->tsum2[tile] = s2; }
+}
+
+/* The lineage of one filter (one block of one wave; lane 0 works): the final particle by
+ * inverse CDF on the last interval's cumulative weights (equal weights when that interval
+ * had no data, was not filtered or every particle failed), its ancestors by the stored
+ * ancestry, then its states regenerated from x0 with the same counter-based noise the
+ * filter drew, interval by interval: s[N-1] = first i with C_i 2^33 >= (2x+1) Q;
+ * s[n-1] = ancestry[n-1][s[n]]; state[n+1] = interval_n(state[n], noise(n, s[n])). */
+template <class M, typename R, int V>
+__global__ __launch_bounds__(64) void k_lineage(plom_dfilter *f)
+{
+    __shared__ double s_icdf[PLOM_ICDF_TAB_SIZE];
+    __shared__ float s_icdf32[PLOM_ICDF_TAB_SIZE];
+    if constexpr (sizeof(R) == 8 && V != 6) {
+        for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += 64) s_icdf[i] = plom_icdf_tab[i];
+        __syncthreads();
+    }
+    if constexpr (sizeof(R) == 4) {
+        for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += 64) s_icdf32[i] = (float)plom_icdf_tab[i];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const int N = f->N, J = f->J;
+    if (N == 0 || !f->lin_slots) return;
+    const uint64_t unum = plom_lineage_num(f->seed, f->stream);
+    int j;
+    if (!(f->flags & PLOM_FLAG_NO_FILTER) && f->nonan[N - 1] > 0 && f->rflag[N - 1]) {
+        const uint64_t Q = f->tincl[f->T - 1];
+        j = select_first(f, (plom_u128)unum * (plom_u128)Q, 1ull << 33);
+    } else {
+        j = (int)((unum * (uint64_t)J + (1ull << 33) - 1ull) >> 33) - 1;
+    }
+    j = j < 0 ? 0 : (j >= J ? J - 1 : j);
+    f->lin_slots[N - 1] = j;
+    for (int n = N - 1; n >= 1; n--) {
+        j = f->ancestry[(size_t)(n - 1) * J + j];
+        f->lin_slots[n - 1] = j;
+    }
+    R X[M::S];
+    const R *x0 = (const R *)f->x0;
+#pragma unroll
+    for (int k = 0; k < M::S; k++) {
+        X[k] = x0[k];
+        f->lin_state[k] = (double)X[k];
+    }
+    for (int n = 0; n < N; n++) {
+        const double t0 = f->times[n], t1 = f->times[n + 1];
+        const uint32_t jn = (uint32_t)f->lin_slots[n];
+        if constexpr (V == 6) {
+            M::interval_ode((double *)X, f->pv, f->cov, f->spl_on ? &f->spl : nullptr, t0, t1, f->dt, f->eps_abs,
+                            f->eps_rel);
+        } else if constexpr (sizeof(R) == 8) {
+            plom_nstream64 ns;
+            plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, jn, s_icdf);
+            M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs,
+                                    f->drift_on ? &f->dv : nullptr, (const double *)nullptr);
+        } else {
+            plom_nstream32 ns;
+            plom_nstream32_init(&ns, f->seed, f->stream, (uint32_t)n, jn, s_icdf32);
+            M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs,
+                                    f->drift_on ? &f->dv : nullptr, (const float *)nullptr);
+        }
+#pragma unroll
+        for (int k = 0; k < M::S; k++) f->lin_state[(size_t)(n + 1) * M::S + k] = (double)X[k];
+    }
+    /* observation mean and one drawn observation per row (print_p_X, print.c:312-341:
+     * obs_mean and observation(), core_template.c:369-412: mean + sd N(0,1), clipped at 0) */
+    for (int n = 0; n <= N; n++) {
+        double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
+#pragma unroll
+        for (int k = 0; k < M::S; k++) xs[k] = f->lin_state[(size_t)n * M::S + k];
+        M::proj2obs(xs, f->pv, cov_at(f, n), obs);
+        M::obs_moments(obs, f->pv, cov_at(f, n), f->times[n], mu, var);
+        plom_lineage_obs(f->seed, f->stream, (uint32_t)n, M::N_TS, mu, var, plom_bm_tab, f->lin_obs + (size_t)n * M::N_TS * 2);
+    }
+}
+
+/* ------------------------------------------------------------------ hat and residuals */
+/* The summaries of interval n (launched after its weights and scan, before the next
+ * propagate reads the cloud).  Weights are the fixed-point integers of the resampling,
+ * q_j = floor(exp(l_j - m)/S 2^62) (0 for failed particles), or 1 for every particle when
+ * the interval was not weighted; Q is their total.
+ *   mean     tree sum over tiles of the tile trees of value x q, divided by Q;
+ *   low95    the smallest value v with 40 W(<= v) >= Q, W(<= v) the total weight of the
+ *            particles whose value is at most v;
+ *   high95   the smallest v with 40 W(<= v) >= 39 Q
+ * (hat.c:29-55: the first sorted value at which the cumulated weight reaches 2.5 % /
+ * 97.5 %; exact in integers here, and independent of the order of equal values).
+ * Each tile sorts its values (bitonic, in LDS) with their weights and keeps the inclusive
+ * weight prefix; one block per quantity and bound then bisects the 64-bit key space,
+ * summing the tiles' prefixes exactly. */
+__device__ inline uint64_t hat_key(double x)
+{
+    const uint64_t u = plom_d2u(x);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__device__ inline double hat_unkey(uint64_t k)
+{
+    return plom_u2d((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k);
+}
+
+__device__ inline bool hat_weighted(const plom_dfilter *f, int n)
+{
+    return !(f->flags & PLOM_FLAG_NO_FILTER) && f->nonan[n] > 0 && f->rflag[n];
+}
+
+template <class M, typename R>
+__global__ __launch_bounds__(TILE) void k_hat_tiles(plom_dfilter *f)
+{
+    __shared__ double lds[8];
+    __shared__ uint64_t lds64[8];
+    __shared__ uint64_t s_key[TILE];
+    __shared__ uint64_t s_w[TILE];
+    const int n = f->step - 1;
+    if (n < 0 || n >= f->N || !f->hkey) return;
+    const int J = f->J, Jp = f->Jp, T = f->T, tile = blockIdx.x;
+    const int j = tile * TILE + threadIdx.x;
+    const bool valid = j < J;
+    double v[M::N_HAT];
+    uint64_t q = 0;
+    if (valid) {
+        const R *Xs = (const R *)f->X[(n + 1) & 1];
+        double xs[M::S];
+#pragma unroll
+        for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
+        M::hat_values(xs, f->pv, cov_at(f, n + 1), f->drift_on ? &f->dv : nullptr, f->times[n + 1], v);
+        if (hat_weighted(f, n)) {
+            const double l = f->logw[j];
+            if (l != -PLOM_INF) q = (uint64_t)((plom_exp(l - f->scal[0]) / f->scal[1]) * 0x1p62);
+        } else {
+            q = 1;
+        }
+    }
+    for (int k = 0; k < M::N_HAT; k++) {
+        const double part = tree256(valid ? v[k] * (double)q : 0.0, lds);
+        if (threadIdx.x == 0) f->hpart[(size_t)k * T + tile] = part;
+        s_key[threadIdx.x] = valid ? hat_key(v[k]) : ~0ull;
+        s_w[threadIdx.x] = q;
+        __syncthreads();
+        for (int size = 2; size <= TILE; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                const int t = threadIdx.x, p = t ^ stride;
+                if (p > t) {
+                    const bool up = (t & size) == 0;
+                    const uint64_t a = s_key[t], b = s_key[p];
+                    if ((a > b) == up) {
+                        s_key[t] = b; s_key[p] = a;
+                        const uint64_t wa = s_w[t];
+                        s_w[t] = s_w[p]; s_w[p] = wa;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        const uint64_t key = s_key[threadIdx.x];
+        const uint64_t pref = scan256(s_w[threadIdx.x], lds64);
+        f->hkey[(size_t)k * Jp + j] = key;
+        f->hpref[(size_t)k * Jp + j] = pref;
+        __syncthreads();
+    }
+}
+
+/* number of systematic points k in [0, J) with (k 2^33 + u) Q <= C J 2^33, i.e. of
+ * particles whose ancestor index is at most the particle with inclusive prefix C */
+__device__ inline uint64_t hat_count_le(uint64_t C, uint64_t J, uint64_t unum, uint64_t Q)
+{
+    const plom_u128 X = (plom_u128)C * J << 33;
+    const double est = (double)C * (double)J / (double)Q - (double)unum * 0x1p-33;
+    int64_t k = est < 0.0 ? 0 : (int64_t)est + 1;
+    if (k > (int64_t)J) k = (int64_t)J;
+    while (k > 0 && (plom_u128)(((uint64_t)(k - 1) << 33) + unum) * Q > X) k--;
+    while (k < (int64_t)J && (plom_u128)(((uint64_t)k << 33) + unum) * Q <= X) k++;
+    return (uint64_t)k;
+}
+
+/* prediction residuals of interval n over the resampled cloud (print.c:632-701): with o_j
+ * the number of copies of particle j, pred = sum o_j mu_j / J, var_state = sum o_j
+ * (mu_j - pred)^2 / (J - 1), var_obs = sum o_j sigma2_j / J, res = (y - pred) /
+ * sqrt(var_state + var_obs); each sum strided by thread, then a tree over the block */
+template <class M, typename R>
+__global__ __launch_bounds__(TILE) void k_pred_res(plom_dfilter *f)
+{
+    __shared__ double lds[8];
+    const int n = f->step - 1;
+    if (n < 0 || n >= f->N || !f->pres) return;
+    double *out = f->pres + (size_t)n * M::N_TS * 2;
+    if ((f->flags & PLOM_FLAG_NO_FILTER) || f->nonan[n] == 0) {
+        if (threadIdx.x < 2 * M::N_TS) out[threadIdx.x] = PLOM_NAN;
+        return;
+    }
+    const int J = f->J, Jp = f->Jp;
+    const bool res_on = f->rflag[n] != 0;
+    const uint64_t Q = res_on ? f->tincl[f->T - 1] : 0;
+    const uint64_t unum = plom_resample_num(f->seed, f->stream, (uint32_t)n);
+    const R *Xs = (const R *)f->X[(n + 1) & 1];
+    const double t1 = f->times[n + 1];
+    double a[M::N_TS], b[M::N_TS];
+#pragma unroll
+    for (int ts = 0; ts < M::N_TS; ts++) { a[ts] = 0.0; b[ts] = 0.0; }
+    for (int j = threadIdx.x; j < J; j += TILE) {
+        uint64_t o = 1;
+        if (res_on) {
+            const int tile = j / TILE;
+            const uint64_t C = (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j];
+            const uint64_t Cm = j ? ((j % TILE) ? (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j - 1] : f->tincl[tile - 1]) : 0ull;
+            o = hat_count_le(C, (uint64_t)J, unum, Q) - hat_count_le(Cm, (uint64_t)J, unum, Q);
+        }
+        if (!o) continue;
+        double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
+#pragma unroll
+        for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
+        M::proj2obs(xs, f->pv, cov_at(f, n + 1), obs);
+        M::obs_moments(obs, f->pv, cov_at(f, n + 1), t1, mu, var);
+#pragma unroll
+        for (int ts = 0; ts < M::N_TS; ts++) { a[ts] += (double)o * mu[ts]; b[ts] += (double)o * var[ts]; }
+    }
+    double pred[M::N_TS], vobs[M::N_TS], c[M::N_TS];
+#pragma unroll
+    for (int ts = 0; ts < M::N_TS; ts++) {
+        pred[ts] = tree256(a[ts], lds) / (double)J;
+        vobs[ts] = tree256(b[ts], lds) / (double)J;
+        c[ts] = 0.0;
+    }
+    for (int j = threadIdx.x; j < J; j += TILE) {
+        uint64_t o = 1;
+        if (res_on) {
+            const int tile = j / TILE;
+            const uint64_t C = (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j];
+            const uint64_t Cm = j ? ((j % TILE) ? (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j - 1] : f->tincl[tile - 1]) : 0ull;
+            o = hat_count_le(C, (uint64_t)J, unum, Q) - hat_count_le(Cm, (uint64_t)J, unum, Q);
+        }
+        if (!o) continue;
+        double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
+#pragma unroll
+        for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
+        M::proj2obs(xs, f->pv, cov_at(f, n + 1), obs);
+        M::obs_moments(obs, f->pv, cov_at(f, n + 1), t1, mu, var);
+#pragma unroll
+        for (int ts = 0; ts < M::N_TS; ts++) {
+            const double d = mu[ts] - pred[ts];
+            c[ts] += (double)o * (d * d);
+        }
+    }
+#pragma unroll
+    for (int ts = 0; ts < M::N_TS; ts++) {
+        const double vstate = tree256(c[ts], lds) / (double)(J - 1);
+        if (threadIdx.x == 0) {
+            const double y = f->data[(size_t)n * M::N_TS + ts];
+            out[2 * ts] = pred[ts];
+            out[2 * ts + 1] = (y - pred[ts]) / plom_sqrt(vstate + vobs[ts]);
+        }
+    }
+}
+
+/* row 0 of the hat: every particle starts at x0 */
+template <class M, typename R>
+__global__ void k_hat0(plom_dfilter *f)
+{
+    if (threadIdx.x != 0 || !f->hat) return;
+    const R *x0 = (const R *)f->x0;
+    double xs[M::S], v[M::N_HAT];
+#pragma unroll
+    for (int k = 0; k < M::S; k++) xs[k] = (double)x0[k];
+    M::hat_values(xs, f->pv, cov_at(f, 0), f->drift_on ? &f->dv : nullptr, f->times[0], v);
+    for (int k = 0; k < M::N_HAT; k++) f->hat[3 * k] = f->hat[3 * k + 1] = f->hat[3 * k + 2] = v[k];
+}
+
+/* ------------------------------------------------------------------ model table */
+/* What a model's translation unit exports: its dimensions and the host launchers of its
+ * kernels.  abi and dfilter_bytes let the library refuse a model library compiled against
+ * another version of this header (plom_model_load). */
+#define PLOM_MODEL_ABI 2
+struct model_entry {
+    int32_t abi;            /* PLOM_MODEL_ABI */
+    int32_t dfilter_bytes;  /* sizeof(plom_dfilter) */
+    const char *name;
+    plom_model_info info;
+    void (*launch_prop)(int prec, int variant, int mif, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles);
+    void (*launch_lineage)(int prec, int variant, hipStream_t s, plom_dfilter *df);
+    void (*launch_hat)(int prec, hipStream_t s, plom_dfilter *df, int ntiles, int n_hat);
+    void (*launch_hat0)(int prec, hipStream_t s, plom_dfilter *df);
+    /* host builds of the generated observation functions (stored clouds, read on the host) */
+    void (*proj2obs_h)(const double *X, const double *pvd, const double *cvd, double *obs);
+    void (*moments_h)(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var);
+};
+
+
+/* variants 0-3: SDE noise switches; 4, 5: psr with / without white noise (FP64 only) */
+template <class M, typename R, bool MIF>
+static void launch_prop_r(int variant, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles)
+{
+    if constexpr (sizeof(R) == 8) {
+        if (variant == 4) { hipLaunchKernelGGL((k_propagate<M, R, 4, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); return; }
+        if (variant == 5) { hipLaunchKernelGGL((k_propagate<M, R, 5, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); return; }
+        if (variant == 6) { hipLaunchKernelGGL((k_propagate<M, R, 6, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); return; }
+    }
+    switch (variant) {
+    case 0: hipLaunchKernelGGL((k_propagate<M, R, 0, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
+    case 1: hipLaunchKernelGGL((k_propagate<M, R, 1, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
+    case 2: hipLaunchKernelGGL((k_propagate<M, R, 2, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
+    default: hipLaunchKernelGGL((k_propagate<M, R, 3, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
+    }
+}
+
+template <class M>
+static void launch_prop(int prec, int variant, int mif, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles)
+{
+    if (prec == PLOM_FP32) {
+        if (mif) launch_prop_r<M, float, true>(variant, nblocks, s, df, tiles);
+        else launch_prop_r<M, float, false>(variant, nblocks, s, df, tiles);
+    } else {
+        if (mif) launch_prop_r<M, double, true>(variant, nblocks, s, df, tiles);
+        else launch_prop_r<M, double, false>(variant, nblocks, s, df, tiles);
+    }
+}
+
+template <class M, typename R>
+static void launch_lineage_r(int variant, hipStream_t s, plom_dfilter *df)
+{
+    if constexpr (sizeof(R) == 8) {
+        if (variant == 4) { hipLaunchKernelGGL((k_lineage<M, R, 4>), dim3(1), dim3(64), 0, s, df); return; }
+        if (variant == 5) { hipLaunchKernelGGL((k_lineage<M, R, 5>), dim3(1), dim3(64), 0, s, df); return; }
+        if (variant == 6) { hipLaunchKernelGGL((k_lineage<M, R, 6>), dim3(1), dim3(64), 0, s, df); return; }
+    }
+    switch (variant) {
+    case 0: hipLaunchKernelGGL((k_lineage<M, R, 0>), dim3(1), dim3(64), 0, s, df); break;
+    case 1: hipLaunchKernelGGL((k_lineage<M, R, 1>), dim3(1), dim3(64), 0, s, df); break;
+    case 2: hipLaunchKernelGGL((k_lineage<M, R, 2>), dim3(1), dim3(64), 0, s, df); break;
+    default: hipLaunchKernelGGL((k_lineage<M, R, 3>), dim3(1), dim3(64), 0, s, df); break;
+    }
+}
+
+template <class M>
+static void launch_lineage(int prec, int variant, hipStream_t s, plom_dfilter *df)
+{
+    if (prec == PLOM_FP32) launch_lineage_r<M, float>(variant, s, df);
+    else launch_lineage_r<M, double>(variant, s, df);
+}
+
+/* the per-tile hat sort and the residuals (the library then launches k_hat_select) */
+template <class M>
+static void launch_hat(int prec, hipStream_t s, plom_dfilter *df, int ntiles, int n_hat)
+{
+    if (prec == PLOM_FP32) {
+        hipLaunchKernelGGL((k_hat_tiles<M, float>), dim3(ntiles), dim3(TILE), 0, s, df);
+        hipLaunchKernelGGL((k_pred_res<M, float>), dim3(1), dim3(TILE), 0, s, df);
+    } else {
+        hipLaunchKernelGGL((k_hat_tiles<M, double>), dim3(ntiles), dim3(TILE), 0, s, df);
+        hipLaunchKernelGGL((k_pred_res<M, double>), dim3(1), dim3(TILE), 0, s, df);
+    }
+    (void)ntiles; (void)n_hat;
+}
+
+template <class M>
+static void launch_hat0(int prec, hipStream_t s, plom_dfilter *df)
+{
+    if (prec == PLOM_FP32) hipLaunchKernelGGL((k_hat0<M, float>), dim3(1), dim3(64), 0, s, df);
+    else hipLaunchKernelGGL((k_hat0<M, double>), dim3(1), dim3(64), 0, s, df);
+}
+
+#define PLOM_CAT_(a, b) a##b
+#define PLOM_CAT(a, b) PLOM_CAT_(a, b)
+#define PLOM_STR_(a) #a
+#define PLOM_STR(a) PLOM_STR_(a)
+
+/* the exported entry of model `lname` (generated constants UNAME_*, generated struct
+ * plom_model_<lname>, host functions plom_<lname>_*) */
+#define PLOM_DEFINE_MODEL(lname, UNAME)                                                              \
+    extern "C" const model_entry *PLOM_CAT(plom_model_entry_, lname)(void)                           \
+    {                                                                                                \
+        static const model_entry e = {                                                               \
+            PLOM_MODEL_ABI, (int32_t)sizeof(plom_dfilter), PLOM_STR(lname),                          \
+            { PLOM_CAT(UNAME, _SIZE_PROJ), PLOM_CAT(UNAME, _N_TS), PLOM_CAT(UNAME, _N_CAC),            \
+              PLOM_CAT(UNAME, _N_PAR_SV), PLOM_CAT(UNAME, _INC_OFFSET), PLOM_CAT(UNAME, _N_TS_INC_UNIQUE), \
+              PLOM_CAT(UNAME, _PV_SIZE), PLOM_CAT(UNAME, _N_DEM), PLOM_CAT(UNAME, _N_ENV),             \
+              PLOM_CAT(UNAME, _N_HAT), PLOM_CAT(UNAME, _N_PP), PLOM_CAT(UNAME, _FIX_OFF),              \
+              PLOM_CAT(UNAME, _N_FIX), PLOM_CAT(UNAME, _N_DRIFT), PLOM_CAT(UNAME, _DRIFT_OFFSET) },     \
+            launch_prop<PLOM_CAT(plom_model_, lname)>, launch_lineage<PLOM_CAT(plom_model_, lname)>,     \
+            launch_hat<PLOM_CAT(plom_model_, lname)>, launch_hat0<PLOM_CAT(plom_model_, lname)>,         \
+            PLOM_CAT(PLOM_CAT(plom_, lname), _proj2obs_f64), PLOM_CAT(PLOM_CAT(plom_, lname), _obs_moments) }; \
+        return &e;                                                                                   \
+    }
+
+#endif /* PLOM_DEVICE_H */

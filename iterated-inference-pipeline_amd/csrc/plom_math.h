@@ -71,6 +71,31 @@ PLOM_HD double plom_sqrt_mid(double x)
 #else
 PLOM_HD double plom_sqrt_mid(double x) { return __builtin_sqrt(x); }
 #endif
+
+/* The square root of a demographic noise term, sqrt(rate x state) of the Euler-Maruyama
+ * step (core_template.c:292-293), as this build defines it on both sides:
+ *   0          for +0 <= x < 2^-767 (a bit-pattern compare: the +0 and tiny range),
+ *   sqrt(x)    (IEEE) for 2^-767 <= x < +inf,
+ *   NaN        otherwise (x < 0, -0, +inf, NaN).
+ * It differs from IEEE sqrt only below 2^-767 (where sqrt is < 2^-383, far below an ulp of
+ * any state it is added to) and at -0 and +inf.  The device computes the Newton core of
+ * plom_sqrt_mid, exact on [2^-767, +inf) and NaN on the last class, and selects 0 for the
+ * first, so the range scaling of the IEEE expansion is not needed. */
+#define PLOM_SQRT_RATE_LO 0x1000000000000000ULL /* the bits of 2^-767 */
+#if defined(__HIP_DEVICE_COMPILE__)
+PLOM_HD double plom_sqrt_rate(double x)
+{
+    const double r = plom_sqrt_mid(x);
+    return plom_d2u(x) < PLOM_SQRT_RATE_LO ? 0.0 : r;
+}
+#else
+PLOM_HD double plom_sqrt_rate(double x)
+{
+    if (plom_d2u(x) < PLOM_SQRT_RATE_LO) return 0.0;
+    if (x > 0.0 && x < PLOM_INF) return __builtin_sqrt(x);
+    return PLOM_NAN;
+}
+#endif
 PLOM_HD double plom_fabs(double x) { return plom_u2d(plom_d2u(x) & 0x7fffffffffffffffULL); }
 PLOM_HD int plom_isnan(double x) { return x != x; }
 /* 2^k for k in [-1022, 1023] */

@@ -22,18 +22,22 @@
  *   PURPOSE_OBS gives the Gaussians of the observations drawn along a lineage.
  *
  * Uniforms are (x + 1/2) 2^-32 for a 32-bit x: never 0 or 1, exact in double.
- * Gaussians: Box-Muller pairs on two uniforms; the generated step code takes the two
- * values of a pair in draw order and drops the second value of a pair left open at the
- * end of an Euler substep, so the pairing is fixed at code-generation time.  FP64 works
- * on the two 32-bit words with small lookup tables (plom_bm_m2log, plom_bm_sincos: no
- * division, no branch) and IEEE sqrt; the FP32 stream (device only) uses the gfx950
- * transcendental instructions.
+ * Gaussians of the stream (the Euler noise): one per 32-bit word, the inverse normal CDF
+ * of the word's uniform, evaluated on piecewise polynomials (plom_icdf_normal, table
+ * plom_icdf_table.h: no division, no branch, no transcendental); the generated step code
+ * draws them two at a time in draw order and drops the second value of a pair left open at
+ * the end of an Euler substep, so the pairing is fixed at code-generation time.  The
+ * Gaussians drawn from Philox words outside the stream (the MIF random walk, the drawn
+ * observations of a lineage) are Box-Muller pairs on two words with small lookup tables
+ * (plom_bm_m2log, plom_bm_sincos) and IEEE sqrt.  The FP32 stream (device only) uses the
+ * gfx950 transcendental instructions.
  */
 #ifndef PLOM_RNG_H
 #define PLOM_RNG_H
 
 #include "plom_math.h"
 #include "plom_bm_tables.h"
+#include "plom_icdf_table.h"
 
 #define PLOM_PURPOSE_PROPAGATE 0u
 #define PLOM_PURPOSE_RESAMPLE 1u
@@ -101,6 +105,16 @@ static __constant__ const double plom_bm_tab[PLOM_BM_TAB_SIZE] = PLOM_BM_TAB_VAL
 static const double plom_bm_tab[PLOM_BM_TAB_SIZE] = PLOM_BM_TAB_VALUES;
 #endif
 
+/* The coefficient table of the stream's Gaussian transform (gen_icdf_table.py); kernels
+ * copy it to LDS and hand the stream a pointer to that copy. */
+#if defined(__HIPCC__)
+static __constant__ const double plom_icdf_tab[PLOM_ICDF_TAB_SIZE] = PLOM_ICDF_TAB_VALUES;
+#else
+static const double plom_icdf_tab[PLOM_ICDF_TAB_SIZE] = PLOM_ICDF_TAB_VALUES;
+#endif
+
+/* A particle's stream over one interval: its xoshiro128** state and the inverse-CDF
+ * table (plom_icdf_tab or a copy of it). */
 typedef struct { plom_xoshiro128 r; const double *tab; } plom_nstream64;
 
 PLOM_HD void plom_nstream64_init(plom_nstream64 *ns, uint32_t seed, uint32_t stream, uint32_t n, uint32_t j,
@@ -166,11 +180,49 @@ PLOM_HD void plom_bm_pair(uint32_t x, uint32_t y, const double *tab, double *z0,
     *z1 = rad * s;
 }
 
+/* The Gaussian of one 32-bit word w: v = w >> 1, u = (v + 1/2) 2^-32 in [2^-33, 1/2)
+ * (exact), z = Phi^-1(u) from the polynomial of u's segment (binary octave of u and the
+ * top PLOM_ICDF_SUB_BITS fraction bits: the top bits of the double's high word) at the
+ * local coordinate t = 32 (1 + r) - 33 in [-1, 1) (r the remaining fraction, t exact),
+ * and the sign flipped when w is odd.  About 2e-14 from Phi^-1(u) (checked against mpmath
+ * by the tests); the mirror image of the low bit makes z symmetric exactly. */
+#if PLOM_ICDF_DEG != 6
+#error "plom_icdf_normal evaluates a degree-6 table"
+#endif
+#define PLOM_ICDF_T_SCALE ((double)(2 << PLOM_ICDF_SUB_BITS))
+PLOM_HD double plom_icdf_normal(uint32_t w, const double *tab)
+{
+    /* (v + 1/2) 2^-32 = (2v + 1) 2^-33 = (w | 1) 2^-33 */
+    const double u = (double)(w | 1u) * 0x1p-33;
+    const uint64_t b = plom_d2u(u);
+    const uint32_t hi = (uint32_t)(b >> 32);
+    /* the segment (E + 33) 16 + s, taken modulo the table size: it is in [0, NSEG) anyway,
+     * and the mask lets the compiler fold the coefficients' offsets into the LDS loads */
+    const uint32_t idx = ((hi >> (20 - PLOM_ICDF_SUB_BITS)) + (uint32_t)PLOM_ICDF_NSEG -
+                          (((uint32_t)PLOM_ICDF_EB_MIN << PLOM_ICDF_SUB_BITS) & (PLOM_ICDF_NSEG - 1u))) &
+                         (PLOM_ICDF_NSEG - 1u);
+    const uint32_t keep = (1u << (20 - PLOM_ICDF_SUB_BITS)) - 1u;
+    const double m = plom_u2d(((uint64_t)((hi & keep) | 0x3ff00000u) << 32) | (b & 0xffffffffULL));
+    /* t = 32 (m - 1) - 1, exact: m - 1 is exact (Sterbenz) and so is the rest */
+    const double t = plom_fma(m - 1.0, PLOM_ICDF_T_SCALE, -1.0);
+    const double *c = tab + idx;
+    double p = c[6 * PLOM_ICDF_NSEG];
+    p = plom_fma(p, t, c[5 * PLOM_ICDF_NSEG]);
+    p = plom_fma(p, t, c[4 * PLOM_ICDF_NSEG]);
+    p = plom_fma(p, t, c[3 * PLOM_ICDF_NSEG]);
+    p = plom_fma(p, t, c[2 * PLOM_ICDF_NSEG]);
+    p = plom_fma(p, t, c[1 * PLOM_ICDF_NSEG]);
+    p = plom_fma(p, t, c[0]);
+    return plom_u2d(plom_d2u(p) ^ ((uint64_t)(w & 1u) << 63));
+}
+
+/* the next two Gaussians of the stream, in draw order */
 PLOM_HD void plom_normal_pair64(plom_nstream64 *ns, double *z0, double *z1)
 {
     const uint32_t x = plom_xoshiro_next(&ns->r);
     const uint32_t y = plom_xoshiro_next(&ns->r);
-    plom_bm_pair(x, y, ns->tab, z0, z1);
+    *z0 = plom_icdf_normal(x, ns->tab);
+    *z1 = plom_icdf_normal(y, ns->tab);
 }
 
 /* the uniform that picks the final particle of a lineage, as the numerator 2x+1 of
@@ -221,30 +273,51 @@ PLOM_HD uint64_t plom_resample_num(uint32_t seed, uint32_t stream, uint32_t n)
 
 /* ---------------------------------------------------------------- FP32 Gaussians */
 #if defined(__HIPCC__)
-typedef plom_xoshiro128 plom_nstream32;
+/* A particle's FP32 stream: the same xoshiro128** words as the FP64 stream and the same
+ * inverse-CDF transform in FP32 (the table's coefficients rounded to float), so that an
+ * FP32 filter draws the FP64 filter's Gaussians rounded to FP32. */
+typedef struct { plom_xoshiro128 r; const float *tab; } plom_nstream32;
 
 __device__ inline __attribute__((always_inline)) void
-plom_nstream32_init(plom_nstream32 *ns, uint32_t seed, uint32_t stream, uint32_t n, uint32_t j)
+plom_nstream32_init(plom_nstream32 *ns, uint32_t seed, uint32_t stream, uint32_t n, uint32_t j, const float *tab)
 {
     uint32_t w[4];
     plom_draw128(seed, stream, n, j, PLOM_PURPOSE_PROPAGATE, w);
-    plom_xoshiro_seed(ns, w);
+    plom_xoshiro_seed(&ns->r, w);
+    ns->tab = tab;
 }
 
-/* u in (0,1) from the top 24 bits: exact in float */
-__device__ inline __attribute__((always_inline)) float plom_u01f(uint32_t x)
+/* plom_icdf_normal in FP32 on the top 24 bits of the word: u = ((w >> 9) + 1/2) 2^-24 =
+ * ((w >> 8) | 1) 2^-25 (exact in float, within 2^-25 of the FP64 transform's u, in
+ * [2^-25, 1/2)), the same segments (the float's exponent and top 4 fraction bits) and the
+ * same sign bit w & 1 */
+#define PLOM_ICDF_EBF_MIN (PLOM_ICDF_EB_MIN - 1023 + 127)   /* float's biased exponent of 2^-33 */
+__device__ inline __attribute__((always_inline)) float plom_icdf_normal32(uint32_t w, const float *tab)
 {
-    return ((float)(x >> 8) + 0.5f) * 0x1p-24f;
+    const float u = (float)((w >> 8) | 1u) * 0x1p-25f;
+    const uint32_t b = __float_as_uint(u);
+    const uint32_t idx = ((b >> (23 - PLOM_ICDF_SUB_BITS)) + (uint32_t)PLOM_ICDF_NSEG -
+                          (((uint32_t)PLOM_ICDF_EBF_MIN << PLOM_ICDF_SUB_BITS) & (PLOM_ICDF_NSEG - 1u))) &
+                         (PLOM_ICDF_NSEG - 1u);
+    const float m = __uint_as_float((b & ((1u << (23 - PLOM_ICDF_SUB_BITS)) - 1u)) | 0x3f800000u);
+    const float t = __builtin_fmaf(m - 1.0f, (float)PLOM_ICDF_T_SCALE, -1.0f);
+    const float *c = tab + idx;
+    float p = c[6 * PLOM_ICDF_NSEG];
+    p = __builtin_fmaf(p, t, c[5 * PLOM_ICDF_NSEG]);
+    p = __builtin_fmaf(p, t, c[4 * PLOM_ICDF_NSEG]);
+    p = __builtin_fmaf(p, t, c[3 * PLOM_ICDF_NSEG]);
+    p = __builtin_fmaf(p, t, c[2 * PLOM_ICDF_NSEG]);
+    p = __builtin_fmaf(p, t, c[1 * PLOM_ICDF_NSEG]);
+    p = __builtin_fmaf(p, t, c[0]);
+    return __uint_as_float(__float_as_uint(p) ^ ((w & 1u) << 31));
 }
 
 __device__ inline __attribute__((always_inline)) void plom_normal_pair32(plom_nstream32 *ns, float *z0, float *z1)
 {
-    float u1 = plom_u01f(plom_xoshiro_next(ns));
-    float u2 = plom_u01f(plom_xoshiro_next(ns));
-    /* -2 ln u1 = -2 ln2 log2(u1); v_log_f32 is log2, v_sin/v_cos_f32 take turns */
-    float rad = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
-    *z0 = rad * __builtin_amdgcn_cosf(u2);
-    *z1 = rad * __builtin_amdgcn_sinf(u2);
+    const uint32_t x = plom_xoshiro_next(&ns->r);
+    const uint32_t y = plom_xoshiro_next(&ns->r);
+    *z0 = plom_icdf_normal32(x, ns->tab);
+    *z1 = plom_icdf_normal32(y, ns->tab);
 }
 #endif
 

@@ -1,24 +1,21 @@
 /*
- * plom_smc.hip -- the PLOM SMC particle filter on MI355X (gfx950): kernels + C ABI.
+ * plom_smc.hip -- the PLOM SMC particle filter on MI355X (gfx950): the model-independent
+ * kernels and the C ABI (filters, batches, the server).
  *
- * One observation interval n of one filter is two kernels over tiles of 256 particles
- * (one particle per lane, 4 waves per tile):
+ * One observation interval n of a filter (or of every filter of a batch, over a tile map
+ * (filter, tile)) is four stream-ordered launches over tiles of 256 particles:
  *
- *   k_propagate<Model, Real, Variant>
- *       select:   ancestor a_j of particle j from the previous interval's fixed-point
- *                 cumulative weights (systematic resampling, reference smc.c:90-112),
- *       gather:   state of a_j from the SoA cloud [S][Jp] (resample_X, smc.c:142-161),
- *       propagate K Euler-Maruyama substeps in registers (generated step, core_template.c
- *                 :215-339, driven as in prediction_util.c:275-287),
- *       weight:   log-weight l_j = sum_ts log like_ts (likelihood.c:56-92),
- *       tile:     max m_t, tree sums of exp(l - m_t) and its square;
- *       the block that completes the filter's last tile reduces the tiles to the global
- *       max m, S = sum exp(l - m), S2, and writes loglik_n = m + log(S/J), ESS = S^2/S2,
- *       the all-fail branch (smc.c:55-70) and loglik += loglik_n (smc.c:80).
- *   k_scan
- *       q_j = trunc(exp(l_j - m)/S * 2^62), tile-local inclusive integer scan, tile totals;
- *       the block completing the last tile scans the tile totals and advances the
- *       filter's interval counter.
+ *   k_propagate<Model, Real, Variant, MIF>   (plom_device.h, one translation unit per model)
+ *       select the ancestor of particle j from the previous interval's fixed-point
+ *       cumulative weights (systematic resampling, smc.c:90-112), gather its state from the
+ *       SoA cloud [S][Jp] (resample_X, smc.c:142-161), run the interval's Euler-Maruyama
+ *       substeps in registers (generated step, core_template.c:215-339, driven as in
+ *       prediction_util.c:275-287), the log-weight l_j (likelihood.c:56-92), and per tile
+ *       the max m_t and the tree sums of exp(l - m_t) and of its square;
+ *   k_weights   one block per filter: the tiles reduced to m, S, S2; loglik_n = m +
+ *               log(S/J), ESS = S^2/S2, the all-fail branch (smc.c:27-83);
+ *   k_scan      q_j = trunc(exp(l_j - m)/S 2^62), inclusive integer scan within the tile;
+ *   k_advance   one block per filter: inclusive prefix of the tile totals, interval + 1.
  *
  * Canonical arithmetic shared with the oracle (oracle/smc_oracle.c): the sums over 256
  * particles and over tiles are the recursive contiguous pairwise tree (zero padded to a
@@ -26,9 +23,8 @@
  * 128-bit integers, C the integer prefix of q, Q its total, u the interval's 32-bit
  * uniform; select[k] is the first i with C_i J 2^33 >= (k 2^33 + 2u + 1) Q.
  *
- * A batch of filters runs the same kernels over a tile map (filter, tile); every filter
- * keeps its own interval counter in device memory, so a launch sequence can be captured
- * in a hipGraph and replayed.
+ * Every filter keeps its interval counter in device memory, so a launch sequence can be
+ * captured in a hipGraph and replayed.
  */
 #include <hip/hip_runtime.h>
 #include <stdio.h>
@@ -39,19 +35,12 @@
 #include <algorithm>
 #include <new>
 
-#include "plom_smc.h"
-#include "plom_math.h"
-#include "plom_cov.h"
-#include "plom_ode.h"
-#include "plom_rng.h"
-#include "plom_mif.h"
-#include "plom_models.h"
+#include <dlfcn.h>
+#include <atomic>
+#include <mutex>
 
-typedef unsigned __int128 plom_u128;
-
-#define TILE 256
-#define PLOM_TF_CAP 512   /* LDS entries of the per-interval time-function table */
-#define MAX_TILES (PLOM_MAX_J / TILE)
+#include "plom_device.h"
+#include "plom_model_list.h"
 
 /* ------------------------------------------------------------------ errors */
 static thread_local char g_err[512];
@@ -74,150 +63,6 @@ static int set_err(int code, const char *fmt, ...)
     } while (0)
 
 extern "C" const char *plom_last_error(void) { return g_err; }
-
-/* ------------------------------------------------------------------ device filter */
-struct plom_dfilter {
-    int32_t J, Jp, T, N, S, n_ts, flags, step;
-    uint32_t seed, stream, prec, pad;
-    double dt, sqdt, like_min, all_fail_llike;
-    double eps_abs, eps_rel;   /* PLOM_FLAG_ODE: error control of the RKF45 integration */
-    void *X[2];
-    const void *x0;
-    const double *pv;
-    const float *pvf;
-    const double *cov;  /* covariate table [N kmax + N + 1][cov_cs] (plom_cov.h), or the covariate
-                           slots of pv with cov_cs = 0 (constant covariates) */
-    int32_t cov_cs, cov_kmax;
-    plom_drift_view dv; /* drifting parameters (used when drift_on) */
-    int32_t drift_on, pad3;
-    const double *times, *data;
-    const int32_t *nonan;
-    double *logw, *tmax, *tsum, *tsum2;
-    uint64_t *cum, *ttot, *tincl;
-    uint32_t *cnt;      /* [0] propagate arrivals, [1] scan arrivals */
-    double *scal;       /* m, S, S2, loglik */
-    int32_t *n_all_fail;
-    double *llike_n, *ess_n;
-    int32_t *rflag;     /* rflag[n] = 1 when interval n ended with a resampling */
-    int32_t *ancestry;  /* [N][J] or null */
-    int32_t *lin_slots; /* [N] or null: the sampled lineage's particle index per interval */
-    double *lin_state;  /* [(N+1)][S] or null: its states */
-    double *lin_obs;    /* [(N+1)][n_ts][2] or null: its observation mean and one drawn observation */
-    int32_t n_hat, pad2;
-    uint64_t *hkey;     /* [n_hat][Jp] or null: per tile, the sorted keys of the hat quantities */
-    uint64_t *hpref;    /* [n_hat][Jp]: inclusive prefix of their weights in that order */
-    double *hpart;      /* [n_hat][T]: per tile, tree sum of value x weight */
-    double *hat;        /* [(N+1)][n_hat][3]: low95, mean, high95 */
-    double *pres;       /* [N][n_ts][2]: prediction mean and residual */
-    /* PLOM_FLAG_MIF: per-particle parameters */
-    int32_t n_theta, n_pp;
-    const int32_t *pm_src;          /* [n_pp] component of each per-particle pv slot, -1 = pv */
-    const int32_t *pm_tr, *pm_sc, *pm_dur;      /* [n_theta] */
-    const double *pm_min, *pm_max, *pm_mult, *pm_sd;   /* [n_theta] */
-    const double *mut_fac;          /* [N] random-walk factor after interval n (with data) */
-    const double *pm_chol;          /* [n_theta][n_theta] lower factor of the walk's covariance, or null */
-    const int32_t *pm_prior, *pm_puntil;   /* [n_theta] prior patch of the weights, or null */
-    const double *pm_ppow;                 /* [n_theta] */
-    double *th[2];                  /* [n_theta][Jp] theta of the particles, ping-pong like X */
-    double *mpw, *mps, *mpm, *mpq;  /* tile partials: [T] sum w, [n_theta][T] sum w theta, mean, M2 */
-    double *mmean, *mvar;           /* [(N+1)][n_theta] weighted mean and variance of theta */
-    void *allX;                     /* PLOM_FLAG_ALL_STATES: [(N+1)][S][Jp] every propagated cloud */
-    /* the ode implementation with time-varying covariates: their splines, evaluated at the
-     * integrator's times (spl_on = 0: the covariates are constants) */
-    plom_spline_set spl;
-    int32_t spl_on, pad_spl;
-};
-
-/* ------------------------------------------------------------------ covariate rows */
-/* the rows of interval n's substeps (the same in every lane: moved to scalar registers so
- * that the generated code reads the values with scalar loads) */
-__device__ inline const double *cov_steps(const plom_dfilter *f, int n)
-{
-    const uint64_t u = (uint64_t)(f->cov + (size_t)n * (size_t)f->cov_kmax * (size_t)f->cov_cs);
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
-    return (const double *)(((uint64_t)hi << 32) | lo);
-}
-
-/* the row of observation time times[r], 0 <= r <= N */
-__device__ inline const double *cov_at(const plom_dfilter *f, int r)
-{
-    return f->cov + ((size_t)f->N * (size_t)f->cov_kmax + (size_t)r) * (size_t)f->cov_cs;
-}
-
-/* ------------------------------------------------------------------ block helpers */
-/* recursive contiguous pairwise sum of the 256 values of a block (every lane gets it) */
-__device__ inline double tree256(double v, double *lds)
-{
-#pragma unroll
-    for (int h = 1; h < 64; h <<= 1) v = v + __shfl_xor(v, h, 64);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) lds[w] = v;
-    __syncthreads();
-    double r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
-    __syncthreads();
-    return r;
-}
-
-__device__ inline double max256(double v, double *lds)
-{
-#pragma unroll
-    for (int h = 1; h < 64; h <<= 1) v = fmax(v, __shfl_xor(v, h, 64));
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) lds[w] = v;
-    __syncthreads();
-    double r = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
-    __syncthreads();
-    return r;
-}
-
-/* inclusive scan of 256 uint64 values (exact: any order gives the same integers) */
-__device__ inline uint64_t scan256(uint64_t v, uint64_t *lds)
-{
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int h = 1; h < 64; h <<= 1) {
-        uint64_t o = __shfl_up(v, h, 64);
-        if (lane >= h) v += o;
-    }
-    if (lane == 63) lds[w] = v;
-    __syncthreads();
-    uint64_t off = 0;
-    for (int k = 0; k < w; k++) off += lds[k];
-    __syncthreads();
-    return v + off;
-}
-
-/* first particle i with C_i * J2 >= Rk in the fixed-point cumulative weights of the
- * interval whose scan is in f->cum / f->tincl */
-__device__ inline int select_first(const plom_dfilter *f, plom_u128 Rk, uint64_t J2)
-{
-    const int T = f->T;
-    const uint64_t *tincl = f->tincl;
-    int lo = 0, hi = T - 1;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if ((plom_u128)tincl[mid] * J2 >= Rk) hi = mid; else lo = mid + 1;
-    }
-    const uint64_t P = lo ? tincl[lo - 1] : 0ull;
-    const uint64_t *c = f->cum + (size_t)lo * TILE;
-    int a = 0, b = TILE - 1;
-    while (a < b) {
-        int mid = (a + b) >> 1;
-        if ((plom_u128)(P + c[mid]) * J2 >= Rk) b = mid; else a = mid + 1;
-    }
-    return lo * TILE + a;
-}
-
-/* systematic resampling of interval n: ancestor of output slot k, the first i with
- * C_i J 2^33 >= (k 2^33 + 2x + 1) Q */
-__device__ inline int select_ancestor(const plom_dfilter *f, int k, int n)
-{
-    const uint64_t Q = f->tincl[f->T - 1];
-    const uint64_t unum = plom_resample_num(f->seed, f->stream, (uint32_t)n);
-    const plom_u128 Rk = (plom_u128)(((uint64_t)k << 33) + unum) * (plom_u128)Q;
-    return select_first(f, Rk, (uint64_t)f->J << 33);
-}
 
 /* global reduction of the tile statistics by the last block of interval n */
 __device__ void finalize_weights(plom_dfilter *f, int n, double *lds)
@@ -279,203 +124,6 @@ __device__ void finalize_weights(plom_dfilter *f, int n, double *lds)
     }
 }
 
-/* ------------------------------------------------------------------ kernels */
-/* PLOM_FLAG_MIF: theta of particle j from its ancestor a, moved by the random walk when
- * interval n-1 had data (resample_and_mut_theta_mif, C/mif/methods.c:261-367: every
- * component with a jump size moves by mut_fac[n-1] * sd * N(0,1)); written to the output
- * table; then the per-particle slots of pv from it (back_transform_theta2par) */
-template <int NPP>
-__device__ inline void mif_particle(const plom_dfilter *f, int n, int j, int a, const double *tab, double *lp)
-{
-    const int Jp = f->Jp, nth = f->n_theta;
-    const double *Tin = f->th[n & 1];
-    double *Tout = f->th[(n + 1) & 1];
-    const double fac = (n > 0 && f->nonan[n - 1] > 0) ? f->mut_fac[n - 1] : 0.0;
-    if (f->pm_chol) {
-        /* correlated walk: theta_o += fac sum_{q<=o} L[o][q] z_q, z from the same counters */
-        double z[PLOM_MIF_MVN_MAX];
-        for (int o = 0; o < nth; o += 2) {
-            double z0 = 0.0, z1 = 0.0;
-            if (fac != 0.0) plom_mif_normals(f->seed, f->stream, (uint32_t)n, (uint32_t)j, (uint32_t)(o >> 1), tab, &z0, &z1);
-            z[o] = z0;
-            if (o + 1 < nth) z[o + 1] = z1;
-        }
-        for (int o = 0; o < nth; o++) {
-            double v = Tin[(size_t)o * Jp + a];
-            if (fac != 0.0) {
-                double acc = 0.0;
-                for (int q = 0; q <= o; q++) acc = acc + f->pm_chol[(size_t)o * nth + q] * z[q];
-                v = v + fac * acc;
-            }
-            Tout[(size_t)o * Jp + j] = v;
-        }
-    } else
-    for (int o = 0; o < nth; o += 2) {
-        const double sd0 = fac * f->pm_sd[o];
-        const double sd1 = (o + 1 < nth) ? fac * f->pm_sd[o + 1] : 0.0;
-        double z0 = 0.0, z1 = 0.0;
-        if (sd0 != 0.0 || sd1 != 0.0) plom_mif_normals(f->seed, f->stream, (uint32_t)n, (uint32_t)j, (uint32_t)(o >> 1), tab, &z0, &z1);
-        double v = Tin[(size_t)o * Jp + a];
-        if (sd0 != 0.0) v = v + sd0 * z0;
-        Tout[(size_t)o * Jp + j] = v;
-        if (o + 1 < nth) {
-            double w = Tin[(size_t)(o + 1) * Jp + a];
-            if (sd1 != 0.0) w = w + sd1 * z1;
-            Tout[(size_t)(o + 1) * Jp + j] = w;
-        }
-    }
-#pragma unroll
-    for (int s = 0; s < NPP; s++) {
-        const int o = f->pm_src[s];
-        lp[s] = o < 0 ? f->pv[s]
-                      : plom_mif_natural(f->pm_tr[o], f->pm_sc[o], f->pm_dur[o], Tout[(size_t)o * Jp + j], f->pm_min[o],
-                                         f->pm_max[o], f->pm_mult[o]);
-    }
-}
-
-template <class M, typename R, int V, bool MIF>
-__global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const int2 *tiles)
-{
-    __shared__ double lds[8];
-    int fi = 0, tile = blockIdx.x;
-    if (tiles) { const int2 tt = tiles[blockIdx.x]; fi = tt.x; tile = tt.y; }
-    plom_dfilter *f = &filters[fi];
-    const int n = f->step;
-    if (n >= f->N) return;
-    /* the Box-Muller tables in LDS (FP64 only; the block returns above as a whole) */
-    __shared__ double s_bm[PLOM_BM_TAB_SIZE];
-    if constexpr (sizeof(R) == 8) {
-        for (int i = threadIdx.x; i < PLOM_BM_TAB_SIZE; i += TILE) s_bm[i] = plom_bm_tab[i];
-        __syncthreads();
-    }
-    const int J = f->J, Jp = f->Jp;
-    const int j = tile * TILE + threadIdx.x;
-    const bool valid = j < J;
-
-    R X[M::S];
-    int a = j;
-    if (n == 0 && !MIF) {
-        const R *x0 = (const R *)f->x0;
-#pragma unroll
-        for (int k = 0; k < M::S; k++) X[k] = x0[k];
-    } else {
-        /* (with MIF, interval 0 starts from the particles' own initial states in X[0]) */
-        if (n > 0) {
-            if (valid && f->rflag[n - 1]) a = select_ancestor(f, j, n - 1);
-            if (valid && f->ancestry) f->ancestry[(size_t)(n - 1) * J + j] = a;
-        }
-        const R *Xin = (const R *)f->X[n & 1];
-        if (!valid) a = 0;
-#pragma unroll
-        for (int k = 0; k < M::S; k++) X[k] = Xin[(size_t)k * Jp + a];
-    }
-    if (n == 0 && f->allX && valid) {
-        R *A = (R *)f->allX;
-#pragma unroll
-        for (int k = 0; k < M::S; k++) A[(size_t)k * Jp + j] = X[k];
-    }
-    /* per-particle slots of pv (MIF only; the array is unused otherwise) */
-    double lp[MIF ? M::NPP : 1];
-    if constexpr (MIF) {
-        if (valid) {
-            if constexpr (sizeof(R) == 8) mif_particle<M::NPP>(f, n, j, a, s_bm, lp);
-            else mif_particle<M::NPP>(f, n, j, a, plom_bm_tab, lp);
-        }
-    }
-
-    const double t0 = f->times[n], t1 = f->times[n + 1];
-    /* the time-only functions of the interval's substeps (the seasonal forcing ...): the same
-     * for every particle of the block, so each substep time is evaluated once, by one
-     * thread, into LDS; the generated step reads the table (identical values: the same
-     * function of the same accumulated t), or evaluates them itself when they do not fit */
-    const R *tft = nullptr;
-    if constexpr (M::NTF > 0 && V != 6) {
-        __shared__ double s_tf_raw[PLOM_TF_CAP];
-        R *s_tf = (R *)s_tf_raw;
-        const int K = plom_substeps(t0, t1, f->dt);
-        if (K * M::NTF <= PLOM_TF_CAP) {
-            for (int k = threadIdx.x; k < K; k += TILE) {
-                double t = t0;
-                for (int q = 0; q < k; q++) t += f->dt;
-                M::time_funcs((R)t, s_tf + k * M::NTF);
-            }
-            __syncthreads();
-            tft = s_tf;
-        }
-    }
-    if (valid) {
-        if constexpr (V == 6) {
-            /* the ode implementation: deterministic, no random numbers */
-            const plom_spline_set *sp = f->spl_on ? &f->spl : nullptr;
-            if constexpr (MIF) M::interval_ode_pp((double *)X, f->pv, lp, f->cov, sp, t0, t1, f->dt, f->eps_abs, f->eps_rel);
-            else M::interval_ode((double *)X, f->pv, f->cov, sp, t0, t1, f->dt, f->eps_abs, f->eps_rel);
-        } else if constexpr (sizeof(R) == 8) {
-            plom_nstream64 ns;
-            plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j, s_bm);
-            if constexpr (MIF) M::template interval_pp<V>((double *)X, f->pv, lp, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs, f->drift_on ? &f->dv : nullptr, (const double *)tft);
-            else M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs, f->drift_on ? &f->dv : nullptr, (const double *)tft);
-        } else {
-            plom_nstream32 ns;
-            plom_nstream32_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j);
-            if constexpr (MIF) {
-                float lpf[M::NPP];
-#pragma unroll
-                for (int k = 0; k < M::NPP; k++) lpf[k] = (float)lp[k];
-                M::template interval_pp<V>((float *)X, f->pvf, lpf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs,
-                                           f->drift_on ? &f->dv : nullptr, (const float *)tft);
-            } else {
-                M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs,
-                                        f->drift_on ? &f->dv : nullptr, (const float *)tft);
-            }
-        }
-        R *Xout = (R *)f->X[(n + 1) & 1];
-#pragma unroll
-        for (int k = 0; k < M::S; k++) Xout[(size_t)k * Jp + j] = X[k];
-        if (f->allX) {
-            R *A = (R *)f->allX + (size_t)(n + 1) * M::S * Jp;
-#pragma unroll
-            for (int k = 0; k < M::S; k++) A[(size_t)k * Jp + j] = X[k];
-        }
-    }
-
-    if ((f->flags & PLOM_FLAG_NO_FILTER) || f->nonan[n] == 0) return;
-
-    double l = -PLOM_INF;
-    if (valid) {
-        double obs[M::N_TS];
-        int nf = 0;
-        const double *cvd = cov_at(f, n + 1);
-        if constexpr (MIF) {
-            M::proj2obs_pp(X, f->pv, lp, cvd, obs);
-            l = M::loglik_pp(obs, f->data + (size_t)n * f->n_ts, f->pv, lp, cvd, t1, f->like_min, &nf);
-        } else {
-            M::proj2obs(X, f->pv, cvd, obs);
-            l = M::loglik(obs, f->data + (size_t)n * f->n_ts, f->pv, cvd, t1, f->like_min, &nf);
-        }
-        if (MIF && f->pm_prior) {
-            /* mif --prior: the prior terms of the particle's theta for this interval (the
-             * table holds what mif_particle wrote for it), then the failure test on the
-             * patched weight */
-            const double *th = f->th[(n + 1) & 1];
-            double pt = 0.0;
-            for (int o = 0; o < f->n_theta; o++)
-                if (f->pm_prior[o] != PLOM_PRIOR_NONE && n < f->pm_puntil[o])
-                    pt = pt + plom_mif_prior_term(f->pm_prior[o], f->pm_ppow[o], f->pm_tr[o], th[(size_t)o * Jp + j],
-                                                  f->pm_min[o], f->pm_max[o], f->like_min);
-            l = plom_mif_prior_patch(l, nf == M::N_TS, pt, f->all_fail_llike);
-        } else if (nf == M::N_TS) {
-            /* failure: every series observed and floored at LIKE_MIN (smc.c:44) */
-            l = -PLOM_INF;
-        }
-    }
-    f->logw[j] = l;
-    const double mt = max256(l, lds);
-    double w = 0.0;
-    if (mt != -PLOM_INF && l != -PLOM_INF) w = plom_exp(l - mt);
-    const double s = tree256(w, lds);
-    const double s2 = tree256(w * w, lds);
-    if (threadIdx.x == 0) { f->tmax[tile] = mt; f->tsum[tile] = s; f->tsum2[tile] = s2; }
-}
 
 /* one block per filter: the tiles' statistics -> loglik_n, ESS, resampling flag */
 __global__ __launch_bounds__(TILE) void k_weights(plom_dfilter *filters, const int *fidx)
@@ -707,161 +355,6 @@ __global__ __launch_bounds__(TILE) void k_final(plom_dfilter *filters, const int
     for (int k = 0; k < S; k++) Xout[(size_t)k * Jp + j] = Xin[(size_t)k * Jp + a];
 }
 
-/* The lineage of one filter (one block of one wave; lane 0 works): the final particle by
- * inverse CDF on the last interval's cumulative weights (equal weights when that interval
- * had no data, was not filtered or every particle failed), its ancestors by the stored
- * ancestry, then its states regenerated from x0 with the same counter-based noise the
- * filter drew, interval by interval: s[N-1] = first i with C_i 2^33 >= (2x+1) Q;
- * s[n-1] = ancestry[n-1][s[n]]; state[n+1] = interval_n(state[n], noise(n, s[n])). */
-template <class M, typename R, int V>
-__global__ __launch_bounds__(64) void k_lineage(plom_dfilter *f)
-{
-    __shared__ double s_bm[PLOM_BM_TAB_SIZE];
-    if constexpr (sizeof(R) == 8) {
-        for (int i = threadIdx.x; i < PLOM_BM_TAB_SIZE; i += 64) s_bm[i] = plom_bm_tab[i];
-        __syncthreads();
-    }
-    if (threadIdx.x != 0) return;
-    const int N = f->N, J = f->J;
-    if (N == 0 || !f->lin_slots) return;
-    const uint64_t unum = plom_lineage_num(f->seed, f->stream);
-    int j;
-    if (!(f->flags & PLOM_FLAG_NO_FILTER) && f->nonan[N - 1] > 0 && f->rflag[N - 1]) {
-        const uint64_t Q = f->tincl[f->T - 1];
-        j = select_first(f, (plom_u128)unum * (plom_u128)Q, 1ull << 33);
-    } else {
-        j = (int)((unum * (uint64_t)J + (1ull << 33) - 1ull) >> 33) - 1;
-    }
-    j = j < 0 ? 0 : (j >= J ? J - 1 : j);
-    f->lin_slots[N - 1] = j;
-    for (int n = N - 1; n >= 1; n--) {
-        j = f->ancestry[(size_t)(n - 1) * J + j];
-        f->lin_slots[n - 1] = j;
-    }
-    R X[M::S];
-    const R *x0 = (const R *)f->x0;
-#pragma unroll
-    for (int k = 0; k < M::S; k++) {
-        X[k] = x0[k];
-        f->lin_state[k] = (double)X[k];
-    }
-    for (int n = 0; n < N; n++) {
-        const double t0 = f->times[n], t1 = f->times[n + 1];
-        const uint32_t jn = (uint32_t)f->lin_slots[n];
-        if constexpr (V == 6) {
-            M::interval_ode((double *)X, f->pv, f->cov, f->spl_on ? &f->spl : nullptr, t0, t1, f->dt, f->eps_abs,
-                            f->eps_rel);
-        } else if constexpr (sizeof(R) == 8) {
-            plom_nstream64 ns;
-            plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, jn, s_bm);
-            M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs,
-                                    f->drift_on ? &f->dv : nullptr, (const double *)nullptr);
-        } else {
-            plom_nstream32 ns;
-            plom_nstream32_init(&ns, f->seed, f->stream, (uint32_t)n, jn);
-            M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs,
-                                    f->drift_on ? &f->dv : nullptr, (const float *)nullptr);
-        }
-#pragma unroll
-        for (int k = 0; k < M::S; k++) f->lin_state[(size_t)(n + 1) * M::S + k] = (double)X[k];
-    }
-    /* observation mean and one drawn observation per row (print_p_X, print.c:312-341:
-     * obs_mean and observation(), core_template.c:369-412: mean + sd N(0,1), clipped at 0) */
-    for (int n = 0; n <= N; n++) {
-        double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
-#pragma unroll
-        for (int k = 0; k < M::S; k++) xs[k] = f->lin_state[(size_t)n * M::S + k];
-        M::proj2obs(xs, f->pv, cov_at(f, n), obs);
-        M::obs_moments(obs, f->pv, cov_at(f, n), f->times[n], mu, var);
-        plom_lineage_obs(f->seed, f->stream, (uint32_t)n, M::N_TS, mu, var, plom_bm_tab, f->lin_obs + (size_t)n * M::N_TS * 2);
-    }
-}
-
-/* ------------------------------------------------------------------ hat and residuals */
-/* The summaries of interval n (launched after its weights and scan, before the next
- * propagate reads the cloud).  Weights are the fixed-point integers of the resampling,
- * q_j = floor(exp(l_j - m)/S 2^62) (0 for failed particles), or 1 for every particle when
- * the interval was not weighted; Q is their total.
- *   mean     tree sum over tiles of the tile trees of value x q, divided by Q;
- *   low95    the smallest value v with 40 W(<= v) >= Q, W(<= v) the total weight of the
- *            particles whose value is at most v;
- *   high95   the smallest v with 40 W(<= v) >= 39 Q
- * (hat.c:29-55: the first sorted value at which the cumulated weight reaches 2.5 % /
- * 97.5 %; exact in integers here, and independent of the order of equal values).
- * Each tile sorts its values (bitonic, in LDS) with their weights and keeps the inclusive
- * weight prefix; one block per quantity and bound then bisects the 64-bit key space,
- * summing the tiles' prefixes exactly. */
-__device__ inline uint64_t hat_key(double x)
-{
-    const uint64_t u = plom_d2u(x);
-    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
-}
-
-__device__ inline double hat_unkey(uint64_t k)
-{
-    return plom_u2d((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k);
-}
-
-__device__ inline bool hat_weighted(const plom_dfilter *f, int n)
-{
-    return !(f->flags & PLOM_FLAG_NO_FILTER) && f->nonan[n] > 0 && f->rflag[n];
-}
-
-template <class M, typename R>
-__global__ __launch_bounds__(TILE) void k_hat_tiles(plom_dfilter *f)
-{
-    __shared__ double lds[8];
-    __shared__ uint64_t lds64[8];
-    __shared__ uint64_t s_key[TILE];
-    __shared__ uint64_t s_w[TILE];
-    const int n = f->step - 1;
-    if (n < 0 || n >= f->N || !f->hkey) return;
-    const int J = f->J, Jp = f->Jp, T = f->T, tile = blockIdx.x;
-    const int j = tile * TILE + threadIdx.x;
-    const bool valid = j < J;
-    double v[M::N_HAT];
-    uint64_t q = 0;
-    if (valid) {
-        const R *Xs = (const R *)f->X[(n + 1) & 1];
-        double xs[M::S];
-#pragma unroll
-        for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
-        M::hat_values(xs, f->pv, cov_at(f, n + 1), f->drift_on ? &f->dv : nullptr, f->times[n + 1], v);
-        if (hat_weighted(f, n)) {
-            const double l = f->logw[j];
-            if (l != -PLOM_INF) q = (uint64_t)((plom_exp(l - f->scal[0]) / f->scal[1]) * 0x1p62);
-        } else {
-            q = 1;
-        }
-    }
-    for (int k = 0; k < M::N_HAT; k++) {
-        const double part = tree256(valid ? v[k] * (double)q : 0.0, lds);
-        if (threadIdx.x == 0) f->hpart[(size_t)k * T + tile] = part;
-        s_key[threadIdx.x] = valid ? hat_key(v[k]) : ~0ull;
-        s_w[threadIdx.x] = q;
-        __syncthreads();
-        for (int size = 2; size <= TILE; size <<= 1) {
-            for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                const int t = threadIdx.x, p = t ^ stride;
-                if (p > t) {
-                    const bool up = (t & size) == 0;
-                    const uint64_t a = s_key[t], b = s_key[p];
-                    if ((a > b) == up) {
-                        s_key[t] = b; s_key[p] = a;
-                        const uint64_t wa = s_w[t];
-                        s_w[t] = s_w[p]; s_w[p] = wa;
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        const uint64_t key = s_key[threadIdx.x];
-        const uint64_t pref = scan256(s_w[threadIdx.x], lds64);
-        f->hkey[(size_t)k * Jp + j] = key;
-        f->hpref[(size_t)k * Jp + j] = pref;
-        __syncthreads();
-    }
-}
 
 __device__ inline uint64_t sum256_u64(uint64_t v, uint64_t *lds)
 {
@@ -926,123 +419,20 @@ __global__ __launch_bounds__(TILE) void k_hat_select(plom_dfilter *f)
     if (threadIdx.x == 0) out[role == 1 ? 0 : 2] = hat_unkey(lo);
 }
 
-/* number of systematic points k in [0, J) with (k 2^33 + u) Q <= C J 2^33, i.e. of
- * particles whose ancestor index is at most the particle with inclusive prefix C */
-__device__ inline uint64_t hat_count_le(uint64_t C, uint64_t J, uint64_t unum, uint64_t Q)
-{
-    const plom_u128 X = (plom_u128)C * J << 33;
-    const double est = (double)C * (double)J / (double)Q - (double)unum * 0x1p-33;
-    int64_t k = est < 0.0 ? 0 : (int64_t)est + 1;
-    if (k > (int64_t)J) k = (int64_t)J;
-    while (k > 0 && (plom_u128)(((uint64_t)(k - 1) << 33) + unum) * Q > X) k--;
-    while (k < (int64_t)J && (plom_u128)(((uint64_t)k << 33) + unum) * Q <= X) k++;
-    return (uint64_t)k;
-}
-
-/* prediction residuals of interval n over the resampled cloud (print.c:632-701): with o_j
- * the number of copies of particle j, pred = sum o_j mu_j / J, var_state = sum o_j
- * (mu_j - pred)^2 / (J - 1), var_obs = sum o_j sigma2_j / J, res = (y - pred) /
- * sqrt(var_state + var_obs); each sum strided by thread, then a tree over the block */
-template <class M, typename R>
-__global__ __launch_bounds__(TILE) void k_pred_res(plom_dfilter *f)
-{
-    __shared__ double lds[8];
-    const int n = f->step - 1;
-    if (n < 0 || n >= f->N || !f->pres) return;
-    double *out = f->pres + (size_t)n * M::N_TS * 2;
-    if ((f->flags & PLOM_FLAG_NO_FILTER) || f->nonan[n] == 0) {
-        if (threadIdx.x < 2 * M::N_TS) out[threadIdx.x] = PLOM_NAN;
-        return;
-    }
-    const int J = f->J, Jp = f->Jp;
-    const bool res_on = f->rflag[n] != 0;
-    const uint64_t Q = res_on ? f->tincl[f->T - 1] : 0;
-    const uint64_t unum = plom_resample_num(f->seed, f->stream, (uint32_t)n);
-    const R *Xs = (const R *)f->X[(n + 1) & 1];
-    const double t1 = f->times[n + 1];
-    double a[M::N_TS], b[M::N_TS];
-#pragma unroll
-    for (int ts = 0; ts < M::N_TS; ts++) { a[ts] = 0.0; b[ts] = 0.0; }
-    for (int j = threadIdx.x; j < J; j += TILE) {
-        uint64_t o = 1;
-        if (res_on) {
-            const int tile = j / TILE;
-            const uint64_t C = (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j];
-            const uint64_t Cm = j ? ((j % TILE) ? (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j - 1] : f->tincl[tile - 1]) : 0ull;
-            o = hat_count_le(C, (uint64_t)J, unum, Q) - hat_count_le(Cm, (uint64_t)J, unum, Q);
-        }
-        if (!o) continue;
-        double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
-#pragma unroll
-        for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
-        M::proj2obs(xs, f->pv, cov_at(f, n + 1), obs);
-        M::obs_moments(obs, f->pv, cov_at(f, n + 1), t1, mu, var);
-#pragma unroll
-        for (int ts = 0; ts < M::N_TS; ts++) { a[ts] += (double)o * mu[ts]; b[ts] += (double)o * var[ts]; }
-    }
-    double pred[M::N_TS], vobs[M::N_TS], c[M::N_TS];
-#pragma unroll
-    for (int ts = 0; ts < M::N_TS; ts++) {
-        pred[ts] = tree256(a[ts], lds) / (double)J;
-        vobs[ts] = tree256(b[ts], lds) / (double)J;
-        c[ts] = 0.0;
-    }
-    for (int j = threadIdx.x; j < J; j += TILE) {
-        uint64_t o = 1;
-        if (res_on) {
-            const int tile = j / TILE;
-            const uint64_t C = (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j];
-            const uint64_t Cm = j ? ((j % TILE) ? (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j - 1] : f->tincl[tile - 1]) : 0ull;
-            o = hat_count_le(C, (uint64_t)J, unum, Q) - hat_count_le(Cm, (uint64_t)J, unum, Q);
-        }
-        if (!o) continue;
-        double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
-#pragma unroll
-        for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
-        M::proj2obs(xs, f->pv, cov_at(f, n + 1), obs);
-        M::obs_moments(obs, f->pv, cov_at(f, n + 1), t1, mu, var);
-#pragma unroll
-        for (int ts = 0; ts < M::N_TS; ts++) {
-            const double d = mu[ts] - pred[ts];
-            c[ts] += (double)o * (d * d);
-        }
-    }
-#pragma unroll
-    for (int ts = 0; ts < M::N_TS; ts++) {
-        const double vstate = tree256(c[ts], lds) / (double)(J - 1);
-        if (threadIdx.x == 0) {
-            const double y = f->data[(size_t)n * M::N_TS + ts];
-            out[2 * ts] = pred[ts];
-            out[2 * ts + 1] = (y - pred[ts]) / plom_sqrt(vstate + vobs[ts]);
-        }
-    }
-}
-
-/* row 0 of the hat: every particle starts at x0 */
-template <class M, typename R>
-__global__ void k_hat0(plom_dfilter *f)
-{
-    if (threadIdx.x != 0 || !f->hat) return;
-    const R *x0 = (const R *)f->x0;
-    double xs[M::S], v[M::N_HAT];
-#pragma unroll
-    for (int k = 0; k < M::S; k++) xs[k] = (double)x0[k];
-    M::hat_values(xs, f->pv, cov_at(f, 0), f->drift_on ? &f->dv : nullptr, f->times[0], v);
-    for (int k = 0; k < M::N_HAT; k++) f->hat[3 * k] = f->hat[3 * k + 1] = f->hat[3 * k + 2] = v[k];
-}
 
 __global__ void k_selftest_math(int n, const double *in, double *out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double x = in[i];
-    out[i * 7 + 0] = plom_exp(x);
-    out[i * 7 + 1] = plom_log(x);
-    out[i * 7 + 2] = plom_Phi(x);
-    out[i * 7 + 3] = plom_sin(x);
-    out[i * 7 + 4] = plom_cos(x);
-    out[i * 7 + 5] = plom_sqrt(x);
-    out[i * 7 + 6] = plom_sqrt_mid(x);
+    out[i * 8 + 0] = plom_exp(x);
+    out[i * 8 + 1] = plom_log(x);
+    out[i * 8 + 2] = plom_Phi(x);
+    out[i * 8 + 3] = plom_sin(x);
+    out[i * 8 + 4] = plom_cos(x);
+    out[i * 8 + 5] = plom_sqrt(x);
+    out[i * 8 + 6] = plom_sqrt_mid(x);
+    out[i * 8 + 7] = plom_sqrt_rate(x);
 }
 
 /* plain copy (16-byte vectors, grid-stride): the bandwidth reference and the calibration
@@ -1053,11 +443,27 @@ __global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, d
         dst[i] = src[i];
 }
 
-__global__ void k_selftest_normals(uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int n, double *out)
+/* the first n Gaussians of a particle's stream over interval n0, FP64 (fp32 = 0) or FP32
+ * (fp32 = 1, widened to double) */
+__global__ void k_selftest_normals(uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int n, int fp32, double *out)
 {
+    __shared__ float s_tab32[PLOM_ICDF_TAB_SIZE];
+    for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += blockDim.x) s_tab32[i] = (float)plom_icdf_tab[i];
+    __syncthreads();
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (fp32) {
+        plom_nstream32 ns;
+        plom_nstream32_init(&ns, seed, stream, n0, j, s_tab32);
+        for (int i = 0; i < n; i += 2) {
+            float a, b;
+            plom_normal_pair32(&ns, &a, &b);
+            out[i] = a;
+            if (i + 1 < n) out[i + 1] = b;
+        }
+        return;
+    }
     plom_nstream64 ns;
-    plom_nstream64_init(&ns, seed, stream, n0, j, plom_bm_tab);
+    plom_nstream64_init(&ns, seed, stream, n0, j, plom_icdf_tab);
     for (int i = 0; i + 1 < n; i += 2) plom_normal_pair64(&ns, &out[i], &out[i + 1]);
     if (n & 1) {
         double spare;
@@ -1065,133 +471,113 @@ __global__ void k_selftest_normals(uint32_t seed, uint32_t stream, uint32_t n0, 
     }
 }
 
-/* ------------------------------------------------------------------ model table */
-struct model_entry {
-    const char *name;
-    plom_model_info info;
-    void (*launch_prop)(int prec, int variant, int mif, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles);
-    void (*launch_lineage)(int prec, int variant, hipStream_t s, plom_dfilter *df);
-    void (*launch_hat)(int prec, hipStream_t s, plom_dfilter *df, int ntiles, int n_hat);
-    void (*launch_hat0)(int prec, hipStream_t s, plom_dfilter *df);
-    /* host builds of the generated observation functions (stored clouds, read on the host) */
-    void (*proj2obs_h)(const double *X, const double *pvd, const double *cvd, double *obs);
-    void (*moments_h)(const double *obs, const double *pvd, const double *cvd, double t, double *mean, double *var);
-};
 
+/* ------------------------------------------------------------------ model table */
 /* the Box-Muller table for host code (the kernels read the __constant__ one) */
 static const double g_bm_tab_host[PLOM_BM_TAB_SIZE] = PLOM_BM_TAB_VALUES;
 
-/* variants 0-3: SDE noise switches; 4, 5: psr with / without white noise (FP64 only) */
-template <class M, typename R, bool MIF>
-static void launch_prop_r(int variant, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles)
+/* the models compiled into the library (one translation unit each), then the models
+ * registered by plom_model_load, in id order; entries are never removed, so a reader
+ * needs only the published count */
+#define PLOM_MAX_MODELS 64
+#define DECLARE_BUILTIN(lname) extern "C" const model_entry *plom_model_entry_##lname(void);
+PLOM_BUILTIN_MODELS(DECLARE_BUILTIN)
+#define LIST_BUILTIN(lname) plom_model_entry_##lname,
+static const model_entry *(*const g_builtin[])(void) = {PLOM_BUILTIN_MODELS(LIST_BUILTIN)};
+static const model_entry *g_model_tab[PLOM_MAX_MODELS];
+static std::atomic<int> g_model_count{0};
+static std::mutex g_model_mu;
+
+static int model_count()
 {
-    if constexpr (sizeof(R) == 8) {
-        if (variant == 4) { hipLaunchKernelGGL((k_propagate<M, R, 4, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); return; }
-        if (variant == 5) { hipLaunchKernelGGL((k_propagate<M, R, 5, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); return; }
-        if (variant == 6) { hipLaunchKernelGGL((k_propagate<M, R, 6, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); return; }
-    }
-    switch (variant) {
-    case 0: hipLaunchKernelGGL((k_propagate<M, R, 0, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
-    case 1: hipLaunchKernelGGL((k_propagate<M, R, 1, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
-    case 2: hipLaunchKernelGGL((k_propagate<M, R, 2, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
-    default: hipLaunchKernelGGL((k_propagate<M, R, 3, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
-    }
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int n = 0;
+        for (auto fn : g_builtin) g_model_tab[n++] = fn();
+        g_model_count.store(n, std::memory_order_release);
+    });
+    return g_model_count.load(std::memory_order_acquire);
 }
 
-template <class M>
-static void launch_prop(int prec, int variant, int mif, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles)
+/* model `id`, or null when out of range */
+static const model_entry *model_at(int id)
 {
-    if (prec == PLOM_FP32) {
-        if (mif) launch_prop_r<M, float, true>(variant, nblocks, s, df, tiles);
-        else launch_prop_r<M, float, false>(variant, nblocks, s, df, tiles);
-    } else {
-        if (mif) launch_prop_r<M, double, true>(variant, nblocks, s, df, tiles);
-        else launch_prop_r<M, double, false>(variant, nblocks, s, df, tiles);
-    }
+    return (id >= 0 && id < model_count()) ? g_model_tab[id] : nullptr;
 }
 
-template <class M, typename R>
-static void launch_lineage_r(int variant, hipStream_t s, plom_dfilter *df)
-{
-    if constexpr (sizeof(R) == 8) {
-        if (variant == 4) { hipLaunchKernelGGL((k_lineage<M, R, 4>), dim3(1), dim3(64), 0, s, df); return; }
-        if (variant == 5) { hipLaunchKernelGGL((k_lineage<M, R, 5>), dim3(1), dim3(64), 0, s, df); return; }
-        if (variant == 6) { hipLaunchKernelGGL((k_lineage<M, R, 6>), dim3(1), dim3(64), 0, s, df); return; }
-    }
-    switch (variant) {
-    case 0: hipLaunchKernelGGL((k_lineage<M, R, 0>), dim3(1), dim3(64), 0, s, df); break;
-    case 1: hipLaunchKernelGGL((k_lineage<M, R, 1>), dim3(1), dim3(64), 0, s, df); break;
-    case 2: hipLaunchKernelGGL((k_lineage<M, R, 2>), dim3(1), dim3(64), 0, s, df); break;
-    default: hipLaunchKernelGGL((k_lineage<M, R, 3>), dim3(1), dim3(64), 0, s, df); break;
-    }
-}
-
-template <class M>
-static void launch_lineage(int prec, int variant, hipStream_t s, plom_dfilter *df)
-{
-    if (prec == PLOM_FP32) launch_lineage_r<M, float>(variant, s, df);
-    else launch_lineage_r<M, double>(variant, s, df);
-}
-
-template <class M>
-static void launch_hat(int prec, hipStream_t s, plom_dfilter *df, int ntiles, int n_hat)
-{
-    if (prec == PLOM_FP32) {
-        hipLaunchKernelGGL((k_hat_tiles<M, float>), dim3(ntiles), dim3(TILE), 0, s, df);
-        hipLaunchKernelGGL((k_pred_res<M, float>), dim3(1), dim3(TILE), 0, s, df);
-    } else {
-        hipLaunchKernelGGL((k_hat_tiles<M, double>), dim3(ntiles), dim3(TILE), 0, s, df);
-        hipLaunchKernelGGL((k_pred_res<M, double>), dim3(1), dim3(TILE), 0, s, df);
-    }
-    hipLaunchKernelGGL(k_hat_select, dim3(3 * n_hat), dim3(TILE), 0, s, df);
-}
-
-template <class M>
-static void launch_hat0(int prec, hipStream_t s, plom_dfilter *df)
-{
-    if (prec == PLOM_FP32) hipLaunchKernelGGL((k_hat0<M, float>), dim3(1), dim3(64), 0, s, df);
-    else hipLaunchKernelGGL((k_hat0<M, double>), dim3(1), dim3(64), 0, s, df);
-}
-
-#define MODEL_ENTRY(lname, UNAME)                                                               \
-    { #lname,                                                                                   \
-      { UNAME##_SIZE_PROJ, UNAME##_N_TS, UNAME##_N_CAC, UNAME##_N_PAR_SV, UNAME##_INC_OFFSET,   \
-        UNAME##_N_TS_INC_UNIQUE, UNAME##_PV_SIZE, UNAME##_N_DEM, UNAME##_N_ENV, UNAME##_N_HAT,    \
-        UNAME##_N_PP, UNAME##_FIX_OFF, UNAME##_N_FIX, UNAME##_N_DRIFT, UNAME##_DRIFT_OFFSET },                                                                      \
-      launch_prop<plom_model_##lname>, launch_lineage<plom_model_##lname>, launch_hat<plom_model_##lname>, \
-      launch_hat0<plom_model_##lname>, plom_##lname##_proj2obs_f64, plom_##lname##_obs_moments }
-
-static const model_entry g_models[] = {
-    MODEL_ENTRY(sir, PLOM_SIR),
-    MODEL_ENTRY(sirs4, PLOM_SIRS4),
-    MODEL_ENTRY(seir10, PLOM_SEIR10),
-    MODEL_ENTRY(noise, PLOM_NOISE),
-    MODEL_ENTRY(drift, PLOM_DRIFT),
-};
-static const int g_n_models = (int)(sizeof g_models / sizeof g_models[0]);
-
-extern "C" int plom_model_count(void) { return g_n_models; }
+extern "C" int plom_model_count(void) { return model_count(); }
 
 extern "C" int plom_model_find(const char *name)
 {
-    for (int i = 0; i < g_n_models; i++)
-        if (name && !strcmp(name, g_models[i].name)) return i;
+    const int n = model_count();
+    for (int i = 0; i < n; i++)
+        if (name && !strcmp(name, g_model_tab[i]->name)) return i;
     return set_err(PLOM_ERR_MODEL, "unknown model '%s'", name ? name : "(null)");
 }
 
 extern "C" int plom_model_name(int id, char *buf, int len)
 {
-    if (id < 0 || id >= g_n_models) return set_err(PLOM_ERR_MODEL, "model id %d out of range", id);
-    if (buf && len > 0) snprintf(buf, (size_t)len, "%s", g_models[id].name);
+    const model_entry *me = model_at(id);
+    if (!me) return set_err(PLOM_ERR_MODEL, "model id %d out of range", id);
+    if (buf && len > 0) snprintf(buf, (size_t)len, "%s", me->name);
     return PLOM_OK;
 }
 
 extern "C" int plom_model_info_get(int id, plom_model_info *out)
 {
-    if (id < 0 || id >= g_n_models) return set_err(PLOM_ERR_MODEL, "model id %d out of range", id);
+    const model_entry *me = model_at(id);
+    if (!me) return set_err(PLOM_ERR_MODEL, "model id %d out of range", id);
     if (!out) return set_err(PLOM_ERR_ARG, "null output");
-    *out = g_models[id].info;
+    *out = me->info;
     return PLOM_OK;
+}
+
+/* Register the model of a model library built by plom_build.py (plom_model.hip compiled
+ * with PLOM_MODEL_PLUGIN): dlopen it, check that it was compiled against this library's
+ * device descriptor, and append it to the table.  Loading a library whose model name is
+ * already registered gives that model's id again. */
+extern "C" int plom_model_load(const char *path, int32_t *id)
+{
+    if (!path || !id) return set_err(PLOM_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lock(g_model_mu);
+    void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return set_err(PLOM_ERR_MODEL, "cannot load model library %s: %s", path, dlerror());
+    typedef const model_entry *(*entry_fn)(void);
+    entry_fn fn = (entry_fn)dlsym(h, "plom_model_plugin");
+    if (!fn) {
+        dlclose(h);
+        return set_err(PLOM_ERR_MODEL, "%s exports no plom_model_plugin (not a model library)", path);
+    }
+    const model_entry *me = fn();
+    if (!me || me->abi != PLOM_MODEL_ABI || me->dfilter_bytes != (int32_t)sizeof(plom_dfilter)) {
+        dlclose(h);
+        return set_err(PLOM_ERR_MODEL, "%s was compiled against another version of the library (abi %d, descriptor "
+                       "%d bytes; this library: abi %d, %d bytes)", path, me ? me->abi : -1, me ? me->dfilter_bytes : -1,
+                       PLOM_MODEL_ABI, (int)sizeof(plom_dfilter));
+    }
+    const int n = model_count();
+    for (int i = 0; i < n; i++)
+        if (!strcmp(g_model_tab[i]->name, me->name)) {
+            /* the same model again: keep the first registration (its library stays open) */
+            if (g_model_tab[i] != me) dlclose(h);
+            *id = i;
+            return PLOM_OK;
+        }
+    if (n >= PLOM_MAX_MODELS) {
+        dlclose(h);
+        return set_err(PLOM_ERR_MODEL, "at most %d models", PLOM_MAX_MODELS);
+    }
+    g_model_tab[n] = me;   /* the library stays open until the process ends */
+    g_model_count.store(n + 1, std::memory_order_release);
+    *id = n;
+    return PLOM_OK;
+}
+
+/* the hat of one interval: the model's tile sort and residuals, then the selection */
+static void launch_hat_all(const model_entry *me, int prec, hipStream_t s, plom_dfilter *df, int ntiles, int n_hat)
+{
+    me->launch_hat(prec, s, df, ntiles, n_hat);
+    hipLaunchKernelGGL(k_hat_select, dim3(3 * n_hat), dim3(TILE), 0, s, df);
 }
 
 /* ------------------------------------------------------------------ context / filters */
@@ -1368,8 +754,9 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
 static int check_desc(const plom_filter_desc *d, plom_model_info *mi)
 {
     if (!d) return set_err(PLOM_ERR_ARG, "null descriptor");
-    if (d->model_id < 0 || d->model_id >= g_n_models) return set_err(PLOM_ERR_MODEL, "model id %d out of range", d->model_id);
-    *mi = g_models[d->model_id].info;
+    const model_entry *me = model_at(d->model_id);
+    if (!me) return set_err(PLOM_ERR_MODEL, "model id %d out of range", d->model_id);
+    *mi = me->info;
     if (d->J < 1 || d->J > PLOM_MAX_J) return set_err(PLOM_ERR_ARG, "J=%d outside [1, %d]", d->J, PLOM_MAX_J);
     if (d->N < 0) return set_err(PLOM_ERR_ARG, "N=%d < 0", d->N);
     if (d->precision != PLOM_FP64 && d->precision != PLOM_FP32) return set_err(PLOM_ERR_ARG, "bad precision %d", d->precision);
@@ -1922,7 +1309,7 @@ extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
         int rc = filter_reset_async(c, f);
         if (rc != PLOM_OK) return rc;
     }
-    const model_entry *me = &g_models[f->desc.model_id];
+    const model_entry *me = model_at(f->desc.model_id);
     const int T = f->h.T;
     const int mif = (f->desc.flags & PLOM_FLAG_MIF) != 0;
     if (mif) {
@@ -1942,7 +1329,7 @@ extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
         const int b = kspan_close(c, &k, c->stream, a, 0, 0);
         enqueue_resample(c->stream, f->df, nullptr, T, nullptr, 1, mif ? f->desc.n_theta : 0);
         if (kspan_close(c, &k, c->stream, b, 1, 0) == -2) return PLOM_ERR_HIP;
-        if (f->desc.flags & PLOM_FLAG_HAT) me->launch_hat(f->desc.precision, c->stream, f->df, T, f->info.n_hat);
+        if (f->desc.flags & PLOM_FLAG_HAT) launch_hat_all(me, f->desc.precision, c->stream, f->df, T, f->info.n_hat);
     }
     if ((f->desc.flags & PLOM_FLAG_HAT) && f->desc.N > 0) me->launch_hat0(f->desc.precision, c->stream, f->df);
     if (needs_final(f)) enqueue_final(c, f, nullptr, f->df, T);
@@ -2025,7 +1412,7 @@ static int filter_read_on(plom_ctx *c, plom_filter *f, plom_filter_outputs *o, h
     if (want_all) {
         /* row n = the cloud after interval n-1 as resampling leaves it: particle j holds its
          * ancestor's state (identity where the interval was not resampled) */
-        const model_entry *me = &g_models[f->desc.model_id];
+        const model_entry *me = model_at(f->desc.model_id);
         const int NTS = f->info.n_ts;
         std::vector<double> x(S), obs((size_t)NTS + 1), mean((size_t)NTS + 1), var((size_t)NTS + 1);
         for (size_t n = 0; n <= N; n++) {
@@ -2168,7 +1555,7 @@ static int chain_propagate(batch_chain *ch, int s, hipStream_t st)
     int launches = 0;
     for (auto &g : ch->groups) {
         const int nb = prefix_len(ch, g.tiles, s);
-        if (nb > 0) { g_models[g.model].launch_prop(g.prec, g.variant, g.mif, nb, st, ch->ddf, g.d); launches++; }
+        if (nb > 0) { model_at(g.model)->launch_prop(g.prec, g.variant, g.mif, nb, st, ch->ddf, g.d); launches++; }
     }
     return launches;
 }
@@ -2182,7 +1569,7 @@ static void chain_resample(batch_chain *ch, int s, hipStream_t st)
     for (int r = 0; r < nfa; r++) {
         plom_filter *f = ch->fs[(size_t)r];
         if (f->desc.flags & PLOM_FLAG_HAT)
-            g_models[f->desc.model_id].launch_hat(f->desc.precision, st, ch->ddf + r, f->h.T, f->info.n_hat);
+            launch_hat_all(model_at(f->desc.model_id), f->desc.precision, st, ch->ddf + r, f->h.T, f->info.n_hat);
     }
 }
 
@@ -2193,9 +1580,9 @@ static void chain_final(plom_ctx *c, batch_chain *ch, hipStream_t st)
         plom_filter *f = ch->fs[(size_t)r];
         while (ch->all_tiles[first].x != r) first++;
         if (f->desc.N > 0 && (f->desc.flags & PLOM_FLAG_LINEAGE))
-            g_models[f->desc.model_id].launch_lineage(f->desc.precision, f->variant, st, ch->ddf + r);
+            model_at(f->desc.model_id)->launch_lineage(f->desc.precision, f->variant, st, ch->ddf + r);
         if (f->desc.N > 0 && (f->desc.flags & PLOM_FLAG_HAT))
-            g_models[f->desc.model_id].launch_hat0(f->desc.precision, st, ch->ddf + r);
+            model_at(f->desc.model_id)->launch_hat0(f->desc.precision, st, ch->ddf + r);
         if (!needs_final(f)) continue;
         if (f->desc.precision == PLOM_FP32)
             hipLaunchKernelGGL((k_final<float>), dim3(f->h.T), dim3(TILE), 0, st, ch->ddf, ch->d_all + first, f->info.size_proj);
@@ -2783,7 +2170,7 @@ static int srv_lane_step(plom_server *S, srv_lane *L)
     }
     (void)hipGetLastError();
     for (auto &g : L->groups)
-        g_models[g.model].launch_prop(g.prec, g.variant, 0, (int)g.n, L->stream, S->d_slots, L->d_tiles + g.off);
+        model_at(g.model)->launch_prop(g.prec, g.variant, 0, (int)g.n, L->stream, S->d_slots, L->d_tiles + g.off);
     enqueue_resample(L->stream, S->d_slots, (const int2 *)L->d_tiles, (int)L->n_all, L->d_slots_active, (int)L->active.size());
     HIPCHK(hipGetLastError());
     for (size_t i = 0; i < L->active.size();) {
@@ -2792,7 +2179,7 @@ static int srv_lane_step(plom_server *S, srv_lane *L)
         if (r->progress >= r->desc.N) {
             /* the slot descriptor carries the interval counter: copy it back, mark done */
             if (r->desc.N > 0 && (r->desc.flags & PLOM_FLAG_LINEAGE))
-                g_models[r->desc.model_id].launch_lineage(r->desc.precision, r->f->variant, L->stream, S->d_slots + r->slot);
+                model_at(r->desc.model_id)->launch_lineage(r->desc.precision, r->f->variant, L->stream, S->d_slots + r->slot);
             HIPCHK(hipMemcpyAsync(r->f->df, S->d_slots + r->slot, sizeof(plom_dfilter), hipMemcpyDeviceToDevice, L->stream));
             HIPCHK(hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming));
             HIPCHK(hipEventRecord(r->done_ev, L->stream));
@@ -2853,7 +2240,7 @@ extern "C" int plom_server_step(plom_server *S, int32_t *n_active, int32_t *n_qu
         srv_request *r = S->queue.front();
         if (S->in_use + r->bytes > S->budget || !ready[r->lane]) break;
         srv_lane *L = &S->lanes[r->lane];
-        plom_model_info mi = g_models[r->desc.model_id].info;
+        plom_model_info mi = model_at(r->desc.model_id)->info;
         layout Lo;
         make_layout(&r->desc, &mi, &Lo, r->cov_vals, 0, 0, r->spl_vals);
         char *img;
@@ -3028,12 +2415,12 @@ extern "C" int plom_selftest_math(plom_ctx *c, int n, const double *in, double *
     if (n == 0) return PLOM_OK;
     HIPCHK(hipSetDevice(c->device));
     double *d = nullptr;
-    HIPCHK(hipMalloc((void **)&d, (size_t)n * 8 * 8));
+    HIPCHK(hipMalloc((void **)&d, (size_t)n * 9 * 8));   /* n inputs, then 8 results each */
     HIPCHK(hipMemcpy(d, in, (size_t)n * 8, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_selftest_math, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, d, d + n);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(out, d + n, (size_t)n * 7 * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out, d + n, (size_t)n * 8 * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipFree(d));
     return PLOM_OK;
 }
@@ -3062,17 +2449,29 @@ extern "C" int plom_selftest_copy(plom_ctx *c, int64_t nbytes, int reps, double 
     return PLOM_OK;
 }
 
-extern "C" int plom_selftest_normals(plom_ctx *c, uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int n, double *out)
+static int selftest_normals(plom_ctx *c, uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int n, int fp32,
+                            double *out)
 {
     if (!c || n < 0 || (n > 0 && !out)) return set_err(PLOM_ERR_ARG, "bad arguments");
     if (n == 0) return PLOM_OK;
     HIPCHK(hipSetDevice(c->device));
     double *d = nullptr;
     HIPCHK(hipMalloc((void **)&d, (size_t)n * 8));
-    hipLaunchKernelGGL(k_selftest_normals, dim3(1), dim3(64), 0, c->stream, seed, stream, n0, j, n, d);
+    hipLaunchKernelGGL(k_selftest_normals, dim3(1), dim3(64), 0, c->stream, seed, stream, n0, j, n, fp32, d);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(out, d, (size_t)n * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipFree(d));
     return PLOM_OK;
+}
+
+extern "C" int plom_selftest_normals(plom_ctx *c, uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int n, double *out)
+{
+    return selftest_normals(c, seed, stream, n0, j, n, 0, out);
+}
+
+extern "C" int plom_selftest_normals32(plom_ctx *c, uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int n,
+                                       double *out)
+{
+    return selftest_normals(c, seed, stream, n0, j, n, 1, out);
 }

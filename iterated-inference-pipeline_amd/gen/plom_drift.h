@@ -70,9 +70,9 @@ PLOM_HD void plom_drift_interval_full_f64(double *X, const double *pv, plom_nstr
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
         const double d4_0 = sqdt * za2;
-        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt((r1_0*X[0]))*d1_0- plom_sqrt((r3_0*X[0]))*d3_0+ plom_sqrt((r0_0))*d0_0;
-        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt((r2_0*X[2]))*d2_0- plom_sqrt((r3_0*X[2]))*d4_0+ plom_sqrt((r1_0*X[0]))*d1_0;
-        s6 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt((r2_0*X[2]))*d2_0+ plom_sqrt((r3_0*X[2]))*d4_0;
+        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt_rate((r1_0*X[0]))*d1_0- plom_sqrt_rate((r3_0*X[0]))*d3_0+ plom_sqrt_rate((r0_0))*d0_0;
+        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt_rate((r2_0*X[2]))*d2_0- plom_sqrt_rate((r3_0*X[2]))*d4_0+ plom_sqrt_rate((r1_0*X[0]))*d1_0;
+        s6 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt_rate((r2_0*X[2]))*d2_0+ plom_sqrt_rate((r3_0*X[2]))*d4_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
@@ -88,10 +88,10 @@ PLOM_HD void plom_drift_interval_full_f64(double *X, const double *pv, plom_nstr
         plom_normal_pair64(ns, &za4, &zb4);
         const double d3_1 = sqdt * za4;
         const double d4_1 = sqdt * zb4;
-        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt((r1_1*X[1]))*d1_1- plom_sqrt((r3_1*X[1]))*d3_1+ plom_sqrt((r0_1))*d0_1;
-        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt((r2_1*X[3]))*d2_1- plom_sqrt((r3_1*X[3]))*d4_1+ plom_sqrt((r1_1*X[1]))*d1_1;
-        s6 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt((r2_1*X[3]))*d2_1+ plom_sqrt((r3_1*X[3]))*d4_1;
-        s7 += ( + (r1_1*X[1]))*dtr + + plom_sqrt((r1_1*X[1]))*d1_1;
+        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt_rate((r1_1*X[1]))*d1_1- plom_sqrt_rate((r3_1*X[1]))*d3_1+ plom_sqrt_rate((r0_1))*d0_1;
+        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt_rate((r2_1*X[3]))*d2_1- plom_sqrt_rate((r3_1*X[3]))*d4_1+ plom_sqrt_rate((r1_1*X[1]))*d1_1;
+        s6 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt_rate((r2_1*X[3]))*d2_1+ plom_sqrt_rate((r3_1*X[3]))*d4_1;
+        s7 += ( + (r1_1*X[1]))*dtr + + plom_sqrt_rate((r1_1*X[1]))*d1_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         const double f6 = X[6] + s6;
@@ -248,9 +248,9 @@ PLOM_HD void plom_drift_interval_no_env_sto_f64(double *X, const double *pv, plo
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
         const double d4_0 = sqdt * za2;
-        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt((r1_0*X[0]))*d1_0- plom_sqrt((r3_0*X[0]))*d3_0+ plom_sqrt((r0_0))*d0_0;
-        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt((r2_0*X[2]))*d2_0- plom_sqrt((r3_0*X[2]))*d4_0+ plom_sqrt((r1_0*X[0]))*d1_0;
-        s6 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt((r2_0*X[2]))*d2_0+ plom_sqrt((r3_0*X[2]))*d4_0;
+        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt_rate((r1_0*X[0]))*d1_0- plom_sqrt_rate((r3_0*X[0]))*d3_0+ plom_sqrt_rate((r0_0))*d0_0;
+        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt_rate((r2_0*X[2]))*d2_0- plom_sqrt_rate((r3_0*X[2]))*d4_0+ plom_sqrt_rate((r1_0*X[0]))*d1_0;
+        s6 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt_rate((r2_0*X[2]))*d2_0+ plom_sqrt_rate((r3_0*X[2]))*d4_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
@@ -266,10 +266,10 @@ PLOM_HD void plom_drift_interval_no_env_sto_f64(double *X, const double *pv, plo
         plom_normal_pair64(ns, &za4, &zb4);
         const double d3_1 = sqdt * za4;
         const double d4_1 = sqdt * zb4;
-        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt((r1_1*X[1]))*d1_1- plom_sqrt((r3_1*X[1]))*d3_1+ plom_sqrt((r0_1))*d0_1;
-        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt((r2_1*X[3]))*d2_1- plom_sqrt((r3_1*X[3]))*d4_1+ plom_sqrt((r1_1*X[1]))*d1_1;
-        s6 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt((r2_1*X[3]))*d2_1+ plom_sqrt((r3_1*X[3]))*d4_1;
-        s7 += ( + (r1_1*X[1]))*dtr + + plom_sqrt((r1_1*X[1]))*d1_1;
+        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt_rate((r1_1*X[1]))*d1_1- plom_sqrt_rate((r3_1*X[1]))*d3_1+ plom_sqrt_rate((r0_1))*d0_1;
+        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt_rate((r2_1*X[3]))*d2_1- plom_sqrt_rate((r3_1*X[3]))*d4_1+ plom_sqrt_rate((r1_1*X[1]))*d1_1;
+        s6 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt_rate((r2_1*X[3]))*d2_1+ plom_sqrt_rate((r3_1*X[3]))*d4_1;
+        s7 += ( + (r1_1*X[1]))*dtr + + plom_sqrt_rate((r1_1*X[1]))*d1_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         const double f6 = X[6] + s6;
@@ -1678,9 +1678,9 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
         const double d4_0 = sqdt * za2;
-        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt((r1_0*X[0]))*d1_0- plom_sqrt((r3_0*X[0]))*d3_0+ plom_sqrt((r0_0))*d0_0;
-        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt((r2_0*X[2]))*d2_0- plom_sqrt((r3_0*X[2]))*d4_0+ plom_sqrt((r1_0*X[0]))*d1_0;
-        s6 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt((r2_0*X[2]))*d2_0+ plom_sqrt((r3_0*X[2]))*d4_0;
+        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt_rate((r1_0*X[0]))*d1_0- plom_sqrt_rate((r3_0*X[0]))*d3_0+ plom_sqrt_rate((r0_0))*d0_0;
+        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt_rate((r2_0*X[2]))*d2_0- plom_sqrt_rate((r3_0*X[2]))*d4_0+ plom_sqrt_rate((r1_0*X[0]))*d1_0;
+        s6 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt_rate((r2_0*X[2]))*d2_0+ plom_sqrt_rate((r3_0*X[2]))*d4_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
@@ -1696,10 +1696,10 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         plom_normal_pair64(ns, &za4, &zb4);
         const double d3_1 = sqdt * za4;
         const double d4_1 = sqdt * zb4;
-        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt((r1_1*X[1]))*d1_1- plom_sqrt((r3_1*X[1]))*d3_1+ plom_sqrt((r0_1))*d0_1;
-        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt((r2_1*X[3]))*d2_1- plom_sqrt((r3_1*X[3]))*d4_1+ plom_sqrt((r1_1*X[1]))*d1_1;
-        s6 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt((r2_1*X[3]))*d2_1+ plom_sqrt((r3_1*X[3]))*d4_1;
-        s7 += ( + (r1_1*X[1]))*dtr + + plom_sqrt((r1_1*X[1]))*d1_1;
+        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt_rate((r1_1*X[1]))*d1_1- plom_sqrt_rate((r3_1*X[1]))*d3_1+ plom_sqrt_rate((r0_1))*d0_1;
+        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt_rate((r2_1*X[3]))*d2_1- plom_sqrt_rate((r3_1*X[3]))*d4_1+ plom_sqrt_rate((r1_1*X[1]))*d1_1;
+        s6 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt_rate((r2_1*X[3]))*d2_1+ plom_sqrt_rate((r3_1*X[3]))*d4_1;
+        s7 += ( + (r1_1*X[1]))*dtr + + plom_sqrt_rate((r1_1*X[1]))*d1_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         const double f6 = X[6] + s6;
@@ -1884,9 +1884,9 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
         const double d4_0 = sqdt * za2;
-        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt((r1_0*X[0]))*d1_0- plom_sqrt((r3_0*X[0]))*d3_0+ plom_sqrt((r0_0))*d0_0;
-        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt((r2_0*X[2]))*d2_0- plom_sqrt((r3_0*X[2]))*d4_0+ plom_sqrt((r1_0*X[0]))*d1_0;
-        s6 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt((r2_0*X[2]))*d2_0+ plom_sqrt((r3_0*X[2]))*d4_0;
+        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt_rate((r1_0*X[0]))*d1_0- plom_sqrt_rate((r3_0*X[0]))*d3_0+ plom_sqrt_rate((r0_0))*d0_0;
+        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt_rate((r2_0*X[2]))*d2_0- plom_sqrt_rate((r3_0*X[2]))*d4_0+ plom_sqrt_rate((r1_0*X[0]))*d1_0;
+        s6 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt_rate((r2_0*X[2]))*d2_0+ plom_sqrt_rate((r3_0*X[2]))*d4_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
@@ -1902,10 +1902,10 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         plom_normal_pair64(ns, &za4, &zb4);
         const double d3_1 = sqdt * za4;
         const double d4_1 = sqdt * zb4;
-        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt((r1_1*X[1]))*d1_1- plom_sqrt((r3_1*X[1]))*d3_1+ plom_sqrt((r0_1))*d0_1;
-        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt((r2_1*X[3]))*d2_1- plom_sqrt((r3_1*X[3]))*d4_1+ plom_sqrt((r1_1*X[1]))*d1_1;
-        s6 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt((r2_1*X[3]))*d2_1+ plom_sqrt((r3_1*X[3]))*d4_1;
-        s7 += ( + (r1_1*X[1]))*dtr + + plom_sqrt((r1_1*X[1]))*d1_1;
+        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt_rate((r1_1*X[1]))*d1_1- plom_sqrt_rate((r3_1*X[1]))*d3_1+ plom_sqrt_rate((r0_1))*d0_1;
+        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt_rate((r2_1*X[3]))*d2_1- plom_sqrt_rate((r3_1*X[3]))*d4_1+ plom_sqrt_rate((r1_1*X[1]))*d1_1;
+        s6 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt_rate((r2_1*X[3]))*d2_1+ plom_sqrt_rate((r3_1*X[3]))*d4_1;
+        s7 += ( + (r1_1*X[1]))*dtr + + plom_sqrt_rate((r1_1*X[1]))*d1_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         const double f6 = X[6] + s6;

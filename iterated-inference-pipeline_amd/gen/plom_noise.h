@@ -65,9 +65,9 @@ PLOM_HD void plom_noise_interval_full_f64(double *X, const double *pv, plom_nstr
         plom_normal_pair64(ns, &za2, &zb2);
         const double d4_0 = sqdt * za2;
         const double w0_0 = sqdt * zb2;
-        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt((r1_0*X[0]))*d1_0- plom_sqrt((r3_0*X[0]))*d3_0+ plom_sqrt((r0_0))*d0_0 + - (r1_0*X[0])*P4*w0_0;
-        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt((r2_0*X[2]))*d2_0- plom_sqrt((r3_0*X[2]))*d4_0+ plom_sqrt((r1_0*X[0]))*d1_0 + + (r1_0*X[0])*P4*w0_0;
-        s4 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt((r2_0*X[2]))*d2_0+ plom_sqrt((r3_0*X[2]))*d4_0;
+        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt_rate((r1_0*X[0]))*d1_0- plom_sqrt_rate((r3_0*X[0]))*d3_0+ plom_sqrt_rate((r0_0))*d0_0 + - (r1_0*X[0])*P4*w0_0;
+        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt_rate((r2_0*X[2]))*d2_0- plom_sqrt_rate((r3_0*X[2]))*d4_0+ plom_sqrt_rate((r1_0*X[0]))*d1_0 + + (r1_0*X[0])*P4*w0_0;
+        s4 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt_rate((r2_0*X[2]))*d2_0+ plom_sqrt_rate((r3_0*X[2]))*d4_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
@@ -86,10 +86,10 @@ PLOM_HD void plom_noise_interval_full_f64(double *X, const double *pv, plom_nstr
         plom_normal_pair64(ns, &za5, &zb5);
         const double d4_1 = sqdt * za5;
         const double w0_1 = sqdt * zb5;
-        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt((r1_1*X[1]))*d1_1- plom_sqrt((r3_1*X[1]))*d3_1+ plom_sqrt((r0_1))*d0_1 + - (r1_1*X[1])*P5*w0_1;
-        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt((r2_1*X[3]))*d2_1- plom_sqrt((r3_1*X[3]))*d4_1+ plom_sqrt((r1_1*X[1]))*d1_1 + + (r1_1*X[1])*P5*w0_1;
-        s4 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt((r2_1*X[3]))*d2_1+ plom_sqrt((r3_1*X[3]))*d4_1;
-        s5 += ( + (r1_1*X[1]))*dtr + + plom_sqrt((r1_1*X[1]))*d1_1 + + (r1_1*X[1])*P5*w0_1;
+        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt_rate((r1_1*X[1]))*d1_1- plom_sqrt_rate((r3_1*X[1]))*d3_1+ plom_sqrt_rate((r0_1))*d0_1 + - (r1_1*X[1])*P5*w0_1;
+        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt_rate((r2_1*X[3]))*d2_1- plom_sqrt_rate((r3_1*X[3]))*d4_1+ plom_sqrt_rate((r1_1*X[1]))*d1_1 + + (r1_1*X[1])*P5*w0_1;
+        s4 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt_rate((r2_1*X[3]))*d2_1+ plom_sqrt_rate((r3_1*X[3]))*d4_1;
+        s5 += ( + (r1_1*X[1]))*dtr + + plom_sqrt_rate((r1_1*X[1]))*d1_1 + + (r1_1*X[1])*P5*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         const double f4 = X[4] + s4;
@@ -194,9 +194,9 @@ PLOM_HD void plom_noise_interval_no_env_sto_f64(double *X, const double *pv, plo
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
         const double d4_0 = sqdt * za2;
-        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt((r1_0*X[0]))*d1_0- plom_sqrt((r3_0*X[0]))*d3_0+ plom_sqrt((r0_0))*d0_0;
-        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt((r2_0*X[2]))*d2_0- plom_sqrt((r3_0*X[2]))*d4_0+ plom_sqrt((r1_0*X[0]))*d1_0;
-        s4 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt((r2_0*X[2]))*d2_0+ plom_sqrt((r3_0*X[2]))*d4_0;
+        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt_rate((r1_0*X[0]))*d1_0- plom_sqrt_rate((r3_0*X[0]))*d3_0+ plom_sqrt_rate((r0_0))*d0_0;
+        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt_rate((r2_0*X[2]))*d2_0- plom_sqrt_rate((r3_0*X[2]))*d4_0+ plom_sqrt_rate((r1_0*X[0]))*d1_0;
+        s4 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt_rate((r2_0*X[2]))*d2_0+ plom_sqrt_rate((r3_0*X[2]))*d4_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
@@ -212,10 +212,10 @@ PLOM_HD void plom_noise_interval_no_env_sto_f64(double *X, const double *pv, plo
         plom_normal_pair64(ns, &za4, &zb4);
         const double d3_1 = sqdt * za4;
         const double d4_1 = sqdt * zb4;
-        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt((r1_1*X[1]))*d1_1- plom_sqrt((r3_1*X[1]))*d3_1+ plom_sqrt((r0_1))*d0_1;
-        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt((r2_1*X[3]))*d2_1- plom_sqrt((r3_1*X[3]))*d4_1+ plom_sqrt((r1_1*X[1]))*d1_1;
-        s4 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt((r2_1*X[3]))*d2_1+ plom_sqrt((r3_1*X[3]))*d4_1;
-        s5 += ( + (r1_1*X[1]))*dtr + + plom_sqrt((r1_1*X[1]))*d1_1;
+        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt_rate((r1_1*X[1]))*d1_1- plom_sqrt_rate((r3_1*X[1]))*d3_1+ plom_sqrt_rate((r0_1))*d0_1;
+        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt_rate((r2_1*X[3]))*d2_1- plom_sqrt_rate((r3_1*X[3]))*d4_1+ plom_sqrt_rate((r1_1*X[1]))*d1_1;
+        s4 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt_rate((r2_1*X[3]))*d2_1+ plom_sqrt_rate((r3_1*X[3]))*d4_1;
+        s5 += ( + (r1_1*X[1]))*dtr + + plom_sqrt_rate((r1_1*X[1]))*d1_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         const double f4 = X[4] + s4;
@@ -1335,9 +1335,9 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         plom_normal_pair64(ns, &za2, &zb2);
         const double d4_0 = sqdt * za2;
         const double w0_0 = sqdt * zb2;
-        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt((r1_0*X[0]))*d1_0- plom_sqrt((r3_0*X[0]))*d3_0+ plom_sqrt((r0_0))*d0_0 + - (r1_0*X[0])*P4*w0_0;
-        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt((r2_0*X[2]))*d2_0- plom_sqrt((r3_0*X[2]))*d4_0+ plom_sqrt((r1_0*X[0]))*d1_0 + + (r1_0*X[0])*P4*w0_0;
-        s4 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt((r2_0*X[2]))*d2_0+ plom_sqrt((r3_0*X[2]))*d4_0;
+        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt_rate((r1_0*X[0]))*d1_0- plom_sqrt_rate((r3_0*X[0]))*d3_0+ plom_sqrt_rate((r0_0))*d0_0 + - (r1_0*X[0])*P4*w0_0;
+        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt_rate((r2_0*X[2]))*d2_0- plom_sqrt_rate((r3_0*X[2]))*d4_0+ plom_sqrt_rate((r1_0*X[0]))*d1_0 + + (r1_0*X[0])*P4*w0_0;
+        s4 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt_rate((r2_0*X[2]))*d2_0+ plom_sqrt_rate((r3_0*X[2]))*d4_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
@@ -1356,10 +1356,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         plom_normal_pair64(ns, &za5, &zb5);
         const double d4_1 = sqdt * za5;
         const double w0_1 = sqdt * zb5;
-        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt((r1_1*X[1]))*d1_1- plom_sqrt((r3_1*X[1]))*d3_1+ plom_sqrt((r0_1))*d0_1 + - (r1_1*X[1])*P5*w0_1;
-        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt((r2_1*X[3]))*d2_1- plom_sqrt((r3_1*X[3]))*d4_1+ plom_sqrt((r1_1*X[1]))*d1_1 + + (r1_1*X[1])*P5*w0_1;
-        s4 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt((r2_1*X[3]))*d2_1+ plom_sqrt((r3_1*X[3]))*d4_1;
-        s5 += ( + (r1_1*X[1]))*dtr + + plom_sqrt((r1_1*X[1]))*d1_1 + + (r1_1*X[1])*P5*w0_1;
+        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt_rate((r1_1*X[1]))*d1_1- plom_sqrt_rate((r3_1*X[1]))*d3_1+ plom_sqrt_rate((r0_1))*d0_1 + - (r1_1*X[1])*P5*w0_1;
+        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt_rate((r2_1*X[3]))*d2_1- plom_sqrt_rate((r3_1*X[3]))*d4_1+ plom_sqrt_rate((r1_1*X[1]))*d1_1 + + (r1_1*X[1])*P5*w0_1;
+        s4 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt_rate((r2_1*X[3]))*d2_1+ plom_sqrt_rate((r3_1*X[3]))*d4_1;
+        s5 += ( + (r1_1*X[1]))*dtr + + plom_sqrt_rate((r1_1*X[1]))*d1_1 + + (r1_1*X[1])*P5*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         const double f4 = X[4] + s4;
@@ -1464,9 +1464,9 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
         const double d4_0 = sqdt * za2;
-        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt((r1_0*X[0]))*d1_0- plom_sqrt((r3_0*X[0]))*d3_0+ plom_sqrt((r0_0))*d0_0;
-        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt((r2_0*X[2]))*d2_0- plom_sqrt((r3_0*X[2]))*d4_0+ plom_sqrt((r1_0*X[0]))*d1_0;
-        s4 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt((r2_0*X[2]))*d2_0+ plom_sqrt((r3_0*X[2]))*d4_0;
+        const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr + - plom_sqrt_rate((r1_0*X[0]))*d1_0- plom_sqrt_rate((r3_0*X[0]))*d3_0+ plom_sqrt_rate((r0_0))*d0_0;
+        const double f2 = X[2] + ( - (r2_0*X[2]) - (r3_0*X[2]) + (r1_0*X[0]))*dtr + - plom_sqrt_rate((r2_0*X[2]))*d2_0- plom_sqrt_rate((r3_0*X[2]))*d4_0+ plom_sqrt_rate((r1_0*X[0]))*d1_0;
+        s4 += ( + (r2_0*X[2]) + (r3_0*X[2]))*dtr + + plom_sqrt_rate((r2_0*X[2]))*d2_0+ plom_sqrt_rate((r3_0*X[2]))*d4_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
@@ -1482,10 +1482,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         plom_normal_pair64(ns, &za4, &zb4);
         const double d3_1 = sqdt * za4;
         const double d4_1 = sqdt * zb4;
-        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt((r1_1*X[1]))*d1_1- plom_sqrt((r3_1*X[1]))*d3_1+ plom_sqrt((r0_1))*d0_1;
-        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt((r2_1*X[3]))*d2_1- plom_sqrt((r3_1*X[3]))*d4_1+ plom_sqrt((r1_1*X[1]))*d1_1;
-        s4 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt((r2_1*X[3]))*d2_1+ plom_sqrt((r3_1*X[3]))*d4_1;
-        s5 += ( + (r1_1*X[1]))*dtr + + plom_sqrt((r1_1*X[1]))*d1_1;
+        const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr + - plom_sqrt_rate((r1_1*X[1]))*d1_1- plom_sqrt_rate((r3_1*X[1]))*d3_1+ plom_sqrt_rate((r0_1))*d0_1;
+        const double f3 = X[3] + ( - (r2_1*X[3]) - (r3_1*X[3]) + (r1_1*X[1]))*dtr + - plom_sqrt_rate((r2_1*X[3]))*d2_1- plom_sqrt_rate((r3_1*X[3]))*d4_1+ plom_sqrt_rate((r1_1*X[1]))*d1_1;
+        s4 += ( + (r2_1*X[3]) + (r3_1*X[3]))*dtr + + plom_sqrt_rate((r2_1*X[3]))*d2_1+ plom_sqrt_rate((r3_1*X[3]))*d4_1;
+        s5 += ( + (r1_1*X[1]))*dtr + + plom_sqrt_rate((r1_1*X[1]))*d1_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         const double f4 = X[4] + s4;

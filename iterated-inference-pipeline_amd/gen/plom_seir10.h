@@ -228,10 +228,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
         const double w0_0 = sqdt * za2;
-        const double f0 = X[0] + ( - (r0_0*X[0]) + (r3_0))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0+ plom_sqrt((r3_0))*d3_0 + - (r0_0*X[0])*P40*w0_0;
-        const double f10 = X[10] + ( - (r1_0*X[10]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[10]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P40*w0_0;
-        const double f20 = X[20] + ( - (r2_0*X[20]) + (r1_0*X[10]))*dtr + - plom_sqrt((r2_0*X[20]))*d2_0+ plom_sqrt((r1_0*X[10]))*d1_0;
-        s30 += ( + (r1_0*X[10]))*dtr + + plom_sqrt((r1_0*X[10]))*d1_0;
+        const double f0 = X[0] + ( - (r0_0*X[0]) + (r3_0))*dtr + - plom_sqrt_rate((r0_0*X[0]))*d0_0+ plom_sqrt_rate((r3_0))*d3_0 + - (r0_0*X[0])*P40*w0_0;
+        const double f10 = X[10] + ( - (r1_0*X[10]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[10]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P40*w0_0;
+        const double f20 = X[20] + ( - (r2_0*X[20]) + (r1_0*X[10]))*dtr + - plom_sqrt_rate((r2_0*X[20]))*d2_0+ plom_sqrt_rate((r1_0*X[10]))*d1_0;
+        s30 += ( + (r1_0*X[10]))*dtr + + plom_sqrt_rate((r1_0*X[10]))*d1_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
         const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
@@ -247,10 +247,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         plom_normal_pair64(ns, &za4, &zb4);
         const double d3_1 = sqdt * za4;
         const double w0_1 = sqdt * zb4;
-        const double f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr + - plom_sqrt((r0_1*X[1]))*d0_1+ plom_sqrt((r3_1))*d3_1 + - (r0_1*X[1])*P41*w0_1;
-        const double f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[11]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P41*w0_1;
-        const double f21 = X[21] + ( - (r2_1*X[21]) + (r1_1*X[11]))*dtr + - plom_sqrt((r2_1*X[21]))*d2_1+ plom_sqrt((r1_1*X[11]))*d1_1;
-        s31 += ( + (r1_1*X[11]))*dtr + + plom_sqrt((r1_1*X[11]))*d1_1;
+        const double f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr + - plom_sqrt_rate((r0_1*X[1]))*d0_1+ plom_sqrt_rate((r3_1))*d3_1 + - (r0_1*X[1])*P41*w0_1;
+        const double f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr + - plom_sqrt_rate((r1_1*X[11]))*d1_1+ plom_sqrt_rate((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P41*w0_1;
+        const double f21 = X[21] + ( - (r2_1*X[21]) + (r1_1*X[11]))*dtr + - plom_sqrt_rate((r2_1*X[21]))*d2_1+ plom_sqrt_rate((r1_1*X[11]))*d1_1;
+        s31 += ( + (r1_1*X[11]))*dtr + + plom_sqrt_rate((r1_1*X[11]))*d1_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
         const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
@@ -268,10 +268,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         double za7, zb7;
         plom_normal_pair64(ns, &za7, &zb7);
         const double w0_2 = sqdt * za7;
-        const double f2 = X[2] + ( - (r0_2*X[2]) + (r3_2))*dtr + - plom_sqrt((r0_2*X[2]))*d0_2+ plom_sqrt((r3_2))*d3_2 + - (r0_2*X[2])*P42*w0_2;
-        const double f12 = X[12] + ( - (r1_2*X[12]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[12]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P42*w0_2;
-        const double f22 = X[22] + ( - (r2_2*X[22]) + (r1_2*X[12]))*dtr + - plom_sqrt((r2_2*X[22]))*d2_2+ plom_sqrt((r1_2*X[12]))*d1_2;
-        s32 += ( + (r1_2*X[12]))*dtr + + plom_sqrt((r1_2*X[12]))*d1_2;
+        const double f2 = X[2] + ( - (r0_2*X[2]) + (r3_2))*dtr + - plom_sqrt_rate((r0_2*X[2]))*d0_2+ plom_sqrt_rate((r3_2))*d3_2 + - (r0_2*X[2])*P42*w0_2;
+        const double f12 = X[12] + ( - (r1_2*X[12]) + (r0_2*X[2]))*dtr + - plom_sqrt_rate((r1_2*X[12]))*d1_2+ plom_sqrt_rate((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P42*w0_2;
+        const double f22 = X[22] + ( - (r2_2*X[22]) + (r1_2*X[12]))*dtr + - plom_sqrt_rate((r2_2*X[22]))*d2_2+ plom_sqrt_rate((r1_2*X[12]))*d1_2;
+        s32 += ( + (r1_2*X[12]))*dtr + + plom_sqrt_rate((r1_2*X[12]))*d1_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         X[12] = (f12 < 0.0) ? 0.0 : f12;
         const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
@@ -287,10 +287,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         plom_normal_pair64(ns, &za9, &zb9);
         const double d3_3 = sqdt * za9;
         const double w0_3 = sqdt * zb9;
-        const double f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr + - plom_sqrt((r0_3*X[3]))*d0_3+ plom_sqrt((r3_3))*d3_3 + - (r0_3*X[3])*P43*w0_3;
-        const double f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr + - plom_sqrt((r1_3*X[13]))*d1_3+ plom_sqrt((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P43*w0_3;
-        const double f23 = X[23] + ( - (r2_3*X[23]) + (r1_3*X[13]))*dtr + - plom_sqrt((r2_3*X[23]))*d2_3+ plom_sqrt((r1_3*X[13]))*d1_3;
-        s33 += ( + (r1_3*X[13]))*dtr + + plom_sqrt((r1_3*X[13]))*d1_3;
+        const double f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr + - plom_sqrt_rate((r0_3*X[3]))*d0_3+ plom_sqrt_rate((r3_3))*d3_3 + - (r0_3*X[3])*P43*w0_3;
+        const double f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr + - plom_sqrt_rate((r1_3*X[13]))*d1_3+ plom_sqrt_rate((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P43*w0_3;
+        const double f23 = X[23] + ( - (r2_3*X[23]) + (r1_3*X[13]))*dtr + - plom_sqrt_rate((r2_3*X[23]))*d2_3+ plom_sqrt_rate((r1_3*X[13]))*d1_3;
+        s33 += ( + (r1_3*X[13]))*dtr + + plom_sqrt_rate((r1_3*X[13]))*d1_3;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[13] = (f13 < 0.0) ? 0.0 : f13;
         const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
@@ -308,10 +308,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         double za12, zb12;
         plom_normal_pair64(ns, &za12, &zb12);
         const double w0_4 = sqdt * za12;
-        const double f4 = X[4] + ( - (r0_4*X[4]) + (r3_4))*dtr + - plom_sqrt((r0_4*X[4]))*d0_4+ plom_sqrt((r3_4))*d3_4 + - (r0_4*X[4])*P44*w0_4;
-        const double f14 = X[14] + ( - (r1_4*X[14]) + (r0_4*X[4]))*dtr + - plom_sqrt((r1_4*X[14]))*d1_4+ plom_sqrt((r0_4*X[4]))*d0_4 + + (r0_4*X[4])*P44*w0_4;
-        const double f24 = X[24] + ( - (r2_4*X[24]) + (r1_4*X[14]))*dtr + - plom_sqrt((r2_4*X[24]))*d2_4+ plom_sqrt((r1_4*X[14]))*d1_4;
-        s34 += ( + (r1_4*X[14]))*dtr + + plom_sqrt((r1_4*X[14]))*d1_4;
+        const double f4 = X[4] + ( - (r0_4*X[4]) + (r3_4))*dtr + - plom_sqrt_rate((r0_4*X[4]))*d0_4+ plom_sqrt_rate((r3_4))*d3_4 + - (r0_4*X[4])*P44*w0_4;
+        const double f14 = X[14] + ( - (r1_4*X[14]) + (r0_4*X[4]))*dtr + - plom_sqrt_rate((r1_4*X[14]))*d1_4+ plom_sqrt_rate((r0_4*X[4]))*d0_4 + + (r0_4*X[4])*P44*w0_4;
+        const double f24 = X[24] + ( - (r2_4*X[24]) + (r1_4*X[14]))*dtr + - plom_sqrt_rate((r2_4*X[24]))*d2_4+ plom_sqrt_rate((r1_4*X[14]))*d1_4;
+        s34 += ( + (r1_4*X[14]))*dtr + + plom_sqrt_rate((r1_4*X[14]))*d1_4;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[14] = (f14 < 0.0) ? 0.0 : f14;
         const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
@@ -327,10 +327,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         plom_normal_pair64(ns, &za14, &zb14);
         const double d3_5 = sqdt * za14;
         const double w0_5 = sqdt * zb14;
-        const double f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr + - plom_sqrt((r0_5*X[5]))*d0_5+ plom_sqrt((r3_5))*d3_5 + - (r0_5*X[5])*P45*w0_5;
-        const double f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr + - plom_sqrt((r1_5*X[15]))*d1_5+ plom_sqrt((r0_5*X[5]))*d0_5 + + (r0_5*X[5])*P45*w0_5;
-        const double f25 = X[25] + ( - (r2_5*X[25]) + (r1_5*X[15]))*dtr + - plom_sqrt((r2_5*X[25]))*d2_5+ plom_sqrt((r1_5*X[15]))*d1_5;
-        s35 += ( + (r1_5*X[15]))*dtr + + plom_sqrt((r1_5*X[15]))*d1_5;
+        const double f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr + - plom_sqrt_rate((r0_5*X[5]))*d0_5+ plom_sqrt_rate((r3_5))*d3_5 + - (r0_5*X[5])*P45*w0_5;
+        const double f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr + - plom_sqrt_rate((r1_5*X[15]))*d1_5+ plom_sqrt_rate((r0_5*X[5]))*d0_5 + + (r0_5*X[5])*P45*w0_5;
+        const double f25 = X[25] + ( - (r2_5*X[25]) + (r1_5*X[15]))*dtr + - plom_sqrt_rate((r2_5*X[25]))*d2_5+ plom_sqrt_rate((r1_5*X[15]))*d1_5;
+        s35 += ( + (r1_5*X[15]))*dtr + + plom_sqrt_rate((r1_5*X[15]))*d1_5;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[15] = (f15 < 0.0) ? 0.0 : f15;
         const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
@@ -348,10 +348,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         double za17, zb17;
         plom_normal_pair64(ns, &za17, &zb17);
         const double w0_6 = sqdt * za17;
-        const double f6 = X[6] + ( - (r0_6*X[6]) + (r3_6))*dtr + - plom_sqrt((r0_6*X[6]))*d0_6+ plom_sqrt((r3_6))*d3_6 + - (r0_6*X[6])*P46*w0_6;
-        const double f16 = X[16] + ( - (r1_6*X[16]) + (r0_6*X[6]))*dtr + - plom_sqrt((r1_6*X[16]))*d1_6+ plom_sqrt((r0_6*X[6]))*d0_6 + + (r0_6*X[6])*P46*w0_6;
-        const double f26 = X[26] + ( - (r2_6*X[26]) + (r1_6*X[16]))*dtr + - plom_sqrt((r2_6*X[26]))*d2_6+ plom_sqrt((r1_6*X[16]))*d1_6;
-        s36 += ( + (r1_6*X[16]))*dtr + + plom_sqrt((r1_6*X[16]))*d1_6;
+        const double f6 = X[6] + ( - (r0_6*X[6]) + (r3_6))*dtr + - plom_sqrt_rate((r0_6*X[6]))*d0_6+ plom_sqrt_rate((r3_6))*d3_6 + - (r0_6*X[6])*P46*w0_6;
+        const double f16 = X[16] + ( - (r1_6*X[16]) + (r0_6*X[6]))*dtr + - plom_sqrt_rate((r1_6*X[16]))*d1_6+ plom_sqrt_rate((r0_6*X[6]))*d0_6 + + (r0_6*X[6])*P46*w0_6;
+        const double f26 = X[26] + ( - (r2_6*X[26]) + (r1_6*X[16]))*dtr + - plom_sqrt_rate((r2_6*X[26]))*d2_6+ plom_sqrt_rate((r1_6*X[16]))*d1_6;
+        s36 += ( + (r1_6*X[16]))*dtr + + plom_sqrt_rate((r1_6*X[16]))*d1_6;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[16] = (f16 < 0.0) ? 0.0 : f16;
         const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
@@ -367,10 +367,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         plom_normal_pair64(ns, &za19, &zb19);
         const double d3_7 = sqdt * za19;
         const double w0_7 = sqdt * zb19;
-        const double f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr + - plom_sqrt((r0_7*X[7]))*d0_7+ plom_sqrt((r3_7))*d3_7 + - (r0_7*X[7])*P47*w0_7;
-        const double f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr + - plom_sqrt((r1_7*X[17]))*d1_7+ plom_sqrt((r0_7*X[7]))*d0_7 + + (r0_7*X[7])*P47*w0_7;
-        const double f27 = X[27] + ( - (r2_7*X[27]) + (r1_7*X[17]))*dtr + - plom_sqrt((r2_7*X[27]))*d2_7+ plom_sqrt((r1_7*X[17]))*d1_7;
-        s37 += ( + (r1_7*X[17]))*dtr + + plom_sqrt((r1_7*X[17]))*d1_7;
+        const double f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr + - plom_sqrt_rate((r0_7*X[7]))*d0_7+ plom_sqrt_rate((r3_7))*d3_7 + - (r0_7*X[7])*P47*w0_7;
+        const double f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr + - plom_sqrt_rate((r1_7*X[17]))*d1_7+ plom_sqrt_rate((r0_7*X[7]))*d0_7 + + (r0_7*X[7])*P47*w0_7;
+        const double f27 = X[27] + ( - (r2_7*X[27]) + (r1_7*X[17]))*dtr + - plom_sqrt_rate((r2_7*X[27]))*d2_7+ plom_sqrt_rate((r1_7*X[17]))*d1_7;
+        s37 += ( + (r1_7*X[17]))*dtr + + plom_sqrt_rate((r1_7*X[17]))*d1_7;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
         X[17] = (f17 < 0.0) ? 0.0 : f17;
         const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
@@ -388,10 +388,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         double za22, zb22;
         plom_normal_pair64(ns, &za22, &zb22);
         const double w0_8 = sqdt * za22;
-        const double f8 = X[8] + ( - (r0_8*X[8]) + (r3_8))*dtr + - plom_sqrt((r0_8*X[8]))*d0_8+ plom_sqrt((r3_8))*d3_8 + - (r0_8*X[8])*P48*w0_8;
-        const double f18 = X[18] + ( - (r1_8*X[18]) + (r0_8*X[8]))*dtr + - plom_sqrt((r1_8*X[18]))*d1_8+ plom_sqrt((r0_8*X[8]))*d0_8 + + (r0_8*X[8])*P48*w0_8;
-        const double f28 = X[28] + ( - (r2_8*X[28]) + (r1_8*X[18]))*dtr + - plom_sqrt((r2_8*X[28]))*d2_8+ plom_sqrt((r1_8*X[18]))*d1_8;
-        s38 += ( + (r1_8*X[18]))*dtr + + plom_sqrt((r1_8*X[18]))*d1_8;
+        const double f8 = X[8] + ( - (r0_8*X[8]) + (r3_8))*dtr + - plom_sqrt_rate((r0_8*X[8]))*d0_8+ plom_sqrt_rate((r3_8))*d3_8 + - (r0_8*X[8])*P48*w0_8;
+        const double f18 = X[18] + ( - (r1_8*X[18]) + (r0_8*X[8]))*dtr + - plom_sqrt_rate((r1_8*X[18]))*d1_8+ plom_sqrt_rate((r0_8*X[8]))*d0_8 + + (r0_8*X[8])*P48*w0_8;
+        const double f28 = X[28] + ( - (r2_8*X[28]) + (r1_8*X[18]))*dtr + - plom_sqrt_rate((r2_8*X[28]))*d2_8+ plom_sqrt_rate((r1_8*X[18]))*d1_8;
+        s38 += ( + (r1_8*X[18]))*dtr + + plom_sqrt_rate((r1_8*X[18]))*d1_8;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
         X[18] = (f18 < 0.0) ? 0.0 : f18;
         const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
@@ -407,10 +407,10 @@ PLOM_HD void plom_seir10_interval_full_f64(double *X, const double *pv, plom_nst
         plom_normal_pair64(ns, &za24, &zb24);
         const double d3_9 = sqdt * za24;
         const double w0_9 = sqdt * zb24;
-        const double f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr + - plom_sqrt((r0_9*X[9]))*d0_9+ plom_sqrt((r3_9))*d3_9 + - (r0_9*X[9])*P49*w0_9;
-        const double f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr + - plom_sqrt((r1_9*X[19]))*d1_9+ plom_sqrt((r0_9*X[9]))*d0_9 + + (r0_9*X[9])*P49*w0_9;
-        const double f29 = X[29] + ( - (r2_9*X[29]) + (r1_9*X[19]))*dtr + - plom_sqrt((r2_9*X[29]))*d2_9+ plom_sqrt((r1_9*X[19]))*d1_9;
-        s39 += ( + (r1_9*X[19]))*dtr + + plom_sqrt((r1_9*X[19]))*d1_9;
+        const double f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr + - plom_sqrt_rate((r0_9*X[9]))*d0_9+ plom_sqrt_rate((r3_9))*d3_9 + - (r0_9*X[9])*P49*w0_9;
+        const double f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr + - plom_sqrt_rate((r1_9*X[19]))*d1_9+ plom_sqrt_rate((r0_9*X[9]))*d0_9 + + (r0_9*X[9])*P49*w0_9;
+        const double f29 = X[29] + ( - (r2_9*X[29]) + (r1_9*X[19]))*dtr + - plom_sqrt_rate((r2_9*X[29]))*d2_9+ plom_sqrt_rate((r1_9*X[19]))*d1_9;
+        s39 += ( + (r1_9*X[19]))*dtr + + plom_sqrt_rate((r1_9*X[19]))*d1_9;
         X[9] = (f9 < 0.0) ? 0.0 : f9;
         X[19] = (f19 < 0.0) ? 0.0 : f19;
         X[20] = (f20 < 0.0) ? 0.0 : f20;
@@ -981,10 +981,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         plom_normal_pair64(ns, &za1, &zb1);
         const double d2_0 = sqdt * za1;
         const double d3_0 = sqdt * zb1;
-        const double f0 = X[0] + ( - (r0_0*X[0]) + (r3_0))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0+ plom_sqrt((r3_0))*d3_0;
-        const double f10 = X[10] + ( - (r1_0*X[10]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[10]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
-        const double f20 = X[20] + ( - (r2_0*X[20]) + (r1_0*X[10]))*dtr + - plom_sqrt((r2_0*X[20]))*d2_0+ plom_sqrt((r1_0*X[10]))*d1_0;
-        s30 += ( + (r1_0*X[10]))*dtr + + plom_sqrt((r1_0*X[10]))*d1_0;
+        const double f0 = X[0] + ( - (r0_0*X[0]) + (r3_0))*dtr + - plom_sqrt_rate((r0_0*X[0]))*d0_0+ plom_sqrt_rate((r3_0))*d3_0;
+        const double f10 = X[10] + ( - (r1_0*X[10]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[10]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0;
+        const double f20 = X[20] + ( - (r2_0*X[20]) + (r1_0*X[10]))*dtr + - plom_sqrt_rate((r2_0*X[20]))*d2_0+ plom_sqrt_rate((r1_0*X[10]))*d1_0;
+        s30 += ( + (r1_0*X[10]))*dtr + + plom_sqrt_rate((r1_0*X[10]))*d1_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
         const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
@@ -999,10 +999,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         plom_normal_pair64(ns, &za3, &zb3);
         const double d2_1 = sqdt * za3;
         const double d3_1 = sqdt * zb3;
-        const double f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr + - plom_sqrt((r0_1*X[1]))*d0_1+ plom_sqrt((r3_1))*d3_1;
-        const double f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[11]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1;
-        const double f21 = X[21] + ( - (r2_1*X[21]) + (r1_1*X[11]))*dtr + - plom_sqrt((r2_1*X[21]))*d2_1+ plom_sqrt((r1_1*X[11]))*d1_1;
-        s31 += ( + (r1_1*X[11]))*dtr + + plom_sqrt((r1_1*X[11]))*d1_1;
+        const double f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr + - plom_sqrt_rate((r0_1*X[1]))*d0_1+ plom_sqrt_rate((r3_1))*d3_1;
+        const double f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr + - plom_sqrt_rate((r1_1*X[11]))*d1_1+ plom_sqrt_rate((r0_1*X[1]))*d0_1;
+        const double f21 = X[21] + ( - (r2_1*X[21]) + (r1_1*X[11]))*dtr + - plom_sqrt_rate((r2_1*X[21]))*d2_1+ plom_sqrt_rate((r1_1*X[11]))*d1_1;
+        s31 += ( + (r1_1*X[11]))*dtr + + plom_sqrt_rate((r1_1*X[11]))*d1_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
         const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
@@ -1017,10 +1017,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         plom_normal_pair64(ns, &za5, &zb5);
         const double d2_2 = sqdt * za5;
         const double d3_2 = sqdt * zb5;
-        const double f2 = X[2] + ( - (r0_2*X[2]) + (r3_2))*dtr + - plom_sqrt((r0_2*X[2]))*d0_2+ plom_sqrt((r3_2))*d3_2;
-        const double f12 = X[12] + ( - (r1_2*X[12]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[12]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2;
-        const double f22 = X[22] + ( - (r2_2*X[22]) + (r1_2*X[12]))*dtr + - plom_sqrt((r2_2*X[22]))*d2_2+ plom_sqrt((r1_2*X[12]))*d1_2;
-        s32 += ( + (r1_2*X[12]))*dtr + + plom_sqrt((r1_2*X[12]))*d1_2;
+        const double f2 = X[2] + ( - (r0_2*X[2]) + (r3_2))*dtr + - plom_sqrt_rate((r0_2*X[2]))*d0_2+ plom_sqrt_rate((r3_2))*d3_2;
+        const double f12 = X[12] + ( - (r1_2*X[12]) + (r0_2*X[2]))*dtr + - plom_sqrt_rate((r1_2*X[12]))*d1_2+ plom_sqrt_rate((r0_2*X[2]))*d0_2;
+        const double f22 = X[22] + ( - (r2_2*X[22]) + (r1_2*X[12]))*dtr + - plom_sqrt_rate((r2_2*X[22]))*d2_2+ plom_sqrt_rate((r1_2*X[12]))*d1_2;
+        s32 += ( + (r1_2*X[12]))*dtr + + plom_sqrt_rate((r1_2*X[12]))*d1_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         X[12] = (f12 < 0.0) ? 0.0 : f12;
         const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
@@ -1035,10 +1035,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         plom_normal_pair64(ns, &za7, &zb7);
         const double d2_3 = sqdt * za7;
         const double d3_3 = sqdt * zb7;
-        const double f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr + - plom_sqrt((r0_3*X[3]))*d0_3+ plom_sqrt((r3_3))*d3_3;
-        const double f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr + - plom_sqrt((r1_3*X[13]))*d1_3+ plom_sqrt((r0_3*X[3]))*d0_3;
-        const double f23 = X[23] + ( - (r2_3*X[23]) + (r1_3*X[13]))*dtr + - plom_sqrt((r2_3*X[23]))*d2_3+ plom_sqrt((r1_3*X[13]))*d1_3;
-        s33 += ( + (r1_3*X[13]))*dtr + + plom_sqrt((r1_3*X[13]))*d1_3;
+        const double f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr + - plom_sqrt_rate((r0_3*X[3]))*d0_3+ plom_sqrt_rate((r3_3))*d3_3;
+        const double f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr + - plom_sqrt_rate((r1_3*X[13]))*d1_3+ plom_sqrt_rate((r0_3*X[3]))*d0_3;
+        const double f23 = X[23] + ( - (r2_3*X[23]) + (r1_3*X[13]))*dtr + - plom_sqrt_rate((r2_3*X[23]))*d2_3+ plom_sqrt_rate((r1_3*X[13]))*d1_3;
+        s33 += ( + (r1_3*X[13]))*dtr + + plom_sqrt_rate((r1_3*X[13]))*d1_3;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[13] = (f13 < 0.0) ? 0.0 : f13;
         const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
@@ -1053,10 +1053,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         plom_normal_pair64(ns, &za9, &zb9);
         const double d2_4 = sqdt * za9;
         const double d3_4 = sqdt * zb9;
-        const double f4 = X[4] + ( - (r0_4*X[4]) + (r3_4))*dtr + - plom_sqrt((r0_4*X[4]))*d0_4+ plom_sqrt((r3_4))*d3_4;
-        const double f14 = X[14] + ( - (r1_4*X[14]) + (r0_4*X[4]))*dtr + - plom_sqrt((r1_4*X[14]))*d1_4+ plom_sqrt((r0_4*X[4]))*d0_4;
-        const double f24 = X[24] + ( - (r2_4*X[24]) + (r1_4*X[14]))*dtr + - plom_sqrt((r2_4*X[24]))*d2_4+ plom_sqrt((r1_4*X[14]))*d1_4;
-        s34 += ( + (r1_4*X[14]))*dtr + + plom_sqrt((r1_4*X[14]))*d1_4;
+        const double f4 = X[4] + ( - (r0_4*X[4]) + (r3_4))*dtr + - plom_sqrt_rate((r0_4*X[4]))*d0_4+ plom_sqrt_rate((r3_4))*d3_4;
+        const double f14 = X[14] + ( - (r1_4*X[14]) + (r0_4*X[4]))*dtr + - plom_sqrt_rate((r1_4*X[14]))*d1_4+ plom_sqrt_rate((r0_4*X[4]))*d0_4;
+        const double f24 = X[24] + ( - (r2_4*X[24]) + (r1_4*X[14]))*dtr + - plom_sqrt_rate((r2_4*X[24]))*d2_4+ plom_sqrt_rate((r1_4*X[14]))*d1_4;
+        s34 += ( + (r1_4*X[14]))*dtr + + plom_sqrt_rate((r1_4*X[14]))*d1_4;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[14] = (f14 < 0.0) ? 0.0 : f14;
         const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
@@ -1071,10 +1071,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         plom_normal_pair64(ns, &za11, &zb11);
         const double d2_5 = sqdt * za11;
         const double d3_5 = sqdt * zb11;
-        const double f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr + - plom_sqrt((r0_5*X[5]))*d0_5+ plom_sqrt((r3_5))*d3_5;
-        const double f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr + - plom_sqrt((r1_5*X[15]))*d1_5+ plom_sqrt((r0_5*X[5]))*d0_5;
-        const double f25 = X[25] + ( - (r2_5*X[25]) + (r1_5*X[15]))*dtr + - plom_sqrt((r2_5*X[25]))*d2_5+ plom_sqrt((r1_5*X[15]))*d1_5;
-        s35 += ( + (r1_5*X[15]))*dtr + + plom_sqrt((r1_5*X[15]))*d1_5;
+        const double f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr + - plom_sqrt_rate((r0_5*X[5]))*d0_5+ plom_sqrt_rate((r3_5))*d3_5;
+        const double f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr + - plom_sqrt_rate((r1_5*X[15]))*d1_5+ plom_sqrt_rate((r0_5*X[5]))*d0_5;
+        const double f25 = X[25] + ( - (r2_5*X[25]) + (r1_5*X[15]))*dtr + - plom_sqrt_rate((r2_5*X[25]))*d2_5+ plom_sqrt_rate((r1_5*X[15]))*d1_5;
+        s35 += ( + (r1_5*X[15]))*dtr + + plom_sqrt_rate((r1_5*X[15]))*d1_5;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[15] = (f15 < 0.0) ? 0.0 : f15;
         const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
@@ -1089,10 +1089,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         plom_normal_pair64(ns, &za13, &zb13);
         const double d2_6 = sqdt * za13;
         const double d3_6 = sqdt * zb13;
-        const double f6 = X[6] + ( - (r0_6*X[6]) + (r3_6))*dtr + - plom_sqrt((r0_6*X[6]))*d0_6+ plom_sqrt((r3_6))*d3_6;
-        const double f16 = X[16] + ( - (r1_6*X[16]) + (r0_6*X[6]))*dtr + - plom_sqrt((r1_6*X[16]))*d1_6+ plom_sqrt((r0_6*X[6]))*d0_6;
-        const double f26 = X[26] + ( - (r2_6*X[26]) + (r1_6*X[16]))*dtr + - plom_sqrt((r2_6*X[26]))*d2_6+ plom_sqrt((r1_6*X[16]))*d1_6;
-        s36 += ( + (r1_6*X[16]))*dtr + + plom_sqrt((r1_6*X[16]))*d1_6;
+        const double f6 = X[6] + ( - (r0_6*X[6]) + (r3_6))*dtr + - plom_sqrt_rate((r0_6*X[6]))*d0_6+ plom_sqrt_rate((r3_6))*d3_6;
+        const double f16 = X[16] + ( - (r1_6*X[16]) + (r0_6*X[6]))*dtr + - plom_sqrt_rate((r1_6*X[16]))*d1_6+ plom_sqrt_rate((r0_6*X[6]))*d0_6;
+        const double f26 = X[26] + ( - (r2_6*X[26]) + (r1_6*X[16]))*dtr + - plom_sqrt_rate((r2_6*X[26]))*d2_6+ plom_sqrt_rate((r1_6*X[16]))*d1_6;
+        s36 += ( + (r1_6*X[16]))*dtr + + plom_sqrt_rate((r1_6*X[16]))*d1_6;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[16] = (f16 < 0.0) ? 0.0 : f16;
         const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
@@ -1107,10 +1107,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         plom_normal_pair64(ns, &za15, &zb15);
         const double d2_7 = sqdt * za15;
         const double d3_7 = sqdt * zb15;
-        const double f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr + - plom_sqrt((r0_7*X[7]))*d0_7+ plom_sqrt((r3_7))*d3_7;
-        const double f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr + - plom_sqrt((r1_7*X[17]))*d1_7+ plom_sqrt((r0_7*X[7]))*d0_7;
-        const double f27 = X[27] + ( - (r2_7*X[27]) + (r1_7*X[17]))*dtr + - plom_sqrt((r2_7*X[27]))*d2_7+ plom_sqrt((r1_7*X[17]))*d1_7;
-        s37 += ( + (r1_7*X[17]))*dtr + + plom_sqrt((r1_7*X[17]))*d1_7;
+        const double f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr + - plom_sqrt_rate((r0_7*X[7]))*d0_7+ plom_sqrt_rate((r3_7))*d3_7;
+        const double f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr + - plom_sqrt_rate((r1_7*X[17]))*d1_7+ plom_sqrt_rate((r0_7*X[7]))*d0_7;
+        const double f27 = X[27] + ( - (r2_7*X[27]) + (r1_7*X[17]))*dtr + - plom_sqrt_rate((r2_7*X[27]))*d2_7+ plom_sqrt_rate((r1_7*X[17]))*d1_7;
+        s37 += ( + (r1_7*X[17]))*dtr + + plom_sqrt_rate((r1_7*X[17]))*d1_7;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
         X[17] = (f17 < 0.0) ? 0.0 : f17;
         const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
@@ -1125,10 +1125,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         plom_normal_pair64(ns, &za17, &zb17);
         const double d2_8 = sqdt * za17;
         const double d3_8 = sqdt * zb17;
-        const double f8 = X[8] + ( - (r0_8*X[8]) + (r3_8))*dtr + - plom_sqrt((r0_8*X[8]))*d0_8+ plom_sqrt((r3_8))*d3_8;
-        const double f18 = X[18] + ( - (r1_8*X[18]) + (r0_8*X[8]))*dtr + - plom_sqrt((r1_8*X[18]))*d1_8+ plom_sqrt((r0_8*X[8]))*d0_8;
-        const double f28 = X[28] + ( - (r2_8*X[28]) + (r1_8*X[18]))*dtr + - plom_sqrt((r2_8*X[28]))*d2_8+ plom_sqrt((r1_8*X[18]))*d1_8;
-        s38 += ( + (r1_8*X[18]))*dtr + + plom_sqrt((r1_8*X[18]))*d1_8;
+        const double f8 = X[8] + ( - (r0_8*X[8]) + (r3_8))*dtr + - plom_sqrt_rate((r0_8*X[8]))*d0_8+ plom_sqrt_rate((r3_8))*d3_8;
+        const double f18 = X[18] + ( - (r1_8*X[18]) + (r0_8*X[8]))*dtr + - plom_sqrt_rate((r1_8*X[18]))*d1_8+ plom_sqrt_rate((r0_8*X[8]))*d0_8;
+        const double f28 = X[28] + ( - (r2_8*X[28]) + (r1_8*X[18]))*dtr + - plom_sqrt_rate((r2_8*X[28]))*d2_8+ plom_sqrt_rate((r1_8*X[18]))*d1_8;
+        s38 += ( + (r1_8*X[18]))*dtr + + plom_sqrt_rate((r1_8*X[18]))*d1_8;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
         X[18] = (f18 < 0.0) ? 0.0 : f18;
         const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
@@ -1143,10 +1143,10 @@ PLOM_HD void plom_seir10_interval_no_env_sto_f64(double *X, const double *pv, pl
         plom_normal_pair64(ns, &za19, &zb19);
         const double d2_9 = sqdt * za19;
         const double d3_9 = sqdt * zb19;
-        const double f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr + - plom_sqrt((r0_9*X[9]))*d0_9+ plom_sqrt((r3_9))*d3_9;
-        const double f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr + - plom_sqrt((r1_9*X[19]))*d1_9+ plom_sqrt((r0_9*X[9]))*d0_9;
-        const double f29 = X[29] + ( - (r2_9*X[29]) + (r1_9*X[19]))*dtr + - plom_sqrt((r2_9*X[29]))*d2_9+ plom_sqrt((r1_9*X[19]))*d1_9;
-        s39 += ( + (r1_9*X[19]))*dtr + + plom_sqrt((r1_9*X[19]))*d1_9;
+        const double f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr + - plom_sqrt_rate((r0_9*X[9]))*d0_9+ plom_sqrt_rate((r3_9))*d3_9;
+        const double f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr + - plom_sqrt_rate((r1_9*X[19]))*d1_9+ plom_sqrt_rate((r0_9*X[9]))*d0_9;
+        const double f29 = X[29] + ( - (r2_9*X[29]) + (r1_9*X[19]))*dtr + - plom_sqrt_rate((r2_9*X[29]))*d2_9+ plom_sqrt_rate((r1_9*X[19]))*d1_9;
+        s39 += ( + (r1_9*X[19]))*dtr + + plom_sqrt_rate((r1_9*X[19]))*d1_9;
         X[9] = (f9 < 0.0) ? 0.0 : f9;
         X[19] = (f19 < 0.0) ? 0.0 : f19;
         X[20] = (f20 < 0.0) ? 0.0 : f20;
@@ -7254,10 +7254,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         double za2, zb2;
         plom_normal_pair64(ns, &za2, &zb2);
         const double w0_0 = sqdt * za2;
-        const double f0 = X[0] + ( - (r0_0*X[0]) + (r3_0))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0+ plom_sqrt((r3_0))*d3_0 + - (r0_0*X[0])*P40*w0_0;
-        const double f10 = X[10] + ( - (r1_0*X[10]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[10]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P40*w0_0;
-        const double f20 = X[20] + ( - (r2_0*X[20]) + (r1_0*X[10]))*dtr + - plom_sqrt((r2_0*X[20]))*d2_0+ plom_sqrt((r1_0*X[10]))*d1_0;
-        s30 += ( + (r1_0*X[10]))*dtr + + plom_sqrt((r1_0*X[10]))*d1_0;
+        const double f0 = X[0] + ( - (r0_0*X[0]) + (r3_0))*dtr + - plom_sqrt_rate((r0_0*X[0]))*d0_0+ plom_sqrt_rate((r3_0))*d3_0 + - (r0_0*X[0])*P40*w0_0;
+        const double f10 = X[10] + ( - (r1_0*X[10]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[10]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P40*w0_0;
+        const double f20 = X[20] + ( - (r2_0*X[20]) + (r1_0*X[10]))*dtr + - plom_sqrt_rate((r2_0*X[20]))*d2_0+ plom_sqrt_rate((r1_0*X[10]))*d1_0;
+        s30 += ( + (r1_0*X[10]))*dtr + + plom_sqrt_rate((r1_0*X[10]))*d1_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
         const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
@@ -7273,10 +7273,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         plom_normal_pair64(ns, &za4, &zb4);
         const double d3_1 = sqdt * za4;
         const double w0_1 = sqdt * zb4;
-        const double f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr + - plom_sqrt((r0_1*X[1]))*d0_1+ plom_sqrt((r3_1))*d3_1 + - (r0_1*X[1])*P41*w0_1;
-        const double f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[11]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P41*w0_1;
-        const double f21 = X[21] + ( - (r2_1*X[21]) + (r1_1*X[11]))*dtr + - plom_sqrt((r2_1*X[21]))*d2_1+ plom_sqrt((r1_1*X[11]))*d1_1;
-        s31 += ( + (r1_1*X[11]))*dtr + + plom_sqrt((r1_1*X[11]))*d1_1;
+        const double f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr + - plom_sqrt_rate((r0_1*X[1]))*d0_1+ plom_sqrt_rate((r3_1))*d3_1 + - (r0_1*X[1])*P41*w0_1;
+        const double f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr + - plom_sqrt_rate((r1_1*X[11]))*d1_1+ plom_sqrt_rate((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P41*w0_1;
+        const double f21 = X[21] + ( - (r2_1*X[21]) + (r1_1*X[11]))*dtr + - plom_sqrt_rate((r2_1*X[21]))*d2_1+ plom_sqrt_rate((r1_1*X[11]))*d1_1;
+        s31 += ( + (r1_1*X[11]))*dtr + + plom_sqrt_rate((r1_1*X[11]))*d1_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
         const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
@@ -7294,10 +7294,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         double za7, zb7;
         plom_normal_pair64(ns, &za7, &zb7);
         const double w0_2 = sqdt * za7;
-        const double f2 = X[2] + ( - (r0_2*X[2]) + (r3_2))*dtr + - plom_sqrt((r0_2*X[2]))*d0_2+ plom_sqrt((r3_2))*d3_2 + - (r0_2*X[2])*P42*w0_2;
-        const double f12 = X[12] + ( - (r1_2*X[12]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[12]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P42*w0_2;
-        const double f22 = X[22] + ( - (r2_2*X[22]) + (r1_2*X[12]))*dtr + - plom_sqrt((r2_2*X[22]))*d2_2+ plom_sqrt((r1_2*X[12]))*d1_2;
-        s32 += ( + (r1_2*X[12]))*dtr + + plom_sqrt((r1_2*X[12]))*d1_2;
+        const double f2 = X[2] + ( - (r0_2*X[2]) + (r3_2))*dtr + - plom_sqrt_rate((r0_2*X[2]))*d0_2+ plom_sqrt_rate((r3_2))*d3_2 + - (r0_2*X[2])*P42*w0_2;
+        const double f12 = X[12] + ( - (r1_2*X[12]) + (r0_2*X[2]))*dtr + - plom_sqrt_rate((r1_2*X[12]))*d1_2+ plom_sqrt_rate((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P42*w0_2;
+        const double f22 = X[22] + ( - (r2_2*X[22]) + (r1_2*X[12]))*dtr + - plom_sqrt_rate((r2_2*X[22]))*d2_2+ plom_sqrt_rate((r1_2*X[12]))*d1_2;
+        s32 += ( + (r1_2*X[12]))*dtr + + plom_sqrt_rate((r1_2*X[12]))*d1_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         X[12] = (f12 < 0.0) ? 0.0 : f12;
         const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
@@ -7313,10 +7313,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         plom_normal_pair64(ns, &za9, &zb9);
         const double d3_3 = sqdt * za9;
         const double w0_3 = sqdt * zb9;
-        const double f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr + - plom_sqrt((r0_3*X[3]))*d0_3+ plom_sqrt((r3_3))*d3_3 + - (r0_3*X[3])*P43*w0_3;
-        const double f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr + - plom_sqrt((r1_3*X[13]))*d1_3+ plom_sqrt((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P43*w0_3;
-        const double f23 = X[23] + ( - (r2_3*X[23]) + (r1_3*X[13]))*dtr + - plom_sqrt((r2_3*X[23]))*d2_3+ plom_sqrt((r1_3*X[13]))*d1_3;
-        s33 += ( + (r1_3*X[13]))*dtr + + plom_sqrt((r1_3*X[13]))*d1_3;
+        const double f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr + - plom_sqrt_rate((r0_3*X[3]))*d0_3+ plom_sqrt_rate((r3_3))*d3_3 + - (r0_3*X[3])*P43*w0_3;
+        const double f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr + - plom_sqrt_rate((r1_3*X[13]))*d1_3+ plom_sqrt_rate((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P43*w0_3;
+        const double f23 = X[23] + ( - (r2_3*X[23]) + (r1_3*X[13]))*dtr + - plom_sqrt_rate((r2_3*X[23]))*d2_3+ plom_sqrt_rate((r1_3*X[13]))*d1_3;
+        s33 += ( + (r1_3*X[13]))*dtr + + plom_sqrt_rate((r1_3*X[13]))*d1_3;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[13] = (f13 < 0.0) ? 0.0 : f13;
         const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
@@ -7334,10 +7334,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         double za12, zb12;
         plom_normal_pair64(ns, &za12, &zb12);
         const double w0_4 = sqdt * za12;
-        const double f4 = X[4] + ( - (r0_4*X[4]) + (r3_4))*dtr + - plom_sqrt((r0_4*X[4]))*d0_4+ plom_sqrt((r3_4))*d3_4 + - (r0_4*X[4])*P44*w0_4;
-        const double f14 = X[14] + ( - (r1_4*X[14]) + (r0_4*X[4]))*dtr + - plom_sqrt((r1_4*X[14]))*d1_4+ plom_sqrt((r0_4*X[4]))*d0_4 + + (r0_4*X[4])*P44*w0_4;
-        const double f24 = X[24] + ( - (r2_4*X[24]) + (r1_4*X[14]))*dtr + - plom_sqrt((r2_4*X[24]))*d2_4+ plom_sqrt((r1_4*X[14]))*d1_4;
-        s34 += ( + (r1_4*X[14]))*dtr + + plom_sqrt((r1_4*X[14]))*d1_4;
+        const double f4 = X[4] + ( - (r0_4*X[4]) + (r3_4))*dtr + - plom_sqrt_rate((r0_4*X[4]))*d0_4+ plom_sqrt_rate((r3_4))*d3_4 + - (r0_4*X[4])*P44*w0_4;
+        const double f14 = X[14] + ( - (r1_4*X[14]) + (r0_4*X[4]))*dtr + - plom_sqrt_rate((r1_4*X[14]))*d1_4+ plom_sqrt_rate((r0_4*X[4]))*d0_4 + + (r0_4*X[4])*P44*w0_4;
+        const double f24 = X[24] + ( - (r2_4*X[24]) + (r1_4*X[14]))*dtr + - plom_sqrt_rate((r2_4*X[24]))*d2_4+ plom_sqrt_rate((r1_4*X[14]))*d1_4;
+        s34 += ( + (r1_4*X[14]))*dtr + + plom_sqrt_rate((r1_4*X[14]))*d1_4;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[14] = (f14 < 0.0) ? 0.0 : f14;
         const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
@@ -7353,10 +7353,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         plom_normal_pair64(ns, &za14, &zb14);
         const double d3_5 = sqdt * za14;
         const double w0_5 = sqdt * zb14;
-        const double f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr + - plom_sqrt((r0_5*X[5]))*d0_5+ plom_sqrt((r3_5))*d3_5 + - (r0_5*X[5])*P45*w0_5;
-        const double f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr + - plom_sqrt((r1_5*X[15]))*d1_5+ plom_sqrt((r0_5*X[5]))*d0_5 + + (r0_5*X[5])*P45*w0_5;
-        const double f25 = X[25] + ( - (r2_5*X[25]) + (r1_5*X[15]))*dtr + - plom_sqrt((r2_5*X[25]))*d2_5+ plom_sqrt((r1_5*X[15]))*d1_5;
-        s35 += ( + (r1_5*X[15]))*dtr + + plom_sqrt((r1_5*X[15]))*d1_5;
+        const double f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr + - plom_sqrt_rate((r0_5*X[5]))*d0_5+ plom_sqrt_rate((r3_5))*d3_5 + - (r0_5*X[5])*P45*w0_5;
+        const double f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr + - plom_sqrt_rate((r1_5*X[15]))*d1_5+ plom_sqrt_rate((r0_5*X[5]))*d0_5 + + (r0_5*X[5])*P45*w0_5;
+        const double f25 = X[25] + ( - (r2_5*X[25]) + (r1_5*X[15]))*dtr + - plom_sqrt_rate((r2_5*X[25]))*d2_5+ plom_sqrt_rate((r1_5*X[15]))*d1_5;
+        s35 += ( + (r1_5*X[15]))*dtr + + plom_sqrt_rate((r1_5*X[15]))*d1_5;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[15] = (f15 < 0.0) ? 0.0 : f15;
         const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
@@ -7374,10 +7374,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         double za17, zb17;
         plom_normal_pair64(ns, &za17, &zb17);
         const double w0_6 = sqdt * za17;
-        const double f6 = X[6] + ( - (r0_6*X[6]) + (r3_6))*dtr + - plom_sqrt((r0_6*X[6]))*d0_6+ plom_sqrt((r3_6))*d3_6 + - (r0_6*X[6])*P46*w0_6;
-        const double f16 = X[16] + ( - (r1_6*X[16]) + (r0_6*X[6]))*dtr + - plom_sqrt((r1_6*X[16]))*d1_6+ plom_sqrt((r0_6*X[6]))*d0_6 + + (r0_6*X[6])*P46*w0_6;
-        const double f26 = X[26] + ( - (r2_6*X[26]) + (r1_6*X[16]))*dtr + - plom_sqrt((r2_6*X[26]))*d2_6+ plom_sqrt((r1_6*X[16]))*d1_6;
-        s36 += ( + (r1_6*X[16]))*dtr + + plom_sqrt((r1_6*X[16]))*d1_6;
+        const double f6 = X[6] + ( - (r0_6*X[6]) + (r3_6))*dtr + - plom_sqrt_rate((r0_6*X[6]))*d0_6+ plom_sqrt_rate((r3_6))*d3_6 + - (r0_6*X[6])*P46*w0_6;
+        const double f16 = X[16] + ( - (r1_6*X[16]) + (r0_6*X[6]))*dtr + - plom_sqrt_rate((r1_6*X[16]))*d1_6+ plom_sqrt_rate((r0_6*X[6]))*d0_6 + + (r0_6*X[6])*P46*w0_6;
+        const double f26 = X[26] + ( - (r2_6*X[26]) + (r1_6*X[16]))*dtr + - plom_sqrt_rate((r2_6*X[26]))*d2_6+ plom_sqrt_rate((r1_6*X[16]))*d1_6;
+        s36 += ( + (r1_6*X[16]))*dtr + + plom_sqrt_rate((r1_6*X[16]))*d1_6;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[16] = (f16 < 0.0) ? 0.0 : f16;
         const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
@@ -7393,10 +7393,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         plom_normal_pair64(ns, &za19, &zb19);
         const double d3_7 = sqdt * za19;
         const double w0_7 = sqdt * zb19;
-        const double f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr + - plom_sqrt((r0_7*X[7]))*d0_7+ plom_sqrt((r3_7))*d3_7 + - (r0_7*X[7])*P47*w0_7;
-        const double f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr + - plom_sqrt((r1_7*X[17]))*d1_7+ plom_sqrt((r0_7*X[7]))*d0_7 + + (r0_7*X[7])*P47*w0_7;
-        const double f27 = X[27] + ( - (r2_7*X[27]) + (r1_7*X[17]))*dtr + - plom_sqrt((r2_7*X[27]))*d2_7+ plom_sqrt((r1_7*X[17]))*d1_7;
-        s37 += ( + (r1_7*X[17]))*dtr + + plom_sqrt((r1_7*X[17]))*d1_7;
+        const double f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr + - plom_sqrt_rate((r0_7*X[7]))*d0_7+ plom_sqrt_rate((r3_7))*d3_7 + - (r0_7*X[7])*P47*w0_7;
+        const double f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr + - plom_sqrt_rate((r1_7*X[17]))*d1_7+ plom_sqrt_rate((r0_7*X[7]))*d0_7 + + (r0_7*X[7])*P47*w0_7;
+        const double f27 = X[27] + ( - (r2_7*X[27]) + (r1_7*X[17]))*dtr + - plom_sqrt_rate((r2_7*X[27]))*d2_7+ plom_sqrt_rate((r1_7*X[17]))*d1_7;
+        s37 += ( + (r1_7*X[17]))*dtr + + plom_sqrt_rate((r1_7*X[17]))*d1_7;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
         X[17] = (f17 < 0.0) ? 0.0 : f17;
         const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
@@ -7414,10 +7414,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         double za22, zb22;
         plom_normal_pair64(ns, &za22, &zb22);
         const double w0_8 = sqdt * za22;
-        const double f8 = X[8] + ( - (r0_8*X[8]) + (r3_8))*dtr + - plom_sqrt((r0_8*X[8]))*d0_8+ plom_sqrt((r3_8))*d3_8 + - (r0_8*X[8])*P48*w0_8;
-        const double f18 = X[18] + ( - (r1_8*X[18]) + (r0_8*X[8]))*dtr + - plom_sqrt((r1_8*X[18]))*d1_8+ plom_sqrt((r0_8*X[8]))*d0_8 + + (r0_8*X[8])*P48*w0_8;
-        const double f28 = X[28] + ( - (r2_8*X[28]) + (r1_8*X[18]))*dtr + - plom_sqrt((r2_8*X[28]))*d2_8+ plom_sqrt((r1_8*X[18]))*d1_8;
-        s38 += ( + (r1_8*X[18]))*dtr + + plom_sqrt((r1_8*X[18]))*d1_8;
+        const double f8 = X[8] + ( - (r0_8*X[8]) + (r3_8))*dtr + - plom_sqrt_rate((r0_8*X[8]))*d0_8+ plom_sqrt_rate((r3_8))*d3_8 + - (r0_8*X[8])*P48*w0_8;
+        const double f18 = X[18] + ( - (r1_8*X[18]) + (r0_8*X[8]))*dtr + - plom_sqrt_rate((r1_8*X[18]))*d1_8+ plom_sqrt_rate((r0_8*X[8]))*d0_8 + + (r0_8*X[8])*P48*w0_8;
+        const double f28 = X[28] + ( - (r2_8*X[28]) + (r1_8*X[18]))*dtr + - plom_sqrt_rate((r2_8*X[28]))*d2_8+ plom_sqrt_rate((r1_8*X[18]))*d1_8;
+        s38 += ( + (r1_8*X[18]))*dtr + + plom_sqrt_rate((r1_8*X[18]))*d1_8;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
         X[18] = (f18 < 0.0) ? 0.0 : f18;
         const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
@@ -7433,10 +7433,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         plom_normal_pair64(ns, &za24, &zb24);
         const double d3_9 = sqdt * za24;
         const double w0_9 = sqdt * zb24;
-        const double f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr + - plom_sqrt((r0_9*X[9]))*d0_9+ plom_sqrt((r3_9))*d3_9 + - (r0_9*X[9])*P49*w0_9;
-        const double f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr + - plom_sqrt((r1_9*X[19]))*d1_9+ plom_sqrt((r0_9*X[9]))*d0_9 + + (r0_9*X[9])*P49*w0_9;
-        const double f29 = X[29] + ( - (r2_9*X[29]) + (r1_9*X[19]))*dtr + - plom_sqrt((r2_9*X[29]))*d2_9+ plom_sqrt((r1_9*X[19]))*d1_9;
-        s39 += ( + (r1_9*X[19]))*dtr + + plom_sqrt((r1_9*X[19]))*d1_9;
+        const double f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr + - plom_sqrt_rate((r0_9*X[9]))*d0_9+ plom_sqrt_rate((r3_9))*d3_9 + - (r0_9*X[9])*P49*w0_9;
+        const double f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr + - plom_sqrt_rate((r1_9*X[19]))*d1_9+ plom_sqrt_rate((r0_9*X[9]))*d0_9 + + (r0_9*X[9])*P49*w0_9;
+        const double f29 = X[29] + ( - (r2_9*X[29]) + (r1_9*X[19]))*dtr + - plom_sqrt_rate((r2_9*X[29]))*d2_9+ plom_sqrt_rate((r1_9*X[19]))*d1_9;
+        s39 += ( + (r1_9*X[19]))*dtr + + plom_sqrt_rate((r1_9*X[19]))*d1_9;
         X[9] = (f9 < 0.0) ? 0.0 : f9;
         X[19] = (f19 < 0.0) ? 0.0 : f19;
         X[20] = (f20 < 0.0) ? 0.0 : f20;
@@ -8007,10 +8007,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         plom_normal_pair64(ns, &za1, &zb1);
         const double d2_0 = sqdt * za1;
         const double d3_0 = sqdt * zb1;
-        const double f0 = X[0] + ( - (r0_0*X[0]) + (r3_0))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0+ plom_sqrt((r3_0))*d3_0;
-        const double f10 = X[10] + ( - (r1_0*X[10]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[10]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
-        const double f20 = X[20] + ( - (r2_0*X[20]) + (r1_0*X[10]))*dtr + - plom_sqrt((r2_0*X[20]))*d2_0+ plom_sqrt((r1_0*X[10]))*d1_0;
-        s30 += ( + (r1_0*X[10]))*dtr + + plom_sqrt((r1_0*X[10]))*d1_0;
+        const double f0 = X[0] + ( - (r0_0*X[0]) + (r3_0))*dtr + - plom_sqrt_rate((r0_0*X[0]))*d0_0+ plom_sqrt_rate((r3_0))*d3_0;
+        const double f10 = X[10] + ( - (r1_0*X[10]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[10]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0;
+        const double f20 = X[20] + ( - (r2_0*X[20]) + (r1_0*X[10]))*dtr + - plom_sqrt_rate((r2_0*X[20]))*d2_0+ plom_sqrt_rate((r1_0*X[10]))*d1_0;
+        s30 += ( + (r1_0*X[10]))*dtr + + plom_sqrt_rate((r1_0*X[10]))*d1_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[10] = (f10 < 0.0) ? 0.0 : f10;
         const double r0_1 = ((P1 * (P90 * X[20] + P91 * X[21] + P92 * X[22] + P93 * X[23] + P94 * X[24] + P95 * X[25] + P96 * X[26] + P97 * X[27] + P98 * X[28] + P99 * X[29])) / C1);
@@ -8025,10 +8025,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         plom_normal_pair64(ns, &za3, &zb3);
         const double d2_1 = sqdt * za3;
         const double d3_1 = sqdt * zb3;
-        const double f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr + - plom_sqrt((r0_1*X[1]))*d0_1+ plom_sqrt((r3_1))*d3_1;
-        const double f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[11]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1;
-        const double f21 = X[21] + ( - (r2_1*X[21]) + (r1_1*X[11]))*dtr + - plom_sqrt((r2_1*X[21]))*d2_1+ plom_sqrt((r1_1*X[11]))*d1_1;
-        s31 += ( + (r1_1*X[11]))*dtr + + plom_sqrt((r1_1*X[11]))*d1_1;
+        const double f1 = X[1] + ( - (r0_1*X[1]) + (r3_1))*dtr + - plom_sqrt_rate((r0_1*X[1]))*d0_1+ plom_sqrt_rate((r3_1))*d3_1;
+        const double f11 = X[11] + ( - (r1_1*X[11]) + (r0_1*X[1]))*dtr + - plom_sqrt_rate((r1_1*X[11]))*d1_1+ plom_sqrt_rate((r0_1*X[1]))*d0_1;
+        const double f21 = X[21] + ( - (r2_1*X[21]) + (r1_1*X[11]))*dtr + - plom_sqrt_rate((r2_1*X[21]))*d2_1+ plom_sqrt_rate((r1_1*X[11]))*d1_1;
+        s31 += ( + (r1_1*X[11]))*dtr + + plom_sqrt_rate((r1_1*X[11]))*d1_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         X[11] = (f11 < 0.0) ? 0.0 : f11;
         const double r0_2 = ((P2 * (P100 * X[20] + P101 * X[21] + P102 * X[22] + P103 * X[23] + P104 * X[24] + P105 * X[25] + P106 * X[26] + P107 * X[27] + P108 * X[28] + P109 * X[29])) / C2);
@@ -8043,10 +8043,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         plom_normal_pair64(ns, &za5, &zb5);
         const double d2_2 = sqdt * za5;
         const double d3_2 = sqdt * zb5;
-        const double f2 = X[2] + ( - (r0_2*X[2]) + (r3_2))*dtr + - plom_sqrt((r0_2*X[2]))*d0_2+ plom_sqrt((r3_2))*d3_2;
-        const double f12 = X[12] + ( - (r1_2*X[12]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[12]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2;
-        const double f22 = X[22] + ( - (r2_2*X[22]) + (r1_2*X[12]))*dtr + - plom_sqrt((r2_2*X[22]))*d2_2+ plom_sqrt((r1_2*X[12]))*d1_2;
-        s32 += ( + (r1_2*X[12]))*dtr + + plom_sqrt((r1_2*X[12]))*d1_2;
+        const double f2 = X[2] + ( - (r0_2*X[2]) + (r3_2))*dtr + - plom_sqrt_rate((r0_2*X[2]))*d0_2+ plom_sqrt_rate((r3_2))*d3_2;
+        const double f12 = X[12] + ( - (r1_2*X[12]) + (r0_2*X[2]))*dtr + - plom_sqrt_rate((r1_2*X[12]))*d1_2+ plom_sqrt_rate((r0_2*X[2]))*d0_2;
+        const double f22 = X[22] + ( - (r2_2*X[22]) + (r1_2*X[12]))*dtr + - plom_sqrt_rate((r2_2*X[22]))*d2_2+ plom_sqrt_rate((r1_2*X[12]))*d1_2;
+        s32 += ( + (r1_2*X[12]))*dtr + + plom_sqrt_rate((r1_2*X[12]))*d1_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         X[12] = (f12 < 0.0) ? 0.0 : f12;
         const double r0_3 = ((P3 * (P110 * X[20] + P111 * X[21] + P112 * X[22] + P113 * X[23] + P114 * X[24] + P115 * X[25] + P116 * X[26] + P117 * X[27] + P118 * X[28] + P119 * X[29])) / C3);
@@ -8061,10 +8061,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         plom_normal_pair64(ns, &za7, &zb7);
         const double d2_3 = sqdt * za7;
         const double d3_3 = sqdt * zb7;
-        const double f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr + - plom_sqrt((r0_3*X[3]))*d0_3+ plom_sqrt((r3_3))*d3_3;
-        const double f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr + - plom_sqrt((r1_3*X[13]))*d1_3+ plom_sqrt((r0_3*X[3]))*d0_3;
-        const double f23 = X[23] + ( - (r2_3*X[23]) + (r1_3*X[13]))*dtr + - plom_sqrt((r2_3*X[23]))*d2_3+ plom_sqrt((r1_3*X[13]))*d1_3;
-        s33 += ( + (r1_3*X[13]))*dtr + + plom_sqrt((r1_3*X[13]))*d1_3;
+        const double f3 = X[3] + ( - (r0_3*X[3]) + (r3_3))*dtr + - plom_sqrt_rate((r0_3*X[3]))*d0_3+ plom_sqrt_rate((r3_3))*d3_3;
+        const double f13 = X[13] + ( - (r1_3*X[13]) + (r0_3*X[3]))*dtr + - plom_sqrt_rate((r1_3*X[13]))*d1_3+ plom_sqrt_rate((r0_3*X[3]))*d0_3;
+        const double f23 = X[23] + ( - (r2_3*X[23]) + (r1_3*X[13]))*dtr + - plom_sqrt_rate((r2_3*X[23]))*d2_3+ plom_sqrt_rate((r1_3*X[13]))*d1_3;
+        s33 += ( + (r1_3*X[13]))*dtr + + plom_sqrt_rate((r1_3*X[13]))*d1_3;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[13] = (f13 < 0.0) ? 0.0 : f13;
         const double r0_4 = ((P4 * (P120 * X[20] + P121 * X[21] + P122 * X[22] + P123 * X[23] + P124 * X[24] + P125 * X[25] + P126 * X[26] + P127 * X[27] + P128 * X[28] + P129 * X[29])) / C4);
@@ -8079,10 +8079,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         plom_normal_pair64(ns, &za9, &zb9);
         const double d2_4 = sqdt * za9;
         const double d3_4 = sqdt * zb9;
-        const double f4 = X[4] + ( - (r0_4*X[4]) + (r3_4))*dtr + - plom_sqrt((r0_4*X[4]))*d0_4+ plom_sqrt((r3_4))*d3_4;
-        const double f14 = X[14] + ( - (r1_4*X[14]) + (r0_4*X[4]))*dtr + - plom_sqrt((r1_4*X[14]))*d1_4+ plom_sqrt((r0_4*X[4]))*d0_4;
-        const double f24 = X[24] + ( - (r2_4*X[24]) + (r1_4*X[14]))*dtr + - plom_sqrt((r2_4*X[24]))*d2_4+ plom_sqrt((r1_4*X[14]))*d1_4;
-        s34 += ( + (r1_4*X[14]))*dtr + + plom_sqrt((r1_4*X[14]))*d1_4;
+        const double f4 = X[4] + ( - (r0_4*X[4]) + (r3_4))*dtr + - plom_sqrt_rate((r0_4*X[4]))*d0_4+ plom_sqrt_rate((r3_4))*d3_4;
+        const double f14 = X[14] + ( - (r1_4*X[14]) + (r0_4*X[4]))*dtr + - plom_sqrt_rate((r1_4*X[14]))*d1_4+ plom_sqrt_rate((r0_4*X[4]))*d0_4;
+        const double f24 = X[24] + ( - (r2_4*X[24]) + (r1_4*X[14]))*dtr + - plom_sqrt_rate((r2_4*X[24]))*d2_4+ plom_sqrt_rate((r1_4*X[14]))*d1_4;
+        s34 += ( + (r1_4*X[14]))*dtr + + plom_sqrt_rate((r1_4*X[14]))*d1_4;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[14] = (f14 < 0.0) ? 0.0 : f14;
         const double r0_5 = ((P5 * (P130 * X[20] + P131 * X[21] + P132 * X[22] + P133 * X[23] + P134 * X[24] + P135 * X[25] + P136 * X[26] + P137 * X[27] + P138 * X[28] + P139 * X[29])) / C5);
@@ -8097,10 +8097,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         plom_normal_pair64(ns, &za11, &zb11);
         const double d2_5 = sqdt * za11;
         const double d3_5 = sqdt * zb11;
-        const double f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr + - plom_sqrt((r0_5*X[5]))*d0_5+ plom_sqrt((r3_5))*d3_5;
-        const double f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr + - plom_sqrt((r1_5*X[15]))*d1_5+ plom_sqrt((r0_5*X[5]))*d0_5;
-        const double f25 = X[25] + ( - (r2_5*X[25]) + (r1_5*X[15]))*dtr + - plom_sqrt((r2_5*X[25]))*d2_5+ plom_sqrt((r1_5*X[15]))*d1_5;
-        s35 += ( + (r1_5*X[15]))*dtr + + plom_sqrt((r1_5*X[15]))*d1_5;
+        const double f5 = X[5] + ( - (r0_5*X[5]) + (r3_5))*dtr + - plom_sqrt_rate((r0_5*X[5]))*d0_5+ plom_sqrt_rate((r3_5))*d3_5;
+        const double f15 = X[15] + ( - (r1_5*X[15]) + (r0_5*X[5]))*dtr + - plom_sqrt_rate((r1_5*X[15]))*d1_5+ plom_sqrt_rate((r0_5*X[5]))*d0_5;
+        const double f25 = X[25] + ( - (r2_5*X[25]) + (r1_5*X[15]))*dtr + - plom_sqrt_rate((r2_5*X[25]))*d2_5+ plom_sqrt_rate((r1_5*X[15]))*d1_5;
+        s35 += ( + (r1_5*X[15]))*dtr + + plom_sqrt_rate((r1_5*X[15]))*d1_5;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
         X[15] = (f15 < 0.0) ? 0.0 : f15;
         const double r0_6 = ((P6 * (P140 * X[20] + P141 * X[21] + P142 * X[22] + P143 * X[23] + P144 * X[24] + P145 * X[25] + P146 * X[26] + P147 * X[27] + P148 * X[28] + P149 * X[29])) / C6);
@@ -8115,10 +8115,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         plom_normal_pair64(ns, &za13, &zb13);
         const double d2_6 = sqdt * za13;
         const double d3_6 = sqdt * zb13;
-        const double f6 = X[6] + ( - (r0_6*X[6]) + (r3_6))*dtr + - plom_sqrt((r0_6*X[6]))*d0_6+ plom_sqrt((r3_6))*d3_6;
-        const double f16 = X[16] + ( - (r1_6*X[16]) + (r0_6*X[6]))*dtr + - plom_sqrt((r1_6*X[16]))*d1_6+ plom_sqrt((r0_6*X[6]))*d0_6;
-        const double f26 = X[26] + ( - (r2_6*X[26]) + (r1_6*X[16]))*dtr + - plom_sqrt((r2_6*X[26]))*d2_6+ plom_sqrt((r1_6*X[16]))*d1_6;
-        s36 += ( + (r1_6*X[16]))*dtr + + plom_sqrt((r1_6*X[16]))*d1_6;
+        const double f6 = X[6] + ( - (r0_6*X[6]) + (r3_6))*dtr + - plom_sqrt_rate((r0_6*X[6]))*d0_6+ plom_sqrt_rate((r3_6))*d3_6;
+        const double f16 = X[16] + ( - (r1_6*X[16]) + (r0_6*X[6]))*dtr + - plom_sqrt_rate((r1_6*X[16]))*d1_6+ plom_sqrt_rate((r0_6*X[6]))*d0_6;
+        const double f26 = X[26] + ( - (r2_6*X[26]) + (r1_6*X[16]))*dtr + - plom_sqrt_rate((r2_6*X[26]))*d2_6+ plom_sqrt_rate((r1_6*X[16]))*d1_6;
+        s36 += ( + (r1_6*X[16]))*dtr + + plom_sqrt_rate((r1_6*X[16]))*d1_6;
         X[6] = (f6 < 0.0) ? 0.0 : f6;
         X[16] = (f16 < 0.0) ? 0.0 : f16;
         const double r0_7 = ((P7 * (P150 * X[20] + P151 * X[21] + P152 * X[22] + P153 * X[23] + P154 * X[24] + P155 * X[25] + P156 * X[26] + P157 * X[27] + P158 * X[28] + P159 * X[29])) / C7);
@@ -8133,10 +8133,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         plom_normal_pair64(ns, &za15, &zb15);
         const double d2_7 = sqdt * za15;
         const double d3_7 = sqdt * zb15;
-        const double f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr + - plom_sqrt((r0_7*X[7]))*d0_7+ plom_sqrt((r3_7))*d3_7;
-        const double f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr + - plom_sqrt((r1_7*X[17]))*d1_7+ plom_sqrt((r0_7*X[7]))*d0_7;
-        const double f27 = X[27] + ( - (r2_7*X[27]) + (r1_7*X[17]))*dtr + - plom_sqrt((r2_7*X[27]))*d2_7+ plom_sqrt((r1_7*X[17]))*d1_7;
-        s37 += ( + (r1_7*X[17]))*dtr + + plom_sqrt((r1_7*X[17]))*d1_7;
+        const double f7 = X[7] + ( - (r0_7*X[7]) + (r3_7))*dtr + - plom_sqrt_rate((r0_7*X[7]))*d0_7+ plom_sqrt_rate((r3_7))*d3_7;
+        const double f17 = X[17] + ( - (r1_7*X[17]) + (r0_7*X[7]))*dtr + - plom_sqrt_rate((r1_7*X[17]))*d1_7+ plom_sqrt_rate((r0_7*X[7]))*d0_7;
+        const double f27 = X[27] + ( - (r2_7*X[27]) + (r1_7*X[17]))*dtr + - plom_sqrt_rate((r2_7*X[27]))*d2_7+ plom_sqrt_rate((r1_7*X[17]))*d1_7;
+        s37 += ( + (r1_7*X[17]))*dtr + + plom_sqrt_rate((r1_7*X[17]))*d1_7;
         X[7] = (f7 < 0.0) ? 0.0 : f7;
         X[17] = (f17 < 0.0) ? 0.0 : f17;
         const double r0_8 = ((P8 * (P160 * X[20] + P161 * X[21] + P162 * X[22] + P163 * X[23] + P164 * X[24] + P165 * X[25] + P166 * X[26] + P167 * X[27] + P168 * X[28] + P169 * X[29])) / C8);
@@ -8151,10 +8151,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         plom_normal_pair64(ns, &za17, &zb17);
         const double d2_8 = sqdt * za17;
         const double d3_8 = sqdt * zb17;
-        const double f8 = X[8] + ( - (r0_8*X[8]) + (r3_8))*dtr + - plom_sqrt((r0_8*X[8]))*d0_8+ plom_sqrt((r3_8))*d3_8;
-        const double f18 = X[18] + ( - (r1_8*X[18]) + (r0_8*X[8]))*dtr + - plom_sqrt((r1_8*X[18]))*d1_8+ plom_sqrt((r0_8*X[8]))*d0_8;
-        const double f28 = X[28] + ( - (r2_8*X[28]) + (r1_8*X[18]))*dtr + - plom_sqrt((r2_8*X[28]))*d2_8+ plom_sqrt((r1_8*X[18]))*d1_8;
-        s38 += ( + (r1_8*X[18]))*dtr + + plom_sqrt((r1_8*X[18]))*d1_8;
+        const double f8 = X[8] + ( - (r0_8*X[8]) + (r3_8))*dtr + - plom_sqrt_rate((r0_8*X[8]))*d0_8+ plom_sqrt_rate((r3_8))*d3_8;
+        const double f18 = X[18] + ( - (r1_8*X[18]) + (r0_8*X[8]))*dtr + - plom_sqrt_rate((r1_8*X[18]))*d1_8+ plom_sqrt_rate((r0_8*X[8]))*d0_8;
+        const double f28 = X[28] + ( - (r2_8*X[28]) + (r1_8*X[18]))*dtr + - plom_sqrt_rate((r2_8*X[28]))*d2_8+ plom_sqrt_rate((r1_8*X[18]))*d1_8;
+        s38 += ( + (r1_8*X[18]))*dtr + + plom_sqrt_rate((r1_8*X[18]))*d1_8;
         X[8] = (f8 < 0.0) ? 0.0 : f8;
         X[18] = (f18 < 0.0) ? 0.0 : f18;
         const double r0_9 = ((P9 * (P170 * X[20] + P171 * X[21] + P172 * X[22] + P173 * X[23] + P174 * X[24] + P175 * X[25] + P176 * X[26] + P177 * X[27] + P178 * X[28] + P179 * X[29])) / C9);
@@ -8169,10 +8169,10 @@ static __device__ inline __attribute__((always_inline)) void plom_seir10_interva
         plom_normal_pair64(ns, &za19, &zb19);
         const double d2_9 = sqdt * za19;
         const double d3_9 = sqdt * zb19;
-        const double f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr + - plom_sqrt((r0_9*X[9]))*d0_9+ plom_sqrt((r3_9))*d3_9;
-        const double f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr + - plom_sqrt((r1_9*X[19]))*d1_9+ plom_sqrt((r0_9*X[9]))*d0_9;
-        const double f29 = X[29] + ( - (r2_9*X[29]) + (r1_9*X[19]))*dtr + - plom_sqrt((r2_9*X[29]))*d2_9+ plom_sqrt((r1_9*X[19]))*d1_9;
-        s39 += ( + (r1_9*X[19]))*dtr + + plom_sqrt((r1_9*X[19]))*d1_9;
+        const double f9 = X[9] + ( - (r0_9*X[9]) + (r3_9))*dtr + - plom_sqrt_rate((r0_9*X[9]))*d0_9+ plom_sqrt_rate((r3_9))*d3_9;
+        const double f19 = X[19] + ( - (r1_9*X[19]) + (r0_9*X[9]))*dtr + - plom_sqrt_rate((r1_9*X[19]))*d1_9+ plom_sqrt_rate((r0_9*X[9]))*d0_9;
+        const double f29 = X[29] + ( - (r2_9*X[29]) + (r1_9*X[19]))*dtr + - plom_sqrt_rate((r2_9*X[29]))*d2_9+ plom_sqrt_rate((r1_9*X[19]))*d1_9;
+        s39 += ( + (r1_9*X[19]))*dtr + + plom_sqrt_rate((r1_9*X[19]))*d1_9;
         X[9] = (f9 < 0.0) ? 0.0 : f9;
         X[19] = (f19 < 0.0) ? 0.0 : f19;
         X[20] = (f20 < 0.0) ? 0.0 : f20;

@@ -44,9 +44,9 @@ PLOM_HD void plom_sir_interval_full_f64(double *X, const double *pv, plom_nstrea
         plom_normal_pair64(ns, &za0, &zb0);
         const double d0_0 = sqdt * za0;
         const double d1_0 = sqdt * zb0;
-        const double f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0;
-        const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[1]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
-        s2 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0;
+        const double f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_sqrt_rate((r0_0*X[0]))*d0_0;
+        const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[1]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0;
+        s2 += ( + (r0_0*X[0]))*dtr + + plom_sqrt_rate((r0_0*X[0]))*d0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         const double f2 = X[2] + s2;
@@ -102,9 +102,9 @@ PLOM_HD void plom_sir_interval_no_env_sto_f64(double *X, const double *pv, plom_
         plom_normal_pair64(ns, &za0, &zb0);
         const double d0_0 = sqdt * za0;
         const double d1_0 = sqdt * zb0;
-        const double f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0;
-        const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[1]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
-        s2 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0;
+        const double f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_sqrt_rate((r0_0*X[0]))*d0_0;
+        const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[1]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0;
+        s2 += ( + (r0_0*X[0]))*dtr + + plom_sqrt_rate((r0_0*X[0]))*d0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         const double f2 = X[2] + s2;
@@ -635,9 +635,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_f
         plom_normal_pair64(ns, &za0, &zb0);
         const double d0_0 = sqdt * za0;
         const double d1_0 = sqdt * zb0;
-        const double f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0;
-        const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[1]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
-        s2 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0;
+        const double f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_sqrt_rate((r0_0*X[0]))*d0_0;
+        const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[1]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0;
+        s2 += ( + (r0_0*X[0]))*dtr + + plom_sqrt_rate((r0_0*X[0]))*d0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         const double f2 = X[2] + s2;
@@ -693,9 +693,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
         plom_normal_pair64(ns, &za0, &zb0);
         const double d0_0 = sqdt * za0;
         const double d1_0 = sqdt * zb0;
-        const double f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0;
-        const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[1]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
-        s2 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0;
+        const double f0 = X[0] + ( - (r0_0*X[0]))*dtr + - plom_sqrt_rate((r0_0*X[0]))*d0_0;
+        const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[1]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0;
+        s2 += ( + (r0_0*X[0]))*dtr + + plom_sqrt_rate((r0_0*X[0]))*d0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         const double f2 = X[2] + s2;

@@ -94,9 +94,9 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
         plom_normal_pair64(ns, &za1, &zb1);
         const double d2_0 = sqdt * za1;
         const double w0_0 = sqdt * zb1;
-        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0+ plom_sqrt((r2_0))*d2_0 + - (r0_0*X[0])*P16*w0_0;
-        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[4]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
-        s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
+        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - plom_sqrt_rate((r0_0*X[0]))*d0_0+ plom_sqrt_rate((r2_0))*d2_0 + - (r0_0*X[0])*P16*w0_0;
+        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[4]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
+        s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt_rate((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
@@ -109,9 +109,9 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
         plom_normal_pair64(ns, &za3, &zb3);
         const double d2_1 = sqdt * za3;
         const double w0_1 = sqdt * zb3;
-        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - plom_sqrt((r0_1*X[1]))*d0_1+ plom_sqrt((r2_1))*d2_1 + - (r0_1*X[1])*P17*w0_1;
-        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[5]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
-        s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
+        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - plom_sqrt_rate((r0_1*X[1]))*d0_1+ plom_sqrt_rate((r2_1))*d2_1 + - (r0_1*X[1])*P17*w0_1;
+        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt_rate((r1_1*X[5]))*d1_1+ plom_sqrt_rate((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
+        s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt_rate((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
@@ -124,9 +124,9 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
         plom_normal_pair64(ns, &za5, &zb5);
         const double d2_2 = sqdt * za5;
         const double w0_2 = sqdt * zb5;
-        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - plom_sqrt((r0_2*X[2]))*d0_2+ plom_sqrt((r2_2))*d2_2 + - (r0_2*X[2])*P18*w0_2;
-        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[6]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
-        s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
+        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - plom_sqrt_rate((r0_2*X[2]))*d0_2+ plom_sqrt_rate((r2_2))*d2_2 + - (r0_2*X[2])*P18*w0_2;
+        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt_rate((r1_2*X[6]))*d1_2+ plom_sqrt_rate((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
+        s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt_rate((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
@@ -139,9 +139,9 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
         plom_normal_pair64(ns, &za7, &zb7);
         const double d2_3 = sqdt * za7;
         const double w0_3 = sqdt * zb7;
-        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - plom_sqrt((r0_3*X[3]))*d0_3+ plom_sqrt((r2_3))*d2_3 + - (r0_3*X[3])*P19*w0_3;
-        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - plom_sqrt((r1_3*X[7]))*d1_3+ plom_sqrt((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
-        s11 += ( + (r0_3*X[3]))*dtr + + plom_sqrt((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
+        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - plom_sqrt_rate((r0_3*X[3]))*d0_3+ plom_sqrt_rate((r2_3))*d2_3 + - (r0_3*X[3])*P19*w0_3;
+        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - plom_sqrt_rate((r1_3*X[7]))*d1_3+ plom_sqrt_rate((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
+        s11 += ( + (r0_3*X[3]))*dtr + + plom_sqrt_rate((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
@@ -333,9 +333,9 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
         double za1, zb1;
         plom_normal_pair64(ns, &za1, &zb1);
         const double d2_0 = sqdt * za1;
-        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0+ plom_sqrt((r2_0))*d2_0;
-        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[4]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
-        s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0;
+        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - plom_sqrt_rate((r0_0*X[0]))*d0_0+ plom_sqrt_rate((r2_0))*d2_0;
+        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[4]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0;
+        s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt_rate((r0_0*X[0]))*d0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
@@ -345,9 +345,9 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
         plom_normal_pair64(ns, &za2, &zb2);
         const double d1_1 = sqdt * za2;
         const double d2_1 = sqdt * zb2;
-        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - plom_sqrt((r0_1*X[1]))*d0_1+ plom_sqrt((r2_1))*d2_1;
-        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[5]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1;
-        s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt((r0_1*X[1]))*d0_1;
+        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - plom_sqrt_rate((r0_1*X[1]))*d0_1+ plom_sqrt_rate((r2_1))*d2_1;
+        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt_rate((r1_1*X[5]))*d1_1+ plom_sqrt_rate((r0_1*X[1]))*d0_1;
+        s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt_rate((r0_1*X[1]))*d0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
@@ -359,9 +359,9 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
         double za4, zb4;
         plom_normal_pair64(ns, &za4, &zb4);
         const double d2_2 = sqdt * za4;
-        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - plom_sqrt((r0_2*X[2]))*d0_2+ plom_sqrt((r2_2))*d2_2;
-        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[6]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2;
-        s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt((r0_2*X[2]))*d0_2;
+        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - plom_sqrt_rate((r0_2*X[2]))*d0_2+ plom_sqrt_rate((r2_2))*d2_2;
+        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt_rate((r1_2*X[6]))*d1_2+ plom_sqrt_rate((r0_2*X[2]))*d0_2;
+        s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt_rate((r0_2*X[2]))*d0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
@@ -371,9 +371,9 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
         plom_normal_pair64(ns, &za5, &zb5);
         const double d1_3 = sqdt * za5;
         const double d2_3 = sqdt * zb5;
-        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - plom_sqrt((r0_3*X[3]))*d0_3+ plom_sqrt((r2_3))*d2_3;
-        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - plom_sqrt((r1_3*X[7]))*d1_3+ plom_sqrt((r0_3*X[3]))*d0_3;
-        s11 += ( + (r0_3*X[3]))*dtr + + plom_sqrt((r0_3*X[3]))*d0_3;
+        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - plom_sqrt_rate((r0_3*X[3]))*d0_3+ plom_sqrt_rate((r2_3))*d2_3;
+        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - plom_sqrt_rate((r1_3*X[7]))*d1_3+ plom_sqrt_rate((r0_3*X[3]))*d0_3;
+        s11 += ( + (r0_3*X[3]))*dtr + + plom_sqrt_rate((r0_3*X[3]))*d0_3;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
@@ -2279,9 +2279,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         plom_normal_pair64(ns, &za1, &zb1);
         const double d2_0 = sqdt * za1;
         const double w0_0 = sqdt * zb1;
-        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0+ plom_sqrt((r2_0))*d2_0 + - (r0_0*X[0])*P16*w0_0;
-        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[4]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
-        s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
+        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - plom_sqrt_rate((r0_0*X[0]))*d0_0+ plom_sqrt_rate((r2_0))*d2_0 + - (r0_0*X[0])*P16*w0_0;
+        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[4]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
+        s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt_rate((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
@@ -2294,9 +2294,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         plom_normal_pair64(ns, &za3, &zb3);
         const double d2_1 = sqdt * za3;
         const double w0_1 = sqdt * zb3;
-        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - plom_sqrt((r0_1*X[1]))*d0_1+ plom_sqrt((r2_1))*d2_1 + - (r0_1*X[1])*P17*w0_1;
-        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[5]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
-        s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
+        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - plom_sqrt_rate((r0_1*X[1]))*d0_1+ plom_sqrt_rate((r2_1))*d2_1 + - (r0_1*X[1])*P17*w0_1;
+        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt_rate((r1_1*X[5]))*d1_1+ plom_sqrt_rate((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
+        s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt_rate((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
@@ -2309,9 +2309,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         plom_normal_pair64(ns, &za5, &zb5);
         const double d2_2 = sqdt * za5;
         const double w0_2 = sqdt * zb5;
-        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - plom_sqrt((r0_2*X[2]))*d0_2+ plom_sqrt((r2_2))*d2_2 + - (r0_2*X[2])*P18*w0_2;
-        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[6]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
-        s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
+        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - plom_sqrt_rate((r0_2*X[2]))*d0_2+ plom_sqrt_rate((r2_2))*d2_2 + - (r0_2*X[2])*P18*w0_2;
+        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt_rate((r1_2*X[6]))*d1_2+ plom_sqrt_rate((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
+        s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt_rate((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
@@ -2324,9 +2324,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         plom_normal_pair64(ns, &za7, &zb7);
         const double d2_3 = sqdt * za7;
         const double w0_3 = sqdt * zb7;
-        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - plom_sqrt((r0_3*X[3]))*d0_3+ plom_sqrt((r2_3))*d2_3 + - (r0_3*X[3])*P19*w0_3;
-        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - plom_sqrt((r1_3*X[7]))*d1_3+ plom_sqrt((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
-        s11 += ( + (r0_3*X[3]))*dtr + + plom_sqrt((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
+        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - plom_sqrt_rate((r0_3*X[3]))*d0_3+ plom_sqrt_rate((r2_3))*d2_3 + - (r0_3*X[3])*P19*w0_3;
+        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - plom_sqrt_rate((r1_3*X[7]))*d1_3+ plom_sqrt_rate((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
+        s11 += ( + (r0_3*X[3]))*dtr + + plom_sqrt_rate((r0_3*X[3]))*d0_3 + + (r0_3*X[3])*P19*w0_3;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[5] = (f5 < 0.0) ? 0.0 : f5;
@@ -2518,9 +2518,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         double za1, zb1;
         plom_normal_pair64(ns, &za1, &zb1);
         const double d2_0 = sqdt * za1;
-        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - plom_sqrt((r0_0*X[0]))*d0_0+ plom_sqrt((r2_0))*d2_0;
-        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt((r1_0*X[4]))*d1_0+ plom_sqrt((r0_0*X[0]))*d0_0;
-        s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt((r0_0*X[0]))*d0_0;
+        const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr + - plom_sqrt_rate((r0_0*X[0]))*d0_0+ plom_sqrt_rate((r2_0))*d2_0;
+        const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[4]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0;
+        s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt_rate((r0_0*X[0]))*d0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
         const double r1_1 = P9;
@@ -2530,9 +2530,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         plom_normal_pair64(ns, &za2, &zb2);
         const double d1_1 = sqdt * za2;
         const double d2_1 = sqdt * zb2;
-        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - plom_sqrt((r0_1*X[1]))*d0_1+ plom_sqrt((r2_1))*d2_1;
-        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt((r1_1*X[5]))*d1_1+ plom_sqrt((r0_1*X[1]))*d0_1;
-        s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt((r0_1*X[1]))*d0_1;
+        const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr + - plom_sqrt_rate((r0_1*X[1]))*d0_1+ plom_sqrt_rate((r2_1))*d2_1;
+        const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt_rate((r1_1*X[5]))*d1_1+ plom_sqrt_rate((r0_1*X[1]))*d0_1;
+        s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt_rate((r0_1*X[1]))*d0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
         const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
         const double r1_2 = P10;
@@ -2544,9 +2544,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         double za4, zb4;
         plom_normal_pair64(ns, &za4, &zb4);
         const double d2_2 = sqdt * za4;
-        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - plom_sqrt((r0_2*X[2]))*d0_2+ plom_sqrt((r2_2))*d2_2;
-        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt((r1_2*X[6]))*d1_2+ plom_sqrt((r0_2*X[2]))*d0_2;
-        s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt((r0_2*X[2]))*d0_2;
+        const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr + - plom_sqrt_rate((r0_2*X[2]))*d0_2+ plom_sqrt_rate((r2_2))*d2_2;
+        const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt_rate((r1_2*X[6]))*d1_2+ plom_sqrt_rate((r0_2*X[2]))*d0_2;
+        s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt_rate((r0_2*X[2]))*d0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
         const double r1_3 = P11;
@@ -2556,9 +2556,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         plom_normal_pair64(ns, &za5, &zb5);
         const double d1_3 = sqdt * za5;
         const double d2_3 = sqdt * zb5;
-        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - plom_sqrt((r0_3*X[3]))*d0_3+ plom_sqrt((r2_3))*d2_3;
-        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - plom_sqrt((r1_3*X[7]))*d1_3+ plom_sqrt((r0_3*X[3]))*d0_3;
-        s11 += ( + (r0_3*X[3]))*dtr + + plom_sqrt((r0_3*X[3]))*d0_3;
+        const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr + - plom_sqrt_rate((r0_3*X[3]))*d0_3+ plom_sqrt_rate((r2_3))*d2_3;
+        const double f7 = X[7] + ( - (r1_3*X[7]) + (r0_3*X[3]))*dtr + - plom_sqrt_rate((r1_3*X[7]))*d1_3+ plom_sqrt_rate((r0_3*X[3]))*d0_3;
+        s11 += ( + (r0_3*X[3]))*dtr + + plom_sqrt_rate((r0_3*X[3]))*d0_3;
         X[3] = (f3 < 0.0) ? 0.0 : f3;
         X[4] = (f4 < 0.0) ? 0.0 : f4;
         X[5] = (f5 < 0.0) ? 0.0 : f5;

@@ -3,11 +3,14 @@
 
 The role of the reference's `pmbuilder` (scripts/pmbuilder:50-105 -> PlomModelBuilder,
 model_builder/Builder.py:70-351): it writes the settings file the `smc` command reads
-(schema of Builder.make_settings_json, Builder.py:173-259) and checks that the model is
-one of the models compiled into libplom_smc (their generated code lives in gen/; adding a
-model means adding it to codegen.BUILTIN_MODELS and rebuilding).
+(schema of Builder.make_settings_json, Builder.py:173-259).  A model compiled into
+libplom_smc (codegen.BUILTIN_MODELS) needs nothing else; any other model is compiled with
+--lib into a model library of its own, libplom_model_<name>.so in out_dir (its generated
+code: the role of Builder.py:283-351 and C/templates/Makefile:28-29), and the settings
+name it ("model_library"), so that the commands register it (plom_model_load) when they
+read the settings.
 
-usage: python plom_build.py <model dir with context/process/link.json> [-o out_dir]
+usage: python plom_build.py <model dir with context/process/link.json> [-o out_dir] [--lib]
 """
 import argparse
 import datetime
@@ -55,10 +58,24 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("model_dir")
     ap.add_argument("-o", "--out", default=None)
+    ap.add_argument("--lib", action="store_true",
+                    help="compile the model into its own model library in out_dir (models outside the library)")
+    ap.add_argument("--model_library", default=None,
+                    help="name an already built model library of this model in the settings instead")
     a = ap.parse_args()
     out = a.out or a.model_dir
     s = make_settings(a.model_dir)
     os.makedirs(out, exist_ok=True)
+    sys.path.insert(0, HERE)
+    from codegen import BUILTIN_MODELS  # noqa: E402
+    if a.lib:
+        import build  # noqa: E402
+        name, path = build.build_model_lib(a.model_dir, out)
+        s["model_library"] = os.path.basename(path)
+    elif a.model_library:
+        s["model_library"] = os.path.abspath(a.model_library)
+    elif s["model"] not in BUILTIN_MODELS:
+        sys.exit("model %s is not compiled into libplom_smc: build it with --lib" % s["model"])
     with open(os.path.join(out, ".settings.json"), "w") as f:
         json.dump(s, f)
     print("wrote %s (model %s)" % (os.path.join(out, ".settings.json"), s["model"]))

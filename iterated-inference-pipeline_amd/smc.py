@@ -82,11 +82,11 @@ class FilterOutputs(C.Structure):
 
 EXPORTS = [
     "plom_last_error", "plom_ctx_create", "plom_ctx_destroy", "plom_model_count", "plom_model_find",
-    "plom_model_name", "plom_model_info_get", "plom_smc_run", "plom_filter_create", "plom_filter_reset",
+    "plom_model_name", "plom_model_info_get", "plom_model_load", "plom_smc_run", "plom_filter_create", "plom_filter_reset",
     "plom_filter_update", "plom_filter_mif_start", "plom_covariate_eval",
     "plom_filter_run", "plom_filter_read", "plom_filter_destroy", "plom_filter_device_bytes",
     "plom_batch_run", "plom_sync", "plom_ctx_set_kernel_timing", "plom_last_timing", "plom_last_timing_lanes", "plom_selftest_math",
-    "plom_selftest_normals", "plom_selftest_copy", "plom_batch_create", "plom_batch_launch",
+    "plom_selftest_normals", "plom_selftest_normals32", "plom_selftest_copy", "plom_batch_create", "plom_batch_launch",
     "plom_batch_destroy", "plom_batch_configure", "plom_batch_autotune", "plom_server_create", "plom_server_submit", "plom_server_step",
     "plom_server_status", "plom_server_done", "plom_server_result", "plom_server_destroy",
 ]
@@ -126,6 +126,7 @@ def lib():
     L.plom_model_find.argtypes = [C.c_char_p]
     L.plom_model_name.argtypes = [C.c_int, C.c_char_p, C.c_int]
     L.plom_model_info_get.argtypes = [C.c_int, C.POINTER(ModelInfo)]
+    L.plom_model_load.argtypes = [C.c_char_p, C.POINTER(C.c_int32)]
     L.plom_smc_run.argtypes = [C.c_void_p, C.POINTER(FilterDesc), C.POINTER(FilterInputs), C.POINTER(FilterOutputs)]
     L.plom_filter_create.argtypes = [C.c_void_p, C.POINTER(FilterDesc), C.POINTER(FilterInputs), C.POINTER(C.c_void_p)]
     L.plom_filter_reset.argtypes = [C.c_void_p, C.c_void_p]
@@ -144,6 +145,7 @@ def lib():
     L.plom_last_timing_lanes.argtypes = [C.c_void_p, dp, dp, C.c_int32, C.POINTER(C.c_int32)]
     L.plom_selftest_math.argtypes = [C.c_void_p, C.c_int, dp, dp]
     L.plom_selftest_normals.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
+    L.plom_selftest_normals32.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
     L.plom_selftest_copy.argtypes = [C.c_void_p, C.c_int64, C.c_int, dp]
     L.plom_batch_create.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
     L.plom_batch_launch.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
@@ -215,14 +217,16 @@ class Context:
 
 # ---------------------------------------------------------------------- models
 class Model:
-    """A built-in model: generated code in the library + its description (codegen)."""
+    """A model: its generated code in the library (a built-in model) or in a model library
+    registered at run time (Model.load), and its description (codegen)."""
 
     _cache = {}
 
-    def __init__(self, name):
+    def __init__(self, name, model_dir=None, model_id=None):
         self.name = name
-        self.desc = load_model(name)
-        self.id = BUILTIN_MODELS.index(name)
+        self.model_dir = model_dir or os.path.join(PKG_DIR, "examples", name)
+        self.desc = load_model(self.model_dir)
+        self.id = BUILTIN_MODELS.index(name) if model_id is None else model_id
         self.S = self.desc.size_proj
         self.n_ts = self.desc.N_TS
         self.n_cac = self.desc.N_CAC
@@ -233,13 +237,29 @@ class Model:
         self.n_fix = self.desc.layout["n_fix"]
         self.fix_off = self.desc.layout["fix_off"]
         self.layout = self.desc.layout
-        self.context = load_json(os.path.join(PKG_DIR, "examples", name, "context.json"))
+        self.context = load_json(os.path.join(self.model_dir, "context.json"))
 
     @classmethod
     def get(cls, name):
         if name not in cls._cache:
             cls._cache[name] = cls(name)
         return cls._cache[name]
+
+    @classmethod
+    def load(cls, model_dir, library=None):
+        """A model outside the library (context / process / link JSON in model_dir) whose
+        model library was built by build.py --model (default: model_dir/build/
+        libplom_model_<name>.so): registered with plom_model_load, then usable like a
+        built-in model (filters, batches beside other models, served requests)."""
+        name = load_model(model_dir).name.lower()
+        if name in cls._cache:
+            return cls._cache[name]
+        library = library or os.path.join(model_dir, "build", "libplom_model_%s.so" % name)
+        mid = C.c_int32()
+        check(lib().plom_model_load(library.encode(), C.byref(mid)))
+        m = cls(name, model_dir, mid.value)
+        cls._cache[name] = m
+        return m
 
     def covariates(self):
         """The covariates of the context as one (x, y) knot pair per covariate slot of pv (pv
@@ -672,7 +692,7 @@ def run(ctx, model, J, times, data, pv, x0, ancestry=False, final_state=False, l
 
 def selftest_math(ctx, x):
     x = np.ascontiguousarray(x, dtype=np.float64)
-    out = np.zeros((x.size, 7))
+    out = np.zeros((x.size, 8))
     check(lib().plom_selftest_math(ctx.h, x.size, dptr(x), dptr(out)))
     return out
 
@@ -680,4 +700,11 @@ def selftest_math(ctx, x):
 def selftest_normals(ctx, seed, stream, n0, j, n):
     out = np.zeros(n)
     check(lib().plom_selftest_normals(ctx.h, seed, stream, n0, j, n, dptr(out)))
+    return out
+
+
+def selftest_normals32(ctx, seed, stream, n0, j, n):
+    """The first n Gaussians of the FP32 stream (widened to double)."""
+    out = np.zeros(n)
+    check(lib().plom_selftest_normals32(ctx.h, seed, stream, n0, j, n, dptr(out)))
     return out

@@ -17,48 +17,82 @@ def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
 
 
+_extra = {}
+
+
+def extra_lib(name, gen_dir):
+    """The oracle with a user's model (generated header plom_<name>.h in gen_dir) compiled
+    in after the built-in models (make extra); its model name then resolves in smc(),
+    mif_pass(), trajectory() ... like a built-in id."""
+    if name not in _extra:
+        path = os.path.join(HERE, "_build", "liboracle_%s.so" % name)
+        if not os.path.exists(path):
+            subprocess.check_call(["make", "-s", "-C", HERE, "extra", "NAME=" + name, "GEN=" + os.path.abspath(gen_dir)])
+        _extra[name] = _setup(C.CDLL(path))
+    return _extra[name]
+
+
+def _resolve(model):
+    """(library, model index) of a built-in model id or name, or of a registered extra model's name."""
+    if not isinstance(model, str):
+        return lib(), model
+    L = lib()
+    for i in range(L.oracle_model_count()):
+        if L.oracle_model_name(i).decode() == model:
+            return L, i
+    if model in _extra:
+        X = _extra[model]
+        return X, X.oracle_model_count() - 1
+    raise KeyError("unknown oracle model %r (register a user's model with extra_lib)" % model)
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
-        dp = C.POINTER(C.c_double)
-        ip = C.POINTER(C.c_int32)
-        L.oracle_smc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-                                 C.c_double, C.c_double, dp, dp, dp, dp, C.c_int, C.c_int,
-                                 dp, ip, dp, dp, ip, dp, dp, dp, ip, dp, dp, dp, dp, dp, ip, dp, dp, C.c_double, C.c_double,
-                                 ip, ip, ip, ip, dp, dp, dp, ip]
-        L.oracle_trajectory.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-                                        C.c_double, dp, dp, dp, dp, ip, dp, dp, C.c_double, C.c_double,
-                                        ip, ip, ip, ip, dp, dp, dp, ip]
-        L.oracle_ode_test.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, ip]
-        L.oracle_cov_table.argtypes = [C.c_int, C.c_int, dp, C.c_double, ip, dp, dp, dp, ip]
-        L.oracle_spline.argtypes = [C.c_int, dp, dp, C.c_int, dp, dp]
-        L.oracle_obs_draw.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, dp]
-        L.oracle_obs_loglik.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, dp, ip]
-        L.oracle_math.argtypes = [C.c_int, dp, dp]
-        L.oracle_normals.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
-        L.oracle_philox.argtypes = [C.POINTER(C.c_uint32)] * 3
-        L.oracle_bm_pairs.argtypes = [C.c_int, C.POINTER(C.c_uint32), dp]
-        L.oracle_xoshiro.argtypes = [C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_uint32)]
-        L.oracle_model_name.restype = C.c_char_p
-        L.oracle_psr_draws.argtypes = [C.c_int, C.c_double, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-                                       C.c_int, dp, dp]
-        L.oracle_mif_natural.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
-        L.oracle_mif_natural.restype = C.c_double
-        i32p = C.POINTER(C.c_int32)
-        L.oracle_mif_pass.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double,
-                                      C.c_double, dp, dp, dp, C.c_int, i32p, i32p, i32p, i32p, dp, dp, dp, dp, dp,
-                                      dp, dp, C.c_int, dp, i32p, dp, dp, dp, dp, dp, dp, i32p, i32p, dp, dp, C.c_double,
-                                      C.c_double, dp, i32p, dp, i32p, ip, ip, ip, ip, dp, dp, dp, ip]
-        L.oracle_mif_prior_pdf.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
-        L.oracle_mif_prior_pdf.restype = C.c_double
-        L.oracle_mif_prior_term.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double,
-                                            C.c_double]
-        L.oracle_mif_prior_term.restype = C.c_double
-        _lib = L
+        _lib = _setup(C.CDLL(LIB))
     return _lib
+
+
+def _setup(L):
+    """argument and result types of the oracle's entry points"""
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    L.oracle_smc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                             C.c_double, C.c_double, dp, dp, dp, dp, C.c_int, C.c_int,
+                             dp, ip, dp, dp, ip, dp, dp, dp, ip, dp, dp, dp, dp, dp, ip, dp, dp, C.c_double, C.c_double,
+                             ip, ip, ip, ip, dp, dp, dp, ip]
+    L.oracle_trajectory.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.c_double, dp, dp, dp, dp, ip, dp, dp, C.c_double, C.c_double,
+                                    ip, ip, ip, ip, dp, dp, dp, ip]
+    L.oracle_ode_test.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, ip]
+    L.oracle_cov_table.argtypes = [C.c_int, C.c_int, dp, C.c_double, ip, dp, dp, dp, ip]
+    L.oracle_spline.argtypes = [C.c_int, dp, dp, C.c_int, dp, dp]
+    L.oracle_obs_draw.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, dp]
+    L.oracle_obs_loglik.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, dp, ip]
+    L.oracle_math.argtypes = [C.c_int, dp, dp]
+    L.oracle_normals.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
+    L.oracle_philox.argtypes = [C.POINTER(C.c_uint32)] * 3
+    L.oracle_bm_pairs.argtypes = [C.c_int, C.POINTER(C.c_uint32), dp]
+    L.oracle_icdf.argtypes = [C.c_int, C.POINTER(C.c_uint32), dp]
+    L.oracle_xoshiro.argtypes = [C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_uint32)]
+    L.oracle_model_name.restype = C.c_char_p
+    L.oracle_psr_draws.argtypes = [C.c_int, C.c_double, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                   C.c_int, dp, dp]
+    L.oracle_mif_natural.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.oracle_mif_natural.restype = C.c_double
+    i32p = C.POINTER(C.c_int32)
+    L.oracle_mif_pass.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double,
+                                  C.c_double, dp, dp, dp, C.c_int, i32p, i32p, i32p, i32p, dp, dp, dp, dp, dp,
+                                  dp, dp, C.c_int, dp, i32p, dp, dp, dp, dp, dp, dp, i32p, i32p, dp, dp, C.c_double,
+                                  C.c_double, dp, i32p, dp, i32p, ip, ip, ip, ip, dp, dp, dp, ip]
+    L.oracle_mif_prior_pdf.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+    L.oracle_mif_prior_pdf.restype = C.c_double
+    L.oracle_mif_prior_term.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_double]
+    L.oracle_mif_prior_term.restype = C.c_double
+    return L
 
 
 def _dp(a):
@@ -131,14 +165,15 @@ def _cov_args(cov):
 
 def cov_table(model_id, times, dt, cov):
     """The covariate table of a filter (csrc/plom_cov.h): (table [rows][n_fix], kmax)."""
+    L, model_id = _resolve(model_id)
     times = np.ascontiguousarray(times, dtype=np.float64)
     N = len(times) - 1
-    nfix = lib().oracle_model_n_fix(model_id)
+    nfix = L.oracle_model_n_fix(model_id)
     kmax_bound = int(np.max(np.diff(times)) / dt) + 2 if N else 0
     out = np.zeros(((N * kmax_bound + N + 1), max(nfix, 1)))
     km = C.c_int32()
     n, xs, ys = _cov(cov)
-    rc = lib().oracle_cov_table(model_id, N, _dp(times), dt, _ip(n), _dp(xs), _dp(ys), _dp(out), C.byref(km))
+    rc = L.oracle_cov_table(model_id, N, _dp(times), dt, _ip(n), _dp(xs), _dp(ys), _dp(out), C.byref(km))
     if rc != 0:
         raise ValueError("covariate table: %d" % rc)
     rows = N * km.value + N + 1
@@ -165,13 +200,14 @@ def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=
     ([N+1][J][S], for tests on small runs); psr: the Euler-multinomial implementation;
     cov: time-varying covariates, one (x, y) knot pair per covariate slot of pv; ode: the
     adaptive RKF45 implementation under eps_abs / eps_rel."""
+    L, model_id = _resolve(model_id)
     noises_off = _impl(noises_off, psr, ode)
     times = np.ascontiguousarray(times, dtype=np.float64)
     N = len(times) - 1
     data = np.ascontiguousarray(data, dtype=np.float64)
     pv = np.ascontiguousarray(pv, dtype=np.float64)
     x0 = np.ascontiguousarray(x0, dtype=np.float64)
-    S = lib().oracle_model_size_proj(model_id)
+    S = L.oracle_model_size_proj(model_id)
     ll = C.c_double()
     naf = C.c_int32()
     ll_n = np.zeros(N)
@@ -182,13 +218,13 @@ def smc(model_id, J, times, data, pv, x0, noises_off=0, flags=0, seed=1, stream=
     lsl = np.zeros(N, dtype=np.int32) if lineage else None
     lst = np.zeros((N + 1, S)) if lineage else None
     alls = np.zeros((N + 1, J, S)) if all_states else None
-    NH = lib().oracle_model_n_hat(model_id)
-    n_ts = lib().oracle_model_n_ts(model_id)
+    NH = L.oracle_model_n_hat(model_id)
+    n_ts = L.oracle_model_n_ts(model_id)
     ht = np.zeros((N + 1, NH, 3)) if hat else None
     lob = np.zeros((N + 1, n_ts, 2)) if lineage else None
     pr = np.zeros((N, n_ts, 2)) if hat else None
     el = C.c_double()
-    rc = lib().oracle_smc(model_id, J, N, noises_off, flags, seed, stream, dt, like_min, _dp(times), _dp(data),
+    rc = L.oracle_smc(model_id, J, N, noises_off, flags, seed, stream, dt, like_min, _dp(times), _dp(data),
                           _dp(pv), _dp(x0), mode, nthreads, C.byref(ll), C.byref(naf), _dp(ll_n), _dp(ess),
                           _ip(anc), _dp(fs), _dp(lw), C.byref(el), _ip(lsl), _dp(lst), _dp(alls), _dp(ht), _dp(pr), _dp(lob),
                           *_cov_args(cov), *_eps(eps_abs, eps_rel), *_drift_args(drift))
@@ -208,6 +244,7 @@ def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=
     [n_theta] (the prior patch of the weights); theta0 [n_theta, J]; x0 [S, J]; mut_fac [N];
     drift: the drift map of a model with diffusions (the volatilities are read from each
     particle's own parameters)."""
+    L, model_id = _resolve(model_id)
     noises_off = _impl(noises_off, psr, ode)
     times = np.ascontiguousarray(times, dtype=np.float64)
     N = len(times) - 1
@@ -227,7 +264,7 @@ def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=
     tho = np.zeros((N + 1, nth, J)) if theta_out else None
     lw = np.zeros((N, J)) if logw else None
     anc = np.zeros((N, J), dtype=np.int32) if ancestry else None
-    rc = lib().oracle_mif_pass(model_id, J, N, noises_off, seed, stream, dt, like_min, _dp(times), _dp(data), _dp(pv),
+    rc = L.oracle_mif_pass(model_id, J, N, noises_off, seed, stream, dt, like_min, _dp(times), _dp(data), _dp(pv),
                                nth, _ip(m["pv_src"]), _ip(m["tr"]), _ip(m["scale"]), _ip(m["duration"]), _dp(m["min"]),
                                _dp(m["max"]), _dp(m["multiplier"]), _dp(m["sd"]), _dp(mut_fac), _dp(theta0), _dp(x0),
                                nthreads, C.byref(ll), C.byref(naf), _dp(ll_n), _dp(ess), _dp(mm), _dp(mv), _dp(tho),
@@ -271,12 +308,13 @@ def mif_prior_term(kind, e, tr, theta, a, b, like_min=1e-17):
 
 def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=0.25, psr=False, cov=None, ode=False,
                eps_abs=0.0, eps_rel=0.0, drift=None):
+    L, model_id = _resolve(model_id)
     noises_off = _impl(noises_off, psr, ode)
     times = np.ascontiguousarray(times, dtype=np.float64)
     N = len(times) - 1
-    S = lib().oracle_model_size_proj(model_id)
+    S = L.oracle_model_size_proj(model_id)
     out = np.zeros((N + 1, S))
-    rc = lib().oracle_trajectory(model_id, N, noises_off, seed, stream, j, dt, _dp(times),
+    rc = L.oracle_trajectory(model_id, N, noises_off, seed, stream, j, dt, _dp(times),
                                  _dp(np.ascontiguousarray(pv, dtype=np.float64)),
                                  _dp(np.ascontiguousarray(x0, dtype=np.float64)), _dp(out), *_cov_args(cov),
                                  *_eps(eps_abs, eps_rel), *_drift_args(drift))
@@ -287,9 +325,10 @@ def trajectory(model_id, times, pv, x0, noises_off=0, seed=1, stream=0, j=0, dt=
 
 def obs_draw(model_id, x, pv, t, seed, stream, n, slot, cvd=None):
     """(observation means [n_ts], drawn observations [n_ts]) of state x (plom_obs_draw)."""
-    n_ts = lib().oracle_model_n_ts(model_id)
+    L, model_id = _resolve(model_id)
+    n_ts = L.oracle_model_n_ts(model_id)
     out = np.zeros(2 * n_ts)
-    lib().oracle_obs_draw(model_id, _dp(np.ascontiguousarray(x, dtype=np.float64)),
+    L.oracle_obs_draw(model_id, _dp(np.ascontiguousarray(x, dtype=np.float64)),
                           _dp(np.ascontiguousarray(pv, dtype=np.float64)),
                           _dp(np.ascontiguousarray(cvd, dtype=np.float64)) if cvd is not None else None, t, seed, stream,
                           n, slot, _dp(out))
@@ -297,11 +336,12 @@ def obs_draw(model_id, x, pv, t, seed, stream, n, slot, cvd=None):
 
 
 def obs_loglik(model_id, x, y, pv, t=0.0, like_min=1e-17):
-    n_ts = lib().oracle_model_n_ts(model_id)
+    L, model_id = _resolve(model_id)
+    n_ts = L.oracle_model_n_ts(model_id)
     obs = np.zeros(max(n_ts, 1))
     ll = C.c_double()
     nf = C.c_int32()
-    lib().oracle_obs_loglik(model_id, _dp(np.ascontiguousarray(x, dtype=np.float64)),
+    L.oracle_obs_loglik(model_id, _dp(np.ascontiguousarray(x, dtype=np.float64)),
                             _dp(np.ascontiguousarray(y, dtype=np.float64)),
                             _dp(np.ascontiguousarray(pv, dtype=np.float64)), t, like_min, _dp(obs),
                             C.byref(ll), C.byref(nf))
@@ -310,7 +350,7 @@ def obs_loglik(model_id, x, y, pv, t=0.0, like_min=1e-17):
 
 def math(x):
     x = np.ascontiguousarray(x, dtype=np.float64)
-    out = np.zeros((x.size, 7))
+    out = np.zeros((x.size, 8))
     lib().oracle_math(x.size, _dp(x), _dp(out))
     return out
 
@@ -326,6 +366,14 @@ def bm_pairs(xy):
     xy = np.ascontiguousarray(xy, dtype=np.uint32).reshape(-1, 2)
     out = np.zeros(xy.shape)
     lib().oracle_bm_pairs(len(xy), xy.ctypes.data_as(C.POINTER(C.c_uint32)), _dp(out))
+    return out
+
+
+def icdf(words):
+    """The stream's Gaussians of the 32-bit words (plom_icdf_normal)."""
+    w = np.ascontiguousarray(words, dtype=np.uint32).ravel()
+    out = np.zeros(w.size)
+    lib().oracle_icdf(w.size, w.ctypes.data_as(C.POINTER(C.c_uint32)), _dp(out))
     return out
 
 

@@ -80,12 +80,23 @@ typedef struct {
       plom_##lname##_interval_f64,                                                                   \
       plom_##lname##_proj2obs_f64, plom_##lname##_loglik, plom_##lname##_obs_moments, plom_##lname##_hat_values }
 
+/* an extra model (a user's model outside the library, built from its generated header in
+ * another directory): -DORACLE_EXTRA_MODEL=<name> -DORACLE_EXTRA_UNAME=PLOM_<NAME>
+ * -DORACLE_EXTRA_HEADER='"plom_<name>.h"'; it takes the id after the built-in models */
+#define ORACLE_MODEL_X(lname, UNAME) ORACLE_MODEL(lname, UNAME)
+#ifdef ORACLE_EXTRA_MODEL
+#include ORACLE_EXTRA_HEADER
+#endif
+
 static const oracle_model MODELS[] = {
     ORACLE_MODEL(sir, PLOM_SIR),
     ORACLE_MODEL(sirs4, PLOM_SIRS4),
     ORACLE_MODEL(seir10, PLOM_SEIR10),
     ORACLE_MODEL(noise, PLOM_NOISE),
     ORACLE_MODEL(drift, PLOM_DRIFT),
+#ifdef ORACLE_EXTRA_MODEL
+    ORACLE_MODEL_X(ORACLE_EXTRA_MODEL, ORACLE_EXTRA_UNAME),
+#endif
 };
 #define N_MODELS ((int)(sizeof MODELS / sizeof MODELS[0]))
 
@@ -102,7 +113,7 @@ void oracle_psr_draws(int kind, double a, double b, uint32_t seed, uint32_t stre
                       const double *p, double *out)
 {
     plom_nstream64 ns;
-    plom_nstream64_init(&ns, seed, stream, n, j, plom_bm_tab);
+    plom_nstream64_init(&ns, seed, stream, n, j, plom_icdf_tab);
     if (kind == 5) {
         plom_multinomial(&ns, count, a, p, out);
         return;
@@ -299,7 +310,7 @@ static void propagate_range(prop_job *p)
         double *x = p->X + (size_t)j * S;
         const double *pv = p->pv + (size_t)j * p->pv_stride;
         plom_nstream64 ns;
-        plom_nstream64_init(&ns, p->seed, p->stream, (uint32_t)p->n, (uint32_t)j, plom_bm_tab);
+        plom_nstream64_init(&ns, p->seed, p->stream, (uint32_t)p->n, (uint32_t)j, plom_icdf_tab);
         M->interval(x, pv, &ns, p->times[p->n], p->times[p->n + 1], p->dt, p->sqdt, p->noises_off, cv_steps(p->cv, p->n),
                     p->cv->cs, p->eps_abs, p->eps_rel, p->dm, cv_splines(p->cv, p->noises_off));
         if (p->filter && p->nonan) {
@@ -701,7 +712,7 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
         memcpy(lin_state, x, (size_t)S * sizeof(double));
         for (int n = 0; n < N; n++) {
             plom_nstream64 ns;
-            plom_nstream64_init(&ns, seed, stream, (uint32_t)n, (uint32_t)lin_slots[n], plom_bm_tab);
+            plom_nstream64_init(&ns, seed, stream, (uint32_t)n, (uint32_t)lin_slots[n], plom_icdf_tab);
             M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off, cv_steps(&cv, n), cv.cs, eps_abs,
                         eps_rel, dm, cv_splines(&cv, noises_off));
             memcpy(lin_state + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
@@ -949,7 +960,7 @@ int oracle_trajectory(int model, int N, uint32_t noises_off, uint32_t seed, uint
     memcpy(out, x0, (size_t)S * sizeof(double));
     for (int n = 0; n < N; n++) {
         plom_nstream64 ns;
-        plom_nstream64_init(&ns, seed, stream, (uint32_t)n, j, plom_bm_tab);
+        plom_nstream64_init(&ns, seed, stream, (uint32_t)n, j, plom_icdf_tab);
         M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off, cv_steps(&cv, n), cv.cs, eps_abs,
                     eps_rel, drift_view(&da), cv_splines(&cv, noises_off));
         memcpy(out + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
@@ -1040,13 +1051,14 @@ void oracle_math(int n, const double *in, double *out)
 {
     for (int i = 0; i < n; i++) {
         const double x = in[i];
-        out[i * 7 + 0] = plom_exp(x);
-        out[i * 7 + 1] = plom_log(x);
-        out[i * 7 + 2] = plom_Phi(x);
-        out[i * 7 + 3] = plom_sin(x);
-        out[i * 7 + 4] = plom_cos(x);
-        out[i * 7 + 5] = plom_sqrt(x);
-        out[i * 7 + 6] = plom_sqrt_mid(x);
+        out[i * 8 + 0] = plom_exp(x);
+        out[i * 8 + 1] = plom_log(x);
+        out[i * 8 + 2] = plom_Phi(x);
+        out[i * 8 + 3] = plom_sin(x);
+        out[i * 8 + 4] = plom_cos(x);
+        out[i * 8 + 5] = plom_sqrt(x);
+        out[i * 8 + 6] = plom_sqrt_mid(x);
+        out[i * 8 + 7] = plom_sqrt_rate(x);
     }
 }
 
@@ -1054,7 +1066,7 @@ void oracle_math(int n, const double *in, double *out)
 void oracle_normals(uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int n, double *out)
 {
     plom_nstream64 ns;
-    plom_nstream64_init(&ns, seed, stream, n0, j, plom_bm_tab);
+    plom_nstream64_init(&ns, seed, stream, n0, j, plom_icdf_tab);
     for (int i = 0; i + 1 < n; i += 2) plom_normal_pair64(&ns, &out[i], &out[i + 1]);
     if (n & 1) {
         double spare;
@@ -1066,6 +1078,12 @@ void oracle_normals(uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int
 void oracle_bm_pairs(int n, const uint32_t *xy, double *out)
 {
     for (int i = 0; i < n; i++) plom_bm_pair(xy[2 * i], xy[2 * i + 1], plom_bm_tab, &out[2 * i], &out[2 * i + 1]);
+}
+
+/* the stream's Gaussian transform of the 32-bit words w[0..n) (plom_icdf_normal) */
+void oracle_icdf(int n, const uint32_t *w, double *out)
+{
+    for (int i = 0; i < n; i++) out[i] = plom_icdf_normal(w[i], plom_icdf_tab);
 }
 
 void oracle_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) { plom_philox4x32_10(ctr, key, out); }

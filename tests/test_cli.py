@@ -532,3 +532,54 @@ def test_cli_prior(tmp_path, ctx):
         assert r.returncode == 0, r.stderr
         lls.append(float([l for l in r.stdout.splitlines() if l.startswith("logV_17g:")][0].split(":")[1]))
     assert lls[1] == pytest.approx(lls[0] + lp, rel=1e-13)
+
+
+def user_model_dir(tmp_path):
+    import user_model as U
+    U.load(build_if_missing=True)
+    d = tmp_path / U.NAME
+    d.mkdir()
+    subprocess.check_call(["python", os.path.join(PKG, "plom_build.py"), U.DIR, "-o", str(d), "--model_library", U.LIBRARY],
+                          stdout=subprocess.DEVNULL)
+    return d, U
+
+
+def test_user_model_dry_run(tmp_path):
+    """A model outside the library through the command: the settings name its model
+    library, which `smc` registers; the theta of the noise example has no initial condition
+    for R, which the model has as a state, so R takes the theta remainder 1 - S - I
+    (build.c:456-468, proposal.c:87-114)."""
+    d, U = user_model_dir(tmp_path)
+    s = json.load(open(d / ".settings.json"))
+    assert s["model"] == U.NAME and s["POP_SIZE_EQ_SUM_SV"] is True and s["model_library"] == U.LIBRARY
+    r = run_cli(d, ["sde", "-J", "10", "--dry_run"], os.path.join(U.DIR, "theta.json"))
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    x0 = np.array(out["x0"])
+    assert len(x0) == 8
+    np.testing.assert_allclose(x0[0:2] + x0[2:4] + x0[4:6], [1e6, 6e5], rtol=1e-12)
+    # without the model library the command refuses the model
+    s.pop("model_library")
+    (d / ".settings.json").write_text(json.dumps(s))
+    r = run_cli(d, ["sde", "-J", "10", "--dry_run"], os.path.join(U.DIR, "theta.json"))
+    assert r.returncode != 0 and "unknown model" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_user_model_matches_api(tmp_path, ctx):
+    """`smc sde` on the model outside the library gives the API's and the oracle's
+    log-likelihood."""
+    from oracle import oracle
+    d, U = user_model_dir(tmp_path)
+    theta = os.path.join(U.DIR, "theta.json")
+    m = smc.Model.load(U.DIR, U.LIBRARY)
+    times, data = m.dataset()
+    cov = m.covariates()
+    dry = json.loads(run_cli(d, ["sde", "-J", "2000", "--dry_run"], theta).stdout)
+    pv, x0 = np.array(dry["pv"]), np.array(dry["x0"])
+    print("cli user model", flush=True)
+    r = run_cli(d, ["sde", "-J", "2000", "-b", "-e", "-d"], theta)
+    assert r.returncode == 0, r.stderr
+    ll_cli = float([l for l in r.stdout.splitlines() if l.startswith("logV_17g:")][0].split(":")[1])
+    assert ll_cli == smc.run(ctx, m, 2000, times, data, pv, x0, seed=2, stream=0, cov=cov).loglik
+    assert ll_cli == oracle.smc(U.NAME, 2000, times, data, pv, x0, seed=2, stream=0, cov=cov, nthreads=8)["loglik"]

@@ -60,10 +60,35 @@ def test_sqrt_mid_bitwise(ctx):
     np.testing.assert_array_equal(got[:, 6], np.sqrt(x))
 
 
+def test_sqrt_rate_bitwise(ctx):
+    """The device's demographic-noise square root (Newton core and one integer compare)
+    equals the oracle's definition on its whole domain: normal and subnormal arguments on
+    both sides of 2^-767, zeros, negative numbers, infinities and NaN."""
+    rng = np.random.default_rng(9)
+    x = np.concatenate([np.exp2(rng.uniform(-1074, 1023, 1000000)), rng.uniform(0, 1e6, 200000),
+                        -np.exp2(rng.uniform(-1074, 1023, 1000)),
+                        [0.0, -0.0, 5e-324, 2.0 ** -767, np.nextafter(2.0 ** -767, 0.0), np.nextafter(2.0 ** -767, 1.0),
+                         np.finfo(float).max, np.inf, -np.inf, np.nan]])
+    np.testing.assert_array_equal(smc.selftest_math(ctx, x)[:, 7], oracle.math(x)[:, 7])
+
+
 def test_normals_bitwise(ctx):
     for (seed, stream, n0, j) in [(1, 0, 0, 0), (7, 3, 11, 12345), (0xffffffff, 0xffffffff, 1039, 999999)]:
         np.testing.assert_array_equal(smc.selftest_normals(ctx, seed, stream, n0, j, 257),
                                       oracle.normals(seed, stream, n0, j, 257))
+
+
+def test_fp32_normals_are_the_fp64_normals_rounded(ctx):
+    """The FP32 stream draws the FP64 stream's Gaussians: the same words through the same
+    inverse-CDF segments in FP32, from the top 24 bits of each word (|u32 - u64| <= 2^-25),
+    so |z32 - z64| <= 2^-25 / phi(z) + FP32 rounding."""
+    from scipy.stats import norm
+    for (seed, stream, n0, j) in [(1, 0, 0, 0), (7, 3, 11, 12345), (0xffffffff, 0xffffffff, 1039, 999999)]:
+        z64 = oracle.normals(seed, stream, n0, j, 20000)
+        z32 = smc.selftest_normals32(ctx, seed, stream, n0, j, 20000)
+        bound = 2.0 ** -25 / norm.pdf(np.maximum(np.abs(z32), np.abs(z64))) * 1.01 + 1e-6
+        assert np.all(np.abs(z32 - z64) <= bound)
+        assert np.all((z32 >= 0) == (z64 >= 0))
 
 
 def test_sir_j1000_n52_parity(ctx):

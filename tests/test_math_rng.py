@@ -68,6 +68,20 @@ def test_exp_log_sin_cos_accuracy():
         assert row[5] == math.sqrt(x)
 
 
+def test_sqrt_rate():
+    """The demographic-noise square root: IEEE sqrt on [2^-767, inf), 0 on [+0, 2^-767),
+    NaN for negative numbers, -0, +inf and NaN."""
+    rng = np.random.default_rng(3)
+    big = np.concatenate([np.exp2(rng.uniform(-767, 1023, 20000)), rng.uniform(0, 1e6, 20000),
+                          [2.0 ** -767, np.nextafter(2.0 ** -767, 1.0), 1.0, 4.0, np.finfo(float).max]])
+    np.testing.assert_array_equal(oracle.math(big)[:, 7], np.sqrt(big))
+    tiny = np.concatenate([[0.0, 5e-324, 2.0 ** -1022, np.nextafter(2.0 ** -767, 0.0)],
+                           np.exp2(rng.uniform(-1074, -767.0001, 1000))])
+    np.testing.assert_array_equal(oracle.math(tiny)[:, 7], 0.0)
+    odd = np.array([-0.0, -1.0, -5e-324, np.inf, -np.inf, np.nan])
+    assert np.all(np.isnan(oracle.math(odd)[:, 7]))
+
+
 def test_special_values():
     r = oracle.math(np.array([0.0, -0.0, np.inf, -np.inf, np.nan, -1.0, 1000.0, -1000.0]))
     assert r[0, 1] == -np.inf and r[2, 1] == np.inf and np.isnan(r[5, 1]) and np.isnan(r[4, 1])
@@ -147,10 +161,60 @@ def test_box_muller_near_one_keeps_absolute_accuracy():
 
 
 def test_normal_streams_are_the_transform_of_xoshiro_words():
-    """oracle.normals(seed, stream, n, j) = Box-Muller of the xoshiro128** words seeded by
-    Philox(ctr = (j, n, 0, 0), key = (seed, stream)), pairs in order."""
+    """oracle.normals(seed, stream, n, j) = the inverse-CDF transform of the xoshiro128**
+    words seeded by Philox(ctr = (j, n, 0, 0), key = (seed, stream)), one word each, in
+    order."""
     seed, stream, n, j = 9, 4, 17, 300
     w = oracle.philox([j, n, 0, 0], [seed, stream])
     words, _ = oracle.xoshiro(w, 40)
     z = oracle.normals(seed, stream, n, j, 40)
-    np.testing.assert_array_equal(z, oracle.bm_pairs(np.array(words, dtype=np.uint32).reshape(-1, 2)).ravel())
+    np.testing.assert_array_equal(z, oracle.icdf(words))
+
+
+def icdf_exact(w):
+    """+-Phi^-1((v + 1/2) 2^-32), v = w >> 1, positive for odd w, in 40-digit arithmetic."""
+    u = (mp.mpf(int(w) >> 1) + mp.mpf(1) / 2) / 2 ** 32
+    z = mp.sqrt(2) * mp.erfinv(2 * u - 1)
+    return -z if int(w) & 1 else z
+
+
+def test_icdf_transform_against_mpmath():
+    """The stream's Gaussian transform (plom_icdf_normal) against the exact inverse normal
+    CDF: random words, the extreme words (u -> 2^-33 and u -> 1/2), and the first and last
+    word of every polynomial segment (binary octave x 16 sub-segments) with its neighbours.
+    Bound: 4e-14 absolute (the fits are 2e-14; |z| <= 6.24)."""
+    rng = np.random.default_rng(12)
+    ws = [int(x) for x in rng.integers(0, 2 ** 32, 3000, dtype=np.uint64)]
+    ws += [0, 1, 2, 3, 4, 5, 2 ** 32 - 1, 2 ** 32 - 2, 2 ** 32 - 3, 2 ** 32 - 4]
+    for e in range(-32, -1):                  # segment starts: u = 2^e (1 + s/16), v = u 2^32 - 1/2
+        for s_ in range(16):
+            v = int((2 ** (e + 32)) * (16 + s_) // 16)
+            for d in (-1, 0, 1):
+                if 0 <= v + d < 2 ** 31:
+                    ws += [2 * (v + d), 2 * (v + d) + 1]
+    z = oracle.icdf(np.array(ws, dtype=np.uint64) % 2 ** 32)
+    worst = max(abs(zi - float(icdf_exact(w))) for w, zi in zip(ws, z))
+    assert worst <= 4e-14, worst
+
+
+def test_icdf_symmetry_and_range():
+    """z(w) = -z(w ^ 1) exactly; the extremes are +-Phi^-1(2^-33) = -+6.2376..."""
+    rng = np.random.default_rng(13)
+    w = rng.integers(0, 2 ** 31, 20000, dtype=np.uint64) * 2
+    np.testing.assert_array_equal(oracle.icdf(w), -oracle.icdf(w + 1))
+    e = oracle.icdf(np.array([0, 1], dtype=np.uint32))
+    assert e[0] == pytest.approx(float(mp.sqrt(2) * mp.erfinv(2 * mp.mpf(2) ** -33 - 1)), abs=4e-14)
+    assert e[1] == -e[0]
+
+
+def test_icdf_distribution():
+    """One million Gaussians of the particle streams: Kolmogorov-Smirnov against N(0, 1)
+    and the first four moments within their sampling error."""
+    from scipy import stats
+    z = np.concatenate([oracle.normals(3, 7, n, j, 1000) for n in range(10) for j in range(100)])
+    assert stats.kstest(z, "norm").pvalue > 1e-3
+    m = len(z)
+    assert abs(z.mean()) < 5 / np.sqrt(m)
+    assert abs(z.var() - 1) < 5 * np.sqrt(2 / m)
+    assert abs(stats.skew(z)) < 5 * np.sqrt(6 / m)
+    assert abs(stats.kurtosis(z)) < 5 * np.sqrt(24 / m)

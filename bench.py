@@ -125,7 +125,8 @@ def batched_leg(smc, ctx, filters, steps, warmup, lanes=0):
     if lanes:
         b.configure(lanes)
     else:
-        lanes, tuned = b.autotune(rounds=32)
+        tuned = b.tune(rounds=32)
+        lanes = tuned["lanes"]
     b.launch(use_graph=True)              # capture + first replay
     for _ in range(warmup):
         b.launch(use_graph=True)
@@ -146,7 +147,7 @@ def batched_leg(smc, ctx, filters, steps, warmup, lanes=0):
     ctx.set_kernel_timing(False)
     b.close()
     return {"wall_s": wall, "loglik": ll, "prop_ms": prop_ms, "scan_ms": scan_ms, "total_ms": total_ms,
-            "launches": nprop, "rounds": nint, "lanes": lanes, "autotune_ms": tuned,
+            "launches": nprop, "rounds": nint, "lanes": lanes, "tuning": tuned,
             "lane_prop_ms": lane_prop, "lane_scan_ms": lane_scan}
 
 
@@ -247,9 +248,27 @@ def memory_leg(smc, ctx, inp, configs, J, N):
             "loglik": r.loglik, "lineage_final_slot": int(r.lineage_slots[-1])}
 
 
+def pick_tiles_per_block(ctx, f):
+    """The tiles per propagate block (1, 2, 4) under which one run of the filter is fastest
+    (a warm run first); returns (chosen, {tpb: ms})."""
+    ms = {}
+    for tpb in (1, 2, 4):
+        ctx.set_tiles_per_block(tpb)
+        f.run()
+        ctx.sync()
+        t0 = time.perf_counter()
+        f.run()
+        ctx.sync()
+        ms[tpb] = round((time.perf_counter() - t0) * 1e3, 3)
+    best = min(ms, key=ms.get)
+    ctx.set_tiles_per_block(best)
+    return best, ms
+
+
 def single_leg(smc, ctx, inp, configs, precision, steps, warmup, J, N):
     m, times, data, pv, x0 = inp.get("sirs4", N, configs.TRUTH["sirs4"])
     f = smc.Filter(ctx, m, J, times, data, pv, x0, precision=precision, seed=configs.FILTER_SEED, stream=0)
+    tpb, tpb_ms = pick_tiles_per_block(ctx, f)
     for _ in range(warmup):
         f.run()
     ctx.sync()
@@ -263,9 +282,11 @@ def single_leg(smc, ctx, inp, configs, precision, steps, warmup, J, N):
     f.run()
     _, prop_ms, _, nint, _ = ctx.last_timing()
     ctx.set_kernel_timing(False)
+    ctx.set_tiles_per_block(1)
     f.close()
     return {"value": round(J * N / wall, 1), "ms_per_filter": round(wall * 1e3, 3), "loglik": res.loglik,
-            "propagate_ms_per_launch": round(prop_ms / max(nint, 1), 5)}
+            "propagate_ms_per_launch": round(prop_ms / max(nint, 1), 5), "tiles_per_block": tpb,
+            "ms_per_filter_by_tiles_per_block": tpb_ms}
 
 
 def psr_leg(smc, ctx, inp, configs, steps, warmup, J, N, cpu, cpu_J, cpu_N):
@@ -408,11 +429,11 @@ def main():
         rel32 = np.abs(fp32["loglik"] - fp64["loglik"]) / np.abs(fp64["loglik"])
         out["batched"] = {
             "fp64": {"value": out["value"], "ms_per_pass": out["ms_per_step"], "graph": True,
-                     "lanes": fp64["lanes"], "autotune_ms_32_rounds": fp64["autotune_ms"],
+                     "lanes": fp64["lanes"], "tuning": fp64["tuning"],
                      "propagate_ms": round(fp64["prop_ms"], 3), "weights_scan_ms": round(fp64["scan_ms"], 3),
                      "launch_rounds": fp64["rounds"], "propagate_launches": fp64["launches"]},
             "fp32": {"value": round(ps_batch / fp32["wall_s"], 1), "ms_per_pass": round(fp32["wall_s"] * 1e3, 3),
-                     "lanes": fp32["lanes"],
+                     "lanes": fp32["lanes"], "tuning": fp32["tuning"],
                      "mean_rel_dev_vs_fp64": float(rel32.mean()), "max_rel_dev_vs_fp64": float(rel32.max()),
                      "note": "FP32 propagation with FP64 weights: a different Monte Carlo estimate, not an "
                              "approximation of the FP64 one; the tests check its mean over replicates "

@@ -314,6 +314,31 @@ int plom_batch_destroy(plom_ctx *ctx, plom_batch *batch);
 #define PLOM_MAX_LANES 8
 int plom_batch_configure(plom_ctx *ctx, plom_batch *batch, int lanes);
 int plom_batch_autotune(plom_ctx *ctx, plom_batch *batch, int rounds, int32_t *lanes_out, double *ms_out);
+
+/* Tiles per propagate block (no counterpart in the reference): a propagate block holds 1, 2
+ * or 4 tiles of 256 particles (256, 512 or 1024 threads; every tile keeps its own weight
+ * statistics, so results do not depend on it).  Larger blocks load the Gaussians' LDS table
+ * once for more particles; smaller ones schedule more finely.  plom_batch_tune extends
+ * plom_batch_autotune: after the lanes it tunes the tiles per block of every (model,
+ * precision, variant, MIF) group of the batch in turn (1, 2, 4; the ode implementation and
+ * MIF filters use 1) and reports every timing; plom_batch_autotune also does this and
+ * reports the lanes part.  plom_ctx_set_tiles_per_block sets the value single filters and
+ * the server use, and batch groups that were not tuned (default 1). */
+#define PLOM_TUNE_MAX_GROUPS 16
+typedef struct {
+    int32_t model_id, precision, variant, mif;  /* variant: 0-3 sde noise switches, 4-5 psr, 6 ode */
+    int32_t tiles_per_block;                     /* chosen */
+    int32_t pad;
+    double ms[3];                                /* the rounds' time with 1, 2, 4 tiles per block (0: not tried) */
+} plom_tune_group;
+typedef struct {
+    int32_t rounds, lanes;                       /* launch rounds timed per candidate; lanes chosen */
+    double lanes_ms[4];                          /* with 1, 2, 4, 8 lanes (0: not tried) */
+    int32_t n_groups, pad;
+    plom_tune_group group[PLOM_TUNE_MAX_GROUPS];
+} plom_batch_tuning;
+int plom_batch_tune(plom_ctx *ctx, plom_batch *batch, int rounds, plom_batch_tuning *out);
+int plom_ctx_set_tiles_per_block(plom_ctx *ctx, int tiles_per_block);
 int plom_batch_run(plom_ctx *ctx, int n_filters, plom_filter **filters, int use_graph); /* create + launch + destroy */
 
 /* Served stream (no counterpart in the reference): continuous batching of filter

@@ -21,13 +21,15 @@ script prints the largest error of the double evaluation against mpmath over 65 
 of every segment (about 2e-14 absolute; the tests check it again).
 
 Layout (coefficient-major, so that one LDS address and constant offsets reach all seven
-coefficients of a segment): tab[i * NSEG + idx] = c_i of segment idx.
+coefficients of a segment): tab[i * NSEG + idx] = c_i of segment idx.  The header also
+holds the coefficients rounded to float, for the FP32 stream.
 
 Run:  python gen_icdf_table.py > plom_icdf_table.h
 """
 import sys
 
 import mpmath as mp
+import numpy as np
 
 mp.mp.dps = 50
 
@@ -82,6 +84,12 @@ def main():
     for i in range(DEG + 1):
         for idx in range(0, NSEG, 4):
             out.append("    " + ", ".join(coef[j][i].hex() for j in range(idx, idx + 4)) + ", \\")
+    out.append("}")
+    out.append("/* the same coefficients rounded to float (the FP32 stream) */")
+    out.append("#define PLOM_ICDF_TAB32_VALUES { \\")
+    for i in range(DEG + 1):
+        for idx in range(0, NSEG, 4):
+            out.append("    " + ", ".join("%sf" % float(np.float32(coef[j][i])).hex() for j in range(idx, idx + 4)) + ", \\")
     out.append("}")
     out.append("#endif")
     print("\n".join(out))

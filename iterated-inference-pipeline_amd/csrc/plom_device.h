@@ -104,7 +104,7 @@ __device__ inline double tree256(double v, double *lds)
 {
 #pragma unroll
     for (int h = 1; h < 64; h <<= 1) v = v + __shfl_xor(v, h, 64);
-    const int w = threadIdx.x >> 6;
+    const int w = (threadIdx.x >> 6) & 3;
     if ((threadIdx.x & 63) == 0) lds[w] = v;
     __syncthreads();
     double r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
@@ -116,7 +116,7 @@ __device__ inline double max256(double v, double *lds)
 {
 #pragma unroll
     for (int h = 1; h < 64; h <<= 1) v = fmax(v, __shfl_xor(v, h, 64));
-    const int w = threadIdx.x >> 6;
+    const int w = (threadIdx.x >> 6) & 3;
     if ((threadIdx.x & 63) == 0) lds[w] = v;
     __syncthreads();
     double r = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
@@ -227,56 +227,75 @@ __device__ inline void mif_particle(const plom_dfilter *f, int n, int j, int a, 
     }
 }
 
-template <class M, typename R, int V, bool MIF>
-__global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const int2 *tiles)
+/* One observation interval for TPB tiles of 256 particles per block (blocks of 256 TPB
+ * threads): sub-block q = threadIdx.x / 256 takes entry blockIdx.x TPB + q of the tile map
+ * (or that tile of the single filter without a map) and has its own time-function table
+ * and tile statistics; with TPB > 1 every thread passes the same barriers, whatever its
+ * sub-block does.  The tile of the statistics stays 256 particles for every TPB: it fixes
+ * the summation tree of the weights (the canonical arithmetic). */
+template <class M, typename R, int V, bool MIF, int TPB>
+__global__ __launch_bounds__(TILE * TPB) void k_propagate(plom_dfilter *filters, const int2 *tiles, int ntiles)
 {
-    __shared__ double lds[8];
-    int fi = 0, tile = blockIdx.x;
-    if (tiles) { const int2 tt = tiles[blockIdx.x]; fi = tt.x; tile = tt.y; }
-    plom_dfilter *f = &filters[fi];
-    const int n = f->step;
-    if (n >= f->N) return;
-    /* in LDS (FP64 only; the block returns above as a whole): the coefficient table of the
-     * stream's Gaussians (not for the ode implementation, which draws none) and, with MIF,
-     * the Box-Muller tables of the random walk */
+    __shared__ double lds[4 * TPB];
+    const int q = TPB > 1 ? (int)(threadIdx.x >> 8) : 0;
+    const int tid = TPB > 1 ? (int)(threadIdx.x & (TILE - 1)) : (int)threadIdx.x;
+    const int e = (int)blockIdx.x * TPB + q;
+    plom_dfilter *f = nullptr;
+    int tile = e, n = 0;
+    bool act = e < ntiles;
+    if (act) {
+        int fi = 0;
+        if (tiles) { const int2 tt = tiles[e]; fi = tt.x; tile = tt.y; }
+        f = &filters[fi];
+        n = f->step;
+        act = n < f->N;
+    }
+    if constexpr (TPB == 1) {
+        if (!act) return;      /* the whole block */
+    }
+    /* in LDS: the coefficient table of the stream's Gaussians (not for the ode
+     * implementation, which draws none; FP64 or its float copy) and, with MIF in FP64, the
+     * Box-Muller tables of the random walk */
     __shared__ double s_icdf[PLOM_ICDF_TAB_SIZE];
     __shared__ float s_icdf32[PLOM_ICDF_TAB_SIZE];
     __shared__ double s_bm[PLOM_BM_TAB_SIZE];
     if constexpr (sizeof(R) == 8 && V != 6) {
-        for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += TILE) s_icdf[i] = plom_icdf_tab[i];
+        for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += TILE * TPB) s_icdf[i] = plom_icdf_tab[i];
     }
     if constexpr (sizeof(R) == 4) {
-        for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += TILE) s_icdf32[i] = (float)plom_icdf_tab[i];
+        for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += TILE * TPB) s_icdf32[i] = plom_icdf_tab32[i];
     }
     if constexpr (sizeof(R) == 8 && MIF) {
-        for (int i = threadIdx.x; i < PLOM_BM_TAB_SIZE; i += TILE) s_bm[i] = plom_bm_tab[i];
+        for (int i = threadIdx.x; i < PLOM_BM_TAB_SIZE; i += TILE * TPB) s_bm[i] = plom_bm_tab[i];
     }
     if constexpr (sizeof(R) == 4 || V != 6 || MIF) __syncthreads();
-    const int J = f->J, Jp = f->Jp;
-    const int j = tile * TILE + threadIdx.x;
-    const bool valid = j < J;
+    const int J = act ? f->J : 0, Jp = act ? f->Jp : 0;
+    const int j = tile * TILE + tid;
+    const bool valid = act && j < J;
 
     R X[M::S];
     int a = j;
-    if (n == 0 && !MIF) {
-        const R *x0 = (const R *)f->x0;
+    if (act) {
+        if (n == 0 && !MIF) {
+            const R *x0 = (const R *)f->x0;
 #pragma unroll
-        for (int k = 0; k < M::S; k++) X[k] = x0[k];
-    } else {
-        /* (with MIF, interval 0 starts from the particles' own initial states in X[0]) */
-        if (n > 0) {
-            if (valid && f->rflag[n - 1]) a = select_ancestor(f, j, n - 1);
-            if (valid && f->ancestry) f->ancestry[(size_t)(n - 1) * J + j] = a;
+            for (int k = 0; k < M::S; k++) X[k] = x0[k];
+        } else {
+            /* (with MIF, interval 0 starts from the particles' own initial states in X[0]) */
+            if (n > 0) {
+                if (valid && f->rflag[n - 1]) a = select_ancestor(f, j, n - 1);
+                if (valid && f->ancestry) f->ancestry[(size_t)(n - 1) * J + j] = a;
+            }
+            const R *Xin = (const R *)f->X[n & 1];
+            if (!valid) a = 0;
+#pragma unroll
+            for (int k = 0; k < M::S; k++) X[k] = Xin[(size_t)k * Jp + a];
         }
-        const R *Xin = (const R *)f->X[n & 1];
-        if (!valid) a = 0;
+        if (n == 0 && f->allX && valid) {
+            R *A = (R *)f->allX;
 #pragma unroll
-        for (int k = 0; k < M::S; k++) X[k] = Xin[(size_t)k * Jp + a];
-    }
-    if (n == 0 && f->allX && valid) {
-        R *A = (R *)f->allX;
-#pragma unroll
-        for (int k = 0; k < M::S; k++) A[(size_t)k * Jp + j] = X[k];
+            for (int k = 0; k < M::S; k++) A[(size_t)k * Jp + j] = X[k];
+        }
     }
     /* per-particle slots of pv (MIF only; the array is unused otherwise) */
     double lp[MIF ? M::NPP : 1];
@@ -287,25 +306,31 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
         }
     }
 
-    const double t0 = f->times[n], t1 = f->times[n + 1];
+    const double t0 = act ? f->times[n] : 0.0, t1 = act ? f->times[n + 1] : 0.0;
     /* the time-only functions of the interval's substeps (the seasonal forcing ...): the same
-     * for every particle of the block, so each substep time is evaluated once, by one
-     * thread, into LDS; the generated step reads the table (identical values: the same
-     * function of the same accumulated t), or evaluates them itself when they do not fit */
+     * for every particle of the tile, so each substep time is evaluated once, by one
+     * thread, into the sub-block's LDS table; the generated step reads the table (identical
+     * values: the same function of the same accumulated t), or evaluates them itself when
+     * they do not fit */
     const R *tft = nullptr;
     if constexpr (M::NTF > 0 && V != 6) {
-        __shared__ double s_tf_raw[PLOM_TF_CAP];
-        R *s_tf = (R *)s_tf_raw;
-        const int K = plom_substeps(t0, t1, f->dt);
-        if (K * M::NTF <= PLOM_TF_CAP) {
-            for (int k = threadIdx.x; k < K; k += TILE) {
+        __shared__ double s_tf_raw[PLOM_TF_CAP * TPB];
+        R *s_tf = (R *)(s_tf_raw + q * PLOM_TF_CAP);
+        const int K = act ? plom_substeps(t0, t1, f->dt) : 0;
+        const bool fits = act && K * M::NTF <= PLOM_TF_CAP;
+        if (fits) {
+            for (int k = tid; k < K; k += TILE) {
                 double t = t0;
-                for (int q = 0; q < k; q++) t += f->dt;
+                for (int r = 0; r < k; r++) t += f->dt;
                 M::time_funcs((R)t, s_tf + k * M::NTF);
             }
-            __syncthreads();
-            tft = s_tf;
         }
+        if constexpr (TPB == 1) {
+            if (fits) __syncthreads();   /* uniform over the block */
+        } else {
+            __syncthreads();
+        }
+        if (fits) tft = s_tf;
     }
     if (valid) {
         if constexpr (V == 6) {
@@ -342,10 +367,12 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
         }
     }
 
-    if ((f->flags & PLOM_FLAG_NO_FILTER) || f->nonan[n] == 0) return;
-
+    const bool wt = act && !(f->flags & PLOM_FLAG_NO_FILTER) && f->nonan[n] > 0;
+    if constexpr (TPB == 1) {
+        if (!wt) return;       /* uniform over the block */
+    }
     double l = -PLOM_INF;
-    if (valid) {
+    if (valid && wt) {
         double obs[M::N_TS];
         int nf = 0;
         const double *cvd = cov_at(f, n + 1);
@@ -372,13 +399,14 @@ __global__ __launch_bounds__(TILE) void k_propagate(plom_dfilter *filters, const
             l = -PLOM_INF;
         }
     }
-    f->logw[j] = l;
-    const double mt = max256(l, lds);
+    if (wt) f->logw[j] = l;
+    double *ql = lds + 4 * q;
+    const double mt = max256(l, ql);
     double w = 0.0;
     if (mt != -PLOM_INF && l != -PLOM_INF) w = plom_exp(l - mt);
-    const double s = tree256(w, lds);
-    const double s2 = tree256(w * w, lds);
-    if (threadIdx.x == 0) { f->tmax[tile] = mt; f->tsum[tile] = s; f->tsum2[tile] = s2; }
+    const double s = tree256(w, ql);
+    const double s2 = tree256(w * w, ql);
+    if (wt && tid == 0) { f->tmax[tile] = mt; f->tsum[tile] = s; f->tsum2[tile] = s2; }
 }
 
 /* The lineage of one filter (one block of one wave; lane 0 works): the final particle by
@@ -397,7 +425,7 @@ __global__ __launch_bounds__(64) void k_lineage(plom_dfilter *f)
         __syncthreads();
     }
     if constexpr (sizeof(R) == 4) {
-        for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += 64) s_icdf32[i] = (float)plom_icdf_tab[i];
+        for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += 64) s_icdf32[i] = plom_icdf_tab32[i];
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
@@ -651,13 +679,15 @@ __global__ void k_hat0(plom_dfilter *f)
 /* What a model's translation unit exports: its dimensions and the host launchers of its
  * kernels.  abi and dfilter_bytes let the library refuse a model library compiled against
  * another version of this header (plom_model_load). */
-#define PLOM_MODEL_ABI 2
+#define PLOM_MODEL_ABI 3
 struct model_entry {
     int32_t abi;            /* PLOM_MODEL_ABI */
     int32_t dfilter_bytes;  /* sizeof(plom_dfilter) */
     const char *name;
     plom_model_info info;
-    void (*launch_prop)(int prec, int variant, int mif, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles);
+    /* one propagate launch over ntiles tiles, tpb (1, 2 or 4) tiles per block */
+    void (*launch_prop)(int prec, int variant, int mif, int ntiles, int tpb, hipStream_t s, plom_dfilter *df,
+                        const int2 *tiles);
     void (*launch_lineage)(int prec, int variant, hipStream_t s, plom_dfilter *df);
     void (*launch_hat)(int prec, hipStream_t s, plom_dfilter *df, int ntiles, int n_hat);
     void (*launch_hat0)(int prec, hipStream_t s, plom_dfilter *df);
@@ -667,32 +697,54 @@ struct model_entry {
 };
 
 
-/* variants 0-3: SDE noise switches; 4, 5: psr with / without white noise (FP64 only) */
+/* one propagate launch over ntiles tiles (tile-map entries, or tiles of one filter without
+ * a map), tpb tiles per block */
+template <class M, typename R, int V, bool MIF, int TPB>
+static void launch_prop_k(int ntiles, hipStream_t s, plom_dfilter *df, const int2 *tiles)
+{
+    hipLaunchKernelGGL((k_propagate<M, R, V, MIF, TPB>), dim3((ntiles + TPB - 1) / TPB), dim3(TILE * TPB), 0, s, df,
+                       tiles, ntiles);
+}
+
+/* tiles per block: 1, 2 or 4 for the sde and psr variants without per-particle parameters,
+ * 1 otherwise (other values fall back to 1) */
+template <class M, typename R, int V, bool MIF>
+static void launch_prop_v(int ntiles, int tpb, hipStream_t s, plom_dfilter *df, const int2 *tiles)
+{
+    if constexpr (!MIF && V != 6) {
+        if (tpb == 2) { launch_prop_k<M, R, V, MIF, 2>(ntiles, s, df, tiles); return; }
+        if (tpb == 4) { launch_prop_k<M, R, V, MIF, 4>(ntiles, s, df, tiles); return; }
+    }
+    launch_prop_k<M, R, V, MIF, 1>(ntiles, s, df, tiles);
+}
+
+/* variants 0-3: SDE noise switches; 4, 5: psr with / without white noise; 6: ode (FP64 only) */
 template <class M, typename R, bool MIF>
-static void launch_prop_r(int variant, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles)
+static void launch_prop_r(int variant, int ntiles, int tpb, hipStream_t s, plom_dfilter *df, const int2 *tiles)
 {
     if constexpr (sizeof(R) == 8) {
-        if (variant == 4) { hipLaunchKernelGGL((k_propagate<M, R, 4, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); return; }
-        if (variant == 5) { hipLaunchKernelGGL((k_propagate<M, R, 5, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); return; }
-        if (variant == 6) { hipLaunchKernelGGL((k_propagate<M, R, 6, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); return; }
+        if (variant == 4) { launch_prop_v<M, R, 4, MIF>(ntiles, tpb, s, df, tiles); return; }
+        if (variant == 5) { launch_prop_v<M, R, 5, MIF>(ntiles, tpb, s, df, tiles); return; }
+        if (variant == 6) { launch_prop_v<M, R, 6, MIF>(ntiles, tpb, s, df, tiles); return; }
     }
     switch (variant) {
-    case 0: hipLaunchKernelGGL((k_propagate<M, R, 0, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
-    case 1: hipLaunchKernelGGL((k_propagate<M, R, 1, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
-    case 2: hipLaunchKernelGGL((k_propagate<M, R, 2, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
-    default: hipLaunchKernelGGL((k_propagate<M, R, 3, MIF>), dim3(nblocks), dim3(TILE), 0, s, df, tiles); break;
+    case 0: launch_prop_v<M, R, 0, MIF>(ntiles, tpb, s, df, tiles); break;
+    case 1: launch_prop_v<M, R, 1, MIF>(ntiles, tpb, s, df, tiles); break;
+    case 2: launch_prop_v<M, R, 2, MIF>(ntiles, tpb, s, df, tiles); break;
+    default: launch_prop_v<M, R, 3, MIF>(ntiles, tpb, s, df, tiles); break;
     }
 }
 
 template <class M>
-static void launch_prop(int prec, int variant, int mif, int nblocks, hipStream_t s, plom_dfilter *df, const int2 *tiles)
+static void launch_prop(int prec, int variant, int mif, int ntiles, int tpb, hipStream_t s, plom_dfilter *df,
+                        const int2 *tiles)
 {
     if (prec == PLOM_FP32) {
-        if (mif) launch_prop_r<M, float, true>(variant, nblocks, s, df, tiles);
-        else launch_prop_r<M, float, false>(variant, nblocks, s, df, tiles);
+        if (mif) launch_prop_r<M, float, true>(variant, ntiles, tpb, s, df, tiles);
+        else launch_prop_r<M, float, false>(variant, ntiles, tpb, s, df, tiles);
     } else {
-        if (mif) launch_prop_r<M, double, true>(variant, nblocks, s, df, tiles);
-        else launch_prop_r<M, double, false>(variant, nblocks, s, df, tiles);
+        if (mif) launch_prop_r<M, double, true>(variant, ntiles, tpb, s, df, tiles);
+        else launch_prop_r<M, double, false>(variant, ntiles, tpb, s, df, tiles);
     }
 }
 

@@ -278,6 +278,9 @@ PLOM_HD uint64_t plom_resample_num(uint32_t seed, uint32_t stream, uint32_t n)
  * FP32 filter draws the FP64 filter's Gaussians rounded to FP32. */
 typedef struct { plom_xoshiro128 r; const float *tab; } plom_nstream32;
 
+/* the coefficients rounded to float; kernels copy them to LDS */
+static __constant__ const float plom_icdf_tab32[PLOM_ICDF_TAB_SIZE] = PLOM_ICDF_TAB32_VALUES;
+
 __device__ inline __attribute__((always_inline)) void
 plom_nstream32_init(plom_nstream32 *ns, uint32_t seed, uint32_t stream, uint32_t n, uint32_t j, const float *tab)
 {

@@ -448,7 +448,7 @@ __global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, d
 __global__ void k_selftest_normals(uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int n, int fp32, double *out)
 {
     __shared__ float s_tab32[PLOM_ICDF_TAB_SIZE];
-    for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += blockDim.x) s_tab32[i] = (float)plom_icdf_tab[i];
+    for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += blockDim.x) s_tab32[i] = plom_icdf_tab32[i];
     __syncthreads();
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     if (fp32) {
@@ -596,6 +596,8 @@ struct plom_ctx {
     std::vector<hipStream_t> lane;  /* batch lane streams, created on first use ([0] unused: lane 0 is `stream`) */
     std::vector<hipEvent_t> join_ev;
     hipEvent_t fork_ev;
+    int tpb;                        /* tiles per propagate block of single filters, the server and
+                                       batch groups without a tuned value */
 };
 
 /* event k of the per-launch timing pool (grown on demand), recorded on stream s */
@@ -818,6 +820,7 @@ extern "C" int plom_ctx_create(int device, plom_ctx **out)
     for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&c->ev[i]));
     c->ktiming = 0;
     c->kspan_rounds = 0;
+    c->tpb = 1;
     *out = c;
     return PLOM_OK;
 }
@@ -1324,7 +1327,7 @@ extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
     c->kspan_rounds = c->ktiming ? f->desc.N : 0;
     for (int n = 0; n < f->desc.N; n++) {
         const int a = kspan_open(c, &k, c->stream);
-        me->launch_prop(f->desc.precision, f->variant, mif, T, c->stream, f->df, nullptr);
+        me->launch_prop(f->desc.precision, f->variant, mif, T, c->tpb, c->stream, f->df, nullptr);
         c->kprop_launches++;
         const int b = kspan_close(c, &k, c->stream, a, 0, 0);
         enqueue_resample(c->stream, f->df, nullptr, T, nullptr, 1, mif ? f->desc.n_theta : 0);
@@ -1477,7 +1480,7 @@ extern "C" int plom_smc_run(plom_ctx *c, const plom_filter_desc *d, const plom_f
  * the results do not depend on the number of chains. */
 struct batch_chain {
     std::vector<plom_filter *> fs;     /* N-descending: row r of the chain's descriptor array */
-    struct group { int model, prec, variant, mif; std::vector<int2> tiles; int2 *d; };
+    struct group { int model, prec, variant, mif; std::vector<int2> tiles; int2 *d; int tpb; };
     std::vector<group> groups;
     std::vector<int2> all_tiles;
     int2 *d_all = nullptr;
@@ -1487,12 +1490,16 @@ struct batch_chain {
     int any_mif = 0;                   /* largest n_theta of the chain's MIF filters (0: none) */
 };
 
+/* tiles per propagate block chosen for the groups of one (model, precision, variant, MIF) */
+struct tpb_rule { int model, prec, variant, mif, tpb; };
+
 struct plom_batch {
     int nf;
     std::vector<plom_filter *> fs;     /* all filters, N-descending */
     int lanes;
     std::vector<batch_chain> chains;
     hipGraphExec_t graph;
+    std::vector<tpb_rule> tpb_rules;   /* set by plom_batch_tune; other groups take the context's */
 };
 
 /* active tiles at interval s: a chain's filters are sorted by N, so the active ones are a prefix */
@@ -1515,7 +1522,7 @@ static int chain_build(plom_ctx *c, batch_chain *ch)
         for (auto &gg : ch->groups)
             if (gg.model == f->desc.model_id && gg.prec == f->desc.precision && gg.variant == f->variant && gg.mif == mif) g = &gg;
         if (!g) {
-            ch->groups.push_back(batch_chain::group{f->desc.model_id, f->desc.precision, f->variant, mif, {}, nullptr});
+            ch->groups.push_back(batch_chain::group{f->desc.model_id, f->desc.precision, f->variant, mif, {}, nullptr, 1});
             g = &ch->groups.back();
         }
         for (int t = 0; t < f->h.T; t++) {
@@ -1555,7 +1562,7 @@ static int chain_propagate(batch_chain *ch, int s, hipStream_t st)
     int launches = 0;
     for (auto &g : ch->groups) {
         const int nb = prefix_len(ch, g.tiles, s);
-        if (nb > 0) { model_at(g.model)->launch_prop(g.prec, g.variant, g.mif, nb, st, ch->ddf, g.d); launches++; }
+        if (nb > 0) { model_at(g.model)->launch_prop(g.prec, g.variant, g.mif, nb, g.tpb, st, ch->ddf, g.d); launches++; }
     }
     return launches;
 }
@@ -1737,6 +1744,11 @@ static int batch_set_lanes(plom_ctx *c, plom_batch *b, int lanes)
     for (auto &ch : b->chains) {
         rc = chain_build(c, &ch);
         if (rc != PLOM_OK) return rc;
+        for (auto &g : ch.groups) {
+            g.tpb = c->tpb;
+            for (auto &t : b->tpb_rules)
+                if (t.model == g.model && t.prec == g.prec && t.variant == g.variant && t.mif == g.mif) g.tpb = t.tpb;
+        }
     }
     return PLOM_OK;
 }
@@ -1774,7 +1786,30 @@ extern "C" int plom_batch_configure(plom_ctx *c, plom_batch *b, int lanes)
     return batch_set_lanes(c, b, lanes);
 }
 
-extern "C" int plom_batch_autotune(plom_ctx *c, plom_batch *b, int rounds, int32_t *lanes_out, double *ms_out)
+/* time `rounds` launch rounds spread over the pass in the batch's current configuration
+ * (a warm pass first: the first use of lanes and kernels) */
+static int batch_time_rounds(plom_ctx *c, plom_batch *b, int rounds, double *ms)
+{
+    int rc = batch_enqueue(c, b, 0, rounds);
+    if (rc != PLOM_OK) return rc;
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    rc = batch_enqueue(c, b, 0, rounds);
+    if (rc != PLOM_OK) return rc;
+    HIPCHK(hipEventRecord(c->ev[3], c->stream));
+    HIPCHK(hipEventSynchronize(c->ev[3]));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, c->ev[2], c->ev[3]));
+    *ms = t;
+    return PLOM_OK;
+}
+
+/* The autotuner (the role of the reference's static J_chunk split of the particles over
+ * threads, C/smc/main.c:298): first the number of lanes (1, 2, 4, 8) with the groups' current
+ * tiles per block, then, lanes fixed, the tiles per block (1, 2, 4) of each (model,
+ * precision, variant, MIF) group in turn, the others fixed; each candidate is the time of
+ * `rounds` launch rounds spread over the pass.  Leaves the batch in the fastest
+ * configuration found; results never depend on it. */
+extern "C" int plom_batch_tune(plom_ctx *c, plom_batch *b, int rounds, plom_batch_tuning *out)
 {
     if (!c || !b) return set_err(PLOM_ERR_ARG, "null argument");
     if (rounds < 1) return set_err(PLOM_ERR_ARG, "rounds=%d < 1", rounds);
@@ -1782,29 +1817,79 @@ extern "C" int plom_batch_autotune(plom_ctx *c, plom_batch *b, int rounds, int32
         if (f->desc.flags & PLOM_FLAG_MIF)
             return set_err(PLOM_ERR_ARG, "autotuning runs the filters and would consume the MIF starting clouds");
     HIPCHK(hipSetDevice(c->device));
-    static const int cand[4] = {1, 2, 4, 8};
+    plom_batch_tuning res;
+    memset(&res, 0, sizeof res);
+    res.rounds = rounds;
+    static const int lane_cand[4] = {1, 2, 4, 8};
     int best = 1;
     double best_ms = 0.0;
     for (int i = 0; i < 4; i++) {
-        if (ms_out) ms_out[i] = 0.0;
-        const int lanes = cand[i];
+        const int lanes = lane_cand[i];
         if (lanes > b->nf) continue;
         int rc = batch_set_lanes(c, b, lanes);
+        if (rc == PLOM_OK) rc = batch_time_rounds(c, b, rounds, &res.lanes_ms[i]);
         if (rc != PLOM_OK) return rc;
-        rc = batch_enqueue(c, b, 0, rounds);           /* warm: first use of the lanes */
-        if (rc != PLOM_OK) return rc;
-        HIPCHK(hipEventRecord(c->ev[2], c->stream));
-        rc = batch_enqueue(c, b, 0, rounds);
-        if (rc != PLOM_OK) return rc;
-        HIPCHK(hipEventRecord(c->ev[3], c->stream));
-        HIPCHK(hipEventSynchronize(c->ev[3]));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-        if (ms_out) ms_out[i] = ms;
-        if (i == 0 || ms < best_ms) { best = lanes; best_ms = ms; }
+        if (i == 0 || res.lanes_ms[i] < best_ms) { best = lanes; best_ms = res.lanes_ms[i]; }
     }
-    if (lanes_out) *lanes_out = best;
-    return batch_set_lanes(c, b, best);
+    res.lanes = best;
+    /* the groups: the same keys in every chain */
+    std::vector<tpb_rule> keys;
+    for (auto f : b->fs) {
+        const int mif = (f->desc.flags & PLOM_FLAG_MIF) != 0;
+        bool seen = false;
+        for (auto &k : keys)
+            seen = seen || (k.model == f->desc.model_id && k.prec == f->desc.precision && k.variant == f->variant && k.mif == mif);
+        if (!seen) keys.push_back(tpb_rule{f->desc.model_id, f->desc.precision, f->variant, mif, c->tpb});
+    }
+    static const int tpb_cand[3] = {1, 2, 4};
+    for (size_t g = 0; g < keys.size(); g++) {
+        tpb_rule &k = keys[g];
+        plom_tune_group *tg = g < PLOM_TUNE_MAX_GROUPS ? &res.group[g] : nullptr;
+        const bool multi = !k.mif && k.variant != 6;
+        int tbest = k.tpb;
+        double tbest_ms = -1.0;
+        for (int i = 0; i < 3; i++) {
+            if (!multi && tpb_cand[i] != 1) continue;
+            k.tpb = tpb_cand[i];
+            b->tpb_rules = keys;
+            double ms = 0.0;
+            int rc = batch_set_lanes(c, b, best);
+            if (rc == PLOM_OK) rc = batch_time_rounds(c, b, rounds, &ms);
+            if (rc != PLOM_OK) return rc;
+            if (tg) tg->ms[i] = ms;
+            if (tbest_ms < 0.0 || ms < tbest_ms) { tbest = k.tpb; tbest_ms = ms; }
+        }
+        k.tpb = tbest;
+        if (tg) {
+            tg->model_id = k.model; tg->precision = k.prec; tg->variant = k.variant; tg->mif = k.mif;
+            tg->tiles_per_block = tbest;
+        }
+    }
+    res.n_groups = (int32_t)(keys.size() < PLOM_TUNE_MAX_GROUPS ? keys.size() : PLOM_TUNE_MAX_GROUPS);
+    b->tpb_rules = keys;
+    int rc = batch_set_lanes(c, b, best);
+    if (rc != PLOM_OK) return rc;
+    if (out) *out = res;
+    return PLOM_OK;
+}
+
+extern "C" int plom_batch_autotune(plom_ctx *c, plom_batch *b, int rounds, int32_t *lanes_out, double *ms_out)
+{
+    plom_batch_tuning t;
+    int rc = plom_batch_tune(c, b, rounds, &t);
+    if (rc != PLOM_OK) return rc;
+    if (lanes_out) *lanes_out = t.lanes;
+    if (ms_out)
+        for (int i = 0; i < 4; i++) ms_out[i] = t.lanes_ms[i];
+    return PLOM_OK;
+}
+
+extern "C" int plom_ctx_set_tiles_per_block(plom_ctx *c, int tpb)
+{
+    if (!c) return set_err(PLOM_ERR_ARG, "null context");
+    if (tpb != 1 && tpb != 2 && tpb != 4) return set_err(PLOM_ERR_ARG, "tiles per block %d is not 1, 2 or 4", tpb);
+    c->tpb = tpb;
+    return PLOM_OK;
 }
 
 extern "C" int plom_batch_launch(plom_ctx *c, plom_batch *b, int use_graph)
@@ -2170,7 +2255,7 @@ static int srv_lane_step(plom_server *S, srv_lane *L)
     }
     (void)hipGetLastError();
     for (auto &g : L->groups)
-        model_at(g.model)->launch_prop(g.prec, g.variant, 0, (int)g.n, L->stream, S->d_slots, L->d_tiles + g.off);
+        model_at(g.model)->launch_prop(g.prec, g.variant, 0, (int)g.n, c->tpb, L->stream, S->d_slots, L->d_tiles + g.off);
     enqueue_resample(L->stream, S->d_slots, (const int2 *)L->d_tiles, (int)L->n_all, L->d_slots_active, (int)L->active.size());
     HIPCHK(hipGetLastError());
     for (size_t i = 0; i < L->active.size();) {

@@ -87,12 +87,23 @@ EXPORTS = [
     "plom_filter_run", "plom_filter_read", "plom_filter_destroy", "plom_filter_device_bytes",
     "plom_batch_run", "plom_sync", "plom_ctx_set_kernel_timing", "plom_last_timing", "plom_last_timing_lanes", "plom_selftest_math",
     "plom_selftest_normals", "plom_selftest_normals32", "plom_selftest_copy", "plom_batch_create", "plom_batch_launch",
-    "plom_batch_destroy", "plom_batch_configure", "plom_batch_autotune", "plom_server_create", "plom_server_submit", "plom_server_step",
+    "plom_batch_destroy", "plom_batch_configure", "plom_batch_autotune", "plom_batch_tune",
+    "plom_ctx_set_tiles_per_block", "plom_server_create", "plom_server_submit", "plom_server_step",
     "plom_server_status", "plom_server_done", "plom_server_result", "plom_server_destroy",
 ]
 
 SRV_QUEUED, SRV_RUNNING, SRV_DONE = 0, 1, 2
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_MODEL, ERR_STATE = -1, -2, -3, -4, -5
+
+
+class TuneGroup(C.Structure):
+    _fields_ = [("model_id", C.c_int32), ("precision", C.c_int32), ("variant", C.c_int32), ("mif", C.c_int32),
+                ("tiles_per_block", C.c_int32), ("pad", C.c_int32), ("ms", C.c_double * 3)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("rounds", C.c_int32), ("lanes", C.c_int32), ("lanes_ms", C.c_double * 4), ("n_groups", C.c_int32),
+                ("pad", C.c_int32), ("group", TuneGroup * 16)]
 
 
 class ServerConfig(C.Structure):
@@ -152,6 +163,8 @@ def lib():
     L.plom_batch_destroy.argtypes = [C.c_void_p, C.c_void_p]
     L.plom_batch_configure.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
     L.plom_batch_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    L.plom_batch_tune.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(BatchTuning)]
+    L.plom_ctx_set_tiles_per_block.argtypes = [C.c_void_p, C.c_int]
     L.plom_server_create.argtypes = [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]
     L.plom_server_submit.argtypes = [C.c_void_p, C.POINTER(FilterDesc), C.POINTER(FilterInputs), C.POINTER(C.c_uint64)]
     L.plom_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -196,6 +209,11 @@ class Context:
 
     def sync(self):
         check(lib().plom_sync(self.h))
+
+    def set_tiles_per_block(self, tpb):
+        """Tiles of 256 particles per propagate block (1, 2, 4) of single filters, the
+        server and untuned batch groups."""
+        check(lib().plom_ctx_set_tiles_per_block(self.h, int(tpb)))
 
     def set_kernel_timing(self, on):
         check(lib().plom_ctx_set_kernel_timing(self.h, 1 if on else 0))
@@ -582,12 +600,30 @@ class Batch:
         check(lib().plom_batch_configure(self.ctx.h, self.h, int(lanes)))
 
     def autotune(self, rounds=16):
-        """Pick the number of lanes (1, 2, 4, 8) by timing the first `rounds` intervals;
-        returns (chosen lanes, {lanes: ms})."""
+        """Pick the number of lanes (1, 2, 4, 8) and the tiles per block of every group by
+        timing `rounds` launch rounds spread over the pass; returns (chosen lanes,
+        {lanes: ms})."""
         lanes = C.c_int32()
         ms = (C.c_double * 4)()
         check(lib().plom_batch_autotune(self.ctx.h, self.h, int(rounds), C.byref(lanes), ms))
         return lanes.value, {k: ms[i] for i, k in enumerate((1, 2, 4, 8)) if ms[i] > 0}
+
+    def tune(self, rounds=16):
+        """plom_batch_tune: lanes, then tiles per block per (model, precision, variant, MIF)
+        group; returns {"lanes", "lanes_ms", "groups": [{model, precision, variant, mif,
+        tiles_per_block, ms: {tpb: ms}}]}."""
+        t = BatchTuning()
+        check(lib().plom_batch_tune(self.ctx.h, self.h, int(rounds), C.byref(t)))
+        groups = []
+        for g in t.group[:t.n_groups]:
+            name = C.create_string_buffer(64)
+            lib().plom_model_name(g.model_id, name, 64)
+            groups.append({"model": name.value.decode(), "precision": "fp32" if g.precision == PLOM_FP32 else "fp64",
+                           "variant": g.variant, "mif": bool(g.mif), "tiles_per_block": g.tiles_per_block,
+                           "ms": {k: round(g.ms[i], 3) for i, k in enumerate((1, 2, 4)) if g.ms[i] > 0}})
+        return {"rounds": t.rounds, "lanes": t.lanes,
+                "lanes_ms": {k: round(t.lanes_ms[i], 3) for i, k in enumerate((1, 2, 4, 8)) if t.lanes_ms[i] > 0},
+                "groups": groups}
 
     def close(self):
         if self.h:

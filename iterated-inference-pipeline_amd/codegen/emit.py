@@ -432,6 +432,22 @@ class Emitter:
                 idx = self.xidx(s, c)
                 K = len(exits) + 1
                 terms = ["%s * dt" % noisy(r) for _, r in exits]
+                if K == 2:
+                    # one exit: the multinomial is one binomial, the stay is the rest (its second
+                    # binomial has probability (1 - p0) / (1 - p0) = 1 and draws nothing)
+                    (ri, r), = exits
+                    L.append("        {")
+                    L.append("            const double sum = %s;" % terms[0])
+                    L.append("            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((%s) / sum) : 0.0;"
+                             % terms[0])
+                    L.append("            const double cnt = plom_floor(X[%d]);" % idx)
+                    # a wave whose counts or probabilities are all zero skips the call
+                    L.append("            n%d_%d = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;" % (ri, c))
+                    # (p0 is NaN only for an infinite rate: the multinomial then gives 0 for both;
+                    # a count that is not positive stays 0, as the second binomial gives)
+                    L.append("            stay%d = (p0 == p0 && cnt > 0.0) ? cnt - n%d_%d : 0.0;" % (idx, ri, c))
+                    L.append("        }")
+                    continue
                 L.append("        {")
                 L.append("            double p[%d], cnt[%d];" % (K, K))
                 L.append("            const double sum = %s;" % " + ".join(terms))
@@ -456,7 +472,10 @@ class Emitter:
                     continue
                 for ri, r in enumerate(m.proc_model):
                     if r["from"] == src:
-                        L.append("        n%d_%d = plom_poisson_draw(ns, %s * dt);" % (ri, c, noisy(r)))
+                        L.append("        {")
+                        L.append("            const double mu = %s * dt;" % noisy(r))
+                        L.append("            n%d_%d = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;" % (ri, c))
+                        L.append("        }")
         # declarations of the counts (inserted before the draws)
         decl = []
         for c in range(m.N_CAC):

@@ -5,7 +5,10 @@
  * draws the same numbers on both).
  *
  * Every draw consumes 32-bit words of the particle's xoshiro128** stream (plom_rng.h) in
- * a fixed order; the arithmetic uses this library's exp / log / sqrt (plom_math.h).
+ * a fixed order; the arithmetic uses this library's exp and sqrt (plom_math.h) and the
+ * table logarithm plom_log_tab (plom_rng.h: no division, no branch) on the stream's log
+ * table ns->ltab.  Each rejection test takes one logarithm of a product instead of a sum
+ * of logarithms.
  *
  *   plom_unif53        (a >> 6) 2^27 + (b >> 5) + 1/2, times 2^-53: a uniform in (0, 1)
  *                      from two words.
@@ -68,13 +71,13 @@ static __constant__ const double plom_lfact_tab[30] = PLOM_LFACT_TAB;
 static const double plom_lfact_tab[30] = PLOM_LFACT_TAB;
 #endif
 
-PLOM_HD double plom_lfact(double k)
+PLOM_HD double plom_lfact(double k, const double *ltab)
 {
     if (k < 30.0) return plom_lfact_tab[(int)k];
     const double x = k + 1.0;
     const double r = 1.0 / x, r2 = r * r;
     /* ln Gamma(x) = (x - 1/2) ln x - x + ln sqrt(2 pi) + 1/(12x) - 1/(360x^3) + 1/(1260x^5) */
-    return (x - 0.5) * plom_log(x) - x + PLOM_LN_SQRT_2PI + r * (1.0 / 12.0 - r2 * (1.0 / 360.0 - r2 * (1.0 / 1260.0)));
+    return (x - 0.5) * plom_log_tab(x, ltab) - x + PLOM_LN_SQRT_2PI + r * (1.0 / 12.0 - r2 * (1.0 / 360.0 - r2 * (1.0 / 1260.0)));
 }
 
 /* Gamma(a, 1), a > 0 */
@@ -83,7 +86,7 @@ PLOM_HD_CALL double plom_gamma_draw(plom_nstream64 *ns, double a)
     double boost = 1.0;
     if (a < 1.0) {
         /* Gamma(a) = Gamma(a + 1) U^(1/a) */
-        boost = plom_exp(plom_log(plom_unif53(ns)) / a);
+        boost = plom_exp(plom_log_tab(plom_unif53(ns), ns->ltab) / a);
         a = a + 1.0;
     }
     const double d = a - 1.0 / 3.0;
@@ -97,7 +100,8 @@ PLOM_HD_CALL double plom_gamma_draw(plom_nstream64 *ns, double a)
         const double u = plom_unif53(ns);
         const double z2 = z * z;
         if (u < 1.0 - 0.0331 * (z2 * z2)) return (d * v) * boost;
-        if (plom_log(u) < 0.5 * z2 + d * (1.0 - v + plom_log(v))) return (d * v) * boost;
+        /* v underflows to 0 only for v1 < 1e-102, where the test fails either way */
+        if (plom_log_tab(u, ns->ltab) < 0.5 * z2 + d * (1.0 - v + plom_log_tab(v, ns->ltab))) return (d * v) * boost;
     }
     return d * boost;
 }
@@ -118,7 +122,7 @@ PLOM_HD_CALL double plom_poisson_draw(plom_nstream64 *ns, double mu)
         return k;
     }
     const double slam = plom_sqrt(mu);
-    const double loglam = plom_log(mu);
+    const double loglam = plom_log_tab(mu, ns->ltab);
     const double b = 0.931 + 2.53 * slam;
     const double a = -0.059 + 0.02483 * b;
     const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
@@ -130,7 +134,10 @@ PLOM_HD_CALL double plom_poisson_draw(plom_nstream64 *ns, double mu)
         const double k = plom_floor((2.0 * a / us + b) * U + mu + 0.43);
         if (us >= 0.07 && V <= vr) return k;
         if (k < 0.0 || (us < 0.013 && V > us)) continue;
-        if (plom_log(V) + plom_log(invalpha) - plom_log(a / (us * us) + b) <= -mu + k * loglam - plom_lfact(k)) return k;
+        /* log(V invalpha / (a / us^2 + b)) */
+        const double us2 = us * us;
+        if (plom_log_tab(V * invalpha * us2 / plom_fma(b, us2, a), ns->ltab) <= -mu + k * loglam - plom_lfact(k, ns->ltab))
+            return k;
     }
     return plom_floor(mu);
 }
@@ -150,7 +157,7 @@ PLOM_HD double plom_binom_half(plom_nstream64 *ns, double n, double p)
             const double p2 = p * p;
             l1p = -(p + p2 * (0.5 + p * (1.0 / 3.0 + p * 0.25)));
         } else {
-            l1p = plom_log(q);
+            l1p = plom_log_tab(q, ns->ltab);
         }
         double r = plom_exp(n * l1p);
         double u = plom_unif53(ns);
@@ -169,9 +176,9 @@ PLOM_HD double plom_binom_half(plom_nstream64 *ns, double n, double p)
     const double c = np + 0.5;
     const double vr = 0.92 - 4.2 / b;
     const double alpha = (2.83 + 5.1 / b) * spq;
-    const double lpq = plom_log(p / q);
+    const double lpq = plom_log_tab(p / q, ns->ltab);
     const double m = plom_floor((n + 1.0) * p);
-    const double h = plom_lfact(m) + plom_lfact(n - m);
+    const double h = plom_lfact(m, ns->ltab) + plom_lfact(n - m, ns->ltab);
     for (int t = 0; t < PLOM_PSR_MAX_TRY; t++) {
         const double u = plom_unif53(ns) - 0.5;
         double v = plom_unif53(ns);
@@ -179,8 +186,10 @@ PLOM_HD double plom_binom_half(plom_nstream64 *ns, double n, double p)
         const double k = plom_floor((2.0 * a / us + b) * u + c);
         if (k < 0.0 || k > n) continue;
         if (us >= 0.07 && v <= vr) return k;
-        v = plom_log(v * alpha / (a / (us * us) + b));
-        if (v <= h - plom_lfact(k) - plom_lfact(n - k) + (k - m) * lpq) return k;
+        /* log(v alpha / (a / us^2 + b)) */
+        const double us2 = us * us;
+        v = plom_log_tab(v * alpha * us2 / plom_fma(b, us2, a), ns->ltab);
+        if (v <= h - plom_lfact(k, ns->ltab) - plom_lfact(n - k, ns->ltab) + (k - m) * lpq) return k;
     }
     return m;
 }

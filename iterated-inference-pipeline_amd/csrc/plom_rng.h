@@ -113,9 +113,10 @@ static __constant__ const double plom_icdf_tab[PLOM_ICDF_TAB_SIZE] = PLOM_ICDF_T
 static const double plom_icdf_tab[PLOM_ICDF_TAB_SIZE] = PLOM_ICDF_TAB_VALUES;
 #endif
 
-/* A particle's stream over one interval: its xoshiro128** state and the inverse-CDF
- * table (plom_icdf_tab or a copy of it). */
-typedef struct { plom_xoshiro128 r; const double *tab; } plom_nstream64;
+/* A particle's stream over one interval: its xoshiro128** state, the inverse-CDF table
+ * (plom_icdf_tab or a copy of it) and the log table of the psr samplers (the Box-Muller
+ * table plom_bm_tab or a copy of it; plom_nstream64_init sets plom_bm_tab). */
+typedef struct { plom_xoshiro128 r; const double *tab; const double *ltab; } plom_nstream64;
 
 PLOM_HD void plom_nstream64_init(plom_nstream64 *ns, uint32_t seed, uint32_t stream, uint32_t n, uint32_t j,
                                  const double *tab)
@@ -124,16 +125,18 @@ PLOM_HD void plom_nstream64_init(plom_nstream64 *ns, uint32_t seed, uint32_t str
     plom_draw128(seed, stream, n, j, PLOM_PURPOSE_PROPAGATE, w);
     plom_xoshiro_seed(&ns->r, w);
     ns->tab = tab;
+    ns->ltab = plom_bm_tab;
 }
 
-/* -2 log u for u = (x + 1/2) 2^-32: u = 2^e m, table entry k from the top fraction bits
- * of m, log u = e ln2 + L_k + log1p(m invc_k - 1) with the degree-7 Taylor polynomial of
- * log1p on |z| < 2^-8 (truncation < 7e-21).  Entries k >= 64 hold L_k - ln2 and count
- * one more power of two, so that log u near u = 1 carries no cancellation. */
-PLOM_HD double plom_bm_m2log(uint32_t x, const double *tab)
+/* log x for a positive, normal, finite x by the log table: x = 2^e m, table entry k from
+ * the top PLOM_BM_LOG_BITS fraction bits of m, log x = e ln2 + L_k + log1p(m invc_k - 1)
+ * with the degree-7 Taylor polynomial of log1p on |z| < 2^-8 (truncation < 7e-21).
+ * Entries k >= 64 hold L_k - ln2 and count one more power of two, so that log x near
+ * x = 1 carries no cancellation.  No division and no branch.  The Box-Muller radius and
+ * the psr samplers' logarithm. */
+PLOM_HD double plom_log_tab(double x, const double *tab)
 {
-    const double u = plom_fma((double)x, 0x1p-32, 0x1p-33); /* exact */
-    const uint64_t b = plom_d2u(u);
+    const uint64_t b = plom_d2u(x);
     const uint32_t hi = (uint32_t)(b >> 32);
     const uint32_t k = (hi >> (20 - PLOM_BM_LOG_BITS)) & ((1u << PLOM_BM_LOG_BITS) - 1u);
     const int e = (int)(hi >> 20) - 1023 + (int)(k >> (PLOM_BM_LOG_BITS - 1));
@@ -146,7 +149,13 @@ PLOM_HD double plom_bm_m2log(uint32_t x, const double *tab)
     q = plom_fma(q, z, -0x1.0000000000000p-1);
     const double p = plom_fma(z * z, q, z);
     const double de = (double)e;
-    return -2.0 * (plom_fma(de, PLOM_LN2_HI, tab[2 * k + 1]) + plom_fma(de, PLOM_LN2_LO, p));
+    return plom_fma(de, PLOM_LN2_HI, tab[2 * k + 1]) + plom_fma(de, PLOM_LN2_LO, p);
+}
+
+/* -2 log u for u = (x + 1/2) 2^-32 */
+PLOM_HD double plom_bm_m2log(uint32_t x, const double *tab)
+{
+    return -2.0 * plom_log_tab(plom_fma((double)x, 0x1p-32, 0x1p-33) /* exact */, tab);
 }
 
 /* sin and cos of 2 pi u for u = (y + 1/2) 2^-32: the angle is a_k + d with a_k =

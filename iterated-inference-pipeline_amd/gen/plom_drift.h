@@ -566,7 +566,10 @@ PLOM_HD void plom_drift_interval_psr_full_f64(double *X, const double *pv, plom_
             n4_0 = cnt[1];
             stay2 = cnt[2];
         }
-        n0_0 = plom_poisson_draw(ns, r0_0 * dt);
+        {
+            const double mu = r0_0 * dt;
+            n0_0 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         {
             double p[3], cnt[3];
             const double sum = r1_1 * dt + r3_1 * dt;
@@ -603,7 +606,10 @@ PLOM_HD void plom_drift_interval_psr_full_f64(double *X, const double *pv, plom_
             n4_1 = cnt[1];
             stay3 = cnt[2];
         }
-        n0_1 = plom_poisson_draw(ns, r0_1 * dt);
+        {
+            const double mu = r0_1 * dt;
+            n0_1 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         X[0] = stay0 + n0_0;
         X[2] = stay2 + n1_0;
         X[1] = stay1 + n0_1;
@@ -722,7 +728,10 @@ PLOM_HD void plom_drift_interval_psr_no_env_sto_f64(double *X, const double *pv,
             n4_0 = cnt[1];
             stay2 = cnt[2];
         }
-        n0_0 = plom_poisson_draw(ns, r0_0 * dt);
+        {
+            const double mu = r0_0 * dt;
+            n0_0 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         {
             double p[3], cnt[3];
             const double sum = r1_1 * dt + r3_1 * dt;
@@ -759,7 +768,10 @@ PLOM_HD void plom_drift_interval_psr_no_env_sto_f64(double *X, const double *pv,
             n4_1 = cnt[1];
             stay3 = cnt[2];
         }
-        n0_1 = plom_poisson_draw(ns, r0_1 * dt);
+        {
+            const double mu = r0_1 * dt;
+            n0_1 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         X[0] = stay0 + n0_0;
         X[2] = stay2 + n1_0;
         X[1] = stay1 + n0_1;
@@ -1325,7 +1337,10 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
             n4_0 = cnt[1];
             stay2 = cnt[2];
         }
-        n0_0 = plom_poisson_draw(ns, r0_0 * dt);
+        {
+            const double mu = r0_0 * dt;
+            n0_0 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         {
             double p[3], cnt[3];
             const double sum = r1_1 * dt + r3_1 * dt;
@@ -1362,7 +1377,10 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
             n4_1 = cnt[1];
             stay3 = cnt[2];
         }
-        n0_1 = plom_poisson_draw(ns, r0_1 * dt);
+        {
+            const double mu = r0_1 * dt;
+            n0_1 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         X[0] = stay0 + n0_0;
         X[2] = stay2 + n1_0;
         X[1] = stay1 + n0_1;
@@ -1495,7 +1513,10 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
             n4_0 = cnt[1];
             stay2 = cnt[2];
         }
-        n0_0 = plom_poisson_draw(ns, r0_0 * dt);
+        {
+            const double mu = r0_0 * dt;
+            n0_0 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         {
             double p[3], cnt[3];
             const double sum = r1_1 * dt + r3_1 * dt;
@@ -1532,7 +1553,10 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
             n4_1 = cnt[1];
             stay3 = cnt[2];
         }
-        n0_1 = plom_poisson_draw(ns, r0_1 * dt);
+        {
+            const double mu = r0_1 * dt;
+            n0_1 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         X[0] = stay0 + n0_0;
         X[2] = stay2 + n1_0;
         X[1] = stay1 + n0_1;

@@ -457,7 +457,10 @@ PLOM_HD void plom_noise_interval_psr_full_f64(double *X, const double *pv, plom_
             n4_0 = cnt[1];
             stay2 = cnt[2];
         }
-        n0_0 = plom_poisson_draw(ns, r0_0 * dt);
+        {
+            const double mu = r0_0 * dt;
+            n0_0 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double sd0_1 = P5;
         const double v0_1 = sd0_1 * sd0_1;
         const double xi0_1 = (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt;
@@ -497,7 +500,10 @@ PLOM_HD void plom_noise_interval_psr_full_f64(double *X, const double *pv, plom_
             n4_1 = cnt[1];
             stay3 = cnt[2];
         }
-        n0_1 = plom_poisson_draw(ns, r0_1 * dt);
+        {
+            const double mu = r0_1 * dt;
+            n0_1 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         X[0] = stay0 + n0_0;
         X[2] = stay2 + n1_0;
         X[1] = stay1 + n0_1;
@@ -588,7 +594,10 @@ PLOM_HD void plom_noise_interval_psr_no_env_sto_f64(double *X, const double *pv,
             n4_0 = cnt[1];
             stay2 = cnt[2];
         }
-        n0_0 = plom_poisson_draw(ns, r0_0 * dt);
+        {
+            const double mu = r0_0 * dt;
+            n0_0 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double xi0_1 = 1.0;
         {
             double p[3], cnt[3];
@@ -626,7 +635,10 @@ PLOM_HD void plom_noise_interval_psr_no_env_sto_f64(double *X, const double *pv,
             n4_1 = cnt[1];
             stay3 = cnt[2];
         }
-        n0_1 = plom_poisson_draw(ns, r0_1 * dt);
+        {
+            const double mu = r0_1 * dt;
+            n0_1 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         X[0] = stay0 + n0_0;
         X[2] = stay2 + n1_0;
         X[1] = stay1 + n0_1;
@@ -1062,7 +1074,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
             n4_0 = cnt[1];
             stay2 = cnt[2];
         }
-        n0_0 = plom_poisson_draw(ns, r0_0 * dt);
+        {
+            const double mu = r0_0 * dt;
+            n0_0 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double sd0_1 = P5;
         const double v0_1 = sd0_1 * sd0_1;
         const double xi0_1 = (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt;
@@ -1102,7 +1117,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
             n4_1 = cnt[1];
             stay3 = cnt[2];
         }
-        n0_1 = plom_poisson_draw(ns, r0_1 * dt);
+        {
+            const double mu = r0_1 * dt;
+            n0_1 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         X[0] = stay0 + n0_0;
         X[2] = stay2 + n1_0;
         X[1] = stay1 + n0_1;
@@ -1193,7 +1211,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
             n4_0 = cnt[1];
             stay2 = cnt[2];
         }
-        n0_0 = plom_poisson_draw(ns, r0_0 * dt);
+        {
+            const double mu = r0_0 * dt;
+            n0_0 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double xi0_1 = 1.0;
         {
             double p[3], cnt[3];
@@ -1231,7 +1252,10 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
             n4_1 = cnt[1];
             stay3 = cnt[2];
         }
-        n0_1 = plom_poisson_draw(ns, r0_1 * dt);
+        {
+            const double mu = r0_1 * dt;
+            n0_1 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         X[0] = stay0 + n0_0;
         X[2] = stay2 + n1_0;
         X[1] = stay1 + n0_1;

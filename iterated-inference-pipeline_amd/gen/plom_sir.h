@@ -205,34 +205,18 @@ PLOM_HD void plom_sir_interval_psr_full_f64(double *X, const double *pv, plom_ns
         const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         {
-            double p[2], cnt[2];
             const double sum = r0_0 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r0_0 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
-            n0_0 = cnt[0];
-            stay0 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r0_0 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[0]);
+            n0_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay0 = (p0 == p0 && cnt > 0.0) ? cnt - n0_0 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_0 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
-            n1_0 = cnt[0];
-            stay1 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_0 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[1]);
+            n1_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay1 = (p0 == p0 && cnt > 0.0) ? cnt - n1_0 : 0.0;
         }
         X[0] = stay0;
         X[1] = stay1 + n0_0;
@@ -261,34 +245,18 @@ PLOM_HD void plom_sir_interval_psr_no_env_sto_f64(double *X, const double *pv, p
         const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         {
-            double p[2], cnt[2];
             const double sum = r0_0 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r0_0 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
-            n0_0 = cnt[0];
-            stay0 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r0_0 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[0]);
+            n0_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay0 = (p0 == p0 && cnt > 0.0) ? cnt - n0_0 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_0 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
-            n1_0 = cnt[0];
-            stay1 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_0 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[1]);
+            n1_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay1 = (p0 == p0 && cnt > 0.0) ? cnt - n1_0 : 0.0;
         }
         X[0] = stay0;
         X[1] = stay1 + n0_0;
@@ -493,34 +461,18 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_p
         const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         {
-            double p[2], cnt[2];
             const double sum = r0_0 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r0_0 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
-            n0_0 = cnt[0];
-            stay0 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r0_0 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[0]);
+            n0_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay0 = (p0 == p0 && cnt > 0.0) ? cnt - n0_0 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_0 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
-            n1_0 = cnt[0];
-            stay1 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_0 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[1]);
+            n1_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay1 = (p0 == p0 && cnt > 0.0) ? cnt - n1_0 : 0.0;
         }
         X[0] = stay0;
         X[1] = stay1 + n0_0;
@@ -549,34 +501,18 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_p
         const double r0_0 = ((P0 * X[1]) / C0);
         const double r1_0 = P1;
         {
-            double p[2], cnt[2];
             const double sum = r0_0 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r0_0 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
-            n0_0 = cnt[0];
-            stay0 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r0_0 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[0]);
+            n0_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay0 = (p0 == p0 && cnt > 0.0) ? cnt - n0_0 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_0 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
-            n1_0 = cnt[0];
-            stay1 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_0 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[1]);
+            n1_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay1 = (p0 == p0 && cnt > 0.0) ? cnt - n1_0 : 0.0;
         }
         X[0] = stay0;
         X[1] = stay1 + n0_0;

@@ -678,138 +678,86 @@ PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_
         const double v0_0 = sd0_0 * sd0_0;
         const double xi0_0 = (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt;
         {
-            double p[2], cnt[2];
             const double sum = (r0_0 * xi0_0) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_0 * xi0_0) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
-            n0_0 = cnt[0];
-            stay0 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_0 * xi0_0) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[0]);
+            n0_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay0 = (p0 == p0 && cnt > 0.0) ? cnt - n0_0 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_0 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[4]), p, cnt);
-            n1_0 = cnt[0];
-            stay4 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_0 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[4]);
+            n1_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay4 = (p0 == p0 && cnt > 0.0) ? cnt - n1_0 : 0.0;
         }
-        n2_0 = plom_poisson_draw(ns, r2_0 * dt);
+        {
+            const double mu = r2_0 * dt;
+            n2_0 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double sd0_1 = P17;
         const double v0_1 = sd0_1 * sd0_1;
         const double xi0_1 = (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt;
         {
-            double p[2], cnt[2];
             const double sum = (r0_1 * xi0_1) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_1 * xi0_1) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
-            n0_1 = cnt[0];
-            stay1 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_1 * xi0_1) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[1]);
+            n0_1 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay1 = (p0 == p0 && cnt > 0.0) ? cnt - n0_1 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_1 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_1 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[5]), p, cnt);
-            n1_1 = cnt[0];
-            stay5 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_1 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[5]);
+            n1_1 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay5 = (p0 == p0 && cnt > 0.0) ? cnt - n1_1 : 0.0;
         }
-        n2_1 = plom_poisson_draw(ns, r2_1 * dt);
+        {
+            const double mu = r2_1 * dt;
+            n2_1 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double sd0_2 = P18;
         const double v0_2 = sd0_2 * sd0_2;
         const double xi0_2 = (v0_2 * plom_gamma_draw(ns, dt / v0_2)) / dt;
         {
-            double p[2], cnt[2];
             const double sum = (r0_2 * xi0_2) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_2 * xi0_2) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[2]), p, cnt);
-            n0_2 = cnt[0];
-            stay2 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_2 * xi0_2) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[2]);
+            n0_2 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay2 = (p0 == p0 && cnt > 0.0) ? cnt - n0_2 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_2 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_2 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[6]), p, cnt);
-            n1_2 = cnt[0];
-            stay6 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_2 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[6]);
+            n1_2 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay6 = (p0 == p0 && cnt > 0.0) ? cnt - n1_2 : 0.0;
         }
-        n2_2 = plom_poisson_draw(ns, r2_2 * dt);
+        {
+            const double mu = r2_2 * dt;
+            n2_2 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double sd0_3 = P19;
         const double v0_3 = sd0_3 * sd0_3;
         const double xi0_3 = (v0_3 * plom_gamma_draw(ns, dt / v0_3)) / dt;
         {
-            double p[2], cnt[2];
             const double sum = (r0_3 * xi0_3) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_3 * xi0_3) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[3]), p, cnt);
-            n0_3 = cnt[0];
-            stay3 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_3 * xi0_3) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[3]);
+            n0_3 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay3 = (p0 == p0 && cnt > 0.0) ? cnt - n0_3 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_3 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_3 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[7]), p, cnt);
-            n1_3 = cnt[0];
-            stay7 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_3 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[7]);
+            n1_3 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay7 = (p0 == p0 && cnt > 0.0) ? cnt - n1_3 : 0.0;
         }
-        n2_3 = plom_poisson_draw(ns, r2_3 * dt);
+        {
+            const double mu = r2_3 * dt;
+            n2_3 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         X[0] = stay0 + n2_0;
         X[4] = stay4 + n0_0;
         X[1] = stay1 + n2_1;
@@ -910,132 +858,80 @@ PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv,
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double xi0_0 = 1.0;
         {
-            double p[2], cnt[2];
             const double sum = (r0_0 * xi0_0) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_0 * xi0_0) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
-            n0_0 = cnt[0];
-            stay0 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_0 * xi0_0) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[0]);
+            n0_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay0 = (p0 == p0 && cnt > 0.0) ? cnt - n0_0 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_0 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[4]), p, cnt);
-            n1_0 = cnt[0];
-            stay4 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_0 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[4]);
+            n1_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay4 = (p0 == p0 && cnt > 0.0) ? cnt - n1_0 : 0.0;
         }
-        n2_0 = plom_poisson_draw(ns, r2_0 * dt);
+        {
+            const double mu = r2_0 * dt;
+            n2_0 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double xi0_1 = 1.0;
         {
-            double p[2], cnt[2];
             const double sum = (r0_1 * xi0_1) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_1 * xi0_1) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
-            n0_1 = cnt[0];
-            stay1 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_1 * xi0_1) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[1]);
+            n0_1 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay1 = (p0 == p0 && cnt > 0.0) ? cnt - n0_1 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_1 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_1 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[5]), p, cnt);
-            n1_1 = cnt[0];
-            stay5 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_1 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[5]);
+            n1_1 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay5 = (p0 == p0 && cnt > 0.0) ? cnt - n1_1 : 0.0;
         }
-        n2_1 = plom_poisson_draw(ns, r2_1 * dt);
+        {
+            const double mu = r2_1 * dt;
+            n2_1 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double xi0_2 = 1.0;
         {
-            double p[2], cnt[2];
             const double sum = (r0_2 * xi0_2) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_2 * xi0_2) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[2]), p, cnt);
-            n0_2 = cnt[0];
-            stay2 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_2 * xi0_2) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[2]);
+            n0_2 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay2 = (p0 == p0 && cnt > 0.0) ? cnt - n0_2 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_2 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_2 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[6]), p, cnt);
-            n1_2 = cnt[0];
-            stay6 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_2 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[6]);
+            n1_2 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay6 = (p0 == p0 && cnt > 0.0) ? cnt - n1_2 : 0.0;
         }
-        n2_2 = plom_poisson_draw(ns, r2_2 * dt);
+        {
+            const double mu = r2_2 * dt;
+            n2_2 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double xi0_3 = 1.0;
         {
-            double p[2], cnt[2];
             const double sum = (r0_3 * xi0_3) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_3 * xi0_3) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[3]), p, cnt);
-            n0_3 = cnt[0];
-            stay3 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_3 * xi0_3) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[3]);
+            n0_3 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay3 = (p0 == p0 && cnt > 0.0) ? cnt - n0_3 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_3 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_3 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[7]), p, cnt);
-            n1_3 = cnt[0];
-            stay7 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_3 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[7]);
+            n1_3 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay7 = (p0 == p0 && cnt > 0.0) ? cnt - n1_3 : 0.0;
         }
-        n2_3 = plom_poisson_draw(ns, r2_3 * dt);
+        {
+            const double mu = r2_3 * dt;
+            n2_3 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         X[0] = stay0 + n2_0;
         X[4] = stay4 + n0_0;
         X[1] = stay1 + n2_1;
@@ -1738,138 +1634,86 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double v0_0 = sd0_0 * sd0_0;
         const double xi0_0 = (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt;
         {
-            double p[2], cnt[2];
             const double sum = (r0_0 * xi0_0) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_0 * xi0_0) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
-            n0_0 = cnt[0];
-            stay0 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_0 * xi0_0) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[0]);
+            n0_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay0 = (p0 == p0 && cnt > 0.0) ? cnt - n0_0 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_0 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[4]), p, cnt);
-            n1_0 = cnt[0];
-            stay4 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_0 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[4]);
+            n1_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay4 = (p0 == p0 && cnt > 0.0) ? cnt - n1_0 : 0.0;
         }
-        n2_0 = plom_poisson_draw(ns, r2_0 * dt);
+        {
+            const double mu = r2_0 * dt;
+            n2_0 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double sd0_1 = P17;
         const double v0_1 = sd0_1 * sd0_1;
         const double xi0_1 = (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt;
         {
-            double p[2], cnt[2];
             const double sum = (r0_1 * xi0_1) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_1 * xi0_1) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
-            n0_1 = cnt[0];
-            stay1 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_1 * xi0_1) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[1]);
+            n0_1 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay1 = (p0 == p0 && cnt > 0.0) ? cnt - n0_1 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_1 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_1 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[5]), p, cnt);
-            n1_1 = cnt[0];
-            stay5 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_1 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[5]);
+            n1_1 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay5 = (p0 == p0 && cnt > 0.0) ? cnt - n1_1 : 0.0;
         }
-        n2_1 = plom_poisson_draw(ns, r2_1 * dt);
+        {
+            const double mu = r2_1 * dt;
+            n2_1 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double sd0_2 = P18;
         const double v0_2 = sd0_2 * sd0_2;
         const double xi0_2 = (v0_2 * plom_gamma_draw(ns, dt / v0_2)) / dt;
         {
-            double p[2], cnt[2];
             const double sum = (r0_2 * xi0_2) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_2 * xi0_2) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[2]), p, cnt);
-            n0_2 = cnt[0];
-            stay2 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_2 * xi0_2) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[2]);
+            n0_2 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay2 = (p0 == p0 && cnt > 0.0) ? cnt - n0_2 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_2 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_2 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[6]), p, cnt);
-            n1_2 = cnt[0];
-            stay6 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_2 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[6]);
+            n1_2 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay6 = (p0 == p0 && cnt > 0.0) ? cnt - n1_2 : 0.0;
         }
-        n2_2 = plom_poisson_draw(ns, r2_2 * dt);
+        {
+            const double mu = r2_2 * dt;
+            n2_2 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double sd0_3 = P19;
         const double v0_3 = sd0_3 * sd0_3;
         const double xi0_3 = (v0_3 * plom_gamma_draw(ns, dt / v0_3)) / dt;
         {
-            double p[2], cnt[2];
             const double sum = (r0_3 * xi0_3) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_3 * xi0_3) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[3]), p, cnt);
-            n0_3 = cnt[0];
-            stay3 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_3 * xi0_3) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[3]);
+            n0_3 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay3 = (p0 == p0 && cnt > 0.0) ? cnt - n0_3 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_3 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_3 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[7]), p, cnt);
-            n1_3 = cnt[0];
-            stay7 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_3 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[7]);
+            n1_3 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay7 = (p0 == p0 && cnt > 0.0) ? cnt - n1_3 : 0.0;
         }
-        n2_3 = plom_poisson_draw(ns, r2_3 * dt);
+        {
+            const double mu = r2_3 * dt;
+            n2_3 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         X[0] = stay0 + n2_0;
         X[4] = stay4 + n0_0;
         X[1] = stay1 + n2_1;
@@ -1970,132 +1814,80 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double xi0_0 = 1.0;
         {
-            double p[2], cnt[2];
             const double sum = (r0_0 * xi0_0) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_0 * xi0_0) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[0]), p, cnt);
-            n0_0 = cnt[0];
-            stay0 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_0 * xi0_0) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[0]);
+            n0_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay0 = (p0 == p0 && cnt > 0.0) ? cnt - n0_0 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_0 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_0 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[4]), p, cnt);
-            n1_0 = cnt[0];
-            stay4 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_0 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[4]);
+            n1_0 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay4 = (p0 == p0 && cnt > 0.0) ? cnt - n1_0 : 0.0;
         }
-        n2_0 = plom_poisson_draw(ns, r2_0 * dt);
+        {
+            const double mu = r2_0 * dt;
+            n2_0 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double xi0_1 = 1.0;
         {
-            double p[2], cnt[2];
             const double sum = (r0_1 * xi0_1) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_1 * xi0_1) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[1]), p, cnt);
-            n0_1 = cnt[0];
-            stay1 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_1 * xi0_1) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[1]);
+            n0_1 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay1 = (p0 == p0 && cnt > 0.0) ? cnt - n0_1 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_1 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_1 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[5]), p, cnt);
-            n1_1 = cnt[0];
-            stay5 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_1 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[5]);
+            n1_1 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay5 = (p0 == p0 && cnt > 0.0) ? cnt - n1_1 : 0.0;
         }
-        n2_1 = plom_poisson_draw(ns, r2_1 * dt);
+        {
+            const double mu = r2_1 * dt;
+            n2_1 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double xi0_2 = 1.0;
         {
-            double p[2], cnt[2];
             const double sum = (r0_2 * xi0_2) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_2 * xi0_2) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[2]), p, cnt);
-            n0_2 = cnt[0];
-            stay2 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_2 * xi0_2) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[2]);
+            n0_2 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay2 = (p0 == p0 && cnt > 0.0) ? cnt - n0_2 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_2 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_2 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[6]), p, cnt);
-            n1_2 = cnt[0];
-            stay6 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_2 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[6]);
+            n1_2 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay6 = (p0 == p0 && cnt > 0.0) ? cnt - n1_2 : 0.0;
         }
-        n2_2 = plom_poisson_draw(ns, r2_2 * dt);
+        {
+            const double mu = r2_2 * dt;
+            n2_2 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         const double xi0_3 = 1.0;
         {
-            double p[2], cnt[2];
             const double sum = (r0_3 * xi0_3) * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * (((r0_3 * xi0_3) * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[3]), p, cnt);
-            n0_3 = cnt[0];
-            stay3 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_3 * xi0_3) * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[3]);
+            n0_3 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay3 = (p0 == p0 && cnt > 0.0) ? cnt - n0_3 : 0.0;
         }
         {
-            double p[2], cnt[2];
             const double sum = r1_3 * dt;
-            if (sum > 0.0) {
-                const double one_minus_exp_sum = 1.0 - plom_exp(-sum);
-                p[0] = one_minus_exp_sum * ((r1_3 * dt) / sum);
-                p[1] = 1.0 - p[0];
-            } else {
-                p[0] = 0.0;
-                p[1] = 1.0;
-            }
-            plom_multinomial(ns, 2, plom_floor(X[7]), p, cnt);
-            n1_3 = cnt[0];
-            stay7 = cnt[1];
+            const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * ((r1_3 * dt) / sum) : 0.0;
+            const double cnt = plom_floor(X[7]);
+            n1_3 = (cnt > 0.0 && p0 > 0.0) ? plom_binom_draw(ns, cnt, p0) : 0.0;
+            stay7 = (p0 == p0 && cnt > 0.0) ? cnt - n1_3 : 0.0;
         }
-        n2_3 = plom_poisson_draw(ns, r2_3 * dt);
+        {
+            const double mu = r2_3 * dt;
+            n2_3 = (mu > 0.0) ? plom_poisson_draw(ns, mu) : 0.0;
+        }
         X[0] = stay0 + n2_0;
         X[4] = stay4 + n0_0;
         X[1] = stay1 + n2_1;

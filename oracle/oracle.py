@@ -76,6 +76,7 @@ def _setup(L):
     L.oracle_philox.argtypes = [C.POINTER(C.c_uint32)] * 3
     L.oracle_bm_pairs.argtypes = [C.c_int, C.POINTER(C.c_uint32), dp]
     L.oracle_icdf.argtypes = [C.c_int, C.POINTER(C.c_uint32), dp]
+    L.oracle_log_tab.argtypes = [C.c_int, dp, dp]
     L.oracle_xoshiro.argtypes = [C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_uint32)]
     L.oracle_model_name.restype = C.c_char_p
     L.oracle_psr_draws.argtypes = [C.c_int, C.c_double, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -374,6 +375,14 @@ def icdf(words):
     w = np.ascontiguousarray(words, dtype=np.uint32).ravel()
     out = np.zeros(w.size)
     lib().oracle_icdf(w.size, w.ctypes.data_as(C.POINTER(C.c_uint32)), _dp(out))
+    return out
+
+
+def log_tab(x):
+    """The psr samplers' table logarithm of positive normal doubles (plom_log_tab)."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    out = np.zeros(x.size)
+    lib().oracle_log_tab(x.size, _dp(x), _dp(out))
     return out
 
 

@@ -124,7 +124,7 @@ void oracle_psr_draws(int kind, double a, double b, uint32_t seed, uint32_t stre
         case 1: out[i] = plom_gamma_draw(&ns, a); break;
         case 2: out[i] = plom_poisson_draw(&ns, a); break;
         case 3: out[i] = plom_binom_draw(&ns, a, b); break;
-        default: out[i] = plom_lfact(a + (double)i); break;
+        default: out[i] = plom_lfact(a + (double)i, plom_bm_tab); break;
         }
     }
 }
@@ -1084,6 +1084,12 @@ void oracle_bm_pairs(int n, const uint32_t *xy, double *out)
 void oracle_icdf(int n, const uint32_t *w, double *out)
 {
     for (int i = 0; i < n; i++) out[i] = plom_icdf_normal(w[i], plom_icdf_tab);
+}
+
+/* the psr samplers' table logarithm of x[0..n) (plom_log_tab) */
+void oracle_log_tab(int n, const double *x, double *out)
+{
+    for (int i = 0; i < n; i++) out[i] = plom_log_tab(x[i], plom_bm_tab);
 }
 
 void oracle_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) { plom_philox4x32_10(ctr, key, out); }

@@ -218,3 +218,24 @@ def test_icdf_distribution():
     assert abs(z.var() - 1) < 5 * np.sqrt(2 / m)
     assert abs(stats.skew(z)) < 5 * np.sqrt(6 / m)
     assert abs(stats.kurtosis(z)) < 5 * np.sqrt(24 / m)
+
+
+def test_log_tab_against_mpmath():
+    """The psr samplers' table logarithm (plom_log_tab) over the normal doubles: random
+    mantissas at exponents -1020..1020, every table boundary (m = 1 + k/128, the k = 64
+    switch) and the neighbourhood of 1, where the error must stay absolute (no
+    cancellation).  Bound: 2 ulps of log x, or 2.5e-16 absolute below |log x| = 1."""
+    rng = np.random.default_rng(5)
+    xs = list(np.ldexp(1.0 + rng.random(3000), rng.integers(-1020, 1021, 3000)))
+    for e in (-1000, -60, -1, 0, 1, 2, 700):
+        for k in (0, 1, 63, 64, 65, 127):
+            v = math.ldexp(1.0 + k / 128.0, e)
+            xs += [np.nextafter(v, 0.0), v, np.nextafter(v, np.inf)]
+    xs += list(1.0 + rng.uniform(-2 ** -7, 2 ** -7, 1000)) + [1.0 - 2 ** -53, 1.0 + 2 ** -52]
+    x = np.array(xs, dtype=np.float64)
+    got = oracle.log_tab(x)
+    for xi, g in zip(x, got):
+        ref = mp.log(mp.mpf(float(xi)))
+        err = abs(mp.mpf(float(g)) - ref)
+        tol = 2.5e-16 if abs(ref) < 1 else 2.0 * math.ulp(float(ref))
+        assert err <= tol, (float(xi), float(g), float(ref))

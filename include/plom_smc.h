@@ -273,6 +273,21 @@ int plom_covariate_eval(const plom_covariate *cov, int nt, const double *t, doub
 int plom_smc_run(plom_ctx *ctx, const plom_filter_desc *desc, const plom_filter_inputs *in,
                  plom_filter_outputs *out);
 
+/* plom_smc_run without device allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The filter is laid out in a fixed device scratch of
+ * PLOM_NOALLOC_MAX_BYTES that the library holds on every GPU it runs on until the process
+ * ends; it makes no hipMalloc / hipFree and no other device or page-locked allocation (the
+ * host-side staging of the inputs uses the C heap).  A filter whose device layout
+ * (plom_filter_device_bytes) exceeds the scratch gets PLOM_ERR_ARG, and a context on a
+ * device past the 64th has no scratch (PLOM_ERR_ARG).  Results are bit-identical to
+ * plom_smc_run.  Every call on one GPU uses that GPU's one scratch, which keeps the last
+ * call's contents: calls that overlap in time on one GPU (two threads, two contexts) race,
+ * so the caller serialises them. */
+#define PLOM_NOALLOC_MAX_BYTES ((size_t)64 << 20)
+int plom_smc_run_noalloc(plom_ctx *ctx, const plom_filter_desc *desc, const plom_filter_inputs *in,
+                         plom_filter_outputs *out);
+
 /* resident filter: upload once, run (asynchronously on the context stream), read */
 int plom_filter_create(plom_ctx *ctx, const plom_filter_desc *desc, const plom_filter_inputs *in,
                        plom_filter **out);

@@ -598,6 +598,7 @@ struct plom_ctx {
     hipEvent_t fork_ev;
     int tpb;                        /* tiles per propagate block of single filters, the server and
                                        batch groups without a tuned value */
+    char *scratch;                  /* this device's fixed scratch (plom_smc_run_noalloc), or null */
 };
 
 /* event k of the per-launch timing pool (grown on demand), recorded on stream s */
@@ -633,7 +634,8 @@ struct plom_filter {
     plom_filter_desc desc;
     plom_model_info info;
     int variant;
-    char *dev;              /* one allocation holding everything */
+    char *dev;              /* one allocation holding everything (or the fixed scratch) */
+    int fixed_dev;          /* dev is the device's fixed scratch: nothing to free */
     size_t dev_bytes;
     int async_alloc;        /* allocated with hipMallocAsync (server) */
     plom_dfilter *df;       /* device copy of the descriptor */
@@ -805,10 +807,44 @@ extern "C" int64_t plom_filter_device_bytes(const plom_filter_desc *d)
     return (int64_t)L.total;
 }
 
+/* ---------------------------------------------------------------- fixed device scratch */
+/* The device memory of plom_smc_run_noalloc: a file-scope array that every GPU the library
+ * runs on holds from the library's load until the process ends.  In host code the name
+ * `scratch` is only a host-side stand-in for the symbol, not a device address: the address
+ * on each GPU comes from hipGetSymbolAddress, taken once per device with that device current
+ * (in plom_ctx_create) and kept in g_scratch_dev; host code uses only that pointer.  Calls
+ * share their GPU's array and find the previous call's contents in it. */
+static __device__ __align__(256) char scratch[PLOM_NOALLOC_MAX_BYTES];
+#define PLOM_SCRATCH_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static char *g_scratch_dev[PLOM_SCRATCH_MAX_DEVICES];
+
+/* the scratch's address on `device` (the current device), looked up on first use */
+static int scratch_bind(int device, char **out)
+{
+    if (device < 0 || device >= PLOM_SCRATCH_MAX_DEVICES)
+        return set_err(PLOM_ERR_ARG, "device %d: the fixed scratch serves devices 0..%d", device, PLOM_SCRATCH_MAX_DEVICES - 1);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_err(PLOM_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+        g_scratch_dev[device] = (char *)ptr;
+    }
+    *out = g_scratch_dev[device];
+    return PLOM_OK;
+}
+
 extern "C" int plom_ctx_create(int device, plom_ctx **out)
 {
     if (!out) return set_err(PLOM_ERR_ARG, "null output");
     HIPCHK(hipSetDevice(device));
+    char *scratch_dev = nullptr;
+    if (device < PLOM_SCRATCH_MAX_DEVICES) {
+        const int rc = scratch_bind(device, &scratch_dev);
+        if (rc != PLOM_OK) return rc;
+    }
     void *mem = calloc(1, sizeof(plom_ctx));
     if (!mem) return set_err(PLOM_ERR_NOMEM, "host allocation failed");
     plom_ctx *c = new (mem) plom_ctx();
@@ -821,6 +857,7 @@ extern "C" int plom_ctx_create(int device, plom_ctx **out)
     c->ktiming = 0;
     c->kspan_rounds = 0;
     c->tpb = 1;
+    c->scratch = scratch_dev;
     *out = c;
     return PLOM_OK;
 }
@@ -942,8 +979,10 @@ static int prepare_inputs(const plom_filter_desc *d, const plom_model_info &mi, 
     return PLOM_OK;
 }
 
+/* fixed: the device memory to lay the filter out in (the fixed scratch, fixed_cap bytes)
+ * instead of an allocation of its own */
 static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom_filter_inputs *in, char *pinned,
-                              plom_filter **out, hipStream_t st = nullptr)
+                              plom_filter **out, hipStream_t st = nullptr, char *fixed = nullptr, size_t fixed_cap = 0)
 {
     if (!st && c) st = c->stream;
     if (!c || !out || !in) return set_err(PLOM_ERR_ARG, "null argument");
@@ -1007,10 +1046,21 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
                  : (int)(d->noises_off & 3u);
     f->dev_bytes = L.total;
     f->async_alloc = async;
-    hipError_t e = async ? hipMallocAsync((void **)&f->dev, L.total, st) : hipMalloc((void **)&f->dev, L.total);
-    if (e != hipSuccess) {
-        delete f;
-        return set_err(PLOM_ERR_NOMEM, "hipMalloc(%zu): %s", L.total, hipGetErrorString(e));
+    hipError_t e = hipSuccess;
+    if (fixed) {
+        if (L.total > fixed_cap) {
+            delete f;
+            return set_err(PLOM_ERR_ARG, "the filter needs %zu bytes of device memory, more than the %zu of the fixed scratch",
+                           L.total, fixed_cap);
+        }
+        f->dev = fixed;
+        f->fixed_dev = 1;
+    } else {
+        e = async ? hipMallocAsync((void **)&f->dev, L.total, st) : hipMalloc((void **)&f->dev, L.total);
+        if (e != hipSuccess) {
+            delete f;
+            return set_err(PLOM_ERR_NOMEM, "hipMalloc(%zu): %s", L.total, hipGetErrorString(e));
+        }
     }
     char *b = f->dev;
     plom_dfilter *h = &f->h;
@@ -1168,7 +1218,9 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     e = hipMemcpyAsync(b + L.off_img, img, image_bytes(&L), hipMemcpyHostToDevice, st);
     if (e == hipSuccess && !async) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
-        if (async) (void)hipFreeAsync(f->dev, st); else (void)hipFree(f->dev);
+        if (f->fixed_dev) (void)hipStreamSynchronize(st);
+        else if (async) (void)hipFreeAsync(f->dev, st);
+        else (void)hipFree(f->dev);
         delete f;
         return set_err(PLOM_ERR_HIP, "filter upload: %s", hipGetErrorString(e));
     }
@@ -1269,7 +1321,8 @@ extern "C" int plom_filter_destroy(plom_ctx *c, plom_filter *f)
 {
     if (!f) return PLOM_OK;
     if (c) HIPCHK(hipStreamSynchronize(c->stream));
-    if (f->async_alloc && c) HIPCHK(hipFreeAsync(f->dev, c->stream));
+    if (f->fixed_dev) { /* the fixed scratch stays */ }
+    else if (f->async_alloc && c) HIPCHK(hipFreeAsync(f->dev, c->stream));
     else HIPCHK(hipFree(f->dev));
     delete f;
     return PLOM_OK;
@@ -1467,6 +1520,26 @@ extern "C" int plom_smc_run(plom_ctx *c, const plom_filter_desc *d, const plom_f
     rc = plom_filter_run(c, f);
     if (rc == PLOM_OK) rc = plom_filter_read(c, f, o);
     int rc2 = plom_filter_destroy(c, f);
+    return rc != PLOM_OK ? rc : rc2;
+}
+
+/* plom_smc_run in the device's fixed scratch: no device allocation or free.  The filter's
+ * whole device layout (work buffers, counters, the uploaded inputs) is written before it is
+ * read: the upload carries the zeroed counters and the inputs, and every interval writes the
+ * states, weights and tile sums before reading them, so the previous call's contents in the
+ * scratch do not reach the results. */
+extern "C" int plom_smc_run_noalloc(plom_ctx *c, const plom_filter_desc *d, const plom_filter_inputs *in,
+                                    plom_filter_outputs *o)
+{
+    if (!c || !d || !in || !o) return set_err(PLOM_ERR_ARG, "null argument");
+    if (!c->scratch) return set_err(PLOM_ERR_ARG, "device %d has no fixed scratch", c->device);
+    HIPCHK(hipSetDevice(c->device));
+    plom_filter *f = nullptr;
+    int rc = filter_create_impl(c, d, in, nullptr, &f, c->stream, c->scratch, PLOM_NOALLOC_MAX_BYTES);
+    if (rc != PLOM_OK) return rc;
+    rc = plom_filter_run(c, f);
+    if (rc == PLOM_OK) rc = plom_filter_read(c, f, o);
+    const int rc2 = plom_filter_destroy(c, f);
     return rc != PLOM_OK ? rc : rc2;
 }
 

@@ -82,7 +82,7 @@ class FilterOutputs(C.Structure):
 
 EXPORTS = [
     "plom_last_error", "plom_ctx_create", "plom_ctx_destroy", "plom_model_count", "plom_model_find",
-    "plom_model_name", "plom_model_info_get", "plom_model_load", "plom_smc_run", "plom_filter_create", "plom_filter_reset",
+    "plom_model_name", "plom_model_info_get", "plom_model_load", "plom_smc_run", "plom_smc_run_noalloc", "plom_filter_create", "plom_filter_reset",
     "plom_filter_update", "plom_filter_mif_start", "plom_covariate_eval",
     "plom_filter_run", "plom_filter_read", "plom_filter_destroy", "plom_filter_device_bytes",
     "plom_batch_run", "plom_sync", "plom_ctx_set_kernel_timing", "plom_last_timing", "plom_last_timing_lanes", "plom_selftest_math",
@@ -139,6 +139,7 @@ def lib():
     L.plom_model_info_get.argtypes = [C.c_int, C.POINTER(ModelInfo)]
     L.plom_model_load.argtypes = [C.c_char_p, C.POINTER(C.c_int32)]
     L.plom_smc_run.argtypes = [C.c_void_p, C.POINTER(FilterDesc), C.POINTER(FilterInputs), C.POINTER(FilterOutputs)]
+    L.plom_smc_run_noalloc.argtypes = L.plom_smc_run.argtypes
     L.plom_filter_create.argtypes = [C.c_void_p, C.POINTER(FilterDesc), C.POINTER(FilterInputs), C.POINTER(C.c_void_p)]
     L.plom_filter_reset.argtypes = [C.c_void_p, C.c_void_p]
     L.plom_filter_update.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -724,6 +725,35 @@ def run(ctx, model, J, times, data, pv, x0, ancestry=False, final_state=False, l
         return f.read(ancestry=ancestry, final_state=final_state, final_logw=True, lineage=lineage)
     finally:
         f.close()
+
+
+NOALLOC_MAX_BYTES = 64 << 20   # PLOM_NOALLOC_MAX_BYTES
+
+
+def smc_run(ctx, model, J, times, data, pv, x0, noalloc=False, precision="fp64", noises_off=0, flags=0, seed=1,
+            stream=0, dt=0.25, like_min=1e-17):
+    """One blocking filter through plom_smc_run (upload, run, read back, free), or with
+    noalloc through plom_smc_run_noalloc (the same in the library's fixed device scratch,
+    no device allocation).  Returns the log-likelihood, per-interval log-likelihoods and
+    ESS, and, as flags ask, the ancestry and the final cloud."""
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    N = len(times) - 1
+    data = np.ascontiguousarray(data, dtype=np.float64).reshape(N, model.n_ts)
+    pv = np.ascontiguousarray(pv, dtype=np.float64)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    if pv.size != model.layout["size"] or x0.size != model.S:
+        raise ValueError("pv / x0 size mismatch for model %s" % model.name)
+    desc = _desc(model, J, N, precision, noises_off, flags, seed, stream, dt, like_min)
+    inp = FilterInputs(dptr(times), dptr(data), dptr(pv), dptr(x0), None, None, None)
+    ll_n, ess = np.zeros(N), np.zeros(N)
+    anc = np.zeros((N, J), dtype=np.int32) if flags & FLAG_STORE_ANCESTRY else None
+    fs = np.zeros((model.S, J)) if flags & FLAG_FINAL_STATE else None
+    fw = np.zeros(J)
+    out = FilterOutputs(0.0, 0, dptr(ll_n), dptr(ess), iptr(anc), dptr(fs), dptr(fw), None, None, None, None,
+                        None, None, None, None, None)
+    fn = lib().plom_smc_run_noalloc if noalloc else lib().plom_smc_run
+    check(fn(ctx.h, C.byref(desc), C.byref(inp), C.byref(out)))
+    return FilterResult(out.loglik, out.n_all_fail, ll_n, ess, anc, fs, fw)
 
 
 def selftest_math(ctx, x):

@@ -19,7 +19,9 @@ Legs on the same JSON line:
             deviation from FP64
   served    a Poisson stream (200 requests/s) of SIR / SIRS requests, J log-uniform in
             [500, 5e4], N ~ U{52..520}, served with continuous batching and admission by
-            free HBM: sustained particle-steps/s and p50 / p99 latency (submit -> result)
+            free HBM: sustained particle-steps/s and p50 / p99 latency (submit -> result);
+            then the same mix offered at 400, 600 and 800 requests/s (10 s each): the capacity is
+            the highest sustained rate
   single    one large filter (SIRS, J = 1e5, N = 520), FP64 and FP32
   psr       the same filter with the psr implementation (Euler-multinomial step, FP64),
             and the C oracle's psr on a bounded sample of it
@@ -373,6 +375,9 @@ def main():
     ap.add_argument("--lanes", type=int, default=0, help="concurrent chains of the batch (0: autotuned)")
     ap.add_argument("--served-seconds", type=float, default=30.0)
     ap.add_argument("--served-rate", type=float, default=200.0)
+    ap.add_argument("--sweep-rates", type=lambda x: [float(v) for v in x.split(",") if v], default=[400.0, 600.0, 800.0],
+                    help="further arrival rates of the served capacity sweep (comma list; empty: none)")
+    ap.add_argument("--sweep-seconds", type=float, default=10.0)
     ap.add_argument("--single-J", type=int, default=100000)
     ap.add_argument("--single-N", type=int, default=520)
     ap.add_argument("--cpu-filters", type=int, default=8)
@@ -489,6 +494,24 @@ def main():
                           "N~U{52..520}, FP64" % (args.served_rate, args.served_seconds))
         if "plain" in out:
             sv["vs_plain"] = round(sv["particle_steps_per_s"] / out["plain"]["value"], 2)
+        # capacity: the same mix offered faster, until the server no longer keeps up
+        points = [{"rate": args.served_rate, "seconds": args.served_seconds,
+                   **{k: sv[k] for k in ("requests", "offered_particle_steps_per_s", "particle_steps_per_s",
+                                         "makespan_s", "p50_ms", "p99_ms", "peak_active")}}]
+        for rate in (r for r in args.sweep_rates if r != args.served_rate):
+            log("served capacity sweep: %.0f s at %.0f requests/s" % (args.sweep_seconds, rate))
+            p = served_leg(smc, ctx, inp, configs, args.sweep_seconds, rate, check_n=4)
+            points.append({"rate": rate, "seconds": args.sweep_seconds,
+                           **{k: round(p[k], 3) if isinstance(p[k], float) else p[k]
+                              for k in ("requests", "offered_particle_steps_per_s", "particle_steps_per_s",
+                                        "makespan_s", "p50_ms", "p99_ms", "peak_active", "max_rel_err_vs_alone")}})
+        best = max(points, key=lambda q: q["particle_steps_per_s"])
+        sv["sweep"] = points
+        sv["capacity"] = {"rate": best["rate"], "particle_steps_per_s": round(best["particle_steps_per_s"], 1),
+                          "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"],
+                          "keeps_up": best["particle_steps_per_s"] >= 0.95 * best["offered_particle_steps_per_s"]}
+        if "plain" in out:
+            sv["capacity"]["vs_plain"] = round(best["particle_steps_per_s"] / out["plain"]["value"], 2)
         out["served"] = sv
     if "memory" in legs or "memory_alone" in legs:
         log("memory-heavy filter alone")
@@ -503,7 +526,11 @@ def main():
                                    "p50_ms": round(sv["p50_ms"], 3), "p99_ms": round(sv["p99_ms"], 3),
                                    "particle_steps_per_s": round((sv["particle_steps"] + args.memory_J * args.memory_N)
                                                                  / sv["makespan_s"], 1),
-                                   "max_rel_err_vs_alone": sv["max_rel_err_vs_alone"]}
+                                   "max_rel_err_vs_alone": sv["max_rel_err_vs_alone"],
+                                   "heavy_latency_vs_alone": round(sv["heavy"]["latency_s"] / mem["s_per_filter"], 2)}
+        if "served" in out:
+            # the same stream without the heavy request
+            mem["served_alongside"]["small_p99_vs_without_heavy"] = round(sv["p99_ms"] / out["served"]["p99_ms"], 2)
         out["memory_heavy"] = mem
     if "single" in legs:
         log("single large filter")

@@ -379,6 +379,13 @@ typedef struct {
     int64_t heavy_threshold;     /* J * size_proj from which a request runs in the heavy lane
                                     (its own stream, stepping independently of the small
                                     requests); 0: 4e6; negative: one lane for all */
+    int32_t heavy_blocks;        /* while small requests are in flight, the heavy lane's
+                                    propagate launches cover at most this many blocks each (tile
+                                    ranges of one interval, one after another on its stream), so
+                                    that the heavy request holds at most that many blocks' share of
+                                    the CUs and the small requests' blocks find room at once;
+                                    0: the device's CU count; negative: no limit */
+    int32_t pad;
 } plom_server_config;
 typedef struct plom_server plom_server;
 int plom_server_create(plom_ctx *ctx, const plom_server_config *cfg, plom_server **out);

@@ -2005,7 +2005,11 @@ extern "C" int plom_batch_run(plom_ctx *c, int nf, plom_filter **fs, int use_gra
  * Two lanes: requests whose work per interval (J x size_proj) reaches the heavy threshold
  * run in lane 1, on a stream of their own, so that the short intervals of the small
  * requests (lane 0, the context stream) do not wait behind theirs; the two lanes run
- * concurrently on the device.  Each lane keeps at most max_inflight_steps steps queued;
+ * concurrently on the device.  While small requests are in flight, the heavy lane's
+ * propagate launches cover at most heavy_blocks blocks each (one block per CU by default):
+ * a launch over all of a heavy request's tiles would hold every CU's slots, and each small
+ * launch would wait for heavy blocks to retire before its own could start.  Each lane keeps
+ * at most max_inflight_steps steps queued;
  * a step skips a lane whose window is full and blocks only when no lane can advance.
  * Host-to-device uploads come from a pool of page-locked blocks, released once the lane
  * step that used them is known to be complete. */
@@ -2087,6 +2091,8 @@ struct srv_lane {
 struct plom_server {
     plom_ctx *c;
     int64_t budget, in_use, heavy;
+    int heavy_blocks;                          /* blocks per heavy propagate launch while small
+                                                  requests are in flight (0: no limit) */
     int max_active;
     uint64_t next_id;
     std::vector<srv_request *> queue;          /* FIFO */
@@ -2205,6 +2211,18 @@ extern "C" int plom_server_create(plom_ctx *c, const plom_server_config *cfg, pl
     S->max_active = (cfg && cfg->max_active > 0) ? cfg->max_active : 4096;
     S->depth = (cfg && cfg->max_inflight_steps > 0) ? cfg->max_inflight_steps : 4;
     S->heavy = (cfg && cfg->heavy_threshold != 0) ? cfg->heavy_threshold : (int64_t)4000000;
+    if (cfg && cfg->heavy_blocks < 0) {
+        S->heavy_blocks = 0;
+    } else if (cfg && cfg->heavy_blocks > 0) {
+        S->heavy_blocks = cfg->heavy_blocks;
+    } else {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || ncu <= 0) {
+            srv_free(S);
+            return set_err(PLOM_ERR_HIP, "the device's CU count is unavailable");
+        }
+        S->heavy_blocks = ncu;
+    }
     S->capacity = S->max_active;
     S->in_use = 0;
     S->next_id = 1;
@@ -2318,7 +2336,19 @@ static int srv_poll(plom_server *S)
     return PLOM_OK;
 }
 
-/* one interval for every active request of a lane whose ring has room */
+/* small requests are active or queued */
+static bool srv_light_busy(const plom_server *S)
+{
+    if (!S->lanes[0].active.empty()) return true;
+    for (auto r : S->queue)
+        if (r->lane == 0) return true;
+    return false;
+}
+
+/* one interval for every active request of a lane whose ring has room.  The heavy lane,
+ * while small requests are in flight, covers each group's tiles in launches of at most
+ * heavy_blocks blocks (consecutive tile ranges; the stream runs them one after another, and
+ * the tiles are independent, so the results are those of one launch). */
 static int srv_lane_step(plom_server *S, srv_lane *L)
 {
     plom_ctx *c = S->c;
@@ -2327,8 +2357,13 @@ static int srv_lane_step(plom_server *S, srv_lane *L)
         if (rc != PLOM_OK) return rc;
     }
     (void)hipGetLastError();
-    for (auto &g : L->groups)
-        model_at(g.model)->launch_prop(g.prec, g.variant, 0, (int)g.n, c->tpb, L->stream, S->d_slots, L->d_tiles + g.off);
+    const bool limit = L == &S->lanes[1] && S->heavy_blocks > 0 && srv_light_busy(S);
+    for (auto &g : L->groups) {
+        const size_t chunk = limit ? (size_t)S->heavy_blocks * (size_t)c->tpb : g.n;
+        for (size_t o = 0; o < g.n; o += chunk)
+            model_at(g.model)->launch_prop(g.prec, g.variant, 0, (int)std::min(chunk, g.n - o), c->tpb, L->stream,
+                                           S->d_slots, L->d_tiles + g.off + o);
+    }
     enqueue_resample(L->stream, S->d_slots, (const int2 *)L->d_tiles, (int)L->n_all, L->d_slots_active, (int)L->active.size());
     HIPCHK(hipGetLastError());
     for (size_t i = 0; i < L->active.size();) {

@@ -108,7 +108,7 @@ class BatchTuning(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active", C.c_int32), ("max_inflight_steps", C.c_int32),
-                ("heavy_threshold", C.c_int64)]
+                ("heavy_threshold", C.c_int64), ("heavy_blocks", C.c_int32), ("pad", C.c_int32)]
 
 _lib = None
 
@@ -635,9 +635,10 @@ class Batch:
 class Server:
     """The served stream (plom_server_*): continuous batching with admission by free HBM."""
 
-    def __init__(self, ctx, hbm_budget_bytes=0, max_active=0, max_inflight_steps=0, heavy_threshold=0):
+    def __init__(self, ctx, hbm_budget_bytes=0, max_active=0, max_inflight_steps=0, heavy_threshold=0, heavy_blocks=0):
         self.ctx = ctx
-        cfg = ServerConfig(int(hbm_budget_bytes), int(max_active), int(max_inflight_steps), int(heavy_threshold))
+        cfg = ServerConfig(int(hbm_budget_bytes), int(max_active), int(max_inflight_steps), int(heavy_threshold),
+                           int(heavy_blocks), 0)
         self._ids = (C.c_uint64 * 4096)()
         self.h = C.c_void_p()
         check(lib().plom_server_create(ctx.h, C.byref(cfg), C.byref(self.h)))

@@ -119,10 +119,14 @@ def test_done_list_and_inflight_bound(ctx):
     srv.close()
 
 
-def test_heavy_lane_equals_alone(ctx):
+@pytest.mark.parametrize("heavy_blocks", [0, 3, -1])
+def test_heavy_lane_equals_alone(ctx, heavy_blocks):
     """A request at or above the heavy threshold steps in its own lane beside the small
-    ones; every result still equals its run alone."""
-    srv = smc.Server(ctx, heavy_threshold=30000)   # J * size_proj: sirs4 (S = 12) with J >= 2500
+    ones, its propagate launches limited to heavy_blocks blocks while they run (0: one per
+    CU; 3: a 6000-particle request's 24 tiles in 8 launches per interval; -1: no limit);
+    every result still equals its run alone."""
+    print("heavy lane, heavy_blocks", heavy_blocks, flush=True)
+    srv = smc.Server(ctx, heavy_threshold=30000, heavy_blocks=heavy_blocks)   # sirs4 (S = 12) with J >= 2500
     reqs = [("sirs4", 6000, 40, 41), ("sir", 800, 30, 42), ("sir", 1500, 12, 43), ("sirs4", 700, 20, 44),
             ("sirs4", 3000, 10, 45)]
     ids = []

@@ -87,6 +87,13 @@ extern "C" {
 
 #define PLOM_MAX_J (1 << 21)          /* particles per filter */
 
+/* school terms (terms_forcing, C/core/special_functions.c:21-47): holiday periods per
+ * population, and the values of one population's record in pv */
+#ifndef PLOM_MAX_TERMS
+#define PLOM_MAX_TERMS 8
+#define PLOM_SCHOOL_REC (2 + 2 * PLOM_MAX_TERMS)
+#endif
+
 typedef struct {
     int32_t size_proj;   /* S: N_PAR_SV*N_CAC + N_TS_INC_UNIQUE (C/smc/main.c:249) */
     int32_t n_ts;        /* N_TS */
@@ -94,7 +101,10 @@ typedef struct {
     int32_t n_par_sv;    /* N_PAR_SV */
     int32_t inc_offset;  /* first incidence slot of the state */
     int32_t n_inc;       /* N_TS_INC_UNIQUE */
-    int32_t pv_size;     /* length of the flat natural-parameter vector */
+    int32_t pv_size;     /* length of the flat natural-parameter vector; a model whose rates use
+                            terms_forcing ends it with one school record per population
+                            (PLOM_SCHOOL_REC values: n_terms, prop_school, PLOM_MAX_TERMS
+                            (begin, end) holiday periods in years) */
     int32_t n_dem;       /* demographic noises per population */
     int32_t n_env;       /* environmental noises per population */
     int32_t n_hat;       /* quantities of a hat row: states, remainders, observation means */

@@ -717,6 +717,25 @@ void cli_pack(const cli_problem *P, const double *theta, double *pv, double *x0)
     pj *coupling = pj_get(pj_get(P->settings, "coupling"), "matrix");
     for (int a = 0; a < pj_size(coupling); a++)
         for (int b = 0; b < pj_size(pj_at(coupling, a)); b++) pv[o++] = pj_number(pj_at(pj_at(coupling, a), b), NULL);
+    /* school terms (IS_SCHOOL_TERMS, build.c:834-868): per population n_terms, prop_school =
+     * 1 - sum(end - begin), then the (begin, end) pairs of data.school_terms, in years */
+    if (pj_number(pj_get(pj_get(P->settings, "cst"), "IS_SCHOOL_TERMS"), NULL) == 1.0) {
+        pj *terms = pj_get(pj_get(P->settings, "data"), "school_terms");
+        if (pj_size(terms) != N_CAC) cli_die("settings data.school_terms needs one list per population");
+        for (int cac = 0; cac < N_CAC; cac++) {
+            pj *lst = pj_at(terms, cac);
+            const int n = pj_size(lst) / 2;
+            if (pj_size(lst) % 2 || n > PLOM_MAX_TERMS) cli_die("settings data.school_terms: (begin, end) pairs, at most 8 per population");
+            double *rec = pv + o;
+            memset(rec, 0, PLOM_SCHOOL_REC * sizeof(double));
+            rec[0] = n;
+            double held = 0.0;
+            for (int k = 0; k < 2 * n; k++) rec[2 + k] = pj_number(pj_at(lst, k), NULL);
+            for (int k = 0; k < n; k++) held += rec[3 + 2 * k] - rec[2 + 2 * k];
+            rec[1] = 1.0 - held;
+            o += PLOM_SCHOOL_REC;
+        }
+    }
     if (o != P->mi.pv_size) {
         snprintf(str, sizeof str, "parameter vector has %d values, the model expects %d", o, P->mi.pv_size);
         cli_die(str);

@@ -21,6 +21,7 @@ whole state lives in registers.  Arithmetic is printed in the reference's term o
 (Ccoder.py:449-505): X + (drift)*dt + demographic terms + environmental terms.
 """
 from . import expr as E
+from .model import SCHOOL_REC
 
 VARIANTS = [
     # name, demographic noise on, environmental noise on
@@ -82,6 +83,10 @@ class Emitter:
                 if text not in self.time_funcs:
                     self.time_funcs[text] = "tf%d" % len(self.time_funcs)
                 return self.time_funcs[text]
+            if fn == "terms_forcing":
+                # the population's school record in pv (Ccoder.py:88-89 passes t, p_data, cac)
+                return "plom_terms_forcing%s(%s, tr, pv + %d)" % ("" if real == "double" else "f", args[0],
+                                                                  lay["school"] + cac * SCHOOL_REC)
             if fn == "correct_rate":
                 # skipped in SDE/ODE code (Ccoder.py:55-56, generator_C with no_correct_rate)
                 return "(%s)" % args[0]

@@ -11,6 +11,15 @@ Follows the reference's parsing and rewriting rules:
     model_builder/Builder.py:70-169;
   * the ``.settings.json`` written for the C side: model_builder/Builder.py:173-259.
 
+School terms (``terms_forcing(amplitude)`` in a rate, special_functions.c:21-47): the
+holiday periods of every population, as fractions of a year, come from the context's
+``data.school_terms`` (one flat list ``[begin_0, end_0, begin_1, end_1, ...]`` per
+population; the reference's loader reads the same list from the settings,
+C/core/build.c:834-868, and its builder leaves writing it to the user, Builder.py:219-220).
+They travel in pv, one record of PLOM_SCHOOL_REC values per population after the other
+blocks: n_terms, prop_school = 1 - sum(end - begin), then the PLOM_MAX_TERMS (begin, end)
+pairs, unused pairs 0.
+
 Extension (absent from the reference grammar, where every state maps to its own
 population, model_builder/Ccoder.py:36-37): ``mix(X)`` inside a rate evaluates, for
 population ``cac``, sum_k C[cac][k] * X[k] with a coupling matrix C given in the context
@@ -25,6 +34,10 @@ from . import expr as E
 
 class ModelError(ValueError):
     pass
+
+
+MAX_TERMS = 8                    # PLOM_MAX_TERMS: holiday periods per population
+SCHOOL_REC = 2 + 2 * MAX_TERMS   # PLOM_SCHOOL_REC: values per population in pv
 
 
 class PlomModel:
@@ -200,6 +213,8 @@ class PlomModel:
                            if r["from"] == d["from"] and r["to"] == d["to"] and r["rate"] == d["rate"]]
                     d["reactions"] = ids
         self.rate_trees = [E.parse(r) for r in self.rates]
+        self.is_school_terms = any(c.fn == "terms_forcing" for t in self.rate_trees for c in E.calls(t))
+        self.school_terms = self._school_terms() if self.is_school_terms else None
         self.layout = self._param_layout()
 
     def check_codegen(self):
@@ -241,10 +256,44 @@ class PlomModel:
                 if s not in known:
                     raise ModelError("unknown symbol %r in rate" % s)
             for c in E.calls(t):
-                if c.fn == "terms_forcing":
-                    raise ModelError("terms_forcing (school terms) is not supported by this build")
+                if c.fn == "terms_forcing" and len(c.args) != 1:
+                    raise ModelError("terms_forcing() takes one argument, the amplitude")
                 if c.fn == "mix" and self.context.get("coupling") is None:
                     raise ModelError("mix() needs a coupling matrix in the context")
+
+    def _school_terms(self):
+        """Per population, the flat list of holiday periods [begin, end, ...] in years
+        (context data.school_terms), checked."""
+        terms = self.context.get("data", {}).get("school_terms")
+        if terms is None:
+            raise ModelError("terms_forcing() needs data.school_terms in the context (one list per population)")
+        if len(terms) != self.N_CAC:
+            raise ModelError("data.school_terms needs one list per population (%d)" % self.N_CAC)
+        out = []
+        for cac, lst in enumerate(terms):
+            v = [float(x) for x in lst]
+            if len(v) % 2 or len(v) // 2 > MAX_TERMS:
+                raise ModelError("school_terms[%d]: pairs (begin, end), at most %d of them" % (cac, MAX_TERMS))
+            for k in range(0, len(v), 2):
+                if not (0.0 <= v[k] <= v[k + 1] <= 1.0):
+                    raise ModelError("school_terms[%d]: a period [begin, end] within [0, 1] years" % cac)
+            out.append(v)
+        return out
+
+    def school_records(self):
+        """The pv values of the school block: per population n_terms, prop_school (the
+        proportion of the year with school in, build.c:860-866), the (begin, end) pairs."""
+        out = []
+        for v in self.school_terms:
+            rec = [0.0] * SCHOOL_REC
+            rec[0] = float(len(v) // 2)
+            held = 0.0
+            for k in range(0, len(v), 2):
+                held += v[k + 1] - v[k]
+            rec[1] = 1.0 - held
+            rec[2:2 + len(v)] = v
+            out += rec
+        return out
 
     def _param_layout(self):
         """Offsets of the flat parameter vector pv shared by host and kernels."""
@@ -268,6 +317,9 @@ class PlomModel:
         lay["coupling"] = off if self.coupling is not None else None
         if self.coupling is not None:
             off += self.N_CAC * self.N_CAC
+        lay["school"] = off if self.is_school_terms else None
+        if self.is_school_terms:
+            off += self.N_CAC * SCHOOL_REC
         lay["size"] = off
         return lay
 
@@ -299,6 +351,7 @@ class PlomModel:
                 "par_fixed_values": par_fixed_values if par_fixed_values is not None else {},
                 "dates": dates if dates is not None else [],
                 "times": times if times is not None else [0],
+                **({"school_terms": self.school_terms} if self.is_school_terms else {}),
             },
             "drift": {"ind_par_Xdrift_applied": [all_order.index(x) for x in self.drift_par_proc],
                       "ind_volatility_Xdrift": [all_order.index(x) for x in self.vol_par_proc]},
@@ -312,7 +365,7 @@ class PlomModel:
                 "N_TS": self.N_TS, "N_TS_INC": self.N_TS_INC, "N_TS_INC_UNIQUE": self.N_TS_INC_UNIQUE,
                 "N_DATA": len(data) if data is not None else 0,
                 "N_OBS_ALL": len(self.obs_var_def), "N_OBS_INC": self.N_OBS_INC, "N_OBS_PREV": self.N_OBS_PREV,
-                "N_DRIFT": self.N_DRIFT, "IS_SCHOOL_TERMS": 0,
+                "N_DRIFT": self.N_DRIFT, "IS_SCHOOL_TERMS": 1 if self.is_school_terms else 0,
             },
             "orders": {
                 "par_sv": self.par_sv, "par_proc": self.par_proc, "par_fixed": self.par_fixed,

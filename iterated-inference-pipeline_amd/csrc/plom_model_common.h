@@ -129,6 +129,26 @@ PLOM_HD double plom_heaviside(double x) { return (x < 0.0) ? 0.0 : 1.0; }
 PLOM_HD double plom_ramp(double x) { return (x >= 0.0) ? x : 0.0; }
 PLOM_HD double plom_slowstep(double x, double d) { return (x >= 0.0) ? ((x >= d) ? d : x) : 0.0; }
 
+/* School-term forcing, terms_forcing(amplitude) of a rate (C/core/special_functions.c:21-47).
+ * rec is the population's school record in pv: n_terms, prop_school (the proportion of the
+ * year with school in: 1 - sum(end - begin), C/core/build.c:860-866), then the holiday
+ * periods (begin, end) in years.  With y = t/365 - floor(t/365) the fraction of the year at
+ * time t (days), the forcing is amplitude / (1 + amplitude (1 - prop_school)) when
+ * begin <= y <= end for some period, and 1 / (1 + amplitude (1 - prop_school)) otherwise. */
+#ifndef PLOM_MAX_TERMS
+#define PLOM_MAX_TERMS 8
+#define PLOM_SCHOOL_REC (2 + 2 * PLOM_MAX_TERMS)
+#endif
+PLOM_HD double plom_terms_forcing(double amp, double t, const double *rec)
+{
+    const double ty = t / 365.0;
+    const double y = ty - __builtin_floor(ty);
+    const int n = (int)rec[0];
+    int hol = 0;
+    for (int k = 0; k < n && k < PLOM_MAX_TERMS; k++) hol |= (y >= rec[2 + 2 * k]) & (y <= rec[3 + 2 * k]);
+    return (hol ? amp : 1.0) / (1.0 + amp * (1.0 - rec[1]));
+}
+
 PLOM_HD double plom_sanitize_likelihood(double like, double like_min, int *n_floor)
 {
     if (plom_isnan(like) || like == PLOM_INF || like == -PLOM_INF || like < 0.0) {
@@ -166,6 +186,15 @@ PLOM_DF float plom_powf(float x, float y) { return powf(x, y); }
 PLOM_DF float plom_heavisidef(float x) { return (x < 0.0f) ? 0.0f : 1.0f; }
 PLOM_DF float plom_rampf(float x) { return (x >= 0.0f) ? x : 0.0f; }
 PLOM_DF float plom_slowstepf(float x, float d) { return (x >= 0.0f) ? ((x >= d) ? d : x) : 0.0f; }
+PLOM_DF float plom_terms_forcingf(float amp, float t, const float *rec)
+{
+    const float ty = t / 365.0f;
+    const float y = ty - __builtin_floorf(ty);
+    const int n = (int)rec[0];
+    int hol = 0;
+    for (int k = 0; k < n && k < PLOM_MAX_TERMS; k++) hol |= (y >= rec[2 + 2 * k]) & (y <= rec[3 + 2 * k]);
+    return (hol ? amp : 1.0f) / (1.0f + amp * (1.0f - rec[1]));
+}
 /* FP32 propagation is its own Monte Carlo estimate (checked against FP64 statistically),
  * so its noise square roots, divisions and clipping use the single-instruction hardware
  * forms (v_sqrt_f32 and v_rcp_f32 within 1 ulp, v_max_f32) instead of the correctly

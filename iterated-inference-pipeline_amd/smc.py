@@ -358,6 +358,9 @@ class Model:
                                                                                     off - lay["fix_off"] + m.N_TS]])
         if lay["coupling"] is not None:
             pv[lay["coupling"]:lay["coupling"] + m.N_CAC ** 2] = np.asarray(m.coupling, dtype=np.float64).ravel()
+        if lay.get("school") is not None:
+            rec = m.school_records()
+            pv[lay["school"]:lay["school"] + len(rec)] = rec
         return pv
 
     def x0(self, props, pop_size=None):

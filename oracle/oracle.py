@@ -26,7 +26,8 @@ def extra_lib(name, gen_dir):
     mif_pass(), trajectory() ... like a built-in id."""
     if name not in _extra:
         path = os.path.join(HERE, "_build", "liboracle_%s.so" % name)
-        if not os.path.exists(path):
+        if not os.path.exists(path) or os.path.exists(os.path.join(HERE, "Makefile")):
+            # make rebuilds it when the oracle's sources or the model's header are newer
             subprocess.check_call(["make", "-s", "-C", HERE, "extra", "NAME=" + name, "GEN=" + os.path.abspath(gen_dir)])
         _extra[name] = _setup(C.CDLL(path))
     return _extra[name]
@@ -77,6 +78,7 @@ def _setup(L):
     L.oracle_bm_pairs.argtypes = [C.c_int, C.POINTER(C.c_uint32), dp]
     L.oracle_icdf.argtypes = [C.c_int, C.POINTER(C.c_uint32), dp]
     L.oracle_log_tab.argtypes = [C.c_int, dp, dp]
+    L.oracle_terms_forcing.argtypes = [C.c_int, C.c_double, dp, dp, dp]
     L.oracle_xoshiro.argtypes = [C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_uint32)]
     L.oracle_model_name.restype = C.c_char_p
     L.oracle_psr_draws.argtypes = [C.c_int, C.c_double, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -375,6 +377,15 @@ def icdf(words):
     w = np.ascontiguousarray(words, dtype=np.uint32).ravel()
     out = np.zeros(w.size)
     lib().oracle_icdf(w.size, w.ctypes.data_as(C.POINTER(C.c_uint32)), _dp(out))
+    return out
+
+
+def terms_forcing(amp, t, rec):
+    """terms_forcing(amp) at the times t (days) for one school record (plom_terms_forcing)."""
+    t = np.ascontiguousarray(t, dtype=np.float64).ravel()
+    rec = np.ascontiguousarray(rec, dtype=np.float64)
+    out = np.zeros(t.size)
+    lib().oracle_terms_forcing(t.size, float(amp), _dp(t), _dp(rec), _dp(out))
     return out
 
 

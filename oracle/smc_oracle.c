@@ -1086,6 +1086,13 @@ void oracle_icdf(int n, const uint32_t *w, double *out)
     for (int i = 0; i < n; i++) out[i] = plom_icdf_normal(w[i], plom_icdf_tab);
 }
 
+/* terms_forcing(amp) at the times t[0..n) (days) for one population's school record rec
+ * (plom_terms_forcing) */
+void oracle_terms_forcing(int n, double amp, const double *t, const double *rec, double *out)
+{
+    for (int i = 0; i < n; i++) out[i] = plom_terms_forcing(amp, t[i], rec);
+}
+
 /* the psr samplers' table logarithm of x[0..n) (plom_log_tab) */
 void oracle_log_tab(int n, const double *x, double *out)
 {

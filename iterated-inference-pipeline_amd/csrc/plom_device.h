@@ -177,7 +177,23 @@ __device__ inline int select_ancestor(const plom_dfilter *f, int k, int n)
 /* PLOM_FLAG_MIF: theta of particle j from its ancestor a, moved by the random walk when
  * interval n-1 had data (resample_and_mut_theta_mif, C/mif/methods.c:261-367: every
  * component with a jump size moves by mut_fac[n-1] * sd * N(0,1)); written to the output
- * table; then the per-particle slots of pv from it (back_transform_theta2par) */
+ * table; then the per-particle slots of pv from it (back_transform_theta2par).  The slots
+ * take each component's natural value as the component is moved, from the register that
+ * holds it (the slot map pm_src is uniform, so the match is a scalar branch), instead of
+ * reading the output table back. */
+template <int NPP>
+__device__ inline void mif_slots(const plom_dfilter *f, const int *src, int o, double v, double *lp)
+{
+    bool used = false;
+#pragma unroll
+    for (int s = 0; s < NPP; s++) used = used || src[s] == o;
+    if (!used) return;
+    const double nat = plom_mif_natural(f->pm_tr[o], f->pm_sc[o], f->pm_dur[o], v, f->pm_min[o], f->pm_max[o], f->pm_mult[o]);
+#pragma unroll
+    for (int s = 0; s < NPP; s++)
+        if (src[s] == o) lp[s] = nat;
+}
+
 template <int NPP>
 __device__ inline void mif_particle(const plom_dfilter *f, int n, int j, int a, const double *tab, double *lp)
 {
@@ -185,6 +201,12 @@ __device__ inline void mif_particle(const plom_dfilter *f, int n, int j, int a, 
     const double *Tin = f->th[n & 1];
     double *Tout = f->th[(n + 1) & 1];
     const double fac = (n > 0 && f->nonan[n - 1] > 0) ? f->mut_fac[n - 1] : 0.0;
+    int src[NPP];
+#pragma unroll
+    for (int s = 0; s < NPP; s++) {
+        src[s] = f->pm_src[s];
+        lp[s] = f->pv[s];          /* slots without a component keep pv */
+    }
     if (f->pm_chol) {
         /* correlated walk: theta_o += fac sum_{q<=o} L[o][q] z_q, z from the same counters */
         double z[PLOM_MIF_MVN_MAX];
@@ -202,6 +224,7 @@ __device__ inline void mif_particle(const plom_dfilter *f, int n, int j, int a, 
                 v = v + fac * acc;
             }
             Tout[(size_t)o * Jp + j] = v;
+            mif_slots<NPP>(f, src, o, v, lp);
         }
     } else
     for (int o = 0; o < nth; o += 2) {
@@ -212,18 +235,13 @@ __device__ inline void mif_particle(const plom_dfilter *f, int n, int j, int a, 
         double v = Tin[(size_t)o * Jp + a];
         if (sd0 != 0.0) v = v + sd0 * z0;
         Tout[(size_t)o * Jp + j] = v;
+        mif_slots<NPP>(f, src, o, v, lp);
         if (o + 1 < nth) {
             double w = Tin[(size_t)(o + 1) * Jp + a];
             if (sd1 != 0.0) w = w + sd1 * z1;
             Tout[(size_t)(o + 1) * Jp + j] = w;
+            mif_slots<NPP>(f, src, o + 1, w, lp);
         }
-    }
-#pragma unroll
-    for (int s = 0; s < NPP; s++) {
-        const int o = f->pm_src[s];
-        lp[s] = o < 0 ? f->pv[s]
-                      : plom_mif_natural(f->pm_tr[o], f->pm_sc[o], f->pm_dur[o], Tout[(size_t)o * Jp + j], f->pm_min[o],
-                                         f->pm_max[o], f->pm_mult[o]);
     }
 }
 

@@ -11,7 +11,10 @@
  * of logarithms.
  *
  *   plom_unif53        (a >> 6) 2^27 + (b >> 5) + 1/2, times 2^-53: a uniform in (0, 1)
- *                      from two words.
+ *                      from two words (the inversion and multiplication methods, whose
+ *                      tails need the resolution).
+ *   plom_unif32        (w + 1/2) 2^-32 from one word: the uniforms of the rejection
+ *                      attempts (transformed rejection, Marsaglia-Tsang acceptance).
  *   plom_lfact         log k!: exact table below 30, Stirling's series above (error
  *                      below 1e-14).
  *   plom_gamma_draw    Gamma(a, 1): Marsaglia and Tsang (2000, ACM TOMS 26:363-372) for
@@ -53,6 +56,11 @@ PLOM_HD double plom_unif53(plom_nstream64 *ns)
     const uint32_t a = plom_xoshiro_next(&ns->r) >> 6;
     const uint32_t b = plom_xoshiro_next(&ns->r) >> 5;
     return ((double)a * 134217728.0 + (double)b + 0.5) * 0x1p-53;
+}
+
+PLOM_HD double plom_unif32(plom_nstream64 *ns)
+{
+    return plom_fma((double)plom_xoshiro_next(&ns->r), 0x1p-32, 0x1p-33); /* exact */
 }
 
 /* log k! for k = 0..29 (50-digit values rounded to nearest) */
@@ -97,7 +105,7 @@ PLOM_HD_CALL double plom_gamma_draw(plom_nstream64 *ns, double a)
         const double v1 = 1.0 + c * z;
         if (v1 <= 0.0) continue;
         const double v = v1 * v1 * v1;
-        const double u = plom_unif53(ns);
+        const double u = plom_unif32(ns);
         const double z2 = z * z;
         if (u < 1.0 - 0.0331 * (z2 * z2)) return (d * v) * boost;
         /* v underflows to 0 only for v1 < 1e-102, where the test fails either way */
@@ -127,11 +135,12 @@ PLOM_HD_CALL double plom_poisson_draw(plom_nstream64 *ns, double mu)
     const double a = -0.059 + 0.02483 * b;
     const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
     const double vr = 0.9277 - 3.6224 / (b - 2.0);
+    const double a2 = 2.0 * a;
     for (int t = 0; t < PLOM_PSR_MAX_TRY; t++) {
-        const double U = plom_unif53(ns) - 0.5;
-        const double V = plom_unif53(ns);
+        const double U = plom_unif32(ns) - 0.5;
+        const double V = plom_unif32(ns);
         const double us = 0.5 - plom_fabs(U);
-        const double k = plom_floor((2.0 * a / us + b) * U + mu + 0.43);
+        const double k = plom_floor((a2 / us + b) * U + mu + 0.43);
         if (us >= 0.07 && V <= vr) return k;
         if (k < 0.0 || (us < 0.013 && V > us)) continue;
         /* log(V invalpha / (a / us^2 + b)) */
@@ -174,16 +183,18 @@ PLOM_HD double plom_binom_half(plom_nstream64 *ns, double n, double p)
     const double b = 1.15 + 2.53 * spq;
     const double a = -0.0873 + 0.0248 * b + 0.01 * p;
     const double c = np + 0.5;
-    const double vr = 0.92 - 4.2 / b;
-    const double alpha = (2.83 + 5.1 / b) * spq;
+    const double rb = 1.0 / b;
+    const double vr = 0.92 - 4.2 * rb;
+    const double alpha = (2.83 + 5.1 * rb) * spq;
+    const double a2 = 2.0 * a;
     const double lpq = plom_log_tab(p / q, ns->ltab);
     const double m = plom_floor((n + 1.0) * p);
     const double h = plom_lfact(m, ns->ltab) + plom_lfact(n - m, ns->ltab);
     for (int t = 0; t < PLOM_PSR_MAX_TRY; t++) {
-        const double u = plom_unif53(ns) - 0.5;
-        double v = plom_unif53(ns);
+        const double u = plom_unif32(ns) - 0.5;
+        double v = plom_unif32(ns);
         const double us = 0.5 - plom_fabs(u);
-        const double k = plom_floor((2.0 * a / us + b) * u + c);
+        const double k = plom_floor((a2 / us + b) * u + c);
         if (k < 0.0 || k > n) continue;
         if (us >= 0.07 && v <= vr) return k;
         /* log(v alpha / (a / us^2 + b)) */

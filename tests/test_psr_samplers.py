@@ -4,7 +4,8 @@ distributions they draw from (scipy), with fixed keys (deterministic):
 * chi-square goodness of fit of Poisson and Binomial draws in both regimes of each
   sampler (inversion below mean 10, transformed rejection above) and at the symmetry
   switch p = 1/2 of the binomial; the test rejects a 1 % shifted parameter;
-* Kolmogorov-Smirnov of Gamma draws for shapes below and above 1 and of the uniforms;
+* Kolmogorov-Smirnov of Gamma draws for shapes below and above 1 and of the uniforms (two
+  words, and the one-word uniforms of the rejection attempts);
 * log k! against mpmath; the multinomial's counts sum to N with the right means; edge
   cases (n = 0, p = 0, p = 1, mu = 0).
 """
@@ -68,6 +69,15 @@ def test_gamma(a):
     assert x.min() > 0
     assert stats.kstest(x, stats.gamma(a).cdf).pvalue > 1e-4
     assert stats.kstest(x, stats.gamma(a * 1.03).cdf).pvalue < 1e-4
+
+
+def test_uniform32():
+    """the one-word uniforms of the rejection attempts: (w + 1/2) 2^-32, symmetric about 1/2"""
+    u = oracle.psr_draws(oracle.PSR_UNIFORM32, 0.0, 100000, seed=9)
+    assert u.min() > 0.0 and u.max() < 1.0
+    assert stats.kstest(u, "uniform").pvalue > 1e-4
+    w = u * 2.0 ** 32 - 0.5
+    assert np.all(w == np.floor(w)) and w.min() >= 0 and w.max() <= 2.0 ** 32 - 1
 
 
 def test_uniform():

@@ -417,9 +417,16 @@ class Emitter:
             for j, w in enumerate(m.white_noise):
                 if env:
                     sd = self.sd_text(w["sd"], c, "double")
+                    # the noise multiplies the rates of its reactions: when none is positive,
+                    # every flow it would scale is not drawn (probability or mean <= 0), so
+                    # the gamma draw is skipped and xi is 1
+                    used = sorted(set("r%d_%d" % (r["ind_cache"], c) for r in m.proc_model
+                                      if r.get("white_noise", {}).get("name") == w["name"]))
+                    cond = " || ".join("%s > 0.0" % u for u in used) or "false"
                     L.append("        const double sd%d_%d = %s;" % (j, c, sd))
                     L.append("        const double v%d_%d = sd%d_%d * sd%d_%d;" % (j, c, j, c, j, c))
-                    L.append("        const double xi%d_%d = (v%d_%d * plom_gamma_draw(ns, dt / v%d_%d)) / dt;" % (j, c, j, c, j, c))
+                    L.append("        const double xi%d_%d = (%s) ? (v%d_%d * plom_gamma_draw(ns, dt / v%d_%d)) / dt : 1.0;"
+                             % (j, c, cond, j, c, j, c))
                 else:
                     L.append("        const double xi%d_%d = 1.0;" % (j, c))
 

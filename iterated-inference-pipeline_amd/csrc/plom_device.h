@@ -21,6 +21,7 @@
 #include "plom_ode.h"
 #include "plom_rng.h"
 #include "plom_mif.h"
+#include "plom_psr.h"
 
 typedef unsigned __int128 plom_u128;
 
@@ -272,19 +273,25 @@ __global__ __launch_bounds__(TILE * TPB) void k_propagate(plom_dfilter *filters,
         if (!act) return;      /* the whole block */
     }
     /* in LDS: the coefficient table of the stream's Gaussians (not for the ode
-     * implementation, which draws none; FP64 or its float copy) and, with MIF in FP64, the
-     * Box-Muller tables of the random walk */
+     * implementation, which draws none; FP64 or its float copy), with MIF in FP64 the
+     * Box-Muller tables of the random walk, and for psr the log and log k! tables of its
+     * samplers */
+    constexpr bool PSR = V == 4 || V == 5;
     __shared__ double s_icdf[PLOM_ICDF_TAB_SIZE];
     __shared__ float s_icdf32[PLOM_ICDF_TAB_SIZE];
     __shared__ double s_bm[PLOM_BM_TAB_SIZE];
+    __shared__ double s_lf[PSR ? 30 : 1];
     if constexpr (sizeof(R) == 8 && V != 6) {
         for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += TILE * TPB) s_icdf[i] = plom_icdf_tab[i];
     }
     if constexpr (sizeof(R) == 4) {
         for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += TILE * TPB) s_icdf32[i] = plom_icdf_tab32[i];
     }
-    if constexpr (sizeof(R) == 8 && MIF) {
+    if constexpr (sizeof(R) == 8 && (MIF || PSR)) {
         for (int i = threadIdx.x; i < PLOM_BM_TAB_SIZE; i += TILE * TPB) s_bm[i] = plom_bm_tab[i];
+    }
+    if constexpr (PSR) {
+        for (int i = threadIdx.x; i < 30; i += TILE * TPB) s_lf[i] = plom_lfact_tab[i];
     }
     if constexpr (sizeof(R) == 4 || V != 6 || MIF) __syncthreads();
     const int J = act ? f->J : 0, Jp = act ? f->Jp : 0;
@@ -359,6 +366,7 @@ __global__ __launch_bounds__(TILE * TPB) void k_propagate(plom_dfilter *filters,
         } else if constexpr (sizeof(R) == 8) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, (uint32_t)j, s_icdf);
+            if constexpr (PSR) plom_psr_tables(&ns, s_bm, s_lf);
             if constexpr (MIF) M::template interval_pp<V>((double *)X, f->pv, lp, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs, f->drift_on ? &f->dv : nullptr, (const double *)tft);
             else M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs, f->drift_on ? &f->dv : nullptr, (const double *)tft);
         } else {
@@ -436,8 +444,15 @@ __global__ __launch_bounds__(TILE * TPB) void k_propagate(plom_dfilter *filters,
 template <class M, typename R, int V>
 __global__ __launch_bounds__(64) void k_lineage(plom_dfilter *f)
 {
+    constexpr bool PSR = V == 4 || V == 5;
     __shared__ double s_icdf[PLOM_ICDF_TAB_SIZE];
     __shared__ float s_icdf32[PLOM_ICDF_TAB_SIZE];
+    __shared__ double s_bm[PSR ? PLOM_BM_TAB_SIZE : 1];
+    __shared__ double s_lf[PSR ? 30 : 1];
+    if constexpr (PSR) {
+        for (int i = threadIdx.x; i < PLOM_BM_TAB_SIZE; i += 64) s_bm[i] = plom_bm_tab[i];
+        for (int i = threadIdx.x; i < 30; i += 64) s_lf[i] = plom_lfact_tab[i];
+    }
     if constexpr (sizeof(R) == 8 && V != 6) {
         for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += 64) s_icdf[i] = plom_icdf_tab[i];
         __syncthreads();
@@ -479,6 +494,7 @@ __global__ __launch_bounds__(64) void k_lineage(plom_dfilter *f)
         } else if constexpr (sizeof(R) == 8) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, jn, s_icdf);
+            if constexpr (PSR) plom_psr_tables(&ns, s_bm, s_lf);
             M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs,
                                     f->drift_on ? &f->dv : nullptr, (const double *)nullptr);
         } else {

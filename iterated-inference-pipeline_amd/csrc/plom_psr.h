@@ -16,7 +16,7 @@
  *   plom_unif32        (w + 1/2) 2^-32 from one word: the uniforms of the rejection
  *                      attempts (transformed rejection, Marsaglia-Tsang acceptance).
  *   plom_lfact         log k!: exact table below 30, Stirling's series above (error
- *                      below 1e-14).
+ *                      below 1e-14); the stream carries both tables (plom_psr_tables).
  *   plom_gamma_draw    Gamma(a, 1): Marsaglia and Tsang (2000, ACM TOMS 26:363-372) for
  *                      a >= 1; Gamma(a + 1) U^(1/a) below.
  *   plom_poisson_draw  Poisson(mu): the multiplication method (Knuth, TAOCP vol. 2,
@@ -79,13 +79,25 @@ static __constant__ const double plom_lfact_tab[30] = PLOM_LFACT_TAB;
 static const double plom_lfact_tab[30] = PLOM_LFACT_TAB;
 #endif
 
-PLOM_HD double plom_lfact(double k, const double *ltab)
+/* log k! for an integer k >= 0: the table (lftab: plom_lfact_tab or a copy) below 30,
+ * Stirling's series above; both are evaluated, without a branch */
+PLOM_HD double plom_lfact(double k, const double *ltab, const double *lftab)
 {
-    if (k < 30.0) return plom_lfact_tab[(int)k];
     const double x = k + 1.0;
     const double r = 1.0 / x, r2 = r * r;
     /* ln Gamma(x) = (x - 1/2) ln x - x + ln sqrt(2 pi) + 1/(12x) - 1/(360x^3) + 1/(1260x^5) */
-    return (x - 0.5) * plom_log_tab(x, ltab) - x + PLOM_LN_SQRT_2PI + r * (1.0 / 12.0 - r2 * (1.0 / 360.0 - r2 * (1.0 / 1260.0)));
+    const double st = (x - 0.5) * plom_log_tab(x, ltab) - x + PLOM_LN_SQRT_2PI
+                      + r * (1.0 / 12.0 - r2 * (1.0 / 360.0 - r2 * (1.0 / 1260.0)));
+    const int small = k < 30.0;
+    const double tv = lftab[small ? (int)k : 0];
+    return small ? tv : st;
+}
+
+/* the stream's psr tables: the log table it holds and the log k! table (the host's) */
+PLOM_HD void plom_psr_tables(plom_nstream64 *ns, const double *ltab, const double *lftab)
+{
+    ns->ltab = ltab;
+    ns->lftab = lftab;
 }
 
 /* Gamma(a, 1), a > 0 */
@@ -141,12 +153,17 @@ PLOM_HD_CALL double plom_poisson_draw(plom_nstream64 *ns, double mu)
         const double V = plom_unif32(ns);
         const double us = 0.5 - plom_fabs(U);
         const double k = plom_floor((a2 / us + b) * U + mu + 0.43);
+        /* the quick acceptance, the early rejection and the full test, evaluated together
+         * (no branch: a wave takes the full test whenever one of its lanes needs it), the
+         * full test at k clamped to >= 0 where it is not used */
+        const double kk = k < 0.0 ? 0.0 : k;
+        const double us2 = us * us;
+        /* log(V invalpha / (a / us^2 + b)) */
+        const double lhs = plom_log_tab(V * invalpha * us2 / plom_fma(b, us2, a), ns->ltab);
+        const double rhs = -mu + kk * loglam - plom_lfact(kk, ns->ltab, ns->lftab);
         if (us >= 0.07 && V <= vr) return k;
         if (k < 0.0 || (us < 0.013 && V > us)) continue;
-        /* log(V invalpha / (a / us^2 + b)) */
-        const double us2 = us * us;
-        if (plom_log_tab(V * invalpha * us2 / plom_fma(b, us2, a), ns->ltab) <= -mu + k * loglam - plom_lfact(k, ns->ltab))
-            return k;
+        if (lhs <= rhs) return k;
     }
     return plom_floor(mu);
 }
@@ -189,18 +206,23 @@ PLOM_HD double plom_binom_half(plom_nstream64 *ns, double n, double p)
     const double a2 = 2.0 * a;
     const double lpq = plom_log_tab(p / q, ns->ltab);
     const double m = plom_floor((n + 1.0) * p);
-    const double h = plom_lfact(m, ns->ltab) + plom_lfact(n - m, ns->ltab);
+    const double h = plom_lfact(m, ns->ltab, ns->lftab) + plom_lfact(n - m, ns->ltab, ns->lftab);
     for (int t = 0; t < PLOM_PSR_MAX_TRY; t++) {
         const double u = plom_unif32(ns) - 0.5;
-        double v = plom_unif32(ns);
+        const double v = plom_unif32(ns);
         const double us = 0.5 - plom_fabs(u);
         const double k = plom_floor((a2 / us + b) * u + c);
+        /* the range check, the quick acceptance and the full test, evaluated together (no
+         * branch: a wave takes the full test whenever one of its lanes needs it), the full
+         * test at k clamped into [0, n] where it is not used */
+        const double kk = k < 0.0 ? 0.0 : (k > n ? n : k);
+        const double us2 = us * us;
+        /* log(v alpha / (a / us^2 + b)) */
+        const double lhs = plom_log_tab(v * alpha * us2 / plom_fma(b, us2, a), ns->ltab);
+        const double rhs = h - plom_lfact(kk, ns->ltab, ns->lftab) - plom_lfact(n - kk, ns->ltab, ns->lftab) + (kk - m) * lpq;
         if (k < 0.0 || k > n) continue;
         if (us >= 0.07 && v <= vr) return k;
-        /* log(v alpha / (a / us^2 + b)) */
-        const double us2 = us * us;
-        v = plom_log_tab(v * alpha * us2 / plom_fma(b, us2, a), ns->ltab);
-        if (v <= h - plom_lfact(k, ns->ltab) - plom_lfact(n - k, ns->ltab) + (k - m) * lpq) return k;
+        if (lhs <= rhs) return k;
     }
     return m;
 }
@@ -210,8 +232,10 @@ PLOM_HD_CALL double plom_binom_draw(plom_nstream64 *ns, double n, double p)
 {
     if (!(n > 0.0) || !(p > 0.0)) return 0.0;
     if (p >= 1.0) return n;
-    if (p > 0.5) return n - plom_binom_half(ns, n, 1.0 - p);
-    return plom_binom_half(ns, n, p);
+    /* one inlined copy of the sampler for both halves */
+    const int hi = p > 0.5;
+    const double x = plom_binom_half(ns, n, hi ? 1.0 - p : p);
+    return hi ? n - x : x;
 }
 
 /* n[0..K) ~ Multinomial(N, p[0..K)), p summing to 1: the sequential binomials of

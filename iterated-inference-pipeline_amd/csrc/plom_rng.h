@@ -114,9 +114,11 @@ static const double plom_icdf_tab[PLOM_ICDF_TAB_SIZE] = PLOM_ICDF_TAB_VALUES;
 #endif
 
 /* A particle's stream over one interval: its xoshiro128** state, the inverse-CDF table
- * (plom_icdf_tab or a copy of it) and the log table of the psr samplers (the Box-Muller
- * table plom_bm_tab or a copy of it; plom_nstream64_init sets plom_bm_tab). */
-typedef struct { plom_xoshiro128 r; const double *tab; const double *ltab; } plom_nstream64;
+ * (plom_icdf_tab or a copy of it) and the tables of the psr samplers: the log table (the
+ * Box-Muller table plom_bm_tab or a copy of it) and log k! for k < 30 (plom_lfact_tab of
+ * plom_psr.h or a copy of it); plom_nstream64_init sets the first and leaves the second
+ * null, which plom_psr.h's users set. */
+typedef struct { plom_xoshiro128 r; const double *tab; const double *ltab; const double *lftab; } plom_nstream64;
 
 PLOM_HD void plom_nstream64_init(plom_nstream64 *ns, uint32_t seed, uint32_t stream, uint32_t n, uint32_t j,
                                  const double *tab)
@@ -126,6 +128,7 @@ PLOM_HD void plom_nstream64_init(plom_nstream64 *ns, uint32_t seed, uint32_t str
     plom_xoshiro_seed(&ns->r, w);
     ns->tab = tab;
     ns->ltab = plom_bm_tab;
+    ns->lftab = 0;
 }
 
 /* log x for a positive, normal, finite x by the log table: x = 2^e m, table entry k from

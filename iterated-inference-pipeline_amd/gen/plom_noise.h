@@ -420,7 +420,7 @@ PLOM_HD void plom_noise_interval_psr_full_f64(double *X, const double *pv, plom_
         const double r3_1 = C7;
         const double sd0_0 = P4;
         const double v0_0 = sd0_0 * sd0_0;
-        const double xi0_0 = (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt;
+        const double xi0_0 = (r1_0 > 0.0) ? (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt : 1.0;
         {
             double p[3], cnt[3];
             const double sum = (r1_0 * xi0_0) * dt + r3_0 * dt;
@@ -463,7 +463,7 @@ PLOM_HD void plom_noise_interval_psr_full_f64(double *X, const double *pv, plom_
         }
         const double sd0_1 = P5;
         const double v0_1 = sd0_1 * sd0_1;
-        const double xi0_1 = (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt;
+        const double xi0_1 = (r1_1 > 0.0) ? (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt : 1.0;
         {
             double p[3], cnt[3];
             const double sum = (r1_1 * xi0_1) * dt + r3_1 * dt;
@@ -1037,7 +1037,7 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         const double r3_1 = C7;
         const double sd0_0 = P4;
         const double v0_0 = sd0_0 * sd0_0;
-        const double xi0_0 = (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt;
+        const double xi0_0 = (r1_0 > 0.0) ? (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt : 1.0;
         {
             double p[3], cnt[3];
             const double sum = (r1_0 * xi0_0) * dt + r3_0 * dt;
@@ -1080,7 +1080,7 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         }
         const double sd0_1 = P5;
         const double v0_1 = sd0_1 * sd0_1;
-        const double xi0_1 = (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt;
+        const double xi0_1 = (r1_1 > 0.0) ? (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt : 1.0;
         {
             double p[3], cnt[3];
             const double sum = (r1_1 * xi0_1) * dt + r3_1 * dt;

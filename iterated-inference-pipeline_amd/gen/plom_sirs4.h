@@ -676,7 +676,7 @@ PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double sd0_0 = P16;
         const double v0_0 = sd0_0 * sd0_0;
-        const double xi0_0 = (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt;
+        const double xi0_0 = (r0_0 > 0.0) ? (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt : 1.0;
         {
             const double sum = (r0_0 * xi0_0) * dt;
             const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_0 * xi0_0) * dt) / sum) : 0.0;
@@ -697,7 +697,7 @@ PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_
         }
         const double sd0_1 = P17;
         const double v0_1 = sd0_1 * sd0_1;
-        const double xi0_1 = (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt;
+        const double xi0_1 = (r0_1 > 0.0) ? (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt : 1.0;
         {
             const double sum = (r0_1 * xi0_1) * dt;
             const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_1 * xi0_1) * dt) / sum) : 0.0;
@@ -718,7 +718,7 @@ PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_
         }
         const double sd0_2 = P18;
         const double v0_2 = sd0_2 * sd0_2;
-        const double xi0_2 = (v0_2 * plom_gamma_draw(ns, dt / v0_2)) / dt;
+        const double xi0_2 = (r0_2 > 0.0) ? (v0_2 * plom_gamma_draw(ns, dt / v0_2)) / dt : 1.0;
         {
             const double sum = (r0_2 * xi0_2) * dt;
             const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_2 * xi0_2) * dt) / sum) : 0.0;
@@ -739,7 +739,7 @@ PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_
         }
         const double sd0_3 = P19;
         const double v0_3 = sd0_3 * sd0_3;
-        const double xi0_3 = (v0_3 * plom_gamma_draw(ns, dt / v0_3)) / dt;
+        const double xi0_3 = (r0_3 > 0.0) ? (v0_3 * plom_gamma_draw(ns, dt / v0_3)) / dt : 1.0;
         {
             const double sum = (r0_3 * xi0_3) * dt;
             const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_3 * xi0_3) * dt) / sum) : 0.0;
@@ -1632,7 +1632,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double sd0_0 = P16;
         const double v0_0 = sd0_0 * sd0_0;
-        const double xi0_0 = (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt;
+        const double xi0_0 = (r0_0 > 0.0) ? (v0_0 * plom_gamma_draw(ns, dt / v0_0)) / dt : 1.0;
         {
             const double sum = (r0_0 * xi0_0) * dt;
             const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_0 * xi0_0) * dt) / sum) : 0.0;
@@ -1653,7 +1653,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         }
         const double sd0_1 = P17;
         const double v0_1 = sd0_1 * sd0_1;
-        const double xi0_1 = (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt;
+        const double xi0_1 = (r0_1 > 0.0) ? (v0_1 * plom_gamma_draw(ns, dt / v0_1)) / dt : 1.0;
         {
             const double sum = (r0_1 * xi0_1) * dt;
             const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_1 * xi0_1) * dt) / sum) : 0.0;
@@ -1674,7 +1674,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         }
         const double sd0_2 = P18;
         const double v0_2 = sd0_2 * sd0_2;
-        const double xi0_2 = (v0_2 * plom_gamma_draw(ns, dt / v0_2)) / dt;
+        const double xi0_2 = (r0_2 > 0.0) ? (v0_2 * plom_gamma_draw(ns, dt / v0_2)) / dt : 1.0;
         {
             const double sum = (r0_2 * xi0_2) * dt;
             const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_2 * xi0_2) * dt) / sum) : 0.0;
@@ -1695,7 +1695,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         }
         const double sd0_3 = P19;
         const double v0_3 = sd0_3 * sd0_3;
-        const double xi0_3 = (v0_3 * plom_gamma_draw(ns, dt / v0_3)) / dt;
+        const double xi0_3 = (r0_3 > 0.0) ? (v0_3 * plom_gamma_draw(ns, dt / v0_3)) / dt : 1.0;
         {
             const double sum = (r0_3 * xi0_3) * dt;
             const double p0 = (sum > 0.0) ? (1.0 - plom_exp(-sum)) * (((r0_3 * xi0_3) * dt) / sum) : 0.0;

@@ -115,6 +115,7 @@ void oracle_psr_draws(int kind, double a, double b, uint32_t seed, uint32_t stre
 {
     plom_nstream64 ns;
     plom_nstream64_init(&ns, seed, stream, n, j, plom_icdf_tab);
+    plom_psr_tables(&ns, plom_bm_tab, plom_lfact_tab);
     if (kind == 5) {
         plom_multinomial(&ns, count, a, p, out);
         return;
@@ -126,7 +127,7 @@ void oracle_psr_draws(int kind, double a, double b, uint32_t seed, uint32_t stre
         case 2: out[i] = plom_poisson_draw(&ns, a); break;
         case 3: out[i] = plom_binom_draw(&ns, a, b); break;
         case 6: out[i] = plom_unif32(&ns); break;
-        default: out[i] = plom_lfact(a + (double)i, plom_bm_tab); break;
+        default: out[i] = plom_lfact(a + (double)i, plom_bm_tab, plom_lfact_tab); break;
         }
     }
 }
@@ -313,6 +314,7 @@ static void propagate_range(prop_job *p)
         const double *pv = p->pv + (size_t)j * p->pv_stride;
         plom_nstream64 ns;
         plom_nstream64_init(&ns, p->seed, p->stream, (uint32_t)p->n, (uint32_t)j, plom_icdf_tab);
+        plom_psr_tables(&ns, plom_bm_tab, plom_lfact_tab);
         M->interval(x, pv, &ns, p->times[p->n], p->times[p->n + 1], p->dt, p->sqdt, p->noises_off, cv_steps(p->cv, p->n),
                     p->cv->cs, p->eps_abs, p->eps_rel, p->dm, cv_splines(p->cv, p->noises_off));
         if (p->filter && p->nonan) {
@@ -715,6 +717,7 @@ int oracle_smc(int model, int J, int N, uint32_t noises_off, uint32_t flags, uin
         for (int n = 0; n < N; n++) {
             plom_nstream64 ns;
             plom_nstream64_init(&ns, seed, stream, (uint32_t)n, (uint32_t)lin_slots[n], plom_icdf_tab);
+            plom_psr_tables(&ns, plom_bm_tab, plom_lfact_tab);
             M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off, cv_steps(&cv, n), cv.cs, eps_abs,
                         eps_rel, dm, cv_splines(&cv, noises_off));
             memcpy(lin_state + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
@@ -963,6 +966,7 @@ int oracle_trajectory(int model, int N, uint32_t noises_off, uint32_t seed, uint
     for (int n = 0; n < N; n++) {
         plom_nstream64 ns;
         plom_nstream64_init(&ns, seed, stream, (uint32_t)n, j, plom_icdf_tab);
+        plom_psr_tables(&ns, plom_bm_tab, plom_lfact_tab);
         M->interval(x, pv, &ns, times[n], times[n + 1], dt, plom_sqrt(dt), noises_off, cv_steps(&cv, n), cv.cs, eps_abs,
                     eps_rel, drift_view(&da), cv_splines(&cv, noises_off));
         memcpy(out + (size_t)(n + 1) * S, x, (size_t)S * sizeof(double));
@@ -1069,6 +1073,7 @@ void oracle_normals(uint32_t seed, uint32_t stream, uint32_t n0, uint32_t j, int
 {
     plom_nstream64 ns;
     plom_nstream64_init(&ns, seed, stream, n0, j, plom_icdf_tab);
+    plom_psr_tables(&ns, plom_bm_tab, plom_lfact_tab);
     for (int i = 0; i + 1 < n; i += 2) plom_normal_pair64(&ns, &out[i], &out[i + 1]);
     if (n & 1) {
         double spare;

@@ -20,6 +20,8 @@ Everything is unrolled over populations and states so that, on the GPU, a partic
 whole state lives in registers.  Arithmetic is printed in the reference's term order
 (Ccoder.py:449-505): X + (drift)*dt + demographic terms + environmental terms.
 """
+import re
+
 from . import expr as E
 from .model import SCHOOL_REC
 
@@ -45,6 +47,8 @@ class Emitter:
         self.time_funcs = None
         self.drift_mode = "pv"
         self.used_drift = set()
+        self.rc_div = False
+        self.used_rc = set()
 
     # ---------------------------------------------------------------- symbols
     def xidx(self, state, cac):
@@ -98,7 +102,20 @@ class Emitter:
                 return "(" + " + ".join("%s * X[%d]" % (self.par(base + k), self.xidx(s, k)) for k in range(m.N_CAC)) + ")"
             return None
 
-        return E.CEmitter(resolve, real, call)
+        def div(node, a, b):
+            # a quotient by a covariate (a population size) inside a substep loop: from the
+            # covariate's reciprocal, computed once per interval when the covariates are
+            # constant (cs == 0) and once per substep otherwise (plom_div_rc)
+            if real != "double" or not self.rc_div or not isinstance(node.b, E.Sym):
+                return None
+            mt = re.match(r"^C(\d+)$", b)
+            if not mt:
+                return None
+            i = int(mt.group(1))
+            self.used_rc.add(i)
+            return "plom_div_rc(%s, C%d, RC%d)" % (a, i, i)
+
+        return E.CEmitter(resolve, real, call, div)
 
     def par(self, k):
         """Name of parameter pv[k] inside an interval function (bound once, uniform)."""
@@ -188,6 +205,16 @@ class Emitter:
                 L.append("        const double C%d = plom_cov_ld(cvk + %d);" % (i, i))
             else:
                 L.append("        const float C%d = (float)plom_cov_ld(cvk + %d);" % (i, i))
+        # the reciprocals of the covariates that divide: from the caller's substep table
+        # after the time-only functions (computed there once per substep for the block)
+        lst = self.tf_list.get("double", []) if getattr(self, "tf_list", None) else []
+        rcl = getattr(self, "rc_list", [])
+        for i in sorted(self.used_rc):
+            if i in rcl:
+                L.append("        const double RC%d = tft ? tft[k * %d + %d] : plom_recip_uniform(C%d);"
+                         % (i, len(lst) + len(rcl), len(lst) + rcl.index(i), i))
+            else:
+                L.append("        const double RC%d = plom_recip_uniform(C%d);" % (i, i))
         return L
 
     # ---------------------------------------------------------------- interval
@@ -240,6 +267,8 @@ class Emitter:
         self.used_pars = set()
         self.used_cov = set()
         self.used_drift = set()
+        self.used_rc = set()
+        self.rc_div = real == "double"
         self.drift_mode = "state"
         bind_at = len(L)
         for u in range(m.N_TS_INC_UNIQUE):
@@ -368,6 +397,8 @@ class Emitter:
         binds = [("    const %s P%d = lp[%d];" % (Rt, k, k)) if pp and k < npp else
                  ("    const %s P%d = %s(pv[%d]);" % (Rt, k, uni, k)) for k in sorted(self.used_pars)]
         L[bind_at:bind_at] = binds
+        self.last_used_rc = set(self.used_rc)
+        self.rc_div = False
         return "\n".join(L)
 
     # ---------------------------------------------------------------- PSR interval
@@ -396,6 +427,8 @@ class Emitter:
         self.used_pars = set()
         self.used_cov = set()
         self.used_drift = set()
+        self.used_rc = set()
+        self.rc_div = True
         self.drift_mode = "state"
         bind_at = len(L)
         for u in range(m.N_TS_INC_UNIQUE):
@@ -532,6 +565,8 @@ class Emitter:
         binds = [("    const double P%d = lp[%d];" % (k, k)) if pp and k < npp else
                  ("    const double P%d = PLOM_UNIFORM64(pv[%d]);" % (k, k)) for k in sorted(self.used_pars)]
         L[bind_at:bind_at] = binds
+        self.last_used_rc = set(self.used_rc)
+        self.rc_div = False
         return "\n".join(L)
 
     # ---------------------------------------------------------------- ODE
@@ -797,29 +832,46 @@ class Emitter:
         same function of the same t), else evaluated here."""
         Rt = "double" if real == "double" else "float"
         lst = self.tf_list.get(real, []) if getattr(self, "tf_list", None) else []
+        ntf = len(lst) + len(getattr(self, "rc_list", []))
         out = []
         for text, name in sorted(self.time_funcs.items(), key=lambda kv: int(kv[1][2:])):
             if text in lst:
-                out.append("        const %s %s = tft ? tft[k * %d + %d] : %s;" % (Rt, name, len(lst), lst.index(text), text))
+                out.append("        const %s %s = tft ? tft[k * %d + %d] : %s;" % (Rt, name, ntf, lst.index(text), text))
             else:
                 out.append("        const %s %s = %s;" % (Rt, name, text))
         return out
 
     def collect_time_funcs(self):
-        """The model's list of time-only functions, per precision, from its full SDE step."""
+        """The model's list of time-only functions, per precision, from its full SDE step,
+        and the covariates that its FP64 steps divide by (their reciprocals follow the time
+        functions in a substep's table row)."""
         self.tf_list = {}
+        self.rc_list = []
+        rc = set()
         for real in ("double", "float"):
             self.interval("full", True, True, real)
             self.tf_list[real] = [t for t, _ in sorted(self.last_time_funcs.items(), key=lambda kv: int(kv[1][2:]))]
+            if real == "double":
+                rc |= self.last_used_rc
+        self.interval_psr(True)
+        rc |= self.last_used_rc
+        self.rc_list = sorted(rc)
         if len(self.tf_list["double"]) != len(self.tf_list["float"]):
             raise ValueError("time functions differ between precisions")
 
     def time_funcs_fn(self, real):
+        """One row of the substep table: the time-only functions at tr, then the
+        reciprocals of the dividing covariates of the substep's covariate row cvk (the FP32
+        steps do not read them)."""
         Rt, tag = ("double", "f64") if real == "double" else ("float", "f32")
         qual = "PLOM_HD" if real == "double" else "static __device__ inline __attribute__((always_inline))"
-        L = ["%s void %s_time_funcs_%s(%s tr, %s *out)" % (qual, self.p, tag, Rt, Rt), "{", "    (void)tr; (void)out;"]
+        L = ["%s void %s_time_funcs_%s(%s tr, const double *cvk, %s *out)" % (qual, self.p, tag, Rt, Rt), "{",
+             "    (void)tr; (void)cvk; (void)out;"]
         for i, text in enumerate(self.tf_list[real]):
             L.append("    out[%d] = %s;" % (i, text))
+        n = len(self.tf_list[real])
+        for r, i in enumerate(self.rc_list):
+            L.append("    out[%d] = (%s)plom_recip_uniform(plom_cov_ld(cvk + %d));" % (n + r, Rt, i))
         L.append("}")
         return "\n".join(L)
 
@@ -915,7 +967,8 @@ class Emitter:
         out.append("    static constexpr int S = %d, N_TS = %d, INC_OFFSET = %d, N_INC = %d, PV = %d, N_CAC = %d, N_HAT = %d, NPP = %d, FIX_OFF = %d, N_FIX = %d, N_DRIFT = %d;"
                    % (m.size_proj, m.N_TS, m.inc_offset, m.N_TS_INC_UNIQUE, m.layout["size"], m.N_CAC, self.n_hat(),
                       m.layout["npp"], m.layout["fix_off"], m.layout["n_fix"], m.N_DRIFT))
-        out.append("    static constexpr int NTF = %d;  /* time-only functions of a substep */" % len(self.tf_list["double"]))
+        out.append("    static constexpr int NTF = %d;  /* time-only functions and covariate reciprocals of a substep */"
+                   % (len(self.tf_list["double"]) + len(self.rc_list)))
         for tag, Rt, stream in (("f64", "double", "plom_nstream64"), ("f32", "float", "plom_nstream32")):
             out.append("    template <int V> static __device__ inline __attribute__((always_inline)) void interval(%s *X, const %s *pv, %s *ns, double t0, double t1, double dt, %s sqdt, const double *cv, int cs, const plom_drift_view *dm, const %s *tft)"
                        % (Rt, Rt, stream, Rt, Rt))
@@ -937,7 +990,7 @@ class Emitter:
                 out.append("        else if constexpr (V == 4) %s_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);" % self.p)
                 out.append("        else if constexpr (V == 5) %s_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);" % self.p)
             out.append("    }")
-            out.append("    static __device__ inline __attribute__((always_inline)) void time_funcs(%s tr, %s *out) { %s_time_funcs_%s(tr, out); }"
+            out.append("    static __device__ inline __attribute__((always_inline)) void time_funcs(%s tr, const double *cvk, %s *out) { %s_time_funcs_%s(tr, cvk, out); }"
                        % (Rt, Rt, self.p, tag))
             out.append("    static __device__ inline __attribute__((always_inline)) void proj2obs(const %s *X, const double *pvd, const double *cvd, double *obs) { %s_proj2obs_%s(X, pvd, cvd, obs); }"
                        % (Rt, self.p, tag))

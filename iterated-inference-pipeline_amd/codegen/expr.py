@@ -218,14 +218,16 @@ class CEmitter:
     """Print a tree as a C expression for one precision.
 
     ``resolve(name)`` returns the C text of a symbol; ``call(fn, arg_texts, node)``
-    returns the C text of a function call (or None to use the defaults below).
-    ``real`` is 'double' or 'float'.
+    returns the C text of a function call (or None to use the defaults below);
+    ``div(node, a_text, b_text)`` the C text of a division a / b in double (or None for
+    the plain quotient).  ``real`` is 'double' or 'float'.
     """
 
-    def __init__(self, resolve, real="double", call=None):
+    def __init__(self, resolve, real="double", call=None, div=None):
         self.resolve = resolve
         self.real = real
         self.call_hook = call
+        self.div_hook = div
         self.sfx = "" if real == "double" else "f"
 
     def lit(self, text):
@@ -249,6 +251,10 @@ class CEmitter:
                 return "plom_pow%s(%s, %s)" % (self.sfx, self.emit(node.a), self.emit(node.b))
             if node.op == "/" and self.real == "float":
                 return "plom_fdivf(%s, %s)" % (self.emit(node.a), self.emit(node.b))
+            if node.op == "/" and self.div_hook:
+                r = self.div_hook(node, self.emit(node.a), self.emit(node.b))
+                if r is not None:
+                    return r
             return "(%s %s %s)" % (self.emit(node.a), node.op, self.emit(node.b))
         if isinstance(node, Call):
             args = [self.emit(a) for a in node.args]

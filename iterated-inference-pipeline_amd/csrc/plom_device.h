@@ -347,7 +347,7 @@ __global__ __launch_bounds__(TILE * TPB) void k_propagate(plom_dfilter *filters,
             for (int k = tid; k < K; k += TILE) {
                 double t = t0;
                 for (int r = 0; r < k; r++) t += f->dt;
-                M::time_funcs((R)t, s_tf + k * M::NTF);
+                M::time_funcs((R)t, cov_steps(f, n) + (size_t)k * (size_t)f->cov_cs, s_tf + k * M::NTF);
             }
         }
         if constexpr (TPB == 1) {

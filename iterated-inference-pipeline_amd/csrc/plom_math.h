@@ -96,6 +96,26 @@ PLOM_HD double plom_sqrt_rate(double x)
     return PLOM_NAN;
 }
 #endif
+/* 1 / c (IEEE), for a divisor shared by many quotients */
+PLOM_HD double plom_recip_uniform(double c) { return 1.0 / c; }
+
+/* x / c from rc = RN(1/c), in three operations instead of a division: q = RN(x rc), the
+ * residual e = RN(x - q c) (one fma, exact when q is within an ulp of x / c), and the
+ * corrected quotient RN(q + e rc) (Markstein's correction step), with the sign of q, which
+ * is the sign of x / c also when the quotient is zero.  For finite x and c with no
+ * underflow the result is the IEEE quotient x / c except in rare cases close to a rounding
+ * midpoint, where it can be one ulp away (the correction step is exact only for a faithful
+ * q; see Brisebarre, Muller and Raina, "Accelerating correctly rounded floating-point
+ * division when the divisor is known in advance", IEEE Trans. Computers 53(8), 2004).
+ * x = +-inf gives NaN.  The same operations on the device and the host. */
+PLOM_HD double plom_div_rc(double x, double c, double rc)
+{
+    const double q = x * rc;
+    const double e = plom_fma(-q, c, x);
+    const double r = plom_fma(e, rc, q);
+    return plom_u2d((plom_d2u(r) & 0x7fffffffffffffffULL) | (plom_d2u(q) & 0x8000000000000000ULL));
+}
+
 PLOM_HD double plom_fabs(double x) { return plom_u2d(plom_d2u(x) & 0x7fffffffffffffffULL); }
 PLOM_HD int plom_isnan(double x) { return x != x; }
 /* 2^k for k in [-1022, 1023] */

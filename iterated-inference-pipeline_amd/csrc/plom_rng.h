@@ -77,6 +77,7 @@ typedef struct { uint32_t s0, s1, s2, s3; } plom_xoshiro128;
 
 PLOM_HD uint32_t plom_rotl32(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
 
+/* xoshiro128** (Blackman and Vigna): the output rotl(5 s1, 7) 9, then the state update */
 PLOM_HD uint32_t plom_xoshiro_next(plom_xoshiro128 *r)
 {
     uint32_t result = plom_rotl32(r->s1 * 5u, 7) * 9u;
@@ -204,14 +205,17 @@ PLOM_HD void plom_bm_pair(uint32_t x, uint32_t y, const double *tab, double *z0,
 #define PLOM_ICDF_T_SCALE ((double)(2 << PLOM_ICDF_SUB_BITS))
 PLOM_HD double plom_icdf_normal(uint32_t w, const double *tab)
 {
-    /* (v + 1/2) 2^-32 = (2v + 1) 2^-33 = (w | 1) 2^-33 */
-    const double u = (double)(w | 1u) * 0x1p-33;
-    const uint64_t b = plom_d2u(u);
+    /* (v + 1/2) 2^-32 = (2v + 1) 2^-33 = (w | 1) 2^-33 = u.  The double v1 = w | 1 (exact)
+     * has u's fraction bits and a biased exponent 33 higher, so the segment is read from v1
+     * directly and u itself is never formed. */
+    const double v1 = (double)(w | 1u);
+    const uint64_t b = plom_d2u(v1);
     const uint32_t hi = (uint32_t)(b >> 32);
-    /* the segment (E + 33) 16 + s, taken modulo the table size: it is in [0, NSEG) anyway,
-     * and the mask lets the compiler fold the coefficients' offsets into the LDS loads */
+    /* the segment (E + 33) 16 + s of u, from v1's exponent E + 33 + 1023, taken modulo the
+     * table size: it is in [0, NSEG) anyway, and the mask lets the compiler fold the
+     * coefficients' offsets into the LDS loads */
     const uint32_t idx = ((hi >> (20 - PLOM_ICDF_SUB_BITS)) + (uint32_t)PLOM_ICDF_NSEG -
-                          (((uint32_t)PLOM_ICDF_EB_MIN << PLOM_ICDF_SUB_BITS) & (PLOM_ICDF_NSEG - 1u))) &
+                          ((((uint32_t)PLOM_ICDF_EB_MIN + 33u) << PLOM_ICDF_SUB_BITS) & (PLOM_ICDF_NSEG - 1u))) &
                          (PLOM_ICDF_NSEG - 1u);
     const uint32_t keep = (1u << (20 - PLOM_ICDF_SUB_BITS)) - 1u;
     const double m = plom_u2d(((uint64_t)((hi & keep) | 0x3ff00000u) << 32) | (b & 0xffffffffULL));
@@ -309,10 +313,12 @@ plom_nstream32_init(plom_nstream32 *ns, uint32_t seed, uint32_t stream, uint32_t
 #define PLOM_ICDF_EBF_MIN (PLOM_ICDF_EB_MIN - 1023 + 127)   /* float's biased exponent of 2^-33 */
 __device__ inline __attribute__((always_inline)) float plom_icdf_normal32(uint32_t w, const float *tab)
 {
-    const float u = (float)((w >> 8) | 1u) * 0x1p-25f;
-    const uint32_t b = __float_as_uint(u);
+    /* the float v1 = (w >> 8) | 1 (exact) has u's fraction bits and a biased exponent 25
+     * higher: the segment is read from v1 directly */
+    const float v1 = (float)((w >> 8) | 1u);
+    const uint32_t b = __float_as_uint(v1);
     const uint32_t idx = ((b >> (23 - PLOM_ICDF_SUB_BITS)) + (uint32_t)PLOM_ICDF_NSEG -
-                          (((uint32_t)PLOM_ICDF_EBF_MIN << PLOM_ICDF_SUB_BITS) & (PLOM_ICDF_NSEG - 1u))) &
+                          ((((uint32_t)PLOM_ICDF_EBF_MIN + 25u) << PLOM_ICDF_SUB_BITS) & (PLOM_ICDF_NSEG - 1u))) &
                          (PLOM_ICDF_NSEG - 1u);
     const float m = __uint_as_float((b & ((1u << (23 - PLOM_ICDF_SUB_BITS)) - 1u)) | 0x3f800000u);
     const float t = __builtin_fmaf(m - 1.0f, (float)PLOM_ICDF_T_SCALE, -1.0f);

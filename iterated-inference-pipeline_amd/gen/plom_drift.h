@@ -19,9 +19,11 @@
 #define PLOM_DRIFT_N_DRIFT 1
 #define PLOM_DRIFT_DRIFT_OFFSET 4
 
-PLOM_HD void plom_drift_time_funcs_f64(double tr, double *out)
+PLOM_HD void plom_drift_time_funcs_f64(double tr, const double *cvk, double *out)
 {
-    (void)tr; (void)out;
+    (void)tr; (void)cvk; (void)out;
+    out[0] = (double)plom_recip_uniform(plom_cov_ld(cvk + 0));
+    out[1] = (double)plom_recip_uniform(plom_cov_ld(cvk + 1));
 }
 
 PLOM_HD void plom_drift_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
@@ -45,6 +47,8 @@ PLOM_HD void plom_drift_interval_full_f64(double *X, const double *pv, plom_nstr
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double D0_0, D0_1;
         if (dm) {
             D0_0 = plom_mif_natural(dm->tr[0], dm->sc[0], dm->dur[0], (double)X[4], dm->min[0], dm->max[0], dm->mult[0]);
@@ -56,7 +60,7 @@ PLOM_HD void plom_drift_interval_full_f64(double *X, const double *pv, plom_nstr
         double s6 = 0.0;
         double s7 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((D0_0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(D0_0, C0, RC0) * P2) * X[2]);
         const double r2_0 = P2;
         const double r3_0 = C6;
         double za0, zb0;
@@ -76,7 +80,7 @@ PLOM_HD void plom_drift_interval_full_f64(double *X, const double *pv, plom_nstr
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((D0_1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(D0_1, C1, RC1) * P3) * X[3]);
         const double r2_1 = P3;
         const double r3_1 = C7;
         const double d0_1 = sqdt * zb2;
@@ -144,6 +148,8 @@ PLOM_HD void plom_drift_interval_no_dem_sto_f64(double *X, const double *pv, plo
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double D0_0, D0_1;
         if (dm) {
             D0_0 = plom_mif_natural(dm->tr[0], dm->sc[0], dm->dur[0], (double)X[4], dm->min[0], dm->max[0], dm->mult[0]);
@@ -155,7 +161,7 @@ PLOM_HD void plom_drift_interval_no_dem_sto_f64(double *X, const double *pv, plo
         double s6 = 0.0;
         double s7 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((D0_0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(D0_0, C0, RC0) * P2) * X[2]);
         const double r2_0 = P2;
         const double r3_0 = C6;
         const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr;
@@ -164,7 +170,7 @@ PLOM_HD void plom_drift_interval_no_dem_sto_f64(double *X, const double *pv, plo
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((D0_1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(D0_1, C1, RC1) * P3) * X[3]);
         const double r2_1 = P3;
         const double r3_1 = C7;
         const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr;
@@ -223,6 +229,8 @@ PLOM_HD void plom_drift_interval_no_env_sto_f64(double *X, const double *pv, plo
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double D0_0, D0_1;
         if (dm) {
             D0_0 = plom_mif_natural(dm->tr[0], dm->sc[0], dm->dur[0], (double)X[4], dm->min[0], dm->max[0], dm->mult[0]);
@@ -234,7 +242,7 @@ PLOM_HD void plom_drift_interval_no_env_sto_f64(double *X, const double *pv, plo
         double s6 = 0.0;
         double s7 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((D0_0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(D0_0, C0, RC0) * P2) * X[2]);
         const double r2_0 = P2;
         const double r3_0 = C6;
         double za0, zb0;
@@ -254,7 +262,7 @@ PLOM_HD void plom_drift_interval_no_env_sto_f64(double *X, const double *pv, plo
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((D0_1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(D0_1, C1, RC1) * P3) * X[3]);
         const double r2_1 = P3;
         const double r3_1 = C7;
         const double d0_1 = sqdt * zb2;
@@ -322,6 +330,8 @@ PLOM_HD void plom_drift_interval_no_dem_sto_no_env_sto_f64(double *X, const doub
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double D0_0, D0_1;
         if (dm) {
             D0_0 = plom_mif_natural(dm->tr[0], dm->sc[0], dm->dur[0], (double)X[4], dm->min[0], dm->max[0], dm->mult[0]);
@@ -333,7 +343,7 @@ PLOM_HD void plom_drift_interval_no_dem_sto_no_env_sto_f64(double *X, const doub
         double s6 = 0.0;
         double s7 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((D0_0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(D0_0, C0, RC0) * P2) * X[2]);
         const double r2_0 = P2;
         const double r3_0 = C6;
         const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr;
@@ -342,7 +352,7 @@ PLOM_HD void plom_drift_interval_no_dem_sto_no_env_sto_f64(double *X, const doub
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((D0_1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(D0_1, C1, RC1) * P3) * X[3]);
         const double r2_1 = P3;
         const double r3_1 = C7;
         const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr;
@@ -499,6 +509,8 @@ PLOM_HD void plom_drift_interval_psr_full_f64(double *X, const double *pv, plom_
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double D0_0, D0_1;
         if (dm) {
             D0_0 = plom_mif_natural(dm->tr[0], dm->sc[0], dm->dur[0], (double)X[4], dm->min[0], dm->max[0], dm->mult[0]);
@@ -523,11 +535,11 @@ PLOM_HD void plom_drift_interval_psr_full_f64(double *X, const double *pv, plom_
         double stay3 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n4_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n4_1;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((D0_0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(D0_0, C0, RC0) * P2) * X[2]);
         const double r2_0 = P2;
         const double r3_0 = C6;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((D0_1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(D0_1, C1, RC1) * P3) * X[3]);
         const double r2_1 = P3;
         const double r3_1 = C7;
         {
@@ -661,6 +673,8 @@ PLOM_HD void plom_drift_interval_psr_no_env_sto_f64(double *X, const double *pv,
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double D0_0, D0_1;
         if (dm) {
             D0_0 = plom_mif_natural(dm->tr[0], dm->sc[0], dm->dur[0], (double)X[4], dm->min[0], dm->max[0], dm->mult[0]);
@@ -685,11 +699,11 @@ PLOM_HD void plom_drift_interval_psr_no_env_sto_f64(double *X, const double *pv,
         double stay3 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n4_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n4_1;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((D0_0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(D0_0, C0, RC0) * P2) * X[2]);
         const double r2_0 = P2;
         const double r3_0 = C6;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((D0_1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(D0_1, C1, RC1) * P3) * X[3]);
         const double r2_1 = P3;
         const double r3_1 = C7;
         {
@@ -876,9 +890,11 @@ PLOM_HD void plom_drift_interval_f64(double *X, const double *pv, plom_nstream64
 }
 
 #if defined(__HIPCC__)
-static __device__ inline __attribute__((always_inline)) void plom_drift_time_funcs_f32(float tr, float *out)
+static __device__ inline __attribute__((always_inline)) void plom_drift_time_funcs_f32(float tr, const double *cvk, float *out)
 {
-    (void)tr; (void)out;
+    (void)tr; (void)cvk; (void)out;
+    out[0] = (float)plom_recip_uniform(plom_cov_ld(cvk + 0));
+    out[1] = (float)plom_recip_uniform(plom_cov_ld(cvk + 1));
 }
 
 static __device__ inline __attribute__((always_inline)) void plom_drift_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
@@ -1270,6 +1286,8 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double D0_0, D0_1;
         if (dm) {
             D0_0 = plom_mif_natural(dm->tr[0], dm->sc[0], dm->dur[0], (double)X[4], dm->min[0], dm->max[0], dm->mult[0]);
@@ -1294,11 +1312,11 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         double stay3 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n4_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n4_1;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((D0_0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(D0_0, C0, RC0) * P2) * X[2]);
         const double r2_0 = P2;
         const double r3_0 = C6;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((D0_1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(D0_1, C1, RC1) * P3) * X[3]);
         const double r2_1 = P3;
         const double r3_1 = C7;
         {
@@ -1446,6 +1464,8 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double D0_0, D0_1;
         if (dm) {
             D0_0 = plom_mif_natural(dm->tr[0], dm->sc[0], dm->dur[0], (double)X[4], dm->min[0], dm->max[0], dm->mult[0]);
@@ -1470,11 +1490,11 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         double stay3 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n4_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n4_1;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((D0_0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(D0_0, C0, RC0) * P2) * X[2]);
         const double r2_0 = P2;
         const double r3_0 = C6;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((D0_1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(D0_1, C1, RC1) * P3) * X[3]);
         const double r2_1 = P3;
         const double r3_1 = C7;
         {
@@ -1677,6 +1697,8 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double D0_0, D0_1;
         if (dm) {
             D0_0 = plom_mif_natural(dm->tr[0], dm->sc[0], dm->dur[0], (double)X[4], dm->min[0], dm->max[0], dm->mult[0]);
@@ -1688,7 +1710,7 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         double s6 = 0.0;
         double s7 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((D0_0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(D0_0, C0, RC0) * P2) * X[2]);
         const double r2_0 = P2;
         const double r3_0 = C6;
         double za0, zb0;
@@ -1708,7 +1730,7 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((D0_1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(D0_1, C1, RC1) * P3) * X[3]);
         const double r2_1 = P3;
         const double r3_1 = C7;
         const double d0_1 = sqdt * zb2;
@@ -1790,6 +1812,8 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double D0_0, D0_1;
         if (dm) {
             D0_0 = plom_mif_natural(dm->tr[0], dm->sc[0], dm->dur[0], (double)X[4], dm->min[0], dm->max[0], dm->mult[0]);
@@ -1801,7 +1825,7 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         double s6 = 0.0;
         double s7 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((D0_0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(D0_0, C0, RC0) * P2) * X[2]);
         const double r2_0 = P2;
         const double r3_0 = C6;
         const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr;
@@ -1810,7 +1834,7 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((D0_1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(D0_1, C1, RC1) * P3) * X[3]);
         const double r2_1 = P3;
         const double r3_1 = C7;
         const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr;
@@ -1883,6 +1907,8 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double D0_0, D0_1;
         if (dm) {
             D0_0 = plom_mif_natural(dm->tr[0], dm->sc[0], dm->dur[0], (double)X[4], dm->min[0], dm->max[0], dm->mult[0]);
@@ -1894,7 +1920,7 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         double s6 = 0.0;
         double s7 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((D0_0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(D0_0, C0, RC0) * P2) * X[2]);
         const double r2_0 = P2;
         const double r3_0 = C6;
         double za0, zb0;
@@ -1914,7 +1940,7 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((D0_1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(D0_1, C1, RC1) * P3) * X[3]);
         const double r2_1 = P3;
         const double r3_1 = C7;
         const double d0_1 = sqdt * zb2;
@@ -1996,6 +2022,8 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double D0_0, D0_1;
         if (dm) {
             D0_0 = plom_mif_natural(dm->tr[0], dm->sc[0], dm->dur[0], (double)X[4], dm->min[0], dm->max[0], dm->mult[0]);
@@ -2007,7 +2035,7 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         double s6 = 0.0;
         double s7 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((D0_0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(D0_0, C0, RC0) * P2) * X[2]);
         const double r2_0 = P2;
         const double r3_0 = C6;
         const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr;
@@ -2016,7 +2044,7 @@ static __device__ inline __attribute__((always_inline)) void plom_drift_interval
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((D0_1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(D0_1, C1, RC1) * P3) * X[3]);
         const double r2_1 = P3;
         const double r3_1 = C7;
         const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr;
@@ -2553,7 +2581,7 @@ static __device__ inline __attribute__((always_inline)) double plom_drift_loglik
 
 struct plom_model_drift {
     static constexpr int S = 8, N_TS = 4, INC_OFFSET = 6, N_INC = 2, PV = 26, N_CAC = 2, N_HAT = 12, NPP = 14, FIX_OFF = 14, N_FIX = 12, N_DRIFT = 1;
-    static constexpr int NTF = 0;  /* time-only functions of a substep */
+    static constexpr int NTF = 2;  /* time-only functions and covariate reciprocals of a substep */
     template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
     {
         if constexpr (V == 0) plom_drift_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
@@ -2572,7 +2600,7 @@ struct plom_model_drift {
         else if constexpr (V == 4) plom_drift_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
         else if constexpr (V == 5) plom_drift_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
-    static __device__ inline __attribute__((always_inline)) void time_funcs(double tr, double *out) { plom_drift_time_funcs_f64(tr, out); }
+    static __device__ inline __attribute__((always_inline)) void time_funcs(double tr, const double *cvk, double *out) { plom_drift_time_funcs_f64(tr, cvk, out); }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, const double *cvd, double *obs) { plom_drift_proj2obs_f64(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_drift_proj2obs_pp_f64(X, pvd, lpd, cvd, obs); }
     template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
@@ -2589,7 +2617,7 @@ struct plom_model_drift {
         else if constexpr (V == 2) plom_drift_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
         else if constexpr (V == 3) plom_drift_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
-    static __device__ inline __attribute__((always_inline)) void time_funcs(float tr, float *out) { plom_drift_time_funcs_f32(tr, out); }
+    static __device__ inline __attribute__((always_inline)) void time_funcs(float tr, const double *cvk, float *out) { plom_drift_time_funcs_f32(tr, cvk, out); }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_drift_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_drift_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double ea, double er) { return plom_drift_interval_ode_f64(X, pv, cv, sp, t0, t1, dt, ea, er); }

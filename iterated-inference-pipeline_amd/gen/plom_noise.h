@@ -19,9 +19,11 @@
 #define PLOM_NOISE_N_DRIFT 0
 #define PLOM_NOISE_DRIFT_OFFSET 4
 
-PLOM_HD void plom_noise_time_funcs_f64(double tr, double *out)
+PLOM_HD void plom_noise_time_funcs_f64(double tr, const double *cvk, double *out)
 {
-    (void)tr; (void)out;
+    (void)tr; (void)cvk; (void)out;
+    out[0] = (double)plom_recip_uniform(plom_cov_ld(cvk + 0));
+    out[1] = (double)plom_recip_uniform(plom_cov_ld(cvk + 1));
 }
 
 PLOM_HD void plom_noise_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
@@ -47,10 +49,12 @@ PLOM_HD void plom_noise_interval_full_f64(double *X, const double *pv, plom_nstr
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double s4 = 0.0;
         double s5 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((P0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(P0, C0, RC0) * P2) * X[2]);
         const double r2_0 = (P2);
         const double r3_0 = C6;
         double za0, zb0;
@@ -71,7 +75,7 @@ PLOM_HD void plom_noise_interval_full_f64(double *X, const double *pv, plom_nstr
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((P1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(P1, C1, RC1) * P3) * X[3]);
         const double r2_1 = (P3);
         const double r3_1 = C7;
         double za3, zb3;
@@ -123,10 +127,12 @@ PLOM_HD void plom_noise_interval_no_dem_sto_f64(double *X, const double *pv, plo
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double s4 = 0.0;
         double s5 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((P0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(P0, C0, RC0) * P2) * X[2]);
         const double r2_0 = (P2);
         const double r3_0 = C6;
         double za0, zb0;
@@ -138,7 +144,7 @@ PLOM_HD void plom_noise_interval_no_dem_sto_f64(double *X, const double *pv, plo
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((P1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(P1, C1, RC1) * P3) * X[3]);
         const double r2_1 = (P3);
         const double r3_1 = C7;
         const double w0_1 = sqdt * zb0;
@@ -177,10 +183,12 @@ PLOM_HD void plom_noise_interval_no_env_sto_f64(double *X, const double *pv, plo
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double s4 = 0.0;
         double s5 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((P0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(P0, C0, RC0) * P2) * X[2]);
         const double r2_0 = (P2);
         const double r3_0 = C6;
         double za0, zb0;
@@ -200,7 +208,7 @@ PLOM_HD void plom_noise_interval_no_env_sto_f64(double *X, const double *pv, plo
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((P1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(P1, C1, RC1) * P3) * X[3]);
         const double r2_1 = (P3);
         const double r3_1 = C7;
         const double d0_1 = sqdt * zb2;
@@ -247,10 +255,12 @@ PLOM_HD void plom_noise_interval_no_dem_sto_no_env_sto_f64(double *X, const doub
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double s4 = 0.0;
         double s5 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((P0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(P0, C0, RC0) * P2) * X[2]);
         const double r2_0 = (P2);
         const double r3_0 = C6;
         const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr;
@@ -259,7 +269,7 @@ PLOM_HD void plom_noise_interval_no_dem_sto_no_env_sto_f64(double *X, const doub
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((P1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(P1, C1, RC1) * P3) * X[3]);
         const double r2_1 = (P3);
         const double r3_1 = C7;
         const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr;
@@ -395,6 +405,8 @@ PLOM_HD void plom_noise_interval_psr_full_f64(double *X, const double *pv, plom_
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -411,11 +423,11 @@ PLOM_HD void plom_noise_interval_psr_full_f64(double *X, const double *pv, plom_
         double stay3 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n4_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n4_1;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((P0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(P0, C0, RC0) * P2) * X[2]);
         const double r2_0 = (P2);
         const double r3_0 = C6;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((P1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(P1, C1, RC1) * P3) * X[3]);
         const double r2_1 = (P3);
         const double r3_1 = C7;
         const double sd0_0 = P4;
@@ -534,6 +546,8 @@ PLOM_HD void plom_noise_interval_psr_no_env_sto_f64(double *X, const double *pv,
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -550,11 +564,11 @@ PLOM_HD void plom_noise_interval_psr_no_env_sto_f64(double *X, const double *pv,
         double stay3 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n4_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n4_1;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((P0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(P0, C0, RC0) * P2) * X[2]);
         const double r2_0 = (P2);
         const double r3_0 = C6;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((P1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(P1, C1, RC1) * P3) * X[3]);
         const double r2_1 = (P3);
         const double r3_1 = C7;
         const double xi0_0 = 1.0;
@@ -720,9 +734,11 @@ PLOM_HD void plom_noise_interval_f64(double *X, const double *pv, plom_nstream64
 }
 
 #if defined(__HIPCC__)
-static __device__ inline __attribute__((always_inline)) void plom_noise_time_funcs_f32(float tr, float *out)
+static __device__ inline __attribute__((always_inline)) void plom_noise_time_funcs_f32(float tr, const double *cvk, float *out)
 {
-    (void)tr; (void)out;
+    (void)tr; (void)cvk; (void)out;
+    out[0] = (float)plom_recip_uniform(plom_cov_ld(cvk + 0));
+    out[1] = (float)plom_recip_uniform(plom_cov_ld(cvk + 1));
 }
 
 static __device__ inline __attribute__((always_inline)) void plom_noise_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
@@ -1012,6 +1028,8 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -1028,11 +1046,11 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         double stay3 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n4_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n4_1;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((P0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(P0, C0, RC0) * P2) * X[2]);
         const double r2_0 = (P2);
         const double r3_0 = C6;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((P1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(P1, C1, RC1) * P3) * X[3]);
         const double r2_1 = (P3);
         const double r3_1 = C7;
         const double sd0_0 = P4;
@@ -1151,6 +1169,8 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -1167,11 +1187,11 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         double stay3 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n3_0; (void)n4_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n3_1; (void)n4_1;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((P0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(P0, C0, RC0) * P2) * X[2]);
         const double r2_0 = (P2);
         const double r3_0 = C6;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((P1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(P1, C1, RC1) * P3) * X[3]);
         const double r2_1 = (P3);
         const double r3_1 = C7;
         const double xi0_0 = 1.0;
@@ -1341,10 +1361,12 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double s4 = 0.0;
         double s5 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((P0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(P0, C0, RC0) * P2) * X[2]);
         const double r2_0 = (P2);
         const double r3_0 = C6;
         double za0, zb0;
@@ -1365,7 +1387,7 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((P1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(P1, C1, RC1) * P3) * X[3]);
         const double r2_1 = (P3);
         const double r3_1 = C7;
         double za3, zb3;
@@ -1417,10 +1439,12 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double s4 = 0.0;
         double s5 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((P0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(P0, C0, RC0) * P2) * X[2]);
         const double r2_0 = (P2);
         const double r3_0 = C6;
         double za0, zb0;
@@ -1432,7 +1456,7 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((P1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(P1, C1, RC1) * P3) * X[3]);
         const double r2_1 = (P3);
         const double r3_1 = C7;
         const double w0_1 = sqdt * zb0;
@@ -1471,10 +1495,12 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double s4 = 0.0;
         double s5 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((P0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(P0, C0, RC0) * P2) * X[2]);
         const double r2_0 = (P2);
         const double r3_0 = C6;
         double za0, zb0;
@@ -1494,7 +1520,7 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((P1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(P1, C1, RC1) * P3) * X[3]);
         const double r2_1 = (P3);
         const double r3_1 = C7;
         const double d0_1 = sqdt * zb2;
@@ -1541,10 +1567,12 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         const double C5 = plom_cov_ld(cvk + 5);
         const double C6 = plom_cov_ld(cvk + 6);
         const double C7 = plom_cov_ld(cvk + 7);
+        const double RC0 = tft ? tft[k * 2 + 0] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 2 + 1] : plom_recip_uniform(C1);
         double s4 = 0.0;
         double s5 = 0.0;
         const double r0_0 = (C4 * C0);
-        const double r1_0 = (((P0 / C0) * P2) * X[2]);
+        const double r1_0 = ((plom_div_rc(P0, C0, RC0) * P2) * X[2]);
         const double r2_0 = (P2);
         const double r3_0 = C6;
         const double f0 = X[0] + ( - (r1_0*X[0]) - (r3_0*X[0]) + (r0_0))*dtr;
@@ -1553,7 +1581,7 @@ static __device__ inline __attribute__((always_inline)) void plom_noise_interval
         X[0] = (f0 < 0.0) ? 0.0 : f0;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
         const double r0_1 = (C5 * C1);
-        const double r1_1 = (((P1 / C1) * P3) * X[3]);
+        const double r1_1 = ((plom_div_rc(P1, C1, RC1) * P3) * X[3]);
         const double r2_1 = (P3);
         const double r3_1 = C7;
         const double f1 = X[1] + ( - (r1_1*X[1]) - (r3_1*X[1]) + (r0_1))*dtr;
@@ -1895,7 +1923,7 @@ static __device__ inline __attribute__((always_inline)) double plom_noise_loglik
 
 struct plom_model_noise {
     static constexpr int S = 6, N_TS = 4, INC_OFFSET = 4, N_INC = 2, PV = 26, N_CAC = 2, N_HAT = 10, NPP = 14, FIX_OFF = 14, N_FIX = 12, N_DRIFT = 0;
-    static constexpr int NTF = 0;  /* time-only functions of a substep */
+    static constexpr int NTF = 2;  /* time-only functions and covariate reciprocals of a substep */
     template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
     {
         if constexpr (V == 0) plom_noise_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
@@ -1914,7 +1942,7 @@ struct plom_model_noise {
         else if constexpr (V == 4) plom_noise_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
         else if constexpr (V == 5) plom_noise_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
-    static __device__ inline __attribute__((always_inline)) void time_funcs(double tr, double *out) { plom_noise_time_funcs_f64(tr, out); }
+    static __device__ inline __attribute__((always_inline)) void time_funcs(double tr, const double *cvk, double *out) { plom_noise_time_funcs_f64(tr, cvk, out); }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, const double *cvd, double *obs) { plom_noise_proj2obs_f64(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_noise_proj2obs_pp_f64(X, pvd, lpd, cvd, obs); }
     template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
@@ -1931,7 +1959,7 @@ struct plom_model_noise {
         else if constexpr (V == 2) plom_noise_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
         else if constexpr (V == 3) plom_noise_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
-    static __device__ inline __attribute__((always_inline)) void time_funcs(float tr, float *out) { plom_noise_time_funcs_f32(tr, out); }
+    static __device__ inline __attribute__((always_inline)) void time_funcs(float tr, const double *cvk, float *out) { plom_noise_time_funcs_f32(tr, cvk, out); }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_noise_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_noise_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double ea, double er) { return plom_noise_interval_ode_f64(X, pv, cv, sp, t0, t1, dt, ea, er); }

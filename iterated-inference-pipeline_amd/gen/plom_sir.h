@@ -19,9 +19,10 @@
 #define PLOM_SIR_N_DRIFT 0
 #define PLOM_SIR_DRIFT_OFFSET 2
 
-PLOM_HD void plom_sir_time_funcs_f64(double tr, double *out)
+PLOM_HD void plom_sir_time_funcs_f64(double tr, const double *cvk, double *out)
 {
-    (void)tr; (void)out;
+    (void)tr; (void)cvk; (void)out;
+    out[0] = (double)plom_recip_uniform(plom_cov_ld(cvk + 0));
 }
 
 PLOM_HD void plom_sir_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
@@ -37,8 +38,9 @@ PLOM_HD void plom_sir_interval_full_f64(double *X, const double *pv, plom_nstrea
         const double tr = (double)t; (void)tr;
         const double *cvk = cv + (size_t)k * (size_t)cs;
         const double C0 = plom_cov_ld(cvk + 0);
+        const double RC0 = tft ? tft[k * 1 + 0] : plom_recip_uniform(C0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / C0);
+        const double r0_0 = plom_div_rc((P0 * X[1]), C0, RC0);
         const double r1_0 = P1;
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
@@ -68,8 +70,9 @@ PLOM_HD void plom_sir_interval_no_dem_sto_f64(double *X, const double *pv, plom_
         const double tr = (double)t; (void)tr;
         const double *cvk = cv + (size_t)k * (size_t)cs;
         const double C0 = plom_cov_ld(cvk + 0);
+        const double RC0 = tft ? tft[k * 1 + 0] : plom_recip_uniform(C0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / C0);
+        const double r0_0 = plom_div_rc((P0 * X[1]), C0, RC0);
         const double r1_0 = P1;
         const double f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
@@ -95,8 +98,9 @@ PLOM_HD void plom_sir_interval_no_env_sto_f64(double *X, const double *pv, plom_
         const double tr = (double)t; (void)tr;
         const double *cvk = cv + (size_t)k * (size_t)cs;
         const double C0 = plom_cov_ld(cvk + 0);
+        const double RC0 = tft ? tft[k * 1 + 0] : plom_recip_uniform(C0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / C0);
+        const double r0_0 = plom_div_rc((P0 * X[1]), C0, RC0);
         const double r1_0 = P1;
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
@@ -126,8 +130,9 @@ PLOM_HD void plom_sir_interval_no_dem_sto_no_env_sto_f64(double *X, const double
         const double tr = (double)t; (void)tr;
         const double *cvk = cv + (size_t)k * (size_t)cs;
         const double C0 = plom_cov_ld(cvk + 0);
+        const double RC0 = tft ? tft[k * 1 + 0] : plom_recip_uniform(C0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / C0);
+        const double r0_0 = plom_div_rc((P0 * X[1]), C0, RC0);
         const double r1_0 = P1;
         const double f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
@@ -197,12 +202,13 @@ PLOM_HD void plom_sir_interval_psr_full_f64(double *X, const double *pv, plom_ns
         const double tr = t; (void)tr;
         const double *cvk = cv + (size_t)k * (size_t)cs;
         const double C0 = plom_cov_ld(cvk + 0);
+        const double RC0 = tft ? tft[k * 1 + 0] : plom_recip_uniform(C0);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double stay0 = 0.0;
         double stay1 = 0.0;
         (void)n0_0; (void)n1_0;
-        const double r0_0 = ((P0 * X[1]) / C0);
+        const double r0_0 = plom_div_rc((P0 * X[1]), C0, RC0);
         const double r1_0 = P1;
         {
             const double sum = r0_0 * dt;
@@ -237,12 +243,13 @@ PLOM_HD void plom_sir_interval_psr_no_env_sto_f64(double *X, const double *pv, p
         const double tr = t; (void)tr;
         const double *cvk = cv + (size_t)k * (size_t)cs;
         const double C0 = plom_cov_ld(cvk + 0);
+        const double RC0 = tft ? tft[k * 1 + 0] : plom_recip_uniform(C0);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double stay0 = 0.0;
         double stay1 = 0.0;
         (void)n0_0; (void)n1_0;
-        const double r0_0 = ((P0 * X[1]) / C0);
+        const double r0_0 = plom_div_rc((P0 * X[1]), C0, RC0);
         const double r1_0 = P1;
         {
             const double sum = r0_0 * dt;
@@ -314,9 +321,10 @@ PLOM_HD void plom_sir_interval_f64(double *X, const double *pv, plom_nstream64 *
 }
 
 #if defined(__HIPCC__)
-static __device__ inline __attribute__((always_inline)) void plom_sir_time_funcs_f32(float tr, float *out)
+static __device__ inline __attribute__((always_inline)) void plom_sir_time_funcs_f32(float tr, const double *cvk, float *out)
 {
-    (void)tr; (void)out;
+    (void)tr; (void)cvk; (void)out;
+    out[0] = (float)plom_recip_uniform(plom_cov_ld(cvk + 0));
 }
 
 static __device__ inline __attribute__((always_inline)) void plom_sir_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
@@ -453,12 +461,13 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_p
         const double tr = t; (void)tr;
         const double *cvk = cv + (size_t)k * (size_t)cs;
         const double C0 = plom_cov_ld(cvk + 0);
+        const double RC0 = tft ? tft[k * 1 + 0] : plom_recip_uniform(C0);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double stay0 = 0.0;
         double stay1 = 0.0;
         (void)n0_0; (void)n1_0;
-        const double r0_0 = ((P0 * X[1]) / C0);
+        const double r0_0 = plom_div_rc((P0 * X[1]), C0, RC0);
         const double r1_0 = P1;
         {
             const double sum = r0_0 * dt;
@@ -493,12 +502,13 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_p
         const double tr = t; (void)tr;
         const double *cvk = cv + (size_t)k * (size_t)cs;
         const double C0 = plom_cov_ld(cvk + 0);
+        const double RC0 = tft ? tft[k * 1 + 0] : plom_recip_uniform(C0);
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double stay0 = 0.0;
         double stay1 = 0.0;
         (void)n0_0; (void)n1_0;
-        const double r0_0 = ((P0 * X[1]) / C0);
+        const double r0_0 = plom_div_rc((P0 * X[1]), C0, RC0);
         const double r1_0 = P1;
         {
             const double sum = r0_0 * dt;
@@ -564,8 +574,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_f
         const double tr = (double)t; (void)tr;
         const double *cvk = cv + (size_t)k * (size_t)cs;
         const double C0 = plom_cov_ld(cvk + 0);
+        const double RC0 = tft ? tft[k * 1 + 0] : plom_recip_uniform(C0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / C0);
+        const double r0_0 = plom_div_rc((P0 * X[1]), C0, RC0);
         const double r1_0 = P1;
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
@@ -595,8 +606,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
         const double tr = (double)t; (void)tr;
         const double *cvk = cv + (size_t)k * (size_t)cs;
         const double C0 = plom_cov_ld(cvk + 0);
+        const double RC0 = tft ? tft[k * 1 + 0] : plom_recip_uniform(C0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / C0);
+        const double r0_0 = plom_div_rc((P0 * X[1]), C0, RC0);
         const double r1_0 = P1;
         const double f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
@@ -622,8 +634,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
         const double tr = (double)t; (void)tr;
         const double *cvk = cv + (size_t)k * (size_t)cs;
         const double C0 = plom_cov_ld(cvk + 0);
+        const double RC0 = tft ? tft[k * 1 + 0] : plom_recip_uniform(C0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / C0);
+        const double r0_0 = plom_div_rc((P0 * X[1]), C0, RC0);
         const double r1_0 = P1;
         double za0, zb0;
         plom_normal_pair64(ns, &za0, &zb0);
@@ -653,8 +666,9 @@ static __device__ inline __attribute__((always_inline)) void plom_sir_interval_n
         const double tr = (double)t; (void)tr;
         const double *cvk = cv + (size_t)k * (size_t)cs;
         const double C0 = plom_cov_ld(cvk + 0);
+        const double RC0 = tft ? tft[k * 1 + 0] : plom_recip_uniform(C0);
         double s2 = 0.0;
-        const double r0_0 = ((P0 * X[1]) / C0);
+        const double r0_0 = plom_div_rc((P0 * X[1]), C0, RC0);
         const double r1_0 = P1;
         const double f0 = X[0] + ( - (r0_0*X[0]))*dtr;
         const double f1 = X[1] + ( - (r1_0*X[1]) + (r0_0*X[0]))*dtr;
@@ -815,7 +829,7 @@ static __device__ inline __attribute__((always_inline)) double plom_sir_loglik_p
 
 struct plom_model_sir {
     static constexpr int S = 3, N_TS = 1, INC_OFFSET = 2, N_INC = 1, PV = 5, N_CAC = 1, N_HAT = 4, NPP = 4, FIX_OFF = 4, N_FIX = 1, N_DRIFT = 0;
-    static constexpr int NTF = 0;  /* time-only functions of a substep */
+    static constexpr int NTF = 1;  /* time-only functions and covariate reciprocals of a substep */
     template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
     {
         if constexpr (V == 0) plom_sir_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
@@ -834,7 +848,7 @@ struct plom_model_sir {
         else if constexpr (V == 4) plom_sir_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
         else if constexpr (V == 5) plom_sir_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
-    static __device__ inline __attribute__((always_inline)) void time_funcs(double tr, double *out) { plom_sir_time_funcs_f64(tr, out); }
+    static __device__ inline __attribute__((always_inline)) void time_funcs(double tr, const double *cvk, double *out) { plom_sir_time_funcs_f64(tr, cvk, out); }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, const double *cvd, double *obs) { plom_sir_proj2obs_f64(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sir_proj2obs_pp_f64(X, pvd, lpd, cvd, obs); }
     template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
@@ -851,7 +865,7 @@ struct plom_model_sir {
         else if constexpr (V == 2) plom_sir_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
         else if constexpr (V == 3) plom_sir_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
-    static __device__ inline __attribute__((always_inline)) void time_funcs(float tr, float *out) { plom_sir_time_funcs_f32(tr, out); }
+    static __device__ inline __attribute__((always_inline)) void time_funcs(float tr, const double *cvk, float *out) { plom_sir_time_funcs_f32(tr, cvk, out); }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_sir_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sir_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double ea, double er) { return plom_sir_interval_ode_f64(X, pv, cv, sp, t0, t1, dt, ea, er); }

@@ -19,10 +19,14 @@
 #define PLOM_SIRS4_N_DRIFT 0
 #define PLOM_SIRS4_DRIFT_OFFSET 8
 
-PLOM_HD void plom_sirs4_time_funcs_f64(double tr, double *out)
+PLOM_HD void plom_sirs4_time_funcs_f64(double tr, const double *cvk, double *out)
 {
-    (void)tr; (void)out;
+    (void)tr; (void)cvk; (void)out;
     out[0] = plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+    out[1] = (double)plom_recip_uniform(plom_cov_ld(cvk + 0));
+    out[2] = (double)plom_recip_uniform(plom_cov_ld(cvk + 1));
+    out[3] = (double)plom_recip_uniform(plom_cov_ld(cvk + 2));
+    out[4] = (double)plom_recip_uniform(plom_cov_ld(cvk + 3));
 }
 
 PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
@@ -78,12 +82,16 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double RC0 = tft ? tft[k * 5 + 1] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 5 + 2] : plom_recip_uniform(C1);
+        const double RC2 = tft ? tft[k * 5 + 3] : plom_recip_uniform(C2);
+        const double RC3 = tft ? tft[k * 5 + 4] : plom_recip_uniform(C3);
+        const double tf0 = tft ? tft[k * 5 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
+        const double r0_0 = plom_div_rc(((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0, RC0);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         double za0, zb0;
@@ -98,7 +106,7 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[4]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt_rate((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
+        const double r0_1 = plom_div_rc(((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1, RC1);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         double za2, zb2;
@@ -113,7 +121,7 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt_rate((r1_1*X[5]))*d1_1+ plom_sqrt_rate((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt_rate((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
+        const double r0_2 = plom_div_rc(((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2, RC2);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         double za4, zb4;
@@ -128,7 +136,7 @@ PLOM_HD void plom_sirs4_interval_full_f64(double *X, const double *pv, plom_nstr
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt_rate((r1_2*X[6]))*d1_2+ plom_sqrt_rate((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt_rate((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
+        const double r0_3 = plom_div_rc(((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3, RC3);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         double za6, zb6;
@@ -212,12 +220,16 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double RC0 = tft ? tft[k * 5 + 1] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 5 + 2] : plom_recip_uniform(C1);
+        const double RC2 = tft ? tft[k * 5 + 3] : plom_recip_uniform(C2);
+        const double RC3 = tft ? tft[k * 5 + 4] : plom_recip_uniform(C3);
+        const double tf0 = tft ? tft[k * 5 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
+        const double r0_0 = plom_div_rc(((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0, RC0);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         double za0, zb0;
@@ -227,7 +239,7 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
+        const double r0_1 = plom_div_rc(((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1, RC1);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const double w0_1 = sqdt * zb0;
@@ -235,7 +247,7 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
+        const double r0_2 = plom_div_rc(((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2, RC2);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         double za1, zb1;
@@ -245,7 +257,7 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_f64(double *X, const double *pv, plo
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
+        const double r0_3 = plom_div_rc(((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3, RC3);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double w0_3 = sqdt * zb1;
@@ -318,12 +330,16 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double RC0 = tft ? tft[k * 5 + 1] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 5 + 2] : plom_recip_uniform(C1);
+        const double RC2 = tft ? tft[k * 5 + 3] : plom_recip_uniform(C2);
+        const double RC3 = tft ? tft[k * 5 + 4] : plom_recip_uniform(C3);
+        const double tf0 = tft ? tft[k * 5 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
+        const double r0_0 = plom_div_rc(((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0, RC0);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         double za0, zb0;
@@ -337,7 +353,7 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[4]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt_rate((r0_0*X[0]))*d0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
+        const double r0_1 = plom_div_rc(((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1, RC1);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const double d0_1 = sqdt * zb1;
@@ -349,7 +365,7 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt_rate((r1_1*X[5]))*d1_1+ plom_sqrt_rate((r0_1*X[1]))*d0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt_rate((r0_1*X[1]))*d0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
+        const double r0_2 = plom_div_rc(((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2, RC2);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         double za3, zb3;
@@ -363,7 +379,7 @@ PLOM_HD void plom_sirs4_interval_no_env_sto_f64(double *X, const double *pv, plo
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt_rate((r1_2*X[6]))*d1_2+ plom_sqrt_rate((r0_2*X[2]))*d0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt_rate((r0_2*X[2]))*d0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
+        const double r0_3 = plom_div_rc(((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3, RC3);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double d0_3 = sqdt * zb4;
@@ -440,33 +456,37 @@ PLOM_HD void plom_sirs4_interval_no_dem_sto_no_env_sto_f64(double *X, const doub
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double RC0 = tft ? tft[k * 5 + 1] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 5 + 2] : plom_recip_uniform(C1);
+        const double RC2 = tft ? tft[k * 5 + 3] : plom_recip_uniform(C2);
+        const double RC3 = tft ? tft[k * 5 + 4] : plom_recip_uniform(C3);
+        const double tf0 = tft ? tft[k * 5 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
+        const double r0_0 = plom_div_rc(((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0, RC0);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr;
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr;
         s8 += ( + (r0_0*X[0]))*dtr;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
+        const double r0_1 = plom_div_rc(((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1, RC1);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr;
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr;
         s9 += ( + (r0_1*X[1]))*dtr;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
+        const double r0_2 = plom_div_rc(((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2, RC2);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr;
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr;
         s10 += ( + (r0_2*X[2]))*dtr;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
+        const double r0_3 = plom_div_rc(((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3, RC3);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr;
@@ -640,7 +660,11 @@ PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double RC0 = tft ? tft[k * 5 + 1] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 5 + 2] : plom_recip_uniform(C1);
+        const double RC2 = tft ? tft[k * 5 + 3] : plom_recip_uniform(C2);
+        const double RC3 = tft ? tft[k * 5 + 4] : plom_recip_uniform(C3);
+        const double tf0 = tft ? tft[k * 5 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -662,16 +686,16 @@ PLOM_HD void plom_sirs4_interval_psr_full_f64(double *X, const double *pv, plom_
         double stay6 = 0.0;
         double stay7 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n0_3; (void)n1_3; (void)n2_3;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
+        const double r0_0 = plom_div_rc(((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0, RC0);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
+        const double r0_1 = plom_div_rc(((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1, RC1);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
+        const double r0_2 = plom_div_rc(((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2, RC2);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
+        const double r0_3 = plom_div_rc(((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3, RC3);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double sd0_0 = P16;
@@ -822,7 +846,11 @@ PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv,
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double RC0 = tft ? tft[k * 5 + 1] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 5 + 2] : plom_recip_uniform(C1);
+        const double RC2 = tft ? tft[k * 5 + 3] : plom_recip_uniform(C2);
+        const double RC3 = tft ? tft[k * 5 + 4] : plom_recip_uniform(C3);
+        const double tf0 = tft ? tft[k * 5 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -844,16 +872,16 @@ PLOM_HD void plom_sirs4_interval_psr_no_env_sto_f64(double *X, const double *pv,
         double stay6 = 0.0;
         double stay7 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n0_3; (void)n1_3; (void)n2_3;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
+        const double r0_0 = plom_div_rc(((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0, RC0);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
+        const double r0_1 = plom_div_rc(((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1, RC1);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
+        const double r0_2 = plom_div_rc(((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2, RC2);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
+        const double r0_3 = plom_div_rc(((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3, RC3);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double xi0_0 = 1.0;
@@ -1065,10 +1093,14 @@ PLOM_HD void plom_sirs4_interval_f64(double *X, const double *pv, plom_nstream64
 }
 
 #if defined(__HIPCC__)
-static __device__ inline __attribute__((always_inline)) void plom_sirs4_time_funcs_f32(float tr, float *out)
+static __device__ inline __attribute__((always_inline)) void plom_sirs4_time_funcs_f32(float tr, const double *cvk, float *out)
 {
-    (void)tr; (void)out;
+    (void)tr; (void)cvk; (void)out;
     out[0] = plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+    out[1] = (float)plom_recip_uniform(plom_cov_ld(cvk + 0));
+    out[2] = (float)plom_recip_uniform(plom_cov_ld(cvk + 1));
+    out[3] = (float)plom_recip_uniform(plom_cov_ld(cvk + 2));
+    out[4] = (float)plom_recip_uniform(plom_cov_ld(cvk + 3));
 }
 
 static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval_full_f32(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
@@ -1124,7 +1156,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 5 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -1258,7 +1290,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 5 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -1364,7 +1396,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 5 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -1486,7 +1518,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 5 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -1596,7 +1628,11 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double RC0 = tft ? tft[k * 5 + 1] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 5 + 2] : plom_recip_uniform(C1);
+        const double RC2 = tft ? tft[k * 5 + 3] : plom_recip_uniform(C2);
+        const double RC3 = tft ? tft[k * 5 + 4] : plom_recip_uniform(C3);
+        const double tf0 = tft ? tft[k * 5 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -1618,16 +1654,16 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         double stay6 = 0.0;
         double stay7 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n0_3; (void)n1_3; (void)n2_3;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
+        const double r0_0 = plom_div_rc(((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0, RC0);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
+        const double r0_1 = plom_div_rc(((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1, RC1);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
+        const double r0_2 = plom_div_rc(((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2, RC2);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
+        const double r0_3 = plom_div_rc(((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3, RC3);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double sd0_0 = P16;
@@ -1778,7 +1814,11 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double RC0 = tft ? tft[k * 5 + 1] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 5 + 2] : plom_recip_uniform(C1);
+        const double RC2 = tft ? tft[k * 5 + 3] : plom_recip_uniform(C2);
+        const double RC3 = tft ? tft[k * 5 + 4] : plom_recip_uniform(C3);
+        const double tf0 = tft ? tft[k * 5 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double n0_0 = 0.0;
         double n1_0 = 0.0;
         double n2_0 = 0.0;
@@ -1800,16 +1840,16 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         double stay6 = 0.0;
         double stay7 = 0.0;
         (void)n0_0; (void)n1_0; (void)n2_0; (void)n0_1; (void)n1_1; (void)n2_1; (void)n0_2; (void)n1_2; (void)n2_2; (void)n0_3; (void)n1_3; (void)n2_3;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
+        const double r0_0 = plom_div_rc(((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0, RC0);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
+        const double r0_1 = plom_div_rc(((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1, RC1);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
+        const double r0_2 = plom_div_rc(((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2, RC2);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
+        const double r0_3 = plom_div_rc(((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3, RC3);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double xi0_0 = 1.0;
@@ -2055,12 +2095,16 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double RC0 = tft ? tft[k * 5 + 1] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 5 + 2] : plom_recip_uniform(C1);
+        const double RC2 = tft ? tft[k * 5 + 3] : plom_recip_uniform(C2);
+        const double RC3 = tft ? tft[k * 5 + 4] : plom_recip_uniform(C3);
+        const double tf0 = tft ? tft[k * 5 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
+        const double r0_0 = plom_div_rc(((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0, RC0);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         double za0, zb0;
@@ -2075,7 +2119,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[4]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt_rate((r0_0*X[0]))*d0_0 + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
+        const double r0_1 = plom_div_rc(((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1, RC1);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         double za2, zb2;
@@ -2090,7 +2134,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt_rate((r1_1*X[5]))*d1_1+ plom_sqrt_rate((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt_rate((r0_1*X[1]))*d0_1 + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
+        const double r0_2 = plom_div_rc(((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2, RC2);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         double za4, zb4;
@@ -2105,7 +2149,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt_rate((r1_2*X[6]))*d1_2+ plom_sqrt_rate((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt_rate((r0_2*X[2]))*d0_2 + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
+        const double r0_3 = plom_div_rc(((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3, RC3);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         double za6, zb6;
@@ -2189,12 +2233,16 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double RC0 = tft ? tft[k * 5 + 1] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 5 + 2] : plom_recip_uniform(C1);
+        const double RC2 = tft ? tft[k * 5 + 3] : plom_recip_uniform(C2);
+        const double RC3 = tft ? tft[k * 5 + 4] : plom_recip_uniform(C3);
+        const double tf0 = tft ? tft[k * 5 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
+        const double r0_0 = plom_div_rc(((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0, RC0);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         double za0, zb0;
@@ -2204,7 +2252,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + (r0_0*X[0])*P16*w0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
+        const double r0_1 = plom_div_rc(((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1, RC1);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const double w0_1 = sqdt * zb0;
@@ -2212,7 +2260,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + (r0_1*X[1])*P17*w0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
+        const double r0_2 = plom_div_rc(((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2, RC2);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         double za1, zb1;
@@ -2222,7 +2270,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + (r0_2*X[2])*P18*w0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
+        const double r0_3 = plom_div_rc(((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3, RC3);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double w0_3 = sqdt * zb1;
@@ -2295,12 +2343,16 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double RC0 = tft ? tft[k * 5 + 1] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 5 + 2] : plom_recip_uniform(C1);
+        const double RC2 = tft ? tft[k * 5 + 3] : plom_recip_uniform(C2);
+        const double RC3 = tft ? tft[k * 5 + 4] : plom_recip_uniform(C3);
+        const double tf0 = tft ? tft[k * 5 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
+        const double r0_0 = plom_div_rc(((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0, RC0);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         double za0, zb0;
@@ -2314,7 +2366,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr + - plom_sqrt_rate((r1_0*X[4]))*d1_0+ plom_sqrt_rate((r0_0*X[0]))*d0_0;
         s8 += ( + (r0_0*X[0]))*dtr + + plom_sqrt_rate((r0_0*X[0]))*d0_0;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
+        const double r0_1 = plom_div_rc(((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1, RC1);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const double d0_1 = sqdt * zb1;
@@ -2326,7 +2378,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr + - plom_sqrt_rate((r1_1*X[5]))*d1_1+ plom_sqrt_rate((r0_1*X[1]))*d0_1;
         s9 += ( + (r0_1*X[1]))*dtr + + plom_sqrt_rate((r0_1*X[1]))*d0_1;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
+        const double r0_2 = plom_div_rc(((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2, RC2);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         double za3, zb3;
@@ -2340,7 +2392,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr + - plom_sqrt_rate((r1_2*X[6]))*d1_2+ plom_sqrt_rate((r0_2*X[2]))*d0_2;
         s10 += ( + (r0_2*X[2]))*dtr + + plom_sqrt_rate((r0_2*X[2]))*d0_2;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
+        const double r0_3 = plom_div_rc(((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3, RC3);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double d0_3 = sqdt * zb4;
@@ -2417,33 +2469,37 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const double C1 = plom_cov_ld(cvk + 1);
         const double C2 = plom_cov_ld(cvk + 2);
         const double C3 = plom_cov_ld(cvk + 3);
-        const double tf0 = tft ? tft[k * 1 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
+        const double RC0 = tft ? tft[k * 5 + 1] : plom_recip_uniform(C0);
+        const double RC1 = tft ? tft[k * 5 + 2] : plom_recip_uniform(C1);
+        const double RC2 = tft ? tft[k * 5 + 3] : plom_recip_uniform(C2);
+        const double RC3 = tft ? tft[k * 5 + 4] : plom_recip_uniform(C3);
+        const double tf0 = tft ? tft[k * 5 + 0] : plom_sin((((2.0 * 3.141592653589793) * tr) / 365.0));
         double s8 = 0.0;
         double s9 = 0.0;
         double s10 = 0.0;
         double s11 = 0.0;
-        const double r0_0 = (((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])) / C0);
+        const double r0_0 = plom_div_rc(((P0 * (1.0 + (P4 * tf0))) * (P32 * X[4] + P33 * X[5] + P34 * X[6] + P35 * X[7])), C0, RC0);
         const double r1_0 = P8;
         const double r2_0 = (P12 * ((C0 - X[0]) - X[4]));
         const double f0 = X[0] + ( - (r0_0*X[0]) + (r2_0))*dtr;
         const double f4 = X[4] + ( - (r1_0*X[4]) + (r0_0*X[0]))*dtr;
         s8 += ( + (r0_0*X[0]))*dtr;
         X[0] = (f0 < 0.0) ? 0.0 : f0;
-        const double r0_1 = (((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])) / C1);
+        const double r0_1 = plom_div_rc(((P1 * (1.0 + (P5 * tf0))) * (P36 * X[4] + P37 * X[5] + P38 * X[6] + P39 * X[7])), C1, RC1);
         const double r1_1 = P9;
         const double r2_1 = (P13 * ((C1 - X[1]) - X[5]));
         const double f1 = X[1] + ( - (r0_1*X[1]) + (r2_1))*dtr;
         const double f5 = X[5] + ( - (r1_1*X[5]) + (r0_1*X[1]))*dtr;
         s9 += ( + (r0_1*X[1]))*dtr;
         X[1] = (f1 < 0.0) ? 0.0 : f1;
-        const double r0_2 = (((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])) / C2);
+        const double r0_2 = plom_div_rc(((P2 * (1.0 + (P6 * tf0))) * (P40 * X[4] + P41 * X[5] + P42 * X[6] + P43 * X[7])), C2, RC2);
         const double r1_2 = P10;
         const double r2_2 = (P14 * ((C2 - X[2]) - X[6]));
         const double f2 = X[2] + ( - (r0_2*X[2]) + (r2_2))*dtr;
         const double f6 = X[6] + ( - (r1_2*X[6]) + (r0_2*X[2]))*dtr;
         s10 += ( + (r0_2*X[2]))*dtr;
         X[2] = (f2 < 0.0) ? 0.0 : f2;
-        const double r0_3 = (((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])) / C3);
+        const double r0_3 = plom_div_rc(((P3 * (1.0 + (P7 * tf0))) * (P44 * X[4] + P45 * X[5] + P46 * X[6] + P47 * X[7])), C3, RC3);
         const double r1_3 = P11;
         const double r2_3 = (P15 * ((C3 - X[3]) - X[7]));
         const double f3 = X[3] + ( - (r0_3*X[3]) + (r2_3))*dtr;
@@ -2528,7 +2584,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 5 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -2662,7 +2718,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 5 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -2768,7 +2824,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 5 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -2890,7 +2946,7 @@ static __device__ inline __attribute__((always_inline)) void plom_sirs4_interval
         const float C1 = (float)plom_cov_ld(cvk + 1);
         const float C2 = (float)plom_cov_ld(cvk + 2);
         const float C3 = (float)plom_cov_ld(cvk + 3);
-        const float tf0 = tft ? tft[k * 1 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
+        const float tf0 = tft ? tft[k * 5 + 0] : plom_sinf(plom_fdivf(((2.0f * 3.141592653589793f) * tr), 365.0f));
         float s8 = 0.0f;
         float s9 = 0.0f;
         float s10 = 0.0f;
@@ -2995,7 +3051,7 @@ static __device__ inline __attribute__((always_inline)) double plom_sirs4_loglik
 
 struct plom_model_sirs4 {
     static constexpr int S = 12, N_TS = 4, INC_OFFSET = 8, N_INC = 4, PV = 48, N_CAC = 4, N_HAT = 16, NPP = 28, FIX_OFF = 28, N_FIX = 4, N_DRIFT = 0;
-    static constexpr int NTF = 1;  /* time-only functions of a substep */
+    static constexpr int NTF = 5;  /* time-only functions and covariate reciprocals of a substep */
     template <int V> static __device__ inline __attribute__((always_inline)) void interval(double *X, const double *pv, plom_nstream64 *ns, double t0, double t1, double dt, double sqdt, const double *cv, int cs, const plom_drift_view *dm, const double *tft)
     {
         if constexpr (V == 0) plom_sirs4_interval_full_f64(X, pv, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
@@ -3014,7 +3070,7 @@ struct plom_model_sirs4 {
         else if constexpr (V == 4) plom_sirs4_interval_psr_full_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
         else if constexpr (V == 5) plom_sirs4_interval_psr_no_env_sto_pp_f64(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
-    static __device__ inline __attribute__((always_inline)) void time_funcs(double tr, double *out) { plom_sirs4_time_funcs_f64(tr, out); }
+    static __device__ inline __attribute__((always_inline)) void time_funcs(double tr, const double *cvk, double *out) { plom_sirs4_time_funcs_f64(tr, cvk, out); }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const double *X, const double *pvd, const double *cvd, double *obs) { plom_sirs4_proj2obs_f64(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const double *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sirs4_proj2obs_pp_f64(X, pvd, lpd, cvd, obs); }
     template <int V> static __device__ inline __attribute__((always_inline)) void interval(float *X, const float *pv, plom_nstream32 *ns, double t0, double t1, double dt, float sqdt, const double *cv, int cs, const plom_drift_view *dm, const float *tft)
@@ -3031,7 +3087,7 @@ struct plom_model_sirs4 {
         else if constexpr (V == 2) plom_sirs4_interval_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
         else if constexpr (V == 3) plom_sirs4_interval_no_dem_sto_no_env_sto_pp_f32(X, pv, lp, ns, t0, t1, dt, sqdt, cv, cs, dm, tft);
     }
-    static __device__ inline __attribute__((always_inline)) void time_funcs(float tr, float *out) { plom_sirs4_time_funcs_f32(tr, out); }
+    static __device__ inline __attribute__((always_inline)) void time_funcs(float tr, const double *cvk, float *out) { plom_sirs4_time_funcs_f32(tr, cvk, out); }
     static __device__ inline __attribute__((always_inline)) void proj2obs(const float *X, const double *pvd, const double *cvd, double *obs) { plom_sirs4_proj2obs_f32(X, pvd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) void proj2obs_pp(const float *X, const double *pvd, const double *lpd, const double *cvd, double *obs) { plom_sirs4_proj2obs_pp_f32(X, pvd, lpd, cvd, obs); }
     static __device__ inline __attribute__((always_inline)) int interval_ode(double *X, const double *pv, const double *cv, const plom_spline_set *sp, double t0, double t1, double dt, double ea, double er) { return plom_sirs4_interval_ode_f64(X, pv, cv, sp, t0, t1, dt, ea, er); }

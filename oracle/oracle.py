@@ -372,6 +372,16 @@ def bm_pairs(xy):
     return out
 
 
+def div_rc(x, c):
+    """x / c as the generated steps divide by a covariate (plom_div_rc with the divisor's
+    reciprocal)."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    c = np.ascontiguousarray(np.broadcast_to(c, x.shape), dtype=np.float64).ravel()
+    out = np.zeros(x.size)
+    lib().oracle_div_rc(x.size, _dp(x), _dp(c), _dp(out))
+    return out
+
+
 def icdf(words):
     """The stream's Gaussians of the 32-bit words (plom_icdf_normal)."""
     w = np.ascontiguousarray(words, dtype=np.uint32).ravel()

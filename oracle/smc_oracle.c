@@ -1087,6 +1087,13 @@ void oracle_bm_pairs(int n, const uint32_t *xy, double *out)
     for (int i = 0; i < n; i++) plom_bm_pair(xy[2 * i], xy[2 * i + 1], plom_bm_tab, &out[2 * i], &out[2 * i + 1]);
 }
 
+/* quotients x[i] / c[i] from the divisor's reciprocal (plom_div_rc, the generated steps'
+ * division by a covariate) */
+void oracle_div_rc(int n, const double *x, const double *c, double *out)
+{
+    for (int i = 0; i < n; i++) out[i] = plom_div_rc(x[i], c[i], plom_recip_uniform(c[i]));
+}
+
 /* the stream's Gaussian transform of the 32-bit words w[0..n) (plom_icdf_normal) */
 void oracle_icdf(int n, const uint32_t *w, double *out)
 {

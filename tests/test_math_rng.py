@@ -31,6 +31,55 @@ def test_xoshiro128ss_first_output_and_uniformity():
     assert abs(bits.mean() - 0.5) < 0.002
 
 
+def xoshiro128ss_reference(s, n):
+    """xoshiro128** 1.1 as published by Blackman and Vigna (prng.di.unimi.it, xoshiro128starstar.c):
+    result = rotl(s1 * 5, 7) * 9; t = s1 << 9; s2 ^= s0; s3 ^= s1; s1 ^= s2; s0 ^= s3;
+    s2 ^= t; s3 = rotl(s3, 11)."""
+    M = 0xffffffff
+    rotl = lambda x, k: ((x << k) | (x >> (32 - k))) & M
+    s = list(s)
+    out = []
+    for _ in range(n):
+        out.append(rotl((s[1] * 5) & M, 7) * 9 & M)
+        t = (s[1] << 9) & M
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = rotl(s[3], 11)
+    return out
+
+
+def test_xoshiro128ss_stream_equals_the_published_algorithm():
+    """The stream's words against the published step, from two seeds, over enough words to
+    exercise every state word."""
+    for seed in ([1, 2, 3, 4], [0x9e3779b9, 0x243f6a88, 0x85a308d3, 0xfffffffe]):
+        out, _ = oracle.xoshiro(seed, 4000)
+        assert list(np.asarray(out, dtype=np.uint64)) == xoshiro128ss_reference(seed, 4000)
+
+
+def test_div_rc_equals_ieee_division():
+    """x / c from the divisor's reciprocal with one correction step (plom_div_rc, the steps'
+    division by a population size) against IEEE division: equal on random quotients with
+    population-size and random divisors, on quotients whose mantissas lie next to 1 and 2
+    (where q = RN(x RN(1/c)) is furthest from x / c), with the sign of x / c for signed
+    zeros, and NaN for infinite x."""
+    rng = np.random.default_rng(11)
+    n = 400000
+    x = np.exp(rng.uniform(np.log(1e-6), np.log(1e9), n))
+    for c in (5e5, 1e6, 3e5, 123457.0, 1e6 / 3):
+        np.testing.assert_array_equal(oracle.div_rc(x, c), x / c)
+    c = np.exp(rng.uniform(-30, 30, n))
+    np.testing.assert_array_equal(oracle.div_rc(x, c), x / c)
+    xm = 2 - np.exp(rng.uniform(np.log(1e-16), np.log(1e-1), n))
+    for cm in (2 - np.exp(rng.uniform(np.log(1e-16), np.log(1e-2), n)), 1 + np.exp(rng.uniform(np.log(1e-16), np.log(1e-2), n))):
+        np.testing.assert_array_equal(oracle.div_rc(xm, cm), xm / cm)
+    z = oracle.div_rc(np.array([0.0, -0.0, 0.0, -0.0]), np.array([5e5, 5e5, -5e5, -5e5]))
+    assert list(np.signbit(z)) == [False, True, True, False] and np.all(z == 0.0)
+    assert np.all(np.isnan(oracle.div_rc(np.array([np.inf, -np.inf]), 5e5)))
+
+
 def test_normals_distribution():
     z = np.concatenate([oracle.normals(1, s, n, j, 64) for s in range(4) for n in range(16) for j in range(64)])
     assert stats.kstest(z, "norm").pvalue > 1e-3

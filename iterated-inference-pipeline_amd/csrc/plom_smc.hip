@@ -140,9 +140,11 @@ __global__ __launch_bounds__(TILE) void k_weights(plom_dfilter *filters, const i
 /* PLOM_FLAG_MIF, interval n with data, after k_weights: per tile, the sum of the weights
  * w_j = exp(l_j - m) (1 each when every particle failed, smc.c:55-70) and, per component,
  * the sum of w_j theta_j, the tile mean and the sum of squared deviations from it.  Every
- * sum is the tree of tree256 (a butterfly within each wave, then (w0 + w1) + (w2 + w3));
- * the components go through in chunks of 64 with one LDS exchange per pass. */
-#define MIF_CHUNK 64
+ * sum is the tree of tree256 (a butterfly within each wave, then (w0 + w1) + (w2 + w3)).
+ * Block (tile, y) takes components [y MIF_COMPS, (y + 1) MIF_COMPS): the grid spreads the
+ * components over many blocks (the sums are chains of dependent shuffles, so the blocks'
+ * number hides their latency).  Block y = 0 sums and writes the tile's weights. */
+#define MIF_COMPS 1
 __device__ inline double wave_sum(double v)
 {
 #pragma unroll
@@ -153,14 +155,17 @@ __device__ inline double wave_sum(double v)
 __global__ __launch_bounds__(TILE) void k_mif_tiles(plom_dfilter *filters, const int2 *tiles)
 {
     __shared__ double lds[8];
-    __shared__ double red[MIF_CHUNK][4][2];
-    __shared__ double smean[MIF_CHUNK];
+    __shared__ double red[MIF_COMPS][4][2];
+    __shared__ double smean[MIF_COMPS];
     int fi = 0, tile = blockIdx.x;
     if (tiles) { const int2 tt = tiles[blockIdx.x]; fi = tt.x; tile = tt.y; }
     plom_dfilter *f = &filters[fi];
     const int n = f->step;
     if (n >= f->N || !(f->flags & PLOM_FLAG_MIF) || f->nonan[n] == 0) return;
     const int J = f->J, Jp = f->Jp, T = f->T, nth = f->n_theta;
+    const int o0 = (int)blockIdx.y * MIF_COMPS;
+    if (o0 >= nth) return;                 /* uniform over the block */
+    const int nc = nth - o0 < MIF_COMPS ? nth - o0 : MIF_COMPS;
     const int j = tile * TILE + threadIdx.x;
     const bool valid = j < J;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -173,41 +178,45 @@ __global__ __launch_bounds__(TILE) void k_mif_tiles(plom_dfilter *filters, const
             w = 1.0;
         }
     }
-    const double sw = tree256(w, lds);
+    const double sw = blockIdx.y == 0 ? tree256(w, lds) : 0.0;   /* uniform over the block */
     const int rem = J - tile * TILE;
     const double ct = (double)(rem < TILE ? rem : TILE);
-    const double *th = f->th[(n + 1) & 1];
-    for (int o0 = 0; o0 < nth; o0 += MIF_CHUNK) {
-        const int nc = nth - o0 < MIF_CHUNK ? nth - o0 : MIF_CHUNK;
-        for (int c = 0; c < nc; c++) {
-            const double v = valid ? th[(size_t)(o0 + c) * Jp + j] : 0.0;
-            const double a = wave_sum(valid ? w * v : 0.0);
-            const double b = wave_sum(v);
+    const double *th = f->th[(n + 1) & 1] + (size_t)o0 * Jp;
+    double v[MIF_COMPS];
+#pragma unroll
+    for (int c = 0; c < MIF_COMPS; c++) v[c] = (valid && c < nc) ? th[(size_t)c * Jp + j] : 0.0;
+#pragma unroll
+    for (int c = 0; c < MIF_COMPS; c++) {
+        if (c < nc) {
+            const double a = wave_sum(valid ? w * v[c] : 0.0);
+            const double b = wave_sum(v[c]);
             if (lane == 0) { red[c][wv][0] = a; red[c][wv][1] = b; }
         }
-        __syncthreads();
-        if (threadIdx.x < nc) {
-            const int c = threadIdx.x;
-            const double s1 = (red[c][0][0] + red[c][1][0]) + (red[c][2][0] + red[c][3][0]);
-            const double mt = ((red[c][0][1] + red[c][1][1]) + (red[c][2][1] + red[c][3][1])) / ct;
-            smean[c] = mt;
-            f->mps[(size_t)(o0 + c) * T + tile] = s1;
-            f->mpm[(size_t)(o0 + c) * T + tile] = mt;
-        }
-        __syncthreads();
-        for (int c = 0; c < nc; c++) {
-            const double d = valid ? th[(size_t)(o0 + c) * Jp + j] - smean[c] : 0.0;
+    }
+    __syncthreads();
+    if (threadIdx.x < nc) {
+        const int c = threadIdx.x;
+        const double s1 = (red[c][0][0] + red[c][1][0]) + (red[c][2][0] + red[c][3][0]);
+        const double mt = ((red[c][0][1] + red[c][1][1]) + (red[c][2][1] + red[c][3][1])) / ct;
+        smean[c] = mt;
+        f->mps[(size_t)(o0 + c) * T + tile] = s1;
+        f->mpm[(size_t)(o0 + c) * T + tile] = mt;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < MIF_COMPS; c++) {
+        if (c < nc) {
+            const double d = valid ? v[c] - smean[c] : 0.0;
             const double q = wave_sum(d * d);
             if (lane == 0) red[c][wv][0] = q;
         }
-        __syncthreads();
-        if (threadIdx.x < nc) {
-            const int c = threadIdx.x;
-            f->mpq[(size_t)(o0 + c) * T + tile] = (red[c][0][0] + red[c][1][0]) + (red[c][2][0] + red[c][3][0]);
-        }
-        __syncthreads();
     }
-    if (threadIdx.x == 0) f->mpw[tile] = sw;
+    __syncthreads();
+    if (threadIdx.x < nc) {
+        const int c = threadIdx.x;
+        f->mpq[(size_t)(o0 + c) * T + tile] = (red[c][0][0] + red[c][1][0]) + (red[c][2][0] + red[c][3][0]);
+    }
+    if (blockIdx.y == 0 && threadIdx.x == 0) f->mpw[tile] = sw;
 }
 
 /* pairwise tree (contiguous, zero padded to a power of two) over the T tile values of a
@@ -1337,7 +1346,7 @@ static void enqueue_resample(hipStream_t s, plom_dfilter *df, const int2 *tiles,
     if (mif_theta > 0) {
         /* theta moments of the weighted cloud, before the scan and the interval counter move
          * on; mif_theta is the largest n_theta of the MIF filters */
-        hipLaunchKernelGGL(k_mif_tiles, dim3(ntiles), dim3(TILE), 0, s, df, tiles);
+        hipLaunchKernelGGL(k_mif_tiles, dim3(ntiles, (mif_theta + MIF_COMPS - 1) / MIF_COMPS), dim3(TILE), 0, s, df, tiles);
         hipLaunchKernelGGL(k_mif_reduce, dim3(nfilters, mif_theta), dim3(TILE), 0, s, df, fidx);
     }
     hipLaunchKernelGGL(k_scan, dim3(ntiles), dim3(TILE), 0, s, df, tiles);
@@ -1672,11 +1681,12 @@ static void chain_final(plom_ctx *c, batch_chain *ch, hipStream_t st)
     (void)c;
 }
 
-/* streams and events of the extra lanes (lane 0 is the context stream) */
+/* streams and events of the extra lanes: lane 0 is the context stream, lane l >= 1 is
+ * c->lane[l - 1] with join event c->join_ev[l - 1] */
 static int ctx_lanes(plom_ctx *c, int lanes)
 {
     if (!c->fork_ev) HIPCHK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
-    while ((int)c->lane.size() < lanes) {
+    while ((int)c->lane.size() < lanes - 1) {
         hipStream_t st;
         hipEvent_t ev;
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -1684,6 +1694,24 @@ static int ctx_lanes(plom_ctx *c, int lanes)
         c->lane.push_back(st);
         c->join_ev.push_back(ev);
     }
+    return PLOM_OK;
+}
+
+/* Destroy the extra lanes' streams (after their work).  The device has a few hardware
+ * queues per process (4 by default) and streams beyond that share them; a stream that
+ * shares the context stream's queue runs in its order, not beside it.  The batch lanes
+ * are released when a batch has been tuned and before a server creates its heavy lane,
+ * so that the heavy lane gets a queue of its own; a later launch with lanes creates
+ * them again. */
+static int ctx_release_lanes(plom_ctx *c)
+{
+    for (size_t l = 0; l < c->lane.size(); l++) {
+        HIPCHK(hipStreamSynchronize(c->lane[l]));
+        HIPCHK(hipStreamDestroy(c->lane[l]));
+        HIPCHK(hipEventDestroy(c->join_ev[l]));
+    }
+    c->lane.clear();
+    c->join_ev.clear();
     return PLOM_OK;
 }
 
@@ -1723,10 +1751,10 @@ static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing, int rounds)
         int rc = ctx_lanes(c, b->lanes);
         if (rc != PLOM_OK) return rc;
         HIPCHK(hipEventRecord(c->fork_ev, c->stream));
-        for (int l = 1; l < b->lanes; l++) HIPCHK(hipStreamWaitEvent(c->lane[(size_t)l], c->fork_ev, 0));
+        for (int l = 1; l < b->lanes; l++) HIPCHK(hipStreamWaitEvent(c->lane[(size_t)l - 1], c->fork_ev, 0));
         for (int l = 0; l < b->lanes; l++) {
             batch_chain *ch = &b->chains[(size_t)l];
-            hipStream_t st = l == 0 ? c->stream : c->lane[(size_t)l];
+            hipStream_t st = l == 0 ? c->stream : c->lane[(size_t)l - 1];
             chain_reset(ch, st);
             /* with timing on, every chain brackets its own launches with events on its own
              * stream: the durations are those of the concurrent launches */
@@ -1740,8 +1768,8 @@ static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing, int rounds)
             if (R == maxN) chain_final(c, ch, st);
         }
         for (int l = 1; l < b->lanes; l++) {
-            HIPCHK(hipEventRecord(c->join_ev[(size_t)l], c->lane[(size_t)l]));
-            HIPCHK(hipStreamWaitEvent(c->stream, c->join_ev[(size_t)l], 0));
+            HIPCHK(hipEventRecord(c->join_ev[(size_t)l - 1], c->lane[(size_t)l - 1]));
+            HIPCHK(hipStreamWaitEvent(c->stream, c->join_ev[(size_t)l - 1], 0));
         }
     }
     c->ktiming = ktiming;
@@ -1765,6 +1793,11 @@ static int batch_launch(plom_ctx *c, plom_batch *b, int use_graph)
     if (use_graph) {
         if (!b->graph) {
             hipGraph_t g;
+            /* the lanes' streams exist before the capture starts (none is created inside it) */
+            if (b->lanes > 1) {
+                int rl = ctx_lanes(c, b->lanes);
+                if (rl != PLOM_OK) return rl;
+            }
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int rc = batch_enqueue(c, b, 0, -1);
             hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -1941,6 +1974,9 @@ extern "C" int plom_batch_tune(plom_ctx *c, plom_batch *b, int rounds, plom_batc
     res.n_groups = (int32_t)(keys.size() < PLOM_TUNE_MAX_GROUPS ? keys.size() : PLOM_TUNE_MAX_GROUPS);
     b->tpb_rules = keys;
     int rc = batch_set_lanes(c, b, best);
+    if (rc != PLOM_OK) return rc;
+    /* the candidates' extra streams go; the batch's next launch creates what it uses */
+    rc = ctx_release_lanes(c);
     if (rc != PLOM_OK) return rc;
     if (out) *out = res;
     return PLOM_OK;
@@ -2229,6 +2265,10 @@ extern "C" int plom_server_create(plom_ctx *c, const plom_server_config *cfg, pl
     S->d_slots = nullptr;
     hipError_t e = hipMalloc((void **)&S->d_slots, (size_t)S->capacity * sizeof(plom_dfilter));
     if (e != hipSuccess) { srv_free(S); return set_err(PLOM_ERR_NOMEM, "hipMalloc slots: %s", hipGetErrorString(e)); }
+    /* the batch lanes' streams go first, so that the heavy lane's stream gets a hardware
+     * queue of its own instead of sharing the context stream's */
+    int rl = ctx_release_lanes(c);
+    if (rl != PLOM_OK) { srv_free(S); return rl; }
     for (int l = 0; l < SRV_LANES; l++) {
         srv_lane *L = &S->lanes[l];
         if (l == 0) {

@@ -57,6 +57,9 @@ struct plom_dfilter {
     int32_t *lin_slots; /* [N] or null: the sampled lineage's particle index per interval */
     double *lin_state;  /* [(N+1)][S] or null: its states */
     double *lin_obs;    /* [(N+1)][n_ts][2] or null: its observation mean and one drawn observation */
+    void *lin_snap;     /* [lin_nseg - 1][S][Jp] or null: the propagated cloud after intervals K-1,
+                           2K-1, ... (K = lin_K), where the lineage replay's segments start */
+    int32_t lin_K, lin_nseg;
     int32_t n_hat, pad2;
     uint64_t *hkey;     /* [n_hat][Jp] or null: per tile, the sorted keys of the hat quantities */
     uint64_t *hpref;    /* [n_hat][Jp]: inclusive prefix of their weights in that order */
@@ -386,6 +389,13 @@ __global__ __launch_bounds__(TILE * TPB) void k_propagate(plom_dfilter *filters,
         R *Xout = (R *)f->X[(n + 1) & 1];
 #pragma unroll
         for (int k = 0; k < M::S; k++) Xout[(size_t)k * Jp + j] = X[k];
+        /* PLOM_FLAG_LINEAGE: after intervals K-1, 2K-1, ... (below N-1) a copy of the cloud,
+         * where the segments of the lineage replay start */
+        if (f->lin_snap && (n + 1) % f->lin_K == 0 && n + 1 < f->N) {
+            R *A = (R *)f->lin_snap + (size_t)((n + 1) / f->lin_K - 1) * M::S * Jp;
+#pragma unroll
+            for (int k = 0; k < M::S; k++) A[(size_t)k * Jp + j] = X[k];
+        }
         if (f->allX) {
             R *A = (R *)f->allX + (size_t)(n + 1) * M::S * Jp;
 #pragma unroll
@@ -435,7 +445,7 @@ __global__ __launch_bounds__(TILE * TPB) void k_propagate(plom_dfilter *filters,
     if (wt && tid == 0) { f->tmax[tile] = mt; f->tsum[tile] = s; f->tsum2[tile] = s2; }
 }
 
-/* The lineage of one filter (one block of one wave; lane 0 works): the final particle by
+/* The lineage of an ode filter (one block of one wave; lane 0 works): the final particle by
  * inverse CDF on the last interval's cumulative weights (equal weights when that interval
  * had no data, was not filtered or every particle failed), its ancestors by the stored
  * ancestry, then its states regenerated from x0 with the same counter-based noise the
@@ -509,6 +519,122 @@ __global__ __launch_bounds__(64) void k_lineage(plom_dfilter *f)
     /* observation mean and one drawn observation per row (print_p_X, print.c:312-341:
      * obs_mean and observation(), core_template.c:369-412: mean + sd N(0,1), clipped at 0) */
     for (int n = 0; n <= N; n++) {
+        double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
+#pragma unroll
+        for (int k = 0; k < M::S; k++) xs[k] = f->lin_state[(size_t)n * M::S + k];
+        M::proj2obs(xs, f->pv, cov_at(f, n), obs);
+        M::obs_moments(obs, f->pv, cov_at(f, n), f->times[n], mu, var);
+        plom_lineage_obs(f->seed, f->stream, (uint32_t)n, M::N_TS, mu, var, plom_bm_tab, f->lin_obs + (size_t)n * M::N_TS * 2);
+    }
+}
+
+/* The lineage of a filter of the sde and psr implementations, in three launches.
+ *
+ * k_lineage_slots (one lane): the final particle by inverse CDF on the last interval's
+ * cumulative weights (equal weights when that interval had no data, was not filtered or
+ * every particle failed), then its ancestors by the stored ancestry:
+ * s[N-1] = first i with C_i 2^33 >= (2x+1) Q; s[n-1] = ancestry[n-1][s[n]].
+ *
+ * k_lineage_seg (one wave per segment, lane 0 works): the states regenerated with the same
+ * counter-based noise the filter drew, interval by interval,
+ * state[n+1] = interval_n(state[n], noise(n, s[n])).  Segment b replays intervals
+ * [bK, min((b+1)K, N)); it starts from x0 (b = 0) or from the cloud the propagate kernel
+ * copied after interval bK - 1, at particle s[bK - 1]: that copy is the state the
+ * sequential replay reaches there (the same function of the same inputs), so the segments
+ * give the sequential replay's states bit for bit.
+ *
+ * k_lineage_obs (one wave per row): the observation mean and one drawn observation of
+ * every row. */
+#define PLOM_LIN_SEG_MAX 16      /* segments of the lineage replay (grid of k_lineage_seg) */
+#define PLOM_LIN_OBS_BLOCKS 1024 /* grid of k_lineage_obs: rows b, b + 1024, ... */
+template <class M>
+__global__ __launch_bounds__(64) void k_lineage_slots(plom_dfilter *f)
+{
+    if (threadIdx.x != 0) return;
+    const int N = f->N, J = f->J;
+    if (N == 0 || !f->lin_slots) return;
+    const uint64_t unum = plom_lineage_num(f->seed, f->stream);
+    int j;
+    if (!(f->flags & PLOM_FLAG_NO_FILTER) && f->nonan[N - 1] > 0 && f->rflag[N - 1]) {
+        const uint64_t Q = f->tincl[f->T - 1];
+        j = select_first(f, (plom_u128)unum * (plom_u128)Q, 1ull << 33);
+    } else {
+        j = (int)((unum * (uint64_t)J + (1ull << 33) - 1ull) >> 33) - 1;
+    }
+    j = j < 0 ? 0 : (j >= J ? J - 1 : j);
+    f->lin_slots[N - 1] = j;
+    for (int n = N - 1; n >= 1; n--) {
+        j = f->ancestry[(size_t)(n - 1) * J + j];
+        f->lin_slots[n - 1] = j;
+    }
+}
+
+template <class M, typename R, int V>
+__global__ __launch_bounds__(64) void k_lineage_seg(plom_dfilter *f)
+{
+    constexpr bool PSR = V == 4 || V == 5;
+    const int N = f->N;
+    if (N == 0 || !f->lin_slots || (int)blockIdx.x >= f->lin_nseg) return;   /* the whole block */
+    __shared__ double s_icdf[PLOM_ICDF_TAB_SIZE];
+    __shared__ float s_icdf32[PLOM_ICDF_TAB_SIZE];
+    __shared__ double s_bm[PSR ? PLOM_BM_TAB_SIZE : 1];
+    __shared__ double s_lf[PSR ? 30 : 1];
+    if constexpr (PSR) {
+        for (int i = threadIdx.x; i < PLOM_BM_TAB_SIZE; i += 64) s_bm[i] = plom_bm_tab[i];
+        for (int i = threadIdx.x; i < 30; i += 64) s_lf[i] = plom_lfact_tab[i];
+    }
+    if constexpr (sizeof(R) == 8) {
+        for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += 64) s_icdf[i] = plom_icdf_tab[i];
+    }
+    if constexpr (sizeof(R) == 4) {
+        for (int i = threadIdx.x; i < PLOM_ICDF_TAB_SIZE; i += 64) s_icdf32[i] = plom_icdf_tab32[i];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int K = f->lin_K, b = (int)blockIdx.x;
+    const int n0 = b * K, n1 = (b + 1) * K < N ? (b + 1) * K : N;
+    R X[M::S];
+    if (b == 0) {
+        const R *x0 = (const R *)f->x0;
+#pragma unroll
+        for (int k = 0; k < M::S; k++) {
+            X[k] = x0[k];
+            f->lin_state[k] = (double)X[k];
+        }
+    } else {
+        const R *A = (const R *)f->lin_snap + (size_t)(b - 1) * M::S * f->Jp;
+        const size_t a = (size_t)f->lin_slots[n0 - 1];
+#pragma unroll
+        for (int k = 0; k < M::S; k++) X[k] = A[(size_t)k * f->Jp + a];
+    }
+    for (int n = n0; n < n1; n++) {
+        const double t0 = f->times[n], t1 = f->times[n + 1];
+        const uint32_t jn = (uint32_t)f->lin_slots[n];
+        if constexpr (sizeof(R) == 8) {
+            plom_nstream64 ns;
+            plom_nstream64_init(&ns, f->seed, f->stream, (uint32_t)n, jn, s_icdf);
+            if constexpr (PSR) plom_psr_tables(&ns, s_bm, s_lf);
+            M::template interval<V>((double *)X, f->pv, &ns, t0, t1, f->dt, f->sqdt, cov_steps(f, n), f->cov_cs,
+                                    f->drift_on ? &f->dv : nullptr, (const double *)nullptr);
+        } else {
+            plom_nstream32 ns;
+            plom_nstream32_init(&ns, f->seed, f->stream, (uint32_t)n, jn, s_icdf32);
+            M::template interval<V>((float *)X, f->pvf, &ns, t0, t1, f->dt, (float)f->sqdt, cov_steps(f, n), f->cov_cs,
+                                    f->drift_on ? &f->dv : nullptr, (const float *)nullptr);
+        }
+#pragma unroll
+        for (int k = 0; k < M::S; k++) f->lin_state[(size_t)(n + 1) * M::S + k] = (double)X[k];
+    }
+}
+
+/* observation mean and one drawn observation per row (print_p_X, print.c:312-341:
+ * obs_mean and observation(), core_template.c:369-412: mean + sd N(0,1), clipped at 0) */
+template <class M>
+__global__ __launch_bounds__(64) void k_lineage_obs(plom_dfilter *f)
+{
+    if (threadIdx.x != 0 || !f->lin_slots) return;
+    const int N = f->N;
+    for (int n = (int)blockIdx.x; n <= N; n += (int)gridDim.x) {
         double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
 #pragma unroll
         for (int k = 0; k < M::S; k++) xs[k] = f->lin_state[(size_t)n * M::S + k];
@@ -713,7 +839,7 @@ __global__ void k_hat0(plom_dfilter *f)
 /* What a model's translation unit exports: its dimensions and the host launchers of its
  * kernels.  abi and dfilter_bytes let the library refuse a model library compiled against
  * another version of this header (plom_model_load). */
-#define PLOM_MODEL_ABI 3
+#define PLOM_MODEL_ABI 4
 struct model_entry {
     int32_t abi;            /* PLOM_MODEL_ABI */
     int32_t dfilter_bytes;  /* sizeof(plom_dfilter) */
@@ -782,19 +908,31 @@ static void launch_prop(int prec, int variant, int mif, int ntiles, int tpb, hip
     }
 }
 
+template <class M, typename R, int V>
+static void launch_lineage_v(hipStream_t s, plom_dfilter *df)
+{
+    if constexpr (V == 6) {
+        hipLaunchKernelGGL((k_lineage<M, R, V>), dim3(1), dim3(64), 0, s, df);
+    } else {
+        hipLaunchKernelGGL((k_lineage_slots<M>), dim3(1), dim3(64), 0, s, df);
+        hipLaunchKernelGGL((k_lineage_seg<M, R, V>), dim3(PLOM_LIN_SEG_MAX), dim3(64), 0, s, df);
+        hipLaunchKernelGGL((k_lineage_obs<M>), dim3(PLOM_LIN_OBS_BLOCKS), dim3(64), 0, s, df);
+    }
+}
+
 template <class M, typename R>
 static void launch_lineage_r(int variant, hipStream_t s, plom_dfilter *df)
 {
     if constexpr (sizeof(R) == 8) {
-        if (variant == 4) { hipLaunchKernelGGL((k_lineage<M, R, 4>), dim3(1), dim3(64), 0, s, df); return; }
-        if (variant == 5) { hipLaunchKernelGGL((k_lineage<M, R, 5>), dim3(1), dim3(64), 0, s, df); return; }
-        if (variant == 6) { hipLaunchKernelGGL((k_lineage<M, R, 6>), dim3(1), dim3(64), 0, s, df); return; }
+        if (variant == 4) { launch_lineage_v<M, R, 4>(s, df); return; }
+        if (variant == 5) { launch_lineage_v<M, R, 5>(s, df); return; }
+        if (variant == 6) { launch_lineage_v<M, R, 6>(s, df); return; }
     }
     switch (variant) {
-    case 0: hipLaunchKernelGGL((k_lineage<M, R, 0>), dim3(1), dim3(64), 0, s, df); break;
-    case 1: hipLaunchKernelGGL((k_lineage<M, R, 1>), dim3(1), dim3(64), 0, s, df); break;
-    case 2: hipLaunchKernelGGL((k_lineage<M, R, 2>), dim3(1), dim3(64), 0, s, df); break;
-    default: hipLaunchKernelGGL((k_lineage<M, R, 3>), dim3(1), dim3(64), 0, s, df); break;
+    case 0: launch_lineage_v<M, R, 0>(s, df); break;
+    case 1: launch_lineage_v<M, R, 1>(s, df); break;
+    case 2: launch_lineage_v<M, R, 2>(s, df); break;
+    default: launch_lineage_v<M, R, 3>(s, df); break;
     }
 }
 

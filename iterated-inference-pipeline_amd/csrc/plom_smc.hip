@@ -661,13 +661,27 @@ struct plom_filter {
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+/* The segments of the lineage replay: K intervals each (at least 8, and at most
+ * PLOM_LIN_SEG_MAX segments), ceil(N / K) of them; the cloud is copied after intervals
+ * K - 1, 2K - 1, ... for every segment after the first.  The ode implementation replays
+ * its lineage in one segment. */
+static void lineage_segments(int N, uint32_t flags, int *K, int *nseg)
+{
+    if (N <= 0) { *K = 1; *nseg = 0; return; }
+    if (flags & PLOM_FLAG_ODE) { *K = N; *nseg = 1; return; }
+    int k = (N + PLOM_LIN_SEG_MAX - 1) / PLOM_LIN_SEG_MAX;
+    if (k < 8) k = 8;
+    *K = k;
+    *nseg = (N + k - 1) / k;
+}
+
 /* One device allocation per filter.  The work buffers come first; the tail from off_img
  * on (descriptor, counters, per-interval outputs, inputs) is written from one host image
  * in a single copy, so that creating a filter is one upload (from pinned memory when the
  * server creates it) and starts from zeroed counters without memset calls. */
 struct layout {
     size_t off_X0, off_X1, off_logw, off_cum, off_tmax, off_tsum, off_tsum2, off_ttot, off_tincl, off_anc,
-        off_lslots, off_lstate, off_lobs, off_hkey, off_hpref, off_hpart, off_hat, off_pres,
+        off_lslots, off_lstate, off_lobs, off_lsnap, off_hkey, off_hpref, off_hpart, off_hat, off_pres,
         off_th0, off_th1, off_mpw, off_mps, off_mpm, off_mpq, off_mmean, off_mvar,
         off_img, off_df, off_cnt, off_scal, off_naf, off_llike_n, off_ess_n, off_rflag, off_nonan, off_times,
         off_data, off_pv, off_pvf, off_x0, off_pm_src, off_pm_tr, off_pm_sc, off_pm_dur, off_pm_min, off_pm_max,
@@ -702,6 +716,11 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     PUT(off_lslots, (d->flags & PLOM_FLAG_LINEAGE) ? N * 4 : 0);
     PUT(off_lstate, (d->flags & PLOM_FLAG_LINEAGE) ? (N + 1) * S * 8 : 0);
     PUT(off_lobs, (d->flags & PLOM_FLAG_LINEAGE) ? (N + 1) * (size_t)mi->n_ts * 2 * 8 : 0);
+    {
+        int K, nseg;
+        lineage_segments((int)N, d->flags, &K, &nseg);
+        PUT(off_lsnap, ((d->flags & PLOM_FLAG_LINEAGE) && nseg > 1) ? (size_t)(nseg - 1) * S * Jp * rs : 0);
+    }
     const size_t NH = (d->flags & PLOM_FLAG_HAT) ? (size_t)mi->n_hat : 0;
     PUT(off_hkey, NH * Jp * 8);
     PUT(off_hpref, NH * Jp * 8);
@@ -1100,6 +1119,13 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     h->lin_slots = (d->flags & PLOM_FLAG_LINEAGE) ? (int32_t *)(b + L.off_lslots) : nullptr;
     h->lin_state = (d->flags & PLOM_FLAG_LINEAGE) ? (double *)(b + L.off_lstate) : nullptr;
     h->lin_obs = (d->flags & PLOM_FLAG_LINEAGE) ? (double *)(b + L.off_lobs) : nullptr;
+    {
+        int K, nseg;
+        lineage_segments(d->N, d->flags, &K, &nseg);
+        h->lin_K = K;
+        h->lin_nseg = nseg;
+        h->lin_snap = ((d->flags & PLOM_FLAG_LINEAGE) && nseg > 1) ? (void *)(b + L.off_lsnap) : nullptr;
+    }
     const int hat_on = (d->flags & PLOM_FLAG_HAT) != 0;
     h->n_hat = hat_on ? mi.n_hat : 0;
     h->hkey = hat_on ? (uint64_t *)(b + L.off_hkey) : nullptr;

@@ -17,6 +17,10 @@
  *                      attempts (transformed rejection, Marsaglia-Tsang acceptance).
  *   plom_lfact         log k!: exact table below 30, Stirling's series above (error
  *                      below 1e-14); the stream carries both tables (plom_psr_tables).
+ *                      plom_lfact2 sums two of them with one division for both.
+ * The transformed-rejection attempts take a2/us and us^2/(a + b us^2) from one reciprocal,
+ * and the setups pair their divisions the same way: fewer FP64 division sequences, each
+ * value within a few ulps of the separate quotient.
  *   plom_gamma_draw    Gamma(a, 1): Marsaglia and Tsang (2000, ACM TOMS 26:363-372) for
  *                      a >= 1; Gamma(a + 1) U^(1/a) below.
  *   plom_poisson_draw  Poisson(mu): the multiplication method (Knuth, TAOCP vol. 2,
@@ -81,16 +85,32 @@ static const double plom_lfact_tab[30] = PLOM_LFACT_TAB;
 
 /* log k! for an integer k >= 0: the table (lftab: plom_lfact_tab or a copy) below 30,
  * Stirling's series above; both are evaluated, without a branch */
-PLOM_HD double plom_lfact(double k, const double *ltab, const double *lftab)
+/* log k! with x = k + 1 and r = 1/x given */
+PLOM_HD double plom_lfact_r(double k, double x, double r, const double *ltab, const double *lftab)
 {
-    const double x = k + 1.0;
-    const double r = 1.0 / x, r2 = r * r;
+    const double r2 = r * r;
     /* ln Gamma(x) = (x - 1/2) ln x - x + ln sqrt(2 pi) + 1/(12x) - 1/(360x^3) + 1/(1260x^5) */
     const double st = (x - 0.5) * plom_log_tab(x, ltab) - x + PLOM_LN_SQRT_2PI
                       + r * (1.0 / 12.0 - r2 * (1.0 / 360.0 - r2 * (1.0 / 1260.0)));
     const int small = k < 30.0;
     const double tv = lftab[small ? (int)k : 0];
     return small ? tv : st;
+}
+
+PLOM_HD double plom_lfact(double k, const double *ltab, const double *lftab)
+{
+    const double x = k + 1.0;
+    return plom_lfact_r(k, x, 1.0 / x, ltab, lftab);
+}
+
+/* log k1! + log k2! with one division for both series: 1/x1 = x2 R and 1/x2 = x1 R from
+ * R = 1/(x1 x2) (x1 x2 exact below 2^53; a few ulps either way, far below what the
+ * 1/(12x) term needs) */
+PLOM_HD double plom_lfact2(double k1, double k2, const double *ltab, const double *lftab)
+{
+    const double x1 = k1 + 1.0, x2 = k2 + 1.0;
+    const double R = 1.0 / (x1 * x2);
+    return plom_lfact_r(k1, x1, x2 * R, ltab, lftab) + plom_lfact_r(k2, x2, x1 * R, ltab, lftab);
 }
 
 /* the stream's psr tables: the log table it holds and the log k! table (the host's) */
@@ -145,21 +165,27 @@ PLOM_HD_CALL double plom_poisson_draw(plom_nstream64 *ns, double mu)
     const double loglam = plom_log_tab(mu, ns->ltab);
     const double b = 0.931 + 2.53 * slam;
     const double a = -0.059 + 0.02483 * b;
-    const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
-    const double vr = 0.9277 - 3.6224 / (b - 2.0);
+    /* 1/(b - 3.4) and 1/(b - 2) from one division */
+    const double bm34 = b - 3.4, bm2 = b - 2.0;
+    const double Rb = 1.0 / (bm34 * bm2);
+    const double invalpha = 1.1239 + 1.1328 * (bm2 * Rb);
+    const double vr = 0.9277 - 3.6224 * (bm34 * Rb);
     const double a2 = 2.0 * a;
     for (int t = 0; t < PLOM_PSR_MAX_TRY; t++) {
         const double U = plom_unif32(ns) - 0.5;
         const double V = plom_unif32(ns);
         const double us = 0.5 - plom_fabs(U);
-        const double k = plom_floor((a2 / us + b) * U + mu + 0.43);
+        /* a2 / us and us^2 / (a + b us^2) from one division: R = 1 / (us (a + b us^2)) */
+        const double us2 = us * us;
+        const double den = plom_fma(b, us2, a);
+        const double R = 1.0 / (us * den);
+        const double k = plom_floor((a2 * (den * R) + b) * U + mu + 0.43);
         /* the quick acceptance, the early rejection and the full test, evaluated together
          * (no branch: a wave takes the full test whenever one of its lanes needs it), the
          * full test at k clamped to >= 0 where it is not used */
         const double kk = k < 0.0 ? 0.0 : k;
-        const double us2 = us * us;
         /* log(V invalpha / (a / us^2 + b)) */
-        const double lhs = plom_log_tab(V * invalpha * us2 / plom_fma(b, us2, a), ns->ltab);
+        const double lhs = plom_log_tab(V * invalpha * (us2 * us * R), ns->ltab);
         const double rhs = -mu + kk * loglam - plom_lfact(kk, ns->ltab, ns->lftab);
         if (us >= 0.07 && V <= vr) return k;
         if (k < 0.0 || (us < 0.013 && V > us)) continue;
@@ -200,26 +226,31 @@ PLOM_HD double plom_binom_half(plom_nstream64 *ns, double n, double p)
     const double b = 1.15 + 2.53 * spq;
     const double a = -0.0873 + 0.0248 * b + 0.01 * p;
     const double c = np + 0.5;
-    const double rb = 1.0 / b;
+    /* 1/b and p/q from one division: Rq = 1 / (b q) */
+    const double Rq = 1.0 / (b * q);
+    const double rb = q * Rq;
     const double vr = 0.92 - 4.2 * rb;
     const double alpha = (2.83 + 5.1 * rb) * spq;
     const double a2 = 2.0 * a;
-    const double lpq = plom_log_tab(p / q, ns->ltab);
+    const double lpq = plom_log_tab(p * (b * Rq), ns->ltab);
     const double m = plom_floor((n + 1.0) * p);
-    const double h = plom_lfact(m, ns->ltab, ns->lftab) + plom_lfact(n - m, ns->ltab, ns->lftab);
+    const double h = plom_lfact2(m, n - m, ns->ltab, ns->lftab);
     for (int t = 0; t < PLOM_PSR_MAX_TRY; t++) {
         const double u = plom_unif32(ns) - 0.5;
         const double v = plom_unif32(ns);
         const double us = 0.5 - plom_fabs(u);
-        const double k = plom_floor((a2 / us + b) * u + c);
+        /* a2 / us and us^2 / (a + b us^2) from one division: R = 1 / (us (a + b us^2)) */
+        const double us2 = us * us;
+        const double den = plom_fma(b, us2, a);
+        const double R = 1.0 / (us * den);
+        const double k = plom_floor((a2 * (den * R) + b) * u + c);
         /* the range check, the quick acceptance and the full test, evaluated together (no
          * branch: a wave takes the full test whenever one of its lanes needs it), the full
          * test at k clamped into [0, n] where it is not used */
         const double kk = k < 0.0 ? 0.0 : (k > n ? n : k);
-        const double us2 = us * us;
         /* log(v alpha / (a / us^2 + b)) */
-        const double lhs = plom_log_tab(v * alpha * us2 / plom_fma(b, us2, a), ns->ltab);
-        const double rhs = h - plom_lfact(kk, ns->ltab, ns->lftab) - plom_lfact(n - kk, ns->ltab, ns->lftab) + (kk - m) * lpq;
+        const double lhs = plom_log_tab(v * alpha * (us2 * us * R), ns->ltab);
+        const double rhs = h - plom_lfact2(kk, n - kk, ns->ltab, ns->lftab) + (kk - m) * lpq;
         if (k < 0.0 || k > n) continue;
         if (us >= 0.07 && v <= vr) return k;
         if (lhs <= rhs) return k;

@@ -279,7 +279,7 @@ def mif_pass(model_id, J, times, data, pv, mif, theta0, x0, mut_fac, noises_off=
             "mif_var": mv, "theta": tho, "logw": lw, "ancestry": anc}
 
 
-PSR_UNIFORM, PSR_GAMMA, PSR_POISSON, PSR_BINOMIAL, PSR_LFACT, PSR_MULTINOMIAL, PSR_UNIFORM32 = range(7)
+PSR_UNIFORM, PSR_GAMMA, PSR_POISSON, PSR_BINOMIAL, PSR_LFACT, PSR_MULTINOMIAL, PSR_UNIFORM32, PSR_LFACT2 = range(8)
 
 
 def psr_draws(kind, a, count, b=0.0, seed=1, stream=0, n=0, j=0, p=None):

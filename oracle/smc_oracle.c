@@ -108,7 +108,7 @@ int oracle_model_n_hat(int m) { return MODELS[m].n_hat; }
 int oracle_model_n_pp(int m) { return MODELS[m].n_pp; }
 /* count draws of one sampler from the particle stream (seed, stream, n, j), in order:
  * kind 0 uniform (two words), 1 Gamma(a, 1), 2 Poisson(a), 3 Binomial(a, b), 4 log a!, 6 uniform
- * (one word); for kind 5 the multinomial of N = a over the K = count probabilities in p,
+ * (one word), 7 log a! + log b! (plom_lfact2); for kind 5 the multinomial of N = a over the K = count probabilities in p,
  * written to out[0..K) */
 void oracle_psr_draws(int kind, double a, double b, uint32_t seed, uint32_t stream, uint32_t n, uint32_t j, int count,
                       const double *p, double *out)
@@ -127,6 +127,7 @@ void oracle_psr_draws(int kind, double a, double b, uint32_t seed, uint32_t stre
         case 2: out[i] = plom_poisson_draw(&ns, a); break;
         case 3: out[i] = plom_binom_draw(&ns, a, b); break;
         case 6: out[i] = plom_unif32(&ns); break;
+        case 7: out[i] = plom_lfact2(a + (double)i, b, plom_bm_tab, plom_lfact_tab); break;
         default: out[i] = plom_lfact(a + (double)i, plom_bm_tab, plom_lfact_tab); break;
         }
     }

@@ -96,6 +96,17 @@ def test_log_factorial():
         assert abs(got - exact) <= 1e-14 * max(1.0, abs(exact)), (k, got, exact)
 
 
+def test_log_factorial_pairs():
+    """log k1! + log k2! with one shared reciprocal (plom_lfact2, the binomial sampler's
+    two terms) against mpmath, on both sides of the table / series switch."""
+    pairs = [(0, 0), (5, 29), (29, 30), (30, 31), (40, 10 ** 6), (1000, 12345), (3 * 10 ** 7, 25),
+             (10 ** 6, 10 ** 6), (123457, 999983)]
+    for k1, k2 in pairs:
+        got = oracle.psr_draws(oracle.PSR_LFACT2, k1, 1, b=k2)[0]
+        exact = float(mp.loggamma(k1 + 1) + mp.loggamma(k2 + 1))
+        assert abs(got - exact) <= 1e-14 * max(1.0, abs(exact)), (k1, k2, got, exact)
+
+
 def test_multinomial_and_edges():
     p = np.array([0.1, 0.25, 0.05, 0.6])
     tot = np.zeros(4)

@@ -148,11 +148,13 @@ def batched_leg(smc, ctx, filters, steps, warmup, lanes=0):
     b.launch(use_graph=False)
     total_ms, prop_ms, scan_ms, nint, nprop = ctx.last_timing()
     lane_prop, lane_scan = ctx.last_timing_lanes()
+    prop_busy, scan_busy = ctx.last_timing_busy()
     ctx.set_kernel_timing(False)
     b.close()
     return {"wall_s": wall, "loglik": ll, "prop_ms": prop_ms, "scan_ms": scan_ms, "total_ms": total_ms,
             "launches": nprop, "rounds": nint, "lanes": lanes, "tuning": tuned,
-            "lane_prop_ms": lane_prop, "lane_scan_ms": lane_scan}
+            "lane_prop_ms": lane_prop, "lane_scan_ms": lane_scan, "prop_busy_ms": prop_busy,
+            "scan_busy_ms": scan_busy}
 
 
 def plain_leg(filters):
@@ -431,12 +433,18 @@ def main():
         out["batched"] = {
             "fp64": {"value": out["value"], "ms_per_pass": out["ms_per_step"], "graph": True,
                      "lanes": fp64["lanes"], "tuning": fp64["tuning"],
-                     "propagate_ms": round(fp64["prop_ms"], 3), "weights_scan_ms": round(fp64["scan_ms"], 3),
+                     "propagate_busy_ms": round(fp64["prop_busy_ms"], 3),
+                     "weights_scan_busy_ms": round(fp64["scan_busy_ms"], 3),
                      "launch_rounds": fp64["rounds"], "propagate_launches": fp64["launches"]}}
         B = bytes_per_particle_step(S, 8)
         nl = max(fp64["launches"], 1)
         per_launch = B * ps_batch / nl
-        avg_ms = fp64["prop_ms"] / nl
+        # the device time per launch: the union of the lanes' propagate spans over the
+        # launches (launches that run beside each other count once), so that the kernel
+        # time of a pass never exceeds the pass; each launch's own span, beside the other
+        # lanes' launches, is reported too
+        avg_ms = fp64["prop_busy_ms"] / nl
+        span_ms = fp64["prop_ms"] / nl
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
         prof = "k_propagate_sirs4_fp64_pmcmc_batch.md"
         valu = read_profile_value(prof, "valu_issue_frac")
@@ -451,7 +459,10 @@ def main():
               "algorithmic_bytes_per_particle_step": B,
               "algorithmic_bytes_per_launch": round(per_launch, 1),
               "avg_launch_ms": round(avg_ms, 5), "launches": fp64["launches"],
-              "propagate_ms_per_pass_by_lane": [round(x, 3) for x in fp64["lane_prop_ms"]],
+              "avg_launch_span_ms": round(span_ms, 5),
+              "propagate_busy_ms_per_pass": round(fp64["prop_busy_ms"], 3),
+              "timing_pass_ms": round(fp64["total_ms"], 3),
+              "propagate_span_ms_per_pass_by_lane": [round(x, 3) for x in fp64["lane_prop_ms"]],
               "pass_wall_GBps": round(B * ps_batch / fp64["wall_s"] / 1e9, 2),
               "measured_copy_GBps": round(copy_gbps, 1)}
         if fpps:

@@ -425,6 +425,10 @@ int plom_last_timing(plom_ctx *ctx, double *total_ms, double *propagate_ms, doub
 /* per lane l < cap of the last timed run: summed propagate and weight / scan durations;
  * n_lanes = lanes that recorded events */
 int plom_last_timing_lanes(plom_ctx *ctx, double *propagate_ms, double *scan_ms, int32_t cap, int32_t *n_lanes);
+/* the device time the last timed run's propagate launches (and its weight / scan
+ * launches) occupied: the length of the union of their event spans over all lanes, so
+ * that launches running beside each other count once; -1 without timing */
+int plom_last_timing_busy(plom_ctx *ctx, double *propagate_busy_ms, double *scan_busy_ms);
 
 /* device self-test of the shared arithmetic: out[i*8+..] = exp, log, Phi, sin, cos, sqrt,
  * sqrt_mid (the Box-Muller radius square root), sqrt_rate (the demographic noise square

@@ -2668,6 +2668,41 @@ extern "C" int plom_last_timing_lanes(plom_ctx *c, double *prop_ms, double *scan
     return PLOM_OK;
 }
 
+extern "C" int plom_last_timing_busy(plom_ctx *c, double *prop_busy_ms, double *scan_busy_ms)
+{
+    if (!c || !prop_busy_ms || !scan_busy_ms) return set_err(PLOM_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *prop_busy_ms = -1.0;
+    *scan_busy_ms = -1.0;
+    if (c->kspans.empty()) return PLOM_OK;
+    /* each span as [start, end] from the run's first event (recorded on the context stream
+     * before the lanes fork from it), then the length of the union per kind */
+    std::vector<std::pair<float, float>> iv[2];
+    for (const auto &sp : c->kspans) {
+        float a = 0.f, b = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, c->ev[0], c->kev[(size_t)sp.a]));
+        HIPCHK(hipEventElapsedTime(&b, c->ev[0], c->kev[(size_t)sp.b]));
+        iv[sp.kind == 0 ? 0 : 1].push_back({a, b});
+    }
+    double out[2] = { 0.0, 0.0 };
+    for (int kd = 0; kd < 2; kd++) {
+        auto &v = iv[kd];
+        std::sort(v.begin(), v.end());
+        double lo = 0.0, hi = 0.0, sum = 0.0;
+        bool open = false;
+        for (const auto &p : v) {
+            if (open && p.first <= hi) { if (p.second > hi) hi = p.second; continue; }
+            if (open) sum += hi - lo;
+            lo = p.first; hi = p.second; open = true;
+        }
+        if (open) sum += hi - lo;
+        out[kd] = sum;
+    }
+    *prop_busy_ms = out[0];
+    *scan_busy_ms = out[1];
+    return PLOM_OK;
+}
+
 extern "C" int plom_selftest_math(plom_ctx *c, int n, const double *in, double *out)
 {
     if (!c || n < 0 || (n > 0 && (!in || !out))) return set_err(PLOM_ERR_ARG, "bad arguments");

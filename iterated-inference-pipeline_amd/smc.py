@@ -85,7 +85,7 @@ EXPORTS = [
     "plom_model_name", "plom_model_info_get", "plom_model_load", "plom_smc_run", "plom_smc_run_noalloc", "plom_filter_create", "plom_filter_reset",
     "plom_filter_update", "plom_filter_mif_start", "plom_covariate_eval",
     "plom_filter_run", "plom_filter_read", "plom_filter_destroy", "plom_filter_device_bytes",
-    "plom_batch_run", "plom_sync", "plom_ctx_set_kernel_timing", "plom_last_timing", "plom_last_timing_lanes", "plom_selftest_math",
+    "plom_batch_run", "plom_sync", "plom_ctx_set_kernel_timing", "plom_last_timing", "plom_last_timing_lanes", "plom_last_timing_busy", "plom_selftest_math",
     "plom_selftest_normals", "plom_selftest_normals32", "plom_selftest_copy", "plom_batch_create", "plom_batch_launch",
     "plom_batch_destroy", "plom_batch_configure", "plom_batch_autotune", "plom_batch_tune",
     "plom_ctx_set_tiles_per_block", "plom_server_create", "plom_server_submit", "plom_server_step",
@@ -155,6 +155,7 @@ def lib():
     L.plom_ctx_set_kernel_timing.argtypes = [C.c_void_p, C.c_int]
     L.plom_last_timing.argtypes = [C.c_void_p, dp, dp, dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.plom_last_timing_lanes.argtypes = [C.c_void_p, dp, dp, C.c_int32, C.POINTER(C.c_int32)]
+    L.plom_last_timing_busy.argtypes = [C.c_void_p, dp, dp]
     L.plom_selftest_math.argtypes = [C.c_void_p, C.c_int, dp, dp]
     L.plom_selftest_normals.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
     L.plom_selftest_normals32.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
@@ -232,6 +233,13 @@ class Context:
         n = C.c_int32()
         check(lib().plom_last_timing_lanes(self.h, dptr(p), dptr(q), cap, C.byref(n)))
         return list(p[:n.value]), list(q[:n.value])
+
+    def last_timing_busy(self):
+        """(propagate_busy_ms, scan_busy_ms) of the last timed run: the union of the launches'
+        event spans over all lanes (concurrent launches count once)."""
+        p, q = C.c_double(), C.c_double()
+        check(lib().plom_last_timing_busy(self.h, C.byref(p), C.byref(q)))
+        return p.value, q.value
 
 
 # ---------------------------------------------------------------------- models

@@ -1744,9 +1744,10 @@ static int ctx_release_lanes(plom_ctx *c)
 /* the launch sequence of a batch (reset, every interval, final resampling); with
  * rounds >= 0 (the autotuner) only `rounds` launch rounds whose grids are those of
  * intervals spread evenly over the pass (a round costs what its active tiles cost, the
- * interval the device is at does not matter) and no final resampling; with timing (or one
- * lane) every chain runs on the context stream, interleaved round by round, between
- * per-round events */
+ * interval the device is at does not matter) and no final resampling.  With one lane every
+ * chain runs on the context stream, interleaved round by round; with several, each lane's
+ * chain runs on that lane's stream.  With timing, the propagate launches and the weight /
+ * scan launches of every round are bracketed by events on the stream they run on. */
 static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing, int rounds)
 {
     (void)hipGetLastError(); /* a not-ready query or an unused event earlier must not be reported as a launch failure */

@@ -187,3 +187,40 @@ def test_batch_autotune_keeps_results(ctx):
     b.close()
     for f in fs:
         f.close()
+
+
+def test_kernel_timing_busy_and_spans(ctx):
+    """Per-launch timing of a 2-lane batch and of a single filter: the union of the
+    propagate spans (plom_last_timing_busy) is at most the run and the sum of all spans,
+    at least each lane's own sum, and equals the sum on one stream, where spans cannot
+    overlap; the results with timing on equal those without."""
+    fs = [make(ctx, *c)[0] for c in CASES]
+    smc.batch_run(ctx, fs)
+    ref = [f.read().loglik for f in fs]
+    b = smc.Batch(ctx, fs)
+    b.configure(2)
+    ctx.set_kernel_timing(True)
+    try:
+        b.launch(use_graph=False)
+        total, prop, scan, nint, nprop = ctx.last_timing()
+        lane_prop, lane_scan = ctx.last_timing_lanes()
+        busy, sbusy = ctx.last_timing_busy()
+        print("total", total, "prop", prop, "busy", busy, "lanes", lane_prop, "scan busy", sbusy, flush=True)
+        assert [f.read().loglik for f in fs] == ref
+        assert len(lane_prop) == 2 and nprop > 0
+        assert 0.0 < busy <= total * (1 + 1e-6) and busy <= prop * (1 + 1e-6)
+        assert busy >= max(lane_prop) * (1 - 1e-4)
+        assert 0.0 < sbusy <= sum(lane_scan) * (1 + 1e-6)
+        b.close()
+        f = fs[1]
+        f.run()
+        total, prop, scan, nint, nprop = ctx.last_timing()
+        busy, sbusy = ctx.last_timing_busy()
+        print("single: total", total, "prop", prop, "busy", busy, flush=True)
+        assert nprop == nint == CASES[1][2]
+        assert busy == pytest.approx(prop, rel=1e-4) and sbusy == pytest.approx(scan, rel=1e-4)
+        assert busy + sbusy <= total * (1 + 1e-6)
+    finally:
+        ctx.set_kernel_timing(False)
+    for f in fs:
+        f.close()

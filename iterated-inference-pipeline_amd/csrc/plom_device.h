@@ -76,6 +76,8 @@ struct plom_dfilter {
     const int32_t *pm_prior, *pm_puntil;   /* [n_theta] prior patch of the weights, or null */
     const double *pm_ppow;                 /* [n_theta] */
     double *th[2];                  /* [n_theta][Jp] theta of the particles, ping-pong like X */
+    double *mw;                     /* [Jp] the weights exp(l - m) of the last resampled interval
+                                       (k_scan), read by the theta moments */
     double *mpw, *mps, *mpm, *mpq;  /* tile partials: [T] sum w, [n_theta][T] sum w theta, mean, M2 */
     double *mmean, *mvar;           /* [(N+1)][n_theta] weighted mean and variance of theta */
     void *allX;                     /* PLOM_FLAG_ALL_STATES: [(N+1)][S][Jp] every propagated cloud */
@@ -839,7 +841,7 @@ __global__ void k_hat0(plom_dfilter *f)
 /* What a model's translation unit exports: its dimensions and the host launchers of its
  * kernels.  abi and dfilter_bytes let the library refuse a model library compiled against
  * another version of this header (plom_model_load). */
-#define PLOM_MODEL_ABI 4
+#define PLOM_MODEL_ABI 5
 struct model_entry {
     int32_t abi;            /* PLOM_MODEL_ABI */
     int32_t dfilter_bytes;  /* sizeof(plom_dfilter) */

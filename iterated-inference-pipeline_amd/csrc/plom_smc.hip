@@ -137,13 +137,16 @@ __global__ __launch_bounds__(TILE) void k_weights(plom_dfilter *filters, const i
     finalize_weights(f, n, lds);
 }
 
-/* PLOM_FLAG_MIF, interval n with data, after k_weights: per tile, the sum of the weights
+/* PLOM_FLAG_MIF, interval n with data, after k_scan: per tile, the sum of the weights
  * w_j = exp(l_j - m) (1 each when every particle failed, smc.c:55-70) and, per component,
  * the sum of w_j theta_j, the tile mean and the sum of squared deviations from it.  Every
  * sum is the tree of tree256 (a butterfly within each wave, then (w0 + w1) + (w2 + w3)).
  * Block (tile, y) takes components [y MIF_COMPS, (y + 1) MIF_COMPS): the grid spreads the
  * components over many blocks (the sums are chains of dependent shuffles, so the blocks'
- * number hides their latency).  Block y = 0 sums and writes the tile's weights. */
+ * number hides their latency).  Block y = 0 sums and writes the tile's weights.  The
+ * interval n is an argument and the weights come from k_scan's copy (mw), not from the
+ * log-weights and the maximum: so the kernel can run beside the next interval's
+ * propagate, which overwrites those. */
 #define MIF_COMPS 1
 __device__ inline double wave_sum(double v)
 {
@@ -152,7 +155,7 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-__global__ __launch_bounds__(TILE) void k_mif_tiles(plom_dfilter *filters, const int2 *tiles)
+__global__ __launch_bounds__(TILE) void k_mif_tiles(plom_dfilter *filters, const int2 *tiles, int n)
 {
     __shared__ double lds[8];
     __shared__ double red[MIF_COMPS][4][2];
@@ -160,7 +163,6 @@ __global__ __launch_bounds__(TILE) void k_mif_tiles(plom_dfilter *filters, const
     int fi = 0, tile = blockIdx.x;
     if (tiles) { const int2 tt = tiles[blockIdx.x]; fi = tt.x; tile = tt.y; }
     plom_dfilter *f = &filters[fi];
-    const int n = f->step;
     if (n >= f->N || !(f->flags & PLOM_FLAG_MIF) || f->nonan[n] == 0) return;
     const int J = f->J, Jp = f->Jp, T = f->T, nth = f->n_theta;
     const int o0 = (int)blockIdx.y * MIF_COMPS;
@@ -170,14 +172,7 @@ __global__ __launch_bounds__(TILE) void k_mif_tiles(plom_dfilter *filters, const
     const bool valid = j < J;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     double w = 0.0;
-    if (valid) {
-        if (f->rflag[n]) {
-            const double l = f->logw[j];
-            w = (l == -PLOM_INF) ? 0.0 : plom_exp(l - f->scal[0]);
-        } else {
-            w = 1.0;
-        }
-    }
+    if (valid) w = f->rflag[n] ? f->mw[j] : 1.0;
     const double sw = blockIdx.y == 0 ? tree256(w, lds) : 0.0;   /* uniform over the block */
     const int rem = J - tile * TILE;
     const double ct = (double)(rem < TILE ? rem : TILE);
@@ -247,7 +242,7 @@ __device__ double tile_tree(const double *v, int T, double *lds, double *chunks)
 
 /* one block per (filter, component): weighted mean sum(w theta)/sum(w) and variance
  * M2/(J-1), the (count, mean, M2) of the tiles merged along the same fixed tree */
-__global__ __launch_bounds__(TILE) void k_mif_reduce(plom_dfilter *filters, const int *fidx)
+__global__ __launch_bounds__(TILE) void k_mif_reduce(plom_dfilter *filters, const int *fidx, int n)
 {
     __shared__ double lds[8];
     __shared__ double chunks[MAX_TILES / TILE + 1];
@@ -255,7 +250,6 @@ __global__ __launch_bounds__(TILE) void k_mif_reduce(plom_dfilter *filters, cons
     __shared__ double cc[MAX_TILES / TILE + 1], cm[MAX_TILES / TILE + 1], cq[MAX_TILES / TILE + 1];
     const int fi = fidx ? fidx[blockIdx.x] : (int)blockIdx.x;
     plom_dfilter *f = &filters[fi];
-    const int n = f->step;
     if (n >= f->N || !(f->flags & PLOM_FLAG_MIF) || f->nonan[n] == 0) return;
     const int J = f->J, T = f->T, nth = f->n_theta;
     const int o = blockIdx.y;
@@ -314,7 +308,12 @@ __global__ __launch_bounds__(TILE) void k_scan(plom_dfilter *filters, const int2
     const int j = tile * TILE + threadIdx.x;
     const double l = f->logw[j];
     uint64_t q = 0;
-    if (l != -PLOM_INF) q = (uint64_t)((plom_exp(l - m) / S) * 0x1p62);
+    double e = 0.0;
+    if (l != -PLOM_INF) {
+        e = plom_exp(l - m);
+        q = (uint64_t)((e / S) * 0x1p62);
+    }
+    if (f->mw) f->mw[j] = e;       /* PLOM_FLAG_MIF: the weights of the theta moments */
     const uint64_t c = scan256(q, lds);
     f->cum[j] = c;
     if (threadIdx.x == TILE - 1) f->ttot[tile] = c;
@@ -605,6 +604,8 @@ struct plom_ctx {
     std::vector<hipStream_t> lane;  /* batch lane streams, created on first use ([0] unused: lane 0 is `stream`) */
     std::vector<hipEvent_t> join_ev;
     hipEvent_t fork_ev;
+    hipStream_t mside;              /* the theta moments of MIF filters beside the next propagate */
+    hipEvent_t mside_fork, mside_done;
     int tpb;                        /* tiles per propagate block of single filters, the server and
                                        batch groups without a tuned value */
     char *scratch;                  /* this device's fixed scratch (plom_smc_run_noalloc), or null */
@@ -682,7 +683,7 @@ static void lineage_segments(int N, uint32_t flags, int *K, int *nseg)
 struct layout {
     size_t off_X0, off_X1, off_logw, off_cum, off_tmax, off_tsum, off_tsum2, off_ttot, off_tincl, off_anc,
         off_lslots, off_lstate, off_lobs, off_lsnap, off_hkey, off_hpref, off_hpart, off_hat, off_pres,
-        off_th0, off_th1, off_mpw, off_mps, off_mpm, off_mpq, off_mmean, off_mvar,
+        off_th0, off_th1, off_mw, off_mpw, off_mps, off_mpm, off_mpq, off_mmean, off_mvar,
         off_img, off_df, off_cnt, off_scal, off_naf, off_llike_n, off_ess_n, off_rflag, off_nonan, off_times,
         off_data, off_pv, off_pvf, off_x0, off_pm_src, off_pm_tr, off_pm_sc, off_pm_dur, off_pm_min, off_pm_max,
         off_pm_mult, off_pm_sd, off_mutfac, off_cov, off_dm_group, off_dm_ng, off_dm_tr, off_dm_sc, off_dm_dur,
@@ -730,6 +731,7 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     const size_t NT = (d->flags & PLOM_FLAG_MIF) ? (size_t)d->n_theta : 0;
     PUT(off_th0, NT * Jp * 8);
     PUT(off_th1, NT * Jp * 8);
+    PUT(off_mw, NT ? Jp * 8 : 0);
     PUT(off_mpw, NT ? T * 8 : 0);
     PUT(off_mps, NT * T * 8);
     PUT(off_mpm, NT * T * 8);
@@ -903,6 +905,12 @@ extern "C" int plom_ctx_destroy(plom_ctx *c)
         HIPCHK(hipEventDestroy(c->join_ev[l]));
     }
     if (c->fork_ev) HIPCHK(hipEventDestroy(c->fork_ev));
+    if (c->mside) {
+        HIPCHK(hipStreamSynchronize(c->mside));
+        HIPCHK(hipStreamDestroy(c->mside));
+        HIPCHK(hipEventDestroy(c->mside_fork));
+        HIPCHK(hipEventDestroy(c->mside_done));
+    }
     HIPCHK(hipStreamDestroy(c->stream));
     HIPCHK(hipStreamDestroy(c->io));
     c->~plom_ctx();
@@ -1151,6 +1159,7 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     h->pm_puntil = prior_on ? (const int32_t *)(b + L.off_pm_puntil) : nullptr;
     h->th[0] = mif_on ? (double *)(b + L.off_th0) : nullptr;
     h->th[1] = mif_on ? (double *)(b + L.off_th1) : nullptr;
+    h->mw = mif_on ? (double *)(b + L.off_mw) : nullptr;
     h->mpw = mif_on ? (double *)(b + L.off_mpw) : nullptr;
     h->mps = mif_on ? (double *)(b + L.off_mps) : nullptr;
     h->mpm = mif_on ? (double *)(b + L.off_mpm) : nullptr;
@@ -1363,20 +1372,71 @@ extern "C" int plom_filter_destroy(plom_ctx *c, plom_filter *f)
     return PLOM_OK;
 }
 
-/* weights, fixed-point scan and tile prefix of one interval: K2 (one block per filter),
- * K3 (one block per tile), K4 (one block per filter) */
-static void enqueue_resample(hipStream_t s, plom_dfilter *df, const int2 *tiles, int ntiles, const int *fidx, int nfilters,
-                             int mif_theta = 0)
+/* A side stream for the theta moments of MIF filters (k_mif_tiles, k_mif_reduce): the
+ * moments of interval n run beside interval n + 1's propagate instead of before it.  They
+ * read the scan's copy of the weights (mw), the interval's flags and the theta table the
+ * next propagate only reads, and write buffers nothing else touches; the next scan, which
+ * rewrites mw, waits for them.  `pending`: moments enqueued and not yet waited for. */
+struct mif_side {
+    hipStream_t st;
+    hipEvent_t fork, done;
+    int pending;
+};
+
+static int ctx_mif_side(plom_ctx *c, mif_side *sd)
+{
+    if (!c->mside) {
+        HIPCHK(hipStreamCreateWithFlags(&c->mside, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&c->mside_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->mside_done, hipEventDisableTiming));
+    }
+    sd->st = c->mside;
+    sd->fork = c->mside_fork;
+    sd->done = c->mside_done;
+    sd->pending = 0;
+    return PLOM_OK;
+}
+
+/* the stream s waits for the moments on the side stream */
+static int mif_side_join(hipStream_t s, mif_side *sd)
+{
+    if (!sd || !sd->pending) return PLOM_OK;
+    HIPCHK(hipStreamWaitEvent(s, sd->done, 0));
+    sd->pending = 0;
+    return PLOM_OK;
+}
+
+/* weights, fixed-point scan and tile prefix of interval n: K2 (one block per filter), K3
+ * (one block per tile), K4 (one block per filter); with MIF filters (mif_theta, the largest
+ * n_theta of them, > 0) the theta moments of the weighted cloud after the scan, on the
+ * stream s or, with a side stream, beside the next interval */
+static int enqueue_resample(hipStream_t s, plom_dfilter *df, const int2 *tiles, int ntiles, const int *fidx, int nfilters,
+                            int mif_theta = 0, int n = 0, mif_side *side = nullptr)
 {
     hipLaunchKernelGGL(k_weights, dim3(nfilters), dim3(TILE), 0, s, df, fidx);
     if (mif_theta > 0) {
-        /* theta moments of the weighted cloud, before the scan and the interval counter move
-         * on; mif_theta is the largest n_theta of the MIF filters */
-        hipLaunchKernelGGL(k_mif_tiles, dim3(ntiles, (mif_theta + MIF_COMPS - 1) / MIF_COMPS), dim3(TILE), 0, s, df, tiles);
-        hipLaunchKernelGGL(k_mif_reduce, dim3(nfilters, mif_theta), dim3(TILE), 0, s, df, fidx);
+        const dim3 gt(ntiles, (mif_theta + MIF_COMPS - 1) / MIF_COMPS), gr(nfilters, mif_theta);
+        if (side) {
+            if (mif_side_join(s, side) != PLOM_OK) return PLOM_ERR_HIP;   /* k_scan rewrites mw */
+            hipLaunchKernelGGL(k_scan, dim3(ntiles), dim3(TILE), 0, s, df, tiles);
+            hipLaunchKernelGGL(k_advance, dim3(nfilters), dim3(TILE), 0, s, df, fidx);
+            HIPCHK(hipEventRecord(side->fork, s));
+            HIPCHK(hipStreamWaitEvent(side->st, side->fork, 0));
+            hipLaunchKernelGGL(k_mif_tiles, gt, dim3(TILE), 0, side->st, df, tiles, n);
+            hipLaunchKernelGGL(k_mif_reduce, gr, dim3(TILE), 0, side->st, df, fidx, n);
+            HIPCHK(hipEventRecord(side->done, side->st));
+            side->pending = 1;
+            return PLOM_OK;
+        }
+        hipLaunchKernelGGL(k_scan, dim3(ntiles), dim3(TILE), 0, s, df, tiles);
+        hipLaunchKernelGGL(k_mif_tiles, gt, dim3(TILE), 0, s, df, tiles, n);
+        hipLaunchKernelGGL(k_mif_reduce, gr, dim3(TILE), 0, s, df, fidx, n);
+        hipLaunchKernelGGL(k_advance, dim3(nfilters), dim3(TILE), 0, s, df, fidx);
+        return PLOM_OK;
     }
     hipLaunchKernelGGL(k_scan, dim3(ntiles), dim3(TILE), 0, s, df, tiles);
     hipLaunchKernelGGL(k_advance, dim3(nfilters), dim3(TILE), 0, s, df, fidx);
+    return PLOM_OK;
 }
 
 static void enqueue_final(plom_ctx *c, plom_filter *f, const int2 *tiles, plom_dfilter *df, int nblocks)
@@ -1413,15 +1473,19 @@ extern "C" int plom_filter_run(plom_ctx *c, plom_filter *f)
     c->kprop_launches = 0;
     c->kspans.clear();
     c->kspan_rounds = c->ktiming ? f->desc.N : 0;
+    mif_side side;
+    if (mif && ctx_mif_side(c, &side) != PLOM_OK) return PLOM_ERR_HIP;
     for (int n = 0; n < f->desc.N; n++) {
         const int a = kspan_open(c, &k, c->stream);
         me->launch_prop(f->desc.precision, f->variant, mif, T, c->tpb, c->stream, f->df, nullptr);
         c->kprop_launches++;
         const int b = kspan_close(c, &k, c->stream, a, 0, 0);
-        enqueue_resample(c->stream, f->df, nullptr, T, nullptr, 1, mif ? f->desc.n_theta : 0);
+        if (enqueue_resample(c->stream, f->df, nullptr, T, nullptr, 1, mif ? f->desc.n_theta : 0, n, mif ? &side : nullptr) != PLOM_OK)
+            return PLOM_ERR_HIP;
         if (kspan_close(c, &k, c->stream, b, 1, 0) == -2) return PLOM_ERR_HIP;
         if (f->desc.flags & PLOM_FLAG_HAT) launch_hat_all(me, f->desc.precision, c->stream, f->df, T, f->info.n_hat);
     }
+    if (mif && mif_side_join(c->stream, &side) != PLOM_OK) return PLOM_ERR_HIP;
     if ((f->desc.flags & PLOM_FLAG_HAT) && f->desc.N > 0) me->launch_hat0(f->desc.precision, c->stream, f->df);
     if (needs_final(f)) enqueue_final(c, f, nullptr, f->df, T);
     if (f->desc.N > 0 && (f->desc.flags & PLOM_FLAG_LINEAGE))
@@ -1675,17 +1739,19 @@ static int chain_propagate(batch_chain *ch, int s, hipStream_t st)
     return launches;
 }
 
-static void chain_resample(batch_chain *ch, int s, hipStream_t st)
+static int chain_resample(batch_chain *ch, int s, hipStream_t st, mif_side *side = nullptr)
 {
     const int nb = prefix_len(ch, ch->all_tiles, s);
     int nfa = 0;
     while (nfa < (int)ch->fs.size() && ch->fs[(size_t)nfa]->desc.N > s) nfa++;
-    enqueue_resample(st, ch->ddf, (const int2 *)ch->d_all, nb, nullptr, nfa, ch->any_mif);
+    if (enqueue_resample(st, ch->ddf, (const int2 *)ch->d_all, nb, nullptr, nfa, ch->any_mif, s, ch->any_mif ? side : nullptr) != PLOM_OK)
+        return PLOM_ERR_HIP;
     for (int r = 0; r < nfa; r++) {
         plom_filter *f = ch->fs[(size_t)r];
         if (f->desc.flags & PLOM_FLAG_HAT)
             launch_hat_all(model_at(f->desc.model_id), f->desc.precision, st, ch->ddf + r, f->h.T, f->info.n_hat);
     }
+    return PLOM_OK;
 }
 
 static void chain_final(plom_ctx *c, batch_chain *ch, hipStream_t st)
@@ -1723,12 +1789,12 @@ static int ctx_lanes(plom_ctx *c, int lanes)
     return PLOM_OK;
 }
 
-/* Destroy the extra lanes' streams (after their work).  The device has a few hardware
- * queues per process (4 by default) and streams beyond that share them; a stream that
- * shares the context stream's queue runs in its order, not beside it.  The batch lanes
- * are released when a batch has been tuned and before a server creates its heavy lane,
- * so that the heavy lane gets a queue of its own; a later launch with lanes creates
- * them again. */
+/* Destroy the extra lanes' streams and the MIF side stream (after their work).  The
+ * device has a few hardware queues per process (4 by default) and streams beyond that
+ * share them; a stream that shares the context stream's queue runs in its order, not
+ * beside it.  They are released when a batch has been tuned and before a server creates
+ * its heavy lane, so that the heavy lane gets a queue of its own; a later launch that
+ * needs them creates them again. */
 static int ctx_release_lanes(plom_ctx *c)
 {
     for (size_t l = 0; l < c->lane.size(); l++) {
@@ -1738,6 +1804,13 @@ static int ctx_release_lanes(plom_ctx *c)
     }
     c->lane.clear();
     c->join_ev.clear();
+    if (c->mside) {
+        HIPCHK(hipStreamSynchronize(c->mside));
+        HIPCHK(hipStreamDestroy(c->mside));
+        HIPCHK(hipEventDestroy(c->mside_fork));
+        HIPCHK(hipEventDestroy(c->mside_done));
+        c->mside = nullptr;
+    }
     return PLOM_OK;
 }
 
@@ -1761,6 +1834,11 @@ static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing, int rounds)
     c->ktiming = timing;
     int k = 0;
     if (b->lanes == 1) {
+        /* one lane: MIF moments on the side stream, beside the next round's propagate */
+        bool any_mif = false;
+        for (auto &ch : b->chains) any_mif = any_mif || ch.any_mif > 0;
+        mif_side side;
+        if (any_mif && ctx_mif_side(c, &side) != PLOM_OK) { c->ktiming = ktiming; return PLOM_ERR_HIP; }
         for (auto &ch : b->chains) chain_reset(&ch, c->stream);
         for (int i = 0; i < R; i++) {
             const int s = i * stride;
@@ -1769,9 +1847,13 @@ static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing, int rounds)
                 if (s < ch.maxN) c->kprop_launches += chain_propagate(&ch, s, c->stream);
             const int m = kspan_close(c, &k, c->stream, a, 0, 0);
             for (auto &ch : b->chains)
-                if (s < ch.maxN) chain_resample(&ch, s, c->stream);
+                if (s < ch.maxN && chain_resample(&ch, s, c->stream, any_mif ? &side : nullptr) != PLOM_OK) {
+                    c->ktiming = ktiming;
+                    return PLOM_ERR_HIP;
+                }
             if (kspan_close(c, &k, c->stream, m, 1, 0) == -2) { c->ktiming = ktiming; return PLOM_ERR_HIP; }
         }
+        if (any_mif && mif_side_join(c->stream, &side) != PLOM_OK) { c->ktiming = ktiming; return PLOM_ERR_HIP; }
         if (R == maxN)
             for (auto &ch : b->chains) chain_final(c, &ch, c->stream);
     } else {
@@ -1789,7 +1871,7 @@ static int batch_enqueue(plom_ctx *c, plom_batch *b, int timing, int rounds)
                 const int a = kspan_open(c, &k, st);
                 c->kprop_launches += chain_propagate(ch, i * stride, st);
                 const int m = kspan_close(c, &k, st, a, 0, l);
-                chain_resample(ch, i * stride, st);
+                if (chain_resample(ch, i * stride, st) != PLOM_OK) { c->ktiming = ktiming; return PLOM_ERR_HIP; }
                 if (kspan_close(c, &k, st, m, 1, l) == -2) { c->ktiming = ktiming; return PLOM_ERR_HIP; }
             }
             if (R == maxN) chain_final(c, ch, st);
@@ -1820,10 +1902,16 @@ static int batch_launch(plom_ctx *c, plom_batch *b, int use_graph)
     if (use_graph) {
         if (!b->graph) {
             hipGraph_t g;
-            /* the lanes' streams exist before the capture starts (none is created inside it) */
+            /* the lanes' streams and the MIF side stream exist before the capture starts (none
+             * is created inside it) */
             if (b->lanes > 1) {
                 int rl = ctx_lanes(c, b->lanes);
                 if (rl != PLOM_OK) return rl;
+            } else {
+                bool any_mif = false;
+                for (auto &ch : b->chains) any_mif = any_mif || ch.any_mif > 0;
+                mif_side tmp;
+                if (any_mif && ctx_mif_side(c, &tmp) != PLOM_OK) return PLOM_ERR_HIP;
             }
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int rc = batch_enqueue(c, b, 0, -1);

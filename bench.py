@@ -17,8 +17,8 @@ Legs on the same JSON line:
             result read before the next one starts), FP64: the baseline of the speedup
   batched   the headline numbers, the FP32-propagation pass of the same batch and its
             deviation from FP64
-  served    a Poisson stream (200 requests/s) of SIR / SIRS requests, J log-uniform in
-            [500, 5e4], N ~ U{52..520}, served with continuous batching and admission by
+  served    a Poisson stream (200 requests/s for 60 s) of SIR / SIRS requests, J log-uniform
+            in [500, 5e4], N ~ U{52..520}, served with continuous batching and admission by
             free HBM: sustained particle-steps/s and p50 / p99 latency (submit -> result);
             then the same mix offered at 400, 600 and 800 requests/s (10 s each): the capacity is
             the highest sustained rate
@@ -28,7 +28,8 @@ Legs on the same JSON line:
   mif       one iterated-filtering pass of the same filter with per-particle parameters
             (28 per-particle slots, random walk after every observation, theta moments)
   roofline  of the dominant kernel (k_propagate of the batch), from HIP events around
-            every propagate launch on the library's stream
+            every propagate launch on the stream it runs on (the device time the lanes'
+            launches occupy)
   cpu_baseline  the C oracle (this project's restatement of the reference filter) on a
             bounded sample of the headline workload, timed on the host cores
 
@@ -167,9 +168,11 @@ def plain_leg(filters):
     return {"wall_s": time.perf_counter() - t0, "loglik": np.array(ll)}
 
 
-def served_leg(smc, ctx, inp, configs, seconds, rate, check_n=16, heavy=None):
+def served_leg(smc, ctx, inp, configs, seconds, rate, check_n=16, heavy=None, window_s=None):
     """heavy: an extra request (dict with model, J, N, stream) submitted with the first
-    arrival, with lineage, served alongside the stream."""
+    arrival, with lineage, served alongside the stream.  window_s: also the p99 latency of
+    the requests that arrived in the first window_s seconds (the stream of that length is
+    a prefix of this one)."""
     reqs = configs.served_mix(rate=rate, duration_s=seconds)
     prepared = [inp.get(r["model"], r["N"], r["params"]) for r in reqs]
     srv = smc.Server(ctx)
@@ -232,6 +235,10 @@ def served_leg(smc, ctx, inp, configs, seconds, rate, check_n=16, heavy=None):
             "checked_alone": len(sample), "max_rel_err_vs_alone": max_rel}
     if heavy is not None:
         out["heavy"] = heavy_out
+    if window_s is not None:
+        w = np.array([lat_ms[k] for k in range(len(reqs)) if reqs[k]["t"] < window_s])
+        out["window"] = {"seconds": window_s, "requests": int(w.size),
+                         "p99_ms": float(np.percentile(w, 99)) if w.size else None}
     return out
 
 
@@ -375,7 +382,9 @@ def main():
     ap.add_argument("--legs", default=",".join(LEGS), help="comma list of " + ",".join(LEGS))
     ap.add_argument("--filters", type=int, default=256)
     ap.add_argument("--lanes", type=int, default=0, help="concurrent chains of the batch (0: autotuned)")
-    ap.add_argument("--served-seconds", type=float, default=30.0)
+    ap.add_argument("--served-seconds", type=float, default=60.0)
+    ap.add_argument("--alongside-seconds", type=float, default=30.0,
+                    help="length of the stream served alongside the memory-heavy filter")
     ap.add_argument("--served-rate", type=float, default=200.0)
     ap.add_argument("--sweep-rates", type=lambda x: [float(v) for v in x.split(",") if v], default=[400.0, 600.0, 800.0],
                     help="further arrival rates of the served capacity sweep (comma list; empty: none)")
@@ -499,7 +508,8 @@ def main():
             f.close()
     if "served" in legs:
         log("served stream, %.0f s at %.0f requests/s" % (args.served_seconds, args.served_rate))
-        sv = served_leg(smc, ctx, inp, configs, args.served_seconds, args.served_rate)
+        sv = served_leg(smc, ctx, inp, configs, args.served_seconds, args.served_rate,
+                        window_s=args.alongside_seconds if "memory" in legs else None)
         sv = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in sv.items()}
         sv["workload"] = ("served_mix: Poisson %.0f req/s for %.0f s, 50/50 SIR / SIRS4, J log-uniform [500, 5e4], "
                           "N~U{52..520}, FP64" % (args.served_rate, args.served_seconds))
@@ -530,7 +540,7 @@ def main():
         out["memory_heavy"] = mem
     if "memory" in legs:
         log("memory-heavy filter served alongside the stream")
-        sv = served_leg(smc, ctx, inp, configs, args.served_seconds, args.served_rate, check_n=4,
+        sv = served_leg(smc, ctx, inp, configs, args.alongside_seconds, args.served_rate, check_n=4,
                         heavy={"model": "seir10", "J": args.memory_J, "N": args.memory_N, "stream": 999})
         mem["served_alongside"] = {"heavy_latency_s": round(sv["heavy"]["latency_s"], 3),
                                    "heavy_loglik_equals_alone": sv["heavy"]["loglik"] == mem["loglik"],
@@ -539,9 +549,13 @@ def main():
                                                                  / sv["makespan_s"], 1),
                                    "max_rel_err_vs_alone": sv["max_rel_err_vs_alone"],
                                    "heavy_latency_vs_alone": round(sv["heavy"]["latency_s"] / mem["s_per_filter"], 2)}
-        if "served" in out:
-            # the same stream without the heavy request
-            mem["served_alongside"]["small_p99_vs_without_heavy"] = round(sv["p99_ms"] / out["served"]["p99_ms"], 2)
+        base = out.get("served", {}).get("window")
+        if base and base.get("p99_ms"):
+            # the same requests (the served stream's first alongside_seconds) without the
+            # heavy request
+            mem["served_alongside"]["stream_seconds"] = args.alongside_seconds
+            mem["served_alongside"]["small_p99_without_heavy_ms"] = round(base["p99_ms"], 3)
+            mem["served_alongside"]["small_p99_vs_without_heavy"] = round(sv["p99_ms"] / base["p99_ms"], 2)
         out["memory_heavy"] = mem
     if "single" in legs:
         log("single large filter")

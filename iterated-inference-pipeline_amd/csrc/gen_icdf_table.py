@@ -21,8 +21,17 @@ script prints the largest error of the double evaluation against mpmath over 65 
 of every segment (about 2e-14 absolute; the tests check it again).
 
 Layout (coefficient-major, so that one LDS address and constant offsets reach all seven
-coefficients of a segment): tab[i * NSEG + idx] = c_i of segment idx.  The header also
-holds the coefficients rounded to float, for the FP32 stream.
+coefficients of a segment): tab[i * NSEG + slot] = c_i of segment idx, where the slot is
+idx rotated to the low bits of the converted word's exponent and top fraction bits, so
+that the transform reads the slot straight from those bits (a shift and a mask):
+
+  FP64: the word is converted as v1 = w | 1 = u 2^33, whose biased exponent is
+        (E + 33) + 1023, so slot = (idx + 1023 * 16) mod NSEG = (idx + ROT64) mod NSEG;
+  FP32: the top 24 bits are converted as v1 = (w >> 8) | 1 = u' 2^25, biased exponent
+        (E + 25) + 127 = (E + 33) + 119, so slot = (idx + ROT32) mod NSEG.
+
+The header also holds the coefficients rounded to float, for the FP32 stream, in the
+FP32 slots.
 
 Run:  python gen_icdf_table.py > plom_icdf_table.h
 """
@@ -37,6 +46,8 @@ SUB_BITS = 4
 DEG = 6
 E_MIN, E_MAX = -33, -2
 NSEG = (E_MAX - E_MIN + 1) << SUB_BITS
+ROT64 = (1023 << SUB_BITS) % NSEG   # slot of segment 0 (u in [2^-33, ...), v1 = 1.0)
+ROT32 = (119 << SUB_BITS) % NSEG    # slot of segment 0 for the float conversion
 
 
 def seg_u(idx, t):
@@ -80,16 +91,20 @@ def main():
            "#define PLOM_ICDF_NSEG %d" % NSEG,
            "#define PLOM_ICDF_EB_MIN %d /* biased exponent of the first octave, 2^%d */" % (E_MIN + 1023, E_MIN),
            "#define PLOM_ICDF_TAB_SIZE %d" % ((DEG + 1) * NSEG),
+           "#define PLOM_ICDF_ROT64 %d /* slot of segment 0 in the FP64 table */" % ROT64,
+           "#define PLOM_ICDF_ROT32 %d /* slot of segment 0 in the FP32 table */" % ROT32,
            "#define PLOM_ICDF_TAB_VALUES { \\"]
+    at64 = [(slot - ROT64) % NSEG for slot in range(NSEG)]   # segment held by each slot
+    at32 = [(slot - ROT32) % NSEG for slot in range(NSEG)]
     for i in range(DEG + 1):
-        for idx in range(0, NSEG, 4):
-            out.append("    " + ", ".join(coef[j][i].hex() for j in range(idx, idx + 4)) + ", \\")
+        for slot in range(0, NSEG, 4):
+            out.append("    " + ", ".join(coef[at64[j]][i].hex() for j in range(slot, slot + 4)) + ", \\")
     out.append("}")
-    out.append("/* the same coefficients rounded to float (the FP32 stream) */")
+    out.append("/* the same coefficients rounded to float (the FP32 stream), in the FP32 slots */")
     out.append("#define PLOM_ICDF_TAB32_VALUES { \\")
     for i in range(DEG + 1):
-        for idx in range(0, NSEG, 4):
-            out.append("    " + ", ".join("%sf" % float(np.float32(coef[j][i])).hex() for j in range(idx, idx + 4)) + ", \\")
+        for slot in range(0, NSEG, 4):
+            out.append("    " + ", ".join("%sf" % float(np.float32(coef[at32[j]][i])).hex() for j in range(slot, slot + 4)) + ", \\")
     out.append("}")
     out.append("#endif")
     print("\n".join(out))

@@ -198,7 +198,10 @@ PLOM_HD void plom_bm_pair(uint32_t x, uint32_t y, const double *tab, double *z0,
  * top PLOM_ICDF_SUB_BITS fraction bits: the top bits of the double's high word) at the
  * local coordinate t = 32 (1 + r) - 33 in [-1, 1) (r the remaining fraction, t exact),
  * and the sign flipped when w is odd.  About 2e-14 from Phi^-1(u) (checked against mpmath
- * by the tests); the mirror image of the low bit makes z symmetric exactly. */
+ * by the tests); the mirror image of the low bit makes z symmetric exactly.  The table
+ * holds segment s in slot (s + PLOM_ICDF_ROT64) mod NSEG, the low bits of the converted
+ * word's exponent and top fraction bits, so the slot's byte offset is a shift and a mask
+ * of the high word. */
 #if PLOM_ICDF_DEG != 6
 #error "plom_icdf_normal evaluates a degree-6 table"
 #endif
@@ -211,17 +214,18 @@ PLOM_HD double plom_icdf_normal(uint32_t w, const double *tab)
     const double v1 = (double)(w | 1u);
     const uint64_t b = plom_d2u(v1);
     const uint32_t hi = (uint32_t)(b >> 32);
-    /* the segment (E + 33) 16 + s of u, from v1's exponent E + 33 + 1023, taken modulo the
-     * table size: it is in [0, NSEG) anyway, and the mask lets the compiler fold the
-     * coefficients' offsets into the LDS loads */
-    const uint32_t idx = ((hi >> (20 - PLOM_ICDF_SUB_BITS)) + (uint32_t)PLOM_ICDF_NSEG -
-                          ((((uint32_t)PLOM_ICDF_EB_MIN + 33u) << PLOM_ICDF_SUB_BITS) & (PLOM_ICDF_NSEG - 1u))) &
-                         (PLOM_ICDF_NSEG - 1u);
+    /* the slot of u's segment (E + 33) 16 + s: v1's biased exponent E + 33 + 1023 and top
+     * fraction bits modulo the table size, as a byte offset (the mask also lets the
+     * compiler fold the coefficients' offsets into the LDS loads) */
+    const uint32_t off = (hi >> (20 - PLOM_ICDF_SUB_BITS - 3)) & ((PLOM_ICDF_NSEG - 1u) << 3);
+    /* x = 32 m, m in [1, 1 + 2^-4) the remaining fraction r after the top fraction bits
+     * (1 + r with v1's exponent replaced by that of 32: the same bits, exact), and
+     * t = 32 (1 + r) - 33 = x - 33, exact: x has at most 48 fraction bits below 2^5, so
+     * x - 33 in [-1, 1) needs 48 bits */
     const uint32_t keep = (1u << (20 - PLOM_ICDF_SUB_BITS)) - 1u;
-    const double m = plom_u2d(((uint64_t)((hi & keep) | 0x3ff00000u) << 32) | (b & 0xffffffffULL));
-    /* t = 32 (m - 1) - 1, exact: m - 1 is exact (Sterbenz) and so is the rest */
-    const double t = plom_fma(m - 1.0, PLOM_ICDF_T_SCALE, -1.0);
-    const double *c = tab + idx;
+    const double x = plom_u2d(((uint64_t)((hi & keep) | ((1023u + PLOM_ICDF_SUB_BITS + 1u) << 20)) << 32) | (b & 0xffffffffULL));
+    const double t = x - (PLOM_ICDF_T_SCALE + 1.0);
+    const double *c = (const double *)((const char *)tab + off);
     double p = c[6 * PLOM_ICDF_NSEG];
     p = plom_fma(p, t, c[5 * PLOM_ICDF_NSEG]);
     p = plom_fma(p, t, c[4 * PLOM_ICDF_NSEG]);
@@ -308,21 +312,21 @@ plom_nstream32_init(plom_nstream32 *ns, uint32_t seed, uint32_t stream, uint32_t
 
 /* plom_icdf_normal in FP32 on the top 24 bits of the word: u = ((w >> 9) + 1/2) 2^-24 =
  * ((w >> 8) | 1) 2^-25 (exact in float, within 2^-25 of the FP64 transform's u, in
- * [2^-25, 1/2)), the same segments (the float's exponent and top 4 fraction bits) and the
- * same sign bit w & 1 */
-#define PLOM_ICDF_EBF_MIN (PLOM_ICDF_EB_MIN - 1023 + 127)   /* float's biased exponent of 2^-33 */
+ * [2^-25, 1/2)), the same segments (the float's exponent and top 4 fraction bits; the FP32
+ * table holds segment s in slot (s + PLOM_ICDF_ROT32) mod NSEG) and the same sign bit
+ * w & 1 */
 __device__ inline __attribute__((always_inline)) float plom_icdf_normal32(uint32_t w, const float *tab)
 {
     /* the float v1 = (w >> 8) | 1 (exact) has u's fraction bits and a biased exponent 25
      * higher: the segment is read from v1 directly */
     const float v1 = (float)((w >> 8) | 1u);
     const uint32_t b = __float_as_uint(v1);
-    const uint32_t idx = ((b >> (23 - PLOM_ICDF_SUB_BITS)) + (uint32_t)PLOM_ICDF_NSEG -
-                          ((((uint32_t)PLOM_ICDF_EBF_MIN + 25u) << PLOM_ICDF_SUB_BITS) & (PLOM_ICDF_NSEG - 1u))) &
-                         (PLOM_ICDF_NSEG - 1u);
+    const uint32_t off = (b >> (23 - PLOM_ICDF_SUB_BITS - 2)) & ((PLOM_ICDF_NSEG - 1u) << 2);
     const float m = __uint_as_float((b & ((1u << (23 - PLOM_ICDF_SUB_BITS)) - 1u)) | 0x3f800000u);
-    const float t = __builtin_fmaf(m - 1.0f, (float)PLOM_ICDF_T_SCALE, -1.0f);
-    const float *c = tab + idx;
+    /* t = 32 (m - 1) - 1 = 32 m - 33 in one fused multiply-add, exact: m in [1, 1 + 2^-4)
+     * has 19 fraction bits, so 32 m - 33 in [-1, 1) needs 19 bits */
+    const float t = __builtin_fmaf(m, (float)PLOM_ICDF_T_SCALE, -(float)(PLOM_ICDF_T_SCALE + 1.0));
+    const float *c = (const float *)((const char *)tab + off);
     float p = c[6 * PLOM_ICDF_NSEG];
     p = __builtin_fmaf(p, t, c[5 * PLOM_ICDF_NSEG]);
     p = __builtin_fmaf(p, t, c[4 * PLOM_ICDF_NSEG]);

@@ -462,8 +462,9 @@ def main():
               "frac": round(achieved / HBM_PEAK_GBPS, 5),
               "traffic": read_profile_value(prof, "traffic_bytes_per_launch"),
               "kernel": "k_propagate<sirs4,double,full> (batched, %d lanes)" % fp64["lanes"],
-              "binding": "VALU issue: the kernel's arithmetic per byte is about 100x the HBM ridge, so the "
-                         "frac above is its HBM share and valu_issue_frac its binding limit",
+              "binding": "VALU issue, in it the FP64 operations: the kernel's FP64 arithmetic per byte is about "
+                         "9x the chip's FP64 ridge, so the frac above is its HBM share and valu_issue_frac "
+                         "(and fp64.frac) its compute use",
               "valu_issue_frac": valu,
               "algorithmic_bytes_per_particle_step": B,
               "algorithmic_bytes_per_launch": round(per_launch, 1),

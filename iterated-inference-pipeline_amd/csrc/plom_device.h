@@ -66,6 +66,8 @@ struct plom_dfilter {
     double *hpart;      /* [n_hat][T]: per tile, tree sum of value x weight */
     double *hat;        /* [(N+1)][n_hat][3]: low95, mean, high95 */
     double *pres;       /* [N][n_ts][2]: prediction mean and residual */
+    double *prw;        /* [T] per tile: the resampled copies (offspring) of its particles */
+    double *prp;        /* [4][n_ts][T] per tile and series: sum o mu, sum o var, mean mu, sum o (mu - mean)^2 */
     /* PLOM_FLAG_MIF: per-particle parameters */
     int32_t n_theta, n_pp;
     const int32_t *pm_src;          /* [n_pp] component of each per-particle pv slot, -1 = pv */
@@ -676,6 +678,97 @@ __device__ inline bool hat_weighted(const plom_dfilter *f, int n)
     return !(f->flags & PLOM_FLAG_NO_FILTER) && f->nonan[n] > 0 && f->rflag[n];
 }
 
+/* number of systematic points k in [0, J) with (k 2^33 + u) Q <= C J 2^33, i.e. of
+ * particles whose ancestor index is at most the particle with inclusive prefix C */
+__device__ inline uint64_t hat_count_le(uint64_t C, uint64_t J, uint64_t unum, uint64_t Q)
+{
+    const plom_u128 X = (plom_u128)C * J << 33;
+    const double est = (double)C * (double)J / (double)Q - (double)unum * 0x1p-33;
+    int64_t k = est < 0.0 ? 0 : (int64_t)est + 1;
+    if (k > (int64_t)J) k = (int64_t)J;
+    while (k > 0 && (plom_u128)(((uint64_t)(k - 1) << 33) + unum) * Q > X) k--;
+    while (k < (int64_t)J && (plom_u128)(((uint64_t)k << 33) + unum) * Q <= X) k++;
+    return (uint64_t)k;
+}
+
+/* the copies o_j of particle j in the resampled cloud of interval n (1 each when the
+ * interval did not resample): the systematic points between its exclusive and inclusive
+ * prefix */
+__device__ inline uint64_t hat_offspring(const plom_dfilter *f, int n, int j, uint64_t unum, uint64_t Q)
+{
+    if (!f->rflag[n]) return 1;
+    const int tile = j / TILE;
+    const uint64_t base = tile ? f->tincl[tile - 1] : 0ull;
+    const uint64_t C = base + f->cum[j];
+    const uint64_t Cm = (j % TILE) ? base + f->cum[j - 1] : base;
+    return hat_count_le(C, (uint64_t)f->J, unum, Q) - hat_count_le(Cm, (uint64_t)f->J, unum, Q);
+}
+
+/* pairwise tree (contiguous, zero padded to a power of two) over the T tile values of a
+ * filter: per chunk of 256 tiles tree256, then over the chunks (at most 32) by thread 0;
+ * every thread gets the total */
+__device__ inline double tiles_tree(const double *v, int T, double *lds, double *chunks)
+{
+    const int nch = (T + TILE - 1) / TILE;
+    for (int c = 0; c < nch; c++) {
+        const int t = c * TILE + threadIdx.x;
+        const double s = tree256(t < T ? v[t] : 0.0, lds);
+        if (threadIdx.x == 0) chunks[c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int P = 1;
+        while (P < nch) P <<= 1;
+        for (int c = nch; c < P; c++) chunks[c] = 0.0;
+        for (int h = 1; h < P; h <<= 1)
+            for (int i = 0; i < P; i += 2 * h) chunks[i] = chunks[i] + chunks[i + h];
+        lds[4] = chunks[0];
+    }
+    __syncthreads();
+    const double out = lds[4];
+    __syncthreads();
+    return out;
+}
+
+/* the (count, mean, M2) of the T tiles merged along the fixed pairwise tree (within a chunk
+ * of 256 tiles, then over the chunks padded with empty sets to a power of two): the M2 of
+ * the union; every thread gets it */
+__device__ inline double tiles_chan(const double *w, const double *m, const double *q, int T, double *sc, double *sm,
+                                    double *sq, double *cc, double *cm, double *cq)
+{
+    const int nch = (T + TILE - 1) / TILE;
+    for (int c = 0; c < nch; c++) {
+        const int t = c * TILE + threadIdx.x;
+        sc[threadIdx.x] = t < T ? w[t] : 0.0;
+        sm[threadIdx.x] = t < T ? m[t] : 0.0;
+        sq[threadIdx.x] = t < T ? q[t] : 0.0;
+        __syncthreads();
+        for (int h = 1; h < TILE; h <<= 1) {
+            const int i = threadIdx.x * 2 * h;
+            double c2 = 0.0, m2 = 0.0, q2 = 0.0;
+            if (i < TILE) plom_mif_combine(sc[i], sm[i], sq[i], sc[i + h], sm[i + h], sq[i + h], &c2, &m2, &q2);
+            __syncthreads();
+            if (i < TILE) { sc[i] = c2; sm[i] = m2; sq[i] = q2; }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) { cc[c] = sc[0]; cm[c] = sm[0]; cq[c] = sq[0]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int P = 1;
+        while (P < nch) P <<= 1;
+        for (int c = nch; c < P; c++) { cc[c] = 0.0; cm[c] = 0.0; cq[c] = 0.0; }
+        for (int h = 1; h < P; h <<= 1)
+            for (int i = 0; i < P; i += 2 * h)
+                plom_mif_combine(cc[i], cm[i], cq[i], cc[i + h], cm[i + h], cq[i + h], &cc[i], &cm[i], &cq[i]);
+        sq[0] = cq[0];
+    }
+    __syncthreads();
+    const double out = sq[0];
+    __syncthreads();
+    return out;
+}
+
 template <class M, typename R>
 __global__ __launch_bounds__(TILE) void k_hat_tiles(plom_dfilter *f)
 {
@@ -689,10 +782,10 @@ __global__ __launch_bounds__(TILE) void k_hat_tiles(plom_dfilter *f)
     const int j = tile * TILE + threadIdx.x;
     const bool valid = j < J;
     double v[M::N_HAT];
+    double xs[M::S];
     uint64_t q = 0;
     if (valid) {
         const R *Xs = (const R *)f->X[(n + 1) & 1];
-        double xs[M::S];
 #pragma unroll
         for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
         M::hat_values(xs, f->pv, cov_at(f, n + 1), f->drift_on ? &f->dv : nullptr, f->times[n + 1], v);
@@ -730,97 +823,78 @@ __global__ __launch_bounds__(TILE) void k_hat_tiles(plom_dfilter *f)
         f->hpref[(size_t)k * Jp + j] = pref;
         __syncthreads();
     }
+    /* the tile's part of the prediction residuals (print.c:632-701) over the resampled
+     * cloud: with o_j the copies of particle j, W = sum o, and per series the sums
+     * A = sum o mu and B = sum o var (tree256), the tile mean A / W and
+     * Q = sum o (mu - A / W)^2; k_pred_res merges the tiles */
+    if (!f->prw || (f->flags & PLOM_FLAG_NO_FILTER) || f->nonan[n] == 0) return;   /* uniform */
+    const bool res_on = f->rflag[n] != 0;
+    const uint64_t Q = res_on ? f->tincl[T - 1] : 0;
+    const uint64_t unum = plom_resample_num(f->seed, f->stream, (uint32_t)n);
+    uint64_t o = 0;
+    double mu[M::N_TS], var[M::N_TS];
+#pragma unroll
+    for (int ts = 0; ts < M::N_TS; ts++) { mu[ts] = 0.0; var[ts] = 0.0; }
+    if (valid) {
+        o = hat_offspring(f, n, j, unum, Q);
+        if (o) {
+            double obs[M::N_TS];
+            M::proj2obs(xs, f->pv, cov_at(f, n + 1), obs);
+            M::obs_moments(obs, f->pv, cov_at(f, n + 1), f->times[n + 1], mu, var);
+        }
+    }
+    const double od = (double)o;
+    const double W = tree256(od, lds);
+    if (threadIdx.x == 0) f->prw[tile] = W;
+    double *pp = f->prp;
+#pragma unroll
+    for (int ts = 0; ts < M::N_TS; ts++) {
+        const double A = tree256(o ? od * mu[ts] : 0.0, lds);
+        const double B = tree256(o ? od * var[ts] : 0.0, lds);
+        const double mt = W > 0.0 ? A / W : 0.0;
+        const double d = mu[ts] - mt;
+        const double Qs = tree256(o ? od * (d * d) : 0.0, lds);
+        if (threadIdx.x == 0) {
+            pp[(size_t)(0 * M::N_TS + ts) * T + tile] = A;
+            pp[(size_t)(1 * M::N_TS + ts) * T + tile] = B;
+            pp[(size_t)(2 * M::N_TS + ts) * T + tile] = mt;
+            pp[(size_t)(3 * M::N_TS + ts) * T + tile] = Qs;
+        }
+    }
 }
 
-/* number of systematic points k in [0, J) with (k 2^33 + u) Q <= C J 2^33, i.e. of
- * particles whose ancestor index is at most the particle with inclusive prefix C */
-__device__ inline uint64_t hat_count_le(uint64_t C, uint64_t J, uint64_t unum, uint64_t Q)
-{
-    const plom_u128 X = (plom_u128)C * J << 33;
-    const double est = (double)C * (double)J / (double)Q - (double)unum * 0x1p-33;
-    int64_t k = est < 0.0 ? 0 : (int64_t)est + 1;
-    if (k > (int64_t)J) k = (int64_t)J;
-    while (k > 0 && (plom_u128)(((uint64_t)(k - 1) << 33) + unum) * Q > X) k--;
-    while (k < (int64_t)J && (plom_u128)(((uint64_t)k << 33) + unum) * Q <= X) k++;
-    return (uint64_t)k;
-}
 
-/* prediction residuals of interval n over the resampled cloud (print.c:632-701): with o_j
- * the number of copies of particle j, pred = sum o_j mu_j / J, var_state = sum o_j
- * (mu_j - pred)^2 / (J - 1), var_obs = sum o_j sigma2_j / J, res = (y - pred) /
- * sqrt(var_state + var_obs); each sum strided by thread, then a tree over the block */
+/* prediction residuals of interval n over the resampled cloud (print.c:632-701), one block
+ * per series from the tiles' parts (k_hat_tiles): pred = sum_t A_t / J, var_obs =
+ * sum_t B_t / J (pairwise trees over the tiles), var_state = M2 / (J - 1) with M2 the
+ * (count, mean, M2) merge of the tiles' (W_t, A_t / W_t, Q_t) along the fixed tree, that is
+ * sum o_j (mu_j - pred)^2; res = (y - pred) / sqrt(var_state + var_obs) */
 template <class M, typename R>
 __global__ __launch_bounds__(TILE) void k_pred_res(plom_dfilter *f)
 {
     __shared__ double lds[8];
+    __shared__ double chunks[MAX_TILES / TILE + 1];
+    __shared__ double sc[TILE], sm[TILE], sq[TILE];
+    __shared__ double cc[MAX_TILES / TILE + 1], cm[MAX_TILES / TILE + 1], cq[MAX_TILES / TILE + 1];
     const int n = f->step - 1;
-    if (n < 0 || n >= f->N || !f->pres) return;
+    const int ts = blockIdx.x;
+    if (n < 0 || n >= f->N || !f->pres || ts >= M::N_TS) return;
     double *out = f->pres + (size_t)n * M::N_TS * 2;
     if ((f->flags & PLOM_FLAG_NO_FILTER) || f->nonan[n] == 0) {
-        if (threadIdx.x < 2 * M::N_TS) out[threadIdx.x] = PLOM_NAN;
+        if (threadIdx.x < 2) out[2 * ts + threadIdx.x] = PLOM_NAN;
         return;
     }
-    const int J = f->J, Jp = f->Jp;
-    const bool res_on = f->rflag[n] != 0;
-    const uint64_t Q = res_on ? f->tincl[f->T - 1] : 0;
-    const uint64_t unum = plom_resample_num(f->seed, f->stream, (uint32_t)n);
-    const R *Xs = (const R *)f->X[(n + 1) & 1];
-    const double t1 = f->times[n + 1];
-    double a[M::N_TS], b[M::N_TS];
-#pragma unroll
-    for (int ts = 0; ts < M::N_TS; ts++) { a[ts] = 0.0; b[ts] = 0.0; }
-    for (int j = threadIdx.x; j < J; j += TILE) {
-        uint64_t o = 1;
-        if (res_on) {
-            const int tile = j / TILE;
-            const uint64_t C = (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j];
-            const uint64_t Cm = j ? ((j % TILE) ? (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j - 1] : f->tincl[tile - 1]) : 0ull;
-            o = hat_count_le(C, (uint64_t)J, unum, Q) - hat_count_le(Cm, (uint64_t)J, unum, Q);
-        }
-        if (!o) continue;
-        double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
-#pragma unroll
-        for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
-        M::proj2obs(xs, f->pv, cov_at(f, n + 1), obs);
-        M::obs_moments(obs, f->pv, cov_at(f, n + 1), t1, mu, var);
-#pragma unroll
-        for (int ts = 0; ts < M::N_TS; ts++) { a[ts] += (double)o * mu[ts]; b[ts] += (double)o * var[ts]; }
-    }
-    double pred[M::N_TS], vobs[M::N_TS], c[M::N_TS];
-#pragma unroll
-    for (int ts = 0; ts < M::N_TS; ts++) {
-        pred[ts] = tree256(a[ts], lds) / (double)J;
-        vobs[ts] = tree256(b[ts], lds) / (double)J;
-        c[ts] = 0.0;
-    }
-    for (int j = threadIdx.x; j < J; j += TILE) {
-        uint64_t o = 1;
-        if (res_on) {
-            const int tile = j / TILE;
-            const uint64_t C = (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j];
-            const uint64_t Cm = j ? ((j % TILE) ? (tile ? f->tincl[tile - 1] : 0ull) + f->cum[j - 1] : f->tincl[tile - 1]) : 0ull;
-            o = hat_count_le(C, (uint64_t)J, unum, Q) - hat_count_le(Cm, (uint64_t)J, unum, Q);
-        }
-        if (!o) continue;
-        double xs[M::S], obs[M::N_TS], mu[M::N_TS], var[M::N_TS];
-#pragma unroll
-        for (int k = 0; k < M::S; k++) xs[k] = (double)Xs[(size_t)k * Jp + j];
-        M::proj2obs(xs, f->pv, cov_at(f, n + 1), obs);
-        M::obs_moments(obs, f->pv, cov_at(f, n + 1), t1, mu, var);
-#pragma unroll
-        for (int ts = 0; ts < M::N_TS; ts++) {
-            const double d = mu[ts] - pred[ts];
-            c[ts] += (double)o * (d * d);
-        }
-    }
-#pragma unroll
-    for (int ts = 0; ts < M::N_TS; ts++) {
-        const double vstate = tree256(c[ts], lds) / (double)(J - 1);
-        if (threadIdx.x == 0) {
-            const double y = f->data[(size_t)n * M::N_TS + ts];
-            out[2 * ts] = pred[ts];
-            out[2 * ts + 1] = (y - pred[ts]) / plom_sqrt(vstate + vobs[ts]);
-        }
+    const int J = f->J, T = f->T;
+    const double *pp = f->prp;
+    const double pred = tiles_tree(pp + (size_t)(0 * M::N_TS + ts) * T, T, lds, chunks) / (double)J;
+    const double vobs = tiles_tree(pp + (size_t)(1 * M::N_TS + ts) * T, T, lds, chunks) / (double)J;
+    const double m2 = tiles_chan(f->prw, pp + (size_t)(2 * M::N_TS + ts) * T, pp + (size_t)(3 * M::N_TS + ts) * T, T,
+                                 sc, sm, sq, cc, cm, cq);
+    if (threadIdx.x == 0) {
+        const double vstate = m2 / (double)(J - 1);
+        const double y = f->data[(size_t)n * M::N_TS + ts];
+        out[2 * ts] = pred;
+        out[2 * ts + 1] = (y - pred) / plom_sqrt(vstate + vobs);
     }
 }
 
@@ -841,7 +915,7 @@ __global__ void k_hat0(plom_dfilter *f)
 /* What a model's translation unit exports: its dimensions and the host launchers of its
  * kernels.  abi and dfilter_bytes let the library refuse a model library compiled against
  * another version of this header (plom_model_load). */
-#define PLOM_MODEL_ABI 5
+#define PLOM_MODEL_ABI 6
 struct model_entry {
     int32_t abi;            /* PLOM_MODEL_ABI */
     int32_t dfilter_bytes;  /* sizeof(plom_dfilter) */
@@ -951,10 +1025,10 @@ static void launch_hat(int prec, hipStream_t s, plom_dfilter *df, int ntiles, in
 {
     if (prec == PLOM_FP32) {
         hipLaunchKernelGGL((k_hat_tiles<M, float>), dim3(ntiles), dim3(TILE), 0, s, df);
-        hipLaunchKernelGGL((k_pred_res<M, float>), dim3(1), dim3(TILE), 0, s, df);
+        hipLaunchKernelGGL((k_pred_res<M, float>), dim3(M::N_TS), dim3(TILE), 0, s, df);
     } else {
         hipLaunchKernelGGL((k_hat_tiles<M, double>), dim3(ntiles), dim3(TILE), 0, s, df);
-        hipLaunchKernelGGL((k_pred_res<M, double>), dim3(1), dim3(TILE), 0, s, df);
+        hipLaunchKernelGGL((k_pred_res<M, double>), dim3(M::N_TS), dim3(TILE), 0, s, df);
     }
     (void)ntiles; (void)n_hat;
 }

@@ -682,7 +682,7 @@ static void lineage_segments(int N, uint32_t flags, int *K, int *nseg)
  * server creates it) and starts from zeroed counters without memset calls. */
 struct layout {
     size_t off_X0, off_X1, off_logw, off_cum, off_tmax, off_tsum, off_tsum2, off_ttot, off_tincl, off_anc,
-        off_lslots, off_lstate, off_lobs, off_lsnap, off_hkey, off_hpref, off_hpart, off_hat, off_pres,
+        off_lslots, off_lstate, off_lobs, off_lsnap, off_hkey, off_hpref, off_hpart, off_hat, off_pres, off_prw, off_prp,
         off_th0, off_th1, off_mw, off_mpw, off_mps, off_mpm, off_mpq, off_mmean, off_mvar,
         off_img, off_df, off_cnt, off_scal, off_naf, off_llike_n, off_ess_n, off_rflag, off_nonan, off_times,
         off_data, off_pv, off_pvf, off_x0, off_pm_src, off_pm_tr, off_pm_sc, off_pm_dur, off_pm_min, off_pm_max,
@@ -728,6 +728,8 @@ static int make_layout(const plom_filter_desc *d, const plom_model_info *mi, lay
     PUT(off_hpart, NH * T * 8);
     PUT(off_hat, NH * (N + 1) * 3 * 8);
     PUT(off_pres, NH ? N * (size_t)mi->n_ts * 2 * 8 : 0);
+    PUT(off_prw, NH ? T * 8 : 0);
+    PUT(off_prp, NH ? 4 * (size_t)mi->n_ts * T * 8 : 0);
     const size_t NT = (d->flags & PLOM_FLAG_MIF) ? (size_t)d->n_theta : 0;
     PUT(off_th0, NT * Jp * 8);
     PUT(off_th1, NT * Jp * 8);
@@ -1141,6 +1143,8 @@ static int filter_create_impl(plom_ctx *c, const plom_filter_desc *d, const plom
     h->hpart = hat_on ? (double *)(b + L.off_hpart) : nullptr;
     h->hat = hat_on ? (double *)(b + L.off_hat) : nullptr;
     h->pres = hat_on ? (double *)(b + L.off_pres) : nullptr;
+    h->prw = hat_on ? (double *)(b + L.off_prw) : nullptr;
+    h->prp = hat_on ? (double *)(b + L.off_prp) : nullptr;
     const int mif_on = (d->flags & PLOM_FLAG_MIF) != 0;
     h->n_theta = mif_on ? d->n_theta : 0;
     h->n_pp = mif_on ? mi.n_pp : 0;

@@ -534,48 +534,65 @@ static void hat_row(const oracle_model *M, const double *X, int J, const double 
     }
 }
 
+static void chan_tree(const double *c, const double *m, const double *q, int n, double *oc, double *om, double *oq);
+
+/* The prediction residuals of interval n (print.c:632-701) over the resampled cloud (o_j
+ * the copies of particle j), in the device's order: per tile of 256 particles W = sum o and,
+ * per series, A = tree(o mu), B = tree(o var) (the tree of tree256), the tile mean A / W and
+ * Q = tree(o (mu - A / W)^2); then pred = tree over the tiles of A, / J, var_obs = the same
+ * of B, and var_state = M2 / (J - 1) with M2 the (count, mean, M2) merge of the tiles'
+ * (W, A / W, Q) along the fixed pairwise tree (= sum o (mu - pred)^2). */
 static void pred_res_row(const oracle_model *M, const double *X, int J, const double *pv, const double *cvd, double t,
-                         const double *y,
-                         const uint32_t *select, uint32_t *offspring, double *row)
+                         const double *y, const uint32_t *select, uint32_t *offspring, double *row)
 {
     const int S = M->S, NT = M->n_ts;
-    double a[TILE][16], b[TILE][16], c[TILE][16], col[TILE], obs[64], mu[64], var[64];
-    double pred[16], vobs[16];
+    const int T = (J + TILE - 1) / TILE, P = next_pow2(T);
+    double *w = calloc((size_t)P * (size_t)(1 + 4 * NT), sizeof(double));   /* W, then per series A, B, mean, Q */
+    double mu[TILE][16], var[TILE][16], obs[64], buf[TILE];
+    uint32_t o[TILE];
+    if (!w) { for (int k = 0; k < 2 * NT; k++) row[k] = NAN; return; }
     memset(offspring, 0, (size_t)J * sizeof(uint32_t));
     for (int k = 0; k < J; k++) offspring[select[k]]++;
-    memset(a, 0, sizeof a);
-    memset(b, 0, sizeof b);
-    memset(c, 0, sizeof c);
-    for (int j = 0; j < J; j++) {
-        if (!offspring[j]) continue;
-        M->proj2obs(X + (size_t)j * S, pv, cvd, obs);
-        M->obs_moments(obs, pv, cvd, t, mu, var);
+    double *A = w + P, *B = A + (size_t)NT * P, *MT = B + (size_t)NT * P, *Q = MT + (size_t)NT * P;
+    for (int tl = 0; tl < T; tl++) {
+        for (int i = 0; i < TILE; i++) {
+            const int j = tl * TILE + i;
+            o[i] = j < J ? offspring[j] : 0u;
+            for (int ts = 0; ts < NT; ts++) { mu[i][ts] = 0.0; var[i][ts] = 0.0; }
+            if (o[i]) {
+                M->proj2obs(X + (size_t)j * S, pv, cvd, obs);
+                M->obs_moments(obs, pv, cvd, t, mu[i], var[i]);
+            }
+            buf[i] = (double)o[i];
+        }
+        const double W = tree_sum(buf, TILE);
+        w[tl] = W;
         for (int ts = 0; ts < NT; ts++) {
-            a[j % TILE][ts] += (double)offspring[j] * mu[ts];
-            b[j % TILE][ts] += (double)offspring[j] * var[ts];
+            for (int i = 0; i < TILE; i++) buf[i] = o[i] ? (double)o[i] * mu[i][ts] : 0.0;
+            const double a = tree_sum(buf, TILE);
+            for (int i = 0; i < TILE; i++) buf[i] = o[i] ? (double)o[i] * var[i][ts] : 0.0;
+            const double bb = tree_sum(buf, TILE);
+            const double mt = W > 0.0 ? a / W : 0.0;
+            for (int i = 0; i < TILE; i++) {
+                const double d = mu[i][ts] - mt;
+                buf[i] = o[i] ? (double)o[i] * (d * d) : 0.0;
+            }
+            A[(size_t)ts * P + tl] = a;
+            B[(size_t)ts * P + tl] = bb;
+            MT[(size_t)ts * P + tl] = mt;
+            Q[(size_t)ts * P + tl] = tree_sum(buf, TILE);
         }
     }
     for (int ts = 0; ts < NT; ts++) {
-        for (int i = 0; i < TILE; i++) col[i] = a[i][ts];
-        pred[ts] = tree_sum(col, TILE) / (double)J;
-        for (int i = 0; i < TILE; i++) col[i] = b[i][ts];
-        vobs[ts] = tree_sum(col, TILE) / (double)J;
+        const double pred = tree_sum(A + (size_t)ts * P, P) / (double)J;
+        const double vobs = tree_sum(B + (size_t)ts * P, P) / (double)J;
+        double c, m, q;
+        chan_tree(w, MT + (size_t)ts * P, Q + (size_t)ts * P, P, &c, &m, &q);
+        const double vstate = q / (double)(J - 1);
+        row[2 * ts] = pred;
+        row[2 * ts + 1] = (y[ts] - pred) / plom_sqrt(vstate + vobs);
     }
-    for (int j = 0; j < J; j++) {
-        if (!offspring[j]) continue;
-        M->proj2obs(X + (size_t)j * S, pv, cvd, obs);
-        M->obs_moments(obs, pv, cvd, t, mu, var);
-        for (int ts = 0; ts < NT; ts++) {
-            const double d = mu[ts] - pred[ts];
-            c[j % TILE][ts] += (double)offspring[j] * (d * d);
-        }
-    }
-    for (int ts = 0; ts < NT; ts++) {
-        for (int i = 0; i < TILE; i++) col[i] = c[i][ts];
-        const double vstate = tree_sum(col, TILE) / (double)(J - 1);
-        row[2 * ts] = pred[ts];
-        row[2 * ts + 1] = (y[ts] - pred[ts]) / plom_sqrt(vstate + vobs[ts]);
-    }
+    free(w);
 }
 
 /* ------------------------------------------------------------------ the filter */

@@ -22,7 +22,8 @@ Legs on the same JSON line:
             free HBM: sustained particle-steps/s and p50 / p99 latency (submit -> result);
             then the same mix offered at 400, 600 and 800 requests/s (10 s each): the capacity is
             the highest sustained rate
-  single    one large filter (SIRS, J = 1e5, N = 520), FP64 and FP32
+  single    one large filter (SIRS, J = 1e5, N = 520), FP64 and FP32, and FP64 with the
+            outputs the smc command writes by default (hat and prediction residuals)
   psr       the same filter with the psr implementation (Euler-multinomial step, FP64),
             and the C oracle's psr on a bounded sample of it
   mif       one iterated-filtering pass of the same filter with per-particle parameters
@@ -302,6 +303,24 @@ def single_leg(smc, ctx, inp, configs, precision, steps, warmup, J, N):
             "ms_per_filter_by_tiles_per_block": tpb_ms}
 
 
+def hat_leg(smc, ctx, inp, configs, steps, warmup, J, N):
+    """The same filter with the outputs the smc command writes by default: the hat (weighted
+    mean and 95 % interval of every state and observation per interval) and the
+    prediction residuals."""
+    m, times, data, pv, x0 = inp.get("sirs4", N, configs.TRUTH["sirs4"])
+    f = smc.Filter(ctx, m, J, times, data, pv, x0, flags=smc.FLAG_HAT, seed=configs.FILTER_SEED, stream=0)
+    for _ in range(max(warmup, 1)):
+        f.run()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f.run()
+    ctx.sync()
+    wall = (time.perf_counter() - t0) / steps
+    f.close()
+    return {"value": round(J * N / wall, 1), "ms_per_filter": round(wall * 1e3, 3)}
+
+
 def psr_leg(smc, ctx, inp, configs, steps, warmup, J, N, cpu, cpu_J, cpu_N):
     m, times, data, pv, x0 = inp.get("sirs4", N, configs.TRUTH["sirs4"])
     x0 = np.round(x0)
@@ -562,7 +581,8 @@ def main():
         log("single large filter")
         out["single"] = {"workload": "SIRS 4x5e5, J=%d, N=%d" % (args.single_J, args.single_N),
                          "fp64": single_leg(smc, ctx, inp, configs, "fp64", args.steps, args.warmup, args.single_J, args.single_N),
-                         "fp32": single_leg(smc, ctx, inp, configs, "fp32", args.steps, args.warmup, args.single_J, args.single_N)}
+                         "fp32": single_leg(smc, ctx, inp, configs, "fp32", args.steps, args.warmup, args.single_J, args.single_N),
+                         "fp64_hat": hat_leg(smc, ctx, inp, configs, args.steps, args.warmup, args.single_J, args.single_N)}
     if "psr" in legs:
         log("psr filter")
         out["psr"] = dict(workload="SIRS 4x5e5 psr (Euler-multinomial), J=%d, N=%d, FP64" % (args.single_J, args.single_N),

@@ -322,15 +322,16 @@ def test_hat_deterministic_limit():
         np.testing.assert_allclose(r["hat"][n, :, 1], q, rtol=1e-13)
 
 
-def test_pred_res_against_numpy():
+@pytest.mark.parametrize("J", [400, 3000])
+def test_pred_res_against_numpy(J):
     """Prediction residuals (print.c:632-701) over the resampled cloud, from the stored
     states and selection: pred = mean of rep x, var_state its sample variance, var_obs the
     mean of rep (1 - rep) x + (rep phi x)^2, res = (y - pred) / sqrt(var_state + var_obs);
-    NaN rows where the interval had no data."""
+    NaN rows where the interval had no data.  The oracle sums per tile of 256 particles and
+    merges the tiles (2 and 12 tiles here)."""
     m, times, data, pv, x0 = setup("sirs4", 9)
     data = data.copy()
     data[3, :] = np.nan
-    J = 400
     r = oracle.smc(m.id, J, times, data, pv, x0, seed=5, stream=7, ancestry=True, all_states=True, hat=True)
     d = m.desc
     lay = d.layout

@@ -1019,7 +1019,8 @@ static void launch_lineage(int prec, int variant, hipStream_t s, plom_dfilter *d
     else launch_lineage_r<M, double>(variant, s, df);
 }
 
-/* the per-tile hat sort and the residuals (the library then launches k_hat_select) */
+/* the per-tile hat sort and the residuals (the library then launches k_hat_mean and
+ * k_hat_bounds) */
 template <class M>
 static void launch_hat(int prec, hipStream_t s, plom_dfilter *df, int ntiles, int n_hat)
 {

@@ -364,31 +364,18 @@ __global__ __launch_bounds__(TILE) void k_final(plom_dfilter *filters, const int
 }
 
 
-__device__ inline uint64_t sum256_u64(uint64_t v, uint64_t *lds)
-{
-#pragma unroll
-    for (int h = 1; h < 64; h <<= 1) v += __shfl_xor(v, h, 64);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) lds[w] = v;
-    __syncthreads();
-    const uint64_t r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
-    __syncthreads();
-    return r;
-}
-
-/* blocks 3k, 3k+1, 3k+2: mean, low95 and high95 of quantity k */
-__global__ __launch_bounds__(TILE) void k_hat_select(plom_dfilter *f)
+/* block k: the weighted mean of quantity k (a tree over the tiles' tree sums) */
+__global__ __launch_bounds__(TILE) void k_hat_mean(plom_dfilter *f)
 {
     __shared__ double lds[8];
-    __shared__ uint64_t lds64[8];
     __shared__ double s_chunk[MAX_TILES / TILE + 1];
     const int n = f->step - 1;
     if (n < 0 || n >= f->N || !f->hkey) return;
-    const int k = blockIdx.x / 3, role = blockIdx.x % 3;
-    const int T = f->T, Jp = f->Jp, NH = f->n_hat;
+    const int k = blockIdx.x;
+    const int T = f->T, NH = f->n_hat;
     const uint64_t Q = hat_weighted(f, n) ? f->tincl[T - 1] : (uint64_t)f->J;
     double *out = f->hat + ((size_t)(n + 1) * NH + k) * 3;
-    if (role == 0) {
+    {
         const int nch = (T + TILE - 1) / TILE;
         for (int c = 0; c < nch; c++) {
             const int t = c * TILE + threadIdx.x;
@@ -404,15 +391,46 @@ __global__ __launch_bounds__(TILE) void k_hat_select(plom_dfilter *f)
                 for (int i = 0; i < P; i += 2 * h) s_chunk[i] = s_chunk[i] + s_chunk[i + h];
             out[1] = s_chunk[0] / (double)Q;
         }
-        return;
     }
+}
+
+/* exact sum of a uint64 over a block of HAT_BOUND_THREADS (every thread gets it) */
+#define HAT_BOUND_THREADS 1024
+__device__ inline uint64_t sum_block_u64(uint64_t v, uint64_t *lds)
+{
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1) v += __shfl_xor(v, h, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    uint64_t r = 0;
+    for (int i = 0; i < HAT_BOUND_THREADS / 64; i++) r += lds[i];
+    __syncthreads();
+    return r;
+}
+
+/* blocks 2k, 2k+1: low95 and high95 of quantity k, the smallest key whose weight prefix
+ * reaches 1/40 (39/40) of the total, by bisection of the 64-bit key space; in every round
+ * each thread counts, in the tiles it covers, the weight of the keys at most the midpoint
+ * (a binary search of the sorted tile), and the block sums the counts (exact integers, so
+ * the order does not matter).  HAT_BOUND_THREADS threads, so that a thread covers one
+ * tile per round up to 1024 tiles. */
+__global__ __launch_bounds__(HAT_BOUND_THREADS) void k_hat_bounds(plom_dfilter *f)
+{
+    __shared__ uint64_t lds64[HAT_BOUND_THREADS / 64];
+    const int n = f->step - 1;
+    if (n < 0 || n >= f->N || !f->hkey) return;
+    const int k = blockIdx.x / 2, role = 1 + (int)(blockIdx.x % 2);
+    const int T = f->T, Jp = f->Jp, NH = f->n_hat;
+    const uint64_t Q = hat_weighted(f, n) ? f->tincl[T - 1] : (uint64_t)f->J;
+    double *out = f->hat + ((size_t)(n + 1) * NH + k) * 3;
     const uint64_t pnum = role == 1 ? 1u : 39u;
     const uint64_t *keys = f->hkey + (size_t)k * Jp, *pref = f->hpref + (size_t)k * Jp;
     uint64_t lo = 0, hi = ~0ull;
     while (lo < hi) {
         const uint64_t mid = lo + ((hi - lo) >> 1);
         uint64_t W = 0;
-        for (int t = threadIdx.x; t < T; t += TILE) {
+        for (int t = threadIdx.x; t < T; t += HAT_BOUND_THREADS) {
             const uint64_t *kt = keys + (size_t)t * TILE;
             int a = 0, b = TILE;                 /* count of keys <= mid in the sorted tile */
             while (a < b) {
@@ -421,7 +439,7 @@ __global__ __launch_bounds__(TILE) void k_hat_select(plom_dfilter *f)
             }
             if (a) W += pref[(size_t)t * TILE + a - 1];
         }
-        W = sum256_u64(W, lds64);
+        W = sum_block_u64(W, lds64);
         if ((plom_u128)W * 40u >= (plom_u128)Q * pnum) hi = mid; else lo = mid + 1;
     }
     if (threadIdx.x == 0) out[role == 1 ? 0 : 2] = hat_unkey(lo);
@@ -585,7 +603,8 @@ extern "C" int plom_model_load(const char *path, int32_t *id)
 static void launch_hat_all(const model_entry *me, int prec, hipStream_t s, plom_dfilter *df, int ntiles, int n_hat)
 {
     me->launch_hat(prec, s, df, ntiles, n_hat);
-    hipLaunchKernelGGL(k_hat_select, dim3(3 * n_hat), dim3(TILE), 0, s, df);
+    hipLaunchKernelGGL(k_hat_mean, dim3(n_hat), dim3(TILE), 0, s, df);
+    hipLaunchKernelGGL(k_hat_bounds, dim3(2 * n_hat), dim3(HAT_BOUND_THREADS), 0, s, df);
 }
 
 /* ------------------------------------------------------------------ context / filters */

@@ -769,6 +769,10 @@ __device__ inline double tiles_chan(const double *w, const double *m, const doub
     return out;
 }
 
+/* Block (tile, y), y < PLOM_HAT_GROUPS, sorts the quantities [y QB, (y + 1) QB) of its
+ * tile, QB = the quantities over PLOM_HAT_GROUPS blocks; block y = PLOM_HAT_GROUPS sums the
+ * tile's parts of the residuals. */
+#define PLOM_HAT_GROUPS 4
 template <class M, typename R>
 __global__ __launch_bounds__(TILE) void k_hat_tiles(plom_dfilter *f)
 {
@@ -796,7 +800,9 @@ __global__ __launch_bounds__(TILE) void k_hat_tiles(plom_dfilter *f)
             q = 1;
         }
     }
-    for (int k = 0; k < M::N_HAT; k++) {
+    constexpr int QB = (M::N_HAT + PLOM_HAT_GROUPS - 1) / PLOM_HAT_GROUPS;
+    const int k0 = (int)blockIdx.y * QB, k1 = k0 + QB < M::N_HAT ? k0 + QB : M::N_HAT;
+    for (int k = k0; k < k1; k++) {
         const double part = tree256(valid ? v[k] * (double)q : 0.0, lds);
         if (threadIdx.x == 0) f->hpart[(size_t)k * T + tile] = part;
         s_key[threadIdx.x] = valid ? hat_key(v[k]) : ~0ull;
@@ -827,7 +833,7 @@ __global__ __launch_bounds__(TILE) void k_hat_tiles(plom_dfilter *f)
      * cloud: with o_j the copies of particle j, W = sum o, and per series the sums
      * A = sum o mu and B = sum o var (tree256), the tile mean A / W and
      * Q = sum o (mu - A / W)^2; k_pred_res merges the tiles */
-    if (!f->prw || (f->flags & PLOM_FLAG_NO_FILTER) || f->nonan[n] == 0) return;   /* uniform */
+    if (blockIdx.y != PLOM_HAT_GROUPS || !f->prw || (f->flags & PLOM_FLAG_NO_FILTER) || f->nonan[n] == 0) return;   /* uniform */
     const bool res_on = f->rflag[n] != 0;
     const uint64_t Q = res_on ? f->tincl[T - 1] : 0;
     const uint64_t unum = plom_resample_num(f->seed, f->stream, (uint32_t)n);
@@ -1025,10 +1031,10 @@ template <class M>
 static void launch_hat(int prec, hipStream_t s, plom_dfilter *df, int ntiles, int n_hat)
 {
     if (prec == PLOM_FP32) {
-        hipLaunchKernelGGL((k_hat_tiles<M, float>), dim3(ntiles), dim3(TILE), 0, s, df);
+        hipLaunchKernelGGL((k_hat_tiles<M, float>), dim3(ntiles, PLOM_HAT_GROUPS + 1), dim3(TILE), 0, s, df);
         hipLaunchKernelGGL((k_pred_res<M, float>), dim3(M::N_TS), dim3(TILE), 0, s, df);
     } else {
-        hipLaunchKernelGGL((k_hat_tiles<M, double>), dim3(ntiles), dim3(TILE), 0, s, df);
+        hipLaunchKernelGGL((k_hat_tiles<M, double>), dim3(ntiles, PLOM_HAT_GROUPS + 1), dim3(TILE), 0, s, df);
         hipLaunchKernelGGL((k_pred_res<M, double>), dim3(M::N_TS), dim3(TILE), 0, s, df);
     }
     (void)ntiles; (void)n_hat;

@@ -409,6 +409,30 @@ __device__ inline uint64_t sum_block_u64(uint64_t v, uint64_t *lds)
     return r;
 }
 
+/* block-wide minimum of lo and maximum of hi (lds: 2 HAT_BOUND_THREADS / 64 entries) */
+__device__ inline void minmax_block_u64(uint64_t *lo, uint64_t *hi, uint64_t *lds)
+{
+    uint64_t a = *lo, b = *hi;
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1) {
+        const uint64_t a2 = __shfl_xor(a, h, 64), b2 = __shfl_xor(b, h, 64);
+        a = a2 < a ? a2 : a;
+        b = b2 > b ? b2 : b;
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { lds[2 * w] = a; lds[2 * w + 1] = b; }
+    __syncthreads();
+    a = ~0ull;
+    b = 0ull;
+    for (int i = 0; i < HAT_BOUND_THREADS / 64; i++) {
+        a = lds[2 * i] < a ? lds[2 * i] : a;
+        b = lds[2 * i + 1] > b ? lds[2 * i + 1] : b;
+    }
+    __syncthreads();
+    *lo = a;
+    *hi = b;
+}
+
 /* blocks 2k, 2k+1: low95 and high95 of quantity k, the smallest key whose weight prefix
  * reaches 1/40 (39/40) of the total, by bisection of the 64-bit key space; in every round
  * each thread counts, in the tiles it covers, the weight of the keys at most the midpoint
@@ -417,7 +441,7 @@ __device__ inline uint64_t sum_block_u64(uint64_t v, uint64_t *lds)
  * tile per round up to 1024 tiles. */
 __global__ __launch_bounds__(HAT_BOUND_THREADS) void k_hat_bounds(plom_dfilter *f)
 {
-    __shared__ uint64_t lds64[HAT_BOUND_THREADS / 64];
+    __shared__ uint64_t lds64[2 * (HAT_BOUND_THREADS / 64)];
     const int n = f->step - 1;
     if (n < 0 || n >= f->N || !f->hkey) return;
     const int k = blockIdx.x / 2, role = 1 + (int)(blockIdx.x % 2);
@@ -426,7 +450,29 @@ __global__ __launch_bounds__(HAT_BOUND_THREADS) void k_hat_bounds(plom_dfilter *
     double *out = f->hat + ((size_t)(n + 1) * NH + k) * 3;
     const uint64_t pnum = role == 1 ? 1u : 39u;
     const uint64_t *keys = f->hkey + (size_t)k * Jp, *pref = f->hpref + (size_t)k * Jp;
+    /* The bound lies between the smallest and the largest of the tiles' own bounds (the
+     * smallest key of a tile whose prefix reaches pnum/40 of the tile's weight), over the
+     * tiles with weight: below the smallest every such tile falls short of its share, so
+     * the sum does; at the largest every tile reaches its share, so the sum does.  The
+     * bisection starts from that bracket (the full key range when no tile has weight), and
+     * finds the same key. */
+    uint64_t tlo = ~0ull, thi = 0ull;
+    for (int t = threadIdx.x; t < T; t += HAT_BOUND_THREADS) {
+        const uint64_t *pt = pref + (size_t)t * TILE;
+        const uint64_t Wt = pt[TILE - 1];
+        if (!Wt) continue;
+        int a = 0, b = TILE - 1;                 /* first position whose prefix reaches the share */
+        while (a < b) {
+            const int m = (a + b) >> 1;
+            if ((plom_u128)pt[m] * 40u >= (plom_u128)Wt * pnum) b = m; else a = m + 1;
+        }
+        const uint64_t x = keys[(size_t)t * TILE + a];
+        tlo = x < tlo ? x : tlo;
+        thi = x > thi ? x : thi;
+    }
+    minmax_block_u64(&tlo, &thi, lds64);
     uint64_t lo = 0, hi = ~0ull;
+    if (tlo <= thi) { lo = tlo; hi = thi; }
     while (lo < hi) {
         const uint64_t mid = lo + ((hi - lo) >> 1);
         uint64_t W = 0;

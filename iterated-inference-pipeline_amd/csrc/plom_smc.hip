@@ -473,12 +473,26 @@ __global__ __launch_bounds__(HAT_BOUND_THREADS) void k_hat_bounds(plom_dfilter *
     minmax_block_u64(&tlo, &thi, lds64);
     uint64_t lo = 0, hi = ~0ull;
     if (tlo <= thi) { lo = tlo; hi = thi; }
+    /* the last key of each 16-key segment of the thread's first tile, in registers: the
+     * segments wholly at most the midpoint are counted there, and a round's search goes
+     * to memory only inside the one segment that straddles it */
+    uint64_t seg[TILE / 16];
+    const bool own = (int)threadIdx.x < T;
+#pragma unroll
+    for (int i = 0; i < TILE / 16; i++) seg[i] = own ? keys[(size_t)threadIdx.x * TILE + 16 * i + 15] : 0ull;
     while (lo < hi) {
         const uint64_t mid = lo + ((hi - lo) >> 1);
         uint64_t W = 0;
         for (int t = threadIdx.x; t < T; t += HAT_BOUND_THREADS) {
             const uint64_t *kt = keys + (size_t)t * TILE;
             int a = 0, b = TILE;                 /* count of keys <= mid in the sorted tile */
+            if (t == (int)threadIdx.x) {
+                int ns = 0;                      /* segments whose keys are all <= mid */
+#pragma unroll
+                for (int i = 0; i < TILE / 16; i++) ns += seg[i] <= mid ? 1 : 0;
+                a = 16 * ns;                     /* kt[a - 1] <= mid, and kt[a + 15] > mid */
+                b = ns < TILE / 16 ? a + 15 : TILE;
+            }
             while (a < b) {
                 const int m = (a + b) >> 1;
                 if (kt[m] <= mid) a = m + 1; else b = m;

@@ -772,7 +772,7 @@ __device__ inline double tiles_chan(const double *w, const double *m, const doub
 /* Block (tile, y), y < PLOM_HAT_GROUPS, sorts the quantities [y QB, (y + 1) QB) of its
  * tile, QB = the quantities over PLOM_HAT_GROUPS blocks; block y = PLOM_HAT_GROUPS sums the
  * tile's parts of the residuals. */
-#define PLOM_HAT_GROUPS 4
+#define PLOM_HAT_GROUPS 8
 template <class M, typename R>
 __global__ __launch_bounds__(TILE) void k_hat_tiles(plom_dfilter *f)
 {
